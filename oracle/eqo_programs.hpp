@@ -1,0 +1,43 @@
+// Workload programs for the oracle, written in the style of the code the
+// Equelle compiler emits (examples/app/out_heateq.cpp:28-73): the same
+// built-in calls in the same order, over the oracle runtime.
+// TEST INFRASTRUCTURE ONLY (see eqo.hpp).
+#pragma once
+#include <vector>
+
+#include "eqo.hpp"
+
+namespace eqo {
+
+// Heat conduction, out_heateq.cpp:28-58, extended with the nonlinear
+// conductivity used by the nonlinear-heat workload:
+//   k_model=const      flux = -(k |f|/|c1-c2|) grad u          (out_heateq.cpp:33-36)
+//   k_model=quadratic  k(u) = k (1 + u^2) on cells, face value = mean of the two
+//                      cells (On FirstCell / On SecondCell), flux = -(|f|/|c1-c2|) k_f grad u
+// Dirichlet faces (dir_boundary/dir_val) follow out_heateq.cpp:37-50; with no
+// Dirichlet faces the boundary is zero-flux.
+struct HeatProgram {
+    HeatProgram(const Runtime& er, const Params& p, const std::vector<int>& dir_faces,
+                const std::vector<double>& dir_vals);
+    ADB residual(const ADB& u, const ADB& u0, double dt) const;
+
+    const Runtime& er;
+    double k = 2.85, cv = 2.17e6;
+    bool quadratic = false;
+    IndexSet ifaces, first, second, all_cells, all_faces, bfaces, dir_boundary, dir_cells;
+    ADB itrans, vol, dir_val, dir_sign, dir_trans;
+    bool has_dir = false;
+};
+
+struct HeatResult {
+    std::vector<double> u;
+    std::vector<NewtonReport> reports;
+};
+HeatResult run_heat(const Params& p, const std::vector<double>& u_initial, const std::vector<int>& dir_faces,
+                    const std::vector<double>& dir_vals, const std::vector<double>& dts);
+
+// Wall time of `iters` Newton iterations (assembly + linear solve + update).
+double time_newton_iterations(const Runtime& rt, const HeatProgram& prog, const ADB& u0, double dt, int iters,
+                              int* lin_iters);
+
+}  // namespace eqo
