@@ -67,6 +67,7 @@ SIGNATURES = {
     "solve_for_update": [_P, _I, _IP, _IP],
     "csr_binop": [_I, _I, _I, _IP, _IP, _DP, _I, _I, _IP, _IP, _DP, _IP, _IP, _IP, _DP, _I],
     "csr_spmv": [_I, _I, _IP, _IP, _DP, _DP, _DP, _I],
+    "sell_spmv": [_P, _I, _DP, _DP, _I],
     "heat_run": [_S, _DP, _I, _IP, _DP, _I, _DP, _I, _DP, _IP, _DP],
     "heat_residual": [_S, _DP, _DP, _I, _IP, _DP, _I, _D, _IP, _DP, _IP, _IP, _DP],
 }

@@ -1,0 +1,460 @@
+// C-ABI of the runtime (include/eqhip.h): handle tables over the device engine.
+#include <eqhip.h>
+
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "core.hpp"
+#include "engine.hpp"
+#include "heat_program.hpp"
+#include "kernels.hpp"
+#include "krylov.hpp"
+#include "runtime_core.hpp"
+
+using namespace eqhip;
+
+namespace {
+thread_local std::string g_err;
+
+struct Rt {
+    std::unique_ptr<RuntimeCore> core;
+    std::vector<SP> scal;
+    std::vector<IP> sets;
+    std::vector<BP> bools;
+    template <class T>
+    static int put(std::vector<T>& v, T x) {
+        for (size_t i = 0; i < v.size(); ++i)
+            if (!v[i]) {
+                v[i] = std::move(x);
+                return (int)i + 1;
+            }
+        v.push_back(std::move(x));
+        return (int)v.size();
+    }
+    template <class T>
+    static const T& get(const std::vector<T>& v, int h) {
+        if (h < 1 || h > (int)v.size() || !v[h - 1]) throw std::runtime_error("invalid handle " + std::to_string(h));
+        return v[h - 1];
+    }
+    const SP& S(int h) const { return get(scal, h); }
+    const IP& I(int h) const { return get(sets, h); }
+    const BP& B(int h) const { return get(bools, h); }
+};
+
+template <class F>
+int guard(F&& f) {
+    try {
+        f();
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return 1;
+    } catch (...) {
+        g_err = "unknown error";
+        return 1;
+    }
+}
+Rt* R(void* p) {
+    if (!p) throw std::runtime_error("null runtime");
+    return static_cast<Rt*>(p);
+}
+
+Opm::ParameterGroup to_group(const char* params) {
+    ParamMap m;
+    m.parse(params ? params : "");
+    Opm::ParameterGroup g;
+    for (auto& e : m.kv) g.insertParameter(e.first, e.second);
+    return g;
+}
+}  // namespace
+
+extern "C" {
+
+const char* eqhip_last_error(void) { return g_err.c_str(); }
+
+int eqhip_runtime_create(const char* params, void** out) {
+    return guard([&] {
+        ParamMap m;
+        m.parse(params ? params : "");
+        auto* r = new Rt();
+        try {
+            r->core.reset(new RuntimeCore(m));
+        } catch (...) {
+            delete r;
+            throw;
+        }
+        *out = r;
+    });
+}
+
+int eqhip_runtime_destroy(void* rt) {
+    return guard([&] { delete static_cast<Rt*>(rt); });
+}
+
+int eqhip_grid_info(void* rt, int* info) {
+    return guard([&] {
+        DeviceGrid& g = R(rt)->core->grid();
+        info[0] = g.dim;
+        info[1] = g.nc;
+        info[2] = g.nf;
+        info[3] = g.nif;
+        info[4] = g.nf - g.nif;
+    });
+}
+
+int eqhip_grid_int_array(void* rt, int which, int* out) {
+    return guard([&] {
+        DeviceGrid& g = R(rt)->core->grid();
+        const DevPtr<int>& d = which == 0 ? g.face_cells : which == 1 ? g.cell_facepos : g.cell_faces;
+        std::vector<int> h = dev_to_host(d->p, d->n);
+        std::memcpy(out, h.data(), h.size() * sizeof(int));
+    });
+}
+
+int eqhip_grid_double_array(void* rt, int which, double* out) {
+    return guard([&] {
+        DeviceGrid& g = R(rt)->core->grid();
+        const DevPtr<double>* a[] = {&g.cell_centroids, &g.cell_volumes, &g.face_centroids, &g.face_areas,
+                                     &g.face_normals};
+        EQ_REQUIRE(which >= 0 && which < 5, "grid_double_array: bad selector " << which);
+        std::vector<double> h = dev_to_host((*a[which])->p, (*a[which])->n);
+        std::memcpy(out, h.data(), h.size() * sizeof(double));
+    });
+}
+
+static const DevCsr& opmat(Rt* r, int which) {
+    DeviceGrid& g = r->core->grid();
+    EQ_REQUIRE(which >= 0 && which < 3, "opmat: bad selector " << which);
+    return which == 0 ? g.grad : which == 1 ? g.div : g.fulldiv;
+}
+int eqhip_opmat_info(void* rt, int which, int* info) {
+    return guard([&] {
+        const DevCsr& m = opmat(R(rt), which);
+        info[0] = m.pat->rows;
+        info[1] = m.pat->cols;
+        info[2] = m.pat->nnz;
+    });
+}
+int eqhip_opmat_copy(void* rt, int which, int* rp, int* ci, double* v) {
+    return guard([&] {
+        const DevCsr& m = opmat(R(rt), which);
+        std::memcpy(rp, m.pat->rp.data(), m.pat->rp.size() * sizeof(int));
+        std::memcpy(ci, m.pat->ci.data(), m.pat->ci.size() * sizeof(int));
+        std::vector<double> h = dev_to_host(m.val->p, m.pat->nnz);
+        std::memcpy(v, h.data(), h.size() * sizeof(double));
+    });
+}
+
+// ---------------- index sets
+int eqhip_set_builtin(void* rt, int which, int* h) {
+    return guard([&] {
+        DeviceGrid& g = R(rt)->core->grid();
+        const IP* s[] = {&g.all_cells, &g.boundary_cells, &g.interior_cells,
+                         &g.all_faces, &g.boundary_faces, &g.interior_faces};
+        EQ_REQUIRE(which >= 0 && which < 6, "unknown builtin set " << which);
+        *h = Rt::put(R(rt)->sets, *s[which]);
+    });
+}
+int eqhip_set_from_host(void* rt, int codim, const int* idx, int n, int* h) {
+    return guard([&] { *h = Rt::put(R(rt)->sets, make_index(codim, false, std::vector<int>(idx, idx + n))); });
+}
+int eqhip_set_size(void* rt, int h, int* n) {
+    return guard([&] { *n = R(rt)->I(h)->size(); });
+}
+int eqhip_set_copy(void* rt, int h, int* out) {
+    return guard([&] {
+        const IP& s = R(rt)->I(h);
+        std::vector<int> d = dev_to_host(s->dev->p, s->size());   // device copy is the source of truth
+        std::memcpy(out, d.data(), d.size() * sizeof(int));
+    });
+}
+int eqhip_first_cell(void* rt, int faces, int* h) {
+    return guard([&] { *h = Rt::put(R(rt)->sets, R(rt)->core->firstCell(R(rt)->I(faces))); });
+}
+int eqhip_second_cell(void* rt, int faces, int* h) {
+    return guard([&] { *h = Rt::put(R(rt)->sets, R(rt)->core->secondCell(R(rt)->I(faces))); });
+}
+int eqhip_set_on(void* rt, int data, int from, int to, int* h) {
+    return guard([&] {
+        Rt* r = R(rt);
+        *h = Rt::put(r->sets, i_on(r->I(data), r->I(from), r->I(to)));
+    });
+}
+int eqhip_set_trinary(void* rt, int pred, int t, int f, int* h) {
+    return guard([&] {
+        Rt* r = R(rt);
+        *h = Rt::put(r->sets, i_select(r->B(pred), r->I(t), r->I(f)));
+    });
+}
+int eqhip_is_empty(void* rt, int set, int* pred) {
+    return guard([&] { *pred = Rt::put(R(rt)->bools, i_is_empty(R(rt)->I(set))); });
+}
+int eqhip_set_contains(void* rt, int super, int sub) {
+    return guard([&] { position_map(R(rt)->I(super), R(rt)->I(sub)); });
+}
+
+// ---------------- booleans
+int eqhip_bool_size(void* rt, int h, int* n) {
+    return guard([&] { *n = R(rt)->B(h)->n; });
+}
+int eqhip_bool_copy(void* rt, int h, unsigned char* out) {
+    return guard([&] {
+        std::vector<uint8_t> b = b_to_host(R(rt)->B(h));
+        std::memcpy(out, b.data(), b.size());
+    });
+}
+int eqhip_compare(void* rt, int op, int a, int b, int* h) {
+    return guard([&] { *h = Rt::put(R(rt)->bools, s_compare(op, R(rt)->S(a), R(rt)->S(b))); });
+}
+int eqhip_compare_scalar(void* rt, int op, int a, double s, int scalar_left, int* h) {
+    return guard([&] { *h = Rt::put(R(rt)->bools, s_compare_scalar(op, R(rt)->S(a), s, scalar_left)); });
+}
+
+// ---------------- scalars
+int eqhip_scalar_from_host(void* rt, const double* v, int n, int* h) {
+    return guard([&] { *h = Rt::put(R(rt)->scal, s_from_host(v, n)); });
+}
+int eqhip_scalar_uniform(void* rt, int n, double s, int* h) {
+    return guard([&] { *h = Rt::put(R(rt)->scal, s_uniform(n, s)); });
+}
+int eqhip_scalar_with_jacobian(void* rt, const double* v, int n, const int* rp, const int* ci, const double* jv,
+                               int cols, int* h) {
+    return guard([&] { *h = Rt::put(R(rt)->scal, s_with_jacobian(v, n, rp, ci, jv, cols)); });
+}
+int eqhip_primary(void* rt, int a, int* h) {
+    return guard([&] { *h = Rt::put(R(rt)->scal, s_primary(R(rt)->S(a))); });
+}
+int eqhip_scalar_info(void* rt, int h, int* info) {
+    return guard([&] {
+        const SP& a = R(rt)->S(h);
+        info[0] = a->n;
+        info[1] = a->ad ? 1 : 0;
+        info[2] = a->pat ? a->pat->rows : 0;
+        info[3] = a->pat ? a->pat->cols : 0;
+        info[4] = a->pat ? a->pat->nnz : 0;
+    });
+}
+int eqhip_scalar_copy(void* rt, int h, double* out) {
+    return guard([&] {
+        std::vector<double> v = s_to_host(R(rt)->S(h));
+        std::memcpy(out, v.data(), v.size() * sizeof(double));
+    });
+}
+int eqhip_jacobian_copy(void* rt, int h, int* rp, int* ci, double* v) {
+    return guard([&] {
+        const SP& a = R(rt)->S(h);
+        EQ_REQUIRE(a->pat, "Trying to copy empty matrix to host");
+        std::memcpy(rp, a->pat->rp.data(), a->pat->rp.size() * sizeof(int));
+        std::memcpy(ci, a->pat->ci.data(), a->pat->ci.size() * sizeof(int));
+        std::vector<double> j = dev_to_host(a->jval->p, a->pat->nnz);
+        std::memcpy(v, j.data(), j.size() * sizeof(double));
+    });
+}
+int eqhip_release(void* rt, int kind, int h) {
+    return guard([&] {
+        Rt* r = R(rt);
+        if (kind == 0) {
+            r->S(h);
+            r->scal[h - 1].reset();
+        } else if (kind == 1) {
+            r->I(h);
+            r->sets[h - 1].reset();
+        } else {
+            r->B(h);
+            r->bools[h - 1].reset();
+        }
+    });
+}
+int eqhip_binop(void* rt, int op, int a, int b, int* h) {
+    return guard([&] {
+        Rt* r = R(rt);
+        const SP &x = r->S(a), &y = r->S(b);
+        SP o = op == 0 ? s_add(x, y) : op == 1 ? s_sub(x, y) : op == 2 ? s_mul(x, y) : s_div(x, y);
+        *h = Rt::put(r->scal, o);
+    });
+}
+int eqhip_scalar_binop(void* rt, int op, double s, int a, int scalar_left, int* h) {
+    return guard([&] {
+        Rt* r = R(rt);
+        const SP& x = r->S(a);
+        SP o;
+        if (op == 2) o = s_scal_mul(s, x);
+        else if (op == 3 && !scalar_left) o = s_div_scal(x, s);
+        else if (op == 3) o = s_scal_div(s, x);
+        else throw std::runtime_error("scalar_binop: unsupported op");
+        *h = Rt::put(r->scal, o);
+    });
+}
+int eqhip_negate(void* rt, int a, int* h) {
+    return guard([&] { *h = Rt::put(R(rt)->scal, s_neg(R(rt)->S(a))); });
+}
+int eqhip_sqrt(void* rt, int a, int* h) {
+    return guard([&] { *h = Rt::put(R(rt)->scal, s_sqrt(R(rt)->S(a))); });
+}
+int eqhip_trinary(void* rt, int pred, int t, int f, int* h) {
+    return guard([&] {
+        Rt* r = R(rt);
+        *h = Rt::put(r->scal, s_select(r->B(pred), r->S(t), r->S(f)));
+    });
+}
+int eqhip_gradient(void* rt, int a, int* h) {
+    return guard([&] { *h = Rt::put(R(rt)->scal, R(rt)->core->gradient(R(rt)->S(a))); });
+}
+int eqhip_divergence(void* rt, int a, int* h) {
+    return guard([&] { *h = Rt::put(R(rt)->scal, R(rt)->core->divergence(R(rt)->S(a))); });
+}
+int eqhip_extend(void* rt, int a, int from, int to, int* h) {
+    return guard([&] {
+        Rt* r = R(rt);
+        *h = Rt::put(r->scal, s_extend(r->S(a), r->I(from), r->I(to)));
+    });
+}
+int eqhip_extend_scalar(void* rt, double s, int to, int* h) {
+    return guard([&] { *h = Rt::put(R(rt)->scal, s_uniform(R(rt)->I(to)->size(), s)); });
+}
+int eqhip_on(void* rt, int a, int from, int to, int* h) {
+    return guard([&] {
+        Rt* r = R(rt);
+        *h = Rt::put(r->scal, s_on(r->S(a), r->I(from), r->I(to)));
+    });
+}
+int eqhip_reduce(void* rt, int kind, int a, double* out) {
+    return guard([&] {
+        const SP& x = R(rt)->S(a);
+        if (kind == 4) *out = s_two_norm(x);
+        else *out = s_reduce(kind == 0 ? RED_SUM : kind == 1 ? RED_PROD : kind == 2 ? RED_MAX : RED_MIN, x);
+    });
+}
+int eqhip_norm_set(void* rt, int set, int* h) {
+    return guard([&] { *h = Rt::put(R(rt)->scal, R(rt)->core->norm(R(rt)->I(set))); });
+}
+int eqhip_centroid(void* rt, int set, double* out) {
+    return guard([&] {
+        std::vector<double> v = v_to_host(R(rt)->core->centroid(R(rt)->I(set)));
+        std::memcpy(out, v.data(), v.size() * sizeof(double));
+    });
+}
+int eqhip_normal(void* rt, int set, double* out) {
+    return guard([&] {
+        std::vector<double> v = v_to_host(R(rt)->core->normal(R(rt)->I(set)));
+        std::memcpy(out, v.data(), v.size() * sizeof(double));
+    });
+}
+int eqhip_solve_for_update(void* rt, int residual, int* du, int* iters) {
+    return guard([&] {
+        LinReport rep;
+        SP d = R(rt)->core->solve_for_update(R(rt)->S(residual), &rep);
+        if (iters) *iters = rep.iterations;
+        *du = Rt::put(R(rt)->scal, d);
+    });
+}
+
+// ---------------- CSR algebra
+int eqhip_csr_binop(int op, int ar, int ac, const int* arp, const int* aci, const double* av, int br, int bc,
+                    const int* brp, const int* bci, const double* bv, int* out_info, int* orp, int* oci, double* ov,
+                    int cap) {
+    return guard([&] {
+        DevCsr A, B;
+        if (arp) A = csr_from_host(ar, ac, arp, aci, av);
+        if (brp) B = csr_from_host(br, bc, brp, bci, bv);
+        DevCsr C;
+        if (op == 2) {
+            if (A.pat && B.pat) EQ_REQUIRE(ac == br, "Error in CudaMatrix * CudaMatrix size checking");
+            C = csr_mul(A, B);
+        } else {
+            if (A.pat && B.pat) EQ_REQUIRE(ar == br && ac == bc, "CudaMatrix +/-: matrices of different sizes");
+            C = csr_add(A, 1.0, B, op == 1 ? -1.0 : 1.0);
+        }
+        out_info[0] = C.pat ? C.pat->rows : 0;
+        out_info[1] = C.pat ? C.pat->cols : 0;
+        out_info[2] = C.pat ? C.pat->nnz : 0;
+        if (!C.pat) return;
+        EQ_REQUIRE(C.pat->nnz <= cap, "csr_binop: output capacity too small");
+        std::memcpy(orp, C.pat->rp.data(), C.pat->rp.size() * sizeof(int));
+        std::memcpy(oci, C.pat->ci.data(), C.pat->ci.size() * sizeof(int));
+        std::vector<double> v = dev_to_host(C.val->p, C.pat->nnz);
+        std::memcpy(ov, v.data(), v.size() * sizeof(double));
+    });
+}
+int eqhip_csr_spmv(int rows, int cols, const int* rp, const int* ci, const double* v, const double* x, double* y,
+                   int transpose) {
+    return guard([&] {
+        DevCsr A = csr_from_host(rows, cols, rp, ci, v);
+        if (transpose) {
+            // structural transpose (symbolic on host), values gathered on the device
+            const Pattern& p = *A.pat;
+            std::vector<int> trp(p.cols + 1, 0), tci(p.nnz), src(p.nnz);
+            for (int c : p.ci) trp[c + 1]++;
+            for (int i = 0; i < p.cols; ++i) trp[i + 1] += trp[i];
+            std::vector<int> pos(trp.begin(), trp.end() - 1);
+            for (int r = 0; r < p.rows; ++r)
+                for (int k = p.rp[r]; k < p.rp[r + 1]; ++k) {
+                    const int q = pos[p.ci[k]]++;
+                    tci[q] = r;
+                    src[q] = k;
+                }
+            DevCsr T;
+            T.pat = make_pattern(p.cols, p.rows, std::move(trp), std::move(tci));
+            T.val = dev_alloc<double>(p.nnz);
+            auto dsrc = dev_from_host(src.data(), src.size());
+            launch_gather(A.val->p, dsrc->p, T.val->p, p.nnz);
+            A = T;
+        }
+        SP xs = s_from_host(x, A.pat->cols);
+        SP ys = s_matmul(A, xs);
+        std::vector<double> h = s_to_host(ys);
+        std::memcpy(y, h.data(), h.size() * sizeof(double));
+    });
+}
+int eqhip_sell_spmv(void* rt, int h, const double* x, double* y, int fp32) {
+    return guard([&] { sell_spmv_test(R(rt)->S(h), x, y, fp32 != 0); });
+}
+
+// ---------------- heat program
+int eqhip_heat_run(const char* params, const double* u_initial, int n, const int* dir_faces, const double* dir_vals,
+                   int ndir, const double* dts, int nsteps, double* u_out, int* newton_iters, double* final_norms) {
+    return guard([&] {
+        Opm::ParameterGroup param = to_group(params);
+        equelleCUDA::EquelleRuntimeCUDA er(param);
+        HeatProgramCUDA prog(er, std::vector<int>(dir_faces, dir_faces + ndir),
+                             std::vector<double>(dir_vals, dir_vals + ndir));
+        equelleCUDA::CollOfScalar u0(std::vector<double>(u_initial, u_initial + n));
+        for (int s = 0; s < nsteps; ++s) {
+            const double dt = dts[s];
+            auto rescomp = [&](const equelleCUDA::CollOfScalar& u) { return prog.residual(u, u0, dt); };
+            equelleCUDA::CollOfScalar u = er.newtonSolve(rescomp, u0);
+            if (newton_iters) newton_iters[s] = er.lastNewtonReport().iterations;
+            if (final_norms) final_norms[s] = er.lastNewtonReport().final_norm;
+            u0 = u;
+        }
+        std::vector<double> h = u0.copyToHost();
+        std::memcpy(u_out, h.data(), h.size() * sizeof(double));
+    });
+}
+
+int eqhip_heat_residual(const char* params, const double* u, const double* u0, int n, const int* dir_faces,
+                        const double* dir_vals, int ndir, double dt, int* info, double* F, int* rp, int* ci,
+                        double* jv) {
+    return guard([&] {
+        Opm::ParameterGroup param = to_group(params);
+        equelleCUDA::EquelleRuntimeCUDA er(param);
+        HeatProgramCUDA prog(er, std::vector<int>(dir_faces, dir_faces + ndir),
+                             std::vector<double>(dir_vals, dir_vals + ndir));
+        equelleCUDA::CollOfScalar U(equelleCUDA::CollOfScalar(std::vector<double>(u, u + n)), true);
+        equelleCUDA::CollOfScalar U0(std::vector<double>(u0, u0 + n));
+        equelleCUDA::CollOfScalar r = prog.residual(U, U0, dt);
+        info[0] = r.size();
+        info[1] = r.data()->pat ? r.data()->pat->nnz : 0;
+        if (!F) return;
+        std::vector<double> fv = r.copyToHost();
+        std::memcpy(F, fv.data(), fv.size() * sizeof(double));
+        equelleCUDA::hostMat m = r.matrixToHost();
+        std::memcpy(rp, m.rowPtr.data(), m.rowPtr.size() * sizeof(int));
+        std::memcpy(ci, m.colInd.data(), m.colInd.size() * sizeof(int));
+        std::memcpy(jv, m.vals.data(), m.vals.size() * sizeof(double));
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,542 @@
+// Forward-mode sparse AD engine: symbolic phase (host, cached per pattern) and
+// numeric phase (HIP kernels).  See engine.hpp.
+#include "engine.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "kernels.hpp"
+
+namespace eqhip {
+
+namespace {
+
+SP make_scalar(int n, DevPtr<double> val, bool ad, PatternP pat, DevPtr<double> jval) {
+    auto s = std::make_shared<ScalarData>();
+    s->n = n;
+    s->val = std::move(val);
+    s->ad = ad;
+    s->pat = std::move(pat);
+    s->jval = std::move(jval);
+    return s;
+}
+
+void check_size(const SP& a, const SP& b, const char* op) {
+    EQ_REQUIRE(a->n == b->n, "CollOfScalar sizes differ in operator" << op << " (" << a->n << " vs " << b->n << ")");
+}
+
+// Union of two patterns with maps from output entries to operand entries.
+SymP sym_union(const PatternP& a, const PatternP& b) {
+    const std::string key = sym_key("union", a->id, b->id);
+    if (SymP e = sym_lookup(key)) return e;
+    EQ_REQUIRE(a->rows == b->rows && a->cols == b->cols,
+               "Jacobians of different sizes (" << a->rows << "x" << a->cols << " vs " << b->rows << "x" << b->cols
+                                                << ")");
+    std::vector<int> rp(a->rows + 1, 0), ci, ia, ib;
+    ci.reserve(std::max(a->nnz, b->nnz));
+    for (int r = 0; r < a->rows; ++r) {
+        int ka = a->rp[r], ea = a->rp[r + 1], kb = b->rp[r], eb = b->rp[r + 1];
+        while (ka < ea || kb < eb) {
+            if (kb >= eb || (ka < ea && a->ci[ka] < b->ci[kb])) {
+                ci.push_back(a->ci[ka]);
+                ia.push_back(ka++);
+                ib.push_back(-1);
+            } else if (ka >= ea || b->ci[kb] < a->ci[ka]) {
+                ci.push_back(b->ci[kb]);
+                ia.push_back(-1);
+                ib.push_back(kb++);
+            } else {
+                ci.push_back(a->ci[ka]);
+                ia.push_back(ka++);
+                ib.push_back(kb++);
+            }
+        }
+        rp[r + 1] = (int)ci.size();
+    }
+    auto e = std::make_shared<SymEntry>();
+    e->out = make_pattern(a->rows, a->cols, std::move(rp), std::move(ci));
+    e->ia = dev_from_host(ia.data(), ia.size());
+    e->ib = dev_from_host(ib.data(), ib.size());
+    e->inputs = {a, b};
+    sym_store(key, e);
+    return e;
+}
+
+// Structural product M * A: output pattern + (matrix entry, operand entry) pairs.
+SymP sym_spgemm(const PatternP& m, const PatternP& a) {
+    const std::string key = sym_key("spgemm", m->id, a->id);
+    if (SymP e = sym_lookup(key)) return e;
+    EQ_REQUIRE(m->cols == a->rows, "Error in CudaMatrix * CudaMatrix size checking: lhs.cols_ = "
+                                       << m->cols << " rhs.rows_ = " << a->rows);
+    std::vector<int> rp(m->rows + 1, 0), ci, seg(1, 0), mi, ai;
+    std::vector<int> mark(a->cols, -1), slot(a->cols, 0), cols;
+    std::vector<std::vector<std::pair<int, int>>> acc;
+    for (int r = 0; r < m->rows; ++r) {
+        cols.clear();
+        acc.clear();
+        for (int t = m->rp[r]; t < m->rp[r + 1]; ++t) {
+            const int k = m->ci[t];
+            for (int e = a->rp[k]; e < a->rp[k + 1]; ++e) {
+                const int c = a->ci[e];
+                if (mark[c] != r) {
+                    mark[c] = r;
+                    slot[c] = (int)cols.size();
+                    cols.push_back(c);
+                    acc.emplace_back();
+                }
+                acc[slot[c]].emplace_back(t, e);
+            }
+        }
+        std::vector<int> order(cols.size());
+        for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
+        std::sort(order.begin(), order.end(), [&](int x, int y) { return cols[x] < cols[y]; });
+        for (int o : order) {
+            ci.push_back(cols[o]);
+            for (auto& pr : acc[o]) {
+                mi.push_back(pr.first);
+                ai.push_back(pr.second);
+            }
+            seg.push_back((int)mi.size());
+        }
+        rp[r + 1] = (int)ci.size();
+    }
+    auto e = std::make_shared<SymEntry>();
+    e->out = make_pattern(m->rows, a->cols, std::move(rp), std::move(ci));
+    e->seg = dev_from_host(seg.data(), seg.size());
+    e->mi = dev_from_host(mi.data(), mi.size());
+    e->ai = dev_from_host(ai.data(), ai.size());
+    e->inputs = {m, a};
+    sym_store(key, e);
+    return e;
+}
+
+// Row gather of a pattern: output row i = input row map[i] (-1 => empty row).
+SymP sym_rows(const PatternP& a, const std::vector<int>& map, uint64_t map_id, int out_rows) {
+    const std::string key = sym_key("rows", a->id, map_id);
+    if (SymP e = sym_lookup(key)) return e;
+    std::vector<int> rp(out_rows + 1, 0), ci, src;
+    for (int i = 0; i < out_rows; ++i) {
+        const int r = map[i];
+        if (r >= 0)
+            for (int k = a->rp[r]; k < a->rp[r + 1]; ++k) {
+                ci.push_back(a->ci[k]);
+                src.push_back(k);
+            }
+        rp[i + 1] = (int)ci.size();
+    }
+    auto e = std::make_shared<SymEntry>();
+    e->out = make_pattern(out_rows, a->cols, std::move(rp), std::move(ci));
+    e->ia = dev_from_host(src.data(), src.size());
+    e->inputs = {a};
+    sym_store(key, e);
+    return e;
+}
+
+struct JacOut {
+    PatternP pat;
+    DevPtr<double> val;
+};
+
+// out = C(r) * (alpha*A(r)*Ja + beta*B(r)*Jb) with the empty-matrix rules of the
+// reference (cudaMatrix.cu:646-694): empty operands drop out.
+JacOut jac_lincomb(const SP& a, const double* A, double alpha, const SP& b, const double* B, double beta,
+                   const double* C) {
+    const PatternP pa = a ? a->pat : nullptr;
+    const PatternP pb = b ? b->pat : nullptr;
+    JacOut o;
+    if (!pa && !pb) return o;
+    if (pa && pb && pa.get() != pb.get()) {
+        SymP e = sym_union(pa, pb);
+        o.pat = e->out;
+        o.val = dev_alloc<double>(o.pat->nnz);
+        launch_jlin(o.pat->nnz, (A || B || C) ? o.pat->row_of() : nullptr, a->jval->p, e->ia->p, A, alpha,
+                    b->jval->p, e->ib->p, B, beta, C, o.val->p);
+        return o;
+    }
+    o.pat = pa ? pa : pb;
+    o.val = dev_alloc<double>(o.pat->nnz);
+    launch_jlin(o.pat->nnz, (A || B || C) ? o.pat->row_of() : nullptr, pa ? a->jval->p : nullptr, nullptr, A, alpha,
+                pb ? b->jval->p : nullptr, nullptr, B, beta, C, o.val->p);
+    return o;
+}
+
+SP binary_values(int op, const SP& a, const SP& b) {
+    auto v = dev_alloc<double>(a->n);
+    launch_ew(op, a->val->p, b->val->p, v->p, a->n);
+    return make_scalar(a->n, v, false, nullptr, nullptr);
+}
+
+}  // namespace
+
+// ===========================================================================
+SP s_from_host(const double* h, int n) { return make_scalar(n, dev_from_host(h, n), false, nullptr, nullptr); }
+
+SP s_uniform(int n, double v) {
+    auto d = dev_alloc<double>(n);
+    launch_fill(d->p, v, n);
+    return make_scalar(n, d, false, nullptr, nullptr);
+}
+
+SP s_from_device(DevPtr<double> v, int n) { return make_scalar(n, std::move(v), false, nullptr, nullptr); }
+
+SP s_primary(const SP& a) {
+    PatternP p = identity_pattern(a->n);
+    auto j = dev_alloc<double>(a->n);
+    launch_fill(j->p, 1.0, a->n);
+    return make_scalar(a->n, a->val, true, p, j);
+}
+
+SP s_with_jacobian(const double* v, int n, const int* rp, const int* ci, const double* jv, int cols) {
+    const int nnz = rp[n];
+    PatternP p = make_pattern(n, cols, std::vector<int>(rp, rp + n + 1), std::vector<int>(ci, ci + nnz));
+    return make_scalar(n, dev_from_host(v, n), true, p, dev_from_host(jv, nnz));
+}
+
+SP s_value_only(const SP& a) { return make_scalar(a->n, a->val, false, nullptr, nullptr); }
+
+SP s_add(const SP& a, const SP& b) {
+    check_size(a, b, "+");
+    SP v = binary_values(EW_ADD, a, b);
+    if (!(a->ad || b->ad)) return v;
+    JacOut j = jac_lincomb(a, nullptr, 1.0, b, nullptr, 1.0, nullptr);
+    return make_scalar(a->n, v->val, true, j.pat, j.val);
+}
+
+SP s_sub(const SP& a, const SP& b) {
+    check_size(a, b, "-");
+    SP v = binary_values(EW_SUB, a, b);
+    if (!(a->ad || b->ad)) return v;
+    JacOut j = jac_lincomb(a, nullptr, 1.0, b, nullptr, -1.0, nullptr);
+    return make_scalar(a->n, v->val, true, j.pat, j.val);
+}
+
+SP s_mul(const SP& a, const SP& b) {
+    check_size(a, b, "*");
+    SP v = binary_values(EW_MUL, a, b);
+    if (!(a->ad || b->ad)) return v;
+    // (uv)' = diag(v) u' + diag(u) v'
+    JacOut j = jac_lincomb(a, b->val->p, 1.0, b, a->val->p, 1.0, nullptr);
+    return make_scalar(a->n, v->val, true, j.pat, j.val);
+}
+
+SP s_div(const SP& a, const SP& b) {
+    check_size(a, b, "/");
+    SP v = binary_values(EW_DIV, a, b);
+    if (!(a->ad || b->ad)) return v;
+    // (u/v)' = diag(1/v^2) (diag(v) u' - diag(u) v')
+    auto c = dev_alloc<double>(a->n);
+    launch_coef(4, nullptr, b->val->p, 0.0, c->p, a->n);
+    JacOut j = jac_lincomb(a, b->val->p, 1.0, b, a->val->p, -1.0, c->p);
+    return make_scalar(a->n, v->val, true, j.pat, j.val);
+}
+
+SP s_scal_mul(double s, const SP& a) {
+    auto v = dev_alloc<double>(a->n);
+    launch_ew_scalar(EW_MUL, s, a->val->p, v->p, a->n, 1);
+    if (!a->ad) return make_scalar(a->n, v, false, nullptr, nullptr);
+    EQ_REQUIRE(a->pat, "Calling CudaMatrix * Scalar with empty matrix...");
+    JacOut j = jac_lincomb(a, nullptr, s, nullptr, nullptr, 0.0, nullptr);
+    return make_scalar(a->n, v, true, j.pat, j.val);
+}
+
+SP s_div_scal(const SP& a, double s) { return s_scal_mul(1.0 / s, a); }
+
+SP s_scal_div(double s, const SP& a) {
+    auto v = dev_alloc<double>(a->n);
+    launch_ew_scalar(EW_DIV, s, a->val->p, v->p, a->n, 1);
+    if (!a->ad) return make_scalar(a->n, v, false, nullptr, nullptr);
+    // (s/u)' = -diag(s/u^2) u'
+    auto c = dev_alloc<double>(a->n);
+    launch_coef(2, a->val->p, nullptr, s, c->p, a->n);
+    JacOut j = jac_lincomb(a, c->p, 1.0, nullptr, nullptr, 0.0, nullptr);
+    return make_scalar(a->n, v, true, j.pat, j.val);
+}
+
+SP s_neg(const SP& a) { return s_scal_mul(-1.0, a); }
+
+SP s_sqrt(const SP& a) {
+    auto v = dev_alloc<double>(a->n);
+    launch_sqrt(a->val->p, v->p, a->n);
+    if (!a->ad) return make_scalar(a->n, v, false, nullptr, nullptr);
+    auto c = dev_alloc<double>(a->n);
+    launch_coef(3, v->p, nullptr, 0.0, c->p, a->n);   // 1/(2 sqrt(x))
+    JacOut j = jac_lincomb(a, c->p, 1.0, nullptr, nullptr, 0.0, nullptr);
+    return make_scalar(a->n, v, true, j.pat, j.val);
+}
+
+BP s_compare(int op, const SP& a, const SP& b) {
+    check_size(a, b, " comparison");
+    auto o = std::make_shared<BoolData>();
+    o->n = a->n;
+    o->val = dev_alloc<uint8_t>(a->n);
+    launch_compare(op, a->val->p, b->val->p, 0.0, 0, o->val->p, a->n);
+    return o;
+}
+
+BP s_compare_scalar(int op, const SP& a, double s, int scalar_left) {
+    auto o = std::make_shared<BoolData>();
+    o->n = a->n;
+    o->val = dev_alloc<uint8_t>(a->n);
+    launch_compare(op, a->val->p, nullptr, s, scalar_left, o->val->p, a->n);
+    return o;
+}
+
+SP s_select(const BP& p, const SP& t, const SP& f) {
+    EQ_REQUIRE(p->n == t->n && p->n == f->n, "Collections are not of the same size");
+    auto v = dev_alloc<double>(t->n);
+    launch_select(p->val->p, t->val->p, f->val->p, v->p, t->n);
+    if (!(t->ad || f->ad)) return make_scalar(t->n, v, false, nullptr, nullptr);
+    // D(p) A' + (I - D(p)) B': union of both patterns regardless of p, zeros kept.
+    const PatternP pa = t->pat, pb = f->pat;
+    if (!pa && !pb) return make_scalar(t->n, v, true, nullptr, nullptr);
+    PatternP out;
+    const int *ia = nullptr, *ib = nullptr;
+    SymP e;
+    if (pa && pb && pa.get() != pb.get()) {
+        e = sym_union(pa, pb);
+        out = e->out;
+        ia = e->ia->p;
+        ib = e->ib->p;
+    } else {
+        out = pa ? pa : pb;
+    }
+    auto j = dev_alloc<double>(out->nnz);
+    launch_jselect(out->nnz, out->row_of(), p->val->p, pa ? t->jval->p : nullptr, ia, pb ? f->jval->p : nullptr, ib,
+                   j->p);
+    return make_scalar(t->n, v, true, out, j);
+}
+
+double s_reduce(int op, const SP& a) {
+    EQ_REQUIRE(a->n > 0 || op == RED_SUM || op == RED_PROD, "reduction of an empty collection");
+    return reduce(op, a->val->p, a->n);
+}
+
+double s_two_norm(const SP& a) { return std::sqrt(reduce(RED_SUMSQ, a->val->p, a->n)); }
+
+std::vector<double> s_to_host(const SP& a) { return dev_to_host(a->val->p, a->n); }
+
+// ---------------------------------------------------------------- matrices
+DevCsr csr_from_host(int rows, int cols, const int* rp, const int* ci, const double* v) {
+    DevCsr m;
+    m.pat = make_pattern(rows, cols, std::vector<int>(rp, rp + rows + 1), std::vector<int>(ci, ci + rp[rows]));
+    m.val = dev_from_host(v, rp[rows]);
+    return m;
+}
+
+SP s_matmul(const DevCsr& m, const SP& a, DevPtr<double> values) {
+    EQ_REQUIRE(m.pat->cols == a->n, "Error in matrix * vector operation as matrix is of size "
+                                        << m.pat->rows << " by " << m.pat->cols << " and the vector of size " << a->n);
+    DevPtr<double> v = values;
+    if (!v) {
+        v = dev_alloc<double>(m.pat->rows);
+        launch_csr_spmv(m.pat->rows, m.pat->d_rp->p, m.pat->d_ci->p, m.val->p, a->val->p, v->p);
+    }
+    if (!a->ad || !a->pat) return make_scalar(m.pat->rows, v, a->ad, nullptr, nullptr);
+    SymP e = sym_spgemm(m.pat, a->pat);
+    auto j = dev_alloc<double>(e->out->nnz);
+    launch_spgemm_num(e->out->nnz, e->seg->p, e->mi->p, e->ai->p, m.val->p, a->jval->p, j->p);
+    return make_scalar(m.pat->rows, v, true, e->out, j);
+}
+
+DevCsr csr_add(const DevCsr& a, double alpha, const DevCsr& b, double beta) {
+    if (!a.pat && !b.pat) return DevCsr();
+    DevCsr o;
+    if (!a.pat || !b.pat || a.pat.get() == b.pat.get()) {
+        o.pat = a.pat ? a.pat : b.pat;
+        o.val = dev_alloc<double>(o.pat->nnz);
+        launch_jlin(o.pat->nnz, nullptr, a.pat ? a.val->p : nullptr, nullptr, nullptr, alpha,
+                    b.pat ? b.val->p : nullptr, nullptr, nullptr, beta, nullptr, o.val->p);
+        return o;
+    }
+    SymP e = sym_union(a.pat, b.pat);
+    o.pat = e->out;
+    o.val = dev_alloc<double>(o.pat->nnz);
+    launch_jlin(o.pat->nnz, nullptr, a.val->p, e->ia->p, nullptr, alpha, b.val->p, e->ib->p, nullptr, beta, nullptr,
+                o.val->p);
+    return o;
+}
+
+DevCsr csr_mul(const DevCsr& a, const DevCsr& b) {
+    if (!a.pat || !b.pat) return DevCsr();
+    SymP e = sym_spgemm(a.pat, b.pat);
+    DevCsr o;
+    o.pat = e->out;
+    o.val = dev_alloc<double>(o.pat->nnz);
+    launch_spgemm_num(o.pat->nnz, e->seg->p, e->mi->p, e->ai->p, a.val->p, b.val->p, o.val->p);
+    return o;
+}
+
+// ---------------------------------------------------------------- index sets
+PMP position_map(const IP& super, const IP& sub) {
+    static std::mutex mu;
+    static std::map<std::pair<uint64_t, uint64_t>, std::weak_ptr<const PosMap>> cache;
+    const auto key = std::make_pair(super->id, sub->id);
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = cache.find(key);
+        if (it != cache.end())
+            if (PMP p = it->second.lock()) return p;
+    }
+    // subsetIndices (EquelleRuntimeCPU_impl.hpp:28-83): superset sorted and unique.
+    const std::vector<int>& sup = super->host;
+    const std::vector<int>& s = sub->host;
+    std::vector<int> pos(s.size(), -1);
+    if (!s.empty()) {
+        std::vector<std::pair<int, int>> si(s.size());
+        for (size_t i = 0; i < s.size(); ++i) si[i] = {s[i], (int)i};
+        std::sort(si.begin(), si.end());
+        size_t k = 0;
+        for (size_t i = 0; i < sup.size() && k < si.size(); ++i)
+            while (k < si.size() && si[k].first == sup[i]) pos[si[k++].second] = (int)i;
+        EQ_REQUIRE(k == si.size(), "set of size " << s.size() << " is not a subset of the given superset (size "
+                                                  << sup.size() << ")");
+    }
+    auto p = std::make_shared<PosMap>();
+    p->id = next_id();
+    p->super_size = (int)sup.size();
+    p->d_pos = dev_from_host(pos.data(), pos.size());
+    std::vector<int> inv(sup.size(), -1);
+    for (size_t i = 0; i < pos.size(); ++i) inv[pos[i]] = (int)i;
+    p->d_inv = dev_from_host(inv.data(), inv.size());
+    p->pos = std::move(pos);
+    std::lock_guard<std::mutex> lk(mu);
+    cache[key] = p;
+    // keep the maps alive as long as both sets live
+    static std::vector<std::pair<std::weak_ptr<const IndexData>, PMP>> owners;
+    owners.emplace_back(super, p);
+    owners.emplace_back(sub, p);
+    if (owners.size() > 8192)
+        owners.erase(std::remove_if(owners.begin(), owners.end(), [](auto& x) { return x.first.expired(); }),
+                     owners.end());
+    return p;
+}
+
+SP s_on(const SP& a, const IP& from, const IP& to) {
+    EQ_REQUIRE(a->n == from->size(), "data_in (size " << a->n << ") and from_set (size " << from->size()
+                                                      << ") have to be of the same size in On function.");
+    PMP pm = position_map(from, to);
+    const int m = to->size();
+    auto v = dev_alloc<double>(m);
+    launch_gather(a->val->p, pm->d_pos->p, v->p, m);
+    if (!a->ad || !a->pat) return make_scalar(m, v, a->ad, nullptr, nullptr);
+    SymP e = sym_rows(a->pat, pm->pos, pm->id, m);
+    auto j = dev_alloc<double>(e->out->nnz);
+    launch_gather(a->jval->p, e->ia->p, j->p, e->out->nnz);
+    return make_scalar(m, v, true, e->out, j);
+}
+
+SP s_extend(const SP& a, const IP& from, const IP& to) {
+    EQ_REQUIRE(a->n == from->size(), "data_in (size " << a->n << ") and from_set (size " << from->size()
+                                                      << ") have to be of the same size in Extend function.");
+    EQ_REQUIRE(from->size() <= to->size(), "From_set (size " << from->size() << ") has to be a subset of to_set (size "
+                                                             << to->size() << ")");
+    PMP pm = position_map(to, from);
+    const int m = to->size();
+    auto v = dev_alloc<double>(m);
+    launch_gather(a->val->p, pm->d_inv->p, v->p, m);
+    if (!a->ad || !a->pat) return make_scalar(m, v, a->ad, nullptr, nullptr);
+    std::vector<int> inv(m, -1);
+    for (size_t i = 0; i < pm->pos.size(); ++i) inv[pm->pos[i]] = (int)i;
+    SymP e = sym_rows(a->pat, inv, pm->id ^ 0x8000000000000000ull, m);
+    auto j = dev_alloc<double>(e->out->nnz);
+    launch_gather(a->jval->p, e->ia->p, j->p, e->out->nnz);
+    return make_scalar(m, v, true, e->out, j);
+}
+
+IP i_from_device(int codim, DevPtr<int> d, int n) {
+    auto out = std::make_shared<IndexData>();
+    out->id = next_id();
+    out->codim = codim;
+    out->host = dev_to_host(d->p, n);
+    out->dev = std::move(d);
+    return out;
+}
+
+IP i_on(const IP& data, const IP& from, const IP& to) {
+    EQ_REQUIRE(data->size() == from->size(), "data_in(size " << data->size() << ") and from_set (size "
+                                                             << from->size()
+                                                             << ") have to be of the same size in On function for "
+                                                                "CollOfIndices.");
+    PMP pm = position_map(from, to);
+    auto d = dev_alloc<int>(to->size());
+    launch_gather_int(data->dev->p, pm->d_pos->p, d->p, to->size());
+    return i_from_device(data->codim, d, to->size());
+}
+
+IP i_select(const BP& p, const IP& t, const IP& f) {
+    EQ_REQUIRE(p->n == t->size() && p->n == f->size(), "The sets are not of the same size");
+    auto d = dev_alloc<int>(p->n);
+    launch_select_int(p->val->p, t->dev->p, f->dev->p, d->p, p->n);
+    return i_from_device(t->codim, d, p->n);
+}
+
+BP i_is_empty(const IP& s) {
+    auto o = std::make_shared<BoolData>();
+    o->n = s->size();
+    o->val = dev_alloc<uint8_t>(o->n);
+    launch_is_empty(s->dev->p, o->val->p, o->n);
+    return o;
+}
+
+BP b_from_host(const uint8_t* h, int n) {
+    auto o = std::make_shared<BoolData>();
+    o->n = n;
+    o->val = dev_from_host(h, n);
+    return o;
+}
+
+std::vector<uint8_t> b_to_host(const BP& b) { return dev_to_host(b->val->p, b->n); }
+
+// ---------------------------------------------------------------- vectors
+VP v_from_host(const double* h, int n, int dim) {
+    auto v = std::make_shared<VectorData>();
+    v->n = n;
+    v->dim = dim;
+    v->val = dev_from_host(h, (size_t)n * dim);
+    return v;
+}
+
+VP v_binary(int op, const VP& a, const VP& b) {
+    EQ_REQUIRE(a->n == b->n && a->dim == b->dim, "CollOfVector sizes differ");
+    auto v = std::make_shared<VectorData>();
+    v->n = a->n;
+    v->dim = a->dim;
+    v->val = dev_alloc<double>((size_t)a->n * a->dim);
+    launch_vec_ew(op, a->val->p, b->val->p, a->n * a->dim, v->val->p);
+    return v;
+}
+
+VP v_scale(const VP& a, const SP& s, double sc, int divide) {
+    if (s) EQ_REQUIRE(s->n == a->n, "CollOfVector and CollOfScalar sizes differ");
+    auto v = std::make_shared<VectorData>();
+    v->n = a->n;
+    v->dim = a->dim;
+    v->val = dev_alloc<double>((size_t)a->n * a->dim);
+    launch_vec_scale(a->val->p, s ? s->val->p : nullptr, sc, a->n, a->dim, divide, v->val->p);
+    return v;
+}
+
+SP v_norm(const VP& a) {
+    auto d = dev_alloc<double>(a->n);
+    launch_vec_norm(a->val->p, a->n, a->dim, d->p);
+    return make_scalar(a->n, d, false, nullptr, nullptr);
+}
+
+SP v_dot(const VP& a, const VP& b) {
+    EQ_REQUIRE(a->n == b->n && a->dim == b->dim, "Non-matching size of Vector collections for dot().");
+    auto d = dev_alloc<double>(a->n);
+    launch_vec_dot(a->val->p, b->val->p, a->n, a->dim, d->p);
+    return make_scalar(a->n, d, false, nullptr, nullptr);
+}
+
+SP v_col(const VP& a, int col) {
+    EQ_REQUIRE(col >= 0 && col < a->dim, "CollOfVector index " << col << " out of range");
+    auto d = dev_alloc<double>(a->n);
+    launch_vec_col(a->val->p, a->n, a->dim, col, d->p);
+    return make_scalar(a->n, d, false, nullptr, nullptr);
+}
+
+std::vector<double> v_to_host(const VP& a) { return dev_to_host(a->val->p, (size_t)a->n * a->dim); }
+
+}  // namespace eqhip

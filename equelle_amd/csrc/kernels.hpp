@@ -1,0 +1,77 @@
+// Host launchers for the HIP kernels of the AD engine, the grid operators and
+// the reductions.  All launches go to Device::get().stream().
+#pragma once
+#include <cstdint>
+
+namespace eqhip {
+
+enum EwOp { EW_ADD = 0, EW_SUB = 1, EW_MUL = 2, EW_DIV = 3 };
+enum CmpOp { CMP_GT = 0, CMP_GE = 1, CMP_LT = 2, CMP_LE = 3, CMP_EQ = 4, CMP_NE = 5 };
+enum RedOp { RED_SUM = 0, RED_PROD = 1, RED_MAX = 2, RED_MIN = 3, RED_SUMSQ = 4 };
+
+void launch_row_of(const int* rp, int rows, int* row);
+void launch_fill(double* out, double v, long n);
+void launch_fill_int(int* out, int v, long n);
+void launch_ew(int op, const double* a, const double* b, double* out, int n);
+// op EW_MUL: s*a ; EW_DIV: scalar_left ? s/a : a/s
+void launch_ew_scalar(int op, double s, const double* a, double* out, int n, int scalar_left);
+void launch_sqrt(const double* a, double* out, int n);
+// Per-row coefficient helpers for the AD product / quotient rules.
+//   kind 0: out = 1/b            kind 1: out = -a/(b*b)
+//   kind 2: out = -s/(a*a)       kind 3: out = 0.5/a
+//   kind 4: out = 1/(b*b)
+void launch_coef(int kind, const double* a, const double* b, double s, double* out, int n);
+void launch_compare(int op, const double* a, const double* b, double s, int scalar_left, uint8_t* out, int n);
+void launch_select(const uint8_t* p, const double* a, const double* b, double* out, int n);
+void launch_select_int(const uint8_t* p, const int* a, const int* b, int* out, int n);
+void launch_is_empty(const int* idx, uint8_t* out, int n);
+// out[i] = map[i] >= 0 ? in[map[i]] : 0
+void launch_gather(const double* in, const int* map, double* out, long n);
+void launch_gather_int(const int* in, const int* map, int* out, int n);
+
+// Jacobian numeric kernels (one thread per output entry).
+//   out[k] = C(r) * ( alpha*A(r)*a[ia[k]] + beta*B(r)*b[ib[k]] )
+// with r = row[k]; a/b may be null (term absent); ia/ib null => identity map;
+// ia/ib entries -1 => term absent for that entry; A, B, C null => 1.
+void launch_jlin(long nnz, const int* row, const double* a, const int* ia, const double* A, double alpha,
+                 const double* b, const int* ib, const double* B, double beta, const double* C, double* out);
+// out[k] = p[r] ? a[ia[k]] : b[ib[k]]  (missing => 0)
+void launch_jselect(long nnz, const int* row, const uint8_t* p, const double* a, const int* ia, const double* b,
+                    const int* ib, double* out);
+// out[k] = sum_{t in [seg[k], seg[k+1])} mv[mi[t]] * av[ai[t]]
+void launch_spgemm_num(long nnz, const int* seg, const int* mi, const int* ai, const double* mv, const double* av,
+                       double* out);
+
+// y = A x, one thread per row, entries summed in column order.
+void launch_csr_spmv(int rows, const int* rp, const int* ci, const double* v, const double* x, double* y);
+
+// Grid operators on values.
+// gradient: out[i] = u[c2[i]] - u[c1[i]]
+void launch_gradient(const double* u, const int* c1, const int* c2, double* out, int nif);
+// divergence: out[c] = sum_k sgn * flux[f] over the cell's faces; signed_faces[k] = f (cell is
+// first cell => +) or ~f (cell is second cell => -).
+void launch_divergence(const double* flux, const int* facepos, const int* signed_faces, double* out, int nc);
+// face_cells -> interior / boundary flags; cells -> boundary flag
+void launch_face_flags(const int* face_cells, int nf, uint8_t* interior);
+void launch_cell_boundary_flags(const int* face_cells, const int* facepos, const int* cell_faces, int nc,
+                                uint8_t* boundary);
+// geometry gathers: out[i*dim + d] = src[idx[i]*dim + d]
+void launch_gather_vec(const double* src, const int* idx, int n, int dim, double* out);
+void launch_gather_face_cells(const int* face_cells, const int* faces, int n, int which, int* out);
+void launch_vec_norm(const double* v, int n, int dim, double* out);
+void launch_vec_dot(const double* a, const double* b, int n, int dim, double* out);
+void launch_vec_ew(int op, const double* a, const double* b, int count, double* out);
+void launch_vec_scale(const double* v, const double* s, double sc, int n, int dim, int divide, double* out);
+void launch_unit_normals(const double* normals, const int* idx, int n, int dim, double* out);
+void launch_vec_col(const double* v, int n, int dim, int col, double* out);
+
+// Deterministic reductions (fixed grid and order for a given n).
+double reduce(int op, const double* a, long n);
+// Deterministic dot product into a device scalar (no host sync).
+void launch_dot_to(const double* a, const double* b, long n, double* d_out);
+
+// Stream compaction of flagged indices: writes the ascending indices i with
+// flag[i] != 0 (flag == want) into out, returns the count.
+int compact_flags(const uint8_t* flags, int n, int want, int* out);
+
+}  // namespace eqhip

@@ -1,0 +1,794 @@
+// Device Krylov solvers, SELL-64 SpMV, Jacobi and multicolour ILU(0); see krylov.hpp.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <mutex>
+#include <unordered_map>
+
+#include "engine.hpp"
+#include "kernels.hpp"
+#include "krylov.hpp"
+
+namespace eqhip {
+
+namespace {
+constexpr int BS = 256;
+constexpr int W = 64;   // SELL slice height = wavefront size
+inline hipStream_t S() { return Device::get().stream(); }
+__device__ inline long gid() { return (long)blockIdx.x * blockDim.x + threadIdx.x; }
+inline int vec_blocks(long n) {
+    long b = (n + BS * 4 - 1) / (BS * 4);
+    return (int)std::max(1L, std::min(b, 2048L));
+}
+#define KL(kernel, grid, ...)                                                  \
+    do {                                                                       \
+        hipLaunchKernelGGL(kernel, dim3(grid), dim3(BS), 0, S(), __VA_ARGS__); \
+        EQ_HIP(hipGetLastError());                                             \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// Solver structure cached per sparsity pattern
+struct Layout {
+    int n = 0, nslices = 0;
+    long nnzp = 0;
+    DevPtr<int> slice_ptr, col, src, diag;
+    // multicolour ILU(0)
+    int ncolors = 0;
+    std::vector<int> color_off;        // host: rows of colour c are color_rows[color_off[c]..color_off[c+1])
+    DevPtr<int> color_rows;
+    DevPtr<uint8_t> kind;              // per CSR entry: 0 L, 1 diag, 2 U, 3 ignored
+    DevPtr<uint8_t> color;             // per row
+    std::weak_ptr<const Pattern> pat;
+};
+using LayoutP = std::shared_ptr<const Layout>;
+
+LayoutP get_layout(const PatternP& p) {
+    static std::mutex mu;
+    static std::unordered_map<uint64_t, LayoutP> cache;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = cache.find(p->id);
+        if (it != cache.end()) return it->second;
+        for (auto i = cache.begin(); i != cache.end();) i = i->second->pat.expired() ? cache.erase(i) : std::next(i);
+    }
+    EQ_REQUIRE(p->rows == p->cols, "linear solve needs a square Jacobian (" << p->rows << "x" << p->cols << ")");
+    auto L = std::make_shared<Layout>();
+    const int n = p->rows;
+    L->n = n;
+    L->pat = p;
+    L->nslices = (n + W - 1) / W;
+    std::vector<int> sp(L->nslices + 1, 0);
+    for (int s = 0; s < L->nslices; ++s) {
+        int w = 0;
+        for (int r = s * W; r < std::min(n, (s + 1) * W); ++r) w = std::max(w, p->rp[r + 1] - p->rp[r]);
+        sp[s + 1] = sp[s] + w * W;
+    }
+    L->nnzp = sp[L->nslices];
+    std::vector<int> col(L->nnzp), src(L->nnzp);
+    for (int s = 0; s < L->nslices; ++s) {
+        const int w = (sp[s + 1] - sp[s]) / W;
+        for (int j = 0; j < w; ++j)
+            for (int lane = 0; lane < W; ++lane) {
+                const int r = s * W + lane;
+                const long k = sp[s] + (long)j * W + lane;
+                if (r < n && j < p->rp[r + 1] - p->rp[r]) {
+                    col[k] = p->ci[p->rp[r] + j];
+                    src[k] = p->rp[r] + j;
+                } else {
+                    col[k] = r < n ? r : 0;
+                    src[k] = -1;
+                }
+            }
+    }
+    std::vector<int> diag(n, -1);
+    for (int r = 0; r < n; ++r)
+        for (int k = p->rp[r]; k < p->rp[r + 1]; ++k)
+            if (p->ci[k] == r) diag[r] = k;
+    // greedy colouring of the symmetrised graph in natural order
+    std::vector<std::vector<int>> adjT(n);
+    for (int r = 0; r < n; ++r)
+        for (int k = p->rp[r]; k < p->rp[r + 1]; ++k)
+            if (p->ci[k] != r) adjT[p->ci[k]].push_back(r);
+    std::vector<int> color(n, -1);
+    std::vector<int> mark(64, -1);
+    int ncol = 0;
+    for (int r = 0; r < n; ++r) {
+        auto use = [&](int j) {
+            if (color[j] >= 0) {
+                if (color[j] >= (int)mark.size()) mark.resize(color[j] + 1, -1);
+                mark[color[j]] = r;
+            }
+        };
+        for (int k = p->rp[r]; k < p->rp[r + 1]; ++k)
+            if (p->ci[k] != r) use(p->ci[k]);
+        for (int j : adjT[r]) use(j);
+        int c = 0;
+        while (c < (int)mark.size() && mark[c] == r) ++c;
+        color[r] = c;
+        ncol = std::max(ncol, c + 1);
+    }
+    EQ_REQUIRE(ncol <= 255, "ILU(0) colouring needs more than 255 colours");
+    L->ncolors = ncol;
+    L->color_off.assign(ncol + 1, 0);
+    for (int r = 0; r < n; ++r) L->color_off[color[r] + 1]++;
+    for (int c = 0; c < ncol; ++c) L->color_off[c + 1] += L->color_off[c];
+    std::vector<int> rows(n), fill(L->color_off.begin(), L->color_off.end() - 1);
+    for (int r = 0; r < n; ++r) rows[fill[color[r]]++] = r;
+    std::vector<uint8_t> kind(p->nnz), col8(n);
+    for (int r = 0; r < n; ++r) {
+        col8[r] = (uint8_t)color[r];
+        for (int k = p->rp[r]; k < p->rp[r + 1]; ++k) {
+            const int c = p->ci[k];
+            kind[k] = c == r ? 1 : color[c] < color[r] ? 0 : color[c] > color[r] ? 2 : 3;
+        }
+    }
+    L->slice_ptr = dev_from_host(sp.data(), sp.size());
+    L->col = dev_from_host(col.data(), col.size());
+    L->src = dev_from_host(src.data(), src.size());
+    L->diag = dev_from_host(diag.data(), diag.size());
+    L->color_rows = dev_from_host(rows.data(), rows.size());
+    L->kind = dev_from_host(kind.data(), kind.size());
+    L->color = dev_from_host(col8.data(), col8.size());
+    std::lock_guard<std::mutex> lk(mu);
+    cache[p->id] = L;
+    return L;
+}
+
+// ---------------------------------------------------------------------------
+// Kernels
+template <class T>
+__global__ void k_sell_vals(long nnzp, const int* __restrict__ src, const double* __restrict__ csr,
+                            T* __restrict__ out) {
+    long k = gid();
+    if (k >= nnzp) return;
+    const int s = src[k];
+    out[k] = s >= 0 ? (T)csr[s] : (T)0;
+}
+
+// One wave64 per slice: lane = row within the slice, entries column-major.
+template <class T>
+__global__ __launch_bounds__(BS) void k_sell_spmv(int n, int nslices, const int* __restrict__ sp,
+                                                  const int* __restrict__ col, const T* __restrict__ val,
+                                                  const T* __restrict__ x, T* __restrict__ y) {
+    const long row = gid();
+    const long slice = row / W;
+    if (slice >= nslices) return;
+    const int lane = (int)(row % W);
+    const int off = sp[slice];
+    const int w = (sp[slice + 1] - off) / W;
+    T acc = 0;
+    const int* c = col + off + lane;
+    const T* v = val + off + lane;
+    for (int j = 0; j < w; ++j) acc += v[j * W] * x[c[j * W]];
+    if (row < n) y[row] = acc;
+}
+
+template <class Tin, class Tout>
+__global__ void k_cast(long n, const Tin* __restrict__ in, double scale, Tout* __restrict__ out) {
+    long i = gid();
+    if (i < n) out[i] = (Tout)(in[i] * scale);
+}
+
+// x64 += scale * d
+template <class T>
+__global__ void k_axpy_mixed(long n, double scale, const T* __restrict__ d, double* __restrict__ x) {
+    long i = gid();
+    if (i < n) x[i] += scale * (double)d[i];
+}
+
+// r = b - y
+__global__ void k_residual(long n, const double* __restrict__ b, const double* __restrict__ y,
+                           double* __restrict__ r) {
+    long i = gid();
+    if (i < n) r[i] = b[i] - y[i];
+}
+
+template <class T>
+__global__ void k_jacobi_setup(int n, const int* __restrict__ diag, const double* __restrict__ csr,
+                               T* __restrict__ dinv) {
+    long i = gid();
+    if (i >= n) return;
+    const int k = diag[i];
+    const double d = k >= 0 ? csr[k] : 0.0;
+    dinv[i] = (T)(d != 0.0 ? 1.0 / d : 1.0);
+}
+
+template <class T>
+__global__ void k_scale(long n, const T* __restrict__ d, const T* __restrict__ a, T* __restrict__ out) {
+    long i = gid();
+    if (i < n) out[i] = d ? d[i] * a[i] : a[i];
+}
+
+// ---- device scalars -------------------------------------------------------
+enum Slot { RHO = 0, RHO_OLD, ALPHA, OMEGA, BETA, RR, BREAK, RZ, NSLOT = 16 };
+enum Act { ACT_STORE = 0, ACT_BICG_ALPHA, ACT_BICG_OMEGA, ACT_BICG_RHO, ACT_CG_ALPHA, ACT_CG_BETA };
+
+__device__ inline double bred(double v) {
+    __shared__ double lds[BS / 64];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) lds[w] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < BS / 64; ++i) r += lds[i];
+    __syncthreads();
+    return r;   // valid in thread 0
+}
+
+// up to 3 dot products, FP64 accumulation, partials[d*nb + block]
+template <class T>
+__global__ void k_dots(long n, int nd, const T* __restrict__ x0, const T* __restrict__ y0, const T* __restrict__ x1,
+                       const T* __restrict__ y1, const T* __restrict__ x2, const T* __restrict__ y2,
+                       double* __restrict__ part) {
+    double a0 = 0, a1 = 0, a2 = 0;
+    for (long i = gid(); i < n; i += (long)gridDim.x * BS) {
+        a0 += (double)x0[i] * (double)y0[i];
+        if (nd > 1) a1 += (double)x1[i] * (double)y1[i];
+        if (nd > 2) a2 += (double)x2[i] * (double)y2[i];
+    }
+    a0 = bred(a0);
+    if (threadIdx.x == 0) part[blockIdx.x] = a0;
+    if (nd > 1) {
+        a1 = bred(a1);
+        if (threadIdx.x == 0) part[gridDim.x + blockIdx.x] = a1;
+    }
+    if (nd > 2) {
+        a2 = bred(a2);
+        if (threadIdx.x == 0) part[2 * gridDim.x + blockIdx.x] = a2;
+    }
+}
+
+__global__ void k_finalize(int act, int nd, const double* __restrict__ part, int nb, double* __restrict__ sc,
+                           int slot) {
+    double d[3] = {0, 0, 0};
+    for (int j = 0; j < nd; ++j) {
+        double a = 0;
+        for (int i = threadIdx.x; i < nb; i += BS) a += part[j * nb + i];
+        a = bred(a);
+        if (threadIdx.x == 0) d[j] = a;
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    switch (act) {
+        case ACT_STORE:
+            for (int j = 0; j < nd; ++j) sc[slot + j] = d[j];
+            break;
+        case ACT_BICG_ALPHA:
+            if (d[0] == 0.0) { sc[BREAK] = 1; sc[ALPHA] = 0; }
+            else sc[ALPHA] = sc[RHO] / d[0];
+            break;
+        case ACT_BICG_OMEGA:
+            if (d[1] == 0.0) { sc[OMEGA] = 0; sc[BREAK] = 2; }
+            else sc[OMEGA] = d[0] / d[1];
+            break;
+        case ACT_BICG_RHO:
+            sc[RHO_OLD] = sc[RHO];
+            sc[RHO] = d[0];
+            sc[RR] = d[1];
+            if (d[0] == 0.0) sc[BREAK] = 3;
+            break;
+        case ACT_CG_ALPHA:
+            if (d[0] == 0.0) { sc[ALPHA] = 0; sc[BREAK] = 4; }
+            else sc[ALPHA] = sc[RZ] / d[0];
+            break;
+        case ACT_CG_BETA:
+            sc[BETA] = sc[RZ] != 0.0 ? d[0] / sc[RZ] : 0.0;
+            sc[RZ] = d[0];
+            sc[RR] = d[1];
+            break;
+    }
+}
+
+// ---- BiCGStab vector kernels ------------------------------------------------
+// first: p = r ; else beta = (rho/rho_old)(alpha/omega), p = r + beta (p - omega v); ph = D p
+template <class T>
+__global__ void k_bicg_p(long n, int first, const T* __restrict__ r, const T* __restrict__ v, T* __restrict__ p,
+                         T* __restrict__ ph, const T* __restrict__ dinv, const double* __restrict__ sc) {
+    long i = gid();
+    if (i >= n) return;
+    T pi;
+    if (first) pi = r[i];
+    else {
+        const double beta = (sc[RHO] / sc[RHO_OLD]) * (sc[ALPHA] / sc[OMEGA]);
+        pi = r[i] + (T)beta * (p[i] - (T)sc[OMEGA] * v[i]);
+    }
+    p[i] = pi;
+    if (ph) ph[i] = dinv ? dinv[i] * pi : pi;
+}
+// s = r - alpha v ; sh = D s
+template <class T>
+__global__ void k_bicg_s(long n, const T* __restrict__ r, const T* __restrict__ v, T* __restrict__ s,
+                         T* __restrict__ sh, const T* __restrict__ dinv, const double* __restrict__ sc) {
+    long i = gid();
+    if (i >= n) return;
+    const T si = r[i] - (T)sc[ALPHA] * v[i];
+    s[i] = si;
+    if (sh) sh[i] = dinv ? dinv[i] * si : si;
+}
+// x += alpha ph + omega sh ; r = s - omega t ; partial dots (rh.r, r.r)
+template <class T>
+__global__ void k_bicg_x(long n, T* __restrict__ x, const T* __restrict__ ph, const T* __restrict__ sh,
+                         T* __restrict__ r, const T* __restrict__ s, const T* __restrict__ t,
+                         const T* __restrict__ rh, const double* __restrict__ sc, double* __restrict__ part) {
+    const T alpha = (T)sc[ALPHA], omega = (T)sc[OMEGA];
+    double a0 = 0, a1 = 0;
+    for (long i = gid(); i < n; i += (long)gridDim.x * BS) {
+        x[i] += alpha * ph[i] + omega * sh[i];
+        const T ri = s[i] - omega * t[i];
+        r[i] = ri;
+        a0 += (double)rh[i] * (double)ri;
+        a1 += (double)ri * (double)ri;
+    }
+    a0 = bred(a0);
+    if (threadIdx.x == 0) part[blockIdx.x] = a0;
+    a1 = bred(a1);
+    if (threadIdx.x == 0) part[gridDim.x + blockIdx.x] = a1;
+}
+
+// ---- CG vector kernels --------------------------------------------------------
+template <class T>
+__global__ void k_cg_update(long n, T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
+                            const T* __restrict__ q, const double* __restrict__ sc) {
+    long i = gid();
+    if (i >= n) return;
+    const T a = (T)sc[ALPHA];
+    x[i] += a * p[i];
+    r[i] -= a * q[i];
+}
+template <class T>
+__global__ void k_cg_p(long n, int first, const T* __restrict__ z, T* __restrict__ p,
+                       const double* __restrict__ sc) {
+    long i = gid();
+    if (i >= n) return;
+    p[i] = first ? z[i] : z[i] + (T)sc[BETA] * p[i];
+}
+
+// ---- GMRES kernels ----------------------------------------------------------------
+// partial dots of w against columns 0..k of V (V column-major, ld = n)
+template <class T>
+__global__ void k_vdots(long n, int k, const T* __restrict__ V, const T* __restrict__ w, double* __restrict__ part,
+                        int nb) {
+    const int col = blockIdx.y;
+    double a = 0;
+    const T* v = V + (long)col * n;
+    for (long i = (long)blockIdx.x * BS + threadIdx.x; i < n; i += (long)nb * BS) a += (double)v[i] * (double)w[i];
+    a = bred(a);
+    if (threadIdx.x == 0) part[(long)col * nb + blockIdx.x] = a;
+}
+__global__ void k_vdots_final(int nb, const double* __restrict__ part, double* __restrict__ h) {
+    const int col = blockIdx.x;
+    double a = 0;
+    for (int i = threadIdx.x; i < nb; i += BS) a += part[(long)col * nb + i];
+    a = bred(a);
+    if (threadIdx.x == 0) h[col] = a;
+}
+// w -= sum_{j<=k} h[j] V[:,j]
+template <class T>
+__global__ void k_vupdate(long n, int k, const T* __restrict__ V, const double* __restrict__ h, T* __restrict__ w) {
+    long i = gid();
+    if (i >= n) return;
+    double s = (double)w[i];
+    for (int j = 0; j <= k; ++j) s -= h[j] * (double)V[(long)j * n + i];
+    w[i] = (T)s;
+}
+// out = sum_j y[j] V[:,j]
+template <class T>
+__global__ void k_vcombine(long n, int k, const T* __restrict__ V, const double* __restrict__ y, T* __restrict__ out) {
+    long i = gid();
+    if (i >= n) return;
+    double s = 0;
+    for (int j = 0; j < k; ++j) s += y[j] * (double)V[(long)j * n + i];
+    out[i] = (T)s;
+}
+template <class T>
+__global__ void k_scal(long n, double a, const T* __restrict__ x, T* __restrict__ y) {
+    long i = gid();
+    if (i < n) y[i] = (T)(a * (double)x[i]);
+}
+template <class T>
+__global__ void k_add(long n, const T* __restrict__ a, T* __restrict__ x) {
+    long i = gid();
+    if (i < n) x[i] += a[i];
+}
+
+// ---- multicolour ILU(0) ----------------------------------------------------------
+template <class T>
+__global__ void k_ilu_factor(int cnt, const int* __restrict__ rows, const int* __restrict__ rp,
+                             const int* __restrict__ ci, const uint8_t* __restrict__ kind,
+                             const uint8_t* __restrict__ color, const int* __restrict__ diag, int mycolor,
+                             T* __restrict__ lu) {
+    long t = gid();
+    if (t >= cnt) return;
+    const int i = rows[t];
+    const int b = rp[i], e = rp[i + 1];
+    for (int cc = 0; cc < mycolor; ++cc) {
+        for (int q = b; q < e; ++q) {
+            if (kind[q] != 0) continue;
+            const int k = ci[q];
+            if (color[k] != cc) continue;
+            const int dk = diag[k];
+            const T ukk = dk >= 0 ? lu[dk] : (T)0;
+            const T l = ukk != (T)0 ? lu[q] / ukk : (T)0;
+            lu[q] = l;
+            if (l == (T)0) continue;
+            for (int f = rp[k]; f < rp[k + 1]; ++f) {
+                if (kind[f] != 2) continue;   // U part of row k
+                const int j = ci[f];
+                for (int g = b; g < e; ++g)
+                    if (ci[g] == j) {
+                        lu[g] -= l * lu[f];
+                        break;
+                    }
+            }
+        }
+    }
+}
+template <class T>
+__global__ void k_ilu_lower(int cnt, const int* __restrict__ rows, const int* __restrict__ rp,
+                            const int* __restrict__ ci, const uint8_t* __restrict__ kind, const T* __restrict__ lu,
+                            const T* __restrict__ b, T* __restrict__ z) {
+    long t = gid();
+    if (t >= cnt) return;
+    const int i = rows[t];
+    T s = b[i];
+    for (int q = rp[i]; q < rp[i + 1]; ++q)
+        if (kind[q] == 0) s -= lu[q] * z[ci[q]];
+    z[i] = s;
+}
+template <class T>
+__global__ void k_ilu_upper(int cnt, const int* __restrict__ rows, const int* __restrict__ rp,
+                            const int* __restrict__ ci, const uint8_t* __restrict__ kind,
+                            const int* __restrict__ diag, const T* __restrict__ lu, T* __restrict__ z) {
+    long t = gid();
+    if (t >= cnt) return;
+    const int i = rows[t];
+    T s = z[i];
+    for (int q = rp[i]; q < rp[i + 1]; ++q)
+        if (kind[q] == 2) s -= lu[q] * z[ci[q]];
+    const T d = diag[i] >= 0 ? lu[diag[i]] : (T)1;
+    z[i] = d != (T)0 ? s / d : s;
+}
+
+inline unsigned nblk(long n) { return (unsigned)std::max(1L, (n + BS - 1) / BS); }
+
+// ---------------------------------------------------------------------------
+// Solver working set in precision T
+template <class T>
+struct System {
+    int n = 0;
+    LayoutP L;
+    PatternP pat;
+    DevPtr<T> sell;       // SELL values
+    std::string precond;
+    DevPtr<T> dinv;       // Jacobi
+    DevPtr<T> lu;         // ILU(0) factors in CSR layout
+    DevPtr<double> sc, part;
+    int nb = 1;
+
+    void setup(const PatternP& p, const double* csr, const std::string& pc) {
+        pat = p;
+        n = p->rows;
+        L = get_layout(p);
+        sell = dev_alloc<T>(L->nnzp);
+        KL(k_sell_vals<T>, nblk(L->nnzp), L->nnzp, L->src->p, csr, sell->p);
+        precond = pc;
+        if (pc == "jacobi") {
+            dinv = dev_alloc<T>(n);
+            KL(k_jacobi_setup<T>, nblk(n), n, L->diag->p, csr, dinv->p);
+        } else if (pc == "ilu0") {
+            lu = dev_alloc<T>(p->nnz);
+            KL((k_cast<double, T>), nblk(p->nnz), (long)p->nnz, csr, 1.0, lu->p);
+            for (int c = 1; c < L->ncolors; ++c) {
+                const int cnt = L->color_off[c + 1] - L->color_off[c];
+                if (cnt > 0)
+                    KL(k_ilu_factor<T>, nblk(cnt), cnt, L->color_rows->p + L->color_off[c], p->d_rp->p, p->d_ci->p,
+                       L->kind->p, L->color->p, L->diag->p, c, lu->p);
+            }
+        }
+        nb = vec_blocks(n);
+        sc = dev_alloc<double>(NSLOT);
+        part = dev_alloc<double>(3 * (size_t)nb + 8);
+    }
+    void spmv(const T* x, T* y) const {
+        KL(k_sell_spmv<T>, nblk((long)L->nslices * W), n, L->nslices, L->slice_ptr->p, L->col->p, sell->p, x, y);
+    }
+    // z = M^{-1} r  (not used for the fused Jacobi paths)
+    void apply_precond(const T* r, T* z) const {
+        if (precond == "ilu0") {
+            for (int c = 0; c < L->ncolors; ++c) {
+                const int cnt = L->color_off[c + 1] - L->color_off[c];
+                if (cnt > 0)
+                    KL(k_ilu_lower<T>, nblk(cnt), cnt, L->color_rows->p + L->color_off[c], pat->d_rp->p, pat->d_ci->p,
+                       L->kind->p, lu->p, r, z);
+            }
+            for (int c = L->ncolors - 1; c >= 0; --c) {
+                const int cnt = L->color_off[c + 1] - L->color_off[c];
+                if (cnt > 0)
+                    KL(k_ilu_upper<T>, nblk(cnt), cnt, L->color_rows->p + L->color_off[c], pat->d_rp->p, pat->d_ci->p,
+                       L->kind->p, L->diag->p, lu->p, z);
+            }
+        } else {
+            KL(k_scale<T>, nblk(n), (long)n, precond == "jacobi" ? (const T*)dinv->p : (const T*)nullptr, r, z);
+        }
+    }
+    void dots(int act, int nd, const T* x0, const T* y0, const T* x1 = nullptr, const T* y1 = nullptr,
+              int slot = 0) const {
+        KL(k_dots<T>, nb, (long)n, nd, x0, y0, x1, y1, (const T*)nullptr, (const T*)nullptr, part->p);
+        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(BS), 0, S(), act, nd, part->p, nb, sc->p, slot);
+        EQ_HIP(hipGetLastError());
+    }
+    double read_slot(int slot) const {
+        double* h = Device::get().host_scratch();
+        EQ_HIP(hipMemcpyAsync(h, sc->p + slot, sizeof(double), hipMemcpyDeviceToHost, S()));
+        Device::get().sync();
+        return h[0];
+    }
+    void read_slots(double* out, int count) const {
+        double* h = Device::get().host_scratch();
+        EQ_HIP(hipMemcpyAsync(h, sc->p, count * sizeof(double), hipMemcpyDeviceToHost, S()));
+        Device::get().sync();
+        for (int i = 0; i < count; ++i) out[i] = h[i];
+    }
+};
+
+// BiCGStab, x0 = 0 (x zeroed here).  Relative tolerance on ||r|| / ||b||.
+template <class T>
+int bicgstab(const System<T>& A, const T* b, T* x, double tol, int maxit, double* relres) {
+    const long n = A.n;
+    auto r = dev_alloc<T>(n), rh = dev_alloc<T>(n), p = dev_alloc<T>(n), v = dev_alloc<T>(n), s = dev_alloc<T>(n),
+         t = dev_alloc<T>(n);
+    const bool ilu = A.precond == "ilu0";
+    auto ph = dev_alloc<T>(n), sh = dev_alloc<T>(n);
+    const T* dinv = A.precond == "jacobi" ? A.dinv->p : nullptr;
+    EQ_HIP(hipMemsetAsync(x, 0, n * sizeof(T), S()));
+    EQ_HIP(hipMemsetAsync(A.sc->p, 0, NSLOT * sizeof(double), S()));
+    EQ_HIP(hipMemcpyAsync(r->p, b, n * sizeof(T), hipMemcpyDeviceToDevice, S()));
+    EQ_HIP(hipMemcpyAsync(rh->p, b, n * sizeof(T), hipMemcpyDeviceToDevice, S()));
+    A.dots(ACT_BICG_RHO, 2, rh->p, r->p, r->p, r->p);   // rho = rh.r, rr = r.r
+    double h[NSLOT];
+    A.read_slots(h, NSLOT);
+    const double bn = std::sqrt(h[RR]);
+    if (relres) *relres = 0.0;
+    if (bn == 0.0) return 0;
+    int it = 0;
+    double rn = bn;
+    while (it < maxit && rn > tol * bn) {
+        KL(k_bicg_p<T>, nblk(n), n, it == 0 ? 1 : 0, r->p, v->p, p->p, ilu ? (T*)nullptr : ph->p, dinv, A.sc->p);
+        if (ilu) A.apply_precond(p->p, ph->p);
+        A.spmv(ph->p, v->p);
+        A.dots(ACT_BICG_ALPHA, 1, rh->p, v->p);
+        KL(k_bicg_s<T>, nblk(n), n, r->p, v->p, s->p, ilu ? (T*)nullptr : sh->p, dinv, A.sc->p);
+        if (ilu) A.apply_precond(s->p, sh->p);
+        A.spmv(sh->p, t->p);
+        A.dots(ACT_BICG_OMEGA, 2, t->p, s->p, t->p, t->p);
+        KL(k_bicg_x<T>, A.nb, n, x, ph->p, sh->p, r->p, s->p, t->p, rh->p, A.sc->p, A.part->p);
+        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(BS), 0, S(), (int)ACT_BICG_RHO, 2, A.part->p, A.nb, A.sc->p, 0);
+        EQ_HIP(hipGetLastError());
+        ++it;
+        A.read_slots(h, NSLOT);
+        rn = std::sqrt(std::max(h[RR], 0.0));
+        if (h[BREAK] != 0.0 || !std::isfinite(rn)) break;
+    }
+    if (relres) *relres = rn / bn;
+    return it;
+}
+
+template <class T>
+int cg(const System<T>& A, const T* b, T* x, double tol, int maxit, double* relres) {
+    const long n = A.n;
+    auto r = dev_alloc<T>(n), z = dev_alloc<T>(n), p = dev_alloc<T>(n), q = dev_alloc<T>(n);
+    EQ_HIP(hipMemsetAsync(x, 0, n * sizeof(T), S()));
+    EQ_HIP(hipMemsetAsync(A.sc->p, 0, NSLOT * sizeof(double), S()));
+    EQ_HIP(hipMemcpyAsync(r->p, b, n * sizeof(T), hipMemcpyDeviceToDevice, S()));
+    A.apply_precond(r->p, z->p);
+    A.dots(ACT_CG_BETA, 2, r->p, z->p, r->p, r->p);   // rz, rr
+    double h[NSLOT];
+    A.read_slots(h, NSLOT);
+    const double bn = std::sqrt(h[RR]);
+    if (relres) *relres = 0.0;
+    if (bn == 0.0) return 0;
+    KL(k_cg_p<T>, nblk(n), n, 1, z->p, p->p, A.sc->p);
+    int it = 0;
+    double rn = bn;
+    while (it < maxit && rn > tol * bn) {
+        A.spmv(p->p, q->p);
+        A.dots(ACT_CG_ALPHA, 1, p->p, q->p);
+        KL(k_cg_update<T>, nblk(n), n, x, r->p, p->p, q->p, A.sc->p);
+        A.apply_precond(r->p, z->p);
+        A.dots(ACT_CG_BETA, 2, r->p, z->p, r->p, r->p);
+        KL(k_cg_p<T>, nblk(n), n, 0, z->p, p->p, A.sc->p);
+        ++it;
+        A.read_slots(h, NSLOT);
+        rn = std::sqrt(std::max(h[RR], 0.0));
+        if (h[BREAK] != 0.0 || !std::isfinite(rn)) break;
+    }
+    if (relres) *relres = rn / bn;
+    return it;
+}
+
+// GMRES(m), right preconditioning, CGS2; x0 = 0.
+template <class T>
+int gmres(const System<T>& A, const T* b, T* x, double tol, int maxit, int m, double* relres) {
+    const long n = A.n;
+    m = std::max(1, std::min(m, 200));
+    auto V = dev_alloc<T>((size_t)n * (m + 1));
+    auto w = dev_alloc<T>(n), z = dev_alloc<T>(n), r = dev_alloc<T>(n);
+    auto dh = dev_alloc<double>(m + 2);
+    const int nb = A.nb;
+    auto part = dev_alloc<double>((size_t)nb * (m + 2));
+    EQ_HIP(hipMemsetAsync(x, 0, n * sizeof(T), S()));
+    double* hs = Device::get().host_scratch();
+    auto norm2 = [&](const T* a) {
+        A.dots(ACT_STORE, 1, a, a, nullptr, nullptr, RR);
+        return std::sqrt(std::max(A.read_slot(RR), 0.0));
+    };
+    const double bn = norm2(b);
+    if (relres) *relres = 0.0;
+    if (bn == 0.0) return 0;
+    std::vector<double> H((m + 1) * m), cs(m), sn(m), g(m + 1), y(m);
+    int it = 0;
+    double rn = bn;
+    EQ_HIP(hipMemcpyAsync(r->p, b, n * sizeof(T), hipMemcpyDeviceToDevice, S()));
+    while (it < maxit) {
+        const double beta = it == 0 ? bn : norm2(r->p);
+        rn = beta;
+        if (beta <= tol * bn) break;
+        KL(k_scal<T>, nblk(n), n, 1.0 / beta, r->p, V->p);
+        std::fill(g.begin(), g.end(), 0.0);
+        g[0] = beta;
+        int j = 0;
+        for (; j < m && it < maxit; ++j, ++it) {
+            T* vj = V->p + (size_t)j * n;
+            A.apply_precond(vj, z->p);
+            A.spmv(z->p, w->p);
+            std::vector<double> hc(j + 2, 0.0);
+            for (int pass = 0; pass < 2; ++pass) {   // CGS2
+                hipLaunchKernelGGL(k_vdots<T>, dim3(nb, j + 1), dim3(BS), 0, S(), n, j, (const T*)V->p,
+                                   (const T*)w->p, part->p, nb);
+                EQ_HIP(hipGetLastError());
+                hipLaunchKernelGGL(k_vdots_final, dim3(j + 1), dim3(BS), 0, S(), nb, part->p, dh->p);
+                EQ_HIP(hipGetLastError());
+                KL(k_vupdate<T>, nblk(n), n, j, V->p, dh->p, w->p);
+                EQ_HIP(hipMemcpyAsync(hs, dh->p, (j + 1) * sizeof(double), hipMemcpyDeviceToHost, S()));
+                Device::get().sync();
+                for (int i = 0; i <= j; ++i) hc[i] += hs[i];
+            }
+            const double hn = norm2(w->p);
+            hc[j + 1] = hn;
+            if (hn > 0) KL(k_scal<T>, nblk(n), n, 1.0 / hn, w->p, V->p + (size_t)(j + 1) * n);
+            for (int i = 0; i < j; ++i) {   // previous Givens rotations
+                const double t0 = cs[i] * hc[i] + sn[i] * hc[i + 1];
+                hc[i + 1] = -sn[i] * hc[i] + cs[i] * hc[i + 1];
+                hc[i] = t0;
+            }
+            const double den = std::hypot(hc[j], hc[j + 1]);
+            cs[j] = den > 0 ? hc[j] / den : 1.0;
+            sn[j] = den > 0 ? hc[j + 1] / den : 0.0;
+            hc[j] = den;
+            hc[j + 1] = 0;
+            g[j + 1] = -sn[j] * g[j];
+            g[j] = cs[j] * g[j];
+            for (int i = 0; i <= j; ++i) H[i * m + j] = hc[i];
+            rn = std::fabs(g[j + 1]);
+            if (rn <= tol * bn || hn == 0.0 || !std::isfinite(rn)) {
+                ++j;
+                ++it;
+                break;
+            }
+        }
+        // y = H^{-1} g, x += M^{-1} V y
+        for (int i = j - 1; i >= 0; --i) {
+            double s = g[i];
+            for (int k = i + 1; k < j; ++k) s -= H[i * m + k] * y[k];
+            y[i] = H[i * m + i] != 0 ? s / H[i * m + i] : 0.0;
+        }
+        EQ_HIP(hipMemcpyAsync(dh->p, y.data(), j * sizeof(double), hipMemcpyHostToDevice, S()));
+        KL(k_vcombine<T>, nblk(n), n, j, V->p, dh->p, w->p);
+        A.apply_precond(w->p, z->p);
+        KL(k_add<T>, nblk(n), n, z->p, x);
+        Device::get().sync();   // y is read from host memory by the copy above
+        if (rn <= tol * bn || !std::isfinite(rn)) break;
+        // true residual for the restart: r = b - A x
+        A.spmv(x, w->p);
+        KL(k_scal<T>, nblk(n), n, 1.0, b, r->p);
+        KL(k_scal<T>, nblk(n), n, -1.0, w->p, z->p);
+        KL(k_add<T>, nblk(n), n, z->p, r->p);
+    }
+    if (relres) *relres = rn / bn;
+    return it;
+}
+
+template <class T>
+int run_method(const SolverConfig& cfg, const System<T>& A, const T* b, T* x, double tol, int maxit, double* rr) {
+    if (cfg.method == "bicgstab") return bicgstab<T>(A, b, x, tol, maxit, rr);
+    if (cfg.method == "cg") return cg<T>(A, b, x, tol, maxit, rr);
+    if (cfg.method == "gmres") return gmres<T>(A, b, x, tol, maxit, cfg.gmres_restart, rr);
+    throw_error("unknown Krylov method " + cfg.method);
+}
+
+double dnorm(const double* a, long n) { return std::sqrt(reduce(RED_SUMSQ, a, n)); }
+
+}  // namespace
+
+// ===========================================================================
+SP linear_solve(const SolverConfig& cfg, const SP& res, LinReport* rep) {
+    EQ_REQUIRE(res->pat, "linear solve: the residual has no Jacobian");
+    const PatternP& P = res->pat;
+    const int n = res->n;
+    EQ_REQUIRE(P->rows == n && P->cols == n, "linear solve: Jacobian is " << P->rows << "x" << P->cols
+                                                                          << " for " << n << " unknowns");
+    std::string pc = cfg.precond == "diagonal" ? "jacobi" : cfg.precond;
+    LinReport R;
+    auto x = dev_alloc<double>(n);
+    if (cfg.precision == "fp64") {
+        System<double> A;
+        A.setup(P, res->jval->p, pc);
+        double rr = 0;
+        R.iterations = run_method<double>(cfg, A, res->val->p, x->p, cfg.tol, cfg.max_iter, &rr);
+        R.residual = rr;
+        R.converged = rr <= cfg.tol;
+    } else if (cfg.precision == "mixed") {
+        System<float> A;
+        A.setup(P, res->jval->p, pc);
+        auto r64 = dev_alloc<double>(n), y64 = dev_alloc<double>(n);
+        auto r32 = dev_alloc<float>(n), d32 = dev_alloc<float>(n);
+        EQ_HIP(hipMemsetAsync(x->p, 0, n * sizeof(double), S()));
+        EQ_HIP(hipMemcpyAsync(r64->p, res->val->p, n * sizeof(double), hipMemcpyDeviceToDevice, S()));
+        const double bn = dnorm(res->val->p, n);
+        double rn = bn;
+        int total = 0;
+        while (bn > 0 && rn > cfg.tol * bn && R.refinements < cfg.max_refine && total < cfg.max_iter) {
+            // FP32 solve of A d = r / ||r|| (scaled to unit norm), then x += ||r|| d
+            KL((k_cast<double, float>), nblk(n), (long)n, r64->p, 1.0 / rn, r32->p);
+            double rr = 0;
+            total += run_method<float>(cfg, A, r32->p, d32->p, std::max(cfg.inner_tol, 1e-6),
+                                       cfg.max_iter - total, &rr);
+            KL(k_axpy_mixed<float>, nblk(n), (long)n, rn, d32->p, x->p);
+            // FP64 defect
+            launch_csr_spmv(n, P->d_rp->p, P->d_ci->p, res->jval->p, x->p, y64->p);
+            KL(k_residual, nblk(n), (long)n, res->val->p, y64->p, r64->p);
+            const double rnew = dnorm(r64->p, n);
+            ++R.refinements;
+            if (!std::isfinite(rnew) || rnew >= rn) {   // stagnation
+                rn = rnew;
+                break;
+            }
+            rn = rnew;
+        }
+        R.iterations = total;
+        R.residual = bn > 0 ? rn / bn : 0.0;
+        R.converged = R.residual <= cfg.tol;
+    } else {
+        throw_error("unknown precision " + cfg.precision + " (fp64 | mixed)");
+    }
+    if (cfg.verbose > 0)
+        fprintf(stdout, "Linear solver (%s/%s/%s): %d iterations, relative residual %g%s\n", cfg.method.c_str(),
+                pc.c_str(), cfg.precision.c_str(), R.iterations, R.residual, R.converged ? "" : " (not converged)");
+    if (rep) *rep = R;
+    return s_from_device(x, n);
+}
+
+void sell_spmv_test(const SP& a, const double* xh, double* yh, bool fp32) {
+    EQ_REQUIRE(a->pat, "sell_spmv_test: no Jacobian");
+    const int n = a->n;
+    auto x = dev_from_host(xh, n);
+    auto y = dev_alloc<double>(n);
+    if (fp32) {
+        System<float> A;
+        A.setup(a->pat, a->jval->p, "none");
+        auto x32 = dev_alloc<float>(n), y32 = dev_alloc<float>(n);
+        KL((k_cast<double, float>), nblk(n), (long)n, x->p, 1.0, x32->p);
+        A.spmv(x32->p, y32->p);
+        KL((k_cast<float, double>), nblk(n), (long)n, y32->p, 1.0, y->p);
+    } else {
+        System<double> A;
+        A.setup(a->pat, a->jval->p, "none");
+        A.spmv(x->p, y->p);
+    }
+    std::vector<double> h = dev_to_host(y->p, n);
+    std::copy(h.begin(), h.end(), yh);
+}
+
+}  // namespace eqhip

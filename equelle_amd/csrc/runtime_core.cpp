@@ -1,0 +1,328 @@
+// Runtime core: device grid, HelperOps and grid built-ins (see runtime_core.hpp).
+#include "runtime_core.hpp"
+
+#include <algorithm>
+#include <cstdlib>
+#include <sstream>
+
+#include "kernels.hpp"
+
+namespace eqhip {
+
+// ---------------------------------------------------------------- params
+void ParamMap::parse(const std::string& text) {
+    std::istringstream is(text);
+    std::string tok;
+    while (is >> tok) {
+        const size_t e = tok.find('=');
+        if (e != std::string::npos) kv[tok.substr(0, e)] = tok.substr(e + 1);
+    }
+}
+double ParamMap::get_double(const std::string& k, double def) const {
+    auto it = kv.find(k);
+    return it == kv.end() ? def : std::strtod(it->second.c_str(), nullptr);
+}
+int ParamMap::get_int(const std::string& k, int def) const {
+    auto it = kv.find(k);
+    return it == kv.end() ? def : (int)std::strtol(it->second.c_str(), nullptr, 10);
+}
+bool ParamMap::get_bool(const std::string& k, bool def) const {
+    auto it = kv.find(k);
+    if (it == kv.end()) return def;
+    return it->second == "true" || it->second == "1";
+}
+std::string ParamMap::get_string(const std::string& k, const std::string& def) const {
+    auto it = kv.find(k);
+    return it == kv.end() ? def : it->second;
+}
+std::string ParamMap::require(const std::string& k) const {
+    auto it = kv.find(k);
+    EQ_REQUIRE(it != kv.end(), "parameter " << k << " not found");
+    return it->second;
+}
+
+// ---------------------------------------------------------------- grid
+// Numbering of OPM's Cartesian GridManager: cells x-fastest; x-faces, then
+// y-faces, then z-faces, each x-fastest; face_cells = (-axis cell, +axis cell)
+// with -1 outside; normals area-scaled along +axis; cell_faces x-,x+,y-,y+,z-,z+.
+GridHost build_cartesian(int dim, int nx, int ny, int nz, double dx, double dy, double dz) {
+    EQ_REQUIRE(dim == 2 || dim == 3, "Cannot handle " << dim << " dimensions.");
+    EQ_REQUIRE(nx > 0 && ny > 0 && (dim == 2 || nz > 0), "grid sizes must be positive");
+    GridHost g;
+    g.dim = dim;
+    if (dim == 2) {
+        nz = 1;
+        dz = 1.0;
+    }
+    g.n[0] = nx;
+    g.n[1] = ny;
+    g.n[2] = nz;
+    g.h[0] = dx;
+    g.h[1] = dy;
+    g.h[2] = dz;
+    const long nc = (long)nx * ny * nz;
+    const long fx = (long)(nx + 1) * ny * nz, fy = (long)nx * (ny + 1) * nz;
+    const long fz = dim == 3 ? (long)nx * ny * (nz + 1) : 0;
+    const long nf = fx + fy + fz;
+    EQ_REQUIRE(nf < (1L << 31) && 2 * nf < (1L << 31), "grid too large for 32-bit indices");
+    g.nc = (int)nc;
+    g.nf = (int)nf;
+    g.face_cells.assign(2 * nf, -1);
+    g.face_areas.resize(nf);
+    g.face_centroids.assign(dim * nf, 0.0);
+    g.face_normals.assign(dim * nf, 0.0);
+    const double area[3] = {dim == 3 ? dy * dz : dy, dim == 3 ? dx * dz : dx, dx * dy};
+    auto C = [&](long i, long j, long k) { return (int)(i + nx * (j + (long)ny * k)); };
+    // face blocks: axis a, extents (nx+da, ny+db, nz+dc)
+    long base = 0;
+    for (int a = 0; a < dim; ++a) {
+        const long ex = nx + (a == 0), ey = ny + (a == 1), ez = nz + (a == 2);
+        for (long k = 0; k < ez; ++k)
+            for (long j = 0; j < ey; ++j)
+                for (long i = 0; i < ex; ++i) {
+                    const long f = base + i + ex * (j + ey * k);
+                    long lo[3] = {i, j, k};
+                    long hi[3] = {i, j, k};
+                    lo[a] -= 1;
+                    const long ext[3] = {nx, ny, nz};
+                    g.face_cells[2 * f] = lo[a] >= 0 ? C(lo[0], lo[1], lo[2]) : -1;
+                    g.face_cells[2 * f + 1] = hi[a] < ext[a] ? C(hi[0], hi[1], hi[2]) : -1;
+                    g.face_areas[f] = area[a];
+                    g.face_normals[dim * f + a] = area[a];
+                    const double cc[3] = {(i + (a == 0 ? 0.0 : 0.5)) * dx, (j + (a == 1 ? 0.0 : 0.5)) * dy,
+                                          (k + (a == 2 ? 0.0 : 0.5)) * dz};
+                    for (int d = 0; d < dim; ++d) g.face_centroids[dim * f + d] = cc[d];
+                }
+        base += ex * ey * ez;
+    }
+    g.cell_volumes.assign(nc, dim == 3 ? dx * dy * dz : dx * dy);
+    g.cell_centroids.resize(dim * nc);
+    g.cell_facepos.resize(nc + 1);
+    g.cell_faces.resize(2 * dim * nc);
+    for (long k = 0; k < nz; ++k)
+        for (long j = 0; j < ny; ++j)
+            for (long i = 0; i < nx; ++i) {
+                const long c = C(i, j, k);
+                g.cell_centroids[dim * c] = (i + 0.5) * dx;
+                g.cell_centroids[dim * c + 1] = (j + 0.5) * dy;
+                if (dim == 3) g.cell_centroids[dim * c + 2] = (k + 0.5) * dz;
+                const long p = 2 * dim * c;
+                g.cell_facepos[c] = (int)p;
+                g.cell_faces[p + 0] = (int)(i + (nx + 1) * (j + (long)ny * k));
+                g.cell_faces[p + 1] = (int)(i + 1 + (nx + 1) * (j + (long)ny * k));
+                g.cell_faces[p + 2] = (int)(fx + i + (long)nx * (j + (long)(ny + 1) * k));
+                g.cell_faces[p + 3] = (int)(fx + i + (long)nx * (j + 1 + (long)(ny + 1) * k));
+                if (dim == 3) {
+                    g.cell_faces[p + 4] = (int)(fx + fy + i + (long)nx * (j + (long)ny * k));
+                    g.cell_faces[p + 5] = (int)(fx + fy + i + (long)nx * (j + (long)ny * (k + 1)));
+                }
+            }
+    g.cell_facepos[nc] = (int)(2 * dim * nc);
+    return g;
+}
+
+static IP compact_to_set(const DevPtr<uint8_t>& flags, int n, int want, int codim) {
+    auto d = dev_alloc<int>(n > 0 ? n : 1);
+    const int cnt = compact_flags(flags->p, n, want, d->p);
+    return i_from_device(codim, d, cnt);
+}
+
+DeviceGrid::DeviceGrid(GridHost g) : host(std::move(g)) {
+    dim = host.dim;
+    nc = host.nc;
+    nf = host.nf;
+    face_cells = dev_from_host(host.face_cells.data(), host.face_cells.size());
+    cell_facepos = dev_from_host(host.cell_facepos.data(), host.cell_facepos.size());
+    cell_faces = dev_from_host(host.cell_faces.data(), host.cell_faces.size());
+    cell_centroids = dev_from_host(host.cell_centroids.data(), host.cell_centroids.size());
+    cell_volumes = dev_from_host(host.cell_volumes.data(), host.cell_volumes.size());
+    face_centroids = dev_from_host(host.face_centroids.data(), host.face_centroids.size());
+    face_areas = dev_from_host(host.face_areas.data(), host.face_areas.size());
+    face_normals = dev_from_host(host.face_normals.data(), host.face_normals.size());
+    // signed cell->face list for the divergence sweep (+f if the cell is the first cell)
+    std::vector<int> sf(host.cell_faces.size());
+    for (int c = 0; c < nc; ++c)
+        for (int k = host.cell_facepos[c]; k < host.cell_facepos[c + 1]; ++k) {
+            const int f = host.cell_faces[k];
+            sf[k] = host.face_cells[2 * f] == c ? f : ~f;
+        }
+    signed_faces = dev_from_host(sf.data(), sf.size());
+    // subsets: device flag kernels + ordered compaction (deviceGrid.cu:368-492)
+    std::vector<int> all(std::max(nc, nf));
+    for (size_t i = 0; i < all.size(); ++i) all[i] = (int)i;
+    all_cells = make_index(0, true, std::vector<int>(all.begin(), all.begin() + nc));
+    all_faces = make_index(1, true, std::vector<int>(all.begin(), all.begin() + nf));
+    auto fflag = dev_alloc<uint8_t>(nf);
+    launch_face_flags(face_cells->p, nf, fflag->p);
+    interior_faces = compact_to_set(fflag, nf, 1, 1);
+    boundary_faces = compact_to_set(fflag, nf, 0, 1);
+    auto cflag = dev_alloc<uint8_t>(nc);
+    launch_cell_boundary_flags(face_cells->p, cell_facepos->p, cell_faces->p, nc, cflag->p);
+    boundary_cells = compact_to_set(cflag, nc, 1, 0);
+    interior_cells = compact_to_set(cflag, nc, 0, 0);
+    nif = interior_faces->size();
+    // interior face cell pairs for the gradient sweep
+    if_c1 = dev_alloc<int>(nif > 0 ? nif : 1);
+    if_c2 = dev_alloc<int>(nif > 0 ? nif : 1);
+    launch_gather_face_cells(face_cells->p, interior_faces->dev->p, nif, 0, if_c1->p);
+    launch_gather_face_cells(face_cells->p, interior_faces->dev->p, nif, 1, if_c2->p);
+    // HelperOps: grad = -ngrad (nif x nc), div = ngrad^T (nc x nif), fulldiv = fullngrad^T (nc x nf)
+    {
+        std::vector<int> rp(nif + 1), ci(2 * (size_t)nif);
+        std::vector<double> v(2 * (size_t)nif);
+        for (int i = 0; i < nif; ++i) {
+            const int f = interior_faces->host[i];
+            rp[i] = 2 * i;
+            ci[2 * i] = host.face_cells[2 * f];
+            ci[2 * i + 1] = host.face_cells[2 * f + 1];
+            v[2 * i] = -1.0;
+            v[2 * i + 1] = 1.0;
+            if (ci[2 * i] > ci[2 * i + 1]) {
+                std::swap(ci[2 * i], ci[2 * i + 1]);
+                std::swap(v[2 * i], v[2 * i + 1]);
+            }
+        }
+        rp[nif] = 2 * nif;
+        grad = csr_from_host(nif, nc, rp.data(), ci.data(), v.data());
+    }
+    auto transpose_signed = [&](bool interior_only) {
+        // rows = cells; entries over the cell's faces in ascending face order:
+        // +1 where the cell is the first cell, -1 where it is the second.
+        std::vector<int> fidx(nf, -1);
+        if (interior_only)
+            for (int i = 0; i < nif; ++i) fidx[interior_faces->host[i]] = i;
+        std::vector<int> rp(nc + 1, 0), ci;
+        std::vector<double> v;
+        ci.reserve(host.cell_faces.size());
+        v.reserve(host.cell_faces.size());
+        std::vector<std::pair<int, double>> row;
+        for (int c = 0; c < nc; ++c) {
+            row.clear();
+            for (int k = host.cell_facepos[c]; k < host.cell_facepos[c + 1]; ++k) {
+                const int f = host.cell_faces[k];
+                const int col = interior_only ? fidx[f] : f;
+                if (col < 0) continue;
+                row.emplace_back(col, host.face_cells[2 * f] == c ? 1.0 : -1.0);
+            }
+            std::sort(row.begin(), row.end());
+            for (auto& e : row) {
+                ci.push_back(e.first);
+                v.push_back(e.second);
+            }
+            rp[c + 1] = (int)ci.size();
+        }
+        return csr_from_host(nc, interior_only ? nif : nf, rp.data(), ci.data(), v.data());
+    };
+    div = transpose_signed(true);
+    fulldiv = transpose_signed(false);
+}
+
+// ---------------------------------------------------------------- runtime
+RuntimeCore::RuntimeCore(const ParamMap& p) : params_(p) {
+    EQ_REQUIRE(!p.has("grid_filename"), "grid_filename: only Cartesian grids (grid_dim, nx, ny, nz, dx, dy, dz) "
+                                        "are supported by this runtime");
+    // createGridManager (EquelleRuntimeCUDA.cpp:29-60): defaults 6 x 1 (x 1), unit cells
+    const int dim = p.get_int("grid_dim", 2);
+    EQ_REQUIRE(dim == 2 || dim == 3, "Cannot handle " << dim << " dimensions.");
+    grid_.reset(new DeviceGrid(build_cartesian(dim, p.get_int("nx", 6), p.get_int("ny", 1),
+                                               dim == 3 ? p.get_int("nz", 1) : 1, p.get_double("dx", 1.0),
+                                               p.get_double("dy", 1.0), dim == 3 ? p.get_double("dz", 1.0) : 1.0)));
+    // solver options (EquelleRuntimeCUDA.cpp:70-79, linearSolver.cu:33-65)
+    std::string s = p.get_string("solver", "BiCGStab");
+    std::string sl = s;
+    std::transform(sl.begin(), sl.end(), sl.begin(), ::tolower);
+    EQ_REQUIRE(sl != "cpu", "solver=CPU (the host linear solver) is not part of this runtime; "
+                            "use BiCGStab, CG or GMRes");
+    solver.method = sl == "gmres" ? "gmres" : sl == "cg" ? "cg" : "bicgstab";
+    EQ_REQUIRE(sl == "gmres" || sl == "cg" || sl == "bicgstab", "unknown solver " << s);
+    std::string pc = p.get_string("preconditioner", p.get_string("precond", "diagonal"));
+    std::transform(pc.begin(), pc.end(), pc.begin(), ::tolower);
+    solver.precond = (pc == "diagonal" || pc == "jacobi") ? "jacobi" : pc;
+    EQ_REQUIRE(solver.precond == "jacobi" || solver.precond == "ilu0" || solver.precond == "none",
+               "unknown preconditioner " << pc);
+    solver.max_iter = p.get_int("solver_max_iter", 1000);
+    solver.tol = p.get_double("solver_tol", 1e-8);
+    solver.precision = p.get_string("precision", "fp64");
+    solver.gmres_restart = p.get_int("gmres_restart", 30);
+    solver.inner_tol = p.get_double("inner_tol", 1e-4);
+    solver.max_refine = p.get_int("max_refine", 30);
+    verbose = p.get_int("verbose", 0);
+    solver.verbose = verbose;
+    max_iter = p.get_int("max_iter", 10);
+    abs_res_tol = p.get_double("abs_res_tol", 1e-6);
+    rel_res_tol = p.get_double("rel_res_tol", 0.0);
+    output_to_file = p.get_bool("output_to_file", false);
+}
+
+IP RuntimeCore::firstCell(const IP& faces) const {
+    auto d = dev_alloc<int>(std::max(1, faces->size()));
+    launch_gather_face_cells(grid_->face_cells->p, faces->dev->p, faces->size(), 0, d->p);
+    return i_from_device(0, d, faces->size());
+}
+IP RuntimeCore::secondCell(const IP& faces) const {
+    auto d = dev_alloc<int>(std::max(1, faces->size()));
+    launch_gather_face_cells(grid_->face_cells->p, faces->dev->p, faces->size(), 1, d->p);
+    return i_from_device(0, d, faces->size());
+}
+SP RuntimeCore::norm(const IP& s) const {
+    auto d = dev_alloc<double>(std::max(1, s->size()));
+    launch_gather_vec(s->codim == 0 ? grid_->cell_volumes->p : grid_->face_areas->p, s->dev->p, s->size(), 1, d->p);
+    return s_from_device(d, s->size());
+}
+VP RuntimeCore::centroid(const IP& s) const {
+    auto v = std::make_shared<VectorData>();
+    v->n = s->size();
+    v->dim = grid_->dim;
+    v->val = dev_alloc<double>(std::max(1, s->size() * grid_->dim));
+    launch_gather_vec(s->codim == 0 ? grid_->cell_centroids->p : grid_->face_centroids->p, s->dev->p, s->size(),
+                      grid_->dim, v->val->p);
+    return v;
+}
+VP RuntimeCore::normal(const IP& faces) const {
+    // unit normals (EquelleRuntimeCPU.cpp:295-315)
+    auto v = std::make_shared<VectorData>();
+    v->n = faces->size();
+    v->dim = grid_->dim;
+    v->val = dev_alloc<double>(std::max(1, faces->size() * grid_->dim));
+    launch_unit_normals(grid_->face_normals->p, faces->dev->p, faces->size(), grid_->dim, v->val->p);
+    return v;
+}
+
+SP RuntimeCore::gradient(const SP& x) const {
+    EQ_REQUIRE(x->n == grid_->nc, "Gradient need input defined on AllCells()");
+    // values: coalesced face sweep u[second] - u[first]; Jacobian: grad * J (cached pattern)
+    auto v = dev_alloc<double>(std::max(1, grid_->nif));
+    launch_gradient(x->val->p, grid_->if_c1->p, grid_->if_c2->p, v->p, grid_->nif);
+    return s_matmul(grid_->grad, x, v);
+}
+SP RuntimeCore::gradient_matrix(const SP& x) const {
+    EQ_REQUIRE(x->n == grid_->nc, "Gradient need input defined on AllCells()");
+    return s_matmul(grid_->grad, x);
+}
+SP RuntimeCore::divergence(const SP& x) const {
+    const DeviceGrid& g = *grid_;
+    EQ_REQUIRE(x->n == g.nf || x->n == g.nif, "Input for divergence has to be on AllFaces or on InteriorFaces()");
+    // values: cell sweep over the signed face list (ascending face order per cell)
+    const double* flux = x->val->p;
+    DevPtr<double> full;
+    if (x->n != g.nf) {
+        PMP pm = position_map(g.all_faces, g.interior_faces);
+        full = dev_alloc<double>(g.nf);
+        launch_gather(x->val->p, pm->d_inv->p, full->p, g.nf);
+        flux = full->p;
+    }
+    auto v = dev_alloc<double>(std::max(1, g.nc));
+    launch_divergence(flux, g.cell_facepos->p, g.signed_faces->p, v->p, g.nc);
+    return s_matmul(x->n == g.nf ? g.fulldiv : g.div, x, v);
+}
+SP RuntimeCore::divergence_matrix(const SP& x) const {
+    const DeviceGrid& g = *grid_;
+    EQ_REQUIRE(x->n == g.nf || x->n == g.nif, "Input for divergence has to be on AllFaces or on InteriorFaces()");
+    return s_matmul(x->n == g.nf ? g.fulldiv : g.div, x);
+}
+
+SP RuntimeCore::solve_for_update(const SP& residual, LinReport* rep) const {
+    return linear_solve(solver, residual, rep);
+}
+
+}  // namespace eqhip

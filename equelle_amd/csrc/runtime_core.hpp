@@ -1,0 +1,77 @@
+// Runtime core behind EquelleRuntimeCUDA: the device grid, HelperOps matrices,
+// grid built-ins and the Newton / Krylov settings read from the parameters.
+// Follows backends/cuda/src/EquelleRuntimeCUDA.cpp:29-346 and
+// backends/cuda/src/deviceGrid.cu (topology, geometry) for behaviour.
+#pragma once
+#include <map>
+#include <string>
+#include <vector>
+
+#include "core.hpp"
+#include "engine.hpp"
+#include "krylov.hpp"
+
+namespace eqhip {
+
+// Opm::ParameterGroup key=value subset.
+struct ParamMap {
+    std::map<std::string, std::string> kv;
+    void parse(const std::string& text);
+    bool has(const std::string& k) const { return kv.count(k) != 0; }
+    double get_double(const std::string& k, double def) const;
+    int get_int(const std::string& k, int def) const;
+    bool get_bool(const std::string& k, bool def) const;
+    std::string get_string(const std::string& k, const std::string& def) const;
+    std::string require(const std::string& k) const;
+};
+
+// Cartesian grid with OPM GridManager numbering (host build, device copy).
+struct GridHost {
+    int dim = 2, n[3] = {0, 1, 1};
+    double h[3] = {1, 1, 1};
+    int nc = 0, nf = 0;
+    std::vector<int> face_cells, cell_facepos, cell_faces;
+    std::vector<double> cell_centroids, cell_volumes, face_centroids, face_areas, face_normals;
+};
+GridHost build_cartesian(int dim, int nx, int ny, int nz, double dx, double dy, double dz);
+
+struct DeviceGrid {
+    int dim = 2, nc = 0, nf = 0, nif = 0;
+    GridHost host;
+    DevPtr<int> face_cells, cell_facepos, cell_faces, signed_faces, if_c1, if_c2;
+    DevPtr<double> cell_centroids, cell_volumes, face_centroids, face_areas, face_normals;
+    IP all_cells, all_faces, interior_faces, boundary_faces, interior_cells, boundary_cells;
+    DevCsr grad, div, fulldiv;   // HelperOps (deviceHelperOps.cpp:84-148)
+    explicit DeviceGrid(GridHost g);
+};
+
+class RuntimeCore {
+public:
+    explicit RuntimeCore(const ParamMap& p);
+    const ParamMap& params() const { return params_; }
+    DeviceGrid& grid() { return *grid_; }
+
+    IP firstCell(const IP& faces) const;
+    IP secondCell(const IP& faces) const;
+    SP norm(const IP& s) const;
+    VP centroid(const IP& s) const;
+    VP normal(const IP& faces) const;
+    SP gradient(const SP& x) const;
+    SP gradient_matrix(const SP& x) const;
+    SP divergence(const SP& x) const;
+    SP divergence_matrix(const SP& x) const;
+    SP solve_for_update(const SP& residual, LinReport* rep) const;
+
+    SolverConfig solver;
+    int max_iter = 10;
+    double abs_res_tol = 1e-6;
+    double rel_res_tol = 0.0;
+    int verbose = 0;
+    bool output_to_file = false;
+
+private:
+    ParamMap params_;
+    std::unique_ptr<DeviceGrid> grid_;
+};
+
+}  // namespace eqhip

@@ -1,0 +1,220 @@
+// equelleCUDA::EquelleRuntimeCUDA — the runtime class that code emitted by the
+// Equelle compiler's CUDA printer (compiler/PrintCUDABackendASTVisitor.cpp:18-59)
+// is written against.  Member names and signatures follow
+// backends/cuda/include/EquelleRuntimeCUDA.hpp:89-331; the implementation is the
+// MI355X runtime in libeqhip.so (HIP kernels on gfx950).
+//
+// Extensions over the reference (all optional, via parameters):
+//   rel_res_tol      relative Newton stop ||F(u_k)|| <= rel_res_tol ||F(u_0)||
+//                    (beside the reference's absolute abs_res_tol)
+//   precision        fp64 | mixed (FP32 Krylov + FP64 iterative refinement)
+//   preconditioner   diagonal | ilu0 | none
+//   gmres_restart, inner_tol, max_refine
+#ifndef EQUELLERUNTIMECUDA_HEADER_INCLUDED
+#define EQUELLERUNTIMECUDA_HEADER_INCLUDED
+
+#include <opm/common/ErrorMacros.hpp>
+#include <opm/common/utility/parameters/ParameterGroup.hpp>
+#include <opm/grid/GridManager.hpp>
+#include <opm/grid/UnstructuredGrid.h>
+
+#include <chrono>
+#include <iostream>
+#include <map>
+#include <memory>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "equelle_cuda_types.hpp"
+
+namespace eqhip {
+class RuntimeCore;
+}
+
+namespace equelleCUDA {
+
+// Statistics of the last newtonSolve (not in the reference; the reference only prints them).
+struct NewtonReport {
+    int iterations = 0;
+    int linear_iterations = 0;
+    double initial_norm = 0.0;
+    double final_norm = 0.0;
+    int stop_reason = 0;   // 1 = abs_res_tol met, 2 = rel_res_tol met, 0 = max_iter reached
+    double seconds = 0.0;
+};
+
+class EquelleRuntimeCUDA {
+public:
+    EquelleRuntimeCUDA(const Opm::ParameterGroup& param);
+    ~EquelleRuntimeCUDA();
+
+    // ----------- Topology and geometry
+    CollOfCell allCells() const;
+    CollOfCell boundaryCells() const;
+    CollOfCell interiorCells() const;
+    CollOfFace allFaces() const;
+    CollOfFace boundaryFaces() const;
+    CollOfFace interiorFaces() const;
+    CollOfCell firstCell(CollOfFace faces) const;
+    CollOfCell secondCell(CollOfFace faces) const;
+    template <int codim>
+    CollOfScalar norm(const CollOfIndices<codim>& set) const {
+        return normOfSet(set);
+    }
+    CollOfScalar norm(const CollOfVector& vectors) const;
+    template <int codim>
+    CollOfVector centroid(const CollOfIndices<codim>& set) const {
+        return centroidOfSet(set);
+    }
+    CollOfVector normal(const CollOfFace& faces) const;
+
+    // ----------- Operators and math functions
+    CollOfScalar dot(const CollOfVector& v1, const CollOfVector& v2) const;
+    CollOfScalar negGradient(const CollOfScalar& cell_scalarfield) const;
+    CollOfScalar interiorDivergence(const CollOfScalar& face_fluxes) const;
+    CollOfScalar gradient(const CollOfScalar& cell_scalarfield) const;
+    CollOfScalar gradient_matrix(const CollOfScalar& cell_scalarfield) const;
+    CollOfScalar divergence(const CollOfScalar& fluxes) const;
+    CollOfScalar divergence_matrix(const CollOfScalar& fluxes) const;
+    template <int codim>
+    CollOfBool isEmpty(const CollOfIndices<codim>& set) const {
+        return set.isEmpty();
+    }
+
+    // ----------- Extend and On
+    template <int codim>
+    CollOfScalar operatorExtend(const CollOfScalar& data_in, const CollOfIndices<codim>& from_set,
+                                const CollOfIndices<codim>& to_set) const {
+        return extendImpl(data_in, from_set, to_set);
+    }
+    template <int codim>
+    CollOfScalar operatorExtend(const Scalar& data, const CollOfIndices<codim>& set) {
+        return CollOfScalar(set.size(), data);
+    }
+    template <int codim>
+    CollOfScalar operatorOn(const CollOfScalar& data_in, const CollOfIndices<codim>& from_set,
+                            const CollOfIndices<codim>& to_set) {
+        return onImpl(data_in, from_set, to_set);
+    }
+    template <int codim_data, int codim_set>
+    CollOfIndices<codim_data> operatorOn(const CollOfIndices<codim_data>& in_data,
+                                         const CollOfIndices<codim_set>& from_set,
+                                         const CollOfIndices<codim_set>& to_set) {
+        return CollOfIndices<codim_data>(onIndicesImpl(in_data, from_set, to_set));
+    }
+    CollOfScalar trinaryIf(const CollOfBool& predicate, const CollOfScalar& iftrue,
+                           const CollOfScalar& iffalse) const;
+    template <int codim>
+    CollOfIndices<codim> trinaryIf(const CollOfBool& predicate, const CollOfIndices<codim>& iftrue,
+                                   const CollOfIndices<codim>& iffalse) const {
+        return CollOfIndices<codim>(trinaryIndicesImpl(predicate, iftrue, iffalse));
+    }
+
+    // ----------- Reductions and special functions
+    Scalar minReduce(const CollOfScalar& x) const;
+    Scalar maxReduce(const CollOfScalar& x) const;
+    Scalar sumReduce(const CollOfScalar& x) const;
+    Scalar prodReduce(const CollOfScalar& x) const;
+    CollOfScalar sqrt(const CollOfScalar& x) const;
+
+    // ----------- Implicit methods (EquelleRuntimeCUDA_impl.hpp:216-300)
+    template <class ResidualFunctor>
+    CollOfScalar newtonSolve(const ResidualFunctor& rescomp, const CollOfScalar& u_initialguess);
+
+    // ----------- Output
+    void output(const String& tag, Scalar val) const;
+    void output(const String& tag, const CollOfScalar& coll);
+
+    // ----------- Input
+    Scalar inputScalarWithDefault(const String& name, const Scalar default_value);
+    template <int codim>
+    CollOfIndices<codim> inputDomainSubsetOf(const String& name, CollOfIndices<codim> superset) {
+        CollOfIndices<codim> subset(readSubsetFile(name));
+        superset.contains(subset, name);
+        return subset;
+    }
+    template <int codim>
+    CollOfScalar inputCollectionOfScalar(const String& name, const CollOfIndices<codim>& coll) {
+        return inputCollectionImpl(name, coll.size());
+    }
+    SeqOfScalar inputSequenceOfScalar(const String& name);
+    void ensureGridDimensionMin(const int minimum_grid_dimension) const;
+
+    // ----------- Test hooks
+    UnstructuredGrid getGrid() const;
+    CudaMatrix getGradMatrix() const;
+    CudaMatrix getDivMatrix() const;
+    CudaMatrix getFulldivMatrix() const;
+    Scalar twoNormTester(const CollOfScalar& val) const { return twoNorm(val); }
+
+    // ----------- Extensions
+    const NewtonReport& lastNewtonReport() const { return last_newton_; }
+    CollOfScalar solveForUpdate(const CollOfScalar& residual, int* krylov_iterations = nullptr) const;
+    eqhip::RuntimeCore& core() const { return *core_; }
+
+private:
+    Scalar twoNorm(const CollOfScalar& vals) const;
+    CollOfScalar normOfSet(const IndexSetBase& set) const;
+    CollOfVector centroidOfSet(const IndexSetBase& set) const;
+    CollOfScalar extendImpl(const CollOfScalar& data, const IndexSetBase& from, const IndexSetBase& to) const;
+    CollOfScalar onImpl(const CollOfScalar& data, const IndexSetBase& from, const IndexSetBase& to) const;
+    std::shared_ptr<const eqhip::IndexData> onIndicesImpl(const IndexSetBase& data, const IndexSetBase& from,
+                                                          const IndexSetBase& to) const;
+    std::shared_ptr<const eqhip::IndexData> trinaryIndicesImpl(const CollOfBool& p, const IndexSetBase& t,
+                                                               const IndexSetBase& f) const;
+    std::vector<int> readSubsetFile(const String& name) const;
+    CollOfScalar inputCollectionImpl(const String& name, int size) const;
+    int newtonStop(double rn, double r0) const;
+    void newtonIterationMessage(int iter, double rn) const;
+    void newtonDoneMessage(int iter, int stop, double seconds) const;
+
+    std::unique_ptr<eqhip::RuntimeCore> core_;
+    const Opm::ParameterGroup& param_;
+    std::map<std::string, int> outputcount_;
+    NewtonReport last_newton_;
+    int max_iter_;
+    int verbose_;
+};
+
+template <class ResidualFunctor>
+CollOfScalar EquelleRuntimeCUDA::newtonSolve(const ResidualFunctor& rescomp, const CollOfScalar& u_initialguess) {
+    const auto t0 = std::chrono::steady_clock::now();
+    // Primary variable with identity Jacobian; the residual and its Jacobian are
+    // assembled by the AD engine on the device.
+    CollOfScalar u = CollOfScalar(u_initialguess, true);
+    CollOfScalar residual = rescomp(u);
+    const double r0 = twoNorm(residual);
+    double rn = r0;
+    int iter = 0;
+    NewtonReport rep;
+    newtonIterationMessage(iter, rn);
+    while (!newtonStop(rn, r0) && iter < max_iter_) {
+        int lin = 0;
+        CollOfScalar du = solveForUpdate(residual, &lin);
+        rep.linear_iterations += lin;
+        // du is constant: u keeps its identity Jacobian
+        u = u - du;
+        residual = rescomp(u);
+        rn = twoNorm(residual);
+        ++iter;
+        newtonIterationMessage(iter, rn);
+    }
+    rep.iterations = iter;
+    rep.initial_norm = r0;
+    rep.final_norm = rn;
+    rep.stop_reason = newtonStop(rn, r0);
+    rep.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    last_newton_ = rep;
+    newtonDoneMessage(iter, rep.stop_reason, rep.seconds);
+    return CollOfScalar(u.value());
+}
+
+}  // namespace equelleCUDA
+
+// Generated code declares `void ensureRequirements(const EquelleRuntimeCUDA& er);`
+// before its `using namespace equelleCUDA;` (PrintCUDABackendASTVisitor.cpp:38),
+// so the class name is visible at global scope as in the reference.
+using equelleCUDA::EquelleRuntimeCUDA;
+
+#endif  // EQUELLERUNTIMECUDA_HEADER_INCLUDED

@@ -1,0 +1,138 @@
+/* eqhip.h — C-ABI of the MI355X-native Equelle runtime (libeqhip.so).
+ *
+ * The reference exposes this path only as the C++ class
+ * equelleCUDA::EquelleRuntimeCUDA (backends/cuda/include/EquelleRuntimeCUDA.hpp:89-331);
+ * the same surface is available to C++ callers through include/EquelleRuntimeCUDA.hpp.
+ * This C-ABI is the binding a foreign caller (ctypes, cgo, JNI, N-API) uses.  Each
+ * entry point cites the reference member it replaces.
+ *
+ * Conventions
+ *   - Every function returns 0 on success and nonzero on error; the message of the
+ *     last error on the calling thread is eqhip_last_error().  Errors are where the
+ *     reference throws std::runtime_error (size mismatches, bad input sets, ...).
+ *   - A runtime (grid + HelperOps + solver settings) is an opaque pointer created from
+ *     a "key=value key=value" parameter string (Opm::ParameterGroup keys).
+ *   - Values are integer handles owned by the runtime: collections of scalars (with an
+ *     optional CSR Jacobian), index sets (CollOfCell / CollOfFace) and boolean
+ *     collections.  They live in device memory (HBM); release them with eqhip_release.
+ *   - All work is ordered on one HIP stream; calls that return host data synchronise.
+ *   - One controller thread per runtime; the library is not re-entrant per runtime.
+ */
+#ifndef EQHIP_H_INCLUDED
+#define EQHIP_H_INCLUDED
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ errors */
+const char* eqhip_last_error(void);
+
+/* ------------------------------------------------------------------ runtime */
+/* EquelleRuntimeCUDA::EquelleRuntimeCUDA(const Opm::ParameterGroup&) — EquelleRuntimeCUDA.cpp:65-82.
+ * Keys: grid_dim nx ny nz dx dy dz; solver (BiCGStab|CG|GMRes) preconditioner (diagonal|ilu0|none)
+ * solver_max_iter solver_tol; max_iter abs_res_tol verbose output_to_file; extensions:
+ * rel_res_tol precision (fp64|mixed) gmres_restart inner_tol max_refine. */
+int eqhip_runtime_create(const char* params, void** rt);
+/* ~EquelleRuntimeCUDA() — EquelleRuntimeCUDA.cpp:85-88 */
+int eqhip_runtime_destroy(void* rt);
+/* info[5] = {dim, cells, faces, interior faces, boundary faces}; UnstructuredGrid fields
+ * read by EquelleRuntimeCUDA::getGrid() — EquelleRuntimeCUDA.hpp:297 */
+int eqhip_grid_info(void* rt, int* info);
+/* which: 0 face_cells (2*faces), 1 cell_facepos (cells+1), 2 cell_faces — getGrid() */
+int eqhip_grid_int_array(void* rt, int which, int* out);
+/* which: 0 cell_centroids, 1 cell_volumes, 2 face_centroids, 3 face_areas, 4 face_normals — getGrid() */
+int eqhip_grid_double_array(void* rt, int which, double* out);
+/* HelperOps matrices, which: 0 grad, 1 div, 2 fulldiv — getGradMatrix/getDivMatrix/getFulldivMatrix,
+ * EquelleRuntimeCUDA.hpp:299-303.  info[3] = {rows, cols, nnz}. */
+int eqhip_opmat_info(void* rt, int which, int* info);
+int eqhip_opmat_copy(void* rt, int which, int* row_ptr, int* col_ind, double* vals);
+
+/* ------------------------------------------------------------------ index sets */
+/* which: 0 allCells 1 boundaryCells 2 interiorCells 3 allFaces 4 boundaryFaces 5 interiorFaces
+ * — EquelleRuntimeCUDA.hpp:100-110 */
+int eqhip_set_builtin(void* rt, int which, int* set);
+/* CollOfIndices<codim>(indices) — CollOfIndices.hpp:57 */
+int eqhip_set_from_host(void* rt, int codim, const int* idx, int n, int* set);
+int eqhip_set_size(void* rt, int set, int* n);                 /* CollOfIndices::size() */
+int eqhip_set_copy(void* rt, int set, int* out);               /* CollOfIndices::stdToHost() */
+int eqhip_first_cell(void* rt, int faces, int* set);           /* firstCell — EquelleRuntimeCUDA.hpp:116 */
+int eqhip_second_cell(void* rt, int faces, int* set);          /* secondCell — EquelleRuntimeCUDA.hpp:122 */
+int eqhip_set_on(void* rt, int data, int from, int to, int* set); /* operatorOn<cd,cs> — .hpp:207-210 */
+int eqhip_set_trinary(void* rt, int pred, int t, int f, int* set); /* trinaryIf<codim> — .hpp:223-226 */
+int eqhip_is_empty(void* rt, int set, int* pred);              /* isEmpty<codim> — .hpp:176-177 */
+/* superset.contains(subset) check of inputDomainSubsetOf — EquelleRuntimeCUDA_impl.hpp:98 */
+int eqhip_set_contains(void* rt, int superset, int subset);
+
+/* ------------------------------------------------------------------ booleans (CollOfBool) */
+int eqhip_bool_size(void* rt, int pred, int* n);
+int eqhip_bool_copy(void* rt, int pred, unsigned char* out);
+/* op: 0 >, 1 >=, 2 <, 3 <=, 4 ==, 5 != — CollOfScalar comparisons, CollOfScalar.hpp:282-407 */
+int eqhip_compare(void* rt, int op, int a, int b, int* pred);
+/* scalar_left: 0 => a OP s, 1 => s OP a */
+int eqhip_compare_scalar(void* rt, int op, int a, double s, int scalar_left, int* pred);
+
+/* ------------------------------------------------------------------ collections of scalars */
+int eqhip_scalar_from_host(void* rt, const double* v, int n, int* h);  /* CollOfScalar(std::vector) */
+int eqhip_scalar_uniform(void* rt, int n, double s, int* h);          /* CollOfScalar(size, value) */
+/* CollOfScalar(CudaArray, CudaMatrix): values + CSR Jacobian (rows = n) */
+int eqhip_scalar_with_jacobian(void* rt, const double* v, int n, const int* row_ptr, const int* col_ind,
+                               const double* jac, int cols, int* h);
+/* CollOfScalar(const CollOfScalar&, bool primaryVariable) — collOfScalar.cu:68-78 */
+int eqhip_primary(void* rt, int a, int* h);
+/* info[5] = {size, useAutoDiff, jac rows, jac cols, jac nnz}; empty Jacobian => zeros */
+int eqhip_scalar_info(void* rt, int h, int* info);
+int eqhip_scalar_copy(void* rt, int h, double* out);                              /* copyToHost() */
+int eqhip_jacobian_copy(void* rt, int h, int* row_ptr, int* col_ind, double* vals); /* matrixToHost() */
+/* kind: 0 scalar collection, 1 index set, 2 boolean collection */
+int eqhip_release(void* rt, int kind, int h);
+/* op: 0 +, 1 -, 2 *, 3 / — CollOfScalar operators, collOfScalar.cu:273-341 */
+int eqhip_binop(void* rt, int op, int a, int b, int* h);
+/* op 2: s*a; op 3: a/s (scalar_left 0) or s/a (scalar_left 1) — collOfScalar.cu:367-399 */
+int eqhip_scalar_binop(void* rt, int op, double s, int a, int scalar_left, int* h);
+int eqhip_negate(void* rt, int a, int* h);                      /* unary minus, collOfScalar.cu:401-403 */
+int eqhip_sqrt(void* rt, int a, int* h);                        /* sqrt — EquelleRuntimeCUDA.cpp:324-327 */
+int eqhip_trinary(void* rt, int pred, int t, int f, int* h);    /* trinaryIf — EquelleRuntimeCUDA.cpp:239-248 */
+int eqhip_gradient(void* rt, int a, int* h);                    /* gradient — EquelleRuntimeCUDA.cpp:251-264 */
+int eqhip_divergence(void* rt, int a, int* h);                  /* divergence — EquelleRuntimeCUDA.cpp:273-291 */
+int eqhip_extend(void* rt, int a, int from, int to, int* h);    /* operatorExtend — _impl.hpp:112-125 */
+int eqhip_extend_scalar(void* rt, double s, int to, int* h);    /* operatorExtend(Scalar, set) — _impl.hpp:127-131 */
+int eqhip_on(void* rt, int a, int from, int to, int* h);        /* operatorOn — _impl.hpp:134-147 */
+/* kind: 0 sum, 1 product, 2 max, 3 min, 4 twoNorm — EquelleRuntimeCUDA.cpp:148-151,332-346 */
+int eqhip_reduce(void* rt, int kind, int a, double* out);
+int eqhip_norm_set(void* rt, int set, int* h);                  /* norm<codim>(set) — _impl.hpp:171-182 */
+int eqhip_centroid(void* rt, int set, double* out);             /* centroid — _impl.hpp:203-210 (n*dim) */
+int eqhip_normal(void* rt, int faces, double* out);             /* normal — EquelleRuntimeCUDA.cpp:137-140 (unit) */
+/* The linear solve of one Newton step: J du = r with r = residual value, J its Jacobian
+ * — LinearSolver::solve, linearSolver.cu:107-201.  *iters = Krylov iterations. */
+int eqhip_solve_for_update(void* rt, int residual, int* du, int* iters);
+
+/* ------------------------------------------------------------------ CSR algebra (CudaMatrix) */
+/* op: 0 A+B, 1 A-B, 2 A*B on host CSR inputs, computed on the device — cudaMatrix.cu:640-694.
+ * An absent operand (null row_ptr) is the empty (zero) matrix.  out_info = {rows, cols, nnz}. */
+int eqhip_csr_binop(int op, int ar, int ac, const int* arp, const int* aci, const double* av, int br, int bc,
+                    const int* brp, const int* bci, const double* bv, int* out_info, int* orp, int* oci,
+                    double* ov, int cap);
+/* y = op(A) x — operator*(CudaMatrix, CudaArray), cudaMatrix.cu:698-733 */
+int eqhip_csr_spmv(int rows, int cols, const int* rp, const int* ci, const double* v, const double* x, double* y,
+                   int transpose);
+/* y = J x through the Krylov solver's SELL-64 kernels (fp32 != 0: FP32 kernel) for the
+ * Jacobian of collection h — the SpMV inside LinearSolver::solve. */
+int eqhip_sell_spmv(void* rt, int h, const double* x, double* y, int fp32);
+
+/* ------------------------------------------------------------------ workloads (heat) */
+/* Runs the heat program of examples/app/out_heateq.cpp:28-73 (k_model=const) or its
+ * nonlinear-conductivity form (k_model=quadratic) through the device AD engine:
+ * one newtonSolve per time step.  newton_iters / final_norms: per step (may be null). */
+int eqhip_heat_run(const char* params, const double* u_initial, int n, const int* dir_faces,
+                   const double* dir_vals, int ndir, const double* dts, int nsteps, double* u_out,
+                   int* newton_iters, double* final_norms);
+/* One residual + Jacobian evaluation of the same program; info = {n, nnz}; null outputs => query sizes. */
+int eqhip_heat_residual(const char* params, const double* u, const double* u0, int n, const int* dir_faces,
+                        const double* dir_vals, int ndir, double dt, int* info, double* F, int* row_ptr,
+                        int* col_ind, double* jac);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* EQHIP_H_INCLUDED */
