@@ -68,6 +68,14 @@ SIGNATURES = {
     "csr_binop": [_I, _I, _I, _IP, _IP, _DP, _I, _I, _IP, _IP, _DP, _IP, _IP, _IP, _DP, _I],
     "csr_spmv": [_I, _I, _IP, _IP, _DP, _DP, _DP, _I],
     "sell_spmv": [_P, _I, _DP, _DP, _I],
+    "heat_create": [_S, _DP, _I, _IP, _DP, _I, C.POINTER(_P)],
+    "heat_step": [_P, _D, _IP, _DP],
+    "heat_state": [_P, _DP],
+    "heat_destroy": [_P],
+    "timing": [_I],
+    "timing_query": [_S, _IP, _DP],
+    "device_sync": [],
+    "heat_newton_sample": [_S, _DP, _I, _I, _D, _DP, _IP],
     "heat_run": [_S, _DP, _I, _IP, _DP, _I, _DP, _I, _DP, _IP, _DP],
     "heat_residual": [_S, _DP, _DP, _I, _IP, _DP, _I, _D, _IP, _DP, _IP, _IP, _DP],
 }

@@ -12,6 +12,7 @@
 #include "kernels.hpp"
 #include "krylov.hpp"
 #include "runtime_core.hpp"
+#include "timing.hpp"
 
 using namespace eqhip;
 
@@ -432,6 +433,77 @@ int eqhip_heat_run(const char* params, const double* u_initial, int n, const int
         std::vector<double> h = u0.copyToHost();
         std::memcpy(u_out, h.data(), h.size() * sizeof(double));
     });
+}
+
+// ---------------- heat simulation sessions (device-resident state between steps)
+namespace {
+struct HeatSim {
+    Opm::ParameterGroup param;
+    std::unique_ptr<equelleCUDA::EquelleRuntimeCUDA> er;
+    std::unique_ptr<HeatProgramCUDA> prog;
+    equelleCUDA::CollOfScalar u;
+};
+}  // namespace
+
+int eqhip_heat_create(const char* params, const double* u_initial, int n, const int* dir_faces,
+                      const double* dir_vals, int ndir, void** sim) {
+    return guard([&] {
+        auto s = std::make_unique<HeatSim>();
+        s->param = to_group(params);
+        s->er.reset(new equelleCUDA::EquelleRuntimeCUDA(s->param));
+        EQ_REQUIRE(n == s->er->core().grid().nc, "u_initial has " << n << " values for "
+                                                               << s->er->core().grid().nc << " cells");
+        s->prog.reset(new HeatProgramCUDA(*s->er, std::vector<int>(dir_faces, dir_faces + ndir),
+                                          std::vector<double>(dir_vals, dir_vals + ndir)));
+        s->u = equelleCUDA::CollOfScalar(std::vector<double>(u_initial, u_initial + n));
+        *sim = s.release();
+    });
+}
+
+int eqhip_heat_step(void* sim, double dt, int* stats, double* norms) {
+    return guard([&] {
+        EQ_REQUIRE(sim, "null simulation");
+        HeatSim& s = *static_cast<HeatSim*>(sim);
+        const equelleCUDA::CollOfScalar u0 = s.u;
+        auto rescomp = [&](const equelleCUDA::CollOfScalar& u) { return s.prog->residual(u, u0, dt); };
+        s.u = s.er->newtonSolve(rescomp, u0);
+        const equelleCUDA::NewtonReport& r = s.er->lastNewtonReport();
+        if (stats) {
+            stats[0] = r.iterations;
+            stats[1] = r.linear_iterations;
+            stats[2] = r.stop_reason;
+        }
+        if (norms) {
+            norms[0] = r.initial_norm;
+            norms[1] = r.final_norm;
+        }
+    });
+}
+
+int eqhip_heat_state(void* sim, double* u) {
+    return guard([&] {
+        EQ_REQUIRE(sim, "null simulation");
+        std::vector<double> h = static_cast<HeatSim*>(sim)->u.copyToHost();
+        std::memcpy(u, h.data(), h.size() * sizeof(double));
+    });
+}
+
+int eqhip_heat_destroy(void* sim) {
+    return guard([&] { delete static_cast<HeatSim*>(sim); });
+}
+
+// ---------------- timing / device
+int eqhip_timing(int mode) {
+    return guard([&] {
+        if (mode == 2) timing_reset();
+        else timing_enable(mode != 0);
+    });
+}
+int eqhip_timing_query(const char* name, int* count, double* total_ms) {
+    return guard([&] { timing_query(name ? name : "", count, total_ms); });
+}
+int eqhip_device_sync(void) {
+    return guard([&] { Device::get().sync(); });
 }
 
 int eqhip_heat_residual(const char* params, const double* u, const double* u0, int n, const int* dir_faces,

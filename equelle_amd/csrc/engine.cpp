@@ -70,35 +70,33 @@ SymP sym_spgemm(const PatternP& m, const PatternP& a) {
     EQ_REQUIRE(m->cols == a->rows, "Error in CudaMatrix * CudaMatrix size checking: lhs.cols_ = "
                                        << m->cols << " rhs.rows_ = " << a->rows);
     std::vector<int> rp(m->rows + 1, 0), ci, seg(1, 0), mi, ai;
-    std::vector<int> mark(a->cols, -1), slot(a->cols, 0), cols;
-    std::vector<std::vector<std::pair<int, int>>> acc;
+    ci.reserve((size_t)m->nnz * 3);
+    seg.reserve((size_t)m->nnz * 3 + 1);
+    mi.reserve((size_t)m->nnz * 4);
+    ai.reserve((size_t)m->nnz * 4);
+    struct Tri {
+        int col, t, e;
+    };
+    std::vector<Tri> row;
     for (int r = 0; r < m->rows; ++r) {
-        cols.clear();
-        acc.clear();
+        row.clear();
         for (int t = m->rp[r]; t < m->rp[r + 1]; ++t) {
             const int k = m->ci[t];
-            for (int e = a->rp[k]; e < a->rp[k + 1]; ++e) {
-                const int c = a->ci[e];
-                if (mark[c] != r) {
-                    mark[c] = r;
-                    slot[c] = (int)cols.size();
-                    cols.push_back(c);
-                    acc.emplace_back();
-                }
-                acc[slot[c]].emplace_back(t, e);
-            }
+            for (int e = a->rp[k]; e < a->rp[k + 1]; ++e) row.push_back({a->ci[e], t, e});
         }
-        std::vector<int> order(cols.size());
-        for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
-        std::sort(order.begin(), order.end(), [&](int x, int y) { return cols[x] < cols[y]; });
-        for (int o : order) {
-            ci.push_back(cols[o]);
-            for (auto& pr : acc[o]) {
-                mi.push_back(pr.first);
-                ai.push_back(pr.second);
+        // output columns ascending; contributions to one entry in matrix-entry order
+        std::sort(row.begin(), row.end(), [](const Tri& x, const Tri& y) {
+            return x.col != y.col ? x.col < y.col : x.t < y.t;
+        });
+        for (size_t i = 0; i < row.size(); ++i) {
+            if (i == 0 || row[i].col != row[i - 1].col) {
+                if (i > 0) seg.push_back((int)mi.size());
+                ci.push_back(row[i].col);
             }
-            seg.push_back((int)mi.size());
+            mi.push_back(row[i].t);
+            ai.push_back(row[i].e);
         }
+        if (!row.empty()) seg.push_back((int)mi.size());
         rp[r + 1] = (int)ci.size();
     }
     auto e = std::make_shared<SymEntry>();

@@ -9,6 +9,7 @@
 #include "engine.hpp"
 #include "kernels.hpp"
 #include "krylov.hpp"
+#include "timing.hpp"
 
 namespace eqhip {
 
@@ -492,6 +493,7 @@ struct System {
         part = dev_alloc<double>(3 * (size_t)nb + 8);
     }
     void spmv(const T* x, T* y) const {
+        TimedLaunch tl(sizeof(T) == 8 ? "spmv_fp64" : "spmv_fp32");
         KL(k_sell_spmv<T>, nblk((long)L->nslices * W), n, L->nslices, L->slice_ptr->p, L->col->p, sell->p, x, y);
     }
     // z = M^{-1} r  (not used for the fused Jacobi paths)

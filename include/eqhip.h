@@ -132,6 +132,25 @@ int eqhip_heat_residual(const char* params, const double* u, const double* u0, i
                         const double* dir_vals, int ndir, double dt, int* info, double* F, int* row_ptr,
                         int* col_ind, double* jac);
 
+/* A heat simulation session: the program above on one runtime with the state kept in
+ * device memory between steps (the time loop of out_heateq.cpp:64-73).
+ * stats = {Newton iterations, Krylov iterations, stop reason (1 abs, 2 rel, 0 max_iter)},
+ * norms = {initial, final residual 2-norm}; either may be null. */
+int eqhip_heat_create(const char* params, const double* u_initial, int n, const int* dir_faces,
+                      const double* dir_vals, int ndir, void** sim);
+int eqhip_heat_step(void* sim, double dt, int* stats, double* norms);
+int eqhip_heat_state(void* sim, double* u);
+int eqhip_heat_destroy(void* sim);
+
+/* ------------------------------------------------------------------ measurement */
+/* Live kernel timing with HIP events on the runtime stream: mode 1 on, 0 off, 2 reset.
+ * Timed launches: "spmv_fp64", "spmv_fp32" (Krylov SpMV). No reference counterpart
+ * (the reference only prints wall times, EquelleRuntimeCUDA_impl.hpp:295-297). */
+int eqhip_timing(int mode);
+int eqhip_timing_query(const char* name, int* count, double* total_ms);
+/* Waits for all work queued by the runtime. */
+int eqhip_device_sync(void);
+
 #ifdef __cplusplus
 }
 #endif
