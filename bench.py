@@ -1,21 +1,25 @@
 #!/usr/bin/env python3
 """Benchmark: nonlinear heat on a 2048 x 2048 grid, Newton to 1e-8 relative residual.
 
-One step = one implicit time step (one newtonSolve: residual + Jacobian assembly by
-the device AD engine, Krylov solve, update) of u_t = div(k(u) grad u),
-k(u) = 0.1 (1 + u^2), dt = 1, on device-resident state.  The headline run uses the
-mixed-precision solver (FP32 BiCGStab + multicolour ILU(0), FP64 residual, Jacobian,
-defect and convergence test); the plain FP64 solver runs on the same input for
-comparison.  Metric: cell-Newton-iterations per second = cells x Newton iterations /
-wall seconds over the timed steps.
+One step = one implicit time step (one Newton solve: residual + Jacobian assembly,
+Krylov solve, update) of u_t = div(k(u) grad u), k(u) = 0.1 (1 + u^2), dt = 1,
+zero-flux boundary, on device-resident state.  Metric: cell-Newton-iterations per
+second = cells x Newton iterations / wall seconds over the timed steps.
 
-roofline: the Krylov SpMV (dominant kernel), algorithmic bytes per launch
-8 nnz + 12 n + 4 (FP32 values, int32 indices, x read once, y written once),
-timed live with HIP events on the runtime stream over the timed steps.
-cpu_baseline: the CPU oracle (serial C++ restatement of the reference serial
-backend) timed for one Newton iteration of the same workload on one host core.
+Runs on the same seeded input (16 Gaussian bumps + noise):
+  headline     fused engine, mixed precision: FP32 BiCGStab + red-black ILU(0),
+               FP64 residual, Jacobian, defect (iterative refinement) and Newton test
+  plain_fp64   fused engine, FP64 BiCGStab + red-black ILU(0)
+  generic_fp64 the drop-in path: the same program through EquelleRuntimeCUDA's
+               device AD engine (CSR Jacobian, SELL-64 SpMV, multicolour ILU(0))
+roofline: the Krylov SpMV of the headline run (dominant kernel): 5-point stencil
+coefficients in FP32, algorithmic bytes per launch 28 n (5 coefficients + x + y,
+4 B each), average duration from HIP events recorded on the runtime stream around
+every launch in the timed region.
+cpu_baseline: the CPU oracle (serial C++ restatement of the reference serial backend)
+timed for one Newton iteration of the same workload on one host core.
 
-  python bench.py [--steps K] [--warmup W] [--size N] [--no-cpu-baseline]
+  python bench.py [--steps K] [--warmup W] [--size N] [--no-cpu-baseline] [--quick]
 """
 import argparse
 import ctypes as C
@@ -40,51 +44,80 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_heat(lib, params, u0, dt, steps, warmup):
-    n = u0.size
-    sim = C.c_void_p()
-    e, ed = np.zeros(1, dtype=np.int32), np.zeros(1)
-    t0 = time.perf_counter()
-    lib.heat_create(params.encode(), dptr(u0), n, iptr(e), dptr(ed), 0, C.byref(sim))
-    setup_s = time.perf_counter() - t0
-    stats = np.zeros(3, dtype=np.int32)
-    norms = np.zeros(2)
+def timing_avg(lib, name):
+    cnt, tot = C.c_int(), C.c_double()
+    lib.timing_query(name.encode(), C.byref(cnt), C.byref(tot))
+    return cnt.value, (tot.value / cnt.value if cnt.value else float("nan"))
+
+
+def run_fused(lib, precision, u0, nx, ny, dt, steps, warmup, precond="ilu0"):
+    params = (f"precision={precision} preconditioner={precond} rel_res_tol={NONLINEAR_HEAT['rel_res_tol']} "
+              "abs_res_tol=0 max_iter=20 solver_tol=1e-6 inner_tol=1e-4").encode()
+    b = C.c_void_p()
+    lib.batch_create(params, C.byref(b))
+    slot = C.c_int()
+    st = np.zeros(5, dtype=np.int32)
+    nrm = np.zeros(2)
     try:
+        lib.batch_add(b, nx, ny, NONLINEAR_HEAT["k"], 1.0, dptr(u0), C.byref(slot))
+        slots = np.array([slot.value], dtype=np.int32)
         for w in range(warmup):
             t = time.perf_counter()
-            lib.heat_step(sim, dt, iptr(stats), dptr(norms))
-            log(f"  warmup step {w}: newton={stats[0]} krylov={stats[1]} stop={stats[2]} "
-                f"{time.perf_counter() - t:.3f}s")
+            lib.batch_step(b, iptr(slots), 1, dt)
+            lib.batch_stats(b, slot.value, iptr(st), dptr(nrm))
+            log(f"  warmup {w}: newton={st[0]} krylov={st[1]} refine={st[2]} {time.perf_counter() - t:.4f}s")
         lib.device_sync()
-        lib.timing(2)
-        lib.timing(1)
         newton = krylov = 0
-        per_step = []
         t0 = time.perf_counter()
         for s in range(steps):
             t = time.perf_counter()
-            lib.heat_step(sim, dt, iptr(stats), dptr(norms))
-            per_step.append(time.perf_counter() - t)
-            newton += int(stats[0])
-            krylov += int(stats[1])
-            if stats[2] == 0:
-                raise RuntimeError(f"Newton did not converge in step {s}: ||F|| {norms[0]:.3e} -> {norms[1]:.3e}")
-            log(f"  step {s}: newton={stats[0]} krylov={stats[1]} rel={norms[1] / norms[0]:.2e} {per_step[-1]:.3f}s")
+            lib.batch_step(b, iptr(slots), 1, dt)
+            lib.batch_stats(b, slot.value, iptr(st), dptr(nrm))
+            if st[3] == 0 or st[4]:
+                raise RuntimeError(f"step {s} did not converge: stats {st.tolist()} norms {nrm.tolist()}")
+            newton += int(st[0])
+            krylov += int(st[1])
+            log(f"  step {s}: newton={st[0]} krylov={st[1]} refine={st[2]} rel={nrm[1] / nrm[0]:.2e} "
+                f"{time.perf_counter() - t:.4f}s")
         lib.device_sync()
         secs = time.perf_counter() - t0
+        # kernel durations: one more step with HIP events around every timed launch
+        # (eager launches; the timed steps above replay captured graphs)
+        lib.timing(2)
+        lib.timing(1)
+        lib.batch_step(b, iptr(slots), 1, dt)
+        lib.device_sync()
         lib.timing(0)
-        spmv = {}
-        for name in ("spmv_fp32", "spmv_fp64"):
-            cnt, tot = C.c_int(), C.c_double()
-            lib.timing_query(name.encode(), C.byref(cnt), C.byref(tot))
-            if cnt.value:
-                spmv[name] = (cnt.value, tot.value / cnt.value)
-        u = np.zeros(n)
-        lib.heat_state(sim, dptr(u))
+        kern = {name: timing_avg(lib, name) for name in
+                ("stencil_spmv_fp32", "stencil_spmv_fp64", "assemble", "defect_fp64")}
+        u = np.zeros(nx * ny)
+        lib.batch_state(b, slot.value, dptr(u))
+    finally:
+        lib.batch_destroy(b)
+    return dict(seconds=secs, newton=newton, krylov=krylov, kern=kern, u=u)
+
+
+def run_generic(lib, u0, nx, ny, dt, steps, warmup):
+    params = heat_params(nx, ny, NONLINEAR_HEAT["k"], precision="fp64", precond="ilu0",
+                         rel_res_tol=NONLINEAR_HEAT["rel_res_tol"], solver_tol=1e-6)
+    sim = C.c_void_p()
+    e, ed = np.zeros(1, dtype=np.int32), np.zeros(1)
+    lib.heat_create(params.encode(), dptr(u0), u0.size, iptr(e), dptr(ed), 0, C.byref(sim))
+    stats = np.zeros(3, dtype=np.int32)
+    try:
+        for _ in range(warmup):
+            lib.heat_step(sim, dt, iptr(stats), None)
+        lib.device_sync()
+        newton = 0
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            lib.heat_step(sim, dt, iptr(stats), None)
+            newton += int(stats[0])
+        lib.device_sync()
+        secs = time.perf_counter() - t0
     finally:
         lib.heat_destroy(sim)
-    return dict(seconds=secs, newton=newton, krylov=krylov, per_step=per_step, spmv=spmv, setup_s=setup_s,
-                u=u)
+    return dict(seconds=secs, newton=newton)
 
 
 def cpu_baseline(n_side, u0, dt):
@@ -107,11 +140,12 @@ def cpu_baseline(n_side, u0, dt):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=4)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--size", type=int, default=NONLINEAR_HEAT["nx"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 comparison run")
+    ap.add_argument("--quick", action="store_true", help="headline run only (for profiling)")
+    ap.add_argument("--precond", default="ilu0", help="preconditioner of the fused runs: ilu0 | diagonal | none")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -123,17 +157,14 @@ def main():
     nnz = n + 2 * nif
     dt = NONLINEAR_HEAT["dt"]
     u0 = gaussian_bumps(nx, ny, seed=2048)
-    runs = {}
-    for prec in (["mixed"] if args.no_plain else ["mixed", "fp64"]):
-        log(f"[bench] nonlinear heat {nx}x{ny}, precision={prec}")
-        params = heat_params(nx, ny, NONLINEAR_HEAT["k"], precision=prec, precond="ilu0",
-                             rel_res_tol=NONLINEAR_HEAT["rel_res_tol"], solver_tol=1e-6)
-        runs[prec] = run_heat(lib, params, u0, dt, args.steps, args.warmup)
-    head = runs["mixed"]
+    log(f"[bench] nonlinear heat {nx}x{ny}: fused engine, mixed precision")
+    head = run_fused(lib, "mixed", u0, nx, ny, dt, args.steps, args.warmup, args.precond)
     value = n * head["newton"] / head["seconds"]
-    cnt, avg_ms = head["spmv"].get("spmv_fp32", (0, float("nan")))
-    spmv_bytes = 8 * nnz + 12 * n + 4
+    cnt, avg_ms = head["kern"]["stencil_spmv_fp32"]
+    spmv_bytes = 28 * n
     achieved = spmv_bytes / (avg_ms * 1e-3) / 1e9
+    acnt, aavg = head["kern"]["assemble"]
+    asm_bytes = 84 * n   # read u, u0; write F, 5 FP64 + 5 FP32 coefficients
     out = {
         "metric": "cell-Newton-iters/s",
         "value": value,
@@ -147,28 +178,37 @@ def main():
         "vs_baseline": None,
         "dtype": "f64 residual/Jacobian/refinement, f32 Krylov",
         "data": "synthetic",
-        "config": {"workload": f"nonlinear heat {nx}x{ny}, k(u)=0.1(1+u^2), dt=1, Newton to 1e-8 relative "
-                               "residual, mixed precision (FP32 BiCGStab+ILU0, FP64 refinement)",
-                   "cells": n, "jacobian_nnz": nnz, "engine": "device AD engine (generic program path)"},
+        "config": {"workload": f"nonlinear heat {nx}x{ny}, k(u)=0.1(1+u^2), dt=1, zero-flux, Newton to 1e-8 "
+                               "relative residual; mixed precision (FP32 BiCGStab + red-black ILU(0), FP64 "
+                               "refinement to 1e-6)",
+                   "cells": n, "jacobian_nnz": nnz, "engine": "fused heat engine (1 instance)"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_sell_spmv<float>", "launches": cnt, "avg_us": 1e3 * avg_ms,
+                     "kernel": "k_spmv<float> (5-point stencil SpMV)", "launches": cnt, "avg_us": 1e3 * avg_ms,
                      "algorithmic_bytes": spmv_bytes},
+        "assembly_roofline": {"kernel": "k_assemble<true>", "launches": acnt, "avg_us": 1e3 * aavg,
+                              "algorithmic_bytes": asm_bytes,
+                              "achieved_GBs": asm_bytes / (aavg * 1e-3) / 1e9,
+                              "frac": asm_bytes / (aavg * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "newton_iterations": head["newton"],
         "krylov_iterations": head["krylov"],
         "cell_steps_per_s": n * args.steps / head["seconds"],
     }
-    if "fp64" in runs:
-        p = runs["fp64"]
+    if not args.quick:
+        log("[bench] plain FP64 (fused engine)")
+        p = run_fused(lib, "fp64", u0, nx, ny, dt, args.steps, args.warmup, args.precond)
         pv = n * p["newton"] / p["seconds"]
-        c64, a64 = p["spmv"].get("spmv_fp64", (0, float("nan")))
-        b64 = 12 * nnz + 20 * n + 4
+        c64, a64 = p["kern"]["stencil_spmv_fp64"]
         out["plain_fp64"] = {"value": pv, "ms_per_step": 1e3 * p["seconds"] / args.steps,
                              "newton_iterations": p["newton"], "krylov_iterations": p["krylov"],
-                             "spmv_avg_us": 1e3 * a64, "spmv_GBs": b64 / (a64 * 1e-3) / 1e9}
+                             "spmv_avg_us": 1e3 * a64, "spmv_GBs": 56 * n / (a64 * 1e-3) / 1e9}
         out["mixed_over_plain"] = value / pv
-        rel = np.linalg.norm(head["u"] - p["u"]) / np.linalg.norm(p["u"])
-        out["mixed_vs_fp64_rel_error"] = float(rel)
+        out["mixed_vs_fp64_rel_error"] = float(np.linalg.norm(head["u"] - p["u"]) / np.linalg.norm(p["u"]))
+        log("[bench] generic drop-in path (EquelleRuntimeCUDA AD engine, FP64)")
+        gsteps = max(2, args.steps // 4)
+        g = run_generic(lib, u0, nx, ny, dt, gsteps, 1)
+        out["generic_fp64"] = {"value": n * g["newton"] / g["seconds"], "steps": gsteps,
+                               "ms_per_step": 1e3 * g["seconds"] / gsteps}
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (serial oracle, one Newton iteration)")
         out["cpu_baseline"] = cpu_baseline(nx, u0, dt)

@@ -72,6 +72,19 @@ SIGNATURES = {
     "heat_step": [_P, _D, _IP, _DP],
     "heat_state": [_P, _DP],
     "heat_destroy": [_P],
+    "batch_create": [_S, C.POINTER(_P)],
+    "batch_destroy": [_P],
+    "batch_add": [_P, _I, _I, _D, _D, _DP, _IP],
+    "batch_remove": [_P, _I],
+    "batch_begin_step": [_P, _I, _D],
+    "batch_iterate": [_P, _IP, _I, _IP],
+    "batch_in_flight": [_P, _I, _IP],
+    "batch_step": [_P, _IP, _I, _D],
+    "batch_stats": [_P, _I, _IP, _DP],
+    "batch_state": [_P, _I, _DP],
+    "batch_set_state": [_P, _I, _DP],
+    "batch_bytes": [_P, _I, _DP],
+    "batch_residual": [_P, _I, _DP, _DP, _D, _IP, _DP, _IP, _IP, _DP],
     "timing": [_I],
     "timing_query": [_S, _IP, _DP],
     "device_sync": [],

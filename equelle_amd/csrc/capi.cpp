@@ -1,6 +1,7 @@
 // C-ABI of the runtime (include/eqhip.h): handle tables over the device engine.
 #include <eqhip.h>
 
+#include <algorithm>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -8,6 +9,7 @@
 
 #include "core.hpp"
 #include "engine.hpp"
+#include "heat_fused.hpp"
 #include "heat_program.hpp"
 #include "kernels.hpp"
 #include "krylov.hpp"
@@ -490,6 +492,99 @@ int eqhip_heat_state(void* sim, double* u) {
 
 int eqhip_heat_destroy(void* sim) {
     return guard([&] { delete static_cast<HeatSim*>(sim); });
+}
+
+// ---------------- fused heat batches
+static FusedConfig fused_config(const char* params) {
+    ParamMap m;
+    m.parse(params ? params : "");
+    FusedConfig c;
+    c.precision = m.get_string("precision", c.precision);
+    std::string pc = m.get_string("preconditioner", m.get_string("precond", c.precond));
+    std::transform(pc.begin(), pc.end(), pc.begin(), ::tolower);
+    c.precond = pc == "diagonal" ? "jacobi" : pc;
+    c.rel_res_tol = m.get_double("rel_res_tol", c.rel_res_tol);
+    c.abs_res_tol = m.get_double("abs_res_tol", c.abs_res_tol);
+    c.max_iter = m.get_int("max_iter", c.max_iter);
+    c.solver_tol = m.get_double("solver_tol", c.solver_tol);
+    c.solver_max_iter = m.get_int("solver_max_iter", c.solver_max_iter);
+    c.inner_tol = m.get_double("inner_tol", c.inner_tol);
+    c.max_refine = m.get_int("max_refine", c.max_refine);
+    c.graph = m.get_int("graph", 1) != 0;
+    return c;
+}
+static HeatBatch& B(void* b) {
+    EQ_REQUIRE(b, "null batch");
+    return *static_cast<HeatBatch*>(b);
+}
+
+int eqhip_batch_create(const char* params, void** batch) {
+    return guard([&] { *batch = new HeatBatch(fused_config(params)); });
+}
+int eqhip_batch_destroy(void* batch) {
+    return guard([&] { delete static_cast<HeatBatch*>(batch); });
+}
+int eqhip_batch_add(void* batch, int nx, int ny, double k, double cv, const double* u, int* slot) {
+    return guard([&] { *slot = B(batch).add(nx, ny, k, cv, u); });
+}
+int eqhip_batch_remove(void* batch, int slot) {
+    return guard([&] { B(batch).remove(slot); });
+}
+int eqhip_batch_begin_step(void* batch, int slot, double dt) {
+    return guard([&] { B(batch).begin_step(slot, dt); });
+}
+int eqhip_batch_iterate(void* batch, int* finished, int cap, int* nfinished) {
+    return guard([&] {
+        std::vector<int> done = B(batch).iterate();
+        EQ_REQUIRE((int)done.size() <= cap || !finished, "finished-slot buffer too small");
+        if (finished) std::copy(done.begin(), done.end(), finished);
+        *nfinished = (int)done.size();
+    });
+}
+int eqhip_batch_in_flight(void* batch, int slot, int* flag) {
+    return guard([&] { *flag = B(batch).in_flight(slot) ? 1 : 0; });
+}
+int eqhip_batch_step(void* batch, const int* slots, int nslots, double dt) {
+    return guard([&] { B(batch).step(std::vector<int>(slots, slots + nslots), dt); });
+}
+int eqhip_batch_stats(void* batch, int slot, int* stats, double* norms) {
+    return guard([&] {
+        const StepStats& s = B(batch).stats(slot);
+        if (stats) {
+            stats[0] = s.newton;
+            stats[1] = s.krylov;
+            stats[2] = s.refinements;
+            stats[3] = s.stop;
+            stats[4] = s.failed ? 1 : 0;
+        }
+        if (norms) {
+            norms[0] = s.r0;
+            norms[1] = s.rn;
+        }
+    });
+}
+int eqhip_batch_state(void* batch, int slot, double* u) {
+    return guard([&] { B(batch).get_state(slot, u); });
+}
+int eqhip_batch_set_state(void* batch, int slot, const double* u) {
+    return guard([&] { B(batch).set_state(slot, u); });
+}
+int eqhip_batch_bytes(void* batch, int slot, double* bytes) {
+    return guard([&] { *bytes = (double)B(batch).device_bytes(slot); });
+}
+int eqhip_batch_residual(void* batch, int slot, const double* u, const double* u0, double dt, int* info, double* F,
+                         int* rp, int* ci, double* v) {
+    return guard([&] {
+        std::vector<double> f, vv;
+        std::vector<int> r, c;
+        B(batch).export_residual(slot, u, u0, dt, f, r, c, vv);
+        info[0] = (int)f.size();
+        info[1] = (int)c.size();
+        std::copy(f.begin(), f.end(), F);
+        std::copy(r.begin(), r.end(), rp);
+        std::copy(c.begin(), c.end(), ci);
+        std::copy(vv.begin(), vv.end(), v);
+    });
 }
 
 // ---------------- timing / device

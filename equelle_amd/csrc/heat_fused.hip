@@ -1,0 +1,929 @@
+// Fused nonlinear-heat engine (see heat_fused.hpp).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <map>
+
+#include "heat_fused.hpp"
+#include "kernels.hpp"
+#include "timing.hpp"
+
+namespace eqhip {
+
+namespace {
+constexpr int BS = 256;   // cells per block (4 wave64)
+inline hipStream_t S() { return Device::get().stream(); }
+
+enum Slot { S_RHO = 0, S_RHO_OLD, S_ALPHA, S_OMEGA, S_RR, S_BB, S_BREAK, S_IT, S_FN, S_RN, S_BN, S_REF, S_ITT, NSLOT = 16 };
+enum Vec { V_R = 0, V_RH, V_P, V_V, V_S, V_T, V_PH, V_SH, V_X, NVEC };
+enum Act { A_INIT = 0, A_ALPHA, A_OMEGA, A_RHO, A_STORE };
+enum Pc { PC_NONE = 0, PC_JACOBI = 1, PC_ILU = 2 };
+
+struct Desc {
+    int n, nx, ny, pad;
+    double k, cv, dt, gx, gy, vol;
+    double *u, *u0, *F, *du, *r64;
+    double *C, *W, *E, *S, *N;   // FP64 Jacobian stencil coefficients
+    void *Cm, *Wm, *Em, *Sm, *Nm; // Krylov-precision coefficients (alias the FP64 ones in fp64 mode)
+    void* dinv;                   // Jacobi: 1/C ; ILU: 1/pivot (black) or 1/C (red)
+    void* v[NVEC];                // Krylov vectors
+    double* sc;                   // NSLOT scalars
+    int* act;                     // [0] Krylov active, [1] refinement active
+};
+
+// What the last block of an instance does with the summed partials.
+struct Fin {
+    int act, nd, maxit, slot;
+    double tol2;
+};
+
+__device__ inline void bred_store(double v, double* out) {
+    __shared__ double lds[BS / 64];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) lds[w] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int i = 0; i < BS / 64; ++i) s += lds[i];
+        *out = s;
+    }
+    __syncthreads();
+}
+
+// Each block covers CPB = 4*BS consecutive cells of one instance; thread t handles
+// cells base + t, base + BS + t, ... (coalesced), so reductions have one partial per
+// 1024 cells.
+constexpr int CPT = 4;
+constexpr int CPB = CPT * BS;
+#define LOCATE()                                  \
+    const int2 bi = blk[blockIdx.x];              \
+    const Desc& d = D[bi.x];                      \
+    const long base = (long)bi.y * CPB + threadIdx.x
+#define FOR_CELLS(c) for (long c = base, c##_end = min((long)d.n, base + CPB); c < c##_end; c += BS)
+
+// Per-instance finalisation, run by one whole block: sums the instance's block
+// partials in a fixed order (deterministic), then thread 0 applies the action to
+// the instance's scalars.
+__device__ void fin_body(const Desc& d, int2 r, const double* part, int nb, const Fin& f) {
+    __shared__ double res[2];
+    const volatile double* vp = part;   // written by other blocks: read past the L1
+    for (int q = 0; q < f.nd; ++q) {
+        double s = 0.0;
+        for (int b = r.x + threadIdx.x; b < r.y; b += BS) s += vp[(long)q * nb + b];
+        bred_store(s, &res[q]);
+    }
+    if (threadIdx.x != 0) return;
+    const int act = f.act, maxit = f.maxit, slot = f.slot;
+    const double tol2 = f.tol2;
+    double* sc = d.sc;
+    switch (act) {
+        case A_INIT: {   // start of a Krylov solve for instances under refinement
+            if (!d.act[1]) {
+                d.act[0] = 0;
+                break;
+            }
+            sc[S_RHO] = res[0];
+            sc[S_RR] = res[1];
+            sc[S_BB] = res[1];
+            sc[S_IT] = 0.0;
+            sc[S_BREAK] = 0.0;
+            sc[S_ALPHA] = sc[S_OMEGA] = sc[S_RHO_OLD] = 1.0;
+            d.act[0] = res[1] > 0.0 && isfinite(res[1]);
+            break;
+        }
+        case A_ALPHA:
+            if (!d.act[0]) break;
+            if (res[0] == 0.0 || !isfinite(res[0])) {
+                sc[S_BREAK] = 1.0;
+                d.act[0] = 0;
+            } else sc[S_ALPHA] = sc[S_RHO] / res[0];
+            break;
+        case A_OMEGA:
+            if (!d.act[0]) break;
+            if (res[1] == 0.0 || !isfinite(res[1])) {
+                sc[S_OMEGA] = 0.0;   // s already solves the system; finish with x += alpha ph
+            } else sc[S_OMEGA] = res[0] / res[1];
+            break;
+        case A_RHO: {
+            if (!d.act[0]) break;
+            sc[S_RHO_OLD] = sc[S_RHO];
+            sc[S_RHO] = res[0];
+            sc[S_RR] = res[1];
+            sc[S_IT] += 1.0;
+            sc[S_ITT] += 1.0;
+            const bool ok = isfinite(res[1]) && sc[S_OMEGA] != 0.0;
+            if (!ok) sc[S_BREAK] = 2.0;
+            d.act[0] = ok && res[1] > tol2 * sc[S_BB] && sc[S_IT] < maxit && res[0] != 0.0;
+            break;
+        }
+        default:
+            sc[slot] = res[0];
+            break;
+    }
+}
+
+
+__global__ __launch_bounds__(BS) void k_fin(const Desc* __restrict__ D, const int2* __restrict__ rng,
+                                            const double* __restrict__ part, int nb, Fin f) {
+    fin_body(D[blockIdx.x], rng[blockIdx.x], part, nb, f);
+}
+
+// ---------------------------------------------------------------------------
+// Residual + Jacobian assembly: one pass over u, u0; writes F, C/W/E/S/N
+// (FP64 and, when mixed, FP32) and per-block partial ||F||^2.
+template <bool MIXED>
+__global__ __launch_bounds__(BS) void k_assemble(const Desc* __restrict__ D, const int2* __restrict__ blk,
+                                                 double* __restrict__ part) {
+    LOCATE();
+    double f2 = 0.0;
+    const int nx = d.nx, ny = d.ny;
+    const double k = d.k, coef = d.dt / (d.cv * d.vol), gx = d.gx, gy = d.gy;
+    const double* __restrict__ u = d.u;
+    FOR_CELLS(c) {
+        const int i = (int)(c % nx), j = (int)(c / nx);
+        const double uc = u[c];
+        const double kc = k * (1.0 + uc * uc), dkc = 2.0 * k * uc;
+        double div = 0.0, diag = 1.0, cw = 0.0, ce = 0.0, cs = 0.0, cn = 0.0;
+        // faces in ascending face-id order: west (x-), east (x+), south (y-), north (y+)
+        auto face = [&](double um, double g, double& cm) {
+            const double km = k * (1.0 + um * um);
+            const double kf = 0.5 * (kc + km);
+            const double diff = uc - um;
+            div += g * kf * diff;
+            diag += coef * g * (0.5 * dkc * diff + kf);
+            cm = coef * g * (0.5 * (2.0 * k * um) * diff - kf);
+        };
+        if (i > 0) face(u[c - 1], gx, cw);
+        if (i < nx - 1) face(u[c + 1], gx, ce);
+        if (j > 0) face(u[c - nx], gy, cs);
+        if (j < ny - 1) face(u[c + nx], gy, cn);
+        const double F = (uc - d.u0[c]) + coef * div;
+        d.F[c] = F;
+        d.C[c] = diag;
+        d.W[c] = cw;
+        d.E[c] = ce;
+        d.S[c] = cs;
+        d.N[c] = cn;
+        if (MIXED) {
+            ((float*)d.Cm)[c] = (float)diag;
+            ((float*)d.Wm)[c] = (float)cw;
+            ((float*)d.Em)[c] = (float)ce;
+            ((float*)d.Sm)[c] = (float)cs;
+            ((float*)d.Nm)[c] = (float)cn;
+        }
+        f2 += F * F;
+    }
+    bred_store(f2, part + blockIdx.x);
+}
+
+template <class T>
+__device__ inline T stencil_apply(const Desc& d, const T* __restrict__ x, long c, int i, int j) {
+    const int nx = d.nx;
+    T acc = ((const T*)d.Cm)[c] * x[c];
+    if (i > 0) acc += ((const T*)d.Wm)[c] * x[c - 1];
+    if (i < nx - 1) acc += ((const T*)d.Em)[c] * x[c + 1];
+    if (j > 0) acc += ((const T*)d.Sm)[c] * x[c - nx];
+    if (j < d.ny - 1) acc += ((const T*)d.Nm)[c] * x[c + nx];
+    return acc;
+}
+
+// y = A x with fused partial dot products: dot0 = V[a0].y, dot1 = V[a1].y (a1 < 0: y.y).
+// Skipped for inactive instances (partials 0).
+template <class T>
+__global__ __launch_bounds__(BS) void k_spmv_dot(const Desc* __restrict__ D, const int2* __restrict__ blk, int ix,
+                                                 int iy, int nd, int a0, int a1, double* __restrict__ part, int nb) {
+    LOCATE();
+    double s0 = 0.0, s1 = 0.0;
+    if (d.act[0]) {
+        const T* __restrict__ x = (const T*)d.v[ix];
+        T* __restrict__ y = (T*)d.v[iy];
+        const T* __restrict__ va = (const T*)d.v[a0];
+        const T* __restrict__ vb = a1 >= 0 ? (const T*)d.v[a1] : nullptr;
+        FOR_CELLS(c) {
+            const int i = (int)(c % d.nx), j = (int)(c / d.nx);
+            const T yc = stencil_apply<T>(d, x, c, i, j);
+            y[c] = yc;
+            s0 += (double)va[c] * (double)yc;
+            if (nd > 1) s1 += (vb ? (double)vb[c] : (double)yc) * (double)yc;
+        }
+    }
+    bred_store(s0, part + blockIdx.x);
+    if (nd > 1) bred_store(s1, part + nb + blockIdx.x);
+}
+
+// FP64 defect for the instances under refinement: r64 = F - J du.
+__global__ __launch_bounds__(BS) void k_defect(const Desc* __restrict__ D, const int2* __restrict__ blk,
+                                               double* __restrict__ part) {
+    LOCATE();
+    double r2 = 0.0;
+    if (d.act[1]) {
+        const int nx = d.nx;
+        const double* __restrict__ x = d.du;
+        FOR_CELLS(c) {
+            const int i = (int)(c % nx), j = (int)(c / nx);
+            double acc = d.C[c] * x[c];
+            if (i > 0) acc += d.W[c] * x[c - 1];
+            if (i < nx - 1) acc += d.E[c] * x[c + 1];
+            if (j > 0) acc += d.S[c] * x[c - nx];
+            if (j < d.ny - 1) acc += d.N[c] * x[c + nx];
+            const double r = d.F[c] - acc;
+            d.r64[c] = r;
+            r2 += r * r;
+        }
+    }
+    bred_store(r2, part + blockIdx.x);
+}
+
+// Preconditioner set-up: Jacobi 1/C, or red-black ILU(0) pivots.  Red = (i+j) even
+// cells come first in the ordering; red rows have no L part, a black row's L part is its
+// red neighbours and its pivot is C_b - sum_m A_bm A_mb / C_m (fill-in dropped).
+// dinv holds 1/C for red cells and 1/pivot for black cells.
+template <class T>
+__global__ __launch_bounds__(BS) void k_precond_setup(const Desc* __restrict__ D, const int2* __restrict__ blk,
+                                                      int mode) {
+    LOCATE();
+    const T* Cm = (const T*)d.Cm;
+    const T *Wm = (const T*)d.Wm, *Em = (const T*)d.Em, *Sm = (const T*)d.Sm, *Nm = (const T*)d.Nm;
+    const int nx = d.nx;
+    FOR_CELLS(c) {
+        T piv = Cm[c];
+        if (mode == PC_ILU) {
+            const int i = (int)(c % nx), j = (int)(c / nx);
+            if ((i + j) & 1) {
+                if (i > 0) piv -= Wm[c] * Em[c - 1] / Cm[c - 1];
+                if (i < nx - 1) piv -= Em[c] * Wm[c + 1] / Cm[c + 1];
+                if (j > 0) piv -= Sm[c] * Nm[c - nx] / Cm[c - nx];
+                if (j < d.ny - 1) piv -= Nm[c] * Sm[c + nx] / Cm[c + nx];
+            }
+        }
+        ((T*)d.dinv)[c] = piv != (T)0 ? (T)1 / piv : (T)1;
+    }
+}
+
+// z = M^{-1} r for red-black ILU(0) in one pass: a black cell solves
+// z_b = (r_b - sum_m A_bm r_m / C_m) / piv_b; a red cell recomputes the z of its
+// black neighbours (two-hop stencil, cached) and solves z_r = (r_r - sum_b A_rb z_b) / C_r.
+// 1/C_m of a red neighbour is its dinv, so only multiplications remain.
+template <class T>
+__device__ inline T black_value(const Desc& d, const T* __restrict__ r, long b, int i, int j) {
+    const int nx = d.nx, ny = d.ny;
+    const T* dv = (const T*)d.dinv;
+    const T *Wm = (const T*)d.Wm, *Em = (const T*)d.Em, *Sm = (const T*)d.Sm, *Nm = (const T*)d.Nm;
+    T s = r[b];
+    if (i > 0) s -= Wm[b] * (r[b - 1] * dv[b - 1]);
+    if (i < nx - 1) s -= Em[b] * (r[b + 1] * dv[b + 1]);
+    if (j > 0) s -= Sm[b] * (r[b - nx] * dv[b - nx]);
+    if (j < ny - 1) s -= Nm[b] * (r[b + nx] * dv[b + nx]);
+    return s * dv[b];
+}
+template <class T>
+__global__ __launch_bounds__(BS) void k_ilu_apply(const Desc* __restrict__ D, const int2* __restrict__ blk, int ir,
+                                                  int iz) {
+    LOCATE();
+    if (!d.act[0]) return;
+    const int nx = d.nx, ny = d.ny;
+    const T* __restrict__ r = (const T*)d.v[ir];
+    T* __restrict__ zo = (T*)d.v[iz];
+    FOR_CELLS(c) {
+        const int i = (int)(c % nx), j = (int)(c / nx);
+        T z;
+        if ((i + j) & 1) {
+            z = black_value<T>(d, r, c, i, j);
+        } else {
+            T s = r[c];
+            if (i > 0) s -= ((const T*)d.Wm)[c] * black_value<T>(d, r, c - 1, i - 1, j);
+            if (i < nx - 1) s -= ((const T*)d.Em)[c] * black_value<T>(d, r, c + 1, i + 1, j);
+            if (j > 0) s -= ((const T*)d.Sm)[c] * black_value<T>(d, r, c - nx, i, j - 1);
+            if (j < ny - 1) s -= ((const T*)d.Nm)[c] * black_value<T>(d, r, c + nx, i, j + 1);
+            z = s * ((const T*)d.dinv)[c];
+        }
+        zo[c] = z;
+    }
+}
+
+// Krylov start: r = b (already in r), rh = r, x = 0, p = v = 0; partial dots (rh.r, r.r).
+template <class T>
+__global__ __launch_bounds__(BS) void k_kinit(const Desc* __restrict__ D, const int2* __restrict__ blk,
+                                              double* __restrict__ part, int nb) {
+    LOCATE();
+    double rr = 0.0;
+    if (d.act[1]) {
+        FOR_CELLS(c) {
+            const T rc = ((T*)d.v[V_R])[c];
+            ((T*)d.v[V_RH])[c] = rc;
+            ((T*)d.v[V_X])[c] = 0;
+            ((T*)d.v[V_P])[c] = 0;
+            ((T*)d.v[V_V])[c] = 0;
+            rr += (double)rc * (double)rc;
+        }
+    }
+    bred_store(rr, part + blockIdx.x);
+    bred_store(rr, part + nb + blockIdx.x);
+}
+
+// p = r + beta (p - omega v) (first iteration: p = r); Jacobi / none: ph = M p
+template <class T>
+__global__ __launch_bounds__(BS) void k_b1(const Desc* __restrict__ D, const int2* __restrict__ blk, int pc) {
+    LOCATE();
+    if (!d.act[0]) return;
+    const double* sc = d.sc;
+    T* p = (T*)d.v[V_P];
+    const bool first = sc[S_IT] == 0.0;
+    const T beta = first ? (T)0 : (T)((sc[S_RHO] / sc[S_RHO_OLD]) * (sc[S_ALPHA] / sc[S_OMEGA]));
+    const T omega = (T)sc[S_OMEGA];
+    FOR_CELLS(c) {
+        const T r = ((const T*)d.v[V_R])[c];
+        const T pv = first ? r : r + beta * (p[c] - omega * ((const T*)d.v[V_V])[c]);
+        p[c] = pv;
+        if (pc != PC_ILU) ((T*)d.v[V_PH])[c] = pc == PC_JACOBI ? ((const T*)d.dinv)[c] * pv : pv;
+    }
+}
+// s = r - alpha v ; Jacobi / none: sh = M s
+template <class T>
+__global__ __launch_bounds__(BS) void k_b2(const Desc* __restrict__ D, const int2* __restrict__ blk, int pc) {
+    LOCATE();
+    if (!d.act[0]) return;
+    const T alpha = (T)d.sc[S_ALPHA];
+    FOR_CELLS(c) {
+        const T s = ((const T*)d.v[V_R])[c] - alpha * ((const T*)d.v[V_V])[c];
+        ((T*)d.v[V_S])[c] = s;
+        if (pc != PC_ILU) ((T*)d.v[V_SH])[c] = pc == PC_JACOBI ? ((const T*)d.dinv)[c] * s : s;
+    }
+}
+// x += alpha ph + omega sh ; r = s - omega t ; partial dots (rh.r, r.r)
+template <class T>
+__global__ __launch_bounds__(BS) void k_b3(const Desc* __restrict__ D, const int2* __restrict__ blk,
+                                           double* __restrict__ part, int nb) {
+    LOCATE();
+    double a0 = 0.0, a1 = 0.0;
+    if (d.act[0]) {
+        const T alpha = (T)d.sc[S_ALPHA], omega = (T)d.sc[S_OMEGA];
+        T* x = (T*)d.v[V_X];
+        FOR_CELLS(c) {
+            x[c] += alpha * ((const T*)d.v[V_PH])[c] + omega * ((const T*)d.v[V_SH])[c];
+            const T r = ((const T*)d.v[V_S])[c] - omega * ((const T*)d.v[V_T])[c];
+            ((T*)d.v[V_R])[c] = r;
+            a0 += (double)((const T*)d.v[V_RH])[c] * (double)r;
+            a1 += (double)r * (double)r;
+        }
+    }
+    bred_store(a0, part + blockIdx.x);
+    bred_store(a1, part + nb + blockIdx.x);
+}
+
+__global__ void k_count_active(const Desc* __restrict__ D, int ninst, int which, int* __restrict__ out) {
+    __shared__ int lds[BS];
+    int s = 0;
+    for (int i = threadIdx.x; i < ninst; i += BS) s += D[i].act[which] ? 1 : 0;
+    lds[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = BS / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) lds[threadIdx.x] += lds[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = lds[0];
+}
+
+// refinement bookkeeping (one thread per instance):
+//   mode 0: start  -> act[1] = ||F|| > 0; S_BN = S_RN = ||F||; counters reset
+//   mode 1: after a defect: S_RN = sqrt(S_FN); act[1] = S_RN > tol*S_BN && S_REF < max_refine
+//           and the defect decreased (stagnation stops the refinement)
+__global__ void k_refine_flags(const Desc* __restrict__ D, int ninst, int mode, double tol, int max_refine) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ninst) return;
+    const Desc& d = D[i];
+    double* sc = d.sc;
+    if (mode == 0) {
+        sc[S_BN] = sqrt(sc[S_FN]);
+        sc[S_RN] = sc[S_BN];
+        sc[S_REF] = 0.0;
+        sc[S_ITT] = 0.0;
+        d.act[1] = sc[S_BN] > 0.0 && isfinite(sc[S_BN]);
+    } else {
+        if (!d.act[1]) return;
+        const double rn = sqrt(sc[S_FN]);   // defect norm stored by k_fin(A_STORE, S_FN)
+        const bool progress = rn < sc[S_RN];
+        sc[S_RN] = rn;
+        sc[S_REF] += 1.0;
+        d.act[1] = isfinite(rn) && progress && rn > tol * sc[S_BN] && sc[S_REF] < max_refine;
+    }
+}
+
+// r (Krylov precision) = r64 / ||r64||  (mixed) or = F (fp64 mode: r64 == F)
+template <class T>
+__global__ __launch_bounds__(BS) void k_load_rhs(const Desc* __restrict__ D, const int2* __restrict__ blk,
+                                                 int scaled) {
+    LOCATE();
+    if (!d.act[1]) return;
+    const double scale = scaled ? 1.0 / d.sc[S_RN] : 1.0;
+    FOR_CELLS(c)((T*)d.v[V_R])[c] = (T)(d.r64[c] * scale);
+}
+// du += ||r64|| x  (mixed) or du = x (fp64)
+template <class T>
+__global__ __launch_bounds__(BS) void k_accum(const Desc* __restrict__ D, const int2* __restrict__ blk, int scaled) {
+    LOCATE();
+    if (!d.act[1]) return;
+    const double rn = d.sc[S_RN];
+    FOR_CELLS(c) {
+        const double x = (double)((const T*)d.v[V_X])[c];
+        if (scaled) d.du[c] += rn * x;
+        else d.du[c] = x;
+    }
+}
+__global__ __launch_bounds__(BS) void k_zero_du(const Desc* __restrict__ D, const int2* __restrict__ blk) {
+    LOCATE();
+    FOR_CELLS(c) {
+        d.du[c] = 0.0;
+        d.r64[c] = d.F[c];
+    }
+}
+__global__ __launch_bounds__(BS) void k_newton_update(const Desc* __restrict__ D, const int2* __restrict__ blk) {
+    LOCATE();
+    FOR_CELLS(c) d.u[c] -= d.du[c];
+}
+__global__ void k_gather_slot(const Desc* __restrict__ D, int ninst, int slot, double* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < ninst) out[i] = D[i].sc[slot];
+}
+#undef FOR_CELLS
+#undef LOCATE
+
+// ---------------------------------------------------------------------------
+struct Inst {
+    int id = 0;
+    int nx = 0, ny = 0, n = 0;
+    double k = 0, cv = 1, dx = 1, dy = 1, dt = 0;
+    DevPtr<double> u, u0, F, du, r64, C, W, E, S, N, sc;
+    DevPtr<char> coef32, dinv, vecs;
+    DevPtr<int> act;
+    enum State { IDLE, START, SOLVE, DONE } state = IDLE;
+    StepStats st;
+};
+
+struct Table {
+    std::vector<Inst*> inst;
+    DevPtr<Desc> desc;
+    DevPtr<int2> blk, rng;
+    int nb = 0;
+    DevPtr<double> part;
+    uint64_t version = 0;
+    // captured BiCGStab iterations, keyed by (tolerance, max iterations)
+    std::map<std::pair<double, int>, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
+    ~Table() {
+        for (auto& g : graphs) {
+            (void)hipGraphExecDestroy(g.second.second);
+            (void)hipGraphDestroy(g.second.first);
+        }
+    }
+};
+using TableP = std::shared_ptr<Table>;
+}  // namespace
+
+struct HeatBatch::Impl {
+    FusedConfig cfg;
+    bool mixed = true;
+    int pc = PC_ILU;
+    std::map<int, std::unique_ptr<Inst>> insts;
+    int next_id = 1;
+    uint64_t version = 1;   // bumped whenever a descriptor field changes
+    std::map<std::vector<int>, TableP> tcache;
+    bool use_graph = true;
+    int graph_iters = 2;    // BiCGStab iterations per graph launch
+    static constexpr int NRING = 4;
+    DevPtr<int> count;
+    int* host_count = nullptr;
+    hipEvent_t ev[NRING] = {};
+
+    size_t ks() const { return mixed ? sizeof(float) : sizeof(double); }
+
+    Desc desc_of(Inst& I) const {
+        Desc d{};
+        d.n = I.n;
+        d.nx = I.nx;
+        d.ny = I.ny;
+        d.k = I.k;
+        d.cv = I.cv;
+        d.dt = I.dt;
+        d.gx = I.dy / I.dx;   // x-faces: area dy over centroid distance dx
+        d.gy = I.dx / I.dy;
+        d.vol = I.dx * I.dy;
+        d.u = I.u->p;
+        d.u0 = I.u0->p;
+        d.F = I.F->p;
+        d.du = I.du->p;
+        d.r64 = I.r64->p;
+        d.C = I.C->p;
+        d.W = I.W->p;
+        d.E = I.E->p;
+        d.S = I.S->p;
+        d.N = I.N->p;
+        if (mixed) {
+            float* f = (float*)I.coef32->p;
+            d.Cm = f;
+            d.Wm = f + I.n;
+            d.Em = f + 2L * I.n;
+            d.Sm = f + 3L * I.n;
+            d.Nm = f + 4L * I.n;
+        } else {
+            d.Cm = d.C;
+            d.Wm = d.W;
+            d.Em = d.E;
+            d.Sm = d.S;
+            d.Nm = d.N;
+        }
+        d.dinv = I.dinv->p;
+        for (int v = 0; v < NVEC; ++v) d.v[v] = I.vecs->p + (size_t)v * I.n * ks();
+        d.sc = I.sc->p;
+        d.act = I.act->p;
+        return d;
+    }
+
+    TableP table(const std::vector<Inst*>& list) {
+        std::vector<int> key;
+        for (Inst* I : list) key.push_back(I->id);
+        auto it = tcache.find(key);
+        if (it != tcache.end() && it->second->version == version) return it->second;
+        if (tcache.size() > 64) tcache.clear();
+        TableP tp = std::make_shared<Table>();
+        Table& t = *tp;
+        t.version = version;
+        t.inst = list;
+        std::vector<Desc> ds;
+        std::vector<int2> blk, rng;
+        for (size_t q = 0; q < list.size(); ++q) {
+            ds.push_back(desc_of(*list[q]));
+            const int nbk = (list[q]->n + CPB - 1) / CPB;
+            rng.push_back(make_int2((int)blk.size(), (int)blk.size() + nbk));
+            for (int b = 0; b < nbk; ++b) blk.push_back(make_int2((int)q, b));
+        }
+        t.nb = (int)blk.size();
+        t.desc = dev_from_host(ds.data(), ds.size());
+        t.blk = dev_from_host(blk.data(), blk.size());
+        t.rng = dev_from_host(rng.data(), rng.size());
+        t.part = dev_alloc<double>(2 * (size_t)t.nb + 2);
+        tcache[key] = tp;
+        return tp;
+    }
+
+    int count_active(const Table& t, int which) {
+        hipLaunchKernelGGL(k_count_active, dim3(1), dim3(BS), 0, S(), t.desc->p, (int)t.inst.size(), which, count->p);
+        EQ_HIP(hipGetLastError());
+        EQ_HIP(hipMemcpyAsync(host_count, count->p, sizeof(int), hipMemcpyDeviceToHost, S()));
+        Device::get().sync();
+        return *host_count;
+    }
+    // Finalisation of a reduction: one block per instance (kernel boundaries make the
+    // partials of all blocks visible; no device-wide fences inside the kernels).
+    void fin(const Table& t, const Fin& f) {
+        hipLaunchKernelGGL(k_fin, dim3((unsigned)t.inst.size()), dim3(BS), 0, S(), t.desc->p, t.rng->p, t.part->p,
+                           t.nb, f);
+        EQ_HIP(hipGetLastError());
+    }
+    static Fin F(int act, int nd, double tol2 = 0, int maxit = 0, int slot = 0) {
+        Fin f;
+        f.act = act;
+        f.nd = nd;
+        f.tol2 = tol2;
+        f.maxit = maxit;
+        f.slot = slot;
+        return f;
+    }
+#define HL(kernel, ...)                                                                                    \
+    do {                                                                                                   \
+        hipLaunchKernelGGL(kernel, dim3((unsigned)t.nb), dim3(BS), 0, S(), t.desc->p, t.blk->p, ##__VA_ARGS__); \
+        EQ_HIP(hipGetLastError());                                                                         \
+    } while (0)
+
+    void assemble(const Table& t) {
+        {
+            TimedLaunch tl("assemble");
+            if (mixed) HL(k_assemble<true>, t.part->p);
+            else HL(k_assemble<false>, t.part->p);
+        }
+        fin(t, F(A_STORE, 1, 0, 0, S_FN));
+    }
+
+    template <class T>
+    void spmv_dot(const Table& t, int ix, int iy, int nd, int a0, int a1, const Fin& f) {
+        {
+            TimedLaunch tl(sizeof(T) == 4 ? "stencil_spmv_fp32" : "stencil_spmv_fp64");
+            HL(k_spmv_dot<T>, ix, iy, nd, a0, a1, t.part->p, t.nb);
+        }
+        fin(t, f);
+    }
+    template <class T>
+    void precond(const Table& t, int ir, int iz) {
+        if (pc == PC_ILU) HL(k_ilu_apply<T>, ir, iz);
+    }
+
+    // BiCGStab for the instances whose refinement flag is set; rhs in V_R.
+    template <class T>
+    void bicg_iteration(const Table& t, double tol, int maxit) {
+        HL(k_b1<T>, pc);
+        precond<T>(t, V_P, V_PH);
+        spmv_dot<T>(t, V_PH, V_V, 1, (int)V_RH, -1, F(A_ALPHA, 1));   // v = A ph; alpha = rho / rh.v
+        HL(k_b2<T>, pc);
+        precond<T>(t, V_S, V_SH);
+        spmv_dot<T>(t, V_SH, V_T, 2, (int)V_S, -1, F(A_OMEGA, 2));    // t = A sh; omega = t.s / t.t
+        HL(k_b3<T>, t.part->p, t.nb);
+        fin(t, F(A_RHO, 2, tol * tol, maxit));
+    }
+
+    // Device -> host count of active instances without stalling the queue: the count
+    // of launch k is read after launch k+1 has been queued (pinned ring + events).
+    void count_async(const Table& t, int which, int slot) {
+        hipLaunchKernelGGL(k_count_active, dim3(1), dim3(BS), 0, S(), t.desc->p, (int)t.inst.size(), which,
+                           count->p + slot);
+        EQ_HIP(hipGetLastError());
+        EQ_HIP(hipMemcpyAsync(host_count + slot, count->p + slot, sizeof(int), hipMemcpyDeviceToHost, S()));
+        EQ_HIP(hipEventRecord(ev[slot], S()));
+    }
+    int count_wait(int slot) {
+        EQ_HIP(hipEventSynchronize(ev[slot]));
+        return host_count[slot];
+    }
+
+    template <class T>
+    void bicgstab(Table& t, double tol, int maxit) {
+        HL(k_kinit<T>, t.part->p, t.nb);
+        fin(t, F(A_INIT, 2));
+        if (use_graph && !timing_enabled()) {
+            // graph_iters iterations per launch; instances that converge inside a
+            // launch are masked by their device flag, so extra iterations are no-ops.
+            auto key = std::make_pair(tol, maxit);
+            auto g = t.graphs.find(key);
+            if (g == t.graphs.end()) {
+                hipGraph_t graph;
+                hipGraphExec_t exec;
+                EQ_HIP(hipStreamBeginCapture(S(), hipStreamCaptureModeThreadLocal));
+                for (int q = 0; q < graph_iters; ++q) bicg_iteration<T>(t, tol, maxit);
+                EQ_HIP(hipStreamEndCapture(S(), &graph));
+                EQ_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+                g = t.graphs.emplace(key, std::make_pair(graph, exec)).first;
+            }
+            // pipelined: launch k+1 is queued before the active count of launch k is read
+            int k = 0;
+            for (int it = 0; it < maxit; it += graph_iters, ++k) {
+                EQ_HIP(hipGraphLaunch(g->second.second, S()));
+                count_async(t, 0, k % NRING);
+                if (k > 0 && !count_wait((k - 1) % NRING)) break;
+            }
+            Device::get().sync();
+            return;
+        }
+        if (!count_active(t, 0)) return;
+        for (int it = 0; it < maxit; ++it) {
+            bicg_iteration<T>(t, tol, maxit);
+            if (!count_active(t, 0)) break;
+        }
+    }
+
+    template <class T>
+    void linear_solve(Table& t) {
+        // du = 0, r64 = F; refinement flags from ||F||
+        HL(k_zero_du);
+        const unsigned ni = (unsigned)t.inst.size();
+        hipLaunchKernelGGL(k_refine_flags, dim3((ni + 63) / 64), dim3(64), 0, S(), t.desc->p, (int)ni, 0,
+                           cfg.solver_tol, cfg.max_refine);
+        if (!mixed) {
+            HL(k_load_rhs<T>, 0);
+            precond_setup<T>(t);
+            bicgstab<T>(t, cfg.solver_tol, cfg.solver_max_iter);
+            HL(k_accum<T>, 0);
+            return;
+        }
+        precond_setup<T>(t);
+        for (int ref = 0; ref < cfg.max_refine; ++ref) {
+            if (ref > 0 && !count_active(t, 1)) break;   // all start active (||F|| > 0)
+            HL(k_load_rhs<T>, 1);
+            bicgstab<T>(t, cfg.inner_tol, cfg.solver_max_iter);
+            HL(k_accum<T>, 1);
+            {
+                TimedLaunch tl("defect_fp64");
+                HL(k_defect, t.part->p);
+            }
+            fin(t, F(A_STORE, 1, 0, 0, S_FN));
+            hipLaunchKernelGGL(k_refine_flags, dim3((ni + 63) / 64), dim3(64), 0, S(), t.desc->p, (int)ni, 1,
+                               cfg.solver_tol, cfg.max_refine);
+        }
+    }
+    template <class T>
+    void precond_setup(const Table& t) {
+        if (pc != PC_NONE) HL(k_precond_setup<T>, pc);
+    }
+#undef HL
+
+    // per-instance scalars of several slots in one device-to-host copy: [slot][instance]
+    std::vector<double> gather(const Table& t, std::initializer_list<int> slots) {
+        const int ni = (int)t.inst.size();
+        auto d = dev_alloc<double>((size_t)ni * slots.size());
+        int q = 0;
+        for (int slot : slots) {
+            hipLaunchKernelGGL(k_gather_slot, dim3((ni + 63) / 64), dim3(64), 0, S(), t.desc->p, ni, slot,
+                               d->p + (size_t)q * ni);
+            EQ_HIP(hipGetLastError());
+            ++q;
+        }
+        return dev_to_host(d->p, (size_t)ni * slots.size());
+    }
+};
+
+// ===========================================================================
+HeatBatch::HeatBatch(const FusedConfig& cfg) : p_(new Impl()) {
+    p_->cfg = cfg;
+    EQ_REQUIRE(cfg.precision == "fp64" || cfg.precision == "mixed", "precision must be fp64 or mixed");
+    p_->mixed = cfg.precision == "mixed";
+    const std::string pc = cfg.precond == "diagonal" ? "jacobi" : cfg.precond;
+    EQ_REQUIRE(pc == "jacobi" || pc == "ilu0" || pc == "none", "preconditioner must be jacobi, ilu0 or none");
+    p_->pc = pc == "jacobi" ? PC_JACOBI : pc == "ilu0" ? PC_ILU : PC_NONE;
+    p_->use_graph = cfg.graph;
+    p_->count = dev_alloc<int>(Impl::NRING);
+    EQ_HIP(hipHostMalloc((void**)&p_->host_count, Impl::NRING * sizeof(int), hipHostMallocDefault));
+    for (int i = 0; i < Impl::NRING; ++i) EQ_HIP(hipEventCreateWithFlags(&p_->ev[i], hipEventDisableTiming));
+}
+
+HeatBatch::~HeatBatch() {
+    Device::get().sync();
+    for (int i = 0; i < Impl::NRING; ++i)
+        if (p_->ev[i]) (void)hipEventDestroy(p_->ev[i]);
+    if (p_->host_count) (void)hipHostFree(p_->host_count);
+    delete p_;
+}
+
+size_t HeatBatch::estimate_bytes(int n, const std::string& precision) {
+    const size_t ks = precision == "mixed" ? 4 : 8;
+    return (size_t)n * (10 * 8 + (precision == "mixed" ? 5 * 4 : 0) + ks * (1 + NVEC)) + NSLOT * 8 + 64;
+}
+
+int HeatBatch::add(int nx, int ny, double k, double cv, const double* u_host, double dx, double dy) {
+    EQ_REQUIRE(nx > 0 && ny > 0, "instance sizes must be positive");
+    EQ_REQUIRE((long)nx * ny < (1L << 31), "instance too large");
+    auto I = std::make_unique<Inst>();
+    I->id = p_->next_id++;
+    I->nx = nx;
+    I->ny = ny;
+    I->n = nx * ny;
+    I->k = k;
+    I->cv = cv;
+    I->dx = dx;
+    I->dy = dy;
+    const int n = I->n;
+    I->u = dev_from_host(u_host, n);
+    I->u0 = dev_alloc<double>(n);
+    I->F = dev_alloc<double>(n);
+    I->du = dev_alloc<double>(n);
+    I->r64 = dev_alloc<double>(n);
+    I->C = dev_alloc<double>(n);
+    I->W = dev_alloc<double>(n);
+    I->E = dev_alloc<double>(n);
+    I->S = dev_alloc<double>(n);
+    I->N = dev_alloc<double>(n);
+    I->sc = dev_alloc<double>(NSLOT);
+    I->act = dev_alloc<int>(2);
+    EQ_HIP(hipMemsetAsync(I->sc->p, 0, NSLOT * sizeof(double), S()));
+    EQ_HIP(hipMemsetAsync(I->act->p, 0, 2 * sizeof(int), S()));
+    if (p_->mixed) I->coef32 = dev_alloc<char>((size_t)5 * n * sizeof(float));
+    I->dinv = dev_alloc<char>((size_t)n * p_->ks());
+    I->vecs = dev_alloc<char>((size_t)NVEC * n * p_->ks());
+    const int id = I->id;
+    p_->insts[id] = std::move(I);
+    ++p_->version;
+    return id;
+}
+
+static Inst& get_inst(HeatBatch::Impl* p, int slot) {
+    auto it = p->insts.find(slot);
+    EQ_REQUIRE(it != p->insts.end(), "unknown batch slot " << slot);
+    return *it->second;
+}
+
+void HeatBatch::remove(int slot) {
+    get_inst(p_, slot);
+    p_->tcache.clear();
+    p_->insts.erase(slot);
+    ++p_->version;
+}
+int HeatBatch::size_of(int slot) const { return get_inst(p_, slot).n; }
+bool HeatBatch::in_flight(int slot) const {
+    const Inst& I = get_inst(p_, slot);
+    return I.state == Inst::START || I.state == Inst::SOLVE;
+}
+const StepStats& HeatBatch::stats(int slot) const { return get_inst(p_, slot).st; }
+size_t HeatBatch::device_bytes(int slot) const {
+    return estimate_bytes(get_inst(p_, slot).n, p_->cfg.precision);
+}
+
+void HeatBatch::begin_step(int slot, double dt) {
+    Inst& I = get_inst(p_, slot);
+    EQ_REQUIRE(I.state != Inst::START && I.state != Inst::SOLVE, "slot " << slot << " already has a step in flight");
+    if (I.dt != dt) ++p_->version;
+    I.dt = dt;
+    EQ_HIP(hipMemcpyAsync(I.u0->p, I.u->p, (size_t)I.n * sizeof(double), hipMemcpyDeviceToDevice, S()));
+    I.st = StepStats();
+    I.state = Inst::START;
+}
+
+std::vector<int> HeatBatch::iterate() {
+    Impl& P = *p_;
+    std::vector<Inst*> solve, asmb;
+    for (auto& kv : P.insts) {
+        if (kv.second->state == Inst::SOLVE) solve.push_back(kv.second.get());
+        if (kv.second->state == Inst::START || kv.second->state == Inst::SOLVE) asmb.push_back(kv.second.get());
+    }
+    if (!solve.empty()) {
+        TableP tp = P.table(solve);
+        Table& t = *tp;
+        if (P.mixed) P.linear_solve<float>(t);
+        else P.linear_solve<double>(t);
+        hipLaunchKernelGGL(k_newton_update, dim3((unsigned)t.nb), dim3(BS), 0, S(), t.desc->p, t.blk->p);
+        EQ_HIP(hipGetLastError());
+    }
+    std::vector<int> done;
+    if (asmb.empty()) return done;
+    TableP tp = P.table(asmb);
+    Table& t = *tp;
+    P.assemble(t);
+    // one device-to-host copy per Newton iteration: ||F||^2 and the solve statistics
+    const size_t na = asmb.size();
+    std::vector<double> g = P.gather(t, {S_FN, S_ITT, S_REF, S_RN});
+    const FusedConfig& c = P.cfg;
+    for (size_t q = 0; q < na; ++q) {
+        Inst& I = *asmb[q];
+        if (I.state == Inst::SOLVE) {   // it was solved and updated in this call
+            I.st.newton += 1;
+            I.st.krylov += (int)g[na + q];
+            I.st.refinements += (int)g[2 * na + q];
+            if (!std::isfinite(g[3 * na + q])) I.st.failed = true;
+        }
+        const double fn = std::sqrt(g[q]);
+        if (I.state == Inst::START) I.st.r0 = fn;
+        I.st.rn = fn;
+        int stop = 0;
+        if (fn <= c.abs_res_tol) stop = 1;
+        else if (c.rel_res_tol > 0 && fn <= c.rel_res_tol * I.st.r0) stop = 2;
+        const bool bad = !std::isfinite(fn) || I.st.failed;
+        if (stop || bad || I.st.newton >= c.max_iter) {
+            I.st.stop = bad ? 0 : stop;
+            I.st.failed = bad;
+            I.state = Inst::DONE;
+            done.push_back(I.id);
+        } else {
+            I.state = Inst::SOLVE;
+        }
+    }
+    return done;
+}
+
+void HeatBatch::step(const std::vector<int>& slots, double dt) {
+    for (int s : slots) begin_step(s, dt);
+    int guard = 0;
+    for (;;) {
+        bool any = false;
+        for (int s : slots) any |= in_flight(s);
+        if (!any) break;
+        iterate();
+        EQ_REQUIRE(++guard < 10 * (p_->cfg.max_iter + 2), "batch step did not terminate");
+    }
+}
+
+void HeatBatch::get_state(int slot, double* u) const {
+    Inst& I = get_inst(p_, slot);
+    std::vector<double> h = dev_to_host(I.u->p, I.n);
+    std::copy(h.begin(), h.end(), u);
+}
+void HeatBatch::set_state(int slot, const double* u) {
+    Inst& I = get_inst(p_, slot);
+    I.u = dev_from_host(u, I.n);
+    ++p_->version;
+}
+
+void HeatBatch::export_residual(int slot, const double* u, const double* u0, double dt, std::vector<double>& F,
+                                std::vector<int>& rp, std::vector<int>& ci, std::vector<double>& v) {
+    Inst& I = get_inst(p_, slot);
+    I.u = dev_from_host(u, I.n);
+    I.u0 = dev_from_host(u0, I.n);
+    I.dt = dt;
+    ++p_->version;
+    TableP tp = p_->table({&I});
+    p_->assemble(*tp);
+    F = dev_to_host(I.F->p, I.n);
+    std::vector<double> C = dev_to_host(I.C->p, I.n), W = dev_to_host(I.W->p, I.n), E = dev_to_host(I.E->p, I.n),
+                        Sx = dev_to_host(I.S->p, I.n), N = dev_to_host(I.N->p, I.n);
+    // CSR with ascending columns: south, west, diagonal, east, north (existing neighbours only)
+    rp.assign(I.n + 1, 0);
+    ci.clear();
+    v.clear();
+    for (int c = 0; c < I.n; ++c) {
+        const int i = c % I.nx, j = c / I.nx;
+        if (j > 0) ci.push_back(c - I.nx), v.push_back(Sx[c]);
+        if (i > 0) ci.push_back(c - 1), v.push_back(W[c]);
+        ci.push_back(c), v.push_back(C[c]);
+        if (i < I.nx - 1) ci.push_back(c + 1), v.push_back(E[c]);
+        if (j < I.ny - 1) ci.push_back(c + I.nx), v.push_back(N[c]);
+        rp[c + 1] = (int)ci.size();
+    }
+}
+
+}  // namespace eqhip

@@ -1,0 +1,82 @@
+// Fused nonlinear-heat engine on structured 2D grids: many independent
+// instances of different sizes advanced together (batching, serving) with
+// per-instance Newton / Krylov convergence.
+//
+// The program is the one of heat_program.cpp / tests/generated/nonlinear_heat.equelle
+// (zero-flux boundary): F(u) = u - u0 + dt/(cv vol) div(-k_f geo grad u),
+// k(u) = k (1 + u^2), k_f = mean of the two cells.  Instead of one AD op per
+// built-in, one assembly kernel computes F and the five Jacobian stencil
+// coefficients (diagonal, west, east, south, north) per cell; the Jacobian is
+// the same 5-point matrix the AD engine produces (boundary coefficients are
+// structural zeros that are not stored in its CSR form).
+//
+// HBM layout per instance (n = nx*ny cells, x fastest): u, u0, F, du and the
+// coefficients C, W, E, S, N as separate FP64 arrays (FP32 copies in mixed
+// precision), Krylov vectors in the Krylov precision.  Kernels walk a device
+// table of instance descriptors plus a block -> (instance, local block) map,
+// so a batch is a virtual concatenation whose membership can change every
+// Newton iteration without moving any state.  All reductions are per instance
+// and deterministic (fixed block partition and order), so a result does not
+// depend on which other instances share the launch.
+#pragma once
+#include <string>
+#include <vector>
+
+#include "core.hpp"
+
+namespace eqhip {
+
+struct FusedConfig {
+    std::string precision = "mixed";   // fp64 | mixed
+    std::string precond = "ilu0";      // jacobi | ilu0 (red-black) | none
+    double rel_res_tol = 1e-8;         // Newton: ||F_k|| <= rel ||F_0||
+    double abs_res_tol = 0.0;          //         or ||F_k|| <= abs
+    int max_iter = 20;                 // Newton iterations per step
+    double solver_tol = 1e-6;          // linear: ||b - J du|| <= tol ||b|| (FP64 defect)
+    int solver_max_iter = 1000;
+    double inner_tol = 1e-4;           // mixed: FP32 inner solve tolerance
+    int max_refine = 20;
+    bool graph = true;                 // capture BiCGStab iterations in a hipGraph
+};
+
+struct StepStats {
+    int newton = 0;
+    int krylov = 0;
+    int refinements = 0;
+    int stop = 0;                      // 1 abs, 2 rel, 0 max_iter / failure
+    double r0 = 0, rn = 0;
+    bool failed = false;               // non-finite residual or Krylov breakdown
+};
+
+class HeatBatch {
+public:
+    explicit HeatBatch(const FusedConfig& cfg);
+    ~HeatBatch();
+    // Adds an instance (unit cells unless dx, dy given); returns its slot id.
+    int add(int nx, int ny, double k, double cv, const double* u_host, double dx = 1.0, double dy = 1.0);
+    void remove(int slot);
+    int size_of(int slot) const;
+    // Starts one implicit time step of length dt for the slot.
+    void begin_step(int slot, double dt);
+    // Advances every in-flight slot by one Newton iteration (continuous batching).
+    // Returns the slots whose step finished in this call.
+    std::vector<int> iterate();
+    bool in_flight(int slot) const;
+    const StepStats& stats(int slot) const;
+    // Runs whole steps for the given slots (loops iterate()).
+    void step(const std::vector<int>& slots, double dt);
+    void get_state(int slot, double* u) const;
+    void set_state(int slot, const double* u);
+    // Test hook: F and the Jacobian in CSR form at the slot's current state (u0 = prev).
+    void export_residual(int slot, const double* u, const double* u0, double dt, std::vector<double>& F,
+                         std::vector<int>& rp, std::vector<int>& ci, std::vector<double>& v);
+    size_t device_bytes(int slot) const;
+    static size_t estimate_bytes(int n, const std::string& precision);
+
+    struct Impl;
+
+private:
+    Impl* p_;
+};
+
+}  // namespace eqhip

@@ -142,9 +142,39 @@ int eqhip_heat_step(void* sim, double dt, int* stats, double* norms);
 int eqhip_heat_state(void* sim, double* u);
 int eqhip_heat_destroy(void* sim);
 
+/* ------------------------------------------------------------------ fused heat batches */
+/* Many independent nonlinear-heat instances (the program above, k_model=quadratic,
+ * zero-flux boundary, unit or given cells) advanced together by fused kernels with
+ * per-instance Newton and Krylov convergence.  No reference counterpart: the reference
+ * runs one newtonSolve at a time (EquelleRuntimeCUDA_impl.hpp:216-300).
+ * params: precision (fp64|mixed) preconditioner (diagonal|ilu0|none) rel_res_tol abs_res_tol
+ * max_iter solver_tol solver_max_iter inner_tol max_refine. */
+int eqhip_batch_create(const char* params, void** batch);
+int eqhip_batch_destroy(void* batch);
+int eqhip_batch_add(void* batch, int nx, int ny, double k, double cv, const double* u, int* slot);
+int eqhip_batch_remove(void* batch, int slot);
+/* starts one implicit step of length dt for the slot (u0 := u) */
+int eqhip_batch_begin_step(void* batch, int slot, double dt);
+/* one Newton iteration for every slot with a step in flight; returns the slots that finished */
+int eqhip_batch_iterate(void* batch, int* finished, int cap, int* nfinished);
+int eqhip_batch_in_flight(void* batch, int slot, int* flag);
+/* whole steps for the listed slots */
+int eqhip_batch_step(void* batch, const int* slots, int nslots, double dt);
+/* stats = {Newton, Krylov, refinements, stop reason, failed}, norms = {||F(u_0)||, ||F(u_k)||} */
+int eqhip_batch_stats(void* batch, int slot, int* stats, double* norms);
+int eqhip_batch_state(void* batch, int slot, double* u);
+int eqhip_batch_set_state(void* batch, int slot, const double* u);
+/* device-memory footprint of the slot in bytes (admission control) */
+int eqhip_batch_bytes(void* batch, int slot, double* bytes);
+/* F and the Jacobian (CSR, ascending columns) at (u, u0, dt); info = {n, nnz};
+ * caller provides n, n+1, 5n, 5n entries. */
+int eqhip_batch_residual(void* batch, int slot, const double* u, const double* u0, double dt, int* info,
+                         double* F, int* row_ptr, int* col_ind, double* jac);
+
 /* ------------------------------------------------------------------ measurement */
 /* Live kernel timing with HIP events on the runtime stream: mode 1 on, 0 off, 2 reset.
- * Timed launches: "spmv_fp64", "spmv_fp32" (Krylov SpMV). No reference counterpart
+ * Timed launches: "spmv_fp64", "spmv_fp32" (CSR/SELL Krylov SpMV of the AD engine),
+ * "stencil_spmv_fp64", "stencil_spmv_fp32", "assemble", "defect_fp64" (fused batches). No reference counterpart
  * (the reference only prints wall times, EquelleRuntimeCUDA_impl.hpp:295-297). */
 int eqhip_timing(int mode);
 int eqhip_timing_query(const char* name, int* count, double* total_ms);
