@@ -120,6 +120,102 @@ def run_generic(lib, u0, nx, ny, dt, steps, warmup):
     return dict(seconds=secs, newton=newton)
 
 
+def bench_batched(lib, args):
+    """512 independent nonlinear-heat instances (sides log-uniform 32..512, k0 ~ U[0.05,0.5]).
+
+    headline              one fused batch, mixed precision, all instances in every launch
+    sequential_fp64_fused the plain FP64 path of the same engine, one instance at a time
+    sequential_fp64_generic  the drop-in AD engine (FP64), one instance at a time, on every
+                          8th instance (a rate, so the subsample keeps the size mix)
+    Both fused legs run the same W warmup + K timed steps from the same initial fields."""
+    from equelle_amd.workloads import batched_instances
+    inst = batched_instances(args.instances, seed=1234)
+    cells = np.array([s * s for s, _, _ in inst], dtype=np.int64)
+    dt = 1.0
+    steps = args.steps
+    common = f"rel_res_tol={NONLINEAR_HEAT['rel_res_tol']} abs_res_tol=0 max_iter=20 solver_tol=1e-6"
+
+    def run(prec, batched):
+        b = C.c_void_p()
+        lib.batch_create(f"precision={prec} preconditioner={args.precond} {common}".encode(), C.byref(b))
+        try:
+            slots = []
+            for side, k0, u0 in inst:
+                sl = C.c_int()
+                lib.batch_add(b, side, side, k0, 1.0, dptr(u0), C.byref(sl))
+                slots.append(sl.value)
+            slots = np.array(slots, dtype=np.int32)
+            groups = [slots] if batched else [slots[q:q + 1] for q in range(len(slots))]
+            st = np.zeros(5, dtype=np.int32)
+
+            def one_step():
+                work = 0
+                for g in groups:
+                    lib.batch_step(b, iptr(g), len(g), dt)
+                for q, sl in enumerate(slots):
+                    lib.batch_stats(b, int(sl), iptr(st), None)
+                    if st[3] == 0 or st[4]:
+                        raise RuntimeError(f"instance {q} did not converge: {st.tolist()}")
+                    work += int(cells[q]) * int(st[0])
+                return work
+
+            for _ in range(args.warmup):
+                one_step()
+            lib.device_sync()
+            work = 0
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                work += one_step()
+            lib.device_sync()
+            secs = time.perf_counter() - t0
+            u = [np.zeros(s * s) for s, _, _ in inst]
+            for q, sl in enumerate(slots):
+                lib.batch_state(b, int(sl), dptr(u[q]))
+            return dict(value=work / secs, ms_per_step=1e3 * secs / steps, newton_cells=work), u
+        finally:
+            lib.batch_destroy(b)
+
+    log(f"[bench] batched: {len(inst)} instances, {cells.sum()} cells, mixed precision, one batch")
+    res = {}
+    res["batched_mixed"], ub = run("mixed", True)
+    log("[bench] plain FP64, one instance at a time (fused engine)")
+    res["sequential_fp64_fused"], us = run("fp64", False)
+    err = max(float(np.linalg.norm(a - c) / np.linalg.norm(c)) for a, c in zip(ub, us))
+    log("[bench] plain FP64, one instance at a time (generic drop-in AD engine), every 8th instance")
+    e, ed = np.zeros(1, dtype=np.int32), np.zeros(1)
+    stats = np.zeros(3, dtype=np.int32)
+    work, secs = 0, 0.0
+    for side, k0, u0 in inst[::8]:
+        sim = C.c_void_p()
+        params = heat_params(side, side, k0, precision="fp64", precond="ilu0",
+                             rel_res_tol=NONLINEAR_HEAT["rel_res_tol"], solver_tol=1e-6)
+        lib.heat_create(params.encode(), dptr(u0), side * side, iptr(e), dptr(ed), 0, C.byref(sim))
+        try:
+            lib.heat_step(sim, dt, iptr(stats), None)     # first call builds the symbolic cache
+            lib.device_sync()
+            t0 = time.perf_counter()
+            lib.heat_step(sim, dt, iptr(stats), None)
+            lib.device_sync()
+            secs += time.perf_counter() - t0
+            work += side * side * int(stats[0])
+        finally:
+            lib.heat_destroy(sim)
+    res["sequential_fp64_generic"] = dict(value=work / secs, instances=len(inst[::8]))
+    head = res["batched_mixed"]
+    out = {"metric": "cell-Newton-iters/s", "value": head["value"], "unit": "cell-Newton-iters/s",
+           "n_gpus": 1, "steps": steps, "warmup": args.warmup, "ms_per_step": head["ms_per_step"],
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64 residual/Jacobian/refinement, f32 Krylov", "data": "synthetic",
+           "config": {"workload": f"{len(inst)} nonlinear-heat instances in one batch, sides log-uniform "
+                                  "32..512, k0 ~ U[0.05,0.5], dt=1, Newton to 1e-8 relative residual each",
+                      "cells": int(cells.sum()), "precond": args.precond},
+           "runs": res,
+           "batched_over_sequential_fp64_fused": head["value"] / res["sequential_fp64_fused"]["value"],
+           "batched_over_sequential_fp64_generic": head["value"] / res["sequential_fp64_generic"]["value"],
+           "mixed_vs_fp64_max_rel_error": err}
+    print(json.dumps(out), flush=True)
+
+
 def cpu_baseline(n_side, u0, dt):
     """Serial oracle: one Newton iteration (AD assembly + BiCGStab/ILU(0) to the same
     relative tolerance) on the same grid and state."""
@@ -146,11 +242,21 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline run only (for profiling)")
     ap.add_argument("--precond", default="ilu0", help="preconditioner of the fused runs: ilu0 | diagonal | none")
+    ap.add_argument("--workload", default="heat2048", choices=["heat2048", "batched", "serving"])
+    ap.add_argument("--instances", type=int, default=512)
+    ap.add_argument("--rates", default="50,100,200,500,1000,2000", help="serving: arrival rates (req/s)")
+    ap.add_argument("--requests", type=int, default=200, help="serving: requests per rate")
+    ap.add_argument("--hbm-budget-gb", dest="hbm_budget_gb", type=float, default=64.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
     lib = load_library()
+    if args.workload == "batched":
+        return bench_batched(lib, args)
+    if args.workload == "serving":
+        from equelle_amd.serving import bench_serving
+        return bench_serving(lib, args)
     nx = ny = args.size
     n = nx * ny
     nif = (nx - 1) * ny + nx * (ny - 1)

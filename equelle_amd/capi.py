@@ -85,11 +85,19 @@ SIGNATURES = {
     "batch_set_state": [_P, _I, _DP],
     "batch_bytes": [_P, _I, _DP],
     "batch_residual": [_P, _I, _DP, _DP, _D, _IP, _DP, _IP, _IP, _DP],
+    "server_create": [_S, C.POINTER(_P)],
+    "server_destroy": [_P],
+    "server_submit": [_P, _I, _I, _D, _D, _DP, _I, _D, _IP],
+    "server_progress": [_P, _IP, _I, _IP],
+    "server_result": [_P, _I, _DP, _IP, _DP],
+    "server_status": [_P, _IP, _DP],
     "timing": [_I],
     "timing_query": [_S, _IP, _DP],
     "device_sync": [],
     "heat_newton_sample": [_S, _DP, _I, _I, _D, _DP, _IP],
     "heat_run": [_S, _DP, _I, _IP, _DP, _I, _DP, _I, _DP, _IP, _DP],
+    "twophase_run": [_S, _DP, _DP, _DP, _I, _IP, _DP, _I, _DP, _I, _DP, _DP, _IP],
+    "twophase_residual": [_S, _DP, _DP, _I, _IP, _DP, _I, _I, _DP, _DP, _DP, _D, _IP, _DP, _IP, _IP, _DP],
     "heat_residual": [_S, _DP, _DP, _I, _IP, _DP, _I, _D, _IP, _DP, _IP, _IP, _DP],
 }
 _RESTYPES = {"last_error": _S}

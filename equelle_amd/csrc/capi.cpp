@@ -14,6 +14,7 @@
 #include "kernels.hpp"
 #include "krylov.hpp"
 #include "runtime_core.hpp"
+#include "serve.hpp"
 #include "timing.hpp"
 
 using namespace eqhip;
@@ -584,6 +585,67 @@ int eqhip_batch_residual(void* batch, int slot, const double* u, const double* u
         std::copy(r.begin(), r.end(), rp);
         std::copy(c.begin(), c.end(), ci);
         std::copy(vv.begin(), vv.end(), v);
+    });
+}
+
+// ---------------- request loop
+static HeatServer& SV(void* s) {
+    EQ_REQUIRE(s, "null server");
+    return *static_cast<HeatServer*>(s);
+}
+int eqhip_server_create(const char* params, void** server) {
+    return guard([&] {
+        ParamMap m;
+        m.parse(params ? params : "");
+        ServerConfig c;
+        c.batch = fused_config(params);
+        c.hbm_budget = m.get_double("hbm_budget", c.hbm_budget);
+        c.max_in_flight = m.get_int("max_in_flight", c.max_in_flight);
+        *server = new HeatServer(c);
+    });
+}
+int eqhip_server_destroy(void* server) {
+    return guard([&] { delete static_cast<HeatServer*>(server); });
+}
+int eqhip_server_submit(void* server, int nx, int ny, double k, double cv, const double* u0, int steps, double dt,
+                        int* request) {
+    return guard([&] { *request = SV(server).submit(nx, ny, k, cv, u0, steps, dt); });
+}
+int eqhip_server_progress(void* server, int* finished, int cap, int* nfinished) {
+    return guard([&] {
+        std::vector<int> done = SV(server).progress();
+        EQ_REQUIRE((int)done.size() <= cap || !finished, "finished-request buffer too small");
+        if (finished) std::copy(done.begin(), done.end(), finished);
+        *nfinished = (int)done.size();
+    });
+}
+int eqhip_server_result(void* server, int request, double* u, int* stats, double* times) {
+    return guard([&] {
+        RequestInfo info;
+        SV(server).take_result(request, u, &info);
+        if (stats) {
+            stats[0] = info.steps;
+            stats[1] = info.newton;
+            stats[2] = info.krylov;
+            stats[3] = info.failed ? 1 : 0;
+        }
+        if (times) {
+            times[0] = info.queued_s;
+            times[1] = info.latency_s;
+        }
+    });
+}
+int eqhip_server_status(void* server, int* counts, double* bytes) {
+    return guard([&] {
+        HeatServer& s = SV(server);
+        if (counts) {
+            counts[0] = s.queued();
+            counts[1] = s.in_flight();
+        }
+        if (bytes) {
+            bytes[0] = s.used_bytes();
+            bytes[1] = s.config().hbm_budget;
+        }
     });
 }
 

@@ -469,6 +469,7 @@ struct Table {
     int nb = 0;
     DevPtr<double> part;
     uint64_t version = 0;
+    int solves = 0;   // Krylov solves run on this membership (graphs are captured from the second)
     // captured BiCGStab iterations, keyed by (tolerance, max iterations)
     std::map<std::pair<double, int>, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
     ~Table() {
@@ -649,7 +650,8 @@ struct HeatBatch::Impl {
     void bicgstab(Table& t, double tol, int maxit) {
         HL(k_kinit<T>, t.part->p, t.nb);
         fin(t, F(A_INIT, 2));
-        if (use_graph && !timing_enabled()) {
+        // a membership seen once (request churn) runs eagerly; capture pays off from its second solve
+        if (use_graph && !timing_enabled() && t.solves++ > 0) {
             // graph_iters iterations per launch; instances that converge inside a
             // launch are masked by their device flag, so extra iterations are no-ops.
             auto key = std::make_pair(tol, maxit);

@@ -1,0 +1,72 @@
+// Request loop over a continuously re-formed batch of PDE solves.
+//
+// A request is one nonlinear-heat instance (grid, conductivity, initial field) to be
+// advanced by a number of implicit time steps.  Requests wait in a FIFO queue until
+// their HBM footprint fits the budget; admitted requests become slots of one fused
+// batch, their state stays device-resident between steps, and every progress() call
+// advances all in-flight slots by one Newton iteration in shared launches.  A slot
+// whose time step converged starts its next step in the same call; a finished
+// request's field is copied to the host and its slot (and memory) released, so the
+// next queued request can join mid-flight.  Per-instance reductions are
+// deterministic, so a request's result does not depend on which others share the
+// launches.
+#pragma once
+#include <chrono>
+#include <deque>
+#include <map>
+#include <memory>
+#include <vector>
+
+#include "heat_fused.hpp"
+
+namespace eqhip {
+
+struct ServerConfig {
+    FusedConfig batch;
+    double hbm_budget = 64e9;   // bytes of instance state admitted at once
+    int max_in_flight = 4096;
+};
+
+struct RequestInfo {
+    int steps = 0;               // time steps completed
+    int newton = 0, krylov = 0;  // totals over the steps
+    bool failed = false;         // a step stopped without converging
+    double queued_s = 0;         // submit -> admission
+    double latency_s = 0;        // submit -> result available
+};
+
+class HeatServer {
+public:
+    explicit HeatServer(const ServerConfig& cfg);
+    // Queues a request; returns its id.  Throws if it can never fit the budget.
+    int submit(int nx, int ny, double k, double cv, const double* u0, int steps, double dt);
+    // One scheduling tick: admits what fits, advances every in-flight request by one
+    // Newton iteration, retires finished requests.  Returns the ids finished now.
+    std::vector<int> progress();
+    // Copies a finished request's field (n values) and releases it.
+    void take_result(int id, double* u, RequestInfo* info);
+    int queued() const { return (int)queue_.size(); }
+    int in_flight() const { return (int)running_.size(); }
+    double used_bytes() const { return used_; }
+    const ServerConfig& config() const { return cfg_; }
+
+private:
+    struct Req {
+        int id = 0, nx = 0, ny = 0, slot = -1;
+        double k = 0, cv = 1, dt = 0;
+        int steps = 0;
+        std::vector<double> u;    // initial field until admitted, result after completion
+        double bytes = 0;
+        std::chrono::steady_clock::time_point t_submit, t_admit, t_done;
+        RequestInfo info;
+    };
+    ServerConfig cfg_;
+    HeatBatch batch_;
+    std::map<int, Req> reqs_;
+    std::deque<int> queue_;
+    std::map<int, int> running_;   // slot -> request id
+    double used_ = 0;
+    int next_id_ = 1;
+};
+
+}  // namespace eqhip

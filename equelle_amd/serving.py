@@ -1,0 +1,153 @@
+"""Client side of the request loop (eqhip_server_*): a seeded Poisson request stream
+replayed against the native server in wall-clock time, with latency percentiles.
+
+Request mix (2D nonlinear heat, the physics of the 2048^2 workload with
+k(u) = k0 (1 + u^2)): side ~ round(exp(U[ln 16, ln 1024])), k0 ~ U[0.05, 0.5],
+1..20 time steps of dt = 1, Gaussian-bump initial field scaled to the grid,
+each step's Newton solve to 1e-8 relative residual.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+import sys
+import time
+from dataclasses import dataclass
+
+import numpy as np
+
+from .capi import dptr, iptr
+from .workloads import gaussian_bumps
+
+
+@dataclass
+class Request:
+    t: float          # arrival time (s after the start of the stream)
+    side: int
+    k0: float
+    steps: int
+    u0: np.ndarray
+    dt: float = 1.0
+
+
+def poisson_stream(rate: float, count: int, seed: int = 7, lo: int = 16, hi: int = 1024, max_steps: int = 20):
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, count))
+    sides = np.rint(np.exp(rng.uniform(np.log(lo), np.log(hi), count))).astype(int)
+    k0 = rng.uniform(0.05, 0.5, count)
+    steps = rng.integers(1, max_steps + 1, count)
+    return [Request(float(t[i]), int(sides[i]), float(k0[i]), int(steps[i]),
+                    gaussian_bumps(int(sides[i]), int(sides[i]), seed * 100003 + i, width_scale=sides[i] / 2048.0))
+            for i in range(count)]
+
+
+class Server:
+    """Owns one eqhip server handle."""
+
+    def __init__(self, lib, params: str):
+        self.lib = lib
+        self.h = C.c_void_p()
+        lib.server_create(params.encode(), C.byref(self.h))
+        self._fin = np.zeros(4096, dtype=np.int32)
+
+    def submit(self, r: Request) -> int:
+        rid = C.c_int()
+        self.lib.server_submit(self.h, r.side, r.side, r.k0, 1.0, dptr(r.u0), r.steps, r.dt, C.byref(rid))
+        return rid.value
+
+    def progress(self):
+        nf = C.c_int()
+        self.lib.server_progress(self.h, iptr(self._fin), len(self._fin), C.byref(nf))
+        return self._fin[:nf.value].tolist()
+
+    def result(self, rid: int, n: int):
+        u = np.zeros(n)
+        st = np.zeros(4, dtype=np.int32)
+        tm = np.zeros(2)
+        self.lib.server_result(self.h, rid, dptr(u), iptr(st), dptr(tm))
+        return u, st, tm
+
+    def status(self):
+        c = np.zeros(2, dtype=np.int32)
+        b = np.zeros(2)
+        self.lib.server_status(self.h, iptr(c), dptr(b))
+        return c, b
+
+    def close(self):
+        if self.h:
+            self.lib.server_destroy(self.h)
+            self.h = C.c_void_p()
+
+
+def replay(server: Server, stream, keep_results=False):
+    """Submits each request at its arrival time and ticks the server until all finish.
+    Latency = completion time - scheduled arrival time (includes client-side lag)."""
+    out = {}
+    pending = {}
+    i = 0
+    t0 = time.perf_counter()
+    while i < len(stream) or pending:
+        now = time.perf_counter() - t0
+        while i < len(stream) and stream[i].t <= now:
+            pending[server.submit(stream[i])] = i
+            i += 1
+        if pending:
+            done = server.progress()
+            tnow = time.perf_counter() - t0
+            for rid in done:
+                q = pending.pop(rid)
+                r = stream[q]
+                u, st, tm = server.result(rid, r.side * r.side)
+                out[q] = dict(latency=tnow - r.t, server_latency=float(tm[1]), queued=float(tm[0]),
+                              steps=int(st[0]), newton=int(st[1]), failed=bool(st[3]),
+                              cell_newton=int(st[1]) * r.side * r.side, u=u if keep_results else None)
+        elif i < len(stream):
+            time.sleep(max(0.0, min(stream[i].t - now, 0.002)))
+    return out, time.perf_counter() - t0
+
+
+def summarize(stream, out, wall):
+    lat = np.array([out[q]["latency"] for q in range(len(stream))])
+    return dict(requests=len(stream), wall_s=wall,
+                throughput_req_s=len(stream) / wall,
+                cell_newton_iters_per_s=sum(o["cell_newton"] for o in out.values()) / wall,
+                p50_ms=1e3 * float(np.percentile(lat, 50)), p99_ms=1e3 * float(np.percentile(lat, 99)),
+                mean_queue_ms=1e3 * float(np.mean([o["queued"] for o in out.values()])),
+                failed=int(sum(o["failed"] for o in out.values())))
+
+
+def bench_serving(lib, args):
+    """Rate sweep of the request loop; one JSON line (value = sustained cell-Newton-iters/s
+    at the highest rate, latency percentiles per rate under `sweep`)."""
+    rates = [float(r) for r in getattr(args, "rates", "50,100,200,500,1000,2000").split(",")]
+    count = getattr(args, "requests", 200)
+    params = (f"precision=mixed preconditioner={args.precond} rel_res_tol=1e-8 abs_res_tol=0 max_iter=20 "
+              f"solver_tol=1e-6 hbm_budget={getattr(args, 'hbm_budget_gb', 64.0) * 1e9}")
+    sweep = []
+    for rate in rates:
+        stream = poisson_stream(rate, count, seed=7)
+        srv = Server(lib, params)
+        try:
+            # warm the kernels (first launches) outside the measured stream
+            srv.submit(Request(0.0, 64, 0.1, 1, gaussian_bumps(64, 64, 1, width_scale=64 / 2048)))
+            while srv.progress() == [] or srv.status()[0][1]:
+                pass
+            out, wall = replay(srv, stream)
+        finally:
+            srv.close()
+        s = summarize(stream, out, wall)
+        s["rate_req_s"] = rate
+        print(f"[serving] rate {rate:g}/s: p50 {s['p50_ms']:.1f} ms, p99 {s['p99_ms']:.1f} ms, "
+              f"{s['throughput_req_s']:.1f} req/s, {s['cell_newton_iters_per_s']:.3g} cell-Newton/s",
+              file=sys.stderr, flush=True)
+        sweep.append(s)
+    top = sweep[-1]
+    line = {"metric": "cell-Newton-iters/s (serving)", "value": top["cell_newton_iters_per_s"],
+            "unit": "cell-Newton-iters/s", "n_gpus": 1, "steps": count, "warmup": 1,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f64 residual/Jacobian/refinement, f32 Krylov", "data": "synthetic",
+            "config": {"workload": "Poisson request stream of 2D nonlinear-heat solves, side log-uniform "
+                                   "16..1024, 1-20 steps, dt=1, Newton to 1e-8 relative residual",
+                       "requests_per_rate": count, "hbm_budget_gb": getattr(args, "hbm_budget_gb", 64.0)},
+            "p50_ms": top["p50_ms"], "p99_ms": top["p99_ms"], "sweep": sweep}
+    print(json.dumps(line), flush=True)

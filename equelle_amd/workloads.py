@@ -10,6 +10,12 @@ nonlinear heat (2048^2)      k(u) = 0.1 (1 + u^2), face value = mean of the two 
                               A ~ U[0.5, 2], w ~ U[5, 50] cells, centres ~ U(domain), plus
                               U[0, 0.01] noise; zero-flux boundary; dt = 1.0, 20 steps;
                               Newton to ||F(u_k)|| <= 1e-8 ||F(u_0)||.
+two-phase (128 x 128 x 64)   log k ~ N(0,1) filtered by a Gaussian of sigma 4/3 cells truncated at
+                              radius 4 (scipy.ndimage, reflecting edges), renormalised to mean 0 and
+                              variance 1; porosity 0.2; sw_initial = 0; injector column (0,0,k) and
+                              producer column (nx-1,ny-1,k), +-1 per cell; Corey exponent 2;
+                              watervisc 1e-3, oilvisc 5e-3 (ratio 5); dt = 1.0, 30 steps; each
+                              Newton solve to 1e-8 relative residual.
 batched heat (512 instances) side ~ round(exp(U[ln 32, ln 512])), k0 ~ U[0.05, 0.5],
                               bumps scaled to the instance (width U[5,50] * side/2048, at
                               least 1.5 cells), seed = base + instance.
@@ -18,6 +24,8 @@ from __future__ import annotations
 
 import numpy as np
 
+TWOPHASE = dict(nx=128, ny=128, nz=64, dt=1.0, steps=30, watervisc=1e-3, oilvisc=5e-3, corey=2, rate=1.0,
+                porosity=0.2, rel_res_tol=1e-8)
 NONLINEAR_HEAT = dict(nx=2048, ny=2048, k=0.1, cv=1.0, dt=1.0, steps=20, rel_res_tol=1e-8)
 LINEAR_HEAT = dict(nx=100, ny=100, k=0.3, cv=1.0, dt=0.5, steps=10)
 
@@ -58,6 +66,35 @@ def batched_instances(count: int = 512, seed: int = 1234, lo: int = 32, hi: int 
     for i, (s, k) in enumerate(zip(sides, k0)):
         out.append((int(s), float(k), gaussian_bumps(int(s), int(s), seed + 1 + i, width_scale=s / 2048.0)))
     return out
+
+
+def twophase_inputs(nx: int, ny: int, nz: int, seed: int = 42, rate: float = 1.0, porosity: float = 0.2):
+    """perm, poro, sw_initial (cell order x fastest), source cells (ascending), source rates."""
+    from scipy.ndimage import gaussian_filter
+    rng = np.random.default_rng(seed)
+    g = rng.standard_normal((nz, ny, nx))
+    f = gaussian_filter(g, sigma=4.0 / 3.0, truncate=3.0, mode="reflect")
+    f = (f - f.mean()) / f.std()
+    perm = np.ascontiguousarray(np.exp(f).ravel())
+    n = nx * ny * nz
+    poro = np.full(n, porosity)
+    sw0 = np.zeros(n)
+    cells, vals = [], []
+    for k in range(nz):
+        cells += [k * nx * ny, k * nx * ny + nx * ny - 1]
+        vals += [rate, -rate]
+    if nx * ny == 1:
+        raise ValueError("two-phase grid needs at least two columns")
+    return perm, poro, sw0, np.array(cells, dtype=np.int32), np.array(vals)
+
+
+def twophase_params(nx, ny, nz, precision="fp64", solver="GMRes", precond="diagonal", rel_res_tol=1e-8,
+                    solver_tol=1e-6, max_iter=20, extra="") -> str:
+    t = TWOPHASE
+    return (f"grid_dim=3 nx={nx} ny={ny} nz={nz} dx=1 dy=1 dz=1 watervisc={t['watervisc']} oilvisc={t['oilvisc']} "
+            f"corey={t['corey']} abs_res_tol=0 rel_res_tol={rel_res_tol} max_iter={max_iter} solver={solver} "
+            f"preconditioner={precond} precision={precision} solver_tol={solver_tol} solver_max_iter=5000 "
+            f"gmres_restart=30 {extra}").strip()
 
 
 def heat_params(nx, ny, k, cv=1.0, quadratic=True, precision="fp64", precond="ilu0", solver="BiCGStab",

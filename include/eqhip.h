@@ -171,6 +171,26 @@ int eqhip_batch_bytes(void* batch, int slot, double* bytes);
 int eqhip_batch_residual(void* batch, int slot, const double* u, const double* u0, double dt, int* info,
                          double* F, int* row_ptr, int* col_ind, double* jac);
 
+/* ------------------------------------------------------------------ request loop */
+/* Continuous batching of incoming solves.  A request = one nonlinear-heat instance
+ * advanced by `steps` implicit steps of length dt.  Requests queue FIFO until their HBM
+ * footprint fits `hbm_budget` (bytes); admitted requests share launches, one Newton
+ * iteration per progress() tick, and keep their state device-resident between steps.
+ * No reference counterpart (the reference runs one program per process,
+ * compiler/PrintCUDABackendASTVisitor.cpp:18-77); this is the request loop of the
+ * serving workload.  params: the batch parameters above plus hbm_budget, max_in_flight. */
+int eqhip_server_create(const char* params, void** server);
+int eqhip_server_destroy(void* server);
+int eqhip_server_submit(void* server, int nx, int ny, double k, double cv, const double* u0, int steps, double dt,
+                        int* request);
+/* one scheduling tick; returns the requests that finished in it */
+int eqhip_server_progress(void* server, int* finished, int cap, int* nfinished);
+/* copies a finished request's field (nx*ny values) and releases it;
+ * stats = {steps, Newton, Krylov, failed}, times = {queued seconds, latency seconds} */
+int eqhip_server_result(void* server, int request, double* u, int* stats, double* times);
+/* counts = {queued, in flight}, bytes = {admitted footprint, budget} */
+int eqhip_server_status(void* server, int* counts, double* bytes);
+
 /* ------------------------------------------------------------------ measurement */
 /* Live kernel timing with HIP events on the runtime stream: mode 1 on, 0 off, 2 reset.
  * Timed launches: "spmv_fp64", "spmv_fp32" (CSR/SELL Krylov SpMV of the AD engine),

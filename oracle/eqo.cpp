@@ -498,6 +498,14 @@ LinSolveReport solve_banded_direct(const Csr& a, const std::vector<double>& b, s
     std::vector<double> rhs = b;
     std::vector<int> piv(n);
     const int kv = ku + kl;
+    double amax = 0.0, bmax = 0.0;
+    for (double v : a.v) amax = std::max(amax, std::fabs(v));
+    for (double v : b) bmax = std::max(bmax, std::fabs(v));
+    // A connected pure-Neumann operator (the two-phase pressure equation with rate
+    // sources) is singular with the constants as null space: elimination leaves a
+    // round-off last pivot.  For a consistent right-hand side any solution is valid,
+    // so the free last unknown is set to 0; an inconsistent one is a failure.
+    bool free_last = false;
     for (int j = 0; j < n; ++j) {
         const int km = std::min(kl, n - 1 - j);
         int p = j;
@@ -505,6 +513,11 @@ LinSolveReport solve_banded_direct(const Csr& a, const std::vector<double>& b, s
         for (int i = j + 1; i <= j + km; ++i)
             if (std::fabs(AB(i, j)) > best) { best = std::fabs(AB(i, j)); p = i; }
         piv[j] = p;
+        if (j == n - 1 && n > 1 && best <= 1e-11 * amax) {
+            if (std::fabs(rhs[j]) > 1e-9 * bmax) { LinSolveReport r; return r; }
+            free_last = true;
+            break;
+        }
         if (best == 0.0) { LinSolveReport r; return r; }
         const int ju = std::min(n - 1, j + kv);
         if (p != j) {
@@ -525,7 +538,7 @@ LinSolveReport solve_banded_direct(const Csr& a, const std::vector<double>& b, s
         double s = rhs[j];
         const int ju = std::min(n - 1, j + kv);
         for (int c = j + 1; c <= ju; ++c) s -= AB(j, c) * x[c];
-        x[j] = s / AB(j, j);
+        x[j] = (free_last && j == n - 1) ? 0.0 : s / AB(j, j);
     }
     LinSolveReport rep;
     rep.converged = true;

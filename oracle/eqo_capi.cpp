@@ -477,6 +477,63 @@ int eqo_heat_run(const char* params, const double* u_initial, int n, const int* 
     });
 }
 
+// Two-phase program: nsteps time steps; p_out/sw_out final fields;
+// iters[2*s] / iters[2*s+1] = pressure / transport Newton iterations of step s.
+int eqo_twophase_run(const char* params, const double* perm, const double* poro, const double* sw_initial, int n,
+                     const int* src_cells, const double* src_vals, int nsrc, const double* dts, int nsteps,
+                     double* p_out, double* sw_out, int* iters) {
+    return guard([&] {
+        Params p;
+        p.parse(params ? params : "");
+        TwoPhaseResult res = run_twophase(p, std::vector<double>(perm, perm + n), std::vector<double>(poro, poro + n),
+                                          std::vector<double>(sw_initial, sw_initial + n),
+                                          std::vector<int>(src_cells, src_cells + nsrc),
+                                          std::vector<double>(src_vals, src_vals + nsrc),
+                                          std::vector<double>(dts, dts + nsteps));
+        std::memcpy(p_out, res.p.data(), n * sizeof(double));
+        std::memcpy(sw_out, res.sw.data(), n * sizeof(double));
+        for (int s = 0; s < nsteps && iters; ++s) {
+            iters[2 * s] = res.pressure_reports[s].iterations;
+            iters[2 * s + 1] = res.transport_reports[s].iterations;
+        }
+    });
+}
+
+// One residual + Jacobian of the two-phase program: which = 0 the pressure residual
+// at pressure `p` (total mobility from sw0), which = 1 the transport residual at
+// saturation `sw` (total flux from p and sw0, step dt).  info = {n, nnz}.
+int eqo_twophase_residual(const char* params, const double* perm, const double* poro, int n, const int* src_cells,
+                          const double* src_vals, int nsrc, int which, const double* pr, const double* sw0,
+                          const double* sw, double dt, int* info, double* F, int* rp, int* ci, double* jv) {
+    return guard([&] {
+        Params p;
+        p.parse(params ? params : "");
+        Runtime rt(p);
+        TwoPhaseProgram prog(rt, p, std::vector<double>(perm, perm + n), std::vector<double>(poro, poro + n),
+                             std::vector<int>(src_cells, src_cells + nsrc),
+                             std::vector<double>(src_vals, src_vals + nsrc));
+        ADB S0, P;
+        S0.v.assign(sw0, sw0 + n);
+        P.v.assign(pr, pr + n);
+        ADB tm = adb_add(prog.water_mob(S0), prog.oil_mob(S0));
+        ADB r;
+        if (which == 0) {
+            r = prog.pressure_residual(primary_variable(P), tm);
+        } else {
+            ADB S;
+            S.v.assign(sw, sw + n);
+            r = prog.transport_residual(primary_variable(S), S0, prog.total_flux(P, tm), dt);
+        }
+        info[0] = r.size();
+        info[1] = r.J.nnz();
+        if (!F) return;
+        std::memcpy(F, r.v.data(), r.v.size() * sizeof(double));
+        std::memcpy(rp, r.J.rp.data(), r.J.rp.size() * sizeof(int));
+        std::memcpy(ci, r.J.ci.data(), r.J.ci.size() * sizeof(int));
+        std::memcpy(jv, r.J.v.data(), r.J.v.size() * sizeof(double));
+    });
+}
+
 // One residual + Jacobian evaluation of the heat program at state u (prev state u0, step dt).
 // Sizes are returned in info = {n, nnz}; pass null arrays to query first.
 int eqo_heat_residual(const char* params, const double* u, const double* u0, int n, const int* dir_faces,

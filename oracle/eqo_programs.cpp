@@ -98,6 +98,143 @@ HeatResult run_heat(const Params& p, const std::vector<double>& u_initial, const
     return res;
 }
 
+// ---------------------------------------------------------------------------
+// Two-phase program (twophase.equelle:6-113; generated form out_twophase.cpp).
+static ADB vec_dot(const std::vector<double>& a, const std::vector<double>& b, int n, int dim) {
+    ADB o;
+    o.v.resize(n);
+    for (int i = 0; i < n; ++i) {
+        double s = 0.0;
+        for (int d = 0; d < dim; ++d) s += a[(size_t)i * dim + d] * b[(size_t)i * dim + d];
+        o.v[i] = s;
+    }
+    return o;
+}
+static std::vector<double> vec_sub(const std::vector<double>& a, const std::vector<double>& b) {
+    std::vector<double> o(a.size());
+    for (size_t i = 0; i < a.size(); ++i) o[i] = a[i] - b[i];
+    return o;
+}
+static std::vector<uint8_t> cmp(const ADB& a, const ADB& b, int op) {   // 0: >=, 1: >, 2: <
+    std::vector<uint8_t> o(a.v.size());
+    for (size_t i = 0; i < a.v.size(); ++i)
+        o[i] = op == 0 ? a.v[i] >= b.v[i] : op == 1 ? a.v[i] > b.v[i] : a.v[i] < b.v[i];
+    return o;
+}
+
+TwoPhaseProgram::TwoPhaseProgram(const Runtime& er_, const Params& p, const std::vector<double>& perm_in,
+                                 const std::vector<double>& poro_v, const std::vector<int>& source_cells,
+                                 const std::vector<double>& source_values)
+    : er(er_) {
+    const int dim = er.grid().dim;
+    all_cells = er.allCells();
+    if ((int)perm_in.size() != all_cells.size() || (int)poro_v.size() != all_cells.size())
+        throw std::runtime_error("perm/poro must have one value per cell");
+    watervisc = p.get_double("watervisc", 0.0005);
+    oilvisc = p.get_double("oilvisc", 0.005);
+    corey = p.get_int("corey", 1);
+    if (corey != 1 && corey != 2) throw std::runtime_error("corey must be 1 or 2");
+    ADB perm;
+    perm.v = perm_in;
+    ADB poro_in;
+    poro_in.v = poro_v;
+    // poro = (poro_in < min_poro) ? min_poro : poro_in ; pv = poro * |AllCells()|
+    ADB min_poro = er.operatorExtend(p.get_double("min_poro", 0.0001), all_cells);
+    ADB poro = er.trinaryIf(cmp(poro_in, min_poro, 2), min_poro, poro_in);
+    pv = adb_mul(poro, er.norm(all_cells));
+    // computeTransmissibilities
+    ifaces = er.interiorFaces();
+    first = er.firstCell(ifaces);
+    second = er.secondCell(ifaces);
+    const int nif = ifaces.size();
+    const std::vector<double> cf = er.centroid(ifaces), nrm = er.normal(ifaces);
+    const std::vector<double> cdiff1 = vec_sub(er.centroid(first), cf), cdiff2 = vec_sub(er.centroid(second), cf);
+    ADB p1 = er.operatorOn(perm, all_cells, first), p2 = er.operatorOn(perm, all_cells, second);
+    ADB a = er.norm(ifaces);
+    ADB ht1 = adb_mul(adb_mul(adb_neg(a), p1), adb_div(vec_dot(nrm, cdiff1, nif, dim), vec_dot(cdiff1, cdiff1, nif, dim)));
+    ADB ht2 = adb_mul(adb_mul(a, p2), adb_div(vec_dot(nrm, cdiff2, nif, dim), vec_dot(cdiff2, cdiff2, nif, dim)));
+    trans = adb_scal_div_coll(1.0, adb_add(adb_scal_div_coll(1.0, ht1), adb_scal_div_coll(1.0, ht2)));
+    // source = source_values Extend AllCells() (source_cells: InputDomainSubsetOf, ascending)
+    IndexSet sc;
+    sc.codim = 0;
+    sc.idx = source_cells;
+    for (size_t i = 1; i < source_cells.size(); ++i)
+        if (source_cells[i] <= source_cells[i - 1]) throw std::runtime_error("source_cells must be ascending");
+    subset_indices(all_cells, sc);
+    ADB sv;
+    sv.v = source_values;
+    source = er.operatorExtend(sv, sc, all_cells);
+    insource_sw = er.operatorExtend(1.0, all_cells);
+}
+
+ADB TwoPhaseProgram::upwind(const ADB& flux, const ADB& x) const {
+    ADB x1 = er.operatorOn(x, all_cells, er.firstCell(er.interiorFaces()));
+    ADB x2 = er.operatorOn(x, all_cells, er.secondCell(er.interiorFaces()));
+    ADB zero = adb_scal_mul(0.0, flux);
+    return er.trinaryIf(cmp(flux, zero, 0), x1, x2);
+}
+
+ADB TwoPhaseProgram::total_flux(const ADB& pressure, const ADB& total_mobility) const {
+    ADB ngradp = adb_neg(er.gradient(pressure));
+    ADB face_total_mobility = upwind(ngradp, total_mobility);
+    return adb_mul(adb_mul(trans, face_total_mobility), ngradp);
+}
+
+ADB TwoPhaseProgram::pressure_residual(const ADB& pressure, const ADB& total_mobility) const {
+    return adb_sub(er.divergence(total_flux(pressure, total_mobility)), source);
+}
+
+ADB TwoPhaseProgram::water_mob(const ADB& sw) const {
+    ADB krw = corey == 2 ? adb_mul(sw, sw) : sw;
+    return adb_scal_mul(1.0 / watervisc, krw);   // krw / watervisc
+}
+
+ADB TwoPhaseProgram::oil_mob(const ADB& sw) const {
+    ADB s = adb_sub(er.operatorExtend(1.0, all_cells), sw);
+    ADB kro = corey == 2 ? adb_mul(s, s) : s;
+    return adb_scal_mul(1.0 / oilvisc, kro);     // kro / oilvisc
+}
+
+ADB TwoPhaseProgram::transport_residual(const ADB& sw, const ADB& sw0, const ADB& flux, double dt) const {
+    ADB zero = er.operatorExtend(0.0, all_cells);
+    ADB insource = er.trinaryIf(cmp(source, zero, 1), source, zero);
+    ADB outsource = er.trinaryIf(cmp(source, zero, 2), source, zero);
+    ADB mw = water_mob(sw), mo = oil_mob(sw);
+    ADB fracflow = adb_div(mw, adb_add(mw, mo));
+    ADB face_fracflow = upwind(flux, fracflow);
+    ADB water_flux = adb_mul(face_fracflow, flux);
+    ADB q = adb_add(adb_mul(insource, insource_sw), adb_mul(outsource, fracflow));
+    ADB coef = adb_scal_div_coll(dt, pv);
+    return adb_add(adb_sub(sw, sw0), adb_mul(coef, adb_sub(er.divergence(water_flux), q)));
+}
+
+TwoPhaseResult run_twophase(const Params& p, const std::vector<double>& perm, const std::vector<double>& poro,
+                            const std::vector<double>& sw_initial, const std::vector<int>& source_cells,
+                            const std::vector<double>& source_values, const std::vector<double>& dts) {
+    Runtime er(p);
+    TwoPhaseProgram prog(er, p, perm, poro, source_cells, source_values);
+    TwoPhaseResult res;
+    ADB sw0;
+    sw0.v = sw_initial;
+    ADB p0 = er.operatorExtend(0.0, prog.all_cells);
+    for (double dt : dts) {
+        // total_mobility = computeWaterMob(sw0) + computeOilMob(sw0)
+        ADB total_mobility = adb_add(prog.water_mob(sw0), prog.oil_mob(sw0));
+        NewtonReport rp, rs;
+        ADB pr = er.newtonSolve([&](const ADB& x) { return prog.pressure_residual(x, total_mobility); }, p0, &rp);
+        ADB flux = prog.total_flux(pr, total_mobility);
+        ADB sw = er.newtonSolve([&](const ADB& x) { return prog.transport_residual(x, sw0, flux, dt); },
+                                er.operatorExtend(0.5, prog.all_cells), &rs);
+        res.pressure_reports.push_back(rp);
+        res.transport_reports.push_back(rs);
+        p0 = pr;
+        sw0 = sw;
+    }
+    res.p = p0.v;
+    res.sw = sw0.v;
+    return res;
+}
+
 double time_newton_iterations(const Runtime& rt, const HeatProgram& prog, const ADB& u0, double dt, int iters,
                               int* lin_iters) {
     const auto t0 = std::chrono::steady_clock::now();

@@ -36,6 +36,39 @@ struct HeatResult {
 HeatResult run_heat(const Params& p, const std::vector<double>& u_initial, const std::vector<int>& dir_faces,
                     const std::vector<double>& dir_vals, const std::vector<double>& dts);
 
+// Two-phase incompressible flow with operator splitting, examples/dsl/twophase.equelle
+// (generated form examples/app/out_twophase.cpp): per time step one newtonSolve for
+// the pressure (total-mobility upwinded on the sign of -grad p), then one for the
+// water saturation (fractional flow upwinded on the total flux).  Parameters:
+//   watervisc (5e-4), oilvisc (5e-3), min_poro (1e-4) as in the program;
+//   corey = 1 (krw = sw, kro = 1 - sw, the program) or 2 (krw = sw^2, kro = (1-sw)^2).
+// Sources: a subset of cells (ascending) with rates; positive = injection of water.
+struct TwoPhaseProgram {
+    TwoPhaseProgram(const Runtime& er, const Params& p, const std::vector<double>& perm,
+                    const std::vector<double>& poro_in, const std::vector<int>& source_cells,
+                    const std::vector<double>& source_values);
+    ADB upwind(const ADB& flux, const ADB& x) const;
+    ADB total_flux(const ADB& pressure, const ADB& total_mobility) const;
+    ADB pressure_residual(const ADB& pressure, const ADB& total_mobility) const;
+    ADB water_mob(const ADB& sw) const;
+    ADB oil_mob(const ADB& sw) const;
+    ADB transport_residual(const ADB& sw, const ADB& sw0, const ADB& flux, double dt) const;
+
+    const Runtime& er;
+    double watervisc = 5e-4, oilvisc = 5e-3;
+    int corey = 1;
+    IndexSet all_cells, ifaces, first, second;
+    ADB pv, trans, source, insource_sw;
+};
+
+struct TwoPhaseResult {
+    std::vector<double> p, sw;
+    std::vector<NewtonReport> pressure_reports, transport_reports;
+};
+TwoPhaseResult run_twophase(const Params& p, const std::vector<double>& perm, const std::vector<double>& poro,
+                            const std::vector<double>& sw_initial, const std::vector<int>& source_cells,
+                            const std::vector<double>& source_values, const std::vector<double>& dts);
+
 // Wall time of `iters` Newton iterations (assembly + linear solve + update).
 double time_newton_iterations(const Runtime& rt, const HeatProgram& prog, const ADB& u0, double dt, int iters,
                               int* lin_iters);
