@@ -12,10 +12,13 @@ Runs on the same seeded input (16 Gaussian bumps + noise):
   plain_fp64   fused engine, FP64 BiCGStab + red-black ILU(0)
   generic_fp64 the drop-in path: the same program through EquelleRuntimeCUDA's
                device AD engine (CSR Jacobian, SELL-64 SpMV, multicolour ILU(0))
-roofline: the Krylov SpMV of the headline run (dominant kernel): 5-point stencil
-coefficients in FP32, algorithmic bytes per launch 28 n (5 coefficients + x + y,
-4 B each), average duration from HIP events recorded on the runtime stream around
-every launch in the timed region.
+roofline: the Krylov SpMV of the headline run (dominant kernel, k_spmv_dot<float>):
+5-point stencil coefficients in FP32 with the dot products it feeds fused in;
+algorithmic bytes per launch 32 n (5 coefficients + x + dot operand read, y written,
+4 B each); average execution time over 50 launches on the final state of the timed
+run, measured on the device wall clock from the first block's start to the last
+wave's end (the HIP-event time of the same dispatches, which also counts the dispatch,
+is reported as event_us).
 cpu_baseline: the CPU oracle (serial C++ restatement of the reference serial backend)
 timed for one Newton iteration of the same workload on one host core.
 
@@ -38,6 +41,10 @@ from equelle_amd.runtime import load_library  # noqa: E402
 from equelle_amd.workloads import NONLINEAR_HEAT, gaussian_bumps, heat_params  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per cell and launch of the dominant kernel at 2048^2, from rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE per dispatch scaled by a copy kernel of known size (reads
+# x2.0, writes x1.0); profiles/heat2048_fused_hbm_traffic.md has the counts and commands.
+SPMV_TRAFFIC_BYTES_PER_CELL = 33.1
 
 
 def log(msg):
@@ -50,9 +57,9 @@ def timing_avg(lib, name):
     return cnt.value, (tot.value / cnt.value if cnt.value else float("nan"))
 
 
-def run_fused(lib, precision, u0, nx, ny, dt, steps, warmup, precond="ilu0"):
+def run_fused(lib, precision, u0, nx, ny, dt, steps, warmup, precond="ilu0", graph=True):
     params = (f"precision={precision} preconditioner={precond} rel_res_tol={NONLINEAR_HEAT['rel_res_tol']} "
-              "abs_res_tol=0 max_iter=20 solver_tol=1e-6 inner_tol=1e-4").encode()
+              f"abs_res_tol=0 max_iter=20 solver_tol=1e-6 inner_tol=1e-4 graph={int(graph)}").encode()
     b = C.c_void_p()
     lib.batch_create(params, C.byref(b))
     slot = C.c_int()
@@ -81,15 +88,16 @@ def run_fused(lib, precision, u0, nx, ny, dt, steps, warmup, precond="ilu0"):
                 f"{time.perf_counter() - t:.4f}s")
         lib.device_sync()
         secs = time.perf_counter() - t0
-        # kernel durations: one more step with HIP events around every timed launch
-        # (eager launches; the timed steps above replay captured graphs)
-        lib.timing(2)
-        lib.timing(1)
-        lib.batch_step(b, iptr(slots), 1, dt)
-        lib.device_sync()
-        lib.timing(0)
-        kern = {name: timing_avg(lib, name) for name in
-                ("stencil_spmv_fp32", "stencil_spmv_fp64", "assemble", "defect_fp64")}
+        # kernel durations on the final state: HIP events on the runtime stream around
+        # 50 back-to-back launches of each stencil kernel over the whole grid
+        shape = np.zeros(3, dtype=np.int32)
+        lib.batch_shape(b, iptr(shape))
+        kern = {"shape": (int(shape[0]), int(shape[1]))}
+        for name in ("stencil_spmv", "assemble", "ilu_black", "ilu_red", "defect"):
+            ms, ev = C.c_double(), C.c_double()
+            lib.batch_kernel_time(b, name.encode(), 50, C.byref(ms), C.byref(ev))
+            kern[name] = (50, ms.value)
+            kern[name + "_event"] = ev.value
         u = np.zeros(nx * ny)
         lib.batch_state(b, slot.value, dptr(u))
     finally:
@@ -171,7 +179,11 @@ def bench_batched(lib, args):
             u = [np.zeros(s * s) for s, _, _ in inst]
             for q, sl in enumerate(slots):
                 lib.batch_state(b, int(sl), dptr(u[q]))
-            return dict(value=work / secs, ms_per_step=1e3 * secs / steps, newton_cells=work), u
+            shape = np.zeros(3, dtype=np.int32)
+            lib.batch_shape(b, iptr(shape))
+            return dict(value=work / secs, ms_per_step=1e3 * secs / steps, newton_cells=work,
+                        launch_shape={"cells_per_thread": int(shape[0]), "xcd_block_order": int(shape[1]),
+                                      "signatures_tuned": int(shape[2])}), u
         finally:
             lib.batch_destroy(b)
 
@@ -216,6 +228,74 @@ def bench_batched(lib, args):
     print(json.dumps(out), flush=True)
 
 
+def bench_twophase(lib, args):
+    """3D two-phase transport (twophase.equelle, Corey 2, viscosity ratio 5) on the device
+    AD engine: per step a pressure newtonSolve and a saturation newtonSolve, each to 1e-8
+    relative residual.  Work = cells x (pressure + transport Newton iterations)."""
+    from equelle_amd.workloads import TWOPHASE, twophase_inputs, twophase_params
+    nx, ny, nz = (int(v) for v in args.grid3.split("x"))
+    n = nx * ny * nz
+    perm, poro, sw0, sc, sv = twophase_inputs(nx, ny, nz, rate=TWOPHASE["rate"], porosity=TWOPHASE["porosity"])
+    dt = TWOPHASE["dt"]
+    runs = {}
+    for name, extra in (("mixed", f"precision=mixed solver={args.tp_solver} preconditioner=diagonal"),
+                        ("fp64", f"precision=fp64 solver={args.tp_solver} preconditioner=diagonal")):
+        params = twophase_params(nx, ny, nz, rel_res_tol=TWOPHASE["rel_res_tol"], extra=extra).encode()
+        sim = C.c_void_p()
+        lib.twophase_create(params, dptr(perm), dptr(poro), dptr(sw0), n, iptr(sc), dptr(sv), len(sc), C.byref(sim))
+        try:
+            st = np.zeros(6, dtype=np.int32)
+            for _ in range(args.warmup):
+                lib.twophase_step(sim, dt, iptr(st), None)
+            lib.device_sync()
+            newton, krylov = 0, [0, 0]
+            t0 = time.perf_counter()
+            for s in range(args.steps):
+                lib.twophase_step(sim, dt, iptr(st), None)
+                if st[2] == 0 or st[5] == 0:
+                    raise RuntimeError(f"{name}: step {s} did not converge: {st.tolist()}")
+                newton += int(st[0]) + int(st[3])
+                krylov[0] += int(st[1])
+                krylov[1] += int(st[4])
+                log(f"[bench] two-phase {name} step {s}: Newton p/s {st[0]}/{st[3]}, Krylov p/s {st[1]}/{st[4]}")
+            lib.device_sync()
+            secs = time.perf_counter() - t0
+            p, sw = np.zeros(n), np.zeros(n)
+            lib.twophase_state(sim, dptr(p), dptr(sw))
+        finally:
+            lib.twophase_destroy(sim)
+        runs[name] = dict(value=n * newton / secs, ms_per_step=1e3 * secs / args.steps, newton=newton,
+                          krylov_pressure=krylov[0], krylov_transport=krylov[1], sw=sw)
+    err = float(np.linalg.norm(runs["mixed"]["sw"] - runs["fp64"]["sw"]) / np.linalg.norm(runs["fp64"]["sw"]))
+    for r in runs.values():
+        r.pop("sw")
+    head = runs["mixed"]
+    out = {"metric": "cell-Newton-iters/s", "value": head["value"], "unit": "cell-Newton-iters/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": head["ms_per_step"],
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64 residual/Jacobian/refinement, f32 Krylov", "data": "synthetic",
+           "config": {"workload": f"3D two-phase transport {nx}x{ny}x{nz}, log-normal permeability, Corey 2, "
+                                  f"viscosity ratio 5, dt={dt}, Newton to 1e-8 relative residual per solve; "
+                                  f"{args.tp_solver} + Jacobi, device AD engine",
+                      "cells": n},
+           "runs": runs, "mixed_over_fp64": head["value"] / runs["fp64"]["value"],
+           "mixed_vs_fp64_sw_rel_error": err}
+    print(json.dumps(out), flush=True)
+
+
+def bench_copy(args):
+    """Calibration for HBM counters: a device-to-device copy of a known size
+    (torch's elementwise copy kernel), 256 MiB read + 256 MiB written per launch."""
+    import torch
+    a = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(args.warmup + args.steps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"workload": "copy", "bytes_read_per_launch": a.numel() * 4,
+                      "bytes_written_per_launch": b.numel() * 4, "launches": args.warmup + args.steps}), flush=True)
+
+
 def cpu_baseline(n_side, u0, dt):
     """Serial oracle: one Newton iteration (AD assembly + BiCGStab/ILU(0) to the same
     relative tolerance) on the same grid and state."""
@@ -242,8 +322,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline run only (for profiling)")
     ap.add_argument("--precond", default="ilu0", help="preconditioner of the fused runs: ilu0 | diagonal | none")
-    ap.add_argument("--workload", default="heat2048", choices=["heat2048", "batched", "serving"])
+    ap.add_argument("--workload", default="heat2048", choices=["heat2048", "batched", "serving", "twophase", "copy"])
     ap.add_argument("--instances", type=int, default=512)
+    ap.add_argument("--no-graph", dest="no_graph", action="store_true", help="eager launches (profiling)")
+    ap.add_argument("--grid3", default="128x128x64", help="two-phase grid nx x ny x nz")
+    ap.add_argument("--tp-solver", dest="tp_solver", default="GMRes", help="two-phase Krylov solver")
     ap.add_argument("--rates", default="50,100,200,500,1000,2000", help="serving: arrival rates (req/s)")
     ap.add_argument("--requests", type=int, default=200, help="serving: requests per rate")
     ap.add_argument("--hbm-budget-gb", dest="hbm_budget_gb", type=float, default=64.0)
@@ -251,9 +334,13 @@ def main():
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "copy":
+        return bench_copy(args)
     lib = load_library()
     if args.workload == "batched":
         return bench_batched(lib, args)
+    if args.workload == "twophase":
+        return bench_twophase(lib, args)
     if args.workload == "serving":
         from equelle_amd.serving import bench_serving
         return bench_serving(lib, args)
@@ -264,13 +351,19 @@ def main():
     dt = NONLINEAR_HEAT["dt"]
     u0 = gaussian_bumps(nx, ny, seed=2048)
     log(f"[bench] nonlinear heat {nx}x{ny}: fused engine, mixed precision")
-    head = run_fused(lib, "mixed", u0, nx, ny, dt, args.steps, args.warmup, args.precond)
+    head = run_fused(lib, "mixed", u0, nx, ny, dt, args.steps, args.warmup, args.precond, not args.no_graph)
     value = n * head["newton"] / head["seconds"]
-    cnt, avg_ms = head["kern"]["stencil_spmv_fp32"]
-    spmv_bytes = 28 * n
+    cnt, avg_ms = head["kern"]["stencil_spmv"]
+    spmv_bytes = 32 * n   # 5 FP32 coefficients + x + dot operand read, y written
     achieved = spmv_bytes / (avg_ms * 1e-3) / 1e9
     acnt, aavg = head["kern"]["assemble"]
     asm_bytes = 84 * n   # read u, u0; write F, 5 FP64 + 5 FP32 coefficients
+    _, ibavg = head["kern"]["ilu_black"]
+    _, iravg = head["kern"]["ilu_red"]
+    iavg = ibavg + iravg
+    ilu_bytes = 28 * n   # both passes: read r, dinv, 4 off-diagonal FP32 coefficients; write z
+    dcnt, davg = head["kern"]["defect"]
+    def_bytes = 64 * n   # read 5 FP64 coefficients, du, F; write r
     out = {
         "metric": "cell-Newton-iters/s",
         "value": value,
@@ -289,25 +382,34 @@ def main():
                                "refinement to 1e-6)",
                    "cells": n, "jacobian_nnz": nnz, "engine": "fused heat engine (1 instance)"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_spmv<float> (5-point stencil SpMV)", "launches": cnt, "avg_us": 1e3 * avg_ms,
-                     "algorithmic_bytes": spmv_bytes},
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": SPMV_TRAFFIC_BYTES_PER_CELL * n if (nx, ny) == (2048, 2048) else None,
+                     "kernel": "k_spmv_dot<float> (5-point stencil SpMV + fused dot partials)", "launches": cnt, "avg_us": 1e3 * avg_ms,
+                     "event_us": 1e3 * head["kern"]["stencil_spmv_event"], "algorithmic_bytes": spmv_bytes},
         "assembly_roofline": {"kernel": "k_assemble<true>", "launches": acnt, "avg_us": 1e3 * aavg,
                               "algorithmic_bytes": asm_bytes,
                               "achieved_GBs": asm_bytes / (aavg * 1e-3) / 1e9,
                               "frac": asm_bytes / (aavg * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        "other_kernels": {
+            "k_ilu_black<float> + k_ilu_red<float>": {"avg_us": 1e3 * iavg, "black_us": 1e3 * ibavg,
+                                                      "red_us": 1e3 * iravg, "algorithmic_bytes": ilu_bytes,
+                                                      "frac": ilu_bytes / (iavg * 1e-3) / 1e9 / HBM_PEAK_GBS},
+            "k_defect": {"avg_us": 1e3 * davg, "algorithmic_bytes": def_bytes,
+                         "frac": def_bytes / (davg * 1e-3) / 1e9 / HBM_PEAK_GBS}},
+        "launch_shape": {"cells_per_thread": head["kern"]["shape"][0], "xcd_block_order": head["kern"]["shape"][1],
+                         "chosen_by": "launch-shape tuner (timed per batch membership)"},
         "newton_iterations": head["newton"],
         "krylov_iterations": head["krylov"],
         "cell_steps_per_s": n * args.steps / head["seconds"],
     }
     if not args.quick:
         log("[bench] plain FP64 (fused engine)")
-        p = run_fused(lib, "fp64", u0, nx, ny, dt, args.steps, args.warmup, args.precond)
+        p = run_fused(lib, "fp64", u0, nx, ny, dt, args.steps, args.warmup, args.precond, not args.no_graph)
         pv = n * p["newton"] / p["seconds"]
-        c64, a64 = p["kern"]["stencil_spmv_fp64"]
+        c64, a64 = p["kern"]["stencil_spmv"]
         out["plain_fp64"] = {"value": pv, "ms_per_step": 1e3 * p["seconds"] / args.steps,
                              "newton_iterations": p["newton"], "krylov_iterations": p["krylov"],
-                             "spmv_avg_us": 1e3 * a64, "spmv_GBs": 56 * n / (a64 * 1e-3) / 1e9}
+                             "spmv_avg_us": 1e3 * a64, "spmv_GBs": 64 * n / (a64 * 1e-3) / 1e9}
         out["mixed_over_plain"] = value / pv
         out["mixed_vs_fp64_rel_error"] = float(np.linalg.norm(head["u"] - p["u"]) / np.linalg.norm(p["u"]))
         log("[bench] generic drop-in path (EquelleRuntimeCUDA AD engine, FP64)")

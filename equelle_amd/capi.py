@@ -84,7 +84,14 @@ SIGNATURES = {
     "batch_state": [_P, _I, _DP],
     "batch_set_state": [_P, _I, _DP],
     "batch_bytes": [_P, _I, _DP],
+    "batch_kernel_time": [_P, _S, _I, _DP, _DP],
+    "batch_shape": [_P, _IP],
     "batch_residual": [_P, _I, _DP, _DP, _D, _IP, _DP, _IP, _IP, _DP],
+    "twophase_create": [_S, _DP, _DP, _DP, _I, _IP, _DP, _I, C.POINTER(_P)],
+    "twophase_step": [_P, _D, _IP, _DP],
+    "twophase_state": [_P, _DP, _DP],
+    "twophase_residual": [_P, _I, _DP, _DP, _DP, _D, _IP, _DP, _IP, _IP, _DP],
+    "twophase_destroy": [_P],
     "server_create": [_S, C.POINTER(_P)],
     "server_destroy": [_P],
     "server_submit": [_P, _I, _I, _D, _D, _DP, _I, _D, _IP],
@@ -97,7 +104,7 @@ SIGNATURES = {
     "heat_newton_sample": [_S, _DP, _I, _I, _D, _DP, _IP],
     "heat_run": [_S, _DP, _I, _IP, _DP, _I, _DP, _I, _DP, _IP, _DP],
     "twophase_run": [_S, _DP, _DP, _DP, _I, _IP, _DP, _I, _DP, _I, _DP, _DP, _IP],
-    "twophase_residual": [_S, _DP, _DP, _I, _IP, _DP, _I, _I, _DP, _DP, _DP, _D, _IP, _DP, _IP, _IP, _DP],
+    "twophase_residual_params": [_S, _DP, _DP, _I, _IP, _DP, _I, _I, _DP, _DP, _DP, _D, _IP, _DP, _IP, _IP, _DP],
     "heat_residual": [_S, _DP, _DP, _I, _IP, _DP, _I, _D, _IP, _DP, _IP, _IP, _DP],
 }
 _RESTYPES = {"last_error": _S}

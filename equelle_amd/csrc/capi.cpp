@@ -15,6 +15,7 @@
 #include "krylov.hpp"
 #include "runtime_core.hpp"
 #include "serve.hpp"
+#include "twophase_program.hpp"
 #include "timing.hpp"
 
 using namespace eqhip;
@@ -495,6 +496,123 @@ int eqhip_heat_destroy(void* sim) {
     return guard([&] { delete static_cast<HeatSim*>(sim); });
 }
 
+// ---------------- two-phase sessions (generic AD engine, printer-style program)
+}  // extern "C"
+namespace {
+struct TwoPhaseSim {
+    Opm::ParameterGroup param;
+    std::unique_ptr<equelleCUDA::EquelleRuntimeCUDA> er;
+    std::unique_ptr<TwoPhaseProgramCUDA> prog;
+    equelleCUDA::CollOfScalar p, sw;
+};
+TwoPhaseSim& TP(void* s) {
+    EQ_REQUIRE(s, "null two-phase simulation");
+    return *static_cast<TwoPhaseSim*>(s);
+}
+}  // namespace
+extern "C" {
+
+int eqhip_twophase_create(const char* params, const double* perm, const double* poro, const double* sw_initial,
+                          int n, const int* src_cells, const double* src_vals, int nsrc, void** sim) {
+    return guard([&] {
+        auto s = std::make_unique<TwoPhaseSim>();
+        s->param = to_group(params);
+        s->er.reset(new equelleCUDA::EquelleRuntimeCUDA(s->param));
+        EQ_REQUIRE(n == s->er->core().grid().nc, "fields have " << n << " values for " << s->er->core().grid().nc
+                                                              << " cells");
+        s->prog.reset(new TwoPhaseProgramCUDA(*s->er, std::vector<double>(perm, perm + n),
+                                              std::vector<double>(poro, poro + n),
+                                              std::vector<int>(src_cells, src_cells + nsrc),
+                                              std::vector<double>(src_vals, src_vals + nsrc)));
+        s->sw = equelleCUDA::CollOfScalar(std::vector<double>(sw_initial, sw_initial + n));
+        s->p = s->er->operatorExtend(double(0), s->er->allCells());
+        *sim = s.release();
+    });
+}
+
+int eqhip_twophase_step(void* sim, double dt, int* stats, double* norms) {
+    return guard([&] {
+        TwoPhaseSim& s = TP(sim);
+        equelleCUDA::EquelleRuntimeCUDA& er = *s.er;
+        const TwoPhaseProgramCUDA& prog = *s.prog;
+        // twophase.equelle:99-113
+        const equelleCUDA::CollOfScalar total_mobility = (prog.waterMob(s.sw) + prog.oilMob(s.sw));
+        auto pressureResLocal = [&](const equelleCUDA::CollOfScalar& pressure) {
+            return prog.pressureResidual(pressure, total_mobility);
+        };
+        const equelleCUDA::CollOfScalar p = er.newtonSolve(pressureResLocal, s.p);
+        const equelleCUDA::NewtonReport rp = er.lastNewtonReport();
+        const equelleCUDA::CollOfScalar flux = prog.totalFlux(p, total_mobility);
+        const equelleCUDA::CollOfScalar sw0 = s.sw;
+        auto transportResLocal = [&](const equelleCUDA::CollOfScalar& sw) {
+            return prog.transportResidual(sw, sw0, flux, dt);
+        };
+        const equelleCUDA::CollOfScalar sw = er.newtonSolve(transportResLocal, er.operatorExtend(double(0.5), er.allCells()));
+        const equelleCUDA::NewtonReport& rs = er.lastNewtonReport();
+        s.p = p;
+        s.sw = sw;
+        if (stats) {
+            stats[0] = rp.iterations;
+            stats[1] = rp.linear_iterations;
+            stats[2] = rp.stop_reason;
+            stats[3] = rs.iterations;
+            stats[4] = rs.linear_iterations;
+            stats[5] = rs.stop_reason;
+        }
+        if (norms) {
+            norms[0] = rp.initial_norm;
+            norms[1] = rp.final_norm;
+            norms[2] = rs.initial_norm;
+            norms[3] = rs.final_norm;
+        }
+    });
+}
+
+int eqhip_twophase_state(void* sim, double* p, double* sw) {
+    return guard([&] {
+        TwoPhaseSim& s = TP(sim);
+        if (p) {
+            std::vector<double> h = s.p.copyToHost();
+            std::memcpy(p, h.data(), h.size() * sizeof(double));
+        }
+        if (sw) {
+            std::vector<double> h = s.sw.copyToHost();
+            std::memcpy(sw, h.data(), h.size() * sizeof(double));
+        }
+    });
+}
+
+int eqhip_twophase_residual(void* sim, int which, const double* pr, const double* sw0, const double* sw, double dt,
+                            int* info, double* F, int* rp, int* ci, double* jv) {
+    return guard([&] {
+        TwoPhaseSim& s = TP(sim);
+        const int n = s.er->core().grid().nc;
+        const equelleCUDA::CollOfScalar S0(std::vector<double>(sw0, sw0 + n));
+        const equelleCUDA::CollOfScalar P(std::vector<double>(pr, pr + n));
+        const equelleCUDA::CollOfScalar tm = (s.prog->waterMob(S0) + s.prog->oilMob(S0));
+        equelleCUDA::CollOfScalar r;
+        if (which == 0) {
+            r = s.prog->pressureResidual(equelleCUDA::CollOfScalar(P, true), tm);
+        } else {
+            const equelleCUDA::CollOfScalar S(std::vector<double>(sw, sw + n));
+            r = s.prog->transportResidual(equelleCUDA::CollOfScalar(S, true), S0, s.prog->totalFlux(P, tm), dt);
+        }
+        info[0] = r.size();
+        info[1] = r.data()->pat ? r.data()->pat->nnz : 0;
+        if (!F) return;
+        std::vector<double> fv = r.copyToHost();
+        std::memcpy(F, fv.data(), fv.size() * sizeof(double));
+        equelleCUDA::hostMat m = r.matrixToHost();
+        std::memcpy(rp, m.rowPtr.data(), m.rowPtr.size() * sizeof(int));
+        std::memcpy(ci, m.colInd.data(), m.colInd.size() * sizeof(int));
+        std::memcpy(jv, m.vals.data(), m.vals.size() * sizeof(double));
+    });
+}
+
+int eqhip_twophase_destroy(void* sim) {
+    return guard([&] { delete static_cast<TwoPhaseSim*>(sim); });
+}
+
 // ---------------- fused heat batches
 static FusedConfig fused_config(const char* params) {
     ParamMap m;
@@ -512,6 +630,10 @@ static FusedConfig fused_config(const char* params) {
     c.inner_tol = m.get_double("inner_tol", c.inner_tol);
     c.max_refine = m.get_int("max_refine", c.max_refine);
     c.graph = m.get_int("graph", 1) != 0;
+    c.cpt = m.get_int("cpt", c.cpt);
+    EQ_REQUIRE(c.cpt == 0 || c.cpt == 1 || c.cpt == 2 || c.cpt == 4 || c.cpt == 8, "cpt must be 0, 1, 2, 4 or 8");
+    c.xcd = m.get_int("xcd", c.xcd) != 0 ? 1 : 0;
+    c.tune = m.get_int("tune", 1) != 0;
     return c;
 }
 static HeatBatch& B(void* b) {
@@ -572,6 +694,17 @@ int eqhip_batch_set_state(void* batch, int slot, const double* u) {
 }
 int eqhip_batch_bytes(void* batch, int slot, double* bytes) {
     return guard([&] { *bytes = (double)B(batch).device_bytes(slot); });
+}
+int eqhip_batch_shape(void* batch, int* info) {
+    return guard([&] {
+        LaunchShape sh = B(batch).launch_shape();
+        info[0] = sh.cpt;
+        info[1] = sh.xcd;
+        info[2] = sh.signatures;
+    });
+}
+int eqhip_batch_kernel_time(void* batch, const char* kernel, int reps, double* avg_ms, double* event_ms) {
+    return guard([&] { *avg_ms = B(batch).time_kernel(kernel ? kernel : "", reps, event_ms); });
 }
 int eqhip_batch_residual(void* batch, int slot, const double* u, const double* u0, double dt, int* info, double* F,
                          int* rp, int* ci, double* v) {

@@ -2,6 +2,7 @@
 // include/equelle_cuda_types.hpp) over the device engine.
 #include <EquelleRuntimeCUDA.hpp>
 
+#include <algorithm>
 #include <cmath>
 #include <fstream>
 #include <iomanip>

@@ -1,9 +1,13 @@
 // Fused nonlinear-heat engine (see heat_fused.hpp).
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
+#include <functional>
 #include <map>
+#include <tuple>
+#include <type_traits>
 
 #include "heat_fused.hpp"
 #include "kernels.hpp"
@@ -21,7 +25,8 @@ enum Act { A_INIT = 0, A_ALPHA, A_OMEGA, A_RHO, A_STORE };
 enum Pc { PC_NONE = 0, PC_JACOBI = 1, PC_ILU = 2 };
 
 struct Desc {
-    int n, nx, ny, pad;
+    int n, nx, ny, xcd;           // xcd: XCD-contiguous block order (same for all of a table)
+    double rnx;                   // 1 / nx (cell -> (i, j) without integer division)
     double k, cv, dt, gx, gy, vol;
     double *u, *u0, *F, *du, *r64;
     double *C, *W, *E, *S, *N;   // FP64 Jacobian stencil coefficients
@@ -30,6 +35,7 @@ struct Desc {
     void* v[NVEC];                // Krylov vectors
     double* sc;                   // NSLOT scalars
     int* act;                     // [0] Krylov active, [1] refinement active
+    unsigned long long* ts;       // kernel timing only (else null): per block {start, end}
 };
 
 // What the last block of an instance does with the summed partials.
@@ -38,41 +44,125 @@ struct Fin {
     double tol2;
 };
 
+template <int NT = BS>
 __device__ inline void bred_store(double v, double* out) {
-    __shared__ double lds[BS / 64];
+    __shared__ double lds[NT / 64];
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     if (lane == 0) lds[w] = v;
     __syncthreads();
     if (threadIdx.x == 0) {
         double s = 0.0;
-        for (int i = 0; i < BS / 64; ++i) s += lds[i];
+        for (int i = 0; i < NT / 64; ++i) s += lds[i];
         *out = s;
     }
     __syncthreads();
 }
+constexpr int FBS = 1024;   // finaliser block: one per instance, partials read 4+ per thread
 
-// Each block covers CPB = 4*BS consecutive cells of one instance; thread t handles
-// cells base + t, base + BS + t, ... (coalesced), so reductions have one partial per
-// 1024 cells.
-constexpr int CPT = 4;
-constexpr int CPB = CPT * BS;
-#define LOCATE()                                  \
-    const int2 bi = blk[blockIdx.x];              \
-    const Desc& d = D[bi.x];                      \
-    const long base = (long)bi.y * CPB + threadIdx.x
-#define FOR_CELLS(c) for (long c = base, c##_end = min((long)d.n, base + CPB); c < c##_end; c += BS)
+// Launch shape.  Every cell kernel is instantiated for CPT (cells per thread) in
+// {1, 2, 4, 8}: a block covers CPT*BS consecutive cells of one instance and thread t
+// handles cells base + t, base + BS + t, ... (coalesced).  The shape of a batch
+// membership is chosen by measurement (see Impl::shape_for).  Reductions are made
+// shape-independent: a block writes one partial per group of BS cells (group m =
+// cells [m BS, (m+1) BS) of the block, thread t holds cell t of each group) with the
+// same tree for every shape, so results are bit-identical for any CPT and block order.
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2), so
+// workgroup w runs on XCD w % 8.  With xcd set, the logical block (position in cell
+// order) of workgroup w is chosen so that each XCD processes one contiguous range of
+// blocks: stencil neighbours (rows above and below) then hit the same L2.  Bijective
+// for any block count; partials are stored at the logical index.
+constexpr int NXCD = 8;
+__device__ inline int xcd_block(int w, int nblocks) {
+    const int q = nblocks / NXCD, r = nblocks % NXCD, x = w % NXCD;
+    return x * q + min(x, r) + w / NXCD;
+}
+#define LOCATE()                                                                                    \
+    const int lb = D[0].xcd ? xcd_block((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x;         \
+    if (D[0].ts && threadIdx.x == 0) D[0].ts[2 * lb] = wall_clock64();                              \
+    const int2 bi = blk[lb];                                                                        \
+    const Desc& d = D[bi.x];                                                                        \
+    const int base = bi.y * (CPT * BS) + (int)threadIdx.x
+// End stamp of a timed launch: the latest wave of the block (the kernel's execution
+// span is min(start) .. max(end) over blocks, on the constant-rate wall clock).
+#define KTIME_END()                                                                                 \
+    if (D[0].ts && (threadIdx.x & 63) == 0) atomicMax(D[0].ts + 2 * lb + 1, wall_clock64())
+#define FOR_CELLS(c) for (int c = base, c##_end = min(d.n, bi.y * (CPT * BS) + CPT * BS); c < c##_end; c += BS)
+#define FOR_GROUPS(m, c) for (int m = 0, c = base; m < CPT; ++m, c += BS)
+
+// One partial per group: v[m] is this thread's value for group m (0 when it has none).
+// Within a wave the CPT vectors are reduced by recursive halving (reduce-scatter):
+// at offset o the lanes with bit o clear keep the lower half of the groups and add
+// their partner's (lane ^ o) values, the others keep the upper half; once one group
+// per lane is left the usual butterfly finishes.  Every group is summed over the same
+// lane pairs in the same order as a plain shuffle tree (addition is commutative), so
+// the partial is bit-identical for every CPT.  Group g ends in lane g * 64 / CPT.
+template <int CPT>
+__device__ inline void group_store(const double (&v)[CPT], double* out) {
+    __shared__ double lds[CPT][BS / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    double a[CPT];
+    for (int m = 0; m < CPT; ++m) a[m] = v[m];
+    int o = 32;
+#pragma unroll
+    for (int g = CPT; g > 1; g >>= 1, o >>= 1) {
+        const bool up = (lane & o) != 0;
+        const int h = g >> 1;
+#pragma unroll
+        for (int q = 0; q < h; ++q) {
+            const double mine = up ? a[h + q] : a[q];
+            const double send = up ? a[q] : a[h + q];
+            a[q] = mine + __shfl_xor(send, o, 64);
+        }
+    }
+    double x = a[0];
+    for (; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    constexpr int SEG = 64 / CPT;
+    if (lane % SEG == 0) lds[lane / SEG][w] = x;
+    __syncthreads();
+    if ((int)threadIdx.x < CPT) {
+        double s = 0.0;
+        for (int i = 0; i < BS / 64; ++i) s += lds[threadIdx.x][i];
+        out[threadIdx.x] = s;
+    }
+    __syncthreads();
+}
+
+// (i, j) of cell c = i + nx j: a double multiply by 1/nx and one correction step
+// (the product is within one ulp, so the floor is off by at most one).
+__device__ inline void cell_ij(const Desc& d, int c, int& i, int& j) {
+    int jj = (int)((double)c * d.rnx);
+    int ii = c - jj * d.nx;
+    if (ii < 0) {
+        --jj;
+        ii += d.nx;
+    } else if (ii >= d.nx) {
+        ++jj;
+        ii -= d.nx;
+    }
+    i = ii;
+    j = jj;
+}
 
 // Per-instance finalisation, run by one whole block: sums the instance's block
 // partials in a fixed order (deterministic), then thread 0 applies the action to
 // the instance's scalars.
 __device__ void fin_body(const Desc& d, int2 r, const double* part, int nb, const Fin& f) {
     __shared__ double res[2];
-    const volatile double* vp = part;   // written by other blocks: read past the L1
+    // partials were written by the previous kernel (visible at this kernel's start);
+    // four independent sums per thread keep four loads in flight
     for (int q = 0; q < f.nd; ++q) {
-        double s = 0.0;
-        for (int b = r.x + threadIdx.x; b < r.y; b += BS) s += vp[(long)q * nb + b];
-        bred_store(s, &res[q]);
+        const double* __restrict__ pp = part + (long)q * nb;
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+        int b = r.x + (int)threadIdx.x;
+        for (; b + 3 * FBS < r.y; b += 4 * FBS) {
+            a0 += pp[b];
+            a1 += pp[b + FBS];
+            a2 += pp[b + 2 * FBS];
+            a3 += pp[b + 3 * FBS];
+        }
+        for (; b < r.y; b += FBS) a0 += pp[b];
+        bred_store<FBS>((a0 + a1) + (a2 + a3), &res[q]);
     }
     if (threadIdx.x != 0) return;
     const int act = f.act, maxit = f.maxit, slot = f.slot;
@@ -125,7 +215,7 @@ __device__ void fin_body(const Desc& d, int2 r, const double* part, int nb, cons
 }
 
 
-__global__ __launch_bounds__(BS) void k_fin(const Desc* __restrict__ D, const int2* __restrict__ rng,
+__global__ __launch_bounds__(FBS) void k_fin(const Desc* __restrict__ D, const int2* __restrict__ rng,
                                             const double* __restrict__ part, int nb, Fin f) {
     fin_body(D[blockIdx.x], rng[blockIdx.x], part, nb, f);
 }
@@ -133,16 +223,18 @@ __global__ __launch_bounds__(BS) void k_fin(const Desc* __restrict__ D, const in
 // ---------------------------------------------------------------------------
 // Residual + Jacobian assembly: one pass over u, u0; writes F, C/W/E/S/N
 // (FP64 and, when mixed, FP32) and per-block partial ||F||^2.
-template <bool MIXED>
+template <bool MIXED, int CPT>
 __global__ __launch_bounds__(BS) void k_assemble(const Desc* __restrict__ D, const int2* __restrict__ blk,
                                                  double* __restrict__ part) {
     LOCATE();
-    double f2 = 0.0;
+    double f2[CPT] = {};
     const int nx = d.nx, ny = d.ny;
     const double k = d.k, coef = d.dt / (d.cv * d.vol), gx = d.gx, gy = d.gy;
     const double* __restrict__ u = d.u;
-    FOR_CELLS(c) {
-        const int i = (int)(c % nx), j = (int)(c / nx);
+    FOR_GROUPS(m, c) {
+        if (c >= d.n) continue;
+        int i, j;
+        cell_ij(d, c, i, j);
         const double uc = u[c];
         const double kc = k * (1.0 + uc * uc), dkc = 2.0 * k * uc;
         double div = 0.0, diag = 1.0, cw = 0.0, ce = 0.0, cs = 0.0, cn = 0.0;
@@ -173,13 +265,14 @@ __global__ __launch_bounds__(BS) void k_assemble(const Desc* __restrict__ D, con
             ((float*)d.Sm)[c] = (float)cs;
             ((float*)d.Nm)[c] = (float)cn;
         }
-        f2 += F * F;
+        f2[m] = F * F;
     }
-    bred_store(f2, part + blockIdx.x);
+    group_store<CPT>(f2, part + lb * CPT);
+    KTIME_END();
 }
 
 template <class T>
-__device__ inline T stencil_apply(const Desc& d, const T* __restrict__ x, long c, int i, int j) {
+__device__ inline T stencil_apply(const Desc& d, const T* __restrict__ x, int c, int i, int j) {
     const int nx = d.nx;
     T acc = ((const T*)d.Cm)[c] * x[c];
     if (i > 0) acc += ((const T*)d.Wm)[c] * x[c - 1];
@@ -191,38 +284,44 @@ __device__ inline T stencil_apply(const Desc& d, const T* __restrict__ x, long c
 
 // y = A x with fused partial dot products: dot0 = V[a0].y, dot1 = V[a1].y (a1 < 0: y.y).
 // Skipped for inactive instances (partials 0).
-template <class T>
+template <class T, int CPT>
 __global__ __launch_bounds__(BS) void k_spmv_dot(const Desc* __restrict__ D, const int2* __restrict__ blk, int ix,
-                                                 int iy, int nd, int a0, int a1, double* __restrict__ part, int nb) {
+                                                 int iy, int nd, int a0, int a1, double* __restrict__ part, int ng) {
     LOCATE();
-    double s0 = 0.0, s1 = 0.0;
+    double s0[CPT] = {}, s1[CPT] = {};
     if (d.act[0]) {
         const T* __restrict__ x = (const T*)d.v[ix];
         T* __restrict__ y = (T*)d.v[iy];
         const T* __restrict__ va = (const T*)d.v[a0];
         const T* __restrict__ vb = a1 >= 0 ? (const T*)d.v[a1] : nullptr;
-        FOR_CELLS(c) {
-            const int i = (int)(c % d.nx), j = (int)(c / d.nx);
+        FOR_GROUPS(m, c) {
+            if (c >= d.n) continue;
+            int i, j;
+            cell_ij(d, c, i, j);
             const T yc = stencil_apply<T>(d, x, c, i, j);
             y[c] = yc;
-            s0 += (double)va[c] * (double)yc;
-            if (nd > 1) s1 += (vb ? (double)vb[c] : (double)yc) * (double)yc;
+            s0[m] = (double)va[c] * (double)yc;
+            if (nd > 1) s1[m] = (vb ? (double)vb[c] : (double)yc) * (double)yc;
         }
     }
-    bred_store(s0, part + blockIdx.x);
-    if (nd > 1) bred_store(s1, part + nb + blockIdx.x);
+    group_store<CPT>(s0, part + lb * CPT);
+    if (nd > 1) group_store<CPT>(s1, part + ng + lb * CPT);
+    KTIME_END();
 }
 
 // FP64 defect for the instances under refinement: r64 = F - J du.
+template <int CPT>
 __global__ __launch_bounds__(BS) void k_defect(const Desc* __restrict__ D, const int2* __restrict__ blk,
                                                double* __restrict__ part) {
     LOCATE();
-    double r2 = 0.0;
+    double r2[CPT] = {};
     if (d.act[1]) {
         const int nx = d.nx;
         const double* __restrict__ x = d.du;
-        FOR_CELLS(c) {
-            const int i = (int)(c % nx), j = (int)(c / nx);
+        FOR_GROUPS(m, c) {
+            if (c >= d.n) continue;
+            int i, j;
+        cell_ij(d, c, i, j);
             double acc = d.C[c] * x[c];
             if (i > 0) acc += d.W[c] * x[c - 1];
             if (i < nx - 1) acc += d.E[c] * x[c + 1];
@@ -230,17 +329,18 @@ __global__ __launch_bounds__(BS) void k_defect(const Desc* __restrict__ D, const
             if (j < d.ny - 1) acc += d.N[c] * x[c + nx];
             const double r = d.F[c] - acc;
             d.r64[c] = r;
-            r2 += r * r;
+            r2[m] = r * r;
         }
     }
-    bred_store(r2, part + blockIdx.x);
+    group_store<CPT>(r2, part + lb * CPT);
+    KTIME_END();
 }
 
 // Preconditioner set-up: Jacobi 1/C, or red-black ILU(0) pivots.  Red = (i+j) even
 // cells come first in the ordering; red rows have no L part, a black row's L part is its
 // red neighbours and its pivot is C_b - sum_m A_bm A_mb / C_m (fill-in dropped).
 // dinv holds 1/C for red cells and 1/pivot for black cells.
-template <class T>
+template <class T, int CPT>
 __global__ __launch_bounds__(BS) void k_precond_setup(const Desc* __restrict__ D, const int2* __restrict__ blk,
                                                       int mode) {
     LOCATE();
@@ -250,7 +350,8 @@ __global__ __launch_bounds__(BS) void k_precond_setup(const Desc* __restrict__ D
     FOR_CELLS(c) {
         T piv = Cm[c];
         if (mode == PC_ILU) {
-            const int i = (int)(c % nx), j = (int)(c / nx);
+            int i, j;
+        cell_ij(d, c, i, j);
             if ((i + j) & 1) {
                 if (i > 0) piv -= Wm[c] * Em[c - 1] / Cm[c - 1];
                 if (i < nx - 1) piv -= Em[c] * Wm[c + 1] / Cm[c + 1];
@@ -262,12 +363,12 @@ __global__ __launch_bounds__(BS) void k_precond_setup(const Desc* __restrict__ D
     }
 }
 
-// z = M^{-1} r for red-black ILU(0) in one pass: a black cell solves
-// z_b = (r_b - sum_m A_bm r_m / C_m) / piv_b; a red cell recomputes the z of its
-// black neighbours (two-hop stencil, cached) and solves z_r = (r_r - sum_b A_rb z_b) / C_r.
-// 1/C_m of a red neighbour is its dinv, so only multiplications remain.
+// z = M^{-1} r for red-black ILU(0) in two passes: a black cell solves
+// z_b = (r_b - sum_m A_bm r_m / C_m) / piv_b, then a red cell solves
+// z_r = (r_r - sum_b A_rb z_b) / C_r.  1/C_m of a red neighbour is its dinv, so only
+// multiplications remain.
 template <class T>
-__device__ inline T black_value(const Desc& d, const T* __restrict__ r, long b, int i, int j) {
+__device__ inline T black_value(const Desc& d, const T* __restrict__ r, int b, int i, int j) {
     const int nx = d.nx, ny = d.ny;
     const T* dv = (const T*)d.dinv;
     const T *Wm = (const T*)d.Wm, *Em = (const T*)d.Em, *Sm = (const T*)d.Sm, *Nm = (const T*)d.Nm;
@@ -278,53 +379,70 @@ __device__ inline T black_value(const Desc& d, const T* __restrict__ r, long b, 
     if (j < ny - 1) s -= Nm[b] * (r[b + nx] * dv[b + nx]);
     return s * dv[b];
 }
-template <class T>
-__global__ __launch_bounds__(BS) void k_ilu_apply(const Desc* __restrict__ D, const int2* __restrict__ blk, int ir,
+// Pass 1 of the apply: z of the black cells (their L part only touches red cells,
+// whose z = r / C before the U sweep).
+template <class T, int CPT>
+__global__ __launch_bounds__(BS) void k_ilu_black(const Desc* __restrict__ D, const int2* __restrict__ blk, int ir,
                                                   int iz) {
     LOCATE();
-    if (!d.act[0]) return;
-    const int nx = d.nx, ny = d.ny;
-    const T* __restrict__ r = (const T*)d.v[ir];
-    T* __restrict__ zo = (T*)d.v[iz];
-    FOR_CELLS(c) {
-        const int i = (int)(c % nx), j = (int)(c / nx);
-        T z;
-        if ((i + j) & 1) {
-            z = black_value<T>(d, r, c, i, j);
-        } else {
-            T s = r[c];
-            if (i > 0) s -= ((const T*)d.Wm)[c] * black_value<T>(d, r, c - 1, i - 1, j);
-            if (i < nx - 1) s -= ((const T*)d.Em)[c] * black_value<T>(d, r, c + 1, i + 1, j);
-            if (j > 0) s -= ((const T*)d.Sm)[c] * black_value<T>(d, r, c - nx, i, j - 1);
-            if (j < ny - 1) s -= ((const T*)d.Nm)[c] * black_value<T>(d, r, c + nx, i, j + 1);
-            z = s * ((const T*)d.dinv)[c];
+    if (d.act[0]) {
+        const T* __restrict__ r = (const T*)d.v[ir];
+        T* __restrict__ zo = (T*)d.v[iz];
+        FOR_CELLS(c) {
+            int i, j;
+            cell_ij(d, c, i, j);
+            if ((i + j) & 1) zo[c] = black_value<T>(d, r, c, i, j);
         }
-        zo[c] = z;
     }
+    KTIME_END();
+}
+// Pass 2: z of the red cells from the black z of pass 1 (the U part of a red row).
+template <class T, int CPT>
+__global__ __launch_bounds__(BS) void k_ilu_red(const Desc* __restrict__ D, const int2* __restrict__ blk, int ir,
+                                                int iz) {
+    LOCATE();
+    if (d.act[0]) {
+        const int nx = d.nx, ny = d.ny;
+        const T* __restrict__ r = (const T*)d.v[ir];
+        T* __restrict__ z = (T*)d.v[iz];
+        FOR_CELLS(c) {
+            int i, j;
+            cell_ij(d, c, i, j);
+            if ((i + j) & 1) continue;
+            T s = r[c];
+            if (i > 0) s -= ((const T*)d.Wm)[c] * z[c - 1];
+            if (i < nx - 1) s -= ((const T*)d.Em)[c] * z[c + 1];
+            if (j > 0) s -= ((const T*)d.Sm)[c] * z[c - nx];
+            if (j < ny - 1) s -= ((const T*)d.Nm)[c] * z[c + nx];
+            z[c] = s * ((const T*)d.dinv)[c];
+        }
+    }
+    KTIME_END();
 }
 
 // Krylov start: r = b (already in r), rh = r, x = 0, p = v = 0; partial dots (rh.r, r.r).
-template <class T>
+template <class T, int CPT>
 __global__ __launch_bounds__(BS) void k_kinit(const Desc* __restrict__ D, const int2* __restrict__ blk,
-                                              double* __restrict__ part, int nb) {
+                                              double* __restrict__ part, int ng) {
     LOCATE();
-    double rr = 0.0;
+    double rr[CPT] = {};
     if (d.act[1]) {
-        FOR_CELLS(c) {
+        FOR_GROUPS(m, c) {
+            if (c >= d.n) continue;
             const T rc = ((T*)d.v[V_R])[c];
             ((T*)d.v[V_RH])[c] = rc;
             ((T*)d.v[V_X])[c] = 0;
             ((T*)d.v[V_P])[c] = 0;
             ((T*)d.v[V_V])[c] = 0;
-            rr += (double)rc * (double)rc;
+            rr[m] = (double)rc * (double)rc;
         }
     }
-    bred_store(rr, part + blockIdx.x);
-    bred_store(rr, part + nb + blockIdx.x);
+    group_store<CPT>(rr, part + lb * CPT);
+    group_store<CPT>(rr, part + ng + lb * CPT);
 }
 
 // p = r + beta (p - omega v) (first iteration: p = r); Jacobi / none: ph = M p
-template <class T>
+template <class T, int CPT>
 __global__ __launch_bounds__(BS) void k_b1(const Desc* __restrict__ D, const int2* __restrict__ blk, int pc) {
     LOCATE();
     if (!d.act[0]) return;
@@ -341,7 +459,7 @@ __global__ __launch_bounds__(BS) void k_b1(const Desc* __restrict__ D, const int
     }
 }
 // s = r - alpha v ; Jacobi / none: sh = M s
-template <class T>
+template <class T, int CPT>
 __global__ __launch_bounds__(BS) void k_b2(const Desc* __restrict__ D, const int2* __restrict__ blk, int pc) {
     LOCATE();
     if (!d.act[0]) return;
@@ -353,24 +471,25 @@ __global__ __launch_bounds__(BS) void k_b2(const Desc* __restrict__ D, const int
     }
 }
 // x += alpha ph + omega sh ; r = s - omega t ; partial dots (rh.r, r.r)
-template <class T>
+template <class T, int CPT>
 __global__ __launch_bounds__(BS) void k_b3(const Desc* __restrict__ D, const int2* __restrict__ blk,
-                                           double* __restrict__ part, int nb) {
+                                           double* __restrict__ part, int ng) {
     LOCATE();
-    double a0 = 0.0, a1 = 0.0;
+    double a0[CPT] = {}, a1[CPT] = {};
     if (d.act[0]) {
         const T alpha = (T)d.sc[S_ALPHA], omega = (T)d.sc[S_OMEGA];
         T* x = (T*)d.v[V_X];
-        FOR_CELLS(c) {
+        FOR_GROUPS(m, c) {
+            if (c >= d.n) continue;
             x[c] += alpha * ((const T*)d.v[V_PH])[c] + omega * ((const T*)d.v[V_SH])[c];
             const T r = ((const T*)d.v[V_S])[c] - omega * ((const T*)d.v[V_T])[c];
             ((T*)d.v[V_R])[c] = r;
-            a0 += (double)((const T*)d.v[V_RH])[c] * (double)r;
-            a1 += (double)r * (double)r;
+            a0[m] = (double)((const T*)d.v[V_RH])[c] * (double)r;
+            a1[m] = (double)r * (double)r;
         }
     }
-    bred_store(a0, part + blockIdx.x);
-    bred_store(a1, part + nb + blockIdx.x);
+    group_store<CPT>(a0, part + lb * CPT);
+    group_store<CPT>(a1, part + ng + lb * CPT);
 }
 
 __global__ void k_count_active(const Desc* __restrict__ D, int ninst, int which, int* __restrict__ out) {
@@ -412,7 +531,7 @@ __global__ void k_refine_flags(const Desc* __restrict__ D, int ninst, int mode, 
 }
 
 // r (Krylov precision) = r64 / ||r64||  (mixed) or = F (fp64 mode: r64 == F)
-template <class T>
+template <class T, int CPT>
 __global__ __launch_bounds__(BS) void k_load_rhs(const Desc* __restrict__ D, const int2* __restrict__ blk,
                                                  int scaled) {
     LOCATE();
@@ -421,7 +540,7 @@ __global__ __launch_bounds__(BS) void k_load_rhs(const Desc* __restrict__ D, con
     FOR_CELLS(c)((T*)d.v[V_R])[c] = (T)(d.r64[c] * scale);
 }
 // du += ||r64|| x  (mixed) or du = x (fp64)
-template <class T>
+template <class T, int CPT>
 __global__ __launch_bounds__(BS) void k_accum(const Desc* __restrict__ D, const int2* __restrict__ blk, int scaled) {
     LOCATE();
     if (!d.act[1]) return;
@@ -432,6 +551,7 @@ __global__ __launch_bounds__(BS) void k_accum(const Desc* __restrict__ D, const 
         else d.du[c] = x;
     }
 }
+template <int CPT>
 __global__ __launch_bounds__(BS) void k_zero_du(const Desc* __restrict__ D, const int2* __restrict__ blk) {
     LOCATE();
     FOR_CELLS(c) {
@@ -439,6 +559,7 @@ __global__ __launch_bounds__(BS) void k_zero_du(const Desc* __restrict__ D, cons
         d.r64[c] = d.F[c];
     }
 }
+template <int CPT>
 __global__ __launch_bounds__(BS) void k_newton_update(const Desc* __restrict__ D, const int2* __restrict__ blk) {
     LOCATE();
     FOR_CELLS(c) d.u[c] -= d.du[c];
@@ -447,6 +568,8 @@ __global__ void k_gather_slot(const Desc* __restrict__ D, int ninst, int slot, d
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < ninst) out[i] = D[i].sc[slot];
 }
+#undef KTIME_END
+#undef FOR_GROUPS
 #undef FOR_CELLS
 #undef LOCATE
 
@@ -462,11 +585,33 @@ struct Inst {
     StepStats st;
 };
 
+// Launch shape of a table: cells per thread and XCD-contiguous block order.
+struct Shape {
+    int cpt = 4;
+    int xcd = 1;
+    bool operator<(const Shape& o) const { return cpt != o.cpt ? cpt < o.cpt : xcd < o.xcd; }
+};
+constexpr int CPT_MAX = 8;
+
+// Calls f(std::integral_constant<int, CPT>) for the table's cells-per-thread.
+template <class F>
+void by_cpt(int cpt, F&& f) {
+    switch (cpt) {
+        case 1: f(std::integral_constant<int, 1>()); break;
+        case 2: f(std::integral_constant<int, 2>()); break;
+        case 4: f(std::integral_constant<int, 4>()); break;
+        case 8: f(std::integral_constant<int, 8>()); break;
+        default: throw std::runtime_error("unsupported cells per thread " + std::to_string(cpt));
+    }
+}
+
 struct Table {
     std::vector<Inst*> inst;
     DevPtr<Desc> desc;
-    DevPtr<int2> blk, rng;
-    int nb = 0;
+    DevPtr<int2> blk, rng;   // rng: each instance's partials (groups of BS cells)
+    int nb = 0;              // blocks
+    int ng = 0;              // groups = nb * cpt
+    Shape shape;
     DevPtr<double> part;
     uint64_t version = 0;
     int solves = 0;   // Krylov solves run on this membership (graphs are captured from the second)
@@ -504,6 +649,7 @@ struct HeatBatch::Impl {
         d.n = I.n;
         d.nx = I.nx;
         d.ny = I.ny;
+        d.rnx = 1.0 / I.nx;
         d.k = I.k;
         d.cv = I.cv;
         d.dt = I.dt;
@@ -541,31 +687,126 @@ struct HeatBatch::Impl {
         return d;
     }
 
+    TableP build_table(const std::vector<Inst*>& list, Shape sh, unsigned long long* ts = nullptr) {
+        TableP tp = std::make_shared<Table>();
+        Table& t = *tp;
+        t.version = version;
+        t.inst = list;
+        t.shape = sh;
+        const int cpb = sh.cpt * BS;
+        std::vector<Desc> ds;
+        std::vector<int2> blk, rng;
+        for (size_t q = 0; q < list.size(); ++q) {
+            Desc d = desc_of(*list[q]);
+            d.xcd = sh.xcd;
+            d.ts = ts;
+            ds.push_back(d);
+            const int nbk = (list[q]->n + cpb - 1) / cpb;
+            rng.push_back(make_int2((int)blk.size() * sh.cpt, ((int)blk.size() + nbk) * sh.cpt));
+            for (int b = 0; b < nbk; ++b) blk.push_back(make_int2((int)q, b));
+        }
+        t.nb = (int)blk.size();
+        t.ng = t.nb * sh.cpt;
+        t.desc = dev_from_host(ds.data(), ds.size());
+        t.blk = dev_from_host(blk.data(), blk.size());
+        t.rng = dev_from_host(rng.data(), rng.size());
+        t.part = dev_alloc<double>(2 * (size_t)t.ng + 2);
+        return tp;
+    }
     TableP table(const std::vector<Inst*>& list) {
         std::vector<int> key;
         for (Inst* I : list) key.push_back(I->id);
         auto it = tcache.find(key);
         if (it != tcache.end() && it->second->version == version) return it->second;
         if (tcache.size() > 64) tcache.clear();
-        TableP tp = std::make_shared<Table>();
-        Table& t = *tp;
-        t.version = version;
-        t.inst = list;
-        std::vector<Desc> ds;
-        std::vector<int2> blk, rng;
-        for (size_t q = 0; q < list.size(); ++q) {
-            ds.push_back(desc_of(*list[q]));
-            const int nbk = (list[q]->n + CPB - 1) / CPB;
-            rng.push_back(make_int2((int)blk.size(), (int)blk.size() + nbk));
-            for (int b = 0; b < nbk; ++b) blk.push_back(make_int2((int)q, b));
-        }
-        t.nb = (int)blk.size();
-        t.desc = dev_from_host(ds.data(), ds.size());
-        t.blk = dev_from_host(blk.data(), blk.size());
-        t.rng = dev_from_host(rng.data(), rng.size());
-        t.part = dev_alloc<double>(2 * (size_t)t.nb + 2);
+        TableP tp = build_table(list, shape_for(list));
         tcache[key] = tp;
         return tp;
+    }
+
+    // ---- launch-shape tuner -------------------------------------------------------
+    // A membership's signature is (log2 total cells, log2 instance count, log2 largest
+    // instance).  The first time a signature is seen, each shape (cells per thread 1, 2,
+    // 4, 8 x XCD order on/off) is timed on that membership with HIP events around the
+    // Krylov iteration's stencil kernels (SpMV with its dots, and the preconditioner's
+    // passes), and the fastest is kept for the signature.  Results do not depend on the
+    // shape (shape-independent reductions), so tuning changes time only.
+    std::map<std::tuple<int, int, int>, Shape> tuned;
+    Shape last_shape;
+    static int ilog2(long v) {
+        int r = 0;
+        while (v > 1) v >>= 1, ++r;
+        return r;
+    }
+    Shape shape_for(const std::vector<Inst*>& list) {
+        if (cfg.cpt > 0) {
+            Shape sh;
+            sh.cpt = cfg.cpt;
+            sh.xcd = cfg.xcd;
+            return last_shape = sh;
+        }
+        long tot = 0, mx = 0;
+        for (Inst* I : list) tot += I->n, mx = std::max<long>(mx, I->n);
+        const auto sig = std::make_tuple(ilog2(tot), ilog2((long)list.size()), ilog2(mx));
+        auto it = tuned.find(sig);
+        if (it != tuned.end()) return last_shape = it->second;
+        Shape best;
+        if (cfg.tune) {
+            double best_ms = 1e300;
+            for (int cpt : {1, 2, 4, 8})
+                for (int xcd : {0, 1}) {
+                    Shape sh;
+                    sh.cpt = cpt;
+                    sh.xcd = xcd;
+                    const double ms = time_shape(list, sh);
+                    if (ms < best_ms) best_ms = ms, best = sh;
+                }
+        }
+        tuned[sig] = best;
+        return last_shape = best;
+    }
+    double time_shape(const std::vector<Inst*>& list, Shape sh) {
+        TableP tp = build_table(list, sh);
+        Table& t = *tp;
+        // every instance active so the kernels do their full work (a solve resets the flags)
+        const int one[2] = {1, 1};
+        for (Inst* I : list) EQ_HIP(hipMemcpyAsync(I->act->p, one, sizeof(one), hipMemcpyHostToDevice, S()));
+        // a Krylov round (SpMV with dots, preconditioner) and a Newton round; a Newton
+        // iteration runs about eight Krylov rounds
+        auto krylov_round = [&] {
+            if (mixed) spmv_dot<float>(t, V_PH, V_V, 1, (int)V_RH, -1, F(A_STORE, 1, 0, 0, NSLOT - 1), false);
+            else spmv_dot<double>(t, V_PH, V_V, 1, (int)V_RH, -1, F(A_STORE, 1, 0, 0, NSLOT - 1), false);
+            if (mixed) precond<float>(t, V_P, V_PH);
+            else precond<double>(t, V_P, V_PH);
+        };
+        // (the FP64 defect stands for the Newton-level stencil kernels; it only writes
+        // the refinement residual, which every solve re-initialises)
+        auto newton_round = [&] {
+            by_cpt(t.shape.cpt, [&](auto cpt_) {
+                constexpr int CP = decltype(cpt_)::value;
+                hipLaunchKernelGGL((k_defect<CP>), dim3((unsigned)t.nb), dim3(BS), 0, S(), t.desc->p, t.blk->p,
+                                   t.part->p);
+                EQ_HIP(hipGetLastError());
+            });
+        };
+        hipEvent_t e0, e1;
+        EQ_HIP(hipEventCreate(&e0));
+        EQ_HIP(hipEventCreate(&e1));
+        auto timed = [&](auto&& fn, int reps) {
+            fn();   // warm
+            EQ_HIP(hipEventRecord(e0, S()));
+            for (int r = 0; r < reps; ++r) fn();
+            EQ_HIP(hipEventRecord(e1, S()));
+            EQ_HIP(hipEventSynchronize(e1));
+            float ms = 0.f;
+            EQ_HIP(hipEventElapsedTime(&ms, e0, e1));
+            return (double)ms / reps;
+        };
+        const double kr = timed(krylov_round, 3);
+        const double nr = timed(newton_round, 2);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        return 8.0 * kr + nr;
     }
 
     int count_active(const Table& t, int which) {
@@ -578,8 +819,8 @@ struct HeatBatch::Impl {
     // Finalisation of a reduction: one block per instance (kernel boundaries make the
     // partials of all blocks visible; no device-wide fences inside the kernels).
     void fin(const Table& t, const Fin& f) {
-        hipLaunchKernelGGL(k_fin, dim3((unsigned)t.inst.size()), dim3(BS), 0, S(), t.desc->p, t.rng->p, t.part->p,
-                           t.nb, f);
+        hipLaunchKernelGGL(k_fin, dim3((unsigned)t.inst.size()), dim3(FBS), 0, S(), t.desc->p, t.rng->p, t.part->p,
+                           t.ng, f);
         EQ_HIP(hipGetLastError());
     }
     static Fin F(int act, int nd, double tol2 = 0, int maxit = 0, int slot = 0) {
@@ -591,44 +832,50 @@ struct HeatBatch::Impl {
         f.slot = slot;
         return f;
     }
+// Launch over every block of the table; the kernel is named with CP, the table's
+// cells per thread, e.g. HL((k_b1<T, CP>), pc).
 #define HL(kernel, ...)                                                                                    \
-    do {                                                                                                   \
+    by_cpt(t.shape.cpt, [&](auto cpt_) {                                                                   \
+        constexpr int CP = decltype(cpt_)::value;                                                          \
         hipLaunchKernelGGL(kernel, dim3((unsigned)t.nb), dim3(BS), 0, S(), t.desc->p, t.blk->p, ##__VA_ARGS__); \
         EQ_HIP(hipGetLastError());                                                                         \
-    } while (0)
+    })
 
     void assemble(const Table& t) {
         {
             TimedLaunch tl("assemble");
-            if (mixed) HL(k_assemble<true>, t.part->p);
-            else HL(k_assemble<false>, t.part->p);
+            if (mixed) HL((k_assemble<true, CP>), t.part->p);
+            else HL((k_assemble<false, CP>), t.part->p);
         }
         fin(t, F(A_STORE, 1, 0, 0, S_FN));
     }
 
     template <class T>
-    void spmv_dot(const Table& t, int ix, int iy, int nd, int a0, int a1, const Fin& f) {
+    void spmv_dot(const Table& t, int ix, int iy, int nd, int a0, int a1, const Fin& f, bool with_fin = true) {
         {
             TimedLaunch tl(sizeof(T) == 4 ? "stencil_spmv_fp32" : "stencil_spmv_fp64");
-            HL(k_spmv_dot<T>, ix, iy, nd, a0, a1, t.part->p, t.nb);
+            HL((k_spmv_dot<T, CP>), ix, iy, nd, a0, a1, t.part->p, t.ng);
         }
-        fin(t, f);
+        if (with_fin) fin(t, f);
     }
     template <class T>
     void precond(const Table& t, int ir, int iz) {
-        if (pc == PC_ILU) HL(k_ilu_apply<T>, ir, iz);
+        if (pc == PC_ILU) {
+            HL((k_ilu_black<T, CP>), ir, iz);
+            HL((k_ilu_red<T, CP>), ir, iz);
+        }
     }
 
     // BiCGStab for the instances whose refinement flag is set; rhs in V_R.
     template <class T>
     void bicg_iteration(const Table& t, double tol, int maxit) {
-        HL(k_b1<T>, pc);
+        HL((k_b1<T, CP>), pc);
         precond<T>(t, V_P, V_PH);
         spmv_dot<T>(t, V_PH, V_V, 1, (int)V_RH, -1, F(A_ALPHA, 1));   // v = A ph; alpha = rho / rh.v
-        HL(k_b2<T>, pc);
+        HL((k_b2<T, CP>), pc);
         precond<T>(t, V_S, V_SH);
         spmv_dot<T>(t, V_SH, V_T, 2, (int)V_S, -1, F(A_OMEGA, 2));    // t = A sh; omega = t.s / t.t
-        HL(k_b3<T>, t.part->p, t.nb);
+        HL((k_b3<T, CP>), t.part->p, t.ng);
         fin(t, F(A_RHO, 2, tol * tol, maxit));
     }
 
@@ -648,7 +895,7 @@ struct HeatBatch::Impl {
 
     template <class T>
     void bicgstab(Table& t, double tol, int maxit) {
-        HL(k_kinit<T>, t.part->p, t.nb);
+        HL((k_kinit<T, CP>), t.part->p, t.ng);
         fin(t, F(A_INIT, 2));
         // a membership seen once (request churn) runs eagerly; capture pays off from its second solve
         if (use_graph && !timing_enabled() && t.solves++ > 0) {
@@ -685,26 +932,26 @@ struct HeatBatch::Impl {
     template <class T>
     void linear_solve(Table& t) {
         // du = 0, r64 = F; refinement flags from ||F||
-        HL(k_zero_du);
+        HL((k_zero_du<CP>));
         const unsigned ni = (unsigned)t.inst.size();
         hipLaunchKernelGGL(k_refine_flags, dim3((ni + 63) / 64), dim3(64), 0, S(), t.desc->p, (int)ni, 0,
                            cfg.solver_tol, cfg.max_refine);
         if (!mixed) {
-            HL(k_load_rhs<T>, 0);
+            HL((k_load_rhs<T, CP>), 0);
             precond_setup<T>(t);
             bicgstab<T>(t, cfg.solver_tol, cfg.solver_max_iter);
-            HL(k_accum<T>, 0);
+            HL((k_accum<T, CP>), 0);
             return;
         }
         precond_setup<T>(t);
         for (int ref = 0; ref < cfg.max_refine; ++ref) {
             if (ref > 0 && !count_active(t, 1)) break;   // all start active (||F|| > 0)
-            HL(k_load_rhs<T>, 1);
+            HL((k_load_rhs<T, CP>), 1);
             bicgstab<T>(t, cfg.inner_tol, cfg.solver_max_iter);
-            HL(k_accum<T>, 1);
+            HL((k_accum<T, CP>), 1);
             {
                 TimedLaunch tl("defect_fp64");
-                HL(k_defect, t.part->p);
+                HL((k_defect<CP>), t.part->p);
             }
             fin(t, F(A_STORE, 1, 0, 0, S_FN));
             hipLaunchKernelGGL(k_refine_flags, dim3((ni + 63) / 64), dim3(64), 0, S(), t.desc->p, (int)ni, 1,
@@ -713,7 +960,7 @@ struct HeatBatch::Impl {
     }
     template <class T>
     void precond_setup(const Table& t) {
-        if (pc != PC_NONE) HL(k_precond_setup<T>, pc);
+        if (pc != PC_NONE) HL((k_precond_setup<T, CP>), pc);
     }
 #undef HL
 
@@ -761,7 +1008,7 @@ size_t HeatBatch::estimate_bytes(int n, const std::string& precision) {
 
 int HeatBatch::add(int nx, int ny, double k, double cv, const double* u_host, double dx, double dy) {
     EQ_REQUIRE(nx > 0 && ny > 0, "instance sizes must be positive");
-    EQ_REQUIRE((long)nx * ny < (1L << 31), "instance too large");
+    EQ_REQUIRE((long)nx * ny < (1L << 31) - CPT_MAX * BS, "instance too large");
     auto I = std::make_unique<Inst>();
     I->id = p_->next_id++;
     I->nx = nx;
@@ -839,8 +1086,11 @@ std::vector<int> HeatBatch::iterate() {
         Table& t = *tp;
         if (P.mixed) P.linear_solve<float>(t);
         else P.linear_solve<double>(t);
-        hipLaunchKernelGGL(k_newton_update, dim3((unsigned)t.nb), dim3(BS), 0, S(), t.desc->p, t.blk->p);
-        EQ_HIP(hipGetLastError());
+        by_cpt(t.shape.cpt, [&](auto cpt_) {
+            constexpr int CP = decltype(cpt_)::value;
+            hipLaunchKernelGGL((k_newton_update<CP>), dim3((unsigned)t.nb), dim3(BS), 0, S(), t.desc->p, t.blk->p);
+            EQ_HIP(hipGetLastError());
+        });
     }
     std::vector<int> done;
     if (asmb.empty()) return done;
@@ -888,6 +1138,94 @@ void HeatBatch::step(const std::vector<int>& slots, double dt) {
         iterate();
         EQ_REQUIRE(++guard < 10 * (p_->cfg.max_iter + 2), "batch step did not terminate");
     }
+}
+
+LaunchShape HeatBatch::launch_shape() const {
+    LaunchShape r;
+    r.cpt = p_->last_shape.cpt;
+    r.xcd = p_->last_shape.xcd;
+    r.signatures = (int)p_->tuned.size();
+    return r;
+}
+
+double HeatBatch::time_kernel(const std::string& name, int reps, double* event_ms) {
+    Impl& P = *p_;
+    EQ_REQUIRE(reps > 0, "reps must be positive");
+    std::vector<Inst*> all;
+    for (auto& kv : P.insts) all.push_back(kv.second.get());
+    EQ_REQUIRE(!all.empty(), "empty batch");
+    // a table of the whole membership, in its tuned shape, whose launches stamp the
+    // wall clock (block start, wave ends) into ts
+    const Shape sh = P.shape_for(all);
+    const int cpb = sh.cpt * BS;
+    long nb = 0;
+    for (Inst* I : all) nb += (I->n + cpb - 1) / cpb;
+    auto ts = dev_alloc<unsigned long long>(2 * (size_t)nb);
+    TableP tp = P.build_table(all, sh, ts->p);
+    Table& t = *tp;
+    EQ_REQUIRE(t.nb == nb, "timing table mismatch");
+    // every instance active, so the kernels do their full work (the next solve resets the flags)
+    for (Inst* I : all) {
+        const int one[2] = {1, 1};
+        EQ_HIP(hipMemcpyAsync(I->act->p, one, sizeof(one), hipMemcpyHostToDevice, S()));
+    }
+    Device::get().sync();
+    int khz = 0;
+    EQ_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, 0));
+    EQ_REQUIRE(khz > 0, "no wall-clock rate");
+    std::function<void()> launch;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+#define TL(kernel, ...)                                                                                          \
+    launch = [&] {                                                                                               \
+        by_cpt(t.shape.cpt, [&](auto cpt_) {                                                                     \
+            constexpr int CP = decltype(cpt_)::value;                                                            \
+            hipExtLaunchKernelGGL(kernel, dim3((unsigned)t.nb), dim3(BS), 0, S(), e0, e1, 0, t.desc->p, t.blk->p, \
+                                  ##__VA_ARGS__);                                                                \
+        });                                                                                                      \
+    }
+    if (name == "stencil_spmv") {
+        if (P.mixed) TL((k_spmv_dot<float, CP>), (int)V_PH, (int)V_V, 1, (int)V_RH, -1, t.part->p, t.ng);
+        else TL((k_spmv_dot<double, CP>), (int)V_PH, (int)V_V, 1, (int)V_RH, -1, t.part->p, t.ng);
+    } else if (name == "assemble") {
+        if (P.mixed) TL((k_assemble<true, CP>), t.part->p);
+        else TL((k_assemble<false, CP>), t.part->p);
+    } else if (name == "ilu_black") {
+        if (P.mixed) TL((k_ilu_black<float, CP>), (int)V_P, (int)V_PH);
+        else TL((k_ilu_black<double, CP>), (int)V_P, (int)V_PH);
+    } else if (name == "ilu_red") {
+        if (P.mixed) TL((k_ilu_red<float, CP>), (int)V_P, (int)V_PH);
+        else TL((k_ilu_red<double, CP>), (int)V_P, (int)V_PH);
+    } else if (name == "defect") {
+        TL((k_defect<CP>), t.part->p);
+    } else {
+        EQ_REQUIRE(false, "unknown kernel " << name << " (stencil_spmv, assemble, ilu_black, ilu_red, defect)");
+    }
+#undef TL
+    EQ_HIP(hipEventCreate(&e0));
+    EQ_HIP(hipEventCreate(&e1));
+    launch();   // warm
+    EQ_HIP(hipGetLastError());
+    std::vector<unsigned long long> h(2 * (size_t)nb);
+    double total = 0.0, ev_total = 0.0;
+    for (int r = 0; r < reps; ++r) {
+        EQ_HIP(hipMemsetAsync(ts->p, 0, h.size() * sizeof(unsigned long long), S()));
+        launch();
+        EQ_HIP(hipMemcpyAsync(h.data(), ts->p, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, S()));
+        EQ_HIP(hipEventSynchronize(e1));
+        Device::get().sync();
+        float ms = 0.f;
+        EQ_HIP(hipEventElapsedTime(&ms, e0, e1));
+        ev_total += ms;
+        unsigned long long lo = ~0ull, hi = 0;
+        for (long b = 0; b < nb; ++b) lo = std::min(lo, h[2 * b]), hi = std::max(hi, h[2 * b + 1]);
+        EQ_REQUIRE(hi > lo, "kernel clock stamps missing");
+        total += (double)(hi - lo) / (double)khz;   // ms
+    }
+    EQ_HIP(hipGetLastError());
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (event_ms) *event_ms = ev_total / reps;
+    return total / reps;
 }
 
 void HeatBatch::get_state(int slot, double* u) const {

@@ -37,6 +37,14 @@ struct FusedConfig {
     double inner_tol = 1e-4;           // mixed: FP32 inner solve tolerance
     int max_refine = 20;
     bool graph = true;                 // capture BiCGStab iterations in a hipGraph
+    int cpt = 0;                       // cells per thread (1, 2, 4, 8); 0 = chosen by the tuner
+    int xcd = 1;                       // XCD-contiguous block order when cpt is fixed
+    bool tune = true;                  // time the launch shapes per membership signature
+};
+
+struct LaunchShape {
+    int cpt = 0, xcd = 0;
+    int signatures = 0;                // memberships tuned so far
 };
 
 struct StepStats {
@@ -71,7 +79,16 @@ public:
     void export_residual(int slot, const double* u, const double* u0, double dt, std::vector<double>& F,
                          std::vector<int>& rp, std::vector<int>& ci, std::vector<double>& v);
     size_t device_bytes(int slot) const;
+    // Average execution time (ms) of one launch of a kernel over every instance: the
+    // span from the first block's start to the last wave's end on the device wall clock
+    // (wall_clock64, hipDeviceAttributeWallClockRate), over `reps` launches on the
+    // runtime stream; event_ms gets the HIP-event time of the same dispatches, which
+    // also counts the dispatch itself.  Kernels: stencil_spmv | assemble | ilu_black |
+    // ilu_red | defect.
+    double time_kernel(const std::string& name, int reps, double* event_ms = nullptr);
     static size_t estimate_bytes(int n, const std::string& precision);
+    // Launch shape used by the most recent launch and the number of tuned signatures.
+    LaunchShape launch_shape() const;
 
     struct Impl;
 

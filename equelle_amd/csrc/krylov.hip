@@ -16,6 +16,7 @@ namespace eqhip {
 namespace {
 constexpr int BS = 256;
 constexpr int W = 64;   // SELL slice height = wavefront size
+constexpr int GM_MAX = 200;   // largest GMRES restart length
 inline hipStream_t S() { return Device::get().stream(); }
 __device__ inline long gid() { return (long)blockIdx.x * blockDim.x + threadIdx.x; }
 inline int vec_blocks(long n) {
@@ -151,7 +152,9 @@ __global__ void k_sell_vals(long nnzp, const int* __restrict__ src, const double
 template <class T>
 __global__ __launch_bounds__(BS) void k_sell_spmv(int n, int nslices, const int* __restrict__ sp,
                                                   const int* __restrict__ col, const T* __restrict__ val,
-                                                  const T* __restrict__ x, T* __restrict__ y) {
+                                                  const T* __restrict__ x, T* __restrict__ y,
+                                                  const double* __restrict__ gate) {
+    if (gate && gate[0] == 0.0) return;   // solve already finished (device flag)
     const long row = gid();
     const long slice = row / W;
     if (slice >= nslices) return;
@@ -196,14 +199,19 @@ __global__ void k_jacobi_setup(int n, const int* __restrict__ diag, const double
 }
 
 template <class T>
-__global__ void k_scale(long n, const T* __restrict__ d, const T* __restrict__ a, T* __restrict__ out) {
+__global__ void k_scale(long n, const T* __restrict__ d, const T* __restrict__ a, T* __restrict__ out,
+                        const double* __restrict__ gate) {
+    if (gate && gate[0] == 0.0) return;
     long i = gid();
     if (i < n) out[i] = d ? d[i] * a[i] : a[i];
 }
 
 // ---- device scalars -------------------------------------------------------
-enum Slot { RHO = 0, RHO_OLD, ALPHA, OMEGA, BETA, RR, BREAK, RZ, NSLOT = 16 };
-enum Act { ACT_STORE = 0, ACT_BICG_ALPHA, ACT_BICG_OMEGA, ACT_BICG_RHO, ACT_CG_ALPHA, ACT_CG_BETA };
+// ACTIVE (1/0) gates every kernel of an iteration, so the host can queue iterations
+// ahead and read the convergence state only every few iterations; IT counts the
+// iterations that ran while active; BB = ||b||^2.
+enum Slot { RHO = 0, RHO_OLD, ALPHA, OMEGA, BETA, RR, BREAK, RZ, ACTIVE, IT, BB, HN, NSLOT = 16 };
+enum Act { ACT_STORE = 0, ACT_BICG_ALPHA, ACT_BICG_OMEGA, ACT_BICG_RHO, ACT_CG_ALPHA, ACT_CG_BETA, ACT_INIT };
 
 __device__ inline double bred(double v) {
     __shared__ double lds[BS / 64];
@@ -222,7 +230,8 @@ __device__ inline double bred(double v) {
 template <class T>
 __global__ void k_dots(long n, int nd, const T* __restrict__ x0, const T* __restrict__ y0, const T* __restrict__ x1,
                        const T* __restrict__ y1, const T* __restrict__ x2, const T* __restrict__ y2,
-                       double* __restrict__ part) {
+                       double* __restrict__ part, const double* __restrict__ gate) {
+    if (gate && gate[0] == 0.0) return;
     double a0 = 0, a1 = 0, a2 = 0;
     for (long i = gid(); i < n; i += (long)gridDim.x * BS) {
         a0 += (double)x0[i] * (double)y0[i];
@@ -241,8 +250,11 @@ __global__ void k_dots(long n, int nd, const T* __restrict__ x0, const T* __rest
     }
 }
 
+// tol2, maxit: the ACT_BICG_RHO / ACT_CG_BETA / ACT_INIT actions update ACTIVE
+// (continue while ||r||^2 > tol2 ||b||^2, no breakdown, IT < maxit).
 __global__ void k_finalize(int act, int nd, const double* __restrict__ part, int nb, double* __restrict__ sc,
-                           int slot) {
+                           int slot, int gated, double tol2, int maxit) {
+    if (gated && sc[ACTIVE] == 0.0) return;
     double d[3] = {0, 0, 0};
     for (int j = 0; j < nd; ++j) {
         double a = 0;
@@ -268,7 +280,18 @@ __global__ void k_finalize(int act, int nd, const double* __restrict__ part, int
             sc[RHO_OLD] = sc[RHO];
             sc[RHO] = d[0];
             sc[RR] = d[1];
+            sc[IT] += 1.0;
             if (d[0] == 0.0) sc[BREAK] = 3;
+            sc[ACTIVE] = (d[1] > tol2 * sc[BB] && sc[BREAK] == 0.0 && sc[IT] < maxit && isfinite(d[1])) ? 1.0 : 0.0;
+            break;
+        case ACT_INIT:   // d[0] = rh.r (or r.z), d[1] = r.r at the start of a solve
+            sc[RHO] = d[0];
+            sc[RZ] = d[0];
+            sc[RR] = d[1];
+            sc[BB] = d[1];
+            sc[IT] = 0.0;
+            sc[BREAK] = 0.0;
+            sc[ACTIVE] = (d[1] > 0.0 && isfinite(d[1]) && maxit > 0) ? 1.0 : 0.0;
             break;
         case ACT_CG_ALPHA:
             if (d[0] == 0.0) { sc[ALPHA] = 0; sc[BREAK] = 4; }
@@ -278,6 +301,8 @@ __global__ void k_finalize(int act, int nd, const double* __restrict__ part, int
             sc[BETA] = sc[RZ] != 0.0 ? d[0] / sc[RZ] : 0.0;
             sc[RZ] = d[0];
             sc[RR] = d[1];
+            sc[IT] += 1.0;
+            sc[ACTIVE] = (d[1] > tol2 * sc[BB] && sc[BREAK] == 0.0 && sc[IT] < maxit && isfinite(d[1])) ? 1.0 : 0.0;
             break;
     }
 }
@@ -287,6 +312,7 @@ __global__ void k_finalize(int act, int nd, const double* __restrict__ part, int
 template <class T>
 __global__ void k_bicg_p(long n, int first, const T* __restrict__ r, const T* __restrict__ v, T* __restrict__ p,
                          T* __restrict__ ph, const T* __restrict__ dinv, const double* __restrict__ sc) {
+    if (sc[ACTIVE] == 0.0) return;
     long i = gid();
     if (i >= n) return;
     T pi;
@@ -302,6 +328,7 @@ __global__ void k_bicg_p(long n, int first, const T* __restrict__ r, const T* __
 template <class T>
 __global__ void k_bicg_s(long n, const T* __restrict__ r, const T* __restrict__ v, T* __restrict__ s,
                          T* __restrict__ sh, const T* __restrict__ dinv, const double* __restrict__ sc) {
+    if (sc[ACTIVE] == 0.0) return;
     long i = gid();
     if (i >= n) return;
     const T si = r[i] - (T)sc[ALPHA] * v[i];
@@ -313,6 +340,7 @@ template <class T>
 __global__ void k_bicg_x(long n, T* __restrict__ x, const T* __restrict__ ph, const T* __restrict__ sh,
                          T* __restrict__ r, const T* __restrict__ s, const T* __restrict__ t,
                          const T* __restrict__ rh, const double* __restrict__ sc, double* __restrict__ part) {
+    if (sc[ACTIVE] == 0.0) return;
     const T alpha = (T)sc[ALPHA], omega = (T)sc[OMEGA];
     double a0 = 0, a1 = 0;
     for (long i = gid(); i < n; i += (long)gridDim.x * BS) {
@@ -332,6 +360,7 @@ __global__ void k_bicg_x(long n, T* __restrict__ x, const T* __restrict__ ph, co
 template <class T>
 __global__ void k_cg_update(long n, T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
                             const T* __restrict__ q, const double* __restrict__ sc) {
+    if (sc[ACTIVE] == 0.0) return;
     long i = gid();
     if (i >= n) return;
     const T a = (T)sc[ALPHA];
@@ -341,16 +370,34 @@ __global__ void k_cg_update(long n, T* __restrict__ x, T* __restrict__ r, const 
 template <class T>
 __global__ void k_cg_p(long n, int first, const T* __restrict__ z, T* __restrict__ p,
                        const double* __restrict__ sc) {
+    if (sc[ACTIVE] == 0.0) return;
     long i = gid();
     if (i >= n) return;
     p[i] = first ? z[i] : z[i] + (T)sc[BETA] * p[i];
 }
 
 // ---- GMRES kernels ----------------------------------------------------------------
+// The Hessenberg column, Givens rotations and the least-squares solve live in device
+// memory (struct Gm), so an Arnoldi step needs no host round trip; every kernel is
+// gated by gm->active and the host reads the state only at chunk boundaries.
+struct Gm {
+    double active;     // 1 while the cycle continues
+    double rn;         // |g[j+1]|, the residual norm estimate
+    double bn;         // ||b||
+    double tol;
+    double k;          // columns in the current cycle
+    double total;      // Arnoldi steps over all cycles
+    double hn;         // ||w|| after orthogonalisation
+    double pad;
+    double hcol[GM_MAX + 2], h2[GM_MAX + 2];
+    double cs[GM_MAX], sn[GM_MAX], g[GM_MAX + 1], y[GM_MAX];
+    double H[(GM_MAX + 1) * GM_MAX];
+};
 // partial dots of w against columns 0..k of V (V column-major, ld = n)
 template <class T>
 __global__ void k_vdots(long n, int k, const T* __restrict__ V, const T* __restrict__ w, double* __restrict__ part,
-                        int nb) {
+                        int nb, const Gm* __restrict__ gm) {
+    if (gm->active == 0.0) return;
     const int col = blockIdx.y;
     double a = 0;
     const T* v = V + (long)col * n;
@@ -358,29 +405,94 @@ __global__ void k_vdots(long n, int k, const T* __restrict__ V, const T* __restr
     a = bred(a);
     if (threadIdx.x == 0) part[(long)col * nb + blockIdx.x] = a;
 }
-__global__ void k_vdots_final(int nb, const double* __restrict__ part, double* __restrict__ h) {
+// h2[col] = sum of the partials; pass 0 sets hcol, pass 1 adds (CGS2)
+__global__ void k_vdots_final(int nb, const double* __restrict__ part, Gm* __restrict__ gm, int pass) {
+    if (gm->active == 0.0) return;
     const int col = blockIdx.x;
     double a = 0;
     for (int i = threadIdx.x; i < nb; i += BS) a += part[(long)col * nb + i];
     a = bred(a);
-    if (threadIdx.x == 0) h[col] = a;
+    if (threadIdx.x == 0) {
+        gm->h2[col] = a;
+        gm->hcol[col] = pass == 0 ? a : gm->hcol[col] + a;
+    }
 }
-// w -= sum_{j<=k} h[j] V[:,j]
+// w -= sum_{j<=k} h2[j] V[:,j]
 template <class T>
-__global__ void k_vupdate(long n, int k, const T* __restrict__ V, const double* __restrict__ h, T* __restrict__ w) {
+__global__ void k_vupdate(long n, int k, const T* __restrict__ V, const Gm* __restrict__ gm, T* __restrict__ w) {
+    if (gm->active == 0.0) return;
     long i = gid();
     if (i >= n) return;
     double s = (double)w[i];
-    for (int j = 0; j <= k; ++j) s -= h[j] * (double)V[(long)j * n + i];
+    for (int j = 0; j <= k; ++j) s -= gm->h2[j] * (double)V[(long)j * n + i];
     w[i] = (T)s;
 }
-// out = sum_j y[j] V[:,j]
+// hn = ||w|| (from the dot slot), new Givens rotation for column j, residual estimate
+__global__ void k_gmres_step(int j, int m, const double* __restrict__ sc, Gm* __restrict__ gm, int maxit) {
+    if (gm->active == 0.0 || threadIdx.x != 0) return;
+    const double hn = sqrt(fmax(sc[HN], 0.0));
+    gm->hn = hn;
+    double* hc = gm->hcol;
+    hc[j + 1] = hn;
+    for (int i = 0; i < j; ++i) {   // previous rotations
+        const double t0 = gm->cs[i] * hc[i] + gm->sn[i] * hc[i + 1];
+        hc[i + 1] = -gm->sn[i] * hc[i] + gm->cs[i] * hc[i + 1];
+        hc[i] = t0;
+    }
+    const double den = hypot(hc[j], hc[j + 1]);
+    gm->cs[j] = den > 0 ? hc[j] / den : 1.0;
+    gm->sn[j] = den > 0 ? hc[j + 1] / den : 0.0;
+    hc[j] = den;
+    hc[j + 1] = 0;
+    gm->g[j + 1] = -gm->sn[j] * gm->g[j];
+    gm->g[j] = gm->cs[j] * gm->g[j];
+    for (int i = 0; i <= j; ++i) gm->H[i * m + j] = hc[i];
+    gm->rn = fabs(gm->g[j + 1]);
+    gm->k = j + 1;
+    gm->total += 1.0;
+    if (gm->rn <= gm->tol * gm->bn || hn == 0.0 || !isfinite(gm->rn) || gm->total >= maxit) gm->active = 0.0;
+}
+// V[:,j+1] = w / hn
 template <class T>
-__global__ void k_vcombine(long n, int k, const T* __restrict__ V, const double* __restrict__ y, T* __restrict__ out) {
+__global__ void k_gmres_next(long n, const T* __restrict__ w, T* __restrict__ vnext, const Gm* __restrict__ gm) {
+    if (gm->active == 0.0) return;
+    long i = gid();
+    if (i < n) vnext[i] = (T)((double)w[i] / gm->hn);
+}
+// start of a cycle: beta = ||r|| (dot slot RR), V[:,0] = r / beta, g = beta e1
+template <class T>
+__global__ void k_gmres_start(long n, const T* __restrict__ r, T* __restrict__ v0, const double* __restrict__ sc,
+                              Gm* __restrict__ gm, int maxit) {
+    const double beta = sqrt(fmax(sc[RR], 0.0));
+    const bool go = beta > gm->tol * gm->bn && isfinite(beta) && gm->total < maxit;
+    long i = gid();
+    if (go && i < n) v0[i] = (T)((double)r[i] / beta);
+    if (i == 0) {
+        gm->rn = beta;
+        gm->active = go ? 1.0 : 0.0;
+        gm->k = 0;
+        for (int q = 0; q < GM_MAX + 1; ++q) gm->g[q] = 0.0;
+        gm->g[0] = beta;
+    }
+}
+// y = H^{-1} g over the k columns of the finished cycle (one thread)
+__global__ void k_gmres_solve(int m, Gm* __restrict__ gm) {
+    if (threadIdx.x != 0) return;
+    const int k = (int)gm->k;
+    for (int i = k - 1; i >= 0; --i) {
+        double s = gm->g[i];
+        for (int q = i + 1; q < k; ++q) s -= gm->H[i * m + q] * gm->y[q];
+        gm->y[i] = gm->H[i * m + i] != 0 ? s / gm->H[i * m + i] : 0.0;
+    }
+}
+// out = sum_{j<k} y[j] V[:,j]
+template <class T>
+__global__ void k_vcombine(long n, const T* __restrict__ V, const Gm* __restrict__ gm, T* __restrict__ out) {
     long i = gid();
     if (i >= n) return;
+    const int k = (int)gm->k;
     double s = 0;
-    for (int j = 0; j < k; ++j) s += y[j] * (double)V[(long)j * n + i];
+    for (int j = 0; j < k; ++j) s += gm->y[j] * (double)V[(long)j * n + i];
     out[i] = (T)s;
 }
 template <class T>
@@ -392,6 +504,12 @@ template <class T>
 __global__ void k_add(long n, const T* __restrict__ a, T* __restrict__ x) {
     long i = gid();
     if (i < n) x[i] += a[i];
+}
+// r = b - w
+template <class T>
+__global__ void k_sub(long n, const T* __restrict__ b, const T* __restrict__ w, T* __restrict__ r) {
+    long i = gid();
+    if (i < n) r[i] = b[i] - w[i];
 }
 
 // ---- multicolour ILU(0) ----------------------------------------------------------
@@ -429,7 +547,8 @@ __global__ void k_ilu_factor(int cnt, const int* __restrict__ rows, const int* _
 template <class T>
 __global__ void k_ilu_lower(int cnt, const int* __restrict__ rows, const int* __restrict__ rp,
                             const int* __restrict__ ci, const uint8_t* __restrict__ kind, const T* __restrict__ lu,
-                            const T* __restrict__ b, T* __restrict__ z) {
+                            const T* __restrict__ b, T* __restrict__ z, const double* __restrict__ gate) {
+    if (gate && gate[0] == 0.0) return;
     long t = gid();
     if (t >= cnt) return;
     const int i = rows[t];
@@ -441,7 +560,9 @@ __global__ void k_ilu_lower(int cnt, const int* __restrict__ rows, const int* __
 template <class T>
 __global__ void k_ilu_upper(int cnt, const int* __restrict__ rows, const int* __restrict__ rp,
                             const int* __restrict__ ci, const uint8_t* __restrict__ kind,
-                            const int* __restrict__ diag, const T* __restrict__ lu, T* __restrict__ z) {
+                            const int* __restrict__ diag, const T* __restrict__ lu, T* __restrict__ z,
+                            const double* __restrict__ gate) {
+    if (gate && gate[0] == 0.0) return;
     long t = gid();
     if (t >= cnt) return;
     const int i = rows[t];
@@ -492,33 +613,38 @@ struct System {
         sc = dev_alloc<double>(NSLOT);
         part = dev_alloc<double>(3 * (size_t)nb + 8);
     }
-    void spmv(const T* x, T* y) const {
+    void spmv(const T* x, T* y, const double* gate = nullptr) const {
         TimedLaunch tl(sizeof(T) == 8 ? "spmv_fp64" : "spmv_fp32");
-        KL(k_sell_spmv<T>, nblk((long)L->nslices * W), n, L->nslices, L->slice_ptr->p, L->col->p, sell->p, x, y);
+        KL(k_sell_spmv<T>, nblk((long)L->nslices * W), n, L->nslices, L->slice_ptr->p, L->col->p, sell->p, x, y,
+           gate);
     }
-    // z = M^{-1} r  (not used for the fused Jacobi paths)
-    void apply_precond(const T* r, T* z) const {
+    // z = M^{-1} r
+    void apply_precond(const T* r, T* z, const double* gate = nullptr) const {
         if (precond == "ilu0") {
             for (int c = 0; c < L->ncolors; ++c) {
                 const int cnt = L->color_off[c + 1] - L->color_off[c];
                 if (cnt > 0)
                     KL(k_ilu_lower<T>, nblk(cnt), cnt, L->color_rows->p + L->color_off[c], pat->d_rp->p, pat->d_ci->p,
-                       L->kind->p, lu->p, r, z);
+                       L->kind->p, lu->p, r, z, gate);
             }
             for (int c = L->ncolors - 1; c >= 0; --c) {
                 const int cnt = L->color_off[c + 1] - L->color_off[c];
                 if (cnt > 0)
                     KL(k_ilu_upper<T>, nblk(cnt), cnt, L->color_rows->p + L->color_off[c], pat->d_rp->p, pat->d_ci->p,
-                       L->kind->p, L->diag->p, lu->p, z);
+                       L->kind->p, L->diag->p, lu->p, z, gate);
             }
         } else {
-            KL(k_scale<T>, nblk(n), (long)n, precond == "jacobi" ? (const T*)dinv->p : (const T*)nullptr, r, z);
+            KL(k_scale<T>, nblk(n), (long)n, precond == "jacobi" ? (const T*)dinv->p : (const T*)nullptr, r, z, gate);
         }
     }
+    const double* gate() const { return sc->p + ACTIVE; }
+    // dot products + finalisation; gated ones are skipped once the solve finished
     void dots(int act, int nd, const T* x0, const T* y0, const T* x1 = nullptr, const T* y1 = nullptr,
-              int slot = 0) const {
-        KL(k_dots<T>, nb, (long)n, nd, x0, y0, x1, y1, (const T*)nullptr, (const T*)nullptr, part->p);
-        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(BS), 0, S(), act, nd, part->p, nb, sc->p, slot);
+              int slot = 0, bool gated = false, double tol2 = 0.0, int maxit = 0) const {
+        KL(k_dots<T>, nb, (long)n, nd, x0, y0, x1, y1, (const T*)nullptr, (const T*)nullptr, part->p,
+           gated ? gate() : (const double*)nullptr);
+        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(BS), 0, S(), act, nd, part->p, nb, sc->p, slot, gated ? 1 : 0,
+                           tol2, maxit);
         EQ_HIP(hipGetLastError());
     }
     double read_slot(int slot) const {
@@ -535,7 +661,33 @@ struct System {
     }
 };
 
-// BiCGStab, x0 = 0 (x zeroed here).  Relative tolerance on ||r|| / ||b||.
+// Host view of a device flag without stalling the queue: the flag copied after chunk
+// k is read once chunk k+1 has been queued (pinned ring + events).
+struct FlagRing {
+    static constexpr int NR = 4;
+    hipEvent_t ev[NR] = {};
+    double* host = nullptr;
+    FlagRing() {
+        host = Device::get().host_scratch() + 256;
+        for (int i = 0; i < NR; ++i) EQ_HIP(hipEventCreateWithFlags(&ev[i], hipEventDisableTiming));
+    }
+    ~FlagRing() {
+        for (int i = 0; i < NR; ++i)
+            if (ev[i]) (void)hipEventDestroy(ev[i]);
+    }
+    void post(int k, const double* dflag) {
+        EQ_HIP(hipMemcpyAsync(host + k % NR, dflag, sizeof(double), hipMemcpyDeviceToHost, S()));
+        EQ_HIP(hipEventRecord(ev[k % NR], S()));
+    }
+    bool read(int k) {
+        EQ_HIP(hipEventSynchronize(ev[k % NR]));
+        return host[k % NR] != 0.0;
+    }
+};
+constexpr int CHUNK = 4;   // iterations queued between flag reads
+
+// BiCGStab, x0 = 0 (x zeroed here).  Relative tolerance on ||r|| / ||b||.  Iterations
+// are queued in chunks; kernels of iterations past convergence are no-ops (gated).
 template <class T>
 int bicgstab(const System<T>& A, const T* b, T* x, double tol, int maxit, double* relres) {
     const long n = A.n;
@@ -544,162 +696,144 @@ int bicgstab(const System<T>& A, const T* b, T* x, double tol, int maxit, double
     const bool ilu = A.precond == "ilu0";
     auto ph = dev_alloc<T>(n), sh = dev_alloc<T>(n);
     const T* dinv = A.precond == "jacobi" ? A.dinv->p : nullptr;
+    const double* g = A.gate();
     EQ_HIP(hipMemsetAsync(x, 0, n * sizeof(T), S()));
     EQ_HIP(hipMemsetAsync(A.sc->p, 0, NSLOT * sizeof(double), S()));
     EQ_HIP(hipMemcpyAsync(r->p, b, n * sizeof(T), hipMemcpyDeviceToDevice, S()));
     EQ_HIP(hipMemcpyAsync(rh->p, b, n * sizeof(T), hipMemcpyDeviceToDevice, S()));
-    A.dots(ACT_BICG_RHO, 2, rh->p, r->p, r->p, r->p);   // rho = rh.r, rr = r.r
+    A.dots(ACT_INIT, 2, rh->p, r->p, r->p, r->p, 0, false, 0.0, maxit);   // rho = rh.r, rr = bb = r.r
+    const double tol2 = tol * tol;
+    FlagRing ring;
+    int queued = 0;
+    for (int k = 0; queued < maxit; ++k) {
+        for (int q = 0; q < CHUNK && queued < maxit; ++q, ++queued) {
+            KL(k_bicg_p<T>, nblk(n), n, queued == 0 ? 1 : 0, r->p, v->p, p->p, ilu ? (T*)nullptr : ph->p, dinv,
+               A.sc->p);
+            if (ilu) A.apply_precond(p->p, ph->p, g);
+            A.spmv(ph->p, v->p, g);
+            A.dots(ACT_BICG_ALPHA, 1, rh->p, v->p, nullptr, nullptr, 0, true);
+            KL(k_bicg_s<T>, nblk(n), n, r->p, v->p, s->p, ilu ? (T*)nullptr : sh->p, dinv, A.sc->p);
+            if (ilu) A.apply_precond(s->p, sh->p, g);
+            A.spmv(sh->p, t->p, g);
+            A.dots(ACT_BICG_OMEGA, 2, t->p, s->p, t->p, t->p, 0, true);
+            KL(k_bicg_x<T>, A.nb, n, x, ph->p, sh->p, r->p, s->p, t->p, rh->p, A.sc->p, A.part->p);
+            hipLaunchKernelGGL(k_finalize, dim3(1), dim3(BS), 0, S(), (int)ACT_BICG_RHO, 2, A.part->p, A.nb, A.sc->p,
+                               0, 1, tol2, maxit);
+            EQ_HIP(hipGetLastError());
+        }
+        ring.post(k, g);
+        if (k > 0 && !ring.read(k - 1)) break;
+    }
     double h[NSLOT];
     A.read_slots(h, NSLOT);
-    const double bn = std::sqrt(h[RR]);
-    if (relres) *relres = 0.0;
-    if (bn == 0.0) return 0;
-    int it = 0;
-    double rn = bn;
-    while (it < maxit && rn > tol * bn) {
-        KL(k_bicg_p<T>, nblk(n), n, it == 0 ? 1 : 0, r->p, v->p, p->p, ilu ? (T*)nullptr : ph->p, dinv, A.sc->p);
-        if (ilu) A.apply_precond(p->p, ph->p);
-        A.spmv(ph->p, v->p);
-        A.dots(ACT_BICG_ALPHA, 1, rh->p, v->p);
-        KL(k_bicg_s<T>, nblk(n), n, r->p, v->p, s->p, ilu ? (T*)nullptr : sh->p, dinv, A.sc->p);
-        if (ilu) A.apply_precond(s->p, sh->p);
-        A.spmv(sh->p, t->p);
-        A.dots(ACT_BICG_OMEGA, 2, t->p, s->p, t->p, t->p);
-        KL(k_bicg_x<T>, A.nb, n, x, ph->p, sh->p, r->p, s->p, t->p, rh->p, A.sc->p, A.part->p);
-        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(BS), 0, S(), (int)ACT_BICG_RHO, 2, A.part->p, A.nb, A.sc->p, 0);
-        EQ_HIP(hipGetLastError());
-        ++it;
-        A.read_slots(h, NSLOT);
-        rn = std::sqrt(std::max(h[RR], 0.0));
-        if (h[BREAK] != 0.0 || !std::isfinite(rn)) break;
-    }
-    if (relres) *relres = rn / bn;
-    return it;
+    const double bn = std::sqrt(std::max(h[BB], 0.0));
+    if (relres) *relres = bn > 0 ? std::sqrt(std::max(h[RR], 0.0)) / bn : 0.0;
+    return (int)h[IT];
 }
 
 template <class T>
 int cg(const System<T>& A, const T* b, T* x, double tol, int maxit, double* relres) {
     const long n = A.n;
     auto r = dev_alloc<T>(n), z = dev_alloc<T>(n), p = dev_alloc<T>(n), q = dev_alloc<T>(n);
+    const double* g = A.gate();
     EQ_HIP(hipMemsetAsync(x, 0, n * sizeof(T), S()));
     EQ_HIP(hipMemsetAsync(A.sc->p, 0, NSLOT * sizeof(double), S()));
     EQ_HIP(hipMemcpyAsync(r->p, b, n * sizeof(T), hipMemcpyDeviceToDevice, S()));
     A.apply_precond(r->p, z->p);
-    A.dots(ACT_CG_BETA, 2, r->p, z->p, r->p, r->p);   // rz, rr
+    A.dots(ACT_INIT, 2, r->p, z->p, r->p, r->p, 0, false, 0.0, maxit);   // rz, rr = bb
+    const double tol2 = tol * tol;
+    KL(k_cg_p<T>, nblk(n), n, 1, z->p, p->p, A.sc->p);
+    FlagRing ring;
+    int queued = 0;
+    for (int k = 0; queued < maxit; ++k) {
+        for (int c = 0; c < CHUNK && queued < maxit; ++c, ++queued) {
+            A.spmv(p->p, q->p, g);
+            A.dots(ACT_CG_ALPHA, 1, p->p, q->p, nullptr, nullptr, 0, true);
+            KL(k_cg_update<T>, nblk(n), n, x, r->p, p->p, q->p, A.sc->p);
+            A.apply_precond(r->p, z->p, g);
+            A.dots(ACT_CG_BETA, 2, r->p, z->p, r->p, r->p, 0, true, tol2, maxit);
+            KL(k_cg_p<T>, nblk(n), n, 0, z->p, p->p, A.sc->p);
+        }
+        ring.post(k, g);
+        if (k > 0 && !ring.read(k - 1)) break;
+    }
     double h[NSLOT];
     A.read_slots(h, NSLOT);
-    const double bn = std::sqrt(h[RR]);
-    if (relres) *relres = 0.0;
-    if (bn == 0.0) return 0;
-    KL(k_cg_p<T>, nblk(n), n, 1, z->p, p->p, A.sc->p);
-    int it = 0;
-    double rn = bn;
-    while (it < maxit && rn > tol * bn) {
-        A.spmv(p->p, q->p);
-        A.dots(ACT_CG_ALPHA, 1, p->p, q->p);
-        KL(k_cg_update<T>, nblk(n), n, x, r->p, p->p, q->p, A.sc->p);
-        A.apply_precond(r->p, z->p);
-        A.dots(ACT_CG_BETA, 2, r->p, z->p, r->p, r->p);
-        KL(k_cg_p<T>, nblk(n), n, 0, z->p, p->p, A.sc->p);
-        ++it;
-        A.read_slots(h, NSLOT);
-        rn = std::sqrt(std::max(h[RR], 0.0));
-        if (h[BREAK] != 0.0 || !std::isfinite(rn)) break;
-    }
-    if (relres) *relres = rn / bn;
-    return it;
+    const double bn = std::sqrt(std::max(h[BB], 0.0));
+    if (relres) *relres = bn > 0 ? std::sqrt(std::max(h[RR], 0.0)) / bn : 0.0;
+    return (int)h[IT];
 }
 
-// GMRES(m), right preconditioning, CGS2; x0 = 0.
+// GMRES(m), right preconditioning, CGS2; x0 = 0.  The Arnoldi process, Givens
+// rotations and the small least-squares solve run on the device (Gm); the host
+// queues Arnoldi steps in chunks and reads the cycle's state only between chunks.
 template <class T>
 int gmres(const System<T>& A, const T* b, T* x, double tol, int maxit, int m, double* relres) {
     const long n = A.n;
-    m = std::max(1, std::min(m, 200));
+    m = std::max(1, std::min(m, GM_MAX));
     auto V = dev_alloc<T>((size_t)n * (m + 1));
     auto w = dev_alloc<T>(n), z = dev_alloc<T>(n), r = dev_alloc<T>(n);
-    auto dh = dev_alloc<double>(m + 2);
+    auto gmb = dev_alloc<char>(sizeof(Gm));
+    Gm* gm = (Gm*)gmb->p;
     const int nb = A.nb;
     auto part = dev_alloc<double>((size_t)nb * (m + 2));
     EQ_HIP(hipMemsetAsync(x, 0, n * sizeof(T), S()));
-    double* hs = Device::get().host_scratch();
-    auto norm2 = [&](const T* a) {
-        A.dots(ACT_STORE, 1, a, a, nullptr, nullptr, RR);
-        return std::sqrt(std::max(A.read_slot(RR), 0.0));
-    };
-    const double bn = norm2(b);
+    EQ_HIP(hipMemsetAsync(gmb->p, 0, sizeof(Gm), S()));
+    EQ_HIP(hipMemsetAsync(A.sc->p, 0, NSLOT * sizeof(double), S()));
+    A.dots(ACT_STORE, 1, b, b, nullptr, nullptr, RR);
+    const double bn = std::sqrt(std::max(A.read_slot(RR), 0.0));
     if (relres) *relres = 0.0;
-    if (bn == 0.0) return 0;
-    std::vector<double> H((m + 1) * m), cs(m), sn(m), g(m + 1), y(m);
-    int it = 0;
-    double rn = bn;
+    if (bn == 0.0 || maxit <= 0) return 0;
+    const double hdr[4] = {1.0, bn, bn, tol};   // active, rn, bn, tol
+    EQ_HIP(hipMemcpyAsync(gm, hdr, sizeof(hdr), hipMemcpyHostToDevice, S()));
     EQ_HIP(hipMemcpyAsync(r->p, b, n * sizeof(T), hipMemcpyDeviceToDevice, S()));
-    while (it < maxit) {
-        const double beta = it == 0 ? bn : norm2(r->p);
-        rn = beta;
-        if (beta <= tol * bn) break;
-        KL(k_scal<T>, nblk(n), n, 1.0 / beta, r->p, V->p);
-        std::fill(g.begin(), g.end(), 0.0);
-        g[0] = beta;
-        int j = 0;
-        for (; j < m && it < maxit; ++j, ++it) {
-            T* vj = V->p + (size_t)j * n;
-            A.apply_precond(vj, z->p);
-            A.spmv(z->p, w->p);
-            std::vector<double> hc(j + 2, 0.0);
-            for (int pass = 0; pass < 2; ++pass) {   // CGS2
-                hipLaunchKernelGGL(k_vdots<T>, dim3(nb, j + 1), dim3(BS), 0, S(), n, j, (const T*)V->p,
-                                   (const T*)w->p, part->p, nb);
+    const double* act = &gm->active;
+    FlagRing ring;
+    int posted = 0;
+    Gm hg;
+    for (;;) {
+        // cycle start: beta = ||r||, V0 = r / beta (decides whether to continue)
+        A.dots(ACT_STORE, 1, r->p, r->p, nullptr, nullptr, RR);
+        KL(k_gmres_start<T>, nblk(n), n, r->p, V->p, A.sc->p, gm, maxit);
+        for (int j = 0; j < m;) {
+            for (int c = 0; c < CHUNK && j < m; ++c, ++j) {
+                T* vj = V->p + (size_t)j * n;
+                A.apply_precond(vj, z->p, act);
+                A.spmv(z->p, w->p, act);
+                for (int pass = 0; pass < 2; ++pass) {   // CGS2
+                    hipLaunchKernelGGL(k_vdots<T>, dim3(nb, j + 1), dim3(BS), 0, S(), n, j, (const T*)V->p,
+                                       (const T*)w->p, part->p, nb, (const Gm*)gm);
+                    EQ_HIP(hipGetLastError());
+                    hipLaunchKernelGGL(k_vdots_final, dim3(j + 1), dim3(BS), 0, S(), nb, part->p, gm, pass);
+                    EQ_HIP(hipGetLastError());
+                    KL(k_vupdate<T>, nblk(n), n, j, V->p, (const Gm*)gm, w->p);
+                }
+                A.dots(ACT_STORE, 1, w->p, w->p, nullptr, nullptr, HN);
+                hipLaunchKernelGGL(k_gmres_step, dim3(1), dim3(64), 0, S(), j, m, (const double*)A.sc->p, gm, maxit);
                 EQ_HIP(hipGetLastError());
-                hipLaunchKernelGGL(k_vdots_final, dim3(j + 1), dim3(BS), 0, S(), nb, part->p, dh->p);
-                EQ_HIP(hipGetLastError());
-                KL(k_vupdate<T>, nblk(n), n, j, V->p, dh->p, w->p);
-                EQ_HIP(hipMemcpyAsync(hs, dh->p, (j + 1) * sizeof(double), hipMemcpyDeviceToHost, S()));
-                Device::get().sync();
-                for (int i = 0; i <= j; ++i) hc[i] += hs[i];
+                KL(k_gmres_next<T>, nblk(n), n, w->p, V->p + (size_t)(j + 1) * n, (const Gm*)gm);
             }
-            const double hn = norm2(w->p);
-            hc[j + 1] = hn;
-            if (hn > 0) KL(k_scal<T>, nblk(n), n, 1.0 / hn, w->p, V->p + (size_t)(j + 1) * n);
-            for (int i = 0; i < j; ++i) {   // previous Givens rotations
-                const double t0 = cs[i] * hc[i] + sn[i] * hc[i + 1];
-                hc[i + 1] = -sn[i] * hc[i] + cs[i] * hc[i + 1];
-                hc[i] = t0;
-            }
-            const double den = std::hypot(hc[j], hc[j + 1]);
-            cs[j] = den > 0 ? hc[j] / den : 1.0;
-            sn[j] = den > 0 ? hc[j + 1] / den : 0.0;
-            hc[j] = den;
-            hc[j + 1] = 0;
-            g[j + 1] = -sn[j] * g[j];
-            g[j] = cs[j] * g[j];
-            for (int i = 0; i <= j; ++i) H[i * m + j] = hc[i];
-            rn = std::fabs(g[j + 1]);
-            if (rn <= tol * bn || hn == 0.0 || !std::isfinite(rn)) {
-                ++j;
-                ++it;
-                break;
-            }
+            ring.post(posted, act);
+            const bool more = posted == 0 || ring.read(posted - 1);
+            ++posted;
+            if (!more) break;
         }
-        // y = H^{-1} g, x += M^{-1} V y
-        for (int i = j - 1; i >= 0; --i) {
-            double s = g[i];
-            for (int k = i + 1; k < j; ++k) s -= H[i * m + k] * y[k];
-            y[i] = H[i * m + i] != 0 ? s / H[i * m + i] : 0.0;
-        }
-        EQ_HIP(hipMemcpyAsync(dh->p, y.data(), j * sizeof(double), hipMemcpyHostToDevice, S()));
-        KL(k_vcombine<T>, nblk(n), n, j, V->p, dh->p, w->p);
+        // x += M^{-1} V y over the cycle's columns
+        hipLaunchKernelGGL(k_gmres_solve, dim3(1), dim3(64), 0, S(), m, gm);
+        EQ_HIP(hipGetLastError());
+        KL(k_vcombine<T>, nblk(n), n, (const T*)V->p, (const Gm*)gm, w->p);
         A.apply_precond(w->p, z->p);
         KL(k_add<T>, nblk(n), n, z->p, x);
-        Device::get().sync();   // y is read from host memory by the copy above
-        if (rn <= tol * bn || !std::isfinite(rn)) break;
+        EQ_HIP(hipMemcpyAsync(&hg, gm, offsetof(Gm, hcol), hipMemcpyDeviceToHost, S()));
+        Device::get().sync();
+        if (hg.rn <= tol * bn || !std::isfinite(hg.rn) || hg.total >= maxit || hg.k == 0) break;
         // true residual for the restart: r = b - A x
         A.spmv(x, w->p);
-        KL(k_scal<T>, nblk(n), n, 1.0, b, r->p);
-        KL(k_scal<T>, nblk(n), n, -1.0, w->p, z->p);
-        KL(k_add<T>, nblk(n), n, z->p, r->p);
+        KL(k_sub<T>, nblk(n), n, b, w->p, r->p);
+        posted = 0;
     }
-    if (relres) *relres = rn / bn;
-    return it;
+    if (relres) *relres = hg.rn / bn;
+    return (int)hg.total;
 }
 
 template <class T>

@@ -142,13 +142,32 @@ int eqhip_heat_step(void* sim, double dt, int* stats, double* norms);
 int eqhip_heat_state(void* sim, double* u);
 int eqhip_heat_destroy(void* sim);
 
+/* ------------------------------------------------------------------ two-phase sessions */
+/* The two-phase program of examples/dsl/twophase.equelle (examples/app/out_twophase.cpp)
+ * on the device AD engine: per step a pressure newtonSolve then a saturation newtonSolve
+ * (twophase.equelle:99-113).  params: grid (grid_dim=3 nx ny nz dx dy dz), watervisc,
+ * oilvisc, min_poro, corey (1|2), and the solver/Newton parameters of runtime_create.
+ * perm, poro, sw_initial: one value per cell; sources: ascending cells with rates. */
+int eqhip_twophase_create(const char* params, const double* perm, const double* poro, const double* sw_initial,
+                          int n, const int* src_cells, const double* src_vals, int nsrc, void** sim);
+/* stats = {pressure Newton, Krylov, stop; transport Newton, Krylov, stop};
+ * norms = {pressure ||F0||, ||Fn||, transport ||F0||, ||Fn||} */
+int eqhip_twophase_step(void* sim, double dt, int* stats, double* norms);
+int eqhip_twophase_state(void* sim, double* p, double* sw);
+/* residual + Jacobian (CSR): which = 0 pressure residual at p (total mobility of sw0),
+ * which = 1 transport residual at sw (flux from p, sw0; step dt); info = {n, nnz};
+ * pass null F to query sizes. */
+int eqhip_twophase_residual(void* sim, int which, const double* p, const double* sw0, const double* sw, double dt,
+                            int* info, double* F, int* row_ptr, int* col_ind, double* jac);
+int eqhip_twophase_destroy(void* sim);
+
 /* ------------------------------------------------------------------ fused heat batches */
 /* Many independent nonlinear-heat instances (the program above, k_model=quadratic,
  * zero-flux boundary, unit or given cells) advanced together by fused kernels with
  * per-instance Newton and Krylov convergence.  No reference counterpart: the reference
  * runs one newtonSolve at a time (EquelleRuntimeCUDA_impl.hpp:216-300).
  * params: precision (fp64|mixed) preconditioner (diagonal|ilu0|none) rel_res_tol abs_res_tol
- * max_iter solver_tol solver_max_iter inner_tol max_refine. */
+ * max_iter solver_tol solver_max_iter inner_tol max_refine graph (0|1) cpt xcd tune. */
 int eqhip_batch_create(const char* params, void** batch);
 int eqhip_batch_destroy(void* batch);
 int eqhip_batch_add(void* batch, int nx, int ny, double k, double cv, const double* u, int* slot);
@@ -166,6 +185,15 @@ int eqhip_batch_state(void* batch, int slot, double* u);
 int eqhip_batch_set_state(void* batch, int slot, const double* u);
 /* device-memory footprint of the slot in bytes (admission control) */
 int eqhip_batch_bytes(void* batch, int slot, double* bytes);
+/* launch shape of the most recent launch: info = {cells per thread, XCD-contiguous block
+ * order, membership signatures tuned so far}.  Shapes are chosen per membership by
+ * timing (params cpt=1|2|4|8 and xcd=0|1 fix it; tune=0 uses cpt=4, xcd=1). */
+int eqhip_batch_shape(void* batch, int* info);
+/* average execution time in ms of one launch over all slots: first block start to last
+ * wave end on the device wall clock, over `reps` launches; event_ms (optional) gets the
+ * HIP-event time of the same dispatches (hipExtLaunchKernel), which includes the dispatch.
+ * kernel = "stencil_spmv" | "assemble" | "ilu_black" | "ilu_red" | "defect" */
+int eqhip_batch_kernel_time(void* batch, const char* kernel, int reps, double* avg_ms, double* event_ms);
 /* F and the Jacobian (CSR, ascending columns) at (u, u0, dt); info = {n, nnz};
  * caller provides n, n+1, 5n, 5n entries. */
 int eqhip_batch_residual(void* batch, int slot, const double* u, const double* u0, double dt, int* info,

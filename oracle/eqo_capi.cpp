@@ -502,7 +502,7 @@ int eqo_twophase_run(const char* params, const double* perm, const double* poro,
 // One residual + Jacobian of the two-phase program: which = 0 the pressure residual
 // at pressure `p` (total mobility from sw0), which = 1 the transport residual at
 // saturation `sw` (total flux from p and sw0, step dt).  info = {n, nnz}.
-int eqo_twophase_residual(const char* params, const double* perm, const double* poro, int n, const int* src_cells,
+int eqo_twophase_residual_params(const char* params, const double* perm, const double* poro, int n, const int* src_cells,
                           const double* src_vals, int nsrc, int which, const double* pr, const double* sw0,
                           const double* sw, double dt, int* info, double* F, int* rp, int* ci, double* jv) {
     return guard([&] {

@@ -1,0 +1,124 @@
+// This program was created by the Equelle compiler from SINTEF.
+
+#include <opm/common/utility/parameters/ParameterGroup.hpp>
+#include <opm/common/ErrorMacros.hpp>
+#include <opm/grid/UnstructuredGrid.h>
+#include <opm/grid/GridManager.hpp>
+#include <algorithm>
+#include <iterator>
+#include <iostream>
+#include <cmath>
+#include <array>
+
+#include "EquelleRuntimeCUDA.hpp"
+
+void ensureRequirements(const EquelleRuntimeCUDA& er);
+void equelleGeneratedCode(equelleCUDA::EquelleRuntimeCUDA& er);
+
+#ifndef EQUELLE_NO_MAIN
+int main(int argc, char** argv)
+{
+    // Get user parameters.
+    Opm::ParameterGroup param(argc, argv, false);
+
+    // Create the Equelle runtime.
+    equelleCUDA::EquelleRuntimeCUDA er(param);
+    equelleGeneratedCode(er);
+    return 0;
+}
+#endif // EQUELLE_NO_MAIN
+
+void equelleGeneratedCode(equelleCUDA::EquelleRuntimeCUDA& er) {
+    using namespace equelleCUDA;
+    ensureRequirements(er);
+
+    // ============= Generated code starts here ================
+
+    const CollOfScalar perm = er.inputCollectionOfScalar("perm", er.allCells());
+    const CollOfScalar poro_in = er.inputCollectionOfScalar("poro", er.allCells());
+    const Scalar watervisc = er.inputScalarWithDefault("watervisc", double(0.001));
+    const Scalar oilvisc = er.inputScalarWithDefault("oilvisc", double(0.005));
+    const CollOfScalar min_poro = er.operatorExtend(er.inputScalarWithDefault("min_poro", double(0.0001)), er.allCells());
+    const CollOfScalar poro = er.trinaryIf((poro_in < min_poro), min_poro, poro_in);
+    const CollOfScalar pv = (poro * er.norm(er.allCells()));
+    auto computeTransmissibilities = [&](const CollOfScalar& permeability) -> CollOfScalar {
+        const CollOfFace interior_faces = er.interiorFaces();
+        const CollOfCell first = er.firstCell(interior_faces);
+        const CollOfCell second = er.secondCell(interior_faces);
+        const CollOfVector cdiff1 = (er.centroid(first) - er.centroid(interior_faces));
+        const CollOfVector cdiff2 = (er.centroid(second) - er.centroid(interior_faces));
+        const CollOfScalar p1 = er.operatorOn(permeability, er.allCells(), first);
+        const CollOfScalar p2 = er.operatorOn(permeability, er.allCells(), second);
+        const CollOfScalar a = er.norm(interior_faces);
+        const CollOfScalar halftrans1 = ((-a * p1) * (er.dot(er.normal(interior_faces), cdiff1) / er.dot(cdiff1, cdiff1)));
+        const CollOfScalar halftrans2 = ((a * p2) * (er.dot(er.normal(interior_faces), cdiff2) / er.dot(cdiff2, cdiff2)));
+        return (double(1) / ((double(1) / halftrans1) + (double(1) / halftrans2)));
+    };
+    const CollOfScalar trans = computeTransmissibilities(perm);
+    auto upwind_i1_ = [&](const CollOfScalar& flux, const CollOfScalar& x) -> CollOfScalar {
+        const CollOfScalar x1 = er.operatorOn(x, er.allCells(), er.firstCell(er.interiorFaces()));
+        const CollOfScalar x2 = er.operatorOn(x, er.allCells(), er.secondCell(er.interiorFaces()));
+        const CollOfScalar zero = (double(0) * flux);
+        return er.trinaryIf((flux >= zero), x1, x2);
+    };
+    auto computeTotalFlux = [&](const CollOfScalar& pressure, const CollOfScalar& total_mobility) -> CollOfScalar {
+        const CollOfScalar ngradp = -er.gradient(pressure);
+        const CollOfScalar face_total_mobility = upwind_i1_(ngradp, total_mobility);
+        return ((trans * face_total_mobility) * ngradp);
+    };
+    auto computePressureResidual = [&](const CollOfScalar& pressure, const CollOfScalar& total_mobility, const CollOfScalar& source) -> CollOfScalar {
+        const CollOfScalar flux = computeTotalFlux(pressure, total_mobility);
+        return (er.divergence(flux) - source);
+    };
+    auto computeWaterMob = [&](const CollOfScalar& sw) -> CollOfScalar {
+        return ((sw * sw) / watervisc);
+    };
+    auto computeOilMob = [&](const CollOfScalar& sw) -> CollOfScalar {
+        const CollOfScalar so = (er.operatorExtend(double(1), er.allCells()) - sw);
+        return ((so * so) / oilvisc);
+    };
+    auto computeTransportResidual = [&](const CollOfScalar& sw, const CollOfScalar& sw0, const CollOfScalar& flux, const CollOfScalar& source, const CollOfScalar& insource_sw, const Scalar& dt) -> CollOfScalar {
+        const Scalar zero = double(0);
+        const CollOfScalar insource = er.trinaryIf((source > zero), source, er.operatorExtend(zero, er.allCells()));
+        const CollOfScalar outsource = er.trinaryIf((source < zero), source, er.operatorExtend(zero, er.allCells()));
+        const CollOfScalar mw = computeWaterMob(sw);
+        const CollOfScalar mo = computeOilMob(sw);
+        const CollOfScalar fracflow = (mw / (mw + mo));
+        const CollOfScalar face_fracflow = upwind_i1_(flux, fracflow);
+        const CollOfScalar water_flux = (face_fracflow * flux);
+        const CollOfScalar q = ((insource * insource_sw) + (outsource * fracflow));
+        return ((sw - sw0) + ((dt / pv) * (er.divergence(water_flux) - q)));
+    };
+    const SeqOfScalar timesteps = er.inputSequenceOfScalar("timesteps");
+    const CollOfScalar sw_initial = er.inputCollectionOfScalar("sw_initial", er.allCells());
+    const CollOfCell source_cells = er.inputDomainSubsetOf("source_cells", er.allCells());
+    const CollOfScalar source_values = er.inputCollectionOfScalar("source_values", source_cells);
+    const CollOfScalar source = er.operatorExtend(source_values, source_cells, er.allCells());
+    const CollOfScalar insource_sw = er.operatorExtend(double(1), er.allCells());
+    auto sw0 = sw_initial;
+    auto p0 = er.operatorExtend(double(0), er.allCells());
+    for (const Scalar& dt : timesteps) {
+        const CollOfScalar total_mobility = (computeWaterMob(sw0) + computeOilMob(sw0));
+        auto pressureResLocal = [&](const CollOfScalar& pressure) -> CollOfScalar {
+            return computePressureResidual(pressure, total_mobility, source);
+        };
+        const CollOfScalar p = er.newtonSolve(pressureResLocal, p0);
+        const CollOfScalar flux = computeTotalFlux(p, total_mobility);
+        auto transportResLocal = [&](const CollOfScalar& sw) -> CollOfScalar {
+            return computeTransportResidual(sw, sw0, flux, source, insource_sw, dt);
+        };
+        const CollOfScalar sw = er.newtonSolve(transportResLocal, er.operatorExtend(double(0.5), er.allCells()));
+        p0 = p;
+        sw0 = sw;
+        er.output("pressure", p0);
+        er.output("saturation", sw0);
+    }
+
+    // ============= Generated code ends here ================
+
+}
+
+void ensureRequirements(const equelleCUDA::EquelleRuntimeCUDA& er)
+{
+    (void)er;
+}

@@ -207,3 +207,43 @@ def test_graph_replays_match_eager(hip_api):
             hip_api.batch_destroy(b)
     for a, g in zip(res[0], res[1]):
         assert np.array_equal(a, g)
+
+
+def test_launch_shapes_give_identical_results(hip_api):
+    """Every launch shape (cells per thread 1/2/4/8, XCD block order on/off) gives
+    bit-identical steps: reductions use fixed groups of cells, not blocks."""
+    sizes = [(70, 45, 0.2), (33, 130, 0.4), (256, 200, 0.1)]
+    ref = None
+    for cpt in (1, 2, 4, 8):
+        for xcd in (0, 1):
+            b = make_batch(hip_api, f"precision=mixed preconditioner=ilu0 rel_res_tol=1e-8 solver_tol=1e-6 "
+                                    f"cpt={cpt} xcd={xcd}")
+            try:
+                slots = [add(hip_api, b, nx, ny, k, gaussian_bumps(nx, ny, 30 + q, width_scale=0.05))
+                         for q, (nx, ny, k) in enumerate(sizes)]
+                hip_api.batch_step(b, iptr(np.array(slots, dtype=np.int32)), len(slots), 1.0)
+                info = np.zeros(3, dtype=np.int32)
+                hip_api.batch_shape(b, iptr(info))
+                assert info[0] == cpt and info[1] == xcd
+                out = [state(hip_api, b, s, nx * ny) for s, (nx, ny, _) in zip(slots, sizes)]
+            finally:
+                hip_api.batch_destroy(b)
+            if ref is None:
+                ref = out
+            else:
+                for a, c in zip(ref, out):
+                    assert np.array_equal(a, c), (cpt, xcd)
+
+
+def test_tuner_picks_a_shape(hip_api):
+    b = make_batch(hip_api, "precision=mixed preconditioner=ilu0 rel_res_tol=1e-8 solver_tol=1e-6")
+    try:
+        s = add(hip_api, b, 300, 200, 0.2, gaussian_bumps(300, 200, 5, width_scale=0.1))
+        hip_api.batch_step(b, iptr(np.array([s], dtype=np.int32)), 1, 1.0)
+        info = np.zeros(3, dtype=np.int32)
+        hip_api.batch_shape(b, iptr(info))
+        assert info[0] in (1, 2, 4, 8) and info[1] in (0, 1) and info[2] >= 1
+        st, nrm = stats(hip_api, b, s)
+        assert st[3] == 2 and nrm[1] <= 1e-8 * nrm[0]
+    finally:
+        hip_api.batch_destroy(b)

@@ -1,0 +1,36 @@
+"""Shared helpers for the two-phase program (oracle and HIP runtime)."""
+import ctypes as C
+
+import numpy as np
+
+from equelle_amd.capi import dptr, iptr
+from equelle_amd.workloads import twophase_inputs, twophase_params
+
+
+def oracle_residual(api, params, perm, poro, sc, sv, which, p, sw0, sw, dt):
+    """(F, rp, ci, v) of the oracle's pressure (which=0) or transport (which=1) residual."""
+    n = perm.size
+    info = np.zeros(2, dtype=np.int32)
+    args = (params, dptr(perm), dptr(poro), n, iptr(sc), dptr(sv), len(sc), which, dptr(p), dptr(sw0), dptr(sw), dt)
+    api.twophase_residual_params(*args, iptr(info), None, None, None, None)
+    F, rp = np.zeros(n), np.zeros(n + 1, dtype=np.int32)
+    ci, v = np.zeros(info[1], dtype=np.int32), np.zeros(info[1])
+    api.twophase_residual_params(*args, iptr(info), dptr(F), iptr(rp), iptr(ci), dptr(v))
+    return F, rp, ci, v
+
+
+def oracle_run(api, params, perm, poro, sw0, sc, sv, dts):
+    n = perm.size
+    p, sw = np.zeros(n), np.zeros(n)
+    it = np.zeros(2 * len(dts), dtype=np.int32)
+    api.twophase_run(params, dptr(perm), dptr(poro), dptr(sw0), n, iptr(sc), dptr(sv), len(sc),
+                     dptr(np.asarray(dts, dtype=float)), len(dts), dptr(p), dptr(sw), iptr(it))
+    return p, sw, it
+
+
+def case(nx, ny, nz, seed=42):
+    perm, poro, sw0, sc, sv = twophase_inputs(nx, ny, nz, seed=seed)
+    return perm, poro, sw0, sc, sv
+
+
+__all__ = ["oracle_residual", "oracle_run", "case", "twophase_params", "C"]
