@@ -57,9 +57,9 @@ def timing_avg(lib, name):
     return cnt.value, (tot.value / cnt.value if cnt.value else float("nan"))
 
 
-def run_fused(lib, precision, u0, nx, ny, dt, steps, warmup, precond="ilu0", graph=True):
+def run_fused(lib, precision, u0, nx, ny, dt, steps, warmup, precond="ilu0", graph=True, inner_tol=1e-4):
     params = (f"precision={precision} preconditioner={precond} rel_res_tol={NONLINEAR_HEAT['rel_res_tol']} "
-              f"abs_res_tol=0 max_iter=20 solver_tol=1e-6 inner_tol=1e-4 graph={int(graph)}").encode()
+              f"abs_res_tol=0 max_iter=20 solver_tol=1e-6 inner_tol={inner_tol} graph={int(graph)}").encode()
     b = C.c_void_p()
     lib.batch_create(params, C.byref(b))
     slot = C.c_int()
@@ -229,57 +229,104 @@ def bench_batched(lib, args):
 
 
 def bench_twophase(lib, args):
-    """3D two-phase transport (twophase.equelle, Corey 2, viscosity ratio 5) on the device
-    AD engine: per step a pressure newtonSolve and a saturation newtonSolve, each to 1e-8
-    relative residual.  Work = cells x (pressure + transport Newton iterations)."""
+    """3D two-phase transport (twophase.equelle, Corey 2, viscosity ratio 5): per step a
+    pressure Newton solve and a saturation Newton solve, each to 1e-8 relative residual.
+    Work = cells x (pressure + transport Newton iterations).
+
+    headline      fused stencil engine, mixed precision (FP32 BiCGStab + red-black ILU(0),
+                  FP64 residual / Jacobian / refinement)
+    fused_fp64    the same engine, FP64 BiCGStab + ILU(0)
+    generic_*     the drop-in path: the printer-style program on the device AD engine with
+                  GMRES(30) + Jacobi (mixed and FP64)"""
     from equelle_amd.workloads import TWOPHASE, twophase_inputs, twophase_params
     nx, ny, nz = (int(v) for v in args.grid3.split("x"))
     n = nx * ny * nz
     perm, poro, sw0, sc, sv = twophase_inputs(nx, ny, nz, rate=TWOPHASE["rate"], porosity=TWOPHASE["porosity"])
     dt = TWOPHASE["dt"]
     runs = {}
-    for name, extra in (("mixed", f"precision=mixed solver={args.tp_solver} preconditioner=diagonal"),
-                        ("fp64", f"precision=fp64 solver={args.tp_solver} preconditioner=diagonal")):
-        params = twophase_params(nx, ny, nz, rel_res_tol=TWOPHASE["rel_res_tol"], extra=extra).encode()
-        sim = C.c_void_p()
-        lib.twophase_create(params, dptr(perm), dptr(poro), dptr(sw0), n, iptr(sc), dptr(sv), len(sc), C.byref(sim))
+    inst = f"watervisc={TWOPHASE['watervisc']} oilvisc={TWOPHASE['oilvisc']} corey={TWOPHASE['corey']}".encode()
+    for name, prec in (("fused_mixed", "mixed"), ("fused_fp64", "fp64")):
+        b = C.c_void_p()
+        lib.batch_create(f"precision={prec} preconditioner={args.precond} rel_res_tol={TWOPHASE['rel_res_tol']} "
+                         f"abs_res_tol=0 max_iter=20 solver_tol=1e-6 solver_max_iter=5000".encode(), C.byref(b))
         try:
-            st = np.zeros(6, dtype=np.int32)
+            slot = C.c_int()
+            lib.batch_add_twophase(b, inst, nx, ny, nz, dptr(perm), dptr(poro), dptr(sw0), len(sc), iptr(sc),
+                                   dptr(sv), C.byref(slot))
+            sl = np.array([slot.value], dtype=np.int32)
+            st, tps = np.zeros(5, dtype=np.int32), np.zeros(3, dtype=np.int32)
             for _ in range(args.warmup):
-                lib.twophase_step(sim, dt, iptr(st), None)
+                lib.batch_step(b, iptr(sl), 1, dt)
             lib.device_sync()
-            newton, krylov = 0, [0, 0]
+            newton = krylov = kp = 0
             t0 = time.perf_counter()
-            for s in range(args.steps):
-                lib.twophase_step(sim, dt, iptr(st), None)
-                if st[2] == 0 or st[5] == 0:
-                    raise RuntimeError(f"{name}: step {s} did not converge: {st.tolist()}")
-                newton += int(st[0]) + int(st[3])
-                krylov[0] += int(st[1])
-                krylov[1] += int(st[4])
-                log(f"[bench] two-phase {name} step {s}: Newton p/s {st[0]}/{st[3]}, Krylov p/s {st[1]}/{st[4]}")
+            for s_ in range(args.steps):
+                lib.batch_step(b, iptr(sl), 1, dt)
+                lib.batch_stats(b, slot.value, iptr(st), None)
+                lib.batch_twophase_stats(b, slot.value, iptr(tps))
+                if st[3] == 0 or st[4] or tps[2] == 0:
+                    raise RuntimeError(f"{name}: step {s_} did not converge: {st.tolist()} {tps.tolist()}")
+                newton += int(st[0])
+                krylov += int(st[1])
+                kp += int(tps[1])
+                log(f"[bench] two-phase {name} step {s_}: Newton p/s {tps[0]}/{st[0] - tps[0]}, "
+                    f"Krylov p/s {tps[1]}/{st[1] - tps[1]}")
             lib.device_sync()
             secs = time.perf_counter() - t0
             p, sw = np.zeros(n), np.zeros(n)
-            lib.twophase_state(sim, dptr(p), dptr(sw))
+            lib.batch_twophase_state(b, slot.value, dptr(p), dptr(sw))
         finally:
-            lib.twophase_destroy(sim)
+            lib.batch_destroy(b)
         runs[name] = dict(value=n * newton / secs, ms_per_step=1e3 * secs / args.steps, newton=newton,
-                          krylov_pressure=krylov[0], krylov_transport=krylov[1], sw=sw)
-    err = float(np.linalg.norm(runs["mixed"]["sw"] - runs["fp64"]["sw"]) / np.linalg.norm(runs["fp64"]["sw"]))
+                          krylov_pressure=kp, krylov_transport=krylov - kp, sw=sw)
+    if not args.quick:
+        for name, extra in (("generic_mixed", f"precision=mixed solver={args.tp_solver} preconditioner=diagonal"),
+                            ("generic_fp64", f"precision=fp64 solver={args.tp_solver} preconditioner=diagonal")):
+            params = twophase_params(nx, ny, nz, rel_res_tol=TWOPHASE["rel_res_tol"], extra=extra).encode()
+            sim = C.c_void_p()
+            lib.twophase_create(params, dptr(perm), dptr(poro), dptr(sw0), n, iptr(sc), dptr(sv), len(sc),
+                                C.byref(sim))
+            try:
+                st = np.zeros(6, dtype=np.int32)
+                for _ in range(args.warmup):
+                    lib.twophase_step(sim, dt, iptr(st), None)
+                lib.device_sync()
+                newton, krylov = 0, [0, 0]
+                gsteps = max(1, min(args.steps, 2))
+                t0 = time.perf_counter()
+                for s_ in range(gsteps):
+                    lib.twophase_step(sim, dt, iptr(st), None)
+                    if st[2] == 0 or st[5] == 0:
+                        raise RuntimeError(f"{name}: step {s_} did not converge: {st.tolist()}")
+                    newton += int(st[0]) + int(st[3])
+                    krylov[0] += int(st[1])
+                    krylov[1] += int(st[4])
+                    log(f"[bench] two-phase {name} step {s_}: Newton p/s {st[0]}/{st[3]}, Krylov p/s {st[1]}/{st[4]}")
+                lib.device_sync()
+                secs = time.perf_counter() - t0
+                p, sw = np.zeros(n), np.zeros(n)
+                lib.twophase_state(sim, dptr(p), dptr(sw))
+            finally:
+                lib.twophase_destroy(sim)
+            runs[name] = dict(value=n * newton / secs, ms_per_step=1e3 * secs / gsteps, newton=newton, steps=gsteps,
+                              krylov_pressure=krylov[0], krylov_transport=krylov[1], sw=sw)
+    ref = runs["fused_fp64"]["sw"]
+    errs = {k: float(np.linalg.norm(r["sw"] - ref) / np.linalg.norm(ref)) for k, r in runs.items()
+            if k.startswith("fused")}
     for r in runs.values():
         r.pop("sw")
-    head = runs["mixed"]
+    head = runs["fused_mixed"]
     out = {"metric": "cell-Newton-iters/s", "value": head["value"], "unit": "cell-Newton-iters/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": head["ms_per_step"],
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "f64 residual/Jacobian/refinement, f32 Krylov", "data": "synthetic",
            "config": {"workload": f"3D two-phase transport {nx}x{ny}x{nz}, log-normal permeability, Corey 2, "
-                                  f"viscosity ratio 5, dt={dt}, Newton to 1e-8 relative residual per solve; "
-                                  f"{args.tp_solver} + Jacobi, device AD engine",
-                      "cells": n},
-           "runs": runs, "mixed_over_fp64": head["value"] / runs["fp64"]["value"],
-           "mixed_vs_fp64_sw_rel_error": err}
+                                  f"viscosity ratio 5, dt={dt}, Newton to 1e-8 relative residual per solve",
+                      "cells": n, "precond": args.precond},
+           "runs": runs, "mixed_over_fused_fp64": head["value"] / runs["fused_fp64"]["value"],
+           "sw_rel_error_vs_fused_fp64": errs}
+    if "generic_fp64" in runs:
+        out["fused_mixed_over_generic_fp64"] = head["value"] / runs["generic_fp64"]["value"]
     print(json.dumps(out), flush=True)
 
 
@@ -324,11 +371,14 @@ def main():
     ap.add_argument("--precond", default="ilu0", help="preconditioner of the fused runs: ilu0 | diagonal | none")
     ap.add_argument("--workload", default="heat2048", choices=["heat2048", "batched", "serving", "twophase", "copy"])
     ap.add_argument("--instances", type=int, default=512)
+    ap.add_argument("--inner-tol", dest="inner_tol", type=float, default=1e-4,
+                    help="mixed precision: FP32 inner solve tolerance")
     ap.add_argument("--no-graph", dest="no_graph", action="store_true", help="eager launches (profiling)")
     ap.add_argument("--grid3", default="128x128x64", help="two-phase grid nx x ny x nz")
     ap.add_argument("--tp-solver", dest="tp_solver", default="GMRes", help="two-phase Krylov solver")
     ap.add_argument("--rates", default="50,100,200,500,1000,2000", help="serving: arrival rates (req/s)")
     ap.add_argument("--requests", type=int, default=200, help="serving: requests per rate")
+    ap.add_argument("--frac3d", type=float, default=0.3, help="serving: share of 3D two-phase requests")
     ap.add_argument("--hbm-budget-gb", dest="hbm_budget_gb", type=float, default=64.0)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -351,7 +401,8 @@ def main():
     dt = NONLINEAR_HEAT["dt"]
     u0 = gaussian_bumps(nx, ny, seed=2048)
     log(f"[bench] nonlinear heat {nx}x{ny}: fused engine, mixed precision")
-    head = run_fused(lib, "mixed", u0, nx, ny, dt, args.steps, args.warmup, args.precond, not args.no_graph)
+    head = run_fused(lib, "mixed", u0, nx, ny, dt, args.steps, args.warmup, args.precond, not args.no_graph,
+                     args.inner_tol)
     value = n * head["newton"] / head["seconds"]
     cnt, avg_ms = head["kern"]["stencil_spmv"]
     spmv_bytes = 32 * n   # 5 FP32 coefficients + x + dot operand read, y written

@@ -77,6 +77,10 @@ SIGNATURES = {
     "batch_add": [_P, _I, _I, _D, _D, _DP, _IP],
     "batch_remove": [_P, _I],
     "batch_begin_step": [_P, _I, _D],
+    "batch_add_twophase": [_P, _S, _I, _I, _I, _DP, _DP, _DP, _I, _IP, _DP, _IP],
+    "batch_twophase_state": [_P, _I, _DP, _DP],
+    "batch_twophase_stats": [_P, _I, _IP],
+    "batch_twophase_residual": [_P, _I, _I, _DP, _DP, _DP, _D, _IP, _DP, _IP, _IP, _DP],
     "batch_iterate": [_P, _IP, _I, _IP],
     "batch_in_flight": [_P, _I, _IP],
     "batch_step": [_P, _IP, _I, _D],
@@ -96,7 +100,8 @@ SIGNATURES = {
     "server_destroy": [_P],
     "server_submit": [_P, _I, _I, _D, _D, _DP, _I, _D, _IP],
     "server_progress": [_P, _IP, _I, _IP],
-    "server_result": [_P, _I, _DP, _IP, _DP],
+    "server_submit_twophase": [_P, _S, _I, _I, _I, _DP, _DP, _DP, _I, _IP, _DP, _I, _D, _IP],
+    "server_result": [_P, _I, _DP, _DP, _IP, _DP],
     "server_status": [_P, _IP, _DP],
     "timing": [_I],
     "timing_query": [_S, _IP, _DP],

@@ -650,6 +650,51 @@ int eqhip_batch_destroy(void* batch) {
 int eqhip_batch_add(void* batch, int nx, int ny, double k, double cv, const double* u, int* slot) {
     return guard([&] { *slot = B(batch).add(nx, ny, k, cv, u); });
 }
+int eqhip_batch_add_twophase(void* batch, const char* params, int nx, int ny, int nz, const double* perm,
+                             const double* poro, const double* sw, int nsrc, const int* src_cells,
+                             const double* src_vals, int* slot) {
+    return guard([&] {
+        ParamMap m;
+        m.parse(params ? params : "");
+        TwoPhaseSpec sp;
+        sp.nx = nx;
+        sp.ny = ny;
+        sp.nz = nz;
+        sp.dx = m.get_double("dx", 1.0);
+        sp.dy = m.get_double("dy", 1.0);
+        sp.dz = m.get_double("dz", 1.0);
+        sp.watervisc = m.get_double("watervisc", sp.watervisc);
+        sp.oilvisc = m.get_double("oilvisc", sp.oilvisc);
+        sp.min_poro = m.get_double("min_poro", sp.min_poro);
+        sp.corey = m.get_int("corey", sp.corey);
+        *slot = B(batch).add_twophase(sp, perm, poro, sw, nsrc, src_cells, src_vals);
+    });
+}
+int eqhip_batch_twophase_state(void* batch, int slot, double* p, double* sw) {
+    return guard([&] { B(batch).get_twophase_state(slot, p, sw); });
+}
+int eqhip_batch_twophase_stats(void* batch, int slot, int* stats) {
+    return guard([&] {
+        const StepStats& st = B(batch).stats(slot);
+        stats[0] = st.newton_p;
+        stats[1] = st.krylov_p;
+        stats[2] = st.stop_p;
+    });
+}
+int eqhip_batch_twophase_residual(void* batch, int slot, int stage, const double* p, const double* sw0,
+                                  const double* sw, double dt, int* info, double* F, int* rp, int* ci, double* v) {
+    return guard([&] {
+        std::vector<double> f, vv;
+        std::vector<int> r, c;
+        B(batch).export_twophase_residual(slot, stage, p, sw0, sw, dt, f, r, c, vv);
+        info[0] = (int)f.size();
+        info[1] = (int)c.size();
+        std::copy(f.begin(), f.end(), F);
+        std::copy(r.begin(), r.end(), rp);
+        std::copy(c.begin(), c.end(), ci);
+        std::copy(vv.begin(), vv.end(), v);
+    });
+}
 int eqhip_batch_remove(void* batch, int slot) {
     return guard([&] { B(batch).remove(slot); });
 }
@@ -744,6 +789,26 @@ int eqhip_server_submit(void* server, int nx, int ny, double k, double cv, const
                         int* request) {
     return guard([&] { *request = SV(server).submit(nx, ny, k, cv, u0, steps, dt); });
 }
+int eqhip_server_submit_twophase(void* server, const char* params, int nx, int ny, int nz, const double* perm,
+                                 const double* poro, const double* sw, int nsrc, const int* src_cells,
+                                 const double* src_vals, int steps, double dt, int* request) {
+    return guard([&] {
+        ParamMap m;
+        m.parse(params ? params : "");
+        TwoPhaseSpec sp;
+        sp.nx = nx;
+        sp.ny = ny;
+        sp.nz = nz;
+        sp.dx = m.get_double("dx", 1.0);
+        sp.dy = m.get_double("dy", 1.0);
+        sp.dz = m.get_double("dz", 1.0);
+        sp.watervisc = m.get_double("watervisc", sp.watervisc);
+        sp.oilvisc = m.get_double("oilvisc", sp.oilvisc);
+        sp.min_poro = m.get_double("min_poro", sp.min_poro);
+        sp.corey = m.get_int("corey", sp.corey);
+        *request = SV(server).submit_twophase(sp, perm, poro, sw, nsrc, src_cells, src_vals, steps, dt);
+    });
+}
 int eqhip_server_progress(void* server, int* finished, int cap, int* nfinished) {
     return guard([&] {
         std::vector<int> done = SV(server).progress();
@@ -752,10 +817,10 @@ int eqhip_server_progress(void* server, int* finished, int cap, int* nfinished) 
         *nfinished = (int)done.size();
     });
 }
-int eqhip_server_result(void* server, int request, double* u, int* stats, double* times) {
+int eqhip_server_result(void* server, int request, double* u, double* p, int* stats, double* times) {
     return guard([&] {
         RequestInfo info;
-        SV(server).take_result(request, u, &info);
+        SV(server).take_result(request, u, p, &info);
         if (stats) {
             stats[0] = info.steps;
             stats[1] = info.newton;

@@ -24,17 +24,30 @@ enum Vec { V_R = 0, V_RH, V_P, V_V, V_S, V_T, V_PH, V_SH, V_X, NVEC };
 enum Act { A_INIT = 0, A_ALPHA, A_OMEGA, A_RHO, A_STORE };
 enum Pc { PC_NONE = 0, PC_JACOBI = 1, PC_ILU = 2 };
 
+enum Kind { K_HEAT = 0, K_TWOPHASE = 1 };
+
 struct Desc {
-    int n, nx, ny, xcd;           // xcd: XCD-contiguous block order (same for all of a table)
-    double rnx;                   // 1 / nx (cell -> (i, j) without integer division)
-    double k, cv, dt, gx, gy, vol;
-    double *u, *u0, *F, *du, *r64;
-    double *C, *W, *E, *S, *N;   // FP64 Jacobian stencil coefficients
-    void *Cm, *Wm, *Em, *Sm, *Nm; // Krylov-precision coefficients (alias the FP64 ones in fp64 mode)
+    int n, nx, ny, nz;
+    int xcd;                      // XCD-contiguous block order (same for all of a table)
+    int kind;                     // K_HEAT | K_TWOPHASE
+    int corey, pad;
+    double rnx, rnxy;             // 1 / nx, 1 / (nx ny): cell -> (i, j, k) without integer division
+    double dt;
+    // heat: k(u) = k (1 + u^2), x / y face geometry factors, cell volume
+    double k, cv, gx, gy, vol;
+    double *u, *u0;
+    // two-phase: transmissibility of the +x / +y / +z face of each cell, pore volume,
+    // source rate, pressure, saturation and its value at the start of the step
+    double mw_inv, mo_inv;        // 1 / watervisc, 1 / oilvisc
+    const double *tx, *ty, *tz, *pv, *src;
+    double *p, *sw, *sw0;
+    double *F, *du, *r64;
+    double *C, *W, *E, *S, *N, *B, *T;   // FP64 Jacobian stencil coefficients (B, T: 3D only)
+    void *Cm, *Wm, *Em, *Sm, *Nm, *Bm, *Tm; // Krylov-precision copies (alias the FP64 ones in fp64 mode)
     void* dinv;                   // Jacobi: 1/C ; ILU: 1/pivot (black) or 1/C (red)
     void* v[NVEC];                // Krylov vectors
     double* sc;                   // NSLOT scalars
-    int* act;                     // [0] Krylov active, [1] refinement active
+    int* act;                     // [0] Krylov active, [1] refinement active, [2] two-phase stage
     unsigned long long* ts;       // kernel timing only (else null): per block {start, end}
 };
 
@@ -128,20 +141,27 @@ __device__ inline void group_store(const double (&v)[CPT], double* out) {
     __syncthreads();
 }
 
-// (i, j) of cell c = i + nx j: a double multiply by 1/nx and one correction step
-// (the product is within one ulp, so the floor is off by at most one).
-__device__ inline void cell_ij(const Desc& d, int c, int& i, int& j) {
-    int jj = (int)((double)c * d.rnx);
-    int ii = c - jj * d.nx;
-    if (ii < 0) {
-        --jj;
-        ii += d.nx;
-    } else if (ii >= d.nx) {
-        ++jj;
-        ii -= d.nx;
+// (i, j, k) of cell c = i + nx (j + ny k): double multiplies by 1/(nx ny) and 1/nx, each
+// with one correction step (the product is within one ulp, so the floor is off by at
+// most one).
+__device__ inline void div_fix(int c, double rd, int dv, int& q, int& r) {
+    int qq = (int)((double)c * rd);
+    int rr = c - qq * dv;
+    if (rr < 0) {
+        --qq;
+        rr += dv;
+    } else if (rr >= dv) {
+        ++qq;
+        rr -= dv;
     }
-    i = ii;
-    j = jj;
+    q = qq;
+    r = rr;
+}
+__device__ inline void cell_ijk(const Desc& d, int c, int& i, int& j, int& k) {
+    int rem = c;
+    k = 0;
+    if (d.nz > 1) div_fix(c, d.rnxy, d.nx * d.ny, k, rem);
+    div_fix(rem, d.rnx, d.nx, j, i);
 }
 
 // Per-instance finalisation, run by one whole block: sums the instance's block
@@ -223,18 +243,129 @@ __global__ __launch_bounds__(FBS) void k_fin(const Desc* __restrict__ D, const i
 // ---------------------------------------------------------------------------
 // Residual + Jacobian assembly: one pass over u, u0; writes F, C/W/E/S/N
 // (FP64 and, when mixed, FP32) and per-block partial ||F||^2.
+// Two-phase mobilities (twophase.equelle computeWaterMob / computeOilMob):
+// corey 1: krw = s, kro = 1 - s; corey 2: krw = s^2, kro = (1 - s)^2.
+__device__ inline double mob_w(const Desc& d, double s) { return (d.corey == 2 ? s * s : s) * d.mw_inv; }
+__device__ inline double mob_o(const Desc& d, double s) {
+    const double so = 1.0 - s;
+    return (d.corey == 2 ? so * so : so) * d.mo_inv;
+}
+__device__ inline double total_mob(const Desc& d, double s) { return mob_w(d, s) + mob_o(d, s); }
+// fractional flow f = mw / (mw + mo) and its derivative
+__device__ inline void frac_flow(const Desc& d, double s, double& f, double& df) {
+    const double mw = mob_w(d, s), mo = mob_o(d, s);
+    const double dmw = (d.corey == 2 ? 2.0 * s : 1.0) * d.mw_inv;
+    const double dmo = -(d.corey == 2 ? 2.0 * (1.0 - s) : 1.0) * d.mo_inv;
+    const double t = mw + mo;
+    f = mw / t;
+    df = (dmw * mo - mw * dmo) / (t * t);
+}
+// Total flux through the face first -> second (computeTotalFlux): trans * lambda_up *
+// (p_first - p_second), lambda upwinded on the sign of -grad p.
+__device__ inline double face_flux(const Desc& d, double trans, int first, int second) {
+    const double ng = d.p[first] - d.p[second];
+    const double lam = ng >= 0.0 ? total_mob(d, d.sw0[first]) : total_mob(d, d.sw0[second]);
+    return (trans * lam) * ng;
+}
+
+// Two-phase pressure residual and Jacobian at cell c (computePressureResidual):
+// F = sum_faces T lambda_up (p_c - p_nb) - source, J_c,nb = -T lambda_up.
+__device__ inline double tp_pressure(const Desc& d, int c, int i, int j, int k, double* co) {
+    const int nx = d.nx, nxy = d.nx * d.ny;
+    const double pc = d.p[c];
+    double F = -d.src[c];
+    co[0] = 0.0;
+    auto face = [&](int nb, double trans, bool c_first, double& cm) {
+        const double pn = d.p[nb];
+        const double ng = c_first ? pc - pn : pn - pc;   // p_first - p_second
+        const double lam = ng >= 0.0 ? total_mob(d, d.sw0[c_first ? c : nb]) : total_mob(d, d.sw0[c_first ? nb : c]);
+        const double tl = trans * lam;
+        F += tl * (pc - pn);
+        co[0] += tl;
+        cm = -tl;
+    };
+    if (i > 0) face(c - 1, d.tx[c - 1], false, co[1]);
+    if (i < d.nx - 1) face(c + 1, d.tx[c], true, co[2]);
+    if (j > 0) face(c - nx, d.ty[c - nx], false, co[3]);
+    if (j < d.ny - 1) face(c + nx, d.ty[c], true, co[4]);
+    if (k > 0) face(c - nxy, d.tz[c - nxy], false, co[5]);
+    if (k < d.nz - 1) face(c + nxy, d.tz[c], true, co[6]);
+    return F;
+}
+
+// Two-phase transport residual and Jacobian at cell c (computeTransportResidual):
+// F = sw - sw0 + dt/pv (sum_faces +-f_up flux - q), q = max(src,0) + min(src,0) f(sw),
+// f upwinded on the sign of the total flux (fixed from the pressure solve).
+__device__ inline double tp_transport(const Desc& d, int c, int i, int j, int k, double* co) {
+    const int nx = d.nx, nxy = d.nx * d.ny;
+    const double swc = d.sw[c];
+    double fc, dfc;
+    frac_flow(d, swc, fc, dfc);
+    const double a = d.dt / d.pv[c];
+    double div = 0.0, ddiag = 0.0;
+    // face with `other`: c_first -> flux counts +, else -
+    auto face = [&](int nb, double trans, bool c_first, double& cm) {
+        const double flux = c_first ? face_flux(d, trans, c, nb) : face_flux(d, trans, nb, c);
+        const double sgn = c_first ? 1.0 : -1.0;
+        const bool up_c = (flux >= 0.0) == c_first;   // the first cell is upstream when flux >= 0
+        cm = 0.0;
+        if (up_c) {
+            div += sgn * fc * flux;
+            ddiag += sgn * dfc * flux;
+        } else {
+            double fn, dfn;
+            frac_flow(d, d.sw[nb], fn, dfn);
+            div += sgn * fn * flux;
+            cm = a * sgn * dfn * flux;
+        }
+    };
+    if (i > 0) face(c - 1, d.tx[c - 1], false, co[1]);
+    if (i < d.nx - 1) face(c + 1, d.tx[c], true, co[2]);
+    if (j > 0) face(c - nx, d.ty[c - nx], false, co[3]);
+    if (j < d.ny - 1) face(c + nx, d.ty[c], true, co[4]);
+    if (k > 0) face(c - nxy, d.tz[c - nxy], false, co[5]);
+    if (k < d.nz - 1) face(c + nxy, d.tz[c], true, co[6]);
+    const double src = d.src[c];
+    const double insrc = src > 0.0 ? src : 0.0, outsrc = src < 0.0 ? src : 0.0;
+    const double q = insrc + outsrc * fc;
+    co[0] = 1.0 + a * (ddiag - outsrc * dfc);
+    return (swc - d.sw0[c]) + a * (div - q);
+}
+
 template <bool MIXED, int CPT>
 __global__ __launch_bounds__(BS) void k_assemble(const Desc* __restrict__ D, const int2* __restrict__ blk,
                                                  double* __restrict__ part) {
     LOCATE();
     double f2[CPT] = {};
+    if (d.kind == K_TWOPHASE) {
+        const int stage = d.act[2];
+        FOR_GROUPS(m, c) {
+            if (c >= d.n) continue;
+            int i, j, k;
+            cell_ijk(d, c, i, j, k);
+            double co[7] = {0, 0, 0, 0, 0, 0, 0};
+            const double F = stage == 0 ? tp_pressure(d, c, i, j, k, co) : tp_transport(d, c, i, j, k, co);
+            d.F[c] = F;
+            double* dst[7] = {d.C, d.W, d.E, d.S, d.N, d.B, d.T};
+            for (int q = 0; q < (d.nz > 1 ? 7 : 5); ++q) dst[q][c] = co[q];
+            if (MIXED) {
+                float* dm[7] = {(float*)d.Cm, (float*)d.Wm, (float*)d.Em, (float*)d.Sm,
+                                (float*)d.Nm, (float*)d.Bm, (float*)d.Tm};
+                for (int q = 0; q < (d.nz > 1 ? 7 : 5); ++q) dm[q][c] = (float)co[q];
+            }
+            f2[m] = F * F;
+        }
+        group_store<CPT>(f2, part + lb * CPT);
+        KTIME_END();
+        return;
+    }
     const int nx = d.nx, ny = d.ny;
     const double k = d.k, coef = d.dt / (d.cv * d.vol), gx = d.gx, gy = d.gy;
     const double* __restrict__ u = d.u;
     FOR_GROUPS(m, c) {
         if (c >= d.n) continue;
-        int i, j;
-        cell_ij(d, c, i, j);
+        int i, j, kk;
+        cell_ijk(d, c, i, j, kk);
         const double uc = u[c];
         const double kc = k * (1.0 + uc * uc), dkc = 2.0 * k * uc;
         double div = 0.0, diag = 1.0, cw = 0.0, ce = 0.0, cs = 0.0, cn = 0.0;
@@ -272,13 +403,18 @@ __global__ __launch_bounds__(BS) void k_assemble(const Desc* __restrict__ D, con
 }
 
 template <class T>
-__device__ inline T stencil_apply(const Desc& d, const T* __restrict__ x, int c, int i, int j) {
+__device__ inline T stencil_apply(const Desc& d, const T* __restrict__ x, int c, int i, int j, int k) {
     const int nx = d.nx;
     T acc = ((const T*)d.Cm)[c] * x[c];
     if (i > 0) acc += ((const T*)d.Wm)[c] * x[c - 1];
     if (i < nx - 1) acc += ((const T*)d.Em)[c] * x[c + 1];
     if (j > 0) acc += ((const T*)d.Sm)[c] * x[c - nx];
     if (j < d.ny - 1) acc += ((const T*)d.Nm)[c] * x[c + nx];
+    if (d.nz > 1) {
+        const int nxy = nx * d.ny;
+        if (k > 0) acc += ((const T*)d.Bm)[c] * x[c - nxy];
+        if (k < d.nz - 1) acc += ((const T*)d.Tm)[c] * x[c + nxy];
+    }
     return acc;
 }
 
@@ -296,9 +432,9 @@ __global__ __launch_bounds__(BS) void k_spmv_dot(const Desc* __restrict__ D, con
         const T* __restrict__ vb = a1 >= 0 ? (const T*)d.v[a1] : nullptr;
         FOR_GROUPS(m, c) {
             if (c >= d.n) continue;
-            int i, j;
-            cell_ij(d, c, i, j);
-            const T yc = stencil_apply<T>(d, x, c, i, j);
+            int i, j, k;
+            cell_ijk(d, c, i, j, k);
+            const T yc = stencil_apply<T>(d, x, c, i, j, k);
             y[c] = yc;
             s0[m] = (double)va[c] * (double)yc;
             if (nd > 1) s1[m] = (vb ? (double)vb[c] : (double)yc) * (double)yc;
@@ -320,13 +456,18 @@ __global__ __launch_bounds__(BS) void k_defect(const Desc* __restrict__ D, const
         const double* __restrict__ x = d.du;
         FOR_GROUPS(m, c) {
             if (c >= d.n) continue;
-            int i, j;
-        cell_ij(d, c, i, j);
+            int i, j, k;
+            cell_ijk(d, c, i, j, k);
             double acc = d.C[c] * x[c];
             if (i > 0) acc += d.W[c] * x[c - 1];
             if (i < nx - 1) acc += d.E[c] * x[c + 1];
             if (j > 0) acc += d.S[c] * x[c - nx];
             if (j < d.ny - 1) acc += d.N[c] * x[c + nx];
+            if (d.nz > 1) {
+                const int nxy = nx * d.ny;
+                if (k > 0) acc += d.B[c] * x[c - nxy];
+                if (k < d.nz - 1) acc += d.T[c] * x[c + nxy];
+            }
             const double r = d.F[c] - acc;
             d.r64[c] = r;
             r2[m] = r * r;
@@ -336,7 +477,7 @@ __global__ __launch_bounds__(BS) void k_defect(const Desc* __restrict__ D, const
     KTIME_END();
 }
 
-// Preconditioner set-up: Jacobi 1/C, or red-black ILU(0) pivots.  Red = (i+j) even
+// Preconditioner set-up: Jacobi 1/C, or red-black ILU(0) pivots.  Red = (i+j+k) even
 // cells come first in the ordering; red rows have no L part, a black row's L part is its
 // red neighbours and its pivot is C_b - sum_m A_bm A_mb / C_m (fill-in dropped).
 // dinv holds 1/C for red cells and 1/pivot for black cells.
@@ -346,17 +487,22 @@ __global__ __launch_bounds__(BS) void k_precond_setup(const Desc* __restrict__ D
     LOCATE();
     const T* Cm = (const T*)d.Cm;
     const T *Wm = (const T*)d.Wm, *Em = (const T*)d.Em, *Sm = (const T*)d.Sm, *Nm = (const T*)d.Nm;
-    const int nx = d.nx;
+    const T *Bm = (const T*)d.Bm, *Tm = (const T*)d.Tm;
+    const int nx = d.nx, nxy = d.nx * d.ny;
     FOR_CELLS(c) {
         T piv = Cm[c];
         if (mode == PC_ILU) {
-            int i, j;
-        cell_ij(d, c, i, j);
-            if ((i + j) & 1) {
+            int i, j, k;
+            cell_ijk(d, c, i, j, k);
+            if ((i + j + k) & 1) {
                 if (i > 0) piv -= Wm[c] * Em[c - 1] / Cm[c - 1];
                 if (i < nx - 1) piv -= Em[c] * Wm[c + 1] / Cm[c + 1];
                 if (j > 0) piv -= Sm[c] * Nm[c - nx] / Cm[c - nx];
                 if (j < d.ny - 1) piv -= Nm[c] * Sm[c + nx] / Cm[c + nx];
+                if (d.nz > 1) {
+                    if (k > 0) piv -= Bm[c] * Tm[c - nxy] / Cm[c - nxy];
+                    if (k < d.nz - 1) piv -= Tm[c] * Bm[c + nxy] / Cm[c + nxy];
+                }
             }
         }
         ((T*)d.dinv)[c] = piv != (T)0 ? (T)1 / piv : (T)1;
@@ -368,7 +514,7 @@ __global__ __launch_bounds__(BS) void k_precond_setup(const Desc* __restrict__ D
 // z_r = (r_r - sum_b A_rb z_b) / C_r.  1/C_m of a red neighbour is its dinv, so only
 // multiplications remain.
 template <class T>
-__device__ inline T black_value(const Desc& d, const T* __restrict__ r, int b, int i, int j) {
+__device__ inline T black_value(const Desc& d, const T* __restrict__ r, int b, int i, int j, int k) {
     const int nx = d.nx, ny = d.ny;
     const T* dv = (const T*)d.dinv;
     const T *Wm = (const T*)d.Wm, *Em = (const T*)d.Em, *Sm = (const T*)d.Sm, *Nm = (const T*)d.Nm;
@@ -377,6 +523,11 @@ __device__ inline T black_value(const Desc& d, const T* __restrict__ r, int b, i
     if (i < nx - 1) s -= Em[b] * (r[b + 1] * dv[b + 1]);
     if (j > 0) s -= Sm[b] * (r[b - nx] * dv[b - nx]);
     if (j < ny - 1) s -= Nm[b] * (r[b + nx] * dv[b + nx]);
+    if (d.nz > 1) {
+        const int nxy = nx * ny;
+        if (k > 0) s -= ((const T*)d.Bm)[b] * (r[b - nxy] * dv[b - nxy]);
+        if (k < d.nz - 1) s -= ((const T*)d.Tm)[b] * (r[b + nxy] * dv[b + nxy]);
+    }
     return s * dv[b];
 }
 // Pass 1 of the apply: z of the black cells (their L part only touches red cells,
@@ -389,9 +540,9 @@ __global__ __launch_bounds__(BS) void k_ilu_black(const Desc* __restrict__ D, co
         const T* __restrict__ r = (const T*)d.v[ir];
         T* __restrict__ zo = (T*)d.v[iz];
         FOR_CELLS(c) {
-            int i, j;
-            cell_ij(d, c, i, j);
-            if ((i + j) & 1) zo[c] = black_value<T>(d, r, c, i, j);
+            int i, j, k;
+            cell_ijk(d, c, i, j, k);
+            if ((i + j + k) & 1) zo[c] = black_value<T>(d, r, c, i, j, k);
         }
     }
     KTIME_END();
@@ -406,14 +557,19 @@ __global__ __launch_bounds__(BS) void k_ilu_red(const Desc* __restrict__ D, cons
         const T* __restrict__ r = (const T*)d.v[ir];
         T* __restrict__ z = (T*)d.v[iz];
         FOR_CELLS(c) {
-            int i, j;
-            cell_ij(d, c, i, j);
-            if ((i + j) & 1) continue;
+            int i, j, k;
+            cell_ijk(d, c, i, j, k);
+            if ((i + j + k) & 1) continue;
             T s = r[c];
             if (i > 0) s -= ((const T*)d.Wm)[c] * z[c - 1];
             if (i < nx - 1) s -= ((const T*)d.Em)[c] * z[c + 1];
             if (j > 0) s -= ((const T*)d.Sm)[c] * z[c - nx];
             if (j < ny - 1) s -= ((const T*)d.Nm)[c] * z[c + nx];
+            if (d.nz > 1) {
+                const int nxy = nx * ny;
+                if (k > 0) s -= ((const T*)d.Bm)[c] * z[c - nxy];
+                if (k < d.nz - 1) s -= ((const T*)d.Tm)[c] * z[c + nxy];
+            }
             z[c] = s * ((const T*)d.dinv)[c];
         }
     }
@@ -559,10 +715,12 @@ __global__ __launch_bounds__(BS) void k_zero_du(const Desc* __restrict__ D, cons
         d.r64[c] = d.F[c];
     }
 }
+// u -= du for the Newton unknown: heat u; two-phase p (stage 0) or sw (stage 1)
 template <int CPT>
 __global__ __launch_bounds__(BS) void k_newton_update(const Desc* __restrict__ D, const int2* __restrict__ blk) {
     LOCATE();
-    FOR_CELLS(c) d.u[c] -= d.du[c];
+    double* x = d.kind == K_TWOPHASE ? (d.act[2] == 0 ? d.p : d.sw) : d.u;
+    FOR_CELLS(c) x[c] -= d.du[c];
 }
 __global__ void k_gather_slot(const Desc* __restrict__ D, int ninst, int slot, double* __restrict__ out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -576,13 +734,22 @@ __global__ void k_gather_slot(const Desc* __restrict__ D, int ninst, int slot, d
 // ---------------------------------------------------------------------------
 struct Inst {
     int id = 0;
-    int nx = 0, ny = 0, n = 0;
+    int kind = K_HEAT;
+    int nx = 0, ny = 0, nz = 1, n = 0;
     double k = 0, cv = 1, dx = 1, dy = 1, dt = 0;
-    DevPtr<double> u, u0, F, du, r64, C, W, E, S, N, sc;
+    // two-phase
+    double mw_inv = 0, mo_inv = 0;
+    int corey = 1;
+    int stage = 0;                 // 0 pressure, 1 transport
+    double r0_stage = 0;           // ||F|| at the start of the current Newton solve
+    DevPtr<double> tp;             // tx, ty, tz, pv, src (n each)
+    DevPtr<double> p, sw, sw0;
+    DevPtr<double> u, u0, F, du, r64, C, W, E, S, N, B, T, sc;
     DevPtr<char> coef32, dinv, vecs;
     DevPtr<int> act;
     enum State { IDLE, START, SOLVE, DONE } state = IDLE;
     StepStats st;
+    int ncoef() const { return nz > 1 ? 7 : 5; }
 };
 
 // Launch shape of a table: cells per thread and XCD-contiguous block order.
@@ -649,15 +816,33 @@ struct HeatBatch::Impl {
         d.n = I.n;
         d.nx = I.nx;
         d.ny = I.ny;
+        d.nz = I.nz;
+        d.kind = I.kind;
+        d.corey = I.corey;
         d.rnx = 1.0 / I.nx;
-        d.k = I.k;
-        d.cv = I.cv;
+        d.rnxy = 1.0 / ((double)I.nx * I.ny);
         d.dt = I.dt;
-        d.gx = I.dy / I.dx;   // x-faces: area dy over centroid distance dx
-        d.gy = I.dx / I.dy;
-        d.vol = I.dx * I.dy;
-        d.u = I.u->p;
-        d.u0 = I.u0->p;
+        if (I.kind == K_HEAT) {
+            d.k = I.k;
+            d.cv = I.cv;
+            d.gx = I.dy / I.dx;   // x-faces: area dy over centroid distance dx
+            d.gy = I.dx / I.dy;
+            d.vol = I.dx * I.dy;
+            d.u = I.u->p;
+            d.u0 = I.u0->p;
+        } else {
+            d.mw_inv = I.mw_inv;
+            d.mo_inv = I.mo_inv;
+            const double* t = I.tp->p;
+            d.tx = t;
+            d.ty = t + I.n;
+            d.tz = t + 2L * I.n;
+            d.pv = t + 3L * I.n;
+            d.src = t + 4L * I.n;
+            d.p = I.p->p;
+            d.sw = I.sw->p;
+            d.sw0 = I.sw0->p;
+        }
         d.F = I.F->p;
         d.du = I.du->p;
         d.r64 = I.r64->p;
@@ -666,6 +851,8 @@ struct HeatBatch::Impl {
         d.E = I.E->p;
         d.S = I.S->p;
         d.N = I.N->p;
+        d.B = I.B ? I.B->p : nullptr;
+        d.T = I.T ? I.T->p : nullptr;
         if (mixed) {
             float* f = (float*)I.coef32->p;
             d.Cm = f;
@@ -673,12 +860,16 @@ struct HeatBatch::Impl {
             d.Em = f + 2L * I.n;
             d.Sm = f + 3L * I.n;
             d.Nm = f + 4L * I.n;
+            d.Bm = I.nz > 1 ? f + 5L * I.n : nullptr;
+            d.Tm = I.nz > 1 ? f + 6L * I.n : nullptr;
         } else {
             d.Cm = d.C;
             d.Wm = d.W;
             d.Em = d.E;
             d.Sm = d.S;
             d.Nm = d.N;
+            d.Bm = d.B;
+            d.Tm = d.T;
         }
         d.dinv = I.dinv->p;
         for (int v = 0; v < NVEC; ++v) d.v[v] = I.vecs->p + (size_t)v * I.n * ks();
@@ -1001,9 +1192,37 @@ HeatBatch::~HeatBatch() {
     delete p_;
 }
 
-size_t HeatBatch::estimate_bytes(int n, const std::string& precision) {
+size_t HeatBatch::estimate_bytes(int n, const std::string& precision, bool twophase) {
     const size_t ks = precision == "mixed" ? 4 : 8;
-    return (size_t)n * (10 * 8 + (precision == "mixed" ? 5 * 4 : 0) + ks * (1 + NVEC)) + NSLOT * 8 + 64;
+    const bool mixed = precision == "mixed";
+    if (!twophase)   // u, u0, F, du, r64, 5 coefficients (+ FP32 copies), dinv, Krylov vectors
+        return (size_t)n * (10 * 8 + (mixed ? 5 * 4 : 0) + ks * (1 + NVEC)) + NSLOT * 8 + 64;
+    // tx, ty, tz, pv, src, p, sw, sw0, F, du, r64, 7 coefficients (+ FP32), dinv, Krylov vectors
+    return (size_t)n * (18 * 8 + (mixed ? 7 * 4 : 0) + ks * (1 + NVEC)) + NSLOT * 8 + 64;
+}
+
+// Newton / Krylov storage common to both instance kinds.
+static void alloc_solver(Inst& I, bool mixed, size_t ks) {
+    const int n = I.n;
+    I.F = dev_alloc<double>(n);
+    I.du = dev_alloc<double>(n);
+    I.r64 = dev_alloc<double>(n);
+    I.C = dev_alloc<double>(n);
+    I.W = dev_alloc<double>(n);
+    I.E = dev_alloc<double>(n);
+    I.S = dev_alloc<double>(n);
+    I.N = dev_alloc<double>(n);
+    if (I.nz > 1) {
+        I.B = dev_alloc<double>(n);
+        I.T = dev_alloc<double>(n);
+    }
+    I.sc = dev_alloc<double>(NSLOT);
+    I.act = dev_alloc<int>(4);
+    EQ_HIP(hipMemsetAsync(I.sc->p, 0, NSLOT * sizeof(double), S()));
+    EQ_HIP(hipMemsetAsync(I.act->p, 0, 4 * sizeof(int), S()));
+    if (mixed) I.coef32 = dev_alloc<char>((size_t)I.ncoef() * n * sizeof(float));
+    I.dinv = dev_alloc<char>((size_t)n * ks);
+    I.vecs = dev_alloc<char>((size_t)NVEC * n * ks);
 }
 
 int HeatBatch::add(int nx, int ny, double k, double cv, const double* u_host, double dx, double dy) {
@@ -1018,24 +1237,63 @@ int HeatBatch::add(int nx, int ny, double k, double cv, const double* u_host, do
     I->cv = cv;
     I->dx = dx;
     I->dy = dy;
-    const int n = I->n;
-    I->u = dev_from_host(u_host, n);
-    I->u0 = dev_alloc<double>(n);
-    I->F = dev_alloc<double>(n);
-    I->du = dev_alloc<double>(n);
-    I->r64 = dev_alloc<double>(n);
-    I->C = dev_alloc<double>(n);
-    I->W = dev_alloc<double>(n);
-    I->E = dev_alloc<double>(n);
-    I->S = dev_alloc<double>(n);
-    I->N = dev_alloc<double>(n);
-    I->sc = dev_alloc<double>(NSLOT);
-    I->act = dev_alloc<int>(2);
-    EQ_HIP(hipMemsetAsync(I->sc->p, 0, NSLOT * sizeof(double), S()));
-    EQ_HIP(hipMemsetAsync(I->act->p, 0, 2 * sizeof(int), S()));
-    if (p_->mixed) I->coef32 = dev_alloc<char>((size_t)5 * n * sizeof(float));
-    I->dinv = dev_alloc<char>((size_t)n * p_->ks());
-    I->vecs = dev_alloc<char>((size_t)NVEC * n * p_->ks());
+    I->u = dev_from_host(u_host, I->n);
+    I->u0 = dev_alloc<double>(I->n);
+    alloc_solver(*I, p_->mixed, p_->ks());
+    const int id = I->id;
+    p_->insts[id] = std::move(I);
+    ++p_->version;
+    return id;
+}
+
+int HeatBatch::add_twophase(const TwoPhaseSpec& sp, const double* perm, const double* poro, const double* sw_host,
+                            int nsrc, const int* src_cells, const double* src_vals) {
+    const int nx = sp.nx, ny = sp.ny, nz = sp.nz;
+    EQ_REQUIRE(nx > 0 && ny > 0 && nz > 0, "instance sizes must be positive");
+    EQ_REQUIRE((long)nx * ny * nz < (1L << 31) - CPT_MAX * BS, "instance too large");
+    EQ_REQUIRE(sp.corey == 1 || sp.corey == 2, "corey must be 1 or 2");
+    EQ_REQUIRE(sp.watervisc > 0 && sp.oilvisc > 0, "viscosities must be positive");
+    auto I = std::make_unique<Inst>();
+    I->id = p_->next_id++;
+    I->kind = K_TWOPHASE;
+    I->nx = nx;
+    I->ny = ny;
+    I->nz = nz;
+    const int n = nx * ny * nz;
+    I->n = n;
+    I->dx = sp.dx;
+    I->dy = sp.dy;
+    I->corey = sp.corey;
+    I->mw_inv = 1.0 / sp.watervisc;
+    I->mo_inv = 1.0 / sp.oilvisc;
+    // transmissibilities (computeTransmissibilities): half-transmissibility of a cell
+    // towards a face = area * perm * |n . d| / |d|^2 with d the cell-to-face centroid
+    // vector (|d| = h/2 on a Cartesian grid), harmonic sum of the two halves
+    const double dz = nz > 1 ? sp.dz : 1.0;
+    const double ax = sp.dy * dz, ay = sp.dx * dz, az = sp.dx * sp.dy;
+    const double vol = sp.dx * sp.dy * dz;
+    std::vector<double> h((size_t)5 * n, 0.0);
+    auto half = [](double a, double perm_c, double hh) { return a * perm_c * (0.5 * hh) / (0.25 * hh * hh); };
+    auto harm = [](double h1, double h2) { return 1.0 / (1.0 / h1 + 1.0 / h2); };
+    for (int c = 0; c < n; ++c) {
+        const int i = c % nx, j = (c / nx) % ny, k = c / (nx * ny);
+        if (i < nx - 1) h[c] = harm(half(ax, perm[c], sp.dx), half(ax, perm[c + 1], sp.dx));
+        if (j < ny - 1) h[n + c] = harm(half(ay, perm[c], sp.dy), half(ay, perm[c + nx], sp.dy));
+        if (k < nz - 1) h[2L * n + c] = harm(half(az, perm[c], dz), half(az, perm[c + nx * ny], dz));
+        const double ph = poro[c] < sp.min_poro ? sp.min_poro : poro[c];
+        h[3L * n + c] = ph * vol;
+    }
+    for (int q = 0; q < nsrc; ++q) {
+        EQ_REQUIRE(src_cells[q] >= 0 && src_cells[q] < n, "source cell " << src_cells[q] << " outside the grid");
+        EQ_REQUIRE(q == 0 || src_cells[q] > src_cells[q - 1], "source cells must be ascending");
+        h[4L * n + src_cells[q]] = src_vals[q];
+    }
+    I->tp = dev_from_host(h.data(), h.size());
+    I->p = dev_alloc<double>(n);
+    launch_fill(I->p->p, 0.0, n);
+    I->sw = dev_from_host(sw_host, n);
+    I->sw0 = dev_alloc<double>(n);
+    alloc_solver(*I, p_->mixed, p_->ks());
     const int id = I->id;
     p_->insts[id] = std::move(I);
     ++p_->version;
@@ -1061,15 +1319,25 @@ bool HeatBatch::in_flight(int slot) const {
 }
 const StepStats& HeatBatch::stats(int slot) const { return get_inst(p_, slot).st; }
 size_t HeatBatch::device_bytes(int slot) const {
-    return estimate_bytes(get_inst(p_, slot).n, p_->cfg.precision);
+    const Inst& I = get_inst(p_, slot);
+    return estimate_bytes(I.n, p_->cfg.precision, I.kind == K_TWOPHASE);
 }
+
+static const int kStage[2] = {0, 1};
 
 void HeatBatch::begin_step(int slot, double dt) {
     Inst& I = get_inst(p_, slot);
     EQ_REQUIRE(I.state != Inst::START && I.state != Inst::SOLVE, "slot " << slot << " already has a step in flight");
     if (I.dt != dt) ++p_->version;
     I.dt = dt;
-    EQ_HIP(hipMemcpyAsync(I.u0->p, I.u->p, (size_t)I.n * sizeof(double), hipMemcpyDeviceToDevice, S()));
+    if (I.kind == K_HEAT) {
+        EQ_HIP(hipMemcpyAsync(I.u0->p, I.u->p, (size_t)I.n * sizeof(double), hipMemcpyDeviceToDevice, S()));
+    } else {
+        // sw0 = sw; pressure solve first, from the previous step's pressure
+        EQ_HIP(hipMemcpyAsync(I.sw0->p, I.sw->p, (size_t)I.n * sizeof(double), hipMemcpyDeviceToDevice, S()));
+        EQ_HIP(hipMemcpyAsync(I.act->p + 2, &kStage[0], sizeof(int), hipMemcpyHostToDevice, S()));
+        I.stage = 0;
+    }
     I.st = StepStats();
     I.state = Inst::START;
 }
@@ -1107,17 +1375,38 @@ std::vector<int> HeatBatch::iterate() {
             I.st.newton += 1;
             I.st.krylov += (int)g[na + q];
             I.st.refinements += (int)g[2 * na + q];
+            if (I.kind == K_TWOPHASE && I.stage == 0) {
+                I.st.newton_p += 1;
+                I.st.krylov_p += (int)g[na + q];
+            }
             if (!std::isfinite(g[3 * na + q])) I.st.failed = true;
         }
         const double fn = std::sqrt(g[q]);
-        if (I.state == Inst::START) I.st.r0 = fn;
+        if (I.state == Inst::START) {
+            I.r0_stage = fn;
+            if (I.kind == K_HEAT || I.stage == 0) I.st.r0 = fn;
+            I.st.newton_stage = 0;
+        } else {
+            I.st.newton_stage += 1;
+        }
         I.st.rn = fn;
         int stop = 0;
         if (fn <= c.abs_res_tol) stop = 1;
-        else if (c.rel_res_tol > 0 && fn <= c.rel_res_tol * I.st.r0) stop = 2;
+        else if (c.rel_res_tol > 0 && fn <= c.rel_res_tol * I.r0_stage) stop = 2;
         const bool bad = !std::isfinite(fn) || I.st.failed;
-        if (stop || bad || I.st.newton >= c.max_iter) {
+        const bool out = I.st.newton_stage >= c.max_iter;
+        if (I.kind == K_TWOPHASE && I.stage == 0 && !bad && (stop || out)) {
+            // pressure solved: the saturation solve starts from sw = 0.5 (twophase.equelle:107)
+            I.st.stop_p = stop;
+            I.stage = 1;
+            EQ_HIP(hipMemcpyAsync(I.act->p + 2, &kStage[1], sizeof(int), hipMemcpyHostToDevice, S()));
+            launch_fill(I.sw->p, 0.5, I.n);
+            I.state = Inst::START;
+            continue;
+        }
+        if (stop || bad || out) {
             I.st.stop = bad ? 0 : stop;
+            if (I.kind == K_TWOPHASE && I.st.stop_p == 0) I.st.stop = 0;   // pressure did not converge
             I.st.failed = bad;
             I.state = Inst::DONE;
             done.push_back(I.id);
@@ -1230,18 +1519,58 @@ double HeatBatch::time_kernel(const std::string& name, int reps, double* event_m
 
 void HeatBatch::get_state(int slot, double* u) const {
     Inst& I = get_inst(p_, slot);
-    std::vector<double> h = dev_to_host(I.u->p, I.n);
+    std::vector<double> h = dev_to_host(I.kind == K_HEAT ? I.u->p : I.sw->p, I.n);
     std::copy(h.begin(), h.end(), u);
+}
+void HeatBatch::get_twophase_state(int slot, double* p, double* sw) const {
+    Inst& I = get_inst(p_, slot);
+    EQ_REQUIRE(I.kind == K_TWOPHASE, "slot " << slot << " is not a two-phase instance");
+    if (p) {
+        std::vector<double> h = dev_to_host(I.p->p, I.n);
+        std::copy(h.begin(), h.end(), p);
+    }
+    if (sw) {
+        std::vector<double> h = dev_to_host(I.sw->p, I.n);
+        std::copy(h.begin(), h.end(), sw);
+    }
 }
 void HeatBatch::set_state(int slot, const double* u) {
     Inst& I = get_inst(p_, slot);
+    EQ_REQUIRE(I.kind == K_HEAT, "set_state: slot " << slot << " is not a heat instance");
     I.u = dev_from_host(u, I.n);
     ++p_->version;
+}
+
+// CSR with ascending columns: bottom, south, west, diagonal, east, north, top (existing
+// neighbours only; stored zeros kept, as in the AD engine's union patterns).
+static void csr_from_stencil(const Inst& I, std::vector<int>& rp, std::vector<int>& ci, std::vector<double>& v) {
+    std::vector<double> C = dev_to_host(I.C->p, I.n), W = dev_to_host(I.W->p, I.n), E = dev_to_host(I.E->p, I.n),
+                        Sx = dev_to_host(I.S->p, I.n), N = dev_to_host(I.N->p, I.n), Bt, Tp;
+    if (I.nz > 1) {
+        Bt = dev_to_host(I.B->p, I.n);
+        Tp = dev_to_host(I.T->p, I.n);
+    }
+    const int nx = I.nx, ny = I.ny, nz = I.nz, nxy = nx * ny;
+    rp.assign(I.n + 1, 0);
+    ci.clear();
+    v.clear();
+    for (int c = 0; c < I.n; ++c) {
+        const int i = c % nx, j = (c / nx) % ny, k = c / nxy;
+        if (k > 0) ci.push_back(c - nxy), v.push_back(Bt[c]);
+        if (j > 0) ci.push_back(c - nx), v.push_back(Sx[c]);
+        if (i > 0) ci.push_back(c - 1), v.push_back(W[c]);
+        ci.push_back(c), v.push_back(C[c]);
+        if (i < nx - 1) ci.push_back(c + 1), v.push_back(E[c]);
+        if (j < ny - 1) ci.push_back(c + nx), v.push_back(N[c]);
+        if (k < nz - 1) ci.push_back(c + nxy), v.push_back(Tp[c]);
+        rp[c + 1] = (int)ci.size();
+    }
 }
 
 void HeatBatch::export_residual(int slot, const double* u, const double* u0, double dt, std::vector<double>& F,
                                 std::vector<int>& rp, std::vector<int>& ci, std::vector<double>& v) {
     Inst& I = get_inst(p_, slot);
+    EQ_REQUIRE(I.kind == K_HEAT, "slot " << slot << " is not a heat instance");
     I.u = dev_from_host(u, I.n);
     I.u0 = dev_from_host(u0, I.n);
     I.dt = dt;
@@ -1249,21 +1578,26 @@ void HeatBatch::export_residual(int slot, const double* u, const double* u0, dou
     TableP tp = p_->table({&I});
     p_->assemble(*tp);
     F = dev_to_host(I.F->p, I.n);
-    std::vector<double> C = dev_to_host(I.C->p, I.n), W = dev_to_host(I.W->p, I.n), E = dev_to_host(I.E->p, I.n),
-                        Sx = dev_to_host(I.S->p, I.n), N = dev_to_host(I.N->p, I.n);
-    // CSR with ascending columns: south, west, diagonal, east, north (existing neighbours only)
-    rp.assign(I.n + 1, 0);
-    ci.clear();
-    v.clear();
-    for (int c = 0; c < I.n; ++c) {
-        const int i = c % I.nx, j = c / I.nx;
-        if (j > 0) ci.push_back(c - I.nx), v.push_back(Sx[c]);
-        if (i > 0) ci.push_back(c - 1), v.push_back(W[c]);
-        ci.push_back(c), v.push_back(C[c]);
-        if (i < I.nx - 1) ci.push_back(c + 1), v.push_back(E[c]);
-        if (j < I.ny - 1) ci.push_back(c + I.nx), v.push_back(N[c]);
-        rp[c + 1] = (int)ci.size();
-    }
+    csr_from_stencil(I, rp, ci, v);
+}
+
+void HeatBatch::export_twophase_residual(int slot, int stage, const double* p, const double* sw0, const double* sw,
+                                         double dt, std::vector<double>& F, std::vector<int>& rp,
+                                         std::vector<int>& ci, std::vector<double>& v) {
+    Inst& I = get_inst(p_, slot);
+    EQ_REQUIRE(I.kind == K_TWOPHASE, "slot " << slot << " is not a two-phase instance");
+    EQ_REQUIRE(stage == 0 || stage == 1, "stage must be 0 (pressure) or 1 (transport)");
+    EQ_HIP(hipMemcpy(I.p->p, p, (size_t)I.n * sizeof(double), hipMemcpyHostToDevice));
+    EQ_HIP(hipMemcpy(I.sw0->p, sw0, (size_t)I.n * sizeof(double), hipMemcpyHostToDevice));
+    EQ_HIP(hipMemcpy(I.sw->p, sw, (size_t)I.n * sizeof(double), hipMemcpyHostToDevice));
+    EQ_HIP(hipMemcpy(I.act->p + 2, &kStage[stage], sizeof(int), hipMemcpyHostToDevice));
+    I.stage = stage;
+    I.dt = dt;
+    ++p_->version;
+    TableP tp = p_->table({&I});
+    p_->assemble(*tp);
+    F = dev_to_host(I.F->p, I.n);
+    csr_from_stencil(I, rp, ci, v);
 }
 
 }  // namespace eqhip

@@ -48,20 +48,39 @@ struct LaunchShape {
 };
 
 struct StepStats {
-    int newton = 0;
+    int newton = 0;                    // Newton iterations of the step (two-phase: both solves)
     int krylov = 0;
     int refinements = 0;
     int stop = 0;                      // 1 abs, 2 rel, 0 max_iter / failure
     double r0 = 0, rn = 0;
     bool failed = false;               // non-finite residual or Krylov breakdown
+    int newton_p = 0, krylov_p = 0;    // two-phase: the pressure solve's share
+    int stop_p = 0;                    // two-phase: the pressure solve's stop reason
+    int newton_stage = 0;              // iterations of the solve in progress
+};
+
+// A two-phase instance (examples/dsl/twophase.equelle with Corey exponent 1 or 2) on a
+// Cartesian grid; nz = 1 gives a 2D grid.
+struct TwoPhaseSpec {
+    int nx = 1, ny = 1, nz = 1;
+    double dx = 1, dy = 1, dz = 1;
+    double watervisc = 0.0005, oilvisc = 0.005, min_poro = 0.0001;
+    int corey = 1;
 };
 
 class HeatBatch {
 public:
     explicit HeatBatch(const FusedConfig& cfg);
     ~HeatBatch();
+    HeatBatch(const HeatBatch&) = delete;
+    HeatBatch& operator=(const HeatBatch&) = delete;
     // Adds an instance (unit cells unless dx, dy given); returns its slot id.
     int add(int nx, int ny, double k, double cv, const double* u_host, double dx = 1.0, double dy = 1.0);
+    // Adds a two-phase instance: per-cell permeability, porosity, initial saturation and
+    // rate sources (ascending cells); pressure starts at 0.  Each step is a pressure
+    // Newton solve followed by a saturation Newton solve, both on the shared launches.
+    int add_twophase(const TwoPhaseSpec& spec, const double* perm, const double* poro, const double* sw,
+                     int nsrc, const int* src_cells, const double* src_vals);
     void remove(int slot);
     int size_of(int slot) const;
     // Starts one implicit time step of length dt for the slot.
@@ -73,11 +92,17 @@ public:
     const StepStats& stats(int slot) const;
     // Runs whole steps for the given slots (loops iterate()).
     void step(const std::vector<int>& slots, double dt);
-    void get_state(int slot, double* u) const;
+    void get_state(int slot, double* u) const;   // two-phase: the saturation
+    void get_twophase_state(int slot, double* p, double* sw) const;
     void set_state(int slot, const double* u);
     // Test hook: F and the Jacobian in CSR form at the slot's current state (u0 = prev).
     void export_residual(int slot, const double* u, const double* u0, double dt, std::vector<double>& F,
                          std::vector<int>& rp, std::vector<int>& ci, std::vector<double>& v);
+    // Test hook: the pressure (stage 0) or transport (stage 1) residual and Jacobian of a
+    // two-phase slot at (p, sw0, sw, dt); the slot's state is overwritten.
+    void export_twophase_residual(int slot, int stage, const double* p, const double* sw0, const double* sw,
+                                  double dt, std::vector<double>& F, std::vector<int>& rp, std::vector<int>& ci,
+                                  std::vector<double>& v);
     size_t device_bytes(int slot) const;
     // Average execution time (ms) of one launch of a kernel over every instance: the
     // span from the first block's start to the last wave's end on the device wall clock
@@ -86,7 +111,7 @@ public:
     // also counts the dispatch itself.  Kernels: stencil_spmv | assemble | ilu_black |
     // ilu_red | defect.
     double time_kernel(const std::string& name, int reps, double* event_ms = nullptr);
-    static size_t estimate_bytes(int n, const std::string& precision);
+    static size_t estimate_bytes(int n, const std::string& precision, bool twophase = false);
     // Launch shape used by the most recent launch and the number of tuned signatures.
     LaunchShape launch_shape() const;
 

@@ -10,7 +10,7 @@ static double secs(Clock::time_point a, Clock::time_point b) {
     return std::chrono::duration<double>(b - a).count();
 }
 
-HeatServer::HeatServer(const ServerConfig& cfg) : cfg_(cfg), batch_(cfg.batch) {
+HeatServer::HeatServer(const ServerConfig& cfg) : cfg_(cfg), batch_{HeatBatch(cfg.batch), HeatBatch(cfg.batch)} {
     EQ_REQUIRE(cfg.hbm_budget > 0, "hbm_budget must be positive");
     EQ_REQUIRE(cfg.max_in_flight > 0, "max_in_flight must be positive");
 }
@@ -38,39 +38,89 @@ int HeatServer::submit(int nx, int ny, double k, double cv, const double* u0, in
     return id;
 }
 
+int HeatServer::submit_twophase(const TwoPhaseSpec& spec, const double* perm, const double* poro, const double* sw,
+                                int nsrc, const int* src_cells, const double* src_vals, int steps, double dt) {
+    const long n = (long)spec.nx * spec.ny * spec.nz;
+    EQ_REQUIRE(spec.nx > 0 && spec.ny > 0 && spec.nz > 0 && n < (1L << 31), "bad request grid");
+    EQ_REQUIRE(steps > 0 && dt > 0, "a request needs steps > 0 and dt > 0");
+    const double bytes = (double)HeatBatch::estimate_bytes((int)n, cfg_.batch.precision, true);
+    EQ_REQUIRE(bytes <= cfg_.hbm_budget, "request of " << bytes << " bytes exceeds the HBM budget of "
+                                                       << cfg_.hbm_budget);
+    Req r;
+    r.id = next_id_++;
+    r.twophase = true;
+    r.spec = spec;
+    r.nx = spec.nx;
+    r.ny = spec.ny * spec.nz;   // n = nx * ny for the result copies
+    r.dt = dt;
+    r.steps = steps;
+    r.u.assign(sw, sw + n);
+    r.perm.assign(perm, perm + n);
+    r.poro.assign(poro, poro + n);
+    r.src_cells.assign(src_cells, src_cells + nsrc);
+    r.src_vals.assign(src_vals, src_vals + nsrc);
+    r.bytes = bytes;
+    r.t_submit = Clock::now();
+    const int id = r.id;
+    reqs_.emplace(id, std::move(r));
+    queue_.push_back(id);
+    return id;
+}
+
 std::vector<int> HeatServer::progress() {
     // admission: FIFO while the head fits (no overtaking, so large requests cannot starve)
-    while (!queue_.empty() && (int)running_.size() < cfg_.max_in_flight) {
+    while (!queue_.empty() && in_flight() < cfg_.max_in_flight) {
         Req& r = reqs_.at(queue_.front());
         if (used_ + r.bytes > cfg_.hbm_budget) break;
-        r.slot = batch_.add(r.nx, r.ny, r.k, r.cv, r.u.data());
+        const int cls = r.twophase ? 1 : 0;
+        if (r.twophase) {
+            r.slot = batch_[1].add_twophase(r.spec, r.perm.data(), r.poro.data(), r.u.data(),
+                                            (int)r.src_cells.size(), r.src_cells.data(), r.src_vals.data());
+            r.perm = std::vector<double>();
+            r.poro = std::vector<double>();
+        } else {
+            r.slot = batch_[0].add(r.nx, r.ny, r.k, r.cv, r.u.data());
+        }
         r.u.clear();
         r.u.shrink_to_fit();
         used_ += r.bytes;
         r.t_admit = Clock::now();
         r.info.queued_s = secs(r.t_submit, r.t_admit);
-        running_[r.slot] = r.id;
-        batch_.begin_step(r.slot, r.dt);
+        running_[cls][r.slot] = r.id;
+        batch_[cls].begin_step(r.slot, r.dt);
         queue_.pop_front();
     }
     std::vector<int> finished;
-    if (running_.empty()) return finished;
-    for (int slot : batch_.iterate()) {
-        Req& r = reqs_.at(running_.at(slot));
-        const StepStats& st = batch_.stats(slot);
+    int cls;
+    if (running_[0].empty() && running_[1].empty()) return finished;
+    if (running_[0].empty()) cls = 1;
+    else if (running_[1].empty()) cls = 0;
+    else cls = busy_s_[0] <= busy_s_[1] ? 0 : 1;   // the class with less device time so far
+    HeatBatch& batch = batch_[cls];
+    const auto t0 = Clock::now();
+    const std::vector<int> done = batch.iterate();   // ends with a device-to-host copy of the norms
+    busy_s_[cls] += secs(t0, Clock::now());
+    for (int slot : done) {
+        Req& r = reqs_.at(running_[cls].at(slot));
+        const StepStats& st = batch.stats(slot);
         r.info.steps += 1;
         r.info.newton += st.newton;
         r.info.krylov += st.krylov;
         const bool bad = st.failed || st.stop == 0;
         r.info.failed |= bad;
         if (!bad && r.info.steps < r.steps) {
-            batch_.begin_step(slot, r.dt);   // next time step joins the running batch
+            batch.begin_step(slot, r.dt);   // next time step joins the running batch
             continue;
         }
         r.u.resize((size_t)r.nx * r.ny);
-        batch_.get_state(slot, r.u.data());
-        batch_.remove(slot);
-        running_.erase(slot);
+        if (r.twophase) {
+            r.p.resize(r.u.size());
+            batch.get_twophase_state(slot, r.p.data(), r.u.data());
+        } else {
+            batch.get_state(slot, r.u.data());
+        }
+        batch.remove(slot);
+        running_[cls].erase(slot);
         used_ -= r.bytes;
         r.slot = -1;
         r.t_done = Clock::now();
@@ -80,12 +130,13 @@ std::vector<int> HeatServer::progress() {
     return finished;
 }
 
-void HeatServer::take_result(int id, double* u, RequestInfo* info) {
+void HeatServer::take_result(int id, double* u, double* p, RequestInfo* info) {
     auto it = reqs_.find(id);
     EQ_REQUIRE(it != reqs_.end(), "unknown request " << id);
     Req& r = it->second;
     EQ_REQUIRE(r.slot < 0 && !r.u.empty() && r.t_done != Clock::time_point(), "request " << id << " has not finished");
     if (u) std::copy(r.u.begin(), r.u.end(), u);
+    if (p && r.twophase) std::copy(r.p.begin(), r.p.end(), p);
     if (info) *info = r.info;
     reqs_.erase(it);
 }

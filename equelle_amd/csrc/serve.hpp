@@ -1,15 +1,22 @@
 // Request loop over a continuously re-formed batch of PDE solves.
 //
-// A request is one nonlinear-heat instance (grid, conductivity, initial field) to be
-// advanced by a number of implicit time steps.  Requests wait in a FIFO queue until
-// their HBM footprint fits the budget; admitted requests become slots of one fused
-// batch, their state stays device-resident between steps, and every progress() call
-// advances all in-flight slots by one Newton iteration in shared launches.  A slot
-// whose time step converged starts its next step in the same call; a finished
+// A request is one PDE instance to be advanced by a number of implicit time steps: a 2D
+// nonlinear-heat instance (grid, conductivity, initial field) or a two-phase instance
+// (grid, permeability, porosity, initial saturation, sources).  Requests wait in a FIFO
+// queue until their HBM footprint fits the budget; admitted requests become slots of a
+// fused batch, their state stays device-resident between steps, and a progress() call
+// advances all in-flight slots of one batch by one Newton iteration in shared launches.
+// A slot whose time step converged starts its next step in the same call; a finished
 // request's field is copied to the host and its slot (and memory) released, so the
-// next queued request can join mid-flight.  Per-instance reductions are
-// deterministic, so a request's result does not depend on which others share the
-// launches.
+// next queued request can join mid-flight.
+//
+// Heat and two-phase requests live in two batches: a Newton iteration's Krylov solve runs
+// until every member converged, and a 3D pressure solve takes hundreds of Krylov
+// iterations where a heat solve takes a few, so one shared batch would hold every heat
+// request behind the slowest pressure solve.  progress() ticks the batch that has used
+// less device time so far (time-sharing between the two classes).  Per-instance
+// reductions are deterministic, so a request's result does not depend on which others
+// share the launches.
 #pragma once
 #include <chrono>
 #include <deque>
@@ -40,31 +47,39 @@ public:
     explicit HeatServer(const ServerConfig& cfg);
     // Queues a request; returns its id.  Throws if it can never fit the budget.
     int submit(int nx, int ny, double k, double cv, const double* u0, int steps, double dt);
+    int submit_twophase(const TwoPhaseSpec& spec, const double* perm, const double* poro, const double* sw,
+                        int nsrc, const int* src_cells, const double* src_vals, int steps, double dt);
     // One scheduling tick: admits what fits, advances every in-flight request by one
     // Newton iteration, retires finished requests.  Returns the ids finished now.
     std::vector<int> progress();
-    // Copies a finished request's field (n values) and releases it.
-    void take_result(int id, double* u, RequestInfo* info);
+    // Copies a finished request's field (heat: u; two-phase: sw, and p when given; n
+    // values each) and releases it.
+    void take_result(int id, double* u, double* p, RequestInfo* info);
     int queued() const { return (int)queue_.size(); }
-    int in_flight() const { return (int)running_.size(); }
+    int in_flight() const { return (int)(running_[0].size() + running_[1].size()); }
     double used_bytes() const { return used_; }
     const ServerConfig& config() const { return cfg_; }
 
 private:
     struct Req {
         int id = 0, nx = 0, ny = 0, slot = -1;
+        bool twophase = false;
+        TwoPhaseSpec spec;
         double k = 0, cv = 1, dt = 0;
         int steps = 0;
         std::vector<double> u;    // initial field until admitted, result after completion
+        std::vector<double> perm, poro, src_vals, p;
+        std::vector<int> src_cells;
         double bytes = 0;
         std::chrono::steady_clock::time_point t_submit, t_admit, t_done;
         RequestInfo info;
     };
     ServerConfig cfg_;
-    HeatBatch batch_;
+    HeatBatch batch_[2];                // [0] heat, [1] two-phase
+    double busy_s_[2] = {0.0, 0.0};     // device time spent per class
     std::map<int, Req> reqs_;
     std::deque<int> queue_;
-    std::map<int, int> running_;   // slot -> request id
+    std::map<int, int> running_[2];     // slot -> request id, per class
     double used_ = 0;
     int next_id_ = 1;
 };

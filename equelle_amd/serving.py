@@ -1,10 +1,13 @@
 """Client side of the request loop (eqhip_server_*): a seeded Poisson request stream
 replayed against the native server in wall-clock time, with latency percentiles.
 
-Request mix (2D nonlinear heat, the physics of the 2048^2 workload with
-k(u) = k0 (1 + u^2)): side ~ round(exp(U[ln 16, ln 1024])), k0 ~ U[0.05, 0.5],
-1..20 time steps of dt = 1, Gaussian-bump initial field scaled to the grid,
-each step's Newton solve to 1e-8 relative residual.
+Request mix (each request 1..20 time steps, every Newton solve to 1e-8 relative residual):
+* 70% 2D nonlinear heat (the physics of the 2048^2 workload with k(u) = k0 (1 + u^2)):
+  side ~ round(exp(U[ln 16, ln 1024])), k0 ~ U[0.05, 0.5], dt = 1, Gaussian-bump initial
+  field scaled to the grid;
+* 30% 3D two-phase transport (the two-phase workload's physics: log-normal permeability,
+  Corey 2, viscosity ratio 5, injector and producer columns): side ~ round(exp(U[ln 16,
+  ln 128])) cells in each direction, dt = 1.
 """
 from __future__ import annotations
 
@@ -26,19 +29,46 @@ class Request:
     side: int
     k0: float
     steps: int
-    u0: np.ndarray
+    u0: np.ndarray    # heat: initial field; two-phase: initial saturation
     dt: float = 1.0
+    twophase: bool = False
+    perm: np.ndarray = None
+    poro: np.ndarray = None
+    src_cells: np.ndarray = None
+    src_vals: np.ndarray = None
+
+    @property
+    def n(self):
+        return self.side ** 3 if self.twophase else self.side ** 2
 
 
-def poisson_stream(rate: float, count: int, seed: int = 7, lo: int = 16, hi: int = 1024, max_steps: int = 20):
+TWOPHASE_INSTANCE = "watervisc=0.001 oilvisc=0.005 corey=2"
+
+
+def twophase_request(t, side, steps, seed):
+    from .workloads import twophase_inputs
+    perm, poro, sw0, sc, sv = twophase_inputs(side, side, side, seed=seed)
+    return Request(t, side, 0.0, steps, sw0, twophase=True, perm=perm, poro=poro, src_cells=sc, src_vals=sv)
+
+
+def poisson_stream(rate: float, count: int, seed: int = 7, lo: int = 16, hi: int = 1024, max_steps: int = 20,
+                   frac3d: float = 0.0, lo3: int = 16, hi3: int = 128):
     rng = np.random.default_rng(seed)
     t = np.cumsum(rng.exponential(1.0 / rate, count))
     sides = np.rint(np.exp(rng.uniform(np.log(lo), np.log(hi), count))).astype(int)
+    sides3 = np.rint(np.exp(rng.uniform(np.log(lo3), np.log(hi3), count))).astype(int)
+    is3 = rng.uniform(0.0, 1.0, count) < frac3d
     k0 = rng.uniform(0.05, 0.5, count)
     steps = rng.integers(1, max_steps + 1, count)
-    return [Request(float(t[i]), int(sides[i]), float(k0[i]), int(steps[i]),
-                    gaussian_bumps(int(sides[i]), int(sides[i]), seed * 100003 + i, width_scale=sides[i] / 2048.0))
-            for i in range(count)]
+    out = []
+    for i in range(count):
+        if is3[i]:
+            out.append(twophase_request(float(t[i]), int(sides3[i]), int(steps[i]), seed * 100003 + i))
+        else:
+            out.append(Request(float(t[i]), int(sides[i]), float(k0[i]), int(steps[i]),
+                               gaussian_bumps(int(sides[i]), int(sides[i]), seed * 100003 + i,
+                                              width_scale=sides[i] / 2048.0)))
+    return out
 
 
 class Server:
@@ -52,7 +82,12 @@ class Server:
 
     def submit(self, r: Request) -> int:
         rid = C.c_int()
-        self.lib.server_submit(self.h, r.side, r.side, r.k0, 1.0, dptr(r.u0), r.steps, r.dt, C.byref(rid))
+        if r.twophase:
+            self.lib.server_submit_twophase(self.h, TWOPHASE_INSTANCE.encode(), r.side, r.side, r.side, dptr(r.perm),
+                                            dptr(r.poro), dptr(r.u0), len(r.src_cells), iptr(r.src_cells),
+                                            dptr(r.src_vals), r.steps, r.dt, C.byref(rid))
+        else:
+            self.lib.server_submit(self.h, r.side, r.side, r.k0, 1.0, dptr(r.u0), r.steps, r.dt, C.byref(rid))
         return rid.value
 
     def progress(self):
@@ -60,11 +95,12 @@ class Server:
         self.lib.server_progress(self.h, iptr(self._fin), len(self._fin), C.byref(nf))
         return self._fin[:nf.value].tolist()
 
-    def result(self, rid: int, n: int):
+    def result(self, rid: int, n: int, pressure=None):
         u = np.zeros(n)
         st = np.zeros(4, dtype=np.int32)
         tm = np.zeros(2)
-        self.lib.server_result(self.h, rid, dptr(u), iptr(st), dptr(tm))
+        self.lib.server_result(self.h, rid, dptr(u), dptr(pressure) if pressure is not None else None, iptr(st),
+                               dptr(tm))
         return u, st, tm
 
     def status(self):
@@ -97,10 +133,10 @@ def replay(server: Server, stream, keep_results=False):
             for rid in done:
                 q = pending.pop(rid)
                 r = stream[q]
-                u, st, tm = server.result(rid, r.side * r.side)
+                u, st, tm = server.result(rid, r.n)
                 out[q] = dict(latency=tnow - r.t, server_latency=float(tm[1]), queued=float(tm[0]),
-                              steps=int(st[0]), newton=int(st[1]), failed=bool(st[3]),
-                              cell_newton=int(st[1]) * r.side * r.side, u=u if keep_results else None)
+                              steps=int(st[0]), newton=int(st[1]), failed=bool(st[3]), twophase=r.twophase,
+                              cell_newton=int(st[1]) * r.n, u=u if keep_results else None)
         elif i < len(stream):
             time.sleep(max(0.0, min(stream[i].t - now, 0.002)))
     return out, time.perf_counter() - t0
@@ -108,12 +144,18 @@ def replay(server: Server, stream, keep_results=False):
 
 def summarize(stream, out, wall):
     lat = np.array([out[q]["latency"] for q in range(len(stream))])
-    return dict(requests=len(stream), wall_s=wall,
-                throughput_req_s=len(stream) / wall,
-                cell_newton_iters_per_s=sum(o["cell_newton"] for o in out.values()) / wall,
-                p50_ms=1e3 * float(np.percentile(lat, 50)), p99_ms=1e3 * float(np.percentile(lat, 99)),
-                mean_queue_ms=1e3 * float(np.mean([o["queued"] for o in out.values()])),
-                failed=int(sum(o["failed"] for o in out.values())))
+    res = dict(requests=len(stream), wall_s=wall,
+               throughput_req_s=len(stream) / wall,
+               cell_newton_iters_per_s=sum(o["cell_newton"] for o in out.values()) / wall,
+               p50_ms=1e3 * float(np.percentile(lat, 50)), p99_ms=1e3 * float(np.percentile(lat, 99)),
+               mean_queue_ms=1e3 * float(np.mean([o["queued"] for o in out.values()])),
+               failed=int(sum(o["failed"] for o in out.values())))
+    for kind, sel in (("heat2d", False), ("twophase3d", True)):
+        l = np.array([out[q]["latency"] for q in range(len(stream)) if stream[q].twophase == sel])
+        if l.size:
+            res[kind] = dict(requests=int(l.size), p50_ms=1e3 * float(np.percentile(l, 50)),
+                             p99_ms=1e3 * float(np.percentile(l, 99)))
+    return res
 
 
 def bench_serving(lib, args):
@@ -121,11 +163,12 @@ def bench_serving(lib, args):
     at the highest rate, latency percentiles per rate under `sweep`)."""
     rates = [float(r) for r in getattr(args, "rates", "50,100,200,500,1000,2000").split(",")]
     count = getattr(args, "requests", 200)
+    frac3d = getattr(args, "frac3d", 0.3)
     params = (f"precision=mixed preconditioner={args.precond} rel_res_tol=1e-8 abs_res_tol=0 max_iter=20 "
               f"solver_tol=1e-6 hbm_budget={getattr(args, 'hbm_budget_gb', 64.0) * 1e9}")
     sweep = []
     for rate in rates:
-        stream = poisson_stream(rate, count, seed=7)
+        stream = poisson_stream(rate, count, seed=7, frac3d=frac3d)
         srv = Server(lib, params)
         try:
             # warm the kernels (first launches) outside the measured stream
@@ -146,8 +189,9 @@ def bench_serving(lib, args):
             "unit": "cell-Newton-iters/s", "n_gpus": 1, "steps": count, "warmup": 1,
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
             "dtype": "f64 residual/Jacobian/refinement, f32 Krylov", "data": "synthetic",
-            "config": {"workload": "Poisson request stream of 2D nonlinear-heat solves, side log-uniform "
-                                   "16..1024, 1-20 steps, dt=1, Newton to 1e-8 relative residual",
+            "config": {"workload": f"Poisson request stream: {100 * (1 - frac3d):.0f}% 2D nonlinear heat (side "
+                                   f"log-uniform 16..1024), {100 * frac3d:.0f}% 3D two-phase (side log-uniform "
+                                   "16..128), 1-20 steps, dt=1, Newton to 1e-8 relative residual",
                        "requests_per_rate": count, "hbm_budget_gb": getattr(args, "hbm_budget_gb", 64.0)},
             "p50_ms": top["p50_ms"], "p99_ms": top["p99_ms"], "sweep": sweep}
     print(json.dumps(line), flush=True)

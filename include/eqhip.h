@@ -171,6 +171,24 @@ int eqhip_twophase_destroy(void* sim);
 int eqhip_batch_create(const char* params, void** batch);
 int eqhip_batch_destroy(void* batch);
 int eqhip_batch_add(void* batch, int nx, int ny, double k, double cv, const double* u, int* slot);
+/* Adds a two-phase instance (examples/dsl/twophase.equelle; generated form
+ * examples/app/out_twophase.cpp:26-113) to the batch: Cartesian nx x ny x nz cells
+ * (nz = 1: 2D), per-cell perm, poro, initial sw, rate sources on ascending cells.  params:
+ * dx dy dz watervisc oilvisc min_poro corey (1|2).  A step is a pressure Newton solve then
+ * a saturation Newton solve (twophase.equelle:99-113); pressure starts at 0.  Two-phase
+ * and heat instances share the batch's launches. */
+int eqhip_batch_add_twophase(void* batch, const char* params, int nx, int ny, int nz, const double* perm,
+                             const double* poro, const double* sw, int nsrc, const int* src_cells,
+                             const double* src_vals, int* slot);
+int eqhip_batch_twophase_state(void* batch, int slot, double* p, double* sw);
+/* stats = {pressure Newton iterations, pressure Krylov iterations, pressure stop reason};
+ * eqhip_batch_stats gives the totals of the step and the saturation solve's stop reason */
+int eqhip_batch_twophase_stats(void* batch, int slot, int* stats);
+/* pressure (stage 0) or transport (stage 1) residual and Jacobian (CSR, ascending columns)
+ * at (p, sw0, sw, dt); overwrites the slot's state; caller provides n, n+1, 7n, 7n entries */
+int eqhip_batch_twophase_residual(void* batch, int slot, int stage, const double* p, const double* sw0,
+                                  const double* sw, double dt, int* info, double* F, int* row_ptr, int* col_ind,
+                                  double* jac);
 int eqhip_batch_remove(void* batch, int slot);
 /* starts one implicit step of length dt for the slot (u0 := u) */
 int eqhip_batch_begin_step(void* batch, int slot, double dt);
@@ -200,8 +218,8 @@ int eqhip_batch_residual(void* batch, int slot, const double* u, const double* u
                          double* F, int* row_ptr, int* col_ind, double* jac);
 
 /* ------------------------------------------------------------------ request loop */
-/* Continuous batching of incoming solves.  A request = one nonlinear-heat instance
- * advanced by `steps` implicit steps of length dt.  Requests queue FIFO until their HBM
+/* Continuous batching of incoming solves.  A request = one nonlinear-heat or two-phase
+ * instance advanced by `steps` implicit steps of length dt.  Requests queue FIFO until their HBM
  * footprint fits `hbm_budget` (bytes); admitted requests share launches, one Newton
  * iteration per progress() tick, and keep their state device-resident between steps.
  * No reference counterpart (the reference runs one program per process,
@@ -211,11 +229,16 @@ int eqhip_server_create(const char* params, void** server);
 int eqhip_server_destroy(void* server);
 int eqhip_server_submit(void* server, int nx, int ny, double k, double cv, const double* u0, int steps, double dt,
                         int* request);
+/* a two-phase request (the instance of eqhip_batch_add_twophase, same params) */
+int eqhip_server_submit_twophase(void* server, const char* params, int nx, int ny, int nz, const double* perm,
+                                 const double* poro, const double* sw, int nsrc, const int* src_cells,
+                                 const double* src_vals, int steps, double dt, int* request);
 /* one scheduling tick; returns the requests that finished in it */
 int eqhip_server_progress(void* server, int* finished, int cap, int* nfinished);
-/* copies a finished request's field (nx*ny values) and releases it;
+/* copies a finished request's field (heat: u; two-phase: sw into u and, if p is not null,
+ * the pressure; one value per cell) and releases it;
  * stats = {steps, Newton, Krylov, failed}, times = {queued seconds, latency seconds} */
-int eqhip_server_result(void* server, int request, double* u, int* stats, double* times);
+int eqhip_server_result(void* server, int request, double* u, double* p, int* stats, double* times);
 /* counts = {queued, in flight}, bytes = {admitted footprint, budget} */
 int eqhip_server_status(void* server, int* counts, double* bytes);
 

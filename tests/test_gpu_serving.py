@@ -56,7 +56,7 @@ def test_mix_equals_alone_and_budget_respected(hip_api):
             assert used[0] <= used[1]
             for rid in srv.progress():
                 q = ids[rid]
-                u, st, tm = srv.result(rid, reqs[q].side ** 2)
+                u, st, tm = srv.result(rid, reqs[q].n)
                 assert st[0] == reqs[q].steps and st[3] == 0, st
                 assert 0 <= tm[0] <= tm[1]
                 results[q] = u
@@ -112,3 +112,54 @@ def test_poisson_replay_completes(hip_api):
         srv.close()
     assert len(out) == len(stream) and not any(o["failed"] for o in out.values())
     assert all(o["steps"] == stream[q].steps for q, o in out.items())
+
+
+def test_mixed_heat_and_twophase_requests(hip_api):
+    """3D two-phase and 2D heat requests in one continuous batch: each result equals the
+    same request run alone through the batch API, bit for bit."""
+    from equelle_amd.serving import TWOPHASE_INSTANCE, twophase_request
+    reqs = [Request(0.0, 50, 0.2, 2, gaussian_bumps(50, 50, 60, width_scale=0.03)),
+            twophase_request(0.0, 12, 2, seed=61),
+            Request(0.0, 90, 0.4, 3, gaussian_bumps(90, 90, 62, width_scale=0.05)),
+            twophase_request(0.0, 9, 3, seed=63)]
+    srv = Server(hip_api, PARAMS)
+    try:
+        ids = {}
+        results = {}
+        for q, r in enumerate(reqs):
+            ids[srv.submit(r)] = q
+            srv.progress()   # later requests join mid-flight
+        for _ in range(2000):
+            for rid in srv.progress():
+                q = ids[rid]
+                p = np.zeros(reqs[q].n)
+                u, st, _ = srv.result(rid, reqs[q].n, pressure=p)
+                assert st[0] == reqs[q].steps and st[3] == 0, st
+                results[q] = (u, p)
+            if len(results) == len(reqs):
+                break
+        assert len(results) == len(reqs)
+    finally:
+        srv.close()
+    for q, r in enumerate(reqs):
+        b = C.c_void_p()
+        hip_api.batch_create(PARAMS.encode(), C.byref(b))
+        try:
+            s = C.c_int()
+            if r.twophase:
+                hip_api.batch_add_twophase(b, TWOPHASE_INSTANCE.encode(), r.side, r.side, r.side, dptr(r.perm),
+                                           dptr(r.poro), dptr(r.u0), len(r.src_cells), iptr(r.src_cells),
+                                           dptr(r.src_vals), C.byref(s))
+            else:
+                hip_api.batch_add(b, r.side, r.side, r.k0, 1.0, dptr(r.u0), C.byref(s))
+            for _ in range(r.steps):
+                hip_api.batch_step(b, iptr(np.array([s.value], dtype=np.int32)), 1, r.dt)
+            u = np.zeros(r.n)
+            hip_api.batch_state(b, s.value, dptr(u))
+            assert np.array_equal(results[q][0], u), q
+            if r.twophase:
+                p = np.zeros(r.n)
+                hip_api.batch_twophase_state(b, s.value, dptr(p), None)
+                assert np.array_equal(results[q][1], p), q
+        finally:
+            hip_api.batch_destroy(b)

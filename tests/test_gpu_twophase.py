@@ -85,3 +85,124 @@ def test_twophase_steps_match_oracle(hip_api, oracle_api, extra, newton_tol, sol
     assert np.allclose(sh, so, rtol=bar, atol=0.1 * bar)
     dp_h, dp_o = ph - ph[-1], po - po[-1]
     assert np.allclose(dp_h, dp_o, rtol=bar, atol=bar * np.abs(dp_o).max())
+
+
+# ---------------------------------------------------------------- fused batch engine
+def fused_batch(api, params):
+    b = C.c_void_p()
+    api.batch_create(params.encode(), C.byref(b))
+    return b
+
+
+TP_INST = "watervisc=0.001 oilvisc=0.005 corey=2"
+
+
+def fused_add(api, b, nx, ny, nz, perm, poro, sw0, sc, sv):
+    slot = C.c_int()
+    api.batch_add_twophase(b, TP_INST.encode(), nx, ny, nz, dptr(perm), dptr(poro), dptr(sw0), len(sc), iptr(sc),
+                           dptr(sv), C.byref(slot))
+    return slot.value
+
+
+@pytest.mark.parametrize("dims", [(9, 7, 5), (16, 12, 3), (11, 6, 1)])
+@pytest.mark.parametrize("stage", [0, 1])
+def test_fused_twophase_residual_matches_oracle(hip_api, oracle_api, dims, stage):
+    nx, ny, nz = dims
+    perm, poro, sw0, sc, sv = case(nx, ny, nz, seed=nx + nz)
+    n = perm.size
+    rng = np.random.default_rng(3 * nx + stage)
+    p, s0, sw = rng.uniform(0, 1, n), rng.uniform(0, 1, n), rng.uniform(0.05, 0.95, n)
+    b = fused_batch(hip_api, "precision=fp64 preconditioner=ilu0")
+    try:
+        slot = fused_add(hip_api, b, nx, ny, nz, perm, poro, sw0, sc, sv)
+        info = np.zeros(2, dtype=np.int32)
+        F, rp = np.zeros(n), np.zeros(n + 1, dtype=np.int32)
+        ci, v = np.zeros(7 * n, dtype=np.int32), np.zeros(7 * n)
+        hip_api.batch_twophase_residual(b, slot, stage, dptr(p), dptr(s0), dptr(sw), 0.8, iptr(info), dptr(F),
+                                        iptr(rp), iptr(ci), dptr(v))
+    finally:
+        hip_api.batch_destroy(b)
+    params = twophase_params(nx, ny, nz)
+    if nz == 1:   # a 2D grid for the oracle
+        params = params.replace("grid_dim=3", "grid_dim=2")
+    params = params.encode()
+    Fo, rpo, cio, vo = oracle_residual(oracle_api, params, perm, poro, sc, sv, stage, p, s0, sw, 0.8)
+    nnz = info[1]
+    assert nnz == len(cio) and np.array_equal(rp, rpo) and np.array_equal(ci[:nnz], cio)
+    assert np.allclose(F, Fo, rtol=1e-10, atol=1e-12 * np.abs(Fo).max())
+    assert np.allclose(v[:nnz], vo, rtol=1e-10, atol=1e-12 * np.abs(vo).max())
+
+
+# BiCGStab on the singular (no-flow) pressure system stagnates well above rounding, so
+# the solves run to 1e-10 (FP64, and FP32 with Jacobi) and the results are compared at
+# 1e-7; FP32 with ILU(0) runs at the benchmark's tolerances (Newton 1e-8, linear 1e-6),
+# compared at 1e-5.
+@pytest.mark.parametrize("params,newton_tol,solver_tol,bar", [
+    ("precision=fp64 preconditioner=ilu0", 1e-10, 1e-10, 1e-7),
+    ("precision=fp64 preconditioner=diagonal", 1e-10, 1e-10, 1e-7),
+    ("precision=mixed preconditioner=diagonal", 1e-10, 1e-10, 1e-7),
+    ("precision=mixed preconditioner=ilu0", 1e-8, 1e-6, 1e-5)])
+def test_fused_twophase_steps_match_oracle(hip_api, oracle_api, params, newton_tol, solver_tol, bar):
+    nx, ny, nz = 12, 10, 6
+    perm, poro, sw0, sc, sv = case(nx, ny, nz)
+    n = perm.size
+    dts = [1.0, 1.0, 2.0]
+    po, so, _ = oracle_run(oracle_api, twophase_params(nx, ny, nz, rel_res_tol=1e-13, max_iter=40).encode(),
+                           perm, poro, sw0, sc, sv, dts)
+    b = fused_batch(hip_api, params + f" rel_res_tol={newton_tol} abs_res_tol=0 solver_tol={solver_tol} "
+                                      "max_iter=40 solver_max_iter=5000")
+    try:
+        slot = fused_add(hip_api, b, nx, ny, nz, perm, poro, sw0, sc, sv)
+        st = np.zeros(5, dtype=np.int32)
+        tps = np.zeros(3, dtype=np.int32)
+        for dt in dts:
+            hip_api.batch_step(b, iptr(np.array([slot], dtype=np.int32)), 1, dt)
+            hip_api.batch_stats(b, slot, iptr(st), None)
+            hip_api.batch_twophase_stats(b, slot, iptr(tps))
+            assert st[3] in (1, 2) and tps[2] in (1, 2) and st[4] == 0, (st, tps)
+        ph, sh = np.zeros(n), np.zeros(n)
+        hip_api.batch_twophase_state(b, slot, dptr(ph), dptr(sh))
+    finally:
+        hip_api.batch_destroy(b)
+    assert np.allclose(sh, so, rtol=bar, atol=0.1 * bar)
+    assert np.allclose(ph - ph[-1], po - po[-1], rtol=bar, atol=bar * np.abs(po).max())
+
+
+def test_fused_twophase_and_heat_share_a_batch(hip_api):
+    """Two-phase and heat instances in one batch give bit-identical results to each run
+    alone (per-instance reductions, stage-independent launches)."""
+    from equelle_amd.workloads import gaussian_bumps
+    params = "precision=mixed preconditioner=ilu0 rel_res_tol=1e-8 solver_tol=1e-6 max_iter=30"
+    tp = [(10, 9, 7), (20, 15, 4)]
+    heat = [(64, 40, 0.2)]
+
+    def run(which):
+        b = fused_batch(hip_api, params)
+        try:
+            slots = []
+            for q, (nx, ny, nz) in enumerate(tp):
+                if which in ("all", f"tp{q}"):
+                    perm, poro, sw0, sc, sv = case(nx, ny, nz, seed=q)
+                    slots.append(("tp", nx * ny * nz, fused_add(hip_api, b, nx, ny, nz, perm, poro, sw0, sc, sv)))
+            for q, (nx, ny, k) in enumerate(heat):
+                if which in ("all", f"heat{q}"):
+                    s = C.c_int()
+                    hip_api.batch_add(b, nx, ny, k, 1.0, dptr(gaussian_bumps(nx, ny, 4, width_scale=0.05)),
+                                      C.byref(s))
+                    slots.append(("heat", nx * ny, s.value))
+            arr = np.array([s for _, _, s in slots], dtype=np.int32)
+            for dt in (1.0, 0.5):
+                hip_api.batch_step(b, iptr(arr), len(arr), dt)
+            out = []
+            for kind, n, s in slots:
+                u = np.zeros(n)
+                hip_api.batch_state(b, s, dptr(u))
+                out.append(u)
+            return out
+        finally:
+            hip_api.batch_destroy(b)
+
+    together = run("all")
+    alone = run("tp0") + run("tp1") + run("heat0")
+    for a, c in zip(together, alone):
+        assert np.array_equal(a, c)
