@@ -44,7 +44,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per cell and launch of the dominant kernel at 2048^2, from rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE per dispatch scaled by a copy kernel of known size (reads
 # x2.0, writes x1.0); profiles/heat2048_fused_hbm_traffic.md has the counts and commands.
-SPMV_TRAFFIC_BYTES_PER_CELL = 33.1
+TRAFFIC_BYTES_PER_CELL = {}
 
 
 def log(msg):
@@ -74,7 +74,7 @@ def run_fused(lib, precision, u0, nx, ny, dt, steps, warmup, precond="ilu0", gra
             lib.batch_stats(b, slot.value, iptr(st), dptr(nrm))
             log(f"  warmup {w}: newton={st[0]} krylov={st[1]} refine={st[2]} {time.perf_counter() - t:.4f}s")
         lib.device_sync()
-        newton = krylov = 0
+        newton = krylov = refine = 0
         t0 = time.perf_counter()
         for s in range(steps):
             t = time.perf_counter()
@@ -84,16 +84,23 @@ def run_fused(lib, precision, u0, nx, ny, dt, steps, warmup, precond="ilu0", gra
                 raise RuntimeError(f"step {s} did not converge: stats {st.tolist()} norms {nrm.tolist()}")
             newton += int(st[0])
             krylov += int(st[1])
+            refine += int(st[2])
             log(f"  step {s}: newton={st[0]} krylov={st[1]} refine={st[2]} rel={nrm[1] / nrm[0]:.2e} "
                 f"{time.perf_counter() - t:.4f}s")
         lib.device_sync()
         secs = time.perf_counter() - t0
-        # kernel durations on the final state: HIP events on the runtime stream around
-        # 50 back-to-back launches of each stencil kernel over the whole grid
+        # kernel durations on the final state, every instance active: 50 launches of each
+        # stencil kernel over the whole grid, each between its own HIP start/stop events on
+        # the runtime stream (avg), then 50 more back to back (dispatch gaps included)
         shape = np.zeros(3, dtype=np.int32)
         lib.batch_shape(b, iptr(shape))
-        kern = {"shape": (int(shape[0]), int(shape[1]))}
-        for name in ("stencil_spmv", "assemble", "ilu_black", "ilu_red", "defect"):
+        cls = np.zeros(5, dtype=np.int32)
+        lib.batch_shape_classes(b, iptr(cls))
+        kern = {"shape": (int(shape[0]), int(shape[1])),
+                "cpt": dict(zip(("assemble", "defect", "spmv", "precond", "vector"), map(int, cls)))}
+        names = (("operator_red", "operator_black") if precond == "ilu0" else ("spmv",)) + \
+            ("assemble",) + (("defect",) if precision == "mixed" else ()) + ("b1", "b2", "b3", "load_rhs")
+        for name in names:
             ms, ev = C.c_double(), C.c_double()
             lib.batch_kernel_time(b, name.encode(), 50, C.byref(ms), C.byref(ev))
             kern[name] = (50, ms.value)
@@ -102,7 +109,7 @@ def run_fused(lib, precision, u0, nx, ny, dt, steps, warmup, precond="ilu0", gra
         lib.batch_state(b, slot.value, dptr(u))
     finally:
         lib.batch_destroy(b)
-    return dict(seconds=secs, newton=newton, krylov=krylov, kern=kern, u=u)
+    return dict(seconds=secs, newton=newton, krylov=krylov, refine=refine, kern=kern, u=u)
 
 
 def run_generic(lib, u0, nx, ny, dt, steps, warmup):
@@ -343,6 +350,51 @@ def bench_copy(args):
                       "bytes_written_per_launch": b.numel() * 4, "launches": args.warmup + args.steps}), flush=True)
 
 
+# Algorithmic bytes per launch over n cells (h = (n + 1) / 2 pair ranks; ks = bytes of the
+# Krylov precision): every array the kernel must touch, once (stencil neighbours are
+# re-reads of the same array).  Launches per timed region from the iteration counts:
+# every BiCGStab iteration applies the reduced operator twice (both passes) and the three
+# vector updates once; every refinement round loads a right-hand side and runs a defect;
+# every Newton iteration and every step start assembles.
+KERNELS = {
+    "operator_red": ("k_schur_red<T, {cpt}>", "spmv", lambda n, h, ks: 7 * ks * h,
+                     "red ranks: 4 coefficients, 1/C_r, v_b, y_r written (7 ks h)"),
+    "operator_black": ("k_schur_black<T, {cpt}>", "spmv", lambda n, h, ks: 9 * ks * h,
+                       "black ranks: C_b, 4 coefficients, v_b, y_r, dot operand, w written (9 ks h)"),
+    "spmv": ("k_spmv_rb<T, {cpt}>", "spmv", lambda n, h, ks: 2 * 9 * ks * h,
+             "both colours: C, 4 coefficients, x, dot operand, y written (18 ks h)"),
+    "assemble": ("k_assemble<{mixed}, {cpt}>", "assemble", lambda n, h, ks: 24 * n + 5 * ks * 2 * h,
+                 "u, u0 read, F written (FP64), 5 coefficients written (Krylov precision)"),
+    "defect": ("k_defect<{cpt}>", "defect", lambda n, h, ks: 32 * n, "u, du, F read, r written (FP64)"),
+    "b1": ("k_b1<T, {cpt}>", "vector", lambda n, h, ks: 6 * ks * h, "r, p, v, 1/P read; p, ph written"),
+    "b2": ("k_b2<T, {cpt}>", "vector", lambda n, h, ks: 5 * ks * h, "r, v, 1/P read; s, sh written"),
+    "b3": ("k_b3<T, {cpt}>", "vector", lambda n, h, ks: 8 * ks * h, "x, ph, sh, s, t, rh read; x, r written"),
+    "load_rhs": ("k_load_rhs<T, {cpt}>", "precond", lambda n, h, ks: 8 * n + 6 * ks * h,
+                 "r64 read (FP64), 4 black coefficients, 1/C_r read, g written"),
+}
+
+
+def kernel_table(run, n, ks, steps):
+    h = (n + 1) // 2
+    kern = run["kern"]
+    per = {"operator_red": 2 * run["krylov"], "operator_black": 2 * run["krylov"], "spmv": 2 * run["krylov"],
+           "b1": run["krylov"], "b2": run["krylov"], "b3": run["krylov"], "assemble": run["newton"] + steps,
+           "defect": run["refine"], "load_rhs": run["refine"] if ks == 4 else run["newton"]}
+    out = {}
+    for name, (kname, cls, fb, text) in KERNELS.items():
+        if name not in kern:
+            continue
+        _, ms = kern[name]
+        bytes_ = fb(n, h, ks)
+        gbs = bytes_ / (ms * 1e-3) / 1e9
+        out[name] = {"kernel": kname.format(cpt=kern["cpt"][cls], mixed=str(ks == 4).lower()).replace(
+                         "T", "float" if ks == 4 else "double"),
+                     "avg_us": 1e3 * ms, "back_to_back_us": 1e3 * kern[name + "_event"],
+                     "algorithmic_bytes": bytes_, "bytes": text, "GBs": gbs, "frac": gbs / HBM_PEAK_GBS,
+                     "us_per_step": 1e3 * ms * per[name] / steps}
+    return out
+
+
 def cpu_baseline(n_side, u0, dt):
     """Serial oracle: one Newton iteration (AD assembly + BiCGStab/ILU(0) to the same
     relative tolerance) on the same grid and state."""
@@ -404,17 +456,8 @@ def main():
     head = run_fused(lib, "mixed", u0, nx, ny, dt, args.steps, args.warmup, args.precond, not args.no_graph,
                      args.inner_tol)
     value = n * head["newton"] / head["seconds"]
-    cnt, avg_ms = head["kern"]["stencil_spmv"]
-    spmv_bytes = 32 * n   # 5 FP32 coefficients + x + dot operand read, y written
-    achieved = spmv_bytes / (avg_ms * 1e-3) / 1e9
-    acnt, aavg = head["kern"]["assemble"]
-    asm_bytes = 84 * n   # read u, u0; write F, 5 FP64 + 5 FP32 coefficients
-    _, ibavg = head["kern"]["ilu_black"]
-    _, iravg = head["kern"]["ilu_red"]
-    iavg = ibavg + iravg
-    ilu_bytes = 28 * n   # both passes: read r, dinv, 4 off-diagonal FP32 coefficients; write z
-    dcnt, davg = head["kern"]["defect"]
-    def_bytes = 64 * n   # read 5 FP64 coefficients, du, F; write r
+    kt = kernel_table(head, n, 4, args.steps)
+    dom = max(kt, key=lambda k: kt[k]["us_per_step"])
     out = {
         "metric": "cell-Newton-iters/s",
         "value": value,
@@ -432,23 +475,17 @@ def main():
                                "relative residual; mixed precision (FP32 BiCGStab + red-black ILU(0), FP64 "
                                "refinement to 1e-6)",
                    "cells": n, "jacobian_nnz": nnz, "engine": "fused heat engine (1 instance)"},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": SPMV_TRAFFIC_BYTES_PER_CELL * n if (nx, ny) == (2048, 2048) else None,
-                     "kernel": "k_spmv_dot<float> (5-point stencil SpMV + fused dot partials)", "launches": cnt, "avg_us": 1e3 * avg_ms,
-                     "event_us": 1e3 * head["kern"]["stencil_spmv_event"], "algorithmic_bytes": spmv_bytes},
-        "assembly_roofline": {"kernel": "k_assemble<true>", "launches": acnt, "avg_us": 1e3 * aavg,
-                              "algorithmic_bytes": asm_bytes,
-                              "achieved_GBs": asm_bytes / (aavg * 1e-3) / 1e9,
-                              "frac": asm_bytes / (aavg * 1e-3) / 1e9 / HBM_PEAK_GBS},
-        "other_kernels": {
-            "k_ilu_black<float> + k_ilu_red<float>": {"avg_us": 1e3 * iavg, "black_us": 1e3 * ibavg,
-                                                      "red_us": 1e3 * iravg, "algorithmic_bytes": ilu_bytes,
-                                                      "frac": ilu_bytes / (iavg * 1e-3) / 1e9 / HBM_PEAK_GBS},
-            "k_defect": {"avg_us": 1e3 * davg, "algorithmic_bytes": def_bytes,
-                         "frac": def_bytes / (davg * 1e-3) / 1e9 / HBM_PEAK_GBS}},
-        "launch_shape": {"cells_per_thread": head["kern"]["shape"][0], "xcd_block_order": head["kern"]["shape"][1],
-                         "chosen_by": "launch-shape tuner (timed per batch membership)"},
+        "roofline": {"bound": "hbm", "achieved": kt[dom]["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": kt[dom]["frac"],
+                     "traffic": (TRAFFIC_BYTES_PER_CELL[dom] * n
+                                 if (nx, ny) == (2048, 2048) and dom in TRAFFIC_BYTES_PER_CELL else None),
+                     "kernel": kt[dom]["kernel"], "launches": 50, "avg_us": kt[dom]["avg_us"],
+                     "back_to_back_us": kt[dom]["back_to_back_us"],
+                     "algorithmic_bytes": kt[dom]["algorithmic_bytes"], "bytes_formula": kt[dom]["bytes"]},
+        "assembly_roofline": kt["assemble"],
+        "kernels": kt,
+        "launch_shape": {"cells_per_thread": head["kern"]["cpt"], "xcd_block_order": head["kern"]["shape"][1],
+                         "chosen_by": "launch-shape tuner (each kernel class timed per batch membership)"},
         "newton_iterations": head["newton"],
         "krylov_iterations": head["krylov"],
         "cell_steps_per_s": n * args.steps / head["seconds"],
@@ -457,10 +494,10 @@ def main():
         log("[bench] plain FP64 (fused engine)")
         p = run_fused(lib, "fp64", u0, nx, ny, dt, args.steps, args.warmup, args.precond, not args.no_graph)
         pv = n * p["newton"] / p["seconds"]
-        c64, a64 = p["kern"]["stencil_spmv"]
+        kt64 = kernel_table(p, n, 8, args.steps)
         out["plain_fp64"] = {"value": pv, "ms_per_step": 1e3 * p["seconds"] / args.steps,
                              "newton_iterations": p["newton"], "krylov_iterations": p["krylov"],
-                             "spmv_avg_us": 1e3 * a64, "spmv_GBs": 64 * n / (a64 * 1e-3) / 1e9}
+                             "kernels": {k: {"avg_us": v["avg_us"], "frac": v["frac"]} for k, v in kt64.items()}}
         out["mixed_over_plain"] = value / pv
         out["mixed_vs_fp64_rel_error"] = float(np.linalg.norm(head["u"] - p["u"]) / np.linalg.norm(p["u"]))
         log("[bench] generic drop-in path (EquelleRuntimeCUDA AD engine, FP64)")

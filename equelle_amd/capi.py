@@ -90,6 +90,7 @@ SIGNATURES = {
     "batch_bytes": [_P, _I, _DP],
     "batch_kernel_time": [_P, _S, _I, _DP, _DP],
     "batch_shape": [_P, _IP],
+    "batch_shape_classes": [_P, _IP],
     "batch_residual": [_P, _I, _DP, _DP, _D, _IP, _DP, _IP, _IP, _DP],
     "twophase_create": [_S, _DP, _DP, _DP, _I, _IP, _DP, _I, C.POINTER(_P)],
     "twophase_step": [_P, _D, _IP, _DP],

@@ -748,6 +748,12 @@ int eqhip_batch_shape(void* batch, int* info) {
         info[2] = sh.signatures;
     });
 }
+int eqhip_batch_shape_classes(void* batch, int* cpt) {
+    return guard([&] {
+        LaunchShape sh = B(batch).launch_shape();
+        for (int c = 0; c < 5; ++c) cpt[c] = sh.cpt_class[c];
+    });
+}
 int eqhip_batch_kernel_time(void* batch, const char* kernel, int reps, double* avg_ms, double* event_ms) {
     return guard([&] { *avg_ms = B(batch).time_kernel(kernel ? kernel : "", reps, event_ms); });
 }

@@ -20,7 +20,8 @@ constexpr int BS = 256;   // cells per block (4 wave64)
 inline hipStream_t S() { return Device::get().stream(); }
 
 enum Slot { S_RHO = 0, S_RHO_OLD, S_ALPHA, S_OMEGA, S_RR, S_BB, S_BREAK, S_IT, S_FN, S_RN, S_BN, S_REF, S_ITT, NSLOT = 16 };
-enum Vec { V_R = 0, V_RH, V_P, V_V, V_S, V_T, V_PH, V_SH, V_X, NVEC };
+// Krylov vectors; V_Y is the red-rank scratch of the reduced operator (ilu0 only)
+enum Vec { V_R = 0, V_RH, V_P, V_V, V_S, V_T, V_PH, V_SH, V_X, V_Y, NVEC };
 enum Act { A_INIT = 0, A_ALPHA, A_OMEGA, A_RHO, A_STORE };
 enum Pc { PC_NONE = 0, PC_JACOBI = 1, PC_ILU = 2 };
 
@@ -30,7 +31,11 @@ struct Desc {
     int n, nx, ny, nz;
     int xcd;                      // XCD-contiguous block order (same for all of a table)
     int kind;                     // K_HEAT | K_TWOPHASE
-    int corey, pad;
+    int corey;
+    int gbase, ngrp;              // the instance's partials: groups [gbase, gbase + ngrp) of BS cells
+    int h;                        // pair ranks (n + 1) / 2: cells per colour, rounded up
+    int ns;                       // Krylov slots per pair rank: 1 (reduced system) or 2 (full system)
+    int ngrpk;                    // Krylov partials: groups [gbase, gbase + ngrpk) of BS pairs
     double rnx, rnxy;             // 1 / nx, 1 / (nx ny): cell -> (i, j, k) without integer division
     double dt;
     // heat: k(u) = k (1 + u^2), x / y face geometry factors, cell volume
@@ -42,18 +47,22 @@ struct Desc {
     const double *tx, *ty, *tz, *pv, *src;
     double *p, *sw, *sw0;
     double *F, *du, *r64;
-    double *C, *W, *E, *S, *N, *B, *T;   // FP64 Jacobian stencil coefficients (B, T: 3D only)
-    void *Cm, *Wm, *Em, *Sm, *Nm, *Bm, *Tm; // Krylov-precision copies (alias the FP64 ones in fp64 mode)
-    void* dinv;                   // Jacobi: 1/C ; ILU: 1/pivot (black) or 1/C (red)
+    // FP64 Jacobian stencil coefficients in cell order (B, T: 3D only), stored only for
+    // two-phase instances in mixed precision (the refinement defect reads them); a heat
+    // instance recomputes its FP64 rows from u.  Null otherwise.
+    double *C, *W, *E, *S, *N, *B, *T;
+    // Krylov-precision coefficients C, W, E, S, N, B, T in red-black rank order (2 h each)
+    void* K[7];
+    void* dinv;                   // 2 h: red 1/C_r; black 1/pivot (ilu0), 1/C_b (jacobi), 1 (none)
     void* v[NVEC];                // Krylov vectors
     double* sc;                   // NSLOT scalars
     int* act;                     // [0] Krylov active, [1] refinement active, [2] two-phase stage
-    unsigned long long* ts;       // kernel timing only (else null): per block {start, end}
 };
 
 // What the last block of an instance does with the summed partials.
 struct Fin {
     int act, nd, maxit, slot;
+    int kry;                      // partials of a Krylov kernel (groups of pairs) or of a cell kernel
     double tol2;
 };
 
@@ -75,16 +84,18 @@ constexpr int FBS = 1024;   // finaliser block: one per instance, partials read 
 
 // Launch shape.  Every cell kernel is instantiated for CPT (cells per thread) in
 // {1, 2, 4, 8}: a block covers CPT*BS consecutive cells of one instance and thread t
-// handles cells base + t, base + BS + t, ... (coalesced).  The shape of a batch
-// membership is chosen by measurement (see Impl::shape_for).  Reductions are made
-// shape-independent: a block writes one partial per group of BS cells (group m =
-// cells [m BS, (m+1) BS) of the block, thread t holds cell t of each group) with the
-// same tree for every shape, so results are bit-identical for any CPT and block order.
+// handles cells base + t, base + BS + t, ... (coalesced).  Each kernel class (assembly,
+// defect, SpMV, preconditioner, vector updates) has its own CPT, chosen by measurement
+// (see Impl::shape_for).  Reductions are shape-independent: a block writes one partial
+// per group of BS cells (group m = cells [m BS, (m+1) BS) of the block, thread t holds
+// cell t of each group) with the same tree for every shape, into the instance's range of
+// groups [gbase, gbase + ceil(n / BS)), which does not depend on CPT either; results are
+// bit-identical for any CPT and block order.
 // Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2), so
 // workgroup w runs on XCD w % 8.  With xcd set, the logical block (position in cell
 // order) of workgroup w is chosen so that each XCD processes one contiguous range of
 // blocks: stencil neighbours (rows above and below) then hit the same L2.  Bijective
-// for any block count; partials are stored at the logical index.
+// for any block count.
 constexpr int NXCD = 8;
 __device__ inline int xcd_block(int w, int nblocks) {
     const int q = nblocks / NXCD, r = nblocks % NXCD, x = w % NXCD;
@@ -92,15 +103,14 @@ __device__ inline int xcd_block(int w, int nblocks) {
 }
 #define LOCATE()                                                                                    \
     const int lb = D[0].xcd ? xcd_block((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x;         \
-    if (D[0].ts && threadIdx.x == 0) D[0].ts[2 * lb] = wall_clock64();                              \
     const int2 bi = blk[lb];                                                                        \
     const Desc& d = D[bi.x];                                                                        \
     const int base = bi.y * (CPT * BS) + (int)threadIdx.x
-// End stamp of a timed launch: the latest wave of the block (the kernel's execution
-// span is min(start) .. max(end) over blocks, on the constant-rate wall clock).
-#define KTIME_END()                                                                                 \
-    if (D[0].ts && (threadIdx.x & 63) == 0) atomicMax(D[0].ts + 2 * lb + 1, wall_clock64())
+// This block's partials of the instance (pointer, number of its groups that hold cells).
+#define PARTS(p) (p) + d.gbase + bi.y * CPT, d.ngrp - bi.y * CPT
+#define PARTSK(p) (p) + d.gbase + bi.y * CPT, d.ngrpk - bi.y * CPT
 #define FOR_CELLS(c) for (int c = base, c##_end = min(d.n, bi.y * (CPT * BS) + CPT * BS); c < c##_end; c += BS)
+#define FOR_PAIRS(p) for (int p = base, p##_end = min(d.h, bi.y * (CPT * BS) + CPT * BS); p < p##_end; p += BS)
 #define FOR_GROUPS(m, c) for (int m = 0, c = base; m < CPT; ++m, c += BS)
 
 // One partial per group: v[m] is this thread's value for group m (0 when it has none).
@@ -110,8 +120,9 @@ __device__ inline int xcd_block(int w, int nblocks) {
 // per lane is left the usual butterfly finishes.  Every group is summed over the same
 // lane pairs in the same order as a plain shuffle tree (addition is commutative), so
 // the partial is bit-identical for every CPT.  Group g ends in lane g * 64 / CPT.
+// Only the first `lim` groups (those holding cells of the instance) are stored.
 template <int CPT>
-__device__ inline void group_store(const double (&v)[CPT], double* out) {
+__device__ inline void group_store(const double (&v)[CPT], double* out, int lim) {
     __shared__ double lds[CPT][BS / 64];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     double a[CPT];
@@ -133,7 +144,7 @@ __device__ inline void group_store(const double (&v)[CPT], double* out) {
     constexpr int SEG = 64 / CPT;
     if (lane % SEG == 0) lds[lane / SEG][w] = x;
     __syncthreads();
-    if ((int)threadIdx.x < CPT) {
+    if ((int)threadIdx.x < CPT && (int)threadIdx.x < lim) {
         double s = 0.0;
         for (int i = 0; i < BS / 64; ++i) s += lds[threadIdx.x][i];
         out[threadIdx.x] = s;
@@ -167,7 +178,8 @@ __device__ inline void cell_ijk(const Desc& d, int c, int& i, int& j, int& k) {
 // Per-instance finalisation, run by one whole block: sums the instance's block
 // partials in a fixed order (deterministic), then thread 0 applies the action to
 // the instance's scalars.
-__device__ void fin_body(const Desc& d, int2 r, const double* part, int nb, const Fin& f) {
+__device__ void fin_body(const Desc& d, const double* part, int nb, const Fin& f) {
+    const int2 r = make_int2(d.gbase, d.gbase + (f.kry ? d.ngrpk : d.ngrp));
     __shared__ double res[2];
     // partials were written by the previous kernel (visible at this kernel's start);
     // four independent sums per thread keep four loads in flight
@@ -235,9 +247,9 @@ __device__ void fin_body(const Desc& d, int2 r, const double* part, int nb, cons
 }
 
 
-__global__ __launch_bounds__(FBS) void k_fin(const Desc* __restrict__ D, const int2* __restrict__ rng,
-                                            const double* __restrict__ part, int nb, Fin f) {
-    fin_body(D[blockIdx.x], rng[blockIdx.x], part, nb, f);
+__global__ __launch_bounds__(FBS) void k_fin(const Desc* __restrict__ D, const double* __restrict__ part, int nb,
+                                            Fin f) {
+    fin_body(D[blockIdx.x], part, nb, f);
 }
 
 // ---------------------------------------------------------------------------
@@ -332,6 +344,55 @@ __device__ inline double tp_transport(const Desc& d, int c, int i, int j, int k,
     return (swc - d.sw0[c]) + a * (div - q);
 }
 
+// Heat residual and Jacobian row at cell (i, j) from u at the cell (uc) and its four
+// neighbours (values of absent neighbours are not used): F = u - u0 + dt/(cv vol) div
+// with div = sum_faces g k_f (u_c - u_m), k_f = (k(u_c) + k(u_m)) / 2, k(u) = k (1 + u^2);
+// co = {C, W, E, S, N} = dF/du at the cell and the neighbours (0 where absent).  Faces in
+// ascending face-id order: west (x-), east (x+), south (y-), north (y+).  The refinement
+// defect recomputes rows with this same function, so its FP64 Jacobian is bit-identical
+// to the one the assembly derives its Krylov-precision coefficients from.
+__device__ inline double heat_row(const Desc& d, int i, int j, double uc, double uw, double ue, double us,
+                                  double un, double u0c, double (&co)[5]) {
+    const double k = d.k, coef = d.dt / (d.cv * d.vol);
+    const double kc = k * (1.0 + uc * uc), dkc = 2.0 * k * uc;
+    double div = 0.0, diag = 1.0;
+    co[1] = co[2] = co[3] = co[4] = 0.0;
+    auto face = [&](double um, double g, double& cm) {
+        const double km = k * (1.0 + um * um);
+        const double kf = 0.5 * (kc + km);
+        const double diff = uc - um;
+        div += g * kf * diff;
+        diag += coef * g * (0.5 * dkc * diff + kf);
+        cm = coef * g * (0.5 * (2.0 * k * um) * diff - kf);
+    };
+    if (i > 0) face(uw, d.gx, co[1]);
+    if (i < d.nx - 1) face(ue, d.gx, co[2]);
+    if (j > 0) face(us, d.gy, co[3]);
+    if (j < d.ny - 1) face(un, d.gy, co[4]);
+    co[0] = diag;
+    return (uc - u0c) + coef * div;
+}
+
+// u at cell c and its four neighbours (the cell's own value where a neighbour is absent,
+// so every load is in bounds and issued unconditionally).
+struct Nb5 {
+    double c, w, e, s, n;
+};
+__device__ inline Nb5 load_nb5(const double* __restrict__ u, int c, int i, int j, int nx, int ny) {
+    Nb5 r;
+    r.c = u[c];
+    r.w = u[i > 0 ? c - 1 : c];
+    r.e = u[i < nx - 1 ? c + 1 : c];
+    r.s = u[j > 0 ? c - nx : c];
+    r.n = u[j < ny - 1 ? c + nx : c];
+    return r;
+}
+
+// Residual + Jacobian assembly and per-group partial ||F||^2.
+//   heat:      reads u (5-point), u0; writes F and the 5 coefficients in the Krylov
+//              precision (FP32 mixed, FP64 fp64), red-black rank order
+//   two-phase: writes F, the Krylov-precision coefficients and (MIXED) the FP64 ones in
+//              cell order for the refinement defect
 template <bool MIXED, int CPT>
 __global__ __launch_bounds__(BS) void k_assemble(const Desc* __restrict__ D, const int2* __restrict__ blk,
                                                  double* __restrict__ part) {
@@ -346,234 +407,284 @@ __global__ __launch_bounds__(BS) void k_assemble(const Desc* __restrict__ D, con
             double co[7] = {0, 0, 0, 0, 0, 0, 0};
             const double F = stage == 0 ? tp_pressure(d, c, i, j, k, co) : tp_transport(d, c, i, j, k, co);
             d.F[c] = F;
-            double* dst[7] = {d.C, d.W, d.E, d.S, d.N, d.B, d.T};
-            for (int q = 0; q < (d.nz > 1 ? 7 : 5); ++q) dst[q][c] = co[q];
+            const int nq = d.nz > 1 ? 7 : 5;
             if (MIXED) {
-                float* dm[7] = {(float*)d.Cm, (float*)d.Wm, (float*)d.Em, (float*)d.Sm,
-                                (float*)d.Nm, (float*)d.Bm, (float*)d.Tm};
-                for (int q = 0; q < (d.nz > 1 ? 7 : 5); ++q) dm[q][c] = (float)co[q];
+                double* dst[7] = {d.C, d.W, d.E, d.S, d.N, d.B, d.T};
+                for (int q = 0; q < nq; ++q) dst[q][c] = co[q];
             }
+            using K = typename std::conditional<MIXED, float, double>::type;
+            const int sl = ((i + j + k) & 1) * d.h + (c >> 1);
+            for (int q = 0; q < nq; ++q) ((K*)d.K[q])[sl] = (K)co[q];
             f2[m] = F * F;
         }
-        group_store<CPT>(f2, part + lb * CPT);
-        KTIME_END();
+        group_store<CPT>(f2, PARTS(part));
         return;
     }
+    using K = typename std::conditional<MIXED, float, double>::type;
     const int nx = d.nx, ny = d.ny;
-    const double k = d.k, coef = d.dt / (d.cv * d.vol), gx = d.gx, gy = d.gy;
     const double* __restrict__ u = d.u;
+    const double* __restrict__ u0 = d.u0;
+    double* __restrict__ Fo = d.F;
+    K* __restrict__ co_out[5] = {(K*)d.K[0], (K*)d.K[1], (K*)d.K[2], (K*)d.K[3], (K*)d.K[4]};
     FOR_GROUPS(m, c) {
         if (c >= d.n) continue;
         int i, j, kk;
         cell_ijk(d, c, i, j, kk);
-        const double uc = u[c];
-        const double kc = k * (1.0 + uc * uc), dkc = 2.0 * k * uc;
-        double div = 0.0, diag = 1.0, cw = 0.0, ce = 0.0, cs = 0.0, cn = 0.0;
-        // faces in ascending face-id order: west (x-), east (x+), south (y-), north (y+)
-        auto face = [&](double um, double g, double& cm) {
-            const double km = k * (1.0 + um * um);
-            const double kf = 0.5 * (kc + km);
-            const double diff = uc - um;
-            div += g * kf * diff;
-            diag += coef * g * (0.5 * dkc * diff + kf);
-            cm = coef * g * (0.5 * (2.0 * k * um) * diff - kf);
-        };
-        if (i > 0) face(u[c - 1], gx, cw);
-        if (i < nx - 1) face(u[c + 1], gx, ce);
-        if (j > 0) face(u[c - nx], gy, cs);
-        if (j < ny - 1) face(u[c + nx], gy, cn);
-        const double F = (uc - d.u0[c]) + coef * div;
-        d.F[c] = F;
-        d.C[c] = diag;
-        d.W[c] = cw;
-        d.E[c] = ce;
-        d.S[c] = cs;
-        d.N[c] = cn;
-        if (MIXED) {
-            ((float*)d.Cm)[c] = (float)diag;
-            ((float*)d.Wm)[c] = (float)cw;
-            ((float*)d.Em)[c] = (float)ce;
-            ((float*)d.Sm)[c] = (float)cs;
-            ((float*)d.Nm)[c] = (float)cn;
-        }
+        const Nb5 x = load_nb5(u, c, i, j, nx, ny);
+        double co[5];
+        const double F = heat_row(d, i, j, x.c, x.w, x.e, x.s, x.n, u0[c], co);
+        Fo[c] = F;
+        const int sl = ((i + j) & 1) * d.h + (c >> 1);
+#pragma unroll
+        for (int q = 0; q < 5; ++q) co_out[q][sl] = (K)co[q];
         f2[m] = F * F;
     }
-    group_store<CPT>(f2, part + lb * CPT);
-    KTIME_END();
+    group_store<CPT>(f2, PARTS(part));
 }
 
-template <class T>
-__device__ inline T stencil_apply(const Desc& d, const T* __restrict__ x, int c, int i, int j, int k) {
-    const int nx = d.nx;
-    T acc = ((const T*)d.Cm)[c] * x[c];
-    if (i > 0) acc += ((const T*)d.Wm)[c] * x[c - 1];
-    if (i < nx - 1) acc += ((const T*)d.Em)[c] * x[c + 1];
-    if (j > 0) acc += ((const T*)d.Sm)[c] * x[c - nx];
-    if (j < d.ny - 1) acc += ((const T*)d.Nm)[c] * x[c + nx];
-    if (d.nz > 1) {
-        const int nxy = nx * d.ny;
-        if (k > 0) acc += ((const T*)d.Bm)[c] * x[c - nxy];
-        if (k < d.nz - 1) acc += ((const T*)d.Tm)[c] * x[c + nxy];
-    }
-    return acc;
+// FP64 heat rows for export (test hook): F and the 5 coefficients into the given arrays.
+__global__ void k_heat_rows64(const Desc* __restrict__ D, double* __restrict__ F, double* __restrict__ co) {
+    const Desc& d = D[0];
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= d.n) return;
+    int i, j, kk;
+    cell_ijk(d, c, i, j, kk);
+    const Nb5 x = load_nb5(d.u, c, i, j, d.nx, d.ny);
+    double r[5];
+    F[c] = heat_row(d, i, j, x.c, x.w, x.e, x.s, x.n, d.u0[c], r);
+    for (int q = 0; q < 5; ++q) co[(size_t)q * d.n + c] = r[q];
 }
 
-// y = A x with fused partial dot products: dot0 = V[a0].y, dot1 = V[a1].y (a1 < 0: y.y).
-// Skipped for inactive instances (partials 0).
-template <class T, int CPT>
-__global__ __launch_bounds__(BS) void k_spmv_dot(const Desc* __restrict__ D, const int2* __restrict__ blk, int ix,
-                                                 int iy, int nd, int a0, int a1, double* __restrict__ part, int ng) {
-    LOCATE();
-    double s0[CPT] = {}, s1[CPT] = {};
-    if (d.act[0]) {
-        const T* __restrict__ x = (const T*)d.v[ix];
-        T* __restrict__ y = (T*)d.v[iy];
-        const T* __restrict__ va = (const T*)d.v[a0];
-        const T* __restrict__ vb = a1 >= 0 ? (const T*)d.v[a1] : nullptr;
-        FOR_GROUPS(m, c) {
-            if (c >= d.n) continue;
-            int i, j, k;
-            cell_ijk(d, c, i, j, k);
-            const T yc = stencil_apply<T>(d, x, c, i, j, k);
-            y[c] = yc;
-            s0[m] = (double)va[c] * (double)yc;
-            if (nd > 1) s1[m] = (vb ? (double)vb[c] : (double)yc) * (double)yc;
-        }
-    }
-    group_store<CPT>(s0, part + lb * CPT);
-    if (nd > 1) group_store<CPT>(s1, part + ng + lb * CPT);
-    KTIME_END();
-}
-
-// FP64 defect for the instances under refinement: r64 = F - J du.
+// FP64 defect for the instances under refinement: r64 = F - J du.  Heat instances
+// recompute their FP64 Jacobian rows from u (heat_row; reads u, du 5-point and F, writes
+// r64: 32 B per cell); two-phase instances read their stored FP64 coefficients.
 template <int CPT>
 __global__ __launch_bounds__(BS) void k_defect(const Desc* __restrict__ D, const int2* __restrict__ blk,
                                                double* __restrict__ part) {
     LOCATE();
     double r2[CPT] = {};
     if (d.act[1]) {
-        const int nx = d.nx;
+        const int nx = d.nx, ny = d.ny;
         const double* __restrict__ x = d.du;
-        FOR_GROUPS(m, c) {
-            if (c >= d.n) continue;
-            int i, j, k;
-            cell_ijk(d, c, i, j, k);
-            double acc = d.C[c] * x[c];
-            if (i > 0) acc += d.W[c] * x[c - 1];
-            if (i < nx - 1) acc += d.E[c] * x[c + 1];
-            if (j > 0) acc += d.S[c] * x[c - nx];
-            if (j < d.ny - 1) acc += d.N[c] * x[c + nx];
-            if (d.nz > 1) {
-                const int nxy = nx * d.ny;
-                if (k > 0) acc += d.B[c] * x[c - nxy];
-                if (k < d.nz - 1) acc += d.T[c] * x[c + nxy];
+        const double* __restrict__ Fi = d.F;
+        double* __restrict__ ro = d.r64;
+        if (d.kind == K_HEAT) {
+            const double* __restrict__ u = d.u;
+            FOR_GROUPS(m, c) {
+                if (c >= d.n) continue;
+                int i, j, kk;
+                cell_ijk(d, c, i, j, kk);
+                const Nb5 uu = load_nb5(u, c, i, j, nx, ny);
+                const Nb5 xx = load_nb5(x, c, i, j, nx, ny);
+                const double Fc = Fi[c];
+                double co[5];
+                (void)heat_row(d, i, j, uu.c, uu.w, uu.e, uu.s, uu.n, 0.0, co);
+                double acc = co[0] * xx.c;
+                if (i > 0) acc += co[1] * xx.w;
+                if (i < nx - 1) acc += co[2] * xx.e;
+                if (j > 0) acc += co[3] * xx.s;
+                if (j < ny - 1) acc += co[4] * xx.n;
+                const double r = Fc - acc;
+                ro[c] = r;
+                r2[m] = r * r;
             }
-            const double r = d.F[c] - acc;
-            d.r64[c] = r;
-            r2[m] = r * r;
+        } else if (d.C) {   // two-phase: cell-order FP64 coefficients (stored in mixed precision only)
+            FOR_GROUPS(m, c) {
+                if (c >= d.n) continue;
+                int i, j, k;
+                cell_ijk(d, c, i, j, k);
+                double acc = d.C[c] * x[c];
+                if (i > 0) acc += d.W[c] * x[c - 1];
+                if (i < nx - 1) acc += d.E[c] * x[c + 1];
+                if (j > 0) acc += d.S[c] * x[c - nx];
+                if (j < ny - 1) acc += d.N[c] * x[c + nx];
+                if (d.nz > 1) {
+                    const int nxy = nx * ny;
+                    if (k > 0) acc += d.B[c] * x[c - nxy];
+                    if (k < d.nz - 1) acc += d.T[c] * x[c + nxy];
+                }
+                const double r = Fi[c] - acc;
+                ro[c] = r;
+                r2[m] = r * r;
+            }
         }
     }
-    group_store<CPT>(r2, part + lb * CPT);
-    KTIME_END();
+    group_store<CPT>(r2, PARTS(part));
 }
 
-// Preconditioner set-up: Jacobi 1/C, or red-black ILU(0) pivots.  Red = (i+j+k) even
-// cells come first in the ordering; red rows have no L part, a black row's L part is its
-// red neighbours and its pivot is C_b - sum_m A_bm A_mb / C_m (fill-in dropped).
-// dinv holds 1/C for red cells and 1/pivot for black cells.
+// ---------------------------------------------------------------------------
+// Red-black Krylov space.
+//
+// Cell c has colour (i + j + k) & 1 (red 0, black 1) and rank c >> 1 within its colour:
+// every pair of cells (2m, 2m + 1) holds one cell of each colour (neighbours in a row
+// differ in colour; a pair wraps to the next row or plane only when the row or plane
+// length is odd, which flips the colour once more), so each colour's ranks are dense in
+// [0, h), h = (n + 1) / 2.  The Krylov-precision Jacobian coefficients K[q] and dinv are
+// stored red ranks first, black ranks at offset h; a stencil neighbour of a cell has the
+// other colour and rank (c +- 1) >> 1, (c +- nx) >> 1, (c +- nx ny) >> 1.
+//
+// Preconditioner ilu0: with red cells ordered first, ILU(0) of A = [C_r A_rb; A_br C_b]
+// is exact on the red block and drops the black-black fill, M = (P + L) P^-1 (P + U)
+// with P = diag(C_r, P_b), P_b = C_b - diag(A_br C_r^-1 A_rb).  The split-preconditioned
+// operator (P + L)^-1 A (P + U)^-1 P is the identity on red cells and S P_b^-1 on black
+// cells, S = C_b - A_br C_r^-1 A_rb the Schur complement, and P_b = diag(S).  So the
+// solve runs BiCGStab on the black cells only: S x_b = g_b, g_b = b_b - A_br C_r^-1 b_r,
+// right-preconditioned by 1/diag(S) (the ILU(0) black pivots), then x_r = C_r^-1 (b_r -
+// A_rb x_b).  The red residual of the full system is zero by construction and the black
+// one is the reduced system's, so the stopping test (relative to ||g||) is the full
+// system's.  S is applied matrix-free in two passes: y_r = C_r^-1 A_rb v_b over red
+// cells, then S v = C_b v_b - A_br y_r over black cells.  Krylov vectors have one slot
+// per black rank (ns = 1).
+// Preconditioner jacobi / none: BiCGStab on the full system in the same storage, two
+// slots per pair (red rank p at p, black rank p at h + p; ns = 2), y = A x in one pass.
+//
+// Every Krylov kernel runs one thread per pair rank p in [0, h); its partials are groups
+// of BS pairs, [gbase, gbase + ngrpk).
+
+// The cell of colour col and rank p with its (i, j, k), or -1 if the instance has none.
+__device__ inline int cell_of(const Desc& d, int p, int col, int& i, int& j, int& k) {
+    int c = 2 * p;
+    if (c >= d.n) return -1;
+    cell_ijk(d, c, i, j, k);
+    if (((i + j + k) & 1) != col) {
+        if (++c >= d.n) return -1;
+        cell_ijk(d, c, i, j, k);
+    }
+    return c;
+}
+
+// sum over the existing neighbours of cell c (W, E, S, N, B, T order) of the row's
+// coefficient (slot s of K[1..6]) times x at the neighbour's rank.
+template <class T>
+__device__ inline T offdiag(const Desc& d, int s, int c, int i, int j, int k, const T* __restrict__ x) {
+    const int nx = d.nx, ny = d.ny;
+    T acc = (T)0;
+    if (i > 0) acc += ((const T*)d.K[1])[s] * x[(c - 1) >> 1];
+    if (i < nx - 1) acc += ((const T*)d.K[2])[s] * x[(c + 1) >> 1];
+    if (j > 0) acc += ((const T*)d.K[3])[s] * x[(c - nx) >> 1];
+    if (j < ny - 1) acc += ((const T*)d.K[4])[s] * x[(c + nx) >> 1];
+    if (d.nz > 1) {
+        const int nxy = nx * ny;
+        if (k > 0) acc += ((const T*)d.K[5])[s] * x[(c - nxy) >> 1];
+        if (k < d.nz - 1) acc += ((const T*)d.K[6])[s] * x[(c + nxy) >> 1];
+    }
+    return acc;
+}
+
+// Reduced operator, pass 1 (red ranks): y_r = C_r^-1 (A_rb v_b), v = V[iv] (black slots).
+template <class T, int CPT>
+__global__ __launch_bounds__(BS) void k_schur_red(const Desc* __restrict__ D, const int2* __restrict__ blk, int iv) {
+    LOCATE();
+    if (!d.act[0]) return;
+    const T* __restrict__ v = (const T*)d.v[iv];
+    const T* __restrict__ dv = (const T*)d.dinv;
+    T* __restrict__ y = (T*)d.v[V_Y];
+    FOR_PAIRS(p) {
+        int i, j, k;
+        const int c = cell_of(d, p, 0, i, j, k);
+        y[p] = c < 0 ? (T)0 : offdiag<T>(d, p, c, i, j, k, v) * dv[p];
+    }
+}
+// Reduced operator, pass 2 (black ranks): w = S v = C_b v_b - A_br y_r, into V[iw], with
+// fused partial dot products dot0 = V[a0].w, dot1 = V[a1].w (a1 < 0: w.w).
+template <class T, int CPT>
+__global__ __launch_bounds__(BS) void k_schur_black(const Desc* __restrict__ D, const int2* __restrict__ blk,
+                                                    int iv, int iw, int nd, int a0, int a1,
+                                                    double* __restrict__ part, int ng) {
+    LOCATE();
+    double s0[CPT] = {}, s1[CPT] = {};
+    if (d.act[0]) {
+        const int h = d.h;
+        const T* __restrict__ v = (const T*)d.v[iv];
+        const T* __restrict__ y = (const T*)d.v[V_Y];
+        const T* __restrict__ Cb = (const T*)d.K[0] + h;
+        T* __restrict__ w = (T*)d.v[iw];
+        const T* __restrict__ va = (const T*)d.v[a0];
+        const T* __restrict__ vb = a1 >= 0 ? (const T*)d.v[a1] : nullptr;
+        FOR_GROUPS(m, p) {
+            if (p >= h) continue;
+            int i, j, k;
+            const int c = cell_of(d, p, 1, i, j, k);
+            const T wc = c < 0 ? (T)0 : Cb[p] * v[p] - offdiag<T>(d, h + p, c, i, j, k, y);
+            w[p] = wc;
+            s0[m] = (double)va[p] * (double)wc;
+            if (nd > 1) s1[m] = (vb ? (double)vb[p] : (double)wc) * (double)wc;
+        }
+    }
+    group_store<CPT>(s0, PARTSK(part));
+    if (nd > 1) group_store<CPT>(s1, PARTSK(part + ng));
+}
+// Full operator (jacobi / none): y = A x over both slots of each pair, fused dots as above.
+template <class T, int CPT>
+__global__ __launch_bounds__(BS) void k_spmv_rb(const Desc* __restrict__ D, const int2* __restrict__ blk, int ix,
+                                                int iy, int nd, int a0, int a1, double* __restrict__ part, int ng) {
+    LOCATE();
+    double s0[CPT] = {}, s1[CPT] = {};
+    if (d.act[0]) {
+        const int h = d.h;
+        const T* __restrict__ x = (const T*)d.v[ix];
+        T* __restrict__ y = (T*)d.v[iy];
+        const T* __restrict__ K0 = (const T*)d.K[0];
+        const T* __restrict__ va = (const T*)d.v[a0];
+        const T* __restrict__ vb = a1 >= 0 ? (const T*)d.v[a1] : nullptr;
+        FOR_GROUPS(m, p) {
+            if (p >= h) continue;
+#pragma unroll
+            for (int col = 0; col < 2; ++col) {
+                const int s = col * h + p;
+                int i, j, k;
+                const int c = cell_of(d, p, col, i, j, k);
+                const T yc = c < 0 ? (T)0 : K0[s] * x[s] + offdiag<T>(d, s, c, i, j, k, x + (1 - col) * h);
+                y[s] = yc;
+                s0[m] += (double)va[s] * (double)yc;
+                if (nd > 1) s1[m] += (vb ? (double)vb[s] : (double)yc) * (double)yc;
+            }
+        }
+    }
+    group_store<CPT>(s0, PARTSK(part));
+    if (nd > 1) group_store<CPT>(s1, PARTSK(part + ng));
+}
+
+// Preconditioner set-up (pairs): dinv[p] = 1 / C_r of red rank p; dinv[h + p] of black
+// rank p = 1 / P_b (ilu0: C_b - sum_m A_bm A_mb / C_m over the red neighbours m, the
+// ILU(0) pivot = diag of the Schur complement), 1 / C_b (jacobi) or 1 (none); 0 for
+// ranks without a cell.
 template <class T, int CPT>
 __global__ __launch_bounds__(BS) void k_precond_setup(const Desc* __restrict__ D, const int2* __restrict__ blk,
                                                       int mode) {
     LOCATE();
-    const T* Cm = (const T*)d.Cm;
-    const T *Wm = (const T*)d.Wm, *Em = (const T*)d.Em, *Sm = (const T*)d.Sm, *Nm = (const T*)d.Nm;
-    const T *Bm = (const T*)d.Bm, *Tm = (const T*)d.Tm;
-    const int nx = d.nx, nxy = d.nx * d.ny;
-    FOR_CELLS(c) {
-        T piv = Cm[c];
+    const int h = d.h, nx = d.nx, ny = d.ny, nxy = d.nx * d.ny;
+    const T* K[7];
+    for (int q = 0; q < 7; ++q) K[q] = (const T*)d.K[q];
+    T* __restrict__ dv = (T*)d.dinv;
+    FOR_PAIRS(p) {
+        int i, j, k;
+        int c = cell_of(d, p, 0, i, j, k);
+        if (c < 0) dv[p] = (T)0;
+        else {
+            const T cr = K[0][p];
+            dv[p] = mode == PC_NONE ? (T)1 : cr != (T)0 ? (T)1 / cr : (T)1;
+        }
+        c = cell_of(d, p, 1, i, j, k);
+        if (c < 0) {
+            dv[h + p] = (T)0;
+            continue;
+        }
+        const int s = h + p;
+        T piv = K[0][s];
         if (mode == PC_ILU) {
-            int i, j, k;
-            cell_ijk(d, c, i, j, k);
-            if ((i + j + k) & 1) {
-                if (i > 0) piv -= Wm[c] * Em[c - 1] / Cm[c - 1];
-                if (i < nx - 1) piv -= Em[c] * Wm[c + 1] / Cm[c + 1];
-                if (j > 0) piv -= Sm[c] * Nm[c - nx] / Cm[c - nx];
-                if (j < d.ny - 1) piv -= Nm[c] * Sm[c + nx] / Cm[c + nx];
-                if (d.nz > 1) {
-                    if (k > 0) piv -= Bm[c] * Tm[c - nxy] / Cm[c - nxy];
-                    if (k < d.nz - 1) piv -= Tm[c] * Bm[c + nxy] / Cm[c + nxy];
-                }
-            }
-        }
-        ((T*)d.dinv)[c] = piv != (T)0 ? (T)1 / piv : (T)1;
-    }
-}
-
-// z = M^{-1} r for red-black ILU(0) in two passes: a black cell solves
-// z_b = (r_b - sum_m A_bm r_m / C_m) / piv_b, then a red cell solves
-// z_r = (r_r - sum_b A_rb z_b) / C_r.  1/C_m of a red neighbour is its dinv, so only
-// multiplications remain.
-template <class T>
-__device__ inline T black_value(const Desc& d, const T* __restrict__ r, int b, int i, int j, int k) {
-    const int nx = d.nx, ny = d.ny;
-    const T* dv = (const T*)d.dinv;
-    const T *Wm = (const T*)d.Wm, *Em = (const T*)d.Em, *Sm = (const T*)d.Sm, *Nm = (const T*)d.Nm;
-    T s = r[b];
-    if (i > 0) s -= Wm[b] * (r[b - 1] * dv[b - 1]);
-    if (i < nx - 1) s -= Em[b] * (r[b + 1] * dv[b + 1]);
-    if (j > 0) s -= Sm[b] * (r[b - nx] * dv[b - nx]);
-    if (j < ny - 1) s -= Nm[b] * (r[b + nx] * dv[b + nx]);
-    if (d.nz > 1) {
-        const int nxy = nx * ny;
-        if (k > 0) s -= ((const T*)d.Bm)[b] * (r[b - nxy] * dv[b - nxy]);
-        if (k < d.nz - 1) s -= ((const T*)d.Tm)[b] * (r[b + nxy] * dv[b + nxy]);
-    }
-    return s * dv[b];
-}
-// Pass 1 of the apply: z of the black cells (their L part only touches red cells,
-// whose z = r / C before the U sweep).
-template <class T, int CPT>
-__global__ __launch_bounds__(BS) void k_ilu_black(const Desc* __restrict__ D, const int2* __restrict__ blk, int ir,
-                                                  int iz) {
-    LOCATE();
-    if (d.act[0]) {
-        const T* __restrict__ r = (const T*)d.v[ir];
-        T* __restrict__ zo = (T*)d.v[iz];
-        FOR_CELLS(c) {
-            int i, j, k;
-            cell_ijk(d, c, i, j, k);
-            if ((i + j + k) & 1) zo[c] = black_value<T>(d, r, c, i, j, k);
-        }
-    }
-    KTIME_END();
-}
-// Pass 2: z of the red cells from the black z of pass 1 (the U part of a red row).
-template <class T, int CPT>
-__global__ __launch_bounds__(BS) void k_ilu_red(const Desc* __restrict__ D, const int2* __restrict__ blk, int ir,
-                                                int iz) {
-    LOCATE();
-    if (d.act[0]) {
-        const int nx = d.nx, ny = d.ny;
-        const T* __restrict__ r = (const T*)d.v[ir];
-        T* __restrict__ z = (T*)d.v[iz];
-        FOR_CELLS(c) {
-            int i, j, k;
-            cell_ijk(d, c, i, j, k);
-            if ((i + j + k) & 1) continue;
-            T s = r[c];
-            if (i > 0) s -= ((const T*)d.Wm)[c] * z[c - 1];
-            if (i < nx - 1) s -= ((const T*)d.Em)[c] * z[c + 1];
-            if (j > 0) s -= ((const T*)d.Sm)[c] * z[c - nx];
-            if (j < ny - 1) s -= ((const T*)d.Nm)[c] * z[c + nx];
+            // A_bm of the black row times A_mb of the red neighbour's row (opposite direction)
+            if (i > 0) piv -= K[1][s] * K[2][(c - 1) >> 1] / K[0][(c - 1) >> 1];
+            if (i < nx - 1) piv -= K[2][s] * K[1][(c + 1) >> 1] / K[0][(c + 1) >> 1];
+            if (j > 0) piv -= K[3][s] * K[4][(c - nx) >> 1] / K[0][(c - nx) >> 1];
+            if (j < ny - 1) piv -= K[4][s] * K[3][(c + nx) >> 1] / K[0][(c + nx) >> 1];
             if (d.nz > 1) {
-                const int nxy = nx * ny;
-                if (k > 0) s -= ((const T*)d.Bm)[c] * z[c - nxy];
-                if (k < d.nz - 1) s -= ((const T*)d.Tm)[c] * z[c + nxy];
+                if (k > 0) piv -= K[5][s] * K[6][(c - nxy) >> 1] / K[0][(c - nxy) >> 1];
+                if (k < d.nz - 1) piv -= K[6][s] * K[5][(c + nxy) >> 1] / K[0][(c + nxy) >> 1];
             }
-            z[c] = s * ((const T*)d.dinv)[c];
         }
+        dv[s] = mode == PC_NONE ? (T)1 : piv != (T)0 ? (T)1 / piv : (T)1;
     }
-    KTIME_END();
 }
 
 // Krylov start: r = b (already in r), rh = r, x = 0, p = v = 0; partial dots (rh.r, r.r).
@@ -583,23 +694,33 @@ __global__ __launch_bounds__(BS) void k_kinit(const Desc* __restrict__ D, const 
     LOCATE();
     double rr[CPT] = {};
     if (d.act[1]) {
-        FOR_GROUPS(m, c) {
-            if (c >= d.n) continue;
-            const T rc = ((T*)d.v[V_R])[c];
-            ((T*)d.v[V_RH])[c] = rc;
-            ((T*)d.v[V_X])[c] = 0;
-            ((T*)d.v[V_P])[c] = 0;
-            ((T*)d.v[V_V])[c] = 0;
-            rr[m] = (double)rc * (double)rc;
+        FOR_GROUPS(m, p) {
+            if (p >= d.h) continue;
+            for (int w = 0; w < d.ns; ++w) {
+                const int s = p + w * d.h;
+                const T rc = ((T*)d.v[V_R])[s];
+                ((T*)d.v[V_RH])[s] = rc;
+                ((T*)d.v[V_X])[s] = 0;
+                ((T*)d.v[V_P])[s] = 0;
+                ((T*)d.v[V_V])[s] = 0;
+                rr[m] += (double)rc * (double)rc;
+            }
         }
     }
-    group_store<CPT>(rr, part + lb * CPT);
-    group_store<CPT>(rr, part + ng + lb * CPT);
+    group_store<CPT>(rr, PARTSK(part));
+    group_store<CPT>(rr, PARTSK(part + ng));
 }
 
-// p = r + beta (p - omega v) (first iteration: p = r); Jacobi / none: ph = M p
+// The right preconditioner's diagonal for slot s: reduced system -> 1 / P_b of black rank
+// s; full system -> dinv of slot s (jacobi) or 1 (none, dinv = 1).
+template <class T>
+__device__ inline const T* slot_dinv(const Desc& d) {
+    return (const T*)d.dinv + (d.ns == 1 ? d.h : 0);
+}
+
+// p = r + beta (p - omega v) (first iteration: p = r); ph = M^-1 p
 template <class T, int CPT>
-__global__ __launch_bounds__(BS) void k_b1(const Desc* __restrict__ D, const int2* __restrict__ blk, int pc) {
+__global__ __launch_bounds__(BS) void k_b1(const Desc* __restrict__ D, const int2* __restrict__ blk) {
     LOCATE();
     if (!d.act[0]) return;
     const double* sc = d.sc;
@@ -607,23 +728,31 @@ __global__ __launch_bounds__(BS) void k_b1(const Desc* __restrict__ D, const int
     const bool first = sc[S_IT] == 0.0;
     const T beta = first ? (T)0 : (T)((sc[S_RHO] / sc[S_RHO_OLD]) * (sc[S_ALPHA] / sc[S_OMEGA]));
     const T omega = (T)sc[S_OMEGA];
-    FOR_CELLS(c) {
-        const T r = ((const T*)d.v[V_R])[c];
-        const T pv = first ? r : r + beta * (p[c] - omega * ((const T*)d.v[V_V])[c]);
-        p[c] = pv;
-        if (pc != PC_ILU) ((T*)d.v[V_PH])[c] = pc == PC_JACOBI ? ((const T*)d.dinv)[c] * pv : pv;
+    const T* dv = slot_dinv<T>(d);
+    FOR_PAIRS(q) {
+        for (int w = 0; w < d.ns; ++w) {
+            const int s = q + w * d.h;
+            const T r = ((const T*)d.v[V_R])[s];
+            const T pv = first ? r : r + beta * (p[s] - omega * ((const T*)d.v[V_V])[s]);
+            p[s] = pv;
+            ((T*)d.v[V_PH])[s] = dv[s] * pv;
+        }
     }
 }
-// s = r - alpha v ; Jacobi / none: sh = M s
+// s = r - alpha v ; sh = M^-1 s
 template <class T, int CPT>
-__global__ __launch_bounds__(BS) void k_b2(const Desc* __restrict__ D, const int2* __restrict__ blk, int pc) {
+__global__ __launch_bounds__(BS) void k_b2(const Desc* __restrict__ D, const int2* __restrict__ blk) {
     LOCATE();
     if (!d.act[0]) return;
     const T alpha = (T)d.sc[S_ALPHA];
-    FOR_CELLS(c) {
-        const T s = ((const T*)d.v[V_R])[c] - alpha * ((const T*)d.v[V_V])[c];
-        ((T*)d.v[V_S])[c] = s;
-        if (pc != PC_ILU) ((T*)d.v[V_SH])[c] = pc == PC_JACOBI ? ((const T*)d.dinv)[c] * s : s;
+    const T* dv = slot_dinv<T>(d);
+    FOR_PAIRS(q) {
+        for (int w = 0; w < d.ns; ++w) {
+            const int s = q + w * d.h;
+            const T sv = ((const T*)d.v[V_R])[s] - alpha * ((const T*)d.v[V_V])[s];
+            ((T*)d.v[V_S])[s] = sv;
+            ((T*)d.v[V_SH])[s] = dv[s] * sv;
+        }
     }
 }
 // x += alpha ph + omega sh ; r = s - omega t ; partial dots (rh.r, r.r)
@@ -635,30 +764,45 @@ __global__ __launch_bounds__(BS) void k_b3(const Desc* __restrict__ D, const int
     if (d.act[0]) {
         const T alpha = (T)d.sc[S_ALPHA], omega = (T)d.sc[S_OMEGA];
         T* x = (T*)d.v[V_X];
-        FOR_GROUPS(m, c) {
-            if (c >= d.n) continue;
-            x[c] += alpha * ((const T*)d.v[V_PH])[c] + omega * ((const T*)d.v[V_SH])[c];
-            const T r = ((const T*)d.v[V_S])[c] - omega * ((const T*)d.v[V_T])[c];
-            ((T*)d.v[V_R])[c] = r;
-            a0[m] = (double)((const T*)d.v[V_RH])[c] * (double)r;
-            a1[m] = (double)r * (double)r;
+        FOR_GROUPS(m, q) {
+            if (q >= d.h) continue;
+            for (int w = 0; w < d.ns; ++w) {
+                const int s = q + w * d.h;
+                x[s] += alpha * ((const T*)d.v[V_PH])[s] + omega * ((const T*)d.v[V_SH])[s];
+                const T r = ((const T*)d.v[V_S])[s] - omega * ((const T*)d.v[V_T])[s];
+                ((T*)d.v[V_R])[s] = r;
+                a0[m] += (double)((const T*)d.v[V_RH])[s] * (double)r;
+                a1[m] += (double)r * (double)r;
+            }
         }
     }
-    group_store<CPT>(a0, part + lb * CPT);
-    group_store<CPT>(a1, part + ng + lb * CPT);
+    group_store<CPT>(a0, PARTSK(part));
+    group_store<CPT>(a1, PARTSK(part + ng));
 }
 
+// out[0] = number of instances with flag `which` set, out[1] = the largest Krylov
+// iteration count of the current solve (S_IT) over the instances.
 __global__ void k_count_active(const Desc* __restrict__ D, int ninst, int which, int* __restrict__ out) {
-    __shared__ int lds[BS];
-    int s = 0;
-    for (int i = threadIdx.x; i < ninst; i += BS) s += D[i].act[which] ? 1 : 0;
-    lds[threadIdx.x] = s;
+    __shared__ int lds[2][BS];
+    int s = 0, mx = 0;
+    for (int i = threadIdx.x; i < ninst; i += BS) {
+        s += D[i].act[which] ? 1 : 0;
+        mx = max(mx, (int)D[i].sc[S_IT]);
+    }
+    lds[0][threadIdx.x] = s;
+    lds[1][threadIdx.x] = mx;
     __syncthreads();
     for (int w = BS / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) lds[threadIdx.x] += lds[threadIdx.x + w];
+        if ((int)threadIdx.x < w) {
+            lds[0][threadIdx.x] += lds[0][threadIdx.x + w];
+            lds[1][threadIdx.x] = max(lds[1][threadIdx.x], lds[1][threadIdx.x + w]);
+        }
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[0] = lds[0];
+    if (threadIdx.x == 0) {
+        out[0] = lds[0][0];
+        out[1] = lds[1][0];
+    }
 }
 
 // refinement bookkeeping (one thread per instance):
@@ -686,25 +830,72 @@ __global__ void k_refine_flags(const Desc* __restrict__ D, int ninst, int mode, 
     }
 }
 
-// r (Krylov precision) = r64 / ||r64||  (mixed) or = F (fp64 mode: r64 == F)
+// Right-hand side of the Krylov solve from r64 (scaled by 1 / ||r64|| in mixed precision,
+// b = (T)(r64 * scale)): reduced system -> g_b = b_b - A_br (C_r^-1 b_r) on black ranks;
+// full system -> b in both slots of each pair (0 where a rank has no cell).
+template <class T>
+__device__ inline T rhs_value(const Desc& d, int c, double scale) {
+    return (T)(d.r64[c] * scale);
+}
 template <class T, int CPT>
 __global__ __launch_bounds__(BS) void k_load_rhs(const Desc* __restrict__ D, const int2* __restrict__ blk,
                                                  int scaled) {
     LOCATE();
     if (!d.act[1]) return;
     const double scale = scaled ? 1.0 / d.sc[S_RN] : 1.0;
-    FOR_CELLS(c)((T*)d.v[V_R])[c] = (T)(d.r64[c] * scale);
+    const int h = d.h, nx = d.nx, ny = d.ny, nxy = d.nx * d.ny;
+    T* __restrict__ r = (T*)d.v[V_R];
+    const T* __restrict__ dv = (const T*)d.dinv;
+    FOR_PAIRS(p) {
+        int i, j, k;
+        if (d.ns == 2) {
+            int c = cell_of(d, p, 0, i, j, k);
+            r[p] = c < 0 ? (T)0 : rhs_value<T>(d, c, scale);
+            c = cell_of(d, p, 1, i, j, k);
+            r[h + p] = c < 0 ? (T)0 : rhs_value<T>(d, c, scale);
+            continue;
+        }
+        const int c = cell_of(d, p, 1, i, j, k);
+        if (c < 0) {
+            r[p] = (T)0;
+            continue;
+        }
+        const int s = h + p;
+        const T* K[7];
+        for (int q = 1; q < 7; ++q) K[q] = (const T*)d.K[q];
+        T acc = (T)0;
+        if (i > 0) acc += K[1][s] * (rhs_value<T>(d, c - 1, scale) * dv[(c - 1) >> 1]);
+        if (i < nx - 1) acc += K[2][s] * (rhs_value<T>(d, c + 1, scale) * dv[(c + 1) >> 1]);
+        if (j > 0) acc += K[3][s] * (rhs_value<T>(d, c - nx, scale) * dv[(c - nx) >> 1]);
+        if (j < ny - 1) acc += K[4][s] * (rhs_value<T>(d, c + nx, scale) * dv[(c + nx) >> 1]);
+        if (d.nz > 1) {
+            if (k > 0) acc += K[5][s] * (rhs_value<T>(d, c - nxy, scale) * dv[(c - nxy) >> 1]);
+            if (k < d.nz - 1) acc += K[6][s] * (rhs_value<T>(d, c + nxy, scale) * dv[(c + nxy) >> 1]);
+        }
+        r[p] = rhs_value<T>(d, c, scale) - acc;
+    }
 }
-// du += ||r64|| x  (mixed) or du = x (fp64)
+// du += ||r64|| x (mixed) or du = x (fp64), in cell order.  Reduced system: the red
+// unknowns are recovered first, x_r = C_r^-1 (b_r - A_rb x_b).
 template <class T, int CPT>
 __global__ __launch_bounds__(BS) void k_accum(const Desc* __restrict__ D, const int2* __restrict__ blk, int scaled) {
     LOCATE();
     if (!d.act[1]) return;
     const double rn = d.sc[S_RN];
-    FOR_CELLS(c) {
-        const double x = (double)((const T*)d.v[V_X])[c];
-        if (scaled) d.du[c] += rn * x;
-        else d.du[c] = x;
+    const double scale = scaled ? 1.0 / rn : 1.0;
+    const int h = d.h;
+    const T* __restrict__ x = (const T*)d.v[V_X];
+    const T* __restrict__ dv = (const T*)d.dinv;
+    auto put = [&](int c, T xc) {
+        if (scaled) d.du[c] += rn * (double)xc;
+        else d.du[c] = (double)xc;
+    };
+    FOR_PAIRS(p) {
+        int i, j, k;
+        const int cr = cell_of(d, p, 0, i, j, k);
+        if (cr >= 0) put(cr, d.ns == 2 ? x[p] : (rhs_value<T>(d, cr, scale) - offdiag<T>(d, p, cr, i, j, k, x)) * dv[p]);
+        const int cb = cell_of(d, p, 1, i, j, k);
+        if (cb >= 0) put(cb, x[(d.ns == 2 ? h : 0) + p]);
     }
 }
 template <int CPT>
@@ -722,14 +913,22 @@ __global__ __launch_bounds__(BS) void k_newton_update(const Desc* __restrict__ D
     double* x = d.kind == K_TWOPHASE ? (d.act[2] == 0 ? d.p : d.sw) : d.u;
     FOR_CELLS(c) x[c] -= d.du[c];
 }
-__global__ void k_gather_slot(const Desc* __restrict__ D, int ninst, int slot, double* __restrict__ out) {
+// out[q * ninst + i] = scalar slots.x, .y, .z, .w (q = 0..3) of instance i
+__global__ void k_gather_slots(const Desc* __restrict__ D, int ninst, int4 slots, double* __restrict__ out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < ninst) out[i] = D[i].sc[slot];
+    if (i >= ninst) return;
+    const double* sc = D[i].sc;
+    out[i] = sc[slots.x];
+    out[ninst + i] = sc[slots.y];
+    out[2 * ninst + i] = sc[slots.z];
+    out[3 * ninst + i] = sc[slots.w];
 }
-#undef KTIME_END
 #undef FOR_GROUPS
 #undef FOR_CELLS
 #undef LOCATE
+#undef PARTS
+#undef PARTSK
+#undef FOR_PAIRS
 
 // ---------------------------------------------------------------------------
 struct Inst {
@@ -745,22 +944,29 @@ struct Inst {
     DevPtr<double> tp;             // tx, ty, tz, pv, src (n each)
     DevPtr<double> p, sw, sw0;
     DevPtr<double> u, u0, F, du, r64, C, W, E, S, N, B, T, sc;
-    DevPtr<char> coef32, dinv, vecs;
+    DevPtr<char> kcoef, dinv, vecs;   // Krylov-precision coefficients, preconditioner, vectors
     DevPtr<int> act;
     enum State { IDLE, START, SOLVE, DONE } state = IDLE;
     StepStats st;
     int ncoef() const { return nz > 1 ? 7 : 5; }
+    int h() const { return (n + 1) / 2; }
 };
 
-// Launch shape of a table: cells per thread and XCD-contiguous block order.
+// Kernel classes; each has its own cells per thread: assembly, refinement defect, the
+// Krylov operator, the cell-order <-> red-black conversions (preconditioner set-up,
+// right-hand side, solution update) and the Krylov vector updates.
+enum Cls { C_ASM = 0, C_DEF, C_SPMV, C_PC, C_VEC, NCLS };
+
+// Launch shape of a table: cells per thread per kernel class and the XCD-contiguous
+// block order (one for the whole table).
 struct Shape {
-    int cpt = 4;
+    int cpt[NCLS] = {4, 4, 4, 4, 4};
     int xcd = 1;
-    bool operator<(const Shape& o) const { return cpt != o.cpt ? cpt < o.cpt : xcd < o.xcd; }
 };
 constexpr int CPT_MAX = 8;
+inline int cpt_index(int cpt) { return cpt == 1 ? 0 : cpt == 2 ? 1 : cpt == 4 ? 2 : 3; }
 
-// Calls f(std::integral_constant<int, CPT>) for the table's cells-per-thread.
+// Calls f(std::integral_constant<int, CPT>) for a cells-per-thread value.
 template <class F>
 void by_cpt(int cpt, F&& f) {
     switch (cpt) {
@@ -775,15 +981,19 @@ void by_cpt(int cpt, F&& f) {
 struct Table {
     std::vector<Inst*> inst;
     DevPtr<Desc> desc;
-    DevPtr<int2> blk, rng;   // rng: each instance's partials (groups of BS cells)
-    int nb = 0;              // blocks
-    int ng = 0;              // groups = nb * cpt
+    DevPtr<int2> blk[4];     // cell kernels: block -> (instance, local block) for 1, 2, 4, 8 cells per thread
+    int nb[4] = {};          // blocks of each map
+    DevPtr<int2> pblk[4];    // pair kernels (Krylov space): the same over each instance's h pair ranks
+    int pnb[4] = {};
+    int ng = 0;              // partial groups over all instances (an instance has ceil(n / BS))
     Shape shape;
     DevPtr<double> part;
     uint64_t version = 0;
     int solves = 0;   // Krylov solves run on this membership (graphs are captured from the second)
-    // captured BiCGStab iterations, keyed by (tolerance, max iterations)
+    // captured BiCGStab iteration, and the iterations the last solve needed, keyed by
+    // (tolerance, max iterations)
     std::map<std::pair<double, int>, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
+    std::map<std::pair<double, int>, int> pred;
     ~Table() {
         for (auto& g : graphs) {
             (void)hipGraphExecDestroy(g.second.second);
@@ -803,11 +1013,12 @@ struct HeatBatch::Impl {
     uint64_t version = 1;   // bumped whenever a descriptor field changes
     std::map<std::vector<int>, TableP> tcache;
     bool use_graph = true;
-    int graph_iters = 2;    // BiCGStab iterations per graph launch
     static constexpr int NRING = 4;
-    DevPtr<int> count;
+    DevPtr<int> count;      // NRING x {active count, max Krylov iterations}
     int* host_count = nullptr;
     hipEvent_t ev[NRING] = {};
+    DevPtr<double> gbuf;    // gathered per-instance scalars
+    size_t gcap = 0;
 
     size_t ks() const { return mixed ? sizeof(float) : sizeof(double); }
 
@@ -846,61 +1057,57 @@ struct HeatBatch::Impl {
         d.F = I.F->p;
         d.du = I.du->p;
         d.r64 = I.r64->p;
-        d.C = I.C->p;
-        d.W = I.W->p;
-        d.E = I.E->p;
-        d.S = I.S->p;
-        d.N = I.N->p;
-        d.B = I.B ? I.B->p : nullptr;
-        d.T = I.T ? I.T->p : nullptr;
-        if (mixed) {
-            float* f = (float*)I.coef32->p;
-            d.Cm = f;
-            d.Wm = f + I.n;
-            d.Em = f + 2L * I.n;
-            d.Sm = f + 3L * I.n;
-            d.Nm = f + 4L * I.n;
-            d.Bm = I.nz > 1 ? f + 5L * I.n : nullptr;
-            d.Tm = I.nz > 1 ? f + 6L * I.n : nullptr;
-        } else {
-            d.Cm = d.C;
-            d.Wm = d.W;
-            d.Em = d.E;
-            d.Sm = d.S;
-            d.Nm = d.N;
-            d.Bm = d.B;
-            d.Tm = d.T;
-        }
+        auto p64 = [](const DevPtr<double>& a) { return a ? a->p : nullptr; };
+        d.C = p64(I.C);
+        d.W = p64(I.W);
+        d.E = p64(I.E);
+        d.S = p64(I.S);
+        d.N = p64(I.N);
+        d.B = p64(I.B);
+        d.T = p64(I.T);
+        const size_t h = I.h();
+        d.h = (int)h;
+        d.ns = pc == PC_ILU ? 1 : 2;
+        for (int q = 0; q < 7; ++q) d.K[q] = q < I.ncoef() ? I.kcoef->p + q * 2 * h * ks() : nullptr;
         d.dinv = I.dinv->p;
-        for (int v = 0; v < NVEC; ++v) d.v[v] = I.vecs->p + (size_t)v * I.n * ks();
+        for (int v = 0; v < NVEC; ++v) d.v[v] = I.vecs->p + (size_t)v * d.ns * h * ks();
         d.sc = I.sc->p;
         d.act = I.act->p;
         return d;
     }
 
-    TableP build_table(const std::vector<Inst*>& list, Shape sh, unsigned long long* ts = nullptr) {
+    TableP build_table(const std::vector<Inst*>& list, const Shape& sh) {
         TableP tp = std::make_shared<Table>();
         Table& t = *tp;
         t.version = version;
         t.inst = list;
         t.shape = sh;
-        const int cpb = sh.cpt * BS;
         std::vector<Desc> ds;
-        std::vector<int2> blk, rng;
+        int g = 0;
         for (size_t q = 0; q < list.size(); ++q) {
             Desc d = desc_of(*list[q]);
             d.xcd = sh.xcd;
-            d.ts = ts;
+            d.gbase = g;
+            d.ngrp = (list[q]->n + BS - 1) / BS;
+            d.ngrpk = (list[q]->h() + BS - 1) / BS;
+            g += d.ngrp;
             ds.push_back(d);
-            const int nbk = (list[q]->n + cpb - 1) / cpb;
-            rng.push_back(make_int2((int)blk.size() * sh.cpt, ((int)blk.size() + nbk) * sh.cpt));
-            for (int b = 0; b < nbk; ++b) blk.push_back(make_int2((int)q, b));
         }
-        t.nb = (int)blk.size();
-        t.ng = t.nb * sh.cpt;
+        t.ng = g;
+        for (int ci = 0; ci < 4; ++ci) {
+            const int cpb = (1 << ci) * BS;
+            std::vector<int2> blk, pblk;
+            for (size_t q = 0; q < list.size(); ++q) {
+                const int nbk = (list[q]->n + cpb - 1) / cpb, npk = (list[q]->h() + cpb - 1) / cpb;
+                for (int b = 0; b < nbk; ++b) blk.push_back(make_int2((int)q, b));
+                for (int b = 0; b < npk; ++b) pblk.push_back(make_int2((int)q, b));
+            }
+            t.nb[ci] = (int)blk.size();
+            t.blk[ci] = dev_from_host(blk.data(), blk.size());
+            t.pnb[ci] = (int)pblk.size();
+            t.pblk[ci] = dev_from_host(pblk.data(), pblk.size());
+        }
         t.desc = dev_from_host(ds.data(), ds.size());
-        t.blk = dev_from_host(blk.data(), blk.size());
-        t.rng = dev_from_host(rng.data(), rng.size());
         t.part = dev_alloc<double>(2 * (size_t)t.ng + 2);
         return tp;
     }
@@ -917,11 +1124,13 @@ struct HeatBatch::Impl {
 
     // ---- launch-shape tuner -------------------------------------------------------
     // A membership's signature is (log2 total cells, log2 instance count, log2 largest
-    // instance).  The first time a signature is seen, each shape (cells per thread 1, 2,
-    // 4, 8 x XCD order on/off) is timed on that membership with HIP events around the
-    // Krylov iteration's stencil kernels (SpMV with its dots, and the preconditioner's
-    // passes), and the fastest is kept for the signature.  Results do not depend on the
-    // shape (shape-independent reductions), so tuning changes time only.
+    // instance).  The first time a signature is seen, every kernel class is timed on that
+    // membership with HIP events at 1, 2, 4 and 8 cells per thread, with the XCD block
+    // order off and on; each class keeps its fastest cells per thread, and the block
+    // order with the smaller weighted total (weights: launches of the class per Newton
+    // iteration, about 1 assembly, 2 defects, 16 operator applies, 2 conversion rounds
+    // and 8 vector-update rounds) is kept for the table.  Results do not depend on the shape
+    // (shape-independent reductions), so tuning changes time only.
     std::map<std::tuple<int, int, int>, Shape> tuned;
     Shape last_shape;
     static int ilog2(long v) {
@@ -932,7 +1141,7 @@ struct HeatBatch::Impl {
     Shape shape_for(const std::vector<Inst*>& list) {
         if (cfg.cpt > 0) {
             Shape sh;
-            sh.cpt = cfg.cpt;
+            for (int c = 0; c < NCLS; ++c) sh.cpt[c] = cfg.cpt;
             sh.xcd = cfg.xcd;
             return last_shape = sh;
         }
@@ -942,231 +1151,287 @@ struct HeatBatch::Impl {
         auto it = tuned.find(sig);
         if (it != tuned.end()) return last_shape = it->second;
         Shape best;
-        if (cfg.tune) {
-            double best_ms = 1e300;
-            for (int cpt : {1, 2, 4, 8})
-                for (int xcd : {0, 1}) {
-                    Shape sh;
-                    sh.cpt = cpt;
-                    sh.xcd = xcd;
-                    const double ms = time_shape(list, sh);
-                    if (ms < best_ms) best_ms = ms, best = sh;
-                }
-        }
+        if (cfg.tune) best = tune_shape(list);
         tuned[sig] = best;
         return last_shape = best;
     }
-    double time_shape(const std::vector<Inst*>& list, Shape sh) {
-        TableP tp = build_table(list, sh);
-        Table& t = *tp;
-        // every instance active so the kernels do their full work (a solve resets the flags)
+    // One trial launch of a kernel class at the table's shape for that class.
+    void trial(Table& t, int cls) {
+        switch (cls) {
+            case C_ASM: launch_assemble(t); break;
+            case C_DEF: launch_defect(t); break;
+            case C_SPMV:
+                if (mixed) launch_op<float>(t, V_PH, V_V, 1, (int)V_RH, -1);
+                else launch_op<double>(t, V_PH, V_V, 1, (int)V_RH, -1);
+                break;
+            case C_PC:
+                if (mixed) conversions<float>(t);
+                else conversions<double>(t);
+                break;
+            default:
+                if (mixed) vec_round<float>(t);
+                else vec_round<double>(t);
+                break;
+        }
+    }
+    Shape tune_shape(const std::vector<Inst*>& list) {
+        // every instance active so the kernels do their full work.  Trial launches only
+        // write what the next solve re-initialises (Krylov vectors, du, the refinement
+        // residual, partials) or rewrite F and the coefficients from the current state.
         const int one[2] = {1, 1};
         for (Inst* I : list) EQ_HIP(hipMemcpyAsync(I->act->p, one, sizeof(one), hipMemcpyHostToDevice, S()));
-        // a Krylov round (SpMV with dots, preconditioner) and a Newton round; a Newton
-        // iteration runs about eight Krylov rounds
-        auto krylov_round = [&] {
-            if (mixed) spmv_dot<float>(t, V_PH, V_V, 1, (int)V_RH, -1, F(A_STORE, 1, 0, 0, NSLOT - 1), false);
-            else spmv_dot<double>(t, V_PH, V_V, 1, (int)V_RH, -1, F(A_STORE, 1, 0, 0, NSLOT - 1), false);
-            if (mixed) precond<float>(t, V_P, V_PH);
-            else precond<double>(t, V_P, V_PH);
-        };
-        // (the FP64 defect stands for the Newton-level stencil kernels; it only writes
-        // the refinement residual, which every solve re-initialises)
-        auto newton_round = [&] {
-            by_cpt(t.shape.cpt, [&](auto cpt_) {
-                constexpr int CP = decltype(cpt_)::value;
-                hipLaunchKernelGGL((k_defect<CP>), dim3((unsigned)t.nb), dim3(BS), 0, S(), t.desc->p, t.blk->p,
-                                   t.part->p);
-                EQ_HIP(hipGetLastError());
-            });
-        };
+        static const double weight[NCLS] = {1.0, 2.0, 16.0, 2.0, 8.0};
         hipEvent_t e0, e1;
         EQ_HIP(hipEventCreate(&e0));
         EQ_HIP(hipEventCreate(&e1));
-        auto timed = [&](auto&& fn, int reps) {
-            fn();   // warm
-            EQ_HIP(hipEventRecord(e0, S()));
-            for (int r = 0; r < reps; ++r) fn();
-            EQ_HIP(hipEventRecord(e1, S()));
-            EQ_HIP(hipEventSynchronize(e1));
-            float ms = 0.f;
-            EQ_HIP(hipEventElapsedTime(&ms, e0, e1));
-            return (double)ms / reps;
-        };
-        const double kr = timed(krylov_round, 3);
-        const double nr = timed(newton_round, 2);
+        Shape best;
+        double best_total = 1e300;
+        for (int xcd : {0, 1}) {
+            Shape sh;
+            sh.xcd = xcd;
+            TableP tp = build_table(list, sh);
+            Table& t = *tp;
+            double total = 0.0;
+            for (int cls = 0; cls < NCLS; ++cls) {
+                if (cls == C_DEF && !mixed) continue;   // no refinement defect in fp64 mode
+                double best_ms = 1e300;
+                for (int cpt : {1, 2, 4, 8}) {
+                    t.shape.cpt[cls] = cpt;
+                    trial(t, cls);   // warm
+                    EQ_HIP(hipEventRecord(e0, S()));
+                    for (int r = 0; r < 3; ++r) trial(t, cls);
+                    EQ_HIP(hipEventRecord(e1, S()));
+                    EQ_HIP(hipEventSynchronize(e1));
+                    float ms = 0.f;
+                    EQ_HIP(hipEventElapsedTime(&ms, e0, e1));
+                    if (ms < best_ms) best_ms = ms, sh.cpt[cls] = cpt;
+                }
+                total += weight[cls] * best_ms;
+            }
+            if (total < best_total) best_total = total, best = sh;
+        }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
-        return 8.0 * kr + nr;
+        return best;
     }
 
-    int count_active(const Table& t, int which) {
+    // out[0] = active instances (flag `which`), out[1] = max Krylov iterations, synchronously
+    int count_active(const Table& t, int which, int* maxit = nullptr) {
         hipLaunchKernelGGL(k_count_active, dim3(1), dim3(BS), 0, S(), t.desc->p, (int)t.inst.size(), which, count->p);
         EQ_HIP(hipGetLastError());
-        EQ_HIP(hipMemcpyAsync(host_count, count->p, sizeof(int), hipMemcpyDeviceToHost, S()));
+        EQ_HIP(hipMemcpyAsync(host_count, count->p, 2 * sizeof(int), hipMemcpyDeviceToHost, S()));
         Device::get().sync();
-        return *host_count;
+        if (maxit) *maxit = host_count[1];
+        return host_count[0];
+    }
+    // The same without stalling the queue: the result of ring slot k is read later.
+    void count_async(const Table& t, int which, int slot) {
+        hipLaunchKernelGGL(k_count_active, dim3(1), dim3(BS), 0, S(), t.desc->p, (int)t.inst.size(), which,
+                           count->p + 2 * slot);
+        EQ_HIP(hipGetLastError());
+        EQ_HIP(hipMemcpyAsync(host_count + 2 * slot, count->p + 2 * slot, 2 * sizeof(int), hipMemcpyDeviceToHost,
+                              S()));
+        EQ_HIP(hipEventRecord(ev[slot], S()));
+    }
+    int count_wait(int slot, int* maxit = nullptr) {
+        EQ_HIP(hipEventSynchronize(ev[slot]));
+        if (maxit) *maxit = host_count[2 * slot + 1];
+        return host_count[2 * slot];
     }
     // Finalisation of a reduction: one block per instance (kernel boundaries make the
     // partials of all blocks visible; no device-wide fences inside the kernels).
     void fin(const Table& t, const Fin& f) {
-        hipLaunchKernelGGL(k_fin, dim3((unsigned)t.inst.size()), dim3(FBS), 0, S(), t.desc->p, t.rng->p, t.part->p,
-                           t.ng, f);
+        hipLaunchKernelGGL(k_fin, dim3((unsigned)t.inst.size()), dim3(FBS), 0, S(), t.desc->p, t.part->p, t.ng, f);
         EQ_HIP(hipGetLastError());
     }
+    // finaliser of a Krylov reduction (partials over pairs) ...
     static Fin F(int act, int nd, double tol2 = 0, int maxit = 0, int slot = 0) {
         Fin f;
         f.act = act;
         f.nd = nd;
+        f.kry = 1;
         f.tol2 = tol2;
         f.maxit = maxit;
         f.slot = slot;
         return f;
     }
-// Launch over every block of the table; the kernel is named with CP, the table's
-// cells per thread, e.g. HL((k_b1<T, CP>), pc).
-#define HL(kernel, ...)                                                                                    \
-    by_cpt(t.shape.cpt, [&](auto cpt_) {                                                                   \
-        constexpr int CP = decltype(cpt_)::value;                                                          \
-        hipLaunchKernelGGL(kernel, dim3((unsigned)t.nb), dim3(BS), 0, S(), t.desc->p, t.blk->p, ##__VA_ARGS__); \
-        EQ_HIP(hipGetLastError());                                                                         \
+    // ... and of a cell kernel (assembly, defect): partials over cells, stored in `slot`
+    static Fin Fcell(int slot) {
+        Fin f = F(A_STORE, 1, 0, 0, slot);
+        f.kry = 0;
+        return f;
+    }
+// Launch over every block of the table at the cells per thread of kernel class `cls`;
+// the kernel is named with CP, e.g. HL(C_ASM, (k_assemble<true, CP>), t.part->p).
+#define HL(cls, kernel, ...)                                                                                \
+    by_cpt(t.shape.cpt[cls], [&](auto cpt_) {                                                               \
+        constexpr int CP = decltype(cpt_)::value;                                                           \
+        const int ci_ = cpt_index(CP);                                                                      \
+        hipLaunchKernelGGL(kernel, dim3((unsigned)t.nb[ci_]), dim3(BS), 0, S(), t.desc->p, t.blk[ci_]->p,   \
+                           ##__VA_ARGS__);                                                                  \
+        EQ_HIP(hipGetLastError());                                                                          \
+    })
+// The same over the pair ranks of the Krylov space.
+#define HLP(cls, kernel, ...)                                                                               \
+    by_cpt(t.shape.cpt[cls], [&](auto cpt_) {                                                               \
+        constexpr int CP = decltype(cpt_)::value;                                                           \
+        const int ci_ = cpt_index(CP);                                                                      \
+        hipLaunchKernelGGL(kernel, dim3((unsigned)t.pnb[ci_]), dim3(BS), 0, S(), t.desc->p, t.pblk[ci_]->p, \
+                           ##__VA_ARGS__);                                                                  \
+        EQ_HIP(hipGetLastError());                                                                          \
     })
 
+    void launch_assemble(const Table& t) {
+        TimedLaunch tl("assemble");
+        if (mixed) HL(C_ASM, (k_assemble<true, CP>), t.part->p);
+        else HL(C_ASM, (k_assemble<false, CP>), t.part->p);
+    }
     void assemble(const Table& t) {
-        {
-            TimedLaunch tl("assemble");
-            if (mixed) HL((k_assemble<true, CP>), t.part->p);
-            else HL((k_assemble<false, CP>), t.part->p);
-        }
-        fin(t, F(A_STORE, 1, 0, 0, S_FN));
+        launch_assemble(t);
+        fin(t, Fcell(S_FN));
     }
-
-    template <class T>
-    void spmv_dot(const Table& t, int ix, int iy, int nd, int a0, int a1, const Fin& f, bool with_fin = true) {
-        {
-            TimedLaunch tl(sizeof(T) == 4 ? "stencil_spmv_fp32" : "stencil_spmv_fp64");
-            HL((k_spmv_dot<T, CP>), ix, iy, nd, a0, a1, t.part->p, t.ng);
-        }
-        if (with_fin) fin(t, f);
+    void launch_defect(const Table& t) {
+        TimedLaunch tl("defect_fp64");
+        HL(C_DEF, (k_defect<CP>), t.part->p);
     }
+    // w = A v (full system) or S v (reduced system) with the fused dots
     template <class T>
-    void precond(const Table& t, int ir, int iz) {
+    void launch_op(const Table& t, int iv, int iw, int nd, int a0, int a1) {
+        TimedLaunch tl(sizeof(T) == 4 ? "operator_fp32" : "operator_fp64");
         if (pc == PC_ILU) {
-            HL((k_ilu_black<T, CP>), ir, iz);
-            HL((k_ilu_red<T, CP>), ir, iz);
+            HLP(C_SPMV, (k_schur_red<T, CP>), iv);
+            HLP(C_SPMV, (k_schur_black<T, CP>), iv, iw, nd, a0, a1, t.part->p, t.ng);
+        } else {
+            HLP(C_SPMV, (k_spmv_rb<T, CP>), iv, iw, nd, a0, a1, t.part->p, t.ng);
         }
+    }
+    template <class T>
+    void spmv_dot(const Table& t, int ix, int iy, int nd, int a0, int a1, const Fin& f) {
+        launch_op<T>(t, ix, iy, nd, a0, a1);
+        fin(t, f);
+    }
+    template <class T>
+    void conversions(const Table& t) {
+        HLP(C_PC, (k_load_rhs<T, CP>), 1);
+        HLP(C_PC, (k_accum<T, CP>), 1);
+    }
+    template <class T>
+    void vec_round(const Table& t) {
+        HLP(C_VEC, (k_b1<T, CP>));
+        HLP(C_VEC, (k_b2<T, CP>));
+        HLP(C_VEC, (k_b3<T, CP>), t.part->p, t.ng);
     }
 
     // BiCGStab for the instances whose refinement flag is set; rhs in V_R.
+    // (right preconditioner: ph = M^-1 p and sh = M^-1 s are formed in k_b1 / k_b2)
     template <class T>
     void bicg_iteration(const Table& t, double tol, int maxit) {
-        HL((k_b1<T, CP>), pc);
-        precond<T>(t, V_P, V_PH);
+        HLP(C_VEC, (k_b1<T, CP>));
         spmv_dot<T>(t, V_PH, V_V, 1, (int)V_RH, -1, F(A_ALPHA, 1));   // v = A ph; alpha = rho / rh.v
-        HL((k_b2<T, CP>), pc);
-        precond<T>(t, V_S, V_SH);
+        HLP(C_VEC, (k_b2<T, CP>));
         spmv_dot<T>(t, V_SH, V_T, 2, (int)V_S, -1, F(A_OMEGA, 2));    // t = A sh; omega = t.s / t.t
-        HL((k_b3<T, CP>), t.part->p, t.ng);
+        HLP(C_VEC, (k_b3<T, CP>), t.part->p, t.ng);
         fin(t, F(A_RHO, 2, tol * tol, maxit));
     }
 
-    // Device -> host count of active instances without stalling the queue: the count
-    // of launch k is read after launch k+1 has been queued (pinned ring + events).
-    void count_async(const Table& t, int which, int slot) {
-        hipLaunchKernelGGL(k_count_active, dim3(1), dim3(BS), 0, S(), t.desc->p, (int)t.inst.size(), which,
-                           count->p + slot);
-        EQ_HIP(hipGetLastError());
-        EQ_HIP(hipMemcpyAsync(host_count + slot, count->p + slot, sizeof(int), hipMemcpyDeviceToHost, S()));
-        EQ_HIP(hipEventRecord(ev[slot], S()));
-    }
-    int count_wait(int slot) {
-        EQ_HIP(hipEventSynchronize(ev[slot]));
-        return host_count[slot];
-    }
-
+    // One BiCGStab solve.  From the second solve on a membership, one iteration is
+    // captured in a hipGraph and replayed: the iterations the previous solve with the
+    // same tolerance needed, minus one, are queued without looking at the device; then
+    // one synchronous check of the active count; further iterations (if any) are
+    // pipelined, launch k+1 being queued before the count after launch k is read.
+    // Instances that converge earlier are masked by their device flag, so extra
+    // iterations are no-ops.
     template <class T>
     void bicgstab(Table& t, double tol, int maxit) {
-        HL((k_kinit<T, CP>), t.part->p, t.ng);
+        HLP(C_VEC, (k_kinit<T, CP>), t.part->p, t.ng);
         fin(t, F(A_INIT, 2));
-        // a membership seen once (request churn) runs eagerly; capture pays off from its second solve
+        const auto key = std::make_pair(tol, maxit);
+        int iters = 0;
         if (use_graph && !timing_enabled() && t.solves++ > 0) {
-            // graph_iters iterations per launch; instances that converge inside a
-            // launch are masked by their device flag, so extra iterations are no-ops.
-            auto key = std::make_pair(tol, maxit);
             auto g = t.graphs.find(key);
             if (g == t.graphs.end()) {
                 hipGraph_t graph;
                 hipGraphExec_t exec;
                 EQ_HIP(hipStreamBeginCapture(S(), hipStreamCaptureModeThreadLocal));
-                for (int q = 0; q < graph_iters; ++q) bicg_iteration<T>(t, tol, maxit);
+                bicg_iteration<T>(t, tol, maxit);
                 EQ_HIP(hipStreamEndCapture(S(), &graph));
                 EQ_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
                 g = t.graphs.emplace(key, std::make_pair(graph, exec)).first;
             }
-            // pipelined: launch k+1 is queued before the active count of launch k is read
+            const hipGraphExec_t exec = g->second.second;
+            const int blind = std::min(maxit, std::max(0, t.pred[key] - 1));
             int k = 0;
-            for (int it = 0; it < maxit; it += graph_iters, ++k) {
-                EQ_HIP(hipGraphLaunch(g->second.second, S()));
-                count_async(t, 0, k % NRING);
-                if (k > 0 && !count_wait((k - 1) % NRING)) break;
+            for (; k < blind; ++k) EQ_HIP(hipGraphLaunch(exec, S()));
+            bool done = k >= maxit;
+            if (!done) {
+                EQ_HIP(hipGraphLaunch(exec, S()));
+                ++k;
+                done = count_active(t, 0, &iters) == 0;
             }
+            int pending = -1;
+            while (!done && k < maxit) {
+                EQ_HIP(hipGraphLaunch(exec, S()));
+                count_async(t, 0, k % NRING);
+                if (pending >= 0 && count_wait(pending % NRING, &iters) == 0) done = true;
+                pending = k++;
+            }
+            if (!done && pending >= 0) count_wait(pending % NRING, &iters);
             Device::get().sync();
-            return;
+        } else {
+            if (count_active(t, 0)) {
+                for (int it = 0; it < maxit; ++it) {
+                    bicg_iteration<T>(t, tol, maxit);
+                    if (!count_active(t, 0, &iters)) break;
+                }
+            }
         }
-        if (!count_active(t, 0)) return;
-        for (int it = 0; it < maxit; ++it) {
-            bicg_iteration<T>(t, tol, maxit);
-            if (!count_active(t, 0)) break;
-        }
+        t.pred[key] = iters;
     }
 
     template <class T>
     void linear_solve(Table& t) {
         // du = 0, r64 = F; refinement flags from ||F||
-        HL((k_zero_du<CP>));
+        HL(C_VEC, (k_zero_du<CP>));
         const unsigned ni = (unsigned)t.inst.size();
         hipLaunchKernelGGL(k_refine_flags, dim3((ni + 63) / 64), dim3(64), 0, S(), t.desc->p, (int)ni, 0,
                            cfg.solver_tol, cfg.max_refine);
+        precond_setup<T>(t);
         if (!mixed) {
-            HL((k_load_rhs<T, CP>), 0);
-            precond_setup<T>(t);
+            HLP(C_PC, (k_load_rhs<T, CP>), 0);
             bicgstab<T>(t, cfg.solver_tol, cfg.solver_max_iter);
-            HL((k_accum<T, CP>), 0);
+            HLP(C_PC, (k_accum<T, CP>), 0);
             return;
         }
-        precond_setup<T>(t);
         for (int ref = 0; ref < cfg.max_refine; ++ref) {
             if (ref > 0 && !count_active(t, 1)) break;   // all start active (||F|| > 0)
-            HL((k_load_rhs<T, CP>), 1);
+            HLP(C_PC, (k_load_rhs<T, CP>), 1);
             bicgstab<T>(t, cfg.inner_tol, cfg.solver_max_iter);
-            HL((k_accum<T, CP>), 1);
-            {
-                TimedLaunch tl("defect_fp64");
-                HL((k_defect<CP>), t.part->p);
-            }
-            fin(t, F(A_STORE, 1, 0, 0, S_FN));
+            HLP(C_PC, (k_accum<T, CP>), 1);
+            launch_defect(t);
+            fin(t, Fcell(S_FN));
             hipLaunchKernelGGL(k_refine_flags, dim3((ni + 63) / 64), dim3(64), 0, S(), t.desc->p, (int)ni, 1,
                                cfg.solver_tol, cfg.max_refine);
         }
     }
     template <class T>
     void precond_setup(const Table& t) {
-        if (pc != PC_NONE) HL((k_precond_setup<T, CP>), pc);
+        HLP(C_PC, (k_precond_setup<T, CP>), pc);
     }
+    void newton_update(const Table& t) { HL(C_VEC, (k_newton_update<CP>)); }
 #undef HL
+#undef HLP
 
-    // per-instance scalars of several slots in one device-to-host copy: [slot][instance]
-    std::vector<double> gather(const Table& t, std::initializer_list<int> slots) {
+    // per-instance scalars of four slots in one launch and one device-to-host copy:
+    // result[q * ninst + i] = scalar slots[q] of instance i
+    std::vector<double> gather(const Table& t, int4 slots) {
         const int ni = (int)t.inst.size();
-        auto d = dev_alloc<double>((size_t)ni * slots.size());
-        int q = 0;
-        for (int slot : slots) {
-            hipLaunchKernelGGL(k_gather_slot, dim3((ni + 63) / 64), dim3(64), 0, S(), t.desc->p, ni, slot,
-                               d->p + (size_t)q * ni);
-            EQ_HIP(hipGetLastError());
-            ++q;
+        if (gcap < 4 * (size_t)ni) {
+            gcap = 4 * (size_t)ni;
+            gbuf = dev_alloc<double>(gcap);
         }
-        return dev_to_host(d->p, (size_t)ni * slots.size());
+        hipLaunchKernelGGL(k_gather_slots, dim3((ni + 63) / 64), dim3(64), 0, S(), t.desc->p, ni, slots, gbuf->p);
+        EQ_HIP(hipGetLastError());
+        return dev_to_host(gbuf->p, 4 * (size_t)ni);
     }
 };
 
@@ -1179,8 +1444,8 @@ HeatBatch::HeatBatch(const FusedConfig& cfg) : p_(new Impl()) {
     EQ_REQUIRE(pc == "jacobi" || pc == "ilu0" || pc == "none", "preconditioner must be jacobi, ilu0 or none");
     p_->pc = pc == "jacobi" ? PC_JACOBI : pc == "ilu0" ? PC_ILU : PC_NONE;
     p_->use_graph = cfg.graph;
-    p_->count = dev_alloc<int>(Impl::NRING);
-    EQ_HIP(hipHostMalloc((void**)&p_->host_count, Impl::NRING * sizeof(int), hipHostMallocDefault));
+    p_->count = dev_alloc<int>(2 * Impl::NRING);
+    EQ_HIP(hipHostMalloc((void**)&p_->host_count, 2 * Impl::NRING * sizeof(int), hipHostMallocDefault));
     for (int i = 0; i < Impl::NRING; ++i) EQ_HIP(hipEventCreateWithFlags(&p_->ev[i], hipEventDisableTiming));
 }
 
@@ -1192,37 +1457,48 @@ HeatBatch::~HeatBatch() {
     delete p_;
 }
 
-size_t HeatBatch::estimate_bytes(int n, const std::string& precision, bool twophase) {
-    const size_t ks = precision == "mixed" ? 4 : 8;
+size_t HeatBatch::estimate_bytes(int n, const std::string& precision, bool twophase, const std::string& precond) {
     const bool mixed = precision == "mixed";
-    if (!twophase)   // u, u0, F, du, r64, 5 coefficients (+ FP32 copies), dinv, Krylov vectors
-        return (size_t)n * (10 * 8 + (mixed ? 5 * 4 : 0) + ks * (1 + NVEC)) + NSLOT * 8 + 64;
-    // tx, ty, tz, pv, src, p, sw, sw0, F, du, r64, 7 coefficients (+ FP32), dinv, Krylov vectors
-    return (size_t)n * (18 * 8 + (mixed ? 7 * 4 : 0) + ks * (1 + NVEC)) + NSLOT * 8 + 64;
+    const size_t ks = mixed ? 4 : 8;
+    const size_t h = ((size_t)n + 1) / 2;
+    const size_t ncoef = twophase ? 7 : 5;
+    const size_t slots = precond == "ilu0" ? h : 2 * h;
+    // F, du, r64; Krylov-precision coefficients and dinv (2 h each); Krylov vectors
+    size_t b = (size_t)n * 3 * 8 + (ncoef + 1) * 2 * h * ks + (size_t)NVEC * slots * ks;
+    if (!twophase) b += (size_t)n * 2 * 8;   // u, u0
+    else b += (size_t)n * 8 * 8 + (mixed ? (size_t)n * ncoef * 8 : 0);   // tx ty tz pv src p sw sw0 (+ FP64 J)
+    return b + NSLOT * 8 + 64;
 }
 
-// Newton / Krylov storage common to both instance kinds.
-static void alloc_solver(Inst& I, bool mixed, size_t ks) {
+// Newton / Krylov storage common to both instance kinds.  FP64 Jacobian coefficients in
+// cell order are stored only for two-phase instances in mixed precision (their
+// refinement defect reads them); the Krylov-precision coefficients, dinv and the Krylov
+// vectors are in red-black rank order.
+static void alloc_solver(Inst& I, bool mixed, size_t ks, int pc) {
     const int n = I.n;
+    const size_t h = I.h();
     I.F = dev_alloc<double>(n);
     I.du = dev_alloc<double>(n);
     I.r64 = dev_alloc<double>(n);
-    I.C = dev_alloc<double>(n);
-    I.W = dev_alloc<double>(n);
-    I.E = dev_alloc<double>(n);
-    I.S = dev_alloc<double>(n);
-    I.N = dev_alloc<double>(n);
-    if (I.nz > 1) {
-        I.B = dev_alloc<double>(n);
-        I.T = dev_alloc<double>(n);
+    if (I.kind == K_TWOPHASE && mixed) {
+        I.C = dev_alloc<double>(n);
+        I.W = dev_alloc<double>(n);
+        I.E = dev_alloc<double>(n);
+        I.S = dev_alloc<double>(n);
+        I.N = dev_alloc<double>(n);
+        if (I.nz > 1) {
+            I.B = dev_alloc<double>(n);
+            I.T = dev_alloc<double>(n);
+        }
     }
     I.sc = dev_alloc<double>(NSLOT);
     I.act = dev_alloc<int>(4);
     EQ_HIP(hipMemsetAsync(I.sc->p, 0, NSLOT * sizeof(double), S()));
     EQ_HIP(hipMemsetAsync(I.act->p, 0, 4 * sizeof(int), S()));
-    if (mixed) I.coef32 = dev_alloc<char>((size_t)I.ncoef() * n * sizeof(float));
-    I.dinv = dev_alloc<char>((size_t)n * ks);
-    I.vecs = dev_alloc<char>((size_t)NVEC * n * ks);
+    I.kcoef = dev_alloc<char>((size_t)I.ncoef() * 2 * h * ks);
+    I.dinv = dev_alloc<char>(2 * h * ks);
+    const size_t slots = pc == PC_ILU ? h : 2 * h;
+    I.vecs = dev_alloc<char>((size_t)NVEC * slots * ks);
 }
 
 int HeatBatch::add(int nx, int ny, double k, double cv, const double* u_host, double dx, double dy) {
@@ -1239,7 +1515,7 @@ int HeatBatch::add(int nx, int ny, double k, double cv, const double* u_host, do
     I->dy = dy;
     I->u = dev_from_host(u_host, I->n);
     I->u0 = dev_alloc<double>(I->n);
-    alloc_solver(*I, p_->mixed, p_->ks());
+    alloc_solver(*I, p_->mixed, p_->ks(), p_->pc);
     const int id = I->id;
     p_->insts[id] = std::move(I);
     ++p_->version;
@@ -1293,7 +1569,7 @@ int HeatBatch::add_twophase(const TwoPhaseSpec& sp, const double* perm, const do
     launch_fill(I->p->p, 0.0, n);
     I->sw = dev_from_host(sw_host, n);
     I->sw0 = dev_alloc<double>(n);
-    alloc_solver(*I, p_->mixed, p_->ks());
+    alloc_solver(*I, p_->mixed, p_->ks(), p_->pc);
     const int id = I->id;
     p_->insts[id] = std::move(I);
     ++p_->version;
@@ -1320,7 +1596,7 @@ bool HeatBatch::in_flight(int slot) const {
 const StepStats& HeatBatch::stats(int slot) const { return get_inst(p_, slot).st; }
 size_t HeatBatch::device_bytes(int slot) const {
     const Inst& I = get_inst(p_, slot);
-    return estimate_bytes(I.n, p_->cfg.precision, I.kind == K_TWOPHASE);
+    return estimate_bytes(I.n, p_->cfg.precision, I.kind == K_TWOPHASE, p_->cfg.precond);
 }
 
 static const int kStage[2] = {0, 1};
@@ -1354,11 +1630,7 @@ std::vector<int> HeatBatch::iterate() {
         Table& t = *tp;
         if (P.mixed) P.linear_solve<float>(t);
         else P.linear_solve<double>(t);
-        by_cpt(t.shape.cpt, [&](auto cpt_) {
-            constexpr int CP = decltype(cpt_)::value;
-            hipLaunchKernelGGL((k_newton_update<CP>), dim3((unsigned)t.nb), dim3(BS), 0, S(), t.desc->p, t.blk->p);
-            EQ_HIP(hipGetLastError());
-        });
+        P.newton_update(t);
     }
     std::vector<int> done;
     if (asmb.empty()) return done;
@@ -1367,7 +1639,7 @@ std::vector<int> HeatBatch::iterate() {
     P.assemble(t);
     // one device-to-host copy per Newton iteration: ||F||^2 and the solve statistics
     const size_t na = asmb.size();
-    std::vector<double> g = P.gather(t, {S_FN, S_ITT, S_REF, S_RN});
+    std::vector<double> g = P.gather(t, make_int4(S_FN, S_ITT, S_REF, S_RN));
     const FusedConfig& c = P.cfg;
     for (size_t q = 0; q < na; ++q) {
         Inst& I = *asmb[q];
@@ -1431,9 +1703,10 @@ void HeatBatch::step(const std::vector<int>& slots, double dt) {
 
 LaunchShape HeatBatch::launch_shape() const {
     LaunchShape r;
-    r.cpt = p_->last_shape.cpt;
+    r.cpt = p_->last_shape.cpt[C_SPMV];
     r.xcd = p_->last_shape.xcd;
     r.signatures = (int)p_->tuned.size();
+    for (int c = 0; c < NCLS; ++c) r.cpt_class[c] = p_->last_shape.cpt[c];
     return r;
 }
 
@@ -1443,77 +1716,92 @@ double HeatBatch::time_kernel(const std::string& name, int reps, double* event_m
     std::vector<Inst*> all;
     for (auto& kv : P.insts) all.push_back(kv.second.get());
     EQ_REQUIRE(!all.empty(), "empty batch");
-    // a table of the whole membership, in its tuned shape, whose launches stamp the
-    // wall clock (block start, wave ends) into ts
-    const Shape sh = P.shape_for(all);
-    const int cpb = sh.cpt * BS;
-    long nb = 0;
-    for (Inst* I : all) nb += (I->n + cpb - 1) / cpb;
-    auto ts = dev_alloc<unsigned long long>(2 * (size_t)nb);
-    TableP tp = P.build_table(all, sh, ts->p);
+    // the table of the whole membership in its tuned shape
+    TableP tp = P.table(all);
     Table& t = *tp;
-    EQ_REQUIRE(t.nb == nb, "timing table mismatch");
     // every instance active, so the kernels do their full work (the next solve resets the flags)
     for (Inst* I : all) {
         const int one[2] = {1, 1};
         EQ_HIP(hipMemcpyAsync(I->act->p, one, sizeof(one), hipMemcpyHostToDevice, S()));
     }
     Device::get().sync();
-    int khz = 0;
-    EQ_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, 0));
-    EQ_REQUIRE(khz > 0, "no wall-clock rate");
-    std::function<void()> launch;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-#define TL(kernel, ...)                                                                                          \
-    launch = [&] {                                                                                               \
-        by_cpt(t.shape.cpt, [&](auto cpt_) {                                                                     \
-            constexpr int CP = decltype(cpt_)::value;                                                            \
-            hipExtLaunchKernelGGL(kernel, dim3((unsigned)t.nb), dim3(BS), 0, S(), e0, e1, 0, t.desc->p, t.blk->p, \
-                                  ##__VA_ARGS__);                                                                \
-        });                                                                                                      \
+    std::function<void(hipEvent_t, hipEvent_t)> launch;
+#define TL(cls, kernel, ...)                                                                                    \
+    launch = [&](hipEvent_t s0, hipEvent_t s1) {                                                                \
+        by_cpt(t.shape.cpt[cls], [&](auto cpt_) {                                                               \
+            constexpr int CP = decltype(cpt_)::value;                                                           \
+            const int ci_ = cpt_index(CP);                                                                      \
+            hipExtLaunchKernelGGL(kernel, dim3((unsigned)t.nb[ci_]), dim3(BS), 0, S(), s0, s1, 0, t.desc->p,    \
+                                  t.blk[ci_]->p, ##__VA_ARGS__);                                                \
+        });                                                                                                     \
     }
-    if (name == "stencil_spmv") {
-        if (P.mixed) TL((k_spmv_dot<float, CP>), (int)V_PH, (int)V_V, 1, (int)V_RH, -1, t.part->p, t.ng);
-        else TL((k_spmv_dot<double, CP>), (int)V_PH, (int)V_V, 1, (int)V_RH, -1, t.part->p, t.ng);
+#define TLP(cls, kernel, ...)                                                                                   \
+    launch = [&](hipEvent_t s0, hipEvent_t s1) {                                                                \
+        by_cpt(t.shape.cpt[cls], [&](auto cpt_) {                                                               \
+            constexpr int CP = decltype(cpt_)::value;                                                           \
+            const int ci_ = cpt_index(CP);                                                                      \
+            hipExtLaunchKernelGGL(kernel, dim3((unsigned)t.pnb[ci_]), dim3(BS), 0, S(), s0, s1, 0, t.desc->p,   \
+                                  t.pblk[ci_]->p, ##__VA_ARGS__);                                               \
+        });                                                                                                     \
+    }
+    const bool red = P.pc == PC_ILU;
+    if (name == "operator_red" && red) {
+        if (P.mixed) TLP(C_SPMV, (k_schur_red<float, CP>), (int)V_PH);
+        else TLP(C_SPMV, (k_schur_red<double, CP>), (int)V_PH);
+    } else if (name == "operator_black" && red) {
+        if (P.mixed) TLP(C_SPMV, (k_schur_black<float, CP>), (int)V_PH, (int)V_V, 1, (int)V_RH, -1, t.part->p, t.ng);
+        else TLP(C_SPMV, (k_schur_black<double, CP>), (int)V_PH, (int)V_V, 1, (int)V_RH, -1, t.part->p, t.ng);
+    } else if (name == "spmv" && !red) {
+        if (P.mixed) TLP(C_SPMV, (k_spmv_rb<float, CP>), (int)V_PH, (int)V_V, 1, (int)V_RH, -1, t.part->p, t.ng);
+        else TLP(C_SPMV, (k_spmv_rb<double, CP>), (int)V_PH, (int)V_V, 1, (int)V_RH, -1, t.part->p, t.ng);
     } else if (name == "assemble") {
-        if (P.mixed) TL((k_assemble<true, CP>), t.part->p);
-        else TL((k_assemble<false, CP>), t.part->p);
-    } else if (name == "ilu_black") {
-        if (P.mixed) TL((k_ilu_black<float, CP>), (int)V_P, (int)V_PH);
-        else TL((k_ilu_black<double, CP>), (int)V_P, (int)V_PH);
-    } else if (name == "ilu_red") {
-        if (P.mixed) TL((k_ilu_red<float, CP>), (int)V_P, (int)V_PH);
-        else TL((k_ilu_red<double, CP>), (int)V_P, (int)V_PH);
-    } else if (name == "defect") {
-        TL((k_defect<CP>), t.part->p);
+        if (P.mixed) TL(C_ASM, (k_assemble<true, CP>), t.part->p);
+        else TL(C_ASM, (k_assemble<false, CP>), t.part->p);
+    } else if (name == "defect" && P.mixed) {   // the refinement defect exists in mixed precision only
+        TL(C_DEF, (k_defect<CP>), t.part->p);
+    } else if (name == "b1") {
+        if (P.mixed) TLP(C_VEC, (k_b1<float, CP>));
+        else TLP(C_VEC, (k_b1<double, CP>));
+    } else if (name == "b2") {
+        if (P.mixed) TLP(C_VEC, (k_b2<float, CP>));
+        else TLP(C_VEC, (k_b2<double, CP>));
+    } else if (name == "b3") {
+        if (P.mixed) TLP(C_VEC, (k_b3<float, CP>), t.part->p, t.ng);
+        else TLP(C_VEC, (k_b3<double, CP>), t.part->p, t.ng);
+    } else if (name == "load_rhs") {
+        if (P.mixed) TLP(C_PC, (k_load_rhs<float, CP>), 1);
+        else TLP(C_PC, (k_load_rhs<double, CP>), 1);
     } else {
-        EQ_REQUIRE(false, "unknown kernel " << name << " (stencil_spmv, assemble, ilu_black, ilu_red, defect)");
+        EQ_REQUIRE(false, "unknown kernel " << name << " (operator_red, operator_black [ilu0], spmv [jacobi, none], "
+                                                       "assemble, defect [mixed], b1, b2, b3, load_rhs)");
     }
 #undef TL
+#undef TLP
+    hipEvent_t e0 = nullptr, e1 = nullptr;
     EQ_HIP(hipEventCreate(&e0));
     EQ_HIP(hipEventCreate(&e1));
-    launch();   // warm
+    launch(nullptr, nullptr);   // warm
     EQ_HIP(hipGetLastError());
-    std::vector<unsigned long long> h(2 * (size_t)nb);
-    double total = 0.0, ev_total = 0.0;
+    // each launch between its own start and stop events (hipExtLaunchKernel)
+    double total = 0.0;
     for (int r = 0; r < reps; ++r) {
-        EQ_HIP(hipMemsetAsync(ts->p, 0, h.size() * sizeof(unsigned long long), S()));
-        launch();
-        EQ_HIP(hipMemcpyAsync(h.data(), ts->p, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, S()));
+        launch(e0, e1);
         EQ_HIP(hipEventSynchronize(e1));
-        Device::get().sync();
         float ms = 0.f;
         EQ_HIP(hipEventElapsedTime(&ms, e0, e1));
-        ev_total += ms;
-        unsigned long long lo = ~0ull, hi = 0;
-        for (long b = 0; b < nb; ++b) lo = std::min(lo, h[2 * b]), hi = std::max(hi, h[2 * b + 1]);
-        EQ_REQUIRE(hi > lo, "kernel clock stamps missing");
-        total += (double)(hi - lo) / (double)khz;   // ms
+        total += ms;
     }
+    // the same number of launches back to back, one event pair around all of them
+    EQ_HIP(hipEventRecord(e0, S()));
+    for (int r = 0; r < reps; ++r) launch(nullptr, nullptr);
+    EQ_HIP(hipEventRecord(e1, S()));
+    EQ_HIP(hipEventSynchronize(e1));
+    float batch_ms = 0.f;
+    EQ_HIP(hipEventElapsedTime(&batch_ms, e0, e1));
     EQ_HIP(hipGetLastError());
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    if (event_ms) *event_ms = ev_total / reps;
+    if (event_ms) *event_ms = batch_ms / reps;
     return total / reps;
 }
 
@@ -1542,14 +1830,14 @@ void HeatBatch::set_state(int slot, const double* u) {
 }
 
 // CSR with ascending columns: bottom, south, west, diagonal, east, north, top (existing
-// neighbours only; stored zeros kept, as in the AD engine's union patterns).
-static void csr_from_stencil(const Inst& I, std::vector<int>& rp, std::vector<int>& ci, std::vector<double>& v) {
-    std::vector<double> C = dev_to_host(I.C->p, I.n), W = dev_to_host(I.W->p, I.n), E = dev_to_host(I.E->p, I.n),
-                        Sx = dev_to_host(I.S->p, I.n), N = dev_to_host(I.N->p, I.n), Bt, Tp;
-    if (I.nz > 1) {
-        Bt = dev_to_host(I.B->p, I.n);
-        Tp = dev_to_host(I.T->p, I.n);
-    }
+// neighbours only; stored zeros kept, as in the AD engine's union patterns).  co holds the
+// FP64 coefficients C, W, E, S, N (, B, T), n each.
+static void csr_from_stencil(const Inst& I, const std::vector<double>& co, std::vector<int>& rp,
+                             std::vector<int>& ci, std::vector<double>& v) {
+    const size_t n = I.n;
+    const double *C = co.data(), *W = C + n, *E = C + 2 * n, *Sx = C + 3 * n, *N = C + 4 * n;
+    const double* Bt = I.nz > 1 ? C + 5 * n : nullptr;
+    const double* Tp = I.nz > 1 ? C + 6 * n : nullptr;
     const int nx = I.nx, ny = I.ny, nz = I.nz, nxy = nx * ny;
     rp.assign(I.n + 1, 0);
     ci.clear();
@@ -1566,6 +1854,29 @@ static void csr_from_stencil(const Inst& I, std::vector<int>& rp, std::vector<in
         rp[c + 1] = (int)ci.size();
     }
 }
+// The FP64 coefficients of an instance in cell order, n each: the stored cell-order ones
+// (two-phase, mixed precision) or the Krylov ones (fp64 mode) moved out of red-black order.
+static std::vector<double> stored_coefs(const Inst& I, bool mixed) {
+    std::vector<double> co;
+    if (I.C) {
+        for (const DevPtr<double>* a : {&I.C, &I.W, &I.E, &I.S, &I.N, &I.B, &I.T}) {
+            if (!*a) continue;
+            std::vector<double> h = dev_to_host((*a)->p, I.n);
+            co.insert(co.end(), h.begin(), h.end());
+        }
+        return co;
+    }
+    EQ_REQUIRE(!mixed, "no FP64 coefficients stored");
+    const size_t h = I.h(), n = I.n;
+    std::vector<double> rb = dev_to_host((const double*)I.kcoef->p, I.ncoef() * 2 * h);
+    co.resize(I.ncoef() * n);
+    for (int q = 0; q < I.ncoef(); ++q)
+        for (size_t c = 0; c < n; ++c) {
+            const size_t i = c % I.nx, j = (c / I.nx) % I.ny, k = c / ((size_t)I.nx * I.ny);
+            co[q * n + c] = rb[q * 2 * h + ((i + j + k) & 1) * h + (c >> 1)];
+        }
+    return co;
+}
 
 void HeatBatch::export_residual(int slot, const double* u, const double* u0, double dt, std::vector<double>& F,
                                 std::vector<int>& rp, std::vector<int>& ci, std::vector<double>& v) {
@@ -1578,7 +1889,19 @@ void HeatBatch::export_residual(int slot, const double* u, const double* u0, dou
     TableP tp = p_->table({&I});
     p_->assemble(*tp);
     F = dev_to_host(I.F->p, I.n);
-    csr_from_stencil(I, rp, ci, v);
+    std::vector<double> co;
+    if (!p_->mixed) {
+        co = stored_coefs(I, false);
+    } else {   // mixed precision: the FP64 rows the refinement defect applies
+        auto f = dev_alloc<double>(I.n);
+        auto c = dev_alloc<double>(5 * (size_t)I.n);
+        hipLaunchKernelGGL(k_heat_rows64, dim3((unsigned)((I.n + BS - 1) / BS)), dim3(BS), 0, S(), tp->desc->p,
+                           f->p, c->p);
+        EQ_HIP(hipGetLastError());
+        co = dev_to_host(c->p, 5 * (size_t)I.n);
+        EQ_REQUIRE(dev_to_host(f->p, I.n) == F, "FP64 row recomputation differs from the assembly");
+    }
+    csr_from_stencil(I, co, rp, ci, v);
 }
 
 void HeatBatch::export_twophase_residual(int slot, int stage, const double* p, const double* sw0, const double* sw,
@@ -1597,7 +1920,7 @@ void HeatBatch::export_twophase_residual(int slot, int stage, const double* p, c
     TableP tp = p_->table({&I});
     p_->assemble(*tp);
     F = dev_to_host(I.F->p, I.n);
-    csr_from_stencil(I, rp, ci, v);
+    csr_from_stencil(I, stored_coefs(I, p_->mixed), rp, ci, v);
 }
 
 }  // namespace eqhip

@@ -43,8 +43,9 @@ struct FusedConfig {
 };
 
 struct LaunchShape {
-    int cpt = 0, xcd = 0;
+    int cpt = 0, xcd = 0;              // cells per thread of the SpMV; XCD block order
     int signatures = 0;                // memberships tuned so far
+    int cpt_class[5] = {};             // cells per thread: assembly, defect, SpMV, preconditioner, vector updates
 };
 
 struct StepStats {
@@ -104,14 +105,16 @@ public:
                                   double dt, std::vector<double>& F, std::vector<int>& rp, std::vector<int>& ci,
                                   std::vector<double>& v);
     size_t device_bytes(int slot) const;
-    // Average execution time (ms) of one launch of a kernel over every instance: the
-    // span from the first block's start to the last wave's end on the device wall clock
-    // (wall_clock64, hipDeviceAttributeWallClockRate), over `reps` launches on the
-    // runtime stream; event_ms gets the HIP-event time of the same dispatches, which
-    // also counts the dispatch itself.  Kernels: stencil_spmv | assemble | ilu_black |
-    // ilu_red | defect.
+    // Average execution time (ms) of one launch of a kernel over every instance, in the
+    // membership's tuned shape, with every instance active: each of `reps` launches on the
+    // runtime stream is bracketed by its own HIP start and stop events (hipExtLaunchKernel)
+    // and waited for.  event_ms gets the average of `reps` further launches queued back to
+    // back between one event pair (dispatch gaps included).  Kernels: operator_red,
+    // operator_black (preconditioner ilu0), spmv (jacobi, none), assemble, defect, b1, b2,
+    // b3, load_rhs.
     double time_kernel(const std::string& name, int reps, double* event_ms = nullptr);
-    static size_t estimate_bytes(int n, const std::string& precision, bool twophase = false);
+    static size_t estimate_bytes(int n, const std::string& precision, bool twophase = false,
+                                 const std::string& precond = "ilu0");
     // Launch shape used by the most recent launch and the number of tuned signatures.
     LaunchShape launch_shape() const;
 

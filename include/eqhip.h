@@ -203,13 +203,18 @@ int eqhip_batch_state(void* batch, int slot, double* u);
 int eqhip_batch_set_state(void* batch, int slot, const double* u);
 /* device-memory footprint of the slot in bytes (admission control) */
 int eqhip_batch_bytes(void* batch, int slot, double* bytes);
-/* launch shape of the most recent launch: info = {cells per thread, XCD-contiguous block
- * order, membership signatures tuned so far}.  Shapes are chosen per membership by
- * timing (params cpt=1|2|4|8 and xcd=0|1 fix it; tune=0 uses cpt=4, xcd=1). */
+/* launch shape of the most recent launch: info = {cells per thread of the SpMV, XCD-contiguous
+ * block order, membership signatures tuned so far}.  Shapes are chosen per membership by
+ * timing each kernel class (params cpt=1|2|4|8 and xcd=0|1 fix them for every class;
+ * tune=0 uses cpt=4, xcd=1). */
 int eqhip_batch_shape(void* batch, int* info);
-/* average execution time in ms of one launch over all slots: first block start to last
- * wave end on the device wall clock, over `reps` launches; event_ms (optional) gets the
- * HIP-event time of the same dispatches (hipExtLaunchKernel), which includes the dispatch.
+/* cells per thread of each kernel class of the most recent launch: cpt = {assembly, defect,
+ * SpMV, preconditioner, vector updates} (5 entries) */
+int eqhip_batch_shape_classes(void* batch, int* cpt);
+/* average execution time in ms of one launch over all slots (every slot active, tuned
+ * shape): each of `reps` launches is bracketed by its own HIP start/stop events
+ * (hipExtLaunchKernel) and waited for; event_ms (optional) gets the average of `reps`
+ * further launches queued back to back between one event pair (dispatch gaps included).
  * kernel = "stencil_spmv" | "assemble" | "ilu_black" | "ilu_red" | "defect" */
 int eqhip_batch_kernel_time(void* batch, const char* kernel, int reps, double* avg_ms, double* event_ms);
 /* F and the Jacobian (CSR, ascending columns) at (u, u0, dt); info = {n, nnz};
