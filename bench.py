@@ -7,18 +7,18 @@ zero-flux boundary, on device-resident state.  Metric: cell-Newton-iterations pe
 second = cells x Newton iterations / wall seconds over the timed steps.
 
 Runs on the same seeded input (16 Gaussian bumps + noise):
-  headline     fused engine, mixed precision: FP32 BiCGStab + red-black ILU(0),
-               FP64 residual, Jacobian, defect (iterative refinement) and Newton test
-  plain_fp64   fused engine, FP64 BiCGStab + red-black ILU(0)
+  headline     fused engine, mixed precision: FP32 BiCGStab on the red-black reduced
+               (Schur complement) system, preconditioned by its diagonal (= red-black
+               ILU(0)), inner tolerance 1e-6; FP64 residual, Jacobian, defect (iterative
+               refinement to 1e-6) and Newton test
+  plain_fp64   fused engine, the same Krylov method in FP64
   generic_fp64 the drop-in path: the same program through EquelleRuntimeCUDA's
                device AD engine (CSR Jacobian, SELL-64 SpMV, multicolour ILU(0))
-roofline: the Krylov SpMV of the headline run (dominant kernel, k_spmv_dot<float>):
-5-point stencil coefficients in FP32 with the dot products it feeds fused in;
-algorithmic bytes per launch 32 n (5 coefficients + x + dot operand read, y written,
-4 B each); average execution time over 50 launches on the final state of the timed
-run, measured on the device wall clock from the first block's start to the last
-wave's end (the HIP-event time of the same dispatches, which also counts the dispatch,
-is reported as event_us).
+kernels: every kernel of the headline's Newton and Krylov loop, timed on the final state
+of the timed run: 50 launches each between their own HIP start/stop events on the
+runtime stream (avg_us), and 50 more back to back (back_to_back_us); algorithmic bytes
+per launch as stated per kernel.  roofline: the kernel with the largest estimated time
+per step (avg_us x launches per step from the iteration counts).
 cpu_baseline: the CPU oracle (serial C++ restatement of the reference serial backend)
 timed for one Newton iteration of the same workload on one host core.
 
@@ -57,7 +57,7 @@ def timing_avg(lib, name):
     return cnt.value, (tot.value / cnt.value if cnt.value else float("nan"))
 
 
-def run_fused(lib, precision, u0, nx, ny, dt, steps, warmup, precond="ilu0", graph=True, inner_tol=1e-4):
+def run_fused(lib, precision, u0, nx, ny, dt, steps, warmup, precond="ilu0", graph=True, inner_tol=1e-6):
     params = (f"precision={precision} preconditioner={precond} rel_res_tol={NONLINEAR_HEAT['rel_res_tol']} "
               f"abs_res_tol=0 max_iter=20 solver_tol=1e-6 inner_tol={inner_tol} graph={int(graph)}").encode()
     b = C.c_void_p()
@@ -240,9 +240,9 @@ def bench_twophase(lib, args):
     pressure Newton solve and a saturation Newton solve, each to 1e-8 relative residual.
     Work = cells x (pressure + transport Newton iterations).
 
-    headline      fused stencil engine, mixed precision (FP32 BiCGStab + red-black ILU(0),
-                  FP64 residual / Jacobian / refinement)
-    fused_fp64    the same engine, FP64 BiCGStab + ILU(0)
+    headline      fused stencil engine, mixed precision (FP32 BiCGStab on the red-black
+                  reduced system = red-black ILU(0); FP64 residual / Jacobian / refinement)
+    fused_fp64    the same engine, the same Krylov method in FP64
     generic_*     the drop-in path: the printer-style program on the device AD engine with
                   GMRES(30) + Jacobi (mixed and FP64)"""
     from equelle_amd.workloads import TWOPHASE, twophase_inputs, twophase_params
@@ -262,8 +262,12 @@ def bench_twophase(lib, args):
                                    dptr(sv), C.byref(slot))
             sl = np.array([slot.value], dtype=np.int32)
             st, tps = np.zeros(5, dtype=np.int32), np.zeros(3, dtype=np.int32)
-            for _ in range(args.warmup):
+            for w_ in range(args.warmup):
                 lib.batch_step(b, iptr(sl), 1, dt)
+                lib.batch_stats(b, slot.value, iptr(st), None)
+                lib.batch_twophase_stats(b, slot.value, iptr(tps))
+                if st[3] == 0 or st[4] or tps[2] == 0:
+                    raise RuntimeError(f"{name}: warmup step {w_} did not converge: {st.tolist()} {tps.tolist()}")
             lib.device_sync()
             newton = krylov = kp = 0
             t0 = time.perf_counter()
@@ -423,7 +427,7 @@ def main():
     ap.add_argument("--precond", default="ilu0", help="preconditioner of the fused runs: ilu0 | diagonal | none")
     ap.add_argument("--workload", default="heat2048", choices=["heat2048", "batched", "serving", "twophase", "copy"])
     ap.add_argument("--instances", type=int, default=512)
-    ap.add_argument("--inner-tol", dest="inner_tol", type=float, default=1e-4,
+    ap.add_argument("--inner-tol", dest="inner_tol", type=float, default=1e-6,
                     help="mixed precision: FP32 inner solve tolerance")
     ap.add_argument("--no-graph", dest="no_graph", action="store_true", help="eager launches (profiling)")
     ap.add_argument("--grid3", default="128x128x64", help="two-phase grid nx x ny x nz")
@@ -472,8 +476,9 @@ def main():
         "dtype": "f64 residual/Jacobian/refinement, f32 Krylov",
         "data": "synthetic",
         "config": {"workload": f"nonlinear heat {nx}x{ny}, k(u)=0.1(1+u^2), dt=1, zero-flux, Newton to 1e-8 "
-                               "relative residual; mixed precision (FP32 BiCGStab + red-black ILU(0), FP64 "
-                               "refinement to 1e-6)",
+                               "relative residual; mixed precision (FP32 BiCGStab on the red-black reduced "
+                               "system with red-black ILU(0) pivots, inner tolerance "
+                               f"{args.inner_tol:g}, FP64 refinement to 1e-6)",
                    "cells": n, "jacobian_nnz": nnz, "engine": "fused heat engine (1 instance)"},
         "roofline": {"bound": "hbm", "achieved": kt[dom]["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": kt[dom]["frac"],

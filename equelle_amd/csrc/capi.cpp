@@ -628,6 +628,9 @@ static FusedConfig fused_config(const char* params) {
     c.solver_tol = m.get_double("solver_tol", c.solver_tol);
     c.solver_max_iter = m.get_int("solver_max_iter", c.solver_max_iter);
     c.inner_tol = m.get_double("inner_tol", c.inner_tol);
+    c.inner_tol_twophase = m.get_double("inner_tol_twophase", c.inner_tol_twophase);
+    c.krylov_form_heat = m.get_string("krylov_form_heat", c.krylov_form_heat);
+    c.krylov_form_twophase = m.get_string("krylov_form_twophase", c.krylov_form_twophase);
     c.max_refine = m.get_int("max_refine", c.max_refine);
     c.graph = m.get_int("graph", 1) != 0;
     c.cpt = m.get_int("cpt", c.cpt);

@@ -19,11 +19,18 @@ namespace {
 constexpr int BS = 256;   // cells per block (4 wave64)
 inline hipStream_t S() { return Device::get().stream(); }
 
-enum Slot { S_RHO = 0, S_RHO_OLD, S_ALPHA, S_OMEGA, S_RR, S_BB, S_BREAK, S_IT, S_FN, S_RN, S_BN, S_REF, S_ITT, NSLOT = 16 };
+// per-instance scalars; S_TOL2: the square of the instance's inner (mixed-precision) Krylov tolerance
+// S_MEAN: the mean of the Krylov right-hand side over the cells (singular systems only)
+enum Slot { S_RHO = 0, S_RHO_OLD, S_ALPHA, S_OMEGA, S_RR, S_BB, S_BREAK, S_IT, S_FN, S_RN, S_BN, S_REF, S_ITT, S_TOL2,
+            S_MEAN, NSLOT = 16 };
 // Krylov vectors; V_Y is the red-rank scratch of the reduced operator (ilu0 only)
 enum Vec { V_R = 0, V_RH, V_P, V_V, V_S, V_T, V_PH, V_SH, V_X, V_Y, NVEC };
-enum Act { A_INIT = 0, A_ALPHA, A_OMEGA, A_RHO, A_STORE };
+enum Act { A_INIT = 0, A_ALPHA, A_OMEGA, A_RHO, A_STORE, A_MEAN };
 enum Pc { PC_NONE = 0, PC_JACOBI = 1, PC_ILU = 2 };
+// Krylov form and right preconditioner of one instance: the full system with no
+// preconditioner, Jacobi, or red-black ILU(0) applied in two passes; or the red-black
+// reduced (Schur complement) system with its diagonal (= red-black ILU(0)).
+enum Pm { PM_NONE = 0, PM_JACOBI = 1, PM_ILU = 2, PM_SCHUR = 3 };
 
 enum Kind { K_HEAT = 0, K_TWOPHASE = 1 };
 
@@ -34,6 +41,7 @@ struct Desc {
     int corey;
     int gbase, ngrp;              // the instance's partials: groups [gbase, gbase + ngrp) of BS cells
     int h;                        // pair ranks (n + 1) / 2: cells per colour, rounded up
+    int pcm;                      // Krylov form / preconditioner (Pm)
     int ns;                       // Krylov slots per pair rank: 1 (reduced system) or 2 (full system)
     int ngrpk;                    // Krylov partials: groups [gbase, gbase + ngrpk) of BS pairs
     double rnx, rnxy;             // 1 / nx, 1 / (nx ny): cell -> (i, j, k) without integer division
@@ -237,9 +245,13 @@ __device__ void fin_body(const Desc& d, const double* part, int nb, const Fin& f
             sc[S_ITT] += 1.0;
             const bool ok = isfinite(res[1]) && sc[S_OMEGA] != 0.0;
             if (!ok) sc[S_BREAK] = 2.0;
-            d.act[0] = ok && res[1] > tol2 * sc[S_BB] && sc[S_IT] < maxit && res[0] != 0.0;
+            // tol2 < 0: the instance's own tolerance (S_TOL2)
+            d.act[0] = ok && res[1] > (tol2 < 0.0 ? sc[S_TOL2] : tol2) * sc[S_BB] && sc[S_IT] < maxit && res[0] != 0.0;
             break;
         }
+        case A_MEAN:   // Krylov right-hand side sum -> its mean over the cells (singular systems)
+            sc[S_MEAN] = res[0] / (double)(d.ns == 1 ? d.n / 2 : d.n);   // black cells or all cells
+            break;
         default:
             sc[slot] = res[0];
             break;
@@ -535,8 +547,14 @@ __global__ __launch_bounds__(BS) void k_defect(const Desc* __restrict__ D, const
 // system's.  S is applied matrix-free in two passes: y_r = C_r^-1 A_rb v_b over red
 // cells, then S v = C_b v_b - A_br y_r over black cells.  Krylov vectors have one slot
 // per black rank (ns = 1).
-// Preconditioner jacobi / none: BiCGStab on the full system in the same storage, two
-// slots per pair (red rank p at p, black rank p at h + p; ns = 2), y = A x in one pass.
+// Full system (PM_ILU, PM_JACOBI, PM_NONE): BiCGStab on A in the same storage, two slots
+// per pair (red rank p at p, black rank p at h + p; ns = 2), y = A x in one pass; PM_ILU
+// applies M^-1 in two passes (black: z_b = (r_b - A_br C_r^-1 r_r) / P_b; red: z_r =
+// (r_r - A_rb z_b) / C_r).  The reduced form moves fewer bytes per iteration; the full
+// form is kept for systems where FP32 rounding in the reduced operator costs
+// convergence (the singular two-phase pressure system).  The form is chosen per
+// instance kind, and instances of both forms can share a batch: each operator kernel
+// skips the instances of the other form.
 //
 // Every Krylov kernel runs one thread per pair rank p in [0, h); its partials are groups
 // of BS pairs, [gbase, gbase + ngrpk).
@@ -575,7 +593,7 @@ __device__ inline T offdiag(const Desc& d, int s, int c, int i, int j, int k, co
 template <class T, int CPT>
 __global__ __launch_bounds__(BS) void k_schur_red(const Desc* __restrict__ D, const int2* __restrict__ blk, int iv) {
     LOCATE();
-    if (!d.act[0]) return;
+    if (d.pcm != PM_SCHUR || !d.act[0]) return;
     const T* __restrict__ v = (const T*)d.v[iv];
     const T* __restrict__ dv = (const T*)d.dinv;
     T* __restrict__ y = (T*)d.v[V_Y];
@@ -592,6 +610,7 @@ __global__ __launch_bounds__(BS) void k_schur_black(const Desc* __restrict__ D, 
                                                     int iv, int iw, int nd, int a0, int a1,
                                                     double* __restrict__ part, int ng) {
     LOCATE();
+    if (d.pcm != PM_SCHUR) return;   // block-uniform: the full-system kernel writes these partials
     double s0[CPT] = {}, s1[CPT] = {};
     if (d.act[0]) {
         const int h = d.h;
@@ -619,6 +638,7 @@ template <class T, int CPT>
 __global__ __launch_bounds__(BS) void k_spmv_rb(const Desc* __restrict__ D, const int2* __restrict__ blk, int ix,
                                                 int iy, int nd, int a0, int a1, double* __restrict__ part, int ng) {
     LOCATE();
+    if (d.ns != 2) return;   // block-uniform: the reduced-system kernel writes these partials
     double s0[CPT] = {}, s1[CPT] = {};
     if (d.act[0]) {
         const int h = d.h;
@@ -645,15 +665,14 @@ __global__ __launch_bounds__(BS) void k_spmv_rb(const Desc* __restrict__ D, cons
     if (nd > 1) group_store<CPT>(s1, PARTSK(part + ng));
 }
 
-// Preconditioner set-up (pairs): dinv[p] = 1 / C_r of red rank p; dinv[h + p] of black
-// rank p = 1 / P_b (ilu0: C_b - sum_m A_bm A_mb / C_m over the red neighbours m, the
-// ILU(0) pivot = diag of the Schur complement), 1 / C_b (jacobi) or 1 (none); 0 for
-// ranks without a cell.
+// Preconditioner set-up (pairs): dinv[p] = 1 / C_r of red rank p (1 for PM_NONE);
+// dinv[h + p] of black rank p = 1 / P_b (PM_ILU, PM_SCHUR: C_b - sum_m A_bm A_mb / C_m over
+// the red neighbours m, the ILU(0) pivot = diag of the Schur complement), 1 / C_b
+// (PM_JACOBI) or 1 (PM_NONE); 0 for ranks without a cell.
 template <class T, int CPT>
-__global__ __launch_bounds__(BS) void k_precond_setup(const Desc* __restrict__ D, const int2* __restrict__ blk,
-                                                      int mode) {
+__global__ __launch_bounds__(BS) void k_precond_setup(const Desc* __restrict__ D, const int2* __restrict__ blk) {
     LOCATE();
-    const int h = d.h, nx = d.nx, ny = d.ny, nxy = d.nx * d.ny;
+    const int h = d.h, nx = d.nx, ny = d.ny, nxy = d.nx * d.ny, mode = d.pcm;
     const T* K[7];
     for (int q = 0; q < 7; ++q) K[q] = (const T*)d.K[q];
     T* __restrict__ dv = (T*)d.dinv;
@@ -663,7 +682,7 @@ __global__ __launch_bounds__(BS) void k_precond_setup(const Desc* __restrict__ D
         if (c < 0) dv[p] = (T)0;
         else {
             const T cr = K[0][p];
-            dv[p] = mode == PC_NONE ? (T)1 : cr != (T)0 ? (T)1 / cr : (T)1;
+            dv[p] = mode == PM_NONE ? (T)1 : cr != (T)0 ? (T)1 / cr : (T)1;
         }
         c = cell_of(d, p, 1, i, j, k);
         if (c < 0) {
@@ -672,7 +691,7 @@ __global__ __launch_bounds__(BS) void k_precond_setup(const Desc* __restrict__ D
         }
         const int s = h + p;
         T piv = K[0][s];
-        if (mode == PC_ILU) {
+        if (mode == PM_ILU || mode == PM_SCHUR) {
             // A_bm of the black row times A_mb of the red neighbour's row (opposite direction)
             if (i > 0) piv -= K[1][s] * K[2][(c - 1) >> 1] / K[0][(c - 1) >> 1];
             if (i < nx - 1) piv -= K[2][s] * K[1][(c + 1) >> 1] / K[0][(c + 1) >> 1];
@@ -683,22 +702,100 @@ __global__ __launch_bounds__(BS) void k_precond_setup(const Desc* __restrict__ D
                 if (k < d.nz - 1) piv -= K[6][s] * K[5][(c + nxy) >> 1] / K[0][(c + nxy) >> 1];
             }
         }
-        dv[s] = mode == PC_NONE ? (T)1 : piv != (T)0 ? (T)1 / piv : (T)1;
+        dv[s] = mode == PM_NONE ? (T)1 : piv != (T)0 ? (T)1 / piv : (T)1;
     }
 }
 
-// Krylov start: r = b (already in r), rh = r, x = 0, p = v = 0; partial dots (rh.r, r.r).
+// Full-system red-black ILU(0) apply z = M^-1 r (PM_ILU instances), pass 1 over black
+// ranks: z_b = (r_b - sum_m A_bm r_m / C_m) / P_b (1 / C_m of a red neighbour is its dinv).
+template <class T, int CPT>
+__global__ __launch_bounds__(BS) void k_ilu_black(const Desc* __restrict__ D, const int2* __restrict__ blk, int ir,
+                                                  int iz) {
+    LOCATE();
+    if (d.pcm != PM_ILU || !d.act[0]) return;
+    const int h = d.h, nx = d.nx, ny = d.ny, nxy = d.nx * d.ny;
+    const T* __restrict__ r = (const T*)d.v[ir];
+    const T* __restrict__ dv = (const T*)d.dinv;
+    T* __restrict__ z = (T*)d.v[iz];
+    FOR_PAIRS(p) {
+        int i, j, k;
+        const int c = cell_of(d, p, 1, i, j, k);
+        const int s = h + p;
+        if (c < 0) {
+            z[s] = (T)0;
+            continue;
+        }
+        T acc = (T)0;
+        if (i > 0) acc += ((const T*)d.K[1])[s] * (r[(c - 1) >> 1] * dv[(c - 1) >> 1]);
+        if (i < nx - 1) acc += ((const T*)d.K[2])[s] * (r[(c + 1) >> 1] * dv[(c + 1) >> 1]);
+        if (j > 0) acc += ((const T*)d.K[3])[s] * (r[(c - nx) >> 1] * dv[(c - nx) >> 1]);
+        if (j < ny - 1) acc += ((const T*)d.K[4])[s] * (r[(c + nx) >> 1] * dv[(c + nx) >> 1]);
+        if (d.nz > 1) {
+            if (k > 0) acc += ((const T*)d.K[5])[s] * (r[(c - nxy) >> 1] * dv[(c - nxy) >> 1]);
+            if (k < d.nz - 1) acc += ((const T*)d.K[6])[s] * (r[(c + nxy) >> 1] * dv[(c + nxy) >> 1]);
+        }
+        z[s] = (r[s] - acc) * dv[s];
+    }
+}
+// Pass 2 over red ranks: z_r = (r_r - A_rb z_b) / C_r.
+template <class T, int CPT>
+__global__ __launch_bounds__(BS) void k_ilu_red(const Desc* __restrict__ D, const int2* __restrict__ blk, int ir,
+                                                int iz) {
+    LOCATE();
+    if (d.pcm != PM_ILU || !d.act[0]) return;
+    const T* __restrict__ r = (const T*)d.v[ir];
+    const T* __restrict__ dv = (const T*)d.dinv;
+    T* __restrict__ z = (T*)d.v[iz];
+    FOR_PAIRS(p) {
+        int i, j, k;
+        const int c = cell_of(d, p, 0, i, j, k);
+        z[p] = c < 0 ? (T)0 : (r[p] - offdiag<T>(d, p, c, i, j, k, z + d.h)) * dv[p];
+    }
+}
+
+// The two-phase pressure Jacobian (no-flow boundary, balanced sources) is singular with
+// the constants as null space (its row sums are zero).  Rounded to FP32 its row sums are
+// no longer exactly zero, and a right-hand side component along the constants is then
+// amplified without bound by the Krylov solve.  For these instances the right-hand side
+// is projected onto the range before the solve: its mean over the cells is subtracted
+// (Krylov start below).  The FP64 refinement defect is computed in full, so the
+// projection only keeps the inner solves well posed.
+__device__ inline bool singular(const Desc& d) { return d.kind == K_TWOPHASE && d.act[2] == 0; }
+// Partial sums of the Krylov right-hand side over the slots (for the mean above).
+template <class T, int CPT>
+__global__ __launch_bounds__(BS) void k_rhs_sum(const Desc* __restrict__ D, const int2* __restrict__ blk,
+                                                double* __restrict__ part) {
+    LOCATE();
+    double a[CPT] = {};
+    if (d.act[1] && singular(d)) {
+        FOR_GROUPS(m, p) {
+            if (p >= d.h) continue;
+            for (int w = 0; w < d.ns; ++w) a[m] += (double)((const T*)d.v[V_R])[p + w * d.h];
+        }
+    }
+    group_store<CPT>(a, PARTSK(part));
+}
+
+// Krylov start: r = b (already in r; minus its mean for singular systems), rh = r, x = 0,
+// p = v = 0; partial dots (rh.r, r.r).
 template <class T, int CPT>
 __global__ __launch_bounds__(BS) void k_kinit(const Desc* __restrict__ D, const int2* __restrict__ blk,
                                               double* __restrict__ part, int ng) {
     LOCATE();
     double rr[CPT] = {};
     if (d.act[1]) {
+        const bool proj = singular(d);
+        const T mean = proj ? (T)d.sc[S_MEAN] : (T)0;
         FOR_GROUPS(m, p) {
             if (p >= d.h) continue;
             for (int w = 0; w < d.ns; ++w) {
                 const int s = p + w * d.h;
-                const T rc = ((T*)d.v[V_R])[s];
+                T rc = ((T*)d.v[V_R])[s];
+                if (proj) {   // project onto the range: subtract the mean over the slots holding a cell
+                    int i, j, k;
+                    if (cell_of(d, p, d.ns == 1 ? 1 : w, i, j, k) >= 0) rc -= mean;
+                    ((T*)d.v[V_R])[s] = rc;
+                }
                 ((T*)d.v[V_RH])[s] = rc;
                 ((T*)d.v[V_X])[s] = 0;
                 ((T*)d.v[V_P])[s] = 0;
@@ -718,7 +815,8 @@ __device__ inline const T* slot_dinv(const Desc& d) {
     return (const T*)d.dinv + (d.ns == 1 ? d.h : 0);
 }
 
-// p = r + beta (p - omega v) (first iteration: p = r); ph = M^-1 p
+// p = r + beta (p - omega v) (first iteration: p = r); ph = M^-1 p (diagonal M; PM_ILU:
+// by the ILU passes)
 template <class T, int CPT>
 __global__ __launch_bounds__(BS) void k_b1(const Desc* __restrict__ D, const int2* __restrict__ blk) {
     LOCATE();
@@ -735,11 +833,11 @@ __global__ __launch_bounds__(BS) void k_b1(const Desc* __restrict__ D, const int
             const T r = ((const T*)d.v[V_R])[s];
             const T pv = first ? r : r + beta * (p[s] - omega * ((const T*)d.v[V_V])[s]);
             p[s] = pv;
-            ((T*)d.v[V_PH])[s] = dv[s] * pv;
+            if (d.pcm != PM_ILU) ((T*)d.v[V_PH])[s] = dv[s] * pv;
         }
     }
 }
-// s = r - alpha v ; sh = M^-1 s
+// s = r - alpha v ; sh = M^-1 s (diagonal M; PM_ILU: by the ILU passes)
 template <class T, int CPT>
 __global__ __launch_bounds__(BS) void k_b2(const Desc* __restrict__ D, const int2* __restrict__ blk) {
     LOCATE();
@@ -751,7 +849,7 @@ __global__ __launch_bounds__(BS) void k_b2(const Desc* __restrict__ D, const int
             const int s = q + w * d.h;
             const T sv = ((const T*)d.v[V_R])[s] - alpha * ((const T*)d.v[V_V])[s];
             ((T*)d.v[V_S])[s] = sv;
-            ((T*)d.v[V_SH])[s] = dv[s] * sv;
+            if (d.pcm != PM_ILU) ((T*)d.v[V_SH])[s] = dv[s] * sv;
         }
     }
 }
@@ -880,7 +978,9 @@ __global__ __launch_bounds__(BS) void k_load_rhs(const Desc* __restrict__ D, con
 template <class T, int CPT>
 __global__ __launch_bounds__(BS) void k_accum(const Desc* __restrict__ D, const int2* __restrict__ blk, int scaled) {
     LOCATE();
-    if (!d.act[1]) return;
+    // a solve whose last residual is not finite (breakdown, overflow) leaves du unchanged:
+    // the refinement defect then shows no progress and the refinement stops
+    if (!d.act[1] || !isfinite(d.sc[S_RR])) return;
     const double rn = d.sc[S_RN];
     const double scale = scaled ? 1.0 / rn : 1.0;
     const int h = d.h;
@@ -948,8 +1048,10 @@ struct Inst {
     DevPtr<int> act;
     enum State { IDLE, START, SOLVE, DONE } state = IDLE;
     StepStats st;
+    int pcm = PM_SCHUR;            // Krylov form / preconditioner
     int ncoef() const { return nz > 1 ? 7 : 5; }
     int h() const { return (n + 1) / 2; }
+    int ns() const { return pcm == PM_SCHUR ? 1 : 2; }
 };
 
 // Kernel classes; each has its own cells per thread: assembly, refinement defect, the
@@ -986,6 +1088,8 @@ struct Table {
     DevPtr<int2> pblk[4];    // pair kernels (Krylov space): the same over each instance's h pair ranks
     int pnb[4] = {};
     int ng = 0;              // partial groups over all instances (an instance has ceil(n / BS))
+    bool has_schur = false, has_full = false, has_ilu = false;   // Krylov forms present
+    bool has_tp = false;                                         // two-phase instances present
     Shape shape;
     DevPtr<double> part;
     uint64_t version = 0;
@@ -1067,7 +1171,8 @@ struct HeatBatch::Impl {
         d.T = p64(I.T);
         const size_t h = I.h();
         d.h = (int)h;
-        d.ns = pc == PC_ILU ? 1 : 2;
+        d.pcm = I.pcm;
+        d.ns = I.ns();
         for (int q = 0; q < 7; ++q) d.K[q] = q < I.ncoef() ? I.kcoef->p + q * 2 * h * ks() : nullptr;
         d.dinv = I.dinv->p;
         for (int v = 0; v < NVEC; ++v) d.v[v] = I.vecs->p + (size_t)v * d.ns * h * ks();
@@ -1092,6 +1197,10 @@ struct HeatBatch::Impl {
             d.ngrpk = (list[q]->h() + BS - 1) / BS;
             g += d.ngrp;
             ds.push_back(d);
+            t.has_schur |= d.pcm == PM_SCHUR;
+            t.has_full |= d.pcm != PM_SCHUR;
+            t.has_ilu |= d.pcm == PM_ILU;
+            t.has_tp |= d.kind == K_TWOPHASE;
         }
         t.ng = g;
         for (int ci = 0; ci < 4; ++ci) {
@@ -1160,9 +1269,14 @@ struct HeatBatch::Impl {
         switch (cls) {
             case C_ASM: launch_assemble(t); break;
             case C_DEF: launch_defect(t); break;
-            case C_SPMV:
-                if (mixed) launch_op<float>(t, V_PH, V_V, 1, (int)V_RH, -1);
-                else launch_op<double>(t, V_PH, V_V, 1, (int)V_RH, -1);
+            case C_SPMV:   // the operator and the ILU passes
+                if (mixed) {
+                    ilu_apply<float>(t, V_P, V_PH);
+                    launch_op<float>(t, V_PH, V_V, 1, (int)V_RH, -1);
+                } else {
+                    ilu_apply<double>(t, V_P, V_PH);
+                    launch_op<double>(t, V_PH, V_V, 1, (int)V_RH, -1);
+                }
                 break;
             case C_PC:
                 if (mixed) conversions<float>(t);
@@ -1294,16 +1408,22 @@ struct HeatBatch::Impl {
         TimedLaunch tl("defect_fp64");
         HL(C_DEF, (k_defect<CP>), t.part->p);
     }
-    // w = A v (full system) or S v (reduced system) with the fused dots
+    // w = A v (full-system instances) and S v (reduced-system instances) with the fused dots
     template <class T>
     void launch_op(const Table& t, int iv, int iw, int nd, int a0, int a1) {
         TimedLaunch tl(sizeof(T) == 4 ? "operator_fp32" : "operator_fp64");
-        if (pc == PC_ILU) {
+        if (t.has_schur) {
             HLP(C_SPMV, (k_schur_red<T, CP>), iv);
             HLP(C_SPMV, (k_schur_black<T, CP>), iv, iw, nd, a0, a1, t.part->p, t.ng);
-        } else {
-            HLP(C_SPMV, (k_spmv_rb<T, CP>), iv, iw, nd, a0, a1, t.part->p, t.ng);
         }
+        if (t.has_full) HLP(C_SPMV, (k_spmv_rb<T, CP>), iv, iw, nd, a0, a1, t.part->p, t.ng);
+    }
+    // z = M^-1 r for the full-system ILU(0) instances
+    template <class T>
+    void ilu_apply(const Table& t, int ir, int iz) {
+        if (!t.has_ilu) return;
+        HLP(C_SPMV, (k_ilu_black<T, CP>), ir, iz);
+        HLP(C_SPMV, (k_ilu_red<T, CP>), ir, iz);
     }
     template <class T>
     void spmv_dot(const Table& t, int ix, int iy, int nd, int a0, int a1, const Fin& f) {
@@ -1327,11 +1447,13 @@ struct HeatBatch::Impl {
     template <class T>
     void bicg_iteration(const Table& t, double tol, int maxit) {
         HLP(C_VEC, (k_b1<T, CP>));
+        ilu_apply<T>(t, V_P, V_PH);
         spmv_dot<T>(t, V_PH, V_V, 1, (int)V_RH, -1, F(A_ALPHA, 1));   // v = A ph; alpha = rho / rh.v
         HLP(C_VEC, (k_b2<T, CP>));
+        ilu_apply<T>(t, V_S, V_SH);
         spmv_dot<T>(t, V_SH, V_T, 2, (int)V_S, -1, F(A_OMEGA, 2));    // t = A sh; omega = t.s / t.t
         HLP(C_VEC, (k_b3<T, CP>), t.part->p, t.ng);
-        fin(t, F(A_RHO, 2, tol * tol, maxit));
+        fin(t, F(A_RHO, 2, tol < 0.0 ? -1.0 : tol * tol, maxit));
     }
 
     // One BiCGStab solve.  From the second solve on a membership, one iteration is
@@ -1343,6 +1465,10 @@ struct HeatBatch::Impl {
     // iterations are no-ops.
     template <class T>
     void bicgstab(Table& t, double tol, int maxit) {
+        if (t.has_tp) {   // two-phase pressure: mean of the right-hand side
+            HLP(C_VEC, (k_rhs_sum<T, CP>), t.part->p);
+            fin(t, F(A_MEAN, 1));
+        }
         HLP(C_VEC, (k_kinit<T, CP>), t.part->p, t.ng);
         fin(t, F(A_INIT, 2));
         const auto key = std::make_pair(tol, maxit);
@@ -1405,7 +1531,7 @@ struct HeatBatch::Impl {
         for (int ref = 0; ref < cfg.max_refine; ++ref) {
             if (ref > 0 && !count_active(t, 1)) break;   // all start active (||F|| > 0)
             HLP(C_PC, (k_load_rhs<T, CP>), 1);
-            bicgstab<T>(t, cfg.inner_tol, cfg.solver_max_iter);
+            bicgstab<T>(t, -1.0, cfg.solver_max_iter);   // per-instance inner tolerance (S_TOL2)
             HLP(C_PC, (k_accum<T, CP>), 1);
             launch_defect(t);
             fin(t, Fcell(S_FN));
@@ -1415,7 +1541,14 @@ struct HeatBatch::Impl {
     }
     template <class T>
     void precond_setup(const Table& t) {
-        HLP(C_PC, (k_precond_setup<T, CP>), pc);
+        HLP(C_PC, (k_precond_setup<T, CP>));
+    }
+    // Krylov form of a new instance: reduced system for ilu0 where the kind's option asks
+    // for it, else the full system with the batch's preconditioner.
+    int form_of(int kind) const {
+        if (pc != PC_ILU) return pc == PC_JACOBI ? PM_JACOBI : PM_NONE;
+        const std::string& f = kind == K_HEAT ? cfg.krylov_form_heat : cfg.krylov_form_twophase;
+        return f == "reduced" ? PM_SCHUR : PM_ILU;
     }
     void newton_update(const Table& t) { HL(C_VEC, (k_newton_update<CP>)); }
 #undef HL
@@ -1457,12 +1590,12 @@ HeatBatch::~HeatBatch() {
     delete p_;
 }
 
-size_t HeatBatch::estimate_bytes(int n, const std::string& precision, bool twophase, const std::string& precond) {
+size_t HeatBatch::estimate_bytes(int n, const std::string& precision, bool twophase, bool reduced) {
     const bool mixed = precision == "mixed";
     const size_t ks = mixed ? 4 : 8;
     const size_t h = ((size_t)n + 1) / 2;
     const size_t ncoef = twophase ? 7 : 5;
-    const size_t slots = precond == "ilu0" ? h : 2 * h;
+    const size_t slots = reduced ? h : 2 * h;
     // F, du, r64; Krylov-precision coefficients and dinv (2 h each); Krylov vectors
     size_t b = (size_t)n * 3 * 8 + (ncoef + 1) * 2 * h * ks + (size_t)NVEC * slots * ks;
     if (!twophase) b += (size_t)n * 2 * 8;   // u, u0
@@ -1474,7 +1607,7 @@ size_t HeatBatch::estimate_bytes(int n, const std::string& precision, bool twoph
 // cell order are stored only for two-phase instances in mixed precision (their
 // refinement defect reads them); the Krylov-precision coefficients, dinv and the Krylov
 // vectors are in red-black rank order.
-static void alloc_solver(Inst& I, bool mixed, size_t ks, int pc) {
+static void alloc_solver(Inst& I, bool mixed, size_t ks, double inner_tol) {
     const int n = I.n;
     const size_t h = I.h();
     I.F = dev_alloc<double>(n);
@@ -1495,10 +1628,10 @@ static void alloc_solver(Inst& I, bool mixed, size_t ks, int pc) {
     I.act = dev_alloc<int>(4);
     EQ_HIP(hipMemsetAsync(I.sc->p, 0, NSLOT * sizeof(double), S()));
     EQ_HIP(hipMemsetAsync(I.act->p, 0, 4 * sizeof(int), S()));
+    launch_fill(I.sc->p + S_TOL2, inner_tol * inner_tol, 1);
     I.kcoef = dev_alloc<char>((size_t)I.ncoef() * 2 * h * ks);
     I.dinv = dev_alloc<char>(2 * h * ks);
-    const size_t slots = pc == PC_ILU ? h : 2 * h;
-    I.vecs = dev_alloc<char>((size_t)NVEC * slots * ks);
+    I.vecs = dev_alloc<char>((size_t)NVEC * I.ns() * h * ks);
 }
 
 int HeatBatch::add(int nx, int ny, double k, double cv, const double* u_host, double dx, double dy) {
@@ -1515,7 +1648,8 @@ int HeatBatch::add(int nx, int ny, double k, double cv, const double* u_host, do
     I->dy = dy;
     I->u = dev_from_host(u_host, I->n);
     I->u0 = dev_alloc<double>(I->n);
-    alloc_solver(*I, p_->mixed, p_->ks(), p_->pc);
+    I->pcm = p_->form_of(K_HEAT);
+    alloc_solver(*I, p_->mixed, p_->ks(), p_->cfg.inner_tol);
     const int id = I->id;
     p_->insts[id] = std::move(I);
     ++p_->version;
@@ -1569,7 +1703,8 @@ int HeatBatch::add_twophase(const TwoPhaseSpec& sp, const double* perm, const do
     launch_fill(I->p->p, 0.0, n);
     I->sw = dev_from_host(sw_host, n);
     I->sw0 = dev_alloc<double>(n);
-    alloc_solver(*I, p_->mixed, p_->ks(), p_->pc);
+    I->pcm = p_->form_of(K_TWOPHASE);
+    alloc_solver(*I, p_->mixed, p_->ks(), p_->cfg.inner_tol_twophase);
     const int id = I->id;
     p_->insts[id] = std::move(I);
     ++p_->version;
@@ -1596,7 +1731,7 @@ bool HeatBatch::in_flight(int slot) const {
 const StepStats& HeatBatch::stats(int slot) const { return get_inst(p_, slot).st; }
 size_t HeatBatch::device_bytes(int slot) const {
     const Inst& I = get_inst(p_, slot);
-    return estimate_bytes(I.n, p_->cfg.precision, I.kind == K_TWOPHASE, p_->cfg.precond);
+    return estimate_bytes(I.n, p_->cfg.precision, I.kind == K_TWOPHASE, I.pcm == PM_SCHUR);
 }
 
 static const int kStage[2] = {0, 1};
@@ -1744,14 +1879,21 @@ double HeatBatch::time_kernel(const std::string& name, int reps, double* event_m
                                   t.pblk[ci_]->p, ##__VA_ARGS__);                                               \
         });                                                                                                     \
     }
-    const bool red = P.pc == PC_ILU;
+    bool red = false, full = false, ilu = false;
+    for (Inst* I : all) red |= I->pcm == PM_SCHUR, full |= I->pcm != PM_SCHUR, ilu |= I->pcm == PM_ILU;
     if (name == "operator_red" && red) {
         if (P.mixed) TLP(C_SPMV, (k_schur_red<float, CP>), (int)V_PH);
         else TLP(C_SPMV, (k_schur_red<double, CP>), (int)V_PH);
     } else if (name == "operator_black" && red) {
         if (P.mixed) TLP(C_SPMV, (k_schur_black<float, CP>), (int)V_PH, (int)V_V, 1, (int)V_RH, -1, t.part->p, t.ng);
         else TLP(C_SPMV, (k_schur_black<double, CP>), (int)V_PH, (int)V_V, 1, (int)V_RH, -1, t.part->p, t.ng);
-    } else if (name == "spmv" && !red) {
+    } else if (name == "ilu_black" && ilu) {
+        if (P.mixed) TLP(C_SPMV, (k_ilu_black<float, CP>), (int)V_P, (int)V_PH);
+        else TLP(C_SPMV, (k_ilu_black<double, CP>), (int)V_P, (int)V_PH);
+    } else if (name == "ilu_red" && ilu) {
+        if (P.mixed) TLP(C_SPMV, (k_ilu_red<float, CP>), (int)V_P, (int)V_PH);
+        else TLP(C_SPMV, (k_ilu_red<double, CP>), (int)V_P, (int)V_PH);
+    } else if (name == "spmv" && full) {
         if (P.mixed) TLP(C_SPMV, (k_spmv_rb<float, CP>), (int)V_PH, (int)V_V, 1, (int)V_RH, -1, t.part->p, t.ng);
         else TLP(C_SPMV, (k_spmv_rb<double, CP>), (int)V_PH, (int)V_V, 1, (int)V_RH, -1, t.part->p, t.ng);
     } else if (name == "assemble") {
@@ -1772,8 +1914,9 @@ double HeatBatch::time_kernel(const std::string& name, int reps, double* event_m
         if (P.mixed) TLP(C_PC, (k_load_rhs<float, CP>), 1);
         else TLP(C_PC, (k_load_rhs<double, CP>), 1);
     } else {
-        EQ_REQUIRE(false, "unknown kernel " << name << " (operator_red, operator_black [ilu0], spmv [jacobi, none], "
-                                                       "assemble, defect [mixed], b1, b2, b3, load_rhs)");
+        EQ_REQUIRE(false, "unknown kernel " << name << " (operator_red, operator_black [reduced system], spmv, "
+                                                       "ilu_black, ilu_red [full system], assemble, defect [mixed], "
+                                                       "b1, b2, b3, load_rhs)");
     }
 #undef TL
 #undef TLP

@@ -34,7 +34,13 @@ struct FusedConfig {
     int max_iter = 20;                 // Newton iterations per step
     double solver_tol = 1e-6;          // linear: ||b - J du|| <= tol ||b|| (FP64 defect)
     int solver_max_iter = 1000;
-    double inner_tol = 1e-4;           // mixed: FP32 inner solve tolerance
+    double inner_tol = 1e-6;           // mixed: FP32 inner solve tolerance of heat instances
+    double inner_tol_twophase = 1e-4;  //        and of two-phase instances (singular, badly scaled pressure)
+    // Krylov form with preconditioner ilu0, per instance kind: "reduced" (BiCGStab on the
+    // red-black Schur complement, preconditioned by its diagonal) or "full" (BiCGStab on
+    // the whole system with the two-pass red-black ILU(0)); equal in exact arithmetic.
+    std::string krylov_form_heat = "reduced";
+    std::string krylov_form_twophase = "full";
     int max_refine = 20;
     bool graph = true;                 // capture BiCGStab iterations in a hipGraph
     int cpt = 0;                       // cells per thread (1, 2, 4, 8); 0 = chosen by the tuner
@@ -110,11 +116,11 @@ public:
     // runtime stream is bracketed by its own HIP start and stop events (hipExtLaunchKernel)
     // and waited for.  event_ms gets the average of `reps` further launches queued back to
     // back between one event pair (dispatch gaps included).  Kernels: operator_red,
-    // operator_black (preconditioner ilu0), spmv (jacobi, none), assemble, defect, b1, b2,
-    // b3, load_rhs.
+    // operator_black (reduced-system instances), spmv, ilu_black, ilu_red (full-system
+    // instances), assemble, defect (mixed), b1, b2, b3, load_rhs.
     double time_kernel(const std::string& name, int reps, double* event_ms = nullptr);
-    static size_t estimate_bytes(int n, const std::string& precision, bool twophase = false,
-                                 const std::string& precond = "ilu0");
+    // Device bytes of one instance; `reduced`: its Krylov space is the reduced system.
+    static size_t estimate_bytes(int n, const std::string& precision, bool twophase = false, bool reduced = true);
     // Launch shape used by the most recent launch and the number of tuned signatures.
     LaunchShape launch_shape() const;
 

@@ -18,7 +18,7 @@ HeatServer::HeatServer(const ServerConfig& cfg) : cfg_(cfg), batch_{HeatBatch(cf
 int HeatServer::submit(int nx, int ny, double k, double cv, const double* u0, int steps, double dt) {
     EQ_REQUIRE(nx > 0 && ny > 0 && (long)nx * ny < (1L << 31), "bad request grid " << nx << "x" << ny);
     EQ_REQUIRE(steps > 0 && dt > 0, "a request needs steps > 0 and dt > 0");
-    const double bytes = (double)HeatBatch::estimate_bytes(nx * ny, cfg_.batch.precision, false, cfg_.batch.precond);
+    const double bytes = (double)HeatBatch::estimate_bytes(nx * ny, cfg_.batch.precision, false, cfg_.batch.precond == "ilu0" && cfg_.batch.krylov_form_heat == "reduced");
     EQ_REQUIRE(bytes <= cfg_.hbm_budget, "request of " << bytes << " bytes exceeds the HBM budget of "
                                                        << cfg_.hbm_budget);
     Req r;
@@ -43,7 +43,7 @@ int HeatServer::submit_twophase(const TwoPhaseSpec& spec, const double* perm, co
     const long n = (long)spec.nx * spec.ny * spec.nz;
     EQ_REQUIRE(spec.nx > 0 && spec.ny > 0 && spec.nz > 0 && n < (1L << 31), "bad request grid");
     EQ_REQUIRE(steps > 0 && dt > 0, "a request needs steps > 0 and dt > 0");
-    const double bytes = (double)HeatBatch::estimate_bytes((int)n, cfg_.batch.precision, true, cfg_.batch.precond);
+    const double bytes = (double)HeatBatch::estimate_bytes((int)n, cfg_.batch.precision, true, cfg_.batch.precond == "ilu0" && cfg_.batch.krylov_form_twophase == "reduced");
     EQ_REQUIRE(bytes <= cfg_.hbm_budget, "request of " << bytes << " bytes exceeds the HBM budget of "
                                                        << cfg_.hbm_budget);
     Req r;
