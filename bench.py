@@ -240,11 +240,14 @@ def bench_twophase(lib, args):
     pressure Newton solve and a saturation Newton solve, each to 1e-8 relative residual.
     Work = cells x (pressure + transport Newton iterations).
 
-    headline      fused stencil engine, mixed precision (FP32 BiCGStab on the red-black
-                  reduced system = red-black ILU(0); FP64 residual / Jacobian / refinement)
-    fused_fp64    the same engine, the same Krylov method in FP64
+    headline      fused_fp64: fused stencil engine, FP64 BiCGStab with the full-system
+                  red-black ILU(0) (per step a pressure and a saturation Newton solve)
+    fused_mixed   the same with FP32 BiCGStab and FP64 refinement.  The pressure system
+                  (singular, log-normal permeability) has eps32 * cond ~ 0.1-0.5, the edge of
+                  where FP32 refinement converges; a step that does not converge is recorded
+                  under runs.fused_mixed.failed instead of ending the run
     generic_*     the drop-in path: the printer-style program on the device AD engine with
-                  GMRES(30) + Jacobi (mixed and FP64)"""
+                  GMRES(30) + Jacobi, FP32 with FP64 refinement (generic_mixed) and FP64"""
     from equelle_amd.workloads import TWOPHASE, twophase_inputs, twophase_params
     nx, ny, nz = (int(v) for v in args.grid3.split("x"))
     n = nx * ny * nz
@@ -252,11 +255,16 @@ def bench_twophase(lib, args):
     dt = TWOPHASE["dt"]
     runs = {}
     inst = f"watervisc={TWOPHASE['watervisc']} oilvisc={TWOPHASE['oilvisc']} corey={TWOPHASE['corey']}".encode()
-    for name, prec in (("fused_mixed", "mixed"), ("fused_fp64", "fp64")):
+    def failed(name, err):
+        log(f"[bench] two-phase {name}: {err}")
+        runs[name] = {"failed": str(err)}
+
+    for name, prec in (("fused_fp64", "fp64"), ("fused_mixed", "mixed")):
         b = C.c_void_p()
         lib.batch_create(f"precision={prec} preconditioner={args.precond} rel_res_tol={TWOPHASE['rel_res_tol']} "
                          f"abs_res_tol=0 max_iter=20 solver_tol=1e-6 solver_max_iter=5000".encode(), C.byref(b))
         try:
+          try:
             slot = C.c_int()
             lib.batch_add_twophase(b, inst, nx, ny, nz, dptr(perm), dptr(poro), dptr(sw0), len(sc), iptr(sc),
                                    dptr(sv), C.byref(slot))
@@ -286,6 +294,11 @@ def bench_twophase(lib, args):
             secs = time.perf_counter() - t0
             p, sw = np.zeros(n), np.zeros(n)
             lib.batch_twophase_state(b, slot.value, dptr(p), dptr(sw))
+          except RuntimeError as e:   # FP32 Krylov on this pressure system: see the docstring
+            if prec == "fp64":
+                raise
+            failed(name, e)
+            continue
         finally:
             lib.batch_destroy(b)
         runs[name] = dict(value=n * newton / secs, ms_per_step=1e3 * secs / args.steps, newton=newton,
@@ -298,6 +311,7 @@ def bench_twophase(lib, args):
             lib.twophase_create(params, dptr(perm), dptr(poro), dptr(sw0), n, iptr(sc), dptr(sv), len(sc),
                                 C.byref(sim))
             try:
+              try:
                 st = np.zeros(6, dtype=np.int32)
                 for _ in range(args.warmup):
                     lib.twophase_step(sim, dt, iptr(st), None)
@@ -317,27 +331,32 @@ def bench_twophase(lib, args):
                 secs = time.perf_counter() - t0
                 p, sw = np.zeros(n), np.zeros(n)
                 lib.twophase_state(sim, dptr(p), dptr(sw))
+              except RuntimeError as e:
+                if name == "generic_fp64":
+                    raise
+                failed(name, e)
+                continue
             finally:
                 lib.twophase_destroy(sim)
             runs[name] = dict(value=n * newton / secs, ms_per_step=1e3 * secs / gsteps, newton=newton, steps=gsteps,
                               krylov_pressure=krylov[0], krylov_transport=krylov[1], sw=sw)
     ref = runs["fused_fp64"]["sw"]
-    errs = {k: float(np.linalg.norm(r["sw"] - ref) / np.linalg.norm(ref)) for k, r in runs.items()
-            if k.startswith("fused")}
+    errs = {k: float(np.linalg.norm(r["sw"] - ref) / np.linalg.norm(ref)) for k, r in runs.items() if "sw" in r}
     for r in runs.values():
-        r.pop("sw")
-    head = runs["fused_mixed"]
+        r.pop("sw", None)
+    head = runs["fused_fp64"]
     out = {"metric": "cell-Newton-iters/s", "value": head["value"], "unit": "cell-Newton-iters/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": head["ms_per_step"],
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-           "dtype": "f64 residual/Jacobian/refinement, f32 Krylov", "data": "synthetic",
+           "dtype": "f64", "data": "synthetic",
            "config": {"workload": f"3D two-phase transport {nx}x{ny}x{nz}, log-normal permeability, Corey 2, "
                                   f"viscosity ratio 5, dt={dt}, Newton to 1e-8 relative residual per solve",
                       "cells": n, "precond": args.precond},
-           "runs": runs, "mixed_over_fused_fp64": head["value"] / runs["fused_fp64"]["value"],
-           "sw_rel_error_vs_fused_fp64": errs}
+           "runs": runs, "sw_rel_error_vs_fused_fp64": errs}
+    if "value" in runs.get("fused_mixed", {}):
+        out["fused_mixed_over_fused_fp64"] = runs["fused_mixed"]["value"] / head["value"]
     if "generic_fp64" in runs:
-        out["fused_mixed_over_generic_fp64"] = head["value"] / runs["generic_fp64"]["value"]
+        out["fused_fp64_over_generic_fp64"] = head["value"] / runs["generic_fp64"]["value"]
     print(json.dumps(out), flush=True)
 
 

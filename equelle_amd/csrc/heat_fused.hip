@@ -928,19 +928,21 @@ __global__ void k_refine_flags(const Desc* __restrict__ D, int ninst, int mode, 
     }
 }
 
-// Right-hand side of the Krylov solve from r64 (scaled by 1 / ||r64|| in mixed precision,
-// b = (T)(r64 * scale)): reduced system -> g_b = b_b - A_br (C_r^-1 b_r) on black ranks;
+// Right-hand side of the Krylov solve from the refinement residual: F in the first round
+// (du = 0), r64 = F - J du after it; scaled by 1 / ||.|| in mixed precision,
+// b = (T)(src * scale).  Reduced system -> g_b = b_b - A_br (C_r^-1 b_r) on black ranks;
 // full system -> b in both slots of each pair (0 where a rank has no cell).
 template <class T>
-__device__ inline T rhs_value(const Desc& d, int c, double scale) {
-    return (T)(d.r64[c] * scale);
+__device__ inline T rhs_value(const double* __restrict__ src, int c, double scale) {
+    return (T)(src[c] * scale);
 }
 template <class T, int CPT>
 __global__ __launch_bounds__(BS) void k_load_rhs(const Desc* __restrict__ D, const int2* __restrict__ blk,
-                                                 int scaled) {
+                                                 int scaled, int first) {
     LOCATE();
     if (!d.act[1]) return;
     const double scale = scaled ? 1.0 / d.sc[S_RN] : 1.0;
+    const double* __restrict__ src = first ? d.F : d.r64;
     const int h = d.h, nx = d.nx, ny = d.ny, nxy = d.nx * d.ny;
     T* __restrict__ r = (T*)d.v[V_R];
     const T* __restrict__ dv = (const T*)d.dinv;
@@ -948,9 +950,9 @@ __global__ __launch_bounds__(BS) void k_load_rhs(const Desc* __restrict__ D, con
         int i, j, k;
         if (d.ns == 2) {
             int c = cell_of(d, p, 0, i, j, k);
-            r[p] = c < 0 ? (T)0 : rhs_value<T>(d, c, scale);
+            r[p] = c < 0 ? (T)0 : rhs_value<T>(src, c, scale);
             c = cell_of(d, p, 1, i, j, k);
-            r[h + p] = c < 0 ? (T)0 : rhs_value<T>(d, c, scale);
+            r[h + p] = c < 0 ? (T)0 : rhs_value<T>(src, c, scale);
             continue;
         }
         const int c = cell_of(d, p, 1, i, j, k);
@@ -962,48 +964,49 @@ __global__ __launch_bounds__(BS) void k_load_rhs(const Desc* __restrict__ D, con
         const T* K[7];
         for (int q = 1; q < 7; ++q) K[q] = (const T*)d.K[q];
         T acc = (T)0;
-        if (i > 0) acc += K[1][s] * (rhs_value<T>(d, c - 1, scale) * dv[(c - 1) >> 1]);
-        if (i < nx - 1) acc += K[2][s] * (rhs_value<T>(d, c + 1, scale) * dv[(c + 1) >> 1]);
-        if (j > 0) acc += K[3][s] * (rhs_value<T>(d, c - nx, scale) * dv[(c - nx) >> 1]);
-        if (j < ny - 1) acc += K[4][s] * (rhs_value<T>(d, c + nx, scale) * dv[(c + nx) >> 1]);
+        if (i > 0) acc += K[1][s] * (rhs_value<T>(src, c - 1, scale) * dv[(c - 1) >> 1]);
+        if (i < nx - 1) acc += K[2][s] * (rhs_value<T>(src, c + 1, scale) * dv[(c + 1) >> 1]);
+        if (j > 0) acc += K[3][s] * (rhs_value<T>(src, c - nx, scale) * dv[(c - nx) >> 1]);
+        if (j < ny - 1) acc += K[4][s] * (rhs_value<T>(src, c + nx, scale) * dv[(c + nx) >> 1]);
         if (d.nz > 1) {
-            if (k > 0) acc += K[5][s] * (rhs_value<T>(d, c - nxy, scale) * dv[(c - nxy) >> 1]);
-            if (k < d.nz - 1) acc += K[6][s] * (rhs_value<T>(d, c + nxy, scale) * dv[(c + nxy) >> 1]);
+            if (k > 0) acc += K[5][s] * (rhs_value<T>(src, c - nxy, scale) * dv[(c - nxy) >> 1]);
+            if (k < d.nz - 1) acc += K[6][s] * (rhs_value<T>(src, c + nxy, scale) * dv[(c + nxy) >> 1]);
         }
-        r[p] = rhs_value<T>(d, c, scale) - acc;
+        r[p] = rhs_value<T>(src, c, scale) - acc;
     }
 }
-// du += ||r64|| x (mixed) or du = x (fp64), in cell order.  Reduced system: the red
-// unknowns are recovered first, x_r = C_r^-1 (b_r - A_rb x_b).
+// du (+)= ||.|| x (mixed) or du = x (fp64), in cell order; the first round writes du
+// (every instance of the launch: 0 where the refinement is inactive), later rounds add.
+// Reduced system: the red unknowns are recovered first, x_r = C_r^-1 (b_r - A_rb x_b).
+// A solve whose last residual is not finite (breakdown, overflow) contributes nothing:
+// the refinement defect then shows no progress and the refinement stops.
 template <class T, int CPT>
-__global__ __launch_bounds__(BS) void k_accum(const Desc* __restrict__ D, const int2* __restrict__ blk, int scaled) {
+__global__ __launch_bounds__(BS) void k_accum(const Desc* __restrict__ D, const int2* __restrict__ blk, int scaled,
+                                              int first) {
     LOCATE();
-    // a solve whose last residual is not finite (breakdown, overflow) leaves du unchanged:
-    // the refinement defect then shows no progress and the refinement stops
-    if (!d.act[1] || !isfinite(d.sc[S_RR])) return;
+    const bool ok = d.act[1] && isfinite(d.sc[S_RR]);
+    if (!ok && !first) return;
     const double rn = d.sc[S_RN];
     const double scale = scaled ? 1.0 / rn : 1.0;
+    const double* __restrict__ src = first ? d.F : d.r64;
     const int h = d.h;
     const T* __restrict__ x = (const T*)d.v[V_X];
     const T* __restrict__ dv = (const T*)d.dinv;
+    double* __restrict__ du = d.du;
     auto put = [&](int c, T xc) {
-        if (scaled) d.du[c] += rn * (double)xc;
-        else d.du[c] = (double)xc;
+        const double v = scaled ? rn * (double)xc : (double)xc;
+        if (first) du[c] = ok ? v : 0.0;
+        else du[c] += v;
     };
     FOR_PAIRS(p) {
         int i, j, k;
         const int cr = cell_of(d, p, 0, i, j, k);
-        if (cr >= 0) put(cr, d.ns == 2 ? x[p] : (rhs_value<T>(d, cr, scale) - offdiag<T>(d, p, cr, i, j, k, x)) * dv[p]);
+        if (cr >= 0)
+            put(cr, !ok ? (T)0
+                        : d.ns == 2 ? x[p]
+                                    : (rhs_value<T>(src, cr, scale) - offdiag<T>(d, p, cr, i, j, k, x)) * dv[p]);
         const int cb = cell_of(d, p, 1, i, j, k);
-        if (cb >= 0) put(cb, x[(d.ns == 2 ? h : 0) + p]);
-    }
-}
-template <int CPT>
-__global__ __launch_bounds__(BS) void k_zero_du(const Desc* __restrict__ D, const int2* __restrict__ blk) {
-    LOCATE();
-    FOR_CELLS(c) {
-        d.du[c] = 0.0;
-        d.r64[c] = d.F[c];
+        if (cb >= 0) put(cb, ok ? x[(d.ns == 2 ? h : 0) + p] : (T)0);
     }
 }
 // u -= du for the Newton unknown: heat u; two-phase p (stage 0) or sw (stage 1)
@@ -1432,8 +1435,8 @@ struct HeatBatch::Impl {
     }
     template <class T>
     void conversions(const Table& t) {
-        HLP(C_PC, (k_load_rhs<T, CP>), 1);
-        HLP(C_PC, (k_accum<T, CP>), 1);
+        HLP(C_PC, (k_load_rhs<T, CP>), 1, 0);
+        HLP(C_PC, (k_accum<T, CP>), 1, 0);
     }
     template <class T>
     void vec_round(const Table& t) {
@@ -1516,23 +1519,22 @@ struct HeatBatch::Impl {
 
     template <class T>
     void linear_solve(Table& t) {
-        // du = 0, r64 = F; refinement flags from ||F||
-        HL(C_VEC, (k_zero_du<CP>));
+        // refinement flags from ||F||; the first round reads F and writes du (no clearing pass)
         const unsigned ni = (unsigned)t.inst.size();
         hipLaunchKernelGGL(k_refine_flags, dim3((ni + 63) / 64), dim3(64), 0, S(), t.desc->p, (int)ni, 0,
                            cfg.solver_tol, cfg.max_refine);
         precond_setup<T>(t);
         if (!mixed) {
-            HLP(C_PC, (k_load_rhs<T, CP>), 0);
+            HLP(C_PC, (k_load_rhs<T, CP>), 0, 1);
             bicgstab<T>(t, cfg.solver_tol, cfg.solver_max_iter);
-            HLP(C_PC, (k_accum<T, CP>), 0);
+            HLP(C_PC, (k_accum<T, CP>), 0, 1);
             return;
         }
         for (int ref = 0; ref < cfg.max_refine; ++ref) {
             if (ref > 0 && !count_active(t, 1)) break;   // all start active (||F|| > 0)
-            HLP(C_PC, (k_load_rhs<T, CP>), 1);
+            HLP(C_PC, (k_load_rhs<T, CP>), 1, ref == 0);
             bicgstab<T>(t, -1.0, cfg.solver_max_iter);   // per-instance inner tolerance (S_TOL2)
-            HLP(C_PC, (k_accum<T, CP>), 1);
+            HLP(C_PC, (k_accum<T, CP>), 1, ref == 0);
             launch_defect(t);
             fin(t, Fcell(S_FN));
             hipLaunchKernelGGL(k_refine_flags, dim3((ni + 63) / 64), dim3(64), 0, S(), t.desc->p, (int)ni, 1,
@@ -1911,8 +1913,8 @@ double HeatBatch::time_kernel(const std::string& name, int reps, double* event_m
         if (P.mixed) TLP(C_VEC, (k_b3<float, CP>), t.part->p, t.ng);
         else TLP(C_VEC, (k_b3<double, CP>), t.part->p, t.ng);
     } else if (name == "load_rhs") {
-        if (P.mixed) TLP(C_PC, (k_load_rhs<float, CP>), 1);
-        else TLP(C_PC, (k_load_rhs<double, CP>), 1);
+        if (P.mixed) TLP(C_PC, (k_load_rhs<float, CP>), 1, 0);
+        else TLP(C_PC, (k_load_rhs<double, CP>), 1, 0);
     } else {
         EQ_REQUIRE(false, "unknown kernel " << name << " (operator_red, operator_black [reduced system], spmv, "
                                                        "ilu_black, ilu_red [full system], assemble, defect [mixed], "

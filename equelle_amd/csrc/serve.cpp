@@ -10,7 +10,14 @@ static double secs(Clock::time_point a, Clock::time_point b) {
     return std::chrono::duration<double>(b - a).count();
 }
 
-HeatServer::HeatServer(const ServerConfig& cfg) : cfg_(cfg), batch_{HeatBatch(cfg.batch), HeatBatch(cfg.batch)} {
+static FusedConfig twophase_config(const ServerConfig& cfg) {
+    FusedConfig c = cfg.batch;
+    c.precision = cfg.twophase_precision;
+    return c;
+}
+
+HeatServer::HeatServer(const ServerConfig& cfg)
+    : cfg_(cfg), batch_{HeatBatch(cfg.batch), HeatBatch(twophase_config(cfg))} {
     EQ_REQUIRE(cfg.hbm_budget > 0, "hbm_budget must be positive");
     EQ_REQUIRE(cfg.max_in_flight > 0, "max_in_flight must be positive");
 }
@@ -43,7 +50,7 @@ int HeatServer::submit_twophase(const TwoPhaseSpec& spec, const double* perm, co
     const long n = (long)spec.nx * spec.ny * spec.nz;
     EQ_REQUIRE(spec.nx > 0 && spec.ny > 0 && spec.nz > 0 && n < (1L << 31), "bad request grid");
     EQ_REQUIRE(steps > 0 && dt > 0, "a request needs steps > 0 and dt > 0");
-    const double bytes = (double)HeatBatch::estimate_bytes((int)n, cfg_.batch.precision, true, cfg_.batch.precond == "ilu0" && cfg_.batch.krylov_form_twophase == "reduced");
+    const double bytes = (double)HeatBatch::estimate_bytes((int)n, cfg_.twophase_precision, true, cfg_.batch.precond == "ilu0" && cfg_.batch.krylov_form_twophase == "reduced");
     EQ_REQUIRE(bytes <= cfg_.hbm_budget, "request of " << bytes << " bytes exceeds the HBM budget of "
                                                        << cfg_.hbm_budget);
     Req r;

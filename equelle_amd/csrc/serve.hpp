@@ -30,6 +30,10 @@ namespace eqhip {
 
 struct ServerConfig {
     FusedConfig batch;
+    // Krylov precision of the two-phase batch.  FP64 by default: the two-phase pressure
+    // system (singular, log-normal permeability) has eps32 * cond ~ 0.1-0.5, so FP32
+    // inner solves with FP64 refinement stagnate or diverge on part of the stream.
+    std::string twophase_precision = "fp64";
     double hbm_budget = 64e9;   // bytes of instance state admitted at once
     int max_in_flight = 4096;
 };

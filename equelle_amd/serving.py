@@ -188,7 +188,8 @@ def bench_serving(lib, args):
     line = {"metric": "cell-Newton-iters/s (serving)", "value": top["cell_newton_iters_per_s"],
             "unit": "cell-Newton-iters/s", "n_gpus": 1, "steps": count, "warmup": 1,
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-            "dtype": "f64 residual/Jacobian/refinement, f32 Krylov", "data": "synthetic",
+            "dtype": "f64 residual/Jacobian; heat: f32 Krylov + f64 refinement, two-phase: f64 Krylov",
+            "data": "synthetic",
             "config": {"workload": f"Poisson request stream: {100 * (1 - frac3d):.0f}% 2D nonlinear heat (side "
                                    f"log-uniform 16..1024), {100 * frac3d:.0f}% 3D two-phase (side log-uniform "
                                    "16..128), 1-20 steps, dt=1, Newton to 1e-8 relative residual",

@@ -115,8 +115,9 @@ def test_poisson_replay_completes(hip_api):
 
 
 def test_mixed_heat_and_twophase_requests(hip_api):
-    """3D two-phase and 2D heat requests in one continuous batch: each result equals the
-    same request run alone through the batch API, bit for bit."""
+    """3D two-phase and 2D heat requests in one request loop (heat in mixed precision,
+    two-phase with FP64 Krylov): each result equals the same request run alone through
+    the batch API with that precision, bit for bit."""
     from equelle_amd.serving import TWOPHASE_INSTANCE, twophase_request
     reqs = [Request(0.0, 50, 0.2, 2, gaussian_bumps(50, 50, 60, width_scale=0.03)),
             twophase_request(0.0, 12, 2, seed=61),
@@ -143,7 +144,9 @@ def test_mixed_heat_and_twophase_requests(hip_api):
         srv.close()
     for q, r in enumerate(reqs):
         b = C.c_void_p()
-        hip_api.batch_create(PARAMS.encode(), C.byref(b))
+        # the request loop runs two-phase requests with FP64 Krylov (twophase_precision=fp64)
+        params = PARAMS.replace("precision=mixed", "precision=fp64") if r.twophase else PARAMS
+        hip_api.batch_create(params.encode(), C.byref(b))
         try:
             s = C.c_int()
             if r.twophase:
