@@ -41,10 +41,11 @@ from equelle_amd.runtime import load_library  # noqa: E402
 from equelle_amd.workloads import NONLINEAR_HEAT, gaussian_bumps, heat_params  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per cell and launch of the dominant kernel at 2048^2, from rocprofv3 --pmc
-# FETCH_SIZE and WRITE_SIZE per dispatch scaled by a copy kernel of known size (reads
-# x2.0, writes x1.0); profiles/heat2048_fused_hbm_traffic.md has the counts and commands.
-TRAFFIC_BYTES_PER_CELL = {}
+# HBM bytes per cell and launch of each kernel at 2048^2 (mixed precision): rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE per dispatch (mean of the 50 back-to-back timing launches),
+# scaled by what the counters read for a copy of known size (reads x2.0, writes x1.0).
+TRAFFIC_BYTES_PER_CELL = {"operator_red": 14.03, "operator_black": 18.07, "assemble": 44.29, "defect": 32.40,
+                          "b1": 12.02, "b2": 10.02, "b3": 16.06, "load_rhs": 20.13}
 
 
 def log(msg):
