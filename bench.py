@@ -361,6 +361,71 @@ def bench_twophase(lib, args):
     print(json.dumps(out), flush=True)
 
 
+def bench_twophase_fi(lib, args):
+    """Fully implicit two-phase flow (examples/dsl/twophase_fully_implicit.equelle): per
+    step one newtonSolveSystem on (p, sw), i.e. one 2n x 2n Jacobian with the coupling
+    blocks, on the device AD engine (the drop-in path; the fused engine has no fully
+    implicit kind).  The workload's permeability, porosity and sources on a --grid3-fi
+    grid, the program's linear relative permeabilities, dt = 0.05 (plain Newton from the
+    program's initial guess sw = 0.5 converges there), each solve to 1e-8 relative
+    residual; FP64 GMRES(30) + ILU(0) with linear tolerance 1e-7 (the system is singular
+    in the pressure constants; GMRES stays above its rounding floor).  Work = cells x
+    Newton iterations.  cpu_baseline: the oracle (one core) on the same case, direct or
+    BiCGStab/ILU(0) solves, one step."""
+    from equelle_amd.workloads import twophase_inputs, twophase_params
+    nx, ny, nz = (int(v) for v in args.grid3_fi.split("x"))
+    n = nx * ny * nz
+    perm, poro, sw0, sc, sv = twophase_inputs(nx, ny, nz)
+    dt = 0.05
+    params = twophase_params(nx, ny, nz, rel_res_tol=1e-8, max_iter=40, solver_tol=1e-7,
+                             extra="solver=GMRes preconditioner=ILU0 fully_implicit=1").replace("corey=2", "corey=1")
+    params = params.replace("solver=GMRes preconditioner=diagonal", "")
+    sim = C.c_void_p()
+    lib.twophase_create(params.encode(), dptr(perm), dptr(poro), dptr(sw0), n, iptr(sc), dptr(sv), len(sc),
+                        C.byref(sim))
+    st = np.zeros(6, dtype=np.int32)
+    try:
+        for _ in range(args.warmup):
+            lib.twophase_step(sim, dt, iptr(st), None)
+        lib.device_sync()
+        newton = krylov = 0
+        t0 = time.perf_counter()
+        for s_ in range(args.steps):
+            lib.twophase_step(sim, dt, iptr(st), None)
+            if st[2] == 0:
+                raise RuntimeError(f"fully implicit step {s_} did not converge: {st.tolist()}")
+            newton += int(st[0])
+            krylov += int(st[1])
+            log(f"[bench] fully implicit step {s_}: Newton {st[0]}, GMRES {st[1]}")
+        lib.device_sync()
+        secs = time.perf_counter() - t0
+    finally:
+        lib.twophase_destroy(sim)
+    value = n * newton / secs
+    out = {"metric": "cell-Newton-iters/s", "value": value, "unit": "cell-Newton-iters/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * secs / args.steps,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"fully implicit two-phase {nx}x{ny}x{nz} (newtonSolveSystem, 2n unknowns), "
+                                  "log-normal permeability, linear relative permeabilities, dt=0.05, Newton to 1e-8 "
+                                  "relative residual; device AD engine, FP64 GMRES(30)+ILU(0)",
+                      "cells": n, "unknowns": 2 * n},
+           "newton_iterations": newton, "krylov_iterations": krylov}
+    if not args.no_cpu_baseline:
+        oracle = CApi(os.path.join(ROOT, "oracle", "liboracle.so"), "eqo_")
+        oparams = twophase_params(nx, ny, nz, rel_res_tol=1e-8, max_iter=40).replace("corey=2", "corey=1")
+        p_o, s_o = np.zeros(n), np.zeros(n)
+        it = np.zeros(1, dtype=np.int32)
+        t = time.perf_counter()
+        oracle.twophase_fi_run(oparams.encode(), dptr(perm), dptr(poro), dptr(sw0), n, iptr(sc), dptr(sv), len(sc),
+                               dptr(np.array([dt])), 1, dptr(p_o), dptr(s_o), iptr(it))
+        csecs = time.perf_counter() - t
+        out["cpu_baseline"] = {"value": n * int(it[0]) / csecs, "unit": "cell-Newton-iters/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"first step of the same case ({int(it[0])} Newton iterations), serial C++ "
+                                         f"oracle: {csecs:.2f} s"}
+    print(json.dumps(out), flush=True)
+
+
 def bench_copy(args):
     """Calibration for HBM counters: a device-to-device copy of a known size
     (torch's elementwise copy kernel), 256 MiB read + 256 MiB written per launch."""
@@ -445,12 +510,14 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline run only (for profiling)")
     ap.add_argument("--precond", default="ilu0", help="preconditioner of the fused runs: ilu0 | diagonal | none")
-    ap.add_argument("--workload", default="heat2048", choices=["heat2048", "batched", "serving", "twophase", "copy"])
+    ap.add_argument("--workload", default="heat2048",
+                    choices=["heat2048", "batched", "serving", "twophase", "twophase_fi", "copy"])
     ap.add_argument("--instances", type=int, default=512)
     ap.add_argument("--inner-tol", dest="inner_tol", type=float, default=1e-6,
                     help="mixed precision: FP32 inner solve tolerance")
     ap.add_argument("--no-graph", dest="no_graph", action="store_true", help="eager launches (profiling)")
     ap.add_argument("--grid3", default="128x128x64", help="two-phase grid nx x ny x nz")
+    ap.add_argument("--grid3-fi", dest="grid3_fi", default="32x32x16", help="fully implicit two-phase grid")
     ap.add_argument("--tp-solver", dest="tp_solver", default="GMRes", help="two-phase Krylov solver")
     ap.add_argument("--rates", default="50,100,200,500,1000,2000", help="serving: arrival rates (req/s)")
     ap.add_argument("--requests", type=int, default=200, help="serving: requests per rate")
@@ -467,6 +534,8 @@ def main():
         return bench_batched(lib, args)
     if args.workload == "twophase":
         return bench_twophase(lib, args)
+    if args.workload == "twophase_fi":
+        return bench_twophase_fi(lib, args)
     if args.workload == "serving":
         from equelle_amd.serving import bench_serving
         return bench_serving(lib, args)

@@ -504,6 +504,7 @@ struct TwoPhaseSim {
     std::unique_ptr<equelleCUDA::EquelleRuntimeCUDA> er;
     std::unique_ptr<TwoPhaseProgramCUDA> prog;
     equelleCUDA::CollOfScalar p, sw;
+    bool fully_implicit = false;   // twophase_fully_implicit.equelle: one newtonSolveSystem per step
 };
 TwoPhaseSim& TP(void* s) {
     EQ_REQUIRE(s, "null two-phase simulation");
@@ -526,6 +527,7 @@ int eqhip_twophase_create(const char* params, const double* perm, const double* 
                                               std::vector<double>(src_vals, src_vals + nsrc)));
         s->sw = equelleCUDA::CollOfScalar(std::vector<double>(sw_initial, sw_initial + n));
         s->p = s->er->operatorExtend(double(0), s->er->allCells());
+        s->fully_implicit = s->er->core().params().get_bool("fully_implicit", false);
         *sim = s.release();
     });
 }
@@ -535,6 +537,35 @@ int eqhip_twophase_step(void* sim, double dt, int* stats, double* norms) {
         TwoPhaseSim& s = TP(sim);
         equelleCUDA::EquelleRuntimeCUDA& er = *s.er;
         const TwoPhaseProgramCUDA& prog = *s.prog;
+        if (s.fully_implicit) {
+            // twophase_fully_implicit.equelle:117-149 (out_twophase_fully_implicit.cpp:125-141)
+            const equelleCUDA::CollOfScalar sw0 = s.sw;
+            auto pressureResLocal = [&](const equelleCUDA::CollOfScalar& p, const equelleCUDA::CollOfScalar& sw) {
+                return prog.pressureResLocal(p, sw);
+            };
+            auto transportResLocal = [&](const equelleCUDA::CollOfScalar& p, const equelleCUDA::CollOfScalar& sw) {
+                return prog.transportResLocal(p, sw, sw0, dt);
+            };
+            const std::tuple<equelleCUDA::CollOfScalar, equelleCUDA::CollOfScalar> newvals = er.newtonSolveSystem(
+                equelleCUDA::makeArray(pressureResLocal, transportResLocal),
+                equelleCUDA::makeArray(s.p, er.operatorExtend(double(0.5), er.allCells())));
+            const equelleCUDA::NewtonReport& r = er.lastNewtonReport();
+            s.p = std::get<0>(newvals);
+            s.sw = std::get<1>(newvals);
+            if (stats) {   // the system solve in the pressure slots, its stop reason in both
+                stats[0] = r.iterations;
+                stats[1] = r.linear_iterations;
+                stats[2] = r.stop_reason;
+                stats[3] = 0;
+                stats[4] = 0;
+                stats[5] = r.stop_reason;
+            }
+            if (norms) {
+                norms[0] = norms[2] = r.initial_norm;
+                norms[1] = norms[3] = r.final_norm;
+            }
+            return;
+        }
         // twophase.equelle:99-113
         const equelleCUDA::CollOfScalar total_mobility = (prog.waterMob(s.sw) + prog.oilMob(s.sw));
         auto pressureResLocal = [&](const equelleCUDA::CollOfScalar& pressure) {
@@ -591,7 +622,18 @@ int eqhip_twophase_residual(void* sim, int which, const double* pr, const double
         const equelleCUDA::CollOfScalar P(std::vector<double>(pr, pr + n));
         const equelleCUDA::CollOfScalar tm = (s.prog->waterMob(S0) + s.prog->oilMob(S0));
         equelleCUDA::CollOfScalar r;
-        if (which == 0) {
+        if (which == 2) {   // fully implicit system at (p, sw): 2n rows, columns [p; sw]
+            std::vector<double> u(pr, pr + n);
+            u.insert(u.end(), sw, sw + n);
+            auto pres = [&](const equelleCUDA::CollOfScalar& p, const equelleCUDA::CollOfScalar& ss) {
+                return s.prog->pressureResLocal(p, ss);
+            };
+            auto tres = [&](const equelleCUDA::CollOfScalar& p, const equelleCUDA::CollOfScalar& ss) {
+                return s.prog->transportResLocal(p, ss, S0, dt);
+            };
+            r = s.er->systemResidual(equelleCUDA::makeArray(pres, tres),
+                                     equelleCUDA::CollOfScalar(equelleCUDA::CollOfScalar(u), true));
+        } else if (which == 0) {
             r = s.prog->pressureResidual(equelleCUDA::CollOfScalar(P, true), tm);
         } else {
             const equelleCUDA::CollOfScalar S(std::vector<double>(sw, sw + n));

@@ -383,6 +383,18 @@ std::shared_ptr<const eqhip::IndexData> EquelleRuntimeCUDA::trinaryIndicesImpl(c
     return eqhip::i_select(p.data(), t.data(), f.data());
 }
 
+CollOfScalar EquelleRuntimeCUDA::subsetRange(const CollOfScalar& u, int start, int len) const {
+    if (start < 0 || len < 0 || start + len > u.size())
+        OPM_THROW(std::runtime_error, "subset [" << start << ", " << start + len << ") of " << u.size() << " rows");
+    return CollOfScalar(eqhip::s_on(u.data(), core_->range_set(0, u.size()), core_->range_set(start, len)));
+}
+
+CollOfScalar EquelleRuntimeCUDA::supersetRange(const CollOfScalar& a, int start, int total) const {
+    if (start < 0 || start + a.size() > total)
+        OPM_THROW(std::runtime_error, "superset of " << a.size() << " rows at " << start << " in " << total);
+    return CollOfScalar(eqhip::s_extend(a.data(), core_->range_set(start, a.size()), core_->range_set(0, total)));
+}
+
 CollOfScalar EquelleRuntimeCUDA::solveForUpdate(const CollOfScalar& residual, int* iters) const {
     eqhip::LinReport rep;
     SP du = core_->solve_for_update(residual.data(), &rep);

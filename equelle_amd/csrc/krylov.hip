@@ -250,6 +250,8 @@ __global__ void k_dots(long n, int nd, const T* __restrict__ x0, const T* __rest
     }
 }
 
+constexpr double kHuge = 1e30;   // largest Krylov step length accepted
+
 // tol2, maxit: the ACT_BICG_RHO / ACT_CG_BETA / ACT_INIT actions update ACTIVE
 // (continue while ||r||^2 > tol2 ||b||^2, no breakdown, IT < maxit).
 __global__ void k_finalize(int act, int nd, const double* __restrict__ part, int nb, double* __restrict__ sc,
@@ -268,14 +270,25 @@ __global__ void k_finalize(int act, int nd, const double* __restrict__ part, int
         case ACT_STORE:
             for (int j = 0; j < nd; ++j) sc[slot + j] = d[j];
             break;
-        case ACT_BICG_ALPHA:
-            if (d[0] == 0.0) { sc[BREAK] = 1; sc[ALPHA] = 0; }
-            else sc[ALPHA] = sc[RHO] / d[0];
+        // A breakdown (zero, tiny or non-finite denominator) must not reach x: a bad alpha
+        // stops the solve before the update (x keeps the last iterate), a bad omega
+        // falls back to omega = 0 (x += alpha ph, r = s) and ends the solve after it.
+        // kHuge bounds the step so the FP32 vectors cannot overflow either.
+        case ACT_BICG_ALPHA: {
+            const double a = sc[RHO] / d[0];
+            if (d[0] == 0.0 || !isfinite(a) || fabs(a) > kHuge) {
+                sc[BREAK] = 1;
+                sc[ALPHA] = 0;
+                sc[ACTIVE] = 0.0;
+            } else sc[ALPHA] = a;
             break;
-        case ACT_BICG_OMEGA:
-            if (d[1] == 0.0) { sc[OMEGA] = 0; sc[BREAK] = 2; }
-            else sc[OMEGA] = d[0] / d[1];
+        }
+        case ACT_BICG_OMEGA: {
+            const double w = d[0] / d[1];
+            if (d[1] == 0.0 || !isfinite(w) || fabs(w) > kHuge) { sc[OMEGA] = 0; sc[BREAK] = 2; }
+            else sc[OMEGA] = w;
             break;
+        }
         case ACT_BICG_RHO:
             sc[RHO_OLD] = sc[RHO];
             sc[RHO] = d[0];
@@ -293,10 +306,15 @@ __global__ void k_finalize(int act, int nd, const double* __restrict__ part, int
             sc[BREAK] = 0.0;
             sc[ACTIVE] = (d[1] > 0.0 && isfinite(d[1]) && maxit > 0) ? 1.0 : 0.0;
             break;
-        case ACT_CG_ALPHA:
-            if (d[0] == 0.0) { sc[ALPHA] = 0; sc[BREAK] = 4; }
-            else sc[ALPHA] = sc[RZ] / d[0];
+        case ACT_CG_ALPHA: {
+            const double a = sc[RZ] / d[0];
+            if (d[0] == 0.0 || !isfinite(a) || fabs(a) > kHuge) {
+                sc[ALPHA] = 0;
+                sc[BREAK] = 4;
+                sc[ACTIVE] = 0.0;
+            } else sc[ALPHA] = a;
             break;
+        }
         case ACT_CG_BETA:
             sc[BETA] = sc[RZ] != 0.0 ? d[0] / sc[RZ] : 0.0;
             sc[RZ] = d[0];

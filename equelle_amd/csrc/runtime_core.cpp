@@ -321,6 +321,17 @@ SP RuntimeCore::divergence_matrix(const SP& x) const {
     return s_matmul(x->n == g.nf ? g.fulldiv : g.div, x);
 }
 
+IP RuntimeCore::range_set(int start, int len) const {
+    auto key = std::make_pair(start, len);
+    auto it = ranges_.find(key);
+    if (it != ranges_.end()) return it->second;
+    std::vector<int> idx(len);
+    for (int i = 0; i < len; ++i) idx[i] = start + i;
+    IP s = make_index(0, false, std::move(idx));
+    ranges_[key] = s;
+    return s;
+}
+
 SP RuntimeCore::solve_for_update(const SP& residual, LinReport* rep) const {
     return linear_solve(solver, residual, rep);
 }

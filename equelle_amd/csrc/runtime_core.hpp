@@ -61,6 +61,9 @@ public:
     SP divergence(const SP& x) const;
     SP divergence_matrix(const SP& x) const;
     SP solve_for_update(const SP& residual, LinReport* rep) const;
+    // Index set [start, start + len) of a combined unknown (newtonSolveSystem ranges);
+    // cached, so the position maps of subset / superset are built once.
+    IP range_set(int start, int len) const;
 
     SolverConfig solver;
     int max_iter = 10;
@@ -72,6 +75,7 @@ public:
 private:
     ParamMap params_;
     std::unique_ptr<DeviceGrid> grid_;
+    mutable std::map<std::pair<int, int>, IP> ranges_;
 };
 
 }  // namespace eqhip

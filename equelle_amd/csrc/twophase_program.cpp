@@ -86,4 +86,16 @@ CollOfScalar TwoPhaseProgramCUDA::transportResidual(const CollOfScalar& sw, cons
     return ((sw - sw0) + ((dt / pv) * (er.divergence(water_flux) - q)));
 }
 
+CollOfScalar TwoPhaseProgramCUDA::pressureResLocal(const CollOfScalar& p, const CollOfScalar& sw) const {
+    const CollOfScalar total_mobility = (waterMob(sw) + oilMob(sw));
+    return pressureResidual(p, total_mobility);
+}
+
+CollOfScalar TwoPhaseProgramCUDA::transportResLocal(const CollOfScalar& p, const CollOfScalar& sw,
+                                                    const CollOfScalar& sw0, double dt) const {
+    const CollOfScalar total_mobility = (waterMob(sw) + oilMob(sw));
+    const CollOfScalar flux = totalFlux(p, total_mobility);
+    return transportResidual(sw, sw0, flux, dt);
+}
+
 }  // namespace eqhip

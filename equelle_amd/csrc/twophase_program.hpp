@@ -24,6 +24,12 @@ struct TwoPhaseProgramCUDA {
     equelleCUDA::CollOfScalar transportResidual(const equelleCUDA::CollOfScalar& sw,
                                                 const equelleCUDA::CollOfScalar& sw0,
                                                 const equelleCUDA::CollOfScalar& flux, double dt) const;
+    // The fully implicit program (examples/dsl/twophase_fully_implicit.equelle:117-149):
+    // both residuals as functions of (p, sw), for newtonSolveSystem.
+    equelleCUDA::CollOfScalar pressureResLocal(const equelleCUDA::CollOfScalar& p,
+                                               const equelleCUDA::CollOfScalar& sw) const;
+    equelleCUDA::CollOfScalar transportResLocal(const equelleCUDA::CollOfScalar& p, const equelleCUDA::CollOfScalar& sw,
+                                                const equelleCUDA::CollOfScalar& sw0, double dt) const;
 
     equelleCUDA::EquelleRuntimeCUDA& er;
     double watervisc, oilvisc;

@@ -121,6 +121,23 @@ public:
     // ----------- Implicit methods (EquelleRuntimeCUDA_impl.hpp:216-300)
     template <class ResidualFunctor>
     CollOfScalar newtonSolve(const ResidualFunctor& rescomp, const CollOfScalar& u_initialguess);
+    // newtonSolveSystem for two equations, as the serial backend has it
+    // (backends/serial/include/equelle/EquelleRuntimeCPU_impl.hpp:282-339; commented out in
+    // the CUDA backend, EquelleRuntimeCUDA.hpp:252-254): the unknowns are stacked [u0; u1],
+    // each residual function is called with the two subsets of the combined primary
+    // variable, the residuals are placed back by superset and added, and newtonSolve runs
+    // on the combined residual (one Jacobian with the coupling blocks).
+    template <class F0, class F1>
+    std::tuple<CollOfScalar, CollOfScalar> newtonSolveSystem(const std::tuple<F0, F1>& rescomp,
+                                                             const std::tuple<CollOfScalar, CollOfScalar>& u_initialguess);
+    // The combined residual of a two-equation system at the stacked unknown u.
+    template <class F0, class F1>
+    CollOfScalar systemResidual(const std::tuple<F0, F1>& rescomp, const CollOfScalar& u) const;
+    // Rows [start, start + len) of u (values and Jacobian rows), and `a` placed at rows
+    // [start, start + a.size()) of a collection of `total` rows: the subset / superset of
+    // EquelleRuntimeCPU_impl.hpp:85-110 on a contiguous range.
+    CollOfScalar subsetRange(const CollOfScalar& u, int start, int len) const;
+    CollOfScalar supersetRange(const CollOfScalar& a, int start, int total) const;
 
     // ----------- Output
     void output(const String& tag, Scalar val) const;
@@ -208,6 +225,27 @@ CollOfScalar EquelleRuntimeCUDA::newtonSolve(const ResidualFunctor& rescomp, con
     last_newton_ = rep;
     newtonDoneMessage(iter, rep.stop_reason, rep.seconds);
     return CollOfScalar(u.value());
+}
+
+template <class F0, class F1>
+CollOfScalar EquelleRuntimeCUDA::systemResidual(const std::tuple<F0, F1>& rescomp, const CollOfScalar& u) const {
+    const int total = u.size(), n0 = total / 2;
+    const CollOfScalar a = subsetRange(u, 0, n0);
+    const CollOfScalar b = subsetRange(u, n0, total - n0);
+    return supersetRange(std::get<0>(rescomp)(a, b), 0, total) + supersetRange(std::get<1>(rescomp)(a, b), n0, total);
+}
+
+template <class F0, class F1>
+std::tuple<CollOfScalar, CollOfScalar> EquelleRuntimeCUDA::newtonSolveSystem(
+    const std::tuple<F0, F1>& rescomp, const std::tuple<CollOfScalar, CollOfScalar>& u_initialguess) {
+    const CollOfScalar& g0 = std::get<0>(u_initialguess);
+    const CollOfScalar& g1 = std::get<1>(u_initialguess);
+    if (g0.size() != g1.size())
+        OPM_THROW(std::runtime_error, "newtonSolveSystem: unknowns of sizes " << g0.size() << " and " << g1.size());
+    const int n0 = g0.size(), total = 2 * n0;
+    const CollOfScalar guess = supersetRange(g0, 0, total) + supersetRange(g1, n0, total);
+    const CollOfScalar u = newtonSolve([&](const CollOfScalar& x) { return systemResidual(rescomp, x); }, guess);
+    return std::tuple<CollOfScalar, CollOfScalar>(subsetRange(u, 0, n0), subsetRange(u, n0, n0));
 }
 
 }  // namespace equelleCUDA

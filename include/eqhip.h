@@ -147,16 +147,20 @@ int eqhip_heat_destroy(void* sim);
  * on the device AD engine: per step a pressure newtonSolve then a saturation newtonSolve
  * (twophase.equelle:99-113).  params: grid (grid_dim=3 nx ny nz dx dy dz), watervisc,
  * oilvisc, min_poro, corey (1|2), and the solver/Newton parameters of runtime_create.
+ * fully_implicit=1 runs examples/dsl/twophase_fully_implicit.equelle instead: one
+ * newtonSolveSystem on (p, sw) per step (EquelleRuntimeCPU_impl.hpp:282-339).
  * perm, poro, sw_initial: one value per cell; sources: ascending cells with rates. */
 int eqhip_twophase_create(const char* params, const double* perm, const double* poro, const double* sw_initial,
                           int n, const int* src_cells, const double* src_vals, int nsrc, void** sim);
 /* stats = {pressure Newton, Krylov, stop; transport Newton, Krylov, stop};
- * norms = {pressure ||F0||, ||Fn||, transport ||F0||, ||Fn||} */
+ * norms = {pressure ||F0||, ||Fn||, transport ||F0||, ||Fn||}; fully implicit: the system solve's
+ * Newton, Krylov, stop in stats[0..2], stats[3..4] = 0, stats[5] = stop, norms repeated */
 int eqhip_twophase_step(void* sim, double dt, int* stats, double* norms);
 int eqhip_twophase_state(void* sim, double* p, double* sw);
 /* residual + Jacobian (CSR): which = 0 pressure residual at p (total mobility of sw0),
- * which = 1 transport residual at sw (flux from p, sw0; step dt); info = {n, nnz};
- * pass null F to query sizes. */
+ * which = 1 transport residual at sw (flux from p, sw0; step dt), which = 2 the fully
+ * implicit system at (p, sw) (2n rows [pressure; transport], columns [p; sw], previous
+ * saturation sw0); info = {rows, nnz}; pass null F to query sizes. */
 int eqhip_twophase_residual(void* sim, int which, const double* p, const double* sw0, const double* sw, double dt,
                             int* info, double* F, int* row_ptr, int* col_ind, double* jac);
 int eqhip_twophase_destroy(void* sim);

@@ -241,17 +241,20 @@ template <typename T>
 std::tuple<T> makeArray(const T& t) {
     return std::tuple<T>{t};
 }
-template <typename T>
-std::tuple<T, T> makeArray(const T& t1, const T& t2) {
-    return std::tuple<T, T>{t1, t2};
+// The elements may differ in type: generated code passes residual lambdas to
+// newtonSolveSystem as makeArray(pressureResLocal, transportResLocal)
+// (examples/app/out_twophase_fully_implicit.cpp:135).
+template <typename T1, typename T2>
+std::tuple<T1, T2> makeArray(const T1& t1, const T2& t2) {
+    return std::tuple<T1, T2>{t1, t2};
 }
-template <typename T>
-std::tuple<T, T, T> makeArray(const T& t1, const T& t2, const T& t3) {
-    return std::tuple<T, T, T>{t1, t2, t3};
+template <typename T1, typename T2, typename T3>
+std::tuple<T1, T2, T3> makeArray(const T1& t1, const T2& t2, const T3& t3) {
+    return std::tuple<T1, T2, T3>{t1, t2, t3};
 }
-template <typename T>
-std::tuple<T, T, T, T> makeArray(const T& t1, const T& t2, const T& t3, const T& t4) {
-    return std::tuple<T, T, T, T>{t1, t2, t3, t4};
+template <typename T1, typename T2, typename T3, typename T4>
+std::tuple<T1, T2, T3, T4> makeArray(const T1& t1, const T2& t2, const T3& t3, const T4& t4) {
+    return std::tuple<T1, T2, T3, T4>{t1, t2, t3, t4};
 }
 
 }  // namespace equelleCUDA

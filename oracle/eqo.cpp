@@ -501,11 +501,14 @@ LinSolveReport solve_banded_direct(const Csr& a, const std::vector<double>& b, s
     double amax = 0.0;
     for (double v : a.v) amax = std::max(amax, std::fabs(v));
     // A connected pure-Neumann operator (the two-phase pressure equation with rate
-    // sources) is singular with the constants as null space: elimination leaves a
-    // round-off last pivot.  The right-hand side is consistent up to rounding, and any
-    // solution is valid, so the free last unknown is set to 0 (the solution with
-    // x[n-1] = 0; its residual, returned below, shows any inconsistency).
-    bool free_last = false;
+    // sources, alone or as the pressure block of the fully implicit system) is singular
+    // with the constants (of the pressure unknowns) as null space: elimination meets a
+    // round-off pivot in the column where the dependency closes (the last one for the
+    // pressure alone).  The right-hand side is consistent up to rounding and any
+    // solution is valid, so such a column's unknown is set to 0 and its row, a
+    // combination of the others, is left out (the residual returned below shows any
+    // inconsistency).
+    std::vector<char> free_col(n, 0);
     for (int j = 0; j < n; ++j) {
         const int km = std::min(kl, n - 1 - j);
         int p = j;
@@ -513,9 +516,9 @@ LinSolveReport solve_banded_direct(const Csr& a, const std::vector<double>& b, s
         for (int i = j + 1; i <= j + km; ++i)
             if (std::fabs(AB(i, j)) > best) { best = std::fabs(AB(i, j)); p = i; }
         piv[j] = p;
-        if (j == n - 1 && n > 1 && best <= 1e-11 * amax) {
-            free_last = true;
-            break;
+        if (n > 1 && best <= 1e-11 * amax) {
+            free_col[j] = 1;
+            continue;
         }
         if (best == 0.0) { LinSolveReport r; return r; }
         const int ju = std::min(n - 1, j + kv);
@@ -537,7 +540,7 @@ LinSolveReport solve_banded_direct(const Csr& a, const std::vector<double>& b, s
         double s = rhs[j];
         const int ju = std::min(n - 1, j + kv);
         for (int c = j + 1; c <= ju; ++c) s -= AB(j, c) * x[c];
-        x[j] = (free_last && j == n - 1) ? 0.0 : s / AB(j, j);
+        x[j] = free_col[j] ? 0.0 : s / AB(j, j);
     }
     LinSolveReport rep;
     rep.converged = true;
@@ -825,16 +828,53 @@ double Runtime::prodReduce(const ADB& x) const {
 }
 double Runtime::twoNorm(const ADB& x) const { return std::sqrt(dotv(x.v, x.v)); }
 
-std::vector<double> Runtime::solveForUpdate(const ADB& residual, LinSolveReport* rep) const {
+std::vector<double> Runtime::solveForUpdate(const ADB& residual, LinSolveReport* rep, int blocks) const {
     if (residual.J.empty()) fail("solveForUpdate: residual has no Jacobian");
     std::vector<double> du;
     LinSolveReport r;
-    const bool direct = linear_solver == "direct" || (linear_solver == "auto" && residual.size() <= 40000);
-    if (direct) r = solve_banded_direct(residual.J, residual.v, du);
-    else r = solve_bicgstab_ilu0(residual.J, residual.v, du, solver_tol, solver_max_iter);
+    const int N = residual.size();
+    if (blocks > 1) {
+        // Block-ordered unknowns (k * m + c, the system of newtonSolveSystem) are solved in
+        // interleaved order (B * c + k), which keeps the band of a stencil system narrow.
+        if (N % blocks) fail("solveForUpdate: size not a multiple of the block count");
+        const int m = N / blocks;
+        std::vector<int> perm(N);
+        for (int k = 0; k < blocks; ++k)
+            for (int c = 0; c < m; ++c) perm[k * m + c] = blocks * c + k;
+        const Csr& a = residual.J;
+        std::vector<Triplet> t;
+        for (int i = 0; i < N; ++i)
+            for (int q = a.rp[i]; q < a.rp[i + 1]; ++q) t.push_back({perm[i], perm[a.ci[q]], a.v[q]});
+        Csr pa = csr_from_triplets(N, N, t);
+        std::vector<double> pb(N), px;
+        for (int i = 0; i < N; ++i) pb[perm[i]] = residual.v[i];
+        const bool direct = linear_solver == "direct" || (linear_solver == "auto" && N <= 20000);
+        if (direct) r = solve_banded_direct(pa, pb, px);
+        else r = solve_bicgstab_ilu0(pa, pb, px, solver_tol, solver_max_iter);
+        if (r.converged) {
+            du.resize(N);
+            for (int i = 0; i < N; ++i) du[i] = px[perm[i]];
+        }
+    } else {
+        const bool direct = linear_solver == "direct" || (linear_solver == "auto" && N <= 40000);
+        if (direct) r = solve_banded_direct(residual.J, residual.v, du);
+        else r = solve_bicgstab_ilu0(residual.J, residual.v, du, solver_tol, solver_max_iter);
+    }
     if (!r.converged) fail("Linear solver convergence failure.");
     if (rep) *rep = r;
     return du;
+}
+
+ADB adb_subset(const ADB& u, int start, int len) {
+    std::vector<int> rows(len);
+    for (int i = 0; i < len; ++i) rows[i] = start + i;
+    return adb_gather(u, rows);
+}
+
+ADB adb_superset(const ADB& a, int start, int total) {
+    std::vector<int> rows(a.size());
+    for (int i = 0; i < a.size(); ++i) rows[i] = start + i;
+    return adb_scatter(a, rows, total);
 }
 
 }  // namespace eqo

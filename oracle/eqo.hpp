@@ -27,6 +27,7 @@
 #include <vector>
 #include <map>
 #include <stdexcept>
+#include <utility>
 
 namespace eqo {
 
@@ -107,6 +108,12 @@ ADB adb_matmul(const Csr& m, const ADB& a);      // Matrix * CollOfScalar
 ADB adb_gather(const ADB& a, const std::vector<int>& rows);           // On
 ADB adb_scatter(const ADB& a, const std::vector<int>& rows, int n);   // Extend
 ADB adb_select(const std::vector<uint8_t>& p, const ADB& t, const ADB& f);  // trinaryIf
+// Rows [start, start + len) of u, and a placed at rows [start, start + a.size()) of a
+// collection of `total` rows (zero rows elsewhere): the subset / superset of
+// EquelleRuntimeCPU_impl.hpp:28-110 on a contiguous range (ESpan), as used by
+// newtonSolveSystem.
+ADB adb_subset(const ADB& u, int start, int len);
+ADB adb_superset(const ADB& a, int start, int total);
 
 // ----------------------------------------------------------------------------
 // Index sets (CollOfIndices<codim>): explicit ascending-or-not index lists;
@@ -191,10 +198,22 @@ public:
     double twoNorm(const ADB& x) const;
 
     // Solve J du = r (EquelleRuntimeCPU.cpp:413-440).
-    std::vector<double> solveForUpdate(const ADB& residual, LinSolveReport* rep) const;
+    // blocks > 1: the unknowns form `blocks` equal blocks (a newtonSolveSystem system)
+    std::vector<double> solveForUpdate(const ADB& residual, LinSolveReport* rep, int blocks = 1) const;
 
     template <class F>
-    ADB newtonSolve(const F& rescomp, const ADB& u0, NewtonReport* rep = nullptr) const;
+    ADB newtonSolve(const F& rescomp, const ADB& u0, NewtonReport* rep = nullptr, int blocks = 1) const;
+    // newtonSolveSystem for two equations (EquelleRuntimeCPU_impl.hpp:282-339): the
+    // unknowns are stacked [u0; u1] (ranges [0, n0) and [n0, n0 + n1)), each residual
+    // function is called with the two subsets of the combined primary variable, the
+    // residuals are placed back by superset and added, and newtonSolve runs on the
+    // combined residual.
+    template <class F0, class F1>
+    std::pair<ADB, ADB> newtonSolveSystem(const F0& f0, const F1& f1, const ADB& u0, const ADB& u1,
+                                          NewtonReport* rep = nullptr) const;
+    // The combined residual at (u0, u1) (the functor newtonSolveSystem iterates on).
+    template <class F0, class F1>
+    ADB systemResidual(const F0& f0, const F1& f1, const ADB& u) const;
 
     int max_iter = 10;
     double abs_res_tol = 1e-6;
@@ -211,7 +230,7 @@ private:
 ADB primary_variable(const ADB& u0);  // identity Jacobian (EquelleRuntimeCPU.cpp:565-571)
 
 template <class F>
-ADB Runtime::newtonSolve(const F& rescomp, const ADB& u0, NewtonReport* rep) const {
+ADB Runtime::newtonSolve(const F& rescomp, const ADB& u0, NewtonReport* rep, int blocks) const {
     ADB u = primary_variable(u0);
     ADB residual = rescomp(u);
     double r0 = twoNorm(residual);
@@ -224,7 +243,7 @@ ADB Runtime::newtonSolve(const F& rescomp, const ADB& u0, NewtonReport* rep) con
     };
     while (!done(rn) && iter < max_iter) {
         LinSolveReport lr;
-        std::vector<double> du = solveForUpdate(residual, &lr);
+        std::vector<double> du = solveForUpdate(residual, &lr, blocks);
         lin_iters += lr.iterations;
         for (int i = 0; i < u.size(); ++i) u.v[i] -= du[i];   // u = u - du (J unchanged)
         residual = rescomp(u);
@@ -242,6 +261,28 @@ ADB Runtime::newtonSolve(const F& rescomp, const ADB& u0, NewtonReport* rep) con
     ADB out;
     out.v = u.v;
     return out;
+}
+
+template <class F0, class F1>
+ADB Runtime::systemResidual(const F0& f0, const F1& f1, const ADB& u) const {
+    const int total = u.size(), n0 = total / 2;
+    const ADB a = adb_subset(u, 0, n0);
+    const ADB b = adb_subset(u, n0, total - n0);
+    return adb_add(adb_superset(f0(a, b), 0, total), adb_superset(f1(a, b), n0, total));
+}
+
+template <class F0, class F1>
+std::pair<ADB, ADB> Runtime::newtonSolveSystem(const F0& f0, const F1& f1, const ADB& u0, const ADB& u1,
+                                               NewtonReport* rep) const {
+    if (u0.size() != u1.size()) throw std::runtime_error("newtonSolveSystem: blocks of different sizes");
+    const int n0 = u0.size(), total = n0 + u1.size();
+    // combined initial guess: superset(u0) + superset(u1)
+    ADB guess = adb_add(adb_superset(u0, 0, total), adb_superset(u1, n0, total));
+    ADB u = newtonSolve([&](const ADB& x) { return systemResidual(f0, f1, x); }, guess, rep, 2);
+    ADB a, b;
+    a.v.assign(u.v.begin(), u.v.begin() + n0);
+    b.v.assign(u.v.begin() + n0, u.v.end());
+    return {a, b};
 }
 
 }  // namespace eqo

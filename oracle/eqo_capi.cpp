@@ -534,6 +534,56 @@ int eqo_twophase_residual_params(const char* params, const double* perm, const d
     });
 }
 
+// Fully implicit two-phase program: nsteps time steps; p_out/sw_out final fields;
+// iters[s] = Newton iterations of step s (one newtonSolveSystem per step).
+int eqo_twophase_fi_run(const char* params, const double* perm, const double* poro, const double* sw_initial, int n,
+                        const int* src_cells, const double* src_vals, int nsrc, const double* dts, int nsteps,
+                        double* p_out, double* sw_out, int* iters) {
+    return guard([&] {
+        Params p;
+        p.parse(params ? params : "");
+        TwoPhaseResult res = run_twophase_fi(p, std::vector<double>(perm, perm + n),
+                                             std::vector<double>(poro, poro + n),
+                                             std::vector<double>(sw_initial, sw_initial + n),
+                                             std::vector<int>(src_cells, src_cells + nsrc),
+                                             std::vector<double>(src_vals, src_vals + nsrc),
+                                             std::vector<double>(dts, dts + nsteps));
+        std::memcpy(p_out, res.p.data(), n * sizeof(double));
+        std::memcpy(sw_out, res.sw.data(), n * sizeof(double));
+        for (int s = 0; s < nsteps && iters; ++s) iters[s] = res.pressure_reports[s].iterations;
+    });
+}
+
+// The combined residual + Jacobian of the fully implicit system at (p, sw) (previous
+// saturation sw0, step dt): 2n rows, block order [pressure; transport], columns
+// [p; sw] of the combined primary variable.  info = {2n, nnz}.
+int eqo_twophase_fi_residual(const char* params, const double* perm, const double* poro, int n, const int* src_cells,
+                             const double* src_vals, int nsrc, const double* pr, const double* sw0, const double* sw,
+                             double dt, int* info, double* F, int* rp, int* ci, double* jv) {
+    return guard([&] {
+        Params p;
+        p.parse(params ? params : "");
+        Runtime rt(p);
+        TwoPhaseProgram prog(rt, p, std::vector<double>(perm, perm + n), std::vector<double>(poro, poro + n),
+                             std::vector<int>(src_cells, src_cells + nsrc),
+                             std::vector<double>(src_vals, src_vals + nsrc));
+        ADB S0, U;
+        S0.v.assign(sw0, sw0 + n);
+        U.v.assign(pr, pr + n);
+        U.v.insert(U.v.end(), sw, sw + n);
+        auto pres = [&](const ADB& pp, const ADB& ss) { return twophase_fi_pressure(prog, pp, ss); };
+        auto tres = [&](const ADB& pp, const ADB& ss) { return twophase_fi_transport(prog, pp, ss, S0, dt); };
+        ADB r = rt.systemResidual(pres, tres, primary_variable(U));
+        info[0] = r.size();
+        info[1] = r.J.nnz();
+        if (!F) return;
+        std::memcpy(F, r.v.data(), r.v.size() * sizeof(double));
+        std::memcpy(rp, r.J.rp.data(), r.J.rp.size() * sizeof(int));
+        std::memcpy(ci, r.J.ci.data(), r.J.ci.size() * sizeof(int));
+        std::memcpy(jv, r.J.v.data(), r.J.v.size() * sizeof(double));
+    });
+}
+
 // One residual + Jacobian evaluation of the heat program at state u (prev state u0, step dt).
 // Sizes are returned in info = {n, nnz}; pass null arrays to query first.
 int eqo_heat_residual(const char* params, const double* u, const double* u0, int n, const int* dir_faces,

@@ -235,6 +235,40 @@ TwoPhaseResult run_twophase(const Params& p, const std::vector<double>& perm, co
     return res;
 }
 
+ADB twophase_fi_pressure(const TwoPhaseProgram& prog, const ADB& p, const ADB& sw) {
+    const ADB total_mobility = adb_add(prog.water_mob(sw), prog.oil_mob(sw));
+    return prog.pressure_residual(p, total_mobility);
+}
+
+ADB twophase_fi_transport(const TwoPhaseProgram& prog, const ADB& p, const ADB& sw, const ADB& sw0, double dt) {
+    const ADB total_mobility = adb_add(prog.water_mob(sw), prog.oil_mob(sw));
+    const ADB flux = prog.total_flux(p, total_mobility);
+    return prog.transport_residual(sw, sw0, flux, dt);
+}
+
+TwoPhaseResult run_twophase_fi(const Params& p, const std::vector<double>& perm, const std::vector<double>& poro,
+                               const std::vector<double>& sw_initial, const std::vector<int>& source_cells,
+                               const std::vector<double>& source_values, const std::vector<double>& dts) {
+    Runtime er(p);
+    TwoPhaseProgram prog(er, p, perm, poro, source_cells, source_values);
+    TwoPhaseResult res;
+    ADB sw0;
+    sw0.v = sw_initial;
+    ADB p0 = er.operatorExtend(0.0, prog.all_cells);
+    for (double dt : dts) {
+        NewtonReport rep;
+        auto pres = [&](const ADB& pp, const ADB& ss) { return twophase_fi_pressure(prog, pp, ss); };
+        auto tres = [&](const ADB& pp, const ADB& ss) { return twophase_fi_transport(prog, pp, ss, sw0, dt); };
+        std::pair<ADB, ADB> nv = er.newtonSolveSystem(pres, tres, p0, er.operatorExtend(0.5, prog.all_cells), &rep);
+        res.pressure_reports.push_back(rep);
+        p0 = nv.first;
+        sw0 = nv.second;
+    }
+    res.p = p0.v;
+    res.sw = sw0.v;
+    return res;
+}
+
 double time_newton_iterations(const Runtime& rt, const HeatProgram& prog, const ADB& u0, double dt, int iters,
                               int* lin_iters) {
     const auto t0 = std::chrono::steady_clock::now();

@@ -69,6 +69,20 @@ TwoPhaseResult run_twophase(const Params& p, const std::vector<double>& perm, co
                             const std::vector<double>& sw_initial, const std::vector<int>& source_cells,
                             const std::vector<double>& source_values, const std::vector<double>& dts);
 
+// Fully implicit two-phase flow, examples/dsl/twophase_fully_implicit.equelle (generated
+// form examples/app/out_twophase_fully_implicit.cpp:125-141): per time step one
+// newtonSolveSystem on (p, sw) with initial guess (p0, 0.5) and the residuals
+//   pressureResLocal(p, sw)  = computePressureResidual(p, mw(sw) + mo(sw))       (:122-128)
+//   transportResLocal(p, sw) = computeTransportResidual(sw, sw0,
+//                                computeTotalFlux(p, mw(sw) + mo(sw)), dt)     (:130-135)
+// (the sequential program's building blocks, with the mobility now a function of the
+// unknown sw).  Reports: one per step in pressure_reports (the system solve).
+ADB twophase_fi_pressure(const TwoPhaseProgram& prog, const ADB& p, const ADB& sw);
+ADB twophase_fi_transport(const TwoPhaseProgram& prog, const ADB& p, const ADB& sw, const ADB& sw0, double dt);
+TwoPhaseResult run_twophase_fi(const Params& p, const std::vector<double>& perm, const std::vector<double>& poro,
+                               const std::vector<double>& sw_initial, const std::vector<int>& source_cells,
+                               const std::vector<double>& source_values, const std::vector<double>& dts);
+
 // Wall time of `iters` Newton iterations (assembly + linear solve + update).
 double time_newton_iterations(const Runtime& rt, const HeatProgram& prog, const ADB& u0, double dt, int iters,
                               int* lin_iters);

@@ -14,6 +14,7 @@ from equelle_amd.capi import dptr, iptr
 GEN = os.path.join(ROOT, "tests", "generated")
 EXE = os.path.join(GEN, "bin", "nonlinear_heat_cuda")
 EXE_TP = os.path.join(GEN, "bin", "twophase_corey_cuda")
+EXE_FI = os.path.join(GEN, "bin", "twophase_fully_implicit_cuda")
 
 
 @pytest.mark.parametrize("src", ["nonlinear_heat_cuda.cpp", "twophase_corey_cuda.cpp"])
@@ -86,5 +87,36 @@ def test_generated_twophase_runs_and_matches_oracle(tmp_path, oracle_api):
     p_last = np.loadtxt(tmp_path / "pressure-00002.output")
     params = twophase_params(nx, ny, nz, rel_res_tol=1e-13, max_iter=40).encode()
     po, so, _ = oracle_run(oracle_api, params, perm, poro, sw0, sc, sv, dts)
+    assert np.allclose(sw_last, so, rtol=1e-9, atol=1e-10)
+    assert np.allclose(p_last - p_last[-1], po - po[-1], rtol=1e-9, atol=1e-9 * np.abs(po).max())
+
+
+@pytest.mark.gpu
+def test_generated_fully_implicit_runs_and_matches_oracle(tmp_path, oracle_api):
+    """The fully implicit two-phase program (twophase_fully_implicit.equelle: one
+    NewtonSolveSystem on (p, sw) per step, makeArray of two lambdas) compiled against the
+    facade and run, against the oracle's run of the same program."""
+    from twophase_cases import case, fi_params, oracle_fi_run
+    assert os.path.exists(EXE_FI), "run __graft_entry__.build() first"
+    nx, ny, nz = 7, 6, 4
+    perm, poro, sw0, sc, sv = case(nx, ny, nz)
+    dts = np.array([0.05, 0.05, 0.1])   # steps plain Newton converges on from the guess sw = 0.5
+    d = tmp_path
+    for name, arr in (("perm", perm), ("poro", poro), ("sw_initial", sw0), ("source_values", sv), ("timesteps", dts)):
+        (d / f"{name}.txt").write_text("\n".join(f"{x:.17g}" for x in arr))
+    (d / "source_cells.txt").write_text(" ".join(str(int(c)) for c in sc))
+    (d / "run.param").write_text("\n".join([
+        "grid_dim=3", f"nx={nx}", f"ny={ny}", f"nz={nz}", "dx=1", "dy=1", "dz=1", "watervisc=0.001",
+        "perm_from_file=true", "perm_filename=perm.txt", "poro_from_file=true", "poro_filename=poro.txt",
+        "sw_initial_from_file=true", "sw_initial_filename=sw_initial.txt",
+        "source_cells_filename=source_cells.txt",
+        "source_values_from_file=true", "source_values_filename=source_values.txt",
+        "timesteps_filename=timesteps.txt", "output_to_file=true", "solver=GMRes", "preconditioner=diagonal",
+        "abs_res_tol=0", "rel_res_tol=1e-12", "max_iter=40", "solver_tol=1e-7"]) + "\n")
+    subprocess.run([EXE_FI, "run.param"], cwd=tmp_path, check=True, timeout=120, stdout=subprocess.DEVNULL)
+    sw_last = np.loadtxt(tmp_path / "saturation-00003.output")   # 00000 is the initial state
+    p_last = np.loadtxt(tmp_path / "pressure-00003.output")
+    params = fi_params(nx, ny, nz, rel_res_tol=1e-13, max_iter=40).encode()
+    po, so, _ = oracle_fi_run(oracle_api, params, perm, poro, sw0, sc, sv, dts)
     assert np.allclose(sw_last, so, rtol=1e-9, atol=1e-10)
     assert np.allclose(p_last - p_last[-1], po - po[-1], rtol=1e-9, atol=1e-9 * np.abs(po).max())
