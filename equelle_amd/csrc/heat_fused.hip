@@ -44,6 +44,7 @@ struct Desc {
     int pcm;                      // Krylov form / preconditioner (Pm)
     int ns;                       // Krylov slots per pair rank: 1 (reduced system) or 2 (full system)
     int ngrpk;                    // Krylov partials: groups [gbase, gbase + ngrpk) of BS pairs
+    int vec;                      // Krylov kernels in the four-ranks-per-lane form (2D heat, nx % 8 == 0)
     double rnx, rnxy;             // 1 / nx, 1 / (nx ny): cell -> (i, j, k) without integer division
     double dt;
     // heat: k(u) = k (1 + u^2), x / y face geometry factors, cell volume
@@ -589,6 +590,85 @@ __device__ inline T offdiag(const Desc& d, int s, int c, int i, int j, int k, co
     return acc;
 }
 
+// Four-ranks-per-lane form (Desc::vec: 2D heat, reduced system, nx % 8 == 0).  Lane l of
+// wave w holds the four consecutive pair ranks p0 .. p0 + 3, p0 = block start + g BS + 4 l,
+// of each group g = w, w + BS/64, ... < CPT of its block.  A row holds nx / 2 pair ranks, a
+// multiple of 4, so the four ranks lie in one row and every four-rank load is aligned (one
+// 16-byte load in FP32, two in FP64).  Rows and columns of the red-black layout then come
+// out of the loaded vectors: in row j the cells of the pairs p0 .. p0 + 3 are i = 2 q ..
+// 2 q + 7 (q = p0 - j nx / 2), so a cell's west and east neighbours (the other colour) are
+// ranks p - 1 and p when its column is even, p and p + 1 when it is odd, and its south and
+// north neighbours are ranks p -+ nx / 2.  Coefficients of absent neighbours are zero (the
+// assembly writes 0 there), so an in-bounds value of the same vector stands in for them.
+// A group's partial sum is the lane sums ((e0 + e1) + e2) + e3 reduced by a butterfly over
+// the wave: the same order for every CPT and block order.
+template <class T>
+struct V4 {
+    T e[4];
+};
+__device__ inline V4<float> ld4(const float* __restrict__ p) {
+    const float4 v = *reinterpret_cast<const float4*>(p);
+    return {{v.x, v.y, v.z, v.w}};
+}
+__device__ inline V4<double> ld4(const double* __restrict__ p) {
+    const double2 a = *reinterpret_cast<const double2*>(p), b = *reinterpret_cast<const double2*>(p + 2);
+    return {{a.x, a.y, b.x, b.y}};
+}
+__device__ inline void st4(float* __restrict__ p, const V4<float>& v) {
+    *reinterpret_cast<float4*>(p) = make_float4(v.e[0], v.e[1], v.e[2], v.e[3]);
+}
+__device__ inline void st4(double* __restrict__ p, const V4<double>& v) {
+    *reinterpret_cast<double2*>(p) = make_double2(v.e[0], v.e[1]);
+    *reinterpret_cast<double2*>(p + 2) = make_double2(v.e[2], v.e[3]);
+}
+#define FOR_VGROUPS(g, p0)                                                                          \
+    for (int g = (int)threadIdx.x >> 6, p0 = bi.y * (CPT * BS) + g * BS + 4 * ((int)threadIdx.x & 63); \
+         g < CPT; g += BS / 64, p0 += (BS / 64) * BS)
+__device__ inline double wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+// Group g's partial (lane 0 stores it when the group holds ranks of the instance).
+__device__ inline void vgroup_store(const Desc& d, int2 bi, int cpt, int g, double v, double* __restrict__ part) {
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0 && bi.y * cpt + g < d.ngrpk) part[d.gbase + bi.y * cpt + g] = v;
+}
+template <class T>
+__device__ inline T dot4(const V4<T>& a, const V4<T>& b) {
+    return ((a.e[0] * b.e[0] + a.e[1] * b.e[1]) + a.e[2] * b.e[2]) + a.e[3] * b.e[3];
+}
+template <class T>
+__device__ inline double dot4d(const V4<T>& a, const V4<T>& b) {
+    return (((double)a.e[0] * (double)b.e[0] + (double)a.e[1] * (double)b.e[1]) + (double)a.e[2] * (double)b.e[2]) +
+           (double)a.e[3] * (double)b.e[3];
+}
+// Row j and the row's pair offset q0 of rank p0 (2 p0 is the first cell of the pair).
+__device__ inline void vrow(const Desc& d, int p0, int& j, int& q0) {
+    int i0;
+    div_fix(2 * p0, d.rnx, d.nx, j, i0);
+    q0 = i0 >> 1;
+}
+// The off-diagonal sum W + E + S + N (coefficients kq at the ranks p0.., x the other
+// colour's vector) for the cells of colour col of row j.
+template <class T>
+__device__ inline V4<T> voffdiag(const Desc& d, const T* const* kq, int p0, int j, int q0, int col,
+                                 const T* __restrict__ x) {
+    const int hr = d.nx >> 1;
+    const V4<T> xc = ld4(x + p0);
+    const bool even = ((j + col) & 1) == 0;   // the cells of colour col have even columns i = 2 q
+    const T e = x[even ? (q0 > 0 ? p0 - 1 : p0) : (q0 + 4 < hr ? p0 + 4 : p0)];
+    const V4<T> xs = ld4(x + (j > 0 ? p0 - hr : p0)), xn = ld4(x + (j < d.ny - 1 ? p0 + hr : p0));
+    const V4<T> kw = ld4(kq[1] + p0), ke = ld4(kq[2] + p0), ks = ld4(kq[3] + p0), kn = ld4(kq[4] + p0);
+    V4<T> r;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        const T xw = even ? (m == 0 ? e : xc.e[m - 1]) : xc.e[m];
+        const T xe = even ? xc.e[m] : (m == 3 ? e : xc.e[m + 1]);
+        r.e[m] = kw.e[m] * xw + ke.e[m] * xe + ks.e[m] * xs.e[m] + kn.e[m] * xn.e[m];
+    }
+    return r;
+}
+
 // Reduced operator, pass 1 (red ranks): y_r = C_r^-1 (A_rb v_b), v = V[iv] (black slots).
 template <class T, int CPT>
 __global__ __launch_bounds__(BS) void k_schur_red(const Desc* __restrict__ D, const int2* __restrict__ blk, int iv) {
@@ -597,6 +677,21 @@ __global__ __launch_bounds__(BS) void k_schur_red(const Desc* __restrict__ D, co
     const T* __restrict__ v = (const T*)d.v[iv];
     const T* __restrict__ dv = (const T*)d.dinv;
     T* __restrict__ y = (T*)d.v[V_Y];
+    if (d.vec) {
+        const T* kq[5] = {nullptr, (const T*)d.K[1], (const T*)d.K[2], (const T*)d.K[3], (const T*)d.K[4]};
+        FOR_VGROUPS(g, p0) {
+            if (p0 >= d.h) continue;
+            int j, q0;
+            vrow(d, p0, j, q0);
+            const V4<T> o = voffdiag<T>(d, kq, p0, j, q0, 0, v);
+            const V4<T> dd = ld4(dv + p0);
+            V4<T> r;
+#pragma unroll
+            for (int m = 0; m < 4; ++m) r.e[m] = o.e[m] * dd.e[m];
+            st4(y + p0, r);
+        }
+        return;
+    }
     FOR_PAIRS(p) {
         int i, j, k;
         const int c = cell_of(d, p, 0, i, j, k);
@@ -611,6 +706,36 @@ __global__ __launch_bounds__(BS) void k_schur_black(const Desc* __restrict__ D, 
                                                     double* __restrict__ part, int ng) {
     LOCATE();
     if (d.pcm != PM_SCHUR) return;   // block-uniform: the full-system kernel writes these partials
+    if (d.vec) {
+        const int h = d.h;
+        const T* __restrict__ v = (const T*)d.v[iv];
+        const T* __restrict__ y = (const T*)d.v[V_Y];
+        const T* __restrict__ Cb = (const T*)d.K[0] + h;
+        T* __restrict__ w = (T*)d.v[iw];
+        const T* __restrict__ va = (const T*)d.v[a0];
+        const T* __restrict__ vb = a1 >= 0 ? (const T*)d.v[a1] : nullptr;
+        const T* kq[5] = {nullptr, (const T*)d.K[1] + h, (const T*)d.K[2] + h, (const T*)d.K[3] + h,
+                          (const T*)d.K[4] + h};
+        const bool on = d.act[0] != 0;
+        FOR_VGROUPS(g, p0) {
+            double s0 = 0.0, s1 = 0.0;
+            if (on && p0 < h) {
+                int j, q0;
+                vrow(d, p0, j, q0);
+                const V4<T> o = voffdiag<T>(d, kq, p0, j, q0, 1, y);
+                const V4<T> cb = ld4(Cb + p0), vv = ld4(v + p0), aa = ld4(va + p0);
+                V4<T> wc;
+#pragma unroll
+                for (int m = 0; m < 4; ++m) wc.e[m] = cb.e[m] * vv.e[m] - o.e[m];
+                st4(w + p0, wc);
+                s0 = dot4d(aa, wc);
+                if (nd > 1) s1 = dot4d(vb ? ld4(vb + p0) : wc, wc);
+            }
+            vgroup_store(d, bi, CPT, g, s0, part);
+            if (nd > 1) vgroup_store(d, bi, CPT, g, s1, part + ng);
+        }
+        return;
+    }
     double s0[CPT] = {}, s1[CPT] = {};
     if (d.act[0]) {
         const int h = d.h;
@@ -827,6 +952,25 @@ __global__ __launch_bounds__(BS) void k_b1(const Desc* __restrict__ D, const int
     const T beta = first ? (T)0 : (T)((sc[S_RHO] / sc[S_RHO_OLD]) * (sc[S_ALPHA] / sc[S_OMEGA]));
     const T omega = (T)sc[S_OMEGA];
     const T* dv = slot_dinv<T>(d);
+    if (d.vec) {   // reduced system: one slot, ph always written
+        FOR_VGROUPS(g, p0) {
+            if (p0 >= d.h) continue;
+            const V4<T> r = ld4((const T*)d.v[V_R] + p0), dd = ld4(dv + p0);
+            V4<T> pv, ph;
+            if (first) {
+                pv = r;
+            } else {
+                const V4<T> po = ld4(p + p0), vv = ld4((const T*)d.v[V_V] + p0);
+#pragma unroll
+                for (int m = 0; m < 4; ++m) pv.e[m] = r.e[m] + beta * (po.e[m] - omega * vv.e[m]);
+            }
+#pragma unroll
+            for (int m = 0; m < 4; ++m) ph.e[m] = dd.e[m] * pv.e[m];
+            st4(p + p0, pv);
+            st4((T*)d.v[V_PH] + p0, ph);
+        }
+        return;
+    }
     FOR_PAIRS(q) {
         for (int w = 0; w < d.ns; ++w) {
             const int s = q + w * d.h;
@@ -844,6 +988,21 @@ __global__ __launch_bounds__(BS) void k_b2(const Desc* __restrict__ D, const int
     if (!d.act[0]) return;
     const T alpha = (T)d.sc[S_ALPHA];
     const T* dv = slot_dinv<T>(d);
+    if (d.vec) {
+        FOR_VGROUPS(g, p0) {
+            if (p0 >= d.h) continue;
+            const V4<T> r = ld4((const T*)d.v[V_R] + p0), vv = ld4((const T*)d.v[V_V] + p0), dd = ld4(dv + p0);
+            V4<T> sv, sh;
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                sv.e[m] = r.e[m] - alpha * vv.e[m];
+                sh.e[m] = dd.e[m] * sv.e[m];
+            }
+            st4((T*)d.v[V_S] + p0, sv);
+            st4((T*)d.v[V_SH] + p0, sh);
+        }
+        return;
+    }
     FOR_PAIRS(q) {
         for (int w = 0; w < d.ns; ++w) {
             const int s = q + w * d.h;
@@ -858,6 +1017,32 @@ template <class T, int CPT>
 __global__ __launch_bounds__(BS) void k_b3(const Desc* __restrict__ D, const int2* __restrict__ blk,
                                            double* __restrict__ part, int ng) {
     LOCATE();
+    if (d.vec) {
+        const bool on = d.act[0] != 0;
+        const T alpha = on ? (T)d.sc[S_ALPHA] : (T)0, omega = on ? (T)d.sc[S_OMEGA] : (T)0;
+        T* x = (T*)d.v[V_X];
+        FOR_VGROUPS(g, p0) {
+            double s0 = 0.0, s1 = 0.0;
+            if (on && p0 < d.h) {
+                const V4<T> xo = ld4(x + p0), ph = ld4((const T*)d.v[V_PH] + p0), sh = ld4((const T*)d.v[V_SH] + p0);
+                const V4<T> sv = ld4((const T*)d.v[V_S] + p0), tv = ld4((const T*)d.v[V_T] + p0);
+                const V4<T> rh = ld4((const T*)d.v[V_RH] + p0);
+                V4<T> xn, r;
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    xn.e[m] = xo.e[m] + (alpha * ph.e[m] + omega * sh.e[m]);
+                    r.e[m] = sv.e[m] - omega * tv.e[m];
+                }
+                st4(x + p0, xn);
+                st4((T*)d.v[V_R] + p0, r);
+                s0 = dot4d(rh, r);
+                s1 = dot4d(r, r);
+            }
+            vgroup_store(d, bi, CPT, g, s0, part);
+            vgroup_store(d, bi, CPT, g, s1, part + ng);
+        }
+        return;
+    }
     double a0[CPT] = {}, a1[CPT] = {};
     if (d.act[0]) {
         const T alpha = (T)d.sc[S_ALPHA], omega = (T)d.sc[S_OMEGA];
@@ -1176,6 +1361,7 @@ struct HeatBatch::Impl {
         d.h = (int)h;
         d.pcm = I.pcm;
         d.ns = I.ns();
+        d.vec = cfg.pair_vectors && I.kind == K_HEAT && I.pcm == PM_SCHUR && I.nx % 8 == 0;
         for (int q = 0; q < 7; ++q) d.K[q] = q < I.ncoef() ? I.kcoef->p + q * 2 * h * ks() : nullptr;
         d.dinv = I.dinv->p;
         for (int v = 0; v < NVEC; ++v) d.v[v] = I.vecs->p + (size_t)v * d.ns * h * ks();

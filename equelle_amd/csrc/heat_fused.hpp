@@ -41,6 +41,10 @@ struct FusedConfig {
     // the whole system with the two-pass red-black ILU(0)); equal in exact arithmetic.
     std::string krylov_form_heat = "reduced";
     std::string krylov_form_twophase = "full";
+    // 2D heat instances with nx % 8 == 0 in the reduced form: Krylov kernels give each lane
+    // four consecutive pair ranks of one row (16-byte loads) instead of ranks strided by the
+    // block size.  Other instances always take the strided form.
+    bool pair_vectors = true;
     int max_refine = 20;
     bool graph = true;                 // capture BiCGStab iterations in a hipGraph
     int cpt = 0;                       // cells per thread (1, 2, 4, 8); 0 = chosen by the tuner

@@ -113,9 +113,10 @@ def test_fused_batch_steps_match_oracle(hip_api, oracle_api, params):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_result_identical_alone_and_in_batch(hip_api, precision):
+@pytest.mark.parametrize("nx", [77, 80])   # 80: four-ranks-per-lane Krylov kernels
+def test_result_identical_alone_and_in_batch(hip_api, precision, nx):
     params = f"precision={precision} preconditioner=ilu0 rel_res_tol=1e-8 solver_tol=1e-6"
-    nx, ny, k = 77, 53, 0.3
+    ny, k = 53, 0.3
     u0 = gaussian_bumps(nx, ny, 7, width_scale=0.05)
     alone = make_batch(hip_api, params)
     mixed = make_batch(hip_api, params)
@@ -207,6 +208,34 @@ def test_graph_replays_match_eager(hip_api):
             hip_api.batch_destroy(b)
     for a, g in zip(res[0], res[1]):
         assert np.array_equal(a, g)
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("mixed", 1e-9)])
+def test_pair_vectors_match_strided_form(hip_api, precision, tol):
+    """The four-ranks-per-lane Krylov kernels (2D heat, nx % 8 == 0) and the strided
+    ones solve the same systems: the steps agree to the solve tolerance (the partial
+    sums are added in another order, so the iterates differ in rounding)."""
+    sizes = [(256, 192, 0.2), (512, 512, 0.1), (64, 8, 0.4)]
+    out = {}
+    for pv in (0, 1):
+        b = make_batch(hip_api, f"precision={precision} preconditioner=ilu0 rel_res_tol=1e-10 abs_res_tol=0 "
+                                f"solver_tol=1e-10 max_iter=30 pair_vectors={pv}")
+        try:
+            slots = [add(hip_api, b, nx, ny, k, gaussian_bumps(nx, ny, 40 + q, width_scale=nx / 2048))
+                     for q, (nx, ny, k) in enumerate(sizes)]
+            arr = np.array(slots, dtype=np.int32)
+            hip_api.batch_step(b, iptr(arr), len(arr), 1.0)
+            hip_api.batch_step(b, iptr(arr), len(arr), 1.0)
+            res = []
+            for s, (nx, ny, _) in zip(slots, sizes):
+                st, nrm = stats(hip_api, b, s)
+                assert st[3] in (1, 2) and st[4] == 0, (pv, st, nrm)
+                res.append(state(hip_api, b, s, nx * ny))
+            out[pv] = res
+        finally:
+            hip_api.batch_destroy(b)
+    for a, v in zip(out[0], out[1]):
+        assert np.linalg.norm(a - v) <= tol * np.linalg.norm(a), np.linalg.norm(a - v) / np.linalg.norm(a)
 
 
 def test_launch_shapes_give_identical_results(hip_api):
