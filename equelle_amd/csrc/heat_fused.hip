@@ -1735,8 +1735,7 @@ struct HeatBatch::Impl {
     // for it, else the full system with the batch's preconditioner.
     int form_of(int kind) const {
         if (pc != PC_ILU) return pc == PC_JACOBI ? PM_JACOBI : PM_NONE;
-        const std::string& f = kind == K_HEAT ? cfg.krylov_form_heat : cfg.krylov_form_twophase;
-        return f == "reduced" ? PM_SCHUR : PM_ILU;
+        return cfg.reduced_form(kind == K_TWOPHASE) ? PM_SCHUR : PM_ILU;
     }
     void newton_update(const Table& t) { HL(C_VEC, (k_newton_update<CP>)); }
 #undef HL
@@ -1764,6 +1763,11 @@ HeatBatch::HeatBatch(const FusedConfig& cfg) : p_(new Impl()) {
     const std::string pc = cfg.precond == "diagonal" ? "jacobi" : cfg.precond;
     EQ_REQUIRE(pc == "jacobi" || pc == "ilu0" || pc == "none", "preconditioner must be jacobi, ilu0 or none");
     p_->pc = pc == "jacobi" ? PC_JACOBI : pc == "ilu0" ? PC_ILU : PC_NONE;
+    EQ_REQUIRE(cfg.krylov_form_heat == "reduced" || cfg.krylov_form_heat == "full" || cfg.krylov_form_heat == "auto",
+               "krylov_form_heat must be reduced, full or auto");
+    EQ_REQUIRE(cfg.krylov_form_twophase == "reduced" || cfg.krylov_form_twophase == "full" ||
+                   cfg.krylov_form_twophase == "auto",
+               "krylov_form_twophase must be reduced, full or auto");
     p_->use_graph = cfg.graph;
     p_->count = dev_alloc<int>(2 * Impl::NRING);
     EQ_HIP(hipHostMalloc((void**)&p_->host_count, 2 * Impl::NRING * sizeof(int), hipHostMallocDefault));

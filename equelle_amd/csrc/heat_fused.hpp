@@ -39,8 +39,16 @@ struct FusedConfig {
     // Krylov form with preconditioner ilu0, per instance kind: "reduced" (BiCGStab on the
     // red-black Schur complement, preconditioned by its diagonal) or "full" (BiCGStab on
     // the whole system with the two-pass red-black ILU(0)); equal in exact arithmetic.
+    // "auto" (two-phase): reduced in FP64, full in mixed precision, where FP32 rounding in
+    // the reduced operator costs convergence on the singular pressure system.
     std::string krylov_form_heat = "reduced";
-    std::string krylov_form_twophase = "full";
+    std::string krylov_form_twophase = "auto";
+    // Whether instances of the kind run the reduced form (preconditioner ilu0 only).
+    bool reduced_form(bool twophase) const {
+        if (precond != "ilu0") return false;
+        const std::string& f = twophase ? krylov_form_twophase : krylov_form_heat;
+        return f == "reduced" || (f == "auto" && precision != "mixed");
+    }
     // 2D heat instances with nx % 8 == 0 in the reduced form: Krylov kernels give each lane
     // four consecutive pair ranks of one row (16-byte loads) instead of ranks strided by the
     // block size.  Other instances always take the strided form.

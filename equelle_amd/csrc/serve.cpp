@@ -25,7 +25,7 @@ HeatServer::HeatServer(const ServerConfig& cfg)
 int HeatServer::submit(int nx, int ny, double k, double cv, const double* u0, int steps, double dt) {
     EQ_REQUIRE(nx > 0 && ny > 0 && (long)nx * ny < (1L << 31), "bad request grid " << nx << "x" << ny);
     EQ_REQUIRE(steps > 0 && dt > 0, "a request needs steps > 0 and dt > 0");
-    const double bytes = (double)HeatBatch::estimate_bytes(nx * ny, cfg_.batch.precision, false, cfg_.batch.precond == "ilu0" && cfg_.batch.krylov_form_heat == "reduced");
+    const double bytes = (double)HeatBatch::estimate_bytes(nx * ny, cfg_.batch.precision, false, cfg_.batch.reduced_form(false));
     EQ_REQUIRE(bytes <= cfg_.hbm_budget, "request of " << bytes << " bytes exceeds the HBM budget of "
                                                        << cfg_.hbm_budget);
     Req r;
@@ -50,7 +50,8 @@ int HeatServer::submit_twophase(const TwoPhaseSpec& spec, const double* perm, co
     const long n = (long)spec.nx * spec.ny * spec.nz;
     EQ_REQUIRE(spec.nx > 0 && spec.ny > 0 && spec.nz > 0 && n < (1L << 31), "bad request grid");
     EQ_REQUIRE(steps > 0 && dt > 0, "a request needs steps > 0 and dt > 0");
-    const double bytes = (double)HeatBatch::estimate_bytes((int)n, cfg_.twophase_precision, true, cfg_.batch.precond == "ilu0" && cfg_.batch.krylov_form_twophase == "reduced");
+    const double bytes = (double)HeatBatch::estimate_bytes((int)n, cfg_.twophase_precision, true,
+                                                             twophase_config(cfg_).reduced_form(true));
     EQ_REQUIRE(bytes <= cfg_.hbm_budget, "request of " << bytes << " bytes exceeds the HBM budget of "
                                                        << cfg_.hbm_budget);
     Req r;

@@ -173,8 +173,10 @@ int eqhip_twophase_destroy(void* sim);
  * params: precision (fp64|mixed) preconditioner (diagonal|ilu0|none) rel_res_tol abs_res_tol
  * max_iter solver_tol solver_max_iter inner_tol (heat, default 1e-6) inner_tol_twophase (default 1e-4)
  * max_refine graph (0|1) cpt xcd tune krylov_form_heat (reduced|full, default reduced)
- * krylov_form_twophase (reduced|full, default full): with preconditioner ilu0, BiCGStab on the
- * red-black reduced (Schur complement) system or on the full system with red-black ILU(0). */
+ * krylov_form_twophase (reduced|full|auto, default auto = reduced in fp64, full in mixed): with
+ * preconditioner ilu0, BiCGStab on the red-black reduced (Schur complement) system or on the
+ * full system with red-black ILU(0).  pair_vectors (0|1, default 1): four pair ranks per lane
+ * in the Krylov kernels of 2D heat instances with nx % 8 == 0. */
 int eqhip_batch_create(const char* params, void** batch);
 int eqhip_batch_destroy(void* batch);
 int eqhip_batch_add(void* batch, int nx, int ny, double k, double cv, const double* u, int* slot);

@@ -211,7 +211,8 @@ def test_fused_twophase_residual_matches_oracle(hip_api, oracle_api, dims, stage
 # 1e-7; FP32 with ILU(0) runs at the benchmark's tolerances (Newton 1e-8, linear 1e-6),
 # compared at 1e-5.
 @pytest.mark.parametrize("params,newton_tol,solver_tol,bar", [
-    ("precision=fp64 preconditioner=ilu0", 1e-10, 1e-10, 1e-7),
+    ("precision=fp64 preconditioner=ilu0", 1e-10, 1e-10, 1e-7),   # reduced form (auto in FP64)
+    ("precision=fp64 preconditioner=ilu0 krylov_form_twophase=full", 1e-10, 1e-10, 1e-7),
     ("precision=fp64 preconditioner=diagonal", 1e-10, 1e-10, 1e-7),
     ("precision=mixed preconditioner=diagonal", 1e-10, 1e-10, 1e-7),
     ("precision=mixed preconditioner=ilu0", 1e-8, 1e-6, 1e-5)])
@@ -241,11 +242,12 @@ def test_fused_twophase_steps_match_oracle(hip_api, oracle_api, params, newton_t
     assert np.allclose(ph - ph[-1], po - po[-1], rtol=bar, atol=bar * np.abs(po).max())
 
 
-def test_fused_twophase_and_heat_share_a_batch(hip_api):
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])   # fp64: both kinds in the reduced form
+def test_fused_twophase_and_heat_share_a_batch(hip_api, precision):
     """Two-phase and heat instances in one batch give bit-identical results to each run
     alone (per-instance reductions, stage-independent launches)."""
     from equelle_amd.workloads import gaussian_bumps
-    params = "precision=mixed preconditioner=ilu0 rel_res_tol=1e-8 solver_tol=1e-6 max_iter=30"
+    params = f"precision={precision} preconditioner=ilu0 rel_res_tol=1e-8 solver_tol=1e-6 max_iter=30"
     tp = [(10, 9, 7), (20, 15, 4)]
     heat = [(64, 40, 0.2)]
 
