@@ -22,10 +22,13 @@ inline hipStream_t S() { return Device::get().stream(); }
 // per-instance scalars; S_TOL2: the square of the instance's inner (mixed-precision) Krylov tolerance
 // S_MEAN: the mean of the Krylov right-hand side over the cells (singular systems only)
 enum Slot { S_RHO = 0, S_RHO_OLD, S_ALPHA, S_OMEGA, S_RR, S_BB, S_BREAK, S_IT, S_FN, S_RN, S_BN, S_REF, S_ITT, S_TOL2,
-            S_MEAN, NSLOT = 16 };
+            S_MEAN, S_BETA, NSLOT = 16 };
 // Krylov vectors; V_Y is the red-rank scratch of the reduced operator (ilu0 only)
 enum Vec { V_R = 0, V_RH, V_P, V_V, V_S, V_T, V_PH, V_SH, V_X, V_Y, NVEC };
-enum Act { A_INIT = 0, A_ALPHA, A_OMEGA, A_RHO, A_STORE, A_MEAN };
+enum Act { A_INIT = 0, A_ALPHA, A_OMEGA, A_RHO, A_STORE, A_MEAN, A_CG_ALPHA, A_CG_RHO };
+// Krylov methods of a solve (bit mask): BiCGStab, and CG for symmetric systems (the
+// two-phase pressure stage in the reduced form)
+enum Meth { M_BICG = 1, M_CG = 2 };
 enum Pc { PC_NONE = 0, PC_JACOBI = 1, PC_ILU = 2 };
 // Krylov form and right preconditioner of one instance: the full system with no
 // preconditioner, Jacobi, or red-black ILU(0) applied in two passes; or the red-black
@@ -45,6 +48,7 @@ struct Desc {
     int ns;                       // Krylov slots per pair rank: 1 (reduced system) or 2 (full system)
     int ngrpk;                    // Krylov partials: groups [gbase, gbase + ngrpk) of BS pairs
     int vec;                      // Krylov kernels in the four-ranks-per-lane form (2D heat, nx % 8 == 0)
+    int cgp;                      // two-phase, reduced form: the pressure stage (symmetric) is solved by CG
     double rnx, rnxy;             // 1 / nx, 1 / (nx ny): cell -> (i, j, k) without integer division
     double dt;
     // heat: k(u) = k (1 + u^2), x / y face geometry factors, cell volume
@@ -67,6 +71,12 @@ struct Desc {
     double* sc;                   // NSLOT scalars
     int* act;                     // [0] Krylov active, [1] refinement active, [2] two-phase stage
 };
+
+// The Krylov method of an instance's current solve: CG for the two-phase pressure stage
+// (a symmetric M-matrix: both cells of a face see the same upwinded mobility) in the
+// reduced form, BiCGStab otherwise.  Kernels and finaliser actions of one method skip the
+// instances of the other, so both can run in one batch.
+__device__ inline int meth_of(const Desc& d) { return d.cgp && d.act[2] == 0 ? M_CG : M_BICG; }
 
 // What the last block of an instance does with the summed partials.
 struct Fin {
@@ -225,20 +235,20 @@ __device__ void fin_body(const Desc& d, const double* part, int nb, const Fin& f
             break;
         }
         case A_ALPHA:
-            if (!d.act[0]) break;
+            if (!d.act[0] || meth_of(d) != M_BICG) break;
             if (res[0] == 0.0 || !isfinite(res[0])) {
                 sc[S_BREAK] = 1.0;
                 d.act[0] = 0;
             } else sc[S_ALPHA] = sc[S_RHO] / res[0];
             break;
         case A_OMEGA:
-            if (!d.act[0]) break;
+            if (!d.act[0] || meth_of(d) != M_BICG) break;
             if (res[1] == 0.0 || !isfinite(res[1])) {
                 sc[S_OMEGA] = 0.0;   // s already solves the system; finish with x += alpha ph
             } else sc[S_OMEGA] = res[0] / res[1];
             break;
         case A_RHO: {
-            if (!d.act[0]) break;
+            if (!d.act[0] || meth_of(d) != M_BICG) break;
             sc[S_RHO_OLD] = sc[S_RHO];
             sc[S_RHO] = res[0];
             sc[S_RR] = res[1];
@@ -247,6 +257,26 @@ __device__ void fin_body(const Desc& d, const double* part, int nb, const Fin& f
             const bool ok = isfinite(res[1]) && sc[S_OMEGA] != 0.0;
             if (!ok) sc[S_BREAK] = 2.0;
             // tol2 < 0: the instance's own tolerance (S_TOL2)
+            d.act[0] = ok && res[1] > (tol2 < 0.0 ? sc[S_TOL2] : tol2) * sc[S_BB] && sc[S_IT] < maxit && res[0] != 0.0;
+            break;
+        }
+        case A_CG_ALPHA:   // alpha = rho / p.q; a non-positive curvature ends the solve
+            if (!d.act[0] || meth_of(d) != M_CG) break;
+            if (!(res[0] > 0.0) || !isfinite(res[0])) {
+                sc[S_BREAK] = 1.0;
+                d.act[0] = 0;
+            } else sc[S_ALPHA] = sc[S_RHO] / res[0];
+            break;
+        case A_CG_RHO: {   // rho' = r.z, beta = rho' / rho, ||r||^2; convergence as A_RHO
+            if (!d.act[0] || meth_of(d) != M_CG) break;
+            const double beta = res[0] / sc[S_RHO];
+            sc[S_BETA] = beta;
+            sc[S_RHO] = res[0];
+            sc[S_RR] = res[1];
+            sc[S_IT] += 1.0;
+            sc[S_ITT] += 1.0;
+            const bool ok = isfinite(res[1]) && isfinite(beta);
+            if (!ok) sc[S_BREAK] = 2.0;
             d.act[0] = ok && res[1] > (tol2 < 0.0 ? sc[S_TOL2] : tol2) * sc[S_BB] && sc[S_IT] < maxit && res[0] != 0.0;
             break;
         }
@@ -671,9 +701,10 @@ __device__ inline V4<T> voffdiag(const Desc& d, const T* const* kq, int p0, int 
 
 // Reduced operator, pass 1 (red ranks): y_r = C_r^-1 (A_rb v_b), v = V[iv] (black slots).
 template <class T, int CPT>
-__global__ __launch_bounds__(BS) void k_schur_red(const Desc* __restrict__ D, const int2* __restrict__ blk, int iv) {
+__global__ __launch_bounds__(BS) void k_schur_red(const Desc* __restrict__ D, const int2* __restrict__ blk, int iv,
+                                                  int meth) {
     LOCATE();
-    if (d.pcm != PM_SCHUR || !d.act[0]) return;
+    if (d.pcm != PM_SCHUR || !d.act[0] || !(meth & meth_of(d))) return;
     const T* __restrict__ v = (const T*)d.v[iv];
     const T* __restrict__ dv = (const T*)d.dinv;
     T* __restrict__ y = (T*)d.v[V_Y];
@@ -703,9 +734,10 @@ __global__ __launch_bounds__(BS) void k_schur_red(const Desc* __restrict__ D, co
 template <class T, int CPT>
 __global__ __launch_bounds__(BS) void k_schur_black(const Desc* __restrict__ D, const int2* __restrict__ blk,
                                                     int iv, int iw, int nd, int a0, int a1,
-                                                    double* __restrict__ part, int ng) {
+                                                    double* __restrict__ part, int ng, int meth) {
     LOCATE();
-    if (d.pcm != PM_SCHUR) return;   // block-uniform: the full-system kernel writes these partials
+    // block-uniform: the full-system kernel, or the other method's launch, writes these partials
+    if (d.pcm != PM_SCHUR || !(meth & meth_of(d))) return;
     if (d.vec) {
         const int h = d.h;
         const T* __restrict__ v = (const T*)d.v[iv];
@@ -901,14 +933,41 @@ __global__ __launch_bounds__(BS) void k_rhs_sum(const Desc* __restrict__ D, cons
     group_store<CPT>(a, PARTSK(part));
 }
 
-// Krylov start: r = b (already in r; minus its mean for singular systems), rh = r, x = 0,
-// p = v = 0; partial dots (rh.r, r.r).
+// The right preconditioner's diagonal for slot s: reduced system -> 1 / P_b of black rank
+// s; full system -> dinv of slot s (jacobi) or 1 (none, dinv = 1).
+template <class T>
+__device__ inline const T* slot_dinv(const Desc& d) {
+    return (const T*)d.dinv + (d.ns == 1 ? d.h : 0);
+}
+
+// Krylov start: r = b (already in r; minus its mean for singular systems), x = 0.
+// BiCGStab: rh = r, p = v = 0; partial dots (rh.r, r.r).  CG: p = z = M^-1 r; partial
+// dots (r.z, r.r).
 template <class T, int CPT>
 __global__ __launch_bounds__(BS) void k_kinit(const Desc* __restrict__ D, const int2* __restrict__ blk,
                                               double* __restrict__ part, int ng) {
     LOCATE();
-    double rr[CPT] = {};
-    if (d.act[1]) {
+    double rr[CPT] = {}, rz[CPT] = {};
+    const bool cg = meth_of(d) == M_CG;
+    if (d.act[1] && cg) {   // reduced form: one slot per pair
+        const bool proj = singular(d);
+        const T mean = proj ? (T)d.sc[S_MEAN] : (T)0;
+        const T* __restrict__ dv = slot_dinv<T>(d);
+        FOR_GROUPS(m, p) {
+            if (p >= d.h) continue;
+            T rc = ((T*)d.v[V_R])[p];
+            if (proj) {
+                int i, j, k;
+                if (cell_of(d, p, 1, i, j, k) >= 0) rc -= mean;
+                ((T*)d.v[V_R])[p] = rc;
+            }
+            const T z = dv[p] * rc;
+            ((T*)d.v[V_P])[p] = z;
+            ((T*)d.v[V_X])[p] = 0;
+            rz[m] = (double)rc * (double)z;
+            rr[m] = (double)rc * (double)rc;
+        }
+    } else if (d.act[1]) {
         const bool proj = singular(d);
         const T mean = proj ? (T)d.sc[S_MEAN] : (T)0;
         FOR_GROUPS(m, p) {
@@ -929,15 +988,47 @@ __global__ __launch_bounds__(BS) void k_kinit(const Desc* __restrict__ D, const 
             }
         }
     }
-    group_store<CPT>(rr, PARTSK(part));
+    group_store<CPT>(cg ? rz : rr, PARTSK(part));
     group_store<CPT>(rr, PARTSK(part + ng));
 }
 
-// The right preconditioner's diagonal for slot s: reduced system -> 1 / P_b of black rank
-// s; full system -> dinv of slot s (jacobi) or 1 (none, dinv = 1).
-template <class T>
-__device__ inline const T* slot_dinv(const Desc& d) {
-    return (const T*)d.dinv + (d.ns == 1 ? d.h : 0);
+// CG (reduced form, one slot per pair; q = S p in V_V): x += alpha p, r -= alpha q;
+// partial dots (r.z, r.r) with z = M^-1 r.
+template <class T, int CPT>
+__global__ __launch_bounds__(BS) void k_cg2(const Desc* __restrict__ D, const int2* __restrict__ blk,
+                                            double* __restrict__ part, int ng) {
+    LOCATE();
+    if (meth_of(d) != M_CG) return;   // block-uniform; the BiCGStab kernel writes these partials
+    double a0[CPT] = {}, a1[CPT] = {};
+    if (d.act[0]) {
+        const T alpha = (T)d.sc[S_ALPHA];
+        T* __restrict__ x = (T*)d.v[V_X];
+        T* __restrict__ r = (T*)d.v[V_R];
+        const T* __restrict__ pp = (const T*)d.v[V_P];
+        const T* __restrict__ q = (const T*)d.v[V_V];
+        const T* __restrict__ dv = slot_dinv<T>(d);
+        FOR_GROUPS(m, p) {
+            if (p >= d.h) continue;
+            x[p] += alpha * pp[p];
+            const T rn = r[p] - alpha * q[p];
+            r[p] = rn;
+            a0[m] = (double)rn * (double)(dv[p] * rn);
+            a1[m] = (double)rn * (double)rn;
+        }
+    }
+    group_store<CPT>(a0, PARTSK(part));
+    group_store<CPT>(a1, PARTSK(part + ng));
+}
+// CG direction: p = M^-1 r + beta p
+template <class T, int CPT>
+__global__ __launch_bounds__(BS) void k_cg3(const Desc* __restrict__ D, const int2* __restrict__ blk) {
+    LOCATE();
+    if (!d.act[0] || meth_of(d) != M_CG) return;
+    const T beta = (T)d.sc[S_BETA];
+    T* __restrict__ pp = (T*)d.v[V_P];
+    const T* __restrict__ r = (const T*)d.v[V_R];
+    const T* __restrict__ dv = slot_dinv<T>(d);
+    FOR_PAIRS(p) pp[p] = dv[p] * r[p] + beta * pp[p];
 }
 
 // p = r + beta (p - omega v) (first iteration: p = r); ph = M^-1 p (diagonal M; PM_ILU:
@@ -945,7 +1036,7 @@ __device__ inline const T* slot_dinv(const Desc& d) {
 template <class T, int CPT>
 __global__ __launch_bounds__(BS) void k_b1(const Desc* __restrict__ D, const int2* __restrict__ blk) {
     LOCATE();
-    if (!d.act[0]) return;
+    if (!d.act[0] || meth_of(d) != M_BICG) return;
     const double* sc = d.sc;
     T* p = (T*)d.v[V_P];
     const bool first = sc[S_IT] == 0.0;
@@ -985,7 +1076,7 @@ __global__ __launch_bounds__(BS) void k_b1(const Desc* __restrict__ D, const int
 template <class T, int CPT>
 __global__ __launch_bounds__(BS) void k_b2(const Desc* __restrict__ D, const int2* __restrict__ blk) {
     LOCATE();
-    if (!d.act[0]) return;
+    if (!d.act[0] || meth_of(d) != M_BICG) return;
     const T alpha = (T)d.sc[S_ALPHA];
     const T* dv = slot_dinv<T>(d);
     if (d.vec) {
@@ -1017,6 +1108,7 @@ template <class T, int CPT>
 __global__ __launch_bounds__(BS) void k_b3(const Desc* __restrict__ D, const int2* __restrict__ blk,
                                            double* __restrict__ part, int ng) {
     LOCATE();
+    if (meth_of(d) != M_BICG) return;   // block-uniform; the CG kernels write these partials
     if (d.vec) {
         const bool on = d.act[0] != 0;
         const T alpha = on ? (T)d.sc[S_ALPHA] : (T)0, omega = on ? (T)d.sc[S_OMEGA] : (T)0;
@@ -1284,8 +1376,9 @@ struct Table {
     int solves = 0;   // Krylov solves run on this membership (graphs are captured from the second)
     // captured BiCGStab iteration, and the iterations the last solve needed, keyed by
     // (tolerance, max iterations)
-    std::map<std::pair<double, int>, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
-    std::map<std::pair<double, int>, int> pred;
+    // keyed by (tolerance, iteration cap, Krylov methods present)
+    std::map<std::tuple<double, int, int>, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
+    std::map<std::tuple<double, int, int>, int> pred;
     ~Table() {
         for (auto& g : graphs) {
             (void)hipGraphExecDestroy(g.second.second);
@@ -1314,6 +1407,10 @@ struct HeatBatch::Impl {
 
     size_t ks() const { return mixed ? sizeof(float) : sizeof(double); }
 
+    // two-phase instances in the reduced form solve their pressure stage by CG
+    bool cg_capable(const Inst& I) const {
+        return cfg.twophase_pressure_cg && I.kind == K_TWOPHASE && I.pcm == PM_SCHUR;
+    }
     Desc desc_of(Inst& I) const {
         Desc d{};
         d.n = I.n;
@@ -1362,6 +1459,7 @@ struct HeatBatch::Impl {
         d.pcm = I.pcm;
         d.ns = I.ns();
         d.vec = cfg.pair_vectors && I.kind == K_HEAT && I.pcm == PM_SCHUR && I.nx % 8 == 0;
+        d.cgp = cg_capable(I);
         for (int q = 0; q < 7; ++q) d.K[q] = q < I.ncoef() ? I.kcoef->p + q * 2 * h * ks() : nullptr;
         d.dinv = I.dinv->p;
         for (int v = 0; v < NVEC; ++v) d.v[v] = I.vecs->p + (size_t)v * d.ns * h * ks();
@@ -1598,12 +1696,13 @@ struct HeatBatch::Impl {
         HL(C_DEF, (k_defect<CP>), t.part->p);
     }
     // w = A v (full-system instances) and S v (reduced-system instances) with the fused dots
+    // (reduced-system instances of the methods in meth only)
     template <class T>
-    void launch_op(const Table& t, int iv, int iw, int nd, int a0, int a1) {
+    void launch_op(const Table& t, int iv, int iw, int nd, int a0, int a1, int meth = M_BICG | M_CG) {
         TimedLaunch tl(sizeof(T) == 4 ? "operator_fp32" : "operator_fp64");
         if (t.has_schur) {
-            HLP(C_SPMV, (k_schur_red<T, CP>), iv);
-            HLP(C_SPMV, (k_schur_black<T, CP>), iv, iw, nd, a0, a1, t.part->p, t.ng);
+            HLP(C_SPMV, (k_schur_red<T, CP>), iv, meth);
+            HLP(C_SPMV, (k_schur_black<T, CP>), iv, iw, nd, a0, a1, t.part->p, t.ng, meth);
         }
         if (t.has_full) HLP(C_SPMV, (k_spmv_rb<T, CP>), iv, iw, nd, a0, a1, t.part->p, t.ng);
     }
@@ -1615,8 +1714,8 @@ struct HeatBatch::Impl {
         HLP(C_SPMV, (k_ilu_red<T, CP>), ir, iz);
     }
     template <class T>
-    void spmv_dot(const Table& t, int ix, int iy, int nd, int a0, int a1, const Fin& f) {
-        launch_op<T>(t, ix, iy, nd, a0, a1);
+    void spmv_dot(const Table& t, int ix, int iy, int nd, int a0, int a1, const Fin& f, int meth) {
+        launch_op<T>(t, ix, iy, nd, a0, a1, meth);
         fin(t, f);
     }
     template <class T>
@@ -1631,18 +1730,35 @@ struct HeatBatch::Impl {
         HLP(C_VEC, (k_b3<T, CP>), t.part->p, t.ng);
     }
 
-    // BiCGStab for the instances whose refinement flag is set; rhs in V_R.
-    // (right preconditioner: ph = M^-1 p and sh = M^-1 s are formed in k_b1 / k_b2)
+    // One Krylov iteration for the instances whose refinement flag is set (rhs in V_R):
+    // BiCGStab (right preconditioner: ph = M^-1 p and sh = M^-1 s are formed in k_b1 /
+    // k_b2) for the M_BICG instances, preconditioned CG for the M_CG ones; meth = the
+    // methods present in the solve.
     template <class T>
-    void bicg_iteration(const Table& t, double tol, int maxit) {
-        HLP(C_VEC, (k_b1<T, CP>));
-        ilu_apply<T>(t, V_P, V_PH);
-        spmv_dot<T>(t, V_PH, V_V, 1, (int)V_RH, -1, F(A_ALPHA, 1));   // v = A ph; alpha = rho / rh.v
-        HLP(C_VEC, (k_b2<T, CP>));
-        ilu_apply<T>(t, V_S, V_SH);
-        spmv_dot<T>(t, V_SH, V_T, 2, (int)V_S, -1, F(A_OMEGA, 2));    // t = A sh; omega = t.s / t.t
-        HLP(C_VEC, (k_b3<T, CP>), t.part->p, t.ng);
-        fin(t, F(A_RHO, 2, tol < 0.0 ? -1.0 : tol * tol, maxit));
+    void krylov_iteration(const Table& t, double tol, int maxit, int meth) {
+        const double tol2 = tol < 0.0 ? -1.0 : tol * tol;
+        if (meth & M_BICG) {
+            HLP(C_VEC, (k_b1<T, CP>));
+            ilu_apply<T>(t, V_P, V_PH);
+            spmv_dot<T>(t, V_PH, V_V, 1, (int)V_RH, -1, F(A_ALPHA, 1), M_BICG);   // v = A ph; alpha = rho / rh.v
+            HLP(C_VEC, (k_b2<T, CP>));
+            ilu_apply<T>(t, V_S, V_SH);
+            spmv_dot<T>(t, V_SH, V_T, 2, (int)V_S, -1, F(A_OMEGA, 2), M_BICG);    // t = A sh; omega = t.s / t.t
+            HLP(C_VEC, (k_b3<T, CP>), t.part->p, t.ng);
+            fin(t, F(A_RHO, 2, tol2, maxit));
+        }
+        if (meth & M_CG) {
+            spmv_dot<T>(t, V_P, V_V, 1, (int)V_P, -1, F(A_CG_ALPHA, 1), M_CG);    // q = S p; alpha = rho / p.q
+            HLP(C_VEC, (k_cg2<T, CP>), t.part->p, t.ng);                         // x, r; r.z, r.r
+            fin(t, F(A_CG_RHO, 2, tol2, maxit));
+            HLP(C_VEC, (k_cg3<T, CP>));                                          // p = z + beta p
+        }
+    }
+    // The methods of the instances of a solve (host-side stage of each instance).
+    int methods(const Table& t) const {
+        int m = 0;
+        for (const Inst* I : t.inst) m |= cg_capable(*I) && I->stage == 0 ? M_CG : M_BICG;
+        return m;
     }
 
     // One BiCGStab solve.  From the second solve on a membership, one iteration is
@@ -1660,7 +1776,8 @@ struct HeatBatch::Impl {
         }
         HLP(C_VEC, (k_kinit<T, CP>), t.part->p, t.ng);
         fin(t, F(A_INIT, 2));
-        const auto key = std::make_pair(tol, maxit);
+        const int meth = methods(t);
+        const auto key = std::make_tuple(tol, maxit, meth);
         int iters = 0;
         if (use_graph && !timing_enabled() && t.solves++ > 0) {
             auto g = t.graphs.find(key);
@@ -1668,7 +1785,7 @@ struct HeatBatch::Impl {
                 hipGraph_t graph;
                 hipGraphExec_t exec;
                 EQ_HIP(hipStreamBeginCapture(S(), hipStreamCaptureModeThreadLocal));
-                bicg_iteration<T>(t, tol, maxit);
+                krylov_iteration<T>(t, tol, maxit, meth);
                 EQ_HIP(hipStreamEndCapture(S(), &graph));
                 EQ_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
                 g = t.graphs.emplace(key, std::make_pair(graph, exec)).first;
@@ -1695,7 +1812,7 @@ struct HeatBatch::Impl {
         } else {
             if (count_active(t, 0)) {
                 for (int it = 0; it < maxit; ++it) {
-                    bicg_iteration<T>(t, tol, maxit);
+                    krylov_iteration<T>(t, tol, maxit, meth);
                     if (!count_active(t, 0, &iters)) break;
                 }
             }
@@ -2074,11 +2191,15 @@ double HeatBatch::time_kernel(const std::string& name, int reps, double* event_m
     bool red = false, full = false, ilu = false;
     for (Inst* I : all) red |= I->pcm == PM_SCHUR, full |= I->pcm != PM_SCHUR, ilu |= I->pcm == PM_ILU;
     if (name == "operator_red" && red) {
-        if (P.mixed) TLP(C_SPMV, (k_schur_red<float, CP>), (int)V_PH);
-        else TLP(C_SPMV, (k_schur_red<double, CP>), (int)V_PH);
+        if (P.mixed) TLP(C_SPMV, (k_schur_red<float, CP>), (int)V_PH, (int)(M_BICG | M_CG));
+        else TLP(C_SPMV, (k_schur_red<double, CP>), (int)V_PH, (int)(M_BICG | M_CG));
     } else if (name == "operator_black" && red) {
-        if (P.mixed) TLP(C_SPMV, (k_schur_black<float, CP>), (int)V_PH, (int)V_V, 1, (int)V_RH, -1, t.part->p, t.ng);
-        else TLP(C_SPMV, (k_schur_black<double, CP>), (int)V_PH, (int)V_V, 1, (int)V_RH, -1, t.part->p, t.ng);
+        if (P.mixed)
+            TLP(C_SPMV, (k_schur_black<float, CP>), (int)V_PH, (int)V_V, 1, (int)V_RH, -1, t.part->p, t.ng,
+                (int)(M_BICG | M_CG));
+        else
+            TLP(C_SPMV, (k_schur_black<double, CP>), (int)V_PH, (int)V_V, 1, (int)V_RH, -1, t.part->p, t.ng,
+                (int)(M_BICG | M_CG));
     } else if (name == "ilu_black" && ilu) {
         if (P.mixed) TLP(C_SPMV, (k_ilu_black<float, CP>), (int)V_P, (int)V_PH);
         else TLP(C_SPMV, (k_ilu_black<double, CP>), (int)V_P, (int)V_PH);

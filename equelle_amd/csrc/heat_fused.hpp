@@ -53,6 +53,9 @@ struct FusedConfig {
     // four consecutive pair ranks of one row (16-byte loads) instead of ranks strided by the
     // block size.  Other instances always take the strided form.
     bool pair_vectors = true;
+    // Two-phase instances in the reduced form solve the pressure stage (a symmetric
+    // M-matrix) by preconditioned CG instead of BiCGStab.
+    bool twophase_pressure_cg = true;
     int max_refine = 20;
     bool graph = true;                 // capture BiCGStab iterations in a hipGraph
     int cpt = 0;                       // cells per thread (1, 2, 4, 8); 0 = chosen by the tuner

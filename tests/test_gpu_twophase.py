@@ -211,7 +211,8 @@ def test_fused_twophase_residual_matches_oracle(hip_api, oracle_api, dims, stage
 # 1e-7; FP32 with ILU(0) runs at the benchmark's tolerances (Newton 1e-8, linear 1e-6),
 # compared at 1e-5.
 @pytest.mark.parametrize("params,newton_tol,solver_tol,bar", [
-    ("precision=fp64 preconditioner=ilu0", 1e-10, 1e-10, 1e-7),   # reduced form (auto in FP64)
+    ("precision=fp64 preconditioner=ilu0", 1e-10, 1e-10, 1e-7),   # reduced form (auto in FP64), pressure by CG
+    ("precision=fp64 preconditioner=ilu0 twophase_pressure_cg=0", 1e-10, 1e-10, 1e-7),   # pressure by BiCGStab
     ("precision=fp64 preconditioner=ilu0 krylov_form_twophase=full", 1e-10, 1e-10, 1e-7),
     ("precision=fp64 preconditioner=diagonal", 1e-10, 1e-10, 1e-7),
     ("precision=mixed preconditioner=diagonal", 1e-10, 1e-10, 1e-7),
