@@ -675,6 +675,7 @@ static FusedConfig fused_config(const char* params) {
     c.krylov_form_twophase = m.get_string("krylov_form_twophase", c.krylov_form_twophase);
     c.pair_vectors = m.get_int("pair_vectors", c.pair_vectors ? 1 : 0) != 0;
     c.twophase_pressure_cg = m.get_int("twophase_pressure_cg", c.twophase_pressure_cg ? 1 : 0) != 0;
+    c.twophase_pressure_mg = m.get_int("twophase_pressure_mg", c.twophase_pressure_mg ? 1 : 0) != 0;
     c.max_refine = m.get_int("max_refine", c.max_refine);
     c.graph = m.get_int("graph", 1) != 0;
     c.cpt = m.get_int("cpt", c.cpt);

@@ -23,9 +23,10 @@ inline hipStream_t S() { return Device::get().stream(); }
 // S_MEAN: the mean of the Krylov right-hand side over the cells (singular systems only)
 enum Slot { S_RHO = 0, S_RHO_OLD, S_ALPHA, S_OMEGA, S_RR, S_BB, S_BREAK, S_IT, S_FN, S_RN, S_BN, S_REF, S_ITT, S_TOL2,
             S_MEAN, S_BETA, NSLOT = 16 };
-// Krylov vectors; V_Y is the red-rank scratch of the reduced operator (ilu0 only)
-enum Vec { V_R = 0, V_RH, V_P, V_V, V_S, V_T, V_PH, V_SH, V_X, V_Y, NVEC };
-enum Act { A_INIT = 0, A_ALPHA, A_OMEGA, A_RHO, A_STORE, A_MEAN, A_CG_ALPHA, A_CG_RHO };
+// Krylov vectors; V_Y is the red-rank scratch of the reduced operator (ilu0 only), V_Z the
+// V-cycle's z = M^-1 r (two-phase pressure by CG with multigrid)
+enum Vec { V_R = 0, V_RH, V_P, V_V, V_S, V_T, V_PH, V_SH, V_X, V_Y, V_Z, NVEC };
+enum Act { A_INIT = 0, A_ALPHA, A_OMEGA, A_RHO, A_STORE, A_MEAN, A_CG_ALPHA, A_CG_RHO, A_CG_START };
 // Krylov methods of a solve (bit mask): BiCGStab, and CG for symmetric systems (the
 // two-phase pressure stage in the reduced form)
 enum Meth { M_BICG = 1, M_CG = 2 };
@@ -49,6 +50,9 @@ struct Desc {
     int ngrpk;                    // Krylov partials: groups [gbase, gbase + ngrpk) of BS pairs
     int vec;                      // Krylov kernels in the four-ranks-per-lane form (2D heat, nx % 8 == 0)
     int cgp;                      // two-phase, reduced form: the pressure stage (symmetric) is solved by CG
+    int mgp;                      // two-phase, full form: pressure by CG with the multigrid V-cycle
+    int mgl;                      // multigrid: levels below this one (0 on the coarsest level)
+    void* minv;                   // multigrid, coarsest level: dense (A + s 1 1^T)^-1, cell order
     double rnx, rnxy;             // 1 / nx, 1 / (nx ny): cell -> (i, j, k) without integer division
     double dt;
     // heat: k(u) = k (1 + u^2), x / y face geometry factors, cell volume
@@ -73,10 +77,11 @@ struct Desc {
 };
 
 // The Krylov method of an instance's current solve: CG for the two-phase pressure stage
-// (a symmetric M-matrix: both cells of a face see the same upwinded mobility) in the
-// reduced form, BiCGStab otherwise.  Kernels and finaliser actions of one method skip the
+// (a symmetric M-matrix: both cells of a face see the same upwinded mobility), in the
+// reduced form with the Schur complement's diagonal or in the full form with the
+// multigrid V-cycle, BiCGStab otherwise.  Kernels and finaliser actions of one method skip the
 // instances of the other, so both can run in one batch.
-__device__ inline int meth_of(const Desc& d) { return d.cgp && d.act[2] == 0 ? M_CG : M_BICG; }
+__device__ inline int meth_of(const Desc& d) { return (d.cgp || d.mgp) && d.act[2] == 0 ? M_CG : M_BICG; }
 
 // What the last block of an instance does with the summed partials.
 struct Fin {
@@ -280,6 +285,15 @@ __device__ void fin_body(const Desc& d, const double* part, int nb, const Fin& f
             d.act[0] = ok && res[1] > (tol2 < 0.0 ? sc[S_TOL2] : tol2) * sc[S_BB] && sc[S_IT] < maxit && res[0] != 0.0;
             break;
         }
+        case A_CG_START:   // V-cycle instances: rho = r.z of the first z, first direction p = z
+            if (!d.act[0] || meth_of(d) != M_CG || !d.mgp) break;
+            sc[S_RHO] = res[0];
+            sc[S_BETA] = 0.0;
+            if (!(res[0] > 0.0) || !isfinite(res[0])) {
+                sc[S_BREAK] = 1.0;
+                d.act[0] = 0;
+            }
+            break;
         case A_MEAN:   // Krylov right-hand side sum -> its mean over the cells (singular systems)
             sc[S_MEAN] = res[0] / (double)(d.ns == 1 ? d.n / 2 : d.n);   // black cells or all cells
             break;
@@ -793,9 +807,11 @@ __global__ __launch_bounds__(BS) void k_schur_black(const Desc* __restrict__ D, 
 // Full operator (jacobi / none): y = A x over both slots of each pair, fused dots as above.
 template <class T, int CPT>
 __global__ __launch_bounds__(BS) void k_spmv_rb(const Desc* __restrict__ D, const int2* __restrict__ blk, int ix,
-                                                int iy, int nd, int a0, int a1, double* __restrict__ part, int ng) {
+                                                int iy, int nd, int a0, int a1, double* __restrict__ part, int ng,
+                                                int meth) {
     LOCATE();
-    if (d.ns != 2) return;   // block-uniform: the reduced-system kernel writes these partials
+    // block-uniform: the reduced-system kernel, or the other method's launch, writes these partials
+    if (d.ns != 2 || !(meth & meth_of(d))) return;
     double s0[CPT] = {}, s1[CPT] = {};
     if (d.act[0]) {
         const int h = d.h;
@@ -869,7 +885,7 @@ template <class T, int CPT>
 __global__ __launch_bounds__(BS) void k_ilu_black(const Desc* __restrict__ D, const int2* __restrict__ blk, int ir,
                                                   int iz) {
     LOCATE();
-    if (d.pcm != PM_ILU || !d.act[0]) return;
+    if (d.pcm != PM_ILU || !d.act[0] || meth_of(d) != M_BICG) return;
     const int h = d.h, nx = d.nx, ny = d.ny, nxy = d.nx * d.ny;
     const T* __restrict__ r = (const T*)d.v[ir];
     const T* __restrict__ dv = (const T*)d.dinv;
@@ -899,7 +915,7 @@ template <class T, int CPT>
 __global__ __launch_bounds__(BS) void k_ilu_red(const Desc* __restrict__ D, const int2* __restrict__ blk, int ir,
                                                 int iz) {
     LOCATE();
-    if (d.pcm != PM_ILU || !d.act[0]) return;
+    if (d.pcm != PM_ILU || !d.act[0] || meth_of(d) != M_BICG) return;
     const T* __restrict__ r = (const T*)d.v[ir];
     const T* __restrict__ dv = (const T*)d.dinv;
     T* __restrict__ z = (T*)d.v[iz];
@@ -941,31 +957,35 @@ __device__ inline const T* slot_dinv(const Desc& d) {
 }
 
 // Krylov start: r = b (already in r; minus its mean for singular systems), x = 0.
-// BiCGStab: rh = r, p = v = 0; partial dots (rh.r, r.r).  CG: p = z = M^-1 r; partial
-// dots (r.z, r.r).
+// BiCGStab: rh = r, p = v = 0; partial dots (rh.r, r.r).  CG with the diagonal: p = z =
+// M^-1 r; partial dots (r.z, r.r).  CG with the V-cycle: p = 0 (z and r.z follow the
+// first V-cycle); partial dots (0, r.r).
 template <class T, int CPT>
 __global__ __launch_bounds__(BS) void k_kinit(const Desc* __restrict__ D, const int2* __restrict__ blk,
                                               double* __restrict__ part, int ng) {
     LOCATE();
     double rr[CPT] = {}, rz[CPT] = {};
     const bool cg = meth_of(d) == M_CG;
-    if (d.act[1] && cg) {   // reduced form: one slot per pair
+    if (d.act[1] && cg) {
         const bool proj = singular(d);
         const T mean = proj ? (T)d.sc[S_MEAN] : (T)0;
         const T* __restrict__ dv = slot_dinv<T>(d);
         FOR_GROUPS(m, p) {
             if (p >= d.h) continue;
-            T rc = ((T*)d.v[V_R])[p];
-            if (proj) {
-                int i, j, k;
-                if (cell_of(d, p, 1, i, j, k) >= 0) rc -= mean;
-                ((T*)d.v[V_R])[p] = rc;
+            for (int w = 0; w < d.ns; ++w) {
+                const int s = p + w * d.h;
+                T rc = ((T*)d.v[V_R])[s];
+                if (proj) {
+                    int i, j, k;
+                    if (cell_of(d, p, d.ns == 1 ? 1 : w, i, j, k) >= 0) rc -= mean;
+                    ((T*)d.v[V_R])[s] = rc;
+                }
+                const T z = d.mgp ? (T)0 : dv[s] * rc;
+                ((T*)d.v[V_P])[s] = z;
+                ((T*)d.v[V_X])[s] = 0;
+                rz[m] += (double)rc * (double)z;
+                rr[m] += (double)rc * (double)rc;
             }
-            const T z = dv[p] * rc;
-            ((T*)d.v[V_P])[p] = z;
-            ((T*)d.v[V_X])[p] = 0;
-            rz[m] = (double)rc * (double)z;
-            rr[m] = (double)rc * (double)rc;
         }
     } else if (d.act[1]) {
         const bool proj = singular(d);
@@ -992,8 +1012,8 @@ __global__ __launch_bounds__(BS) void k_kinit(const Desc* __restrict__ D, const 
     group_store<CPT>(rr, PARTSK(part + ng));
 }
 
-// CG (reduced form, one slot per pair; q = S p in V_V): x += alpha p, r -= alpha q;
-// partial dots (r.z, r.r) with z = M^-1 r.
+// CG (q = A p in V_V): x += alpha p, r -= alpha q; with the diagonal, partial dots (r.z,
+// r.r) with z = M^-1 r (with the V-cycle k_cg_dots writes them once z is formed).
 template <class T, int CPT>
 __global__ __launch_bounds__(BS) void k_cg2(const Desc* __restrict__ D, const int2* __restrict__ blk,
                                             double* __restrict__ part, int ng) {
@@ -1009,26 +1029,60 @@ __global__ __launch_bounds__(BS) void k_cg2(const Desc* __restrict__ D, const in
         const T* __restrict__ dv = slot_dinv<T>(d);
         FOR_GROUPS(m, p) {
             if (p >= d.h) continue;
-            x[p] += alpha * pp[p];
-            const T rn = r[p] - alpha * q[p];
-            r[p] = rn;
-            a0[m] = (double)rn * (double)(dv[p] * rn);
-            a1[m] = (double)rn * (double)rn;
+            for (int w = 0; w < d.ns; ++w) {
+                const int s = p + w * d.h;
+                x[s] += alpha * pp[s];
+                const T rn = r[s] - alpha * q[s];
+                r[s] = rn;
+                if (!d.mgp) {
+                    a0[m] += (double)rn * (double)(dv[s] * rn);
+                    a1[m] += (double)rn * (double)rn;
+                }
+            }
         }
     }
     group_store<CPT>(a0, PARTSK(part));
     group_store<CPT>(a1, PARTSK(part + ng));
 }
-// CG direction: p = M^-1 r + beta p
+// CG with the V-cycle: partial dots (r.z, r.r), z in V_Z.
 template <class T, int CPT>
-__global__ __launch_bounds__(BS) void k_cg3(const Desc* __restrict__ D, const int2* __restrict__ blk) {
+__global__ __launch_bounds__(BS) void k_cg_dots(const Desc* __restrict__ D, const int2* __restrict__ blk,
+                                                double* __restrict__ part, int ng) {
     LOCATE();
-    if (!d.act[0] || meth_of(d) != M_CG) return;
+    if (!d.mgp || meth_of(d) != M_CG) return;   // block-uniform; k_cg2 wrote the others' partials
+    double a0[CPT] = {}, a1[CPT] = {};
+    if (d.act[0]) {
+        const T* __restrict__ r = (const T*)d.v[V_R];
+        const T* __restrict__ z = (const T*)d.v[V_Z];
+        FOR_GROUPS(m, p) {
+            if (p >= d.h) continue;
+            for (int w = 0; w < d.ns; ++w) {
+                const int s = p + w * d.h;
+                a0[m] += (double)r[s] * (double)z[s];
+                a1[m] += (double)r[s] * (double)r[s];
+            }
+        }
+    }
+    group_store<CPT>(a0, PARTSK(part));
+    group_store<CPT>(a1, PARTSK(part + ng));
+}
+// CG direction: p = z + beta p, z = M^-1 r (the diagonal, or the V-cycle's z in V_Z);
+// only_mg: the V-cycle instances only (their first direction, beta = 0).
+template <class T, int CPT>
+__global__ __launch_bounds__(BS) void k_cg3(const Desc* __restrict__ D, const int2* __restrict__ blk, int only_mg) {
+    LOCATE();
+    if (!d.act[0] || meth_of(d) != M_CG || (only_mg && !d.mgp)) return;
     const T beta = (T)d.sc[S_BETA];
     T* __restrict__ pp = (T*)d.v[V_P];
     const T* __restrict__ r = (const T*)d.v[V_R];
+    const T* __restrict__ z = (const T*)d.v[V_Z];
     const T* __restrict__ dv = slot_dinv<T>(d);
-    FOR_PAIRS(p) pp[p] = dv[p] * r[p] + beta * pp[p];
+    FOR_PAIRS(p) {
+        for (int w = 0; w < d.ns; ++w) {
+            const int s = p + w * d.h;
+            pp[s] = (d.mgp ? z[s] : dv[s] * r[s]) + beta * pp[s];
+        }
+    }
 }
 
 // p = r + beta (p - omega v) (first iteration: p = r); ph = M^-1 p (diagonal M; PM_ILU:
@@ -1153,6 +1207,228 @@ __global__ __launch_bounds__(BS) void k_b3(const Desc* __restrict__ D, const int
     }
     group_store<CPT>(a0, PARTSK(part));
     group_store<CPT>(a1, PARTSK(part + ng));
+}
+
+
+// ---------------------------------------------------------------------------
+// Aggregation multigrid for the two-phase pressure stage (Desc::mgp: full red-black
+// system, preconditioned CG).  Level l + 1 aggregates the 2 x 2 x 2 cells of level l:
+// cell (i, j, k) belongs to aggregate (i/2, j/2, k/2), so a coarse grid has
+// ceil(n/2) cells per axis.  Prolongation P is piecewise constant and the coarse
+// operator is the Galerkin product A_{l+1} = P^T A_l P, again a 7-point stencil: two
+// neighbouring aggregates are coupled by the sum of the fine couplings across their
+// interface, and the diagonal is the sum of the aggregate's fine block, so zero row
+// sums (the pressure null space) carry over.  Every level is stored like the fine one:
+// red-black rank order, coefficients K[q], right-hand side in V_R, correction in V_Z.
+// One V-cycle computes z = M r from zero.  On each level except the coarsest: a red,
+// then a black Gauss-Seidel half sweep, and the residual summed over each aggregate as
+// the next level's right-hand side.  On the coarsest level (at most MG_COARSE cells):
+// z = (A + s 1 1^T)^-1 b with a dense inverse formed at set-up; s 1 1^T (s = mean
+// diagonal / n) removes the constant null space of the singular pressure operator, and
+// for b with zero sum the result is the solution with zero sum.  Going up: z += P
+// z_coarse, then a black and a red half sweep.  Pre- and post-smoothing are adjoint and
+// the coarse solve is symmetric, so M is symmetric positive definite and CG applies.
+// Desc::mgl counts the levels below a level (0 on the coarsest); the kernels of one
+// level skip the instances whose coarsest level it is.
+constexpr int MG_COARSE = 64;
+constexpr int MG_BS = 256;   // threads of the dense coarsest-level kernels
+
+__device__ inline bool mg_on(const Desc& d) { return d.mgp && d.act[0] && meth_of(d) == M_CG; }
+// red-black slot of cell c = (i, j, k) of a level
+__device__ inline int rb_slot(const Desc& d, int c, int i, int j, int k) { return ((i + j + k) & 1) * d.h + (c >> 1); }
+
+// Gauss-Seidel half sweep over the cells of colour col of a level: x = (b - offdiag x) / C
+// (zero: x is zero before the sweep, so the neighbours are not read).  Slots without a
+// cell get 0 (they stand in the dot products of the fine level).
+template <class T, int CPT>
+__global__ __launch_bounds__(BS) void k_mg_smooth(const Desc* __restrict__ D, const int2* __restrict__ blk, int col,
+                                                  int zero) {
+    LOCATE();
+    if (!mg_on(d) || d.mgl == 0) return;
+    const int h = d.h;
+    const T* __restrict__ b = (const T*)d.v[V_R] + col * h;
+    const T* __restrict__ C = (const T*)d.K[0] + col * h;
+    T* __restrict__ x = (T*)d.v[V_Z];
+    FOR_PAIRS(p) {
+        int i, j, k;
+        const int c = cell_of(d, p, col, i, j, k);
+        const int s = col * h + p;
+        if (c < 0) {
+            x[s] = (T)0;
+            continue;
+        }
+        const T off = zero ? (T)0 : offdiag<T>(d, s, c, i, j, k, x + (1 - col) * h);
+        x[s] = (b[p] - off) / C[p];
+    }
+}
+
+// Fine level F and coarse level Cd of one instance (thread per cell or pair of a level).
+#define MG_LOCATE(N)                                                                               \
+    const int2 bi = blk[blockIdx.x];                                                               \
+    const Desc& f = F[bi.x];                                                                       \
+    const Desc& c = Cd[bi.x];                                                                      \
+    const int t_ = bi.y * BS + (int)threadIdx.x;                                                   \
+    if (t_ >= (N)) return
+
+// Galerkin coarse operator (thread per coarse cell), before the solve of a pressure
+// Newton iteration (refinement flag set).
+template <class T>
+__global__ __launch_bounds__(BS) void k_mg_galerkin(const Desc* __restrict__ F, const Desc* __restrict__ Cd,
+                                                    const int2* __restrict__ blk) {
+    MG_LOCATE(c.n);
+    if (!c.mgp || !c.act[1] || meth_of(c) != M_CG) return;
+    int ic, jc, kc;
+    cell_ijk(c, t_, ic, jc, kc);
+    T co[7] = {0, 0, 0, 0, 0, 0, 0};
+    const int nxf = f.nx, nyf = f.ny, nzf = f.nz;
+    for (int k = 2 * kc; k < min(2 * kc + 2, nzf); ++k)
+        for (int j = 2 * jc; j < min(2 * jc + 2, nyf); ++j)
+            for (int i = 2 * ic; i < min(2 * ic + 2, nxf); ++i) {
+                const int fc = i + nxf * (j + nyf * k);
+                const int s = rb_slot(f, fc, i, j, k);
+                co[0] += ((const T*)f.K[0])[s];
+                // a coupling to a cell of the same aggregate goes to the diagonal, otherwise
+                // to the direction of the neighbouring aggregate
+                if (i > 0) co[(i & 1) ? 0 : 1] += ((const T*)f.K[1])[s];
+                if (i < nxf - 1) co[(i & 1) ? 2 : 0] += ((const T*)f.K[2])[s];
+                if (j > 0) co[(j & 1) ? 0 : 3] += ((const T*)f.K[3])[s];
+                if (j < nyf - 1) co[(j & 1) ? 4 : 0] += ((const T*)f.K[4])[s];
+                if (nzf > 1) {
+                    if (k > 0) co[(k & 1) ? 0 : 5] += ((const T*)f.K[5])[s];
+                    if (k < nzf - 1) co[(k & 1) ? 6 : 0] += ((const T*)f.K[6])[s];
+                }
+            }
+    const int sc = rb_slot(c, t_, ic, jc, kc);
+    for (int q = 0; q < 7; ++q) ((T*)c.K[q])[sc] = co[q];
+}
+
+// Restriction (thread per coarse cell): b_c = sum over the aggregate of (b - A z) on the
+// fine level.
+template <class T>
+__global__ __launch_bounds__(BS) void k_mg_restrict(const Desc* __restrict__ F, const Desc* __restrict__ Cd,
+                                                    const int2* __restrict__ blk) {
+    MG_LOCATE(c.n);
+    if (!mg_on(c)) return;
+    int ic, jc, kc;
+    cell_ijk(c, t_, ic, jc, kc);
+    const T* __restrict__ b = (const T*)f.v[V_R];
+    const T* __restrict__ x = (const T*)f.v[V_Z];
+    const int h = f.h;
+    T acc = (T)0;
+    for (int k = 2 * kc; k < min(2 * kc + 2, f.nz); ++k)
+        for (int j = 2 * jc; j < min(2 * jc + 2, f.ny); ++j)
+            for (int i = 2 * ic; i < min(2 * ic + 2, f.nx); ++i) {
+                const int fc = i + f.nx * (j + f.ny * k);
+                const int col = (i + j + k) & 1, s = col * h + (fc >> 1);
+                acc += b[s] - (((const T*)f.K[0])[s] * x[s] + offdiag<T>(f, s, fc, i, j, k, x + (1 - col) * h));
+            }
+    ((T*)c.v[V_R])[rb_slot(c, t_, ic, jc, kc)] = acc;
+}
+
+// Prolongation (thread per fine pair): z_f += z_c of the cell's aggregate.
+template <class T>
+__global__ __launch_bounds__(BS) void k_mg_prolong(const Desc* __restrict__ F, const Desc* __restrict__ Cd,
+                                                   const int2* __restrict__ blk) {
+    MG_LOCATE(f.h);
+    if (!mg_on(c)) return;
+    T* __restrict__ xf = (T*)f.v[V_Z];
+    const T* __restrict__ xc = (const T*)c.v[V_Z];
+    for (int col = 0; col < 2; ++col) {
+        int i, j, k;
+        if (cell_of(f, t_, col, i, j, k) < 0) continue;
+        const int ic = i >> 1, jc = j >> 1, kc = k >> 1;
+        const int cc = ic + c.nx * (jc + c.ny * kc);
+        xf[col * f.h + t_] += xc[rb_slot(c, cc, ic, jc, kc)];
+    }
+}
+#undef MG_LOCATE
+
+// Coarsest level set-up (one block per instance): the dense inverse of A + s 1 1^T by
+// Gauss-Jordan elimination in FP64 (symmetric positive definite: no pivoting), stored in
+// the Krylov precision.
+template <class T>
+__global__ __launch_bounds__(MG_BS) void k_mg_dense_setup(const Desc* __restrict__ Cd) {
+    const Desc& c = Cd[blockIdx.x];
+    if (!c.mgp || !c.act[1] || meth_of(c) != M_CG) return;
+    __shared__ double A[MG_COARSE][MG_COARSE];
+    __shared__ double col[MG_COARSE];
+    __shared__ double sdiag;
+    const int n = c.n, tid = threadIdx.x;
+    // diagonal mean -> s
+    if (tid == 0) {
+        double sum = 0.0;
+        for (int q = 0; q < n; ++q) {
+            int i, j, k;
+            cell_ijk(c, q, i, j, k);
+            sum += (double)((const T*)c.K[0])[rb_slot(c, q, i, j, k)];
+        }
+        sdiag = sum / ((double)n * (double)n);
+    }
+    __syncthreads();
+    const double sh = sdiag;
+    for (int e = tid; e < n * n; e += MG_BS) A[e / n][e % n] = sh;
+    __syncthreads();
+    for (int q = tid; q < n; q += MG_BS) {
+        int i, j, k;
+        cell_ijk(c, q, i, j, k);
+        const int s = rb_slot(c, q, i, j, k);
+        A[q][q] += (double)((const T*)c.K[0])[s];
+        const int nxy = c.nx * c.ny;
+        if (i > 0) A[q][q - 1] += (double)((const T*)c.K[1])[s];
+        if (i < c.nx - 1) A[q][q + 1] += (double)((const T*)c.K[2])[s];
+        if (j > 0) A[q][q - c.nx] += (double)((const T*)c.K[3])[s];
+        if (j < c.ny - 1) A[q][q + c.nx] += (double)((const T*)c.K[4])[s];
+        if (c.nz > 1) {
+            if (k > 0) A[q][q - nxy] += (double)((const T*)c.K[5])[s];
+            if (k < c.nz - 1) A[q][q + nxy] += (double)((const T*)c.K[6])[s];
+        }
+    }
+    __syncthreads();
+    // in-place Gauss-Jordan inversion
+    for (int m = 0; m < n; ++m) {
+        const double piv = 1.0 / A[m][m];
+        for (int i = tid; i < n; i += MG_BS) col[i] = A[i][m];
+        __syncthreads();
+        for (int j = tid; j < n; j += MG_BS) A[m][j] = (j == m ? 1.0 : A[m][j]) * piv;
+        __syncthreads();
+        for (int e = tid; e < n * n; e += MG_BS) {
+            const int i = e / n, j = e % n;
+            if (i != m) A[i][j] = (j == m ? 0.0 : A[i][j]) - col[i] * A[m][j];
+        }
+        __syncthreads();
+    }
+    T* __restrict__ inv = (T*)c.minv;
+    for (int e = tid; e < n * n; e += MG_BS) inv[e] = (T)A[e / n][e % n];
+}
+
+// Coarsest level solve (one block per instance): z = inverse * b.
+template <class T>
+__global__ __launch_bounds__(MG_BS) void k_mg_dense_solve(const Desc* __restrict__ Cd) {
+    const Desc& c = Cd[blockIdx.x];
+    if (!mg_on(c)) return;
+    __shared__ double bb[MG_COARSE];
+    const int n = c.n, tid = threadIdx.x;
+    const T* __restrict__ b = (const T*)c.v[V_R];
+    T* __restrict__ x = (T*)c.v[V_Z];
+    for (int q = tid; q < n; q += MG_BS) {
+        int i, j, k;
+        cell_ijk(c, q, i, j, k);
+        bb[q] = (double)b[rb_slot(c, q, i, j, k)];
+    }
+    __syncthreads();
+    const T* __restrict__ inv = (const T*)c.minv;
+    for (int q = tid; q < n; q += MG_BS) {
+        double acc = 0.0;
+        for (int e = 0; e < n; ++e) acc += (double)inv[q * n + e] * bb[e];
+        int i, j, k;
+        cell_ijk(c, q, i, j, k);
+        x[rb_slot(c, q, i, j, k)] = (T)acc;
+    }
+    if ((n & 1) && tid == 0) {   // the last pair holds one cell; the other colour's slot stays 0
+        int i, j, k;
+        cell_ijk(c, n - 1, i, j, k);
+        x[(1 - ((i + j + k) & 1)) * c.h + (n >> 1)] = (T)0;
+    }
 }
 
 // out[0] = number of instances with flag `which` set, out[1] = the largest Krylov
@@ -1329,6 +1605,17 @@ struct Inst {
     enum State { IDLE, START, SOLVE, DONE } state = IDLE;
     StepStats st;
     int pcm = PM_SCHUR;            // Krylov form / preconditioner
+    // multigrid hierarchy (two-phase pressure by CG with the V-cycle): coarse levels 1, 2,
+    // ... (coefficients 7 x 2 h, vectors V_R and V_Z 2 h each, Krylov precision) and the
+    // coarsest level's dense inverse (the fine level's when it is already small)
+    struct Level {
+        int nx = 1, ny = 1, nz = 1, n = 0;
+        DevPtr<char> K, v;
+        int h() const { return (n + 1) / 2; }
+    };
+    bool mg_on = false;
+    std::vector<Level> mg;
+    DevPtr<char> minv;
     int ncoef() const { return nz > 1 ? 7 : 5; }
     int h() const { return (n + 1) / 2; }
     int ns() const { return pcm == PM_SCHUR ? 1 : 2; }
@@ -1370,6 +1657,18 @@ struct Table {
     int ng = 0;              // partial groups over all instances (an instance has ceil(n / BS))
     bool has_schur = false, has_full = false, has_ilu = false;   // Krylov forms present
     bool has_tp = false;                                         // two-phase instances present
+    // Multigrid plan over the instances with a hierarchy; index l = 1 .. levels - 1: the
+    // instances that have level l, their level l - 1 (fine) and level l (coarse)
+    // descriptors, blocks over the coarse cells, the fine pairs and the coarse pairs.
+    struct Mg {
+        int levels = 0;
+        std::vector<DevPtr<Desc>> fine, coarse;
+        std::vector<DevPtr<int2>> cblk, fpblk, cpblk;
+        std::vector<int> ncb, nfpb, ncpb;
+        DevPtr<Desc> coarsest;   // each instance's coarsest level
+        int ncoarsest = 0;
+    };
+    std::unique_ptr<Mg> mg;
     Shape shape;
     DevPtr<double> part;
     uint64_t version = 0;
@@ -1407,9 +1706,79 @@ struct HeatBatch::Impl {
 
     size_t ks() const { return mixed ? sizeof(float) : sizeof(double); }
 
-    // two-phase instances in the reduced form solve their pressure stage by CG
+    // two-phase instances in the reduced form solve their pressure stage by CG with the
+    // Schur complement's diagonal (those in the full form with the V-cycle: Inst::mg_on)
     bool cg_capable(const Inst& I) const {
         return cfg.twophase_pressure_cg && I.kind == K_TWOPHASE && I.pcm == PM_SCHUR;
+    }
+    // Level l >= 1 of an instance's hierarchy (the fields the multigrid kernels read).
+    Desc level_desc(Inst& I, int l) const {
+        Desc d = desc_of(I);
+        const Inst::Level& L = I.mg[l - 1];
+        d.n = L.n;
+        d.nx = L.nx;
+        d.ny = L.ny;
+        d.nz = L.nz;
+        d.rnx = 1.0 / L.nx;
+        d.rnxy = 1.0 / ((double)L.nx * L.ny);
+        d.xcd = 0;
+        const size_t h = L.h();
+        d.h = (int)h;
+        d.ns = 2;
+        d.vec = 0;
+        for (int q = 0; q < 7; ++q) d.K[q] = L.K->p + q * 2 * h * ks();
+        for (int v = 0; v < NVEC; ++v) d.v[v] = nullptr;
+        d.v[V_R] = L.v->p;
+        d.v[V_Z] = L.v->p + 2 * h * ks();
+        d.mgl = (int)I.mg.size() - l;
+        d.minv = d.mgl == 0 ? (void*)I.minv->p : nullptr;
+        return d;
+    }
+    void build_mg(Table& t) {
+        std::vector<Inst*> mi;
+        for (Inst* I : t.inst)
+            if (I->mg_on) mi.push_back(I);
+        if (mi.empty()) return;
+        auto m = std::make_unique<Table::Mg>();
+        for (Inst* I : mi) m->levels = std::max(m->levels, (int)I->mg.size() + 1);
+        m->fine.resize(m->levels);
+        m->coarse.resize(m->levels);
+        m->cblk.resize(m->levels);
+        m->fpblk.resize(m->levels);
+        m->cpblk.resize(m->levels);
+        m->ncb.assign(m->levels, 0);
+        m->nfpb.assign(m->levels, 0);
+        m->ncpb.assign(m->levels, 0);
+        auto blocks = [](std::vector<int2>& b, int q, int count) {
+            for (int k = 0; k < (count + BS - 1) / BS; ++k) b.push_back(make_int2(q, k));
+        };
+        for (int l = 1; l < m->levels; ++l) {
+            std::vector<Desc> fd, cd;
+            std::vector<int2> cb, fpb, cpb;
+            for (Inst* I : mi) {
+                if ((int)I->mg.size() < l) continue;
+                const int q = (int)fd.size();
+                fd.push_back(l == 1 ? desc_of(*I) : level_desc(*I, l - 1));
+                fd.back().xcd = 0;
+                cd.push_back(level_desc(*I, l));
+                blocks(cb, q, cd.back().n);
+                blocks(fpb, q, fd.back().h);
+                blocks(cpb, q, cd.back().h);
+            }
+            m->fine[l] = dev_from_host(fd.data(), fd.size());
+            m->coarse[l] = dev_from_host(cd.data(), cd.size());
+            m->ncb[l] = (int)cb.size();
+            m->nfpb[l] = (int)fpb.size();
+            m->ncpb[l] = (int)cpb.size();
+            m->cblk[l] = dev_from_host(cb.data(), cb.size());
+            m->fpblk[l] = dev_from_host(fpb.data(), fpb.size());
+            m->cpblk[l] = dev_from_host(cpb.data(), cpb.size());
+        }
+        std::vector<Desc> cs;
+        for (Inst* I : mi) cs.push_back(I->mg.empty() ? desc_of(*I) : level_desc(*I, (int)I->mg.size()));
+        m->coarsest = dev_from_host(cs.data(), cs.size());
+        m->ncoarsest = (int)cs.size();
+        t.mg = std::move(m);
     }
     Desc desc_of(Inst& I) const {
         Desc d{};
@@ -1460,6 +1829,9 @@ struct HeatBatch::Impl {
         d.ns = I.ns();
         d.vec = cfg.pair_vectors && I.kind == K_HEAT && I.pcm == PM_SCHUR && I.nx % 8 == 0;
         d.cgp = cg_capable(I);
+        d.mgp = I.mg_on;
+        d.mgl = (int)I.mg.size();
+        d.minv = I.mg_on && I.mg.empty() ? (void*)I.minv->p : nullptr;
         for (int q = 0; q < 7; ++q) d.K[q] = q < I.ncoef() ? I.kcoef->p + q * 2 * h * ks() : nullptr;
         d.dinv = I.dinv->p;
         for (int v = 0; v < NVEC; ++v) d.v[v] = I.vecs->p + (size_t)v * d.ns * h * ks();
@@ -1505,6 +1877,7 @@ struct HeatBatch::Impl {
         }
         t.desc = dev_from_host(ds.data(), ds.size());
         t.part = dev_alloc<double>(2 * (size_t)t.ng + 2);
+        build_mg(t);
         return tp;
     }
     TableP table(const std::vector<Inst*>& list) {
@@ -1704,7 +2077,51 @@ struct HeatBatch::Impl {
             HLP(C_SPMV, (k_schur_red<T, CP>), iv, meth);
             HLP(C_SPMV, (k_schur_black<T, CP>), iv, iw, nd, a0, a1, t.part->p, t.ng, meth);
         }
-        if (t.has_full) HLP(C_SPMV, (k_spmv_rb<T, CP>), iv, iw, nd, a0, a1, t.part->p, t.ng);
+        if (t.has_full) HLP(C_SPMV, (k_spmv_rb<T, CP>), iv, iw, nd, a0, a1, t.part->p, t.ng, meth);
+    }
+    // Multigrid: coarse operators and coarsest inverses of the instances about to solve a
+    // pressure Newton iteration by CG with the V-cycle.
+    template <class T>
+    void mg_setup(const Table& t) {
+        if (!t.mg) return;
+        const Table::Mg& m = *t.mg;
+        for (int l = 1; l < m.levels; ++l) {
+            hipLaunchKernelGGL((k_mg_galerkin<T>), dim3((unsigned)m.ncb[l]), dim3(BS), 0, S(), m.fine[l]->p,
+                               m.coarse[l]->p, m.cblk[l]->p);
+            EQ_HIP(hipGetLastError());
+        }
+        hipLaunchKernelGGL((k_mg_dense_setup<T>), dim3((unsigned)m.ncoarsest), dim3(MG_BS), 0, S(), m.coarsest->p);
+        EQ_HIP(hipGetLastError());
+    }
+    // One V-cycle, z = M r (V_R -> V_Z), for the V-cycle instances of the table.
+    template <class T>
+    void vcycle(const Table& t) {
+        const Table::Mg& m = *t.mg;
+        auto smooth = [&](int l, int col, int zero) {
+            if (l == 0) {
+                HLP(C_PC, (k_mg_smooth<T, CP>), col, zero);
+            } else {
+                hipLaunchKernelGGL((k_mg_smooth<T, 1>), dim3((unsigned)m.ncpb[l]), dim3(BS), 0, S(), m.coarse[l]->p,
+                                   m.cpblk[l]->p, col, zero);
+                EQ_HIP(hipGetLastError());
+            }
+        };
+        for (int l = 0; l + 1 < m.levels; ++l) {
+            smooth(l, 0, 1);
+            smooth(l, 1, 0);
+            hipLaunchKernelGGL((k_mg_restrict<T>), dim3((unsigned)m.ncb[l + 1]), dim3(BS), 0, S(), m.fine[l + 1]->p,
+                               m.coarse[l + 1]->p, m.cblk[l + 1]->p);
+            EQ_HIP(hipGetLastError());
+        }
+        hipLaunchKernelGGL((k_mg_dense_solve<T>), dim3((unsigned)m.ncoarsest), dim3(MG_BS), 0, S(), m.coarsest->p);
+        EQ_HIP(hipGetLastError());
+        for (int l = m.levels - 2; l >= 0; --l) {
+            hipLaunchKernelGGL((k_mg_prolong<T>), dim3((unsigned)m.nfpb[l + 1]), dim3(BS), 0, S(), m.fine[l + 1]->p,
+                               m.coarse[l + 1]->p, m.fpblk[l + 1]->p);
+            EQ_HIP(hipGetLastError());
+            smooth(l, 1, 0);
+            smooth(l, 0, 0);
+        }
     }
     // z = M^-1 r for the full-system ILU(0) instances
     template <class T>
@@ -1748,16 +2165,20 @@ struct HeatBatch::Impl {
             fin(t, F(A_RHO, 2, tol2, maxit));
         }
         if (meth & M_CG) {
-            spmv_dot<T>(t, V_P, V_V, 1, (int)V_P, -1, F(A_CG_ALPHA, 1), M_CG);    // q = S p; alpha = rho / p.q
+            spmv_dot<T>(t, V_P, V_V, 1, (int)V_P, -1, F(A_CG_ALPHA, 1), M_CG);    // q = A p; alpha = rho / p.q
             HLP(C_VEC, (k_cg2<T, CP>), t.part->p, t.ng);                         // x, r; r.z, r.r
+            if (t.mg) {                                                          // z = V-cycle(r); r.z, r.r
+                vcycle<T>(t);
+                HLP(C_VEC, (k_cg_dots<T, CP>), t.part->p, t.ng);
+            }
             fin(t, F(A_CG_RHO, 2, tol2, maxit));
-            HLP(C_VEC, (k_cg3<T, CP>));                                          // p = z + beta p
+            HLP(C_VEC, (k_cg3<T, CP>), 0);                                       // p = z + beta p
         }
     }
     // The methods of the instances of a solve (host-side stage of each instance).
     int methods(const Table& t) const {
         int m = 0;
-        for (const Inst* I : t.inst) m |= cg_capable(*I) && I->stage == 0 ? M_CG : M_BICG;
+        for (const Inst* I : t.inst) m |= (cg_capable(*I) || I->mg_on) && I->stage == 0 ? M_CG : M_BICG;
         return m;
     }
 
@@ -1777,6 +2198,12 @@ struct HeatBatch::Impl {
         HLP(C_VEC, (k_kinit<T, CP>), t.part->p, t.ng);
         fin(t, F(A_INIT, 2));
         const int meth = methods(t);
+        if (t.mg && (meth & M_CG)) {   // V-cycle instances: first z, rho = r.z, p = z
+            vcycle<T>(t);
+            HLP(C_VEC, (k_cg_dots<T, CP>), t.part->p, t.ng);
+            fin(t, F(A_CG_START, 1));
+            HLP(C_VEC, (k_cg3<T, CP>), 1);
+        }
         const auto key = std::make_tuple(tol, maxit, meth);
         int iters = 0;
         if (use_graph && !timing_enabled() && t.solves++ > 0) {
@@ -1827,6 +2254,7 @@ struct HeatBatch::Impl {
         hipLaunchKernelGGL(k_refine_flags, dim3((ni + 63) / 64), dim3(64), 0, S(), t.desc->p, (int)ni, 0,
                            cfg.solver_tol, cfg.max_refine);
         precond_setup<T>(t);
+        mg_setup<T>(t);
         if (!mixed) {
             HLP(C_PC, (k_load_rhs<T, CP>), 0, 1);
             bicgstab<T>(t, cfg.solver_tol, cfg.solver_max_iter);
@@ -1909,6 +2337,9 @@ size_t HeatBatch::estimate_bytes(int n, const std::string& precision, bool twoph
     size_t b = (size_t)n * 3 * 8 + (ncoef + 1) * 2 * h * ks + (size_t)NVEC * slots * ks;
     if (!twophase) b += (size_t)n * 2 * 8;   // u, u0
     else b += (size_t)n * 8 * 8 + (mixed ? (size_t)n * ncoef * 8 : 0);   // tx ty tz pv src p sw sw0 (+ FP64 J)
+    // multigrid levels (full form): 7 coefficients and 2 vectors in 2 h slots, n / 8 + n / 64 + ...
+    // cells, and the coarsest inverse
+    if (twophase && !reduced) b += ((size_t)n / 7 + 64) * 9 * ks + (size_t)MG_COARSE * MG_COARSE * ks;
     return b + NSLOT * 8 + 64;
 }
 
@@ -1941,6 +2372,29 @@ static void alloc_solver(Inst& I, bool mixed, size_t ks, double inner_tol) {
     I.kcoef = dev_alloc<char>((size_t)I.ncoef() * 2 * h * ks);
     I.dinv = dev_alloc<char>(2 * h * ks);
     I.vecs = dev_alloc<char>((size_t)NVEC * I.ns() * h * ks);
+}
+
+// Multigrid hierarchy of a two-phase instance: halve every axis longer than one cell
+// (rounding up) until at most MG_COARSE cells are left.
+static void alloc_mg(Inst& I, size_t ks) {
+    I.mg_on = true;
+    int nx = I.nx, ny = I.ny, nz = I.nz, n = I.n;
+    while (n > MG_COARSE && (nx > 1 || ny > 1 || nz > 1)) {
+        nx = (nx + 1) / 2;
+        ny = (ny + 1) / 2;
+        nz = (nz + 1) / 2;
+        n = nx * ny * nz;
+        Inst::Level L;
+        L.nx = nx;
+        L.ny = ny;
+        L.nz = nz;
+        L.n = n;
+        L.K = dev_alloc<char>((size_t)7 * 2 * L.h() * ks);
+        L.v = dev_alloc<char>((size_t)2 * 2 * L.h() * ks);
+        I.mg.push_back(std::move(L));
+    }
+    EQ_REQUIRE(n <= MG_COARSE, "multigrid coarsest level of " << n << " cells");
+    I.minv = dev_alloc<char>((size_t)n * n * ks);
 }
 
 int HeatBatch::add(int nx, int ny, double k, double cv, const double* u_host, double dx, double dy) {
@@ -2014,6 +2468,7 @@ int HeatBatch::add_twophase(const TwoPhaseSpec& sp, const double* perm, const do
     I->sw0 = dev_alloc<double>(n);
     I->pcm = p_->form_of(K_TWOPHASE);
     alloc_solver(*I, p_->mixed, p_->ks(), p_->cfg.inner_tol_twophase);
+    if (p_->cfg.twophase_pressure_cg && p_->cfg.twophase_pressure_mg && I->pcm == PM_ILU) alloc_mg(*I, p_->ks());
     const int id = I->id;
     p_->insts[id] = std::move(I);
     ++p_->version;
@@ -2207,8 +2662,12 @@ double HeatBatch::time_kernel(const std::string& name, int reps, double* event_m
         if (P.mixed) TLP(C_SPMV, (k_ilu_red<float, CP>), (int)V_P, (int)V_PH);
         else TLP(C_SPMV, (k_ilu_red<double, CP>), (int)V_P, (int)V_PH);
     } else if (name == "spmv" && full) {
-        if (P.mixed) TLP(C_SPMV, (k_spmv_rb<float, CP>), (int)V_PH, (int)V_V, 1, (int)V_RH, -1, t.part->p, t.ng);
-        else TLP(C_SPMV, (k_spmv_rb<double, CP>), (int)V_PH, (int)V_V, 1, (int)V_RH, -1, t.part->p, t.ng);
+        if (P.mixed)
+            TLP(C_SPMV, (k_spmv_rb<float, CP>), (int)V_PH, (int)V_V, 1, (int)V_RH, -1, t.part->p, t.ng,
+                (int)(M_BICG | M_CG));
+        else
+            TLP(C_SPMV, (k_spmv_rb<double, CP>), (int)V_PH, (int)V_V, 1, (int)V_RH, -1, t.part->p, t.ng,
+                (int)(M_BICG | M_CG));
     } else if (name == "assemble") {
         if (P.mixed) TL(C_ASM, (k_assemble<true, CP>), t.part->p);
         else TL(C_ASM, (k_assemble<false, CP>), t.part->p);

@@ -39,7 +39,8 @@ struct FusedConfig {
     // Krylov form with preconditioner ilu0, per instance kind: "reduced" (BiCGStab on the
     // red-black Schur complement, preconditioned by its diagonal) or "full" (BiCGStab on
     // the whole system with the two-pass red-black ILU(0)); equal in exact arithmetic.
-    // "auto" (two-phase): reduced in FP64, full in mixed precision, where FP32 rounding in
+    // "auto" (two-phase): the full form when the pressure stage runs CG with the multigrid
+    // V-cycle; otherwise reduced in FP64, full in mixed precision, where FP32 rounding in
     // the reduced operator costs convergence on the singular pressure system.
     std::string krylov_form_heat = "reduced";
     std::string krylov_form_twophase = "auto";
@@ -47,15 +48,18 @@ struct FusedConfig {
     bool reduced_form(bool twophase) const {
         if (precond != "ilu0") return false;
         const std::string& f = twophase ? krylov_form_twophase : krylov_form_heat;
+        if (f == "auto" && twophase && twophase_pressure_cg && twophase_pressure_mg) return false;
         return f == "reduced" || (f == "auto" && precision != "mixed");
     }
     // 2D heat instances with nx % 8 == 0 in the reduced form: Krylov kernels give each lane
     // four consecutive pair ranks of one row (16-byte loads) instead of ranks strided by the
     // block size.  Other instances always take the strided form.
     bool pair_vectors = true;
-    // Two-phase instances in the reduced form solve the pressure stage (a symmetric
-    // M-matrix) by preconditioned CG instead of BiCGStab.
+    // Two-phase instances solve the pressure stage (a symmetric M-matrix) by preconditioned
+    // CG instead of BiCGStab: in the reduced form with the Schur complement's diagonal, in
+    // the full form with an aggregation multigrid V-cycle (twophase_pressure_mg).
     bool twophase_pressure_cg = true;
+    bool twophase_pressure_mg = true;
     int max_refine = 20;
     bool graph = true;                 // capture BiCGStab iterations in a hipGraph
     int cpt = 0;                       // cells per thread (1, 2, 4, 8); 0 = chosen by the tuner

@@ -211,12 +211,17 @@ def test_fused_twophase_residual_matches_oracle(hip_api, oracle_api, dims, stage
 # 1e-7; FP32 with ILU(0) runs at the benchmark's tolerances (Newton 1e-8, linear 1e-6),
 # compared at 1e-5.
 @pytest.mark.parametrize("params,newton_tol,solver_tol,bar", [
-    ("precision=fp64 preconditioner=ilu0", 1e-10, 1e-10, 1e-7),   # reduced form (auto in FP64), pressure by CG
-    ("precision=fp64 preconditioner=ilu0 twophase_pressure_cg=0", 1e-10, 1e-10, 1e-7),   # pressure by BiCGStab
-    ("precision=fp64 preconditioner=ilu0 krylov_form_twophase=full", 1e-10, 1e-10, 1e-7),
+    # pressure by CG with the multigrid V-cycle (the default: full form)
+    ("precision=fp64 preconditioner=ilu0", 1e-10, 1e-10, 1e-7),
+    # pressure by CG with the Schur complement's diagonal (reduced form)
+    ("precision=fp64 preconditioner=ilu0 krylov_form_twophase=reduced", 1e-10, 1e-10, 1e-7),
+    # pressure by BiCGStab: reduced form, full form with red-black ILU(0)
+    ("precision=fp64 preconditioner=ilu0 twophase_pressure_cg=0", 1e-10, 1e-10, 1e-7),
+    ("precision=fp64 preconditioner=ilu0 twophase_pressure_mg=0 krylov_form_twophase=full", 1e-10, 1e-10, 1e-7),
     ("precision=fp64 preconditioner=diagonal", 1e-10, 1e-10, 1e-7),
     ("precision=mixed preconditioner=diagonal", 1e-10, 1e-10, 1e-7),
-    ("precision=mixed preconditioner=ilu0", 1e-8, 1e-6, 1e-5)])
+    ("precision=mixed preconditioner=ilu0", 1e-8, 1e-6, 1e-5),
+    ("precision=mixed preconditioner=ilu0 twophase_pressure_mg=0", 1e-8, 1e-6, 1e-5)])
 def test_fused_twophase_steps_match_oracle(hip_api, oracle_api, params, newton_tol, solver_tol, bar):
     nx, ny, nz = 12, 10, 6
     perm, poro, sw0, sc, sv = case(nx, ny, nz)
