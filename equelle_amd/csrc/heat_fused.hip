@@ -1044,12 +1044,15 @@ __global__ __launch_bounds__(BS) void k_cg2(const Desc* __restrict__ D, const in
     group_store<CPT>(a0, PARTSK(part));
     group_store<CPT>(a1, PARTSK(part + ng));
 }
-// CG with the V-cycle: partial dots (r.z, r.r), z in V_Z.
+// CG with the V-cycle on an instance without coarse levels (the V-cycle is the dense
+// solve): partial dots (r.z, r.r), z in V_Z.
 template <class T, int CPT>
 __global__ __launch_bounds__(BS) void k_cg_dots(const Desc* __restrict__ D, const int2* __restrict__ blk,
                                                 double* __restrict__ part, int ng) {
     LOCATE();
-    if (!d.mgp || meth_of(d) != M_CG) return;   // block-uniform; k_cg2 wrote the others' partials
+    // block-uniform; k_cg2 wrote the others' partials, the fine level's last half sweep
+    // those of instances with coarse levels
+    if (!d.mgp || d.mgl != 0 || meth_of(d) != M_CG) return;
     double a0[CPT] = {}, a1[CPT] = {};
     if (d.act[0]) {
         const T* __restrict__ r = (const T*)d.v[V_R];
@@ -1239,26 +1242,60 @@ __device__ inline int rb_slot(const Desc& d, int c, int i, int j, int k) { retur
 
 // Gauss-Seidel half sweep over the cells of colour col of a level: x = (b - offdiag x) / C
 // (zero: x is zero before the sweep, so the neighbours are not read).  Slots without a
-// cell get 0 (they stand in the dot products of the fine level).
+// cell get 0 (they stand in the dot products of the fine level).  dots (the fine level's
+// last half sweep, red): also the CG partial dots (r.z, r.r) over both slots of each
+// pair (r = V_R, z = V_Z, the black half final already).
 template <class T, int CPT>
 __global__ __launch_bounds__(BS) void k_mg_smooth(const Desc* __restrict__ D, const int2* __restrict__ blk, int col,
-                                                  int zero) {
+                                                  int zero, int dots, double* __restrict__ part, int ng) {
+    LOCATE();
+    if (d.mgl == 0 || !d.mgp || meth_of(d) != M_CG) return;   // block-uniform; k_cg_dots covers mgl == 0
+    double a0[CPT] = {}, a1[CPT] = {};
+    if (d.act[0]) {
+        const int h = d.h;
+        const T* __restrict__ b = (const T*)d.v[V_R] + col * h;
+        const T* __restrict__ C = (const T*)d.K[0] + col * h;
+        T* __restrict__ x = (T*)d.v[V_Z];
+        FOR_GROUPS(m, p) {
+            if (p >= h) continue;
+            int i, j, k;
+            const int c = cell_of(d, p, col, i, j, k);
+            const int s = col * h + p;
+            T xs = (T)0;
+            if (c >= 0) {
+                const T off = zero ? (T)0 : offdiag<T>(d, s, c, i, j, k, x + (1 - col) * h);
+                xs = (b[p] - off) / C[p];
+            }
+            x[s] = xs;
+            if (dots) {
+                const T* __restrict__ r = (const T*)d.v[V_R];
+                const int o = (1 - col) * h + p;
+                const T ro = r[o], zo = x[o];
+                a0[m] = (double)b[p] * (double)xs + (double)ro * (double)zo;
+                a1[m] = (double)b[p] * (double)b[p] + (double)ro * (double)ro;
+            }
+        }
+    }
+    if (dots) {
+        group_store<CPT>(a0, PARTSK(part));
+        group_store<CPT>(a1, PARTSK(part + ng));
+    }
+}
+// Residual on the red cells after a red-black half-sweep pair (the black residual is zero
+// then): t_r = b_r - C_r x_r - A_rb x_b into V_T, for the restriction.
+template <class T, int CPT>
+__global__ __launch_bounds__(BS) void k_mg_resid(const Desc* __restrict__ D, const int2* __restrict__ blk) {
     LOCATE();
     if (!mg_on(d) || d.mgl == 0) return;
     const int h = d.h;
-    const T* __restrict__ b = (const T*)d.v[V_R] + col * h;
-    const T* __restrict__ C = (const T*)d.K[0] + col * h;
-    T* __restrict__ x = (T*)d.v[V_Z];
+    const T* __restrict__ b = (const T*)d.v[V_R];
+    const T* __restrict__ C = (const T*)d.K[0];
+    const T* __restrict__ x = (const T*)d.v[V_Z];
+    T* __restrict__ t = (T*)d.v[V_T];
     FOR_PAIRS(p) {
         int i, j, k;
-        const int c = cell_of(d, p, col, i, j, k);
-        const int s = col * h + p;
-        if (c < 0) {
-            x[s] = (T)0;
-            continue;
-        }
-        const T off = zero ? (T)0 : offdiag<T>(d, s, c, i, j, k, x + (1 - col) * h);
-        x[s] = (b[p] - off) / C[p];
+        const int c = cell_of(d, p, 0, i, j, k);
+        t[p] = c < 0 ? (T)0 : b[p] - (C[p] * x[p] + offdiag<T>(d, p, c, i, j, k, x + h));
     }
 }
 
@@ -1302,8 +1339,8 @@ __global__ __launch_bounds__(BS) void k_mg_galerkin(const Desc* __restrict__ F, 
     for (int q = 0; q < 7; ++q) ((T*)c.K[q])[sc] = co[q];
 }
 
-// Restriction (thread per coarse cell): b_c = sum over the aggregate of (b - A z) on the
-// fine level.
+// Restriction (thread per coarse cell): b_c = sum over the aggregate of the fine
+// residual, i.e. of the red cells' residuals in V_T (k_mg_resid).
 template <class T>
 __global__ __launch_bounds__(BS) void k_mg_restrict(const Desc* __restrict__ F, const Desc* __restrict__ Cd,
                                                     const int2* __restrict__ blk) {
@@ -1311,17 +1348,12 @@ __global__ __launch_bounds__(BS) void k_mg_restrict(const Desc* __restrict__ F, 
     if (!mg_on(c)) return;
     int ic, jc, kc;
     cell_ijk(c, t_, ic, jc, kc);
-    const T* __restrict__ b = (const T*)f.v[V_R];
-    const T* __restrict__ x = (const T*)f.v[V_Z];
-    const int h = f.h;
+    const T* __restrict__ t = (const T*)f.v[V_T];
     T acc = (T)0;
     for (int k = 2 * kc; k < min(2 * kc + 2, f.nz); ++k)
         for (int j = 2 * jc; j < min(2 * jc + 2, f.ny); ++j)
-            for (int i = 2 * ic; i < min(2 * ic + 2, f.nx); ++i) {
-                const int fc = i + f.nx * (j + f.ny * k);
-                const int col = (i + j + k) & 1, s = col * h + (fc >> 1);
-                acc += b[s] - (((const T*)f.K[0])[s] * x[s] + offdiag<T>(f, s, fc, i, j, k, x + (1 - col) * h));
-            }
+            for (int i = 2 * ic + ((2 * ic + j + k) & 1); i < min(2 * ic + 2, f.nx); i += 2)   // the red cell of the row
+                acc += t[(i + f.nx * (j + f.ny * k)) >> 1];
     ((T*)c.v[V_R])[rb_slot(c, t_, ic, jc, kc)] = acc;
 }
 
@@ -1606,7 +1638,7 @@ struct Inst {
     StepStats st;
     int pcm = PM_SCHUR;            // Krylov form / preconditioner
     // multigrid hierarchy (two-phase pressure by CG with the V-cycle): coarse levels 1, 2,
-    // ... (coefficients 7 x 2 h, vectors V_R and V_Z 2 h each, Krylov precision) and the
+    // ... (coefficients 7 x 2 h, vectors V_R, V_Z, V_T 2 h each, Krylov precision) and the
     // coarsest level's dense inverse (the fine level's when it is already small)
     struct Level {
         int nx = 1, ny = 1, nz = 1, n = 0;
@@ -1667,6 +1699,7 @@ struct Table {
         std::vector<int> ncb, nfpb, ncpb;
         DevPtr<Desc> coarsest;   // each instance's coarsest level
         int ncoarsest = 0;
+        bool shallow = false;    // an instance without coarse levels (its V-cycle is the dense solve)
     };
     std::unique_ptr<Mg> mg;
     Shape shape;
@@ -1730,6 +1763,7 @@ struct HeatBatch::Impl {
         for (int v = 0; v < NVEC; ++v) d.v[v] = nullptr;
         d.v[V_R] = L.v->p;
         d.v[V_Z] = L.v->p + 2 * h * ks();
+        d.v[V_T] = L.v->p + 4 * h * ks();
         d.mgl = (int)I.mg.size() - l;
         d.minv = d.mgl == 0 ? (void*)I.minv->p : nullptr;
         return d;
@@ -1740,7 +1774,10 @@ struct HeatBatch::Impl {
             if (I->mg_on) mi.push_back(I);
         if (mi.empty()) return;
         auto m = std::make_unique<Table::Mg>();
-        for (Inst* I : mi) m->levels = std::max(m->levels, (int)I->mg.size() + 1);
+        for (Inst* I : mi) {
+            m->levels = std::max(m->levels, (int)I->mg.size() + 1);
+            m->shallow |= I->mg.empty();
+        }
         m->fine.resize(m->levels);
         m->coarse.resize(m->levels);
         m->cblk.resize(m->levels);
@@ -2093,22 +2130,30 @@ struct HeatBatch::Impl {
         hipLaunchKernelGGL((k_mg_dense_setup<T>), dim3((unsigned)m.ncoarsest), dim3(MG_BS), 0, S(), m.coarsest->p);
         EQ_HIP(hipGetLastError());
     }
-    // One V-cycle, z = M r (V_R -> V_Z), for the V-cycle instances of the table.
+    // One V-cycle, z = M r (V_R -> V_Z), for the V-cycle instances of the table, followed
+    // by the CG partial dots (r.z, r.r) of every V-cycle instance.
     template <class T>
-    void vcycle(const Table& t) {
+    void vcycle_dots(const Table& t) {
         const Table::Mg& m = *t.mg;
-        auto smooth = [&](int l, int col, int zero) {
+        auto smooth = [&](int l, int col, int zero, int dots) {
             if (l == 0) {
-                HLP(C_PC, (k_mg_smooth<T, CP>), col, zero);
+                HLP(C_PC, (k_mg_smooth<T, CP>), col, zero, dots, t.part->p, t.ng);
             } else {
                 hipLaunchKernelGGL((k_mg_smooth<T, 1>), dim3((unsigned)m.ncpb[l]), dim3(BS), 0, S(), m.coarse[l]->p,
-                                   m.cpblk[l]->p, col, zero);
+                                   m.cpblk[l]->p, col, zero, 0, (double*)nullptr, 0);
                 EQ_HIP(hipGetLastError());
             }
         };
         for (int l = 0; l + 1 < m.levels; ++l) {
-            smooth(l, 0, 1);
-            smooth(l, 1, 0);
+            smooth(l, 0, 1, 0);
+            smooth(l, 1, 0, 0);
+            if (l == 0) {
+                HLP(C_PC, (k_mg_resid<T, CP>));
+            } else {
+                hipLaunchKernelGGL((k_mg_resid<T, 1>), dim3((unsigned)m.ncpb[l]), dim3(BS), 0, S(), m.coarse[l]->p,
+                                   m.cpblk[l]->p);
+                EQ_HIP(hipGetLastError());
+            }
             hipLaunchKernelGGL((k_mg_restrict<T>), dim3((unsigned)m.ncb[l + 1]), dim3(BS), 0, S(), m.fine[l + 1]->p,
                                m.coarse[l + 1]->p, m.cblk[l + 1]->p);
             EQ_HIP(hipGetLastError());
@@ -2119,9 +2164,10 @@ struct HeatBatch::Impl {
             hipLaunchKernelGGL((k_mg_prolong<T>), dim3((unsigned)m.nfpb[l + 1]), dim3(BS), 0, S(), m.fine[l + 1]->p,
                                m.coarse[l + 1]->p, m.fpblk[l + 1]->p);
             EQ_HIP(hipGetLastError());
-            smooth(l, 1, 0);
-            smooth(l, 0, 0);
+            smooth(l, 1, 0, 0);
+            smooth(l, 0, 0, l == 0);
         }
+        if (m.shallow) HLP(C_VEC, (k_cg_dots<T, CP>), t.part->p, t.ng);
     }
     // z = M^-1 r for the full-system ILU(0) instances
     template <class T>
@@ -2167,10 +2213,7 @@ struct HeatBatch::Impl {
         if (meth & M_CG) {
             spmv_dot<T>(t, V_P, V_V, 1, (int)V_P, -1, F(A_CG_ALPHA, 1), M_CG);    // q = A p; alpha = rho / p.q
             HLP(C_VEC, (k_cg2<T, CP>), t.part->p, t.ng);                         // x, r; r.z, r.r
-            if (t.mg) {                                                          // z = V-cycle(r); r.z, r.r
-                vcycle<T>(t);
-                HLP(C_VEC, (k_cg_dots<T, CP>), t.part->p, t.ng);
-            }
+            if (t.mg) vcycle_dots<T>(t);                                         // z = V-cycle(r); r.z, r.r
             fin(t, F(A_CG_RHO, 2, tol2, maxit));
             HLP(C_VEC, (k_cg3<T, CP>), 0);                                       // p = z + beta p
         }
@@ -2199,8 +2242,7 @@ struct HeatBatch::Impl {
         fin(t, F(A_INIT, 2));
         const int meth = methods(t);
         if (t.mg && (meth & M_CG)) {   // V-cycle instances: first z, rho = r.z, p = z
-            vcycle<T>(t);
-            HLP(C_VEC, (k_cg_dots<T, CP>), t.part->p, t.ng);
+            vcycle_dots<T>(t);
             fin(t, F(A_CG_START, 1));
             HLP(C_VEC, (k_cg3<T, CP>), 1);
         }
@@ -2390,7 +2432,7 @@ static void alloc_mg(Inst& I, size_t ks) {
         L.nz = nz;
         L.n = n;
         L.K = dev_alloc<char>((size_t)7 * 2 * L.h() * ks);
-        L.v = dev_alloc<char>((size_t)2 * 2 * L.h() * ks);
+        L.v = dev_alloc<char>((size_t)3 * 2 * L.h() * ks);   // V_R, V_Z, V_T
         I.mg.push_back(std::move(L));
     }
     EQ_REQUIRE(n <= MG_COARSE, "multigrid coarsest level of " << n << " cells");
