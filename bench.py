@@ -260,7 +260,11 @@ def bench_twophase(lib, args):
         runs[name] = {"failed": str(err)}
 
     gsteps = max(1, min(args.steps, 2))   # timed steps of the generic legs (about 2 s each)
-    for name, prec in (("fused_fp64", "fp64"), ("fused_mixed", "mixed")):
+    legs = [(f"fused_{p}", p) for p in args.tp_legs.split(",")]
+    EQ = {"fp64", "mixed"}
+    if not legs or any(p not in EQ for _, p in legs) or legs[0][1] != "fp64":
+        raise SystemExit("--tp-legs: fp64 first, then optionally mixed")
+    for name, prec in legs:
         b = C.c_void_p()
         lib.batch_create(f"precision={prec} preconditioner={args.precond} rel_res_tol={TWOPHASE['rel_res_tol']} "
                          f"abs_res_tol=0 max_iter=20 solver_tol=1e-6 solver_max_iter=5000".encode(), C.byref(b))
@@ -532,6 +536,8 @@ def main():
     ap.add_argument("--grid3", default="128x128x64", help="two-phase grid nx x ny x nz")
     ap.add_argument("--grid3-fi", dest="grid3_fi", default="32x32x16", help="fully implicit two-phase grid")
     ap.add_argument("--tp-solver", dest="tp_solver", default="GMRes", help="two-phase Krylov solver")
+    ap.add_argument("--tp-legs", dest="tp_legs", default="fp64,mixed",
+                    help="two-phase fused legs: fp64 (the headline) and optionally mixed")
     ap.add_argument("--rates", default="50,100,200,500,1000,2000", help="serving: arrival rates (req/s)")
     ap.add_argument("--requests", type=int, default=200, help="serving: requests per rate")
     ap.add_argument("--frac3d", type=float, default=0.3, help="serving: share of 3D two-phase requests")
