@@ -52,6 +52,7 @@ struct Desc {
     int cgp;                      // two-phase, reduced form: the pressure stage (symmetric) is solved by CG
     int mgp;                      // two-phase, full form: pressure by CG with the multigrid V-cycle
     int mgl;                      // multigrid: levels below this one (0 on the coarsest level)
+    int mgt;                      // multigrid: the level belongs to the instance's workgroup tail (k_mg_tail)
     void* minv;                   // multigrid, coarsest level: dense (A + s 1 1^T)^-1, cell order
     double rnx, rnxy;             // 1 / nx, 1 / (nx ny): cell -> (i, j, k) without integer division
     double dt;
@@ -1044,15 +1045,15 @@ __global__ __launch_bounds__(BS) void k_cg2(const Desc* __restrict__ D, const in
     group_store<CPT>(a0, PARTSK(part));
     group_store<CPT>(a1, PARTSK(part + ng));
 }
-// CG with the V-cycle on an instance without coarse levels (the V-cycle is the dense
-// solve): partial dots (r.z, r.r), z in V_Z.
+// CG with the V-cycle on an instance whose fine level is in the workgroup tail (the whole
+// V-cycle runs in k_mg_tail): partial dots (r.z, r.r), z in V_Z.
 template <class T, int CPT>
 __global__ __launch_bounds__(BS) void k_cg_dots(const Desc* __restrict__ D, const int2* __restrict__ blk,
                                                 double* __restrict__ part, int ng) {
     LOCATE();
     // block-uniform; k_cg2 wrote the others' partials, the fine level's last half sweep
-    // those of instances with coarse levels
-    if (!d.mgp || d.mgl != 0 || meth_of(d) != M_CG) return;
+    // those of instances whose fine level is above the tail
+    if (!d.mgp || !d.mgt || meth_of(d) != M_CG) return;
     double a0[CPT] = {}, a1[CPT] = {};
     if (d.act[0]) {
         const T* __restrict__ r = (const T*)d.v[V_R];
@@ -1236,43 +1237,90 @@ __global__ __launch_bounds__(BS) void k_b3(const Desc* __restrict__ D, const int
 constexpr int MG_COARSE = 64;
 constexpr int MG_BS = 256;   // threads of the dense coarsest-level kernels
 
+constexpr int MG_TAIL = 4096;   // levels of at most this many cells run in one workgroup (k_mg_tail)
+constexpr int TAIL_BS = 1024;
+
 __device__ inline bool mg_on(const Desc& d) { return d.mgp && d.act[0] && meth_of(d) == M_CG; }
 // red-black slot of cell c = (i, j, k) of a level
 __device__ inline int rb_slot(const Desc& d, int c, int i, int j, int k) { return ((i + j + k) & 1) * d.h + (c >> 1); }
 
-// Gauss-Seidel half sweep over the cells of colour col of a level: x = (b - offdiag x) / C
-// (zero: x is zero before the sweep, so the neighbours are not read).  Slots without a
-// cell get 0 (they stand in the dot products of the fine level).  dots (the fine level's
-// last half sweep, red): also the CG partial dots (r.z, r.r) over both slots of each
-// pair (r = V_R, z = V_Z, the black half final already).
+// Per-element bodies, shared by the one-level kernels and the workgroup tail.
+// Gauss-Seidel update of the colour-col cell of pair p: x = (b - offdiag x) / C (zero: x
+// is zero before the sweep, so the neighbours are not read); a slot without a cell gets 0.
+template <class T>
+__device__ inline T mg_smooth_at(const Desc& d, int p, int col, int zero) {
+    const int h = d.h;
+    T* __restrict__ x = (T*)d.v[V_Z];
+    int i, j, k;
+    const int c = cell_of(d, p, col, i, j, k);
+    const int s = col * h + p;
+    T xs = (T)0;
+    if (c >= 0) {
+        const T off = zero ? (T)0 : offdiag<T>(d, s, c, i, j, k, x + (1 - col) * h);
+        xs = (((const T*)d.v[V_R])[s] - off) / ((const T*)d.K[0])[s];
+    }
+    x[s] = xs;
+    return xs;
+}
+// Red residual of pair p after a red-black pass (the black residual is zero then):
+// t_r = b_r - C_r x_r - A_rb x_b into V_T.
+template <class T>
+__device__ inline void mg_resid_at(const Desc& d, int p) {
+    const T* __restrict__ x = (const T*)d.v[V_Z];
+    int i, j, k;
+    const int c = cell_of(d, p, 0, i, j, k);
+    ((T*)d.v[V_T])[p] = c < 0 ? (T)0
+                              : ((const T*)d.v[V_R])[p] - (((const T*)d.K[0])[p] * x[p] + offdiag<T>(d, p, c, i, j, k, x + d.h));
+}
+// Coarse cell cc: b_c = the sum of its aggregate's red residuals (fine level f).
+template <class T>
+__device__ inline void mg_restrict_at(const Desc& f, const Desc& c, int cc) {
+    int ic, jc, kc;
+    cell_ijk(c, cc, ic, jc, kc);
+    const T* __restrict__ t = (const T*)f.v[V_T];
+    T acc = (T)0;
+    for (int k = 2 * kc; k < min(2 * kc + 2, f.nz); ++k)
+        for (int j = 2 * jc; j < min(2 * jc + 2, f.ny); ++j)
+            for (int i = 2 * ic + ((2 * ic + j + k) & 1); i < min(2 * ic + 2, f.nx); i += 2)   // the red cell of the row
+                acc += t[(i + f.nx * (j + f.ny * k)) >> 1];
+    ((T*)c.v[V_R])[rb_slot(c, cc, ic, jc, kc)] = acc;
+}
+// Fine pair p: z_f += z_c of each cell's aggregate.
+template <class T>
+__device__ inline void mg_prolong_at(const Desc& f, const Desc& c, int p) {
+    T* __restrict__ xf = (T*)f.v[V_Z];
+    const T* __restrict__ xc = (const T*)c.v[V_Z];
+    for (int col = 0; col < 2; ++col) {
+        int i, j, k;
+        if (cell_of(f, p, col, i, j, k) < 0) continue;
+        const int ic = i >> 1, jc = j >> 1, kc = k >> 1;
+        xf[col * f.h + p] += xc[rb_slot(c, ic + c.nx * (jc + c.ny * kc), ic, jc, kc)];
+    }
+}
+
+// One-level kernels for the levels above an instance's tail (Desc::mgt = 0).
+// Gauss-Seidel half sweep over the cells of colour col; dots (the fine level's last half
+// sweep, red): also the CG partial dots (r.z, r.r) over both slots of each pair (r = V_R,
+// z = V_Z, the black half final already).
 template <class T, int CPT>
 __global__ __launch_bounds__(BS) void k_mg_smooth(const Desc* __restrict__ D, const int2* __restrict__ blk, int col,
                                                   int zero, int dots, double* __restrict__ part, int ng) {
     LOCATE();
-    if (d.mgl == 0 || !d.mgp || meth_of(d) != M_CG) return;   // block-uniform; k_cg_dots covers mgl == 0
+    // block-uniform; k_cg_dots covers the instances whose fine level is in the tail
+    if (d.mgt || !d.mgp || meth_of(d) != M_CG) return;
     double a0[CPT] = {}, a1[CPT] = {};
     if (d.act[0]) {
         const int h = d.h;
-        const T* __restrict__ b = (const T*)d.v[V_R] + col * h;
-        const T* __restrict__ C = (const T*)d.K[0] + col * h;
-        T* __restrict__ x = (T*)d.v[V_Z];
+        const T* __restrict__ r = (const T*)d.v[V_R];
+        const T* __restrict__ z = (const T*)d.v[V_Z];
         FOR_GROUPS(m, p) {
             if (p >= h) continue;
-            int i, j, k;
-            const int c = cell_of(d, p, col, i, j, k);
-            const int s = col * h + p;
-            T xs = (T)0;
-            if (c >= 0) {
-                const T off = zero ? (T)0 : offdiag<T>(d, s, c, i, j, k, x + (1 - col) * h);
-                xs = (b[p] - off) / C[p];
-            }
-            x[s] = xs;
+            const T xs = mg_smooth_at<T>(d, p, col, zero);
             if (dots) {
-                const T* __restrict__ r = (const T*)d.v[V_R];
+                const T b = r[col * h + p];
                 const int o = (1 - col) * h + p;
-                const T ro = r[o], zo = x[o];
-                a0[m] = (double)b[p] * (double)xs + (double)ro * (double)zo;
-                a1[m] = (double)b[p] * (double)b[p] + (double)ro * (double)ro;
+                a0[m] = (double)b * (double)xs + (double)r[o] * (double)z[o];
+                a1[m] = (double)b * (double)b + (double)r[o] * (double)r[o];
             }
         }
     }
@@ -1281,22 +1329,11 @@ __global__ __launch_bounds__(BS) void k_mg_smooth(const Desc* __restrict__ D, co
         group_store<CPT>(a1, PARTSK(part + ng));
     }
 }
-// Residual on the red cells after a red-black half-sweep pair (the black residual is zero
-// then): t_r = b_r - C_r x_r - A_rb x_b into V_T, for the restriction.
 template <class T, int CPT>
 __global__ __launch_bounds__(BS) void k_mg_resid(const Desc* __restrict__ D, const int2* __restrict__ blk) {
     LOCATE();
-    if (!mg_on(d) || d.mgl == 0) return;
-    const int h = d.h;
-    const T* __restrict__ b = (const T*)d.v[V_R];
-    const T* __restrict__ C = (const T*)d.K[0];
-    const T* __restrict__ x = (const T*)d.v[V_Z];
-    T* __restrict__ t = (T*)d.v[V_T];
-    FOR_PAIRS(p) {
-        int i, j, k;
-        const int c = cell_of(d, p, 0, i, j, k);
-        t[p] = c < 0 ? (T)0 : b[p] - (C[p] * x[p] + offdiag<T>(d, p, c, i, j, k, x + h));
-    }
+    if (d.mgt || !mg_on(d)) return;
+    FOR_PAIRS(p) mg_resid_at<T>(d, p);
 }
 
 // Fine level F and coarse level Cd of one instance (thread per cell or pair of a level).
@@ -1339,39 +1376,22 @@ __global__ __launch_bounds__(BS) void k_mg_galerkin(const Desc* __restrict__ F, 
     for (int q = 0; q < 7; ++q) ((T*)c.K[q])[sc] = co[q];
 }
 
-// Restriction (thread per coarse cell): b_c = sum over the aggregate of the fine
-// residual, i.e. of the red cells' residuals in V_T (k_mg_resid).
+// Restriction (thread per coarse cell), for fine levels above the tail.
 template <class T>
 __global__ __launch_bounds__(BS) void k_mg_restrict(const Desc* __restrict__ F, const Desc* __restrict__ Cd,
                                                     const int2* __restrict__ blk) {
     MG_LOCATE(c.n);
-    if (!mg_on(c)) return;
-    int ic, jc, kc;
-    cell_ijk(c, t_, ic, jc, kc);
-    const T* __restrict__ t = (const T*)f.v[V_T];
-    T acc = (T)0;
-    for (int k = 2 * kc; k < min(2 * kc + 2, f.nz); ++k)
-        for (int j = 2 * jc; j < min(2 * jc + 2, f.ny); ++j)
-            for (int i = 2 * ic + ((2 * ic + j + k) & 1); i < min(2 * ic + 2, f.nx); i += 2)   // the red cell of the row
-                acc += t[(i + f.nx * (j + f.ny * k)) >> 1];
-    ((T*)c.v[V_R])[rb_slot(c, t_, ic, jc, kc)] = acc;
+    if (f.mgt || !mg_on(c)) return;
+    mg_restrict_at<T>(f, c, t_);
 }
 
-// Prolongation (thread per fine pair): z_f += z_c of the cell's aggregate.
+// Prolongation (thread per fine pair), for fine levels above the tail.
 template <class T>
 __global__ __launch_bounds__(BS) void k_mg_prolong(const Desc* __restrict__ F, const Desc* __restrict__ Cd,
                                                    const int2* __restrict__ blk) {
     MG_LOCATE(f.h);
-    if (!mg_on(c)) return;
-    T* __restrict__ xf = (T*)f.v[V_Z];
-    const T* __restrict__ xc = (const T*)c.v[V_Z];
-    for (int col = 0; col < 2; ++col) {
-        int i, j, k;
-        if (cell_of(f, t_, col, i, j, k) < 0) continue;
-        const int ic = i >> 1, jc = j >> 1, kc = k >> 1;
-        const int cc = ic + c.nx * (jc + c.ny * kc);
-        xf[col * f.h + t_] += xc[rb_slot(c, cc, ic, jc, kc)];
-    }
+    if (f.mgt || !mg_on(c)) return;
+    mg_prolong_at<T>(f, c, t_);
 }
 #undef MG_LOCATE
 
@@ -1433,33 +1453,64 @@ __global__ __launch_bounds__(MG_BS) void k_mg_dense_setup(const Desc* __restrict
     for (int e = tid; e < n * n; e += MG_BS) inv[e] = (T)A[e / n][e % n];
 }
 
-// Coarsest level solve (one block per instance): z = inverse * b.
+// The tail of the V-cycle (one workgroup per instance): levels [first, first + count) of
+// the instance, top first (its levels of at most MG_TAIL cells; the last one is the
+// coarsest, at most MG_COARSE cells).  The top level's right-hand side is in place; the
+// workgroup runs the descent, the dense coarsest solve and the ascent, with a workgroup
+// barrier between stages (the data of these levels stays in L2).  On return the top
+// level's V_Z holds its correction.
 template <class T>
-__global__ __launch_bounds__(MG_BS) void k_mg_dense_solve(const Desc* __restrict__ Cd) {
-    const Desc& c = Cd[blockIdx.x];
-    if (!mg_on(c)) return;
-    __shared__ double bb[MG_COARSE];
-    const int n = c.n, tid = threadIdx.x;
-    const T* __restrict__ b = (const T*)c.v[V_R];
-    T* __restrict__ x = (T*)c.v[V_Z];
-    for (int q = tid; q < n; q += MG_BS) {
-        int i, j, k;
-        cell_ijk(c, q, i, j, k);
-        bb[q] = (double)b[rb_slot(c, q, i, j, k)];
+__global__ __launch_bounds__(TAIL_BS) void k_mg_tail(const Desc* __restrict__ L, const int2* __restrict__ range) {
+    const int2 rg = range[blockIdx.x];
+    if (!mg_on(L[rg.x])) return;   // uniform over the workgroup
+    const int tid = threadIdx.x, top = rg.x, last = rg.x + rg.y - 1;
+    for (int q = top; q < last; ++q) {
+        const Desc& d = L[q];
+        for (int p = tid; p < d.h; p += TAIL_BS) mg_smooth_at<T>(d, p, 0, 1);
+        __syncthreads();
+        for (int p = tid; p < d.h; p += TAIL_BS) mg_smooth_at<T>(d, p, 1, 0);
+        __syncthreads();
+        for (int p = tid; p < d.h; p += TAIL_BS) mg_resid_at<T>(d, p);
+        __syncthreads();
+        const Desc& c = L[q + 1];
+        for (int cc = tid; cc < c.n; cc += TAIL_BS) mg_restrict_at<T>(d, c, cc);
+        __syncthreads();
     }
-    __syncthreads();
-    const T* __restrict__ inv = (const T*)c.minv;
-    for (int q = tid; q < n; q += MG_BS) {
-        double acc = 0.0;
-        for (int e = 0; e < n; ++e) acc += (double)inv[q * n + e] * bb[e];
-        int i, j, k;
-        cell_ijk(c, q, i, j, k);
-        x[rb_slot(c, q, i, j, k)] = (T)acc;
+    {   // coarsest: z = inverse * b
+        const Desc& c = L[last];
+        __shared__ double bb[MG_COARSE];
+        const int n = c.n;
+        const T* __restrict__ b = (const T*)c.v[V_R];
+        T* __restrict__ x = (T*)c.v[V_Z];
+        for (int q = tid; q < n; q += TAIL_BS) {
+            int i, j, k;
+            cell_ijk(c, q, i, j, k);
+            bb[q] = (double)b[rb_slot(c, q, i, j, k)];
+        }
+        __syncthreads();
+        const T* __restrict__ inv = (const T*)c.minv;
+        for (int q = tid; q < n; q += TAIL_BS) {
+            double acc = 0.0;
+            for (int e = 0; e < n; ++e) acc += (double)inv[q * n + e] * bb[e];
+            int i, j, k;
+            cell_ijk(c, q, i, j, k);
+            x[rb_slot(c, q, i, j, k)] = (T)acc;
+        }
+        if ((n & 1) && tid == 0) {   // the last pair holds one cell; the other colour's slot stays 0
+            int i, j, k;
+            cell_ijk(c, n - 1, i, j, k);
+            x[(1 - ((i + j + k) & 1)) * c.h + (n >> 1)] = (T)0;
+        }
+        __syncthreads();
     }
-    if ((n & 1) && tid == 0) {   // the last pair holds one cell; the other colour's slot stays 0
-        int i, j, k;
-        cell_ijk(c, n - 1, i, j, k);
-        x[(1 - ((i + j + k) & 1)) * c.h + (n >> 1)] = (T)0;
+    for (int q = last - 1; q >= top; --q) {
+        const Desc& d = L[q];
+        for (int p = tid; p < d.h; p += TAIL_BS) mg_prolong_at<T>(d, L[q + 1], p);
+        __syncthreads();
+        for (int p = tid; p < d.h; p += TAIL_BS) mg_smooth_at<T>(d, p, 1, 0);
+        __syncthreads();
+        for (int p = tid; p < d.h; p += TAIL_BS) mg_smooth_at<T>(d, p, 0, 0);
+        __syncthreads();
     }
 }
 
@@ -1648,6 +1699,7 @@ struct Inst {
     bool mg_on = false;
     std::vector<Level> mg;
     DevPtr<char> minv;
+    int mg_tail = 0;               // first level (0 = fine) of at most MG_TAIL cells: k_mg_tail runs it and below
     int ncoef() const { return nz > 1 ? 7 : 5; }
     int h() const { return (n + 1) / 2; }
     int ns() const { return pcm == PM_SCHUR ? 1 : 2; }
@@ -1699,7 +1751,13 @@ struct Table {
         std::vector<int> ncb, nfpb, ncpb;
         DevPtr<Desc> coarsest;   // each instance's coarsest level
         int ncoarsest = 0;
-        bool shallow = false;    // an instance without coarse levels (its V-cycle is the dense solve)
+        // workgroup tails: each instance's levels from its tail top down (flat), and per
+        // instance the (first, count) range
+        DevPtr<Desc> tail;
+        DevPtr<int2> trange;
+        int ntail = 0;
+        std::vector<char> sep;   // level l has one-level launches (some instance has it above its tail)
+        bool tail0 = false;      // an instance whose fine level is in its tail (dots by k_cg_dots)
     };
     std::unique_ptr<Mg> mg;
     Shape shape;
@@ -1765,6 +1823,7 @@ struct HeatBatch::Impl {
         d.v[V_Z] = L.v->p + 2 * h * ks();
         d.v[V_T] = L.v->p + 4 * h * ks();
         d.mgl = (int)I.mg.size() - l;
+        d.mgt = l >= I.mg_tail;
         d.minv = d.mgl == 0 ? (void*)I.minv->p : nullptr;
         return d;
     }
@@ -1774,10 +1833,20 @@ struct HeatBatch::Impl {
             if (I->mg_on) mi.push_back(I);
         if (mi.empty()) return;
         auto m = std::make_unique<Table::Mg>();
+        for (Inst* I : mi) m->levels = std::max(m->levels, (int)I->mg.size() + 1);
+        m->sep.assign(m->levels, 0);
+        std::vector<Desc> td;
+        std::vector<int2> tr;
         for (Inst* I : mi) {
-            m->levels = std::max(m->levels, (int)I->mg.size() + 1);
-            m->shallow |= I->mg.empty();
+            for (int l = 0; l < I->mg_tail; ++l) m->sep[l] = 1;
+            m->tail0 |= I->mg_tail == 0;
+            const int first = (int)td.size();
+            for (int l = I->mg_tail; l <= (int)I->mg.size(); ++l) td.push_back(l == 0 ? desc_of(*I) : level_desc(*I, l));
+            tr.push_back(make_int2(first, (int)td.size() - first));
         }
+        m->tail = dev_from_host(td.data(), td.size());
+        m->trange = dev_from_host(tr.data(), tr.size());
+        m->ntail = (int)tr.size();
         m->fine.resize(m->levels);
         m->coarse.resize(m->levels);
         m->cblk.resize(m->levels);
@@ -1868,6 +1937,7 @@ struct HeatBatch::Impl {
         d.cgp = cg_capable(I);
         d.mgp = I.mg_on;
         d.mgl = (int)I.mg.size();
+        d.mgt = I.mg_on && I.mg_tail == 0;
         d.minv = I.mg_on && I.mg.empty() ? (void*)I.minv->p : nullptr;
         for (int q = 0; q < 7; ++q) d.K[q] = q < I.ncoef() ? I.kcoef->p + q * 2 * h * ks() : nullptr;
         d.dinv = I.dinv->p;
@@ -2144,7 +2214,7 @@ struct HeatBatch::Impl {
                 EQ_HIP(hipGetLastError());
             }
         };
-        for (int l = 0; l + 1 < m.levels; ++l) {
+        for (int l = 0; l + 1 < m.levels && m.sep[l]; ++l) {
             smooth(l, 0, 1, 0);
             smooth(l, 1, 0, 0);
             if (l == 0) {
@@ -2158,16 +2228,17 @@ struct HeatBatch::Impl {
                                m.coarse[l + 1]->p, m.cblk[l + 1]->p);
             EQ_HIP(hipGetLastError());
         }
-        hipLaunchKernelGGL((k_mg_dense_solve<T>), dim3((unsigned)m.ncoarsest), dim3(MG_BS), 0, S(), m.coarsest->p);
+        hipLaunchKernelGGL((k_mg_tail<T>), dim3((unsigned)m.ntail), dim3(TAIL_BS), 0, S(), m.tail->p, m.trange->p);
         EQ_HIP(hipGetLastError());
         for (int l = m.levels - 2; l >= 0; --l) {
+            if (!m.sep[l]) continue;
             hipLaunchKernelGGL((k_mg_prolong<T>), dim3((unsigned)m.nfpb[l + 1]), dim3(BS), 0, S(), m.fine[l + 1]->p,
                                m.coarse[l + 1]->p, m.fpblk[l + 1]->p);
             EQ_HIP(hipGetLastError());
             smooth(l, 1, 0, 0);
             smooth(l, 0, 0, l == 0);
         }
-        if (m.shallow) HLP(C_VEC, (k_cg_dots<T, CP>), t.part->p, t.ng);
+        if (m.tail0) HLP(C_VEC, (k_cg_dots<T, CP>), t.part->p, t.ng);
     }
     // z = M^-1 r for the full-system ILU(0) instances
     template <class T>
@@ -2437,6 +2508,8 @@ static void alloc_mg(Inst& I, size_t ks) {
     }
     EQ_REQUIRE(n <= MG_COARSE, "multigrid coarsest level of " << n << " cells");
     I.minv = dev_alloc<char>((size_t)n * n * ks);
+    I.mg_tail = 0;
+    while (I.mg_tail < (int)I.mg.size() && (I.mg_tail == 0 ? I.n : I.mg[I.mg_tail - 1].n) > MG_TAIL) ++I.mg_tail;
 }
 
 int HeatBatch::add(int nx, int ny, double k, double cv, const double* u_host, double dx, double dy) {

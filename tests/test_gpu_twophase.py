@@ -248,6 +248,38 @@ def test_fused_twophase_steps_match_oracle(hip_api, oracle_api, params, newton_t
     assert np.allclose(ph - ph[-1], po - po[-1], rtol=bar, atol=bar * np.abs(po).max())
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_multigrid_levels_above_the_tail_match_bicgstab(hip_api, precision):
+    """24x20x16 (7680 cells): the fine level runs the one-level multigrid kernels, the
+    levels from 12x10x8 down the workgroup tail.  CG with the V-cycle and BiCGStab with
+    the red-black ILU(0) reach the same steps."""
+    nx, ny, nz = 24, 20, 16
+    perm, poro, sw0, sc, sv = case(nx, ny, nz, seed=3)
+    n = perm.size
+    out = {}
+    for mg in (1, 0):
+        b = fused_batch(hip_api, f"precision={precision} preconditioner=ilu0 rel_res_tol=1e-10 abs_res_tol=0 "
+                                 f"solver_tol=1e-10 max_iter=40 solver_max_iter=5000 twophase_pressure_mg={mg} "
+                                 "krylov_form_twophase=full" + ("" if mg else " twophase_pressure_cg=0"))
+        try:
+            slot = fused_add(hip_api, b, nx, ny, nz, perm, poro, sw0, sc, sv)
+            st, tps = np.zeros(5, dtype=np.int32), np.zeros(3, dtype=np.int32)
+            for dt in (1.0, 1.0):
+                hip_api.batch_step(b, iptr(np.array([slot], dtype=np.int32)), 1, dt)
+                hip_api.batch_stats(b, slot, iptr(st), None)
+                hip_api.batch_twophase_stats(b, slot, iptr(tps))
+                assert st[3] in (1, 2) and tps[2] in (1, 2) and st[4] == 0, (mg, st, tps)
+            ph, sh = np.zeros(n), np.zeros(n)
+            hip_api.batch_twophase_state(b, slot, dptr(ph), dptr(sh))
+            out[mg] = (ph - ph.mean(), sh, int(tps[1]))
+        finally:
+            hip_api.batch_destroy(b)
+    (p1, s1, it1), (p0, s0, it0) = out[1], out[0]
+    assert np.allclose(s1, s0, rtol=1e-7, atol=1e-8)
+    assert np.allclose(p1, p0, rtol=1e-7, atol=1e-7 * np.abs(p0).max())
+    assert it1 < it0, (it1, it0)   # fewer pressure iterations with the V-cycle
+
+
 @pytest.mark.parametrize("precision", ["mixed", "fp64"])   # fp64: both kinds in the reduced form
 def test_fused_twophase_and_heat_share_a_batch(hip_api, precision):
     """Two-phase and heat instances in one batch give bit-identical results to each run
