@@ -1789,10 +1789,12 @@ struct HeatBatch::Impl {
     std::map<std::vector<int>, TableP> tcache;
     bool use_graph = true;
     static constexpr int NRING = 4;
-    DevPtr<int> count;      // NRING x {active count, max Krylov iterations}
+    // NRING x {active count, max Krylov iterations} and the gathered per-instance scalars,
+    // in pinned host memory the kernels store into directly (visible once the stream or the
+    // event has completed): a host check costs the kernel and the wait, no copy launch
     int* host_count = nullptr;
     hipEvent_t ev[NRING] = {};
-    DevPtr<double> gbuf;    // gathered per-instance scalars
+    double* gpin = nullptr;
     size_t gcap = 0;
 
     size_t ks() const { return mixed ? sizeof(float) : sizeof(double); }
@@ -2098,9 +2100,8 @@ struct HeatBatch::Impl {
 
     // out[0] = active instances (flag `which`), out[1] = max Krylov iterations, synchronously
     int count_active(const Table& t, int which, int* maxit = nullptr) {
-        hipLaunchKernelGGL(k_count_active, dim3(1), dim3(BS), 0, S(), t.desc->p, (int)t.inst.size(), which, count->p);
+        hipLaunchKernelGGL(k_count_active, dim3(1), dim3(BS), 0, S(), t.desc->p, (int)t.inst.size(), which, host_count);
         EQ_HIP(hipGetLastError());
-        EQ_HIP(hipMemcpyAsync(host_count, count->p, 2 * sizeof(int), hipMemcpyDeviceToHost, S()));
         Device::get().sync();
         if (maxit) *maxit = host_count[1];
         return host_count[0];
@@ -2108,10 +2109,8 @@ struct HeatBatch::Impl {
     // The same without stalling the queue: the result of ring slot k is read later.
     void count_async(const Table& t, int which, int slot) {
         hipLaunchKernelGGL(k_count_active, dim3(1), dim3(BS), 0, S(), t.desc->p, (int)t.inst.size(), which,
-                           count->p + 2 * slot);
+                           host_count + 2 * slot);
         EQ_HIP(hipGetLastError());
-        EQ_HIP(hipMemcpyAsync(host_count + 2 * slot, count->p + 2 * slot, 2 * sizeof(int), hipMemcpyDeviceToHost,
-                              S()));
         EQ_HIP(hipEventRecord(ev[slot], S()));
     }
     int count_wait(int slot, int* maxit = nullptr) {
@@ -2399,17 +2398,21 @@ struct HeatBatch::Impl {
 #undef HL
 #undef HLP
 
-    // per-instance scalars of four slots in one launch and one device-to-host copy:
+    // per-instance scalars of four slots in one launch, stored into pinned host memory:
     // result[q * ninst + i] = scalar slots[q] of instance i
     std::vector<double> gather(const Table& t, int4 slots) {
         const int ni = (int)t.inst.size();
         if (gcap < 4 * (size_t)ni) {
+            Device::get().sync();
+            if (gpin) EQ_HIP(hipHostFree(gpin));
+            gpin = nullptr;
             gcap = 4 * (size_t)ni;
-            gbuf = dev_alloc<double>(gcap);
+            EQ_HIP(hipHostMalloc((void**)&gpin, gcap * sizeof(double), hipHostMallocCoherent));
         }
-        hipLaunchKernelGGL(k_gather_slots, dim3((ni + 63) / 64), dim3(64), 0, S(), t.desc->p, ni, slots, gbuf->p);
+        hipLaunchKernelGGL(k_gather_slots, dim3((ni + 63) / 64), dim3(64), 0, S(), t.desc->p, ni, slots, gpin);
         EQ_HIP(hipGetLastError());
-        return dev_to_host(gbuf->p, 4 * (size_t)ni);
+        Device::get().sync();
+        return std::vector<double>(gpin, gpin + 4 * (size_t)ni);
     }
 };
 
@@ -2427,8 +2430,7 @@ HeatBatch::HeatBatch(const FusedConfig& cfg) : p_(new Impl()) {
                    cfg.krylov_form_twophase == "auto",
                "krylov_form_twophase must be reduced, full or auto");
     p_->use_graph = cfg.graph;
-    p_->count = dev_alloc<int>(2 * Impl::NRING);
-    EQ_HIP(hipHostMalloc((void**)&p_->host_count, 2 * Impl::NRING * sizeof(int), hipHostMallocDefault));
+    EQ_HIP(hipHostMalloc((void**)&p_->host_count, 2 * Impl::NRING * sizeof(int), hipHostMallocCoherent));
     for (int i = 0; i < Impl::NRING; ++i) EQ_HIP(hipEventCreateWithFlags(&p_->ev[i], hipEventDisableTiming));
 }
 
@@ -2437,6 +2439,7 @@ HeatBatch::~HeatBatch() {
     for (int i = 0; i < Impl::NRING; ++i)
         if (p_->ev[i]) (void)hipEventDestroy(p_->ev[i]);
     if (p_->host_count) (void)hipHostFree(p_->host_count);
+    if (p_->gpin) (void)hipHostFree(p_->gpin);
     delete p_;
 }
 
