@@ -542,6 +542,8 @@ def main():
     ap.add_argument("--requests", type=int, default=200, help="serving: requests per rate")
     ap.add_argument("--frac3d", type=float, default=0.3, help="serving: share of 3D two-phase requests")
     ap.add_argument("--hbm-budget-gb", dest="hbm_budget_gb", type=float, default=64.0)
+    ap.add_argument("--max-in-flight-twophase", dest="max_in_flight_twophase", type=int, default=4096,
+                    help="serving: two-phase requests in flight at once")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)

@@ -833,6 +833,7 @@ int eqhip_server_create(const char* params, void** server) {
         c.batch = fused_config(params);
         c.hbm_budget = m.get_double("hbm_budget", c.hbm_budget);
         c.max_in_flight = m.get_int("max_in_flight", c.max_in_flight);
+        c.max_in_flight_twophase = m.get_int("max_in_flight_twophase", c.max_in_flight_twophase);
         c.twophase_precision = m.get_string("twophase_precision", c.twophase_precision);
         EQ_REQUIRE(c.twophase_precision == "fp64" || c.twophase_precision == "mixed",
                    "twophase_precision must be fp64 or mixed");

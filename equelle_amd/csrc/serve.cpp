@@ -1,5 +1,7 @@
 #include "serve.hpp"
 
+#include <utility>
+
 #include "core.hpp"
 
 namespace eqhip {
@@ -20,6 +22,7 @@ HeatServer::HeatServer(const ServerConfig& cfg)
     : cfg_(cfg), batch_{HeatBatch(cfg.batch), HeatBatch(twophase_config(cfg))} {
     EQ_REQUIRE(cfg.hbm_budget > 0, "hbm_budget must be positive");
     EQ_REQUIRE(cfg.max_in_flight > 0, "max_in_flight must be positive");
+    EQ_REQUIRE(cfg.max_in_flight_twophase > 0, "max_in_flight_twophase must be positive");
 }
 
 int HeatServer::submit(int nx, int ny, double k, double cv, const double* u0, int steps, double dt) {
@@ -41,7 +44,7 @@ int HeatServer::submit(int nx, int ny, double k, double cv, const double* u0, in
     r.t_submit = Clock::now();
     const int id = r.id;
     reqs_.emplace(id, std::move(r));
-    queue_.push_back(id);
+    queue_[0].push_back(id);
     return id;
 }
 
@@ -71,32 +74,42 @@ int HeatServer::submit_twophase(const TwoPhaseSpec& spec, const double* perm, co
     r.t_submit = Clock::now();
     const int id = r.id;
     reqs_.emplace(id, std::move(r));
-    queue_.push_back(id);
+    queue_[1].push_back(id);
     return id;
 }
 
 std::vector<int> HeatServer::progress() {
-    // admission: FIFO while the head fits (no overtaking, so large requests cannot starve)
-    while (!queue_.empty() && in_flight() < cfg_.max_in_flight) {
-        Req& r = reqs_.at(queue_.front());
-        if (used_ + r.bytes > cfg_.hbm_budget) break;
-        const int cls = r.twophase ? 1 : 0;
-        if (r.twophase) {
-            r.slot = batch_[1].add_twophase(r.spec, r.perm.data(), r.poro.data(), r.u.data(),
-                                            (int)r.src_cells.size(), r.src_cells.data(), r.src_vals.data());
-            r.perm = std::vector<double>();
-            r.poro = std::vector<double>();
-        } else {
-            r.slot = batch_[0].add(r.nx, r.ny, r.k, r.cv, r.u.data());
+    // admission: FIFO per request kind while the head fits (no overtaking within a kind, so
+    // large requests cannot starve), the older of the two heads first; two-phase requests
+    // also up to their in-flight cap
+    for (bool admitted = true; admitted;) {
+        admitted = false;
+        int order[2] = {0, 1};
+        if (!queue_[0].empty() && !queue_[1].empty() && queue_[1].front() < queue_[0].front()) std::swap(order[0], order[1]);
+        for (int cls : order) {
+            if (queue_[cls].empty() || in_flight() >= cfg_.max_in_flight) continue;
+            if (cls == 1 && (int)running_[1].size() >= cfg_.max_in_flight_twophase) continue;
+            Req& r = reqs_.at(queue_[cls].front());
+            if (used_ + r.bytes > cfg_.hbm_budget) continue;
+            if (r.twophase) {
+                r.slot = batch_[1].add_twophase(r.spec, r.perm.data(), r.poro.data(), r.u.data(),
+                                                (int)r.src_cells.size(), r.src_cells.data(), r.src_vals.data());
+                r.perm = std::vector<double>();
+                r.poro = std::vector<double>();
+            } else {
+                r.slot = batch_[0].add(r.nx, r.ny, r.k, r.cv, r.u.data());
+            }
+            r.u.clear();
+            r.u.shrink_to_fit();
+            used_ += r.bytes;
+            r.t_admit = Clock::now();
+            r.info.queued_s = secs(r.t_submit, r.t_admit);
+            running_[cls][r.slot] = r.id;
+            batch_[cls].begin_step(r.slot, r.dt);
+            queue_[cls].pop_front();
+            admitted = true;
+            break;   // re-order the heads
         }
-        r.u.clear();
-        r.u.shrink_to_fit();
-        used_ += r.bytes;
-        r.t_admit = Clock::now();
-        r.info.queued_s = secs(r.t_submit, r.t_admit);
-        running_[cls][r.slot] = r.id;
-        batch_[cls].begin_step(r.slot, r.dt);
-        queue_.pop_front();
     }
     std::vector<int> finished;
     int cls;

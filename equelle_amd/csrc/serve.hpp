@@ -30,12 +30,15 @@ namespace eqhip {
 
 struct ServerConfig {
     FusedConfig batch;
-    // Krylov precision of the two-phase batch.  FP64 by default: the two-phase pressure
-    // system (singular, log-normal permeability) has eps32 * cond ~ 0.1-0.5, so FP32
-    // inner solves with FP64 refinement stagnate or diverge on part of the stream.
+    // Krylov precision of the two-phase batch.  FP64 by default: with the multigrid CG
+    // pressure solve both precisions converge, and FP64 needs fewer Krylov iterations.
     std::string twophase_precision = "fp64";
     double hbm_budget = 64e9;   // bytes of instance state admitted at once
-    int max_in_flight = 4096;
+    int max_in_flight = 4096;   // requests in flight, both kinds
+    // two-phase requests in flight at once (the rest wait in their FIFO queue): the batch
+    // advances its requests in lock step, so admitting fewer at a time lets the early ones
+    // finish early instead of every request sharing the device until the backlog ends
+    int max_in_flight_twophase = 4096;
 };
 
 struct RequestInfo {
@@ -59,7 +62,7 @@ public:
     // Copies a finished request's field (heat: u; two-phase: sw, and p when given; n
     // values each) and releases it.
     void take_result(int id, double* u, double* p, RequestInfo* info);
-    int queued() const { return (int)queue_.size(); }
+    int queued() const { return (int)(queue_[0].size() + queue_[1].size()); }
     int in_flight() const { return (int)(running_[0].size() + running_[1].size()); }
     double used_bytes() const { return used_; }
     const ServerConfig& config() const { return cfg_; }
@@ -82,7 +85,7 @@ private:
     HeatBatch batch_[2];                // [0] heat, [1] two-phase
     double busy_s_[2] = {0.0, 0.0};     // device time spent per class
     std::map<int, Req> reqs_;
-    std::deque<int> queue_;
+    std::deque<int> queue_[2];   // FIFO per request kind: heat, two-phase
     std::map<int, int> running_[2];     // slot -> request id, per class
     double used_ = 0;
     int next_id_ = 1;

@@ -164,8 +164,10 @@ def bench_serving(lib, args):
     rates = [float(r) for r in getattr(args, "rates", "50,100,200,500,1000,2000").split(",")]
     count = getattr(args, "requests", 200)
     frac3d = getattr(args, "frac3d", 0.3)
+    cap = getattr(args, "max_in_flight_twophase", 4096)
     params = (f"precision=mixed preconditioner={args.precond} rel_res_tol=1e-8 abs_res_tol=0 max_iter=20 "
-              f"solver_tol=1e-6 hbm_budget={getattr(args, 'hbm_budget_gb', 64.0) * 1e9}")
+              f"solver_tol=1e-6 hbm_budget={getattr(args, 'hbm_budget_gb', 64.0) * 1e9} "
+              f"max_in_flight_twophase={cap}")
     sweep = []
     for rate in rates:
         stream = poisson_stream(rate, count, seed=7, frac3d=frac3d)
@@ -193,6 +195,7 @@ def bench_serving(lib, args):
             "config": {"workload": f"Poisson request stream: {100 * (1 - frac3d):.0f}% 2D nonlinear heat (side "
                                    f"log-uniform 16..1024), {100 * frac3d:.0f}% 3D two-phase (side log-uniform "
                                    "16..128), 1-20 steps, dt=1, Newton to 1e-8 relative residual",
-                       "requests_per_rate": count, "hbm_budget_gb": getattr(args, "hbm_budget_gb", 64.0)},
+                       "requests_per_rate": count, "hbm_budget_gb": getattr(args, "hbm_budget_gb", 64.0),
+                       "max_in_flight_twophase": cap},
             "p50_ms": top["p50_ms"], "p99_ms": top["p99_ms"], "sweep": sweep}
     print(json.dumps(line), flush=True)

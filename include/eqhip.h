@@ -238,8 +238,10 @@ int eqhip_batch_residual(void* batch, int slot, const double* u, const double* u
  * iteration per progress() tick, and keep their state device-resident between steps.
  * No reference counterpart (the reference runs one program per process,
  * compiler/PrintCUDABackendASTVisitor.cpp:18-77); this is the request loop of the
- * serving workload.  params: the batch parameters above plus hbm_budget, max_in_flight and
- * twophase_precision (fp64|mixed, default fp64: Krylov precision of the two-phase requests). */
+ * serving workload.  params: the batch parameters above plus hbm_budget, max_in_flight,
+ * max_in_flight_twophase (two-phase requests in flight at once; default unlimited) and
+ * twophase_precision (fp64|mixed, default fp64: Krylov precision of the two-phase requests).
+ * Admission is FIFO per request kind (heat, two-phase), the older queue head first. */
 int eqhip_server_create(const char* params, void** server);
 int eqhip_server_destroy(void* server);
 int eqhip_server_submit(void* server, int nx, int ny, double k, double cv, const double* u0, int steps, double dt,
