@@ -446,6 +446,96 @@ __device__ inline Nb5 load_nb5(const double* __restrict__ u, int c, int i, int j
     return r;
 }
 
+// Four-ranks-per-lane form (Desc::vec: 2D heat, reduced system, nx % 8 == 0).  Lane l of
+// wave w holds the four consecutive pair ranks p0 .. p0 + 3, p0 = block start + g BS + 4 l,
+// of each group g = w, w + BS/64, ... < CPT of its block.  A row holds nx / 2 pair ranks, a
+// multiple of 4, so the four ranks lie in one row and every four-rank load is aligned (one
+// 16-byte load in FP32, two in FP64).  Rows and columns of the red-black layout then come
+// out of the loaded vectors: in row j the cells of the pairs p0 .. p0 + 3 are i = 2 q ..
+// 2 q + 7 (q = p0 - j nx / 2), so a cell's west and east neighbours (the other colour) are
+// ranks p - 1 and p when its column is even, p and p + 1 when it is odd, and its south and
+// north neighbours are ranks p -+ nx / 2.  Coefficients of absent neighbours are zero (the
+// assembly writes 0 there), so an in-bounds value of the same vector stands in for them.
+// A group's partial sum is the lane sums ((e0 + e1) + e2) + e3 reduced by a butterfly over
+// the wave: the same order for every CPT and block order.
+template <class T>
+struct V4 {
+    T e[4];
+};
+__device__ inline V4<float> ld4(const float* __restrict__ p) {
+    const float4 v = *reinterpret_cast<const float4*>(p);
+    return {{v.x, v.y, v.z, v.w}};
+}
+__device__ inline V4<double> ld4(const double* __restrict__ p) {
+    const double2 a = *reinterpret_cast<const double2*>(p), b = *reinterpret_cast<const double2*>(p + 2);
+    return {{a.x, a.y, b.x, b.y}};
+}
+__device__ inline void st4(float* __restrict__ p, const V4<float>& v) {
+    *reinterpret_cast<float4*>(p) = make_float4(v.e[0], v.e[1], v.e[2], v.e[3]);
+}
+__device__ inline void st4(double* __restrict__ p, const V4<double>& v) {
+    *reinterpret_cast<double2*>(p) = make_double2(v.e[0], v.e[1]);
+    *reinterpret_cast<double2*>(p + 2) = make_double2(v.e[2], v.e[3]);
+}
+#define FOR_VGROUPS(g, p0)                                                                          \
+    for (int g = (int)threadIdx.x >> 6, p0 = bi.y * (CPT * BS) + g * BS + 4 * ((int)threadIdx.x & 63); \
+         g < CPT; g += BS / 64, p0 += (BS / 64) * BS)
+__device__ inline double wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+// Group g's partial (lane 0 stores it when the group holds ranks of the instance; cells:
+// groups of cells, for the cell-order kernels).
+__device__ inline void vgroup_store(const Desc& d, int2 bi, int cpt, int g, double v, double* __restrict__ part,
+                                    bool cells = false) {
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0 && bi.y * cpt + g < (cells ? d.ngrp : d.ngrpk)) part[d.gbase + bi.y * cpt + g] = v;
+}
+template <class T>
+__device__ inline T dot4(const V4<T>& a, const V4<T>& b) {
+    return ((a.e[0] * b.e[0] + a.e[1] * b.e[1]) + a.e[2] * b.e[2]) + a.e[3] * b.e[3];
+}
+template <class T>
+__device__ inline double dot4d(const V4<T>& a, const V4<T>& b) {
+    return (((double)a.e[0] * (double)b.e[0] + (double)a.e[1] * (double)b.e[1]) + (double)a.e[2] * (double)b.e[2]) +
+           (double)a.e[3] * (double)b.e[3];
+}
+// Eight consecutive FP64 values (cells 2 p0 .. 2 p0 + 7 of a row, 64-byte aligned).
+__device__ inline void ld8(const double* __restrict__ p, double (&v)[8]) {
+    const V4<double> a = ld4(p), b = ld4(p + 4);
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        v[m] = a.e[m];
+        v[4 + m] = b.e[m];
+    }
+}
+// Row j and the row's pair offset q0 of rank p0 (2 p0 is the first cell of the pair).
+__device__ inline void vrow(const Desc& d, int p0, int& j, int& q0) {
+    int i0;
+    div_fix(2 * p0, d.rnx, d.nx, j, i0);
+    q0 = i0 >> 1;
+}
+// The off-diagonal sum W + E + S + N (coefficients kq at the ranks p0.., x the other
+// colour's vector) for the cells of colour col of row j.
+template <class T>
+__device__ inline V4<T> voffdiag(const Desc& d, const T* const* kq, int p0, int j, int q0, int col,
+                                 const T* __restrict__ x) {
+    const int hr = d.nx >> 1;
+    const V4<T> xc = ld4(x + p0);
+    const bool even = ((j + col) & 1) == 0;   // the cells of colour col have even columns i = 2 q
+    const T e = x[even ? (q0 > 0 ? p0 - 1 : p0) : (q0 + 4 < hr ? p0 + 4 : p0)];
+    const V4<T> xs = ld4(x + (j > 0 ? p0 - hr : p0)), xn = ld4(x + (j < d.ny - 1 ? p0 + hr : p0));
+    const V4<T> kw = ld4(kq[1] + p0), ke = ld4(kq[2] + p0), ks = ld4(kq[3] + p0), kn = ld4(kq[4] + p0);
+    V4<T> r;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        const T xw = even ? (m == 0 ? e : xc.e[m - 1]) : xc.e[m];
+        const T xe = even ? xc.e[m] : (m == 3 ? e : xc.e[m + 1]);
+        r.e[m] = kw.e[m] * xw + ke.e[m] * xe + ks.e[m] * xs.e[m] + kn.e[m] * xn.e[m];
+    }
+    return r;
+}
+
 // Residual + Jacobian assembly and per-group partial ||F||^2.
 //   heat:      reads u (5-point), u0; writes F and the 5 coefficients in the Krylov
 //              precision (FP32 mixed, FP64 fp64), red-black rank order
@@ -520,6 +610,45 @@ template <int CPT>
 __global__ __launch_bounds__(BS) void k_defect(const Desc* __restrict__ D, const int2* __restrict__ blk,
                                                double* __restrict__ part) {
     LOCATE();
+    if (d.vec) {   // heat, nx % 8 == 0: four consecutive cells of one row per lane
+        const bool on = d.act[1] != 0;
+        const int nx = d.nx, ny = d.ny;
+        const double* __restrict__ u = d.u;
+        const double* __restrict__ x = d.du;
+        FOR_VGROUPS(g, c0) {
+            double r2 = 0.0;
+            if (on && c0 < d.n) {
+                int j, i0;
+                div_fix(c0, d.rnx, nx, j, i0);
+                // absent neighbours: an in-bounds stand-in that heat_row and the sum below ignore
+                const int w = i0 > 0 ? c0 - 1 : c0, e = i0 + 4 < nx ? c0 + 4 : c0;
+                const int so = j > 0 ? c0 - nx : c0, no = j < ny - 1 ? c0 + nx : c0;
+                const V4<double> uc = ld4(u + c0), us = ld4(u + so), un = ld4(u + no);
+                const V4<double> xc = ld4(x + c0), xs = ld4(x + so), xn = ld4(x + no);
+                const V4<double> Fc = ld4(d.F + c0);
+                const double uw0 = u[w], ue3 = u[e], xw0 = x[w], xe3 = x[e];
+                V4<double> r;
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    const int i = i0 + m;
+                    const double uw = m == 0 ? uw0 : uc.e[m > 0 ? m - 1 : 0], ue = m == 3 ? ue3 : uc.e[m < 3 ? m + 1 : 3];
+                    const double xw = m == 0 ? xw0 : xc.e[m > 0 ? m - 1 : 0], xe = m == 3 ? xe3 : xc.e[m < 3 ? m + 1 : 3];
+                    double co[5];
+                    (void)heat_row(d, i, j, uc.e[m], uw, ue, us.e[m], un.e[m], 0.0, co);
+                    double acc = co[0] * xc.e[m];
+                    if (i > 0) acc += co[1] * xw;
+                    if (i < nx - 1) acc += co[2] * xe;
+                    if (j > 0) acc += co[3] * xs.e[m];
+                    if (j < ny - 1) acc += co[4] * xn.e[m];
+                    r.e[m] = Fc.e[m] - acc;
+                }
+                st4(d.r64 + c0, r);
+                r2 = ((r.e[0] * r.e[0] + r.e[1] * r.e[1]) + r.e[2] * r.e[2]) + r.e[3] * r.e[3];
+            }
+            vgroup_store(d, bi, CPT, g, r2, part, true);
+        }
+        return;
+    }
     double r2[CPT] = {};
     if (d.act[1]) {
         const int nx = d.nx, ny = d.ny;
@@ -633,85 +762,6 @@ __device__ inline T offdiag(const Desc& d, int s, int c, int i, int j, int k, co
         if (k < d.nz - 1) acc += ((const T*)d.K[6])[s] * x[(c + nxy) >> 1];
     }
     return acc;
-}
-
-// Four-ranks-per-lane form (Desc::vec: 2D heat, reduced system, nx % 8 == 0).  Lane l of
-// wave w holds the four consecutive pair ranks p0 .. p0 + 3, p0 = block start + g BS + 4 l,
-// of each group g = w, w + BS/64, ... < CPT of its block.  A row holds nx / 2 pair ranks, a
-// multiple of 4, so the four ranks lie in one row and every four-rank load is aligned (one
-// 16-byte load in FP32, two in FP64).  Rows and columns of the red-black layout then come
-// out of the loaded vectors: in row j the cells of the pairs p0 .. p0 + 3 are i = 2 q ..
-// 2 q + 7 (q = p0 - j nx / 2), so a cell's west and east neighbours (the other colour) are
-// ranks p - 1 and p when its column is even, p and p + 1 when it is odd, and its south and
-// north neighbours are ranks p -+ nx / 2.  Coefficients of absent neighbours are zero (the
-// assembly writes 0 there), so an in-bounds value of the same vector stands in for them.
-// A group's partial sum is the lane sums ((e0 + e1) + e2) + e3 reduced by a butterfly over
-// the wave: the same order for every CPT and block order.
-template <class T>
-struct V4 {
-    T e[4];
-};
-__device__ inline V4<float> ld4(const float* __restrict__ p) {
-    const float4 v = *reinterpret_cast<const float4*>(p);
-    return {{v.x, v.y, v.z, v.w}};
-}
-__device__ inline V4<double> ld4(const double* __restrict__ p) {
-    const double2 a = *reinterpret_cast<const double2*>(p), b = *reinterpret_cast<const double2*>(p + 2);
-    return {{a.x, a.y, b.x, b.y}};
-}
-__device__ inline void st4(float* __restrict__ p, const V4<float>& v) {
-    *reinterpret_cast<float4*>(p) = make_float4(v.e[0], v.e[1], v.e[2], v.e[3]);
-}
-__device__ inline void st4(double* __restrict__ p, const V4<double>& v) {
-    *reinterpret_cast<double2*>(p) = make_double2(v.e[0], v.e[1]);
-    *reinterpret_cast<double2*>(p + 2) = make_double2(v.e[2], v.e[3]);
-}
-#define FOR_VGROUPS(g, p0)                                                                          \
-    for (int g = (int)threadIdx.x >> 6, p0 = bi.y * (CPT * BS) + g * BS + 4 * ((int)threadIdx.x & 63); \
-         g < CPT; g += BS / 64, p0 += (BS / 64) * BS)
-__device__ inline double wave_sum(double v) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-// Group g's partial (lane 0 stores it when the group holds ranks of the instance).
-__device__ inline void vgroup_store(const Desc& d, int2 bi, int cpt, int g, double v, double* __restrict__ part) {
-    v = wave_sum(v);
-    if ((threadIdx.x & 63) == 0 && bi.y * cpt + g < d.ngrpk) part[d.gbase + bi.y * cpt + g] = v;
-}
-template <class T>
-__device__ inline T dot4(const V4<T>& a, const V4<T>& b) {
-    return ((a.e[0] * b.e[0] + a.e[1] * b.e[1]) + a.e[2] * b.e[2]) + a.e[3] * b.e[3];
-}
-template <class T>
-__device__ inline double dot4d(const V4<T>& a, const V4<T>& b) {
-    return (((double)a.e[0] * (double)b.e[0] + (double)a.e[1] * (double)b.e[1]) + (double)a.e[2] * (double)b.e[2]) +
-           (double)a.e[3] * (double)b.e[3];
-}
-// Row j and the row's pair offset q0 of rank p0 (2 p0 is the first cell of the pair).
-__device__ inline void vrow(const Desc& d, int p0, int& j, int& q0) {
-    int i0;
-    div_fix(2 * p0, d.rnx, d.nx, j, i0);
-    q0 = i0 >> 1;
-}
-// The off-diagonal sum W + E + S + N (coefficients kq at the ranks p0.., x the other
-// colour's vector) for the cells of colour col of row j.
-template <class T>
-__device__ inline V4<T> voffdiag(const Desc& d, const T* const* kq, int p0, int j, int q0, int col,
-                                 const T* __restrict__ x) {
-    const int hr = d.nx >> 1;
-    const V4<T> xc = ld4(x + p0);
-    const bool even = ((j + col) & 1) == 0;   // the cells of colour col have even columns i = 2 q
-    const T e = x[even ? (q0 > 0 ? p0 - 1 : p0) : (q0 + 4 < hr ? p0 + 4 : p0)];
-    const V4<T> xs = ld4(x + (j > 0 ? p0 - hr : p0)), xn = ld4(x + (j < d.ny - 1 ? p0 + hr : p0));
-    const V4<T> kw = ld4(kq[1] + p0), ke = ld4(kq[2] + p0), ks = ld4(kq[3] + p0), kn = ld4(kq[4] + p0);
-    V4<T> r;
-#pragma unroll
-    for (int m = 0; m < 4; ++m) {
-        const T xw = even ? (m == 0 ? e : xc.e[m - 1]) : xc.e[m];
-        const T xe = even ? xc.e[m] : (m == 3 ? e : xc.e[m + 1]);
-        r.e[m] = kw.e[m] * xw + ke.e[m] * xe + ks.e[m] * xs.e[m] + kn.e[m] * xn.e[m];
-    }
-    return r;
 }
 
 // Reduced operator, pass 1 (red ranks): y_r = C_r^-1 (A_rb v_b), v = V[iv] (black slots).
@@ -965,6 +1015,22 @@ template <class T, int CPT>
 __global__ __launch_bounds__(BS) void k_kinit(const Desc* __restrict__ D, const int2* __restrict__ blk,
                                               double* __restrict__ part, int ng) {
     LOCATE();
+    if (d.vec) {   // heat, reduced form (BiCGStab; not singular): rh = r, x = 0 (p and v are not
+                   // read before the first iteration writes them)
+        const bool on = d.act[1] != 0;
+        FOR_VGROUPS(g, p0) {
+            double rr = 0.0;
+            if (on && p0 < d.h) {
+                const V4<T> r = ld4((const T*)d.v[V_R] + p0);
+                st4((T*)d.v[V_RH] + p0, r);
+                st4((T*)d.v[V_X] + p0, V4<T>{{(T)0, (T)0, (T)0, (T)0}});
+                rr = dot4d(r, r);
+            }
+            vgroup_store(d, bi, CPT, g, rr, part);
+            vgroup_store(d, bi, CPT, g, rr, part + ng);
+        }
+        return;
+    }
     double rr[CPT] = {}, rz[CPT] = {};
     const bool cg = meth_of(d) == M_CG;
     if (d.act[1] && cg) {
@@ -1582,6 +1648,47 @@ __global__ __launch_bounds__(BS) void k_load_rhs(const Desc* __restrict__ D, con
     const int h = d.h, nx = d.nx, ny = d.ny, nxy = d.nx * d.ny;
     T* __restrict__ r = (T*)d.v[V_R];
     const T* __restrict__ dv = (const T*)d.dinv;
+    if (d.vec) {   // heat, reduced form: lanes of four black ranks (cells 2 p0 .. 2 p0 + 7 of row j)
+        const int hr = nx >> 1;
+        FOR_VGROUPS(g, p0) {
+            if (p0 >= h) continue;
+            int j, q0;
+            vrow(d, p0, j, q0);
+            const int c0 = 2 * p0;
+            const bool evb = (j & 1) != 0;   // black cells at the even columns of the row
+            double sc[8], ss[8], sn[8];
+            ld8(src + c0, sc);
+            ld8(src + (j > 0 ? c0 - nx : c0), ss);
+            ld8(src + (j < ny - 1 ? c0 + nx : c0), sn);
+            // the red cell just outside the segment (west of the first black cell or east of
+            // the last one), and its rank; absent at the grid edge (zero coefficient)
+            const int ce = evb ? (q0 > 0 ? c0 - 1 : c0) : (q0 + 4 < hr ? c0 + 8 : c0);
+            const int pe = evb ? (q0 > 0 ? p0 - 1 : p0) : (q0 + 4 < hr ? p0 + 4 : p0);
+            const T se = rhs_value<T>(src, ce, scale) * dv[pe];
+            const V4<T> dc = ld4(dv + p0), ds = ld4(dv + (j > 0 ? p0 - hr : p0)), dn = ld4(dv + (j < ny - 1 ? p0 + hr : p0));
+            const V4<T> kw = ld4((const T*)d.K[1] + h + p0), ke = ld4((const T*)d.K[2] + h + p0);
+            const V4<T> ks = ld4((const T*)d.K[3] + h + p0), kn = ld4((const T*)d.K[4] + h + p0);
+            V4<T> out;
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                const int bc = evb ? 2 * m : 2 * m + 1;   // the black cell's place in the segment
+                // red neighbours: west / east in the row, the cells above and below
+                const T w = evb ? (m == 0 ? se : (T)((T)(sc[bc > 0 ? bc - 1 : 0] * scale) * dc.e[m > 0 ? m - 1 : 0]))
+                                : (T)((T)(sc[bc - 1] * scale) * dc.e[m]);
+                const T e = evb ? (T)((T)(sc[bc + 1] * scale) * dc.e[m])
+                                : (m == 3 ? se : (T)((T)(sc[bc < 7 ? bc + 1 : 7] * scale) * dc.e[m < 3 ? m + 1 : 3]));
+                const T sv = (T)((T)(ss[bc] * scale) * ds.e[m]), nv = (T)((T)(sn[bc] * scale) * dn.e[m]);
+                T acc = (T)0;
+                acc += kw.e[m] * w;
+                acc += ke.e[m] * e;
+                acc += ks.e[m] * sv;
+                acc += kn.e[m] * nv;
+                out.e[m] = (T)(sc[bc] * scale) - acc;
+            }
+            st4(r + p0, out);
+        }
+        return;
+    }
     FOR_PAIRS(p) {
         int i, j, k;
         if (d.ns == 2) {
@@ -1634,6 +1741,43 @@ __global__ __launch_bounds__(BS) void k_accum(const Desc* __restrict__ D, const 
         if (first) du[c] = ok ? v : 0.0;
         else du[c] += v;
     };
+    if (d.vec) {   // heat, reduced form: lanes of four pairs (cells 2 p0 .. 2 p0 + 7 of row j)
+        const T* kq[5] = {nullptr, (const T*)d.K[1], (const T*)d.K[2], (const T*)d.K[3], (const T*)d.K[4]};
+        FOR_VGROUPS(g, p0) {
+            if (p0 >= h) continue;
+            const int c0 = 2 * p0;
+            double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            if (ok) {
+                int j, q0;
+                vrow(d, p0, j, q0);
+                const bool evr = (j & 1) == 0;   // red cells at the even columns of the row
+                // red unknowns recovered: x_r = C_r^-1 (b_r - A_rb x_b)
+                const V4<T> o = voffdiag<T>(d, kq, p0, j, q0, 0, x);
+                const V4<T> dc = ld4(dv + p0), xb = ld4(x + p0);
+                double sc[8];
+                ld8(src + c0, sc);
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    const int rc = evr ? 2 * m : 2 * m + 1;
+                    const T xr = ((T)(sc[rc] * scale) - o.e[m]) * dc.e[m];
+                    v[rc] = scaled ? rn * (double)xr : (double)xr;
+                    v[rc ^ 1] = scaled ? rn * (double)xb.e[m] : (double)xb.e[m];
+                }
+            }
+            V4<double> a{{v[0], v[1], v[2], v[3]}}, b{{v[4], v[5], v[6], v[7]}};
+            if (!first) {
+                const V4<double> a0 = ld4(du + c0), b0 = ld4(du + c0 + 4);
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    a.e[m] += a0.e[m];
+                    b.e[m] += b0.e[m];
+                }
+            }
+            st4(du + c0, a);
+            st4(du + c0 + 4, b);
+        }
+        return;
+    }
     FOR_PAIRS(p) {
         int i, j, k;
         const int cr = cell_of(d, p, 0, i, j, k);
@@ -1649,6 +1793,17 @@ __global__ __launch_bounds__(BS) void k_accum(const Desc* __restrict__ D, const 
 template <int CPT>
 __global__ __launch_bounds__(BS) void k_newton_update(const Desc* __restrict__ D, const int2* __restrict__ blk) {
     LOCATE();
+    if (d.vec) {   // heat, four consecutive cells per lane
+        FOR_VGROUPS(g, c0) {
+            if (c0 >= d.n) continue;
+            V4<double> a = ld4(d.u + c0);
+            const V4<double> b = ld4(d.du + c0);
+#pragma unroll
+            for (int m = 0; m < 4; ++m) a.e[m] -= b.e[m];
+            st4(d.u + c0, a);
+        }
+        return;
+    }
     double* x = d.kind == K_TWOPHASE ? (d.act[2] == 0 ? d.p : d.sw) : d.u;
     FOR_CELLS(c) x[c] -= d.du[c];
 }
