@@ -76,6 +76,7 @@ def run_fused(lib, precision, u0, nx, ny, dt, steps, warmup, precond="ilu0", gra
             log(f"  warmup {w}: newton={st[0]} krylov={st[1]} refine={st[2]} {time.perf_counter() - t:.4f}s")
         lib.device_sync()
         newton = krylov = refine = 0
+        lines = []   # printed after the timed steps
         t0 = time.perf_counter()
         for s in range(steps):
             t = time.perf_counter()
@@ -86,10 +87,11 @@ def run_fused(lib, precision, u0, nx, ny, dt, steps, warmup, precond="ilu0", gra
             newton += int(st[0])
             krylov += int(st[1])
             refine += int(st[2])
-            log(f"  step {s}: newton={st[0]} krylov={st[1]} refine={st[2]} rel={nrm[1] / nrm[0]:.2e} "
-                f"{time.perf_counter() - t:.4f}s")
+            lines.append((s, int(st[0]), int(st[1]), int(st[2]), nrm[1] / nrm[0], time.perf_counter() - t))
         lib.device_sync()
         secs = time.perf_counter() - t0
+        for s, nw, kr, rf, rel, dtm in lines:
+            log(f"  step {s}: newton={nw} krylov={kr} refine={rf} rel={rel:.2e} {dtm:.4f}s")
         # kernel durations on the final state, every instance active: 50 launches of each
         # stencil kernel over the whole grid, each between its own HIP start/stop events on
         # the runtime stream (avg), then 50 more back to back (dispatch gaps included)
@@ -284,6 +286,7 @@ def bench_twophase(lib, args):
             lib.device_sync()
             newton = krylov = kp = 0
             excluded = 0.0
+            lines = []   # printed after the timed steps
             t0 = time.perf_counter()
             for s_ in range(args.steps):
                 lib.batch_step(b, iptr(sl), 1, dt)
@@ -294,8 +297,8 @@ def bench_twophase(lib, args):
                 newton += int(st[0])
                 krylov += int(st[1])
                 kp += int(tps[1])
-                log(f"[bench] two-phase {name} step {s_}: Newton p/s {tps[0]}/{st[0] - tps[0]}, "
-                    f"Krylov p/s {tps[1]}/{st[1] - tps[1]}")
+                lines.append(f"[bench] two-phase {name} step {s_}: Newton p/s {tps[0]}/{st[0] - tps[0]}, "
+                             f"Krylov p/s {tps[1]}/{st[1] - tps[1]}")
                 if s_ == gsteps - 1:   # the state the generic legs end in (they run gsteps steps)
                     tr = time.perf_counter()
                     pg, sw_g = np.zeros(n), np.zeros(n)
@@ -303,6 +306,8 @@ def bench_twophase(lib, args):
                     excluded += time.perf_counter() - tr   # the copy is not part of the step
             lib.device_sync()
             secs = time.perf_counter() - t0 - excluded
+            for ln in lines:
+                log(ln)
             p, sw = np.zeros(n), np.zeros(n)
             lib.batch_twophase_state(b, slot.value, dptr(p), dptr(sw))
           except RuntimeError as e:   # FP32 Krylov on this pressure system: see the docstring
