@@ -22,7 +22,7 @@ inline hipStream_t S() { return Device::get().stream(); }
 // per-instance scalars; S_TOL2: the square of the instance's inner (mixed-precision) Krylov tolerance
 // S_MEAN: the mean of the Krylov right-hand side over the cells (singular systems only)
 enum Slot { S_RHO = 0, S_RHO_OLD, S_ALPHA, S_OMEGA, S_RR, S_BB, S_BREAK, S_IT, S_FN, S_RN, S_BN, S_REF, S_ITT, S_TOL2,
-            S_MEAN, S_BETA, NSLOT = 16 };
+            S_MEAN, S_BETA, S_CONT, NSLOT = 20 };
 // Krylov vectors; V_Y is the red-rank scratch of the reduced operator (ilu0 only), V_Z the
 // V-cycle's z = M^-1 r (two-phase pressure by CG with multigrid)
 enum Vec { V_R = 0, V_RH, V_P, V_V, V_S, V_T, V_PH, V_SH, V_X, V_Y, V_Z, NVEC };
@@ -1606,15 +1606,18 @@ __global__ void k_count_active(const Desc* __restrict__ D, int ninst, int which,
 }
 
 // refinement bookkeeping (one thread per instance):
-//   mode 0: start  -> act[1] = ||F|| > 0; S_BN = S_RN = ||F||; counters reset
+//   mode 0: start  -> act[1] = ||F|| > 0; S_BN = S_RN = ||F||; counters reset (skipped while
+//           S_CONT marks a refinement that continues from the previous round)
 //   mode 1: after a defect: S_RN = sqrt(S_FN); act[1] = S_RN > tol*S_BN && S_REF < max_refine
-//           and the defect decreased (stagnation stops the refinement)
+//           and the defect decreased (stagnation stops the refinement); S_CONT = act[1]: the
+//           Newton update waits for the next round
 __global__ void k_refine_flags(const Desc* __restrict__ D, int ninst, int mode, double tol, int max_refine) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ninst) return;
     const Desc& d = D[i];
     double* sc = d.sc;
     if (mode == 0) {
+        if (sc[S_CONT] != 0.0) return;   // refinement of the current Newton iteration continues
         sc[S_BN] = sqrt(sc[S_FN]);
         sc[S_RN] = sc[S_BN];
         sc[S_REF] = 0.0;
@@ -1627,6 +1630,7 @@ __global__ void k_refine_flags(const Desc* __restrict__ D, int ninst, int mode, 
         sc[S_RN] = rn;
         sc[S_REF] += 1.0;
         d.act[1] = isfinite(rn) && progress && rn > tol * sc[S_BN] && sc[S_REF] < max_refine;
+        sc[S_CONT] = d.act[1] ? 1.0 : 0.0;   // another round before the Newton update
     }
 }
 
@@ -1644,7 +1648,8 @@ __global__ __launch_bounds__(BS) void k_load_rhs(const Desc* __restrict__ D, con
     LOCATE();
     if (!d.act[1]) return;
     const double scale = scaled ? 1.0 / d.sc[S_RN] : 1.0;
-    const double* __restrict__ src = first ? d.F : d.r64;
+    const bool fst = first < 0 ? d.sc[S_REF] == 0.0 : first != 0;   // first < 0: the instance's own round
+    const double* __restrict__ src = fst ? d.F : d.r64;
     const int h = d.h, nx = d.nx, ny = d.ny, nxy = d.nx * d.ny;
     T* __restrict__ r = (T*)d.v[V_R];
     const T* __restrict__ dv = (const T*)d.dinv;
@@ -1727,18 +1732,19 @@ template <class T, int CPT>
 __global__ __launch_bounds__(BS) void k_accum(const Desc* __restrict__ D, const int2* __restrict__ blk, int scaled,
                                               int first) {
     LOCATE();
+    const bool first_round = first < 0 ? d.sc[S_REF] == 0.0 : first != 0;   // first < 0: the instance's own round
     const bool ok = d.act[1] && isfinite(d.sc[S_RR]);
-    if (!ok && !first) return;
+    if (!ok && !first_round) return;
     const double rn = d.sc[S_RN];
     const double scale = scaled ? 1.0 / rn : 1.0;
-    const double* __restrict__ src = first ? d.F : d.r64;
+    const double* __restrict__ src = first_round ? d.F : d.r64;
     const int h = d.h;
     const T* __restrict__ x = (const T*)d.v[V_X];
     const T* __restrict__ dv = (const T*)d.dinv;
     double* __restrict__ du = d.du;
     auto put = [&](int c, T xc) {
         const double v = scaled ? rn * (double)xc : (double)xc;
-        if (first) du[c] = ok ? v : 0.0;
+        if (first_round) du[c] = ok ? v : 0.0;
         else du[c] += v;
     };
     if (d.vec) {   // heat, reduced form: lanes of four pairs (cells 2 p0 .. 2 p0 + 7 of row j)
@@ -1765,7 +1771,7 @@ __global__ __launch_bounds__(BS) void k_accum(const Desc* __restrict__ D, const 
                 }
             }
             V4<double> a{{v[0], v[1], v[2], v[3]}}, b{{v[4], v[5], v[6], v[7]}};
-            if (!first) {
+            if (!first_round) {
                 const V4<double> a0 = ld4(du + c0), b0 = ld4(du + c0 + 4);
 #pragma unroll
                 for (int m = 0; m < 4; ++m) {
@@ -1793,6 +1799,7 @@ __global__ __launch_bounds__(BS) void k_accum(const Desc* __restrict__ D, const 
 template <int CPT>
 __global__ __launch_bounds__(BS) void k_newton_update(const Desc* __restrict__ D, const int2* __restrict__ blk) {
     LOCATE();
+    if (d.sc[S_CONT] != 0.0) return;   // the refinement continues: no update yet
     if (d.vec) {   // heat, four consecutive cells per lane
         FOR_VGROUPS(g, c0) {
             if (c0 >= d.n) continue;
@@ -1807,8 +1814,8 @@ __global__ __launch_bounds__(BS) void k_newton_update(const Desc* __restrict__ D
     double* x = d.kind == K_TWOPHASE ? (d.act[2] == 0 ? d.p : d.sw) : d.u;
     FOR_CELLS(c) x[c] -= d.du[c];
 }
-// out[q * ninst + i] = scalar slots.x, .y, .z, .w (q = 0..3) of instance i
-__global__ void k_gather_slots(const Desc* __restrict__ D, int ninst, int4 slots, double* __restrict__ out) {
+// out[q * ninst + i] = scalar slots.x, .y, .z, .w, s4 (q = 0..4) of instance i
+__global__ void k_gather_slots(const Desc* __restrict__ D, int ninst, int4 slots, int s4, double* __restrict__ out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ninst) return;
     const double* sc = D[i].sc;
@@ -1816,6 +1823,7 @@ __global__ void k_gather_slots(const Desc* __restrict__ D, int ninst, int4 slots
     out[ninst + i] = sc[slots.y];
     out[2 * ninst + i] = sc[slots.z];
     out[3 * ninst + i] = sc[slots.w];
+    out[4 * ninst + i] = sc[s4];
 }
 #undef FOR_GROUPS
 #undef FOR_CELLS
@@ -1835,6 +1843,7 @@ struct Inst {
     int corey = 1;
     int stage = 0;                 // 0 pressure, 1 transport
     double r0_stage = 0;           // ||F|| at the start of the current Newton solve
+    int round = 0;                 // refinement round of the current Newton iteration (host count)
     DevPtr<double> tp;             // tx, ty, tz, pv, src (n each)
     DevPtr<double> p, sw, sw0;
     DevPtr<double> u, u0, F, du, r64, C, W, E, S, N, B, T, sc;
@@ -1921,9 +1930,11 @@ struct Table {
     int solves = 0;   // Krylov solves run on this membership (graphs are captured from the second)
     // captured BiCGStab iteration, and the iterations the last solve needed, keyed by
     // (tolerance, max iterations)
-    // keyed by (tolerance, iteration cap, Krylov methods present)
+    // keyed by (tolerance, iteration cap, Krylov methods present); the predictions also by
+    // the position of the solve in its time step (Newton iteration, refinement round), whose
+    // iteration counts repeat from step to step
     std::map<std::tuple<double, int, int>, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
-    std::map<std::tuple<double, int, int>, int> pred;
+    std::map<std::tuple<double, int, int, int>, int> pred;
     ~Table() {
         for (auto& g : graphs) {
             (void)hipGraphExecDestroy(g.second.second);
@@ -2214,8 +2225,10 @@ struct HeatBatch::Impl {
     }
     Shape tune_shape(const std::vector<Inst*>& list) {
         // every instance active so the kernels do their full work.  Trial launches only
-        // write what the next solve re-initialises (Krylov vectors, du, the refinement
-        // residual, partials) or rewrite F and the coefficients from the current state.
+        // write what the next refinement round re-initialises (Krylov vectors, partials) or
+        // rewrite F, the coefficients and the refinement residual from the current u and du
+        // (unchanged values); du itself, which an instance between refinement rounds keeps,
+        // is not written.
         const int one[2] = {1, 1};
         for (Inst* I : list) EQ_HIP(hipMemcpyAsync(I->act->p, one, sizeof(one), hipMemcpyHostToDevice, S()));
         static const double weight[NCLS] = {1.0, 2.0, 16.0, 2.0, 8.0};
@@ -2406,10 +2419,11 @@ struct HeatBatch::Impl {
         launch_op<T>(t, ix, iy, nd, a0, a1, meth);
         fin(t, f);
     }
+    // the conversion class is timed on the right-hand side kernel alone: k_accum adds into
+    // du, which must survive between the refinement rounds of an instance
     template <class T>
     void conversions(const Table& t) {
         HLP(C_PC, (k_load_rhs<T, CP>), 1, 0);
-        HLP(C_PC, (k_accum<T, CP>), 1, 0);
     }
     template <class T>
     void vec_round(const Table& t) {
@@ -2466,6 +2480,9 @@ struct HeatBatch::Impl {
         HLP(C_VEC, (k_kinit<T, CP>), t.part->p, t.ng);
         fin(t, F(A_INIT, 2));
         const int meth = methods(t);
+        int at = 0;   // the solve's position in the step (the latest instance's)
+        for (const Inst* I : t.inst) at = std::max(at, 16 * I->st.newton_stage + std::min(I->round, 15));
+        const auto pkey = std::make_tuple(tol, maxit, meth, at);
         if (t.mg && (meth & M_CG)) {   // V-cycle instances: first z, rho = r.z, p = z
             vcycle_dots<T>(t);
             fin(t, F(A_CG_START, 1));
@@ -2485,7 +2502,7 @@ struct HeatBatch::Impl {
                 g = t.graphs.emplace(key, std::make_pair(graph, exec)).first;
             }
             const hipGraphExec_t exec = g->second.second;
-            const int blind = std::min(maxit, std::max(0, t.pred[key] - 1));
+            const int blind = std::min(maxit, std::max(0, t.pred[pkey] - 1));
             int k = 0;
             for (; k < blind; ++k) EQ_HIP(hipGraphLaunch(exec, S()));
             bool done = k >= maxit;
@@ -2511,7 +2528,7 @@ struct HeatBatch::Impl {
                 }
             }
         }
-        t.pred[key] = iters;
+        t.pred[pkey] = iters;
     }
 
     template <class T>
@@ -2528,16 +2545,17 @@ struct HeatBatch::Impl {
             HLP(C_PC, (k_accum<T, CP>), 0, 1);
             return;
         }
-        for (int ref = 0; ref < cfg.max_refine; ++ref) {
-            if (ref > 0 && !count_active(t, 1)) break;   // all start active (||F|| > 0)
-            HLP(C_PC, (k_load_rhs<T, CP>), 1, ref == 0);
-            bicgstab<T>(t, -1.0, cfg.solver_max_iter);   // per-instance inner tolerance (S_TOL2)
-            HLP(C_PC, (k_accum<T, CP>), 1, ref == 0);
-            launch_defect(t);
-            fin(t, Fcell(S_FN));
-            hipLaunchKernelGGL(k_refine_flags, dim3((ni + 63) / 64), dim3(64), 0, S(), t.desc->p, (int)ni, 1,
-                               cfg.solver_tol, cfg.max_refine);
-        }
+        // one refinement round per call: an instance whose refinement continues (S_CONT)
+        // skips the Newton update and runs its next round in the next call, so no host
+        // check is needed between rounds; the first round of an instance reads F and
+        // writes du (its own round counter decides)
+        HLP(C_PC, (k_load_rhs<T, CP>), 1, -1);
+        bicgstab<T>(t, -1.0, cfg.solver_max_iter);   // per-instance inner tolerance (S_TOL2)
+        HLP(C_PC, (k_accum<T, CP>), 1, -1);
+        launch_defect(t);
+        fin(t, Fcell(S_FN));
+        hipLaunchKernelGGL(k_refine_flags, dim3((ni + 63) / 64), dim3(64), 0, S(), t.desc->p, (int)ni, 1,
+                           cfg.solver_tol, cfg.max_refine);
     }
     template <class T>
     void precond_setup(const Table& t) {
@@ -2553,21 +2571,21 @@ struct HeatBatch::Impl {
 #undef HL
 #undef HLP
 
-    // per-instance scalars of four slots in one launch, stored into pinned host memory:
-    // result[q * ninst + i] = scalar slots[q] of instance i
-    std::vector<double> gather(const Table& t, int4 slots) {
+    // per-instance scalars of five slots in one launch, stored into pinned host memory:
+    // result[q * ninst + i] = scalar slot q of instance i
+    std::vector<double> gather(const Table& t, int4 slots, int s4) {
         const int ni = (int)t.inst.size();
-        if (gcap < 4 * (size_t)ni) {
+        if (gcap < 5 * (size_t)ni) {
             Device::get().sync();
             if (gpin) EQ_HIP(hipHostFree(gpin));
             gpin = nullptr;
-            gcap = 4 * (size_t)ni;
+            gcap = 5 * (size_t)ni;
             EQ_HIP(hipHostMalloc((void**)&gpin, gcap * sizeof(double), hipHostMallocCoherent));
         }
-        hipLaunchKernelGGL(k_gather_slots, dim3((ni + 63) / 64), dim3(64), 0, S(), t.desc->p, ni, slots, gpin);
+        hipLaunchKernelGGL(k_gather_slots, dim3((ni + 63) / 64), dim3(64), 0, S(), t.desc->p, ni, slots, s4, gpin);
         EQ_HIP(hipGetLastError());
         Device::get().sync();
-        return std::vector<double>(gpin, gpin + 4 * (size_t)ni);
+        return std::vector<double>(gpin, gpin + 5 * (size_t)ni);
     }
 };
 
@@ -2809,12 +2827,18 @@ std::vector<int> HeatBatch::iterate() {
     TableP tp = P.table(asmb);
     Table& t = *tp;
     P.assemble(t);
-    // one device-to-host copy per Newton iteration: ||F||^2 and the solve statistics
+    // one device-to-host copy per Newton iteration: ||F||^2, the solve statistics and
+    // whether the refinement continues (then u was not updated and F is unchanged)
     const size_t na = asmb.size();
-    std::vector<double> g = P.gather(t, make_int4(S_FN, S_ITT, S_REF, S_RN));
+    std::vector<double> g = P.gather(t, make_int4(S_FN, S_ITT, S_REF, S_RN), S_CONT);
     const FusedConfig& c = P.cfg;
     for (size_t q = 0; q < na; ++q) {
         Inst& I = *asmb[q];
+        if (I.state == Inst::SOLVE && g[4 * na + q] != 0.0) {   // next refinement round
+            ++I.round;
+            continue;
+        }
+        I.round = 0;
         if (I.state == Inst::SOLVE) {   // it was solved and updated in this call
             I.st.newton += 1;
             I.st.krylov += (int)g[na + q];
@@ -2868,8 +2892,8 @@ void HeatBatch::step(const std::vector<int>& slots, double dt) {
         bool any = false;
         for (int s : slots) any |= in_flight(s);
         if (!any) break;
-        iterate();
-        EQ_REQUIRE(++guard < 10 * (p_->cfg.max_iter + 2), "batch step did not terminate");
+        iterate();   // a Newton iteration takes one call per refinement round
+        EQ_REQUIRE(++guard < 4 * (p_->cfg.max_iter + 2) * (p_->cfg.max_refine + 2), "batch step did not terminate");
     }
 }
 

@@ -111,8 +111,9 @@ public:
     int size_of(int slot) const;
     // Starts one implicit time step of length dt for the slot.
     void begin_step(int slot, double dt);
-    // Advances every in-flight slot by one Newton iteration (continuous batching).
-    // Returns the slots whose step finished in this call.
+    // Advances every in-flight slot by one Newton iteration (continuous batching); in mixed
+    // precision a slot whose FP64 refinement needs another round runs that round instead
+    // and takes its Newton update in a later call.  Returns the slots whose step finished.
     std::vector<int> iterate();
     bool in_flight(int slot) const;
     const StepStats& stats(int slot) const;

@@ -201,7 +201,8 @@ int eqhip_batch_twophase_residual(void* batch, int slot, int stage, const double
 int eqhip_batch_remove(void* batch, int slot);
 /* starts one implicit step of length dt for the slot (u0 := u) */
 int eqhip_batch_begin_step(void* batch, int slot, double dt);
-/* one Newton iteration for every slot with a step in flight; returns the slots that finished */
+/* one Newton iteration for every slot with a step in flight (mixed precision: or one more
+ * refinement round of its linear solve); returns the slots that finished */
 int eqhip_batch_iterate(void* batch, int* finished, int cap, int* nfinished);
 int eqhip_batch_in_flight(void* batch, int slot, int* flag);
 /* whole steps for the listed slots */
