@@ -2488,22 +2488,31 @@ struct HeatBatch::Impl {
             fin(t, F(A_CG_START, 1));
             HLP(C_VEC, (k_cg3<T, CP>), 1);
         }
-        const auto key = std::make_tuple(tol, maxit, meth);
         int iters = 0;
         if (use_graph && !timing_enabled() && t.solves++ > 0) {
-            auto g = t.graphs.find(key);
-            if (g == t.graphs.end()) {
-                hipGraph_t graph;
-                hipGraphExec_t exec;
-                EQ_HIP(hipStreamBeginCapture(S(), hipStreamCaptureModeThreadLocal));
-                krylov_iteration<T>(t, tol, maxit, meth);
-                EQ_HIP(hipStreamEndCapture(S(), &graph));
-                EQ_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-                g = t.graphs.emplace(key, std::make_pair(graph, exec)).first;
-            }
-            const hipGraphExec_t exec = g->second.second;
+            // graphs of one and of two iterations (the blind launches go in pairs: one
+            // inter-graph gap per two iterations)
+            auto graph_of = [&](int reps) {
+                const auto gk = std::make_tuple(tol, maxit, meth + 8 * reps);
+                auto g = t.graphs.find(gk);
+                if (g == t.graphs.end()) {
+                    hipGraph_t graph;
+                    hipGraphExec_t exec;
+                    EQ_HIP(hipStreamBeginCapture(S(), hipStreamCaptureModeThreadLocal));
+                    for (int r = 0; r < reps; ++r) krylov_iteration<T>(t, tol, maxit, meth);
+                    EQ_HIP(hipStreamEndCapture(S(), &graph));
+                    EQ_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+                    g = t.graphs.emplace(gk, std::make_pair(graph, exec)).first;
+                }
+                return g->second.second;
+            };
+            const hipGraphExec_t exec = graph_of(1);
             const int blind = std::min(maxit, std::max(0, t.pred[pkey] - 1));
             int k = 0;
+            if (blind >= 2) {
+                const hipGraphExec_t exec2 = graph_of(2);
+                for (; k + 2 <= blind; k += 2) EQ_HIP(hipGraphLaunch(exec2, S()));
+            }
             for (; k < blind; ++k) EQ_HIP(hipGraphLaunch(exec, S()));
             bool done = k >= maxit;
             if (!done) {
