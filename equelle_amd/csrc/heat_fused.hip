@@ -206,20 +206,39 @@ __device__ inline void cell_ijk(const Desc& d, int c, int& i, int& j, int& k) {
 __device__ void fin_body(const Desc& d, const double* part, int nb, const Fin& f) {
     const int2 r = make_int2(d.gbase, d.gbase + (f.kry ? d.ngrpk : d.ngrp));
     __shared__ double res[2];
-    // partials were written by the previous kernel (visible at this kernel's start);
-    // four independent sums per thread keep four loads in flight
-    for (int q = 0; q < f.nd; ++q) {
-        const double* __restrict__ pp = part + (long)q * nb;
-        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-        int b = r.x + (int)threadIdx.x;
-        for (; b + 3 * FBS < r.y; b += 4 * FBS) {
-            a0 += pp[b];
-            a1 += pp[b + FBS];
-            a2 += pp[b + 2 * FBS];
-            a3 += pp[b + 3 * FBS];
+    __shared__ double lds[2][FBS / 64];
+    // partials were written by the previous kernel (visible at this kernel's start); both
+    // dots' partials are loaded together, eight per thread, before any is added, so the
+    // kernel waits for memory once per 8 x FBS groups
+    double s0 = 0.0, s1 = 0.0;
+    for (int base = r.x; base < r.y; base += 8 * FBS) {
+        double v0[8], v1[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int b = base + (int)threadIdx.x + u * FBS;
+            v0[u] = b < r.y ? part[b] : 0.0;
+            v1[u] = (f.nd > 1 && b < r.y) ? part[(long)nb + b] : 0.0;
         }
-        for (; b < r.y; b += FBS) a0 += pp[b];
-        bred_store<FBS>((a0 + a1) + (a2 + a3), &res[q]);
+        s0 += ((v0[0] + v0[1]) + (v0[2] + v0[3])) + ((v0[4] + v0[5]) + (v0[6] + v0[7]));
+        s1 += ((v1[0] + v1[1]) + (v1[2] + v1[3])) + ((v1[4] + v1[5]) + (v1[6] + v1[7]));
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        s0 += __shfl_down(s0, off, 64);
+        s1 += __shfl_down(s1, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        lds[0][threadIdx.x >> 6] = s0;
+        lds[1][threadIdx.x >> 6] = s1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0.0, b = 0.0;
+        for (int i = 0; i < FBS / 64; ++i) {
+            a += lds[0][i];
+            b += lds[1][i];
+        }
+        res[0] = a;
+        res[1] = b;
     }
     if (threadIdx.x != 0) return;
     const int act = f.act, maxit = f.maxit, slot = f.slot;
