@@ -40,6 +40,7 @@ enum Kind { K_HEAT = 0, K_TWOPHASE = 1 };
 
 struct Desc {
     int n, nx, ny, nz;
+    int nxr;                      // real row length; heat rows may be padded to nx (see Inst::nxr)
     int xcd;                      // XCD-contiguous block order (same for all of a table)
     int kind;                     // K_HEAT | K_TWOPHASE
     int corey;
@@ -430,6 +431,11 @@ __device__ inline double tp_transport(const Desc& d, int c, int i, int j, int k,
 // to the one the assembly derives its Krylov-precision coefficients from.
 __device__ inline double heat_row(const Desc& d, int i, int j, double uc, double uw, double ue, double us,
                                   double un, double u0c, double (&co)[5]) {
+    if (i >= d.nxr) {   // padding cell (rows padded to the Krylov layout): identity row, F = 0
+        co[0] = 1.0;
+        co[1] = co[2] = co[3] = co[4] = 0.0;
+        return 0.0;
+    }
     const double k = d.k, coef = d.dt / (d.cv * d.vol);
     const double kc = k * (1.0 + uc * uc), dkc = 2.0 * k * uc;
     double div = 0.0, diag = 1.0;
@@ -443,7 +449,7 @@ __device__ inline double heat_row(const Desc& d, int i, int j, double uc, double
         cm = coef * g * (0.5 * (2.0 * k * um) * diff - kf);
     };
     if (i > 0) face(uw, d.gx, co[1]);
-    if (i < d.nx - 1) face(ue, d.gx, co[2]);
+    if (i < d.nxr - 1) face(ue, d.gx, co[2]);
     if (j > 0) face(us, d.gy, co[3]);
     if (j < d.ny - 1) face(un, d.gy, co[4]);
     co[0] = diag;
@@ -1856,7 +1862,14 @@ struct Inst {
     int id = 0;
     int kind = K_HEAT;
     int nx = 0, ny = 0, nz = 1, n = 0;
+    // heat instances in the four-ranks-per-lane form store rows padded to a multiple of 8
+    // cells (nx = the padded stride, nxr = the real row length; 0: not padded).  A padding
+    // cell has an identity row and zero right-hand side and is coupled to nothing, so its
+    // Krylov components stay exactly zero; the state crosses the boundary without padding.
+    int nxr = 0;
     double k = 0, cv = 1, dx = 1, dy = 1, dt = 0;
+    int real_nx() const { return nxr ? nxr : nx; }
+    int real_n() const { return real_nx() * ny * nz; }
     // two-phase
     double mw_inv = 0, mo_inv = 0;
     int corey = 1;
@@ -2077,6 +2090,7 @@ struct HeatBatch::Impl {
         Desc d{};
         d.n = I.n;
         d.nx = I.nx;
+        d.nxr = I.real_nx();
         d.ny = I.ny;
         d.nz = I.nz;
         d.kind = I.kind;
@@ -2691,6 +2705,30 @@ static void alloc_solver(Inst& I, bool mixed, size_t ks, double inner_tol) {
     I.vecs = dev_alloc<char>((size_t)NVEC * I.ns() * h * ks);
 }
 
+// Row stride of a heat instance: padded to a multiple of 8 cells where its Krylov kernels
+// run four pair ranks per lane (reduced form, pair_vectors).
+static int heat_stride(int nx, const FusedConfig& cfg) {
+    return cfg.pair_vectors && cfg.reduced_form(false) ? (nx + 7) / 8 * 8 : nx;
+}
+// Cell-order values of the real cells -> the instance's (padded) layout, zeros in the padding.
+static std::vector<double> pad_rows(const Inst& I, const double* v) {
+    std::vector<double> out((size_t)I.n, 0.0);
+    const int nr = I.real_nx(), rows = I.ny * I.nz;
+    for (int r = 0; r < rows; ++r) std::copy(v + (size_t)r * nr, v + (size_t)r * nr + nr, out.begin() + (size_t)r * I.nx);
+    return out;
+}
+// The instance's layout -> the real cells (cnt arrays of I.n values each, back to back).
+static std::vector<double> unpad_rows(const Inst& I, const std::vector<double>& v, int cnt = 1) {
+    if (!I.nxr) return v;
+    const int nr = I.nxr, rows = I.ny * I.nz;
+    std::vector<double> out((size_t)cnt * I.real_n());
+    for (int q = 0; q < cnt; ++q)
+        for (int r = 0; r < rows; ++r)
+            std::copy(v.begin() + (size_t)q * I.n + (size_t)r * I.nx, v.begin() + (size_t)q * I.n + (size_t)r * I.nx + nr,
+                      out.begin() + (size_t)q * I.real_n() + (size_t)r * nr);
+    return out;
+}
+
 // Multigrid hierarchy of a two-phase instance: halve every axis longer than one cell
 // (rounding up) until at most MG_COARSE cells are left.
 static void alloc_mg(Inst& I, size_t ks) {
@@ -2721,16 +2759,18 @@ int HeatBatch::add(int nx, int ny, double k, double cv, const double* u_host, do
     EQ_REQUIRE((long)nx * ny < (1L << 31) - CPT_MAX * BS, "instance too large");
     auto I = std::make_unique<Inst>();
     I->id = p_->next_id++;
-    I->nx = nx;
+    I->pcm = p_->form_of(K_HEAT);
+    const int nxp = heat_stride(nx, p_->cfg);
+    I->nx = nxp;
+    I->nxr = nxp != nx ? nx : 0;
     I->ny = ny;
-    I->n = nx * ny;
+    I->n = nxp * ny;
     I->k = k;
     I->cv = cv;
     I->dx = dx;
     I->dy = dy;
-    I->u = dev_from_host(u_host, I->n);
+    I->u = dev_from_host(pad_rows(*I, u_host).data(), I->n);
     I->u0 = dev_alloc<double>(I->n);
-    I->pcm = p_->form_of(K_HEAT);
     alloc_solver(*I, p_->mixed, p_->ks(), p_->cfg.inner_tol);
     const int id = I->id;
     p_->insts[id] = std::move(I);
@@ -2806,7 +2846,8 @@ void HeatBatch::remove(int slot) {
     p_->insts.erase(slot);
     ++p_->version;
 }
-int HeatBatch::size_of(int slot) const { return get_inst(p_, slot).n; }
+int HeatBatch::size_of(int slot) const { return get_inst(p_, slot).real_n(); }
+int HeatBatch::heat_cells(int nx, int ny, const FusedConfig& cfg) { return heat_stride(nx, cfg) * ny; }
 bool HeatBatch::in_flight(int slot) const {
     const Inst& I = get_inst(p_, slot);
     return I.state == Inst::START || I.state == Inst::SOLVE;
@@ -3047,7 +3088,7 @@ double HeatBatch::time_kernel(const std::string& name, int reps, double* event_m
 
 void HeatBatch::get_state(int slot, double* u) const {
     Inst& I = get_inst(p_, slot);
-    std::vector<double> h = dev_to_host(I.kind == K_HEAT ? I.u->p : I.sw->p, I.n);
+    std::vector<double> h = unpad_rows(I, dev_to_host(I.kind == K_HEAT ? I.u->p : I.sw->p, I.n));
     std::copy(h.begin(), h.end(), u);
 }
 void HeatBatch::get_twophase_state(int slot, double* p, double* sw) const {
@@ -3065,7 +3106,7 @@ void HeatBatch::get_twophase_state(int slot, double* p, double* sw) const {
 void HeatBatch::set_state(int slot, const double* u) {
     Inst& I = get_inst(p_, slot);
     EQ_REQUIRE(I.kind == K_HEAT, "set_state: slot " << slot << " is not a heat instance");
-    I.u = dev_from_host(u, I.n);
+    I.u = dev_from_host(pad_rows(I, u).data(), I.n);
     ++p_->version;
 }
 
@@ -3074,15 +3115,15 @@ void HeatBatch::set_state(int slot, const double* u) {
 // FP64 coefficients C, W, E, S, N (, B, T), n each.
 static void csr_from_stencil(const Inst& I, const std::vector<double>& co, std::vector<int>& rp,
                              std::vector<int>& ci, std::vector<double>& v) {
-    const size_t n = I.n;
+    const size_t n = I.real_n();   // co: the real cells' coefficients
     const double *C = co.data(), *W = C + n, *E = C + 2 * n, *Sx = C + 3 * n, *N = C + 4 * n;
     const double* Bt = I.nz > 1 ? C + 5 * n : nullptr;
     const double* Tp = I.nz > 1 ? C + 6 * n : nullptr;
-    const int nx = I.nx, ny = I.ny, nz = I.nz, nxy = nx * ny;
-    rp.assign(I.n + 1, 0);
+    const int nx = I.real_nx(), ny = I.ny, nz = I.nz, nxy = nx * ny;
+    rp.assign(n + 1, 0);
     ci.clear();
     v.clear();
-    for (int c = 0; c < I.n; ++c) {
+    for (int c = 0; c < (int)n; ++c) {
         const int i = c % nx, j = (c / nx) % ny, k = c / nxy;
         if (k > 0) ci.push_back(c - nxy), v.push_back(Bt[c]);
         if (j > 0) ci.push_back(c - nx), v.push_back(Sx[c]);
@@ -3122,13 +3163,13 @@ void HeatBatch::export_residual(int slot, const double* u, const double* u0, dou
                                 std::vector<int>& rp, std::vector<int>& ci, std::vector<double>& v) {
     Inst& I = get_inst(p_, slot);
     EQ_REQUIRE(I.kind == K_HEAT, "slot " << slot << " is not a heat instance");
-    I.u = dev_from_host(u, I.n);
-    I.u0 = dev_from_host(u0, I.n);
+    I.u = dev_from_host(pad_rows(I, u).data(), I.n);
+    I.u0 = dev_from_host(pad_rows(I, u0).data(), I.n);
     I.dt = dt;
     ++p_->version;
     TableP tp = p_->table({&I});
     p_->assemble(*tp);
-    F = dev_to_host(I.F->p, I.n);
+    const std::vector<double> Fp = dev_to_host(I.F->p, I.n);
     std::vector<double> co;
     if (!p_->mixed) {
         co = stored_coefs(I, false);
@@ -3139,9 +3180,10 @@ void HeatBatch::export_residual(int slot, const double* u, const double* u0, dou
                            f->p, c->p);
         EQ_HIP(hipGetLastError());
         co = dev_to_host(c->p, 5 * (size_t)I.n);
-        EQ_REQUIRE(dev_to_host(f->p, I.n) == F, "FP64 row recomputation differs from the assembly");
+        EQ_REQUIRE(dev_to_host(f->p, I.n) == Fp, "FP64 row recomputation differs from the assembly");
     }
-    csr_from_stencil(I, co, rp, ci, v);
+    F = unpad_rows(I, Fp);
+    csr_from_stencil(I, unpad_rows(I, co, 5), rp, ci, v);
 }
 
 void HeatBatch::export_twophase_residual(int slot, int stage, const double* p, const double* sw0, const double* sw,

@@ -141,6 +141,8 @@ public:
     double time_kernel(const std::string& name, int reps, double* event_ms = nullptr);
     // Device bytes of one instance; `reduced`: its Krylov space is the reduced system.
     static size_t estimate_bytes(int n, const std::string& precision, bool twophase = false, bool reduced = true);
+    // Cells a heat instance of nx x ny stores (rows padded for the four-ranks-per-lane kernels).
+    static int heat_cells(int nx, int ny, const FusedConfig& cfg);
     // Launch shape used by the most recent launch and the number of tuned signatures.
     LaunchShape launch_shape() const;
 

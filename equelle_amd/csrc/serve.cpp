@@ -28,7 +28,8 @@ HeatServer::HeatServer(const ServerConfig& cfg)
 int HeatServer::submit(int nx, int ny, double k, double cv, const double* u0, int steps, double dt) {
     EQ_REQUIRE(nx > 0 && ny > 0 && (long)nx * ny < (1L << 31), "bad request grid " << nx << "x" << ny);
     EQ_REQUIRE(steps > 0 && dt > 0, "a request needs steps > 0 and dt > 0");
-    const double bytes = (double)HeatBatch::estimate_bytes(nx * ny, cfg_.batch.precision, false, cfg_.batch.reduced_form(false));
+    const double bytes = (double)HeatBatch::estimate_bytes(HeatBatch::heat_cells(nx, ny, cfg_.batch), cfg_.batch.precision,
+                                                             false, cfg_.batch.reduced_form(false));
     EQ_REQUIRE(bytes <= cfg_.hbm_budget, "request of " << bytes << " bytes exceeds the HBM budget of "
                                                        << cfg_.hbm_budget);
     Req r;
