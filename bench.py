@@ -539,7 +539,7 @@ def main():
                     help="mixed precision: FP32 inner solve tolerance")
     ap.add_argument("--no-graph", dest="no_graph", action="store_true", help="eager launches (profiling)")
     ap.add_argument("--grid3", default="128x128x64", help="two-phase grid nx x ny x nz")
-    ap.add_argument("--grid3-fi", dest="grid3_fi", default="32x32x16", help="fully implicit two-phase grid")
+    ap.add_argument("--grid3-fi", dest="grid3_fi", default="64x64x32", help="fully implicit two-phase grid")
     ap.add_argument("--tp-solver", dest="tp_solver", default="GMRes", help="two-phase Krylov solver")
     ap.add_argument("--tp-legs", dest="tp_legs", default="fp64,mixed",
                     help="two-phase fused legs: fp64 (the headline) and optionally mixed")
