@@ -380,7 +380,14 @@ PMP position_map(const IP& super, const IP& sub) {
     const std::vector<int>& sup = super->host;
     const std::vector<int>& s = sub->host;
     std::vector<int> pos(s.size(), -1);
-    if (!s.empty()) {
+    if (super->full) {   // super holds 0..n-1 in order: the position of an element is its value
+        const int ns = (int)sup.size();
+        for (size_t i = 0; i < s.size(); ++i) {
+            EQ_REQUIRE(s[i] >= 0 && s[i] < ns, "set of size " << s.size() << " is not a subset of the given superset (size "
+                                                             << ns << ")");
+            pos[i] = s[i];
+        }
+    } else if (!s.empty()) {
         std::vector<std::pair<int, int>> si(s.size());
         for (size_t i = 0; i < s.size(); ++i) si[i] = {s[i], (int)i};
         std::sort(si.begin(), si.end());

@@ -809,27 +809,54 @@ int gmres(const System<T>& A, const T* b, T* x, double tol, int maxit, int m, do
     FlagRing ring;
     int posted = 0;
     Gm hg;
+    // One Arnoldi step (column j): every kernel is gated by the device flag, so a step queued
+    // past convergence is a no-op.  From the second restart cycle on, step j is replayed from
+    // a graph captured once (the buffers of this solve do not move).
+    auto arnoldi = [&](int j) {
+        T* vj = V->p + (size_t)j * n;
+        A.apply_precond(vj, z->p, act);
+        A.spmv(z->p, w->p, act);
+        for (int pass = 0; pass < 2; ++pass) {   // CGS2
+            hipLaunchKernelGGL(k_vdots<T>, dim3(nb, j + 1), dim3(BS), 0, S(), n, j, (const T*)V->p, (const T*)w->p,
+                               part->p, nb, (const Gm*)gm);
+            EQ_HIP(hipGetLastError());
+            hipLaunchKernelGGL(k_vdots_final, dim3(j + 1), dim3(BS), 0, S(), nb, part->p, gm, pass);
+            EQ_HIP(hipGetLastError());
+            KL(k_vupdate<T>, nblk(n), n, j, V->p, (const Gm*)gm, w->p);
+        }
+        A.dots(ACT_STORE, 1, w->p, w->p, nullptr, nullptr, HN);
+        hipLaunchKernelGGL(k_gmres_step, dim3(1), dim3(64), 0, S(), j, m, (const double*)A.sc->p, gm, maxit);
+        EQ_HIP(hipGetLastError());
+        KL(k_gmres_next<T>, nblk(n), n, w->p, V->p + (size_t)(j + 1) * n, (const Gm*)gm);
+    };
+    std::vector<hipGraphExec_t> step_graph(m, nullptr);
+    struct GraphSet {
+        std::vector<hipGraphExec_t>& g;
+        ~GraphSet() {
+            for (hipGraphExec_t e : g)
+                if (e) (void)hipGraphExecDestroy(e);
+        }
+    } graphs_owner{step_graph};
+    int cycles = 0;
     for (;;) {
         // cycle start: beta = ||r||, V0 = r / beta (decides whether to continue)
         A.dots(ACT_STORE, 1, r->p, r->p, nullptr, nullptr, RR);
         KL(k_gmres_start<T>, nblk(n), n, r->p, V->p, A.sc->p, gm, maxit);
         for (int j = 0; j < m;) {
             for (int c = 0; c < CHUNK && j < m; ++c, ++j) {
-                T* vj = V->p + (size_t)j * n;
-                A.apply_precond(vj, z->p, act);
-                A.spmv(z->p, w->p, act);
-                for (int pass = 0; pass < 2; ++pass) {   // CGS2
-                    hipLaunchKernelGGL(k_vdots<T>, dim3(nb, j + 1), dim3(BS), 0, S(), n, j, (const T*)V->p,
-                                       (const T*)w->p, part->p, nb, (const Gm*)gm);
-                    EQ_HIP(hipGetLastError());
-                    hipLaunchKernelGGL(k_vdots_final, dim3(j + 1), dim3(BS), 0, S(), nb, part->p, gm, pass);
-                    EQ_HIP(hipGetLastError());
-                    KL(k_vupdate<T>, nblk(n), n, j, V->p, (const Gm*)gm, w->p);
+                if (step_graph[j]) {
+                    EQ_HIP(hipGraphLaunch(step_graph[j], S()));
+                } else if (cycles > 0 && !timing_enabled()) {   // second cycle on: capture step j once
+                    hipGraph_t g;
+                    EQ_HIP(hipStreamBeginCapture(S(), hipStreamCaptureModeThreadLocal));
+                    arnoldi(j);
+                    EQ_HIP(hipStreamEndCapture(S(), &g));
+                    EQ_HIP(hipGraphInstantiate(&step_graph[j], g, nullptr, nullptr, 0));
+                    EQ_HIP(hipGraphDestroy(g));
+                    EQ_HIP(hipGraphLaunch(step_graph[j], S()));
+                } else {
+                    arnoldi(j);
                 }
-                A.dots(ACT_STORE, 1, w->p, w->p, nullptr, nullptr, HN);
-                hipLaunchKernelGGL(k_gmres_step, dim3(1), dim3(64), 0, S(), j, m, (const double*)A.sc->p, gm, maxit);
-                EQ_HIP(hipGetLastError());
-                KL(k_gmres_next<T>, nblk(n), n, w->p, V->p + (size_t)(j + 1) * n, (const Gm*)gm);
             }
             ring.post(posted, act);
             const bool more = posted == 0 || ring.read(posted - 1);
@@ -849,6 +876,7 @@ int gmres(const System<T>& A, const T* b, T* x, double tol, int maxit, int m, do
         A.spmv(x, w->p);
         KL(k_sub<T>, nblk(n), n, b, w->p, r->p);
         posted = 0;
+        ++cycles;
     }
     if (relres) *relres = hg.rn / bn;
     return (int)hg.total;

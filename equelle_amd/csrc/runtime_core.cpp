@@ -254,16 +254,21 @@ RuntimeCore::RuntimeCore(const ParamMap& p) : params_(p) {
     output_to_file = p.get_bool("output_to_file", false);
 }
 
-IP RuntimeCore::firstCell(const IP& faces) const {
+IP RuntimeCore::face_cell(const IP& faces, int which) const {
+    const auto key = std::make_pair(faces->id, which);
+    auto it = face_cells_.find(key);
+    if (it != face_cells_.end() && !it->second.first.expired()) return it->second.second;
+    if (face_cells_.size() > 64)   // entries whose face set died can never hit again
+        for (auto e = face_cells_.begin(); e != face_cells_.end();)
+            e = e->second.first.expired() ? face_cells_.erase(e) : std::next(e);
     auto d = dev_alloc<int>(std::max(1, faces->size()));
-    launch_gather_face_cells(grid_->face_cells->p, faces->dev->p, faces->size(), 0, d->p);
-    return i_from_device(0, d, faces->size());
+    launch_gather_face_cells(grid_->face_cells->p, faces->dev->p, faces->size(), which, d->p);
+    IP out = i_from_device(0, d, faces->size());
+    face_cells_[key] = {faces, out};
+    return out;
 }
-IP RuntimeCore::secondCell(const IP& faces) const {
-    auto d = dev_alloc<int>(std::max(1, faces->size()));
-    launch_gather_face_cells(grid_->face_cells->p, faces->dev->p, faces->size(), 1, d->p);
-    return i_from_device(0, d, faces->size());
-}
+IP RuntimeCore::firstCell(const IP& faces) const { return face_cell(faces, 0); }
+IP RuntimeCore::secondCell(const IP& faces) const { return face_cell(faces, 1); }
 SP RuntimeCore::norm(const IP& s) const {
     auto d = dev_alloc<double>(std::max(1, s->size()));
     launch_gather_vec(s->codim == 0 ? grid_->cell_volumes->p : grid_->face_areas->p, s->dev->p, s->size(), 1, d->p);

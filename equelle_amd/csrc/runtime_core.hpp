@@ -76,6 +76,12 @@ private:
     ParamMap params_;
     std::unique_ptr<DeviceGrid> grid_;
     mutable std::map<std::pair<int, int>, IP> ranges_;
+    // firstCell / secondCell results per (face set, side).  Generated code calls them inside
+    // the residual function (e.g. FirstCell(InteriorFaces()) in an upwind helper), so a
+    // cached result keeps one index set id per Newton iteration and the position maps and
+    // row-gather patterns keyed by it are built once.
+    mutable std::map<std::pair<uint64_t, int>, std::pair<std::weak_ptr<const IndexData>, IP>> face_cells_;
+    IP face_cell(const IP& faces, int which) const;
 };
 
 }  // namespace eqhip

@@ -300,6 +300,38 @@ def test_linear_solve_residual(hip_api, oracle_api, solver, precond, precision):
     assert iters > 0
 
 
+@pytest.mark.parametrize("precond", ["diagonal", "ilu0"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_gmres_restart_cycles_replay_bit_identical(hip_api, precond, precision):
+    """GMRES replays each Arnoldi step from a graph from the second restart cycle on.  With
+    kernel timing on the steps are launched one kernel at a time; both must give the same
+    update bit for bit, over many restart cycles (restart 4)."""
+    nx, ny = 64, 48
+    er = EquelleRuntimeCUDA(f"grid_dim=2 nx={nx} ny={ny} k=0.2 cv=1 k_model=quadratic solver=GMRes "
+                            f"preconditioner={precond} precision={precision} gmres_restart=4 "
+                            "solver_tol=1e-9 solver_max_iter=3000", api=hip_api)
+    res = heat_program(er, k=0.2, cv=1.0, quadratic=True)
+    rng = np.random.default_rng(6)
+    u = er.collOfScalar(rng.uniform(0, 2, er.nc))
+    u0 = er.collOfScalar(rng.uniform(0, 2, er.nc))
+    R = res(er.primaryVariable(u), u0, 2.0)
+    out = {}
+    try:
+        for mode in (0, 1):
+            hip_api.timing(mode)
+            du, iters = er.solveForUpdate(R)
+            out[mode] = (du.value().copy(), iters)
+    finally:
+        hip_api.timing(0)
+        hip_api.timing(2)
+    assert out[0][1] == out[1][1] and out[0][1] > 12, (out[0][1], out[1][1])
+    assert np.array_equal(out[0][0], out[1][0])
+    rp, ci, v, rows, cols = R.derivative()
+    b = R.value()
+    rel = np.linalg.norm(b - csr_dense(rp, ci, v, rows, cols) @ out[0][0]) / np.linalg.norm(b)
+    assert rel <= 1e-8, rel
+
+
 @pytest.mark.parametrize("fp32", [0, 1])
 def test_sell_spmv_parity(hip_api, oracle_api, fp32):
     nx, ny = 123, 77
