@@ -405,6 +405,7 @@ PMP position_map(const IP& super, const IP& sub) {
     for (size_t i = 0; i < pos.size(); ++i) inv[pos[i]] = (int)i;
     p->d_inv = dev_from_host(inv.data(), inv.size());
     p->pos = std::move(pos);
+    p->inv = std::move(inv);
     std::lock_guard<std::mutex> lk(mu);
     cache[key] = p;
     // keep the maps alive as long as both sets live
@@ -441,9 +442,7 @@ SP s_extend(const SP& a, const IP& from, const IP& to) {
     auto v = dev_alloc<double>(m);
     launch_gather(a->val->p, pm->d_inv->p, v->p, m);
     if (!a->ad || !a->pat) return make_scalar(m, v, a->ad, nullptr, nullptr);
-    std::vector<int> inv(m, -1);
-    for (size_t i = 0; i < pm->pos.size(); ++i) inv[pm->pos[i]] = (int)i;
-    SymP e = sym_rows(a->pat, inv, pm->id ^ 0x8000000000000000ull, m);
+    SymP e = sym_rows(a->pat, pm->inv, pm->id ^ 0x8000000000000000ull, m);
     auto j = dev_alloc<double>(e->out->nnz);
     launch_gather(a->jval->p, e->ia->p, j->p, e->out->nnz);
     return make_scalar(m, v, true, e->out, j);

@@ -54,6 +54,7 @@ DevCsr csr_mul(const DevCsr& a, const DevCsr& b);
 struct PosMap {
     uint64_t id = 0;
     std::vector<int> pos;   // positions of sub's elements inside super
+    std::vector<int> inv;   // |super| entries: index into sub or -1
     DevPtr<int> d_pos;
     DevPtr<int> d_inv;      // |super| entries: index into sub or -1
     int super_size = 0;

@@ -41,6 +41,13 @@ struct Layout {
     DevPtr<int> color_rows;
     DevPtr<uint8_t> kind;              // per CSR entry: 0 L, 1 diag, 2 U, 3 ignored
     DevPtr<uint8_t> color;             // per row
+    // Triangular sweeps in SELL-64 per colour: the rows of colour c (colour order) form
+    // slices of 64 that never cross a colour; slice s of the L part holds its rows' L
+    // entries column-major from lsp[s] (row order within a row kept, so every sweep sums
+    // in CSR order), padded with column -1.  Colour c's first slice is cslice[c].
+    std::vector<int> cslice;
+    DevPtr<int> lsp, lcol, lsrc, usp, ucol, usrc;
+    long lnnz = 0, unnz = 0;
     std::weak_ptr<const Pattern> pat;
 };
 using LayoutP = std::shared_ptr<const Layout>;
@@ -124,6 +131,55 @@ LayoutP get_layout(const PatternP& p) {
             const int c = p->ci[k];
             kind[k] = c == r ? 1 : color[c] < color[r] ? 0 : color[c] > color[r] ? 2 : 3;
         }
+    }
+    // per-colour SELL-64 of the L (kind 0) and U (kind 2) entries
+    {
+        std::vector<int> slc(1, 0), lsp(1, 0), usp(1, 0), lcol, lsrc, ucol, usrc;
+        L->cslice.assign(ncol + 1, 0);
+        for (int c = 0; c < ncol; ++c) {
+            const int b = L->color_off[c], e = L->color_off[c + 1];
+            for (int s0 = b; s0 < e; s0 += W) {
+                const int s1 = std::min(e, s0 + W);
+                for (int part = 0; part < 2; ++part) {
+                    const uint8_t want = part == 0 ? 0 : 2;
+                    std::vector<int>& cols = part == 0 ? lcol : ucol;
+                    std::vector<int>& srcs = part == 0 ? lsrc : usrc;
+                    std::vector<int>& ptr = part == 0 ? lsp : usp;
+                    int w = 0;
+                    for (int t = s0; t < s1; ++t) {
+                        int m = 0;
+                        for (int k = p->rp[rows[t]]; k < p->rp[rows[t] + 1]; ++k) m += kind[k] == want;
+                        w = std::max(w, m);
+                    }
+                    const size_t base = cols.size();
+                    cols.resize(base + (size_t)w * W, -1);
+                    srcs.resize(base + (size_t)w * W, -1);
+                    for (int t = s0; t < s1; ++t) {
+                        int j = 0;
+                        for (int k = p->rp[rows[t]]; k < p->rp[rows[t] + 1]; ++k)
+                            if (kind[k] == want) {
+                                cols[base + (size_t)j * W + (t - s0)] = p->ci[k];
+                                srcs[base + (size_t)j * W + (t - s0)] = k;
+                                ++j;
+                            }
+                    }
+                    ptr.push_back((int)cols.size());
+                }
+            }
+            L->cslice[c + 1] = (int)lsp.size() - 1;
+        }
+        EQ_REQUIRE(lcol.size() < (size_t)INT32_MAX && ucol.size() < (size_t)INT32_MAX,
+                   "ILU(0) sweep arrays exceed 2^31 entries");
+        L->lnnz = (long)lcol.size();
+        L->unnz = (long)ucol.size();
+        for (auto* v : {&lcol, &lsrc, &ucol, &usrc})
+            if (v->empty()) v->push_back(-1);   // keep every device array non-empty
+        L->lsp = dev_from_host(lsp.data(), lsp.size());
+        L->usp = dev_from_host(usp.data(), usp.size());
+        L->lcol = dev_from_host(lcol.data(), lcol.size());
+        L->lsrc = dev_from_host(lsrc.data(), lsrc.size());
+        L->ucol = dev_from_host(ucol.data(), ucol.size());
+        L->usrc = dev_from_host(usrc.data(), usrc.size());
     }
     L->slice_ptr = dev_from_host(sp.data(), sp.size());
     L->col = dev_from_host(col.data(), col.size());
@@ -562,33 +618,63 @@ __global__ void k_ilu_factor(int cnt, const int* __restrict__ rows, const int* _
         }
     }
 }
+// Triangular sweeps over the rows of one colour (t = position within the colour), from the
+// per-colour SELL-64 arrays: lane t % 64 of slice t / 64, entries column-major, so the
+// column and value loads of a wave are contiguous.  Entries of one row are visited in CSR
+// order; column -1 marks padding.
 template <class T>
-__global__ void k_ilu_lower(int cnt, const int* __restrict__ rows, const int* __restrict__ rp,
-                            const int* __restrict__ ci, const uint8_t* __restrict__ kind, const T* __restrict__ lu,
-                            const T* __restrict__ b, T* __restrict__ z, const double* __restrict__ gate) {
+__global__ void k_ilu_lower(int cnt, const int* __restrict__ rows, const int* __restrict__ sp,
+                            const int* __restrict__ col, const T* __restrict__ val, const T* __restrict__ b,
+                            T* __restrict__ z, const double* __restrict__ gate) {
     if (gate && gate[0] == 0.0) return;
     long t = gid();
     if (t >= cnt) return;
     const int i = rows[t];
+    const int s0 = sp[t / W], w = (sp[t / W + 1] - s0) / W;
+    const int lane = (int)(t % W);
     T s = b[i];
-    for (int q = rp[i]; q < rp[i + 1]; ++q)
-        if (kind[q] == 0) s -= lu[q] * z[ci[q]];
+    for (int j = 0; j < w; ++j) {
+        const int k = s0 + j * W + lane;
+        const int c = col[k];
+        if (c >= 0) s -= val[k] * z[c];
+    }
     z[i] = s;
 }
 template <class T>
-__global__ void k_ilu_upper(int cnt, const int* __restrict__ rows, const int* __restrict__ rp,
-                            const int* __restrict__ ci, const uint8_t* __restrict__ kind,
-                            const int* __restrict__ diag, const T* __restrict__ lu, T* __restrict__ z,
-                            const double* __restrict__ gate) {
+__global__ void k_ilu_upper(int cnt, const int* __restrict__ rows, const int* __restrict__ sp,
+                            const int* __restrict__ col, const T* __restrict__ val, const T* __restrict__ dg,
+                            T* __restrict__ z, const double* __restrict__ gate) {
     if (gate && gate[0] == 0.0) return;
     long t = gid();
     if (t >= cnt) return;
     const int i = rows[t];
+    const int s0 = sp[t / W], w = (sp[t / W + 1] - s0) / W;
+    const int lane = (int)(t % W);
     T s = z[i];
-    for (int q = rp[i]; q < rp[i + 1]; ++q)
-        if (kind[q] == 2) s -= lu[q] * z[ci[q]];
-    const T d = diag[i] >= 0 ? lu[diag[i]] : (T)1;
+    for (int j = 0; j < w; ++j) {
+        const int k = s0 + j * W + lane;
+        const int c = col[k];
+        if (c >= 0) s -= val[k] * z[c];
+    }
+    const T d = dg[t];
     z[i] = d != (T)0 ? s / d : s;
+}
+// Sweep values from the CSR factors: out[k] = lu[src[k]] (0 for padding)
+template <class T>
+__global__ void k_ilu_sweep_vals(long m, const int* __restrict__ src, const T* __restrict__ lu, T* __restrict__ out) {
+    long k = gid();
+    if (k >= m) return;
+    const int s = src[k];
+    out[k] = s >= 0 ? lu[s] : (T)0;
+}
+// Pivot per colour position: U(i,i), or 1 where row i has no stored diagonal
+template <class T>
+__global__ void k_ilu_pivots(int n, const int* __restrict__ rows, const int* __restrict__ diag,
+                             const T* __restrict__ lu, T* __restrict__ dg) {
+    long t = gid();
+    if (t >= n) return;
+    const int d = diag[rows[t]];
+    dg[t] = d >= 0 ? lu[d] : (T)1;
 }
 
 inline unsigned nblk(long n) { return (unsigned)std::max(1L, (n + BS - 1) / BS); }
@@ -604,6 +690,7 @@ struct System {
     std::string precond;
     DevPtr<T> dinv;       // Jacobi
     DevPtr<T> lu;         // ILU(0) factors in CSR layout
+    DevPtr<T> lval, uval, piv;   // the factors in the per-colour sweep layout, pivots by colour position
     DevPtr<double> sc, part;
     int nb = 1;
 
@@ -626,6 +713,12 @@ struct System {
                     KL(k_ilu_factor<T>, nblk(cnt), cnt, L->color_rows->p + L->color_off[c], p->d_rp->p, p->d_ci->p,
                        L->kind->p, L->color->p, L->diag->p, c, lu->p);
             }
+            lval = dev_alloc<T>(std::max(1L, L->lnnz));
+            uval = dev_alloc<T>(std::max(1L, L->unnz));
+            piv = dev_alloc<T>(std::max(1, n));
+            if (L->lnnz > 0) KL(k_ilu_sweep_vals<T>, nblk(L->lnnz), L->lnnz, L->lsrc->p, (const T*)lu->p, lval->p);
+            if (L->unnz > 0) KL(k_ilu_sweep_vals<T>, nblk(L->unnz), L->unnz, L->usrc->p, (const T*)lu->p, uval->p);
+            KL(k_ilu_pivots<T>, nblk(n), n, L->color_rows->p, L->diag->p, (const T*)lu->p, piv->p);
         }
         nb = vec_blocks(n);
         sc = dev_alloc<double>(NSLOT);
@@ -642,14 +735,14 @@ struct System {
             for (int c = 0; c < L->ncolors; ++c) {
                 const int cnt = L->color_off[c + 1] - L->color_off[c];
                 if (cnt > 0)
-                    KL(k_ilu_lower<T>, nblk(cnt), cnt, L->color_rows->p + L->color_off[c], pat->d_rp->p, pat->d_ci->p,
-                       L->kind->p, lu->p, r, z, gate);
+                    KL(k_ilu_lower<T>, nblk(cnt), cnt, L->color_rows->p + L->color_off[c], L->lsp->p + L->cslice[c],
+                       L->lcol->p, (const T*)lval->p, r, z, gate);
             }
             for (int c = L->ncolors - 1; c >= 0; --c) {
                 const int cnt = L->color_off[c + 1] - L->color_off[c];
                 if (cnt > 0)
-                    KL(k_ilu_upper<T>, nblk(cnt), cnt, L->color_rows->p + L->color_off[c], pat->d_rp->p, pat->d_ci->p,
-                       L->kind->p, L->diag->p, lu->p, z, gate);
+                    KL(k_ilu_upper<T>, nblk(cnt), cnt, L->color_rows->p + L->color_off[c], L->usp->p + L->cslice[c],
+                       L->ucol->p, (const T*)uval->p, (const T*)piv->p + L->color_off[c], z, gate);
             }
         } else {
             KL(k_scale<T>, nblk(n), (long)n, precond == "jacobi" ? (const T*)dinv->p : (const T*)nullptr, r, z, gate);
