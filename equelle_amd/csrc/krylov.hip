@@ -903,8 +903,7 @@ int gmres(const System<T>& A, const T* b, T* x, double tol, int maxit, int m, do
     int posted = 0;
     Gm hg;
     // One Arnoldi step (column j): every kernel is gated by the device flag, so a step queued
-    // past convergence is a no-op.  From the second restart cycle on, step j is replayed from
-    // a graph captured once (the buffers of this solve do not move).
+    // past convergence is a no-op.
     auto arnoldi = [&](int j) {
         T* vj = V->p + (size_t)j * n;
         A.apply_precond(vj, z->p, act);
@@ -922,34 +921,13 @@ int gmres(const System<T>& A, const T* b, T* x, double tol, int maxit, int m, do
         EQ_HIP(hipGetLastError());
         KL(k_gmres_next<T>, nblk(n), n, w->p, V->p + (size_t)(j + 1) * n, (const Gm*)gm);
     };
-    std::vector<hipGraphExec_t> step_graph(m, nullptr);
-    struct GraphSet {
-        std::vector<hipGraphExec_t>& g;
-        ~GraphSet() {
-            for (hipGraphExec_t e : g)
-                if (e) (void)hipGraphExecDestroy(e);
-        }
-    } graphs_owner{step_graph};
-    int cycles = 0;
     for (;;) {
         // cycle start: beta = ||r||, V0 = r / beta (decides whether to continue)
         A.dots(ACT_STORE, 1, r->p, r->p, nullptr, nullptr, RR);
         KL(k_gmres_start<T>, nblk(n), n, r->p, V->p, A.sc->p, gm, maxit);
         for (int j = 0; j < m;) {
             for (int c = 0; c < CHUNK && j < m; ++c, ++j) {
-                if (step_graph[j]) {
-                    EQ_HIP(hipGraphLaunch(step_graph[j], S()));
-                } else if (cycles > 0 && !timing_enabled()) {   // second cycle on: capture step j once
-                    hipGraph_t g;
-                    EQ_HIP(hipStreamBeginCapture(S(), hipStreamCaptureModeThreadLocal));
-                    arnoldi(j);
-                    EQ_HIP(hipStreamEndCapture(S(), &g));
-                    EQ_HIP(hipGraphInstantiate(&step_graph[j], g, nullptr, nullptr, 0));
-                    EQ_HIP(hipGraphDestroy(g));
-                    EQ_HIP(hipGraphLaunch(step_graph[j], S()));
-                } else {
-                    arnoldi(j);
-                }
+                arnoldi(j);
             }
             ring.post(posted, act);
             const bool more = posted == 0 || ring.read(posted - 1);
@@ -969,7 +947,6 @@ int gmres(const System<T>& A, const T* b, T* x, double tol, int maxit, int m, do
         A.spmv(x, w->p);
         KL(k_sub<T>, nblk(n), n, b, w->p, r->p);
         posted = 0;
-        ++cycles;
     }
     if (relres) *relres = hg.rn / bn;
     return (int)hg.total;

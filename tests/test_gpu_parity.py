@@ -302,10 +302,10 @@ def test_linear_solve_residual(hip_api, oracle_api, solver, precond, precision):
 
 @pytest.mark.parametrize("precond", ["diagonal", "ilu0"])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_gmres_restart_cycles_replay_bit_identical(hip_api, precond, precision):
-    """GMRES replays each Arnoldi step from a graph from the second restart cycle on.  With
-    kernel timing on the steps are launched one kernel at a time; both must give the same
-    update bit for bit, over many restart cycles (restart 4)."""
+def test_gmres_many_restart_cycles(hip_api, precond, precision):
+    """GMRES(4) over many restart cycles (true residual recomputed at each restart) reaches
+    the tolerance, and the same solve with kernel timing on (event records between the
+    launches) gives the same update bit for bit."""
     nx, ny = 64, 48
     er = EquelleRuntimeCUDA(f"grid_dim=2 nx={nx} ny={ny} k=0.2 cv=1 k_model=quadratic solver=GMRes "
                             f"preconditioner={precond} precision={precision} gmres_restart=4 "
