@@ -39,8 +39,6 @@ struct Layout {
     int ncolors = 0;
     std::vector<int> color_off;        // host: rows of colour c are color_rows[color_off[c]..color_off[c+1])
     DevPtr<int> color_rows;
-    DevPtr<uint8_t> kind;              // per CSR entry: 0 L, 1 diag, 2 U, 3 ignored
-    DevPtr<uint8_t> color;             // per row
     // Triangular sweeps in SELL-64 per colour: the rows of colour c (colour order) form
     // slices of 64 that never cross a colour; slice s of the L part holds its rows' L
     // entries column-major from lsp[s] (row order within a row kept, so every sweep sums
@@ -48,6 +46,12 @@ struct Layout {
     std::vector<int> cslice;
     DevPtr<int> lsp, lcol, lsrc, usp, ucol, usrc;
     long lnnz = 0, unnz = 0;
+    // ILU(0) factorisation as precomputed elimination steps: the row at colour position t
+    // runs steps fsp[t]..fsp[t+1]; step s divides L entry fq[s] by the pivot at fdk[s]
+    // (-1: none) and, unless the multiplier is zero, applies the updates fpp[s]..fpp[s+1]:
+    // lu[fg[p]] -= l * lu[ff[p]].  Steps follow the colours of the eliminated rows, then
+    // CSR order; updates the U entries of the pivot row in CSR order.
+    DevPtr<int> fsp, fq, fdk, fpp, fg, ff;
     std::weak_ptr<const Pattern> pat;
 };
 using LayoutP = std::shared_ptr<const Layout>;
@@ -124,9 +128,9 @@ LayoutP get_layout(const PatternP& p) {
     for (int c = 0; c < ncol; ++c) L->color_off[c + 1] += L->color_off[c];
     std::vector<int> rows(n), fill(L->color_off.begin(), L->color_off.end() - 1);
     for (int r = 0; r < n; ++r) rows[fill[color[r]]++] = r;
-    std::vector<uint8_t> kind(p->nnz), col8(n);
+    // per CSR entry: 0 L, 1 diagonal, 2 U, 3 same colour (dropped by ILU(0) here)
+    std::vector<uint8_t> kind(p->nnz);
     for (int r = 0; r < n; ++r) {
-        col8[r] = (uint8_t)color[r];
         for (int k = p->rp[r]; k < p->rp[r + 1]; ++k) {
             const int c = p->ci[k];
             kind[k] = c == r ? 1 : color[c] < color[r] ? 0 : color[c] > color[r] ? 2 : 3;
@@ -181,13 +185,48 @@ LayoutP get_layout(const PatternP& p) {
         L->ucol = dev_from_host(ucol.data(), ucol.size());
         L->usrc = dev_from_host(usrc.data(), usrc.size());
     }
+    // elimination steps of the factorisation (rows of colour 0 have none)
+    {
+        std::vector<int> fsp(1, 0), fq, fdk, fpp(1, 0), fg, ff;
+        for (int t = 0; t < n; ++t) {
+            const int i = rows[t];
+            const int b = p->rp[i], e = p->rp[i + 1];
+            for (int cc = 0; cc < color[i]; ++cc)
+                for (int q = b; q < e; ++q) {
+                    if (kind[q] != 0) continue;
+                    const int k = p->ci[q];
+                    if (color[k] != cc) continue;
+                    fq.push_back(q);
+                    fdk.push_back(diag[k]);
+                    for (int f = p->rp[k]; f < p->rp[k + 1]; ++f) {
+                        if (kind[f] != 2) continue;
+                        const int j = p->ci[f];
+                        for (int g = b; g < e; ++g)
+                            if (p->ci[g] == j) {
+                                fg.push_back(g);
+                                ff.push_back(f);
+                                break;
+                            }
+                    }
+                    EQ_REQUIRE(fg.size() < (size_t)INT32_MAX, "ILU(0) update list exceeds 2^31 entries");
+                    fpp.push_back((int)fg.size());
+                }
+            fsp.push_back((int)fq.size());
+        }
+        for (auto* v : {&fq, &fdk, &fg, &ff})
+            if (v->empty()) v->push_back(-1);   // keep every device array non-empty
+        L->fsp = dev_from_host(fsp.data(), fsp.size());
+        L->fq = dev_from_host(fq.data(), fq.size());
+        L->fdk = dev_from_host(fdk.data(), fdk.size());
+        L->fpp = dev_from_host(fpp.data(), fpp.size());
+        L->fg = dev_from_host(fg.data(), fg.size());
+        L->ff = dev_from_host(ff.data(), ff.size());
+    }
     L->slice_ptr = dev_from_host(sp.data(), sp.size());
     L->col = dev_from_host(col.data(), col.size());
     L->src = dev_from_host(src.data(), src.size());
     L->diag = dev_from_host(diag.data(), diag.size());
     L->color_rows = dev_from_host(rows.data(), rows.size());
-    L->kind = dev_from_host(kind.data(), kind.size());
-    L->color = dev_from_host(col8.data(), col8.size());
     std::lock_guard<std::mutex> lk(mu);
     cache[p->id] = L;
     return L;
@@ -587,35 +626,21 @@ __global__ void k_sub(long n, const T* __restrict__ b, const T* __restrict__ w, 
 }
 
 // ---- multicolour ILU(0) ----------------------------------------------------------
+// ILU(0) rows of one colour (t = colour position, rows of earlier colours final):
+// l = lu[q] / U(k,k) (0 without a pivot), then lu[g] -= l * lu[f] over the step's updates.
 template <class T>
-__global__ void k_ilu_factor(int cnt, const int* __restrict__ rows, const int* __restrict__ rp,
-                             const int* __restrict__ ci, const uint8_t* __restrict__ kind,
-                             const uint8_t* __restrict__ color, const int* __restrict__ diag, int mycolor,
-                             T* __restrict__ lu) {
+__global__ void k_ilu_factor(int cnt, const int* __restrict__ fsp, const int* __restrict__ fq,
+                             const int* __restrict__ fdk, const int* __restrict__ fpp, const int* __restrict__ fg,
+                             const int* __restrict__ ff, T* __restrict__ lu) {
     long t = gid();
     if (t >= cnt) return;
-    const int i = rows[t];
-    const int b = rp[i], e = rp[i + 1];
-    for (int cc = 0; cc < mycolor; ++cc) {
-        for (int q = b; q < e; ++q) {
-            if (kind[q] != 0) continue;
-            const int k = ci[q];
-            if (color[k] != cc) continue;
-            const int dk = diag[k];
-            const T ukk = dk >= 0 ? lu[dk] : (T)0;
-            const T l = ukk != (T)0 ? lu[q] / ukk : (T)0;
-            lu[q] = l;
-            if (l == (T)0) continue;
-            for (int f = rp[k]; f < rp[k + 1]; ++f) {
-                if (kind[f] != 2) continue;   // U part of row k
-                const int j = ci[f];
-                for (int g = b; g < e; ++g)
-                    if (ci[g] == j) {
-                        lu[g] -= l * lu[f];
-                        break;
-                    }
-            }
-        }
+    for (int st = fsp[t]; st < fsp[t + 1]; ++st) {
+        const int q = fq[st], dk = fdk[st];
+        const T ukk = dk >= 0 ? lu[dk] : (T)0;
+        const T l = ukk != (T)0 ? lu[q] / ukk : (T)0;
+        lu[q] = l;
+        if (l == (T)0) continue;
+        for (int u = fpp[st]; u < fpp[st + 1]; ++u) lu[fg[u]] -= l * lu[ff[u]];
     }
 }
 // Triangular sweeps over the rows of one colour (t = position within the colour), from the
@@ -710,8 +735,8 @@ struct System {
             for (int c = 1; c < L->ncolors; ++c) {
                 const int cnt = L->color_off[c + 1] - L->color_off[c];
                 if (cnt > 0)
-                    KL(k_ilu_factor<T>, nblk(cnt), cnt, L->color_rows->p + L->color_off[c], p->d_rp->p, p->d_ci->p,
-                       L->kind->p, L->color->p, L->diag->p, c, lu->p);
+                    KL(k_ilu_factor<T>, nblk(cnt), cnt, L->fsp->p + L->color_off[c], L->fq->p, L->fdk->p, L->fpp->p,
+                       L->fg->p, L->ff->p, lu->p);
             }
             lval = dev_alloc<T>(std::max(1L, L->lnnz));
             uval = dev_alloc<T>(std::max(1L, L->unnz));
