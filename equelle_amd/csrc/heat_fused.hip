@@ -1389,6 +1389,33 @@ __device__ inline void mg_prolong_at(const Desc& f, const Desc& c, int p) {
     }
 }
 
+// offdiag of a black cell whose red neighbours are read as z_r + z_c of the neighbour's
+// aggregate (level cd): the prolongation applied on the fly.  Going up, the red values
+// after z += P z_c are read only by the black half sweep that follows (the red half sweep
+// after it overwrites them, and the black sweep overwrites the black ones), so the black
+// sweep computes them itself: the same sums in the same order as offdiag on the
+// prolonged vector.
+template <class T>
+__device__ inline T offdiag_prolonged(const Desc& d, const Desc& cd, int s, int c, int i, int j, int k,
+                                      const T* __restrict__ x, const T* __restrict__ xc) {
+    const int nx = d.nx, ny = d.ny;
+    auto z = [&](int nb, int ni, int nj, int nk) -> T {
+        const int ic = ni >> 1, jc = nj >> 1, kc = nk >> 1;
+        return x[nb >> 1] + xc[rb_slot(cd, ic + cd.nx * (jc + cd.ny * kc), ic, jc, kc)];
+    };
+    T acc = (T)0;
+    if (i > 0) acc += ((const T*)d.K[1])[s] * z(c - 1, i - 1, j, k);
+    if (i < nx - 1) acc += ((const T*)d.K[2])[s] * z(c + 1, i + 1, j, k);
+    if (j > 0) acc += ((const T*)d.K[3])[s] * z(c - nx, i, j - 1, k);
+    if (j < ny - 1) acc += ((const T*)d.K[4])[s] * z(c + nx, i, j + 1, k);
+    if (d.nz > 1) {
+        const int nxy = nx * ny;
+        if (k > 0) acc += ((const T*)d.K[5])[s] * z(c - nxy, i, j, k - 1);
+        if (k < d.nz - 1) acc += ((const T*)d.K[6])[s] * z(c + nxy, i, j, k + 1);
+    }
+    return acc;
+}
+
 // One-level kernels for the levels above an instance's tail (Desc::mgt = 0).
 // Gauss-Seidel half sweep over the cells of colour col; dots (the fine level's last half
 // sweep, red): also the CG partial dots (r.z, r.r) over both slots of each pair (r = V_R,
@@ -1476,13 +1503,24 @@ __global__ __launch_bounds__(BS) void k_mg_restrict(const Desc* __restrict__ F, 
     mg_restrict_at<T>(f, c, t_);
 }
 
-// Prolongation (thread per fine pair), for fine levels above the tail.
+// Prolongation and the black half sweep after it (thread per fine pair), for fine levels
+// above the tail: z_b = (b_b - A_br (z_r + P z_c)) / C_b (offdiag_prolonged); z_r keeps
+// its pre-prolongation value until the red half sweep overwrites it.
 template <class T>
-__global__ __launch_bounds__(BS) void k_mg_prolong(const Desc* __restrict__ F, const Desc* __restrict__ Cd,
-                                                   const int2* __restrict__ blk) {
+__global__ __launch_bounds__(BS) void k_mg_prolong_black(const Desc* __restrict__ F, const Desc* __restrict__ Cd,
+                                                         const int2* __restrict__ blk) {
     MG_LOCATE(f.h);
     if (f.mgt || !mg_on(c)) return;
-    mg_prolong_at<T>(f, c, t_);
+    T* __restrict__ x = (T*)f.v[V_Z];
+    int i, j, k;
+    const int cell = cell_of(f, t_, 1, i, j, k);
+    const int s = f.h + t_;
+    T xs = (T)0;
+    if (cell >= 0) {
+        const T off = offdiag_prolonged<T>(f, c, s, cell, i, j, k, x, (const T*)c.v[V_Z]);
+        xs = (((const T*)f.v[V_R])[s] - off) / ((const T*)f.K[0])[s];
+    }
+    x[s] = xs;
 }
 #undef MG_LOCATE
 
@@ -2432,10 +2470,9 @@ struct HeatBatch::Impl {
         EQ_HIP(hipGetLastError());
         for (int l = m.levels - 2; l >= 0; --l) {
             if (!m.sep[l]) continue;
-            hipLaunchKernelGGL((k_mg_prolong<T>), dim3((unsigned)m.nfpb[l + 1]), dim3(BS), 0, S(), m.fine[l + 1]->p,
-                               m.coarse[l + 1]->p, m.fpblk[l + 1]->p);
+            hipLaunchKernelGGL((k_mg_prolong_black<T>), dim3((unsigned)m.nfpb[l + 1]), dim3(BS), 0, S(),
+                               m.fine[l + 1]->p, m.coarse[l + 1]->p, m.fpblk[l + 1]->p);
             EQ_HIP(hipGetLastError());
-            smooth(l, 1, 0, 0);
             smooth(l, 0, 0, l == 0);
         }
         if (m.tail0) HLP(C_VEC, (k_cg_dots<T, CP>), t.part->p, t.ng);
