@@ -2288,9 +2288,14 @@ struct HeatBatch::Impl {
                 if (mixed) conversions<float>(t);
                 else conversions<double>(t);
                 break;
-            default:
-                if (mixed) vec_round<float>(t);
-                else vec_round<double>(t);
+            default:   // the vector updates of both Krylov methods (each skips the other's instances)
+                if (mixed) {
+                    vec_round<float>(t);
+                    cg_round<float>(t);
+                } else {
+                    vec_round<double>(t);
+                    cg_round<double>(t);
+                }
                 break;
         }
     }
@@ -2500,6 +2505,11 @@ struct HeatBatch::Impl {
         HLP(C_VEC, (k_b1<T, CP>));
         HLP(C_VEC, (k_b2<T, CP>));
         HLP(C_VEC, (k_b3<T, CP>), t.part->p, t.ng);
+    }
+    template <class T>
+    void cg_round(const Table& t) {
+        HLP(C_VEC, (k_cg2<T, CP>), t.part->p, t.ng);
+        HLP(C_VEC, (k_cg3<T, CP>), 0);
     }
 
     // One Krylov iteration for the instances whose refinement flag is set (rhs in V_R):
