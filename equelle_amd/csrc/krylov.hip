@@ -540,9 +540,37 @@ __global__ void k_vupdate(long n, int k, const T* __restrict__ V, const Gm* __re
     for (int j = 0; j <= k; ++j) s -= gm->h2[j] * (double)V[(long)j * n + i];
     w[i] = (T)s;
 }
-// hn = ||w|| (from the dot slot), new Givens rotation for column j, residual estimate
-__global__ void k_gmres_step(int j, int m, const double* __restrict__ sc, Gm* __restrict__ gm, int maxit) {
-    if (gm->active == 0.0 || threadIdx.x != 0) return;
+// CGS2 pass 2's update, w -= sum_{c<=k} h2[c] V[:,c] (as k_vupdate), with the partial
+// sums of ||w||^2 over a grid-stride loop of gridDim.x blocks
+template <class T>
+__global__ __launch_bounds__(BS) void k_vupdate_norm(long n, int k, const T* __restrict__ V, const Gm* __restrict__ gm,
+                                                     T* __restrict__ w, double* __restrict__ part) {
+    if (gm->active == 0.0) return;
+    double a = 0.0;
+    for (long i = gid(); i < n; i += (long)gridDim.x * BS) {
+        double s = (double)w[i];
+        for (int j = 0; j <= k; ++j) s -= gm->h2[j] * (double)V[(long)j * n + i];
+        const T wt = (T)s;
+        w[i] = wt;
+        a += (double)wt * (double)wt;
+    }
+    a = bred(a);
+    if (threadIdx.x == 0) part[blockIdx.x] = a;
+}
+__device__ void gmres_step_body(int j, int m, const double* __restrict__ sc, Gm* __restrict__ gm, int maxit);
+// hn = ||w|| from the partials of k_vupdate_norm (summed into slot HN), the new Givens
+// rotation for column j, the residual estimate
+__global__ __launch_bounds__(BS) void k_gmres_step_norm(int j, int m, const double* __restrict__ part, int nb,
+                                                        double* __restrict__ sc, Gm* __restrict__ gm, int maxit) {
+    if (gm->active == 0.0) return;
+    double a = 0.0;
+    for (int i = threadIdx.x; i < nb; i += BS) a += part[i];
+    a = bred(a);
+    if (threadIdx.x != 0) return;
+    sc[HN] = a;
+    gmres_step_body(j, m, sc, gm, maxit);
+}
+__device__ void gmres_step_body(int j, int m, const double* __restrict__ sc, Gm* __restrict__ gm, int maxit) {
     const double hn = sqrt(fmax(sc[HN], 0.0));
     gm->hn = hn;
     double* hc = gm->hcol;
@@ -913,6 +941,8 @@ int gmres(const System<T>& A, const T* b, T* x, double tol, int maxit, int m, do
     Gm* gm = (Gm*)gmb->p;
     const int nb = A.nb;
     auto part = dev_alloc<double>((size_t)nb * (m + 2));
+    const unsigned nnb = std::min(nblk(n), 4096u);   // blocks of the update that sums ||w||^2
+    auto npart = dev_alloc<double>(nnb);
     EQ_HIP(hipMemsetAsync(x, 0, n * sizeof(T), S()));
     EQ_HIP(hipMemsetAsync(gmb->p, 0, sizeof(Gm), S()));
     EQ_HIP(hipMemsetAsync(A.sc->p, 0, NSLOT * sizeof(double), S()));
@@ -939,11 +969,15 @@ int gmres(const System<T>& A, const T* b, T* x, double tol, int maxit, int m, do
             EQ_HIP(hipGetLastError());
             hipLaunchKernelGGL(k_vdots_final, dim3(j + 1), dim3(BS), 0, S(), nb, part->p, gm, pass);
             EQ_HIP(hipGetLastError());
-            KL(k_vupdate<T>, nblk(n), n, j, V->p, (const Gm*)gm, w->p);
+            if (pass == 0) {
+                KL(k_vupdate<T>, nblk(n), n, j, V->p, (const Gm*)gm, w->p);
+            } else {   // the second update also sums ||w||^2; the Givens step finishes the sum
+                KL(k_vupdate_norm<T>, nnb, n, j, (const T*)V->p, (const Gm*)gm, w->p, npart->p);
+                hipLaunchKernelGGL(k_gmres_step_norm, dim3(1), dim3(BS), 0, S(), j, m, (const double*)npart->p,
+                                   (int)nnb, A.sc->p, gm, maxit);
+                EQ_HIP(hipGetLastError());
+            }
         }
-        A.dots(ACT_STORE, 1, w->p, w->p, nullptr, nullptr, HN);
-        hipLaunchKernelGGL(k_gmres_step, dim3(1), dim3(64), 0, S(), j, m, (const double*)A.sc->p, gm, maxit);
-        EQ_HIP(hipGetLastError());
         KL(k_gmres_next<T>, nblk(n), n, w->p, V->p + (size_t)(j + 1) * n, (const Gm*)gm);
     };
     for (;;) {

@@ -302,13 +302,15 @@ def test_linear_solve_residual(hip_api, oracle_api, solver, precond, precision):
 
 @pytest.mark.parametrize("precond", ["diagonal", "ilu0"])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_gmres_many_restart_cycles(hip_api, precond, precision):
-    """GMRES(4) over many restart cycles (true residual recomputed at each restart) reaches
+@pytest.mark.parametrize("restart", [4, 40])
+def test_gmres_many_restart_cycles(hip_api, precond, precision, restart):
+    """GMRES over several restart cycles (true residual recomputed at each restart) reaches
     the tolerance, and the same solve with kernel timing on (event records between the
-    launches) gives the same update bit for bit."""
+    launches) gives the same update bit for bit.  Restart 40: Arnoldi steps past 32 columns
+    take the unfused orthogonalisation kernels."""
     nx, ny = 64, 48
     er = EquelleRuntimeCUDA(f"grid_dim=2 nx={nx} ny={ny} k=0.2 cv=1 k_model=quadratic solver=GMRes "
-                            f"preconditioner={precond} precision={precision} gmres_restart=4 "
+                            f"preconditioner={precond} precision={precision} gmres_restart={restart} "
                             "solver_tol=1e-9 solver_max_iter=3000", api=hip_api)
     res = heat_program(er, k=0.2, cv=1.0, quadratic=True)
     rng = np.random.default_rng(6)
