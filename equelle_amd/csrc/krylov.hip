@@ -42,7 +42,9 @@ struct Layout {
     // Triangular sweeps in SELL-64 per colour: the rows of colour c (colour order) form
     // slices of 64 that never cross a colour; slice s of the L part holds its rows' L
     // entries column-major from lsp[s] (row order within a row kept, so every sweep sums
-    // in CSR order), padded with column -1.  Colour c's first slice is cslice[c].
+    // in CSR order), padded with column -1; an L column of colour 0 is stored as -2 - col
+    // (its z equals the right-hand side, so the sweeps read that).  Colour c's first slice
+    // is cslice[c].
     std::vector<int> cslice;
     DevPtr<int> lsp, lcol, lsrc, usp, ucol, usrc;
     long lnnz = 0, unnz = 0;
@@ -162,7 +164,9 @@ LayoutP get_layout(const PatternP& p) {
                         int j = 0;
                         for (int k = p->rp[rows[t]]; k < p->rp[rows[t] + 1]; ++k)
                             if (kind[k] == want) {
-                                cols[base + (size_t)j * W + (t - s0)] = p->ci[k];
+                                const int cc = p->ci[k];
+                                // an L column of colour 0 reads the right-hand side (-2 - col)
+                                cols[base + (size_t)j * W + (t - s0)] = part == 0 && color[cc] == 0 ? -2 - cc : cc;
                                 srcs[base + (size_t)j * W + (t - s0)] = k;
                                 ++j;
                             }
@@ -674,11 +678,13 @@ __global__ void k_ilu_factor(int cnt, const int* __restrict__ fsp, const int* __
 // Triangular sweeps over the rows of one colour (t = position within the colour), from the
 // per-colour SELL-64 arrays: lane t % 64 of slice t / 64, entries column-major, so the
 // column and value loads of a wave are contiguous.  Entries of one row are visited in CSR
-// order; column -1 marks padding.
+// order; column -1 marks padding, -2 - col an L column of colour 0, whose z is b (colour 0
+// has no L entries, so its lower sweep is not run).  dg: the last colour, which has no U
+// entries, divides by its pivot here and skips its upper sweep.
 template <class T>
 __global__ void k_ilu_lower(int cnt, const int* __restrict__ rows, const int* __restrict__ sp,
                             const int* __restrict__ col, const T* __restrict__ val, const T* __restrict__ b,
-                            T* __restrict__ z, const double* __restrict__ gate) {
+                            T* __restrict__ z, const T* __restrict__ dg, const double* __restrict__ gate) {
     if (gate && gate[0] == 0.0) return;
     long t = gid();
     if (t >= cnt) return;
@@ -690,20 +696,27 @@ __global__ void k_ilu_lower(int cnt, const int* __restrict__ rows, const int* __
         const int k = s0 + j * W + lane;
         const int c = col[k];
         if (c >= 0) s -= val[k] * z[c];
+        else if (c < -1) s -= val[k] * b[-2 - c];
     }
-    z[i] = s;
+    if (dg) {
+        const T d = dg[t];
+        z[i] = d != (T)0 ? s / d : s;
+    } else {
+        z[i] = s;
+    }
 }
+// b0: colour 0, whose lower sweep would have copied b into z, starts from b
 template <class T>
 __global__ void k_ilu_upper(int cnt, const int* __restrict__ rows, const int* __restrict__ sp,
                             const int* __restrict__ col, const T* __restrict__ val, const T* __restrict__ dg,
-                            T* __restrict__ z, const double* __restrict__ gate) {
+                            T* __restrict__ z, const T* __restrict__ b0, const double* __restrict__ gate) {
     if (gate && gate[0] == 0.0) return;
     long t = gid();
     if (t >= cnt) return;
     const int i = rows[t];
     const int s0 = sp[t / W], w = (sp[t / W + 1] - s0) / W;
     const int lane = (int)(t % W);
-    T s = z[i];
+    T s = b0 ? b0[i] : z[i];
     for (int j = 0; j < w; ++j) {
         const int k = s0 + j * W + lane;
         const int c = col[k];
@@ -785,17 +798,22 @@ struct System {
     // z = M^{-1} r
     void apply_precond(const T* r, T* z, const double* gate = nullptr) const {
         if (precond == "ilu0") {
-            for (int c = 0; c < L->ncolors; ++c) {
+            // colour 0 has no L entries (z = r there: read from r, no sweep); the last colour
+            // has no U entries (its lower sweep divides by the pivots, no upper sweep)
+            const int nc = L->ncolors;
+            for (int c = 1; c < nc; ++c) {
                 const int cnt = L->color_off[c + 1] - L->color_off[c];
                 if (cnt > 0)
                     KL(k_ilu_lower<T>, nblk(cnt), cnt, L->color_rows->p + L->color_off[c], L->lsp->p + L->cslice[c],
-                       L->lcol->p, (const T*)lval->p, r, z, gate);
+                       L->lcol->p, (const T*)lval->p, r, z,
+                       c == nc - 1 ? (const T*)piv->p + L->color_off[c] : (const T*)nullptr, gate);
             }
-            for (int c = L->ncolors - 1; c >= 0; --c) {
+            for (int c = nc >= 2 ? nc - 2 : 0; c >= 0; --c) {
                 const int cnt = L->color_off[c + 1] - L->color_off[c];
                 if (cnt > 0)
                     KL(k_ilu_upper<T>, nblk(cnt), cnt, L->color_rows->p + L->color_off[c], L->usp->p + L->cslice[c],
-                       L->ucol->p, (const T*)uval->p, (const T*)piv->p + L->color_off[c], z, gate);
+                       L->ucol->p, (const T*)uval->p, (const T*)piv->p + L->color_off[c], z,
+                       c == 0 ? r : (const T*)nullptr, gate);
             }
         } else {
             KL(k_scale<T>, nblk(n), (long)n, precond == "jacobi" ? (const T*)dinv->p : (const T*)nullptr, r, z, gate);
