@@ -808,7 +808,7 @@ struct System {
                        L->lcol->p, (const T*)lval->p, r, z,
                        c == nc - 1 ? (const T*)piv->p + L->color_off[c] : (const T*)nullptr, gate);
             }
-            for (int c = nc >= 2 ? nc - 2 : 0; c >= 0; --c) {
+            for (int c = nc >= 2 ? nc - 2 : nc - 1; c >= 0; --c) {   // nc == 1: colour 0 alone
                 const int cnt = L->color_off[c + 1] - L->color_off[c];
                 if (cnt > 0)
                     KL(k_ilu_upper<T>, nblk(cnt), cnt, L->color_rows->p + L->color_off[c], L->usp->p + L->cslice[c],

@@ -300,6 +300,22 @@ def test_linear_solve_residual(hip_api, oracle_api, solver, precond, precision):
     assert iters > 0
 
 
+@pytest.mark.parametrize("solver", ["BiCGStab", "GMRes"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_ilu0_on_a_diagonal_jacobian(hip_api, solver, precision):
+    """A residual without couplings (u*u - c): its Jacobian diag(2u) takes one colour, so
+    the ILU(0) apply is the pivot division alone; the update is (u*u - c) / (2u)."""
+    er = EquelleRuntimeCUDA(f"grid_dim=2 nx=33 ny=17 solver={solver} preconditioner=ilu0 precision={precision} "
+                            "solver_tol=1e-12 solver_max_iter=50", api=hip_api)
+    rng = np.random.default_rng(8)
+    uv, cv = rng.uniform(0.5, 2.0, er.nc), rng.uniform(0.1, 3.0, er.nc)
+    u = er.primaryVariable(er.collOfScalar(uv))
+    du, iters = er.solveForUpdate(u * u - er.collOfScalar(cv))
+    want = (uv * uv - cv) / (2.0 * uv)
+    assert np.allclose(du.value(), want, rtol=1e-12 if precision == "fp64" else 1e-6, atol=0), iters
+    assert 1 <= iters <= 6, iters   # the preconditioner is exact: one iteration per refinement round
+
+
 @pytest.mark.parametrize("precond", ["diagonal", "ilu0"])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 @pytest.mark.parametrize("restart", [4, 40])
