@@ -561,27 +561,36 @@ __global__ __launch_bounds__(BS) void k_vupdate_norm(long n, int k, const T* __r
     a = bred(a);
     if (threadIdx.x == 0) part[blockIdx.x] = a;
 }
-__device__ void gmres_step_body(int j, int m, const double* __restrict__ sc, Gm* __restrict__ gm, int maxit);
+__device__ void gmres_step_body(int j, int m, const double* __restrict__ sc, Gm* __restrict__ gm, int maxit,
+                                double* hc, const double* cs, const double* sn);
 // hn = ||w|| from the partials of k_vupdate_norm (summed into slot HN), the new Givens
 // rotation for column j, the residual estimate
 __global__ __launch_bounds__(BS) void k_gmres_step_norm(int j, int m, const double* __restrict__ part, int nb,
                                                         double* __restrict__ sc, Gm* __restrict__ gm, int maxit) {
     if (gm->active == 0.0) return;
+    // the column and the previous rotations staged in LDS by the whole block (one round
+    // trip instead of a chain of dependent loads in the serial loop below)
+    __shared__ double hc[GM_MAX + 2], cs[GM_MAX], sn[GM_MAX];
+    for (int i = threadIdx.x; i <= j; i += BS) hc[i] = gm->hcol[i];
+    for (int i = threadIdx.x; i < j; i += BS) {
+        cs[i] = gm->cs[i];
+        sn[i] = gm->sn[i];
+    }
     double a = 0.0;
     for (int i = threadIdx.x; i < nb; i += BS) a += part[i];
-    a = bred(a);
+    a = bred(a);   // its barriers also order the staging above
     if (threadIdx.x != 0) return;
     sc[HN] = a;
-    gmres_step_body(j, m, sc, gm, maxit);
+    gmres_step_body(j, m, sc, gm, maxit, hc, cs, sn);
 }
-__device__ void gmres_step_body(int j, int m, const double* __restrict__ sc, Gm* __restrict__ gm, int maxit) {
+__device__ void gmres_step_body(int j, int m, const double* __restrict__ sc, Gm* __restrict__ gm, int maxit,
+                                double* hc, const double* cs, const double* sn) {
     const double hn = sqrt(fmax(sc[HN], 0.0));
     gm->hn = hn;
-    double* hc = gm->hcol;
     hc[j + 1] = hn;
     for (int i = 0; i < j; ++i) {   // previous rotations
-        const double t0 = gm->cs[i] * hc[i] + gm->sn[i] * hc[i + 1];
-        hc[i + 1] = -gm->sn[i] * hc[i] + gm->cs[i] * hc[i + 1];
+        const double t0 = cs[i] * hc[i] + sn[i] * hc[i + 1];
+        hc[i + 1] = -sn[i] * hc[i] + cs[i] * hc[i + 1];
         hc[i] = t0;
     }
     const double den = hypot(hc[j], hc[j + 1]);
@@ -620,10 +629,31 @@ __global__ void k_gmres_start(long n, const T* __restrict__ r, T* __restrict__ v
         gm->g[0] = beta;
     }
 }
-// y = H^{-1} g over the k columns of the finished cycle (one thread)
+// y = H^{-1} g over the k columns of the finished cycle: the triangle staged in LDS by the
+// block when it fits (k <= GM_LDS), then back substitution by one thread
+constexpr int GM_LDS = 64;
 __global__ void k_gmres_solve(int m, Gm* __restrict__ gm) {
-    if (threadIdx.x != 0) return;
     const int k = (int)gm->k;
+    __shared__ double H[GM_LDS * GM_LDS], y[GM_LDS], g[GM_LDS];
+    const bool lds = k <= GM_LDS;
+    if (lds) {
+        for (int e = threadIdx.x; e < k * k; e += blockDim.x) {
+            const int i = e / k, q = e % k;
+            if (q >= i) H[i * GM_LDS + q] = gm->H[i * m + q];
+        }
+        for (int i = threadIdx.x; i < k; i += blockDim.x) g[i] = gm->g[i];
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    if (lds) {
+        for (int i = k - 1; i >= 0; --i) {
+            double s = g[i];
+            for (int q = i + 1; q < k; ++q) s -= H[i * GM_LDS + q] * y[q];
+            y[i] = H[i * GM_LDS + i] != 0 ? s / H[i * GM_LDS + i] : 0.0;
+        }
+        for (int i = 0; i < k; ++i) gm->y[i] = y[i];
+        return;
+    }
     for (int i = k - 1; i >= 0; --i) {
         double s = gm->g[i];
         for (int q = i + 1; q < k; ++q) s -= gm->H[i * m + q] * gm->y[q];
@@ -1012,7 +1042,7 @@ int gmres(const System<T>& A, const T* b, T* x, double tol, int maxit, int m, do
             if (!more) break;
         }
         // x += M^{-1} V y over the cycle's columns
-        hipLaunchKernelGGL(k_gmres_solve, dim3(1), dim3(64), 0, S(), m, gm);
+        hipLaunchKernelGGL(k_gmres_solve, dim3(1), dim3(BS), 0, S(), m, gm);
         EQ_HIP(hipGetLastError());
         KL(k_vcombine<T>, nblk(n), n, (const T*)V->p, (const Gm*)gm, w->p);
         A.apply_precond(w->p, z->p);
