@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstring>
 #include <cmath>
 #include <functional>
 #include <map>
@@ -1643,6 +1644,26 @@ __global__ __launch_bounds__(TAIL_BS) void k_mg_tail(const Desc* __restrict__ L,
     }
 }
 
+// Copies of the step start (u0 = u, sw0 = sw) of many instances in one launch: job q
+// copies n doubles and owns the blocks from b0 on (MC_EPB elements each); bj maps a block
+// to its job.
+struct McJob {
+    double* dst;
+    const double* src;
+    long n, b0;
+};
+constexpr int MC_EPT = 8;
+constexpr long MC_EPB = (long)BS * MC_EPT;
+__global__ __launch_bounds__(BS) void k_multi_copy(const McJob* __restrict__ jobs, const int* __restrict__ bj) {
+    const McJob j = jobs[bj[blockIdx.x]];
+    const long base = ((long)blockIdx.x - j.b0) * MC_EPB + threadIdx.x;
+#pragma unroll
+    for (int e = 0; e < MC_EPT; ++e) {
+        const long i = base + (long)e * BS;
+        if (i < j.n) j.dst[i] = j.src[i];
+    }
+}
+
 // out[0] = number of instances with flag `which` set, out[1] = the largest Krylov
 // iteration count of the current solve (S_IT) over the instances.
 __global__ void k_count_active(const Desc* __restrict__ D, int ninst, int which, int* __restrict__ out) {
@@ -2032,8 +2053,44 @@ struct HeatBatch::Impl {
     hipEvent_t ev[NRING] = {};
     double* gpin = nullptr;
     size_t gcap = 0;
+    // step-start copies queued by begin_step, issued together before the next device work
+    // that reads or replaces those fields (flush_copies)
+    std::vector<McJob> pend;
+    DevPtr<char> mc_dev;
+    // host image of the job table; it outlives the upload, and the next flush comes after a
+    // host synchronisation (every iterate() waits on the device)
+    std::vector<char> mc_host;
 
     size_t ks() const { return mixed ? sizeof(float) : sizeof(double); }
+
+    void flush_copies() {
+        if (pend.empty()) return;
+        if (pend.size() == 1) {
+            EQ_HIP(hipMemcpyAsync(pend[0].dst, pend[0].src, (size_t)pend[0].n * sizeof(double),
+                                  hipMemcpyDeviceToDevice, S()));
+            pend.clear();
+            return;
+        }
+        std::vector<int> bj;
+        for (size_t q = 0; q < pend.size(); ++q) {
+            pend[q].b0 = (long)bj.size();
+            const long nb = (pend[q].n + MC_EPB - 1) / MC_EPB;
+            bj.insert(bj.end(), (size_t)nb, (int)q);
+        }
+        const size_t jb = pend.size() * sizeof(McJob), bb = bj.size() * sizeof(int);
+        std::vector<char>& h = mc_host;
+        h.resize(jb + bb);
+        std::memcpy(h.data(), pend.data(), jb);
+        std::memcpy(h.data() + jb, bj.data(), bb);
+        if (!mc_dev || mc_dev->n < h.size()) mc_dev = dev_alloc<char>(std::max(h.size(), (size_t)65536));
+        EQ_HIP(hipMemcpyAsync(mc_dev->p, h.data(), h.size(), hipMemcpyHostToDevice, S()));
+        if (!bj.empty()) {
+            hipLaunchKernelGGL(k_multi_copy, dim3((unsigned)bj.size()), dim3(BS), 0, S(), (const McJob*)mc_dev->p,
+                               (const int*)(mc_dev->p + jb));
+            EQ_HIP(hipGetLastError());
+        }
+        pend.clear();
+    }
 
     // two-phase instances in the reduced form solve their pressure stage by CG with the
     // Schur complement's diagonal (those in the full form with the V-cycle: Inst::mg_on)
@@ -2888,6 +2945,7 @@ static Inst& get_inst(HeatBatch::Impl* p, int slot) {
 }
 
 void HeatBatch::remove(int slot) {
+    p_->flush_copies();   // a queued copy may name this instance's fields
     get_inst(p_, slot);
     p_->tcache.clear();
     p_->insts.erase(slot);
@@ -2913,10 +2971,10 @@ void HeatBatch::begin_step(int slot, double dt) {
     if (I.dt != dt) ++p_->version;
     I.dt = dt;
     if (I.kind == K_HEAT) {
-        EQ_HIP(hipMemcpyAsync(I.u0->p, I.u->p, (size_t)I.n * sizeof(double), hipMemcpyDeviceToDevice, S()));
+        p_->pend.push_back({I.u0->p, I.u->p, (long)I.n, 0});   // u0 = u, issued by flush_copies
     } else {
         // sw0 = sw; pressure solve first, from the previous step's pressure
-        EQ_HIP(hipMemcpyAsync(I.sw0->p, I.sw->p, (size_t)I.n * sizeof(double), hipMemcpyDeviceToDevice, S()));
+        p_->pend.push_back({I.sw0->p, I.sw->p, (long)I.n, 0});
         EQ_HIP(hipMemcpyAsync(I.act->p + 2, &kStage[0], sizeof(int), hipMemcpyHostToDevice, S()));
         I.stage = 0;
     }
@@ -2926,6 +2984,7 @@ void HeatBatch::begin_step(int slot, double dt) {
 
 std::vector<int> HeatBatch::iterate() {
     Impl& P = *p_;
+    P.flush_copies();
     std::vector<Inst*> solve, asmb;
     for (auto& kv : P.insts) {
         if (kv.second->state == Inst::SOLVE) solve.push_back(kv.second.get());
@@ -3024,6 +3083,7 @@ LaunchShape HeatBatch::launch_shape() const {
 
 double HeatBatch::time_kernel(const std::string& name, int reps, double* event_ms) {
     Impl& P = *p_;
+    P.flush_copies();
     EQ_REQUIRE(reps > 0, "reps must be positive");
     std::vector<Inst*> all;
     for (auto& kv : P.insts) all.push_back(kv.second.get());
@@ -3134,11 +3194,13 @@ double HeatBatch::time_kernel(const std::string& name, int reps, double* event_m
 }
 
 void HeatBatch::get_state(int slot, double* u) const {
+    p_->flush_copies();
     Inst& I = get_inst(p_, slot);
     std::vector<double> h = unpad_rows(I, dev_to_host(I.kind == K_HEAT ? I.u->p : I.sw->p, I.n));
     std::copy(h.begin(), h.end(), u);
 }
 void HeatBatch::get_twophase_state(int slot, double* p, double* sw) const {
+    p_->flush_copies();
     Inst& I = get_inst(p_, slot);
     EQ_REQUIRE(I.kind == K_TWOPHASE, "slot " << slot << " is not a two-phase instance");
     if (p) {
@@ -3151,6 +3213,7 @@ void HeatBatch::get_twophase_state(int slot, double* p, double* sw) const {
     }
 }
 void HeatBatch::set_state(int slot, const double* u) {
+    p_->flush_copies();
     Inst& I = get_inst(p_, slot);
     EQ_REQUIRE(I.kind == K_HEAT, "set_state: slot " << slot << " is not a heat instance");
     I.u = dev_from_host(pad_rows(I, u).data(), I.n);
@@ -3208,6 +3271,7 @@ static std::vector<double> stored_coefs(const Inst& I, bool mixed) {
 
 void HeatBatch::export_residual(int slot, const double* u, const double* u0, double dt, std::vector<double>& F,
                                 std::vector<int>& rp, std::vector<int>& ci, std::vector<double>& v) {
+    p_->flush_copies();
     Inst& I = get_inst(p_, slot);
     EQ_REQUIRE(I.kind == K_HEAT, "slot " << slot << " is not a heat instance");
     I.u = dev_from_host(pad_rows(I, u).data(), I.n);
@@ -3236,6 +3300,7 @@ void HeatBatch::export_residual(int slot, const double* u, const double* u0, dou
 void HeatBatch::export_twophase_residual(int slot, int stage, const double* p, const double* sw0, const double* sw,
                                          double dt, std::vector<double>& F, std::vector<int>& rp,
                                          std::vector<int>& ci, std::vector<double>& v) {
+    p_->flush_copies();
     Inst& I = get_inst(p_, slot);
     EQ_REQUIRE(I.kind == K_TWOPHASE, "slot " << slot << " is not a two-phase instance");
     EQ_REQUIRE(stage == 0 || stage == 1, "stage must be 0 (pressure) or 1 (transport)");
