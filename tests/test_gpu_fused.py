@@ -172,6 +172,62 @@ def test_continuous_batching_join_mid_flight(hip_api):
         hip_api.batch_destroy(ref)
 
 
+def _run_to_done(api, b, slots):
+    fin = np.zeros(8, dtype=np.int32)
+    nf = C.c_int()
+    for _ in range(200):
+        api.batch_iterate(b, iptr(fin), 8, C.byref(nf))
+        flags = []
+        for s in slots:
+            f = C.c_int()
+            api.batch_in_flight(b, s, C.byref(f))
+            flags.append(f.value)
+        if not any(flags):
+            return
+    raise AssertionError("batch did not finish")
+
+
+def test_step_start_copies_before_remove_and_set_state(hip_api):
+    """begin_step queues each instance's step-start copy u0 = u, and the queued copies are
+    issued together before the next device work.  Removing a slot while copies are queued
+    leaves the others stepping exactly as a plain batch step; set_state after begin_step
+    keeps u0 at the state the step began from (the step solves the same implicit equation
+    from another Newton start)."""
+    params = "precision=mixed preconditioner=ilu0 rel_res_tol=1e-10 solver_tol=1e-6"
+    dims = [(64, 48, 0.3), (40, 56, 0.2), (72, 24, 0.15)]
+    us = [gaussian_bumps(nx, ny, 21 + q, width_scale=0.1) for q, (nx, ny, _) in enumerate(dims)]
+    b, ref = make_batch(hip_api, params), make_batch(hip_api, params)
+    try:
+        slots = [add(hip_api, b, nx, ny, k, u) for (nx, ny, k), u in zip(dims, us)]
+        for s in slots:
+            hip_api.batch_begin_step(b, s, 1.0)
+        hip_api.batch_remove(b, slots[0])   # three copies queued, one of them the removed slot's
+        _run_to_done(hip_api, b, slots[1:])
+        rs = [add(hip_api, ref, nx, ny, k, u) for (nx, ny, k), u in zip(dims[1:], us[1:])]
+        hip_api.batch_step(ref, iptr(np.array(rs, dtype=np.int32)), len(rs), 1.0)
+        for s, r, (nx, ny, _) in zip(slots[1:], rs, dims[1:]):
+            assert np.array_equal(state(hip_api, b, s, nx * ny), state(hip_api, ref, r, nx * ny))
+    finally:
+        hip_api.batch_destroy(b)
+        hip_api.batch_destroy(ref)
+    nx, ny, k = dims[0]
+    b, ref = make_batch(hip_api, params), make_batch(hip_api, params)
+    try:
+        s = add(hip_api, b, nx, ny, k, us[0])
+        hip_api.batch_begin_step(b, s, 1.0)
+        start = np.ascontiguousarray(1.1 * us[0] + 0.01)
+        hip_api.batch_set_state(b, s, dptr(start))   # the Newton start changes, u0 does not
+        _run_to_done(hip_api, b, [s])
+        r = add(hip_api, ref, nx, ny, k, us[0])
+        hip_api.batch_step(ref, iptr(np.array([r], dtype=np.int32)), 1, 1.0)
+        got, want = state(hip_api, b, s, nx * ny), state(hip_api, ref, r, nx * ny)
+        # u0 taken after set_state would solve the step from `start`: a 10% different field
+        assert np.allclose(got, want, rtol=1e-7, atol=1e-9 * np.abs(want).max())
+    finally:
+        hip_api.batch_destroy(b)
+        hip_api.batch_destroy(ref)
+
+
 def test_mixed_matches_fp64_on_large_grid(hip_api):
     nx = ny = 512
     u0 = gaussian_bumps(nx, ny, 2048, width_scale=nx / 2048)
