@@ -164,18 +164,16 @@ def bench_batched(lib, args):
                 slots.append(sl.value)
             slots = np.array(slots, dtype=np.int32)
             groups = [slots] if batched else [slots[q:q + 1] for q in range(len(slots))]
-            st = np.zeros(5, dtype=np.int32)
+            st = np.zeros((len(slots), 5), dtype=np.int32)
 
             def one_step():
-                work = 0
                 for g in groups:
                     lib.batch_step(b, iptr(g), len(g), dt)
-                for q, sl in enumerate(slots):
-                    lib.batch_stats(b, int(sl), iptr(st), None)
-                    if st[3] == 0 or st[4]:
-                        raise RuntimeError(f"instance {q} did not converge: {st.tolist()}")
-                    work += int(cells[q]) * int(st[0])
-                return work
+                lib.batch_stats_many(b, iptr(slots), len(slots), iptr(st))   # host-side counters
+                bad = np.nonzero((st[:, 3] == 0) | (st[:, 4] != 0))[0]
+                if bad.size:
+                    raise RuntimeError(f"instance {bad[0]} did not converge: {st[bad[0]].tolist()}")
+                return int(np.dot(cells, st[:, 0].astype(np.int64)))
 
             for _ in range(args.warmup):
                 one_step()

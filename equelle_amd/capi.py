@@ -85,6 +85,7 @@ SIGNATURES = {
     "batch_in_flight": [_P, _I, _IP],
     "batch_step": [_P, _IP, _I, _D],
     "batch_stats": [_P, _I, _IP, _DP],
+    "batch_stats_many": [_P, _IP, _I, _IP],
     "batch_state": [_P, _I, _DP],
     "batch_set_state": [_P, _I, _DP],
     "batch_bytes": [_P, _I, _DP],

@@ -779,6 +779,20 @@ int eqhip_batch_stats(void* batch, int slot, int* stats, double* norms) {
         }
     });
 }
+int eqhip_batch_stats_many(void* batch, const int* slots, int nslots, int* stats) {
+    return guard([&] {
+        EQ_REQUIRE(nslots >= 0 && (nslots == 0 || (slots && stats)), "batch_stats_many: bad arguments");
+        for (int q = 0; q < nslots; ++q) {
+            const StepStats& s = B(batch).stats(slots[q]);
+            int* o = stats + 5 * (size_t)q;
+            o[0] = s.newton;
+            o[1] = s.krylov;
+            o[2] = s.refinements;
+            o[3] = s.stop;
+            o[4] = s.failed ? 1 : 0;
+        }
+    });
+}
 int eqhip_batch_state(void* batch, int slot, double* u) {
     return guard([&] { B(batch).get_state(slot, u); });
 }

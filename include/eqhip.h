@@ -173,10 +173,13 @@ int eqhip_twophase_destroy(void* sim);
  * params: precision (fp64|mixed) preconditioner (diagonal|ilu0|none) rel_res_tol abs_res_tol
  * max_iter solver_tol solver_max_iter inner_tol (heat, default 1e-6) inner_tol_twophase (default 1e-4)
  * max_refine graph (0|1) cpt xcd tune krylov_form_heat (reduced|full, default reduced)
- * krylov_form_twophase (reduced|full|auto, default auto = reduced in fp64, full in mixed): with
- * preconditioner ilu0, BiCGStab on the red-black reduced (Schur complement) system or on the
- * full system with red-black ILU(0).  pair_vectors (0|1, default 1): four pair ranks per lane
- * in the Krylov kernels of 2D heat instances with nx % 8 == 0. */
+ * krylov_form_twophase (reduced|full|auto, default auto: full when the pressure stage runs
+ * multigrid CG, else reduced in fp64 and full in mixed): with preconditioner ilu0, the Krylov
+ * solve on the red-black reduced (Schur complement) system or on the full system with
+ * red-black ILU(0).  twophase_pressure_cg (0|1, default 1): CG for the symmetric pressure
+ * stage of two-phase instances; twophase_pressure_mg (0|1, default 1): with the aggregation
+ * multigrid V-cycle as its preconditioner (full form).  pair_vectors (0|1, default 1): four
+ * pair ranks per lane in the Krylov kernels of 2D heat instances with nx % 8 == 0. */
 int eqhip_batch_create(const char* params, void** batch);
 int eqhip_batch_destroy(void* batch);
 int eqhip_batch_add(void* batch, int nx, int ny, double k, double cv, const double* u, int* slot);
@@ -209,6 +212,9 @@ int eqhip_batch_in_flight(void* batch, int slot, int* flag);
 int eqhip_batch_step(void* batch, const int* slots, int nslots, double dt);
 /* stats = {Newton, Krylov, refinements, stop reason, failed}, norms = {||F(u_0)||, ||F(u_k)||} */
 int eqhip_batch_stats(void* batch, int slot, int* stats, double* norms);
+/* the same stats for nslots slots in one call: stats[5 q .. 5 q + 4] of slots[q] (host-side
+ * counters, no device access) */
+int eqhip_batch_stats_many(void* batch, const int* slots, int nslots, int* stats);
 int eqhip_batch_state(void* batch, int slot, double* u);
 int eqhip_batch_set_state(void* batch, int slot, const double* u);
 /* device-memory footprint of the slot in bytes (admission control) */
