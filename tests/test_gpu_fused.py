@@ -228,6 +228,22 @@ def test_step_start_copies_before_remove_and_set_state(hip_api):
         hip_api.batch_destroy(ref)
 
 
+def test_stats_many_matches_stats(hip_api):
+    b = make_batch(hip_api, "precision=mixed preconditioner=ilu0 rel_res_tol=1e-8 solver_tol=1e-6")
+    try:
+        dims = [(48, 40, 0.2), (64, 32, 0.4), (30, 30, 0.1)]
+        slots = np.array([add(hip_api, b, nx, ny, k, gaussian_bumps(nx, ny, 31 + q, width_scale=0.1))
+                          for q, (nx, ny, k) in enumerate(dims)], dtype=np.int32)
+        hip_api.batch_step(b, iptr(slots), len(slots), 1.0)
+        many = np.zeros((len(slots), 5), dtype=np.int32)
+        hip_api.batch_stats_many(b, iptr(slots), len(slots), iptr(many))
+        for q, sl in enumerate(slots):
+            assert many[q].tolist() == stats(hip_api, b, int(sl))[0].tolist()
+        assert (many[:, 0] > 0).all() and (many[:, 3] > 0).all()
+    finally:
+        hip_api.batch_destroy(b)
+
+
 def test_mixed_matches_fp64_on_large_grid(hip_api):
     nx = ny = 512
     u0 = gaussian_bumps(nx, ny, 2048, width_scale=nx / 2048)
