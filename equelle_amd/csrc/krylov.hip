@@ -606,12 +606,17 @@ __device__ void gmres_step_body(int j, int m, const double* __restrict__ sc, Gm*
     gm->total += 1.0;
     if (gm->rn <= gm->tol * gm->bn || hn == 0.0 || !isfinite(gm->rn) || gm->total >= maxit) gm->active = 0.0;
 }
-// V[:,j+1] = w / hn
+// V[:,j+1] = w / hn; with z: also the next step's diagonal preconditioner, z = d V[:,j+1]
+// (as k_scale computes it; d null: no preconditioner)
 template <class T>
-__global__ void k_gmres_next(long n, const T* __restrict__ w, T* __restrict__ vnext, const Gm* __restrict__ gm) {
+__global__ void k_gmres_next(long n, const T* __restrict__ w, T* __restrict__ vnext, const Gm* __restrict__ gm,
+                             const T* __restrict__ d, T* __restrict__ z) {
     if (gm->active == 0.0) return;
     long i = gid();
-    if (i < n) vnext[i] = (T)((double)w[i] / gm->hn);
+    if (i >= n) return;
+    const T v = (T)((double)w[i] / gm->hn);
+    vnext[i] = v;
+    if (z) z[i] = d ? d[i] * v : v;
 }
 // start of a cycle: beta = ||r|| (dot slot RR), V[:,0] = r / beta, g = beta e1
 template <class T>
@@ -1007,9 +1012,12 @@ int gmres(const System<T>& A, const T* b, T* x, double tol, int maxit, int m, do
     Gm hg;
     // One Arnoldi step (column j): every kernel is gated by the device flag, so a step queued
     // past convergence is a no-op.
+    // Jacobi or no preconditioner: each step's k_gmres_next also forms the next step's z
+    const bool diag_pc = A.precond != "ilu0";
+    const T* dpc = A.precond == "jacobi" ? (const T*)A.dinv->p : (const T*)nullptr;
     auto arnoldi = [&](int j) {
         T* vj = V->p + (size_t)j * n;
-        A.apply_precond(vj, z->p, act);
+        if (j == 0 || !diag_pc) A.apply_precond(vj, z->p, act);
         A.spmv(z->p, w->p, act);
         for (int pass = 0; pass < 2; ++pass) {   // CGS2
             hipLaunchKernelGGL(k_vdots<T>, dim3(nb, j + 1), dim3(BS), 0, S(), n, j, (const T*)V->p, (const T*)w->p,
@@ -1026,7 +1034,8 @@ int gmres(const System<T>& A, const T* b, T* x, double tol, int maxit, int m, do
                 EQ_HIP(hipGetLastError());
             }
         }
-        KL(k_gmres_next<T>, nblk(n), n, w->p, V->p + (size_t)(j + 1) * n, (const Gm*)gm);
+        KL(k_gmres_next<T>, nblk(n), n, w->p, V->p + (size_t)(j + 1) * n, (const Gm*)gm, dpc,
+           diag_pc ? z->p : (T*)nullptr);
     };
     for (;;) {
         // cycle start: beta = ||r||, V0 = r / beta (decides whether to continue)
