@@ -209,6 +209,21 @@ __device__ void fin_body(const Desc& d, const double* part, int nb, const Fin& f
     const int2 r = make_int2(d.gbase, d.gbase + (f.kry ? d.ngrpk : d.ngrp));
     __shared__ double res[2];
     __shared__ double lds[2][FBS / 64];
+    // the scalars the action reads, loaded by thread 0 before the partials so both round
+    // trips overlap (only this block writes the instance's scalars and flags)
+    int a0 = 0, a1 = 0, mth = M_BICG;
+    double p_rho = 0.0, p_omega = 0.0, p_it = 0.0, p_itt = 0.0, p_tol2 = 0.0, p_bb = 0.0;
+    if (threadIdx.x == 0) {
+        a0 = d.act[0];
+        a1 = d.act[1];
+        mth = meth_of(d);
+        p_rho = d.sc[S_RHO];
+        p_omega = d.sc[S_OMEGA];
+        p_it = d.sc[S_IT];
+        p_itt = d.sc[S_ITT];
+        p_tol2 = d.sc[S_TOL2];
+        p_bb = d.sc[S_BB];
+    }
     // partials were written by the previous kernel (visible at this kernel's start); both
     // dots' partials are loaded together, eight per thread, before any is added, so the
     // kernel waits for memory once per 8 x FBS groups
@@ -248,7 +263,7 @@ __device__ void fin_body(const Desc& d, const double* part, int nb, const Fin& f
     double* sc = d.sc;
     switch (act) {
         case A_INIT: {   // start of a Krylov solve for instances under refinement
-            if (!d.act[1]) {
+            if (!a1) {
                 d.act[0] = 0;
                 break;
             }
@@ -262,53 +277,55 @@ __device__ void fin_body(const Desc& d, const double* part, int nb, const Fin& f
             break;
         }
         case A_ALPHA:
-            if (!d.act[0] || meth_of(d) != M_BICG) break;
+            if (!a0 || mth != M_BICG) break;
             if (res[0] == 0.0 || !isfinite(res[0])) {
                 sc[S_BREAK] = 1.0;
                 d.act[0] = 0;
-            } else sc[S_ALPHA] = sc[S_RHO] / res[0];
+            } else sc[S_ALPHA] = p_rho / res[0];
             break;
         case A_OMEGA:
-            if (!d.act[0] || meth_of(d) != M_BICG) break;
+            if (!a0 || mth != M_BICG) break;
             if (res[1] == 0.0 || !isfinite(res[1])) {
                 sc[S_OMEGA] = 0.0;   // s already solves the system; finish with x += alpha ph
             } else sc[S_OMEGA] = res[0] / res[1];
             break;
         case A_RHO: {
-            if (!d.act[0] || meth_of(d) != M_BICG) break;
-            sc[S_RHO_OLD] = sc[S_RHO];
+            if (!a0 || mth != M_BICG) break;
+            sc[S_RHO_OLD] = p_rho;
             sc[S_RHO] = res[0];
             sc[S_RR] = res[1];
-            sc[S_IT] += 1.0;
-            sc[S_ITT] += 1.0;
-            const bool ok = isfinite(res[1]) && sc[S_OMEGA] != 0.0;
+            const double it = p_it + 1.0;
+            sc[S_IT] = it;
+            sc[S_ITT] = p_itt + 1.0;
+            const bool ok = isfinite(res[1]) && p_omega != 0.0;
             if (!ok) sc[S_BREAK] = 2.0;
             // tol2 < 0: the instance's own tolerance (S_TOL2)
-            d.act[0] = ok && res[1] > (tol2 < 0.0 ? sc[S_TOL2] : tol2) * sc[S_BB] && sc[S_IT] < maxit && res[0] != 0.0;
+            d.act[0] = ok && res[1] > (tol2 < 0.0 ? p_tol2 : tol2) * p_bb && it < maxit && res[0] != 0.0;
             break;
         }
         case A_CG_ALPHA:   // alpha = rho / p.q; a non-positive curvature ends the solve
-            if (!d.act[0] || meth_of(d) != M_CG) break;
+            if (!a0 || mth != M_CG) break;
             if (!(res[0] > 0.0) || !isfinite(res[0])) {
                 sc[S_BREAK] = 1.0;
                 d.act[0] = 0;
-            } else sc[S_ALPHA] = sc[S_RHO] / res[0];
+            } else sc[S_ALPHA] = p_rho / res[0];
             break;
         case A_CG_RHO: {   // rho' = r.z, beta = rho' / rho, ||r||^2; convergence as A_RHO
-            if (!d.act[0] || meth_of(d) != M_CG) break;
-            const double beta = res[0] / sc[S_RHO];
+            if (!a0 || mth != M_CG) break;
+            const double beta = res[0] / p_rho;
             sc[S_BETA] = beta;
             sc[S_RHO] = res[0];
             sc[S_RR] = res[1];
-            sc[S_IT] += 1.0;
-            sc[S_ITT] += 1.0;
+            const double it = p_it + 1.0;
+            sc[S_IT] = it;
+            sc[S_ITT] = p_itt + 1.0;
             const bool ok = isfinite(res[1]) && isfinite(beta);
             if (!ok) sc[S_BREAK] = 2.0;
-            d.act[0] = ok && res[1] > (tol2 < 0.0 ? sc[S_TOL2] : tol2) * sc[S_BB] && sc[S_IT] < maxit && res[0] != 0.0;
+            d.act[0] = ok && res[1] > (tol2 < 0.0 ? p_tol2 : tol2) * p_bb && it < maxit && res[0] != 0.0;
             break;
         }
         case A_CG_START:   // V-cycle instances: rho = r.z of the first z, first direction p = z
-            if (!d.act[0] || meth_of(d) != M_CG || !d.mgp) break;
+            if (!a0 || mth != M_CG || !d.mgp) break;
             sc[S_RHO] = res[0];
             sc[S_BETA] = 0.0;
             if (!(res[0] > 0.0) || !isfinite(res[0])) {
