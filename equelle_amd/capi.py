@@ -61,6 +61,7 @@ SIGNATURES = {
     "extend_scalar": [_P, _D, _I, _IP],
     "on": [_P, _I, _I, _I, _IP],
     "reduce": [_P, _I, _I, _DP],
+    "reduce_noalloc": [_P, _I, _I, _DP],
     "norm_set": [_P, _I, _IP],
     "centroid": [_P, _I, _DP],
     "normal": [_P, _I, _DP],

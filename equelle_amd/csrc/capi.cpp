@@ -332,6 +332,16 @@ int eqhip_reduce(void* rt, int kind, int a, double* out) {
         else *out = s_reduce(kind == 0 ? RED_SUM : kind == 1 ? RED_PROD : kind == 2 ? RED_MAX : RED_MIN, x);
     });
 }
+int eqhip_reduce_noalloc(void* rt, int kind, int a, double* out) {
+    return guard([&] {
+        const SP& x = R(rt)->S(a);
+        EQ_REQUIRE(kind >= 0 && kind <= 4, "reduction kind must be 0..4, got " << kind);
+        EQ_REQUIRE(x->n > 0 || kind == 0 || kind == 1 || kind == 4, "reduction of an empty collection");
+        const double* v = x->val ? x->val->p : nullptr;
+        if (kind == 4) *out = std::sqrt(reduce_noalloc(RED_SUMSQ, v, x->n));
+        else *out = reduce_noalloc(kind == 0 ? RED_SUM : kind == 1 ? RED_PROD : kind == 2 ? RED_MAX : RED_MIN, v, x->n);
+    });
+}
 int eqhip_norm_set(void* rt, int set, int* h) {
     return guard([&] { *h = Rt::put(R(rt)->scal, R(rt)->core->norm(R(rt)->I(set))); });
 }

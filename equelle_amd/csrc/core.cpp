@@ -39,6 +39,7 @@ Device::Device() {
     num_cus_ = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
     EQ_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     EQ_HIP(hipHostMalloc((void**)&host_scratch_, 4096, hipHostMallocDefault));
+    noalloc_scratch_init();   // device address of the allocation-free reduction's scratch
 }
 
 Device::~Device() {}

@@ -7,6 +7,13 @@
 #include "core.hpp"
 #include "kernels.hpp"
 
+// Scratch of the allocation-free reduction (reduce_noalloc): the block partials of
+// k_reduce_partial and the final value, NOALLOC_SCRATCH_DOUBLES doubles (16 KiB).  Each GPU
+// holds its own copy for the life of the process.  Host code never passes this name to a
+// kernel or a copy: it is a host-side stand-in, not a device address; the address comes
+// from hipGetSymbolAddress (noalloc_scratch below).
+static __device__ __align__(256) double scratch[eqhip::NOALLOC_SCRATCH_DOUBLES];
+
 namespace eqhip {
 
 namespace {
@@ -522,6 +529,49 @@ double reduce(int op, const double* a, long n) {
     EQ_HIP(hipGetLastError());
     double* h = dev.host_scratch();
     EQ_HIP(hipMemcpyAsync(h, part->p + nb, sizeof(double), hipMemcpyDeviceToHost, S()));
+    dev.sync();
+    return h[0];
+}
+
+// Device address of `scratch` on each device, looked up once per device with that device
+// current (hipGetSymbolAddress; each device has its own copy at its own address).
+namespace {
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_dev[NOALLOC_MAX_DEVICES] = {};
+}  // namespace
+
+double* noalloc_scratch() {
+    int dev = 0;
+    EQ_HIP(hipGetDevice(&dev));
+    EQ_REQUIRE(dev >= 0 && dev < NOALLOC_MAX_DEVICES, "device " << dev << " out of range for the no-alloc scratch");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        EQ_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        EQ_REQUIRE(p != nullptr, "hipGetSymbolAddress returned a null address for the no-alloc scratch");
+        g_scratch_dev[dev] = static_cast<double*>(p);
+    }
+    return g_scratch_dev[dev];
+}
+
+void noalloc_scratch_init() { (void)noalloc_scratch(); }
+
+// The kernels and launch shape of reduce(), with the partials in the device's scratch array
+// instead of a pool block.  Every partial and the final slot are written before they are
+// read, so nothing needs clearing between calls.
+double reduce_noalloc(int op, const double* a, long n) {
+    EQ_REQUIRE(n >= 0, "negative collection size " << n);
+    const int nb = reduce_blocks(n);
+    EQ_REQUIRE(nb + 1 <= NOALLOC_SCRATCH_DOUBLES, "reduction needs " << (nb + 1) << " scratch doubles, the no-alloc "
+                                                  "scratch holds " << NOALLOC_SCRATCH_DOUBLES);
+    double* part = noalloc_scratch();
+    EQ_LAUNCH(k_reduce_partial, nb, op, a, (const double*)nullptr, n, part);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(BS), 0, S(), op, (const double*)part, nb, part + nb);
+    EQ_HIP(hipGetLastError());
+    Device& dev = Device::get();
+    double* h = dev.host_scratch();   // pinned at device set-up
+    EQ_HIP(hipMemcpyAsync(h, part + nb, sizeof(double), hipMemcpyDeviceToHost, S()));
     dev.sync();
     return h[0];
 }

@@ -67,6 +67,15 @@ void launch_vec_col(const double* v, int n, int dim, int col, double* out);
 
 // Deterministic reductions (fixed grid and order for a given n).
 double reduce(int op, const double* a, long n);
+// The same reduction with its partials in a fixed device array instead of a pool block:
+// no device memory is allocated or freed.  NOALLOC_SCRATCH_DOUBLES bounds the partials
+// (reduce uses at most 1024 blocks, so every n fits).  Calls overlapping on one GPU race.
+constexpr int NOALLOC_SCRATCH_DOUBLES = 2048;
+double reduce_noalloc(int op, const double* a, long n);
+// Device address of that array on the current device (looked up once per device).
+double* noalloc_scratch();
+// Looks the address up for the current device (called at device set-up).
+void noalloc_scratch_init();
 // Deterministic dot product into a device scalar (no host sync).
 void launch_dot_to(const double* a, const double* b, long n, double* d_out);
 

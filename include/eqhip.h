@@ -100,6 +100,13 @@ int eqhip_extend_scalar(void* rt, double s, int to, int* h);    /* operatorExten
 int eqhip_on(void* rt, int a, int from, int to, int* h);        /* operatorOn — _impl.hpp:134-147 */
 /* kind: 0 sum, 1 product, 2 max, 3 min, 4 twoNorm — EquelleRuntimeCUDA.cpp:148-151,332-346 */
 int eqhip_reduce(void* rt, int kind, int a, double* out);
+/* The same reduction, allocating no device memory (no hipMalloc/hipFree, no pool block).
+ * No counterpart in the reference: it is for callers that cannot allocate (code under
+ * graph capture, tight serving loops, harnesses that forbid allocation).  Its partials live
+ * in a fixed 16 KiB device array per GPU (2048 doubles; at most 1025 are used, so every
+ * collection size fits); calls that overlap on one GPU share that array and race.  Results
+ * are bit-identical to eqhip_reduce.  kind outside 0..4 is an error. */
+int eqhip_reduce_noalloc(void* rt, int kind, int a, double* out);
 int eqhip_norm_set(void* rt, int set, int* h);                  /* norm<codim>(set) — _impl.hpp:171-182 */
 int eqhip_centroid(void* rt, int set, double* out);             /* centroid — _impl.hpp:203-210 (n*dim) */
 int eqhip_normal(void* rt, int faces, double* out);             /* normal — EquelleRuntimeCUDA.cpp:137-140 (unit) */
