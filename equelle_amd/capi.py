@@ -57,6 +57,9 @@ SIGNATURES = {
     "trinary": [_P, _I, _I, _I, _IP],
     "gradient": [_P, _I, _IP],
     "divergence": [_P, _I, _IP],
+    "neg_gradient": [_P, _I, _IP],
+    "interior_divergence": [_P, _I, _IP],
+    "norm_scalar": [_P, _I, _IP],
     "extend": [_P, _I, _I, _I, _IP],
     "extend_scalar": [_P, _D, _I, _IP],
     "on": [_P, _I, _I, _I, _IP],

@@ -310,6 +310,15 @@ int eqhip_gradient(void* rt, int a, int* h) {
 int eqhip_divergence(void* rt, int a, int* h) {
     return guard([&] { *h = Rt::put(R(rt)->scal, R(rt)->core->divergence(R(rt)->S(a))); });
 }
+int eqhip_neg_gradient(void* rt, int a, int* h) {
+    return guard([&] { *h = Rt::put(R(rt)->scal, R(rt)->core->neg_gradient(R(rt)->S(a))); });
+}
+int eqhip_interior_divergence(void* rt, int a, int* h) {
+    return guard([&] { *h = Rt::put(R(rt)->scal, R(rt)->core->interior_divergence(R(rt)->S(a))); });
+}
+int eqhip_norm_scalar(void* rt, int a, int* h) {
+    return guard([&] { *h = Rt::put(R(rt)->scal, s_norm(R(rt)->S(a))); });
+}
 int eqhip_extend(void* rt, int a, int from, int to, int* h) {
     return guard([&] {
         Rt* r = R(rt);

@@ -253,6 +253,14 @@ SP s_scal_div(double s, const SP& a) {
 
 SP s_neg(const SP& a) { return s_scal_mul(-1.0, a); }
 
+SP s_norm(const SP& a) {
+    // factors (+1 where a >= 0, else -1) times a, as a constant times a collection
+    // (EquelleRuntimeCPU.cpp:241-254): values |a|, Jacobian diag(factors) a' (a's pattern)
+    auto c = dev_alloc<double>(std::max(1, a->n));
+    launch_coef(5, a->val->p, nullptr, 0.0, c->p, a->n);
+    return s_mul(make_scalar(a->n, c, false, nullptr, nullptr), a);
+}
+
 SP s_sqrt(const SP& a) {
     auto v = dev_alloc<double>(a->n);
     launch_sqrt(a->val->p, v->p, a->n);

@@ -29,6 +29,7 @@ SP s_scal_mul(double s, const SP& a);
 SP s_div_scal(const SP& a, double s);   // a / s = (1/s)*a  (collOfScalar.cu:385-387)
 SP s_scal_div(double s, const SP& a);   // s / a
 SP s_neg(const SP& a);
+SP s_norm(const SP& a);   // norm(CollOfScalar): elementwise |a| with the sign-factor Jacobian
 SP s_sqrt(const SP& a);
 BP s_compare(int op, const SP& a, const SP& b);
 BP s_compare_scalar(int op, const SP& a, double s, int scalar_left);

@@ -331,12 +331,15 @@ CollOfVector EquelleRuntimeCUDA::normal(const CollOfFace& faces) const {
     return CollOfVector(core_->normal(faces.data()));
 }
 CollOfScalar EquelleRuntimeCUDA::dot(const CollOfVector& a, const CollOfVector& b) const { return a.dot(b); }
-CollOfScalar EquelleRuntimeCUDA::negGradient(const CollOfScalar&) const {
-    OPM_THROW(std::runtime_error, "Not yet implemented, as it is not generated by Equelle front-end...");
+// The reference CUDA backend throws on these two (EquelleRuntimeCUDA.cpp:311-319); they
+// follow the serial backend here (EquelleRuntimeCPU.cpp:345-365).
+CollOfScalar EquelleRuntimeCUDA::negGradient(const CollOfScalar& x) const {
+    return CollOfScalar(core_->neg_gradient(x.data()));
 }
-CollOfScalar EquelleRuntimeCUDA::interiorDivergence(const CollOfScalar&) const {
-    OPM_THROW(std::runtime_error, "Not yet implemented, as it is not generated by Equelle front-end...");
+CollOfScalar EquelleRuntimeCUDA::interiorDivergence(const CollOfScalar& x) const {
+    return CollOfScalar(core_->interior_divergence(x.data()));
 }
+CollOfScalar EquelleRuntimeCUDA::norm(const CollOfScalar& x) const { return CollOfScalar(eqhip::s_norm(x.data())); }
 CollOfScalar EquelleRuntimeCUDA::gradient(const CollOfScalar& x) const { return CollOfScalar(core_->gradient(x.data())); }
 CollOfScalar EquelleRuntimeCUDA::gradient_matrix(const CollOfScalar& x) const {
     return CollOfScalar(core_->gradient_matrix(x.data()));

@@ -2074,9 +2074,11 @@ struct HeatBatch::Impl {
     // that reads or replaces those fields (flush_copies)
     std::vector<McJob> pend;
     DevPtr<char> mc_dev;
-    // host image of the job table; it outlives the upload, and the next flush comes after a
-    // host synchronisation (every iterate() waits on the device)
-    std::vector<char> mc_host;
+    // pinned host image of the job table (mc_cap bytes): the upload reads it asynchronously,
+    // so the next flush waits on mc_ev (recorded after the upload) before rewriting it
+    char* mc_host = nullptr;
+    size_t mc_cap = 0;
+    hipEvent_t mc_ev = nullptr;
 
     size_t ks() const { return mixed ? sizeof(float) : sizeof(double); }
 
@@ -2094,13 +2096,19 @@ struct HeatBatch::Impl {
             const long nb = (pend[q].n + MC_EPB - 1) / MC_EPB;
             bj.insert(bj.end(), (size_t)nb, (int)q);
         }
-        const size_t jb = pend.size() * sizeof(McJob), bb = bj.size() * sizeof(int);
-        std::vector<char>& h = mc_host;
-        h.resize(jb + bb);
-        std::memcpy(h.data(), pend.data(), jb);
-        std::memcpy(h.data() + jb, bj.data(), bb);
-        if (!mc_dev || mc_dev->n < h.size()) mc_dev = dev_alloc<char>(std::max(h.size(), (size_t)65536));
-        EQ_HIP(hipMemcpyAsync(mc_dev->p, h.data(), h.size(), hipMemcpyHostToDevice, S()));
+        const size_t jb = pend.size() * sizeof(McJob), bb = bj.size() * sizeof(int), hb = jb + bb;
+        EQ_HIP(hipEventSynchronize(mc_ev));   // the previous upload has read mc_host
+        if (mc_cap < hb) {
+            if (mc_host) EQ_HIP(hipHostFree(mc_host));
+            mc_host = nullptr;
+            mc_cap = std::max(hb, (size_t)65536);
+            EQ_HIP(hipHostMalloc((void**)&mc_host, mc_cap, hipHostMallocDefault));
+        }
+        std::memcpy(mc_host, pend.data(), jb);
+        std::memcpy(mc_host + jb, bj.data(), bb);
+        if (!mc_dev || mc_dev->n < hb) mc_dev = dev_alloc<char>(std::max(hb, (size_t)65536));
+        EQ_HIP(hipMemcpyAsync(mc_dev->p, mc_host, hb, hipMemcpyHostToDevice, S()));
+        EQ_HIP(hipEventRecord(mc_ev, S()));
         if (!bj.empty()) {
             hipLaunchKernelGGL(k_multi_copy, dim3((unsigned)bj.size()), dim3(BS), 0, S(), (const McJob*)mc_dev->p,
                                (const int*)(mc_dev->p + jb));
@@ -2669,7 +2677,9 @@ struct HeatBatch::Impl {
             }
             for (; k < blind; ++k) EQ_HIP(hipGraphLaunch(exec, S()));
             bool done = k >= maxit;
-            if (!done) {
+            if (done) {   // every allowed iteration was queued blind: read the count once
+                count_active(t, 0, &iters);
+            } else {
                 EQ_HIP(hipGraphLaunch(exec, S()));
                 ++k;
                 done = count_active(t, 0, &iters) == 0;
@@ -2768,6 +2778,7 @@ HeatBatch::HeatBatch(const FusedConfig& cfg) : p_(new Impl()) {
     p_->use_graph = cfg.graph;
     EQ_HIP(hipHostMalloc((void**)&p_->host_count, 2 * Impl::NRING * sizeof(int), hipHostMallocCoherent));
     for (int i = 0; i < Impl::NRING; ++i) EQ_HIP(hipEventCreateWithFlags(&p_->ev[i], hipEventDisableTiming));
+    EQ_HIP(hipEventCreateWithFlags(&p_->mc_ev, hipEventDisableTiming));
 }
 
 HeatBatch::~HeatBatch() {
@@ -2776,6 +2787,8 @@ HeatBatch::~HeatBatch() {
         if (p_->ev[i]) (void)hipEventDestroy(p_->ev[i]);
     if (p_->host_count) (void)hipHostFree(p_->host_count);
     if (p_->gpin) (void)hipHostFree(p_->gpin);
+    if (p_->mc_ev) (void)hipEventDestroy(p_->mc_ev);
+    if (p_->mc_host) (void)hipHostFree(p_->mc_host);
     delete p_;
 }
 

@@ -86,6 +86,7 @@ __global__ void k_coef(int kind, const double* __restrict__ a, const double* __r
         case 1: r = -(a[i] / (b[i] * b[i])); break;
         case 2: r = -(s / (a[i] * a[i])); break;
         case 3: r = 0.5 / a[i]; break;
+        case 5: r = a[i] >= 0.0 ? 1.0 : -1.0; break;   // sign factor of norm(CollOfScalar)
         default: r = 1.0 / (b[i] * b[i]); break;
     }
     out[i] = r;

@@ -184,6 +184,8 @@ DeviceGrid::DeviceGrid(GridHost g) : host(std::move(g)) {
         }
         rp[nif] = 2 * nif;
         grad = csr_from_host(nif, nc, rp.data(), ci.data(), v.data());
+        for (double& x : v) x = -x;   // ngrad: +1 first cell, -1 second cell (same pattern)
+        ngrad = csr_from_host(nif, nc, rp.data(), ci.data(), v.data());
     }
     auto transpose_signed = [&](bool interior_only) {
         // rows = cells; entries over the cell's faces in ascending face order:
@@ -303,6 +305,18 @@ SP RuntimeCore::gradient(const SP& x) const {
 SP RuntimeCore::gradient_matrix(const SP& x) const {
     EQ_REQUIRE(x->n == grid_->nc, "Gradient need input defined on AllCells()");
     return s_matmul(grid_->grad, x);
+}
+SP RuntimeCore::neg_gradient(const SP& x) const {
+    EQ_REQUIRE(x->n == grid_->nc, "negGradient needs input defined on AllCells()");
+    // values u[first] - u[second] (the gradient sweep with the face cells swapped);
+    // Jacobian ngrad * J, the pattern of grad * J with negated values
+    auto v = dev_alloc<double>(std::max(1, grid_->nif));
+    launch_gradient(x->val->p, grid_->if_c2->p, grid_->if_c1->p, v->p, grid_->nif);
+    return s_matmul(grid_->ngrad, x, v);
+}
+SP RuntimeCore::interior_divergence(const SP& x) const {
+    EQ_REQUIRE(x->n == grid_->nif, "interiorDivergence needs input defined on InteriorFaces()");
+    return divergence(x);   // div * x: the interior branch of divergence
 }
 SP RuntimeCore::divergence(const SP& x) const {
     const DeviceGrid& g = *grid_;

@@ -41,7 +41,7 @@ struct DeviceGrid {
     DevPtr<int> face_cells, cell_facepos, cell_faces, signed_faces, if_c1, if_c2;
     DevPtr<double> cell_centroids, cell_volumes, face_centroids, face_areas, face_normals;
     IP all_cells, all_faces, interior_faces, boundary_faces, interior_cells, boundary_cells;
-    DevCsr grad, div, fulldiv;   // HelperOps (deviceHelperOps.cpp:84-148)
+    DevCsr grad, ngrad, div, fulldiv;   // HelperOps (deviceHelperOps.cpp:84-148)
     explicit DeviceGrid(GridHost g);
 };
 
@@ -59,6 +59,10 @@ public:
     SP gradient(const SP& x) const;
     SP gradient_matrix(const SP& x) const;
     SP divergence(const SP& x) const;
+    // EquelleRuntimeCPU::negGradient / interiorDivergence (EquelleRuntimeCPU.cpp:345-365):
+    // ngrad * x on AllCells() and div * x on InteriorFaces()
+    SP neg_gradient(const SP& x) const;
+    SP interior_divergence(const SP& x) const;
     SP divergence_matrix(const SP& x) const;
     SP solve_for_update(const SP& residual, LinReport* rep) const;
     // Index set [start, start + len) of a combined unknown (newtonSolveSystem ranges);

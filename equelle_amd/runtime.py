@@ -376,10 +376,14 @@ class EquelleRuntimeCUDA:
 
     # ---------------- geometry ----------------
     def norm(self, x):
-        """|set| (volumes/areas), or the 2-norm of a CollOfVector."""
+        """|set| (volumes/areas), the 2-norm of a CollOfVector, or the elementwise absolute
+        value of a CollOfScalar (EquelleRuntimeCPU.cpp:241-254)."""
         if isinstance(x, CollOfVector):
             return x.norm()
         out = C.c_int()
+        if isinstance(x, CollOfScalar):
+            self.api.norm_scalar(self._rt, x.h, C.byref(out))
+            return CollOfScalar(self, out.value)
         self.api.norm_set(self._rt, x.h, C.byref(out))
         return CollOfScalar(self, out.value)
 
@@ -409,11 +413,17 @@ class EquelleRuntimeCUDA:
         self.api.divergence(self._rt, x.h, C.byref(out))
         return CollOfScalar(self, out.value)
 
-    def negGradient(self, x):
-        raise EquelleError("Not yet implemented, as it is not generated by Equelle front-end...")
+    def negGradient(self, x: CollOfScalar) -> CollOfScalar:
+        """ngrad * x (EquelleRuntimeCPU.cpp:345-348)."""
+        out = C.c_int()
+        self.api.neg_gradient(self._rt, x.h, C.byref(out))
+        return CollOfScalar(self, out.value)
 
-    def interiorDivergence(self, x):
-        raise EquelleError("Not yet implemented, as it is not generated by Equelle front-end...")
+    def interiorDivergence(self, x: CollOfScalar) -> CollOfScalar:
+        """div * x on InteriorFaces() (EquelleRuntimeCPU.cpp:362-365)."""
+        out = C.c_int()
+        self.api.interior_divergence(self._rt, x.h, C.byref(out))
+        return CollOfScalar(self, out.value)
 
     def operatorExtend(self, data, a, b=None):
         out = C.c_int()

@@ -63,6 +63,9 @@ public:
         return normOfSet(set);
     }
     CollOfScalar norm(const CollOfVector& vectors) const;
+    /// Elementwise absolute value of a collection, derivatives included
+    /// (EquelleRuntimeCPU::norm(const CollOfScalar&), EquelleRuntimeCPU.cpp:241-254).
+    CollOfScalar norm(const CollOfScalar& scalars) const;
     template <int codim>
     CollOfVector centroid(const CollOfIndices<codim>& set) const {
         return centroidOfSet(set);

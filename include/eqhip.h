@@ -95,6 +95,13 @@ int eqhip_sqrt(void* rt, int a, int* h);                        /* sqrt — Eque
 int eqhip_trinary(void* rt, int pred, int t, int f, int* h);    /* trinaryIf — EquelleRuntimeCUDA.cpp:239-248 */
 int eqhip_gradient(void* rt, int a, int* h);                    /* gradient — EquelleRuntimeCUDA.cpp:251-264 */
 int eqhip_divergence(void* rt, int a, int* h);                  /* divergence — EquelleRuntimeCUDA.cpp:273-291 */
+/* negGradient: ngrad * a on AllCells() (u[first] - u[second] per interior face) and
+ * interiorDivergence: div * a on InteriorFaces() — EquelleRuntimeCPU.cpp:345-365 (the CUDA
+ * backend throws on both, EquelleRuntimeCUDA.cpp:311-319) */
+int eqhip_neg_gradient(void* rt, int a, int* h);
+int eqhip_interior_divergence(void* rt, int a, int* h);
+/* norm(CollOfScalar): |a| with Jacobian diag(sign) a' — EquelleRuntimeCPU.cpp:241-254 */
+int eqhip_norm_scalar(void* rt, int a, int* h);
 int eqhip_extend(void* rt, int a, int from, int to, int* h);    /* operatorExtend — _impl.hpp:112-125 */
 int eqhip_extend_scalar(void* rt, double s, int to, int* h);    /* operatorExtend(Scalar, set) — _impl.hpp:127-131 */
 int eqhip_on(void* rt, int a, int from, int to, int* h);        /* operatorOn — _impl.hpp:134-147 */

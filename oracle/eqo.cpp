@@ -367,6 +367,15 @@ ADB adb_scal_div_coll(double s, const ADB& a) {
 
 ADB adb_neg(const ADB& a) { return adb_scal_mul(-1.0, a); }
 
+// norm(CollOfScalar) (EquelleRuntimeCPU.cpp:241-254): a constant collection of factors,
+// +1 where the value is >= 0 and -1 elsewhere, times the collection
+ADB adb_norm(const ADB& a) {
+    ADB f;
+    f.v.resize(a.size());
+    for (int i = 0; i < a.size(); ++i) f.v[i] = a.v[i] >= 0.0 ? 1.0 : -1.0;
+    return adb_mul(f, a);
+}
+
 ADB adb_sqrt(const ADB& a) {
     ADB o;
     o.v.resize(a.size());

@@ -103,6 +103,7 @@ ADB adb_div(const ADB& a, const ADB& b);
 ADB adb_scal_mul(double s, const ADB& a);
 ADB adb_scal_div_coll(double s, const ADB& a);   // s / a
 ADB adb_neg(const ADB& a);
+ADB adb_norm(const ADB& a);   // norm(CollOfScalar): elementwise absolute value
 ADB adb_sqrt(const ADB& a);
 ADB adb_matmul(const Csr& m, const ADB& a);      // Matrix * CollOfScalar
 ADB adb_gather(const ADB& a, const std::vector<int>& rows);           // On

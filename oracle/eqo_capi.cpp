@@ -354,6 +354,24 @@ int eqo_divergence(void* rt, int a, int* h) {
         *h = Rt::put(r->scal, r->rt->divergence(r->S(a)));
     });
 }
+int eqo_neg_gradient(void* rt, int a, int* h) {
+    return guard([&] {
+        Rt* r = R(rt);
+        if (r->S(a).size() != r->rt->grid().nc) throw std::runtime_error("negGradient needs input defined on AllCells()");
+        *h = Rt::put(r->scal, r->rt->negGradient(r->S(a)));
+    });
+}
+int eqo_interior_divergence(void* rt, int a, int* h) {
+    return guard([&] {
+        Rt* r = R(rt);
+        if (r->S(a).size() != (int)r->rt->ops().internal_faces.size())
+            throw std::runtime_error("interiorDivergence needs input defined on InteriorFaces()");
+        *h = Rt::put(r->scal, r->rt->interiorDivergence(r->S(a)));
+    });
+}
+int eqo_norm_scalar(void* rt, int a, int* h) {
+    return guard([&] { *h = Rt::put(R(rt)->scal, adb_norm(R(rt)->S(a))); });
+}
 int eqo_extend(void* rt, int a, int from, int to, int* h) {
     return guard([&] {
         Rt* r = R(rt);

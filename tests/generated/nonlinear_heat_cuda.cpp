@@ -1,5 +1,7 @@
 
 // This program was created by the Equelle compiler from SINTEF.
+// (Written by hand in the CUDA printer's conventions for nonlinear_heat.equelle in this
+// folder; the compiler is not built here.)
 
 #include <opm/common/utility/parameters/ParameterGroup.hpp>
 #include <opm/common/ErrorMacros.hpp>

@@ -1,4 +1,7 @@
 // This program was created by the Equelle compiler from SINTEF.
+// (Written by hand in the CUDA printer's conventions for twophase_corey.equelle in this
+// folder, the examples/dsl/twophase.equelle program with Corey exponent 2; the compiler
+// is not built here.)
 
 #include <opm/common/utility/parameters/ParameterGroup.hpp>
 #include <opm/common/ErrorMacros.hpp>

@@ -170,6 +170,10 @@ def autodiff_sequence(er):
     r["trinary"] = er.trinaryIf(x > c, x * x, c - x)
     r["trinary2"] = er.trinaryIf(x >= 0.0, c, x)
     r["on_first"] = er.operatorOn(x, allc, er.firstCell(ifs))
+    r["neg_grad"] = er.negGradient(x * x - c)
+    r["int_div"] = er.interiorDivergence(r["grad"] * r["neg_grad"])
+    r["norm"] = er.norm(x - c)
+    r["norm_const"] = er.norm(c - pos)
     red = {k: getattr(er, k + "Reduce")(x) for k in ("min", "max", "sum")}
     red["twoNorm"] = er.twoNorm(x)
     red["prod"] = er.prodReduce(er.operatorOn(pos, allc, bc))

@@ -250,6 +250,52 @@ def test_ad_rules_finite_difference(oracle_api):
     assert np.allclose(_ad_expr(er, er.collOfScalar(u0), c).value(), R.value(), rtol=1e-14)
 
 
+def test_neg_gradient_interior_divergence_and_norm(oracle_api):
+    """negGradient, interiorDivergence and norm(CollOfScalar) (EquelleRuntimeCPU.cpp:241-254,
+    345-365) against what the mathematics fixes: ngrad = -grad entry by entry, div * x equals
+    the divergence of interior fluxes, |x| with Jacobian diag(sign x) x' (finite differences)."""
+    er, rng = _rand_grid_runtime(oracle_api, 2)
+    n = er.nc
+    u0 = rng.uniform(0.5, 1.5, n)
+    U = er.primaryVariable(er.collOfScalar(u0))
+    one = er.collOfScalar(np.ones(n))
+    q = U * U - one
+    g, ng = er.gradient(q), er.negGradient(q)
+    rp, ci, v, rows, cols = g.derivative()
+    nrp, nci, nv, nrows, ncols = ng.derivative()
+    assert (rows, cols) == (nrows, ncols) and np.array_equal(rp, nrp) and np.array_equal(ci, nci)
+    assert np.array_equal(nv, -v) and np.array_equal(ng.value(), -g.value())
+    # u[first] - u[second] on each interior face
+    ifs = er.interiorFaces()
+    f1 = er.operatorOn(q, er.allCells(), er.firstCell(ifs)).value()
+    f2 = er.operatorOn(q, er.allCells(), er.secondCell(ifs)).value()
+    assert np.allclose(ng.value(), f1 - f2, rtol=0, atol=1e-15)
+    flux = g * ng
+    d1, d2 = er.interiorDivergence(flux), er.divergence(flux)
+    assert np.array_equal(d1.value(), d2.value())
+    assert all(np.array_equal(a, b) for a, b in zip(d1.derivative()[:3], d2.derivative()[:3]))
+    with pytest.raises(Exception, match="InteriorFaces"):
+        er.interiorDivergence(er.operatorExtend(flux, ifs, er.allFaces()))
+    with pytest.raises(Exception, match="AllCells"):
+        er.negGradient(flux)
+    # norm: values |x| and the derivative of |x| where x stays away from 0
+    shift = np.where(np.arange(n) % 3 == 0, 1.6, 0.1)   # mixes signs of u0 - shift
+    s = er.collOfScalar(shift)
+    N = er.norm(U - s)
+    assert np.array_equal(N.value(), np.abs(u0 - shift))
+    rp, ci, v, rows, cols = N.derivative()
+    assert np.array_equal(rp, np.arange(n + 1)) and np.array_equal(ci, np.arange(n))
+    assert np.array_equal(v, np.sign(u0 - shift))
+    h = 1e-7
+    for j in rng.choice(n, 5, replace=False):
+        e = np.zeros(n)
+        e[j] = h
+        fd = (np.abs(u0 + e - shift) - np.abs(u0 - e - shift)) / (2 * h)
+        assert np.allclose(csr_dense(rp, ci, v, rows, cols)[:, j], fd, atol=1e-7)
+    assert er.norm(s - one).derivative() is None     # a constant stays a constant
+    assert np.array_equal(er.norm(s - one).value(), np.abs(shift - 1.0))
+
+
 def test_pattern_conventions(oracle_api):
     er, rng = _rand_grid_runtime(oracle_api, 1)
     n = er.nc
