@@ -119,6 +119,7 @@ SIGNATURES = {
     "twophase_fi_run": [_S, _DP, _DP, _DP, _I, _IP, _DP, _I, _DP, _I, _DP, _DP, _IP],
     "twophase_fi_residual": [_S, _DP, _DP, _I, _IP, _DP, _I, _DP, _DP, _DP, _D, _IP, _DP, _IP, _IP, _DP],
     "heat_residual": [_S, _DP, _DP, _I, _IP, _DP, _I, _D, _IP, _DP, _IP, _IP, _DP],
+    "heat_residual_values": [_S, _DP, _DP, _I, _IP, _DP, _I, _D, _DP],
 }
 _RESTYPES = {"last_error": _S}
 

@@ -628,6 +628,24 @@ int eqo_heat_residual(const char* params, const double* u, const double* u0, int
     });
 }
 
+// Residual values only (no Jacobian) of the same program at (u, u0, dt): F has n entries.
+int eqo_heat_residual_values(const char* params, const double* u, const double* u0, int n, const int* dir_faces,
+                             const double* dir_vals, int ndir, double dt, double* F) {
+    return guard([&] {
+        Params p;
+        p.parse(params ? params : "");
+        Runtime rt(p);
+        HeatProgram prog(rt, p, std::vector<int>(dir_faces, dir_faces + ndir),
+                         std::vector<double>(dir_vals, dir_vals + ndir));
+        ADB U, U0;
+        U.v.assign(u, u + n);
+        U0.v.assign(u0, u0 + n);
+        ADB r = prog.residual(U, U0, dt);
+        if ((int)r.v.size() != n) throw std::runtime_error("residual size differs from the state size");
+        std::memcpy(F, r.v.data(), r.v.size() * sizeof(double));
+    });
+}
+
 // Timed sample of the serial path: `iters` Newton iterations (residual + Jacobian
 // assembly + linear solve) of the heat program at state u.  Returns wall seconds.
 int eqo_heat_newton_sample(const char* params, const double* u0, int n, int iters, double dt, double* seconds,

@@ -442,3 +442,31 @@ def test_heat_program_c_entry_matches_python_sequence(oracle_api):
                              dptr(F), iptr(rp2), iptr(ci2), dptr(v2))
     assert np.array_equal(rp, rp2) and np.array_equal(ci, ci2)
     assert np.allclose(v, v2, rtol=1e-14, atol=1e-15) and np.allclose(R.value(), F, rtol=1e-14)
+
+
+def test_oracle_bicgstab_ilu0_matches_direct_solve(oracle_api):
+    """The oracle's iterative solver (BiCGStab + ILU(0), taken above 40000 unknowns) is the
+    checker of every large parity test: pin it against the oracle's banded direct solve
+    and an independent scipy sparse LU on a 43,200-cell nonlinear-heat Jacobian (not
+    symmetric: k(u) differs between the two cells of a face)."""
+    nx, ny = 240, 180
+    rng = np.random.default_rng(5)
+    u = rng.uniform(0.0, 2.0, nx * ny)
+    u0 = rng.uniform(0.0, 2.0, nx * ny)
+    sol = {}
+    for solver in ("direct", "bicgstab"):
+        er = mk(oracle_api, f"grid_dim=2 nx={nx} ny={ny} oracle_linear_solver={solver} oracle_solver_tol=1e-13 "
+                            "solver_max_iter=2000")
+        res = heat_program(er, k=0.1, quadratic=True)(er.primaryVariable(er.collOfScalar(u)),
+                                                       er.collOfScalar(u0), 1.0)
+        du, iters = er.solveForUpdate(res)
+        sol[solver] = du.value()
+        rp, ci, v, rows, cols = res.derivative()
+        F = res.value()
+    J = sp.csr_matrix((v, ci, rp), shape=(rows, cols))
+    assert abs(J - J.T).max() > 1e-3                                   # not symmetric
+    ref = spla.spsolve(J.tocsc(), F)
+    scale = np.linalg.norm(ref)
+    assert np.linalg.norm(sol["direct"] - ref) <= 1e-12 * scale
+    assert np.linalg.norm(sol["bicgstab"] - ref) <= 1e-10 * scale
+    assert np.linalg.norm(J @ sol["bicgstab"] - F) <= 1e-12 * np.linalg.norm(F)
