@@ -109,6 +109,7 @@ SIGNATURES = {
     "server_submit_twophase": [_P, _S, _I, _I, _I, _DP, _DP, _DP, _I, _IP, _DP, _I, _D, _IP],
     "server_result": [_P, _I, _DP, _DP, _IP, _DP],
     "server_status": [_P, _IP, _DP],
+    "server_set_max_newton": [_P, _I, _I],
     "timing": [_I],
     "timing_query": [_S, _IP, _DP],
     "device_sync": [],

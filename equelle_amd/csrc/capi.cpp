@@ -924,6 +924,9 @@ int eqhip_server_result(void* server, int request, double* u, double* p, int* st
         }
     });
 }
+int eqhip_server_set_max_newton(void* server, int request, int max_newton) {
+    return guard([&] { SV(server).set_max_newton(request, max_newton); });
+}
 int eqhip_server_status(void* server, int* counts, double* bytes) {
     return guard([&] {
         HeatServer& s = SV(server);

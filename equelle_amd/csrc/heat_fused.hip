@@ -1951,6 +1951,7 @@ struct Inst {
     int corey = 1;
     int stage = 0;                 // 0 pressure, 1 transport
     double r0_stage = 0;           // ||F|| at the start of the current Newton solve
+    int max_newton = 0;            // per-slot Newton limit per solve (0: the batch's max_iter)
     int round = 0;                 // refinement round of the current Newton iteration (host count)
     DevPtr<double> tp;             // tx, ty, tz, pv, src (n each)
     DevPtr<double> p, sw, sw0;
@@ -2981,6 +2982,10 @@ void HeatBatch::remove(int slot) {
     p_->insts.erase(slot);
     ++p_->version;
 }
+void HeatBatch::set_max_newton(int slot, int max_newton) {
+    EQ_REQUIRE(max_newton >= 0, "max_newton must be >= 0, got " << max_newton);
+    get_inst(p_, slot).max_newton = max_newton;
+}
 int HeatBatch::size_of(int slot) const { return get_inst(p_, slot).real_n(); }
 int HeatBatch::heat_cells(int nx, int ny, const FusedConfig& cfg) { return heat_stride(nx, cfg) * ny; }
 bool HeatBatch::in_flight(int slot) const {
@@ -3067,7 +3072,8 @@ std::vector<int> HeatBatch::iterate() {
         if (fn <= c.abs_res_tol) stop = 1;
         else if (c.rel_res_tol > 0 && fn <= c.rel_res_tol * I.r0_stage) stop = 2;
         const bool bad = !std::isfinite(fn) || I.st.failed;
-        const bool out = I.st.newton_stage >= c.max_iter;
+        const int mi = I.max_newton > 0 ? std::min(I.max_newton, c.max_iter) : c.max_iter;
+        const bool out = I.st.newton_stage >= mi;
         if (I.kind == K_TWOPHASE && I.stage == 0 && !bad && (stop || out)) {
             // pressure solved: the saturation solve starts from sw = 0.5 (twophase.equelle:107)
             I.st.stop_p = stop;

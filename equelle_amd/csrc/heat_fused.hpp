@@ -108,6 +108,9 @@ public:
     int add_twophase(const TwoPhaseSpec& spec, const double* perm, const double* poro, const double* sw,
                      int nsrc, const int* src_cells, const double* src_vals);
     void remove(int slot);
+    // Newton iterations allowed per solve for this slot (at most the batch's max_iter;
+    // 0 restores the batch's limit).  A solve that reaches it stops with reason 0.
+    void set_max_newton(int slot, int max_newton);
     int size_of(int slot) const;
     // Starts one implicit time step of length dt for the slot.
     void begin_step(int slot, double dt);

@@ -106,6 +106,7 @@ std::vector<int> HeatServer::progress() {
             r.t_admit = Clock::now();
             r.info.queued_s = secs(r.t_submit, r.t_admit);
             running_[cls][r.slot] = r.id;
+            if (r.max_newton) batch_[cls].set_max_newton(r.slot, r.max_newton);
             batch_[cls].begin_step(r.slot, r.dt);
             queue_[cls].pop_front();
             admitted = true;
@@ -150,6 +151,15 @@ std::vector<int> HeatServer::progress() {
         finished.push_back(r.id);
     }
     return finished;
+}
+
+void HeatServer::set_max_newton(int id, int max_newton) {
+    auto it = reqs_.find(id);
+    EQ_REQUIRE(it != reqs_.end(), "unknown request " << id);
+    EQ_REQUIRE(max_newton >= 0, "max_newton must be >= 0, got " << max_newton);
+    Req& r = it->second;
+    r.max_newton = max_newton;
+    if (r.slot >= 0) batch_[r.twophase ? 1 : 0].set_max_newton(r.slot, max_newton);
 }
 
 void HeatServer::take_result(int id, double* u, double* p, RequestInfo* info) {

@@ -62,6 +62,10 @@ public:
     // Copies a finished request's field (heat: u; two-phase: sw, and p when given; n
     // values each) and releases it.
     void take_result(int id, double* u, double* p, RequestInfo* info);
+    // Newton iterations per solve allowed for one request (at most the server's max_iter;
+    // 0: the server's limit), queued or in flight.  A step that reaches it without
+    // converging fails the request; the other requests are unaffected.
+    void set_max_newton(int id, int max_newton);
     int queued() const { return (int)(queue_[0].size() + queue_[1].size()); }
     int in_flight() const { return (int)(running_[0].size() + running_[1].size()); }
     double used_bytes() const { return used_; }
@@ -74,6 +78,7 @@ private:
         TwoPhaseSpec spec;
         double k = 0, cv = 1, dt = 0;
         int steps = 0;
+        int max_newton = 0;
         std::vector<double> u;    // initial field until admitted, result after completion
         std::vector<double> perm, poro, src_vals, p;
         std::vector<int> src_cells;

@@ -90,6 +90,9 @@ class Server:
             self.lib.server_submit(self.h, r.side, r.side, r.k0, 1.0, dptr(r.u0), r.steps, r.dt, C.byref(rid))
         return rid.value
 
+    def set_max_newton(self, rid: int, max_newton: int):
+        self.lib.server_set_max_newton(self.h, rid, max_newton)
+
     def progress(self):
         nf = C.c_int()
         self.lib.server_progress(self.h, iptr(self._fin), len(self._fin), C.byref(nf))

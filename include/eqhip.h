@@ -277,6 +277,10 @@ int eqhip_server_progress(void* server, int* finished, int cap, int* nfinished);
  * the pressure; one value per cell) and releases it;
  * stats = {steps, Newton, Krylov, failed}, times = {queued seconds, latency seconds} */
 int eqhip_server_result(void* server, int request, double* u, double* p, int* stats, double* times);
+/* Newton iterations per solve allowed for one request, queued or in flight (at most the
+ * server's max_iter; 0 restores it).  A step that reaches the limit unconverged ends the
+ * request with failed = 1; no other request is affected. */
+int eqhip_server_set_max_newton(void* server, int request, int max_newton);
 /* counts = {queued, in flight}, bytes = {admitted footprint, budget} */
 int eqhip_server_status(void* server, int* counts, double* bytes);
 

@@ -17,7 +17,8 @@ EXE_TP = os.path.join(GEN, "bin", "twophase_corey_cuda")
 EXE_FI = os.path.join(GEN, "bin", "twophase_fully_implicit_cuda")
 
 
-@pytest.mark.parametrize("src", ["nonlinear_heat_cuda.cpp", "twophase_corey_cuda.cpp"])
+@pytest.mark.parametrize("src", ["nonlinear_heat_cuda.cpp", "twophase_corey_cuda.cpp", "heateq_cuda.cpp",
+                                 "exp_heateq_cuda.cpp", "twophase_fully_implicit_cuda.cpp"])
 def test_generated_program_compiles(tmp_path, src):
     exe = tmp_path / "prog"
     subprocess.check_call(["g++", "-std=c++17", "-O1", "-I" + os.path.join(ROOT, "include"),
@@ -120,3 +121,127 @@ def test_generated_fully_implicit_runs_and_matches_oracle(tmp_path, oracle_api):
     po, so, _ = oracle_fi_run(oracle_api, params, perm, poro, sw0, sc, sv, dts)
     assert np.allclose(sw_last, so, rtol=1e-9, atol=1e-10)
     assert np.allclose(p_last - p_last[-1], po - po[-1], rtol=1e-9, atol=1e-9 * np.abs(po).max())
+
+
+EXE_HEATEQ = os.path.join(GEN, "bin", "heateq_cuda")
+EXE_EXP = os.path.join(GEN, "bin", "exp_heateq_cuda")
+WEBDEMO = os.path.join(ROOT, "tests", "golden", "webdemo_2dheateq")
+
+
+def _webdemo_params():
+    """The parameter file the web demo's runner writes for webdemo/srv/examples/2dheateq
+    (config.json: 2D, 100 x 100 cells of 10 x 10, abs_res_tol 1e-4; one `<tag>_from_file` /
+    `<tag>_filename=../<tag>` pair per input file; executableRun.js:26-61), in its order."""
+    lines = ["grid_dim=2", "ny=100", "dy=10", "nx=100", "dx=10", "abs_res_tol=0.0001"]
+    for tag in ("timesteps", "dir_boundary", "dir_val", "u_initial"):
+        lines += [f"{tag}_from_file=true", f"{tag}_filename=../{tag}"]
+    return "\n".join(lines + ["output_to_file=true", "verbose=2"]) + "\n"
+
+
+@pytest.mark.gpu
+def test_generated_heateq_runs_the_webdemo_inputs(tmp_path, oracle_api):
+    """examples/dsl/heateq.equelle in printer form on the reference's own web-demo inputs,
+    unchanged (the four files of webdemo/srv/examples/2dheateq under their input tags),
+    with the parameter file the demo writes.  Every u-NNNNN.output (the initial field and
+    ten steps of dt = 1e8 s; granite defaults k = 2.85, cv = 2.17e6) against the oracle's
+    run of the same program solved to 1e-12.  Bar: the program stops at |F| <= 1e-4 after
+    a BiCGStab solve to 1e-8 relative, one Newton iteration per step (the problem is
+    linear); with ||J^-1|| <= 1 the fields may differ by 1e-8 ||F0|| per step, so 1e-6 of
+    the largest temperature."""
+    import shutil
+    assert os.path.exists(EXE_HEATEQ), "run __graft_entry__.build() first"
+    files = {"timesteps": "timesteps.mat", "dir_boundary": "dir_boundary", "dir_val": "dir_val",
+             "u_initial": "u_initial"}
+    for tag, name in files.items():
+        shutil.copyfile(os.path.join(WEBDEMO, name), tmp_path / tag)
+    (tmp_path / "params.param").write_text(_webdemo_params())
+    run = tmp_path / "outputs"
+    run.mkdir()
+    out = subprocess.run([EXE_HEATEQ, "../params.param"], cwd=run, check=True, timeout=300, capture_output=True,
+                         text=True).stdout
+    assert out.count("Newton solver converged") == 10, out[-2000:]
+    u0 = np.loadtxt(os.path.join(WEBDEMO, "u_initial"))
+    dirf = np.loadtxt(os.path.join(WEBDEMO, "dir_boundary"), dtype=np.int32)
+    dirv = np.loadtxt(os.path.join(WEBDEMO, "dir_val"))
+    dts = np.loadtxt(os.path.join(WEBDEMO, "timesteps.mat"))
+    assert np.array_equal(np.loadtxt(run / "u-00000.output"), u0)
+    params = ("grid_dim=2 nx=100 ny=100 dx=10 dy=10 k=2.85 cv=2170000 k_model=const abs_res_tol=0 "
+              "rel_res_tol=1e-12 max_iter=10").encode()
+    uo = u0.copy()
+    for s, dt in enumerate(dts):
+        nxt = np.zeros_like(uo)
+        oracle_api.heat_run(params, dptr(uo), uo.size, iptr(dirf), dptr(dirv), len(dirf), dptr(np.array([dt])), 1,
+                            dptr(nxt), None, None)
+        uo = nxt
+        got = np.loadtxt(run / f"u-{s + 1:05d}.output")
+        assert np.abs(got - uo).max() <= 1e-6 * np.abs(uo).max(), (s, np.abs(got - uo).max())
+    maxima = [float(line.split("=")[1]) for line in out.splitlines() if line.startswith("maximum of u = ")]
+    assert len(maxima) == 11 and maxima[-1] == pytest.approx(uo.max(), rel=1e-5)   # printed to 6 digits
+
+
+def _simulator_inputs(d, nx, ny, nz, timesteps, dt):
+    """The input files of examples/simulators/heateq/createDirichlet.m for an nx x ny x nz
+    grid: timesteps.mat (timesteps x dt), dir_bnd.mat (both x-faces of every row of
+    cells, ascending), dir_val.mat (per (y, z): a value falling linearly from 1000 to 0 over
+    y at x = 0, its mirror in y at x = max, listed y-major as the script writes them)."""
+    (d / "timesteps.mat").write_text(" ".join(f"{dt:.7e}" for _ in range(timesteps)) + "\n")
+    idx = np.zeros(2 * ny * nz, dtype=np.int64)
+    for i in range(1, ny * nz):
+        idx[2 * i - 1] = i * (nx + 1) - 1
+        idx[2 * i] = i * (nx + 1)
+    idx[-1] = ny * nz * (nx + 1) - 1
+    (d / "dir_bnd.mat").write_text("".join(f"{v}\n" for v in idx))
+    val_y = np.linspace(1000.0, 0.0, ny)
+    vals = np.zeros(2 * ny * nz)
+    for y in range(ny):
+        for z in range(nz):
+            vals[2 * (y * nz + z)] = val_y[y]
+            vals[2 * (y * nz + z) + 1] = val_y[ny - 1 - y]
+    (d / "dir_val.mat").write_text("".join(f"{v:.7e}\n" for v in vals))
+    return idx.astype(np.int32), np.array([float(f"{v:.7e}") for v in vals]), np.full(timesteps, float(f"{dt:.7e}"))
+
+
+@pytest.mark.gpu
+def test_generated_exp_heateq_runs_the_simulator_params(tmp_path, oracle_api):
+    """examples/simulators/heateq/exp_heateq.equelle (explicit heat with Dirichlet faces) in
+    printer form, run with that simulator's params.param keys unchanged except the grid
+    size, which performanceTest.sh also rewrites per case, and the inputs its
+    createDirichlet.m writes for that size (120 steps of dt = 0.5, uniform u_initial = 500).
+    Every expU output against the same program evaluated on the oracle."""
+    from equelle_amd.runtime import EquelleRuntimeCUDA
+    assert os.path.exists(EXE_EXP), "run __graft_entry__.build() first"
+    nx, ny, nz, nsteps = 20, 24, 10, 120
+    dirf, dirv, dts = _simulator_inputs(tmp_path, nx, ny, nz, nsteps, 0.5)
+    (tmp_path / "params.param").write_text("\n".join([
+        "grid_dim=3", f"nx={nx}", "dx=1", f"ny={ny}", "dy=1", f"nz={nz}", "dz=1", "k=0.3",
+        "u_initial_from_file=false", "u_initial=500",
+        "dirichlet_boundary_from_file=true", "dirichlet_boundary_filename=../dir_bnd.mat",
+        "dirichlet_val_from_file=true", "dirichlet_val_filename=../dir_val.mat",
+        "timesteps_from_file=true", "timesteps_filename=../timesteps.mat",
+        "output_to_file=true", "solver=CG", "preconditioner=diagonal"]))
+    run = tmp_path / "run"
+    run.mkdir()
+    subprocess.run([EXE_EXP, "../params.param"], cwd=run, check=True, timeout=300, stdout=subprocess.DEVNULL)
+    er = EquelleRuntimeCUDA(f"grid_dim=3 nx={nx} ny={ny} nz={nz} dx=1 dy=1 dz=1", api=oracle_api)
+    allc, allf, bf, ifs = er.allCells(), er.allFaces(), er.boundaryFaces(), er.interiorFaces()
+    db = er.indices(dirf, 1)
+    dval = er.collOfScalar(dirv)
+    vol = er.norm(allc)
+    first, second = er.firstCell(ifs), er.secondCell(ifs)
+    itrans = 0.3 * (er.norm(ifs) / er.norm(er.centroid(first) - er.centroid(second)))
+    emp = er.isEmpty(er.firstCell(bf))
+    bf_cells = er.trinaryIf(emp, er.secondCell(bf), er.firstCell(bf))
+    bf_sign = er.trinaryIf(emp, er.operatorExtend(-1.0, bf), er.operatorExtend(1.0, bf))
+    btrans = 0.3 * (er.norm(bf) / er.norm(er.centroid(bf) - er.centroid(bf_cells)))
+    dir_sign = er.operatorOn(bf_sign, bf, db)
+    u = er.collOfScalar(np.full(nx * ny * nz, 500.0))
+    for s, dt in enumerate(dts):
+        udir = er.operatorOn(u, allc, er.operatorOn(bf_cells, bf, db))
+        dflux = (er.operatorOn(btrans, bf, db) * dir_sign) * (udir - dval)
+        fluxes = er.operatorExtend((-itrans) * er.gradient(u), ifs, allf) + \
+            er.operatorExtend(er.operatorExtend(dflux, db, bf), bf, allf)
+        u = u - (dt / vol) * er.divergence(fluxes)
+        if s % 20 == 19 or s == 0:
+            got = np.loadtxt(run / f"expU-{s:05d}.output")
+            assert np.allclose(got, u.value(), rtol=1e-12, atol=1e-9), (s, np.abs(got - u.value()).max())
+    assert np.allclose(np.loadtxt(run / f"expU-{nsteps:05d}.output"), u.value(), rtol=1e-12, atol=1e-9)

@@ -22,14 +22,22 @@ PARAMS = "precision=mixed preconditioner=ilu0 rel_res_tol=1e-8 abs_res_tol=0 max
 
 
 def alone(api, r):
+    """The request run alone through the batch API (two-phase: the request loop's FP64
+    Krylov precision, saturation returned)."""
+    from equelle_amd.serving import TWOPHASE_INSTANCE
     b = C.c_void_p()
-    api.batch_create(PARAMS.encode(), C.byref(b))
+    params = PARAMS.replace("precision=mixed", "precision=fp64") if r.twophase else PARAMS
+    api.batch_create(params.encode(), C.byref(b))
     try:
         s = C.c_int()
-        api.batch_add(b, r.side, r.side, r.k0, 1.0, dptr(r.u0), C.byref(s))
+        if r.twophase:
+            api.batch_add_twophase(b, TWOPHASE_INSTANCE.encode(), r.side, r.side, r.side, dptr(r.perm), dptr(r.poro),
+                                   dptr(r.u0), len(r.src_cells), iptr(r.src_cells), dptr(r.src_vals), C.byref(s))
+        else:
+            api.batch_add(b, r.side, r.side, r.k0, 1.0, dptr(r.u0), C.byref(s))
         for _ in range(r.steps):
             api.batch_step(b, iptr(np.array([s.value], dtype=np.int32)), 1, r.dt)
-        u = np.zeros(r.side * r.side)
+        u = np.zeros(r.n)
         api.batch_state(b, s.value, dptr(u))
         return u
     finally:
@@ -166,3 +174,54 @@ def test_mixed_heat_and_twophase_requests(hip_api):
                 assert np.array_equal(results[q][1], p), q
         finally:
             hip_api.batch_destroy(b)
+
+
+def test_failed_requests_are_isolated(hip_api):
+    """A request with a NaN in its initial field and a request limited to one Newton
+    iteration per step (which needs two or more at 1e-8) run inside a mix of ordinary
+    heat and two-phase requests.  Both come back flagged failed after their first step;
+    every other request is bit-identical to its run alone; at the end no request is queued
+    or in flight and the admitted footprint is back to zero.  The NaN is data, not a
+    fault: the batch finds it in the first residual norm and retires the request before
+    any Krylov solve (SURVEY's failure-isolation requirement; the reference's serial solve
+    throws instead, EquelleRuntimeCPU.cpp:436-438)."""
+    from equelle_amd.serving import twophase_request
+    nan_field = gaussian_bumps(70, 70, 71, width_scale=0.04)
+    nan_field[70 * 35 + 12] = np.nan
+    reqs = [Request(0.0, 64, 0.2, 3, gaussian_bumps(64, 64, 70, width_scale=0.04)),
+            Request(0.0, 70, 0.3, 3, nan_field),                       # NaN initial field
+            twophase_request(0.0, 10, 2, seed=72),
+            Request(0.0, 120, 0.45, 3, gaussian_bumps(120, 120, 73, width_scale=0.06)),   # Newton-limited
+            Request(0.0, 45, 0.1, 2, gaussian_bumps(45, 45, 74, width_scale=0.03))]
+    bad = {1, 3}
+    srv = Server(hip_api, PARAMS + " hbm_budget=1e9")
+    try:
+        ids, results = {}, {}
+
+        def tick():
+            for rid in srv.progress():
+                q = ids[rid]
+                results[q] = srv.result(rid, reqs[q].n)
+
+        for q, r in enumerate(reqs):
+            rid = srv.submit(r)
+            ids[rid] = q
+            if q == 3:
+                srv.set_max_newton(rid, 1)
+            tick()   # later requests join mid-flight; the NaN request is retired in its first tick
+        for _ in range(4000):
+            if len(results) == len(reqs):
+                break
+            tick()
+        assert len(results) == len(reqs)
+        counts, used = srv.status()
+        assert counts.tolist() == [0, 0] and used[0] == 0.0
+    finally:
+        srv.close()
+    for q, r in enumerate(reqs):
+        u, st, _ = results[q]
+        if q in bad:
+            assert st[3] == 1 and st[0] == 1, (q, st)   # failed after its first step
+            continue
+        assert st[3] == 0 and st[0] == r.steps, (q, st)
+        assert np.array_equal(u, alone(hip_api, r)), q
