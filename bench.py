@@ -21,8 +21,18 @@ per launch as stated per kernel.  roofline: the kernel with the largest estimate
 per step (avg_us x launches per step from the iteration counts).
 cpu_baseline: the CPU oracle (serial C++ restatement of the reference serial backend)
 timed for one Newton iteration of the same workload on one host core.
+Every leg runs its W warmup steps and is then reset to its initial state, so the K timed
+steps are the workload's steps 1..K.
 
-  python bench.py [--steps K] [--warmup W] [--size N] [--no-cpu-baseline] [--quick]
+The default run also carries compact summaries of the other configured workloads:
+  batched      512 nonlinear-heat instances (sides 32..512) in one mixed-precision batch
+               against the plain FP64 path one instance at a time
+  twophase     the 3D two-phase workload (128 x 128 x 64), all 30 configured steps: the
+               fused engine (FP64 and mixed) and the configured method (FP32 GMRES(30) +
+               Jacobi with FP64 refinement on the drop-in AD engine)
+  serving      the request loop at two Poisson arrival rates, p50/p99 per request kind
+
+  python bench.py [--steps K] [--warmup W] [--size N] [--no-cpu-baseline] [--quick] [--no-extras]
 """
 import argparse
 import ctypes as C
@@ -74,6 +84,7 @@ def run_fused(lib, precision, u0, nx, ny, dt, steps, warmup, precond="ilu0", gra
             lib.batch_step(b, iptr(slots), 1, dt)
             lib.batch_stats(b, slot.value, iptr(st), dptr(nrm))
             log(f"  warmup {w}: newton={st[0]} krylov={st[1]} refine={st[2]} {time.perf_counter() - t:.4f}s")
+        lib.batch_set_state(b, slot.value, dptr(u0))   # the timed steps are steps 1..K from u0
         lib.device_sync()
         newton = krylov = refine = 0
         lines = []   # printed after the timed steps
@@ -125,6 +136,7 @@ def run_generic(lib, u0, nx, ny, dt, steps, warmup):
     try:
         for _ in range(warmup):
             lib.heat_step(sim, dt, iptr(stats), None)
+        lib.heat_set_state(sim, dptr(u0))   # timed steps are steps 1..K from u0
         lib.device_sync()
         newton = 0
         t0 = time.perf_counter()
@@ -138,22 +150,24 @@ def run_generic(lib, u0, nx, ny, dt, steps, warmup):
     return dict(seconds=secs, newton=newton)
 
 
-def bench_batched(lib, args):
+def batched_runs(lib, args, steps, warmup, seq_steps, generic=True):
     """512 independent nonlinear-heat instances (sides log-uniform 32..512, k0 ~ U[0.05,0.5]).
 
-    headline              one fused batch, mixed precision, all instances in every launch
+    batched_mixed         one fused batch, mixed precision, all instances in every launch
+                          (`steps` timed steps)
     sequential_fp64_fused the plain FP64 path of the same engine, one instance at a time
+                          (`seq_steps` timed steps: a rate)
     sequential_fp64_generic  the drop-in AD engine (FP64), one instance at a time, on every
                           8th instance (a rate, so the subsample keeps the size mix)
-    Both fused legs run the same W warmup + K timed steps from the same initial fields."""
+    Every leg runs its warmup steps, then resets each instance to its initial field, so the
+    timed steps are the workload's first steps."""
     from equelle_amd.workloads import batched_instances
     inst = batched_instances(args.instances, seed=1234)
     cells = np.array([s * s for s, _, _ in inst], dtype=np.int64)
     dt = 1.0
-    steps = args.steps
     common = f"rel_res_tol={NONLINEAR_HEAT['rel_res_tol']} abs_res_tol=0 max_iter=20 solver_tol=1e-6"
 
-    def run(prec, batched):
+    def run(prec, batched, nsteps):
         b = C.c_void_p()
         lib.batch_create(f"precision={prec} preconditioner={args.precond} {common}".encode(), C.byref(b))
         try:
@@ -175,12 +189,14 @@ def bench_batched(lib, args):
                     raise RuntimeError(f"instance {bad[0]} did not converge: {st[bad[0]].tolist()}")
                 return int(np.dot(cells, st[:, 0].astype(np.int64)))
 
-            for _ in range(args.warmup):
+            for _ in range(warmup):
                 one_step()
+            for (_, _, u0), sl in zip(inst, slots):
+                lib.batch_set_state(b, int(sl), dptr(u0))
             lib.device_sync()
             work = 0
             t0 = time.perf_counter()
-            for _ in range(steps):
+            for _ in range(nsteps):
                 work += one_step()
             lib.device_sync()
             secs = time.perf_counter() - t0
@@ -189,7 +205,7 @@ def bench_batched(lib, args):
                 lib.batch_state(b, int(sl), dptr(u[q]))
             shape = np.zeros(3, dtype=np.int32)
             lib.batch_shape(b, iptr(shape))
-            return dict(value=work / secs, ms_per_step=1e3 * secs / steps, newton_cells=work,
+            return dict(value=work / secs, ms_per_step=1e3 * secs / nsteps, steps=nsteps, newton_cells=work,
                         launch_shape={"cells_per_thread": int(shape[0]), "xcd_block_order": int(shape[1]),
                                       "signatures_tuned": int(shape[2])}), u
         finally:
@@ -197,77 +213,98 @@ def bench_batched(lib, args):
 
     log(f"[bench] batched: {len(inst)} instances, {cells.sum()} cells, mixed precision, one batch")
     res = {}
-    res["batched_mixed"], ub = run("mixed", True)
-    log("[bench] plain FP64, one instance at a time (fused engine)")
-    res["sequential_fp64_fused"], us = run("fp64", False)
-    err = max(float(np.linalg.norm(a - c) / np.linalg.norm(c)) for a, c in zip(ub, us))
-    log("[bench] plain FP64, one instance at a time (generic drop-in AD engine), every 8th instance")
-    e, ed = np.zeros(1, dtype=np.int32), np.zeros(1)
-    stats = np.zeros(3, dtype=np.int32)
-    work, secs = 0, 0.0
-    for side, k0, u0 in inst[::8]:
-        sim = C.c_void_p()
-        params = heat_params(side, side, k0, precision="fp64", precond="ilu0",
-                             rel_res_tol=NONLINEAR_HEAT["rel_res_tol"], solver_tol=1e-6)
-        lib.heat_create(params.encode(), dptr(u0), side * side, iptr(e), dptr(ed), 0, C.byref(sim))
-        try:
-            lib.heat_step(sim, dt, iptr(stats), None)     # first call builds the symbolic cache
-            lib.device_sync()
-            t0 = time.perf_counter()
-            lib.heat_step(sim, dt, iptr(stats), None)
-            lib.device_sync()
-            secs += time.perf_counter() - t0
-            work += side * side * int(stats[0])
-        finally:
-            lib.heat_destroy(sim)
-    res["sequential_fp64_generic"] = dict(value=work / secs, instances=len(inst[::8]))
+    res["batched_mixed"], ub = run("mixed", True, steps)
+    log("[bench] batched: plain FP64, one instance at a time (fused engine)")
+    res["sequential_fp64_fused"], us = run("fp64", False, seq_steps)
+    if seq_steps == steps:
+        err = max(float(np.linalg.norm(a - c) / np.linalg.norm(c)) for a, c in zip(ub, us))
+    else:
+        err = None
+    if generic:
+        log("[bench] batched: plain FP64, one instance at a time (generic drop-in AD engine), every 8th instance")
+        e, ed = np.zeros(1, dtype=np.int32), np.zeros(1)
+        stats = np.zeros(3, dtype=np.int32)
+        work, secs = 0, 0.0
+        for side, k0, u0 in inst[::8]:
+            sim = C.c_void_p()
+            params = heat_params(side, side, k0, precision="fp64", precond="ilu0",
+                                 rel_res_tol=NONLINEAR_HEAT["rel_res_tol"], solver_tol=1e-6)
+            lib.heat_create(params.encode(), dptr(u0), side * side, iptr(e), dptr(ed), 0, C.byref(sim))
+            try:
+                lib.heat_step(sim, dt, iptr(stats), None)     # first call builds the symbolic cache
+                lib.heat_set_state(sim, dptr(u0))
+                lib.device_sync()
+                t0 = time.perf_counter()
+                lib.heat_step(sim, dt, iptr(stats), None)
+                lib.device_sync()
+                secs += time.perf_counter() - t0
+                work += side * side * int(stats[0])
+            finally:
+                lib.heat_destroy(sim)
+        res["sequential_fp64_generic"] = dict(value=work / secs, instances=len(inst[::8]), steps=1)
     head = res["batched_mixed"]
-    out = {"metric": "cell-Newton-iters/s", "value": head["value"], "unit": "cell-Newton-iters/s",
-           "n_gpus": 1, "steps": steps, "warmup": args.warmup, "ms_per_step": head["ms_per_step"],
+    out = {"value": head["value"], "unit": "cell-Newton-iters/s", "ms_per_step": head["ms_per_step"],
+           "instances": len(inst), "cells": int(cells.sum()), "runs": res,
+           "batched_over_sequential_fp64_fused": head["value"] / res["sequential_fp64_fused"]["value"]}
+    if generic:
+        out["batched_over_sequential_fp64_generic"] = head["value"] / res["sequential_fp64_generic"]["value"]
+    if err is not None:
+        out["mixed_vs_fp64_max_rel_error"] = err
+    return out
+
+
+def bench_batched(lib, args):
+    """The batched workload on its own: one JSON line (value = batched mixed precision)."""
+    r = batched_runs(lib, args, args.steps, args.warmup, args.steps)
+    out = {"metric": "cell-Newton-iters/s", "value": r["value"], "unit": "cell-Newton-iters/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": r["ms_per_step"],
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "f64 residual/Jacobian/refinement, f32 Krylov", "data": "synthetic",
-           "config": {"workload": f"{len(inst)} nonlinear-heat instances in one batch, sides log-uniform "
+           "config": {"workload": f"{r['instances']} nonlinear-heat instances in one batch, sides log-uniform "
                                   "32..512, k0 ~ U[0.05,0.5], dt=1, Newton to 1e-8 relative residual each",
-                      "cells": int(cells.sum()), "precond": args.precond},
-           "runs": res,
-           "batched_over_sequential_fp64_fused": head["value"] / res["sequential_fp64_fused"]["value"],
-           "batched_over_sequential_fp64_generic": head["value"] / res["sequential_fp64_generic"]["value"],
-           "mixed_vs_fp64_max_rel_error": err}
+                      "cells": r["cells"], "precond": args.precond}}
+    out.update({k: v for k, v in r.items() if k not in ("value", "unit", "ms_per_step")})
     print(json.dumps(out), flush=True)
 
 
-def bench_twophase(lib, args):
+def twophase_runs(lib, args, steps, warmup, generic_legs):
     """3D two-phase transport (twophase.equelle, Corey 2, viscosity ratio 5): per step a
     pressure Newton solve and a saturation Newton solve, each to 1e-8 relative residual.
     Work = cells x (pressure + transport Newton iterations).
 
-    headline      fused_fp64: fused stencil engine, FP64: the pressure stage by CG with the
-                  aggregation multigrid V-cycle, the saturation stage by BiCGStab with the
-                  red-black ILU(0) (per step a pressure and a saturation Newton solve)
+    fused_fp64    fused stencil engine, FP64: the pressure stage by CG with the aggregation
+                  multigrid V-cycle, the saturation stage by BiCGStab with the red-black ILU(0)
     fused_mixed   the same with FP32 Krylov and FP64 refinement; a step that does not
                   converge is recorded under runs.fused_mixed.failed instead of ending the run
-    generic_*     the drop-in path: the printer-style program on the device AD engine with
-                  GMRES(30) + Jacobi, FP32 with FP64 refinement (generic_mixed) and FP64"""
+    generic_*     the drop-in path: the printer-style program on the device AD engine;
+                  generic_legs = [(name, solver settings, steps)], e.g. the configured
+                  method FP32 GMRES(30) + Jacobi with FP64 refinement
+    Every leg runs its warmup steps and is then reset to the initial state (pressure 0,
+    the initial saturation), so the timed steps are the workload's steps 1..K; each
+    generic leg's saturation is compared with fused_fp64's after the same steps."""
     from equelle_amd.workloads import TWOPHASE, twophase_inputs, twophase_params
     nx, ny, nz = (int(v) for v in args.grid3.split("x"))
     n = nx * ny * nz
     perm, poro, sw0, sc, sv = twophase_inputs(nx, ny, nz, rate=TWOPHASE["rate"], porosity=TWOPHASE["porosity"])
+    p0 = np.zeros(n)
     dt = TWOPHASE["dt"]
     runs = {}
     inst = f"watervisc={TWOPHASE['watervisc']} oilvisc={TWOPHASE['oilvisc']} corey={TWOPHASE['corey']}".encode()
+    marks = sorted({g for _, _, g in generic_legs})   # fused_fp64 states kept after these step counts
+    ref_at = {}
+
     def failed(name, err):
         log(f"[bench] two-phase {name}: {err}")
         runs[name] = {"failed": str(err)}
 
-    gsteps = max(1, min(args.steps, 2))   # timed steps of the generic legs (about 2 s each)
     legs = [(f"fused_{p}", p) for p in args.tp_legs.split(",")]
-    EQ = {"fp64", "mixed"}
-    if not legs or any(p not in EQ for _, p in legs) or legs[0][1] != "fp64":
+    if not legs or any(p not in ("fp64", "mixed") for _, p in legs) or legs[0][1] != "fp64":
         raise SystemExit("--tp-legs: fp64 first, then optionally mixed")
     for name, prec in legs:
         b = C.c_void_p()
         lib.batch_create(f"precision={prec} preconditioner={args.precond} rel_res_tol={TWOPHASE['rel_res_tol']} "
-                         f"abs_res_tol=0 max_iter=20 solver_tol=1e-6 solver_max_iter=5000".encode(), C.byref(b))
+                         f"abs_res_tol=0 max_iter=20 solver_tol=1e-6 solver_max_iter=5000 "
+                         f"graph={0 if args.no_graph else 1}".encode(), C.byref(b))
         try:
           try:
             slot = C.c_int()
@@ -275,18 +312,19 @@ def bench_twophase(lib, args):
                                    dptr(sv), C.byref(slot))
             sl = np.array([slot.value], dtype=np.int32)
             st, tps = np.zeros(5, dtype=np.int32), np.zeros(3, dtype=np.int32)
-            for w_ in range(args.warmup):
+            for w_ in range(warmup):
                 lib.batch_step(b, iptr(sl), 1, dt)
                 lib.batch_stats(b, slot.value, iptr(st), None)
                 lib.batch_twophase_stats(b, slot.value, iptr(tps))
                 if st[3] == 0 or st[4] or tps[2] == 0:
                     raise RuntimeError(f"{name}: warmup step {w_} did not converge: {st.tolist()} {tps.tolist()}")
+            lib.batch_set_twophase_state(b, slot.value, dptr(p0), dptr(sw0))
             lib.device_sync()
             newton = krylov = kp = 0
             excluded = 0.0
             lines = []   # printed after the timed steps
             t0 = time.perf_counter()
-            for s_ in range(args.steps):
+            for s_ in range(steps):
                 lib.batch_step(b, iptr(sl), 1, dt)
                 lib.batch_stats(b, slot.value, iptr(st), None)
                 lib.batch_twophase_stats(b, slot.value, iptr(tps))
@@ -297,17 +335,18 @@ def bench_twophase(lib, args):
                 kp += int(tps[1])
                 lines.append(f"[bench] two-phase {name} step {s_}: Newton p/s {tps[0]}/{st[0] - tps[0]}, "
                              f"Krylov p/s {tps[1]}/{st[1] - tps[1]}")
-                if s_ == gsteps - 1:   # the state the generic legs end in (they run gsteps steps)
+                if prec == "fp64" and s_ + 1 in marks:   # the states the generic legs end in
                     tr = time.perf_counter()
-                    pg, sw_g = np.zeros(n), np.zeros(n)
-                    lib.batch_twophase_state(b, slot.value, dptr(pg), dptr(sw_g))
+                    sw_m = np.zeros(n)
+                    lib.batch_twophase_state(b, slot.value, None, dptr(sw_m))
+                    ref_at[s_ + 1] = sw_m
                     excluded += time.perf_counter() - tr   # the copy is not part of the step
             lib.device_sync()
             secs = time.perf_counter() - t0 - excluded
             for ln in lines:
                 log(ln)
-            p, sw = np.zeros(n), np.zeros(n)
-            lib.batch_twophase_state(b, slot.value, dptr(p), dptr(sw))
+            sw = np.zeros(n)
+            lib.batch_twophase_state(b, slot.value, None, dptr(sw))
           except RuntimeError as e:   # FP32 Krylov on this pressure system: see the docstring
             if prec == "fp64":
                 raise
@@ -315,69 +354,75 @@ def bench_twophase(lib, args):
             continue
         finally:
             lib.batch_destroy(b)
-        runs[name] = dict(value=n * newton / secs, ms_per_step=1e3 * secs / args.steps, newton=newton,
-                          krylov_pressure=kp, krylov_transport=krylov - kp, sw=sw, sw_g=sw_g)
-    if not args.quick:
-        for name, extra in (("generic_mixed", f"precision=mixed solver={args.tp_solver} preconditioner=diagonal"),
-                            ("generic_fp64", f"precision=fp64 solver={args.tp_solver} preconditioner=diagonal")):
-            params = twophase_params(nx, ny, nz, rel_res_tol=TWOPHASE["rel_res_tol"], extra=extra).encode()
-            sim = C.c_void_p()
-            lib.twophase_create(params, dptr(perm), dptr(poro), dptr(sw0), n, iptr(sc), dptr(sv), len(sc),
-                                C.byref(sim))
-            try:
-              try:
-                st = np.zeros(6, dtype=np.int32)
-                for _ in range(args.warmup):
-                    lib.twophase_step(sim, dt, iptr(st), None)
-                lib.device_sync()
-                newton, krylov = 0, [0, 0]
-                t0 = time.perf_counter()
-                for s_ in range(gsteps):
-                    lib.twophase_step(sim, dt, iptr(st), None)
-                    if st[2] == 0 or st[5] == 0:
-                        raise RuntimeError(f"{name}: step {s_} did not converge: {st.tolist()}")
-                    newton += int(st[0]) + int(st[3])
-                    krylov[0] += int(st[1])
-                    krylov[1] += int(st[4])
-                    log(f"[bench] two-phase {name} step {s_}: Newton p/s {st[0]}/{st[3]}, Krylov p/s {st[1]}/{st[4]}")
-                lib.device_sync()
-                secs = time.perf_counter() - t0
-                p, sw = np.zeros(n), np.zeros(n)
-                lib.twophase_state(sim, dptr(p), dptr(sw))
-              except RuntimeError as e:
-                if name == "generic_fp64":
-                    raise
-                failed(name, e)
-                continue
-            finally:
-                lib.twophase_destroy(sim)
-            runs[name] = dict(value=n * newton / secs, ms_per_step=1e3 * secs / gsteps, newton=newton, steps=gsteps,
-                              krylov_pressure=krylov[0], krylov_transport=krylov[1], sw=sw)
-    # fused legs against fused_fp64 after the same steps; generic legs (fewer steps) against
-    # fused_fp64's state after their step count
-    ref, ref_g = runs["fused_fp64"]["sw"], runs["fused_fp64"]["sw_g"]
+        runs[name] = dict(value=n * newton / secs, ms_per_step=1e3 * secs / steps, steps=steps, newton=newton,
+                          krylov_pressure=kp, krylov_transport=krylov - kp, sw=sw)
+    for name, extra, gsteps in generic_legs:
+        params = twophase_params(nx, ny, nz, rel_res_tol=TWOPHASE["rel_res_tol"], extra=extra).encode()
+        sim = C.c_void_p()
+        lib.twophase_create(params, dptr(perm), dptr(poro), dptr(sw0), n, iptr(sc), dptr(sv), len(sc), C.byref(sim))
+        try:
+          try:
+            st = np.zeros(6, dtype=np.int32)
+            for _ in range(min(warmup, 1)):   # one warmup step builds the symbolic cache
+                lib.twophase_step(sim, dt, iptr(st), None)
+            lib.twophase_set_state(sim, dptr(p0), dptr(sw0))
+            lib.device_sync()
+            newton, krylov = 0, [0, 0]
+            t0 = time.perf_counter()
+            for s_ in range(gsteps):
+                lib.twophase_step(sim, dt, iptr(st), None)
+                if st[2] == 0 or st[5] == 0:
+                    raise RuntimeError(f"{name}: step {s_} did not converge: {st.tolist()}")
+                newton += int(st[0]) + int(st[3])
+                krylov[0] += int(st[1])
+                krylov[1] += int(st[4])
+                log(f"[bench] two-phase {name} step {s_}: Newton p/s {st[0]}/{st[3]}, Krylov p/s {st[1]}/{st[4]}")
+            lib.device_sync()
+            secs = time.perf_counter() - t0
+            p, sw = np.zeros(n), np.zeros(n)
+            lib.twophase_state(sim, dptr(p), dptr(sw))
+          except RuntimeError as e:
+            failed(name, e)
+            continue
+        finally:
+            lib.twophase_destroy(sim)
+        runs[name] = dict(value=n * newton / secs, ms_per_step=1e3 * secs / gsteps, newton=newton, steps=gsteps,
+                          method=extra, krylov_pressure=krylov[0], krylov_transport=krylov[1], sw=sw)
+    ref = runs["fused_fp64"]["sw"]
     errs = {}
     for k, r in runs.items():
         if "sw" not in r:
             continue
-        want = ref_g if k.startswith("generic") else ref
+        want = ref_at[r["steps"]] if k.startswith("generic") else ref
         errs[k] = float(np.linalg.norm(r["sw"] - want) / np.linalg.norm(want))
     for r in runs.values():
         r.pop("sw", None)
-        r.pop("sw_g", None)
     head = runs["fused_fp64"]
-    out = {"metric": "cell-Newton-iters/s", "value": head["value"], "unit": "cell-Newton-iters/s", "n_gpus": 1,
-           "steps": args.steps, "warmup": args.warmup, "ms_per_step": head["ms_per_step"],
-           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-           "dtype": "f64", "data": "synthetic",
-           "config": {"workload": f"3D two-phase transport {nx}x{ny}x{nz}, log-normal permeability, Corey 2, "
-                                  f"viscosity ratio 5, dt={dt}, Newton to 1e-8 relative residual per solve",
-                      "cells": n, "precond": args.precond},
-           "runs": runs, "sw_rel_error_vs_fused_fp64": errs}
+    out = {"value": head["value"], "unit": "cell-Newton-iters/s", "ms_per_step": head["ms_per_step"],
+           "steps": steps, "cells": n, "runs": runs, "sw_rel_error_vs_fused_fp64": errs}
     if "value" in runs.get("fused_mixed", {}):
         out["fused_mixed_over_fused_fp64"] = runs["fused_mixed"]["value"] / head["value"]
-    if "generic_fp64" in runs:
-        out["fused_fp64_over_generic_fp64"] = head["value"] / runs["generic_fp64"]["value"]
+    for name, _, _ in generic_legs:
+        if "value" in runs.get(name, {}):
+            out[f"fused_fp64_over_{name}"] = head["value"] / runs[name]["value"]
+    return out
+
+
+def bench_twophase(lib, args):
+    """The two-phase workload on its own: one JSON line (value = fused FP64)."""
+    from equelle_amd.workloads import TWOPHASE
+    legs = [] if args.quick else [
+        ("generic_mixed", f"precision=mixed solver={args.tp_solver} preconditioner=diagonal", args.steps),
+        ("generic_fp64", f"precision=fp64 solver={args.tp_solver} preconditioner=diagonal", max(1, min(args.steps, 2)))]
+    r = twophase_runs(lib, args, args.steps, args.warmup, legs)
+    out = {"metric": "cell-Newton-iters/s", "value": r["value"], "unit": "cell-Newton-iters/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": r["ms_per_step"],
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"3D two-phase transport {args.grid3}, log-normal permeability, Corey 2, "
+                                  f"viscosity ratio 5, dt={TWOPHASE['dt']}, Newton to 1e-8 relative residual per "
+                                  "solve", "cells": r["cells"], "precond": args.precond}}
+    out.update({k: v for k, v in r.items() if k not in ("value", "unit", "ms_per_step", "steps", "cells")})
     print(json.dumps(out), flush=True)
 
 
@@ -516,9 +561,32 @@ def cpu_baseline(n_side, u0, dt):
     secs, lin = C.c_double(), C.c_int()
     api.heat_newton_sample(params, dptr(u0), u0.size, 1, dt, C.byref(secs), C.byref(lin))
     return dict(value=u0.size / secs.value, unit="cell-Newton-iters/s", cores=1, kind="port",
+                cpu_model=cpu_model(),
                 sample=f"1 Newton iteration (AD residual+Jacobian, BiCGStab/ILU(0) to 1e-6, {lin.value} "
                        f"iterations) of the {n_side}x{n_side} nonlinear-heat step, serial C++ oracle: "
                        f"{secs.value:.2f} s")
+
+
+def cpu_model():
+    """The host CPU's model name and logical CPU count (the box's own /proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            names = [ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")]
+        return f"{names[0]} ({len(names)} logical CPUs visible)" if names else "unknown"
+    except OSError:
+        return "unknown"
+
+
+def serving_compact(lib, args):
+    """The request loop at two arrival rates: latency percentiles per request kind."""
+    from equelle_amd.serving import serving_sweep
+    sweep = serving_sweep(lib, [float(r) for r in args.default_rates.split(",")], args.default_requests,
+                          frac3d=args.frac3d, precond=args.precond, hbm_budget_gb=args.hbm_budget_gb)
+    keep = ("rate_req_s", "requests", "p50_ms", "p99_ms", "heat2d", "twophase3d", "throughput_req_s",
+            "cell_newton_iters_per_s", "failed")
+    return {"workload": f"Poisson stream of {args.default_requests} requests per rate, {100 * (1 - args.frac3d):.0f}% "
+                        f"2D heat (16..1024), {100 * args.frac3d:.0f}% 3D two-phase (16..128), 1-20 steps each",
+            "sweep": [{k: s[k] for k in keep if k in s} for s in sweep]}
 
 
 def main():
@@ -529,6 +597,11 @@ def main():
     ap.add_argument("--size", type=int, default=NONLINEAR_HEAT["nx"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline run only (for profiling)")
+    ap.add_argument("--no-extras", dest="no_extras", action="store_true",
+                    help="heat headline legs only (no batched, two-phase and serving summaries)")
+    ap.add_argument("--default-rates", dest="default_rates", default="50,500",
+                    help="serving rates of the default run's compact summary")
+    ap.add_argument("--default-requests", dest="default_requests", type=int, default=100)
     ap.add_argument("--precond", default="ilu0", help="preconditioner of the fused runs: ilu0 | diagonal | none")
     ap.add_argument("--workload", default="heat2048",
                     choices=["heat2048", "batched", "serving", "twophase", "twophase_fi", "copy"])
@@ -626,6 +699,18 @@ def main():
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (serial oracle, one Newton iteration)")
         out["cpu_baseline"] = cpu_baseline(nx, u0, dt)
+    if not args.quick and not args.no_extras:
+        # the other configured workloads, compact: batched (512 instances), the 3D two-phase
+        # workload's 30 steps with its configured method beside the fused engine, the request loop
+        from equelle_amd.workloads import TWOPHASE
+        w = min(args.warmup, 2)
+        out["batched"] = batched_runs(lib, args, args.steps, w, 2)
+        out["twophase"] = twophase_runs(
+            lib, args, TWOPHASE["steps"], w,
+            [("generic_configured", "precision=mixed solver=GMRes preconditioner=diagonal", TWOPHASE["steps"])])
+        out["twophase"]["configured_method"] = ("generic_configured: FP64 residual/Jacobian, FP32 GMRES(30) + "
+                                                "Jacobi, FP64 iterative refinement (the drop-in AD engine)")
+        out["serving"] = serving_compact(lib, args)
     print(json.dumps(out), flush=True)
 
 

@@ -75,6 +75,7 @@ SIGNATURES = {
     "heat_create": [_S, _DP, _I, _IP, _DP, _I, C.POINTER(_P)],
     "heat_step": [_P, _D, _IP, _DP],
     "heat_state": [_P, _DP],
+    "heat_set_state": [_P, _DP],
     "heat_destroy": [_P],
     "batch_create": [_S, C.POINTER(_P)],
     "batch_destroy": [_P],
@@ -92,6 +93,7 @@ SIGNATURES = {
     "batch_stats_many": [_P, _IP, _I, _IP],
     "batch_state": [_P, _I, _DP],
     "batch_set_state": [_P, _I, _DP],
+    "batch_set_twophase_state": [_P, _I, _DP, _DP],
     "batch_bytes": [_P, _I, _DP],
     "batch_kernel_time": [_P, _S, _I, _DP, _DP],
     "batch_shape": [_P, _IP],
@@ -100,6 +102,7 @@ SIGNATURES = {
     "twophase_create": [_S, _DP, _DP, _DP, _I, _IP, _DP, _I, C.POINTER(_P)],
     "twophase_step": [_P, _D, _IP, _DP],
     "twophase_state": [_P, _DP, _DP],
+    "twophase_set_state": [_P, _DP, _DP],
     "twophase_residual": [_P, _I, _DP, _DP, _DP, _D, _IP, _DP, _IP, _IP, _DP],
     "twophase_destroy": [_P],
     "server_create": [_S, C.POINTER(_P)],

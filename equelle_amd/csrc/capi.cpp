@@ -511,6 +511,14 @@ int eqhip_heat_state(void* sim, double* u) {
     });
 }
 
+int eqhip_heat_set_state(void* sim, const double* u) {
+    return guard([&] {
+        EQ_REQUIRE(sim, "null simulation");
+        HeatSim* s = static_cast<HeatSim*>(sim);
+        s->u = equelleCUDA::CollOfScalar(std::vector<double>(u, u + s->u.size()));
+    });
+}
+
 int eqhip_heat_destroy(void* sim) {
     return guard([&] { delete static_cast<HeatSim*>(sim); });
 }
@@ -618,6 +626,14 @@ int eqhip_twophase_step(void* sim, double dt, int* stats, double* norms) {
     });
 }
 
+int eqhip_twophase_set_state(void* sim, const double* p, const double* sw) {
+    return guard([&] {
+        TwoPhaseSim& s = TP(sim);
+        const int n = s.sw.size();
+        if (p) s.p = equelleCUDA::CollOfScalar(std::vector<double>(p, p + n));
+        if (sw) s.sw = equelleCUDA::CollOfScalar(std::vector<double>(sw, sw + n));
+    });
+}
 int eqhip_twophase_state(void* sim, double* p, double* sw) {
     return guard([&] {
         TwoPhaseSim& s = TP(sim);
@@ -817,6 +833,9 @@ int eqhip_batch_state(void* batch, int slot, double* u) {
 }
 int eqhip_batch_set_state(void* batch, int slot, const double* u) {
     return guard([&] { B(batch).set_state(slot, u); });
+}
+int eqhip_batch_set_twophase_state(void* batch, int slot, const double* p, const double* sw) {
+    return guard([&] { B(batch).set_twophase_state(slot, p, sw); });
 }
 int eqhip_batch_bytes(void* batch, int slot, double* bytes) {
     return guard([&] { *bytes = (double)B(batch).device_bytes(slot); });

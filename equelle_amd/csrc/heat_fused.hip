@@ -3248,12 +3248,26 @@ void HeatBatch::get_twophase_state(int slot, double* p, double* sw) const {
         std::copy(h.begin(), h.end(), sw);
     }
 }
+// State uploads copy into the instance's existing buffers: descriptors, tables and
+// captured graphs keep their pointers, so nothing is rebuilt.  A queued step-start copy
+// (u0 = u, sw0 = sw) is issued first, so an upload inside a step changes the Newton
+// start, not the state the step began from.
+static void upload(double* dst, const std::vector<double>& h) {
+    EQ_HIP(hipMemcpyAsync(dst, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, S()));
+    Device::get().sync();   // pageable source
+}
 void HeatBatch::set_state(int slot, const double* u) {
     p_->flush_copies();
     Inst& I = get_inst(p_, slot);
     EQ_REQUIRE(I.kind == K_HEAT, "set_state: slot " << slot << " is not a heat instance");
-    I.u = dev_from_host(pad_rows(I, u).data(), I.n);
-    ++p_->version;
+    upload(I.u->p, pad_rows(I, u));
+}
+void HeatBatch::set_twophase_state(int slot, const double* p, const double* sw) {
+    p_->flush_copies();
+    Inst& I = get_inst(p_, slot);
+    EQ_REQUIRE(I.kind == K_TWOPHASE, "slot " << slot << " is not a two-phase instance");
+    if (p) upload(I.p->p, std::vector<double>(p, p + I.n));
+    if (sw) upload(I.sw->p, std::vector<double>(sw, sw + I.n));
 }
 
 // CSR with ascending columns: bottom, south, west, diagonal, east, north, top (existing

@@ -125,6 +125,8 @@ public:
     void get_state(int slot, double* u) const;   // two-phase: the saturation
     void get_twophase_state(int slot, double* p, double* sw) const;
     void set_state(int slot, const double* u);
+    // Two-phase state (pressure and/or saturation; null leaves a field as it is).
+    void set_twophase_state(int slot, const double* p, const double* sw);
     // Test hook: F and the Jacobian in CSR form at the slot's current state (u0 = prev).
     void export_residual(int slot, const double* u, const double* u0, double dt, std::vector<double>& F,
                          std::vector<int>& rp, std::vector<int>& ci, std::vector<double>& v);

@@ -161,16 +161,11 @@ def summarize(stream, out, wall):
     return res
 
 
-def bench_serving(lib, args):
-    """Rate sweep of the request loop; one JSON line (value = sustained cell-Newton-iters/s
-    at the highest rate, latency percentiles per rate under `sweep`)."""
-    rates = [float(r) for r in getattr(args, "rates", "50,100,200,500,1000,2000").split(",")]
-    count = getattr(args, "requests", 200)
-    frac3d = getattr(args, "frac3d", 0.3)
-    cap = getattr(args, "max_in_flight_twophase", 4096)
-    params = (f"precision=mixed preconditioner={args.precond} rel_res_tol=1e-8 abs_res_tol=0 max_iter=20 "
-              f"solver_tol=1e-6 hbm_budget={getattr(args, 'hbm_budget_gb', 64.0) * 1e9} "
-              f"max_in_flight_twophase={cap}")
+def serving_sweep(lib, rates, count, frac3d=0.3, precond="ilu0", hbm_budget_gb=64.0, cap=4096):
+    """Replays a seeded Poisson stream of `count` requests at each rate; one summary per rate
+    (latency percentiles overall and per request kind, throughput, failures)."""
+    params = (f"precision=mixed preconditioner={precond} rel_res_tol=1e-8 abs_res_tol=0 max_iter=20 "
+              f"solver_tol=1e-6 hbm_budget={hbm_budget_gb * 1e9} max_in_flight_twophase={cap}")
     sweep = []
     for rate in rates:
         stream = poisson_stream(rate, count, seed=7, frac3d=frac3d)
@@ -189,6 +184,17 @@ def bench_serving(lib, args):
               f"{s['throughput_req_s']:.1f} req/s, {s['cell_newton_iters_per_s']:.3g} cell-Newton/s",
               file=sys.stderr, flush=True)
         sweep.append(s)
+    return sweep
+
+
+def bench_serving(lib, args):
+    """Rate sweep of the request loop; one JSON line (value = sustained cell-Newton-iters/s
+    at the highest rate, latency percentiles per rate under `sweep`)."""
+    rates = [float(r) for r in getattr(args, "rates", "50,100,200,500,1000,2000").split(",")]
+    count = getattr(args, "requests", 200)
+    frac3d = getattr(args, "frac3d", 0.3)
+    cap = getattr(args, "max_in_flight_twophase", 4096)
+    sweep = serving_sweep(lib, rates, count, frac3d, args.precond, getattr(args, "hbm_budget_gb", 64.0), cap)
     top = sweep[-1]
     line = {"metric": "cell-Newton-iters/s (serving)", "value": top["cell_newton_iters_per_s"],
             "unit": "cell-Newton-iters/s", "n_gpus": 1, "steps": count, "warmup": 1,

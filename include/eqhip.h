@@ -154,6 +154,8 @@ int eqhip_heat_create(const char* params, const double* u_initial, int n, const 
                       const double* dir_vals, int ndir, void** sim);
 int eqhip_heat_step(void* sim, double dt, int* stats, double* norms);
 int eqhip_heat_state(void* sim, double* u);
+/* replaces the session's state (the field the next step starts from) */
+int eqhip_heat_set_state(void* sim, const double* u);
 int eqhip_heat_destroy(void* sim);
 
 /* ------------------------------------------------------------------ two-phase sessions */
@@ -171,6 +173,8 @@ int eqhip_twophase_create(const char* params, const double* perm, const double* 
  * Newton, Krylov, stop in stats[0..2], stats[3..4] = 0, stats[5] = stop, norms repeated */
 int eqhip_twophase_step(void* sim, double dt, int* stats, double* norms);
 int eqhip_twophase_state(void* sim, double* p, double* sw);
+/* replaces the session's pressure and/or saturation (null leaves that field) */
+int eqhip_twophase_set_state(void* sim, const double* p, const double* sw);
 /* residual + Jacobian (CSR): which = 0 pressure residual at p (total mobility of sw0),
  * which = 1 transport residual at sw (flux from p, sw0; step dt), which = 2 the fully
  * implicit system at (p, sw) (2n rows [pressure; transport], columns [p; sw], previous
@@ -231,6 +235,8 @@ int eqhip_batch_stats(void* batch, int slot, int* stats, double* norms);
 int eqhip_batch_stats_many(void* batch, const int* slots, int nslots, int* stats);
 int eqhip_batch_state(void* batch, int slot, double* u);
 int eqhip_batch_set_state(void* batch, int slot, const double* u);
+/* two-phase slot: pressure and/or saturation (null leaves that field) */
+int eqhip_batch_set_twophase_state(void* batch, int slot, const double* p, const double* sw);
 /* device-memory footprint of the slot in bytes (admission control) */
 int eqhip_batch_bytes(void* batch, int slot, double* bytes);
 /* launch shape of the most recent launch: info = {cells per thread of the SpMV, XCD-contiguous
