@@ -37,13 +37,16 @@ enum Pc { PC_NONE = 0, PC_JACOBI = 1, PC_ILU = 2 };
 // reduced (Schur complement) system with its diagonal (= red-black ILU(0)).
 enum Pm { PM_NONE = 0, PM_JACOBI = 1, PM_ILU = 2, PM_SCHUR = 3 };
 
-enum Kind { K_HEAT = 0, K_TWOPHASE = 1 };
+// K_LINEAR: a linear system J x = b whose Jacobian has the pattern of a Cartesian 5- or
+// 7-point stencil, gathered from CSR (the drop-in AD engine's Newton step); only the
+// Krylov solve and its refinement run on it (no assembly, no Newton update).
+enum Kind { K_HEAT = 0, K_TWOPHASE = 1, K_LINEAR = 2 };
 
 struct Desc {
     int n, nx, ny, nz;
     int nxr;                      // real row length; heat rows may be padded to nx (see Inst::nxr)
     int xcd;                      // XCD-contiguous block order (same for all of a table)
-    int kind;                     // K_HEAT | K_TWOPHASE
+    int kind;                     // K_HEAT | K_TWOPHASE | K_LINEAR
     int corey;
     int gbase, ngrp;              // the instance's partials: groups [gbase, gbase + ngrp) of BS cells
     int h;                        // pair ranks (n + 1) / 2: cells per colour, rounded up
@@ -68,8 +71,8 @@ struct Desc {
     double *p, *sw, *sw0;
     double *F, *du, *r64;
     // FP64 Jacobian stencil coefficients in cell order (B, T: 3D only), stored only for
-    // two-phase instances in mixed precision (the refinement defect reads them); a heat
-    // instance recomputes its FP64 rows from u.  Null otherwise.
+    // two-phase and linear-system instances in mixed precision (the refinement defect reads
+    // them); a heat instance recomputes its FP64 rows from u.  Null otherwise.
     double *C, *W, *E, *S, *N, *B, *T;
     // Krylov-precision coefficients C, W, E, S, N, B, T in red-black rank order (2 h each)
     void* K[7];
@@ -589,6 +592,10 @@ __global__ __launch_bounds__(BS) void k_assemble(const Desc* __restrict__ D, con
                                                  double* __restrict__ part) {
     LOCATE();
     double f2[CPT] = {};
+    if (d.kind == K_LINEAR) {   // coefficients come from k_gather_stencil
+        group_store<CPT>(f2, PARTS(part));
+        return;
+    }
     if (d.kind == K_TWOPHASE) {
         const int stage = d.act[2];
         FOR_GROUPS(m, c) {
@@ -646,6 +653,53 @@ __global__ void k_heat_rows64(const Desc* __restrict__ D, double* __restrict__ F
     for (int q = 0; q < 5; ++q) co[(size_t)q * d.n + c] = r[q];
 }
 
+// Linear-system instances: the right-hand side and the stencil coefficients of a CSR
+// Jacobian whose pattern is a Cartesian 5- or 7-point stencil of the instance's grid
+// (checked on the host once per pattern): each entry of row c goes to the coefficient of
+// the neighbour it couples (C, W, E, S, N, B, T; zero where the row has no entry), in the
+// Krylov precision in red-black rank order and (MIXED) in FP64 cell order for the
+// refinement defect; F = rhs; per-group partials of ||F||^2.  Reads 12 B per entry and
+// 12 B per row, writes 8 + (MIXED: 8 +) ks bytes per coefficient and row.
+template <bool MIXED, int CPT>
+__global__ __launch_bounds__(BS) void k_gather_stencil(const Desc* __restrict__ D, const int2* __restrict__ blk,
+                                                       double* __restrict__ part, const int* __restrict__ rp,
+                                                       const int* __restrict__ ci, const double* __restrict__ val,
+                                                       const double* __restrict__ rhs) {
+    LOCATE();
+    using K = typename std::conditional<MIXED, float, double>::type;
+    double f2[CPT] = {};
+    const int nx = d.nx, ny = d.ny, nz = d.nz, nxy = nx * ny, nq = nz > 1 ? 7 : 5;
+    FOR_GROUPS(m, c) {
+        if (c >= d.n) continue;
+        int i, j, k;
+        cell_ijk(d, c, i, j, k);
+        double co[7] = {0, 0, 0, 0, 0, 0, 0};
+        const int e0 = rp[c], e1 = rp[c + 1];
+        for (int e = e0; e < e1; ++e) {
+            const int col = ci[e];
+            const double v = val[e];
+            const int q = col == c                              ? 0
+                          : (i > 0 && col == c - 1)             ? 1
+                          : (i < nx - 1 && col == c + 1)        ? 2
+                          : (j > 0 && col == c - nx)            ? 3
+                          : (j < ny - 1 && col == c + nx)       ? 4
+                          : (nz > 1 && k > 0 && col == c - nxy) ? 5
+                          : (nz > 1 && k < nz - 1 && col == c + nxy) ? 6 : -1;
+            if (q >= 0) co[q] = v;
+        }
+        const double F = rhs[c];
+        d.F[c] = F;
+        if (MIXED) {
+            double* dst[7] = {d.C, d.W, d.E, d.S, d.N, d.B, d.T};
+            for (int q = 0; q < nq; ++q) dst[q][c] = co[q];
+        }
+        const int sl = ((i + j + k) & 1) * d.h + (c >> 1);
+        for (int q = 0; q < nq; ++q) ((K*)d.K[q])[sl] = (K)co[q];
+        f2[m] = F * F;
+    }
+    group_store<CPT>(f2, PARTS(part));
+}
+
 // FP64 defect for the instances under refinement: r64 = F - J du.  Heat instances
 // recompute their FP64 Jacobian rows from u (heat_row; reads u, du 5-point and F, writes
 // r64: 32 B per cell); two-phase instances read their stored FP64 coefficients.
@@ -653,7 +707,7 @@ template <int CPT>
 __global__ __launch_bounds__(BS) void k_defect(const Desc* __restrict__ D, const int2* __restrict__ blk,
                                                double* __restrict__ part) {
     LOCATE();
-    if (d.vec) {   // heat, nx % 8 == 0: four consecutive cells of one row per lane
+    if (d.vec && d.kind == K_HEAT) {   // heat, nx % 8 == 0: four consecutive cells of one row per lane
         const bool on = d.act[1] != 0;
         const int nx = d.nx, ny = d.ny;
         const double* __restrict__ u = d.u;
@@ -718,7 +772,7 @@ __global__ __launch_bounds__(BS) void k_defect(const Desc* __restrict__ D, const
                 ro[c] = r;
                 r2[m] = r * r;
             }
-        } else if (d.C) {   // two-phase: cell-order FP64 coefficients (stored in mixed precision only)
+        } else if (d.C) {   // two-phase, linear system: cell-order FP64 coefficients (mixed precision only)
             FOR_GROUPS(m, c) {
                 if (c >= d.n) continue;
                 int i, j, k;
@@ -2227,7 +2281,7 @@ struct HeatBatch::Impl {
             d.vol = I.dx * I.dy;
             d.u = I.u->p;
             d.u0 = I.u0->p;
-        } else {
+        } else if (I.kind == K_TWOPHASE) {
             d.mw_inv = I.mw_inv;
             d.mo_inv = I.mo_inv;
             const double* t = I.tp->p;
@@ -2255,7 +2309,8 @@ struct HeatBatch::Impl {
         d.h = (int)h;
         d.pcm = I.pcm;
         d.ns = I.ns();
-        d.vec = cfg.pair_vectors && I.kind == K_HEAT && I.pcm == PM_SCHUR && I.nx % 8 == 0;
+        d.vec = cfg.pair_vectors && (I.kind == K_HEAT || (I.kind == K_LINEAR && I.nz == 1)) && I.pcm == PM_SCHUR &&
+                I.nx % 8 == 0;
         d.cgp = cg_capable(I);
         d.mgp = I.mg_on;
         d.mgl = (int)I.mg.size();
@@ -2742,6 +2797,31 @@ struct HeatBatch::Impl {
         return cfg.reduced_form(kind == K_TWOPHASE) ? PM_SCHUR : PM_ILU;
     }
     void newton_update(const Table& t) { HL(C_VEC, (k_newton_update<CP>)); }
+
+    // Linear-system instance: right-hand side and coefficients from a CSR Jacobian, ||F||^2
+    // into S_FN; a refinement left over from an earlier solve is cleared (S_CONT).
+    void gather_linear(const Table& t, const int* rp, const int* ci, const double* val, const double* rhs) {
+        if (mixed) HL(C_ASM, (k_gather_stencil<true, CP>), t.part->p, rp, ci, val, rhs);
+        else HL(C_ASM, (k_gather_stencil<false, CP>), t.part->p, rp, ci, val, rhs);
+        fin(t, Fcell(S_FN));
+        for (const Inst* I : t.inst) launch_fill(I->sc->p + S_CONT, 0.0, 1);
+    }
+    // J du = F for linear-system instances: one solve in FP64, or FP32 solves with FP64
+    // refinement rounds (one per linear_solve call) until every instance's refinement has
+    // stopped; du holds the solution.
+    void solve_linear_table(Table& t) {
+        if (!mixed) {
+            linear_solve<double>(t);
+            return;
+        }
+        // the round keys the Krylov iteration predictions (the counts repeat from call to call)
+        for (int round = 0; round <= cfg.max_refine; ++round) {
+            for (Inst* I : t.inst) I->round = round;
+            linear_solve<float>(t);
+            if (count_active(t, 1) == 0) break;
+        }
+        for (Inst* I : t.inst) I->round = 0;
+    }
 #undef HL
 #undef HLP
 
@@ -2809,9 +2889,9 @@ size_t HeatBatch::estimate_bytes(int n, const std::string& precision, bool twoph
     return b + NSLOT * 8 + 64;
 }
 
-// Newton / Krylov storage common to both instance kinds.  FP64 Jacobian coefficients in
-// cell order are stored only for two-phase instances in mixed precision (their
-// refinement defect reads them); the Krylov-precision coefficients, dinv and the Krylov
+// Newton / Krylov storage common to every instance kind.  FP64 Jacobian coefficients in
+// cell order are stored only for two-phase and linear-system instances in mixed precision
+// (their refinement defect reads them); the Krylov-precision coefficients, dinv and the Krylov
 // vectors are in red-black rank order.
 static void alloc_solver(Inst& I, bool mixed, size_t ks, double inner_tol) {
     const int n = I.n;
@@ -2819,7 +2899,7 @@ static void alloc_solver(Inst& I, bool mixed, size_t ks, double inner_tol) {
     I.F = dev_alloc<double>(n);
     I.du = dev_alloc<double>(n);
     I.r64 = dev_alloc<double>(n);
-    if (I.kind == K_TWOPHASE && mixed) {
+    if ((I.kind == K_TWOPHASE || I.kind == K_LINEAR) && mixed) {
         I.C = dev_alloc<double>(n);
         I.W = dev_alloc<double>(n);
         I.E = dev_alloc<double>(n);
@@ -2982,6 +3062,47 @@ void HeatBatch::remove(int slot) {
     p_->insts.erase(slot);
     ++p_->version;
 }
+int HeatBatch::add_linear(int nx, int ny, int nz) {
+    EQ_REQUIRE(nx > 0 && ny > 0 && nz > 0, "instance sizes must be positive");
+    EQ_REQUIRE((long)nx * ny * nz < (1L << 31) - CPT_MAX * BS, "instance too large");
+    auto I = std::make_unique<Inst>();
+    I->id = p_->next_id++;
+    I->kind = K_LINEAR;
+    I->nx = nx;
+    I->ny = ny;
+    I->nz = nz;
+    I->n = nx * ny * nz;
+    I->pcm = p_->form_of(K_LINEAR);
+    alloc_solver(*I, p_->mixed, p_->ks(), p_->cfg.inner_tol);
+    const int id = I->id;
+    p_->insts[id] = std::move(I);
+    ++p_->version;
+    return id;
+}
+
+LinearSolveStats HeatBatch::solve_linear(int slot, const int* rp, const int* ci, const double* val, const double* rhs,
+                                         double* x) {
+    Impl& P = *p_;
+    P.flush_copies();
+    Inst& I = get_inst(p_, slot);
+    EQ_REQUIRE(I.kind == K_LINEAR, "slot " << slot << " is not a linear-system instance");
+    TableP tp = P.table({&I});
+    Table& t = *tp;
+    P.gather_linear(t, rp, ci, val, rhs);
+    P.solve_linear_table(t);
+    // S_ITT: Krylov iterations of the solve; S_REF: refinement rounds; S_RN / S_BN: FP64
+    // defect over ||F|| (mixed); S_RR / S_BB: the Krylov residual over its right-hand side
+    std::vector<double> g = P.gather(t, make_int4(S_ITT, S_REF, S_RN, S_BN), S_RR);
+    std::vector<double> bb = P.gather(t, make_int4(S_BB, S_BB, S_BB, S_BB), S_BB);
+    LinearSolveStats st;
+    st.iterations = (int)g[0];
+    st.refinements = (int)g[1];
+    st.relres = P.mixed ? (g[3] > 0 ? g[2] / g[3] : 0.0) : (bb[0] > 0 ? std::sqrt(std::max(g[4], 0.0) / bb[0]) : 0.0);
+    if (g[3] == 0.0) st.relres = 0.0;   // zero right-hand side: du = 0
+    EQ_HIP(hipMemcpyAsync(x, I.du->p, (size_t)I.n * sizeof(double), hipMemcpyDeviceToDevice, S()));
+    return st;
+}
+
 void HeatBatch::set_max_newton(int slot, int max_newton) {
     EQ_REQUIRE(max_newton >= 0, "max_newton must be >= 0, got " << max_newton);
     get_inst(p_, slot).max_newton = max_newton;

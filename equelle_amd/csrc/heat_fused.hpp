@@ -94,6 +94,13 @@ struct TwoPhaseSpec {
     int corey = 1;
 };
 
+// Result of HeatBatch::solve_linear.
+struct LinearSolveStats {
+    int iterations = 0;    // Krylov iterations (all refinement rounds)
+    int refinements = 0;   // FP64 refinement rounds (mixed precision)
+    double relres = 0.0;   // FP64 defect / ||b|| (mixed), Krylov residual / its right-hand side (fp64)
+};
+
 class HeatBatch {
 public:
     explicit HeatBatch(const FusedConfig& cfg);
@@ -108,6 +115,15 @@ public:
     int add_twophase(const TwoPhaseSpec& spec, const double* perm, const double* poro, const double* sw,
                      int nsrc, const int* src_cells, const double* src_vals);
     void remove(int slot);
+    // A linear-system instance on an nx x ny x nz Cartesian grid (nz = 1: 2D): no physics;
+    // solve_linear gathers a CSR Jacobian with the grid's 5- or 7-point stencil pattern
+    // (device pointers rp, ci, val) and right-hand side b, solves J x = b with the batch's
+    // Krylov method, precision and tolerances (solver_tol, inner_tol, max_refine), and
+    // writes x (device memory, cell order).  Every CSR entry must couple a cell with itself
+    // or a grid neighbour; the caller checks the pattern.
+    int add_linear(int nx, int ny, int nz);
+    LinearSolveStats solve_linear(int slot, const int* rp, const int* ci, const double* val, const double* b,
+                                  double* x);
     // Newton iterations allowed per solve for this slot (at most the batch's max_iter;
     // 0 restores the batch's limit).  A solve that reaches it stops with reason 0.
     void set_max_newton(int slot, int max_newton);

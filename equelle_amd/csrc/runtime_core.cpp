@@ -2,6 +2,7 @@
 #include "runtime_core.hpp"
 
 #include <algorithm>
+#include <cstdio>
 #include <cstdlib>
 #include <sstream>
 
@@ -248,6 +249,7 @@ RuntimeCore::RuntimeCore(const ParamMap& p) : params_(p) {
     solver.gmres_restart = p.get_int("gmres_restart", 30);
     solver.inner_tol = p.get_double("inner_tol", 1e-4);
     solver.max_refine = p.get_int("max_refine", 30);
+    stencil_solver = p.get_bool("stencil_solver", true);
     verbose = p.get_int("verbose", 0);
     solver.verbose = verbose;
     max_iter = p.get_int("max_iter", 10);
@@ -351,7 +353,65 @@ IP RuntimeCore::range_set(int start, int len) const {
     return s;
 }
 
+// The pattern of J couples each cell only with itself (always present) and its Cartesian
+// neighbours (W, E, S, N, B, T where they exist): the 5- or 7-point stencil of the grid.
+bool RuntimeCore::stencil_pattern(const PatternP& P) const {
+    auto it = stencil_patterns_.find(P->id);
+    if (it != stencil_patterns_.end()) return it->second;
+    const GridHost& g = grid_->host;
+    const int nx = g.n[0], ny = g.n[1], nz = g.dim == 3 ? g.n[2] : 1, nxy = nx * ny;
+    bool ok = P->rows == g.nc && P->cols == g.nc;
+    for (int c = 0; ok && c < g.nc; ++c) {
+        const int i = c % nx, j = (c / nx) % ny, k = c / nxy;
+        bool diag = false;
+        for (int e = P->rp[c]; ok && e < P->rp[c + 1]; ++e) {
+            const int col = P->ci[e];
+            diag |= col == c;
+            ok = col == c || (i > 0 && col == c - 1) || (i < nx - 1 && col == c + 1) || (j > 0 && col == c - nx) ||
+                 (j < ny - 1 && col == c + nx) || (nz > 1 && k > 0 && col == c - nxy) ||
+                 (nz > 1 && k < nz - 1 && col == c + nxy);
+        }
+        ok = ok && diag;
+    }
+    stencil_patterns_[P->id] = ok;
+    return ok;
+}
+
 SP RuntimeCore::solve_for_update(const SP& residual, LinReport* rep) const {
+    // BiCGStab with red-black ILU(0) or Jacobi on a Jacobian with the grid's stencil
+    // pattern: the coefficients are gathered out of the CSR values and the solve runs on
+    // the stencil kernels (no column indices in the iteration); other systems take the
+    // CSR / SELL-64 path.
+    if (stencil_solver && solver.method == "bicgstab" && (solver.precond == "ilu0" || solver.precond == "jacobi") &&
+        residual->pat && stencil_pattern(residual->pat)) {
+        if (!stencil_) {
+            FusedConfig fc;
+            fc.precision = solver.precision;
+            fc.precond = solver.precond;
+            fc.solver_tol = solver.tol;
+            fc.solver_max_iter = solver.max_iter;
+            fc.inner_tol = solver.inner_tol;
+            fc.max_refine = solver.max_refine;
+            const GridHost& g = grid_->host;
+            stencil_.reset(new HeatBatch(fc));
+            stencil_slot_ = stencil_->add_linear(g.n[0], g.n[1], g.dim == 3 ? g.n[2] : 1);
+        }
+        const Pattern& P = *residual->pat;
+        auto x = dev_alloc<double>(std::max(1, residual->n));
+        const LinearSolveStats st = stencil_->solve_linear(stencil_slot_, P.d_rp->p, P.d_ci->p, residual->jval->p,
+                                                           residual->val->p, x->p);
+        LinReport R;
+        R.iterations = st.iterations;
+        R.refinements = st.refinements;
+        R.residual = st.relres;
+        R.converged = st.relres <= solver.tol;
+        if (solver.verbose > 0)
+            fprintf(stdout, "Linear solver (stencil %s/%s/%s): %d iterations, relative residual %g%s\n",
+                    solver.method.c_str(), solver.precond.c_str(), solver.precision.c_str(), R.iterations,
+                    R.residual, R.converged ? "" : " (not converged)");
+        if (rep) *rep = R;
+        return s_from_device(x, residual->n);
+    }
     return linear_solve(solver, residual, rep);
 }
 

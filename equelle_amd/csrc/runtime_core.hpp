@@ -9,6 +9,7 @@
 
 #include "core.hpp"
 #include "engine.hpp"
+#include "heat_fused.hpp"
 #include "krylov.hpp"
 
 namespace eqhip {
@@ -70,6 +71,7 @@ public:
     IP range_set(int start, int len) const;
 
     SolverConfig solver;
+    bool stencil_solver = true;   // stencil-pattern Jacobians: solve on the stencil kernels
     int max_iter = 10;
     double abs_res_tol = 1e-6;
     double rel_res_tol = 0.0;
@@ -77,7 +79,11 @@ public:
     bool output_to_file = false;
 
 private:
+    bool stencil_pattern(const PatternP& P) const;
     ParamMap params_;
+    mutable std::map<uint64_t, bool> stencil_patterns_;   // pattern id -> grid stencil
+    mutable std::unique_ptr<HeatBatch> stencil_;         // linear-system instance of the grid
+    mutable int stencil_slot_ = 0;
     std::unique_ptr<DeviceGrid> grid_;
     mutable std::map<std::pair<int, int>, IP> ranges_;
     // firstCell / secondCell results per (face set, side).  Generated code calls them inside

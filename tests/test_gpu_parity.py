@@ -280,6 +280,36 @@ def test_nonlinear_heat_newton_parity(hip_api, oracle_api, extra):
     assert close(out["hip"], out["oracle"], rtol=1e-8)
 
 
+@pytest.mark.parametrize("grid", ["grid_dim=2 nx=96 ny=80",            # four pair ranks per lane (nx % 8 == 0)
+                                  "grid_dim=2 nx=77 ny=53",            # strided Krylov kernels
+                                  "grid_dim=3 nx=20 ny=14 nz=9"])      # 7-point stencil
+@pytest.mark.parametrize("extra", ["preconditioner=ilu0 precision=fp64", "preconditioner=ilu0 precision=mixed",
+                                   "preconditioner=diagonal precision=fp64",
+                                   "preconditioner=diagonal precision=mixed"])
+def test_stencil_solve_matches_csr_solve_and_oracle(hip_api, oracle_api, grid, extra):
+    """A Jacobian with the grid's stencil pattern is solved on the stencil kernels (the
+    coefficients gathered out of the CSR values): the Newton trajectory equals the CSR
+    (SELL-64) solve's and the oracle's within the tolerance the solves ran to."""
+    dims = dict(kv.split("=") for kv in grid.split())
+    n = int(dims["nx"]) * int(dims["ny"]) * int(dims.get("nz", 1))
+    base = (f"{grid} k=0.1 cv=1 k_model=quadratic abs_res_tol=0 rel_res_tol=1e-10 max_iter=20 "
+            f"solver=BiCGStab solver_tol=1e-11 solver_max_iter=3000 inner_tol=1e-6 {extra}")
+    rng = np.random.default_rng(n)
+    u0 = rng.uniform(0, 2, n)
+    dts = np.array([1.0, 0.5])
+    e, ed = np.zeros(1, dtype=np.int32), np.zeros(1)
+    out = {}
+    for name, api, params in (("stencil", hip_api, base), ("csr", hip_api, base + " stencil_solver=0"),
+                              ("oracle", oracle_api, base)):
+        u = np.zeros(n)
+        it = np.zeros(len(dts), dtype=np.int32)
+        api.heat_run(params.encode(), dptr(u0), n, iptr(e), dptr(ed), 0, dptr(dts), len(dts), dptr(u), iptr(it), None)
+        out[name] = (u, it)
+    assert close(out["stencil"][0], out["oracle"][0], rtol=1e-8)
+    assert close(out["stencil"][0], out["csr"][0], rtol=1e-8)
+    assert np.array_equal(out["stencil"][1], out["csr"][1])
+
+
 # ------------------------------------------------------------------ Krylov
 @pytest.mark.parametrize("solver", ["BiCGStab", "CG", "GMRes"])
 @pytest.mark.parametrize("precond", ["diagonal", "ilu0", "none"])
