@@ -126,9 +126,9 @@ def run_fused(lib, precision, u0, nx, ny, dt, steps, warmup, precond="ilu0", gra
     return dict(seconds=secs, newton=newton, krylov=krylov, refine=refine, kern=kern, u=u)
 
 
-def run_generic(lib, u0, nx, ny, dt, steps, warmup):
-    params = heat_params(nx, ny, NONLINEAR_HEAT["k"], precision="fp64", precond="ilu0",
-                         rel_res_tol=NONLINEAR_HEAT["rel_res_tol"], solver_tol=1e-6)
+def run_generic(lib, u0, nx, ny, dt, steps, warmup, precision="fp64", extra=""):
+    params = heat_params(nx, ny, NONLINEAR_HEAT["k"], precision=precision, precond="ilu0",
+                         rel_res_tol=NONLINEAR_HEAT["rel_res_tol"], solver_tol=1e-6, extra=extra)
     sim = C.c_void_p()
     e, ed = np.zeros(1, dtype=np.int32), np.zeros(1)
     lib.heat_create(params.encode(), dptr(u0), u0.size, iptr(e), dptr(ed), 0, C.byref(sim))
@@ -138,16 +138,51 @@ def run_generic(lib, u0, nx, ny, dt, steps, warmup):
             lib.heat_step(sim, dt, iptr(stats), None)
         lib.heat_set_state(sim, dptr(u0))   # timed steps are steps 1..K from u0
         lib.device_sync()
-        newton = 0
+        newton = krylov = 0
         t0 = time.perf_counter()
         for _ in range(steps):
             lib.heat_step(sim, dt, iptr(stats), None)
             newton += int(stats[0])
+            krylov += int(stats[1])
         lib.device_sync()
         secs = time.perf_counter() - t0
+        u = np.zeros(nx * ny)
+        lib.heat_state(sim, dptr(u))
     finally:
         lib.heat_destroy(sim)
-    return dict(seconds=secs, newton=newton)
+    return dict(seconds=secs, newton=newton, krylov=krylov, u=u)
+
+
+def bench_generic(lib, args):
+    """The drop-in path alone: the nonlinear heat program through EquelleRuntimeCUDA's
+    device AD engine (one newtonSolve per step, BiCGStab + ILU(0)), FP64 and mixed
+    precision, with the Jacobian solved on the stencil kernels (its pattern is the grid's
+    5-point stencil) and, for comparison, on the CSR / SELL-64 kernels (stencil_solver=0)."""
+    nx = ny = args.size
+    n = nx * ny
+    u0 = gaussian_bumps(nx, ny, seed=2048)
+    legs = {}
+    ref = None
+    for name, prec, extra in (("fp64_stencil", "fp64", ""), ("mixed_stencil", "mixed", "inner_tol=1e-6"),
+                              ("fp64_csr", "fp64", "stencil_solver=0"),
+                              ("mixed_csr", "mixed", "stencil_solver=0 inner_tol=1e-6")):
+        if args.quick and name.endswith("_csr"):
+            continue
+        log(f"[bench] generic drop-in path {name}")
+        g = run_generic(lib, u0, nx, ny, NONLINEAR_HEAT["dt"], args.steps, args.warmup, prec, extra)
+        if ref is None:
+            ref = g["u"]
+        legs[name] = {"value": n * g["newton"] / g["seconds"], "ms_per_step": 1e3 * g["seconds"] / args.steps,
+                      "newton_iterations": g["newton"], "krylov_iterations": g["krylov"],
+                      "rel_error_vs_fp64_stencil": float(np.linalg.norm(g["u"] - ref) / np.linalg.norm(ref))}
+    head = legs["fp64_stencil"]
+    out = {"metric": "cell-Newton-iters/s", "value": head["value"], "unit": "cell-Newton-iters/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": head["ms_per_step"], "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"nonlinear heat {nx}x{ny} through the drop-in EquelleRuntimeCUDA AD engine, "
+                                  "BiCGStab + ILU(0), Newton to 1e-8 relative residual", "cells": n},
+           "legs": legs}
+    print(json.dumps(out), flush=True)
 
 
 def batched_runs(lib, args, steps, warmup, seq_steps, generic=True):
@@ -604,7 +639,7 @@ def main():
     ap.add_argument("--default-requests", dest="default_requests", type=int, default=100)
     ap.add_argument("--precond", default="ilu0", help="preconditioner of the fused runs: ilu0 | diagonal | none")
     ap.add_argument("--workload", default="heat2048",
-                    choices=["heat2048", "batched", "serving", "twophase", "twophase_fi", "copy"])
+                    choices=["heat2048", "batched", "serving", "twophase", "twophase_fi", "copy", "generic"])
     ap.add_argument("--instances", type=int, default=512)
     ap.add_argument("--inner-tol", dest="inner_tol", type=float, default=1e-6,
                     help="mixed precision: FP32 inner solve tolerance")
@@ -633,6 +668,8 @@ def main():
         return bench_twophase(lib, args)
     if args.workload == "twophase_fi":
         return bench_twophase_fi(lib, args)
+    if args.workload == "generic":
+        return bench_generic(lib, args)
     if args.workload == "serving":
         from equelle_amd.serving import bench_serving
         return bench_serving(lib, args)

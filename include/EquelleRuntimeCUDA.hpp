@@ -209,15 +209,28 @@ CollOfScalar EquelleRuntimeCUDA::newtonSolve(const ResidualFunctor& rescomp, con
     int iter = 0;
     NewtonReport rep;
     newtonIterationMessage(iter, rn);
+    // Where this solve is expected to converge (the previous newtonSolve's iteration count;
+    // every iteration of the first solve), the new residual is first evaluated on the
+    // values alone (a constant u: no Jacobian is assembled); the value kernels are those of
+    // the AD evaluation, so the norm and the stop decision are bit-identical, and the
+    // Jacobian is assembled only when another Newton step follows.
+    const int expected = last_newton_.iterations;
     while (!newtonStop(rn, r0) && iter < max_iter_) {
         int lin = 0;
         CollOfScalar du = solveForUpdate(residual, &lin);
         rep.linear_iterations += lin;
         // du is constant: u keeps its identity Jacobian
         u = u - du;
+        ++iter;
+        if (iter >= expected) {
+            const CollOfScalar values = rescomp(CollOfScalar(u.value()));
+            rn = twoNorm(values);
+            newtonIterationMessage(iter, rn);
+            if (!newtonStop(rn, r0) && iter < max_iter_) residual = rescomp(u);
+            continue;
+        }
         residual = rescomp(u);
         rn = twoNorm(residual);
-        ++iter;
         newtonIterationMessage(iter, rn);
     }
     rep.iterations = iter;
