@@ -63,6 +63,7 @@ struct Desc {
     double dt;
     // heat: k(u) = k (1 + u^2), x / y face geometry factors, cell volume
     double k, cv, gx, gy, vol;
+    double hcgx, hcgy;            // dt gx / (cv vol), dt gy / (cv vol): set with dt, once per instance
     double *u, *u0;
     // two-phase: transmissibility of the +x / +y / +z face of each cell, pore volume,
     // source rate, pressure, saturation and its value at the start of the step
@@ -457,24 +458,28 @@ __device__ inline double heat_row(const Desc& d, int i, int j, double uc, double
         co[1] = co[2] = co[3] = co[4] = 0.0;
         return 0.0;
     }
-    const double k = d.k, coef = d.dt / (d.cv * d.vol);
-    const double kc = k * (1.0 + uc * uc), dkc = 2.0 * k * uc;
-    double div = 0.0, diag = 1.0;
+    // per face (cg = dt g / (cv vol), the face's geometry factor g = area / centroid distance):
+    // k_f = (k(u_c) + k(u_m)) / 2, flux term cg k_f (u_c - u_m), dF/du_c += cg (k u_c (u_c - u_m)
+    // + k_f), dF/du_m = cg (k u_m (u_c - u_m) - k_f); about ten FP64 operations per face
+    const double k = d.k;
+    const double kfc = 0.5 * k * (2.0 + uc * uc);   // k_f less the neighbour's k u_m^2 / 2
+    const double kuc = k * uc;
+    double flux = 0.0, diag = 1.0;
     co[1] = co[2] = co[3] = co[4] = 0.0;
-    auto face = [&](double um, double g, double& cm) {
-        const double km = k * (1.0 + um * um);
-        const double kf = 0.5 * (kc + km);
+    auto face = [&](double um, double cg, double& cm) {
         const double diff = uc - um;
-        div += g * kf * diff;
-        diag += coef * g * (0.5 * dkc * diff + kf);
-        cm = coef * g * (0.5 * (2.0 * k * um) * diff - kf);
+        const double kum = k * um;
+        const double kf = (0.5 * kum) * um + kfc;
+        flux += (cg * kf) * diff;
+        diag += cg * (kuc * diff + kf);
+        cm = cg * (kum * diff - kf);
     };
-    if (i > 0) face(uw, d.gx, co[1]);
-    if (i < d.nxr - 1) face(ue, d.gx, co[2]);
-    if (j > 0) face(us, d.gy, co[3]);
-    if (j < d.ny - 1) face(un, d.gy, co[4]);
+    if (i > 0) face(uw, d.hcgx, co[1]);
+    if (i < d.nxr - 1) face(ue, d.hcgx, co[2]);
+    if (j > 0) face(us, d.hcgy, co[3]);
+    if (j < d.ny - 1) face(un, d.hcgy, co[4]);
     co[0] = diag;
-    return (uc - u0c) + coef * div;
+    return (uc - u0c) + flux;
 }
 
 // u at cell c and its four neighbours (the cell's own value where a neighbour is absent,
@@ -712,6 +717,8 @@ __global__ __launch_bounds__(BS) void k_defect(const Desc* __restrict__ D, const
         const int nx = d.nx, ny = d.ny;
         const double* __restrict__ u = d.u;
         const double* __restrict__ x = d.du;
+        const double* __restrict__ Fi = d.F;
+        double* __restrict__ ro = d.r64;
         FOR_VGROUPS(g, c0) {
             double r2 = 0.0;
             if (on && c0 < d.n) {
@@ -722,7 +729,7 @@ __global__ __launch_bounds__(BS) void k_defect(const Desc* __restrict__ D, const
                 const int so = j > 0 ? c0 - nx : c0, no = j < ny - 1 ? c0 + nx : c0;
                 const V4<double> uc = ld4(u + c0), us = ld4(u + so), un = ld4(u + no);
                 const V4<double> xc = ld4(x + c0), xs = ld4(x + so), xn = ld4(x + no);
-                const V4<double> Fc = ld4(d.F + c0);
+                const V4<double> Fc = ld4(Fi + c0);
                 const double uw0 = u[w], ue3 = u[e], xw0 = x[w], xe3 = x[e];
                 V4<double> r;
 #pragma unroll
@@ -739,7 +746,7 @@ __global__ __launch_bounds__(BS) void k_defect(const Desc* __restrict__ D, const
                     if (j < ny - 1) acc += co[4] * xn.e[m];
                     r.e[m] = Fc.e[m] - acc;
                 }
-                st4(d.r64 + c0, r);
+                st4(ro + c0, r);
                 r2 = ((r.e[0] * r.e[0] + r.e[1] * r.e[1]) + r.e[2] * r.e[2]) + r.e[3] * r.e[3];
             }
             vgroup_store(d, bi, CPT, g, r2, part, true);
@@ -2279,6 +2286,8 @@ struct HeatBatch::Impl {
             d.gx = I.dy / I.dx;   // x-faces: area dy over centroid distance dx
             d.gy = I.dx / I.dy;
             d.vol = I.dx * I.dy;
+            d.hcgx = I.dt / (I.cv * d.vol) * d.gx;
+            d.hcgy = I.dt / (I.cv * d.vol) * d.gy;
             d.u = I.u->p;
             d.u0 = I.u0->p;
         } else if (I.kind == K_TWOPHASE) {
