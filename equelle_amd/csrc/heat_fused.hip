@@ -81,7 +81,13 @@ struct Desc {
     void* v[NVEC];                // Krylov vectors
     double* sc;                   // NSLOT scalars
     int* act;                     // [0] Krylov active, [1] refinement active, [2] two-phase stage
+    // Batch-wide gate of the work that follows a Krylov solve (the solution update, the
+    // refinement defect and flags, the Newton update, the assembly after it, its norms):
+    // 1 except between a speculative solve whose blind iterations left an instance
+    // unconverged and the host's catch-up, when that work is a no-op.
+    const int* gate;
 };
+__device__ inline bool gated_off(const Desc& d) { return *d.gate == 0; }
 
 // The Krylov method of an instance's current solve: CG for the two-phase pressure stage
 // (a symmetric M-matrix: both cells of a face see the same upwinded mobility), in the
@@ -210,6 +216,7 @@ __device__ inline void cell_ijk(const Desc& d, int c, int& i, int& j, int& k) {
 // partials in a fixed order (deterministic), then thread 0 applies the action to
 // the instance's scalars.
 __device__ void fin_body(const Desc& d, const double* part, int nb, const Fin& f) {
+    if (!f.kry && gated_off(d)) return;   // norms of gated cell kernels (assembly, defect)
     const int2 r = make_int2(d.gbase, d.gbase + (f.kry ? d.ngrpk : d.ngrp));
     __shared__ double res[2];
     __shared__ double lds[2][FBS / 64];
@@ -596,6 +603,7 @@ template <bool MIXED, int CPT>
 __global__ __launch_bounds__(BS) void k_assemble(const Desc* __restrict__ D, const int2* __restrict__ blk,
                                                  double* __restrict__ part) {
     LOCATE();
+    if (gated_off(d)) return;
     double f2[CPT] = {};
     if (d.kind == K_LINEAR) {   // coefficients come from k_gather_stencil
         group_store<CPT>(f2, PARTS(part));
@@ -712,6 +720,7 @@ template <int CPT>
 __global__ __launch_bounds__(BS) void k_defect(const Desc* __restrict__ D, const int2* __restrict__ blk,
                                                double* __restrict__ part) {
     LOCATE();
+    if (gated_off(d)) return;
     if (d.vec && d.kind == K_HEAT) {   // heat, nx % 8 == 0: four consecutive cells of one row per lane
         const bool on = d.act[1] != 0;
         const int nx = d.nx, ny = d.ny;
@@ -1744,7 +1753,8 @@ __global__ __launch_bounds__(BS) void k_multi_copy(const McJob* __restrict__ job
 
 // out[0] = number of instances with flag `which` set, out[1] = the largest Krylov
 // iteration count of the current solve (S_IT) over the instances.
-__global__ void k_count_active(const Desc* __restrict__ D, int ninst, int which, int* __restrict__ out) {
+__global__ void k_count_active(const Desc* __restrict__ D, int ninst, int which, int* __restrict__ out,
+                               int* __restrict__ gate) {
     __shared__ int lds[2][BS];
     int s = 0, mx = 0;
     for (int i = threadIdx.x; i < ninst; i += BS) {
@@ -1764,6 +1774,7 @@ __global__ void k_count_active(const Desc* __restrict__ D, int ninst, int which,
     if (threadIdx.x == 0) {
         out[0] = lds[0][0];
         out[1] = lds[1][0];
+        if (gate) *gate = lds[0][0] == 0 ? 1 : 0;   // the post-solve work runs once every instance converged
     }
 }
 
@@ -1778,6 +1789,7 @@ __global__ void k_refine_flags(const Desc* __restrict__ D, int ninst, int mode, 
     if (i >= ninst) return;
     const Desc& d = D[i];
     double* sc = d.sc;
+    if (mode == 1 && gated_off(d)) return;
     if (mode == 0) {
         if (sc[S_CONT] != 0.0) return;   // refinement of the current Newton iteration continues
         sc[S_BN] = sqrt(sc[S_FN]);
@@ -1894,6 +1906,7 @@ template <class T, int CPT>
 __global__ __launch_bounds__(BS) void k_accum(const Desc* __restrict__ D, const int2* __restrict__ blk, int scaled,
                                               int first) {
     LOCATE();
+    if (gated_off(d)) return;
     const bool first_round = first < 0 ? d.sc[S_REF] == 0.0 : first != 0;   // first < 0: the instance's own round
     const bool ok = d.act[1] && isfinite(d.sc[S_RR]);
     if (!ok && !first_round) return;
@@ -1961,7 +1974,7 @@ __global__ __launch_bounds__(BS) void k_accum(const Desc* __restrict__ D, const 
 template <int CPT>
 __global__ __launch_bounds__(BS) void k_newton_update(const Desc* __restrict__ D, const int2* __restrict__ blk) {
     LOCATE();
-    if (d.sc[S_CONT] != 0.0) return;   // the refinement continues: no update yet
+    if (gated_off(d) || d.sc[S_CONT] != 0.0) return;   // the refinement continues: no update yet
     if (d.vec) {   // heat, four consecutive cells per lane
         FOR_VGROUPS(g, c0) {
             if (c0 >= d.n) continue;
@@ -1979,7 +1992,7 @@ __global__ __launch_bounds__(BS) void k_newton_update(const Desc* __restrict__ D
 // out[q * ninst + i] = scalar slots.x, .y, .z, .w, s4 (q = 0..4) of instance i
 __global__ void k_gather_slots(const Desc* __restrict__ D, int ninst, int4 slots, int s4, double* __restrict__ out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ninst) return;
+    if (i >= ninst || gated_off(D[i])) return;
     const double* sc = D[i].sc;
     out[i] = sc[slots.x];
     out[ninst + i] = sc[slots.y];
@@ -2128,8 +2141,28 @@ struct HeatBatch::Impl {
     // NRING x {active count, max Krylov iterations} and the gathered per-instance scalars,
     // in pinned host memory the kernels store into directly (visible once the stream or the
     // event has completed): a host check costs the kernel and the wait, no copy launch
-    int* host_count = nullptr;
+    int* host_count = nullptr;   // NRING ring slots, then the speculative solve's slot (SPEC)
+    static constexpr int SPEC = NRING;
     hipEvent_t ev[NRING] = {};
+    // Speculative solve: after the predicted number of blind Krylov iterations, the work that
+    // follows the solve is queued at once, gated on device (gate = every instance converged,
+    // set by the count after the blind iterations); the host resolves it at the next
+    // synchronisation: a short prediction means the gated work was a no-op, and the solve is
+    // finished with checked iterations and the work queued again (same results either way).
+    DevPtr<int> gate;        // 1 int, batch-wide; 1 outside an unresolved speculative solve
+    bool allow_spec = false;   // set by iterate() around the solve it can resolve
+    struct Spec {
+        bool on = false;
+        Table* t = nullptr;   // alive: iterate() holds the solve's table until resolved
+        std::tuple<double, int, int, int> pkey;
+        double tol = 0;
+        int maxit = 0, meth = 0, k = 0;
+    } spec;
+    void open_gate() {
+        const int one = 1;
+        EQ_HIP(hipMemcpyAsync(gate->p, &one, sizeof(int), hipMemcpyHostToDevice, S()));
+        Device::get().sync();
+    }
     double* gpin = nullptr;
     size_t gcap = 0;
     // step-start copies queued by begin_step, issued together before the next device work
@@ -2330,6 +2363,7 @@ struct HeatBatch::Impl {
         for (int v = 0; v < NVEC; ++v) d.v[v] = I.vecs->p + (size_t)v * d.ns * h * ks();
         d.sc = I.sc->p;
         d.act = I.act->p;
+        d.gate = gate->p;
         return d;
     }
 
@@ -2490,8 +2524,10 @@ struct HeatBatch::Impl {
     }
 
     // out[0] = active instances (flag `which`), out[1] = max Krylov iterations, synchronously
-    int count_active(const Table& t, int which, int* maxit = nullptr) {
-        hipLaunchKernelGGL(k_count_active, dim3(1), dim3(BS), 0, S(), t.desc->p, (int)t.inst.size(), which, host_count);
+    // (gate: the post-solve gate is set from the count as well)
+    int count_active(const Table& t, int which, int* maxit = nullptr, int* gate_dev = nullptr) {
+        hipLaunchKernelGGL(k_count_active, dim3(1), dim3(BS), 0, S(), t.desc->p, (int)t.inst.size(), which, host_count,
+                           gate_dev);
         EQ_HIP(hipGetLastError());
         Device::get().sync();
         if (maxit) *maxit = host_count[1];
@@ -2500,7 +2536,7 @@ struct HeatBatch::Impl {
     // The same without stalling the queue: the result of ring slot k is read later.
     void count_async(const Table& t, int which, int slot) {
         hipLaunchKernelGGL(k_count_active, dim3(1), dim3(BS), 0, S(), t.desc->p, (int)t.inst.size(), which,
-                           host_count + 2 * slot);
+                           host_count + 2 * slot, (int*)nullptr);
         EQ_HIP(hipGetLastError());
         EQ_HIP(hipEventRecord(ev[slot], S()));
     }
@@ -2719,22 +2755,32 @@ struct HeatBatch::Impl {
         if (use_graph && !timing_enabled() && t.solves++ > 0) {
             // graphs of one and of two iterations (the blind launches go in pairs: one
             // inter-graph gap per two iterations)
-            auto graph_of = [&](int reps) {
-                const auto gk = std::make_tuple(tol, maxit, meth + 8 * reps);
-                auto g = t.graphs.find(gk);
-                if (g == t.graphs.end()) {
-                    hipGraph_t graph;
-                    hipGraphExec_t exec;
-                    EQ_HIP(hipStreamBeginCapture(S(), hipStreamCaptureModeThreadLocal));
-                    for (int r = 0; r < reps; ++r) krylov_iteration<T>(t, tol, maxit, meth);
-                    EQ_HIP(hipStreamEndCapture(S(), &graph));
-                    EQ_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-                    g = t.graphs.emplace(gk, std::make_pair(graph, exec)).first;
-                }
-                return g->second.second;
-            };
+            auto graph_of = [&](int reps) { return graph_exec<T>(t, tol, maxit, meth, reps); };
             const hipGraphExec_t exec = graph_of(1);
-            const int blind = std::min(maxit, std::max(0, t.pred[pkey] - 1));
+            const int pred = t.pred[pkey];
+            if (allow_spec && pred >= 1) {
+                // speculative: the predicted iterations, then the count (into the SPEC slot
+                // and the gate) without waiting; iterate() resolves it (finish_spec)
+                const int nb = std::min(maxit, pred);
+                int k = 0;
+                if (nb >= 2) {
+                    const hipGraphExec_t exec2 = graph_of(2);
+                    for (; k + 2 <= nb; k += 2) EQ_HIP(hipGraphLaunch(exec2, S()));
+                }
+                for (; k < nb; ++k) EQ_HIP(hipGraphLaunch(exec, S()));
+                hipLaunchKernelGGL(k_count_active, dim3(1), dim3(BS), 0, S(), t.desc->p, (int)t.inst.size(), 0,
+                                   host_count + 2 * SPEC, gate->p);
+                EQ_HIP(hipGetLastError());
+                spec.on = true;
+                spec.t = &t;
+                spec.pkey = pkey;
+                spec.tol = tol;
+                spec.maxit = maxit;
+                spec.meth = meth;
+                spec.k = k;
+                return;
+            }
+            const int blind = std::min(maxit, std::max(0, pred - 1));
             int k = 0;
             if (blind >= 2) {
                 const hipGraphExec_t exec2 = graph_of(2);
@@ -2769,6 +2815,56 @@ struct HeatBatch::Impl {
         t.pred[pkey] = iters;
     }
 
+    // The captured graph of `reps` Krylov iterations for (tolerance, iteration cap, methods).
+    template <class T>
+    hipGraphExec_t graph_exec(Table& t, double tol, int maxit, int meth, int reps) {
+        const auto gk = std::make_tuple(tol, maxit, meth + 8 * reps);
+        auto g = t.graphs.find(gk);
+        if (g == t.graphs.end()) {
+            hipGraph_t graph;
+            hipGraphExec_t exec;
+            EQ_HIP(hipStreamBeginCapture(S(), hipStreamCaptureModeThreadLocal));
+            for (int r = 0; r < reps; ++r) krylov_iteration<T>(t, tol, maxit, meth);
+            EQ_HIP(hipStreamEndCapture(S(), &graph));
+            EQ_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+            g = t.graphs.emplace(gk, std::make_pair(graph, exec)).first;
+        }
+        return g->second.second;
+    }
+    // Resolves a speculative solve after the host synchronisation that follows its gated
+    // work: with every instance converged after the blind iterations (the count in the SPEC
+    // slot), that work ran; otherwise the solve continues with checked iterations, the gate
+    // is reopened, and the caller queues the gated work again (returns true).
+    template <class T>
+    bool finish_spec() {
+        Table& t = *spec.t;
+        spec.on = false;
+        int iters = host_count[2 * SPEC + 1];
+        if (host_count[2 * SPEC] == 0) {
+            t.pred[spec.pkey] = iters;
+            return false;
+        }
+        const hipGraphExec_t exec = graph_exec<T>(t, spec.tol, spec.maxit, spec.meth, 1);
+        int k = spec.k, pending = -1;
+        bool done = false;
+        while (!done && k < spec.maxit) {
+            EQ_HIP(hipGraphLaunch(exec, S()));
+            count_async(t, 0, k % NRING);
+            if (pending >= 0 && count_wait(pending % NRING, &iters) == 0) done = true;
+            pending = k++;
+        }
+        if (!done && pending >= 0) count_wait(pending % NRING, &iters);
+        open_gate();
+        t.pred[spec.pkey] = iters;
+        return true;
+    }
+
+    // Linear solve of one Newton iteration: flags and preconditioner, right-hand side, the
+    // Krylov solve, then (linear_tail) the solution and, in mixed precision, the FP64 defect
+    // and refinement flags.  One refinement round per call: an instance whose refinement
+    // continues (S_CONT) skips the Newton update and runs its next round in the next call, so
+    // no host check is needed between rounds; the first round of an instance reads F and
+    // writes du (its own round counter decides).
     template <class T>
     void linear_solve(Table& t) {
         // refinement flags from ||F||; the first round reads F and writes du (no clearing pass)
@@ -2780,15 +2876,19 @@ struct HeatBatch::Impl {
         if (!mixed) {
             HLP(C_PC, (k_load_rhs<T, CP>), 0, 1);
             bicgstab<T>(t, cfg.solver_tol, cfg.solver_max_iter);
+        } else {
+            HLP(C_PC, (k_load_rhs<T, CP>), 1, -1);
+            bicgstab<T>(t, -1.0, cfg.solver_max_iter);   // per-instance inner tolerance (S_TOL2)
+        }
+        linear_tail<T>(t);
+    }
+    template <class T>
+    void linear_tail(Table& t) {
+        if (!mixed) {
             HLP(C_PC, (k_accum<T, CP>), 0, 1);
             return;
         }
-        // one refinement round per call: an instance whose refinement continues (S_CONT)
-        // skips the Newton update and runs its next round in the next call, so no host
-        // check is needed between rounds; the first round of an instance reads F and
-        // writes du (its own round counter decides)
-        HLP(C_PC, (k_load_rhs<T, CP>), 1, -1);
-        bicgstab<T>(t, -1.0, cfg.solver_max_iter);   // per-instance inner tolerance (S_TOL2)
+        const unsigned ni = (unsigned)t.inst.size();
         HLP(C_PC, (k_accum<T, CP>), 1, -1);
         launch_defect(t);
         fin(t, Fcell(S_FN));
@@ -2866,7 +2966,10 @@ HeatBatch::HeatBatch(const FusedConfig& cfg) : p_(new Impl()) {
                    cfg.krylov_form_twophase == "auto",
                "krylov_form_twophase must be reduced, full or auto");
     p_->use_graph = cfg.graph;
-    EQ_HIP(hipHostMalloc((void**)&p_->host_count, 2 * Impl::NRING * sizeof(int), hipHostMallocCoherent));
+    EQ_HIP(hipHostMalloc((void**)&p_->host_count, 2 * (Impl::NRING + 1) * sizeof(int), hipHostMallocCoherent));
+    p_->gate = dev_alloc<int>(1);
+    const int one = 1;
+    EQ_HIP(hipMemcpy(p_->gate->p, &one, sizeof(int), hipMemcpyHostToDevice));
     for (int i = 0; i < Impl::NRING; ++i) EQ_HIP(hipEventCreateWithFlags(&p_->ev[i], hipEventDisableTiming));
     EQ_HIP(hipEventCreateWithFlags(&p_->mc_ev, hipEventDisableTiming));
 }
@@ -3149,17 +3252,24 @@ void HeatBatch::begin_step(int slot, double dt) {
 
 std::vector<int> HeatBatch::iterate() {
     Impl& P = *p_;
+    if (P.spec.on) {   // left unresolved by an error in the previous call
+        P.spec.on = false;
+        P.open_gate();
+    }
     P.flush_copies();
     std::vector<Inst*> solve, asmb;
     for (auto& kv : P.insts) {
         if (kv.second->state == Inst::SOLVE) solve.push_back(kv.second.get());
         if (kv.second->state == Inst::START || kv.second->state == Inst::SOLVE) asmb.push_back(kv.second.get());
     }
+    TableP tps;
     if (!solve.empty()) {
-        TableP tp = P.table(solve);
-        Table& t = *tp;
+        tps = P.table(solve);
+        Table& t = *tps;
+        P.allow_spec = P.cfg.speculate;
         if (P.mixed) P.linear_solve<float>(t);
         else P.linear_solve<double>(t);
+        P.allow_spec = false;
         P.newton_update(t);
     }
     std::vector<int> done;
@@ -3171,6 +3281,15 @@ std::vector<int> HeatBatch::iterate() {
     // whether the refinement continues (then u was not updated and F is unchanged)
     const size_t na = asmb.size();
     std::vector<double> g = P.gather(t, make_int4(S_FN, S_ITT, S_REF, S_RN), S_CONT);
+    if (P.spec.on && (P.mixed ? P.finish_spec<float>() : P.finish_spec<double>())) {
+        // the blind iterations were too few: the work after the solve was gated off and is
+        // queued again now that the solve has finished
+        if (P.mixed) P.linear_tail<float>(*tps);
+        else P.linear_tail<double>(*tps);
+        P.newton_update(*tps);
+        P.assemble(t);
+        g = P.gather(t, make_int4(S_FN, S_ITT, S_REF, S_RN), S_CONT);
+    }
     const FusedConfig& c = P.cfg;
     for (size_t q = 0; q < na; ++q) {
         Inst& I = *asmb[q];

@@ -62,6 +62,10 @@ struct FusedConfig {
     bool twophase_pressure_mg = true;
     int max_refine = 20;
     bool graph = true;                 // capture BiCGStab iterations in a hipGraph
+    // with graphs: queue the work after a Newton iteration's Krylov solve behind the predicted
+    // iterations without waiting for the convergence count (gated on device; resolved at
+    // the iteration's norm check)
+    bool speculate = true;
     int cpt = 0;                       // cells per thread (1, 2, 4, 8); 0 = chosen by the tuner
     int xcd = 1;                       // XCD-contiguous block order when cpt is fixed
     bool tune = true;                  // time the launch shapes per membership signature
