@@ -18,7 +18,9 @@ kernels: every kernel of the headline's Newton and Krylov loop, timed on the fin
 of the timed run: 50 launches each between their own HIP start/stop events on the
 runtime stream (avg_us), and 50 more back to back (back_to_back_us); algorithmic bytes
 per launch as stated per kernel.  roofline: the kernel with the largest estimated time
-per step (avg_us x launches per step from the iteration counts).
+per step (avg_us x launches per step from the iteration counts); roofline.traffic: its HBM
+bytes per launch measured in the same run (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE over
+child runs of this script, scaled by the counters of a 256 MiB device-to-device copy).
 cpu_baseline: the CPU oracle (serial C++ restatement of the reference serial backend)
 timed for one Newton iteration of the same workload on one host core.
 Every leg runs its W warmup steps and is then reset to its initial state, so the K timed
@@ -33,6 +35,7 @@ The default run also carries compact summaries of the other configured workloads
   serving      the request loop at two Poisson arrival rates, p50/p99 per request kind
 
   python bench.py [--steps K] [--warmup W] [--size N] [--no-cpu-baseline] [--quick] [--no-extras]
+                  [--no-traffic]
 """
 import argparse
 import ctypes as C
@@ -51,11 +54,7 @@ from equelle_amd.runtime import load_library  # noqa: E402
 from equelle_amd.workloads import NONLINEAR_HEAT, gaussian_bumps, heat_params  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per cell and launch of each kernel at 2048^2 (mixed precision): rocprofv3 --pmc
-# FETCH_SIZE and WRITE_SIZE per dispatch (mean of the 50 back-to-back timing launches),
-# scaled by what the counters read for a copy of known size (reads x2.0, writes x1.0).
-TRAFFIC_BYTES_PER_CELL = {"operator_red": 14.06, "operator_black": 18.10, "assemble": 44.16, "defect": 32.21,
-                          "b1": 12.03, "b2": 10.02, "b3": 16.07, "load_rhs": 20.14}
+COPY_BYTES = 256 << 20   # the counter calibration copies 256 MiB device to device per launch
 
 
 def log(msg):
@@ -526,17 +525,130 @@ def bench_twophase_fi(lib, args):
     print(json.dumps(out), flush=True)
 
 
+def hip_runtime():
+    """The HIP runtime library (libamdhip64), through ctypes."""
+    for name in ("libamdhip64.so", "/opt/rocm/lib/libamdhip64.so"):
+        try:
+            return C.CDLL(name)
+        except OSError:
+            continue
+    raise RuntimeError("libamdhip64.so not found")
+
+
 def bench_copy(args):
-    """Calibration for HBM counters: a device-to-device copy of a known size
-    (torch's elementwise copy kernel), 256 MiB read + 256 MiB written per launch."""
-    import torch
-    a = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
-    b = torch.empty_like(a)
-    for _ in range(args.warmup + args.steps):
-        b.copy_(a)
-    torch.cuda.synchronize()
-    print(json.dumps({"workload": "copy", "bytes_read_per_launch": a.numel() * 4,
-                      "bytes_written_per_launch": b.numel() * 4, "launches": args.warmup + args.steps}), flush=True)
+    """Calibration for HBM counters: a device-to-device hipMemcpy of a known size (the HIP
+    runtime's copy kernel), 256 MiB read + 256 MiB written per launch."""
+    hip = hip_runtime()
+
+    def ok(rc, what):
+        if rc != 0:
+            raise RuntimeError(f"{what} failed with HIP error {rc}")
+
+    a, b = C.c_void_p(), C.c_void_p()
+    ok(hip.hipMalloc(C.byref(a), C.c_size_t(COPY_BYTES)), "hipMalloc")
+    ok(hip.hipMalloc(C.byref(b), C.c_size_t(COPY_BYTES)), "hipMalloc")
+    try:
+        ok(hip.hipMemset(a, C.c_int(1), C.c_size_t(COPY_BYTES)), "hipMemset")
+        for _ in range(args.warmup + args.steps):
+            ok(hip.hipMemcpy(b, a, C.c_size_t(COPY_BYTES), C.c_int(3)), "hipMemcpy")   # device to device
+        ok(hip.hipDeviceSynchronize(), "hipDeviceSynchronize")
+    finally:
+        hip.hipFree(a)
+        hip.hipFree(b)
+    print(json.dumps({"workload": "copy", "bytes_read_per_launch": COPY_BYTES,
+                      "bytes_written_per_launch": COPY_BYTES, "launches": args.warmup + args.steps}), flush=True)
+
+
+def pmc_per_dispatch(csv_path):
+    """Kernel name -> counter values per dispatch (in dispatch order) from a rocprofv3
+    counter-collection CSV of one counter."""
+    import csv
+    rows = list(csv.DictReader(open(csv_path)))
+    if rows and "Dispatch_Id" in rows[0]:
+        rows.sort(key=lambda r: int(r["Dispatch_Id"]))
+    per = {}
+    for r in rows:
+        per.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]))
+    return per
+
+
+def measure_traffic(args):
+    """HBM bytes per launch of every kernel of the headline loop, measured in this run:
+    rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE (one counter per run, no other tracing)
+    over child runs of this script (the headline at one timed step, then the 256 MiB copy),
+    each counter scaled to bytes by what it reads for the copy.  Per kernel: the mean over
+    the last 50 dispatches of its most-dispatched launch shape (the back-to-back timing
+    launches on the final state).  Runs before this process touches the GPU."""
+    import glob
+    import shutil
+    import signal
+    import subprocess
+    import tempfile
+    if shutil.which("rocprofv3") is None:
+        return {"error": "rocprofv3 not on PATH"}
+    me = os.path.abspath(__file__)
+    legs = {"head": ["--steps", "1", "--warmup", "0", "--no-cpu-baseline", "--quick", "--size", str(args.size),
+                     "--precond", args.precond, "--inner-tol", repr(args.inner_tol)],
+            "copy": ["--workload", "copy", "--steps", "3", "--warmup", "1"]}
+    got = {}
+    env = dict(os.environ, TMPDIR="/tmp")
+    with tempfile.TemporaryDirectory(dir="/tmp") as d:
+        for leg, extra in legs.items():
+            for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
+                out = os.path.join(d, f"{leg}_{ctr}")
+                cmd = ["rocprofv3", "--pmc", ctr, "--output-format", "csv", "-d", out, "-o", "run", "--",
+                       sys.executable, me] + extra
+                log(f"[bench] traffic: {ctr} over the {leg} run")
+                p = subprocess.Popen(cmd, cwd=d, env=env, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE,
+                                     start_new_session=True)
+                try:
+                    _, err = p.communicate(timeout=240)
+                except subprocess.TimeoutExpired:
+                    os.killpg(p.pid, signal.SIGKILL)
+                    p.communicate()
+                    return {"error": f"rocprofv3 --pmc {ctr} over the {leg} run timed out"}
+                files = glob.glob(os.path.join(out, "**", "*counter_collection.csv"), recursive=True)
+                if p.returncode != 0 and not files:
+                    return {"error": f"rocprofv3 --pmc {ctr} over the {leg} run exited {p.returncode}: "
+                                     f"{err.decode(errors='replace')[-300:]}"}
+                if not files:
+                    return {"error": f"rocprofv3 --pmc {ctr} over the {leg} run wrote no counter file"}
+                got[(leg, ctr)] = pmc_per_dispatch(files[0])
+    copy = {ctr: [v for k, v in got[("copy", ctr)].items() if "copy" in k.lower()] for ctr in ("FETCH_SIZE",
+                                                                                               "WRITE_SIZE")}
+    if not copy["FETCH_SIZE"] or not copy["WRITE_SIZE"]:
+        return {"error": "no copy kernel in the calibration run"}
+    kib_r = max(v[-1] for v in copy["FETCH_SIZE"])
+    kib_w = max(v[-1] for v in copy["WRITE_SIZE"])
+    scale_r, scale_w = COPY_BYTES / (kib_r * 1024), COPY_BYTES / (kib_w * 1024)
+    per_kernel = {}
+    fetch, write = got[("head", "FETCH_SIZE")], got[("head", "WRITE_SIZE")]
+    for name, vals in fetch.items():
+        if "eqhip" not in name or name not in write:
+            continue
+        short = name.split("(eqhip")[0].replace("void ", "").replace("eqhip::(anonymous namespace)::", "")
+        f, w = vals[-50:], write[name][-50:]
+        per_kernel[short] = {"dispatches": len(vals), "read_bytes": sum(f) / len(f) * 1024 * scale_r,
+                             "write_bytes": sum(w) / len(w) * 1024 * scale_w}
+    return {"per_kernel": per_kernel, "scale_read": scale_r, "scale_write": scale_w,
+            "source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE child runs of this script (headline at 1 timed "
+                      "step; 256 MiB device-to-device copy for the byte scale)"}
+
+
+def traffic_of(measured, kernel):
+    """Measured bytes per launch of `kernel` (e.g. "k_schur_black<float, 4>"): the
+    most-dispatched launch shape with the same name and first template argument."""
+    if not measured or "per_kernel" not in measured:
+        return None
+    base, _, rest = kernel.partition("<")
+    first = rest.split(",")[0].rstrip(">") if "," in rest else None
+    cands = [(v["dispatches"], k, v) for k, v in measured["per_kernel"].items()
+             if k.startswith(base + "<") and (first is None or k[len(base) + 1:].split(",")[0] == first)]
+    if not cands:
+        return None
+    _, k, v = max(cands)
+    return {"kernel": k, "bytes": v["read_bytes"] + v["write_bytes"], "read_bytes": v["read_bytes"],
+            "write_bytes": v["write_bytes"]}
 
 
 # Algorithmic bytes per launch over n cells (h = (n + 1) / 2 pair ranks; ks = bytes of the
@@ -632,6 +744,8 @@ def main():
     ap.add_argument("--size", type=int, default=NONLINEAR_HEAT["nx"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline run only (for profiling)")
+    ap.add_argument("--no-traffic", dest="no_traffic", action="store_true",
+                    help="skip the rocprofv3 counter runs that measure roofline.traffic")
     ap.add_argument("--no-extras", dest="no_extras", action="store_true",
                     help="heat headline legs only (no batched, two-phase and serving summaries)")
     ap.add_argument("--default-rates", dest="default_rates", default="50,500",
@@ -661,6 +775,9 @@ def main():
         sys.exit(2)
     if args.workload == "copy":
         return bench_copy(args)
+    # the counter runs are child processes started before this process touches the GPU
+    measured = measure_traffic(args) if args.workload == "heat2048" and not (args.quick or args.no_traffic) \
+        else None
     lib = load_library()
     if args.workload == "batched":
         return bench_batched(lib, args)
@@ -705,8 +822,7 @@ def main():
                    "cells": n, "jacobian_nnz": nnz, "engine": "fused heat engine (1 instance)"},
         "roofline": {"bound": "hbm", "achieved": kt[dom]["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": kt[dom]["frac"],
-                     "traffic": (TRAFFIC_BYTES_PER_CELL[dom] * n
-                                 if (nx, ny) == (2048, 2048) and dom in TRAFFIC_BYTES_PER_CELL else None),
+                     "traffic": None,
                      "kernel": kt[dom]["kernel"], "launches": 50, "avg_us": kt[dom]["avg_us"],
                      "back_to_back_us": kt[dom]["back_to_back_us"],
                      "algorithmic_bytes": kt[dom]["algorithmic_bytes"], "bytes_formula": kt[dom]["bytes"]},
@@ -718,6 +834,18 @@ def main():
         "krylov_iterations": head["krylov"],
         "cell_steps_per_s": n * args.steps / head["seconds"],
     }
+    tr = traffic_of(measured, kt[dom]["kernel"])
+    if tr is not None:
+        out["roofline"]["traffic"] = tr["bytes"]
+        out["roofline"]["traffic_detail"] = dict(tr, scale_read=measured["scale_read"],
+                                                 scale_write=measured["scale_write"], source=measured["source"],
+                                                 over_algorithmic=tr["bytes"] / kt[dom]["algorithmic_bytes"])
+        for name, row in kt.items():
+            t = traffic_of(measured, row["kernel"])
+            if t is not None:
+                row["traffic"] = t["bytes"]
+    elif measured is not None:
+        out["roofline"]["traffic_detail"] = {"error": measured.get("error", "kernel not in the counter runs")}
     if not args.quick:
         log("[bench] plain FP64 (fused engine)")
         p = run_fused(lib, "fp64", u0, nx, ny, dt, args.steps, args.warmup, args.precond, not args.no_graph)
