@@ -253,7 +253,7 @@ int eqhip_jacobian_copy(void* rt, int h, int* rp, int* ci, double* v) {
         EQ_REQUIRE(a->pat, "Trying to copy empty matrix to host");
         std::memcpy(rp, a->pat->rp.data(), a->pat->rp.size() * sizeof(int));
         std::memcpy(ci, a->pat->ci.data(), a->pat->ci.size() * sizeof(int));
-        std::vector<double> j = dev_to_host(a->jval->p, a->pat->nnz);
+        std::vector<double> j = dev_to_host(jac_values(*a), a->pat->nnz);
         std::memcpy(v, j.data(), j.size() * sizeof(double));
     });
 }

@@ -141,6 +141,20 @@ const int* Pattern::row_of() const {
     return d_row->p;
 }
 
+bool Pattern::ascending() const {
+    std::lock_guard<std::mutex> lk(mu);
+    if (canon < 0) {
+        canon = 1;
+        for (int r = 0; r < rows && canon; ++r)
+            for (int k = rp[r] + 1; k < rp[r + 1]; ++k)
+                if (ci[k] <= ci[k - 1]) {
+                    canon = 0;
+                    break;
+                }
+    }
+    return canon == 1;
+}
+
 PatternP make_pattern(int rows, int cols, std::vector<int> rp, std::vector<int> ci) {
     uint64_t h = 1469598103934665603ull;
     const int dims[2] = {rows, cols};

@@ -117,8 +117,10 @@ struct Pattern {
     std::vector<int> rp, ci;          // host copies (symbolic phase)
     DevPtr<int> d_rp, d_ci;           // device copies (numeric phase, SpMV)
     mutable DevPtr<int> d_row;        // row index of every entry (lazy)
+    mutable int canon = -1;           // columns strictly ascending in every row (lazy; -1 unknown)
     mutable std::mutex mu;
     const int* row_of() const;        // device row-of-entry map
+    bool ascending() const;           // columns strictly ascending in every row
 };
 using PatternP = std::shared_ptr<const Pattern>;
 
@@ -148,7 +150,15 @@ struct ScalarData {
     DevPtr<double> val;
     bool ad = false;       // useAutoDiff flag of the reference CollOfScalar
     PatternP pat;          // null => empty (zero) Jacobian
-    DevPtr<double> jval;   // pat->nnz values
+    DevPtr<double> jval;   // pat->nnz stored values
+    // The Jacobian is jscale * diag(jrow) * (pat, jval): scalings by constants and
+    // collections without derivatives are kept as factors and applied by the next kernel
+    // that reads the values (or once, by jac_values, where a caller needs them).
+    double jscale = 1.0;
+    DevPtr<double> jrow;   // n row factors, or null
+    bool jones = false;    // (pat, jval) is the n x n identity with unit values (a primary variable)
+    mutable DevPtr<double> jeff;   // the factors applied (materialized on first request)
+    bool jpending() const { return jscale != 1.0 || (bool)jrow; }
     int jcols() const { return pat ? pat->cols : 0; }
 };
 using SP = std::shared_ptr<const ScalarData>;

@@ -131,31 +131,79 @@ SymP sym_rows(const PatternP& a, const std::vector<int>& map, uint64_t map_id, i
     return e;
 }
 
+// A Jacobian: pattern, stored values and the factors still to apply (jscale, jrow).
 struct JacOut {
     PatternP pat;
     DevPtr<double> val;
+    double s = 1.0;
+    DevPtr<double> row;
+    bool ones = false;
 };
 
+SP make_ad(int n, DevPtr<double> val, const JacOut& j) {
+    auto s = std::make_shared<ScalarData>();
+    s->n = n;
+    s->val = std::move(val);
+    s->ad = true;
+    s->pat = j.pat;
+    s->jval = j.val;
+    s->jscale = j.s;
+    s->jrow = j.row;
+    s->jones = j.ones;
+    return s;
+}
+
+// The elementwise product of the given row-factor arrays (null entries skipped; null if
+// none): one array is shared, more are multiplied into a fresh one.
+DevPtr<double> row_product(std::initializer_list<DevPtr<double>> fs, int n) {
+    DevPtr<double> out;
+    for (const DevPtr<double>& f : fs) {
+        if (!f) continue;
+        if (!out) {
+            out = f;
+            continue;
+        }
+        auto m = dev_alloc<double>(std::max(1, n));
+        launch_ew(EW_MUL, out->p, f->p, m->p, n);
+        out = m;
+    }
+    return out;
+}
+
+const double* rowp(const DevPtr<double>& r) { return r ? r->p : nullptr; }
+
 // out = C(r) * (alpha*A(r)*Ja + beta*B(r)*Jb) with the empty-matrix rules of the
-// reference (cudaMatrix.cu:646-694): empty operands drop out.
-JacOut jac_lincomb(const SP& a, const double* A, double alpha, const SP& b, const double* B, double beta,
-                   const double* C) {
+// reference (cudaMatrix.cu:646-694): empty operands drop out.  With two operands one
+// kernel applies every factor (the operands' pending ones included) over the union
+// pattern; with one, its values are shared and the factors stay pending.
+JacOut jac_lincomb(const SP& a, const DevPtr<double>& A, double alpha, const SP& b, const DevPtr<double>& B,
+                   double beta, const DevPtr<double>& C) {
     const PatternP pa = a ? a->pat : nullptr;
     const PatternP pb = b ? b->pat : nullptr;
     JacOut o;
     if (!pa && !pb) return o;
-    if (pa && pb && pa.get() != pb.get()) {
-        SymP e = sym_union(pa, pb);
-        o.pat = e->out;
+    if (pa && pb) {
+        const int *ia = nullptr, *ib = nullptr;
+        if (pa.get() != pb.get()) {
+            SymP e = sym_union(pa, pb);
+            o.pat = e->out;
+            ia = e->ia->p;
+            ib = e->ib->p;
+        } else {
+            o.pat = pa;
+        }
         o.val = dev_alloc<double>(o.pat->nnz);
-        launch_jlin(o.pat->nnz, (A || B || C) ? o.pat->row_of() : nullptr, a->jval->p, e->ia->p, A, alpha,
-                    b->jval->p, e->ib->p, B, beta, C, o.val->p);
+        const bool rows = A || B || C || a->jrow || b->jrow;
+        launch_jlin(o.pat->nnz, rows ? o.pat->row_of() : nullptr, a->jval->p, ia, rowp(A), rowp(a->jrow),
+                    alpha * a->jscale, b->jval->p, ib, rowp(B), rowp(b->jrow), beta * b->jscale, rowp(C), o.val->p);
         return o;
     }
-    o.pat = pa ? pa : pb;
-    o.val = dev_alloc<double>(o.pat->nnz);
-    launch_jlin(o.pat->nnz, (A || B || C) ? o.pat->row_of() : nullptr, pa ? a->jval->p : nullptr, nullptr, A, alpha,
-                pb ? b->jval->p : nullptr, nullptr, B, beta, C, o.val->p);
+    const SP& x = pa ? a : b;
+    o.pat = x->pat;
+    o.val = x->jval;
+    o.s = (pa ? alpha : beta) * x->jscale;
+    o.row = row_product({C, pa ? A : B, x->jrow}, x->n);
+    o.ones = x->jones;
     return o;
 }
 
@@ -179,10 +227,12 @@ SP s_uniform(int n, double v) {
 SP s_from_device(DevPtr<double> v, int n) { return make_scalar(n, std::move(v), false, nullptr, nullptr); }
 
 SP s_primary(const SP& a) {
-    PatternP p = identity_pattern(a->n);
-    auto j = dev_alloc<double>(a->n);
-    launch_fill(j->p, 1.0, a->n);
-    return make_scalar(a->n, a->val, true, p, j);
+    JacOut j;
+    j.pat = identity_pattern(a->n);
+    j.val = dev_alloc<double>(a->n);
+    launch_fill(j.val->p, 1.0, a->n);
+    j.ones = true;
+    return make_ad(a->n, a->val, j);
 }
 
 SP s_with_jacobian(const double* v, int n, const int* rp, const int* ci, const double* jv, int cols) {
@@ -197,16 +247,14 @@ SP s_add(const SP& a, const SP& b) {
     check_size(a, b, "+");
     SP v = binary_values(EW_ADD, a, b);
     if (!(a->ad || b->ad)) return v;
-    JacOut j = jac_lincomb(a, nullptr, 1.0, b, nullptr, 1.0, nullptr);
-    return make_scalar(a->n, v->val, true, j.pat, j.val);
+    return make_ad(a->n, v->val, jac_lincomb(a, nullptr, 1.0, b, nullptr, 1.0, nullptr));
 }
 
 SP s_sub(const SP& a, const SP& b) {
     check_size(a, b, "-");
     SP v = binary_values(EW_SUB, a, b);
     if (!(a->ad || b->ad)) return v;
-    JacOut j = jac_lincomb(a, nullptr, 1.0, b, nullptr, -1.0, nullptr);
-    return make_scalar(a->n, v->val, true, j.pat, j.val);
+    return make_ad(a->n, v->val, jac_lincomb(a, nullptr, 1.0, b, nullptr, -1.0, nullptr));
 }
 
 SP s_mul(const SP& a, const SP& b) {
@@ -214,8 +262,7 @@ SP s_mul(const SP& a, const SP& b) {
     SP v = binary_values(EW_MUL, a, b);
     if (!(a->ad || b->ad)) return v;
     // (uv)' = diag(v) u' + diag(u) v'
-    JacOut j = jac_lincomb(a, b->val->p, 1.0, b, a->val->p, 1.0, nullptr);
-    return make_scalar(a->n, v->val, true, j.pat, j.val);
+    return make_ad(a->n, v->val, jac_lincomb(a, b->val, 1.0, b, a->val, 1.0, nullptr));
 }
 
 SP s_div(const SP& a, const SP& b) {
@@ -225,8 +272,7 @@ SP s_div(const SP& a, const SP& b) {
     // (u/v)' = diag(1/v^2) (diag(v) u' - diag(u) v')
     auto c = dev_alloc<double>(a->n);
     launch_coef(4, nullptr, b->val->p, 0.0, c->p, a->n);
-    JacOut j = jac_lincomb(a, b->val->p, 1.0, b, a->val->p, -1.0, c->p);
-    return make_scalar(a->n, v->val, true, j.pat, j.val);
+    return make_ad(a->n, v->val, jac_lincomb(a, b->val, 1.0, b, a->val, -1.0, c));
 }
 
 SP s_scal_mul(double s, const SP& a) {
@@ -234,8 +280,7 @@ SP s_scal_mul(double s, const SP& a) {
     launch_ew_scalar(EW_MUL, s, a->val->p, v->p, a->n, 1);
     if (!a->ad) return make_scalar(a->n, v, false, nullptr, nullptr);
     EQ_REQUIRE(a->pat, "Calling CudaMatrix * Scalar with empty matrix...");
-    JacOut j = jac_lincomb(a, nullptr, s, nullptr, nullptr, 0.0, nullptr);
-    return make_scalar(a->n, v, true, j.pat, j.val);
+    return make_ad(a->n, v, jac_lincomb(a, nullptr, s, nullptr, nullptr, 0.0, nullptr));
 }
 
 SP s_div_scal(const SP& a, double s) { return s_scal_mul(1.0 / s, a); }
@@ -247,8 +292,7 @@ SP s_scal_div(double s, const SP& a) {
     // (s/u)' = -diag(s/u^2) u'
     auto c = dev_alloc<double>(a->n);
     launch_coef(2, a->val->p, nullptr, s, c->p, a->n);
-    JacOut j = jac_lincomb(a, c->p, 1.0, nullptr, nullptr, 0.0, nullptr);
-    return make_scalar(a->n, v, true, j.pat, j.val);
+    return make_ad(a->n, v, jac_lincomb(a, c, 1.0, nullptr, nullptr, 0.0, nullptr));
 }
 
 SP s_neg(const SP& a) { return s_scal_mul(-1.0, a); }
@@ -267,8 +311,7 @@ SP s_sqrt(const SP& a) {
     if (!a->ad) return make_scalar(a->n, v, false, nullptr, nullptr);
     auto c = dev_alloc<double>(a->n);
     launch_coef(3, v->p, nullptr, 0.0, c->p, a->n);   // 1/(2 sqrt(x))
-    JacOut j = jac_lincomb(a, c->p, 1.0, nullptr, nullptr, 0.0, nullptr);
-    return make_scalar(a->n, v, true, j.pat, j.val);
+    return make_ad(a->n, v, jac_lincomb(a, c, 1.0, nullptr, nullptr, 0.0, nullptr));
 }
 
 BP s_compare(int op, const SP& a, const SP& b) {
@@ -308,8 +351,9 @@ SP s_select(const BP& p, const SP& t, const SP& f) {
         out = pa ? pa : pb;
     }
     auto j = dev_alloc<double>(out->nnz);
-    launch_jselect(out->nnz, out->row_of(), p->val->p, pa ? t->jval->p : nullptr, ia, pb ? f->jval->p : nullptr, ib,
-                   j->p);
+    launch_jselect(out->nnz, out->row_of(), p->val->p, pa ? t->jval->p : nullptr, ia, pa ? t->jscale : 1.0,
+                   pa ? rowp(t->jrow) : nullptr, pb ? f->jval->p : nullptr, ib, pb ? f->jscale : 1.0,
+                   pb ? rowp(f->jrow) : nullptr, j->p);
     return make_scalar(t->n, v, true, out, j);
 }
 
@@ -339,10 +383,21 @@ SP s_matmul(const DevCsr& m, const SP& a, DevPtr<double> values) {
         launch_csr_spmv(m.pat->rows, m.pat->d_rp->p, m.pat->d_ci->p, m.val->p, a->val->p, v->p);
     }
     if (!a->ad || !a->pat) return make_scalar(m.pat->rows, v, a->ad, nullptr, nullptr);
+    JacOut o;
+    o.s = a->jscale;
+    if (a->jones && !a->jrow && m.pat->ascending()) {
+        // M * (s I) = s M: the structural product is M's own pattern, one contribution per entry
+        o.pat = m.pat;
+        o.val = m.val;
+        return make_ad(m.pat->rows, v, o);
+    }
     SymP e = sym_spgemm(m.pat, a->pat);
-    auto j = dev_alloc<double>(e->out->nnz);
-    launch_spgemm_num(e->out->nnz, e->seg->p, e->mi->p, e->ai->p, m.val->p, a->jval->p, j->p);
-    return make_scalar(m.pat->rows, v, true, e->out, j);
+    o.pat = e->out;
+    o.val = dev_alloc<double>(e->out->nnz);
+    o.s = 1.0;
+    launch_spgemm_num(e->out->nnz, e->seg->p, e->mi->p, e->ai->p, m.val->p, a->jval->p,
+                      a->jrow ? a->pat->row_of() : nullptr, rowp(a->jrow), a->jscale, o.val->p);
+    return make_ad(m.pat->rows, v, o);
 }
 
 DevCsr csr_add(const DevCsr& a, double alpha, const DevCsr& b, double beta) {
@@ -351,15 +406,15 @@ DevCsr csr_add(const DevCsr& a, double alpha, const DevCsr& b, double beta) {
     if (!a.pat || !b.pat || a.pat.get() == b.pat.get()) {
         o.pat = a.pat ? a.pat : b.pat;
         o.val = dev_alloc<double>(o.pat->nnz);
-        launch_jlin(o.pat->nnz, nullptr, a.pat ? a.val->p : nullptr, nullptr, nullptr, alpha,
-                    b.pat ? b.val->p : nullptr, nullptr, nullptr, beta, nullptr, o.val->p);
+        launch_jlin(o.pat->nnz, nullptr, a.pat ? a.val->p : nullptr, nullptr, nullptr, nullptr, alpha,
+                    b.pat ? b.val->p : nullptr, nullptr, nullptr, nullptr, beta, nullptr, o.val->p);
         return o;
     }
     SymP e = sym_union(a.pat, b.pat);
     o.pat = e->out;
     o.val = dev_alloc<double>(o.pat->nnz);
-    launch_jlin(o.pat->nnz, nullptr, a.val->p, e->ia->p, nullptr, alpha, b.val->p, e->ib->p, nullptr, beta, nullptr,
-                o.val->p);
+    launch_jlin(o.pat->nnz, nullptr, a.val->p, e->ia->p, nullptr, nullptr, alpha, b.val->p, e->ib->p, nullptr, nullptr,
+                beta, nullptr, o.val->p);
     return o;
 }
 
@@ -369,8 +424,39 @@ DevCsr csr_mul(const DevCsr& a, const DevCsr& b) {
     DevCsr o;
     o.pat = e->out;
     o.val = dev_alloc<double>(o.pat->nnz);
-    launch_spgemm_num(o.pat->nnz, e->seg->p, e->mi->p, e->ai->p, a.val->p, b.val->p, o.val->p);
+    launch_spgemm_num(o.pat->nnz, e->seg->p, e->mi->p, e->ai->p, a.val->p, b.val->p, nullptr, nullptr, 1.0,
+                      o.val->p);
     return o;
+}
+
+// Row gather (On / Extend) of a Jacobian: the stored values of the selected rows, the
+// pending factors carried along (row factors gathered with the rows; -1 => empty row).
+static JacOut gather_rows(const SP& a, const SymP& e, const int* rowmap, int m) {
+    JacOut o;
+    o.pat = e->out;
+    o.val = dev_alloc<double>(e->out->nnz);
+    launch_gather(a->jval->p, e->ia->p, o.val->p, e->out->nnz);
+    o.s = a->jscale;
+    if (a->jrow) {
+        o.row = dev_alloc<double>(std::max(1, m));
+        launch_gather(a->jrow->p, rowmap, o.row->p, m);
+    }
+    return o;
+}
+
+DevPtr<double> jac_effective(const ScalarData& a) {
+    if (!a.pat) return nullptr;
+    if (!a.jpending()) return a.jval;
+    if (!a.jeff) {
+        auto v = dev_alloc<double>(std::max(1, a.pat->nnz));
+        launch_jscale(a.pat->nnz, a.jrow ? a.pat->row_of() : nullptr, a.jscale, rowp(a.jrow), a.jval->p, v->p);
+        a.jeff = v;
+    }
+    return a.jeff;
+}
+const double* jac_values(const ScalarData& a) {
+    DevPtr<double> v = jac_effective(a);
+    return v ? v->p : nullptr;
 }
 
 // ---------------------------------------------------------------- index sets
@@ -435,9 +521,7 @@ SP s_on(const SP& a, const IP& from, const IP& to) {
     launch_gather(a->val->p, pm->d_pos->p, v->p, m);
     if (!a->ad || !a->pat) return make_scalar(m, v, a->ad, nullptr, nullptr);
     SymP e = sym_rows(a->pat, pm->pos, pm->id, m);
-    auto j = dev_alloc<double>(e->out->nnz);
-    launch_gather(a->jval->p, e->ia->p, j->p, e->out->nnz);
-    return make_scalar(m, v, true, e->out, j);
+    return make_ad(m, v, gather_rows(a, e, pm->d_pos->p, m));
 }
 
 SP s_extend(const SP& a, const IP& from, const IP& to) {
@@ -451,9 +535,7 @@ SP s_extend(const SP& a, const IP& from, const IP& to) {
     launch_gather(a->val->p, pm->d_inv->p, v->p, m);
     if (!a->ad || !a->pat) return make_scalar(m, v, a->ad, nullptr, nullptr);
     SymP e = sym_rows(a->pat, pm->inv, pm->id ^ 0x8000000000000000ull, m);
-    auto j = dev_alloc<double>(e->out->nnz);
-    launch_gather(a->jval->p, e->ia->p, j->p, e->out->nnz);
-    return make_scalar(m, v, true, e->out, j);
+    return make_ad(m, v, gather_rows(a, e, pm->d_inv->p, m));
 }
 
 IP i_from_device(int codim, DevPtr<int> d, int n) {

@@ -37,6 +37,10 @@ SP s_select(const BP& p, const SP& t, const SP& f);
 double s_reduce(int op, const SP& a);
 double s_two_norm(const SP& a);
 std::vector<double> s_to_host(const SP& a);
+// The Jacobian's values with its pending factors applied (null without a Jacobian); the
+// first request on a value with pending factors runs one kernel and keeps the result.
+DevPtr<double> jac_effective(const ScalarData& a);
+const double* jac_values(const ScalarData& a);
 
 // Matrix * CollOfScalar with a device CSR matrix (values on device, pattern cached)
 struct DevCsr {

@@ -183,6 +183,10 @@ CollOfScalar::CollOfScalar(const CudaArray& val) : d_(val.data() ? val.data() : 
 CollOfScalar::CollOfScalar(const CudaArray& val, const CudaMatrix& der) {
     auto s = std::make_shared<eqhip::ScalarData>(*val.data());
     s->ad = true;
+    s->jscale = 1.0;
+    s->jrow = nullptr;
+    s->jones = false;
+    s->jeff = nullptr;
     if (!der.isEmpty()) {
         if (der.rows() != s->n) OPM_THROW(std::runtime_error, "CollOfScalar: Jacobian rows != collection size");
         s->pat = der.data()->m.pat;
@@ -215,7 +219,7 @@ CudaMatrix CollOfScalar::derivative() const {
     if (!d_->pat) return CudaMatrix();
     eqhip::DevCsr m;
     m.pat = d_->pat;
-    m.val = d_->jval;
+    m.val = eqhip::jac_effective(*d_);
     return CudaMatrix(mat(d_->pat->rows, d_->pat->cols, m));
 }
 CudaArray CollOfScalar::value() const { return CudaArray(d_); }

@@ -144,28 +144,30 @@ __global__ void k_gather_int(const int* __restrict__ in, const int* __restrict__
 // ---------------------------------------------------------------------------
 // Jacobian numeric kernels
 __global__ void k_jlin(long nnz, const int* __restrict__ row, const double* __restrict__ a,
-                       const int* __restrict__ ia, const double* __restrict__ A, double alpha,
-                       const double* __restrict__ b, const int* __restrict__ ib, const double* __restrict__ B,
-                       double beta, const double* __restrict__ C, double* __restrict__ out) {
+                       const int* __restrict__ ia, const double* __restrict__ A, const double* __restrict__ RA,
+                       double alpha, const double* __restrict__ b, const int* __restrict__ ib,
+                       const double* __restrict__ B, const double* __restrict__ RB, double beta,
+                       const double* __restrict__ C, double* __restrict__ out) {
     long k = gid();
     if (k >= nnz) return;
-    const bool need_row = A || B || C;
+    const bool need_row = A || B || C || RA || RB;
     const int r = need_row ? row[k] : 0;
     double s = 0.0;
     if (a) {
         const long j = ia ? ia[k] : k;
-        if (j >= 0) s = alpha * (A ? A[r] : 1.0) * a[j];
+        if (j >= 0) s = alpha * (A ? A[r] : 1.0) * (RA ? RA[r] : 1.0) * a[j];
     }
     if (b) {
         const long j = ib ? ib[k] : k;
-        if (j >= 0) s += beta * (B ? B[r] : 1.0) * b[j];
+        if (j >= 0) s += beta * (B ? B[r] : 1.0) * (RB ? RB[r] : 1.0) * b[j];
     }
     out[k] = C ? C[r] * s : s;
 }
 
 __global__ void k_jselect(long nnz, const int* __restrict__ row, const uint8_t* __restrict__ p,
-                          const double* __restrict__ a, const int* __restrict__ ia, const double* __restrict__ b,
-                          const int* __restrict__ ib, double* __restrict__ out) {
+                          const double* __restrict__ a, const int* __restrict__ ia, double sa,
+                          const double* __restrict__ RA, const double* __restrict__ b, const int* __restrict__ ib,
+                          double sb, const double* __restrict__ RB, double* __restrict__ out) {
     long k = gid();
     if (k >= nnz) return;
     const int r = row[k];
@@ -173,23 +175,35 @@ __global__ void k_jselect(long nnz, const int* __restrict__ row, const uint8_t* 
     if (p[r]) {
         if (a) {
             const long j = ia ? ia[k] : k;
-            if (j >= 0) v = a[j];
+            if (j >= 0) v = sa * (RA ? RA[r] : 1.0) * a[j];
         }
     } else if (b) {
         const long j = ib ? ib[k] : k;
-        if (j >= 0) v = b[j];
+        if (j >= 0) v = sb * (RB ? RB[r] : 1.0) * b[j];
     }
     out[k] = v;
 }
 
 __global__ void k_spgemm_num(long nnz, const int* __restrict__ seg, const int* __restrict__ mi,
                              const int* __restrict__ ai, const double* __restrict__ mv, const double* __restrict__ av,
+                             const int* __restrict__ arow, const double* __restrict__ ar, double scale,
                              double* __restrict__ out) {
     long k = gid();
     if (k >= nnz) return;
     double s = 0.0;
-    for (int t = seg[k]; t < seg[k + 1]; ++t) s += mv[mi[t]] * av[ai[t]];
-    out[k] = s;
+    if (ar) {
+        for (int t = seg[k]; t < seg[k + 1]; ++t) s += mv[mi[t]] * (ar[arow[ai[t]]] * av[ai[t]]);
+    } else {
+        for (int t = seg[k]; t < seg[k + 1]; ++t) s += mv[mi[t]] * av[ai[t]];
+    }
+    out[k] = scale * s;
+}
+
+__global__ void k_jscale(long nnz, const int* __restrict__ row, double s, const double* __restrict__ R,
+                         const double* __restrict__ v, double* __restrict__ out) {
+    long k = gid();
+    if (k >= nnz) return;
+    out[k] = s * (R ? R[row[k]] : 1.0) * v[k];
 }
 
 __global__ void k_csr_spmv(int rows, const int* __restrict__ rp, const int* __restrict__ ci,
@@ -468,17 +482,21 @@ void launch_gather(const double* in, const int* map, double* out, long n) {
 void launch_gather_int(const int* in, const int* map, int* out, int n) {
     if (n > 0) EQ_LAUNCH(k_gather_int, blocks_for(n), in, map, out, n);
 }
-void launch_jlin(long nnz, const int* row, const double* a, const int* ia, const double* A, double alpha,
-                 const double* b, const int* ib, const double* B, double beta, const double* C, double* out) {
-    if (nnz > 0) EQ_LAUNCH(k_jlin, blocks_for(nnz), nnz, row, a, ia, A, alpha, b, ib, B, beta, C, out);
+void launch_jlin(long nnz, const int* row, const double* a, const int* ia, const double* A, const double* RA,
+                 double alpha, const double* b, const int* ib, const double* B, const double* RB, double beta,
+                 const double* C, double* out) {
+    if (nnz > 0) EQ_LAUNCH(k_jlin, blocks_for(nnz), nnz, row, a, ia, A, RA, alpha, b, ib, B, RB, beta, C, out);
 }
-void launch_jselect(long nnz, const int* row, const uint8_t* p, const double* a, const int* ia, const double* b,
-                    const int* ib, double* out) {
-    if (nnz > 0) EQ_LAUNCH(k_jselect, blocks_for(nnz), nnz, row, p, a, ia, b, ib, out);
+void launch_jselect(long nnz, const int* row, const uint8_t* p, const double* a, const int* ia, double sa,
+                    const double* RA, const double* b, const int* ib, double sb, const double* RB, double* out) {
+    if (nnz > 0) EQ_LAUNCH(k_jselect, blocks_for(nnz), nnz, row, p, a, ia, sa, RA, b, ib, sb, RB, out);
 }
 void launch_spgemm_num(long nnz, const int* seg, const int* mi, const int* ai, const double* mv, const double* av,
-                       double* out) {
-    if (nnz > 0) EQ_LAUNCH(k_spgemm_num, blocks_for(nnz), nnz, seg, mi, ai, mv, av, out);
+                       const int* arow, const double* ar, double scale, double* out) {
+    if (nnz > 0) EQ_LAUNCH(k_spgemm_num, blocks_for(nnz), nnz, seg, mi, ai, mv, av, arow, ar, scale, out);
+}
+void launch_jscale(long nnz, const int* row, double s, const double* R, const double* v, double* out) {
+    if (nnz > 0) EQ_LAUNCH(k_jscale, blocks_for(nnz), nnz, row, s, R, v, out);
 }
 void launch_csr_spmv(int rows, const int* rp, const int* ci, const double* v, const double* x, double* y) {
     if (rows > 0) EQ_LAUNCH(k_csr_spmv, blocks_for(rows), rows, rp, ci, v, x, y);

@@ -30,17 +30,21 @@ void launch_gather(const double* in, const int* map, double* out, long n);
 void launch_gather_int(const int* in, const int* map, int* out, int n);
 
 // Jacobian numeric kernels (one thread per output entry).
-//   out[k] = C(r) * ( alpha*A(r)*a[ia[k]] + beta*B(r)*b[ib[k]] )
+//   out[k] = C(r) * ( alpha*A(r)*RA(r)*a[ia[k]] + beta*B(r)*RB(r)*b[ib[k]] )
 // with r = row[k]; a/b may be null (term absent); ia/ib null => identity map;
-// ia/ib entries -1 => term absent for that entry; A, B, C null => 1.
-void launch_jlin(long nnz, const int* row, const double* a, const int* ia, const double* A, double alpha,
-                 const double* b, const int* ib, const double* B, double beta, const double* C, double* out);
-// out[k] = p[r] ? a[ia[k]] : b[ib[k]]  (missing => 0)
-void launch_jselect(long nnz, const int* row, const uint8_t* p, const double* a, const int* ia, const double* b,
-                    const int* ib, double* out);
-// out[k] = sum_{t in [seg[k], seg[k+1])} mv[mi[t]] * av[ai[t]]
+// ia/ib entries -1 => term absent for that entry; A, RA, B, RB, C null => 1.
+void launch_jlin(long nnz, const int* row, const double* a, const int* ia, const double* A, const double* RA,
+                 double alpha, const double* b, const int* ib, const double* B, const double* RB, double beta,
+                 const double* C, double* out);
+// out[k] = p[r] ? sa*RA(r)*a[ia[k]] : sb*RB(r)*b[ib[k]]  (missing => 0)
+void launch_jselect(long nnz, const int* row, const uint8_t* p, const double* a, const int* ia, double sa,
+                    const double* RA, const double* b, const int* ib, double sb, const double* RB, double* out);
+// out[k] = scale * sum_{t in [seg[k], seg[k+1])} mv[mi[t]] * ar(arow[ai[t]]) * av[ai[t]]
+// (ar: row factors of the operand, indexed by the row of its entry; null => 1)
 void launch_spgemm_num(long nnz, const int* seg, const int* mi, const int* ai, const double* mv, const double* av,
-                       double* out);
+                       const int* arow, const double* ar, double scale, double* out);
+// out[k] = s * R(row[k]) * v[k]  (R null => 1)
+void launch_jscale(long nnz, const int* row, double s, const double* R, const double* v, double* out);
 
 // y = A x, one thread per row, entries summed in column order.
 void launch_csr_spmv(int rows, const int* rp, const int* ci, const double* v, const double* x, double* y);

@@ -1092,14 +1092,14 @@ SP linear_solve(const SolverConfig& cfg, const SP& res, LinReport* rep) {
     auto x = dev_alloc<double>(n);
     if (cfg.precision == "fp64") {
         System<double> A;
-        A.setup(P, res->jval->p, pc);
+        A.setup(P, jac_values(*res), pc);
         double rr = 0;
         R.iterations = run_method<double>(cfg, A, res->val->p, x->p, cfg.tol, cfg.max_iter, &rr);
         R.residual = rr;
         R.converged = rr <= cfg.tol;
     } else if (cfg.precision == "mixed") {
         System<float> A;
-        A.setup(P, res->jval->p, pc);
+        A.setup(P, jac_values(*res), pc);
         auto r64 = dev_alloc<double>(n), y64 = dev_alloc<double>(n);
         auto r32 = dev_alloc<float>(n), d32 = dev_alloc<float>(n);
         EQ_HIP(hipMemsetAsync(x->p, 0, n * sizeof(double), S()));
@@ -1115,7 +1115,7 @@ SP linear_solve(const SolverConfig& cfg, const SP& res, LinReport* rep) {
                                        cfg.max_iter - total, &rr);
             KL(k_axpy_mixed<float>, nblk(n), (long)n, rn, d32->p, x->p);
             // FP64 defect
-            launch_csr_spmv(n, P->d_rp->p, P->d_ci->p, res->jval->p, x->p, y64->p);
+            launch_csr_spmv(n, P->d_rp->p, P->d_ci->p, jac_values(*res), x->p, y64->p);
             KL(k_residual, nblk(n), (long)n, res->val->p, y64->p, r64->p);
             const double rnew = dnorm(r64->p, n);
             ++R.refinements;
@@ -1145,14 +1145,14 @@ void sell_spmv_test(const SP& a, const double* xh, double* yh, bool fp32) {
     auto y = dev_alloc<double>(n);
     if (fp32) {
         System<float> A;
-        A.setup(a->pat, a->jval->p, "none");
+        A.setup(a->pat, jac_values(*a), "none");
         auto x32 = dev_alloc<float>(n), y32 = dev_alloc<float>(n);
         KL((k_cast<double, float>), nblk(n), (long)n, x->p, 1.0, x32->p);
         A.spmv(x32->p, y32->p);
         KL((k_cast<float, double>), nblk(n), (long)n, y32->p, 1.0, y->p);
     } else {
         System<double> A;
-        A.setup(a->pat, a->jval->p, "none");
+        A.setup(a->pat, jac_values(*a), "none");
         A.spmv(x->p, y->p);
     }
     std::vector<double> h = dev_to_host(y->p, n);

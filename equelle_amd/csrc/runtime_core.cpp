@@ -398,7 +398,7 @@ SP RuntimeCore::solve_for_update(const SP& residual, LinReport* rep) const {
         }
         const Pattern& P = *residual->pat;
         auto x = dev_alloc<double>(std::max(1, residual->n));
-        const LinearSolveStats st = stencil_->solve_linear(stencil_slot_, P.d_rp->p, P.d_ci->p, residual->jval->p,
+        const LinearSolveStats st = stencil_->solve_linear(stencil_slot_, P.d_rp->p, P.d_ci->p, jac_values(*residual),
                                                            residual->val->p, x->p);
         LinReport R;
         R.iterations = st.iterations;
