@@ -711,6 +711,8 @@ static FusedConfig fused_config(const char* params) {
     c.pair_vectors = m.get_int("pair_vectors", c.pair_vectors ? 1 : 0) != 0;
     c.twophase_pressure_cg = m.get_int("twophase_pressure_cg", c.twophase_pressure_cg ? 1 : 0) != 0;
     c.twophase_pressure_mg = m.get_int("twophase_pressure_mg", c.twophase_pressure_mg ? 1 : 0) != 0;
+    c.mg_scale = m.get_double("mg_scale", c.mg_scale);
+    EQ_REQUIRE(c.mg_scale > 0.0 && c.mg_scale < 2.0, "mg_scale must lie in (0, 2), got " << c.mg_scale);
     c.max_refine = m.get_int("max_refine", c.max_refine);
     c.graph = m.get_int("graph", 1) != 0;
     c.speculate = m.get_int("speculate", 1) != 0;

@@ -59,6 +59,7 @@ struct Desc {
     int mgl;                      // multigrid: levels below this one (0 on the coarsest level)
     int mgt;                      // multigrid: the level belongs to the instance's workgroup tail (k_mg_tail)
     void* minv;                   // multigrid, coarsest level: dense (A + s 1 1^T)^-1, cell order
+    double mgs;                   // multigrid: factor of the coarse-grid correction (z += mgs P z_c)
     double rnx, rnxy;             // 1 / nx, 1 / (nx ny): cell -> (i, j, k) without integer division
     double dt;
     // heat: k(u) = k (1 + u^2), x / y face geometry factors, cell volume
@@ -1408,9 +1409,10 @@ __global__ __launch_bounds__(BS) void k_b3(const Desc* __restrict__ D, const int
 // the next level's right-hand side.  On the coarsest level (at most MG_COARSE cells):
 // z = (A + s 1 1^T)^-1 b with a dense inverse formed at set-up; s 1 1^T (s = mean
 // diagonal / n) removes the constant null space of the singular pressure operator, and
-// for b with zero sum the result is the solution with zero sum.  Going up: z += P
-// z_coarse, then a black and a red half sweep.  Pre- and post-smoothing are adjoint and
-// the coarse solve is symmetric, so M is symmetric positive definite and CG applies.
+// for b with zero sum the result is the solution with zero sum.  Going up: z += s P
+// z_coarse (s = Desc::mgs, the over-correction factor, 0 < s < 2), then a black and a
+// red half sweep.  Pre- and post-smoothing are adjoint and the coarse solve is symmetric,
+// so M is symmetric positive definite and CG applies.
 // Desc::mgl counts the levels below a level (0 on the coarsest); the kernels of one
 // level skip the instances whose coarsest level it is.
 constexpr int MG_COARSE = 64;
@@ -1473,13 +1475,13 @@ __device__ inline void mg_prolong_at(const Desc& f, const Desc& c, int p) {
         int i, j, k;
         if (cell_of(f, p, col, i, j, k) < 0) continue;
         const int ic = i >> 1, jc = j >> 1, kc = k >> 1;
-        xf[col * f.h + p] += xc[rb_slot(c, ic + c.nx * (jc + c.ny * kc), ic, jc, kc)];
+        xf[col * f.h + p] += (T)f.mgs * xc[rb_slot(c, ic + c.nx * (jc + c.ny * kc), ic, jc, kc)];
     }
 }
 
-// offdiag of a black cell whose red neighbours are read as z_r + z_c of the neighbour's
+// offdiag of a black cell whose red neighbours are read as z_r + s z_c of the neighbour's
 // aggregate (level cd): the prolongation applied on the fly.  Going up, the red values
-// after z += P z_c are read only by the black half sweep that follows (the red half sweep
+// after z += s P z_c are read only by the black half sweep that follows (the red half sweep
 // after it overwrites them, and the black sweep overwrites the black ones), so the black
 // sweep computes them itself: the same sums in the same order as offdiag on the
 // prolonged vector.
@@ -1489,7 +1491,7 @@ __device__ inline T offdiag_prolonged(const Desc& d, const Desc& cd, int s, int 
     const int nx = d.nx, ny = d.ny;
     auto z = [&](int nb, int ni, int nj, int nk) -> T {
         const int ic = ni >> 1, jc = nj >> 1, kc = nk >> 1;
-        return x[nb >> 1] + xc[rb_slot(cd, ic + cd.nx * (jc + cd.ny * kc), ic, jc, kc)];
+        return x[nb >> 1] + (T)d.mgs * xc[rb_slot(cd, ic + cd.nx * (jc + cd.ny * kc), ic, jc, kc)];
     };
     T acc = (T)0;
     if (i > 0) acc += ((const T*)d.K[1])[s] * z(c - 1, i - 1, j, k);
@@ -2355,6 +2357,7 @@ struct HeatBatch::Impl {
                 I.nx % 8 == 0;
         d.cgp = cg_capable(I);
         d.mgp = I.mg_on;
+        d.mgs = cfg.mg_scale;
         d.mgl = (int)I.mg.size();
         d.mgt = I.mg_on && I.mg_tail == 0;
         d.minv = I.mg_on && I.mg.empty() ? (void*)I.minv->p : nullptr;
