@@ -135,6 +135,7 @@ struct SymEntry {
     DevPtr<int> ia, ib;        // per output entry: index into operand a / b values, -1 if absent
     DevPtr<int> seg;           // spgemm: segment offsets (out.nnz+1)
     DevPtr<int> mi, ai;        // spgemm: (matrix entry, operand entry) pairs
+    DevPtr<int> sai;           // spgemm with a +-1 matrix: operand entry, ~entry where the sign is -1
     std::vector<std::weak_ptr<const Pattern>> inputs;
 };
 using SymP = std::shared_ptr<const SymEntry>;

@@ -63,9 +63,12 @@ SymP sym_union(const PatternP& a, const PatternP& b) {
     return e;
 }
 
-// Structural product M * A: output pattern + (matrix entry, operand entry) pairs.
-SymP sym_spgemm(const PatternP& m, const PatternP& a) {
-    const std::string key = sym_key("spgemm", m->id, a->id);
+// Structural product M * A: output pattern + (matrix entry, operand entry) pairs.  With
+// `sign` (M's values, all +1 or -1): the operand entry of each pair with M's sign folded
+// in (sai = ai, or ~ai where M's entry is -1) instead of the two index arrays.
+SymP sym_spgemm(const PatternP& m, const PatternP& a, const std::vector<int8_t>* sign = nullptr,
+                uint64_t sign_id = 0) {
+    const std::string key = sym_key(sign ? "spgemm_s" : "spgemm", m->id, a->id, sign_id);
     if (SymP e = sym_lookup(key)) return e;
     EQ_REQUIRE(m->cols == a->rows, "Error in CudaMatrix * CudaMatrix size checking: lhs.cols_ = "
                                        << m->cols << " rhs.rows_ = " << a->rows);
@@ -102,8 +105,14 @@ SymP sym_spgemm(const PatternP& m, const PatternP& a) {
     auto e = std::make_shared<SymEntry>();
     e->out = make_pattern(m->rows, a->cols, std::move(rp), std::move(ci));
     e->seg = dev_from_host(seg.data(), seg.size());
-    e->mi = dev_from_host(mi.data(), mi.size());
-    e->ai = dev_from_host(ai.data(), ai.size());
+    if (sign) {
+        for (size_t t = 0; t < ai.size(); ++t)
+            if ((*sign)[mi[t]] < 0) ai[t] = ~ai[t];
+        e->sai = dev_from_host(ai.data(), ai.size());
+    } else {
+        e->mi = dev_from_host(mi.data(), mi.size());
+        e->ai = dev_from_host(ai.data(), ai.size());
+    }
     e->inputs = {m, a};
     sym_store(key, e);
     return e;
@@ -371,6 +380,15 @@ DevCsr csr_from_host(int rows, int cols, const int* rp, const int* ci, const dou
     DevCsr m;
     m.pat = make_pattern(rows, cols, std::vector<int>(rp, rp + rows + 1), std::vector<int>(ci, ci + rp[rows]));
     m.val = dev_from_host(v, rp[rows]);
+    const int nnz = rp[rows];
+    bool unit = nnz > 0;
+    for (int k = 0; k < nnz && unit; ++k) unit = v[k] == 1.0 || v[k] == -1.0;
+    if (unit) {   // a signed incidence matrix (the grid's divergence and gradient operators)
+        auto sg = std::make_shared<std::vector<int8_t>>(nnz);
+        for (int k = 0; k < nnz; ++k) (*sg)[k] = v[k] > 0.0 ? 1 : -1;
+        m.sign = sg;
+        m.sign_id = next_id();
+    }
     return m;
 }
 
@@ -391,12 +409,16 @@ SP s_matmul(const DevCsr& m, const SP& a, DevPtr<double> values) {
         o.val = m.val;
         return make_ad(m.pat->rows, v, o);
     }
-    SymP e = sym_spgemm(m.pat, a->pat);
+    SymP e = sym_spgemm(m.pat, a->pat, m.sign.get(), m.sign_id);
     o.pat = e->out;
     o.val = dev_alloc<double>(e->out->nnz);
     o.s = 1.0;
-    launch_spgemm_num(e->out->nnz, e->seg->p, e->mi->p, e->ai->p, m.val->p, a->jval->p,
-                      a->jrow ? a->pat->row_of() : nullptr, rowp(a->jrow), a->jscale, o.val->p);
+    if (m.sign)   // M's entries are +-1: the signs are folded into the operand indices
+        launch_spgemm_signed(e->out->nnz, e->seg->p, e->sai->p, a->jval->p, a->jrow ? a->pat->row_of() : nullptr,
+                             rowp(a->jrow), a->jscale, o.val->p);
+    else
+        launch_spgemm_num(e->out->nnz, e->seg->p, e->mi->p, e->ai->p, m.val->p, a->jval->p,
+                          a->jrow ? a->pat->row_of() : nullptr, rowp(a->jrow), a->jscale, o.val->p);
     return make_ad(m.pat->rows, v, o);
 }
 

@@ -46,6 +46,10 @@ const double* jac_values(const ScalarData& a);
 struct DevCsr {
     PatternP pat;
     DevPtr<double> val;
+    // host copy of the values' signs when every value is +1 or -1 (set by csr_from_host),
+    // with an id for the symbolic cache; products with it skip the matrix values
+    std::shared_ptr<const std::vector<int8_t>> sign;
+    uint64_t sign_id = 0;
 };
 // `values`: precomputed output values (e.g. from a grid sweep); null => CSR SpMV.
 SP s_matmul(const DevCsr& m, const SP& a, DevPtr<double> values = nullptr);

@@ -199,6 +199,21 @@ __global__ void k_spgemm_num(long nnz, const int* __restrict__ seg, const int* _
     out[k] = scale * s;
 }
 
+__global__ void k_spgemm_signed(long nnz, const int* __restrict__ seg, const int* __restrict__ sai,
+                                const double* __restrict__ av, const int* __restrict__ arow,
+                                const double* __restrict__ ar, double scale, double* __restrict__ out) {
+    long k = gid();
+    if (k >= nnz) return;
+    double s = 0.0;
+    for (int t = seg[k]; t < seg[k + 1]; ++t) {
+        const int e = sai[t];
+        const int j = e >= 0 ? e : ~e;
+        const double v = ar ? ar[arow[j]] * av[j] : av[j];
+        s += e >= 0 ? v : -v;
+    }
+    out[k] = scale * s;
+}
+
 __global__ void k_jscale(long nnz, const int* __restrict__ row, double s, const double* __restrict__ R,
                          const double* __restrict__ v, double* __restrict__ out) {
     long k = gid();
@@ -494,6 +509,10 @@ void launch_jselect(long nnz, const int* row, const uint8_t* p, const double* a,
 void launch_spgemm_num(long nnz, const int* seg, const int* mi, const int* ai, const double* mv, const double* av,
                        const int* arow, const double* ar, double scale, double* out) {
     if (nnz > 0) EQ_LAUNCH(k_spgemm_num, blocks_for(nnz), nnz, seg, mi, ai, mv, av, arow, ar, scale, out);
+}
+void launch_spgemm_signed(long nnz, const int* seg, const int* sai, const double* av, const int* arow,
+                          const double* ar, double scale, double* out) {
+    if (nnz > 0) EQ_LAUNCH(k_spgemm_signed, blocks_for(nnz), nnz, seg, sai, av, arow, ar, scale, out);
 }
 void launch_jscale(long nnz, const int* row, double s, const double* R, const double* v, double* out) {
     if (nnz > 0) EQ_LAUNCH(k_jscale, blocks_for(nnz), nnz, row, s, R, v, out);

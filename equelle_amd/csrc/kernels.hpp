@@ -43,6 +43,9 @@ void launch_jselect(long nnz, const int* row, const uint8_t* p, const double* a,
 // (ar: row factors of the operand, indexed by the row of its entry; null => 1)
 void launch_spgemm_num(long nnz, const int* seg, const int* mi, const int* ai, const double* mv, const double* av,
                        const int* arow, const double* ar, double scale, double* out);
+// the same for a matrix of +-1 entries: sai[t] = operand entry, or ~entry for a -1
+void launch_spgemm_signed(long nnz, const int* seg, const int* sai, const double* av, const int* arow,
+                          const double* ar, double scale, double* out);
 // out[k] = s * R(row[k]) * v[k]  (R null => 1)
 void launch_jscale(long nnz, const int* row, double s, const double* R, const double* v, double* out);
 
