@@ -130,7 +130,9 @@ PatternP identity_pattern(int n);
 
 // Symbolic results cached by (op, input ids): output pattern plus device index
 // maps that drive the numeric kernels.
+uint64_t next_id();
 struct SymEntry {
+    uint64_t id = next_id();   // names the entry's index maps (compositions are cached by it)
     PatternP out;
     DevPtr<int> ia, ib;        // per output entry: index into operand a / b values, -1 if absent
     DevPtr<int> seg;           // spgemm: segment offsets (out.nnz+1)
@@ -157,6 +159,10 @@ struct ScalarData {
     // that reads the values (or once, by jac_values, where a caller needs them).
     double jscale = 1.0;
     DevPtr<double> jrow;   // n row factors, or null
+    // stored value of entry k: jval[jmap[k]] (jmap null: jval[k]); set by row gathers (On,
+    // Extend), which then share the operand's values instead of copying them
+    DevPtr<int> jmap;
+    uint64_t jmap_id = 0;
     bool jones = false;    // (pat, jval) is the n x n identity with unit values (a primary variable)
     mutable DevPtr<double> jeff;   // the factors applied (materialized on first request)
     bool jpending() const { return jscale != 1.0 || (bool)jrow; }

@@ -147,6 +147,8 @@ struct JacOut {
     double s = 1.0;
     DevPtr<double> row;
     bool ones = false;
+    DevPtr<int> map;
+    uint64_t map_id = 0;
 };
 
 SP make_ad(int n, DevPtr<double> val, const JacOut& j) {
@@ -159,7 +161,36 @@ SP make_ad(int n, DevPtr<double> val, const JacOut& j) {
     s->jscale = j.s;
     s->jrow = j.row;
     s->jones = j.ones;
+    s->jmap = j.map;
+    s->jmap_id = j.map_id;
     return s;
+}
+
+// Id of map `which` (0: ia, 1: ib) of a symbolic entry.
+uint64_t map_id(const SymEntry& e, int which) { return (e.id << 1) | (uint64_t)which; }
+
+// The composition inner[outer[k]] (-1 where outer is -1) of two entry maps, cached by
+// their ids; `pats` are the patterns the maps belong to (the entry dies with them).
+std::pair<DevPtr<int>, uint64_t> compose_maps(const DevPtr<int>& outer, uint64_t outer_id, long n,
+                                              const DevPtr<int>& inner, uint64_t inner_id,
+                                              std::vector<std::weak_ptr<const Pattern>> pats) {
+    const std::string key = sym_key("compose", outer_id, inner_id);
+    if (SymP e = sym_lookup(key)) return {e->ia, map_id(*e, 0)};
+    auto e = std::make_shared<SymEntry>();
+    e->ia = dev_alloc<int>(std::max(1L, n));
+    if (n > 0) launch_gather_int(inner->p, outer->p, e->ia->p, (int)n);
+    e->inputs = std::move(pats);
+    sym_store(key, e);
+    return {e->ia, map_id(*e, 0)};
+}
+
+// The index into x's stored values of every entry of pattern `out`, whose entries map to
+// x's (logical) entries by `ia` (id ia_id; null: the same pattern; -1: absent), with x's
+// own entry map composed in; null when neither maps.
+const int* value_index(const SP& x, const DevPtr<int>& ia, uint64_t ia_id, const PatternP& out) {
+    if (!x->jmap) return ia ? ia->p : nullptr;
+    if (!ia) return x->jmap->p;
+    return compose_maps(ia, ia_id, (long)out->nnz, x->jmap, x->jmap_id, {out, x->pat}).first->p;
 }
 
 // The elementwise product of the given row-factor arrays (null entries skipped; null if
@@ -196,10 +227,12 @@ JacOut jac_lincomb(const SP& a, const DevPtr<double>& A, double alpha, const SP&
         if (pa.get() != pb.get()) {
             SymP e = sym_union(pa, pb);
             o.pat = e->out;
-            ia = e->ia->p;
-            ib = e->ib->p;
+            ia = value_index(a, e->ia, map_id(*e, 0), o.pat);
+            ib = value_index(b, e->ib, map_id(*e, 1), o.pat);
         } else {
             o.pat = pa;
+            ia = value_index(a, nullptr, 0, o.pat);
+            ib = value_index(b, nullptr, 0, o.pat);
         }
         o.val = dev_alloc<double>(o.pat->nnz);
         const bool rows = A || B || C || a->jrow || b->jrow;
@@ -210,6 +243,8 @@ JacOut jac_lincomb(const SP& a, const DevPtr<double>& A, double alpha, const SP&
     const SP& x = pa ? a : b;
     o.pat = x->pat;
     o.val = x->jval;
+    o.map = x->jmap;
+    o.map_id = x->jmap_id;
     o.s = (pa ? alpha : beta) * x->jscale;
     o.row = row_product({C, pa ? A : B, x->jrow}, x->n);
     o.ones = x->jones;
@@ -354,10 +389,12 @@ SP s_select(const BP& p, const SP& t, const SP& f) {
     if (pa && pb && pa.get() != pb.get()) {
         e = sym_union(pa, pb);
         out = e->out;
-        ia = e->ia->p;
-        ib = e->ib->p;
+        ia = value_index(t, e->ia, map_id(*e, 0), out);
+        ib = value_index(f, e->ib, map_id(*e, 1), out);
     } else {
         out = pa ? pa : pb;
+        if (pa) ia = value_index(t, nullptr, 0, out);
+        if (pb) ib = value_index(f, nullptr, 0, out);
     }
     auto j = dev_alloc<double>(out->nnz);
     launch_jselect(out->nnz, out->row_of(), p->val->p, pa ? t->jval->p : nullptr, ia, pa ? t->jscale : 1.0,
@@ -413,11 +450,16 @@ SP s_matmul(const DevCsr& m, const SP& a, DevPtr<double> values) {
     o.pat = e->out;
     o.val = dev_alloc<double>(e->out->nnz);
     o.s = 1.0;
+    DevPtr<double> av = a->jval;
+    if (a->jmap) {   // the operand's values in entry order
+        av = dev_alloc<double>(std::max(1, a->pat->nnz));
+        launch_gather(a->jval->p, a->jmap->p, av->p, a->pat->nnz);
+    }
     if (m.sign)   // M's entries are +-1: the signs are folded into the operand indices
-        launch_spgemm_signed(e->out->nnz, e->seg->p, e->sai->p, a->jval->p, a->jrow ? a->pat->row_of() : nullptr,
+        launch_spgemm_signed(e->out->nnz, e->seg->p, e->sai->p, av->p, a->jrow ? a->pat->row_of() : nullptr,
                              rowp(a->jrow), a->jscale, o.val->p);
     else
-        launch_spgemm_num(e->out->nnz, e->seg->p, e->mi->p, e->ai->p, m.val->p, a->jval->p,
+        launch_spgemm_num(e->out->nnz, e->seg->p, e->mi->p, e->ai->p, m.val->p, av->p,
                           a->jrow ? a->pat->row_of() : nullptr, rowp(a->jrow), a->jscale, o.val->p);
     return make_ad(m.pat->rows, v, o);
 }
@@ -456,8 +498,14 @@ DevCsr csr_mul(const DevCsr& a, const DevCsr& b) {
 static JacOut gather_rows(const SP& a, const SymP& e, const int* rowmap, int m) {
     JacOut o;
     o.pat = e->out;
-    o.val = dev_alloc<double>(e->out->nnz);
-    launch_gather(a->jval->p, e->ia->p, o.val->p, e->out->nnz);
+    o.val = a->jval;   // shared: the selected entries are addressed through the entry map
+    if (!a->jmap) {
+        o.map = e->ia;
+        o.map_id = map_id(*e, 0);
+    } else {
+        std::tie(o.map, o.map_id) =
+            compose_maps(e->ia, map_id(*e, 0), (long)e->out->nnz, a->jmap, a->jmap_id, {e->out, a->pat});
+    }
     o.s = a->jscale;
     if (a->jrow) {
         o.row = dev_alloc<double>(std::max(1, m));
@@ -468,10 +516,11 @@ static JacOut gather_rows(const SP& a, const SymP& e, const int* rowmap, int m) 
 
 DevPtr<double> jac_effective(const ScalarData& a) {
     if (!a.pat) return nullptr;
-    if (!a.jpending()) return a.jval;
+    if (!a.jpending() && !a.jmap) return a.jval;
     if (!a.jeff) {
         auto v = dev_alloc<double>(std::max(1, a.pat->nnz));
-        launch_jscale(a.pat->nnz, a.jrow ? a.pat->row_of() : nullptr, a.jscale, rowp(a.jrow), a.jval->p, v->p);
+        launch_jscale(a.pat->nnz, a.jrow ? a.pat->row_of() : nullptr, a.jscale, rowp(a.jrow),
+                      a.jmap ? a.jmap->p : nullptr, a.jval->p, v->p);
         a.jeff = v;
     }
     return a.jeff;

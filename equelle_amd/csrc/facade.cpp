@@ -186,6 +186,8 @@ CollOfScalar::CollOfScalar(const CudaArray& val, const CudaMatrix& der) {
     s->jscale = 1.0;
     s->jrow = nullptr;
     s->jones = false;
+    s->jmap = nullptr;
+    s->jmap_id = 0;
     s->jeff = nullptr;
     if (!der.isEmpty()) {
         if (der.rows() != s->n) OPM_THROW(std::runtime_error, "CollOfScalar: Jacobian rows != collection size");

@@ -215,10 +215,10 @@ __global__ void k_spgemm_signed(long nnz, const int* __restrict__ seg, const int
 }
 
 __global__ void k_jscale(long nnz, const int* __restrict__ row, double s, const double* __restrict__ R,
-                         const double* __restrict__ v, double* __restrict__ out) {
+                         const int* __restrict__ map, const double* __restrict__ v, double* __restrict__ out) {
     long k = gid();
     if (k >= nnz) return;
-    out[k] = s * (R ? R[row[k]] : 1.0) * v[k];
+    out[k] = s * (R ? R[row[k]] : 1.0) * v[map ? map[k] : k];
 }
 
 __global__ void k_csr_spmv(int rows, const int* __restrict__ rp, const int* __restrict__ ci,
@@ -514,8 +514,9 @@ void launch_spgemm_signed(long nnz, const int* seg, const int* sai, const double
                           const double* ar, double scale, double* out) {
     if (nnz > 0) EQ_LAUNCH(k_spgemm_signed, blocks_for(nnz), nnz, seg, sai, av, arow, ar, scale, out);
 }
-void launch_jscale(long nnz, const int* row, double s, const double* R, const double* v, double* out) {
-    if (nnz > 0) EQ_LAUNCH(k_jscale, blocks_for(nnz), nnz, row, s, R, v, out);
+void launch_jscale(long nnz, const int* row, double s, const double* R, const int* map, const double* v,
+                   double* out) {
+    if (nnz > 0) EQ_LAUNCH(k_jscale, blocks_for(nnz), nnz, row, s, R, map, v, out);
 }
 void launch_csr_spmv(int rows, const int* rp, const int* ci, const double* v, const double* x, double* y) {
     if (rows > 0) EQ_LAUNCH(k_csr_spmv, blocks_for(rows), rows, rp, ci, v, x, y);

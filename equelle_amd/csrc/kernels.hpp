@@ -46,8 +46,9 @@ void launch_spgemm_num(long nnz, const int* seg, const int* mi, const int* ai, c
 // the same for a matrix of +-1 entries: sai[t] = operand entry, or ~entry for a -1
 void launch_spgemm_signed(long nnz, const int* seg, const int* sai, const double* av, const int* arow,
                           const double* ar, double scale, double* out);
-// out[k] = s * R(row[k]) * v[k]  (R null => 1)
-void launch_jscale(long nnz, const int* row, double s, const double* R, const double* v, double* out);
+// out[k] = s * R(row[k]) * v[map[k]]  (R null => 1, map null => k)
+void launch_jscale(long nnz, const int* row, double s, const double* R, const int* map, const double* v,
+                   double* out);
 
 // y = A x, one thread per row, entries summed in column order.
 void launch_csr_spmv(int rows, const int* rp, const int* ci, const double* v, const double* x, double* y);
