@@ -712,6 +712,7 @@ static FusedConfig fused_config(const char* params) {
     c.twophase_pressure_cg = m.get_int("twophase_pressure_cg", c.twophase_pressure_cg ? 1 : 0) != 0;
     c.twophase_pressure_mg = m.get_int("twophase_pressure_mg", c.twophase_pressure_mg ? 1 : 0) != 0;
     c.mg_scale = m.get_double("mg_scale", c.mg_scale);
+    c.twophase_split_form = m.get_int("twophase_split_form", c.twophase_split_form ? 1 : 0) != 0;
     EQ_REQUIRE(c.mg_scale > 0.0 && c.mg_scale < 2.0, "mg_scale must lie in (0, 2), got " << c.mg_scale);
     c.max_refine = m.get_int("max_refine", c.max_refine);
     c.graph = m.get_int("graph", 1) != 0;

@@ -2036,7 +2036,11 @@ struct Inst {
     DevPtr<int> act;
     enum State { IDLE, START, SOLVE, DONE } state = IDLE;
     StepStats st;
-    int pcm = PM_SCHUR;            // Krylov form / preconditioner
+    int pcm = PM_SCHUR;            // Krylov form / preconditioner (storage is sized for it)
+    // two-phase in the full form (multigrid pressure stage): the transport stage runs in the
+    // reduced form (the full form's storage holds it)
+    bool split = false;
+    int pcm_now() const { return split && stage == 1 ? PM_SCHUR : pcm; }
     // multigrid hierarchy (two-phase pressure by CG with the V-cycle): coarse levels 1, 2,
     // ... (coefficients 7 x 2 h, vectors V_R, V_Z, V_T 2 h each, Krylov precision) and the
     // coarsest level's dense inverse (the fine level's when it is already small)
@@ -2351,8 +2355,8 @@ struct HeatBatch::Impl {
         d.T = p64(I.T);
         const size_t h = I.h();
         d.h = (int)h;
-        d.pcm = I.pcm;
-        d.ns = I.ns();
+        d.pcm = I.pcm_now();
+        d.ns = d.pcm == PM_SCHUR ? 1 : 2;
         d.vec = cfg.pair_vectors && (I.kind == K_HEAT || (I.kind == K_LINEAR && I.nz == 1)) && I.pcm == PM_SCHUR &&
                 I.nx % 8 == 0;
         d.cgp = cg_capable(I);
@@ -2411,8 +2415,8 @@ struct HeatBatch::Impl {
         return tp;
     }
     TableP table(const std::vector<Inst*>& list) {
-        std::vector<int> key;
-        for (Inst* I : list) key.push_back(I->id);
+        std::vector<int> key;   // the instances and, for split ones, the stage (their Krylov form)
+        for (Inst* I : list) key.push_back(2 * I->id + (I->split && I->stage == 1 ? 1 : 0));
         auto it = tcache.find(key);
         if (it != tcache.end() && it->second->version == version) return it->second;
         if (tcache.size() > 64) tcache.clear();
@@ -3158,6 +3162,7 @@ int HeatBatch::add_twophase(const TwoPhaseSpec& sp, const double* perm, const do
     I->pcm = p_->form_of(K_TWOPHASE);
     alloc_solver(*I, p_->mixed, p_->ks(), p_->cfg.inner_tol_twophase);
     if (p_->cfg.twophase_pressure_cg && p_->cfg.twophase_pressure_mg && I->pcm == PM_ILU) alloc_mg(*I, p_->ks());
+    I->split = p_->cfg.twophase_split_form && I->mg_on;
     const int id = I->id;
     p_->insts[id] = std::move(I);
     ++p_->version;

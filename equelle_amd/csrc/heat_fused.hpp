@@ -66,6 +66,10 @@ struct FusedConfig {
     // 307 over ten steps); 0 < mg_scale < 2 keeps the preconditioner symmetric positive
     // definite
     double mg_scale = 1.8;
+    // two-phase instances whose pressure stage runs multigrid CG on the full system solve
+    // their transport stage by BiCGStab on the red-black reduced system (equal to the full
+    // system with red-black ILU(0) in exact arithmetic, fewer bytes per iteration)
+    bool twophase_split_form = true;
     int max_refine = 20;
     bool graph = true;                 // capture BiCGStab iterations in a hipGraph
     // with graphs: queue the work after a Newton iteration's Krylov solve behind the predicted
