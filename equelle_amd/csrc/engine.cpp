@@ -56,8 +56,15 @@ SymP sym_union(const PatternP& a, const PatternP& b) {
     }
     auto e = std::make_shared<SymEntry>();
     e->out = make_pattern(a->rows, a->cols, std::move(rp), std::move(ci));
-    e->ia = dev_from_host(ia.data(), ia.size());
-    e->ib = dev_from_host(ib.data(), ib.size());
+    // an operand whose pattern is the union itself maps every entry to itself: no map
+    // (null), and its numeric kernels read its values in entry order
+    auto same = [](const std::vector<int>& m) {
+        for (size_t k = 0; k < m.size(); ++k)
+            if (m[k] != (int)k) return false;
+        return true;
+    };
+    if (!same(ia)) e->ia = dev_from_host(ia.data(), ia.size());
+    if (!same(ib)) e->ib = dev_from_host(ib.data(), ib.size());
     e->inputs = {a, b};
     sym_store(key, e);
     return e;
@@ -227,7 +234,7 @@ JacOut jac_lincomb(const SP& a, const DevPtr<double>& A, double alpha, const SP&
         if (pa.get() != pb.get()) {
             SymP e = sym_union(pa, pb);
             o.pat = e->out;
-            ia = value_index(a, e->ia, map_id(*e, 0), o.pat);
+            ia = value_index(a, e->ia, map_id(*e, 0), o.pat);   // null map: identity
             ib = value_index(b, e->ib, map_id(*e, 1), o.pat);
         } else {
             o.pat = pa;
@@ -477,7 +484,8 @@ DevCsr csr_add(const DevCsr& a, double alpha, const DevCsr& b, double beta) {
     SymP e = sym_union(a.pat, b.pat);
     o.pat = e->out;
     o.val = dev_alloc<double>(o.pat->nnz);
-    launch_jlin(o.pat->nnz, nullptr, a.val->p, e->ia->p, nullptr, nullptr, alpha, b.val->p, e->ib->p, nullptr, nullptr,
+    launch_jlin(o.pat->nnz, nullptr, a.val->p, e->ia ? e->ia->p : nullptr, nullptr, nullptr, alpha, b.val->p,
+                e->ib ? e->ib->p : nullptr, nullptr, nullptr,
                 beta, nullptr, o.val->p);
     return o;
 }

@@ -683,15 +683,30 @@ __global__ __launch_bounds__(BS) void k_gather_stencil(const Desc* __restrict__ 
     using K = typename std::conditional<MIXED, float, double>::type;
     double f2[CPT] = {};
     const int nx = d.nx, ny = d.ny, nz = d.nz, nxy = nx * ny, nq = nz > 1 ? 7 : 5;
+    // the rows of a group of BS consecutive cells are one contiguous range of at most 7 BS
+    // entries (stencil pattern): staged through LDS by coalesced loads
+    __shared__ int lci[7 * BS];
+    __shared__ double lval[7 * BS];
     FOR_GROUPS(m, c) {
+        const int g0 = c - (int)threadIdx.x, g1 = min(d.n, g0 + BS);
+        if (g0 >= d.n) break;   // block-uniform
+        const int eb = rp[g0], ee = rp[g1];
+        const bool staged = ee - eb <= 7 * BS;   // always, for a stencil pattern
+        __syncthreads();
+        if (staged)
+            for (int e = eb + (int)threadIdx.x; e < ee; e += BS) {
+                lci[e - eb] = ci[e];
+                lval[e - eb] = val[e];
+            }
+        __syncthreads();
         if (c >= d.n) continue;
         int i, j, k;
         cell_ijk(d, c, i, j, k);
         double co[7] = {0, 0, 0, 0, 0, 0, 0};
         const int e0 = rp[c], e1 = rp[c + 1];
         for (int e = e0; e < e1; ++e) {
-            const int col = ci[e];
-            const double v = val[e];
+            const int col = staged ? lci[e - eb] : ci[e];
+            const double v = staged ? lval[e - eb] : val[e];
             const int q = col == c                              ? 0
                           : (i > 0 && col == c - 1)             ? 1
                           : (i < nx - 1 && col == c + 1)        ? 2
