@@ -145,8 +145,8 @@ int eqhip_opmat_info(void* rt, int which, int* info) {
 int eqhip_opmat_copy(void* rt, int which, int* rp, int* ci, double* v) {
     return guard([&] {
         const DevCsr& m = opmat(R(rt), which);
-        std::memcpy(rp, m.pat->rp.data(), m.pat->rp.size() * sizeof(int));
-        std::memcpy(ci, m.pat->ci.data(), m.pat->ci.size() * sizeof(int));
+        std::memcpy(rp, m.pat->rp().data(), m.pat->rp().size() * sizeof(int));
+        std::memcpy(ci, m.pat->ci().data(), m.pat->ci().size() * sizeof(int));
         std::vector<double> h = dev_to_host(m.val->p, m.pat->nnz);
         std::memcpy(v, h.data(), h.size() * sizeof(double));
     });
@@ -251,8 +251,8 @@ int eqhip_jacobian_copy(void* rt, int h, int* rp, int* ci, double* v) {
     return guard([&] {
         const SP& a = R(rt)->S(h);
         EQ_REQUIRE(a->pat, "Trying to copy empty matrix to host");
-        std::memcpy(rp, a->pat->rp.data(), a->pat->rp.size() * sizeof(int));
-        std::memcpy(ci, a->pat->ci.data(), a->pat->ci.size() * sizeof(int));
+        std::memcpy(rp, a->pat->rp().data(), a->pat->rp().size() * sizeof(int));
+        std::memcpy(ci, a->pat->ci().data(), a->pat->ci().size() * sizeof(int));
         std::vector<double> j = dev_to_host(jac_values(*a), a->pat->nnz);
         std::memcpy(v, j.data(), j.size() * sizeof(double));
     });
@@ -396,8 +396,8 @@ int eqhip_csr_binop(int op, int ar, int ac, const int* arp, const int* aci, cons
         out_info[2] = C.pat ? C.pat->nnz : 0;
         if (!C.pat) return;
         EQ_REQUIRE(C.pat->nnz <= cap, "csr_binop: output capacity too small");
-        std::memcpy(orp, C.pat->rp.data(), C.pat->rp.size() * sizeof(int));
-        std::memcpy(oci, C.pat->ci.data(), C.pat->ci.size() * sizeof(int));
+        std::memcpy(orp, C.pat->rp().data(), C.pat->rp().size() * sizeof(int));
+        std::memcpy(oci, C.pat->ci().data(), C.pat->ci().size() * sizeof(int));
         std::vector<double> v = dev_to_host(C.val->p, C.pat->nnz);
         std::memcpy(ov, v.data(), v.size() * sizeof(double));
     });
@@ -410,12 +410,12 @@ int eqhip_csr_spmv(int rows, int cols, const int* rp, const int* ci, const doubl
             // structural transpose (symbolic on host), values gathered on the device
             const Pattern& p = *A.pat;
             std::vector<int> trp(p.cols + 1, 0), tci(p.nnz), src(p.nnz);
-            for (int c : p.ci) trp[c + 1]++;
+            for (int c : p.ci()) trp[c + 1]++;
             for (int i = 0; i < p.cols; ++i) trp[i + 1] += trp[i];
             std::vector<int> pos(trp.begin(), trp.end() - 1);
             for (int r = 0; r < p.rows; ++r)
-                for (int k = p.rp[r]; k < p.rp[r + 1]; ++k) {
-                    const int q = pos[p.ci[k]]++;
+                for (int k = p.rp()[r]; k < p.rp()[r + 1]; ++k) {
+                    const int q = pos[p.ci()[k]]++;
                     tci[q] = r;
                     src[q] = k;
                 }

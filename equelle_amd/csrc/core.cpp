@@ -115,20 +115,36 @@ Registry& reg() {
     static Registry* r = new Registry();
     return *r;
 }
-uint64_t fnv(uint64_t h, const void* data, size_t bytes) {
-    const unsigned char* p = static_cast<const unsigned char*>(data);
-    for (size_t i = 0; i < bytes; ++i) {
-        h ^= p[i];
-        h *= 1099511628211ull;
-    }
+// The host half of the content hash (pattern_mix over the entries, plus the dimensions);
+// launch_pattern_hash computes the entry sum on the device.
+uint64_t dims_hash(int rows, int cols) {
+    return pattern_mix((uint64_t)(uint32_t)rows) * 3 + pattern_mix(((uint64_t)(uint32_t)cols) | (1ull << 40)) * 5;
+}
+uint64_t entries_hash(const std::vector<int>& rp, const std::vector<int>& ci) {
+    uint64_t h = 0;
+    for (size_t k = 0; k < rp.size(); ++k) h += pattern_mix(((uint64_t)k << 32) | (uint32_t)rp[k]);
+    for (size_t k = 0; k < ci.size(); ++k) h += pattern_mix(((uint64_t)(k | 0x80000000ull) << 32) | (uint32_t)ci[k]);
     return h;
 }
-uint64_t hash_words(uint64_t h, const std::vector<int>& v) {
-    // 64-bit mixing over 32-bit words (fast path for large patterns)
-    for (int x : v) {
-        h ^= (uint32_t)x + 0x9e3779b97f4a7c15ull + (h << 6) + (h >> 2);
+bool dev_equal(const int* a, const int* b, long n) { return n <= 0 || !int_arrays_differ(a, b, n); }
+// A registered pattern with this structure, or null (registry lock held by the caller).
+PatternP find_equal(Registry& r, uint64_t h, int rows, int cols, int nnz, const std::vector<int>* hrp,
+                    const std::vector<int>* hci, const int* drp, const int* dci) {
+    auto range = r.by_hash.equal_range(h);
+    for (auto it = range.first; it != range.second;) {
+        PatternP p = it->second.lock();
+        if (!p) {
+            it = r.by_hash.erase(it);
+            continue;
+        }
+        if (p->rows == rows && p->cols == cols && p->nnz == nnz) {
+            const bool same = hrp ? (p->rp() == *hrp && p->ci() == *hci)
+                                  : (dev_equal(p->d_rp->p, drp, rows + 1L) && dev_equal(p->d_ci->p, dci, nnz));
+            if (same) return p;
+        }
+        ++it;
     }
-    return h;
+    return nullptr;
 }
 }  // namespace
 
@@ -141,50 +157,80 @@ const int* Pattern::row_of() const {
     return d_row->p;
 }
 
-bool Pattern::ascending() const {
+const std::vector<int>& Pattern::rp() const {
+    if (on_host.load(std::memory_order_acquire)) return h_rp;
     std::lock_guard<std::mutex> lk(mu);
-    if (canon < 0) {
-        canon = 1;
-        for (int r = 0; r < rows && canon; ++r)
-            for (int k = rp[r] + 1; k < rp[r + 1]; ++k)
-                if (ci[k] <= ci[k - 1]) {
-                    canon = 0;
-                    break;
-                }
+    if (!on_host.load(std::memory_order_relaxed)) {
+        h_rp = dev_to_host(d_rp->p, (size_t)rows + 1);
+        h_ci = dev_to_host(d_ci->p, (size_t)nnz);
+        on_host.store(true, std::memory_order_release);
     }
-    return canon == 1;
+    return h_rp;
+}
+const std::vector<int>& Pattern::ci() const {
+    rp();
+    return h_ci;
+}
+
+bool Pattern::ascending() const {
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        if (canon >= 0) return canon == 1;
+    }
+    const std::vector<int>& R = rp();
+    const std::vector<int>& Cc = ci();
+    int ok = 1;
+    for (int r = 0; r < rows && ok; ++r)
+        for (int k = R[r] + 1; k < R[r + 1]; ++k)
+            if (Cc[k] <= Cc[k - 1]) {
+                ok = 0;
+                break;
+            }
+    std::lock_guard<std::mutex> lk(mu);
+    canon = ok;
+    return ok == 1;
 }
 
 PatternP make_pattern(int rows, int cols, std::vector<int> rp, std::vector<int> ci) {
-    uint64_t h = 1469598103934665603ull;
-    const int dims[2] = {rows, cols};
-    h = fnv(h, dims, sizeof(dims));
-    h = hash_words(h, rp);
-    h = hash_words(h, ci);
+    const int nnz = rp.empty() ? 0 : rp.back();
+    const uint64_t h = dims_hash(rows, cols) + entries_hash(rp, ci);
     Registry& r = reg();
     {
         std::lock_guard<std::mutex> lk(r.mu);
-        auto range = r.by_hash.equal_range(h);
-        for (auto it = range.first; it != range.second;) {
-            PatternP p = it->second.lock();
-            if (!p) {
-                it = r.by_hash.erase(it);
-                continue;
-            }
-            if (p->rows == rows && p->cols == cols && p->rp == rp && p->ci == ci) return p;
-            ++it;
-        }
+        if (PatternP p = find_equal(r, h, rows, cols, nnz, &rp, &ci, nullptr, nullptr)) return p;
     }
     auto p = std::make_shared<Pattern>();
     p->id = next_id();
     p->hash = h;
     p->rows = rows;
     p->cols = cols;
-    p->nnz = rp.empty() ? 0 : rp.back();
+    p->nnz = nnz;
     p->d_rp = dev_from_host(rp.data(), rp.size());
     p->d_ci = dev_from_host(ci.data(), ci.size() ? ci.size() : 0);
-    p->rp = std::move(rp);
-    p->ci = std::move(ci);
+    p->h_rp = std::move(rp);
+    p->h_ci = std::move(ci);
+    p->on_host.store(true);
+    std::lock_guard<std::mutex> lk(r.mu);
+    r.by_hash.emplace(h, std::weak_ptr<const Pattern>(p));
+    return p;
+}
+
+PatternP make_pattern_dev(int rows, int cols, int nnz, DevPtr<int> rp, DevPtr<int> ci, bool ascending) {
+    const uint64_t h = dims_hash(rows, cols) + pattern_entries_hash(rp->p, rows + 1, ci->p, nnz);
+    Registry& r = reg();
+    {
+        std::lock_guard<std::mutex> lk(r.mu);
+        if (PatternP p = find_equal(r, h, rows, cols, nnz, nullptr, nullptr, rp->p, ci->p)) return p;
+    }
+    auto p = std::make_shared<Pattern>();
+    p->id = next_id();
+    p->hash = h;
+    p->rows = rows;
+    p->cols = cols;
+    p->nnz = nnz;
+    p->d_rp = std::move(rp);
+    p->d_ci = std::move(ci);
+    p->canon = ascending ? 1 : -1;
     std::lock_guard<std::mutex> lk(r.mu);
     r.by_hash.emplace(h, std::weak_ptr<const Pattern>(p));
     return p;

@@ -14,6 +14,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <map>
 #include <memory>
@@ -114,18 +115,36 @@ struct Pattern {
     uint64_t id = 0;
     uint64_t hash = 0;
     int rows = 0, cols = 0, nnz = 0;
-    std::vector<int> rp, ci;          // host copies (symbolic phase)
-    DevPtr<int> d_rp, d_ci;           // device copies (numeric phase, SpMV)
+    DevPtr<int> d_rp, d_ci;           // the structure (symbolic passes run on the device)
     mutable DevPtr<int> d_row;        // row index of every entry (lazy)
     mutable int canon = -1;           // columns strictly ascending in every row (lazy; -1 unknown)
     mutable std::mutex mu;
     const int* row_of() const;        // device row-of-entry map
     bool ascending() const;           // columns strictly ascending in every row
+    // host copies, for host-side consumers (copies to the caller, the CSR Krylov set-up):
+    // kept for patterns made from host arrays, downloaded on first use otherwise
+    const std::vector<int>& rp() const;
+    const std::vector<int>& ci() const;
+    mutable std::vector<int> h_rp, h_ci;
+    mutable std::atomic<bool> on_host{false};
 };
 using PatternP = std::shared_ptr<const Pattern>;
 
+// Content hash of a pattern, computed the same way on the host and on the device: a sum
+// over the entries of pattern_mix((position << 32) | value) (row pointers at positions
+// 0..rows, column indices at positions 2^31 + k), plus the dimensions.
+inline uint64_t pattern_mix(uint64_t x) {
+    x += 0x9e3779b97f4a7c15ull;
+    x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
+    x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
+    return x ^ (x >> 31);
+}
+
 // Returns the canonical (deduplicated) pattern with this structure.
 PatternP make_pattern(int rows, int cols, std::vector<int> rp, std::vector<int> ci);
+// The same from device arrays (rows + 1 row pointers, nnz columns); `ascending`: columns
+// known to be strictly ascending in every row.
+PatternP make_pattern_dev(int rows, int cols, int nnz, DevPtr<int> rp, DevPtr<int> ci, bool ascending);
 PatternP identity_pattern(int n);
 
 // Symbolic results cached by (op, input ids): output pattern plus device index

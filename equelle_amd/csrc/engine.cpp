@@ -26,73 +26,51 @@ void check_size(const SP& a, const SP& b, const char* op) {
     EQ_REQUIRE(a->n == b->n, "CollOfScalar sizes differ in operator" << op << " (" << a->n << " vs " << b->n << ")");
 }
 
-// Union of two patterns with maps from output entries to operand entries.
+// The last entry of a device prefix-sum array (one synchronising read).
+int read_last(const DevPtr<int>& a, long idx) { return dev_to_host(a->p + idx, 1)[0]; }
+
+// Union of two patterns with maps from output entries to operand entries: two passes on
+// the device (entries per row, prefix sum, fill).
 SymP sym_union(const PatternP& a, const PatternP& b) {
     const std::string key = sym_key("union", a->id, b->id);
     if (SymP e = sym_lookup(key)) return e;
     EQ_REQUIRE(a->rows == b->rows && a->cols == b->cols,
                "Jacobians of different sizes (" << a->rows << "x" << a->cols << " vs " << b->rows << "x" << b->cols
                                                 << ")");
-    std::vector<int> rp(a->rows + 1, 0), ci, ia, ib;
-    ci.reserve(std::max(a->nnz, b->nnz));
-    for (int r = 0; r < a->rows; ++r) {
-        int ka = a->rp[r], ea = a->rp[r + 1], kb = b->rp[r], eb = b->rp[r + 1];
-        while (ka < ea || kb < eb) {
-            if (kb >= eb || (ka < ea && a->ci[ka] < b->ci[kb])) {
-                ci.push_back(a->ci[ka]);
-                ia.push_back(ka++);
-                ib.push_back(-1);
-            } else if (ka >= ea || b->ci[kb] < a->ci[ka]) {
-                ci.push_back(b->ci[kb]);
-                ia.push_back(-1);
-                ib.push_back(kb++);
-            } else {
-                ci.push_back(a->ci[ka]);
-                ia.push_back(ka++);
-                ib.push_back(kb++);
-            }
-        }
-        rp[r + 1] = (int)ci.size();
-    }
+    const int rows = a->rows;
+    auto cnt = dev_alloc<int>(rows + 1);
+    launch_union_count(rows, a->d_rp->p, a->d_ci->p, b->d_rp->p, b->d_ci->p, cnt->p);
+    auto rp = dev_alloc<int>(rows + 1);
+    exclusive_scan(cnt->p, rp->p, rows + 1);
+    const int nnz = read_last(rp, rows);
+    auto ci = dev_alloc<int>(std::max(1, nnz)), ia = dev_alloc<int>(std::max(1, nnz)),
+         ib = dev_alloc<int>(std::max(1, nnz));
+    launch_union_fill(rows, a->d_rp->p, a->d_ci->p, b->d_rp->p, b->d_ci->p, rp->p, ci->p, ia->p, ib->p);
     auto e = std::make_shared<SymEntry>();
-    e->out = make_pattern(a->rows, a->cols, std::move(rp), std::move(ci));
-    // an operand whose pattern is the union itself maps every entry to itself: no map
-    // (null), and its numeric kernels read its values in entry order
-    auto same = [](const std::vector<int>& m) {
-        for (size_t k = 0; k < m.size(); ++k)
-            if (m[k] != (int)k) return false;
-        return true;
-    };
-    if (!same(ia)) e->ia = dev_from_host(ia.data(), ia.size());
-    if (!same(ib)) e->ib = dev_from_host(ib.data(), ib.size());
+    e->out = make_pattern_dev(rows, a->cols, nnz, rp, ci, a->ascending() && b->ascending());
+    // an operand whose pattern is the union itself (as many entries) maps every entry to
+    // itself: no map (null), and its numeric kernels read its values in entry order
+    if (nnz != a->nnz) e->ia = ia;
+    if (nnz != b->nnz) e->ib = ib;
     e->inputs = {a, b};
     sym_store(key, e);
     return e;
 }
 
-// Structural product M * A: output pattern + (matrix entry, operand entry) pairs.  With
-// `sign` (M's values, all +1 or -1): the operand entry of each pair with M's sign folded
-// in (sai = ai, or ~ai where M's entry is -1) instead of the two index arrays.
-SymP sym_spgemm(const PatternP& m, const PatternP& a, const std::vector<int8_t>* sign = nullptr,
-                uint64_t sign_id = 0) {
-    const std::string key = sym_key(sign ? "spgemm_s" : "spgemm", m->id, a->id, sign_id);
-    if (SymP e = sym_lookup(key)) return e;
-    EQ_REQUIRE(m->cols == a->rows, "Error in CudaMatrix * CudaMatrix size checking: lhs.cols_ = "
-                                       << m->cols << " rhs.rows_ = " << a->rows);
+// Structural product M * A on the host (rows with more contributions than the device
+// pass takes): output pattern + (matrix entry, operand entry) pairs.
+std::shared_ptr<SymEntry> spgemm_host(const PatternP& m, const PatternP& a, const std::vector<int8_t>* sign) {
+    const std::vector<int>&mrp = m->rp(), &mci = m->ci(), &arp = a->rp(), &aci = a->ci();
     std::vector<int> rp(m->rows + 1, 0), ci, seg(1, 0), mi, ai;
-    ci.reserve((size_t)m->nnz * 3);
-    seg.reserve((size_t)m->nnz * 3 + 1);
-    mi.reserve((size_t)m->nnz * 4);
-    ai.reserve((size_t)m->nnz * 4);
     struct Tri {
         int col, t, e;
     };
     std::vector<Tri> row;
     for (int r = 0; r < m->rows; ++r) {
         row.clear();
-        for (int t = m->rp[r]; t < m->rp[r + 1]; ++t) {
-            const int k = m->ci[t];
-            for (int e = a->rp[k]; e < a->rp[k + 1]; ++e) row.push_back({a->ci[e], t, e});
+        for (int t = mrp[r]; t < mrp[r + 1]; ++t) {
+            const int k = mci[t];
+            for (int e = arp[k]; e < arp[k + 1]; ++e) row.push_back({aci[e], t, e});
         }
         // output columns ascending; contributions to one entry in matrix-entry order
         std::sort(row.begin(), row.end(), [](const Tri& x, const Tri& y) {
@@ -120,28 +98,71 @@ SymP sym_spgemm(const PatternP& m, const PatternP& a, const std::vector<int8_t>*
         e->mi = dev_from_host(mi.data(), mi.size());
         e->ai = dev_from_host(ai.data(), ai.size());
     }
-    e->inputs = {m, a};
-    sym_store(key, e);
     return e;
 }
 
-// Row gather of a pattern: output row i = input row map[i] (-1 => empty row).
-SymP sym_rows(const PatternP& a, const std::vector<int>& map, uint64_t map_id, int out_rows) {
+// Structural product M * A: output pattern + (matrix entry, operand entry) pairs.  With
+// `sign` (M's values, all +1 or -1): the operand entry of each pair with M's sign folded
+// in (sai = ai, or ~ai where M's entry is -1) instead of the two index arrays.  On the
+// device in two passes (distinct columns and contributions per row, prefix sums, fill);
+// rows of more than SPGEMM_CMAX contributions send the product to the host pass.
+constexpr int SPGEMM_CMAX = 256;
+SymP sym_spgemm(const PatternP& m, const PatternP& a, const std::vector<int8_t>* sign = nullptr,
+                uint64_t sign_id = 0) {
+    const std::string key = sym_key(sign ? "spgemm_s" : "spgemm", m->id, a->id, sign_id);
+    if (SymP e = sym_lookup(key)) return e;
+    EQ_REQUIRE(m->cols == a->rows, "Error in CudaMatrix * CudaMatrix size checking: lhs.cols_ = "
+                                       << m->cols << " rhs.rows_ = " << a->rows);
+    const int rows = m->rows;
+    auto ucnt = dev_alloc<int>(rows + 1), ccnt = dev_alloc<int>(rows + 1), over = dev_alloc<int>(1);
+    launch_fill_int(over->p, 0, 1);
+    launch_spgemm_count(rows, m->d_rp->p, m->d_ci->p, a->d_rp->p, a->d_ci->p, SPGEMM_CMAX, ucnt->p, ccnt->p, over->p);
+    std::shared_ptr<SymEntry> out;
+    if (dev_to_host(over->p, 1)[0] != 0) {
+        out = spgemm_host(m, a, sign);
+    } else {
+        auto rp = dev_alloc<int>(rows + 1), cp = dev_alloc<int>(rows + 1);
+        exclusive_scan(ucnt->p, rp->p, rows + 1);
+        exclusive_scan(ccnt->p, cp->p, rows + 1);
+        const int nnz = read_last(rp, rows), nc = read_last(cp, rows);
+        auto e = std::make_shared<SymEntry>();
+        auto ci = dev_alloc<int>(std::max(1, nnz));
+        e->seg = dev_alloc<int>(nnz + 1);
+        launch_fill_int(e->seg->p + nnz, nc, 1);
+        DevPtr<int8_t> dsign;
+        if (sign) {
+            dsign = dev_from_host(sign->data(), sign->size());
+            e->sai = dev_alloc<int>(std::max(1, nc));
+        } else {
+            e->mi = dev_alloc<int>(std::max(1, nc));
+            e->ai = dev_alloc<int>(std::max(1, nc));
+        }
+        launch_spgemm_fill(rows, m->d_rp->p, m->d_ci->p, a->d_rp->p, a->d_ci->p, dsign ? dsign->p : nullptr, rp->p,
+                           cp->p, ci->p, e->seg->p, e->mi ? e->mi->p : nullptr, e->ai ? e->ai->p : nullptr,
+                           e->sai ? e->sai->p : nullptr);
+        e->out = make_pattern_dev(rows, a->cols, nnz, rp, ci, true);
+        out = e;
+    }
+    out->inputs = {m, a};
+    sym_store(key, out);
+    return out;
+}
+
+// Row gather of a pattern: output row i = input row map[i] (map: device array, -1 =>
+// empty row), in two passes on the device.
+SymP sym_rows(const PatternP& a, const int* map, uint64_t map_id, int out_rows) {
     const std::string key = sym_key("rows", a->id, map_id);
     if (SymP e = sym_lookup(key)) return e;
-    std::vector<int> rp(out_rows + 1, 0), ci, src;
-    for (int i = 0; i < out_rows; ++i) {
-        const int r = map[i];
-        if (r >= 0)
-            for (int k = a->rp[r]; k < a->rp[r + 1]; ++k) {
-                ci.push_back(a->ci[k]);
-                src.push_back(k);
-            }
-        rp[i + 1] = (int)ci.size();
-    }
+    auto cnt = dev_alloc<int>(out_rows + 1);
+    launch_rows_count(out_rows, a->d_rp->p, map, cnt->p);
+    auto rp = dev_alloc<int>(out_rows + 1);
+    exclusive_scan(cnt->p, rp->p, out_rows + 1);
+    const int nnz = read_last(rp, out_rows);
+    auto ci = dev_alloc<int>(std::max(1, nnz)), src = dev_alloc<int>(std::max(1, nnz));
+    launch_rows_fill(out_rows, a->d_rp->p, a->d_ci->p, map, rp->p, ci->p, src->p);
     auto e = std::make_shared<SymEntry>();
-    e->out = make_pattern(out_rows, a->cols, std::move(rp), std::move(ci));
-    e->ia = dev_from_host(src.data(), src.size());
+    e->out = make_pattern_dev(out_rows, a->cols, nnz, rp, ci, a->ascending());
+    e->ia = src;
     e->inputs = {a};
     sym_store(key, e);
     return e;
@@ -599,7 +620,7 @@ SP s_on(const SP& a, const IP& from, const IP& to) {
     auto v = dev_alloc<double>(m);
     launch_gather(a->val->p, pm->d_pos->p, v->p, m);
     if (!a->ad || !a->pat) return make_scalar(m, v, a->ad, nullptr, nullptr);
-    SymP e = sym_rows(a->pat, pm->pos, pm->id, m);
+    SymP e = sym_rows(a->pat, pm->d_pos->p, pm->id, m);
     return make_ad(m, v, gather_rows(a, e, pm->d_pos->p, m));
 }
 
@@ -613,7 +634,7 @@ SP s_extend(const SP& a, const IP& from, const IP& to) {
     auto v = dev_alloc<double>(m);
     launch_gather(a->val->p, pm->d_inv->p, v->p, m);
     if (!a->ad || !a->pat) return make_scalar(m, v, a->ad, nullptr, nullptr);
-    SymP e = sym_rows(a->pat, pm->inv, pm->id ^ 0x8000000000000000ull, m);
+    SymP e = sym_rows(a->pat, pm->d_inv->p, pm->id ^ 0x8000000000000000ull, m);
     return make_ad(m, v, gather_rows(a, e, pm->d_inv->p, m));
 }
 

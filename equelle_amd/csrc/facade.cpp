@@ -112,8 +112,8 @@ hostMat CudaMatrix::toHost() const {
     h.cols = d_->cols;
     h.nnz = nnz();
     if (d_->m.pat) {
-        h.rowPtr = d_->m.pat->rp;
-        h.colInd = d_->m.pat->ci;
+        h.rowPtr = d_->m.pat->rp();
+        h.colInd = d_->m.pat->ci();
         h.vals = eqhip::dev_to_host(d_->m.val->p, h.nnz);
     }
     return h;
@@ -123,12 +123,12 @@ CudaMatrix CudaMatrix::transpose() const {
     // structural transpose on the host (symbolic), values permuted on the device
     const eqhip::Pattern& p = *d_->m.pat;
     std::vector<int> rp(p.cols + 1, 0), ci(p.nnz), src(p.nnz);
-    for (int c : p.ci) rp[c + 1]++;
+    for (int c : p.ci()) rp[c + 1]++;
     for (int i = 0; i < p.cols; ++i) rp[i + 1] += rp[i];
     std::vector<int> pos(rp.begin(), rp.end() - 1);
     for (int r = 0; r < p.rows; ++r)
-        for (int k = p.rp[r]; k < p.rp[r + 1]; ++k) {
-            const int q = pos[p.ci[k]]++;
+        for (int k = p.rp()[r]; k < p.rp()[r + 1]; ++k) {
+            const int q = pos[p.ci()[k]]++;
             ci[q] = r;
             src[q] = k;
         }

@@ -1,8 +1,10 @@
 // HIP kernels (gfx950, wave64) for the AD engine, grid operators and reductions.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <hipcub/hipcub.hpp>
 
 #include "core.hpp"
 #include "kernels.hpp"
@@ -37,6 +39,174 @@ __global__ void k_row_of(const int* __restrict__ rp, int rows, int* __restrict__
     long r = gid();
     if (r >= rows) return;
     for (int k = rp[r]; k < rp[r + 1]; ++k) row[k] = (int)r;
+}
+
+// ---------------------------------------------------------------------------
+// Symbolic passes (thread per row; rows hold ascending columns).
+__device__ inline unsigned long long pmix(unsigned long long x) {   // pattern_mix
+    x += 0x9e3779b97f4a7c15ull;
+    x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
+    x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
+    return x ^ (x >> 31);
+}
+__global__ void k_pattern_hash(const int* __restrict__ rp, int nrp, const int* __restrict__ ci, int nnz,
+                               unsigned long long* __restrict__ acc) {
+    unsigned long long h = 0;
+    const long n = (long)nrp + nnz, stride = (long)gridDim.x * blockDim.x;
+    for (long k = gid(); k < n; k += stride)
+        h += k < nrp ? pmix(((unsigned long long)k << 32) | (unsigned)rp[k])
+                     : pmix(((unsigned long long)((k - nrp) | 0x80000000ull) << 32) | (unsigned)ci[k - nrp]);
+    for (int o = 32; o > 0; o >>= 1) h += __shfl_xor(h, o, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(acc, h);   // integer sum: order-independent
+}
+__global__ void k_int_differ(const int* __restrict__ a, const int* __restrict__ b, long n, int* __restrict__ flag) {
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long k = gid(); k < n; k += stride)
+        if (a[k] != b[k]) *flag = 1;
+}
+__global__ void k_set_int(int* p, int v) { *p = v; }
+__global__ void k_set_u64(unsigned long long* p) { *p = 0; }
+
+__global__ void k_union_count(int rows, const int* __restrict__ arp, const int* __restrict__ aci,
+                              const int* __restrict__ brp, const int* __restrict__ bci, int* __restrict__ cnt) {
+    const long r = gid();
+    if (r > rows) return;
+    if (r == rows) {
+        cnt[r] = 0;
+        return;
+    }
+    int ka = arp[r], kb = brp[r], c = 0;
+    const int ea = arp[r + 1], eb = brp[r + 1];
+    while (ka < ea || kb < eb) {
+        if (kb >= eb || (ka < ea && aci[ka] < bci[kb])) ++ka;
+        else if (ka >= ea || bci[kb] < aci[ka]) ++kb;
+        else ++ka, ++kb;
+        ++c;
+    }
+    cnt[r] = c;
+}
+__global__ void k_union_fill(int rows, const int* __restrict__ arp, const int* __restrict__ aci,
+                             const int* __restrict__ brp, const int* __restrict__ bci, const int* __restrict__ orp,
+                             int* __restrict__ oci, int* __restrict__ ia, int* __restrict__ ib) {
+    const long r = gid();
+    if (r >= rows) return;
+    int ka = arp[r], kb = brp[r], o = orp[r];
+    const int ea = arp[r + 1], eb = brp[r + 1];
+    while (ka < ea || kb < eb) {
+        if (kb >= eb || (ka < ea && aci[ka] < bci[kb])) {
+            oci[o] = aci[ka];
+            ia[o] = ka++;
+            ib[o] = -1;
+        } else if (ka >= ea || bci[kb] < aci[ka]) {
+            oci[o] = bci[kb];
+            ia[o] = -1;
+            ib[o] = kb++;
+        } else {
+            oci[o] = aci[ka];
+            ia[o] = ka++;
+            ib[o] = kb++;
+        }
+        ++o;
+    }
+}
+__global__ void k_rows_count(int out_rows, const int* __restrict__ arp, const int* __restrict__ map,
+                             int* __restrict__ cnt) {
+    const long i = gid();
+    if (i > out_rows) return;
+    const int r = i < out_rows ? map[i] : -1;
+    cnt[i] = r >= 0 ? arp[r + 1] - arp[r] : 0;
+}
+__global__ void k_rows_fill(int out_rows, const int* __restrict__ arp, const int* __restrict__ aci,
+                            const int* __restrict__ map, const int* __restrict__ orp, int* __restrict__ oci,
+                            int* __restrict__ src) {
+    const long i = gid();
+    if (i >= out_rows) return;
+    const int r = map[i];
+    if (r < 0) return;
+    int o = orp[i];
+    for (int k = arp[r]; k < arp[r + 1]; ++k, ++o) {
+        oci[o] = aci[k];
+        src[o] = k;
+    }
+}
+// Structural product, row r: the contributions are the pairs (t, e) with t an entry of M's
+// row r and e an entry of A's row mci[t], in (t, e) order; a column is new at its first
+// contribution (A's rows have distinct columns, so earlier contributions of the same t
+// never repeat it).
+__global__ void k_spgemm_count(int rows, const int* __restrict__ mrp, const int* __restrict__ mci,
+                               const int* __restrict__ arp, const int* __restrict__ aci, int cmax,
+                               int* __restrict__ ucnt, int* __restrict__ ccnt, int* __restrict__ over) {
+    const long r = gid();
+    if (r > rows) return;
+    if (r == rows) {
+        ucnt[r] = ccnt[r] = 0;
+        return;
+    }
+    int nc = 0;
+    for (int t = mrp[r]; t < mrp[r + 1]; ++t) nc += arp[mci[t] + 1] - arp[mci[t]];
+    ccnt[r] = nc;
+    if (nc > cmax) {
+        *over = 1;
+        ucnt[r] = 0;
+        return;
+    }
+    int u = 0;
+    for (int t = mrp[r]; t < mrp[r + 1]; ++t) {
+        const int k = mci[t];
+        for (int e = arp[k]; e < arp[k + 1]; ++e) {
+            const int col = aci[e];
+            bool seen = false;
+            for (int t2 = mrp[r]; t2 < t && !seen; ++t2) {
+                const int k2 = mci[t2];
+                for (int e2 = arp[k2]; e2 < arp[k2 + 1]; ++e2)
+                    if (aci[e2] == col) {
+                        seen = true;
+                        break;
+                    }
+            }
+            u += seen ? 0 : 1;
+        }
+    }
+    ucnt[r] = u;
+}
+__global__ void k_spgemm_fill(int rows, const int* __restrict__ mrp, const int* __restrict__ mci,
+                              const int* __restrict__ arp, const int* __restrict__ aci,
+                              const int8_t* __restrict__ msign, const int* __restrict__ orp,
+                              const int* __restrict__ cp, int* __restrict__ oci, int* __restrict__ seg,
+                              int* __restrict__ mi, int* __restrict__ ai, int* __restrict__ sai) {
+    const long r = gid();
+    if (r >= rows) return;
+    int o = orp[r], c = cp[r];
+    long last = -1;   // columns emitted so far are <= last
+    while (true) {
+        int mn = 0x7fffffff;
+        for (int t = mrp[r]; t < mrp[r + 1]; ++t) {
+            const int k = mci[t];
+            for (int e = arp[k]; e < arp[k + 1]; ++e) {
+                const int col = aci[e];
+                if (col > last && col < mn) mn = col;
+            }
+        }
+        if (mn == 0x7fffffff) break;
+        oci[o] = mn;
+        seg[o] = c;
+        for (int t = mrp[r]; t < mrp[r + 1]; ++t) {   // the column's contributions, in t order
+            const int k = mci[t];
+            for (int e = arp[k]; e < arp[k + 1]; ++e)
+                if (aci[e] == mn) {
+                    if (sai) {
+                        sai[c] = msign[t] < 0 ? ~e : e;
+                    } else {
+                        mi[c] = t;
+                        ai[c] = e;
+                    }
+                    ++c;
+                    break;
+                }
+        }
+        ++o;
+        last = mn;
+    }
 }
 
 __global__ void k_fill(double* out, double v, long n) {
@@ -461,6 +631,52 @@ __global__ void k_compact_write(const uint8_t* __restrict__ f, int n, int want, 
 
 // ===========================================================================
 void launch_row_of(const int* rp, int rows, int* row) { EQ_LAUNCH(k_row_of, blocks_for(rows), rp, rows, row); }
+
+uint64_t pattern_entries_hash(const int* rp, int nrp, const int* ci, int nnz) {
+    auto acc = dev_alloc<unsigned long long>(1);
+    EQ_LAUNCH(k_set_u64, 1, acc->p);
+    const long n = (long)nrp + nnz;
+    EQ_LAUNCH(k_pattern_hash, std::min<long>(blocks_for(n), 4096), rp, nrp, ci, nnz, acc->p);
+    return (uint64_t)dev_to_host(acc->p, 1)[0];
+}
+bool int_arrays_differ(const int* a, const int* b, long n) {
+    if (n <= 0) return false;
+    auto flag = dev_alloc<int>(1);
+    EQ_LAUNCH(k_set_int, 1, flag->p, 0);
+    EQ_LAUNCH(k_int_differ, std::min<long>(blocks_for(n), 4096), a, b, n, flag->p);
+    return dev_to_host(flag->p, 1)[0] != 0;
+}
+void exclusive_scan(const int* in, int* out, int n) {
+    if (n <= 0) return;
+    size_t bytes = 0;
+    EQ_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, n, S()));
+    auto tmp = dev_alloc<char>(std::max<size_t>(bytes, 1));
+    EQ_HIP(hipcub::DeviceScan::ExclusiveSum(tmp->p, bytes, in, out, n, S()));
+}
+void launch_union_count(int rows, const int* arp, const int* aci, const int* brp, const int* bci, int* cnt) {
+    EQ_LAUNCH(k_union_count, blocks_for(rows + 1L), rows, arp, aci, brp, bci, cnt);
+}
+void launch_union_fill(int rows, const int* arp, const int* aci, const int* brp, const int* bci, const int* orp,
+                       int* oci, int* ia, int* ib) {
+    if (rows > 0) EQ_LAUNCH(k_union_fill, blocks_for(rows), rows, arp, aci, brp, bci, orp, oci, ia, ib);
+}
+void launch_rows_count(int out_rows, const int* arp, const int* map, int* cnt) {
+    EQ_LAUNCH(k_rows_count, blocks_for(out_rows + 1L), out_rows, arp, map, cnt);
+}
+void launch_rows_fill(int out_rows, const int* arp, const int* aci, const int* map, const int* orp, int* oci,
+                      int* src) {
+    if (out_rows > 0) EQ_LAUNCH(k_rows_fill, blocks_for(out_rows), out_rows, arp, aci, map, orp, oci, src);
+}
+void launch_spgemm_count(int rows, const int* mrp, const int* mci, const int* arp, const int* aci, int cmax,
+                         int* ucnt, int* ccnt, int* over) {
+    EQ_LAUNCH(k_spgemm_count, blocks_for(rows + 1L), rows, mrp, mci, arp, aci, cmax, ucnt, ccnt, over);
+}
+void launch_spgemm_fill(int rows, const int* mrp, const int* mci, const int* arp, const int* aci,
+                        const int8_t* msign, const int* orp, const int* cp, int* oci, int* seg, int* mi, int* ai,
+                        int* sai) {
+    if (rows > 0)
+        EQ_LAUNCH(k_spgemm_fill, blocks_for(rows), rows, mrp, mci, arp, aci, msign, orp, cp, oci, seg, mi, ai, sai);
+}
 void launch_fill(double* out, double v, long n) {
     if (n > 0) EQ_LAUNCH(k_fill, blocks_for(n), out, v, n);
 }

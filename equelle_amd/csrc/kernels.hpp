@@ -50,6 +50,35 @@ void launch_spgemm_signed(long nnz, const int* seg, const int* sai, const double
 void launch_jscale(long nnz, const int* row, double s, const double* R, const int* map, const double* v,
                    double* out);
 
+// ---- symbolic passes on the device (CSR patterns with ascending columns per row) ----
+// Sum over the entries of the content hash of a pattern (rows + 1 row pointers at
+// positions 0.., nnz column indices at positions 2^31 + k; see pattern_mix).
+uint64_t pattern_entries_hash(const int* rp, int nrp, const int* ci, int nnz);
+// Whether two device int arrays of n entries differ anywhere.
+bool int_arrays_differ(const int* a, const int* b, long n);
+// out[0..n) = exclusive prefix sums of in[0..n) (out[0] = 0).
+void exclusive_scan(const int* in, int* out, int n);
+// Union of two patterns: entries per row (cnt[rows] = 0), then the columns and, per output
+// entry, the entry of each operand (-1 where absent).
+void launch_union_count(int rows, const int* arp, const int* aci, const int* brp, const int* bci, int* cnt);
+void launch_union_fill(int rows, const int* arp, const int* aci, const int* brp, const int* bci, const int* orp,
+                       int* oci, int* ia, int* ib);
+// Row gather: output row i = input row map[i] (-1: empty): entries per row, then the
+// columns and each entry's source entry.
+void launch_rows_count(int out_rows, const int* arp, const int* map, int* cnt);
+void launch_rows_fill(int out_rows, const int* arp, const int* aci, const int* map, const int* orp, int* oci,
+                      int* src);
+// Structural product M * A: per row the distinct output columns (ucnt) and the
+// contributions (ccnt; ucnt[rows] = ccnt[rows] = 0); *over = 1 when a row has more than
+// cmax contributions.  The fill writes the columns ascending and, per output entry, its
+// contributions in matrix-entry order: (mi, ai) pairs, or sai (ai, or ~ai where msign < 0)
+// when msign is given.
+void launch_spgemm_count(int rows, const int* mrp, const int* mci, const int* arp, const int* aci, int cmax,
+                         int* ucnt, int* ccnt, int* over);
+void launch_spgemm_fill(int rows, const int* mrp, const int* mci, const int* arp, const int* aci,
+                        const int8_t* msign, const int* orp, const int* cp, int* oci, int* seg, int* mi, int* ai,
+                        int* sai);
+
 // y = A x, one thread per row, entries summed in column order.
 void launch_csr_spmv(int rows, const int* rp, const int* ci, const double* v, const double* x, double* y);
 

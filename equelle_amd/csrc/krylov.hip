@@ -76,7 +76,7 @@ LayoutP get_layout(const PatternP& p) {
     std::vector<int> sp(L->nslices + 1, 0);
     for (int s = 0; s < L->nslices; ++s) {
         int w = 0;
-        for (int r = s * W; r < std::min(n, (s + 1) * W); ++r) w = std::max(w, p->rp[r + 1] - p->rp[r]);
+        for (int r = s * W; r < std::min(n, (s + 1) * W); ++r) w = std::max(w, p->rp()[r + 1] - p->rp()[r]);
         sp[s + 1] = sp[s] + w * W;
     }
     L->nnzp = sp[L->nslices];
@@ -87,9 +87,9 @@ LayoutP get_layout(const PatternP& p) {
             for (int lane = 0; lane < W; ++lane) {
                 const int r = s * W + lane;
                 const long k = sp[s] + (long)j * W + lane;
-                if (r < n && j < p->rp[r + 1] - p->rp[r]) {
-                    col[k] = p->ci[p->rp[r] + j];
-                    src[k] = p->rp[r] + j;
+                if (r < n && j < p->rp()[r + 1] - p->rp()[r]) {
+                    col[k] = p->ci()[p->rp()[r] + j];
+                    src[k] = p->rp()[r] + j;
                 } else {
                     col[k] = r < n ? r : 0;
                     src[k] = -1;
@@ -98,13 +98,13 @@ LayoutP get_layout(const PatternP& p) {
     }
     std::vector<int> diag(n, -1);
     for (int r = 0; r < n; ++r)
-        for (int k = p->rp[r]; k < p->rp[r + 1]; ++k)
-            if (p->ci[k] == r) diag[r] = k;
+        for (int k = p->rp()[r]; k < p->rp()[r + 1]; ++k)
+            if (p->ci()[k] == r) diag[r] = k;
     // greedy colouring of the symmetrised graph in natural order
     std::vector<std::vector<int>> adjT(n);
     for (int r = 0; r < n; ++r)
-        for (int k = p->rp[r]; k < p->rp[r + 1]; ++k)
-            if (p->ci[k] != r) adjT[p->ci[k]].push_back(r);
+        for (int k = p->rp()[r]; k < p->rp()[r + 1]; ++k)
+            if (p->ci()[k] != r) adjT[p->ci()[k]].push_back(r);
     std::vector<int> color(n, -1);
     std::vector<int> mark(64, -1);
     int ncol = 0;
@@ -115,8 +115,8 @@ LayoutP get_layout(const PatternP& p) {
                 mark[color[j]] = r;
             }
         };
-        for (int k = p->rp[r]; k < p->rp[r + 1]; ++k)
-            if (p->ci[k] != r) use(p->ci[k]);
+        for (int k = p->rp()[r]; k < p->rp()[r + 1]; ++k)
+            if (p->ci()[k] != r) use(p->ci()[k]);
         for (int j : adjT[r]) use(j);
         int c = 0;
         while (c < (int)mark.size() && mark[c] == r) ++c;
@@ -133,8 +133,8 @@ LayoutP get_layout(const PatternP& p) {
     // per CSR entry: 0 L, 1 diagonal, 2 U, 3 same colour (dropped by ILU(0) here)
     std::vector<uint8_t> kind(p->nnz);
     for (int r = 0; r < n; ++r) {
-        for (int k = p->rp[r]; k < p->rp[r + 1]; ++k) {
-            const int c = p->ci[k];
+        for (int k = p->rp()[r]; k < p->rp()[r + 1]; ++k) {
+            const int c = p->ci()[k];
             kind[k] = c == r ? 1 : color[c] < color[r] ? 0 : color[c] > color[r] ? 2 : 3;
         }
     }
@@ -154,7 +154,7 @@ LayoutP get_layout(const PatternP& p) {
                     int w = 0;
                     for (int t = s0; t < s1; ++t) {
                         int m = 0;
-                        for (int k = p->rp[rows[t]]; k < p->rp[rows[t] + 1]; ++k) m += kind[k] == want;
+                        for (int k = p->rp()[rows[t]]; k < p->rp()[rows[t] + 1]; ++k) m += kind[k] == want;
                         w = std::max(w, m);
                     }
                     const size_t base = cols.size();
@@ -162,9 +162,9 @@ LayoutP get_layout(const PatternP& p) {
                     srcs.resize(base + (size_t)w * W, -1);
                     for (int t = s0; t < s1; ++t) {
                         int j = 0;
-                        for (int k = p->rp[rows[t]]; k < p->rp[rows[t] + 1]; ++k)
+                        for (int k = p->rp()[rows[t]]; k < p->rp()[rows[t] + 1]; ++k)
                             if (kind[k] == want) {
-                                const int cc = p->ci[k];
+                                const int cc = p->ci()[k];
                                 // an L column of colour 0 reads the right-hand side (-2 - col)
                                 cols[base + (size_t)j * W + (t - s0)] = part == 0 && color[cc] == 0 ? -2 - cc : cc;
                                 srcs[base + (size_t)j * W + (t - s0)] = k;
@@ -194,19 +194,19 @@ LayoutP get_layout(const PatternP& p) {
         std::vector<int> fsp(1, 0), fq, fdk, fpp(1, 0), fg, ff;
         for (int t = 0; t < n; ++t) {
             const int i = rows[t];
-            const int b = p->rp[i], e = p->rp[i + 1];
+            const int b = p->rp()[i], e = p->rp()[i + 1];
             for (int cc = 0; cc < color[i]; ++cc)
                 for (int q = b; q < e; ++q) {
                     if (kind[q] != 0) continue;
-                    const int k = p->ci[q];
+                    const int k = p->ci()[q];
                     if (color[k] != cc) continue;
                     fq.push_back(q);
                     fdk.push_back(diag[k]);
-                    for (int f = p->rp[k]; f < p->rp[k + 1]; ++f) {
+                    for (int f = p->rp()[k]; f < p->rp()[k + 1]; ++f) {
                         if (kind[f] != 2) continue;
-                        const int j = p->ci[f];
+                        const int j = p->ci()[f];
                         for (int g = b; g < e; ++g)
-                            if (p->ci[g] == j) {
+                            if (p->ci()[g] == j) {
                                 fg.push_back(g);
                                 ff.push_back(f);
                                 break;

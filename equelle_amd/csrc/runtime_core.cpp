@@ -364,8 +364,8 @@ bool RuntimeCore::stencil_pattern(const PatternP& P) const {
     for (int c = 0; ok && c < g.nc; ++c) {
         const int i = c % nx, j = (c / nx) % ny, k = c / nxy;
         bool diag = false;
-        for (int e = P->rp[c]; ok && e < P->rp[c + 1]; ++e) {
-            const int col = P->ci[e];
+        for (int e = P->rp()[c]; ok && e < P->rp()[c + 1]; ++e) {
+            const int col = P->ci()[e];
             diag |= col == c;
             ok = col == c || (i > 0 && col == c - 1) || (i < nx - 1 && col == c + 1) || (j > 0 && col == c - nx) ||
                  (j < ny - 1 && col == c + nx) || (nz > 1 && k > 0 && col == c - nxy) ||
