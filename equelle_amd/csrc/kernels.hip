@@ -408,13 +408,21 @@ __global__ void k_gradient(const double* __restrict__ u, const int* __restrict__
     if (i < nif) out[i] = u[c2[i]] - u[c1[i]];
 }
 
+// The block's cells hold one contiguous range of the signed face lists (2 dim faces per
+// cell, at most 6 BS): staged through LDS by coalesced loads, then each cell sums its own.
 __global__ void k_divergence(const double* __restrict__ flux, const int* __restrict__ facepos,
                              const int* __restrict__ sfaces, double* __restrict__ out, int nc) {
-    long c = gid();
+    __shared__ int lf[6 * BS];
+    const long c0 = (long)blockIdx.x * BS, c = c0 + threadIdx.x;
+    const int b = facepos[c0], e = facepos[min(c0 + BS, (long)nc)];
+    const bool staged = e - b <= 6 * BS;   // block-uniform; always for a Cartesian grid
+    if (staged)
+        for (int k = b + (int)threadIdx.x; k < e; k += BS) lf[k - b] = sfaces[k];
+    __syncthreads();
     if (c >= nc) return;
     double s = 0.0;
     for (int k = facepos[c]; k < facepos[c + 1]; ++k) {
-        const int f = sfaces[k];
+        const int f = staged ? lf[k - b] : sfaces[k];
         s += f >= 0 ? flux[f] : -flux[~f];
     }
     out[c] = s;
