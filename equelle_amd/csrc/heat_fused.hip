@@ -2450,6 +2450,7 @@ struct HeatBatch::Impl {
     // and 8 vector-update rounds) is kept for the table.  Results do not depend on the shape
     // (shape-independent reductions), so tuning changes time only.
     std::map<std::tuple<int, int, int>, Shape> tuned;
+    std::vector<std::pair<long, Shape>> measured;   // memberships tuned by measurement: total cells, shape
     Shape last_shape;
     static int ilog2(long v) {
         int r = 0;
@@ -2468,8 +2469,20 @@ struct HeatBatch::Impl {
         const auto sig = std::make_tuple(ilog2(tot), ilog2((long)list.size()), ilog2(mx));
         auto it = tuned.find(sig);
         if (it != tuned.end()) return last_shape = it->second;
+        // a membership of less than half the cells of one already measured (the instances
+        // still iterating late in a step, which come and go with their convergence) takes
+        // the shape of the smallest such measured membership instead of a tuning run of its
+        // own; results do not depend on the shape
+        const std::pair<long, Shape>* ref = nullptr;
+        for (const auto& m : measured)
+            if (m.first >= 2 * tot && (!ref || m.first < ref->first)) ref = &m;
         Shape best;
-        if (cfg.tune) best = tune_shape(list);
+        if (ref) {
+            best = ref->second;
+        } else {
+            if (cfg.tune) best = tune_shape(list);
+            measured.emplace_back(tot, best);
+        }
         tuned[sig] = best;
         return last_shape = best;
     }
@@ -3384,7 +3397,7 @@ LaunchShape HeatBatch::launch_shape() const {
     LaunchShape r;
     r.cpt = p_->last_shape.cpt[C_SPMV];
     r.xcd = p_->last_shape.xcd;
-    r.signatures = (int)p_->tuned.size();
+    r.signatures = (int)p_->measured.size();   // memberships tuned by measurement
     for (int c = 0; c < NCLS; ++c) r.cpt_class[c] = p_->last_shape.cpt[c];
     return r;
 }
