@@ -2450,7 +2450,12 @@ struct HeatBatch::Impl {
     // and 8 vector-update rounds) is kept for the table.  Results do not depend on the shape
     // (shape-independent reductions), so tuning changes time only.
     std::map<std::tuple<int, int, int>, Shape> tuned;
-    std::vector<std::pair<long, Shape>> measured;   // memberships tuned by measurement: total cells, shape
+    struct Measured {   // a membership tuned by measurement
+        long cells;       // its total cells
+        int lmax;         // log2 of its largest instance
+        Shape shape;
+    };
+    std::vector<Measured> measured;
     Shape last_shape;
     static int ilog2(long v) {
         int r = 0;
@@ -2469,19 +2474,20 @@ struct HeatBatch::Impl {
         const auto sig = std::make_tuple(ilog2(tot), ilog2((long)list.size()), ilog2(mx));
         auto it = tuned.find(sig);
         if (it != tuned.end()) return last_shape = it->second;
-        // a membership of less than half the cells of one already measured (the instances
-        // still iterating late in a step, which come and go with their convergence) takes
-        // the shape of the smallest such measured membership instead of a tuning run of its
-        // own; results do not depend on the shape
-        const std::pair<long, Shape>* ref = nullptr;
-        for (const auto& m : measured)
-            if (m.first >= 2 * tot && (!ref || m.first < ref->first)) ref = &m;
+        // a membership no larger (in cells) than one already measured whose largest
+        // instance is of the same size class (the instances still iterating late in a step,
+        // which come and go with their convergence) takes the shape of the smallest such
+        // measured membership instead of a tuning run of its own; results do not depend on
+        // the shape
+        const Measured* ref = nullptr;
+        for (const Measured& m : measured)
+            if (m.cells >= tot && m.lmax == ilog2(mx) && (!ref || m.cells < ref->cells)) ref = &m;
         Shape best;
         if (ref) {
-            best = ref->second;
+            best = ref->shape;
         } else {
             if (cfg.tune) best = tune_shape(list);
-            measured.emplace_back(tot, best);
+            measured.push_back({tot, ilog2(mx), best});
         }
         tuned[sig] = best;
         return last_shape = best;
