@@ -125,7 +125,7 @@ def run_fused(lib, precision, u0, nx, ny, dt, steps, warmup, precond="ilu0", gra
     return dict(seconds=secs, newton=newton, krylov=krylov, refine=refine, kern=kern, u=u)
 
 
-def run_generic(lib, u0, nx, ny, dt, steps, warmup, precision="fp64", extra=""):
+def run_generic(lib, u0, nx, ny, dt, steps, warmup, precision="fp64", extra="", eval_reps=0):
     params = heat_params(nx, ny, NONLINEAR_HEAT["k"], precision=precision, precond="ilu0",
                          rel_res_tol=NONLINEAR_HEAT["rel_res_tol"], solver_tol=1e-6, extra=extra)
     sim = C.c_void_p()
@@ -152,9 +152,27 @@ def run_generic(lib, u0, nx, ny, dt, steps, warmup, precision="fp64", extra=""):
         secs = time.perf_counter() - t0
         u = np.zeros(nx * ny)
         lib.heat_state(sim, dptr(u))
+        assembly = None
+        if eval_reps:
+            ms, info = C.c_double(), np.zeros(2, dtype=np.int32)
+            lib.heat_eval_time(sim, dt, eval_reps, C.byref(ms), iptr(info))
+            assembly = assembly_roofline(nx, ny, int(info[1]), ms.value, eval_reps)
     finally:
         lib.heat_destroy(sim)
-    return dict(seconds=secs, newton=newton, krylov=krylov, u=u, first_step_s=first)
+    return dict(seconds=secs, newton=newton, krylov=krylov, u=u, first_step_s=first, assembly=assembly)
+
+
+def assembly_roofline(nx, ny, nnz, ms, reps):
+    """One residual + Jacobian evaluation of the drop-in program against the assembly floor:
+    8 nnz (Jacobian values written) + 8 n (F written) + 16 n (u, u0 read) + 8 nif (the
+    per-face transmissibility read) bytes."""
+    n, nif = nx * ny, (nx - 1) * ny + nx * (ny - 1)
+    floor = 8 * nnz + 8 * n + 16 * n + 8 * nif
+    gbs = floor / (ms * 1e-3) / 1e9
+    return {"ms_per_eval": ms, "evaluations": reps, "jacobian_nnz": nnz, "algorithmic_bytes": floor,
+            "bytes_formula": "8 nnz + 24 n + 8 nif", "GBs": gbs, "frac": gbs / HBM_PEAK_GBS,
+            "measured": "host clock over the evaluations (kernels and host work of the AD engine), "
+                        "after one untimed evaluation"}
 
 
 def bench_generic(lib, args):
@@ -167,15 +185,19 @@ def bench_generic(lib, args):
     u0 = gaussian_bumps(nx, ny, seed=2048)
     legs = {}
     ref = None
+    assembly = None
     for name, prec, extra in (("fp64_stencil", "fp64", ""), ("mixed_stencil", "mixed", "inner_tol=1e-6"),
                               ("fp64_csr", "fp64", "stencil_solver=0"),
                               ("mixed_csr", "mixed", "stencil_solver=0 inner_tol=1e-6")):
         if args.quick and name.endswith("_csr"):
             continue
         log(f"[bench] generic drop-in path {name}")
-        g = run_generic(lib, u0, nx, ny, NONLINEAR_HEAT["dt"], args.steps, args.warmup, prec, extra)
+        g = run_generic(lib, u0, nx, ny, NONLINEAR_HEAT["dt"], args.steps, args.warmup, prec, extra,
+                        eval_reps=10 if name == "fp64_stencil" else 0)
         if ref is None:
             ref = g["u"]
+        if g["assembly"] is not None:
+            assembly = g["assembly"]
         legs[name] = {"value": n * g["newton"] / g["seconds"], "ms_per_step": 1e3 * g["seconds"] / args.steps,
                       "newton_iterations": g["newton"], "krylov_iterations": g["krylov"],
                       "setup_and_first_step_s": g["first_step_s"],
@@ -186,7 +208,7 @@ def bench_generic(lib, args):
            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
            "config": {"workload": f"nonlinear heat {nx}x{ny} through the drop-in EquelleRuntimeCUDA AD engine, "
                                   "BiCGStab + ILU(0), Newton to 1e-8 relative residual", "cells": n},
-           "legs": legs}
+           "legs": legs, "assembly_roofline": assembly}
     print(json.dumps(out), flush=True)
 
 
@@ -864,9 +886,10 @@ def main():
         out["mixed_vs_fp64_rel_error"] = float(np.linalg.norm(head["u"] - p["u"]) / np.linalg.norm(p["u"]))
         log("[bench] generic drop-in path (EquelleRuntimeCUDA AD engine, FP64)")
         gsteps = max(2, args.steps // 4)
-        g = run_generic(lib, u0, nx, ny, dt, gsteps, 1)
+        g = run_generic(lib, u0, nx, ny, dt, gsteps, 1, eval_reps=10)
         out["generic_fp64"] = {"value": n * g["newton"] / g["seconds"], "steps": gsteps,
                                "ms_per_step": 1e3 * g["seconds"] / gsteps}
+        out["generic_assembly_roofline"] = g["assembly"]
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (serial oracle, one Newton iteration)")
         out["cpu_baseline"] = cpu_baseline(nx, u0, dt)

@@ -76,6 +76,7 @@ SIGNATURES = {
     "heat_step": [_P, _D, _IP, _DP],
     "heat_state": [_P, _DP],
     "heat_set_state": [_P, _DP],
+    "heat_eval_time": [_P, _D, _I, _DP, _IP],
     "heat_destroy": [_P],
     "batch_create": [_S, C.POINTER(_P)],
     "batch_destroy": [_P],

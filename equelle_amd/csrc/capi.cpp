@@ -2,6 +2,7 @@
 #include <eqhip.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -516,6 +517,29 @@ int eqhip_heat_set_state(void* sim, const double* u) {
         EQ_REQUIRE(sim, "null simulation");
         HeatSim* s = static_cast<HeatSim*>(sim);
         s->u = equelleCUDA::CollOfScalar(std::vector<double>(u, u + s->u.size()));
+    });
+}
+
+int eqhip_heat_eval_time(void* sim, double dt, int reps, double* ms_per_eval, int* info) {
+    return guard([&] {
+        EQ_REQUIRE(sim, "null simulation");
+        EQ_REQUIRE(reps > 0, "reps must be positive");
+        HeatSim* s = static_cast<HeatSim*>(sim);
+        const equelleCUDA::CollOfScalar u0 = s->u;
+        const equelleCUDA::CollOfScalar u(s->u, true);   // the primary variable of the Newton solve
+        auto eval = [&] {
+            equelleCUDA::CollOfScalar r = s->prog->residual(u, u0, dt);
+            (void)jac_values(*r.data());   // the Jacobian's values, pending factors applied
+            return r;
+        };
+        equelleCUDA::CollOfScalar r = eval();   // symbolic passes and caches outside the timing
+        Device::get().sync();
+        const auto t0 = std::chrono::steady_clock::now();
+        for (int q = 0; q < reps; ++q) r = eval();
+        Device::get().sync();
+        *ms_per_eval = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / reps;
+        info[0] = r.size();
+        info[1] = r.data()->pat ? r.data()->pat->nnz : 0;
     });
 }
 

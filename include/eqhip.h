@@ -156,6 +156,11 @@ int eqhip_heat_step(void* sim, double dt, int* stats, double* norms);
 int eqhip_heat_state(void* sim, double* u);
 /* replaces the session's state (the field the next step starts from) */
 int eqhip_heat_set_state(void* sim, const double* u);
+/* Time of one residual + Jacobian evaluation of the program at the session's state (the
+ * work of one Newton iteration before its linear solve, EquelleRuntimeCUDA_impl.hpp:216-300
+ * evaluated by the device AD engine), averaged over reps evaluations after one untimed one;
+ * info = {n, Jacobian nnz}.  Measurement only: the state is not changed. */
+int eqhip_heat_eval_time(void* sim, double dt, int reps, double* ms_per_eval, int* info);
 int eqhip_heat_destroy(void* sim);
 
 /* ------------------------------------------------------------------ two-phase sessions */
