@@ -67,9 +67,9 @@ def timing_avg(lib, name):
     return cnt.value, (tot.value / cnt.value if cnt.value else float("nan"))
 
 
-def run_fused(lib, precision, u0, nx, ny, dt, steps, warmup, precond="ilu0", graph=True, inner_tol=1e-6):
+def run_fused(lib, precision, u0, nx, ny, dt, steps, warmup, precond="ilu0", graph=True, inner_tol=1e-6, extra=""):
     params = (f"precision={precision} preconditioner={precond} rel_res_tol={NONLINEAR_HEAT['rel_res_tol']} "
-              f"abs_res_tol=0 max_iter=20 solver_tol=1e-6 inner_tol={inner_tol} graph={int(graph)}").encode()
+              f"abs_res_tol=0 max_iter=20 solver_tol=1e-6 inner_tol={inner_tol} graph={int(graph)} {extra}").encode()
     b = C.c_void_p()
     lib.batch_create(params, C.byref(b))
     slot = C.c_int()
@@ -615,33 +615,44 @@ def measure_traffic(args):
     if shutil.which("rocprofv3") is None:
         return {"error": "rocprofv3 not on PATH"}
     me = os.path.abspath(__file__)
-    legs = {"head": ["--steps", "1", "--warmup", "0", "--no-cpu-baseline", "--quick", "--size", str(args.size),
-                     "--precond", args.precond, "--inner-tol", repr(args.inner_tol)],
-            "copy": ["--workload", "copy", "--steps", "3", "--warmup", "1"]}
+    head = ["--steps", "1", "--warmup", "0", "--no-cpu-baseline", "--quick", "--size", str(args.size),
+            "--precond", args.precond, "--inner-tol", repr(args.inner_tol)]
     got = {}
+    shape = None   # the launch shape the first counter run's tuner chose; the later runs use it
     env = dict(os.environ, TMPDIR="/tmp")
     with tempfile.TemporaryDirectory(dir="/tmp") as d:
-        for leg, extra in legs.items():
-            for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
-                out = os.path.join(d, f"{leg}_{ctr}")
-                cmd = ["rocprofv3", "--pmc", ctr, "--output-format", "csv", "-d", out, "-o", "run", "--",
-                       sys.executable, me] + extra
-                log(f"[bench] traffic: {ctr} over the {leg} run")
-                p = subprocess.Popen(cmd, cwd=d, env=env, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE,
-                                     start_new_session=True)
+        for leg, ctr in (("head", "FETCH_SIZE"), ("head", "WRITE_SIZE"), ("copy", "FETCH_SIZE"),
+                         ("copy", "WRITE_SIZE")):
+            extra = (head + (["--shape", shape] if shape else [])) if leg == "head" else \
+                ["--workload", "copy", "--steps", "3", "--warmup", "1"]
+            out = os.path.join(d, f"{leg}_{ctr}")
+            cmd = ["rocprofv3", "--pmc", ctr, "--output-format", "csv", "-d", out, "-o", "run", "--",
+                   sys.executable, me] + extra
+            log(f"[bench] traffic: {ctr} over the {leg} run")
+            p = subprocess.Popen(cmd, cwd=d, env=env, stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                                 start_new_session=True)
+            try:
+                sout, err = p.communicate(timeout=240)
+            except subprocess.TimeoutExpired:
+                os.killpg(p.pid, signal.SIGKILL)
+                p.communicate()
+                return {"error": f"rocprofv3 --pmc {ctr} over the {leg} run timed out"}
+            if leg == "head" and shape is None:
                 try:
-                    _, err = p.communicate(timeout=240)
-                except subprocess.TimeoutExpired:
-                    os.killpg(p.pid, signal.SIGKILL)
-                    p.communicate()
-                    return {"error": f"rocprofv3 --pmc {ctr} over the {leg} run timed out"}
-                files = glob.glob(os.path.join(out, "**", "*counter_collection.csv"), recursive=True)
-                if p.returncode != 0 and not files:
-                    return {"error": f"rocprofv3 --pmc {ctr} over the {leg} run exited {p.returncode}: "
-                                     f"{err.decode(errors='replace')[-300:]}"}
-                if not files:
-                    return {"error": f"rocprofv3 --pmc {ctr} over the {leg} run wrote no counter file"}
-                got[(leg, ctr)] = pmc_per_dispatch(files[0])
+                    line = json.loads([ln for ln in sout.decode(errors="replace").splitlines()
+                                       if ln.startswith("{")][-1])
+                    cp, ls = line["launch_shape"]["cells_per_thread"], line["launch_shape"]
+                    shape = ",".join(str(cp[k]) for k in ("assemble", "defect", "spmv", "precond", "vector")) + \
+                        f",{ls['xcd_block_order']}"
+                except (IndexError, KeyError, ValueError):
+                    return {"error": "the first counter run printed no launch shape"}
+            files = glob.glob(os.path.join(out, "**", "*counter_collection.csv"), recursive=True)
+            if p.returncode != 0 and not files:
+                return {"error": f"rocprofv3 --pmc {ctr} over the {leg} run exited {p.returncode}: "
+                                 f"{err.decode(errors='replace')[-300:]}"}
+            if not files:
+                return {"error": f"rocprofv3 --pmc {ctr} over the {leg} run wrote no counter file"}
+            got[(leg, ctr)] = pmc_per_dispatch(files[0])
     copy = {ctr: [v for k, v in got[("copy", ctr)].items() if "copy" in k.lower()] for ctr in ("FETCH_SIZE",
                                                                                                "WRITE_SIZE")}
     if not copy["FETCH_SIZE"] or not copy["WRITE_SIZE"]:
@@ -658,9 +669,10 @@ def measure_traffic(args):
         f, w = vals[-50:], write[name][-50:]
         per_kernel[short] = {"dispatches": len(vals), "read_bytes": sum(f) / len(f) * 1024 * scale_r,
                              "write_bytes": sum(w) / len(w) * 1024 * scale_w}
-    return {"per_kernel": per_kernel, "scale_read": scale_r, "scale_write": scale_w,
+    return {"per_kernel": per_kernel, "scale_read": scale_r, "scale_write": scale_w, "shape": shape,
             "source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE child runs of this script (headline at 1 timed "
-                      "step; 256 MiB device-to-device copy for the byte scale)"}
+                      "step; 256 MiB device-to-device copy for the byte scale); the headline of this run uses "
+                      "the launch shape the first counter run chose"}
 
 
 def traffic_of(measured, kernel):
@@ -774,6 +786,9 @@ def main():
     ap.add_argument("--quick", action="store_true", help="headline run only (for profiling)")
     ap.add_argument("--no-traffic", dest="no_traffic", action="store_true",
                     help="skip the rocprofv3 counter runs that measure roofline.traffic")
+    ap.add_argument("--shape", default=None,
+                    help="headline launch shape a,d,s,p,v,xcd (cells per thread of assembly, defect, SpMV, "
+                         "preconditioner and vector kernels; XCD block order) instead of the tuner's")
     ap.add_argument("--no-extras", dest="no_extras", action="store_true",
                     help="heat headline legs only (no batched, two-phase and serving summaries)")
     ap.add_argument("--default-rates", dest="default_rates", default="50,500",
@@ -825,8 +840,9 @@ def main():
     dt = NONLINEAR_HEAT["dt"]
     u0 = gaussian_bumps(nx, ny, seed=2048)
     log(f"[bench] nonlinear heat {nx}x{ny}: fused engine, mixed precision")
+    shape = args.shape or (measured or {}).get("shape")
     head = run_fused(lib, "mixed", u0, nx, ny, dt, args.steps, args.warmup, args.precond, not args.no_graph,
-                     args.inner_tol)
+                     args.inner_tol, extra=f"shape={shape}" if shape else "")
     value = n * head["newton"] / head["seconds"]
     kt = kernel_table(head, n, 4, args.steps)
     dom = max(kt, key=lambda k: kt[k]["us_per_step"])

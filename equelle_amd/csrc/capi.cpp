@@ -2,6 +2,7 @@
 #include <eqhip.h>
 
 #include <algorithm>
+#include <cstdio>
 #include <chrono>
 #include <cstring>
 #include <memory>
@@ -744,6 +745,17 @@ static FusedConfig fused_config(const char* params) {
     c.cpt = m.get_int("cpt", c.cpt);
     EQ_REQUIRE(c.cpt == 0 || c.cpt == 1 || c.cpt == 2 || c.cpt == 4 || c.cpt == 8, "cpt must be 0, 1, 2, 4 or 8");
     c.xcd = m.get_int("xcd", c.xcd) != 0 ? 1 : 0;
+    const std::string shape = m.get_string("shape", "");
+    if (!shape.empty()) {   // a,d,s,p,v,xcd: cells per thread of every kernel class, XCD order
+        int v[6];
+        EQ_REQUIRE(std::sscanf(shape.c_str(), "%d,%d,%d,%d,%d,%d", &v[0], &v[1], &v[2], &v[3], &v[4], &v[5]) == 6,
+                   "shape must be six comma-separated integers, got " << shape);
+        for (int q = 0; q < 5; ++q) {
+            EQ_REQUIRE(v[q] == 1 || v[q] == 2 || v[q] == 4 || v[q] == 8, "shape: cells per thread must be 1, 2, 4 or 8");
+            c.cpt_fixed[q] = v[q];
+        }
+        c.xcd = v[5] != 0 ? 1 : 0;
+    }
     c.tune = m.get_int("tune", 1) != 0;
     return c;
 }

@@ -2463,9 +2463,9 @@ struct HeatBatch::Impl {
         return r;
     }
     Shape shape_for(const std::vector<Inst*>& list) {
-        if (cfg.cpt > 0) {
+        if (cfg.cpt > 0 || cfg.cpt_fixed[0] > 0) {
             Shape sh;
-            for (int c = 0; c < NCLS; ++c) sh.cpt[c] = cfg.cpt;
+            for (int c = 0; c < NCLS; ++c) sh.cpt[c] = cfg.cpt > 0 ? cfg.cpt : cfg.cpt_fixed[c];
             sh.xcd = cfg.xcd;
             return last_shape = sh;
         }

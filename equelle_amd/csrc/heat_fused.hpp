@@ -77,6 +77,9 @@ struct FusedConfig {
     // the iteration's norm check)
     bool speculate = true;
     int cpt = 0;                       // cells per thread (1, 2, 4, 8); 0 = chosen by the tuner
+    // a fixed shape per kernel class (assembly, defect, SpMV, preconditioner, vector
+    // updates; 0 = none): set with the XCD order by parameter "shape=a,d,s,p,v,xcd"
+    int cpt_fixed[5] = {0, 0, 0, 0, 0};
     int xcd = 1;                       // XCD-contiguous block order when cpt is fixed
     bool tune = true;                  // time the launch shapes per membership signature
 };
