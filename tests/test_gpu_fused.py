@@ -282,6 +282,45 @@ def test_graph_replays_match_eager(hip_api):
         assert np.array_equal(a, g)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_speculative_solves_match_checked_solves(hip_api, precision):
+    """Queuing the work after a Krylov solve behind the predicted iterations (gated on the
+    device by the convergence count) gives the same results as waiting for the count.  The
+    state is replaced by a rough field between steps, so the iteration predictions of the
+    smooth steps fall short and the gated work is queued again after the catch-up."""
+    sizes = [(96, 72), (64, 64)]
+    rng = np.random.default_rng(5)
+    smooth = [1.0 + 0.01 * gaussian_bumps(nx, ny, 3 + q, width_scale=0.2) for q, (nx, ny) in enumerate(sizes)]
+    rough = [rng.uniform(0.0, 2.0, nx * ny) for nx, ny in sizes]
+    res = {}
+    for spec in (0, 1):
+        b = make_batch(hip_api, f"precision={precision} preconditioner=ilu0 rel_res_tol=1e-8 solver_tol=1e-6 "
+                                f"graph=1 speculate={spec}")
+        try:
+            slots = np.array([add(hip_api, b, nx, ny, 0.3, u) for (nx, ny), u in zip(sizes, smooth)],
+                             dtype=np.int32)
+            traj = []
+            for phase, fields in (("smooth", None), ("rough", rough)):
+                if fields is not None:
+                    for sl, u in zip(slots, fields):
+                        hip_api.batch_set_state(b, int(sl), dptr(u))
+                for _ in range(3):
+                    hip_api.batch_step(b, iptr(slots), len(slots), 1.0)
+                    for sl, (nx, ny) in zip(slots, sizes):
+                        st, _ = stats(hip_api, b, int(sl))
+                        assert st[3] != 0 and st[4] == 0, (phase, st)
+                        traj.append((state(hip_api, b, int(sl), nx * ny), st[:3].tolist()))
+            res[spec] = traj
+        finally:
+            hip_api.batch_destroy(b)
+    for (ua, sa), (ub, sb) in zip(res[0], res[1]):
+        assert sa == sb
+        assert np.array_equal(ua, ub)
+    # the rough steps need more Krylov iterations than the smooth ones predicted
+    half = len(res[1]) // 2
+    assert sum(s[1] for _, s in res[1][half:]) > sum(s[1] for _, s in res[1][:half])
+
+
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("mixed", 1e-9)])
 def test_pair_vectors_match_strided_form(hip_api, precision, tol):
     """The four-ranks-per-lane Krylov kernels (2D heat, nx % 8 == 0) and the strided

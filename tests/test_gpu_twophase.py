@@ -319,3 +319,66 @@ def test_fused_twophase_and_heat_share_a_batch(hip_api, precision):
     alone = run("tp0") + run("tp1") + run("heat0")
     for a, c in zip(together, alone):
         assert np.array_equal(a, c)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_fused_twophase_speculative_matches_checked(hip_api, precision):
+    """Two-phase instances (multigrid CG pressure stage, reduced-form transport stage) give
+    the same steps whether the work after each Krylov solve waits for the convergence count
+    or is queued behind the predicted iterations."""
+    cases = [(24, 20, 16, 3), (16, 12, 8, 4)]
+    data = [case(nx, ny, nz, seed=sd) for nx, ny, nz, sd in cases]
+    res = {}
+    for spec in (0, 1):
+        b = fused_batch(hip_api, f"precision={precision} preconditioner=ilu0 rel_res_tol=1e-9 abs_res_tol=0 "
+                                 f"solver_tol=1e-8 max_iter=40 solver_max_iter=5000 graph=1 speculate={spec}")
+        try:
+            slots = np.array([fused_add(hip_api, b, nx, ny, nz, *d) for (nx, ny, nz, _), d in zip(cases, data)],
+                             dtype=np.int32)
+            st, tps = np.zeros(5, dtype=np.int32), np.zeros(3, dtype=np.int32)
+            traj = []
+            for dt in (1.0, 2.0, 2.0):
+                hip_api.batch_step(b, iptr(slots), len(slots), dt)
+                for sl, (nx, ny, nz, _) in zip(slots, cases):
+                    hip_api.batch_stats(b, int(sl), iptr(st), None)
+                    hip_api.batch_twophase_stats(b, int(sl), iptr(tps))
+                    assert st[3] in (1, 2) and tps[2] in (1, 2) and st[4] == 0, (spec, st, tps)
+                    ph, sh = np.zeros(nx * ny * nz), np.zeros(nx * ny * nz)
+                    hip_api.batch_twophase_state(b, int(sl), dptr(ph), dptr(sh))
+                    traj.append((ph, sh, st[:3].tolist(), tps.tolist()))
+            res[spec] = traj
+        finally:
+            hip_api.batch_destroy(b)
+    for a, g in zip(res[0], res[1]):
+        assert a[2] == g[2] and a[3] == g[3]
+        assert np.array_equal(a[0], g[0]) and np.array_equal(a[1], g[1])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_fused_twophase_split_form_matches_full_form(hip_api, precision):
+    """The transport stage in the reduced form inside the full form's storage (the
+    default with the multigrid pressure stage) against the full form with red-black
+    ILU(0): equal in exact arithmetic, so the steps agree to the solver tolerances."""
+    nx, ny, nz = 24, 20, 16
+    perm, poro, sw0, sc, sv = case(nx, ny, nz, seed=7)
+    n = perm.size
+    out = {}
+    for split in (0, 1):
+        b = fused_batch(hip_api, f"precision={precision} preconditioner=ilu0 rel_res_tol=1e-10 abs_res_tol=0 "
+                                 f"solver_tol=1e-10 max_iter=40 solver_max_iter=5000 twophase_split_form={split}")
+        try:
+            slot = fused_add(hip_api, b, nx, ny, nz, perm, poro, sw0, sc, sv)
+            st, tps = np.zeros(5, dtype=np.int32), np.zeros(3, dtype=np.int32)
+            for dt in (1.0, 1.0):
+                hip_api.batch_step(b, iptr(np.array([slot], dtype=np.int32)), 1, dt)
+                hip_api.batch_stats(b, slot, iptr(st), None)
+                hip_api.batch_twophase_stats(b, slot, iptr(tps))
+                assert st[3] in (1, 2) and tps[2] in (1, 2) and st[4] == 0, (split, st, tps)
+            ph, sh = np.zeros(n), np.zeros(n)
+            hip_api.batch_twophase_state(b, slot, dptr(ph), dptr(sh))
+            out[split] = (ph - ph.mean(), sh)
+        finally:
+            hip_api.batch_destroy(b)
+    (p1, s1), (p0, s0) = out[1], out[0]
+    assert np.allclose(s1, s0, rtol=1e-7, atol=1e-8)
+    assert np.allclose(p1, p0, rtol=1e-7, atol=1e-7 * np.abs(p0).max())
