@@ -154,7 +154,7 @@ def run_generic(lib, u0, nx, ny, dt, steps, warmup, precision="fp64", extra="", 
         lib.heat_state(sim, dptr(u))
         assembly = None
         if eval_reps:
-            ms, info = C.c_double(), np.zeros(2, dtype=np.int32)
+            ms, info = C.c_double(), np.zeros(3, dtype=np.int32)
             lib.heat_eval_time(sim, dt, eval_reps, C.byref(ms), iptr(info))
             assembly = assembly_roofline(nx, ny, int(info[1]), ms.value, eval_reps)
     finally:

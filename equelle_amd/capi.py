@@ -78,6 +78,8 @@ SIGNATURES = {
     "heat_set_state": [_P, _DP],
     "heat_eval_time": [_P, _D, _I, _DP, _IP],
     "heat_destroy": [_P],
+    "residual_stats": [C.POINTER(C.c_long), _DP],
+    "residual_codegen_selftest": [C.c_char_p, _I],
     "batch_create": [_S, C.POINTER(_P)],
     "batch_destroy": [_P],
     "batch_add": [_P, _I, _I, _D, _D, _DP, _IP],

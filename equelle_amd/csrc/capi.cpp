@@ -16,6 +16,7 @@
 #include "kernels.hpp"
 #include "krylov.hpp"
 #include "runtime_core.hpp"
+#include "tape.hpp"
 #include "serve.hpp"
 #include "twophase_program.hpp"
 #include "timing.hpp"
@@ -528,12 +529,27 @@ int eqhip_heat_eval_time(void* sim, double dt, int reps, double* ms_per_eval, in
         HeatSim* s = static_cast<HeatSim*>(sim);
         const equelleCUDA::CollOfScalar u0 = s->u;
         const equelleCUDA::CollOfScalar u(s->u, true);   // the primary variable of the Newton solve
+        auto rescomp = [&](const equelleCUDA::CollOfScalar& x) { return s->prog->residual(x, u0, dt); };
+        // the recorded program, as newtonSolve builds it (one call site of its own)
+        static int site_state = 0;
+        equelleCUDA::FusedResidual fr(*s->er, &site_state, u);
+        equelleCUDA::CollOfScalar r;
+        for (int pass = 0;; ++pass) {
+            fr.begin(pass);
+            try {
+                if (fr.finish(rescomp(u), &r)) break;
+            } catch (const std::exception& e) {
+                fr.aborted(e);
+            }
+        }
+        const bool fused = fr.fused();
         auto eval = [&] {
-            equelleCUDA::CollOfScalar r = s->prog->residual(u, u0, dt);
-            (void)jac_values(*r.data());   // the Jacobian's values, pending factors applied
-            return r;
+            if (fused) return fr.eval(u, true);
+            equelleCUDA::CollOfScalar q = rescomp(u);
+            (void)jac_values(*q.data());   // the Jacobian's values, pending factors applied
+            return q;
         };
-        equelleCUDA::CollOfScalar r = eval();   // symbolic passes and caches outside the timing
+        r = eval();   // symbolic passes and caches outside the timing
         Device::get().sync();
         const auto t0 = std::chrono::steady_clock::now();
         for (int q = 0; q < reps; ++q) r = eval();
@@ -541,7 +557,27 @@ int eqhip_heat_eval_time(void* sim, double dt, int reps, double* ms_per_eval, in
         *ms_per_eval = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / reps;
         info[0] = r.size();
         info[1] = r.data()->pat ? r.data()->pat->nnz : 0;
+        info[2] = fused ? 1 : 0;
     });
+}
+
+int eqhip_residual_stats(long* out, double* seconds) {
+    return guard([&] {
+        const ResidualStats st = residual_stats();
+        const long v[6] = {st.recorded, st.symbolic, st.compiled, st.evaluations, st.aborted, st.rejected};
+        for (int i = 0; i < 6; ++i) out[i] = v[i];
+        if (seconds) *seconds = st.compile_seconds;
+    });
+}
+
+int eqhip_residual_codegen_selftest(char* log, int len) {
+    std::string lg;
+    const int rc = residual_codegen_selftest(&lg);
+    if (log && len > 0) {
+        std::strncpy(log, lg.c_str(), (size_t)len - 1);
+        log[len - 1] = '\0';
+    }
+    return rc;
 }
 
 int eqhip_heat_destroy(void* sim) {

@@ -184,6 +184,10 @@ struct ScalarData {
     uint64_t jmap_id = 0;
     bool jones = false;    // (pat, jval) is the n x n identity with unit values (a primary variable)
     mutable DevPtr<double> jeff;   // the factors applied (materialized on first request)
+    // Residual recording: the recording session and node this value is (0 / -1: none).
+    // A value made by a symbolic recording pass has no device values (val is null).
+    mutable uint64_t tsess = 0;
+    mutable int tnode = -1;
     bool jpending() const { return jscale != 1.0 || (bool)jrow; }
     int jcols() const { return pat ? pat->cols : 0; }
 };
@@ -203,6 +207,8 @@ IP make_index(int codim, bool full, std::vector<int> host);
 struct BoolData {
     int n = 0;
     DevPtr<uint8_t> val;
+    mutable uint64_t tsess = 0;   // recording session and node, as for ScalarData
+    mutable int tnode = -1;
 };
 using BP = std::shared_ptr<const BoolData>;
 

@@ -7,6 +7,7 @@
 #include <cstring>
 
 #include "kernels.hpp"
+#include "tape.hpp"
 
 namespace eqhip {
 
@@ -299,6 +300,7 @@ SP s_uniform(int n, double v) {
 SP s_from_device(DevPtr<double> v, int n) { return make_scalar(n, std::move(v), false, nullptr, nullptr); }
 
 SP s_primary(const SP& a) {
+    tape_read(*a, "a primary variable made inside the residual");
     JacOut j;
     j.pat = identity_pattern(a->n);
     j.val = dev_alloc<double>(a->n);
@@ -313,10 +315,15 @@ SP s_with_jacobian(const double* v, int n, const int* rp, const int* ci, const d
     return make_scalar(n, dev_from_host(v, n), true, p, dev_from_host(jv, nnz));
 }
 
-SP s_value_only(const SP& a) { return make_scalar(a->n, a->val, false, nullptr, nullptr); }
+SP s_value_only(const SP& a) {
+    if (Tape* tp = tape_for(*a)) EQ_TAPE_RETURN(tp, tp->node(T_VAL, tp->operand(a)), a->n, false, s_value_only(a));
+    return make_scalar(a->n, a->val, false, nullptr, nullptr);
+}
 
 SP s_add(const SP& a, const SP& b) {
     check_size(a, b, "+");
+    if (Tape* tp = tape_for(*a, *b))
+        EQ_TAPE_RETURN(tp, tp->node(T_ADD, tp->operand(a), tp->operand(b)), a->n, a->ad || b->ad, s_add(a, b));
     SP v = binary_values(EW_ADD, a, b);
     if (!(a->ad || b->ad)) return v;
     return make_ad(a->n, v->val, jac_lincomb(a, nullptr, 1.0, b, nullptr, 1.0, nullptr));
@@ -324,6 +331,8 @@ SP s_add(const SP& a, const SP& b) {
 
 SP s_sub(const SP& a, const SP& b) {
     check_size(a, b, "-");
+    if (Tape* tp = tape_for(*a, *b))
+        EQ_TAPE_RETURN(tp, tp->node(T_SUB, tp->operand(a), tp->operand(b)), a->n, a->ad || b->ad, s_sub(a, b));
     SP v = binary_values(EW_SUB, a, b);
     if (!(a->ad || b->ad)) return v;
     return make_ad(a->n, v->val, jac_lincomb(a, nullptr, 1.0, b, nullptr, -1.0, nullptr));
@@ -331,6 +340,8 @@ SP s_sub(const SP& a, const SP& b) {
 
 SP s_mul(const SP& a, const SP& b) {
     check_size(a, b, "*");
+    if (Tape* tp = tape_for(*a, *b))
+        EQ_TAPE_RETURN(tp, tp->node(T_MUL, tp->operand(a), tp->operand(b)), a->n, a->ad || b->ad, s_mul(a, b));
     SP v = binary_values(EW_MUL, a, b);
     if (!(a->ad || b->ad)) return v;
     // (uv)' = diag(v) u' + diag(u) v'
@@ -339,6 +350,8 @@ SP s_mul(const SP& a, const SP& b) {
 
 SP s_div(const SP& a, const SP& b) {
     check_size(a, b, "/");
+    if (Tape* tp = tape_for(*a, *b))
+        EQ_TAPE_RETURN(tp, tp->node(T_DIV, tp->operand(a), tp->operand(b)), a->n, a->ad || b->ad, s_div(a, b));
     SP v = binary_values(EW_DIV, a, b);
     if (!(a->ad || b->ad)) return v;
     // (u/v)' = diag(1/v^2) (diag(v) u' - diag(u) v')
@@ -348,6 +361,8 @@ SP s_div(const SP& a, const SP& b) {
 }
 
 SP s_scal_mul(double s, const SP& a) {
+    if (Tape* tp = tape_for(*a))
+        EQ_TAPE_RETURN(tp, tp->node(T_SMUL, tp->operand(a), -1, -1, tp->constant(s)), a->n, a->ad, s_scal_mul(s, a));
     auto v = dev_alloc<double>(a->n);
     launch_ew_scalar(EW_MUL, s, a->val->p, v->p, a->n, 1);
     if (!a->ad) return make_scalar(a->n, v, false, nullptr, nullptr);
@@ -358,6 +373,8 @@ SP s_scal_mul(double s, const SP& a) {
 SP s_div_scal(const SP& a, double s) { return s_scal_mul(1.0 / s, a); }
 
 SP s_scal_div(double s, const SP& a) {
+    if (Tape* tp = tape_for(*a))
+        EQ_TAPE_RETURN(tp, tp->node(T_SDIV, tp->operand(a), -1, -1, tp->constant(s)), a->n, a->ad, s_scal_div(s, a));
     auto v = dev_alloc<double>(a->n);
     launch_ew_scalar(EW_DIV, s, a->val->p, v->p, a->n, 1);
     if (!a->ad) return make_scalar(a->n, v, false, nullptr, nullptr);
@@ -370,6 +387,7 @@ SP s_scal_div(double s, const SP& a) {
 SP s_neg(const SP& a) { return s_scal_mul(-1.0, a); }
 
 SP s_norm(const SP& a) {
+    if (Tape* tp = tape_for(*a)) EQ_TAPE_RETURN(tp, tp->node(T_NORM, tp->operand(a)), a->n, a->ad, s_norm(a));
     // factors (+1 where a >= 0, else -1) times a, as a constant times a collection
     // (EquelleRuntimeCPU.cpp:241-254): values |a|, Jacobian diag(factors) a' (a's pattern)
     auto c = dev_alloc<double>(std::max(1, a->n));
@@ -378,6 +396,7 @@ SP s_norm(const SP& a) {
 }
 
 SP s_sqrt(const SP& a) {
+    if (Tape* tp = tape_for(*a)) EQ_TAPE_RETURN(tp, tp->node(T_SQRT, tp->operand(a)), a->n, a->ad, s_sqrt(a));
     auto v = dev_alloc<double>(a->n);
     launch_sqrt(a->val->p, v->p, a->n);
     if (!a->ad) return make_scalar(a->n, v, false, nullptr, nullptr);
@@ -388,6 +407,17 @@ SP s_sqrt(const SP& a) {
 
 BP s_compare(int op, const SP& a, const SP& b) {
     check_size(a, b, " comparison");
+    if (Tape* tp = tape_for(*a, *b)) {
+        const int id = tp->node(T_CMP, tp->operand(a), tp->operand(b), -1, -1, op);
+        if (tp->symbolic()) return tp->symbolic_bool(id, a->n);
+        BP r;
+        {
+            TapeSuspend ts;
+            r = s_compare(op, a, b);
+        }
+        tp->tag(*r, id);
+        return r;
+    }
     auto o = std::make_shared<BoolData>();
     o->n = a->n;
     o->val = dev_alloc<uint8_t>(a->n);
@@ -396,6 +426,17 @@ BP s_compare(int op, const SP& a, const SP& b) {
 }
 
 BP s_compare_scalar(int op, const SP& a, double s, int scalar_left) {
+    if (Tape* tp = tape_for(*a)) {
+        const int id = tp->node(T_CMPS, tp->operand(a), -1, -1, tp->constant(s), op | (scalar_left ? 16 : 0));
+        if (tp->symbolic()) return tp->symbolic_bool(id, a->n);
+        BP r;
+        {
+            TapeSuspend ts;
+            r = s_compare_scalar(op, a, s, scalar_left);
+        }
+        tp->tag(*r, id);
+        return r;
+    }
     auto o = std::make_shared<BoolData>();
     o->n = a->n;
     o->val = dev_alloc<uint8_t>(a->n);
@@ -405,6 +446,10 @@ BP s_compare_scalar(int op, const SP& a, double s, int scalar_left) {
 
 SP s_select(const BP& p, const SP& t, const SP& f) {
     EQ_REQUIRE(p->n == t->n && p->n == f->n, "Collections are not of the same size");
+    if (Tape* tp = tape_current())
+        if (tp->node_of(*p) >= 0 || tp->node_of(*t) >= 0 || tp->node_of(*f) >= 0)
+            EQ_TAPE_RETURN(tp, tp->node(T_SEL, tp->operand(p), tp->operand(t), tp->operand(f)), t->n, t->ad || f->ad,
+                           s_select(p, t, f));
     auto v = dev_alloc<double>(t->n);
     launch_select(p->val->p, t->val->p, f->val->p, v->p, t->n);
     if (!(t->ad || f->ad)) return make_scalar(t->n, v, false, nullptr, nullptr);
@@ -432,13 +477,19 @@ SP s_select(const BP& p, const SP& t, const SP& f) {
 }
 
 double s_reduce(int op, const SP& a) {
+    tape_read(*a, "a reduction");
     EQ_REQUIRE(a->n > 0 || op == RED_SUM || op == RED_PROD, "reduction of an empty collection");
     return reduce(op, a->val->p, a->n);
 }
 
-double s_two_norm(const SP& a) { return std::sqrt(reduce(RED_SUMSQ, a->val->p, a->n)); }
+double s_two_norm(const SP& a) {
+    tape_read(*a, "a norm");
+    return std::sqrt(reduce(RED_SUMSQ, a->val->p, a->n)); }
 
-std::vector<double> s_to_host(const SP& a) { return dev_to_host(a->val->p, a->n); }
+std::vector<double> s_to_host(const SP& a) {
+    tape_read(*a, "a copy to the host");
+    return dev_to_host(a->val->p, a->n);
+}
 
 // ---------------------------------------------------------------- matrices
 DevCsr csr_from_host(int rows, int cols, const int* rp, const int* ci, const double* v) {
@@ -460,6 +511,9 @@ DevCsr csr_from_host(int rows, int cols, const int* rp, const int* ci, const dou
 SP s_matmul(const DevCsr& m, const SP& a, DevPtr<double> values) {
     EQ_REQUIRE(m.pat->cols == a->n, "Error in matrix * vector operation as matrix is of size "
                                         << m.pat->rows << " by " << m.pat->cols << " and the vector of size " << a->n);
+    if (Tape* tp = tape_for(*a))
+        EQ_TAPE_RETURN(tp, tp->node(T_MAT, tp->operand(a), -1, -1, tp->matrix(m)), m.pat->rows, a->ad,
+                       s_matmul(m, a, values));
     DevPtr<double> v = values;
     if (!v) {
         v = dev_alloc<double>(m.pat->rows);
@@ -544,6 +598,7 @@ static JacOut gather_rows(const SP& a, const SymP& e, const int* rowmap, int m) 
 }
 
 DevPtr<double> jac_effective(const ScalarData& a) {
+    tape_read(a, "a Jacobian read");
     if (!a.pat) return nullptr;
     if (!a.jpending() && !a.jmap) return a.jval;
     if (!a.jeff) {
@@ -617,6 +672,9 @@ SP s_on(const SP& a, const IP& from, const IP& to) {
                                                       << ") have to be of the same size in On function.");
     PMP pm = position_map(from, to);
     const int m = to->size();
+    if (Tape* tp = tape_for(*a))
+        EQ_TAPE_RETURN(tp, tp->node(T_ON, tp->operand(a), -1, -1, tp->map(pm->d_pos, pm->id, pm)), m, a->ad,
+                       s_on(a, from, to));
     auto v = dev_alloc<double>(m);
     launch_gather(a->val->p, pm->d_pos->p, v->p, m);
     if (!a->ad || !a->pat) return make_scalar(m, v, a->ad, nullptr, nullptr);
@@ -631,6 +689,9 @@ SP s_extend(const SP& a, const IP& from, const IP& to) {
                                                              << to->size() << ")");
     PMP pm = position_map(to, from);
     const int m = to->size();
+    if (Tape* tp = tape_for(*a))
+        EQ_TAPE_RETURN(tp, tp->node(T_EXT, tp->operand(a), -1, -1, tp->map(pm->d_inv, pm->id ^ 0x8000000000000000ull, pm)),
+                       m, a->ad, s_extend(a, from, to));
     auto v = dev_alloc<double>(m);
     launch_gather(a->val->p, pm->d_inv->p, v->p, m);
     if (!a->ad || !a->pat) return make_scalar(m, v, a->ad, nullptr, nullptr);
@@ -659,6 +720,7 @@ IP i_on(const IP& data, const IP& from, const IP& to) {
 }
 
 IP i_select(const BP& p, const IP& t, const IP& f) {
+    tape_read(*p, "an index selection by a predicate on the unknown");
     EQ_REQUIRE(p->n == t->size() && p->n == f->size(), "The sets are not of the same size");
     auto d = dev_alloc<int>(p->n);
     launch_select_int(p->val->p, t->dev->p, f->dev->p, d->p, p->n);
@@ -680,7 +742,9 @@ BP b_from_host(const uint8_t* h, int n) {
     return o;
 }
 
-std::vector<uint8_t> b_to_host(const BP& b) { return dev_to_host(b->val->p, b->n); }
+std::vector<uint8_t> b_to_host(const BP& b) {
+    tape_read(*b, "a copy to the host");
+    return dev_to_host(b->val->p, b->n); }
 
 // ---------------------------------------------------------------- vectors
 VP v_from_host(const double* h, int n, int dim) {
@@ -702,6 +766,7 @@ VP v_binary(int op, const VP& a, const VP& b) {
 }
 
 VP v_scale(const VP& a, const SP& s, double sc, int divide) {
+    if (s) tape_read(*s, "a vector scaled by the unknown");
     if (s) EQ_REQUIRE(s->n == a->n, "CollOfVector and CollOfScalar sizes differ");
     auto v = std::make_shared<VectorData>();
     v->n = a->n;

@@ -12,6 +12,7 @@
 #include "core.hpp"
 #include "engine.hpp"
 #include "kernels.hpp"
+#include "tape.hpp"
 #include "runtime_core.hpp"
 
 namespace eqhip {
@@ -95,6 +96,7 @@ CudaMatrix::CudaMatrix(int size) {
 }
 CudaMatrix::CudaMatrix(const CollOfScalar& diag) {
     const SP& s = diag.data();
+    eqhip::tape_read(*s, "a matrix made from the unknown");
     eqhip::DevCsr m;
     m.pat = eqhip::identity_pattern(s->n);
     m.val = s->val;
@@ -181,7 +183,10 @@ CollOfScalar::CollOfScalar(const int size, const double value) : d_(eqhip::s_uni
 CollOfScalar::CollOfScalar(const std::vector<double>& h) : d_(eqhip::s_from_host(h.data(), (int)h.size())) {}
 CollOfScalar::CollOfScalar(const CudaArray& val) : d_(val.data() ? val.data() : eqhip::s_uniform(0, 0.0)) {}
 CollOfScalar::CollOfScalar(const CudaArray& val, const CudaMatrix& der) {
+    eqhip::tape_read(*val.data(), "a collection made from values of the unknown and a matrix");
     auto s = std::make_shared<eqhip::ScalarData>(*val.data());
+    s->tsess = 0;
+    s->tnode = -1;
     s->ad = true;
     s->jscale = 1.0;
     s->jrow = nullptr;
@@ -307,6 +312,27 @@ CollOfVector operator/(const CollOfVector& v, const Scalar s) { return CollOfVec
 CollOfVector operator/(const CollOfVector& v, const CollOfScalar& s) {
     return CollOfVector(eqhip::v_scale(v.data(), s.data(), 1.0, 1));
 }
+
+// ---------------------------------------------------------------- fused residual
+FusedResidual::FusedResidual(const EquelleRuntimeCUDA& er, int* site_state, const CollOfScalar& u)
+    : s_(new eqhip::ResidualSession(site_state, u.data(), er.core().fuse_residual)) {}
+FusedResidual::~FusedResidual() {}
+void FusedResidual::begin(int pass) { s_->begin(pass); }
+bool FusedResidual::finish(const CollOfScalar& residual, CollOfScalar* out) {
+    SP r = s_->finish(residual.data());
+    if (!r) return false;
+    *out = CollOfScalar(r);
+    return true;
+}
+void FusedResidual::aborted(const std::exception& e) {
+    if (!eqhip::ResidualSession::is_abort(e)) {
+        s_->abort();
+        throw;
+    }
+    s_->abort();
+}
+bool FusedResidual::fused() const { return s_->fused(); }
+CollOfScalar FusedResidual::eval(const CollOfScalar& u, bool jacobian) { return CollOfScalar(s_->eval(u.data(), jacobian)); }
 
 // ---------------------------------------------------------------- runtime
 static eqhip::ParamMap to_params(const Opm::ParameterGroup& p) {

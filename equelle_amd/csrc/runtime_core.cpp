@@ -1,6 +1,8 @@
 // Runtime core: device grid, HelperOps and grid built-ins (see runtime_core.hpp).
 #include "runtime_core.hpp"
 
+#include "tape.hpp"
+
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
@@ -250,6 +252,7 @@ RuntimeCore::RuntimeCore(const ParamMap& p) : params_(p) {
     solver.inner_tol = p.get_double("inner_tol", 1e-4);
     solver.max_refine = p.get_int("max_refine", 30);
     stencil_solver = p.get_bool("stencil_solver", true);
+    fuse_residual = p.get_bool("fuse_residual", true);
     verbose = p.get_int("verbose", 0);
     solver.verbose = verbose;
     max_iter = p.get_int("max_iter", 10);
@@ -299,6 +302,9 @@ VP RuntimeCore::normal(const IP& faces) const {
 
 SP RuntimeCore::gradient(const SP& x) const {
     EQ_REQUIRE(x->n == grid_->nc, "Gradient need input defined on AllCells()");
+    if (Tape* tp = tape_for(*x))
+        EQ_TAPE_RETURN(tp, tp->node(T_MAT, tp->operand(x), -1, -1, tp->matrix(grid_->grad)), grid_->nif, x->ad,
+                       gradient(x));
     // values: coalesced face sweep u[second] - u[first]; Jacobian: grad * J (cached pattern)
     auto v = dev_alloc<double>(std::max(1, grid_->nif));
     launch_gradient(x->val->p, grid_->if_c1->p, grid_->if_c2->p, v->p, grid_->nif);
@@ -310,6 +316,9 @@ SP RuntimeCore::gradient_matrix(const SP& x) const {
 }
 SP RuntimeCore::neg_gradient(const SP& x) const {
     EQ_REQUIRE(x->n == grid_->nc, "negGradient needs input defined on AllCells()");
+    if (Tape* tp = tape_for(*x))
+        EQ_TAPE_RETURN(tp, tp->node(T_MAT, tp->operand(x), -1, -1, tp->matrix(grid_->ngrad)), grid_->nif, x->ad,
+                       neg_gradient(x));
     // values u[first] - u[second] (the gradient sweep with the face cells swapped);
     // Jacobian ngrad * J, the pattern of grad * J with negated values
     auto v = dev_alloc<double>(std::max(1, grid_->nif));
@@ -323,6 +332,9 @@ SP RuntimeCore::interior_divergence(const SP& x) const {
 SP RuntimeCore::divergence(const SP& x) const {
     const DeviceGrid& g = *grid_;
     EQ_REQUIRE(x->n == g.nf || x->n == g.nif, "Input for divergence has to be on AllFaces or on InteriorFaces()");
+    if (Tape* tp = tape_for(*x))
+        EQ_TAPE_RETURN(tp, tp->node(T_MAT, tp->operand(x), -1, -1, tp->matrix(x->n == g.nf ? g.fulldiv : g.div)), g.nc,
+                       x->ad, divergence(x));
     // values: cell sweep over the signed face list (ascending face order per cell)
     const double* flux = x->val->p;
     DevPtr<double> full;
@@ -378,6 +390,7 @@ bool RuntimeCore::stencil_pattern(const PatternP& P) const {
 }
 
 SP RuntimeCore::solve_for_update(const SP& residual, LinReport* rep) const {
+    tape_read(*residual, "a linear solve");
     // BiCGStab with red-black ILU(0) or Jacobi on a Jacobian with the grid's stencil
     // pattern: the coefficients are gathered out of the CSR values and the solve runs on
     // the stencil kernels (no column indices in the iteration); other systems take the

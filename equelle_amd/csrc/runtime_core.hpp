@@ -72,6 +72,7 @@ public:
 
     SolverConfig solver;
     bool stencil_solver = true;   // stencil-pattern Jacobians: solve on the stencil kernels
+    bool fuse_residual = true;    // newtonSolve residuals: recorded once, then one fused kernel
     int max_iter = 10;
     double abs_res_tol = 1e-6;
     double rel_res_tol = 0.0;

@@ -30,6 +30,7 @@
 
 namespace eqhip {
 class RuntimeCore;
+class ResidualSession;
 }
 
 namespace equelleCUDA {
@@ -42,6 +43,27 @@ struct NewtonReport {
     double final_norm = 0.0;
     int stop_reason = 0;   // 1 = abs_res_tol met, 2 = rel_res_tol met, 0 = max_iter reached
     double seconds = 0.0;
+};
+
+class EquelleRuntimeCUDA;
+
+// The residual program of one newtonSolve (not in the reference): the first evaluation
+// records the operations on the unknown, later evaluations run one fused kernel
+// (libeqhip: recorded residual programs).  `site_state` is one int per call site.
+class FusedResidual {
+public:
+    FusedResidual(const EquelleRuntimeCUDA& er, int* site_state, const CollOfScalar& u);
+    ~FusedResidual();
+    void begin(int pass);
+    // true: `out` is the residual of this pass; false: run the next pass
+    bool finish(const CollOfScalar& residual, CollOfScalar* out);
+    // a recording pass stopped (rethrows exceptions that are not the recording's own)
+    void aborted(const std::exception& e);
+    bool fused() const;
+    CollOfScalar eval(const CollOfScalar& u, bool jacobian);
+
+private:
+    std::unique_ptr<eqhip::ResidualSession> s_;
 };
 
 class EquelleRuntimeCUDA {
@@ -201,9 +223,25 @@ template <class ResidualFunctor>
 CollOfScalar EquelleRuntimeCUDA::newtonSolve(const ResidualFunctor& rescomp, const CollOfScalar& u_initialguess) {
     const auto t0 = std::chrono::steady_clock::now();
     // Primary variable with identity Jacobian; the residual and its Jacobian are
-    // assembled by the AD engine on the device.
+    // assembled by the AD engine on the device.  The first evaluation records the
+    // operations on u; later evaluations run them as one fused kernel (one recorded
+    // program per call site, i.e. per residual functor type).
+    static int site_state = 0;
     CollOfScalar u = CollOfScalar(u_initialguess, true);
-    CollOfScalar residual = rescomp(u);
+    FusedResidual fr(*this, &site_state, u);
+    CollOfScalar residual;
+    for (int pass = 0;; ++pass) {
+        fr.begin(pass);
+        try {
+            if (fr.finish(rescomp(u), &residual)) break;
+        } catch (const std::exception& e) {
+            fr.aborted(e);
+        }
+    }
+    auto evaluate = [&](const CollOfScalar& x, bool jacobian) {
+        if (fr.fused()) return fr.eval(x, jacobian);
+        return jacobian ? rescomp(x) : rescomp(CollOfScalar(x.value()));
+    };
     const double r0 = twoNorm(residual);
     double rn = r0;
     int iter = 0;
@@ -223,13 +261,13 @@ CollOfScalar EquelleRuntimeCUDA::newtonSolve(const ResidualFunctor& rescomp, con
         u = u - du;
         ++iter;
         if (iter >= expected) {
-            const CollOfScalar values = rescomp(CollOfScalar(u.value()));
+            const CollOfScalar values = evaluate(u, false);
             rn = twoNorm(values);
             newtonIterationMessage(iter, rn);
-            if (!newtonStop(rn, r0) && iter < max_iter_) residual = rescomp(u);
+            if (!newtonStop(rn, r0) && iter < max_iter_) residual = evaluate(u, true);
             continue;
         }
-        residual = rescomp(u);
+        residual = evaluate(u, true);
         rn = twoNorm(residual);
         newtonIterationMessage(iter, rn);
     }

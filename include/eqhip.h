@@ -159,9 +159,23 @@ int eqhip_heat_set_state(void* sim, const double* u);
 /* Time of one residual + Jacobian evaluation of the program at the session's state (the
  * work of one Newton iteration before its linear solve, EquelleRuntimeCUDA_impl.hpp:216-300
  * evaluated by the device AD engine), averaged over reps evaluations after one untimed one;
- * info = {n, Jacobian nnz}.  Measurement only: the state is not changed. */
+ * info = {n, Jacobian nnz, 1 when the recorded residual program ran as one fused kernel
+ * (fuse_residual=1, the default) or 0 when it ran op by op}.  Measurement only: the state
+ * is not changed. */
 int eqhip_heat_eval_time(void* sim, double dt, int reps, double* ms_per_eval, int* info);
 int eqhip_heat_destroy(void* sim);
+
+/* ------------------------------------------------------------------ recorded residuals */
+/* No counterpart in the reference (which evaluates the residual functor op by op,
+ * EquelleRuntimeCUDA_impl.hpp:216-300).  Counters of the recorded residual programs of
+ * this process: out = {recording evaluations, symbolic recordings that found their kernel,
+ * kernels built, fused evaluations, recordings stopped by an unsupported operation,
+ * kernels rejected by the check against the recorded evaluation}; seconds = total kernel
+ * build time. */
+int eqhip_residual_stats(long* out, double* seconds);
+/* Builds the fused kernel of a small heat-like residual program with the run-time compiler
+ * for gfx950 (no device needed); 0 on success, the compiler log (truncated) in log. */
+int eqhip_residual_codegen_selftest(char* log, int len);
 
 /* ------------------------------------------------------------------ two-phase sessions */
 /* The two-phase program of examples/dsl/twophase.equelle (examples/app/out_twophase.cpp)
