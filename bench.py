@@ -157,6 +157,7 @@ def run_generic(lib, u0, nx, ny, dt, steps, warmup, precision="fp64", extra="", 
             ms, info = C.c_double(), np.zeros(3, dtype=np.int32)
             lib.heat_eval_time(sim, dt, eval_reps, C.byref(ms), iptr(info))
             assembly = assembly_roofline(nx, ny, int(info[1]), ms.value, eval_reps)
+            assembly["evaluation"] = "one fused kernel (recorded residual program)" if info[2] else "op by op"
     finally:
         lib.heat_destroy(sim)
     return dict(seconds=secs, newton=newton, krylov=krylov, u=u, first_step_s=first, assembly=assembly)
@@ -186,18 +187,23 @@ def bench_generic(lib, args):
     legs = {}
     ref = None
     assembly = None
+    assembly_op = None
     for name, prec, extra in (("fp64_stencil", "fp64", ""), ("mixed_stencil", "mixed", "inner_tol=1e-6"),
+                              ("fp64_stencil_op_by_op", "fp64", "fuse_residual=0"),
                               ("fp64_csr", "fp64", "stencil_solver=0"),
                               ("mixed_csr", "mixed", "stencil_solver=0 inner_tol=1e-6")):
-        if args.quick and name.endswith("_csr"):
+        if args.quick and (name.endswith("_csr") or name.endswith("op_by_op")):
             continue
         log(f"[bench] generic drop-in path {name}")
         g = run_generic(lib, u0, nx, ny, NONLINEAR_HEAT["dt"], args.steps, args.warmup, prec, extra,
-                        eval_reps=10 if name == "fp64_stencil" else 0)
+                        eval_reps=10 if name in ("fp64_stencil", "fp64_stencil_op_by_op") else 0)
         if ref is None:
             ref = g["u"]
         if g["assembly"] is not None:
-            assembly = g["assembly"]
+            if name == "fp64_stencil":
+                assembly = g["assembly"]
+            else:
+                assembly_op = g["assembly"]
         legs[name] = {"value": n * g["newton"] / g["seconds"], "ms_per_step": 1e3 * g["seconds"] / args.steps,
                       "newton_iterations": g["newton"], "krylov_iterations": g["krylov"],
                       "setup_and_first_step_s": g["first_step_s"],
@@ -208,7 +214,7 @@ def bench_generic(lib, args):
            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
            "config": {"workload": f"nonlinear heat {nx}x{ny} through the drop-in EquelleRuntimeCUDA AD engine, "
                                   "BiCGStab + ILU(0), Newton to 1e-8 relative residual", "cells": n},
-           "legs": legs, "assembly_roofline": assembly}
+           "legs": legs, "assembly_roofline": assembly, "assembly_roofline_op_by_op": assembly_op}
     print(json.dumps(out), flush=True)
 
 

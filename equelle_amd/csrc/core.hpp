@@ -188,6 +188,9 @@ struct ScalarData {
     // A value made by a symbolic recording pass has no device values (val is null).
     mutable uint64_t tsess = 0;
     mutable int tnode = -1;
+    // every value equals uvalue (a collection made from one Scalar)
+    bool uniform = false;
+    double uvalue = 0.0;
     bool jpending() const { return jscale != 1.0 || (bool)jrow; }
     int jcols() const { return pat ? pat->cols : 0; }
 };

@@ -294,7 +294,12 @@ SP s_from_host(const double* h, int n) { return make_scalar(n, dev_from_host(h, 
 SP s_uniform(int n, double v) {
     auto d = dev_alloc<double>(n);
     launch_fill(d->p, v, n);
-    return make_scalar(n, d, false, nullptr, nullptr);
+    auto s = std::make_shared<ScalarData>();
+    s->n = n;
+    s->val = d;
+    s->uniform = true;
+    s->uvalue = v;
+    return s;
 }
 
 SP s_from_device(DevPtr<double> v, int n) { return make_scalar(n, std::move(v), false, nullptr, nullptr); }
@@ -690,7 +695,9 @@ SP s_extend(const SP& a, const IP& from, const IP& to) {
     PMP pm = position_map(to, from);
     const int m = to->size();
     if (Tape* tp = tape_for(*a))
-        EQ_TAPE_RETURN(tp, tp->node(T_EXT, tp->operand(a), -1, -1, tp->map(pm->d_inv, pm->id ^ 0x8000000000000000ull, pm)),
+        EQ_TAPE_RETURN(tp,
+                       tp->node(T_EXT, tp->operand(a), -1, -1, tp->map(pm->d_inv, pm->id ^ 0x8000000000000000ull, pm),
+                                a->n == 0 ? 1 : 0),
                        m, a->ad, s_extend(a, from, to));
     auto v = dev_alloc<double>(m);
     launch_gather(a->val->p, pm->d_inv->p, v->p, m);

@@ -187,6 +187,7 @@ CollOfScalar::CollOfScalar(const CudaArray& val, const CudaMatrix& der) {
     auto s = std::make_shared<eqhip::ScalarData>(*val.data());
     s->tsess = 0;
     s->tnode = -1;
+    s->uniform = false;
     s->ad = true;
     s->jscale = 1.0;
     s->jrow = nullptr;

@@ -77,6 +77,12 @@ int Tape::operand(const SP& s) {
     if (s->pat) unsupported("a collection with its own Jacobian");
     TapeNode nd;
     nd.op = T_LEAF;
+    if (s->uniform) {   // one value: a constant of the kernel, not an array it reads
+        nd.sub = 1;
+        nd.res = constant(s->uvalue);
+        keep.push_back(s);
+        return add(nd);
+    }
     nd.res = (int)leaves.size();
     leaves.push_back(s);
     return add(nd);
@@ -111,10 +117,46 @@ int Tape::map(const DevPtr<int>& m, uint64_t id, std::shared_ptr<const void> k) 
     return (int)maps.size() - 1;
 }
 
+namespace {
+// Longest row and signed-column array of a matrix, cached by pattern (and sign) id.
+struct MatInfo {
+    int maxrow = 0;
+    bool regular = false;
+    DevPtr<int> sci;
+};
+MatInfo matrix_info(const DevCsr& m) {
+    static std::mutex mu;
+    static std::map<std::pair<uint64_t, uint64_t>, MatInfo> cache;
+    const auto key = std::make_pair(m.pat->id, m.sign ? m.sign_id : 0);
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = cache.find(key);
+        if (it != cache.end()) return it->second;
+    }
+    MatInfo mi;
+    const std::vector<int>& rp = m.pat->rp();
+    for (int r = 0; r < m.pat->rows; ++r) mi.maxrow = std::max(mi.maxrow, rp[r + 1] - rp[r]);
+    mi.regular = (long)mi.maxrow * m.pat->rows == m.pat->nnz;
+    if (m.sign && m.pat->nnz > 0) {
+        const std::vector<int>& ci = m.pat->ci();
+        std::vector<int> sci(ci.size());
+        for (size_t k = 0; k < ci.size(); ++k) sci[k] = (*m.sign)[k] > 0 ? ci[k] : ~ci[k];
+        mi.sci = dev_from_host(sci.data(), sci.size());
+    }
+    std::lock_guard<std::mutex> lk(mu);
+    cache[key] = mi;
+    return mi;
+}
+}  // namespace
+
 int Tape::matrix(const DevCsr& m) {
     for (size_t i = 0; i < mats.size(); ++i)
         if (mats[i].pat.get() == m.pat.get() && mats[i].val.get() == m.val.get()) return (int)i;
+    const MatInfo mi = matrix_info(m);
     mats.push_back(m);
+    mat_maxrow.push_back(mi.maxrow);
+    mat_regular.push_back(mi.regular);
+    mat_sci.push_back(mi.sci);
     return (int)mats.size() - 1;
 }
 
@@ -184,6 +226,10 @@ std::string structure_key(const Tape& t) {
     std::ostringstream o;
     o << "L" << t.leaves.size() << "B" << t.bleaves.size() << "M" << t.maps.size() << "X" << t.mats.size() << "C"
       << t.consts.size() << ":";
+    for (size_t i = 0; i < t.mats.size(); ++i)
+        o << (t.mats[i].pat->nnz > 0 ? t.mat_maxrow[i] : 0) << (t.mat_sci[i] ? "s" : "v")
+          << (t.mat_regular[i] ? "r" : "") << ",";
+    o << ":";
     for (const TapeNode& n : t.nodes)
         o << (int)n.op << "," << (int)n.sub << "," << n.a << "," << n.b << "," << n.c << "," << n.res << ","
           << (int)n.ad << (int)n.boolean << ";";
@@ -312,13 +358,21 @@ public:
         };
         switch (n.op) {
             case T_PRIM:
-                if (T == "D")
-                    line("const D " + v + " = prim(PD(" + w(W_U) + ")[" + e + "], " + e + ", col);" +
-                         " miss |= !seen(" + e + ", col);");
+                if (T == "D") {
+                    // an unknown outside the row's pattern on a path that contributes
+                    const std::string p = pred(e);
+                    line("const D " + v + " = prim(PD(" + w(W_U) + ")[" + e + "], " + e + ", col); miss |= " +
+                         (p.empty() ? "" : p + " && ") + "!seen(" + e + ", col);");
+                }
                 else
                     line("const double " + v + " = PD(" + w(W_U) + ")[" + e + "];");
                 break;
-            case T_LEAF: line("const double " + v + " = PD(" + w(L_.leaf0 + n.res) + ")[" + e + "];"); break;
+            case T_LEAF:
+                if (n.sub & 1)
+                    line("const double " + v + " = CD(" + w(L_.const0 + n.res) + ");");
+                else
+                    line("const double " + v + " = PD(" + w(L_.leaf0 + n.res) + ")[" + e + "];");
+                break;
             case T_BLEAF: line("const bool " + v + " = PB(" + w(L_.bleaf0 + n.res) + ")[" + e + "] != 0;"); break;
             case T_ADD: bin("+"); break;
             case T_SUB: bin("-"); break;
@@ -371,34 +425,81 @@ public:
             case T_ON: {
                 const std::string ix = "e" + std::to_string(tmp_++);
                 line("const int " + ix + " = PI(" + w(L_.map0 + n.res) + ")[" + e + "];");
+                guard(ix, e, "");
                 const std::string a = opnd(n.a, ix);
                 line("const " + T + " " + v + " = " + a + ";");
                 break;
             }
             case T_EXT: {
-                const std::string ix = "e" + std::to_string(tmp_++);
-                line("const int " + ix + " = PI(" + w(L_.map0 + n.res) + ")[" + e + "];");
-                line(T + " " + v + " = zero<" + T + ">();");
-                line("if (" + ix + " >= 0) {");
-                push();
+                if (n.sub & 1) {   // empty operand: every row is a zero row
+                    line("const " + T + " " + v + " = zero<" + T + ">();");
+                    break;
+                }
+                // straight-line: rows outside the subset evaluate the operand at its element 0
+                // and select zero, so the loads of all branches can be in flight together
+                const std::string raw = "e" + std::to_string(tmp_++), ix = "e" + std::to_string(tmp_++);
+                line("const int " + raw + " = PI(" + w(L_.map0 + n.res) + ")[" + e + "];");
+                line("const int " + ix + " = " + raw + " >= 0 ? " + raw + " : 0;");
+                guard(ix, e, raw + " >= 0");
                 const std::string a = opnd(n.a, ix);
-                line(v + " = cv<" + T + ">(" + a + ");");
-                pop();
-                line("}");
+                line("const " + T + " " + v + " = " + raw + " >= 0 ? cv<" + T + ">(" + a + ") : zero<" + T + ">();");
                 break;
             }
             case T_MAT: {
                 const int m = L_.mat0 + 3 * n.res;
-                const std::string q = "q" + std::to_string(tmp_++), ix = "e" + std::to_string(tmp_++);
+                const int R = t_.mats[n.res].pat->nnz > 0 ? t_.mat_maxrow[n.res] : 0;
+                const bool sgn = (bool)t_.mat_sci[n.res];
+                const bool reg = t_.mat_regular[n.res] && R <= 8;
                 line(T + " " + v + " = zero<" + T + ">();");
-                line("for (int " + q + " = PI(" + w(m) + ")[" + e + "]; " + q + " < PI(" + w(m) + ")[" + e + " + 1]; ++" +
-                     q + ") {");
-                push();
-                line("const int " + ix + " = PI(" + w(m + 1) + ")[" + q + "];");
-                const std::string a = opnd(n.a, ix);
-                line(v + " = " + v + " + PD(" + w(m + 2) + ")[" + q + "] * " + a + ";");
-                pop();
-                line("}");
+                if (R == 0) break;
+                const std::string qb = "q" + std::to_string(tmp_++), qe = "q" + std::to_string(tmp_++);
+                if (reg)   // every row has R entries: row e starts at R e
+                    line("const int " + qb + " = " + std::to_string(R) + " * " + e + ";");
+                else
+                    line("const int " + qb + " = PI(" + w(m) + ")[" + e + "], " + qe + " = PI(" + w(m) + ")[" + e +
+                         " + 1];");
+                // one entry: its column (sign folded in for a +-1 matrix) and contribution
+                auto entry = [&](const std::string& q, const std::string& on) {
+                    const std::string ix = "e" + std::to_string(tmp_++);
+                    if (sgn) {
+                        const std::string sc = "e" + std::to_string(tmp_++);
+                        line("const int " + sc + " = PI(" + w(m + 1) + ")[" + q + "];");
+                        line("const int " + ix + " = " + sc + " >= 0 ? " + sc + " : ~" + sc + ";");
+                        guard(ix, e, on);
+                        const std::string a = opnd(n.a, ix);
+                        const std::string add = "(" + sc + " >= 0 ? " + v + " + " + a + " : " + v + " - " + a + ")";
+                        line(v + " = " + (on.empty() ? add : on + " ? " + add + " : " + v) + ";");
+                    } else {
+                        line("const int " + ix + " = PI(" + w(m + 1) + ")[" + q + "];");
+                        guard(ix, e, on);
+                        const std::string a = opnd(n.a, ix);
+                        const std::string add = v + " + PD(" + w(m + 2) + ")[" + q + "] * " + a;
+                        line(v + " = " + (on.empty() ? add : on + " ? " + add + " : " + v) + ";");
+                    }
+                };
+                if (R <= 8) {
+                    // short rows: unrolled, every entry's loads issued together (entries past
+                    // the row's end read entry 0 and are not added)
+                    for (int j = 0; j < R; ++j) {
+                        const std::string q = "q" + std::to_string(tmp_++);
+                        if (reg) {
+                            line("const int " + q + " = " + qb + " + " + std::to_string(j) + ";");
+                            entry(q, "");
+                            continue;
+                        }
+                        const std::string on = "o" + std::to_string(tmp_++);
+                        line("const bool " + on + " = " + qb + " + " + std::to_string(j) + " < " + qe + ";");
+                        line("const int " + q + " = " + on + " ? " + qb + " + " + std::to_string(j) + " : 0;");
+                        entry(q, on);
+                    }
+                } else {
+                    const std::string q = "q" + std::to_string(tmp_++);
+                    line("for (int " + q + " = " + qb + "; " + q + " < " + qe + "; ++" + q + ") {");
+                    push();
+                    entry(q, "");
+                    pop();
+                    line("}");
+                }
                 break;
             }
             default: throw_error("residual lowering: unknown tape operation");
@@ -411,6 +512,23 @@ public:
 
 private:
     static std::string w(int i) { return std::to_string(i); }
+    // Whether element variable `e` is on a contributing path: "" (always), or a bool variable.
+    std::string pred(const std::string& e) const {
+        auto it = pred_.find(e);
+        return it == pred_.end() ? std::string() : it->second;
+    }
+    // `ix` is reached from `from` when `cond` holds ("" : always)
+    void guard(const std::string& ix, const std::string& from, const std::string& cond) {
+        const std::string p = pred(from);
+        if (cond.empty() && p.empty()) return;
+        if (cond.empty()) {
+            pred_[ix] = p;
+            return;
+        }
+        const std::string b = "p" + std::to_string(tmp_++);
+        line("const bool " + b + " = " + (p.empty() ? "" : p + " && ") + "(" + cond + ");");
+        pred_[ix] = b;
+    }
     void line(const std::string& s) { out_ << std::string(2 * (scopes_.size() + 1), ' ') << s << "\n"; }
     void push() { scopes_.emplace_back(); }
     void pop() { scopes_.pop_back(); }
@@ -419,6 +537,7 @@ private:
     const Layout& L_;
     bool jac_;
     std::vector<char> need_;
+    std::map<std::string, std::string> pred_;
     int tmp_ = 0;
     std::vector<std::map<std::pair<int, std::string>, std::string>> scopes_;
     std::ostringstream out_;
@@ -585,6 +704,14 @@ std::string residual_codegen_sample(int K, bool jac) {
     };
     t.maps.resize(3);
     t.mats.resize(2);
+    t.mat_maxrow = {2, 4};
+    t.mat_regular = {1, 0};
+    t.mat_sci.resize(2);
+    for (DevCsr& m : t.mats) {
+        auto p = std::make_shared<Pattern>();
+        p->nnz = 1;
+        m.pat = p;
+    }
     t.consts = {0.1, 0.5};
     const int one = leaf(0), u0 = leaf(1), coef = leaf(2), trans = leaf(3);
     const int uu = node(T_MUL, 0, 0);
@@ -637,7 +764,8 @@ struct ResidualSession::Impl {
         for (size_t i = 0; i < t.maps.size(); ++i) words[L.map0 + i] = (uint64_t)(uintptr_t)t.maps[i].first->p;
         for (size_t i = 0; i < t.mats.size(); ++i) {
             words[L.mat0 + 3 * i] = (uint64_t)(uintptr_t)t.mats[i].pat->d_rp->p;
-            words[L.mat0 + 3 * i + 1] = (uint64_t)(uintptr_t)t.mats[i].pat->d_ci->p;
+            words[L.mat0 + 3 * i + 1] =
+                (uint64_t)(uintptr_t)(t.mat_sci[i] ? t.mat_sci[i]->p : t.mats[i].pat->d_ci->p);
             words[L.mat0 + 3 * i + 2] = (uint64_t)(uintptr_t)t.mats[i].val->p;
         }
         for (size_t i = 0; i < t.consts.size(); ++i) std::memcpy(&words[L.const0 + i], &t.consts[i], 8);

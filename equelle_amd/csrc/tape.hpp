@@ -61,7 +61,9 @@ enum TapeOp : uint8_t {
 
 struct TapeNode {
     uint8_t op = 0;
-    int8_t sub = 0;       // comparison code; T_CMPS: 1 when the constant is on the left
+    int8_t sub = 0;       // comparison code (T_CMPS: +16 when the constant is on the left);
+                          // T_EXT: 1 when the operand is empty (the result is zero);
+                          // T_LEAF: 1 for a uniform collection (res: its constant slot)
     int a = -1, b = -1, c = -1;
     int res = -1;         // leaf, map, matrix or constant slot
     bool ad = false;      // carries derivatives
@@ -114,6 +116,11 @@ public:
     std::vector<BP> bleaves;
     std::vector<std::pair<DevPtr<int>, uint64_t>> maps;
     std::vector<DevCsr> mats;
+    // per matrix: longest row, and (for a matrix of +-1 values) each entry's column with
+    // its sign folded in (~column where the value is -1)
+    std::vector<int> mat_maxrow;
+    std::vector<char> mat_regular;   // every row has mat_maxrow entries (rows need no row pointers)
+    std::vector<DevPtr<int>> mat_sci;
     std::vector<double> consts;
     std::vector<std::shared_ptr<const void>> keep;
     int n_primary = 0;
