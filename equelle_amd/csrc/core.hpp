@@ -188,6 +188,9 @@ struct ScalarData {
     // A value made by a symbolic recording pass has no device values (val is null).
     mutable uint64_t tsess = 0;
     mutable int tnode = -1;
+    // nonzero: the Jacobian's values were written into the stencil linear solver's layout by
+    // a fused residual evaluation (this token names it), not into jval
+    uint64_t jsink = 0;
     // every value equals uvalue (a collection made from one Scalar)
     bool uniform = false;
     double uvalue = 0.0;

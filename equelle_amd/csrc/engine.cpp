@@ -605,6 +605,8 @@ static JacOut gather_rows(const SP& a, const SymP& e, const int* rowmap, int m) 
 DevPtr<double> jac_effective(const ScalarData& a) {
     tape_read(a, "a Jacobian read");
     if (!a.pat) return nullptr;
+    EQ_REQUIRE(!a.jsink, "this residual's Jacobian was written into the stencil linear solver's layout "
+                         "(a fused newtonSolve evaluation); it has no CSR values");
     if (!a.jpending() && !a.jmap) return a.jval;
     if (!a.jeff) {
         auto v = dev_alloc<double>(std::max(1, a.pat->nnz));

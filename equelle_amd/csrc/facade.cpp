@@ -316,7 +316,11 @@ CollOfVector operator/(const CollOfVector& v, const CollOfScalar& s) {
 
 // ---------------------------------------------------------------- fused residual
 FusedResidual::FusedResidual(const EquelleRuntimeCUDA& er, int* site_state, const CollOfScalar& u)
-    : s_(new eqhip::ResidualSession(site_state, u.data(), er.core().fuse_residual)) {}
+    : s_(new eqhip::ResidualSession(site_state, u.data(), er.core().fuse_residual)) {
+    // a Jacobian solved on the stencil kernels is written there by the fused evaluation
+    const eqhip::RuntimeCore* core = &er.core();
+    s_->set_sink([core](const eqhip::PatternP& P, eqhip::StencilSink* out) { return core->stencil_sink(P, out); });
+}
 FusedResidual::~FusedResidual() {}
 void FusedResidual::begin(int pass) { s_->begin(pass); }
 bool FusedResidual::finish(const CollOfScalar& residual, CollOfScalar* out) {

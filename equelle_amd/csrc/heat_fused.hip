@@ -729,6 +729,21 @@ __global__ __launch_bounds__(BS) void k_gather_stencil(const Desc* __restrict__ 
     group_store<CPT>(f2, PARTS(part));
 }
 
+// Linear-system instances whose right-hand side and coefficients were written in place (the
+// drop-in path's fused residual kernel): per-group partials of ||F||^2, as the gather stores them.
+template <int CPT>
+__global__ __launch_bounds__(BS) void k_linear_fnorm(const Desc* __restrict__ D, const int2* __restrict__ blk,
+                                                     double* __restrict__ part) {
+    LOCATE();
+    double f2[CPT] = {};
+    FOR_GROUPS(m, c) {
+        if (c >= d.n) continue;
+        const double F = d.F[c];
+        f2[m] = F * F;
+    }
+    group_store<CPT>(f2, PARTS(part));
+}
+
 // FP64 defect for the instances under refinement: r64 = F - J du.  Heat instances
 // recompute their FP64 Jacobian rows from u (heat_row; reads u, du 5-point and F, writes
 // r64: 32 B per cell); two-phase instances read their stored FP64 coefficients.
@@ -2963,6 +2978,12 @@ struct HeatBatch::Impl {
         fin(t, Fcell(S_FN));
         for (const Inst* I : t.inst) launch_fill(I->sc->p + S_CONT, 0.0, 1);
     }
+    // The same with F and the coefficients already in place: ||F||^2 only.
+    void prepared_linear(const Table& t) {
+        HL(C_ASM, (k_linear_fnorm<CP>), t.part->p);
+        fin(t, Fcell(S_FN));
+        for (const Inst* I : t.inst) launch_fill(I->sc->p + S_CONT, 0.0, 1);
+    }
     // J du = F for linear-system instances: one solve in FP64, or FP32 solves with FP64
     // refinement rounds (one per linear_solve call) until every instance's refinement has
     // stopped; du holds the solution.
@@ -3241,15 +3262,47 @@ int HeatBatch::add_linear(int nx, int ny, int nz) {
     return id;
 }
 
+HeatBatch::LinearTargets HeatBatch::linear_targets(int slot) {
+    Inst& I = get_inst(p_, slot);
+    EQ_REQUIRE(I.kind == K_LINEAR, "slot " << slot << " is not a linear-system instance");
+    LinearTargets L;
+    L.F = I.F->p;
+    const size_t ks = p_->ks(), h = I.h();
+    for (int q = 0; q < I.ncoef(); ++q) L.K[q] = I.kcoef->p + q * 2 * h * ks;
+    if (p_->mixed) {
+        const DevPtr<double>* c64[7] = {&I.C, &I.W, &I.E, &I.S, &I.N, &I.B, &I.T};
+        for (int q = 0; q < I.ncoef(); ++q) L.C64[q] = (*c64[q])->p;
+    }
+    L.nx = I.nx;
+    L.ny = I.ny;
+    L.nz = I.nz;
+    L.h = (int)h;
+    L.ncoef = I.ncoef();
+    L.fp32 = p_->mixed;
+    return L;
+}
+
 LinearSolveStats HeatBatch::solve_linear(int slot, const int* rp, const int* ci, const double* val, const double* rhs,
                                          double* x) {
+    return solve_linear_impl(slot, rp, ci, val, rhs, x);
+}
+
+LinearSolveStats HeatBatch::solve_linear_prepared(int slot, double* x) {
+    return solve_linear_impl(slot, nullptr, nullptr, nullptr, nullptr, x);
+}
+
+LinearSolveStats HeatBatch::solve_linear_impl(int slot, const int* rp, const int* ci, const double* val,
+                                              const double* rhs, double* x) {
     Impl& P = *p_;
     P.flush_copies();
     Inst& I = get_inst(p_, slot);
     EQ_REQUIRE(I.kind == K_LINEAR, "slot " << slot << " is not a linear-system instance");
     TableP tp = P.table({&I});
     Table& t = *tp;
-    P.gather_linear(t, rp, ci, val, rhs);
+    if (rp)
+        P.gather_linear(t, rp, ci, val, rhs);
+    else
+        P.prepared_linear(t);
     P.solve_linear_table(t);
     // S_ITT: Krylov iterations of the solve; S_REF: refinement rounds; S_RN / S_BN: FP64
     // defect over ||F|| (mixed); S_RR / S_BB: the Krylov residual over its right-hand side

@@ -141,6 +141,22 @@ public:
     int add_linear(int nx, int ny, int nz);
     LinearSolveStats solve_linear(int slot, const int* rp, const int* ci, const double* val, const double* b,
                                   double* x);
+    // Where a caller writes a linear-system instance's right-hand side and stencil
+    // coefficients in place, instead of solve_linear's gather from CSR: F (cell order), the
+    // Krylov-precision coefficients K[q] (q = C, W, E, S, N, B, T; red-black rank order: cell
+    // c = (i, j, k) at ((i + j + k) & 1) h + c / 2; float when fp32, else double) and, in
+    // mixed precision, the FP64 coefficients in cell order (C64[q], else null).  Absent
+    // neighbours take coefficient 0.  Valid while the slot lives.
+    struct LinearTargets {
+        double* F = nullptr;
+        void* K[7] = {};
+        double* C64[7] = {};
+        int nx = 0, ny = 0, nz = 1, h = 0, ncoef = 5;
+        bool fp32 = false;
+    };
+    LinearTargets linear_targets(int slot);
+    // solve_linear on coefficients and right-hand side already written at linear_targets
+    LinearSolveStats solve_linear_prepared(int slot, double* x);
     // Newton iterations allowed per solve for this slot (at most the batch's max_iter;
     // 0 restores the batch's limit).  A solve that reaches it stops with reason 0.
     void set_max_newton(int slot, int max_newton);
@@ -187,6 +203,8 @@ public:
     struct Impl;
 
 private:
+    LinearSolveStats solve_linear_impl(int slot, const int* rp, const int* ci, const double* val, const double* rhs,
+                                       double* x);
     Impl* p_;
 };
 

@@ -389,30 +389,63 @@ bool RuntimeCore::stencil_pattern(const PatternP& P) const {
     return ok;
 }
 
+bool RuntimeCore::stencil_solves(const PatternP& P) const {
+    return stencil_solver && solver.method == "bicgstab" && (solver.precond == "ilu0" || solver.precond == "jacobi") &&
+           P && stencil_pattern(P);
+}
+
+void RuntimeCore::ensure_stencil() const {
+    if (stencil_) return;
+    FusedConfig fc;
+    fc.precision = solver.precision;
+    fc.precond = solver.precond;
+    fc.solver_tol = solver.tol;
+    fc.solver_max_iter = solver.max_iter;
+    fc.inner_tol = solver.inner_tol;
+    fc.max_refine = solver.max_refine;
+    const GridHost& g = grid_->host;
+    stencil_.reset(new HeatBatch(fc));
+    stencil_slot_ = stencil_->add_linear(g.n[0], g.n[1], g.dim == 3 ? g.n[2] : 1);
+}
+
+bool RuntimeCore::stencil_sink(const PatternP& P, StencilSink* out) const {
+    if (!stencil_solves(P)) return false;
+    ensure_stencil();
+    const HeatBatch::LinearTargets L = stencil_->linear_targets(stencil_slot_);
+    out->F = L.F;
+    for (int q = 0; q < 7; ++q) {
+        out->K[q] = L.K[q];
+        out->C64[q] = L.C64[q];
+    }
+    out->nx = L.nx;
+    out->ny = L.ny;
+    out->nz = L.nz;
+    out->h = L.h;
+    out->ncoef = L.ncoef;
+    out->fp32 = L.fp32;
+    out->token = ++sink_token_;
+    return true;
+}
+
 SP RuntimeCore::solve_for_update(const SP& residual, LinReport* rep) const {
     tape_read(*residual, "a linear solve");
     // BiCGStab with red-black ILU(0) or Jacobi on a Jacobian with the grid's stencil
-    // pattern: the coefficients are gathered out of the CSR values and the solve runs on
-    // the stencil kernels (no column indices in the iteration); other systems take the
-    // CSR / SELL-64 path.
-    if (stencil_solver && solver.method == "bicgstab" && (solver.precond == "ilu0" || solver.precond == "jacobi") &&
-        residual->pat && stencil_pattern(residual->pat)) {
-        if (!stencil_) {
-            FusedConfig fc;
-            fc.precision = solver.precision;
-            fc.precond = solver.precond;
-            fc.solver_tol = solver.tol;
-            fc.solver_max_iter = solver.max_iter;
-            fc.inner_tol = solver.inner_tol;
-            fc.max_refine = solver.max_refine;
-            const GridHost& g = grid_->host;
-            stencil_.reset(new HeatBatch(fc));
-            stencil_slot_ = stencil_->add_linear(g.n[0], g.n[1], g.dim == 3 ? g.n[2] : 1);
-        }
-        const Pattern& P = *residual->pat;
+    // pattern: the coefficients are gathered out of the CSR values (or were written in place
+    // by a fused residual evaluation) and the solve runs on the stencil kernels (no column
+    // indices in the iteration); other systems take the CSR / SELL-64 path.
+    if (residual->jsink || stencil_solves(residual->pat)) {
+        ensure_stencil();
         auto x = dev_alloc<double>(std::max(1, residual->n));
-        const LinearSolveStats st = stencil_->solve_linear(stencil_slot_, P.d_rp->p, P.d_ci->p, jac_values(*residual),
-                                                           residual->val->p, x->p);
+        LinearSolveStats st;
+        if (residual->jsink) {
+            EQ_REQUIRE(residual->jsink == sink_token_,
+                       "the stencil solver's coefficients were overwritten after this residual was evaluated");
+            st = stencil_->solve_linear_prepared(stencil_slot_, x->p);
+        } else {
+            const Pattern& P = *residual->pat;
+            st = stencil_->solve_linear(stencil_slot_, P.d_rp->p, P.d_ci->p, jac_values(*residual), residual->val->p,
+                                        x->p);
+        }
         LinReport R;
         R.iterations = st.iterations;
         R.refinements = st.refinements;

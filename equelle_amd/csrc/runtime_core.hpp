@@ -66,6 +66,10 @@ public:
     SP interior_divergence(const SP& x) const;
     SP divergence_matrix(const SP& x) const;
     SP solve_for_update(const SP& residual, LinReport* rep) const;
+    // The stencil solver's in-place targets for a residual whose Jacobian has pattern P,
+    // when its Newton step is solved there (a fused evaluation then writes its Jacobian
+    // straight into them; StencilSink in tape.hpp).
+    bool stencil_sink(const PatternP& P, struct StencilSink* out) const;
     // Index set [start, start + len) of a combined unknown (newtonSolveSystem ranges);
     // cached, so the position maps of subset / superset are built once.
     IP range_set(int start, int len) const;
@@ -85,6 +89,9 @@ private:
     mutable std::map<uint64_t, bool> stencil_patterns_;   // pattern id -> grid stencil
     mutable std::unique_ptr<HeatBatch> stencil_;         // linear-system instance of the grid
     mutable int stencil_slot_ = 0;
+    mutable uint64_t sink_token_ = 0;                    // the last fused evaluation written in place
+    bool stencil_solves(const PatternP& P) const;        // the Newton step of pattern P is solved on the stencil kernels
+    void ensure_stencil() const;
     std::unique_ptr<DeviceGrid> grid_;
     mutable std::map<std::pair<int, int>, IP> ranges_;
     // firstCell / secondCell results per (face set, side).  Generated code calls them inside

@@ -207,7 +207,11 @@ namespace {
 
 // Word layout of the kernel argument block (one 8-byte word per pointer, integer or
 // constant); the generated code addresses the same words.
-enum : int { W_U = 0, W_RP, W_CI, W_OUTV, W_OUTJ, W_N, W_ERR, W_FIXED };
+enum : int {
+    W_U = 0, W_RP, W_CI, W_OUTV, W_OUTJ, W_N, W_ERR,
+    // stencil output (mode 1): the linear solver's right-hand side and coefficients in place
+    W_MODE, W_NX, W_NY, W_NZ, W_H, W_F32, W_SF, W_K0, W_C0 = W_K0 + 7, W_FIXED = W_C0 + 7
+};
 
 struct Layout {
     int leaf0, bleaf0, map0, mat0, const0, words;
@@ -547,30 +551,72 @@ std::string kernel_source(const Tape& t, int root, int K, bool jac, bool have_pa
     const Layout L(t);
     Emitter em(t, L, jac, root);
     const std::string rv = em.emit(root, "r");
+    const bool dres = jac && em.type(root) == "D";
+    // a 5- or 7-point stencil row can go straight into the stencil solver's layout
+    const bool stencil = jac && have_pat && (K == 5 || K == 7);
     std::ostringstream s;
     s << "#define NW_ " << L.words << "\n#define K_ " << K << "\n" << kPrelude;
     s << "extern \"C\" __global__ void __launch_bounds__(256) eqhip_residual(Args A) {\n";
     s << "  const int r = blockIdx.x * 256 + threadIdx.x;\n";
     s << "  if (r >= (int)A.w[" << W_N << "]) return;\n";
     if (jac) {
+        s << "  int col[K_];\n  int b0 = 0, len = 0;\n";
+        if (stencil) s << "  const int mode = (int)A.w[" << W_MODE << "];\n  int si = 0, sj = 0, sk = 0;\n";
+        s << "  " << (stencil ? "if (mode == 0) " : "") << "{\n";
         if (have_pat) {
-            s << "  const int b0 = PI(" << W_RP << ")[r], len = PI(" << W_RP << ")[r + 1] - b0;\n";
-            s << "  int col[K_];\n#pragma unroll\n  for (int s = 0; s < K_; ++s) col[s] = s < len ? PI(" << W_CI
+            s << "    b0 = PI(" << W_RP << ")[r];\n    len = PI(" << W_RP << ")[r + 1] - b0;\n";
+            s << "#pragma unroll\n    for (int s = 0; s < K_; ++s) col[s] = s < len ? PI(" << W_CI
               << ")[b0 + s] : -1;\n";
         } else {
-            s << "  int col[K_];\n#pragma unroll\n  for (int s = 0; s < K_; ++s) col[s] = -1;\n";
+            s << "#pragma unroll\n    for (int s = 0; s < K_; ++s) col[s] = -1;\n";
+        }
+        s << "  }\n";
+        if (stencil) {
+            // one slot per neighbour in ascending column order (B, S, W, C, E, N, T; 2D:
+            // S, W, C, E, N), -1 where the neighbour is outside the grid
+            s << "  else {\n";
+            s << "    const int nx = (int)A.w[" << W_NX << "], ny = (int)A.w[" << W_NY << "], nz = (int)A.w[" << W_NZ
+              << "], nxy = nx * ny;\n";
+            s << "    si = r % nx; sj = (r / nx) % ny; sk = r / nxy;\n";
+            if (K == 7)
+                s << "    col[0] = sk > 0 ? r - nxy : -1; col[1] = sj > 0 ? r - nx : -1; col[2] = si > 0 ? r - 1 : -1;"
+                     " col[3] = r; col[4] = si < nx - 1 ? r + 1 : -1; col[5] = sj < ny - 1 ? r + nx : -1;"
+                     " col[6] = sk < nz - 1 ? r + nxy : -1;\n";
+            else
+                s << "    col[0] = sj > 0 ? r - nx : -1; col[1] = si > 0 ? r - 1 : -1; col[2] = r;"
+                     " col[3] = si < nx - 1 ? r + 1 : -1; col[4] = sj < ny - 1 ? r + nx : -1;\n";
+            s << "  }\n";
         }
         s << "  bool miss = false;\n";
     }
     s << em.body();
     s << "  ((double*)A.w[" << W_OUTV << "])[r] = V(" << rv << ");\n";
     if (jac) {
-        if (have_pat && em.type(root) == "D") {
-            s << "  double* J = (double*)A.w[" << W_OUTJ << "] + b0;\n";
-            s << "#pragma unroll\n  for (int s = 0; s < K_; ++s) if (s < len) J[s] = " << rv << ".d[s];\n";
+        if (stencil) s << "  if (mode == 0) {\n";
+        if (have_pat && dres) {
+            s << "    double* J = (double*)A.w[" << W_OUTJ << "] + b0;\n";
+            s << "#pragma unroll\n    for (int s = 0; s < K_; ++s) if (s < len) J[s] = " << rv << ".d[s];\n";
         } else if (have_pat) {
-            s << "  double* J = (double*)A.w[" << W_OUTJ << "] + b0;\n";
-            s << "  for (int s = 0; s < len; ++s) J[s] = 0.0;\n";
+            s << "    double* J = (double*)A.w[" << W_OUTJ << "] + b0;\n";
+            s << "    for (int s = 0; s < len; ++s) J[s] = 0.0;\n";
+        }
+        if (stencil) {
+            // the stencil solver's right-hand side, its coefficients (q = C, W, E, S, N, B, T)
+            // in red-black rank order and, in mixed precision, FP64 in cell order
+            s << "  } else {\n";
+            s << "    ((double*)A.w[" << W_SF << "])[r] = V(" << rv << ");\n";
+            s << "    const int sl = ((si + sj + sk) & 1) * (int)A.w[" << W_H << "] + (r >> 1);\n";
+            s << "    const bool f32 = A.w[" << W_F32 << "] != 0;\n";
+            static const int cls7[7] = {5, 3, 1, 0, 2, 4, 6}, cls5[5] = {3, 1, 0, 2, 4};
+            for (int q = 0; q < K; ++q) {
+                const int c = K == 7 ? cls7[q] : cls5[q];
+                const std::string v = dres ? rv + ".d[" + std::to_string(q) + "]" : std::string("0.0");
+                s << "    { const double v = " << v << ";\n";
+                s << "      if (f32) ((float*)A.w[" << W_K0 + c << "])[sl] = (float)v; else ((double*)A.w["
+                  << W_K0 + c << "])[sl] = v;\n";
+                s << "      if (A.w[" << W_C0 + c << "]) ((double*)A.w[" << W_C0 + c << "])[r] = v; }\n";
+            }
+            s << "  }\n";
         }
         // an unknown outside the row's pattern would lose its derivative: flag it
         s << "  if (miss) ((int*)A.w[" << W_ERR << "])[0] = 1;\n";
@@ -744,6 +790,7 @@ struct ResidualSession::Impl {
     RootInfo root;
     std::vector<uint64_t> words;
     DevPtr<int> err;
+    SinkFn sink;
 
     void stop() {
         if (g_tape == tape.get()) g_tape = nullptr;
@@ -771,14 +818,31 @@ struct ResidualSession::Impl {
         for (size_t i = 0; i < t.consts.size(); ++i) std::memcpy(&words[L.const0 + i], &t.consts[i], 8);
     }
 
-    SP run(const SP& x, bool jac) {
+    // `sink`: the Jacobian may go straight into the stencil solver's layout
+    SP run(const SP& x, bool jac, bool sink_ok = true) {
         EQ_REQUIRE(x->n == bound->n_primary, "fused residual: unknown of size " << x->n << ", recorded for "
                                                                                  << bound->n_primary);
         const Kernel& k = jac ? *kj : *kv;
         auto v = dev_alloc<double>(std::max(1, root.n));
-        DevPtr<double> j;
-        if (jac && root.pat) j = dev_alloc<double>(std::max(1, root.pat->nnz));
         std::vector<uint64_t> a = words;
+        StencilSink st;
+        const bool to_sink = jac && sink_ok && sink && root.pat && (root.K == 5 || root.K == 7) && sink(root.pat, &st) &&
+                             st.ncoef == root.K && (long)st.nx * st.ny * st.nz == root.n;
+        DevPtr<double> j;
+        if (jac && root.pat && !to_sink) j = dev_alloc<double>(std::max(1, root.pat->nnz));
+        if (to_sink) {
+            a[W_MODE] = 1;
+            a[W_NX] = (uint64_t)st.nx;
+            a[W_NY] = (uint64_t)st.ny;
+            a[W_NZ] = (uint64_t)st.nz;
+            a[W_H] = (uint64_t)st.h;
+            a[W_F32] = st.fp32 ? 1 : 0;
+            a[W_SF] = (uint64_t)(uintptr_t)st.F;
+            for (int q = 0; q < 7; ++q) {
+                a[W_K0 + q] = (uint64_t)(uintptr_t)st.K[q];
+                a[W_C0 + q] = (uint64_t)(uintptr_t)st.C64[q];
+            }
+        }
         a[W_U] = (uint64_t)(uintptr_t)x->val->p;
         a[W_OUTV] = (uint64_t)(uintptr_t)v->p;
         a[W_OUTJ] = j ? (uint64_t)(uintptr_t)j->p : 0;
@@ -797,6 +861,7 @@ struct ResidualSession::Impl {
             s->ad = root.ad;
             s->pat = root.pat;
             s->jval = j;
+            s->jsink = to_sink ? st.token : 0;
         }
         return s;
     }
@@ -804,7 +869,7 @@ struct ResidualSession::Impl {
     // The kernels against the recorded evaluation (values, Jacobian, unknowns in the pattern).
     bool verify(const SP& eager) {
         launch_fill_int(err->p, 0, 1);
-        const SP a = run(u, true), b = run(u, false);
+        const SP a = run(u, true, false), b = run(u, false);
         const std::vector<double> ve = s_to_host(eager), va = s_to_host(a), vb = s_to_host(b);
         double w1 = 0, w2 = 0, w3 = 0;
         bool ok = close(ve, va, 1e-10, 1e-13, &w1) && close(ve, vb, 1e-10, 1e-13, &w2);
@@ -852,6 +917,8 @@ void ResidualSession::abort() {
 }
 
 bool ResidualSession::fused() const { return impl_->bound && impl_->kj && impl_->kv; }
+
+void ResidualSession::set_sink(SinkFn f) { impl_->sink = std::move(f); }
 
 SP ResidualSession::eval(const SP& u, bool jacobian) { return impl_->run(u, jacobian); }
 

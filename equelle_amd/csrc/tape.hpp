@@ -22,6 +22,7 @@
 //   value that depends on the unknown, user-supplied Jacobians, vectors scaled by it)
 //   stop the recording, and that newtonSolve runs op by op.
 #pragma once
+#include <functional>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -177,6 +178,20 @@ struct ResidualStats {
 };
 ResidualStats residual_stats();
 
+// Where a fused evaluation may write its Jacobian instead of CSR values: the stencil linear
+// solver's right-hand side and coefficients in place (HeatBatch::linear_targets), for a
+// residual whose Jacobian has the grid's 5- or 7-point stencil pattern.
+struct StencilSink {
+    double* F = nullptr;
+    void* K[7] = {};
+    double* C64[7] = {};
+    int nx = 0, ny = 0, nz = 1, h = 0, ncoef = 5;
+    bool fp32 = false;
+    uint64_t token = 0;   // names the evaluation for the solve that reads it
+};
+// true (with the targets) when the Newton step of a residual with pattern P is solved there
+using SinkFn = std::function<bool(const PatternP& P, StencilSink* out)>;
+
 // One newtonSolve's residual program.
 class ResidualSession {
 public:
@@ -193,6 +208,8 @@ public:
     void abort();   // the pass threw TapeAbort
     static bool is_abort(const std::exception& e) { return dynamic_cast<const TapeAbort*>(&e) != nullptr; }
     bool fused() const;
+    // the linear solver's in-place targets for fused Jacobian evaluations
+    void set_sink(SinkFn f);
     // Residual (and, with `jacobian`, its Jacobian) at the values of `u`, by the kernel.
     SP eval(const SP& u, bool jacobian);
 
