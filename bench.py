@@ -54,6 +54,7 @@ from equelle_amd.runtime import load_library  # noqa: E402
 from equelle_amd.workloads import NONLINEAR_HEAT, gaussian_bumps, heat_params  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+FORCING = ""   # "forcing=<x>" from --forcing (fused runs)
 COPY_BYTES = 256 << 20   # the counter calibration copies 256 MiB device to device per launch
 
 
@@ -69,7 +70,8 @@ def timing_avg(lib, name):
 
 def run_fused(lib, precision, u0, nx, ny, dt, steps, warmup, precond="ilu0", graph=True, inner_tol=1e-6, extra=""):
     params = (f"precision={precision} preconditioner={precond} rel_res_tol={NONLINEAR_HEAT['rel_res_tol']} "
-              f"abs_res_tol=0 max_iter=20 solver_tol=1e-6 inner_tol={inner_tol} graph={int(graph)} {extra}").encode()
+              f"abs_res_tol=0 max_iter=20 solver_tol=1e-6 inner_tol={inner_tol} graph={int(graph)} {FORCING} "
+              f"{extra}").encode()
     b = C.c_void_p()
     lib.batch_create(params, C.byref(b))
     slot = C.c_int()
@@ -801,6 +803,8 @@ def main():
                     help="serving rates of the default run's compact summary")
     ap.add_argument("--default-requests", dest="default_requests", type=int, default=100)
     ap.add_argument("--precond", default="ilu0", help="preconditioner of the fused runs: ilu0 | diagonal | none")
+    ap.add_argument("--forcing", type=float, default=None,
+                    help="fused heat runs: inexact-Newton margin of the linear tolerance (0: solver_tol always)")
     ap.add_argument("--workload", default="heat2048",
                     choices=["heat2048", "batched", "serving", "twophase", "twophase_fi", "copy", "generic"])
     ap.add_argument("--instances", type=int, default=512)
@@ -819,6 +823,9 @@ def main():
     ap.add_argument("--max-in-flight-twophase", dest="max_in_flight_twophase", type=int, default=4096,
                     help="serving: two-phase requests in flight at once")
     args = ap.parse_args()
+    if args.forcing is not None:
+        global FORCING
+        FORCING = f"forcing={args.forcing}"
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)

@@ -767,6 +767,9 @@ static FusedConfig fused_config(const char* params) {
     c.solver_max_iter = m.get_int("solver_max_iter", c.solver_max_iter);
     c.inner_tol = m.get_double("inner_tol", c.inner_tol);
     c.inner_tol_twophase = m.get_double("inner_tol_twophase", c.inner_tol_twophase);
+    c.forcing = m.get_double("forcing", c.forcing);
+    c.eta_max = m.get_double("eta_max", c.eta_max);
+    EQ_REQUIRE(c.forcing >= 0.0 && c.eta_max > 0.0 && c.eta_max < 1.0, "forcing must be >= 0 and eta_max in (0, 1)");
     c.krylov_form_heat = m.get_string("krylov_form_heat", c.krylov_form_heat);
     c.krylov_form_twophase = m.get_string("krylov_form_twophase", c.krylov_form_twophase);
     c.pair_vectors = m.get_int("pair_vectors", c.pair_vectors ? 1 : 0) != 0;

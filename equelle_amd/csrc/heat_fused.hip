@@ -22,8 +22,11 @@ inline hipStream_t S() { return Device::get().stream(); }
 
 // per-instance scalars; S_TOL2: the square of the instance's inner (mixed-precision) Krylov tolerance
 // S_MEAN: the mean of the Krylov right-hand side over the cells (singular systems only)
+// S_F0: ||F||^2 at the first Newton iteration of the step (stage); 0 until then
+// S_LTOL: the linear tolerance of the current Newton iteration; S_ITOL: the square of the
+// instance's base inner (mixed-precision) tolerance
 enum Slot { S_RHO = 0, S_RHO_OLD, S_ALPHA, S_OMEGA, S_RR, S_BB, S_BREAK, S_IT, S_FN, S_RN, S_BN, S_REF, S_ITT, S_TOL2,
-            S_MEAN, S_BETA, S_CONT, NSLOT = 20 };
+            S_MEAN, S_BETA, S_CONT, S_F0, S_LTOL, S_ITOL, NSLOT = 20 };
 // Krylov vectors; V_Y is the red-rank scratch of the reduced operator (ilu0 only), V_Z the
 // V-cycle's z = M^-1 r (two-phase pressure by CG with multigrid)
 enum Vec { V_R = 0, V_RH, V_P, V_V, V_S, V_T, V_PH, V_SH, V_X, V_Y, V_Z, NVEC };
@@ -1816,7 +1819,12 @@ __global__ void k_count_active(const Desc* __restrict__ D, int ninst, int which,
 //   mode 1: after a defect: S_RN = sqrt(S_FN); act[1] = S_RN > tol*S_BN && S_REF < max_refine
 //           and the defect decreased (stagnation stops the refinement); S_CONT = act[1]: the
 //           Newton update waits for the next round
-__global__ void k_refine_flags(const Desc* __restrict__ D, int ninst, int mode, double tol, int max_refine) {
+struct Forcing {
+    double rel, abs, forcing, eta_max;
+    int mixed;
+};
+__global__ void k_refine_flags(const Desc* __restrict__ D, int ninst, int mode, double tol, int max_refine,
+                               Forcing fc) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ninst) return;
     const Desc& d = D[i];
@@ -1829,13 +1837,22 @@ __global__ void k_refine_flags(const Desc* __restrict__ D, int ninst, int mode, 
         sc[S_REF] = 0.0;
         sc[S_ITT] = 0.0;
         d.act[1] = sc[S_BN] > 0.0 && isfinite(sc[S_BN]);
+        // this Newton iteration's linear tolerance (inexact Newton; FusedConfig::forcing)
+        double eta = tol;
+        if (fc.forcing > 0.0 && d.kind != K_LINEAR) {
+            if (sc[S_F0] <= 0.0) sc[S_F0] = sc[S_FN];   // the step's (stage's) first iteration
+            const double fn = sc[S_BN], target = fmax(fc.abs, fc.rel > 0.0 ? fc.rel * sqrt(sc[S_F0]) : 0.0);
+            if (fn > 0.0 && target > 0.0 && isfinite(fn)) eta = fmin(fc.eta_max, fmax(tol, fc.forcing * target / fn));
+        }
+        sc[S_LTOL] = eta;
+        sc[S_TOL2] = fc.mixed ? fmax(sc[S_ITOL], eta * eta) : eta * eta;
     } else {
         if (!d.act[1]) return;
         const double rn = sqrt(sc[S_FN]);   // defect norm stored by k_fin(A_STORE, S_FN)
         const bool progress = rn < sc[S_RN];
         sc[S_RN] = rn;
         sc[S_REF] += 1.0;
-        d.act[1] = isfinite(rn) && progress && rn > tol * sc[S_BN] && sc[S_REF] < max_refine;
+        d.act[1] = isfinite(rn) && progress && rn > sc[S_LTOL] * sc[S_BN] && sc[S_REF] < max_refine;
         sc[S_CONT] = d.act[1] ? 1.0 : 0.0;   // another round before the Newton update
     }
 }
@@ -2933,12 +2950,12 @@ struct HeatBatch::Impl {
         // refinement flags from ||F||; the first round reads F and writes du (no clearing pass)
         const unsigned ni = (unsigned)t.inst.size();
         hipLaunchKernelGGL(k_refine_flags, dim3((ni + 63) / 64), dim3(64), 0, S(), t.desc->p, (int)ni, 0,
-                           cfg.solver_tol, cfg.max_refine);
+                           cfg.solver_tol, cfg.max_refine, forcing());
         precond_setup<T>(t);
         mg_setup<T>(t);
         if (!mixed) {
             HLP(C_PC, (k_load_rhs<T, CP>), 0, 1);
-            bicgstab<T>(t, cfg.solver_tol, cfg.solver_max_iter);
+            bicgstab<T>(t, -1.0, cfg.solver_max_iter);   // per-instance tolerance (S_TOL2: the iteration's eta)
         } else {
             HLP(C_PC, (k_load_rhs<T, CP>), 1, -1);
             bicgstab<T>(t, -1.0, cfg.solver_max_iter);   // per-instance inner tolerance (S_TOL2)
@@ -2956,7 +2973,16 @@ struct HeatBatch::Impl {
         launch_defect(t);
         fin(t, Fcell(S_FN));
         hipLaunchKernelGGL(k_refine_flags, dim3((ni + 63) / 64), dim3(64), 0, S(), t.desc->p, (int)ni, 1,
-                           cfg.solver_tol, cfg.max_refine);
+                           cfg.solver_tol, cfg.max_refine, forcing());
+    }
+    Forcing forcing() const {
+        Forcing f;
+        f.rel = cfg.rel_res_tol;
+        f.abs = cfg.abs_res_tol;
+        f.forcing = cfg.forcing;
+        f.eta_max = cfg.eta_max;
+        f.mixed = mixed ? 1 : 0;
+        return f;
     }
     template <class T>
     void precond_setup(const Table& t) {
@@ -3096,6 +3122,7 @@ static void alloc_solver(Inst& I, bool mixed, size_t ks, double inner_tol) {
     EQ_HIP(hipMemsetAsync(I.sc->p, 0, NSLOT * sizeof(double), S()));
     EQ_HIP(hipMemsetAsync(I.act->p, 0, 4 * sizeof(int), S()));
     launch_fill(I.sc->p + S_TOL2, inner_tol * inner_tol, 1);
+    launch_fill(I.sc->p + S_ITOL, inner_tol * inner_tol, 1);
     I.kcoef = dev_alloc<char>((size_t)I.ncoef() * 2 * h * ks);
     I.dinv = dev_alloc<char>(2 * h * ks);
     I.vecs = dev_alloc<char>((size_t)NVEC * I.ns() * h * ks);
@@ -3350,6 +3377,7 @@ void HeatBatch::begin_step(int slot, double dt) {
     }
     I.st = StepStats();
     I.state = Inst::START;
+    launch_fill(I.sc->p + S_F0, 0.0, 1);   // the step's first Newton iteration sets it
 }
 
 std::vector<int> HeatBatch::iterate() {
@@ -3431,6 +3459,7 @@ std::vector<int> HeatBatch::iterate() {
             I.stage = 1;
             EQ_HIP(hipMemcpyAsync(I.act->p + 2, &kStage[1], sizeof(int), hipMemcpyHostToDevice, S()));
             launch_fill(I.sw->p, 0.5, I.n);
+            launch_fill(I.sc->p + S_F0, 0.0, 1);   // the transport stage's own first iteration
             I.state = Inst::START;
             continue;
         }

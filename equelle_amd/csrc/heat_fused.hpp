@@ -36,6 +36,14 @@ struct FusedConfig {
     int solver_max_iter = 1000;
     double inner_tol = 1e-6;           // mixed: FP32 inner solve tolerance of heat instances
     double inner_tol_twophase = 1e-4;  //        and of two-phase instances (singular, badly scaled pressure)
+    // Inexact Newton: the linear solve of a Newton iteration stops at
+    //   eta = min(eta_max, max(solver_tol, forcing * target / ||F_k||)),
+    // target = max(abs_res_tol, rel_res_tol ||F_0||): only as far as the next residual needs
+    // to go to meet the Newton test (with a margin 1 / forcing), never tighter than
+    // solver_tol.  The first iteration of a step takes solver_tol.  forcing = 0: solver_tol
+    // in every iteration.  Linear-system instances always take solver_tol.
+    double forcing = 0.1;
+    double eta_max = 1e-2;
     // Krylov form with preconditioner ilu0, per instance kind: "reduced" (BiCGStab on the
     // red-black Schur complement, preconditioned by its diagonal) or "full" (BiCGStab on
     // the whole system with the two-pass red-black ILU(0)); equal in exact arithmetic.

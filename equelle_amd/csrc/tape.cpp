@@ -9,6 +9,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -684,7 +685,17 @@ std::shared_ptr<Kernel> get_kernel(const std::string& key, const Tape& t, int ro
     auto k = std::make_shared<Kernel>();
     const auto t0 = std::chrono::steady_clock::now();
     std::string log;
-    const std::vector<char> code = compile_source(kernel_source(t, root, K, jac, have_pat), &log);
+    const std::string src = kernel_source(t, root, K, jac, have_pat);
+    // EQHIP_DUMP_RESIDUAL=<directory>: keep each generated source (for inspection)
+    if (const char* dir = std::getenv("EQHIP_DUMP_RESIDUAL")) {
+        const std::string path = std::string(dir) + "/eqhip_residual_" + std::to_string(std::hash<std::string>()(key)) +
+                                 (jac ? "_jac" : "_val") + ".hip";
+        if (FILE* f = std::fopen(path.c_str(), "w")) {
+            std::fputs(src.c_str(), f);
+            std::fclose(f);
+        }
+    }
+    const std::vector<char> code = compile_source(src, &log);
     if (code.empty()) {
         fprintf(stderr, "eqhip: residual kernel did not compile; evaluating op by op\n%s\n", log.c_str());
         k->bad = true;
