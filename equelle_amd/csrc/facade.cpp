@@ -435,9 +435,11 @@ CollOfScalar EquelleRuntimeCUDA::supersetRange(const CollOfScalar& a, int start,
     return CollOfScalar(eqhip::s_extend(a.data(), core_->range_set(start, a.size()), core_->range_set(0, total)));
 }
 
-CollOfScalar EquelleRuntimeCUDA::solveForUpdate(const CollOfScalar& residual, int* iters) const {
+double EquelleRuntimeCUDA::linearTolerance(double rn, double r0) const { return core_->linear_tolerance(rn, r0); }
+
+CollOfScalar EquelleRuntimeCUDA::solveForUpdate(const CollOfScalar& residual, int* iters, double tol) const {
     eqhip::LinReport rep;
-    SP du = core_->solve_for_update(residual.data(), &rep);
+    SP du = core_->solve_for_update(residual.data(), &rep, tol);
     if (iters) *iters = rep.iterations;
     return CollOfScalar(du);
 }

@@ -2950,7 +2950,7 @@ struct HeatBatch::Impl {
         // refinement flags from ||F||; the first round reads F and writes du (no clearing pass)
         const unsigned ni = (unsigned)t.inst.size();
         hipLaunchKernelGGL(k_refine_flags, dim3((ni + 63) / 64), dim3(64), 0, S(), t.desc->p, (int)ni, 0,
-                           cfg.solver_tol, cfg.max_refine, forcing());
+                           tol_call > 0.0 ? tol_call : cfg.solver_tol, cfg.max_refine, forcing());
         precond_setup<T>(t);
         mg_setup<T>(t);
         if (!mixed) {
@@ -2975,6 +2975,7 @@ struct HeatBatch::Impl {
         hipLaunchKernelGGL(k_refine_flags, dim3((ni + 63) / 64), dim3(64), 0, S(), t.desc->p, (int)ni, 1,
                            cfg.solver_tol, cfg.max_refine, forcing());
     }
+    double tol_call = 0.0;   // > 0: the tolerance of the current solve_linear call (linear-system instances)
     Forcing forcing() const {
         Forcing f;
         f.rel = cfg.rel_res_tol;
@@ -3310,16 +3311,16 @@ HeatBatch::LinearTargets HeatBatch::linear_targets(int slot) {
 }
 
 LinearSolveStats HeatBatch::solve_linear(int slot, const int* rp, const int* ci, const double* val, const double* rhs,
-                                         double* x) {
-    return solve_linear_impl(slot, rp, ci, val, rhs, x);
+                                         double* x, double tol) {
+    return solve_linear_impl(slot, rp, ci, val, rhs, x, tol);
 }
 
-LinearSolveStats HeatBatch::solve_linear_prepared(int slot, double* x) {
-    return solve_linear_impl(slot, nullptr, nullptr, nullptr, nullptr, x);
+LinearSolveStats HeatBatch::solve_linear_prepared(int slot, double* x, double tol) {
+    return solve_linear_impl(slot, nullptr, nullptr, nullptr, nullptr, x, tol);
 }
 
 LinearSolveStats HeatBatch::solve_linear_impl(int slot, const int* rp, const int* ci, const double* val,
-                                              const double* rhs, double* x) {
+                                              const double* rhs, double* x, double tol) {
     Impl& P = *p_;
     P.flush_copies();
     Inst& I = get_inst(p_, slot);
@@ -3330,7 +3331,14 @@ LinearSolveStats HeatBatch::solve_linear_impl(int slot, const int* rp, const int
         P.gather_linear(t, rp, ci, val, rhs);
     else
         P.prepared_linear(t);
-    P.solve_linear_table(t);
+    P.tol_call = tol;
+    try {
+        P.solve_linear_table(t);
+    } catch (...) {
+        P.tol_call = 0.0;
+        throw;
+    }
+    P.tol_call = 0.0;
     // S_ITT: Krylov iterations of the solve; S_REF: refinement rounds; S_RN / S_BN: FP64
     // defect over ||F|| (mixed); S_RR / S_BB: the Krylov residual over its right-hand side
     std::vector<double> g = P.gather(t, make_int4(S_ITT, S_REF, S_RN, S_BN), S_RR);

@@ -147,8 +147,9 @@ public:
     // writes x (device memory, cell order).  Every CSR entry must couple a cell with itself
     // or a grid neighbour; the caller checks the pattern.
     int add_linear(int nx, int ny, int nz);
+    // tol > 0: this solve's tolerance in place of solver_tol
     LinearSolveStats solve_linear(int slot, const int* rp, const int* ci, const double* val, const double* b,
-                                  double* x);
+                                  double* x, double tol = 0.0);
     // Where a caller writes a linear-system instance's right-hand side and stencil
     // coefficients in place, instead of solve_linear's gather from CSR: F (cell order), the
     // Krylov-precision coefficients K[q] (q = C, W, E, S, N, B, T; red-black rank order: cell
@@ -164,7 +165,7 @@ public:
     };
     LinearTargets linear_targets(int slot);
     // solve_linear on coefficients and right-hand side already written at linear_targets
-    LinearSolveStats solve_linear_prepared(int slot, double* x);
+    LinearSolveStats solve_linear_prepared(int slot, double* x, double tol = 0.0);
     // Newton iterations allowed per solve for this slot (at most the batch's max_iter;
     // 0 restores the batch's limit).  A solve that reaches it stops with reason 0.
     void set_max_newton(int slot, int max_newton);
@@ -212,7 +213,7 @@ public:
 
 private:
     LinearSolveStats solve_linear_impl(int slot, const int* rp, const int* ci, const double* val, const double* rhs,
-                                       double* x);
+                                       double* x, double tol);
     Impl* p_;
 };
 

@@ -4,6 +4,7 @@
 #include "tape.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <sstream>
@@ -253,6 +254,9 @@ RuntimeCore::RuntimeCore(const ParamMap& p) : params_(p) {
     solver.max_refine = p.get_int("max_refine", 30);
     stencil_solver = p.get_bool("stencil_solver", true);
     fuse_residual = p.get_bool("fuse_residual", true);
+    forcing = p.get_double("forcing", forcing);
+    eta_max = p.get_double("eta_max", eta_max);
+    EQ_REQUIRE(forcing >= 0.0 && eta_max > 0.0 && eta_max < 1.0, "forcing must be >= 0 and eta_max in (0, 1)");
     verbose = p.get_int("verbose", 0);
     solver.verbose = verbose;
     max_iter = p.get_int("max_iter", 10);
@@ -427,7 +431,14 @@ bool RuntimeCore::stencil_sink(const PatternP& P, StencilSink* out) const {
     return true;
 }
 
-SP RuntimeCore::solve_for_update(const SP& residual, LinReport* rep) const {
+double RuntimeCore::linear_tolerance(double rn, double r0) const {
+    const double target = std::max(abs_res_tol, rel_res_tol > 0.0 ? rel_res_tol * r0 : 0.0);
+    if (forcing <= 0.0 || target <= 0.0 || !(rn > 0.0) || !std::isfinite(rn)) return solver.tol;
+    return std::min(eta_max, std::max(solver.tol, forcing * target / rn));
+}
+
+SP RuntimeCore::solve_for_update(const SP& residual, LinReport* rep, double tol) const {
+    const double eta = tol > 0.0 ? tol : solver.tol;
     tape_read(*residual, "a linear solve");
     // BiCGStab with red-black ILU(0) or Jacobi on a Jacobian with the grid's stencil
     // pattern: the coefficients are gathered out of the CSR values (or were written in place
@@ -440,17 +451,17 @@ SP RuntimeCore::solve_for_update(const SP& residual, LinReport* rep) const {
         if (residual->jsink) {
             EQ_REQUIRE(residual->jsink == sink_token_,
                        "the stencil solver's coefficients were overwritten after this residual was evaluated");
-            st = stencil_->solve_linear_prepared(stencil_slot_, x->p);
+            st = stencil_->solve_linear_prepared(stencil_slot_, x->p, eta);
         } else {
             const Pattern& P = *residual->pat;
             st = stencil_->solve_linear(stencil_slot_, P.d_rp->p, P.d_ci->p, jac_values(*residual), residual->val->p,
-                                        x->p);
+                                        x->p, eta);
         }
         LinReport R;
         R.iterations = st.iterations;
         R.refinements = st.refinements;
         R.residual = st.relres;
-        R.converged = st.relres <= solver.tol;
+        R.converged = st.relres <= eta;
         if (solver.verbose > 0)
             fprintf(stdout, "Linear solver (stencil %s/%s/%s): %d iterations, relative residual %g%s\n",
                     solver.method.c_str(), solver.precond.c_str(), solver.precision.c_str(), R.iterations,
@@ -458,7 +469,10 @@ SP RuntimeCore::solve_for_update(const SP& residual, LinReport* rep) const {
         if (rep) *rep = R;
         return s_from_device(x, residual->n);
     }
-    return linear_solve(solver, residual, rep);
+    if (eta == solver.tol) return linear_solve(solver, residual, rep);
+    SolverConfig sc = solver;
+    sc.tol = eta;
+    return linear_solve(sc, residual, rep);
 }
 
 }  // namespace eqhip

@@ -65,7 +65,8 @@ public:
     SP neg_gradient(const SP& x) const;
     SP interior_divergence(const SP& x) const;
     SP divergence_matrix(const SP& x) const;
-    SP solve_for_update(const SP& residual, LinReport* rep) const;
+    // tol > 0: this solve's relative tolerance in place of solver_tol (inexact Newton)
+    SP solve_for_update(const SP& residual, LinReport* rep, double tol = 0.0) const;
     // The stencil solver's in-place targets for a residual whose Jacobian has pattern P,
     // when its Newton step is solved there (a fused evaluation then writes its Jacobian
     // straight into them; StencilSink in tape.hpp).
@@ -77,6 +78,12 @@ public:
     SolverConfig solver;
     bool stencil_solver = true;   // stencil-pattern Jacobians: solve on the stencil kernels
     bool fuse_residual = true;    // newtonSolve residuals: recorded once, then one fused kernel
+    // newtonSolve's inexact Newton: the linear solve of an iteration stops at
+    // min(eta_max, max(solver_tol, forcing * target / ||F_k||)), target = max(abs_res_tol,
+    // rel_res_tol ||F_0||); forcing = 0: solver_tol in every iteration
+    double forcing = 0.1;
+    double eta_max = 1e-2;
+    double linear_tolerance(double rn, double r0) const;
     int max_iter = 10;
     double abs_res_tol = 1e-6;
     double rel_res_tol = 0.0;

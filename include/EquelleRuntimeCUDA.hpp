@@ -192,7 +192,10 @@ public:
 
     // ----------- Extensions
     const NewtonReport& lastNewtonReport() const { return last_newton_; }
-    CollOfScalar solveForUpdate(const CollOfScalar& residual, int* krylov_iterations = nullptr) const;
+    // tol > 0: the solve's relative tolerance in place of solver_tol
+    CollOfScalar solveForUpdate(const CollOfScalar& residual, int* krylov_iterations = nullptr, double tol = 0.0) const;
+    // newtonSolve's inexact-Newton tolerance for an iteration at ||F_k|| = rn (||F_0|| = r0)
+    double linearTolerance(double rn, double r0) const;
     eqhip::RuntimeCore& core() const { return *core_; }
 
 private:
@@ -255,7 +258,7 @@ CollOfScalar EquelleRuntimeCUDA::newtonSolve(const ResidualFunctor& rescomp, con
     const int expected = last_newton_.iterations;
     while (!newtonStop(rn, r0) && iter < max_iter_) {
         int lin = 0;
-        CollOfScalar du = solveForUpdate(residual, &lin);
+        CollOfScalar du = solveForUpdate(residual, &lin, linearTolerance(rn, r0));
         rep.linear_iterations += lin;
         // du is constant: u keeps its identity Jacobian
         u = u - du;
