@@ -805,6 +805,8 @@ def main():
     ap.add_argument("--precond", default="ilu0", help="preconditioner of the fused runs: ilu0 | diagonal | none")
     ap.add_argument("--forcing", type=float, default=None,
                     help="fused heat runs: inexact-Newton margin of the linear tolerance (0: solver_tol always)")
+    ap.add_argument("--forcing0", type=float, default=None,
+                    help="fused heat runs: first-iteration factor on the latest step's contraction (0: solver_tol)")
     ap.add_argument("--workload", default="heat2048",
                     choices=["heat2048", "batched", "serving", "twophase", "twophase_fi", "copy", "generic"])
     ap.add_argument("--instances", type=int, default=512)
@@ -823,9 +825,11 @@ def main():
     ap.add_argument("--max-in-flight-twophase", dest="max_in_flight_twophase", type=int, default=4096,
                     help="serving: two-phase requests in flight at once")
     args = ap.parse_args()
+    global FORCING
     if args.forcing is not None:
-        global FORCING
         FORCING = f"forcing={args.forcing}"
+    if args.forcing0 is not None:
+        FORCING += f" forcing0={args.forcing0}"
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)

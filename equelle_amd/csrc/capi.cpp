@@ -769,6 +769,7 @@ static FusedConfig fused_config(const char* params) {
     c.inner_tol_twophase = m.get_double("inner_tol_twophase", c.inner_tol_twophase);
     c.forcing = m.get_double("forcing", c.forcing);
     c.eta_max = m.get_double("eta_max", c.eta_max);
+    c.forcing0 = m.get_double("forcing0", c.forcing0);
     EQ_REQUIRE(c.forcing >= 0.0 && c.eta_max > 0.0 && c.eta_max < 1.0, "forcing must be >= 0 and eta_max in (0, 1)");
     c.krylov_form_heat = m.get_string("krylov_form_heat", c.krylov_form_heat);
     c.krylov_form_twophase = m.get_string("krylov_form_twophase", c.krylov_form_twophase);

@@ -24,9 +24,11 @@ inline hipStream_t S() { return Device::get().stream(); }
 // S_MEAN: the mean of the Krylov right-hand side over the cells (singular systems only)
 // S_F0: ||F||^2 at the first Newton iteration of the step (stage); 0 until then
 // S_LTOL: the linear tolerance of the current Newton iteration; S_ITOL: the square of the
-// instance's base inner (mixed-precision) tolerance
+// instance's base inner (mixed-precision) tolerance; S_NK: Newton iterations of the step
+// (stage) so far; S_R1, S_R1B: ||F_1|| / ||F_0|| of the latest step (two-phase: of its
+// pressure / transport stage), 0 until measured
 enum Slot { S_RHO = 0, S_RHO_OLD, S_ALPHA, S_OMEGA, S_RR, S_BB, S_BREAK, S_IT, S_FN, S_RN, S_BN, S_REF, S_ITT, S_TOL2,
-            S_MEAN, S_BETA, S_CONT, S_F0, S_LTOL, S_ITOL, NSLOT = 20 };
+            S_MEAN, S_BETA, S_CONT, S_F0, S_LTOL, S_ITOL, S_NK, S_R1, S_R1B, NSLOT = 24 };
 // Krylov vectors; V_Y is the red-rank scratch of the reduced operator (ilu0 only), V_Z the
 // V-cycle's z = M^-1 r (two-phase pressure by CG with multigrid)
 enum Vec { V_R = 0, V_RH, V_P, V_V, V_S, V_T, V_PH, V_SH, V_X, V_Y, V_Z, NVEC };
@@ -1821,6 +1823,7 @@ __global__ void k_count_active(const Desc* __restrict__ D, int ninst, int which,
 //           Newton update waits for the next round
 struct Forcing {
     double rel, abs, forcing, eta_max;
+    double forcing0;   // first iteration of a step: forcing0 * (||F_1|| / ||F_0|| of the latest step)
     int mixed;
 };
 __global__ void k_refine_flags(const Desc* __restrict__ D, int ninst, int mode, double tol, int max_refine,
@@ -1840,9 +1843,20 @@ __global__ void k_refine_flags(const Desc* __restrict__ D, int ninst, int mode, 
         // this Newton iteration's linear tolerance (inexact Newton; FusedConfig::forcing)
         double eta = tol;
         if (fc.forcing > 0.0 && d.kind != K_LINEAR) {
-            if (sc[S_F0] <= 0.0) sc[S_F0] = sc[S_FN];   // the step's (stage's) first iteration
-            const double fn = sc[S_BN], target = fmax(fc.abs, fc.rel > 0.0 ? fc.rel * sqrt(sc[S_F0]) : 0.0);
-            if (fn > 0.0 && target > 0.0 && isfinite(fn)) eta = fmin(fc.eta_max, fmax(tol, fc.forcing * target / fn));
+            const int r1 = d.kind == K_TWOPHASE && d.act[2] == 1 ? S_R1B : S_R1;
+            if (sc[S_F0] <= 0.0) {   // the step's (stage's) first iteration
+                sc[S_F0] = sc[S_FN];
+                sc[S_NK] = 0.0;
+                // no tighter than the nonlinear error the iteration will leave, as the latest
+                // step's first iteration left it
+                if (fc.forcing0 > 0.0 && sc[r1] > 0.0) eta = fmin(fc.eta_max, fmax(tol, fc.forcing0 * sc[r1]));
+            } else {
+                sc[S_NK] += 1.0;
+                if (sc[S_NK] == 1.0 && sc[S_F0] > 0.0) sc[r1] = sqrt(sc[S_FN] / sc[S_F0]);
+                const double fn = sc[S_BN], target = fmax(fc.abs, fc.rel > 0.0 ? fc.rel * sqrt(sc[S_F0]) : 0.0);
+                if (fn > 0.0 && target > 0.0 && isfinite(fn))
+                    eta = fmin(fc.eta_max, fmax(tol, fc.forcing * target / fn));
+            }
         }
         sc[S_LTOL] = eta;
         sc[S_TOL2] = fc.mixed ? fmax(sc[S_ITOL], eta * eta) : eta * eta;
@@ -2982,6 +2996,7 @@ struct HeatBatch::Impl {
         f.abs = cfg.abs_res_tol;
         f.forcing = cfg.forcing;
         f.eta_max = cfg.eta_max;
+        f.forcing0 = cfg.forcing0;
         f.mixed = mixed ? 1 : 0;
         return f;
     }

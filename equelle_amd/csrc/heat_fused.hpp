@@ -44,6 +44,10 @@ struct FusedConfig {
     // in every iteration.  Linear-system instances always take solver_tol.
     double forcing = 0.1;
     double eta_max = 1e-2;
+    // the first iteration of a step stops at max(solver_tol, forcing0 * rho), rho =
+    // ||F_1|| / ||F_0|| of the latest step (the nonlinear error a first iteration leaves);
+    // 0: solver_tol
+    double forcing0 = 0.1;
     // Krylov form with preconditioner ilu0, per instance kind: "reduced" (BiCGStab on the
     // red-black Schur complement, preconditioned by its diagonal) or "full" (BiCGStab on
     // the whole system with the two-pass red-black ILU(0)); equal in exact arithmetic.
