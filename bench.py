@@ -614,7 +614,9 @@ def measure_traffic(args):
     over child runs of this script (the headline at one timed step, then the 256 MiB copy),
     each counter scaled to bytes by what it reads for the copy.  Per kernel: the mean over
     the last 50 dispatches of its most-dispatched launch shape (the back-to-back timing
-    launches on the final state).  Runs before this process touches the GPU."""
+    launches on the final state).  The launch shape comes from the tuner of an unprofiled
+    child run first; the counter runs and the headline run in it.  Runs before this process
+    touches the GPU."""
     import glob
     import shutil
     import signal
@@ -625,9 +627,30 @@ def measure_traffic(args):
     me = os.path.abspath(__file__)
     head = ["--steps", "1", "--warmup", "0", "--no-cpu-baseline", "--quick", "--size", str(args.size),
             "--precond", args.precond, "--inner-tol", repr(args.inner_tol)]
+    if args.forcing is not None:
+        head += ["--forcing", repr(args.forcing)]
+    if args.forcing0 is not None:
+        head += ["--forcing0", repr(args.forcing0)]
     got = {}
-    shape = None   # the launch shape the first counter run's tuner chose; the later runs use it
     env = dict(os.environ, TMPDIR="/tmp")
+    # the launch shape: the tuner's choice in an unprofiled child run (counter collection
+    # distorts the tuner's timings); the counter runs and the headline use it
+    shape = args.shape
+    if shape is None:
+        log("[bench] launch shape: tuner in an unprofiled child run")
+        p = subprocess.Popen([sys.executable, me] + head, env=env, stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                             start_new_session=True)
+        try:
+            sout, _ = p.communicate(timeout=240)
+            line = json.loads([ln for ln in sout.decode(errors="replace").splitlines() if ln.startswith("{")][-1])
+            cp, ls = line["launch_shape"]["cells_per_thread"], line["launch_shape"]
+            shape = ",".join(str(cp[k]) for k in ("assemble", "defect", "spmv", "precond", "vector")) + \
+                f",{ls['xcd_block_order']}"
+        except subprocess.TimeoutExpired:
+            os.killpg(p.pid, signal.SIGKILL)
+            p.communicate()
+        except (IndexError, KeyError, ValueError):
+            pass
     with tempfile.TemporaryDirectory(dir="/tmp") as d:
         for leg, ctr in (("head", "FETCH_SIZE"), ("head", "WRITE_SIZE"), ("copy", "FETCH_SIZE"),
                          ("copy", "WRITE_SIZE")):
