@@ -702,8 +702,8 @@ def measure_traffic(args):
                              "write_bytes": sum(w) / len(w) * 1024 * scale_w}
     return {"per_kernel": per_kernel, "scale_read": scale_r, "scale_write": scale_w, "shape": shape,
             "source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE child runs of this script (headline at 1 timed "
-                      "step; 256 MiB device-to-device copy for the byte scale); the headline of this run uses "
-                      "the launch shape the first counter run chose"}
+                      "step; 256 MiB device-to-device copy for the byte scale); the counter runs and the "
+                      "headline of this run use the launch shape an unprofiled child run tuned"}
 
 
 def traffic_of(measured, kernel):
