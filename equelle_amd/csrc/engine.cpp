@@ -1,5 +1,6 @@
-// Forward-mode sparse AD engine: symbolic phase (host, cached per pattern) and
-// numeric phase (HIP kernels).  See engine.hpp.
+// Forward-mode sparse AD engine: symbolic phase (count / prefix-sum / fill kernels on the
+// device, cached per pattern; a host pass only for structural products with rows of more
+// than SPGEMM_CMAX contributions) and numeric phase (HIP kernels).  See engine.hpp.
 #include "engine.hpp"
 
 #include <algorithm>

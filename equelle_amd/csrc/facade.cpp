@@ -435,7 +435,9 @@ CollOfScalar EquelleRuntimeCUDA::supersetRange(const CollOfScalar& a, int start,
     return CollOfScalar(eqhip::s_extend(a.data(), core_->range_set(start, a.size()), core_->range_set(0, total)));
 }
 
-double EquelleRuntimeCUDA::linearTolerance(double rn, double r0) const { return core_->linear_tolerance(rn, r0); }
+double EquelleRuntimeCUDA::linearTolerance(int k, double rn, double r0, double first_ratio) const {
+    return k == 0 ? core_->first_tolerance(first_ratio) : core_->linear_tolerance(rn, r0);
+}
 
 CollOfScalar EquelleRuntimeCUDA::solveForUpdate(const CollOfScalar& residual, int* iters, double tol) const {
     eqhip::LinReport rep;

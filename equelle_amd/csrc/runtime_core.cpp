@@ -256,6 +256,7 @@ RuntimeCore::RuntimeCore(const ParamMap& p) : params_(p) {
     fuse_residual = p.get_bool("fuse_residual", true);
     forcing = p.get_double("forcing", forcing);
     eta_max = p.get_double("eta_max", eta_max);
+    forcing0 = p.get_double("forcing0", forcing0);
     EQ_REQUIRE(forcing >= 0.0 && eta_max > 0.0 && eta_max < 1.0, "forcing must be >= 0 and eta_max in (0, 1)");
     verbose = p.get_int("verbose", 0);
     solver.verbose = verbose;
@@ -435,6 +436,11 @@ double RuntimeCore::linear_tolerance(double rn, double r0) const {
     const double target = std::max(abs_res_tol, rel_res_tol > 0.0 ? rel_res_tol * r0 : 0.0);
     if (forcing <= 0.0 || target <= 0.0 || !(rn > 0.0) || !std::isfinite(rn)) return solver.tol;
     return std::min(eta_max, std::max(solver.tol, forcing * target / rn));
+}
+
+double RuntimeCore::first_tolerance(double rho) const {
+    if (forcing <= 0.0 || forcing0 <= 0.0 || !(rho > 0.0) || !std::isfinite(rho)) return solver.tol;
+    return std::min(eta_max, std::max(solver.tol, forcing0 * rho));
 }
 
 SP RuntimeCore::solve_for_update(const SP& residual, LinReport* rep, double tol) const {

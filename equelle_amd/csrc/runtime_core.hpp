@@ -83,7 +83,11 @@ public:
     // rel_res_tol ||F_0||); forcing = 0: solver_tol in every iteration
     double forcing = 0.1;
     double eta_max = 1e-2;
+    // the first iteration: max(solver_tol, forcing0 * rho), rho = ||F_1|| / ||F_0|| of the
+    // call site's previous solve (0: none; forcing0 = 0: solver_tol)
+    double forcing0 = 0.1;
     double linear_tolerance(double rn, double r0) const;
+    double first_tolerance(double rho) const;
     int max_iter = 10;
     double abs_res_tol = 1e-6;
     double rel_res_tol = 0.0;

@@ -194,8 +194,9 @@ public:
     const NewtonReport& lastNewtonReport() const { return last_newton_; }
     // tol > 0: the solve's relative tolerance in place of solver_tol
     CollOfScalar solveForUpdate(const CollOfScalar& residual, int* krylov_iterations = nullptr, double tol = 0.0) const;
-    // newtonSolve's inexact-Newton tolerance for an iteration at ||F_k|| = rn (||F_0|| = r0)
-    double linearTolerance(double rn, double r0) const;
+    // newtonSolve's inexact-Newton tolerance for iteration k at ||F_k|| = rn (||F_0|| = r0);
+    // first_ratio: ||F_1|| / ||F_0|| of the call site's previous solve (0: none)
+    double linearTolerance(int k, double rn, double r0, double first_ratio) const;
     eqhip::RuntimeCore& core() const { return *core_; }
 
 private:
@@ -218,6 +219,7 @@ private:
     const Opm::ParameterGroup& param_;
     std::map<std::string, int> outputcount_;
     NewtonReport last_newton_;
+    std::map<const void*, double> first_ratio_;   // per newtonSolve call site
     int max_iter_;
     int verbose_;
 };
@@ -230,6 +232,8 @@ CollOfScalar EquelleRuntimeCUDA::newtonSolve(const ResidualFunctor& rescomp, con
     // operations on u; later evaluations run them as one fused kernel (one recorded
     // program per call site, i.e. per residual functor type).
     static int site_state = 0;
+    // ||F_1|| / ||F_0|| of this call site's previous solve on this runtime (0: none)
+    double& site_ratio = first_ratio_[&site_state];
     CollOfScalar u = CollOfScalar(u_initialguess, true);
     FusedResidual fr(*this, &site_state, u);
     CollOfScalar residual;
@@ -258,7 +262,7 @@ CollOfScalar EquelleRuntimeCUDA::newtonSolve(const ResidualFunctor& rescomp, con
     const int expected = last_newton_.iterations;
     while (!newtonStop(rn, r0) && iter < max_iter_) {
         int lin = 0;
-        CollOfScalar du = solveForUpdate(residual, &lin, linearTolerance(rn, r0));
+        CollOfScalar du = solveForUpdate(residual, &lin, linearTolerance(iter, rn, r0, site_ratio));
         rep.linear_iterations += lin;
         // du is constant: u keeps its identity Jacobian
         u = u - du;
@@ -266,12 +270,14 @@ CollOfScalar EquelleRuntimeCUDA::newtonSolve(const ResidualFunctor& rescomp, con
         if (iter >= expected) {
             const CollOfScalar values = evaluate(u, false);
             rn = twoNorm(values);
+            if (iter == 1 && r0 > 0.0) site_ratio = rn / r0;
             newtonIterationMessage(iter, rn);
             if (!newtonStop(rn, r0) && iter < max_iter_) residual = evaluate(u, true);
             continue;
         }
         residual = evaluate(u, true);
         rn = twoNorm(residual);
+        if (iter == 1 && r0 > 0.0) site_ratio = rn / r0;
         newtonIterationMessage(iter, rn);
     }
     rep.iterations = iter;
