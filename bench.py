@@ -513,7 +513,8 @@ def bench_twophase_fi(lib, args):
     perm, poro, sw0, sc, sv = twophase_inputs(nx, ny, nz)
     dt = 0.05
     params = twophase_params(nx, ny, nz, rel_res_tol=1e-8, max_iter=40, solver_tol=1e-7,
-                             extra="solver=GMRes preconditioner=ILU0 fully_implicit=1").replace("corey=2", "corey=1")
+                             extra=f"solver=GMRes preconditioner={args.fi_precond} fully_implicit=1").replace(
+                                 "corey=2", "corey=1")
     params = params.replace("solver=GMRes preconditioner=diagonal", "")
     sim = C.c_void_p()
     lib.twophase_create(params.encode(), dptr(perm), dptr(poro), dptr(sw0), n, iptr(sc), dptr(sv), len(sc),
@@ -838,6 +839,8 @@ def main():
     ap.add_argument("--no-graph", dest="no_graph", action="store_true", help="eager launches (profiling)")
     ap.add_argument("--grid3", default="128x128x64", help="two-phase grid nx x ny x nz")
     ap.add_argument("--grid3-fi", dest="grid3_fi", default="64x64x32", help="fully implicit two-phase grid")
+    ap.add_argument("--fi-precond", dest="fi_precond", default="ILU0",
+                    help="fully implicit: GMRES preconditioner (ILU0 | cpr | diagonal)")
     ap.add_argument("--tp-solver", dest="tp_solver", default="GMRes", help="two-phase Krylov solver")
     ap.add_argument("--tp-legs", dest="tp_legs", default="fp64,mixed",
                     help="two-phase fused legs: fp64 (the headline) and optionally mixed")

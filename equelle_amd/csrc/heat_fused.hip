@@ -721,7 +721,7 @@ __global__ __launch_bounds__(BS) void k_gather_stencil(const Desc* __restrict__ 
                           : (nz > 1 && k < nz - 1 && col == c + nxy) ? 6 : -1;
             if (q >= 0) co[q] = v;
         }
-        const double F = rhs[c];
+        const double F = rhs ? rhs[c] : 0.0;
         d.F[c] = F;
         if (MIXED) {
             double* dst[7] = {d.C, d.W, d.E, d.S, d.N, d.B, d.T};
@@ -732,6 +732,27 @@ __global__ __launch_bounds__(BS) void k_gather_stencil(const Desc* __restrict__ 
         f2[m] = F * F;
     }
     group_store<CPT>(f2, PARTS(part));
+}
+
+// One instance's vector in cell order (FP64) to red-black rank order in the Krylov
+// precision (full form: both slots of every pair), and back.
+template <class T>
+__global__ void k_cells_to_rb(const Desc* __restrict__ D, const double* __restrict__ src, int vec) {
+    const Desc& d = D[0];
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= d.n) return;
+    int i, j, k;
+    cell_ijk(d, c, i, j, k);
+    ((T*)d.v[vec])[((i + j + k) & 1) * d.h + (c >> 1)] = (T)src[c];
+}
+template <class T>
+__global__ void k_rb_to_cells(const Desc* __restrict__ D, int vec, double* __restrict__ dst) {
+    const Desc& d = D[0];
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= d.n) return;
+    int i, j, k;
+    cell_ijk(d, c, i, j, k);
+    dst[c] = (double)((const T*)d.v[vec])[((i + j + k) & 1) * d.h + (c >> 1)];
 }
 
 // Linear-system instances whose right-hand side and coefficients were written in place (the
@@ -3287,6 +3308,63 @@ void HeatBatch::remove(int slot) {
     p_->insts.erase(slot);
     ++p_->version;
 }
+int HeatBatch::add_linear_multigrid(int nx, int ny, int nz) {
+    EQ_REQUIRE(nx > 0 && ny > 0 && nz > 0, "instance sizes must be positive");
+    EQ_REQUIRE((long)nx * ny * nz < (1L << 31) - CPT_MAX * BS, "instance too large");
+    auto I = std::make_unique<Inst>();
+    I->id = p_->next_id++;
+    I->kind = K_LINEAR;
+    I->nx = nx;
+    I->ny = ny;
+    I->nz = nz;
+    I->n = nx * ny * nz;
+    I->pcm = PM_ILU;   // full red-black form: the V-cycle works on both colours
+    alloc_solver(*I, p_->mixed, p_->ks(), p_->cfg.inner_tol);
+    alloc_mg(*I, p_->ks());
+    const int id = I->id;
+    p_->insts[id] = std::move(I);
+    ++p_->version;
+    return id;
+}
+
+void HeatBatch::set_linear_matrix(int slot, const int* rp, const int* ci, const double* val) {
+    Impl& P = *p_;
+    P.flush_copies();
+    Inst& I = get_inst(p_, slot);
+    EQ_REQUIRE(I.kind == K_LINEAR && I.mg_on, "slot " << slot << " is not a multigrid linear-system instance");
+    TableP tp = P.table({&I});
+    Table& t = *tp;
+    P.gather_linear(t, rp, ci, val, nullptr);
+    // the set-up kernels (Galerkin levels, coarsest inverse) run for instances whose solve
+    // is about to start: act[1] set, act[2] = 0 (a CG stage)
+    launch_fill_int(I.act->p + 1, 1, 1);
+    if (P.mixed) P.mg_setup<float>(t);
+    else P.mg_setup<double>(t);
+}
+
+void HeatBatch::vcycle(int slot, const double* r, double* z) {
+    Impl& P = *p_;
+    P.flush_copies();
+    Inst& I = get_inst(p_, slot);
+    EQ_REQUIRE(I.kind == K_LINEAR && I.mg_on, "slot " << slot << " is not a multigrid linear-system instance");
+    TableP tp = P.table({&I});
+    Table& t = *tp;
+    const unsigned nb = (unsigned)((I.n + BS - 1) / BS);
+    launch_fill_int(I.act->p, 1, 1);   // the V-cycle kernels run for active CG instances
+    if (P.mixed) {
+        hipLaunchKernelGGL(k_cells_to_rb<float>, dim3(nb), dim3(BS), 0, S(), t.desc->p, r, (int)V_R);
+        EQ_HIP(hipGetLastError());
+        P.vcycle_dots<float>(t);
+        hipLaunchKernelGGL(k_rb_to_cells<float>, dim3(nb), dim3(BS), 0, S(), t.desc->p, (int)V_Z, z);
+    } else {
+        hipLaunchKernelGGL(k_cells_to_rb<double>, dim3(nb), dim3(BS), 0, S(), t.desc->p, r, (int)V_R);
+        EQ_HIP(hipGetLastError());
+        P.vcycle_dots<double>(t);
+        hipLaunchKernelGGL(k_rb_to_cells<double>, dim3(nb), dim3(BS), 0, S(), t.desc->p, (int)V_Z, z);
+    }
+    EQ_HIP(hipGetLastError());
+}
+
 int HeatBatch::add_linear(int nx, int ny, int nz) {
     EQ_REQUIRE(nx > 0 && ny > 0 && nz > 0, "instance sizes must be positive");
     EQ_REQUIRE((long)nx * ny * nz < (1L << 31) - CPT_MAX * BS, "instance too large");

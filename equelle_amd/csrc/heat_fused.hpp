@@ -151,6 +151,15 @@ public:
     // writes x (device memory, cell order).  Every CSR entry must couple a cell with itself
     // or a grid neighbour; the caller checks the pattern.
     int add_linear(int nx, int ny, int nz);
+    // A linear-system instance for a symmetric stencil M-matrix (e.g. the pressure block of
+    // a two-phase Jacobian, zero row sums allowed) that carries the aggregation multigrid
+    // hierarchy: set_linear_matrix gathers its coefficients from CSR and builds the coarse
+    // levels; vcycle applies one V-cycle z = M r (cell order, FP64 in and out; the cycle
+    // runs in the batch's Krylov precision), a fixed linear operator usable as a
+    // preconditioner.
+    int add_linear_multigrid(int nx, int ny, int nz);
+    void set_linear_matrix(int slot, const int* rp, const int* ci, const double* val);
+    void vcycle(int slot, const double* r, double* z);
     // tol > 0: this solve's tolerance in place of solver_tol
     LinearSolveStats solve_linear(int slot, const int* rp, const int* ci, const double* val, const double* b,
                                   double* x, double tol = 0.0);

@@ -691,6 +691,25 @@ __global__ void k_sub(long n, const T* __restrict__ b, const T* __restrict__ w, 
     long i = gid();
     if (i < n) r[i] = b[i] - w[i];
 }
+// CPR glue (skipped once the solve finished): x1 = [xp; 0], w = r - J x1, z[0:np] += xp
+__global__ void k_cpr_embed(long n, long np, const double* __restrict__ xp, double* __restrict__ x1,
+                            const double* __restrict__ gate) {
+    long i = gid();
+    if (i >= n || (gate && *gate == 0.0)) return;
+    x1[i] = i < np ? xp[i] : 0.0;
+}
+__global__ void k_cpr_resid(long n, const double* __restrict__ r, const double* __restrict__ jx, double* __restrict__ w,
+                            const double* __restrict__ gate) {
+    long i = gid();
+    if (i >= n || (gate && *gate == 0.0)) return;
+    w[i] = r[i] - jx[i];
+}
+__global__ void k_cpr_add(long np, const double* __restrict__ xp, double* __restrict__ z,
+                          const double* __restrict__ gate) {
+    long i = gid();
+    if (i >= np || (gate && *gate == 0.0)) return;
+    z[i] += xp[i];
+}
 
 // ---- multicolour ILU(0) ----------------------------------------------------------
 // ILU(0) rows of one colour (t = colour position, rows of earlier colours final):
@@ -794,6 +813,8 @@ struct System {
     DevPtr<T> lval, uval, piv;   // the factors in the per-colour sweep layout, pivots by colour position
     DevPtr<double> sc, part;
     int nb = 1;
+    const CprStage* cpr = nullptr;        // CPR (FP64): stage 1 on the pressure block, stage 2 ILU(0)
+    DevPtr<double> cxp, cx1, cjx, cw;     // its work vectors
 
     void setup(const PatternP& p, const double* csr, const std::string& pc) {
         pat = p;
@@ -832,7 +853,25 @@ struct System {
     }
     // z = M^{-1} r
     void apply_precond(const T* r, T* z, const double* gate = nullptr) const {
+        if (cpr) {
+            // stage 1: the pressure block's approximate inverse on the pressure rows
+            cpr->apply((const double*)r, cxp->p);
+            KL(k_cpr_embed, nblk(n), (long)n, (long)cpr->np, (const double*)cxp->p, cx1->p, gate);
+            spmv((const T*)cx1->p, (T*)cjx->p, gate);
+            KL(k_cpr_resid, nblk(n), (long)n, (const double*)r, (const double*)cjx->p, cw->p, gate);
+            // stage 2: ILU(0) of the whole system on what stage 1 left, plus stage 1's part
+            ilu_apply((const T*)cw->p, z, gate);
+            KL(k_cpr_add, nblk(cpr->np), (long)cpr->np, (const double*)cxp->p, (double*)z, gate);
+            return;
+        }
         if (precond == "ilu0") {
+            ilu_apply(r, z, gate);
+        } else {
+            KL(k_scale<T>, nblk(n), (long)n, precond == "jacobi" ? (const T*)dinv->p : (const T*)nullptr, r, z, gate);
+        }
+    }
+    void ilu_apply(const T* r, T* z, const double* gate) const {
+        {
             // colour 0 has no L entries (z = r there: read from r, no sweep); the last colour
             // has no U entries (its lower sweep divides by the pivots, no upper sweep)
             const int nc = L->ncolors;
@@ -850,8 +889,6 @@ struct System {
                        L->ucol->p, (const T*)uval->p, (const T*)piv->p + L->color_off[c], z,
                        c == 0 ? r : (const T*)nullptr, gate);
             }
-        } else {
-            KL(k_scale<T>, nblk(n), (long)n, precond == "jacobi" ? (const T*)dinv->p : (const T*)nullptr, r, z, gate);
         }
     }
     const double* gate() const { return sc->p + ACTIVE; }
@@ -1081,7 +1118,7 @@ double dnorm(const double* a, long n) { return std::sqrt(reduce(RED_SUMSQ, a, n)
 }  // namespace
 
 // ===========================================================================
-SP linear_solve(const SolverConfig& cfg, const SP& res, LinReport* rep) {
+SP linear_solve(const SolverConfig& cfg, const SP& res, LinReport* rep, const CprStage* cpr) {
     EQ_REQUIRE(res->pat, "linear solve: the residual has no Jacobian");
     const PatternP& P = res->pat;
     const int n = res->n;
@@ -1090,9 +1127,22 @@ SP linear_solve(const SolverConfig& cfg, const SP& res, LinReport* rep) {
     std::string pc = cfg.precond == "diagonal" ? "jacobi" : cfg.precond;
     LinReport R;
     auto x = dev_alloc<double>(n);
+    if (cpr) {
+        EQ_REQUIRE(cfg.precision == "fp64", "preconditioner=cpr runs in FP64 (precision=fp64)");
+        EQ_REQUIRE(2 * cpr->np == n, "preconditioner=cpr: " << n << " unknowns are not two stacked halves");
+        pc = "ilu0";
+    }
     if (cfg.precision == "fp64") {
         System<double> A;
         A.setup(P, jac_values(*res), pc);
+        if (cpr) {
+            cpr->setup(P, jac_values(*res));
+            A.cpr = cpr;
+            A.cxp = dev_alloc<double>(std::max(1, cpr->np));
+            A.cx1 = dev_alloc<double>(n);
+            A.cjx = dev_alloc<double>(n);
+            A.cw = dev_alloc<double>(n);
+        }
         double rr = 0;
         R.iterations = run_method<double>(cfg, A, res->val->p, x->p, cfg.tol, cfg.max_iter, &rr);
         R.residual = rr;

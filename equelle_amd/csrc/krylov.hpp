@@ -13,6 +13,7 @@
 //    r = b - J x drives iterative refinement until the FP64 tolerance is met.
 //  * All Krylov scalars stay on the device; the host reads one norm per iteration.
 #pragma once
+#include <functional>
 #include <string>
 
 #include "core.hpp"
@@ -38,8 +39,21 @@ struct LinReport {
     double residual = 0.0;  // final ||b - A x|| / ||b|| (FP64)
 };
 
-// Solves residual.jacobian * du = residual.value; returns du (no Jacobian).
-SP linear_solve(const SolverConfig& cfg, const SP& residual, LinReport* rep);
+// Constrained-pressure-residual (CPR) preconditioning of a stacked two-unknown system
+// [p; s] (newtonSolveSystem's Jacobian, the pressure unknowns and equations first): stage
+// 1 applies an approximate inverse of the pressure block (the rows and columns of the
+// first np unknowns) to the pressure rows of the residual, stage 2 applies ILU(0) of the
+// whole system to the residual that leaves; M^-1 r = x1 + ILU(0)^-1 (r - J x1), x1 =
+// [M_p^-1 r_p; 0].  A fixed linear operator when stage 1 is (one multigrid V-cycle).
+struct CprStage {
+    int np = 0;
+    std::function<void(const PatternP& P, const double* vals)> setup;   // the whole Jacobian (CSR values)
+    std::function<void(const double* r, double* z)> apply;              // z = M_p^-1 r (np each, device)
+};
+
+// Solves residual.jacobian * du = residual.value; returns du (no Jacobian).  With cpr
+// (FP64 only): the CPR preconditioner with the configured Krylov method.
+SP linear_solve(const SolverConfig& cfg, const SP& residual, LinReport* rep, const CprStage* cpr = nullptr);
 
 // SpMV through the solver's SELL-64 storage (test hook / benchmark):
 // y = A x for the Jacobian of `a`, in FP64 or FP32 (x, y converted on the fly).

@@ -244,7 +244,8 @@ RuntimeCore::RuntimeCore(const ParamMap& p) : params_(p) {
     std::string pc = p.get_string("preconditioner", p.get_string("precond", "diagonal"));
     std::transform(pc.begin(), pc.end(), pc.begin(), ::tolower);
     solver.precond = (pc == "diagonal" || pc == "jacobi") ? "jacobi" : pc;
-    EQ_REQUIRE(solver.precond == "jacobi" || solver.precond == "ilu0" || solver.precond == "none",
+    EQ_REQUIRE(solver.precond == "jacobi" || solver.precond == "ilu0" || solver.precond == "none" ||
+                   solver.precond == "cpr",
                "unknown preconditioner " << pc);
     solver.max_iter = p.get_int("solver_max_iter", 1000);
     solver.tol = p.get_double("solver_tol", 1e-8);
@@ -475,10 +476,58 @@ SP RuntimeCore::solve_for_update(const SP& residual, LinReport* rep, double tol)
         if (rep) *rep = R;
         return s_from_device(x, residual->n);
     }
-    if (eta == solver.tol) return linear_solve(solver, residual, rep);
     SolverConfig sc = solver;
     sc.tol = eta;
+    if (solver.precond == "cpr") {
+        // stacked [p; s] (newtonSolveSystem): stage 1 is one multigrid V-cycle on the
+        // pressure block, the grid's stencil in the first nc unknowns
+        const int np = grid_->nc;
+        EQ_REQUIRE(residual->n == 2 * np, "preconditioner=cpr needs two stacked unknowns on AllCells() ("
+                                              << residual->n << " unknowns for " << np << " cells)");
+        if (!mg_) {
+            FusedConfig fc;
+            fc.precision = "mixed";   // the V-cycle in FP32: a preconditioner
+            fc.cpt = 4;               // fixed launch shape (no tuning trials on this instance)
+            fc.mg_scale = params_.get_double("mg_scale", fc.mg_scale);
+            const GridHost& g = grid_->host;
+            mg_.reset(new HeatBatch(fc));
+            mg_slot_ = mg_->add_linear_multigrid(g.n[0], g.n[1], g.dim == 3 ? g.n[2] : 1);
+        }
+        CprStage st;
+        st.np = np;
+        st.setup = [this, np](const PatternP& P, const double* vals) {
+            const PressureBlock& b = pressure_block(P, np);
+            auto sv = dev_alloc<double>(std::max(1, b.pat->nnz));
+            launch_gather(vals, b.map->p, sv->p, b.pat->nnz);
+            mg_->set_linear_matrix(mg_slot_, b.pat->d_rp->p, b.pat->d_ci->p, sv->p);
+        };
+        st.apply = [this](const double* r, double* z) { mg_->vcycle(mg_slot_, r, z); };
+        return linear_solve(sc, residual, rep, &st);
+    }
     return linear_solve(sc, residual, rep);
+}
+
+// Rows and columns of the first np unknowns of P (host pass, once per pattern): the
+// sub-pattern and each of its entries' position in P.
+const RuntimeCore::PressureBlock& RuntimeCore::pressure_block(const PatternP& P, int np) const {
+    auto it = pblocks_.find(P->id);
+    if (it != pblocks_.end()) return it->second;
+    const std::vector<int>&rp = P->rp(), &ci = P->ci();
+    std::vector<int> srp(np + 1, 0), sci, map;
+    for (int r = 0; r < np; ++r) {
+        for (int k = rp[r]; k < rp[r + 1]; ++k)
+            if (ci[k] < np) {
+                sci.push_back(ci[k]);
+                map.push_back(k);
+            }
+        srp[r + 1] = (int)sci.size();
+    }
+    PressureBlock b;
+    b.pat = make_pattern(np, np, std::move(srp), std::move(sci));
+    b.map = dev_from_host(map.data(), map.size());
+    EQ_REQUIRE(stencil_pattern(b.pat), "preconditioner=cpr: the pressure block is not the grid's stencil");
+    b.owner = P;
+    return pblocks_[P->id] = b;
 }
 
 }  // namespace eqhip

@@ -101,6 +101,15 @@ private:
     mutable std::unique_ptr<HeatBatch> stencil_;         // linear-system instance of the grid
     mutable int stencil_slot_ = 0;
     mutable uint64_t sink_token_ = 0;                    // the last fused evaluation written in place
+    // CPR: the multigrid instance of the pressure block and the blocks per Jacobian pattern
+    mutable std::unique_ptr<HeatBatch> mg_;
+    mutable int mg_slot_ = 0;
+    struct PressureBlock {
+        PatternP pat, owner;
+        DevPtr<int> map;
+    };
+    mutable std::map<uint64_t, PressureBlock> pblocks_;
+    const PressureBlock& pressure_block(const PatternP& P, int np) const;
     bool stencil_solves(const PatternP& P) const;        // the Newton step of pattern P is solved on the stencil kernels
     void ensure_stencil() const;
     std::unique_ptr<DeviceGrid> grid_;
