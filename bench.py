@@ -503,8 +503,10 @@ def bench_twophase_fi(lib, args):
     implicit kind).  The workload's permeability, porosity and sources on a --grid3-fi
     grid, the program's linear relative permeabilities, dt = 0.05 (plain Newton from the
     program's initial guess sw = 0.5 converges there), each solve to 1e-8 relative
-    residual; FP64 GMRES(30) + ILU(0) with linear tolerance 1e-7 (the system is singular
-    in the pressure constants; GMRES stays above its rounding floor).  Work = cells x
+    residual; FP64 GMRES(30) with linear tolerance 1e-7 (the system is singular in the
+    pressure constants; GMRES stays above its rounding floor), preconditioned by CPR (one
+    FP32 multigrid V-cycle on the pressure block, then ILU(0) of the coupled Jacobian;
+    --fi-precond ILU0 for ILU(0) alone).  Work = cells x
     Newton iterations.  cpu_baseline: the oracle (one core) on the same case, direct or
     BiCGStab/ILU(0) solves, one step."""
     from equelle_amd.workloads import twophase_inputs, twophase_params
@@ -543,7 +545,9 @@ def bench_twophase_fi(lib, args):
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
            "config": {"workload": f"fully implicit two-phase {nx}x{ny}x{nz} (newtonSolveSystem, 2n unknowns), "
                                   "log-normal permeability, linear relative permeabilities, dt=0.05, Newton to 1e-8 "
-                                  "relative residual; device AD engine, FP64 GMRES(30)+ILU(0)",
+                                  "relative residual; device AD engine, FP64 GMRES(30) with preconditioner "
+                                  f"{args.fi_precond} (cpr: one FP32 multigrid V-cycle on the pressure block, then "
+                                  "ILU(0) of the coupled system)",
                       "cells": n, "unknowns": 2 * n},
            "newton_iterations": newton, "krylov_iterations": krylov}
     if not args.no_cpu_baseline:
@@ -839,7 +843,7 @@ def main():
     ap.add_argument("--no-graph", dest="no_graph", action="store_true", help="eager launches (profiling)")
     ap.add_argument("--grid3", default="128x128x64", help="two-phase grid nx x ny x nz")
     ap.add_argument("--grid3-fi", dest="grid3_fi", default="64x64x32", help="fully implicit two-phase grid")
-    ap.add_argument("--fi-precond", dest="fi_precond", default="ILU0",
+    ap.add_argument("--fi-precond", dest="fi_precond", default="cpr",
                     help="fully implicit: GMRES preconditioner (ILU0 | cpr | diagonal)")
     ap.add_argument("--tp-solver", dest="tp_solver", default="GMRes", help="two-phase Krylov solver")
     ap.add_argument("--tp-legs", dest="tp_legs", default="fp64,mixed",
