@@ -8,7 +8,10 @@
 //   rel_res_tol      relative Newton stop ||F(u_k)|| <= rel_res_tol ||F(u_0)||
 //                    (beside the reference's absolute abs_res_tol)
 //   precision        fp64 | mixed (FP32 Krylov + FP64 iterative refinement)
-//   preconditioner   diagonal | ilu0 | none
+//   preconditioner   diagonal | ilu0 | none | cpr (newtonSolveSystem: one multigrid V-cycle
+//                    on the pressure block, then ILU(0) of the coupled system)
+//   forcing, forcing0, eta_max   inexact Newton: each iteration's linear tolerance
+//   fuse_residual    0: evaluate newtonSolve residuals op by op (1: recorded programs)
 //   gmres_restart, inner_tol, max_refine
 #ifndef EQUELLERUNTIMECUDA_HEADER_INCLUDED
 #define EQUELLERUNTIMECUDA_HEADER_INCLUDED
