@@ -827,7 +827,7 @@ def main():
                          "preconditioner and vector kernels; XCD block order) instead of the tuner's")
     ap.add_argument("--no-extras", dest="no_extras", action="store_true",
                     help="heat headline legs only (no batched, two-phase and serving summaries)")
-    ap.add_argument("--default-rates", dest="default_rates", default="50,500",
+    ap.add_argument("--default-rates", dest="default_rates", default="20,50,500",
                     help="serving rates of the default run's compact summary")
     ap.add_argument("--default-requests", dest="default_requests", type=int, default=100)
     ap.add_argument("--precond", default="ilu0", help="preconditioner of the fused runs: ilu0 | diagonal | none")
