@@ -18,7 +18,8 @@ EXE_FI = os.path.join(GEN, "bin", "twophase_fully_implicit_cuda")
 
 
 @pytest.mark.parametrize("src", ["nonlinear_heat_cuda.cpp", "twophase_corey_cuda.cpp", "heateq_cuda.cpp",
-                                 "exp_heateq_cuda.cpp", "twophase_fully_implicit_cuda.cpp"])
+                                 "exp_heateq_cuda.cpp", "twophase_fully_implicit_cuda.cpp",
+                                 "residual_forms_cuda.cpp"])
 def test_generated_program_compiles(tmp_path, src):
     exe = tmp_path / "prog"
     subprocess.check_call(["g++", "-std=c++17", "-O1", "-I" + os.path.join(ROOT, "include"),
@@ -245,3 +246,84 @@ def test_generated_exp_heateq_runs_the_simulator_params(tmp_path, oracle_api):
             got = np.loadtxt(run / f"expU-{s:05d}.output")
             assert np.allclose(got, u.value(), rtol=1e-12, atol=1e-9), (s, np.abs(got - u.value()).max())
     assert np.allclose(np.loadtxt(run / f"expU-{nsteps:05d}.output"), u.value(), rtol=1e-12, atol=1e-9)
+
+
+EXE_FORMS = os.path.join(GEN, "bin", "residual_forms_cuda")
+
+
+def _forms_residual(form, u, uprev, dt, nx, ny, k=0.1):
+    """The residuals of residual_forms_cuda.cpp on the unit-spaced nx x ny Cartesian grid,
+    restated in numpy from the program text (face lists of the grid built here: x faces
+    between cells i and i+1, y faces between rows j and j+1; the first cell is the lower
+    one, so the gradient is u[second] - u[first] and the transmissibility is 1)."""
+    c = np.arange(nx * ny).reshape(ny, nx)
+    first = np.concatenate([c[:, :-1].ravel(), c[:-1, :].ravel()])
+    second = np.concatenate([c[:, 1:].ravel(), c[1:, :].ravel()])
+
+    def div(flux):
+        return np.bincount(first, flux, nx * ny) - np.bincount(second, flux, nx * ny)
+
+    if form == 0:
+        kc = k * (1 + np.sqrt(u * u + 0.25))
+    elif form == 1:
+        kc = k * (1 + u * u)
+    else:
+        kc = k / (1 + u.max()) * (1 + u * u)
+    kf = 0.5 * (kc[first] + kc[second])
+    flux = -kf * (u[second] - u[first])
+    sink = np.zeros_like(u)
+    if form == 0:
+        up = np.where(u[first] > u[second], u[first], u[second])
+        flux = flux * (1 + 0.1 * np.abs(up)) / (1 + 0.05 * up * up)
+        bnd = np.zeros((ny, nx), dtype=bool)
+        bnd[0, :] = bnd[-1, :] = bnd[:, 0] = bnd[:, -1] = True
+        sink = np.where(bnd.ravel(), 0.01 * u, 0.0)
+    return (u - uprev) + dt * (div(flux) + sink)
+
+
+@pytest.mark.gpu
+def test_generated_residual_forms_fused_and_op_by_op(tmp_path):
+    """Recorded residual programs behind the facade (residual_forms_cuda.cpp): a residual
+    with sqrt, norm, division, a trinaryIf on a comparison of the unknown and On / Extend of
+    the boundary cells; one with a values-only coefficient (u.value()); and one with a
+    reduction of the unknown inside the functor, which stops the recording.  The fused run
+    (fuse_residual=1) and the op-by-op run (fuse_residual=0, the reference's evaluation)
+    agree to 1e-10, and every saved state solves its step's equation, evaluated in numpy."""
+    assert os.path.exists(EXE_FORMS), "run __graft_entry__.build() first"
+    nx, ny = 40, 32
+    n = nx * ny
+    outs, logs = {}, {}
+    for fuse in (1, 0):
+        d = tmp_path / f"fuse{fuse}"
+        d.mkdir()
+        (d / "run.param").write_text("\n".join([
+            "grid_dim=2", f"nx={nx}", f"ny={ny}", "dx=1", "dy=1", "k=0.1", "output_to_file=true",
+            "abs_res_tol=0", "rel_res_tol=1e-12", "max_iter=80", "solver_tol=1e-13",
+            f"fuse_residual={fuse}"]) + "\n")
+        r = subprocess.run([EXE_FORMS, "run.param"], cwd=d, check=True, timeout=120, capture_output=True,
+                           text=True)
+        logs[fuse] = {}
+        for line in r.stdout.splitlines():
+            if " stats " in line:
+                head, rest = line.split(" stats ")
+                logs[fuse][int(head.split()[1])] = {kv.split("=")[0]: int(kv.split("=")[1]) for kv in rest.split()}
+        outs[fuse] = {(f, s): np.loadtxt(d / f"form{f}-{s:05d}.output") for f in range(3) for s in range(3)}
+    s = logs[1]
+    assert set(s) == {0, 1, 2}, s
+    assert s[2]["rejected"] == 0, s
+    # forms 0 and 1 run fused: recorded once, then found symbolically on the later steps
+    assert s[1]["aborted"] == 0 and s[0]["evaluations"] > 0 and s[1]["evaluations"] > s[0]["evaluations"], s
+    assert s[0]["symbolic"] >= 2 and s[1]["symbolic"] >= s[0]["symbolic"] + 2, s
+    # form 2 reads a reduction of the unknown: its recording stops, and it runs op by op
+    assert s[2]["aborted"] >= 1 and s[2]["evaluations"] == s[1]["evaluations"], s
+    assert all(v["evaluations"] == 0 for v in logs[0].values()), logs[0]
+    u_init = np.array([0.5 + 0.4 * np.sin(0.37 * c) + 0.3 * np.cos(0.11 * c) for c in range(n)])
+    dts = [0.5, 1.0, 1.0]
+    for f in range(3):
+        prev = u_init
+        for st in range(3):
+            uf, ue = outs[1][(f, st)], outs[0][(f, st)]
+            assert np.allclose(uf, ue, rtol=1e-10, atol=1e-10), (f, st, np.abs(uf - ue).max())
+            res = _forms_residual(f, uf, prev, dts[st], nx, ny)
+            assert np.abs(res).max() < 1e-9, (f, st, np.abs(res).max())
+            prev = uf
