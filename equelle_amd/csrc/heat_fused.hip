@@ -27,8 +27,10 @@ inline hipStream_t S() { return Device::get().stream(); }
 // instance's base inner (mixed-precision) tolerance; S_NK: Newton iterations of the step
 // (stage) so far; S_R1, S_R1B: ||F_1|| / ||F_0|| of the latest step (two-phase: of its
 // pressure / transport stage), 0 until measured
+// S_XUPD: the four-ranks-per-lane BiCGStab's pending solution update (set when the
+// iteration's second finaliser ran for the instance, cleared by the next first finaliser)
 enum Slot { S_RHO = 0, S_RHO_OLD, S_ALPHA, S_OMEGA, S_RR, S_BB, S_BREAK, S_IT, S_FN, S_RN, S_BN, S_REF, S_ITT, S_TOL2,
-            S_MEAN, S_BETA, S_CONT, S_F0, S_LTOL, S_ITOL, S_NK, S_R1, S_R1B, NSLOT = 24 };
+            S_MEAN, S_BETA, S_CONT, S_F0, S_LTOL, S_ITOL, S_NK, S_R1, S_R1B, S_XUPD, NSLOT = 24 };
 // Krylov vectors; V_Y is the red-rank scratch of the reduced operator (ilu0 only), V_Z the
 // V-cycle's z = M^-1 r (two-phase pressure by CG with multigrid)
 enum Vec { V_R = 0, V_RH, V_P, V_V, V_S, V_T, V_PH, V_SH, V_X, V_Y, V_Z, NVEC };
@@ -220,12 +222,14 @@ __device__ inline void cell_ijk(const Desc& d, int c, int& i, int& j, int& k) {
 
 // Per-instance finalisation, run by one whole block: sums the instance's block
 // partials in a fixed order (deterministic), then thread 0 applies the action to
-// the instance's scalars.
+// the instance's scalars.  ND: the dot products (partial sets) the kernel can sum, 2 or 5;
+// each set is summed in the same order for either ND.
+template <int ND>
 __device__ void fin_body(const Desc& d, const double* part, int nb, const Fin& f) {
     if (!f.kry && gated_off(d)) return;   // norms of gated cell kernels (assembly, defect)
     const int2 r = make_int2(d.gbase, d.gbase + (f.kry ? d.ngrpk : d.ngrp));
-    __shared__ double res[2];
-    __shared__ double lds[2][FBS / 64];
+    __shared__ double res[5];
+    __shared__ double lds[ND][FBS / 64];
     // the scalars the action reads, loaded by thread 0 before the partials so both round
     // trips overlap (only this block writes the instance's scalars and flags)
     int a0 = 0, a1 = 0, mth = M_BICG;
@@ -241,38 +245,51 @@ __device__ void fin_body(const Desc& d, const double* part, int nb, const Fin& f
         p_tol2 = d.sc[S_TOL2];
         p_bb = d.sc[S_BB];
     }
-    // partials were written by the previous kernel (visible at this kernel's start); both
-    // dots' partials are loaded together, eight per thread, before any is added, so the
-    // kernel waits for memory once per 8 x FBS groups
-    double s0 = 0.0, s1 = 0.0;
-    for (int base = r.x; base < r.y; base += 8 * FBS) {
-        double v0[8], v1[8];
+    // partials were written by the previous kernel (visible at this kernel's start); the
+    // sets' partials are loaded together before any is added (eight per thread and set for
+    // ND = 2, two rounds of four for ND = 5), and each set adds a chunk of eight partials
+    // as ((v0 + v1) + (v2 + v3)) + ((v4 + v5) + (v6 + v7)) either way
+    constexpr int U = ND > 2 ? 4 : 8;
+    double s[ND];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const int b = base + (int)threadIdx.x + u * FBS;
-            v0[u] = b < r.y ? part[b] : 0.0;
-            v1[u] = (f.nd > 1 && b < r.y) ? part[(long)nb + b] : 0.0;
+    for (int q = 0; q < ND; ++q) s[q] = 0.0;
+    for (int base = r.x; base < r.y; base += 8 * FBS) {
+        double c[ND][8 / U];
+#pragma unroll
+        for (int half = 0; half < 8 / U; ++half) {
+            double v[ND][U];
+#pragma unroll
+            for (int q = 0; q < ND; ++q)
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int b = base + (int)threadIdx.x + (half * U + u) * FBS;
+                    v[q][u] = (q < f.nd && b < r.y) ? part[(long)q * nb + b] : 0.0;
+                }
+#pragma unroll
+            for (int q = 0; q < ND; ++q) {
+                if (U == 8)
+                    c[q][half] = ((v[q][0] + v[q][1]) + (v[q][2 % U] + v[q][3 % U])) +
+                                 ((v[q][4 % U] + v[q][5 % U]) + (v[q][6 % U] + v[q][7 % U]));
+                else
+                    c[q][half] = (v[q][0] + v[q][1]) + (v[q][2 % U] + v[q][3 % U]);
+            }
         }
-        s0 += ((v0[0] + v0[1]) + (v0[2] + v0[3])) + ((v0[4] + v0[5]) + (v0[6] + v0[7]));
-        s1 += ((v1[0] + v1[1]) + (v1[2] + v1[3])) + ((v1[4] + v1[5]) + (v1[6] + v1[7]));
+#pragma unroll
+        for (int q = 0; q < ND; ++q) s[q] += U == 8 ? c[q][0] : c[q][0] + c[q][(8 / U) - 1];
     }
-    for (int off = 32; off > 0; off >>= 1) {
-        s0 += __shfl_down(s0, off, 64);
-        s1 += __shfl_down(s1, off, 64);
-    }
-    if ((threadIdx.x & 63) == 0) {
-        lds[0][threadIdx.x >> 6] = s0;
-        lds[1][threadIdx.x >> 6] = s1;
+#pragma unroll
+    for (int q = 0; q < ND; ++q) {
+        for (int off = 32; off > 0; off >>= 1) s[q] += __shfl_down(s[q], off, 64);
+        if ((threadIdx.x & 63) == 0) lds[q][threadIdx.x >> 6] = s[q];
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        double a = 0.0, b = 0.0;
-        for (int i = 0; i < FBS / 64; ++i) {
-            a += lds[0][i];
-            b += lds[1][i];
+        for (int q = 0; q < 5; ++q) {
+            double a = 0.0;
+            if (q < ND)
+                for (int i = 0; i < FBS / 64; ++i) a += lds[q][i];
+            res[q] = a;
         }
-        res[0] = a;
-        res[1] = b;
     }
     if (threadIdx.x != 0) return;
     const int act = f.act, maxit = f.maxit, slot = f.slot;
@@ -289,25 +306,45 @@ __device__ void fin_body(const Desc& d, const double* part, int nb, const Fin& f
             sc[S_BB] = res[1];
             sc[S_IT] = 0.0;
             sc[S_BREAK] = 0.0;
+            sc[S_XUPD] = 0.0;
             sc[S_ALPHA] = sc[S_OMEGA] = sc[S_RHO_OLD] = 1.0;
             d.act[0] = res[1] > 0.0 && isfinite(res[1]);
             break;
         }
         case A_ALPHA:
+            sc[S_XUPD] = 0.0;   // the previous iteration's update has been applied
             if (!a0 || mth != M_BICG) break;
             if (res[0] == 0.0 || !isfinite(res[0])) {
                 sc[S_BREAK] = 1.0;
                 d.act[0] = 0;
             } else sc[S_ALPHA] = p_rho / res[0];
             break;
-        case A_OMEGA:
+        case A_OMEGA: {
             if (!a0 || mth != M_BICG) break;
-            if (res[1] == 0.0 || !isfinite(res[1])) {
-                sc[S_OMEGA] = 0.0;   // s already solves the system; finish with x += alpha ph
-            } else sc[S_OMEGA] = res[0] / res[1];
+            const double omega = (res[1] == 0.0 || !isfinite(res[1])) ? 0.0   // s already solves the system
+                                                                       : res[0] / res[1];
+            sc[S_OMEGA] = omega;
+            if (!d.vec) break;
+            // four-ranks-per-lane form: the end of the iteration without a third reduction.
+            // From t.s, t.t, rh.t, rh.s, s.s of the stored s and t: the next rho = rh.r and
+            // ||r||^2 of r = s - omega t, as rh.s - omega rh.t and s.s - 2 omega t.s + omega^2
+            // t.t (clamped at 0); k_b31 then forms x, r and the next search direction.
+            const double rho = res[3] - omega * res[2];
+            const double rr = fmax(0.0, res[4] - 2.0 * omega * res[0] + omega * omega * res[1]);
+            sc[S_RHO_OLD] = p_rho;
+            sc[S_RHO] = rho;
+            sc[S_RR] = rr;
+            const double it = p_it + 1.0;
+            sc[S_IT] = it;
+            sc[S_ITT] = p_itt + 1.0;
+            sc[S_XUPD] = 1.0;
+            const bool ok = isfinite(rr) && isfinite(rho) && omega != 0.0;
+            if (!ok) sc[S_BREAK] = 2.0;
+            d.act[0] = ok && rr > (tol2 < 0.0 ? p_tol2 : tol2) * p_bb && it < maxit && rho != 0.0;
             break;
+        }
         case A_RHO: {
-            if (!a0 || mth != M_BICG) break;
+            if (!a0 || mth != M_BICG || d.vec) break;   // (the four-ranks-per-lane form: in A_OMEGA)
             sc[S_RHO_OLD] = p_rho;
             sc[S_RHO] = res[0];
             sc[S_RR] = res[1];
@@ -360,9 +397,10 @@ __device__ void fin_body(const Desc& d, const double* part, int nb, const Fin& f
 }
 
 
+template <int ND>
 __global__ __launch_bounds__(FBS) void k_fin(const Desc* __restrict__ D, const double* __restrict__ part, int nb,
                                             Fin f) {
-    fin_body(D[blockIdx.x], part, nb, f);
+    fin_body<ND>(D[blockIdx.x], part, nb, f);
 }
 
 // ---------------------------------------------------------------------------
@@ -965,7 +1003,9 @@ __global__ __launch_bounds__(BS) void k_schur_red(const Desc* __restrict__ D, co
     }
 }
 // Reduced operator, pass 2 (black ranks): w = S v = C_b v_b - A_br y_r, into V[iw], with
-// fused partial dot products dot0 = V[a0].w, dot1 = V[a1].w (a1 < 0: w.w).
+// fused partial dot products dot0 = V[a0].w, dot1 = V[a1].w (a1 < 0: w.w).  nd = 5 (the
+// four-ranks-per-lane form; the other forms take it as 2): also dot2 = rh.w, dot3 =
+// rh.V[a0], dot4 = V[a0].V[a0] (a1 < 0).
 template <class T, int CPT>
 __global__ __launch_bounds__(BS) void k_schur_black(const Desc* __restrict__ D, const int2* __restrict__ blk,
                                                     int iv, int iw, int nd, int a0, int a1,
@@ -985,7 +1025,7 @@ __global__ __launch_bounds__(BS) void k_schur_black(const Desc* __restrict__ D, 
                           (const T*)d.K[4] + h};
         const bool on = d.act[0] != 0;
         FOR_VGROUPS(g, p0) {
-            double s0 = 0.0, s1 = 0.0;
+            double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0;
             if (on && p0 < h) {
                 int j, q0;
                 vrow(d, p0, j, q0);
@@ -997,9 +1037,20 @@ __global__ __launch_bounds__(BS) void k_schur_black(const Desc* __restrict__ D, 
                 st4(w + p0, wc);
                 s0 = dot4d(aa, wc);
                 if (nd > 1) s1 = dot4d(vb ? ld4(vb + p0) : wc, wc);
+                if (nd > 2) {
+                    const V4<T> rh = ld4((const T*)d.v[V_RH] + p0);
+                    s2 = dot4d(rh, wc);
+                    s3 = dot4d(rh, aa);
+                    s4 = dot4d(aa, aa);
+                }
             }
             vgroup_store(d, bi, CPT, g, s0, part);
             if (nd > 1) vgroup_store(d, bi, CPT, g, s1, part + ng);
+            if (nd > 2) {
+                vgroup_store(d, bi, CPT, g, s2, part + 2 * (long)ng);
+                vgroup_store(d, bi, CPT, g, s3, part + 3 * (long)ng);
+                vgroup_store(d, bi, CPT, g, s4, part + 4 * (long)ng);
+            }
         }
         return;
     }
@@ -1185,14 +1236,20 @@ template <class T, int CPT>
 __global__ __launch_bounds__(BS) void k_kinit(const Desc* __restrict__ D, const int2* __restrict__ blk,
                                               double* __restrict__ part, int ng) {
     LOCATE();
-    if (d.vec) {   // heat, reduced form (BiCGStab; not singular): rh = r, x = 0 (p and v are not
-                   // read before the first iteration writes them)
+    if (d.vec) {   // reduced form, four ranks per lane (BiCGStab; not singular): rh = r, x = 0,
+                   // the first search direction p = r and ph = M^-1 p (later ones come from k_b31)
         const bool on = d.act[1] != 0;
+        const T* __restrict__ dv = slot_dinv<T>(d);
         FOR_VGROUPS(g, p0) {
             double rr = 0.0;
             if (on && p0 < d.h) {
-                const V4<T> r = ld4((const T*)d.v[V_R] + p0);
+                const V4<T> r = ld4((const T*)d.v[V_R] + p0), dd = ld4(dv + p0);
+                V4<T> ph;
+#pragma unroll
+                for (int m = 0; m < 4; ++m) ph.e[m] = dd.e[m] * r.e[m];
                 st4((T*)d.v[V_RH] + p0, r);
+                st4((T*)d.v[V_P] + p0, r);
+                st4((T*)d.v[V_PH] + p0, ph);
                 st4((T*)d.v[V_X] + p0, V4<T>{{(T)0, (T)0, (T)0, (T)0}});
                 rr = dot4d(r, r);
             }
@@ -1326,36 +1383,17 @@ __global__ __launch_bounds__(BS) void k_cg3(const Desc* __restrict__ D, const in
 }
 
 // p = r + beta (p - omega v) (first iteration: p = r); ph = M^-1 p (diagonal M; PM_ILU:
-// by the ILU passes)
+// by the ILU passes).  Strided forms (the four-ranks-per-lane form: k_kinit and k_b31).
 template <class T, int CPT>
 __global__ __launch_bounds__(BS) void k_b1(const Desc* __restrict__ D, const int2* __restrict__ blk) {
     LOCATE();
-    if (!d.act[0] || meth_of(d) != M_BICG) return;
+    if (d.vec || !d.act[0] || meth_of(d) != M_BICG) return;
     const double* sc = d.sc;
     T* p = (T*)d.v[V_P];
     const bool first = sc[S_IT] == 0.0;
     const T beta = first ? (T)0 : (T)((sc[S_RHO] / sc[S_RHO_OLD]) * (sc[S_ALPHA] / sc[S_OMEGA]));
     const T omega = (T)sc[S_OMEGA];
     const T* dv = slot_dinv<T>(d);
-    if (d.vec) {   // reduced system: one slot, ph always written
-        FOR_VGROUPS(g, p0) {
-            if (p0 >= d.h) continue;
-            const V4<T> r = ld4((const T*)d.v[V_R] + p0), dd = ld4(dv + p0);
-            V4<T> pv, ph;
-            if (first) {
-                pv = r;
-            } else {
-                const V4<T> po = ld4(p + p0), vv = ld4((const T*)d.v[V_V] + p0);
-#pragma unroll
-                for (int m = 0; m < 4; ++m) pv.e[m] = r.e[m] + beta * (po.e[m] - omega * vv.e[m]);
-            }
-#pragma unroll
-            for (int m = 0; m < 4; ++m) ph.e[m] = dd.e[m] * pv.e[m];
-            st4(p + p0, pv);
-            st4((T*)d.v[V_PH] + p0, ph);
-        }
-        return;
-    }
     FOR_PAIRS(q) {
         for (int w = 0; w < d.ns; ++w) {
             const int s = q + w * d.h;
@@ -1397,38 +1435,14 @@ __global__ __launch_bounds__(BS) void k_b2(const Desc* __restrict__ D, const int
         }
     }
 }
-// x += alpha ph + omega sh ; r = s - omega t ; partial dots (rh.r, r.r)
+// x += alpha ph + omega sh ; r = s - omega t ; partial dots (rh.r, r.r).  Strided forms.
 template <class T, int CPT>
 __global__ __launch_bounds__(BS) void k_b3(const Desc* __restrict__ D, const int2* __restrict__ blk,
                                            double* __restrict__ part, int ng) {
     LOCATE();
-    if (meth_of(d) != M_BICG) return;   // block-uniform; the CG kernels write these partials
-    if (d.vec) {
-        const bool on = d.act[0] != 0;
-        const T alpha = on ? (T)d.sc[S_ALPHA] : (T)0, omega = on ? (T)d.sc[S_OMEGA] : (T)0;
-        T* x = (T*)d.v[V_X];
-        FOR_VGROUPS(g, p0) {
-            double s0 = 0.0, s1 = 0.0;
-            if (on && p0 < d.h) {
-                const V4<T> xo = ld4(x + p0), ph = ld4((const T*)d.v[V_PH] + p0), sh = ld4((const T*)d.v[V_SH] + p0);
-                const V4<T> sv = ld4((const T*)d.v[V_S] + p0), tv = ld4((const T*)d.v[V_T] + p0);
-                const V4<T> rh = ld4((const T*)d.v[V_RH] + p0);
-                V4<T> xn, r;
-#pragma unroll
-                for (int m = 0; m < 4; ++m) {
-                    xn.e[m] = xo.e[m] + (alpha * ph.e[m] + omega * sh.e[m]);
-                    r.e[m] = sv.e[m] - omega * tv.e[m];
-                }
-                st4(x + p0, xn);
-                st4((T*)d.v[V_R] + p0, r);
-                s0 = dot4d(rh, r);
-                s1 = dot4d(r, r);
-            }
-            vgroup_store(d, bi, CPT, g, s0, part);
-            vgroup_store(d, bi, CPT, g, s1, part + ng);
-        }
-        return;
-    }
+    // block-uniform; the CG kernels write these partials, and the four-ranks-per-lane form
+    // ends its iterations in k_b31 (no partials)
+    if (meth_of(d) != M_BICG || d.vec) return;
     double a0[CPT] = {}, a1[CPT] = {};
     if (d.act[0]) {
         const T alpha = (T)d.sc[S_ALPHA], omega = (T)d.sc[S_OMEGA];
@@ -1449,6 +1463,50 @@ __global__ __launch_bounds__(BS) void k_b3(const Desc* __restrict__ D, const int
     group_store<CPT>(a1, PARTSK(part + ng));
 }
 
+// Four-ranks-per-lane form, the end of a BiCGStab iteration and the start of the next in
+// one pass (rho, ||r||^2 and beta come from the second finaliser, A_OMEGA):
+//   x += alpha ph + omega sh ; r = s - omega t ;
+//   while the solve goes on: p = r + beta (p - omega v), ph = M^-1 p.
+// S_XUPD says the iteration's update is pending (the instance was active when it started).
+template <class T, int CPT>
+__global__ __launch_bounds__(BS) void k_b31(const Desc* __restrict__ D, const int2* __restrict__ blk) {
+    LOCATE();
+    (void)base;
+    if (!d.vec || meth_of(d) != M_BICG || d.sc[S_XUPD] == 0.0) return;
+    const double* sc = d.sc;
+    const bool more = d.act[0] != 0;
+    const T alpha = (T)sc[S_ALPHA], omega = (T)sc[S_OMEGA];
+    const T beta = more ? (T)((sc[S_RHO] / sc[S_RHO_OLD]) * (sc[S_ALPHA] / sc[S_OMEGA])) : (T)0;
+    T* __restrict__ x = (T*)d.v[V_X];
+    T* __restrict__ r = (T*)d.v[V_R];
+    T* __restrict__ p = (T*)d.v[V_P];
+    T* __restrict__ ph = (T*)d.v[V_PH];
+    const T* __restrict__ dv = slot_dinv<T>(d);
+    FOR_VGROUPS(g, p0) {
+        if (p0 >= d.h) continue;
+        const V4<T> xo = ld4(x + p0), pho = ld4(ph + p0), sh = ld4((const T*)d.v[V_SH] + p0);
+        const V4<T> sv = ld4((const T*)d.v[V_S] + p0), tv = ld4((const T*)d.v[V_T] + p0);
+        V4<T> xn, rn;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            xn.e[m] = xo.e[m] + (alpha * pho.e[m] + omega * sh.e[m]);
+            rn.e[m] = sv.e[m] - omega * tv.e[m];
+        }
+        st4(x + p0, xn);
+        st4(r + p0, rn);
+        if (more) {
+            const V4<T> po = ld4(p + p0), vv = ld4((const T*)d.v[V_V] + p0), dd = ld4(dv + p0);
+            V4<T> pn, phn;
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                pn.e[m] = rn.e[m] + beta * (po.e[m] - omega * vv.e[m]);
+                phn.e[m] = dd.e[m] * pn.e[m];
+            }
+            st4(p + p0, pn);
+            st4(ph + p0, phn);
+        }
+    }
+}
 
 // ---------------------------------------------------------------------------
 // Aggregation multigrid for the two-phase pressure stage (Desc::mgp: full red-black
@@ -2176,6 +2234,8 @@ struct Table {
     int ng = 0;              // partial groups over all instances (an instance has ceil(n / BS))
     bool has_schur = false, has_full = false, has_ilu = false;   // Krylov forms present
     bool has_tp = false;                                         // two-phase instances present
+    // instances whose Krylov kernels run in the four-ranks-per-lane form, and the others
+    bool has_vec = false, has_nonvec = false;
     // Multigrid plan over the instances with a hierarchy; index l = 1 .. levels - 1: the
     // instances that have level l, their level l - 1 (fine) and level l (coarse)
     // descriptors, blocks over the coarse cells, the fine pairs and the coarse pairs.
@@ -2476,6 +2536,8 @@ struct HeatBatch::Impl {
             t.has_full |= d.pcm != PM_SCHUR;
             t.has_ilu |= d.pcm == PM_ILU;
             t.has_tp |= d.kind == K_TWOPHASE;
+            t.has_vec |= d.vec != 0;
+            t.has_nonvec |= d.vec == 0;
         }
         t.ng = g;
         for (int ci = 0; ci < 4; ++ci) {
@@ -2492,7 +2554,7 @@ struct HeatBatch::Impl {
             t.pblk[ci] = dev_from_host(pblk.data(), pblk.size());
         }
         t.desc = dev_from_host(ds.data(), ds.size());
-        t.part = dev_alloc<double>(2 * (size_t)t.ng + 2);
+        t.part = dev_alloc<double>(5 * (size_t)t.ng + 2);   // up to five dot products per launch
         build_mg(t);
         return tp;
     }
@@ -2595,7 +2657,10 @@ struct HeatBatch::Impl {
         // (unchanged values); du itself, which an instance between refinement rounds keeps,
         // is not written.
         const int one[2] = {1, 1};
-        for (Inst* I : list) EQ_HIP(hipMemcpyAsync(I->act->p, one, sizeof(one), hipMemcpyHostToDevice, S()));
+        for (Inst* I : list) {
+            EQ_HIP(hipMemcpyAsync(I->act->p, one, sizeof(one), hipMemcpyHostToDevice, S()));
+            launch_fill(I->sc->p + S_XUPD, 1.0, 1);   // k_b31 does its full work (the next solve clears it)
+        }
         static const double weight[NCLS] = {1.0, 2.0, 16.0, 2.0, 8.0};
         hipEvent_t e0, e1;
         EQ_HIP(hipEventCreate(&e0));
@@ -2663,7 +2728,12 @@ struct HeatBatch::Impl {
     // Finalisation of a reduction: one block per instance (kernel boundaries make the
     // partials of all blocks visible; no device-wide fences inside the kernels).
     void fin(const Table& t, const Fin& f) {
-        hipLaunchKernelGGL(k_fin, dim3((unsigned)t.inst.size()), dim3(FBS), 0, S(), t.desc->p, t.part->p, t.ng, f);
+        if (f.nd > 2)
+            hipLaunchKernelGGL(k_fin<5>, dim3((unsigned)t.inst.size()), dim3(FBS), 0, S(), t.desc->p, t.part->p, t.ng,
+                               f);
+        else
+            hipLaunchKernelGGL(k_fin<2>, dim3((unsigned)t.inst.size()), dim3(FBS), 0, S(), t.desc->p, t.part->p, t.ng,
+                               f);
         EQ_HIP(hipGetLastError());
     }
     // finaliser of a Krylov reduction (partials over pairs) ...
@@ -2800,9 +2870,10 @@ struct HeatBatch::Impl {
     }
     template <class T>
     void vec_round(const Table& t) {
-        HLP(C_VEC, (k_b1<T, CP>));
+        if (t.has_nonvec) HLP(C_VEC, (k_b1<T, CP>));
         HLP(C_VEC, (k_b2<T, CP>));
-        HLP(C_VEC, (k_b3<T, CP>), t.part->p, t.ng);
+        if (t.has_nonvec) HLP(C_VEC, (k_b3<T, CP>), t.part->p, t.ng);
+        if (t.has_vec) HLP(C_VEC, (k_b31<T, CP>));
     }
     template <class T>
     void cg_round(const Table& t) {
@@ -2813,19 +2884,29 @@ struct HeatBatch::Impl {
     // One Krylov iteration for the instances whose refinement flag is set (rhs in V_R):
     // BiCGStab (right preconditioner: ph = M^-1 p and sh = M^-1 s are formed in k_b1 /
     // k_b2) for the M_BICG instances, preconditioned CG for the M_CG ones; meth = the
-    // methods present in the solve.
+    // methods present in the solve.  The four-ranks-per-lane instances take their first
+    // direction from k_kinit, and the second operator's five dot products give them the
+    // iteration's end in A_OMEGA: k_b31 updates x and r and forms the next direction (two
+    // reductions per iteration instead of three).
     template <class T>
     void krylov_iteration(const Table& t, double tol, int maxit, int meth) {
         const double tol2 = tol < 0.0 ? -1.0 : tol * tol;
         if (meth & M_BICG) {
-            HLP(C_VEC, (k_b1<T, CP>));
-            ilu_apply<T>(t, V_P, V_PH);
+            const int nd2 = t.has_vec ? 5 : 2;
+            if (t.has_nonvec) {
+                HLP(C_VEC, (k_b1<T, CP>));
+                ilu_apply<T>(t, V_P, V_PH);
+            }
             spmv_dot<T>(t, V_PH, V_V, 1, (int)V_RH, -1, F(A_ALPHA, 1), M_BICG);   // v = A ph; alpha = rho / rh.v
             HLP(C_VEC, (k_b2<T, CP>));
             ilu_apply<T>(t, V_S, V_SH);
-            spmv_dot<T>(t, V_SH, V_T, 2, (int)V_S, -1, F(A_OMEGA, 2), M_BICG);    // t = A sh; omega = t.s / t.t
-            HLP(C_VEC, (k_b3<T, CP>), t.part->p, t.ng);
-            fin(t, F(A_RHO, 2, tol2, maxit));
+            // t = A sh; omega = t.s / t.t (four ranks per lane: rho, ||r|| and the convergence too)
+            spmv_dot<T>(t, V_SH, V_T, nd2, (int)V_S, -1, F(A_OMEGA, nd2, tol2, maxit), M_BICG);
+            if (t.has_nonvec) {
+                HLP(C_VEC, (k_b3<T, CP>), t.part->p, t.ng);
+                fin(t, F(A_RHO, 2, tol2, maxit));
+            }
+            if (t.has_vec) HLP(C_VEC, (k_b31<T, CP>));
         }
         if (meth & M_CG) {
             spmv_dot<T>(t, V_P, V_V, 1, (int)V_P, -1, F(A_CG_ALPHA, 1), M_CG);    // q = A p; alpha = rho / p.q
@@ -3612,6 +3693,7 @@ double HeatBatch::time_kernel(const std::string& name, int reps, double* event_m
     for (Inst* I : all) {
         const int one[2] = {1, 1};
         EQ_HIP(hipMemcpyAsync(I->act->p, one, sizeof(one), hipMemcpyHostToDevice, S()));
+        launch_fill(I->sc->p + S_XUPD, 1.0, 1);
     }
     Device::get().sync();
     std::function<void(hipEvent_t, hipEvent_t)> launch;
@@ -3672,13 +3754,17 @@ double HeatBatch::time_kernel(const std::string& name, int reps, double* event_m
     } else if (name == "b3") {
         if (P.mixed) TLP(C_VEC, (k_b3<float, CP>), t.part->p, t.ng);
         else TLP(C_VEC, (k_b3<double, CP>), t.part->p, t.ng);
+    } else if (name == "b31") {
+        if (P.mixed) TLP(C_VEC, (k_b31<float, CP>));
+        else TLP(C_VEC, (k_b31<double, CP>));
     } else if (name == "load_rhs") {
         if (P.mixed) TLP(C_PC, (k_load_rhs<float, CP>), 1, 0);
         else TLP(C_PC, (k_load_rhs<double, CP>), 1, 0);
     } else {
         EQ_REQUIRE(false, "unknown kernel " << name << " (operator_red, operator_black [reduced system], spmv, "
                                                        "ilu_black, ilu_red [full system], assemble, defect [mixed], "
-                                                       "b1, b2, b3, load_rhs)");
+                                                       "b1, b2, b3 [strided form], b31 [four ranks per lane], "
+                                                       "load_rhs)");
     }
 #undef TL
 #undef TLP
