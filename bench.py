@@ -116,7 +116,7 @@ def run_fused(lib, precision, u0, nx, ny, dt, steps, warmup, precond="ilu0", gra
         # the reduced form with nx % 8 == 0 runs its Krylov vectors four ranks per lane, whose
         # iterations end in one fused update (b31) instead of b1 and b3
         vec = precond == "ilu0" and nx % 8 == 0
-        names = (("operator_red", "operator_black") if precond == "ilu0" else ("spmv",)) + \
+        names = (("operator",) if vec else ("operator_red", "operator_black") if precond == "ilu0" else ("spmv",)) + \
             ("assemble",) + (("defect",) if precision == "mixed" else ()) + \
             (("b2", "b31") if vec else ("b1", "b2", "b3")) + ("load_rhs",)
         for name in names:
@@ -738,6 +738,9 @@ def traffic_of(measured, kernel):
 # vector updates once; every refinement round loads a right-hand side and runs a defect;
 # every Newton iteration and every step start assembles.
 KERNELS = {
+    "operator": ("k_schur_vec<T, {cpt}>", "spmv", lambda n, h, ks: 13 * ks * h,
+                 "one pass (y_r formed in registers): red 4 coefficients, 1/C_r; black C_b, 4 coefficients; "
+                 "v_b, dot operand, w written (13 ks h)"),
     "operator_red": ("k_schur_red<T, {cpt}>", "spmv", lambda n, h, ks: 7 * ks * h,
                      "red ranks: 4 coefficients, 1/C_r, v_b, y_r written (7 ks h)"),
     "operator_black": ("k_schur_black<T, {cpt}>", "spmv", lambda n, h, ks: 9 * ks * h,
@@ -760,7 +763,8 @@ KERNELS = {
 def kernel_table(run, n, ks, steps):
     h = (n + 1) // 2
     kern = run["kern"]
-    per = {"operator_red": 2 * run["krylov"], "operator_black": 2 * run["krylov"], "spmv": 2 * run["krylov"],
+    per = {"operator": 2 * run["krylov"], "operator_red": 2 * run["krylov"], "operator_black": 2 * run["krylov"],
+           "spmv": 2 * run["krylov"],
            "b1": run["krylov"], "b2": run["krylov"], "b3": run["krylov"], "b31": run["krylov"],
            "assemble": run["newton"] + steps,
            "defect": run["refine"], "load_rhs": run["refine"] if ks == 4 else run["newton"]}

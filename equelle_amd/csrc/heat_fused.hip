@@ -973,29 +973,15 @@ __device__ inline T offdiag(const Desc& d, int s, int c, int i, int j, int k, co
 }
 
 // Reduced operator, pass 1 (red ranks): y_r = C_r^-1 (A_rb v_b), v = V[iv] (black slots).
+// Strided form (the four-ranks-per-lane form applies the operator in one pass, k_schur_vec).
 template <class T, int CPT>
 __global__ __launch_bounds__(BS) void k_schur_red(const Desc* __restrict__ D, const int2* __restrict__ blk, int iv,
                                                   int meth) {
     LOCATE();
-    if (d.pcm != PM_SCHUR || !d.act[0] || !(meth & meth_of(d))) return;
+    if (d.pcm != PM_SCHUR || d.vec || !d.act[0] || !(meth & meth_of(d))) return;
     const T* __restrict__ v = (const T*)d.v[iv];
     const T* __restrict__ dv = (const T*)d.dinv;
     T* __restrict__ y = (T*)d.v[V_Y];
-    if (d.vec) {
-        const T* kq[5] = {nullptr, (const T*)d.K[1], (const T*)d.K[2], (const T*)d.K[3], (const T*)d.K[4]};
-        FOR_VGROUPS(g, p0) {
-            if (p0 >= d.h) continue;
-            int j, q0;
-            vrow(d, p0, j, q0);
-            const V4<T> o = voffdiag<T>(d, kq, p0, j, q0, 0, v);
-            const V4<T> dd = ld4(dv + p0);
-            V4<T> r;
-#pragma unroll
-            for (int m = 0; m < 4; ++m) r.e[m] = o.e[m] * dd.e[m];
-            st4(y + p0, r);
-        }
-        return;
-    }
     FOR_PAIRS(p) {
         int i, j, k;
         const int c = cell_of(d, p, 0, i, j, k);
@@ -1003,57 +989,15 @@ __global__ __launch_bounds__(BS) void k_schur_red(const Desc* __restrict__ D, co
     }
 }
 // Reduced operator, pass 2 (black ranks): w = S v = C_b v_b - A_br y_r, into V[iw], with
-// fused partial dot products dot0 = V[a0].w, dot1 = V[a1].w (a1 < 0: w.w).  nd = 5 (the
-// four-ranks-per-lane form; the other forms take it as 2): also dot2 = rh.w, dot3 =
-// rh.V[a0], dot4 = V[a0].V[a0] (a1 < 0).
+// fused partial dot products dot0 = V[a0].w, dot1 = V[a1].w (a1 < 0: w.w).  Strided form.
 template <class T, int CPT>
 __global__ __launch_bounds__(BS) void k_schur_black(const Desc* __restrict__ D, const int2* __restrict__ blk,
                                                     int iv, int iw, int nd, int a0, int a1,
                                                     double* __restrict__ part, int ng, int meth) {
     LOCATE();
-    // block-uniform: the full-system kernel, or the other method's launch, writes these partials
-    if (d.pcm != PM_SCHUR || !(meth & meth_of(d))) return;
-    if (d.vec) {
-        const int h = d.h;
-        const T* __restrict__ v = (const T*)d.v[iv];
-        const T* __restrict__ y = (const T*)d.v[V_Y];
-        const T* __restrict__ Cb = (const T*)d.K[0] + h;
-        T* __restrict__ w = (T*)d.v[iw];
-        const T* __restrict__ va = (const T*)d.v[a0];
-        const T* __restrict__ vb = a1 >= 0 ? (const T*)d.v[a1] : nullptr;
-        const T* kq[5] = {nullptr, (const T*)d.K[1] + h, (const T*)d.K[2] + h, (const T*)d.K[3] + h,
-                          (const T*)d.K[4] + h};
-        const bool on = d.act[0] != 0;
-        FOR_VGROUPS(g, p0) {
-            double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0;
-            if (on && p0 < h) {
-                int j, q0;
-                vrow(d, p0, j, q0);
-                const V4<T> o = voffdiag<T>(d, kq, p0, j, q0, 1, y);
-                const V4<T> cb = ld4(Cb + p0), vv = ld4(v + p0), aa = ld4(va + p0);
-                V4<T> wc;
-#pragma unroll
-                for (int m = 0; m < 4; ++m) wc.e[m] = cb.e[m] * vv.e[m] - o.e[m];
-                st4(w + p0, wc);
-                s0 = dot4d(aa, wc);
-                if (nd > 1) s1 = dot4d(vb ? ld4(vb + p0) : wc, wc);
-                if (nd > 2) {
-                    const V4<T> rh = ld4((const T*)d.v[V_RH] + p0);
-                    s2 = dot4d(rh, wc);
-                    s3 = dot4d(rh, aa);
-                    s4 = dot4d(aa, aa);
-                }
-            }
-            vgroup_store(d, bi, CPT, g, s0, part);
-            if (nd > 1) vgroup_store(d, bi, CPT, g, s1, part + ng);
-            if (nd > 2) {
-                vgroup_store(d, bi, CPT, g, s2, part + 2 * (long)ng);
-                vgroup_store(d, bi, CPT, g, s3, part + 3 * (long)ng);
-                vgroup_store(d, bi, CPT, g, s4, part + 4 * (long)ng);
-            }
-        }
-        return;
-    }
+    // block-uniform: the full-system kernel, the one-pass kernel, or the other method's
+    // launch, writes these partials
+    if (d.pcm != PM_SCHUR || d.vec || !(meth & meth_of(d))) return;
     double s0[CPT] = {}, s1[CPT] = {};
     if (d.act[0]) {
         const int h = d.h;
@@ -1075,6 +1019,96 @@ __global__ __launch_bounds__(BS) void k_schur_black(const Desc* __restrict__ D, 
     }
     group_store<CPT>(s0, PARTSK(part));
     if (nd > 1) group_store<CPT>(s1, PARTSK(part + ng));
+}
+// Reduced operator in one pass, four-ranks-per-lane form: w = S v = C_b v_b - A_br y_r
+// with y_r = C_r^-1 (A_rb v_b) formed where it is used.  A lane's four black cells of row
+// j need y_r of the four red cells of rows j - 1, j and j + 1 at the same pair ranks and of
+// one more red cell of row j (west of the first or east of the last); each is computed
+// exactly as a red-rank lane of the two-pass form computes it (voffdiag, then times 1/C_r),
+// so w is bit-identical to the two-pass result.  No y_r is stored: of the two passes' 16
+// arrays per rank (the red pass: 4 coefficients, 1/C_r, v, y written; the black pass: C_b,
+// 4 coefficients, v, y, the dot operand, w written) 13 remain, neighbour rows coming from
+// L2.  Fused partial dot products: dot0 = V[a0].w, dot1 = w.w; nd = 5: also dot2 = rh.w,
+// dot3 = rh.V[a0], dot4 = V[a0].V[a0] (the BiCGStab iteration's end, A_OMEGA).
+template <class T>
+__device__ inline T red_y1(const Desc& d, const T* const* kq, const T* __restrict__ dvr, const T* __restrict__ v,
+                           int pe, int j, bool red_even) {
+    // the red cell at rank pe of row j (pair qe): W / E neighbours are ranks pe - 1 / pe for
+    // a red cell at an even column, pe / pe + 1 at an odd one (both exist: pe is interior
+    // to the row whenever it is asked for)
+    const int hr = d.nx >> 1;
+    const T xw = v[red_even ? pe - 1 : pe], xe = v[red_even ? pe : pe + 1];
+    const T xs = v[j > 0 ? pe - hr : pe], xn = v[j < d.ny - 1 ? pe + hr : pe];
+    const T r = kq[1][pe] * xw + kq[2][pe] * xe + kq[3][pe] * xs + kq[4][pe] * xn;
+    return r * dvr[pe];
+}
+template <class T, int CPT>
+__global__ __launch_bounds__(BS) void k_schur_vec(const Desc* __restrict__ D, const int2* __restrict__ blk, int iv,
+                                                  int iw, int nd, int a0, double* __restrict__ part, int ng,
+                                                  int meth) {
+    LOCATE();
+    (void)base;
+    if (d.pcm != PM_SCHUR || !d.vec || !(meth & meth_of(d))) return;   // block-uniform
+    const int h = d.h, hr = d.nx >> 1, ny = d.ny;
+    const T* __restrict__ v = (const T*)d.v[iv];
+    const T* __restrict__ dvr = (const T*)d.dinv;
+    const T* __restrict__ Cb = (const T*)d.K[0] + h;
+    T* __restrict__ w = (T*)d.v[iw];
+    const T* __restrict__ va = (const T*)d.v[a0];
+    const T* kr[5] = {nullptr, (const T*)d.K[1], (const T*)d.K[2], (const T*)d.K[3], (const T*)d.K[4]};
+    const T* kb[5] = {nullptr, (const T*)d.K[1] + h, (const T*)d.K[2] + h, (const T*)d.K[3] + h,
+                      (const T*)d.K[4] + h};
+    const bool on = d.act[0] != 0;
+    FOR_VGROUPS(g, p0) {
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0;
+        if (on && p0 < h) {
+            int j, q0;
+            vrow(d, p0, j, q0);
+            auto red4 = [&](int pr, int jr) {
+                const V4<T> o = voffdiag<T>(d, kr, pr, jr, q0, 0, v);
+                const V4<T> dd = ld4(dvr + pr);
+                V4<T> y;
+#pragma unroll
+                for (int m = 0; m < 4; ++m) y.e[m] = o.e[m] * dd.e[m];
+                return y;
+            };
+            const V4<T> zero{{(T)0, (T)0, (T)0, (T)0}};
+            const V4<T> yc = red4(p0, j);
+            const V4<T> ys = j > 0 ? red4(p0 - hr, j - 1) : zero;
+            const V4<T> yn = j < ny - 1 ? red4(p0 + hr, j + 1) : zero;
+            // black cells of row j at even columns (j odd): west neighbour of the first is the
+            // red cell at rank p0 - 1; otherwise the east neighbour of the last, rank p0 + 4
+            const bool even = (j & 1) != 0;
+            const T ye = even ? (q0 > 0 ? red_y1<T>(d, kr, dvr, v, p0 - 1, j, false) : (T)0)
+                              : (q0 + 4 < hr ? red_y1<T>(d, kr, dvr, v, p0 + 4, j, true) : (T)0);
+            const V4<T> kw = ld4(kb[1] + p0), ke = ld4(kb[2] + p0), ks = ld4(kb[3] + p0), kn = ld4(kb[4] + p0);
+            const V4<T> cb = ld4(Cb + p0), vv = ld4(v + p0), aa = ld4(va + p0);
+            V4<T> wc;
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                const T xw = even ? (m == 0 ? ye : yc.e[m - 1]) : yc.e[m];
+                const T xe = even ? yc.e[m] : (m == 3 ? ye : yc.e[m + 1]);
+                const T o = kw.e[m] * xw + ke.e[m] * xe + ks.e[m] * ys.e[m] + kn.e[m] * yn.e[m];
+                wc.e[m] = cb.e[m] * vv.e[m] - o;
+            }
+            st4(w + p0, wc);
+            s0 = dot4d(aa, wc);
+            if (nd > 1) s1 = dot4d(wc, wc);
+            if (nd > 2) {
+                const V4<T> rh = ld4((const T*)d.v[V_RH] + p0);
+                s2 = dot4d(rh, wc);
+                s3 = dot4d(rh, aa);
+                s4 = dot4d(aa, aa);
+            }
+        }
+        vgroup_store(d, bi, CPT, g, s0, part);
+        if (nd > 1) vgroup_store(d, bi, CPT, g, s1, part + ng);
+        if (nd > 2) {
+            vgroup_store(d, bi, CPT, g, s2, part + 2 * (long)ng);
+            vgroup_store(d, bi, CPT, g, s3, part + 3 * (long)ng);
+            vgroup_store(d, bi, CPT, g, s4, part + 4 * (long)ng);
+        }
+    }
 }
 // Full operator (jacobi / none): y = A x over both slots of each pair, fused dots as above.
 template <class T, int CPT>
@@ -2236,6 +2270,7 @@ struct Table {
     bool has_tp = false;                                         // two-phase instances present
     // instances whose Krylov kernels run in the four-ranks-per-lane form, and the others
     bool has_vec = false, has_nonvec = false;
+    bool has_schur_strided = false;   // reduced-form instances in the strided form
     // Multigrid plan over the instances with a hierarchy; index l = 1 .. levels - 1: the
     // instances that have level l, their level l - 1 (fine) and level l (coarse)
     // descriptors, blocks over the coarse cells, the fine pairs and the coarse pairs.
@@ -2266,8 +2301,16 @@ struct Table {
     // iteration counts repeat from step to step
     std::map<std::tuple<double, int, int>, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
     std::map<std::tuple<double, int, int, int>, int> pred;
+    // whole speculative Newton iterations, keyed by (blind Krylov iterations, methods,
+    // Krylov precision, the pinned gather buffer)
+    std::map<std::tuple<int, int, int, uintptr_t>, std::pair<hipGraph_t, hipGraphExec_t>> newton_graphs;
     ~Table() {
-        for (auto& g : graphs) {
+        for (auto* m : {&graphs})
+            for (auto& g : *m) {
+                (void)hipGraphExecDestroy(g.second.second);
+                (void)hipGraphDestroy(g.second.first);
+            }
+        for (auto& g : newton_graphs) {
             (void)hipGraphExecDestroy(g.second.second);
             (void)hipGraphDestroy(g.second.first);
         }
@@ -2538,6 +2581,7 @@ struct HeatBatch::Impl {
             t.has_tp |= d.kind == K_TWOPHASE;
             t.has_vec |= d.vec != 0;
             t.has_nonvec |= d.vec == 0;
+            t.has_schur_strided |= d.pcm == PM_SCHUR && d.vec == 0;
         }
         t.ng = g;
         for (int ci = 0; ci < 4; ++ci) {
@@ -2791,10 +2835,11 @@ struct HeatBatch::Impl {
     template <class T>
     void launch_op(const Table& t, int iv, int iw, int nd, int a0, int a1, int meth = M_BICG | M_CG) {
         TimedLaunch tl(sizeof(T) == 4 ? "operator_fp32" : "operator_fp64");
-        if (t.has_schur) {
+        if (t.has_schur_strided) {
             HLP(C_SPMV, (k_schur_red<T, CP>), iv, meth);
             HLP(C_SPMV, (k_schur_black<T, CP>), iv, iw, nd, a0, a1, t.part->p, t.ng, meth);
         }
+        if (t.has_vec) HLP(C_SPMV, (k_schur_vec<T, CP>), iv, iw, nd, a0, t.part->p, t.ng, meth);
         if (t.has_full) HLP(C_SPMV, (k_spmv_rb<T, CP>), iv, iw, nd, a0, a1, t.part->p, t.ng, meth);
     }
     // Multigrid: coarse operators and coarsest inverses of the instances about to solve a
@@ -2930,23 +2975,34 @@ struct HeatBatch::Impl {
     // pipelined, launch k+1 being queued before the count after launch k is read.
     // Instances that converge earlier are masked by their device flag, so extra
     // iterations are no-ops.
+    // The prediction key of a solve: (tolerance, iteration cap, methods, the solve's position
+    // in the step: the latest instance's Newton iteration and refinement round).
+    std::tuple<double, int, int, int> pred_key(const Table& t, double tol, int maxit) const {
+        int at = 0;
+        for (const Inst* I : t.inst) at = std::max(at, 16 * I->st.newton_stage + std::min(I->round, 15));
+        return std::make_tuple(tol, maxit, methods(t), at);
+    }
+    // The start of a Krylov solve: right-hand side mean (two-phase pressure), r, rh, x and
+    // the first direction, the V-cycle instances' first z.
     template <class T>
-    void bicgstab(Table& t, double tol, int maxit) {
+    void krylov_start(const Table& t, int meth) {
         if (t.has_tp) {   // two-phase pressure: mean of the right-hand side
             HLP(C_VEC, (k_rhs_sum<T, CP>), t.part->p);
             fin(t, F(A_MEAN, 1));
         }
         HLP(C_VEC, (k_kinit<T, CP>), t.part->p, t.ng);
         fin(t, F(A_INIT, 2));
-        const int meth = methods(t);
-        int at = 0;   // the solve's position in the step (the latest instance's)
-        for (const Inst* I : t.inst) at = std::max(at, 16 * I->st.newton_stage + std::min(I->round, 15));
-        const auto pkey = std::make_tuple(tol, maxit, meth, at);
         if (t.mg && (meth & M_CG)) {   // V-cycle instances: first z, rho = r.z, p = z
             vcycle_dots<T>(t);
             fin(t, F(A_CG_START, 1));
             HLP(C_VEC, (k_cg3<T, CP>), 1);
         }
+    }
+    template <class T>
+    void bicgstab(Table& t, double tol, int maxit) {
+        const int meth = methods(t);
+        const auto pkey = pred_key(t, tol, maxit);
+        krylov_start<T>(t, meth);
         int iters = 0;
         if (use_graph && !timing_enabled() && t.solves++ > 0) {
             // graphs of one and of two iterations (the blind launches go in pairs: one
@@ -3063,20 +3119,71 @@ struct HeatBatch::Impl {
     // writes du (its own round counter decides).
     template <class T>
     void linear_solve(Table& t) {
-        // refinement flags from ||F||; the first round reads F and writes du (no clearing pass)
+        solve_front<T>(t);
+        // per-instance tolerance (S_TOL2: the iteration's eta; mixed: the inner tolerance)
+        bicgstab<T>(t, -1.0, cfg.solver_max_iter);
+        linear_tail<T>(t);
+    }
+    // refinement flags from ||F|| (the first round reads F and writes du: no clearing pass),
+    // the preconditioner and the Krylov right-hand side
+    template <class T>
+    void solve_front(const Table& t) {
         const unsigned ni = (unsigned)t.inst.size();
         hipLaunchKernelGGL(k_refine_flags, dim3((ni + 63) / 64), dim3(64), 0, S(), t.desc->p, (int)ni, 0,
                            tol_call > 0.0 ? tol_call : cfg.solver_tol, cfg.max_refine, forcing());
         precond_setup<T>(t);
         mg_setup<T>(t);
-        if (!mixed) {
-            HLP(C_PC, (k_load_rhs<T, CP>), 0, 1);
-            bicgstab<T>(t, -1.0, cfg.solver_max_iter);   // per-instance tolerance (S_TOL2: the iteration's eta)
-        } else {
-            HLP(C_PC, (k_load_rhs<T, CP>), 1, -1);
-            bicgstab<T>(t, -1.0, cfg.solver_max_iter);   // per-instance inner tolerance (S_TOL2)
+        if (!mixed) HLP(C_PC, (k_load_rhs<T, CP>), 0, 1);
+        else HLP(C_PC, (k_load_rhs<T, CP>), 1, -1);
+    }
+    // A whole speculative Newton iteration of a membership whose instances all solve, as
+    // one graph: the solve's front, the Krylov start, nb blind iterations, the count (SPEC
+    // slot and gate), the solution, defect and refinement flags, the Newton update, the
+    // assembly at the new state and the gathered scalars (g) - the launches of the separate
+    // path, with one graph launch and one wait.  False (nothing launched) when the solve
+    // has no prediction yet or is not speculative; the caller then takes the separate path.
+    template <class T>
+    bool newton_graph_iteration(Table& t, std::vector<double>& g) {
+        if (!allow_spec || !use_graph || timing_enabled() || t.solves == 0 || tol_call > 0.0) return false;
+        const int maxit = cfg.solver_max_iter, meth = methods(t), ni = (int)t.inst.size();
+        const auto pkey = pred_key(t, -1.0, maxit);
+        const auto it = t.pred.find(pkey);
+        if (it == t.pred.end() || it->second < 1) return false;
+        const int nb = std::min(maxit, it->second);
+        ensure_gpin(ni);
+        const auto gk = std::make_tuple(nb, meth, (int)sizeof(T), (uintptr_t)gpin);
+        auto gi = t.newton_graphs.find(gk);
+        if (gi == t.newton_graphs.end()) {
+            hipGraph_t graph;
+            hipGraphExec_t exec;
+            EQ_HIP(hipStreamBeginCapture(S(), hipStreamCaptureModeThreadLocal));
+            solve_front<T>(t);
+            krylov_start<T>(t, meth);
+            for (int k = 0; k < nb; ++k) krylov_iteration<T>(t, -1.0, maxit, meth);
+            hipLaunchKernelGGL(k_count_active, dim3(1), dim3(BS), 0, S(), t.desc->p, ni, 0, host_count + 2 * SPEC,
+                               gate->p);
+            EQ_HIP(hipGetLastError());
+            linear_tail<T>(t);
+            newton_update(t);
+            launch_assemble(t);
+            fin(t, Fcell(S_FN));
+            launch_gather(t, make_int4(S_FN, S_ITT, S_REF, S_RN), S_CONT);
+            EQ_HIP(hipStreamEndCapture(S(), &graph));
+            EQ_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+            gi = t.newton_graphs.emplace(gk, std::make_pair(graph, exec)).first;
         }
-        linear_tail<T>(t);
+        ++t.solves;
+        EQ_HIP(hipGraphLaunch(gi->second.second, S()));
+        spec.on = true;
+        spec.t = &t;
+        spec.pkey = pkey;
+        spec.tol = -1.0;
+        spec.maxit = maxit;
+        spec.meth = meth;
+        spec.k = nb;
+        Device::get().sync();
+        g.assign(gpin, gpin + 5 * (size_t)ni);
+        return true;
     }
     template <class T>
     void linear_tail(Table& t) {
@@ -3151,17 +3258,23 @@ struct HeatBatch::Impl {
     // result[q * ninst + i] = scalar slot q of instance i
     std::vector<double> gather(const Table& t, int4 slots, int s4) {
         const int ni = (int)t.inst.size();
-        if (gcap < 5 * (size_t)ni) {
-            Device::get().sync();
-            if (gpin) EQ_HIP(hipHostFree(gpin));
-            gpin = nullptr;
-            gcap = 5 * (size_t)ni;
-            EQ_HIP(hipHostMalloc((void**)&gpin, gcap * sizeof(double), hipHostMallocCoherent));
-        }
-        hipLaunchKernelGGL(k_gather_slots, dim3((ni + 63) / 64), dim3(64), 0, S(), t.desc->p, ni, slots, s4, gpin);
-        EQ_HIP(hipGetLastError());
+        ensure_gpin(ni);
+        launch_gather(t, slots, s4);
         Device::get().sync();
         return std::vector<double>(gpin, gpin + 5 * (size_t)ni);
+    }
+    void ensure_gpin(int ni) {
+        if (gcap >= 5 * (size_t)ni) return;
+        Device::get().sync();
+        if (gpin) EQ_HIP(hipHostFree(gpin));
+        gpin = nullptr;
+        gcap = 5 * (size_t)ni;
+        EQ_HIP(hipHostMalloc((void**)&gpin, gcap * sizeof(double), hipHostMallocCoherent));
+    }
+    void launch_gather(const Table& t, int4 slots, int s4) {
+        const int ni = (int)t.inst.size();
+        hipLaunchKernelGGL(k_gather_slots, dim3((ni + 63) / 64), dim3(64), 0, S(), t.desc->p, ni, slots, s4, gpin);
+        EQ_HIP(hipGetLastError());
     }
 };
 
@@ -3575,24 +3688,32 @@ std::vector<int> HeatBatch::iterate() {
         if (kv.second->state == Inst::START || kv.second->state == Inst::SOLVE) asmb.push_back(kv.second.get());
     }
     TableP tps;
+    // one device-to-host copy per Newton iteration: ||F||^2, the solve statistics and
+    // whether the refinement continues (then u was not updated and F is unchanged)
+    std::vector<double> g;
+    bool whole = false;   // the iteration ran as one graph (every instance solves)
     if (!solve.empty()) {
         tps = P.table(solve);
         Table& t = *tps;
         P.allow_spec = P.cfg.speculate;
-        if (P.mixed) P.linear_solve<float>(t);
-        else P.linear_solve<double>(t);
+        if (solve.size() == asmb.size())
+            whole = P.mixed ? P.newton_graph_iteration<float>(t, g) : P.newton_graph_iteration<double>(t, g);
+        if (!whole) {
+            if (P.mixed) P.linear_solve<float>(t);
+            else P.linear_solve<double>(t);
+        }
         P.allow_spec = false;
-        P.newton_update(t);
+        if (!whole) P.newton_update(t);
     }
     std::vector<int> done;
     if (asmb.empty()) return done;
-    TableP tp = P.table(asmb);
+    TableP tp = whole ? tps : P.table(asmb);
     Table& t = *tp;
-    P.assemble(t);
-    // one device-to-host copy per Newton iteration: ||F||^2, the solve statistics and
-    // whether the refinement continues (then u was not updated and F is unchanged)
     const size_t na = asmb.size();
-    std::vector<double> g = P.gather(t, make_int4(S_FN, S_ITT, S_REF, S_RN), S_CONT);
+    if (!whole) {
+        P.assemble(t);
+        g = P.gather(t, make_int4(S_FN, S_ITT, S_REF, S_RN), S_CONT);
+    }
     if (P.spec.on && (P.mixed ? P.finish_spec<float>() : P.finish_spec<double>())) {
         // the blind iterations were too few: the work after the solve was gated off and is
         // queued again now that the solve has finished
@@ -3717,7 +3838,16 @@ double HeatBatch::time_kernel(const std::string& name, int reps, double* event_m
     }
     bool red = false, full = false, ilu = false;
     for (Inst* I : all) red |= I->pcm == PM_SCHUR, full |= I->pcm != PM_SCHUR, ilu |= I->pcm == PM_ILU;
-    if (name == "operator_red" && red) {
+    bool vec = false;
+    for (Inst* I : all) vec |= P.desc_of(*I).vec != 0;
+    if (name == "operator" && vec) {
+        if (P.mixed)
+            TLP(C_SPMV, (k_schur_vec<float, CP>), (int)V_PH, (int)V_V, 1, (int)V_RH, t.part->p, t.ng,
+                (int)(M_BICG | M_CG));
+        else
+            TLP(C_SPMV, (k_schur_vec<double, CP>), (int)V_PH, (int)V_V, 1, (int)V_RH, t.part->p, t.ng,
+                (int)(M_BICG | M_CG));
+    } else if (name == "operator_red" && red) {
         if (P.mixed) TLP(C_SPMV, (k_schur_red<float, CP>), (int)V_PH, (int)(M_BICG | M_CG));
         else TLP(C_SPMV, (k_schur_red<double, CP>), (int)V_PH, (int)(M_BICG | M_CG));
     } else if (name == "operator_black" && red) {
@@ -3761,7 +3891,8 @@ double HeatBatch::time_kernel(const std::string& name, int reps, double* event_m
         if (P.mixed) TLP(C_PC, (k_load_rhs<float, CP>), 1, 0);
         else TLP(C_PC, (k_load_rhs<double, CP>), 1, 0);
     } else {
-        EQ_REQUIRE(false, "unknown kernel " << name << " (operator_red, operator_black [reduced system], spmv, "
+        EQ_REQUIRE(false, "unknown kernel " << name << " (operator [reduced system, four ranks per lane], "
+                                                       "operator_red, operator_black [reduced system], spmv, "
                                                        "ilu_black, ilu_red [full system], assemble, defect [mixed], "
                                                        "b1, b2, b3 [strided form], b31 [four ranks per lane], "
                                                        "load_rhs)");
