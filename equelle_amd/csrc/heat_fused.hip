@@ -611,6 +611,16 @@ __device__ inline void ld8(const double* __restrict__ p, double (&v)[8]) {
         v[4 + m] = b.e[m];
     }
 }
+// W x_w + E x_e + S x_s + N x_n as one fixed chain of fused multiply-adds, so the kernels
+// that form this sum (the one-pass and two-pass operators, the red recovery) round it the
+// same way wherever the compiler would otherwise contract it.
+__device__ inline float stencil4(float kw, float xw, float ke, float xe, float ks, float xs, float kn, float xn) {
+    return __builtin_fmaf(kn, xn, __builtin_fmaf(ks, xs, __builtin_fmaf(ke, xe, kw * xw)));
+}
+__device__ inline double stencil4(double kw, double xw, double ke, double xe, double ks, double xs, double kn,
+                                  double xn) {
+    return __builtin_fma(kn, xn, __builtin_fma(ks, xs, __builtin_fma(ke, xe, kw * xw)));
+}
 // Row j and the row's pair offset q0 of rank p0 (2 p0 is the first cell of the pair).
 __device__ inline void vrow(const Desc& d, int p0, int& j, int& q0) {
     int i0;
@@ -633,7 +643,7 @@ __device__ inline V4<T> voffdiag(const Desc& d, const T* const* kq, int p0, int 
     for (int m = 0; m < 4; ++m) {
         const T xw = even ? (m == 0 ? e : xc.e[m - 1]) : xc.e[m];
         const T xe = even ? xc.e[m] : (m == 3 ? e : xc.e[m + 1]);
-        r.e[m] = kw.e[m] * xw + ke.e[m] * xe + ks.e[m] * xs.e[m] + kn.e[m] * xn.e[m];
+        r.e[m] = stencil4(kw.e[m], xw, ke.e[m], xe, ks.e[m], xs.e[m], kn.e[m], xn.e[m]);
     }
     return r;
 }
@@ -643,9 +653,12 @@ __device__ inline V4<T> voffdiag(const Desc& d, const T* const* kq, int p0, int 
 //              precision (FP32 mixed, FP64 fp64), red-black rank order
 //   two-phase: writes F, the Krylov-precision coefficients and (MIXED) the FP64 ones in
 //              cell order for the refinement defect
+//   norm_only (heat instances): ||F||^2 alone, nothing stored - the check after a Newton
+//              iteration predicted to be the step's last (F and the coefficients are only
+//              read by a further iteration, which then assembles them first)
 template <bool MIXED, int CPT>
 __global__ __launch_bounds__(BS) void k_assemble(const Desc* __restrict__ D, const int2* __restrict__ blk,
-                                                 double* __restrict__ part) {
+                                                 double* __restrict__ part, int norm_only) {
     LOCATE();
     if (gated_off(d)) return;
     double f2[CPT] = {};
@@ -688,11 +701,12 @@ __global__ __launch_bounds__(BS) void k_assemble(const Desc* __restrict__ D, con
         const Nb5 x = load_nb5(u, c, i, j, nx, ny);
         double co[5];
         const double F = heat_row(d, i, j, x.c, x.w, x.e, x.s, x.n, u0[c], co);
+        f2[m] = F * F;
+        if (norm_only) continue;
         Fo[c] = F;
         const int sl = ((i + j) & 1) * d.h + (c >> 1);
 #pragma unroll
         for (int q = 0; q < 5; ++q) co_out[q][sl] = (K)co[q];
-        f2[m] = F * F;
     }
     group_store<CPT>(f2, PARTS(part));
 }
@@ -1039,7 +1053,7 @@ __device__ inline T red_y1(const Desc& d, const T* const* kq, const T* __restric
     const int hr = d.nx >> 1;
     const T xw = v[red_even ? pe - 1 : pe], xe = v[red_even ? pe : pe + 1];
     const T xs = v[j > 0 ? pe - hr : pe], xn = v[j < d.ny - 1 ? pe + hr : pe];
-    const T r = kq[1][pe] * xw + kq[2][pe] * xe + kq[3][pe] * xs + kq[4][pe] * xn;
+    const T r = stencil4(kq[1][pe], xw, kq[2][pe], xe, kq[3][pe], xs, kq[4][pe], xn);
     return r * dvr[pe];
 }
 template <class T, int CPT>
@@ -1088,7 +1102,7 @@ __global__ __launch_bounds__(BS) void k_schur_vec(const Desc* __restrict__ D, co
             for (int m = 0; m < 4; ++m) {
                 const T xw = even ? (m == 0 ? ye : yc.e[m - 1]) : yc.e[m];
                 const T xe = even ? yc.e[m] : (m == 3 ? ye : yc.e[m + 1]);
-                const T o = kw.e[m] * xw + ke.e[m] * xe + ks.e[m] * ys.e[m] + kn.e[m] * yn.e[m];
+                const T o = stencil4(kw.e[m], xw, ke.e[m], xe, ks.e[m], ys.e[m], kn.e[m], yn.e[m]);
                 wc.e[m] = cb.e[m] * vv.e[m] - o;
             }
             st4(w + p0, wc);
@@ -2203,6 +2217,7 @@ struct Inst {
     double r0_stage = 0;           // ||F|| at the start of the current Newton solve
     int max_newton = 0;            // per-slot Newton limit per solve (0: the batch's max_iter)
     int round = 0;                 // refinement round of the current Newton iteration (host count)
+    int newton_prev = 0;           // Newton iterations of the instance's previous step (0: none yet)
     DevPtr<double> tp;             // tx, ty, tz, pv, src (n each)
     DevPtr<double> p, sw, sw0;
     DevPtr<double> u, u0, F, du, r64, C, W, E, S, N, B, T, sc;
@@ -2301,8 +2316,8 @@ struct Table {
     // iteration counts repeat from step to step
     std::map<std::tuple<double, int, int>, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
     std::map<std::tuple<double, int, int, int>, int> pred;
-    // whole speculative Newton iterations, keyed by (blind Krylov iterations, methods,
-    // Krylov precision, the pinned gather buffer)
+    // whole speculative Newton iterations, keyed by (blind Krylov iterations, methods + 8 x
+    // norm-only assembly, Krylov precision, the pinned gather buffer)
     std::map<std::tuple<int, int, int, uintptr_t>, std::pair<hipGraph_t, hipGraphExec_t>> newton_graphs;
     ~Table() {
         for (auto* m : {&graphs})
@@ -2817,13 +2832,13 @@ struct HeatBatch::Impl {
         EQ_HIP(hipGetLastError());                                                                          \
     })
 
-    void launch_assemble(const Table& t) {
+    void launch_assemble(const Table& t, int norm_only = 0) {
         TimedLaunch tl("assemble");
-        if (mixed) HL(C_ASM, (k_assemble<true, CP>), t.part->p);
-        else HL(C_ASM, (k_assemble<false, CP>), t.part->p);
+        if (mixed) HL(C_ASM, (k_assemble<true, CP>), t.part->p, norm_only);
+        else HL(C_ASM, (k_assemble<false, CP>), t.part->p, norm_only);
     }
-    void assemble(const Table& t) {
-        launch_assemble(t);
+    void assemble(const Table& t, int norm_only = 0) {
+        launch_assemble(t, norm_only);
         fin(t, Fcell(S_FN));
     }
     void launch_defect(const Table& t) {
@@ -3143,7 +3158,7 @@ struct HeatBatch::Impl {
     // path, with one graph launch and one wait.  False (nothing launched) when the solve
     // has no prediction yet or is not speculative; the caller then takes the separate path.
     template <class T>
-    bool newton_graph_iteration(Table& t, std::vector<double>& g) {
+    bool newton_graph_iteration(Table& t, std::vector<double>& g, int norm_only) {
         if (!allow_spec || !use_graph || timing_enabled() || t.solves == 0 || tol_call > 0.0) return false;
         const int maxit = cfg.solver_max_iter, meth = methods(t), ni = (int)t.inst.size();
         const auto pkey = pred_key(t, -1.0, maxit);
@@ -3151,7 +3166,7 @@ struct HeatBatch::Impl {
         if (it == t.pred.end() || it->second < 1) return false;
         const int nb = std::min(maxit, it->second);
         ensure_gpin(ni);
-        const auto gk = std::make_tuple(nb, meth, (int)sizeof(T), (uintptr_t)gpin);
+        const auto gk = std::make_tuple(nb, meth + 8 * norm_only, (int)sizeof(T), (uintptr_t)gpin);
         auto gi = t.newton_graphs.find(gk);
         if (gi == t.newton_graphs.end()) {
             hipGraph_t graph;
@@ -3165,7 +3180,7 @@ struct HeatBatch::Impl {
             EQ_HIP(hipGetLastError());
             linear_tail<T>(t);
             newton_update(t);
-            launch_assemble(t);
+            launch_assemble(t, norm_only);
             fin(t, Fcell(S_FN));
             launch_gather(t, make_int4(S_FN, S_ITT, S_REF, S_RN), S_CONT);
             EQ_HIP(hipStreamEndCapture(S(), &graph));
@@ -3692,12 +3707,19 @@ std::vector<int> HeatBatch::iterate() {
     // whether the refinement continues (then u was not updated and F is unchanged)
     std::vector<double> g;
     bool whole = false;   // the iteration ran as one graph (every instance solves)
+    // Every instance solves, is a heat instance, and its previous step converged after as
+    // many Newton iterations as this one will have done: the assembly after the update
+    // computes ||F|| alone (if some instance goes on after all, the full assembly follows).
+    int norm_only = !solve.empty() && solve.size() == asmb.size();
+    for (const Inst* I : solve)
+        if (I->kind != K_HEAT || I->newton_prev <= 0 || I->st.newton_stage + 1 < I->newton_prev) norm_only = 0;
     if (!solve.empty()) {
         tps = P.table(solve);
         Table& t = *tps;
         P.allow_spec = P.cfg.speculate;
         if (solve.size() == asmb.size())
-            whole = P.mixed ? P.newton_graph_iteration<float>(t, g) : P.newton_graph_iteration<double>(t, g);
+            whole = P.mixed ? P.newton_graph_iteration<float>(t, g, norm_only)
+                            : P.newton_graph_iteration<double>(t, g, norm_only);
         if (!whole) {
             if (P.mixed) P.linear_solve<float>(t);
             else P.linear_solve<double>(t);
@@ -3711,7 +3733,7 @@ std::vector<int> HeatBatch::iterate() {
     Table& t = *tp;
     const size_t na = asmb.size();
     if (!whole) {
-        P.assemble(t);
+        P.assemble(t, norm_only);
         g = P.gather(t, make_int4(S_FN, S_ITT, S_REF, S_RN), S_CONT);
     }
     if (P.spec.on && (P.mixed ? P.finish_spec<float>() : P.finish_spec<double>())) {
@@ -3720,7 +3742,7 @@ std::vector<int> HeatBatch::iterate() {
         if (P.mixed) P.linear_tail<float>(*tps);
         else P.linear_tail<double>(*tps);
         P.newton_update(*tps);
-        P.assemble(t);
+        P.assemble(t, norm_only);
         g = P.gather(t, make_int4(S_FN, S_ITT, S_REF, S_RN), S_CONT);
     }
     const FusedConfig& c = P.cfg;
@@ -3771,10 +3793,16 @@ std::vector<int> HeatBatch::iterate() {
             if (I.kind == K_TWOPHASE && I.st.stop_p == 0) I.st.stop = 0;   // pressure did not converge
             I.st.failed = bad;
             I.state = Inst::DONE;
+            I.newton_prev = I.st.newton_stage;
             done.push_back(I.id);
         } else {
             I.state = Inst::SOLVE;
         }
+    }
+    if (norm_only) {   // an instance goes on: its next solve reads F and the coefficients
+        bool more = false;
+        for (const Inst* I : asmb) more |= I->state == Inst::SOLVE;
+        if (more) P.assemble(t);
     }
     return done;
 }
@@ -3871,8 +3899,8 @@ double HeatBatch::time_kernel(const std::string& name, int reps, double* event_m
             TLP(C_SPMV, (k_spmv_rb<double, CP>), (int)V_PH, (int)V_V, 1, (int)V_RH, -1, t.part->p, t.ng,
                 (int)(M_BICG | M_CG));
     } else if (name == "assemble") {
-        if (P.mixed) TL(C_ASM, (k_assemble<true, CP>), t.part->p);
-        else TL(C_ASM, (k_assemble<false, CP>), t.part->p);
+        if (P.mixed) TL(C_ASM, (k_assemble<true, CP>), t.part->p, 0);
+        else TL(C_ASM, (k_assemble<false, CP>), t.part->p, 0);
     } else if (name == "defect" && P.mixed) {   // the refinement defect exists in mixed precision only
         TL(C_DEF, (k_defect<CP>), t.part->p);
     } else if (name == "b1") {

@@ -321,6 +321,46 @@ def test_speculative_solves_match_checked_solves(hip_api, precision):
     assert sum(s[1] for _, s in res[1][half:]) > sum(s[1] for _, s in res[1][:half])
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_predicted_last_newton_check_matches_full_assembly(hip_api, precision):
+    """After a Newton iteration that the instance's previous step says is the last, the
+    assembly forms ||F|| alone.  A rough field after smooth steps needs more iterations
+    than predicted: the full assembly then follows, and the step is bit-identical to the
+    same step taken by a fresh batch (no prediction, every assembly full).  forcing0=0: the
+    first linear tolerance of a step does not depend on the previous step either."""
+    nx, ny = 96, 80
+    rng = np.random.default_rng(11)
+    smooth = 1.0 + 0.01 * gaussian_bumps(nx, ny, 7, width_scale=0.2)
+    rough = rng.uniform(0.0, 2.0, nx * ny)
+    params = (f"precision={precision} preconditioner=ilu0 rel_res_tol=1e-9 abs_res_tol=0 solver_tol=1e-6 "
+              "forcing0=0")
+    b = make_batch(hip_api, params)
+    try:
+        s = add(hip_api, b, nx, ny, 0.3, smooth)
+        arr = np.array([s], dtype=np.int32)
+        hip_api.batch_step(b, iptr(arr), 1, 1.0)
+        hip_api.batch_step(b, iptr(arr), 1, 1.0)
+        n_smooth = stats(hip_api, b, s)[0][0]
+        hip_api.batch_set_state(b, s, dptr(rough))
+        hip_api.batch_step(b, iptr(arr), 1, 1.0)
+        st_a, nrm_a = stats(hip_api, b, s)
+        ua = state(hip_api, b, s, nx * ny)
+    finally:
+        hip_api.batch_destroy(b)
+    assert st_a[0] > n_smooth, (st_a, n_smooth)   # the prediction fell short
+    b = make_batch(hip_api, params)
+    try:
+        s = add(hip_api, b, nx, ny, 0.3, rough)
+        hip_api.batch_step(b, iptr(np.array([s], dtype=np.int32)), 1, 1.0)
+        st_b, nrm_b = stats(hip_api, b, s)
+        ub = state(hip_api, b, s, nx * ny)
+    finally:
+        hip_api.batch_destroy(b)
+    assert st_a[0] == st_b[0] and st_a[3] != 0, (st_a, st_b)
+    assert np.array_equal(nrm_a, nrm_b)
+    assert np.array_equal(ua, ub)
+
+
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("mixed", 1e-9)])
 def test_pair_vectors_match_strided_form(hip_api, precision, tol):
     """The four-ranks-per-lane Krylov kernels (2D heat, nx % 8 == 0) and the strided
