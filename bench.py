@@ -118,7 +118,7 @@ def run_fused(lib, precision, u0, nx, ny, dt, steps, warmup, precond="ilu0", gra
         vec = precond == "ilu0" and nx % 8 == 0
         names = (("operator",) if vec else ("operator_red", "operator_black") if precond == "ilu0" else ("spmv",)) + \
             ("assemble",) + (("defect",) if precision == "mixed" else ()) + \
-            (("b2", "b31") if vec else ("b1", "b2", "b3")) + ("load_rhs",)
+            (("b2", "b31", "setup") if vec else ("b1", "b2", "b3", "load_rhs"))
         for name in names:
             ms, ev = C.c_double(), C.c_double()
             lib.batch_kernel_time(b, name.encode(), 50, C.byref(ms), C.byref(ev))
@@ -757,6 +757,8 @@ KERNELS = {
             "x, ph, sh, s, t, p, v, 1/P read; x, r, p, ph written"),
     "load_rhs": ("k_load_rhs<T, {cpt}>", "precond", lambda n, h, ks: 8 * n + 6 * ks * h,
                  "r64 read (FP64), 4 black coefficients, 1/C_r read, g written"),
+    "setup": ("k_setup_vec<T, {cpt}>", "precond", lambda n, h, ks: 8 * n + 17 * ks * h,
+              "r64 read (FP64), 10 coefficient arrays read; 1/C_r, 1/P_b, r, rh, p, ph, x written"),
 }
 
 
@@ -767,7 +769,8 @@ def kernel_table(run, n, ks, steps):
            "spmv": 2 * run["krylov"],
            "b1": run["krylov"], "b2": run["krylov"], "b3": run["krylov"], "b31": run["krylov"],
            "assemble": run["newton"] + steps,
-           "defect": run["refine"], "load_rhs": run["refine"] if ks == 4 else run["newton"]}
+           "defect": run["refine"], "load_rhs": run["refine"] if ks == 4 else run["newton"],
+           "setup": run["refine"] if ks == 4 else run["newton"]}
     out = {}
     for name, (kname, cls, fb, text) in KERNELS.items():
         if name not in kern:

@@ -1165,6 +1165,7 @@ __global__ __launch_bounds__(BS) void k_spmv_rb(const Desc* __restrict__ D, cons
 template <class T, int CPT>
 __global__ __launch_bounds__(BS) void k_precond_setup(const Desc* __restrict__ D, const int2* __restrict__ blk) {
     LOCATE();
+    if (d.vec) return;   // block-uniform (the four-ranks-per-lane form: k_setup_vec)
     const int h = d.h, nx = d.nx, ny = d.ny, nxy = d.nx * d.ny, mode = d.pcm;
     const T* K[7];
     for (int q = 0; q < 7; ++q) K[q] = (const T*)d.K[q];
@@ -1284,28 +1285,7 @@ template <class T, int CPT>
 __global__ __launch_bounds__(BS) void k_kinit(const Desc* __restrict__ D, const int2* __restrict__ blk,
                                               double* __restrict__ part, int ng) {
     LOCATE();
-    if (d.vec) {   // reduced form, four ranks per lane (BiCGStab; not singular): rh = r, x = 0,
-                   // the first search direction p = r and ph = M^-1 p (later ones come from k_b31)
-        const bool on = d.act[1] != 0;
-        const T* __restrict__ dv = slot_dinv<T>(d);
-        FOR_VGROUPS(g, p0) {
-            double rr = 0.0;
-            if (on && p0 < d.h) {
-                const V4<T> r = ld4((const T*)d.v[V_R] + p0), dd = ld4(dv + p0);
-                V4<T> ph;
-#pragma unroll
-                for (int m = 0; m < 4; ++m) ph.e[m] = dd.e[m] * r.e[m];
-                st4((T*)d.v[V_RH] + p0, r);
-                st4((T*)d.v[V_P] + p0, r);
-                st4((T*)d.v[V_PH] + p0, ph);
-                st4((T*)d.v[V_X] + p0, V4<T>{{(T)0, (T)0, (T)0, (T)0}});
-                rr = dot4d(r, r);
-            }
-            vgroup_store(d, bi, CPT, g, rr, part);
-            vgroup_store(d, bi, CPT, g, rr, part + ng);
-        }
-        return;
-    }
+    if (d.vec) return;   // block-uniform (k_setup_vec starts these solves)
     double rr[CPT] = {}, rz[CPT] = {};
     const bool cg = meth_of(d) == M_CG;
     if (d.act[1] && cg) {
@@ -2010,54 +1990,13 @@ template <class T, int CPT>
 __global__ __launch_bounds__(BS) void k_load_rhs(const Desc* __restrict__ D, const int2* __restrict__ blk,
                                                  int scaled, int first) {
     LOCATE();
-    if (!d.act[1]) return;
+    if (!d.act[1] || d.vec) return;   // (the four-ranks-per-lane form: k_setup_vec)
     const double scale = scaled ? 1.0 / d.sc[S_RN] : 1.0;
     const bool fst = first < 0 ? d.sc[S_REF] == 0.0 : first != 0;   // first < 0: the instance's own round
     const double* __restrict__ src = fst ? d.F : d.r64;
     const int h = d.h, nx = d.nx, ny = d.ny, nxy = d.nx * d.ny;
     T* __restrict__ r = (T*)d.v[V_R];
     const T* __restrict__ dv = (const T*)d.dinv;
-    if (d.vec) {   // heat, reduced form: lanes of four black ranks (cells 2 p0 .. 2 p0 + 7 of row j)
-        const int hr = nx >> 1;
-        FOR_VGROUPS(g, p0) {
-            if (p0 >= h) continue;
-            int j, q0;
-            vrow(d, p0, j, q0);
-            const int c0 = 2 * p0;
-            const bool evb = (j & 1) != 0;   // black cells at the even columns of the row
-            double sc[8], ss[8], sn[8];
-            ld8(src + c0, sc);
-            ld8(src + (j > 0 ? c0 - nx : c0), ss);
-            ld8(src + (j < ny - 1 ? c0 + nx : c0), sn);
-            // the red cell just outside the segment (west of the first black cell or east of
-            // the last one), and its rank; absent at the grid edge (zero coefficient)
-            const int ce = evb ? (q0 > 0 ? c0 - 1 : c0) : (q0 + 4 < hr ? c0 + 8 : c0);
-            const int pe = evb ? (q0 > 0 ? p0 - 1 : p0) : (q0 + 4 < hr ? p0 + 4 : p0);
-            const T se = rhs_value<T>(src, ce, scale) * dv[pe];
-            const V4<T> dc = ld4(dv + p0), ds = ld4(dv + (j > 0 ? p0 - hr : p0)), dn = ld4(dv + (j < ny - 1 ? p0 + hr : p0));
-            const V4<T> kw = ld4((const T*)d.K[1] + h + p0), ke = ld4((const T*)d.K[2] + h + p0);
-            const V4<T> ks = ld4((const T*)d.K[3] + h + p0), kn = ld4((const T*)d.K[4] + h + p0);
-            V4<T> out;
-#pragma unroll
-            for (int m = 0; m < 4; ++m) {
-                const int bc = evb ? 2 * m : 2 * m + 1;   // the black cell's place in the segment
-                // red neighbours: west / east in the row, the cells above and below
-                const T w = evb ? (m == 0 ? se : (T)((T)(sc[bc > 0 ? bc - 1 : 0] * scale) * dc.e[m > 0 ? m - 1 : 0]))
-                                : (T)((T)(sc[bc - 1] * scale) * dc.e[m]);
-                const T e = evb ? (T)((T)(sc[bc + 1] * scale) * dc.e[m])
-                                : (m == 3 ? se : (T)((T)(sc[bc < 7 ? bc + 1 : 7] * scale) * dc.e[m < 3 ? m + 1 : 3]));
-                const T sv = (T)((T)(ss[bc] * scale) * ds.e[m]), nv = (T)((T)(sn[bc] * scale) * dn.e[m]);
-                T acc = (T)0;
-                acc += kw.e[m] * w;
-                acc += ke.e[m] * e;
-                acc += ks.e[m] * sv;
-                acc += kn.e[m] * nv;
-                out.e[m] = (T)(sc[bc] * scale) - acc;
-            }
-            st4(r + p0, out);
-        }
-        return;
-    }
     FOR_PAIRS(p) {
         int i, j, k;
         if (d.ns == 2) {
@@ -2087,6 +2026,112 @@ __global__ __launch_bounds__(BS) void k_load_rhs(const Desc* __restrict__ D, con
         r[p] = rhs_value<T>(src, c, scale) - acc;
     }
 }
+// Four-ranks-per-lane form, the start of a solve in one pass (k_precond_setup, k_load_rhs
+// and k_kinit of the strided forms): a lane takes the four red and the four black cells of
+// pair ranks p0 .. p0 + 3 of row j and forms
+//   1 / C_r of its red cells and 1 / P_b of its black ones, P_b = C_b - sum over the red
+//     neighbours m of A_bm A_mb / C_m (the red-black ILU(0) pivot), with the neighbours'
+//     1 / C_m formed from their C_m as their own lanes form them;
+//   g_b = b_b - sum_m A_bm (b_m / C_m), b = (T)(src scale) (src = F in the first refinement
+//     round, r64 after it; scale = 1 / ||src|| in mixed precision);
+//   r = rh = p = g, ph = g / P_b, x = 0, and the partial dots (r.r, r.r) for A_INIT.
+// Reads the 10 coefficient arrays and src once, writes dinv and the five vectors.
+template <class T>
+__device__ inline T inv_or_one(T c) {
+    return c != (T)0 ? (T)1 / c : (T)1;
+}
+template <class T, int CPT>
+__global__ __launch_bounds__(BS) void k_setup_vec(const Desc* __restrict__ D, const int2* __restrict__ blk,
+                                                  double* __restrict__ part, int ng, int scaled, int first) {
+    LOCATE();
+    (void)base;
+    if (!d.vec) return;   // block-uniform
+    const bool on = d.act[1] != 0;
+    const double scale = scaled ? 1.0 / d.sc[S_RN] : 1.0;
+    const bool fst = first < 0 ? d.sc[S_REF] == 0.0 : first != 0;   // first < 0: the instance's own round
+    const double* __restrict__ src = fst ? d.F : d.r64;
+    const int h = d.h, nx = d.nx, ny = d.ny, hr = nx >> 1;
+    const T* __restrict__ K0 = (const T*)d.K[0];
+    const T* __restrict__ K1 = (const T*)d.K[1];
+    const T* __restrict__ K2 = (const T*)d.K[2];
+    const T* __restrict__ K3 = (const T*)d.K[3];
+    const T* __restrict__ K4 = (const T*)d.K[4];
+    T* __restrict__ dv = (T*)d.dinv;
+    FOR_VGROUPS(g, p0) {
+        double rr = 0.0;
+        if (p0 < h) {
+            int j, q0;
+            vrow(d, p0, j, q0);
+            const bool evb = (j & 1) != 0;   // black cells at the even columns of row j
+            // red cells of row j at ranks p0 .. p0 + 3, and the one just outside (west of the
+            // first black cell when evb, east of the last otherwise; absent at the row's end)
+            const V4<T> cr = ld4(K0 + p0), kwr = ld4(K1 + p0), ker = ld4(K2 + p0);
+            const int pe = evb ? p0 - 1 : p0 + 4;
+            const bool has_e = evb ? q0 > 0 : q0 + 4 < hr;
+            const T cre = has_e ? K0[pe] : (T)1, kwe = has_e ? K1[pe] : (T)0, kee = has_e ? K2[pe] : (T)0;
+            V4<T> dr;
+#pragma unroll
+            for (int m = 0; m < 4; ++m) dr.e[m] = inv_or_one(cr.e[m]);
+            st4(dv + p0, dr);
+            // red cells of rows j - 1 and j + 1 at the same ranks (their N / S coefficients
+            // couple them to row j's black cells)
+            const V4<T> one{{(T)1, (T)1, (T)1, (T)1}}, zero{{(T)0, (T)0, (T)0, (T)0}};
+            const V4<T> crs = j > 0 ? ld4(K0 + p0 - hr) : one, kns = j > 0 ? ld4(K4 + p0 - hr) : zero;
+            const V4<T> crn = j < ny - 1 ? ld4(K0 + p0 + hr) : one, ksn = j < ny - 1 ? ld4(K3 + p0 + hr) : zero;
+            const V4<T> cb = ld4(K0 + h + p0), kw = ld4(K1 + h + p0), ke = ld4(K2 + h + p0);
+            const V4<T> ks = ld4(K3 + h + p0), kn = ld4(K4 + h + p0);
+            V4<T> db;
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                // the black cell's red W / E neighbours: (C, the coefficient towards the black cell)
+                const T cW = evb ? (m == 0 ? cre : cr.e[m > 0 ? m - 1 : 0]) : cr.e[m];
+                const T aW = evb ? (m == 0 ? kee : ker.e[m > 0 ? m - 1 : 0]) : ker.e[m];
+                const T cE = evb ? cr.e[m] : (m == 3 ? cre : cr.e[m < 3 ? m + 1 : 3]);
+                const T aE = evb ? kwr.e[m] : (m == 3 ? kwe : kwr.e[m < 3 ? m + 1 : 3]);
+                T piv = cb.e[m];
+                piv -= kw.e[m] * aW / cW;
+                piv -= ke.e[m] * aE / cE;
+                piv -= ks.e[m] * kns.e[m] / crs.e[m];
+                piv -= kn.e[m] * ksn.e[m] / crn.e[m];
+                db.e[m] = inv_or_one(piv);
+            }
+            st4(dv + h + p0, db);
+            if (on) {
+                // right-hand side: b at the row's eight cells, the red cells above and below,
+                // and the outside red cell
+                const int c0 = 2 * p0;
+                double sc[8], ss[8], sn[8];
+                ld8(src + c0, sc);
+                ld8(src + (j > 0 ? c0 - nx : c0), ss);
+                ld8(src + (j < ny - 1 ? c0 + nx : c0), sn);
+                const T be = has_e ? (T)(src[evb ? c0 - 1 : c0 + 8] * scale) : (T)0;
+                V4<T> gv, ph;
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    const int bc = evb ? 2 * m : 2 * m + 1;   // the black cell's place among the eight
+                    const T bW = evb ? (m == 0 ? be : (T)(sc[bc > 0 ? bc - 1 : 0] * scale)) : (T)(sc[bc - 1] * scale);
+                    const T bE = evb ? (T)(sc[bc + 1] * scale) : (m == 3 ? be : (T)(sc[bc < 7 ? bc + 1 : 7] * scale));
+                    const T cW = evb ? (m == 0 ? cre : cr.e[m > 0 ? m - 1 : 0]) : cr.e[m];
+                    const T cE = evb ? cr.e[m] : (m == 3 ? cre : cr.e[m < 3 ? m + 1 : 3]);
+                    const T yW = bW * inv_or_one(cW), yE = bE * inv_or_one(cE);
+                    const T yS = (T)(ss[bc] * scale) * inv_or_one(crs.e[m]);
+                    const T yN = (T)(sn[bc] * scale) * inv_or_one(crn.e[m]);
+                    gv.e[m] = (T)(sc[bc] * scale) - stencil4(kw.e[m], yW, ke.e[m], yE, ks.e[m], yS, kn.e[m], yN);
+                    ph.e[m] = db.e[m] * gv.e[m];
+                }
+                st4((T*)d.v[V_R] + p0, gv);
+                st4((T*)d.v[V_RH] + p0, gv);
+                st4((T*)d.v[V_P] + p0, gv);
+                st4((T*)d.v[V_PH] + p0, ph);
+                st4((T*)d.v[V_X] + p0, zero);
+                rr = dot4d(gv, gv);
+            }
+        }
+        vgroup_store(d, bi, CPT, g, rr, part);
+        vgroup_store(d, bi, CPT, g, rr, part + ng);
+    }
+}
+
 // du (+)= ||.|| x (mixed) or du = x (fp64), in cell order; the first round writes du
 // (every instance of the launch: 0 where the refinement is inactive), later rounds add.
 // Reduced system: the red unknowns are recovered first, x_r = C_r^-1 (b_r - A_rb x_b).
@@ -2926,7 +2971,8 @@ struct HeatBatch::Impl {
     // du, which must survive between the refinement rounds of an instance
     template <class T>
     void conversions(const Table& t) {
-        HLP(C_PC, (k_load_rhs<T, CP>), 1, 0);
+        if (t.has_nonvec) HLP(C_PC, (k_load_rhs<T, CP>), 1, 0);
+        if (t.has_vec) HLP(C_PC, (k_setup_vec<T, CP>), t.part->p, t.ng, 1, 0);
     }
     template <class T>
     void vec_round(const Table& t) {
@@ -3005,7 +3051,11 @@ struct HeatBatch::Impl {
             HLP(C_VEC, (k_rhs_sum<T, CP>), t.part->p);
             fin(t, F(A_MEAN, 1));
         }
-        HLP(C_VEC, (k_kinit<T, CP>), t.part->p, t.ng);
+        if (t.has_vec) {   // the four-ranks-per-lane instances' pivots, right-hand side and start
+            if (mixed) HLP(C_PC, (k_setup_vec<T, CP>), t.part->p, t.ng, 1, -1);
+            else HLP(C_PC, (k_setup_vec<T, CP>), t.part->p, t.ng, 0, 1);
+        }
+        if (t.has_nonvec) HLP(C_VEC, (k_kinit<T, CP>), t.part->p, t.ng);
         fin(t, F(A_INIT, 2));
         if (t.mg && (meth & M_CG)) {   // V-cycle instances: first z, rho = r.z, p = z
             vcycle_dots<T>(t);
@@ -3918,12 +3968,15 @@ double HeatBatch::time_kernel(const std::string& name, int reps, double* event_m
     } else if (name == "load_rhs") {
         if (P.mixed) TLP(C_PC, (k_load_rhs<float, CP>), 1, 0);
         else TLP(C_PC, (k_load_rhs<double, CP>), 1, 0);
+    } else if (name == "setup" && vec) {
+        if (P.mixed) TLP(C_PC, (k_setup_vec<float, CP>), t.part->p, t.ng, 1, 0);
+        else TLP(C_PC, (k_setup_vec<double, CP>), t.part->p, t.ng, 1, 0);
     } else {
         EQ_REQUIRE(false, "unknown kernel " << name << " (operator [reduced system, four ranks per lane], "
                                                        "operator_red, operator_black [reduced system], spmv, "
                                                        "ilu_black, ilu_red [full system], assemble, defect [mixed], "
                                                        "b1, b2, b3 [strided form], b31 [four ranks per lane], "
-                                                       "load_rhs)");
+                                                       "load_rhs [strided form], setup [four ranks per lane])");
     }
 #undef TL
 #undef TLP
