@@ -3196,8 +3196,10 @@ struct HeatBatch::Impl {
         const unsigned ni = (unsigned)t.inst.size();
         hipLaunchKernelGGL(k_refine_flags, dim3((ni + 63) / 64), dim3(64), 0, S(), t.desc->p, (int)ni, 0,
                            tol_call > 0.0 ? tol_call : cfg.solver_tol, cfg.max_refine, forcing());
-        precond_setup<T>(t);
+        // (the four-ranks-per-lane instances' pivots and right-hand side: k_setup_vec)
+        if (t.has_nonvec) precond_setup<T>(t);
         mg_setup<T>(t);
+        if (!t.has_nonvec) return;
         if (!mixed) HLP(C_PC, (k_load_rhs<T, CP>), 0, 1);
         else HLP(C_PC, (k_load_rhs<T, CP>), 1, -1);
     }
