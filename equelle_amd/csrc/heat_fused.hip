@@ -34,7 +34,7 @@ enum Slot { S_RHO = 0, S_RHO_OLD, S_ALPHA, S_OMEGA, S_RR, S_BB, S_BREAK, S_IT, S
 // Krylov vectors; V_Y is the red-rank scratch of the reduced operator (ilu0 only), V_Z the
 // V-cycle's z = M^-1 r (two-phase pressure by CG with multigrid)
 enum Vec { V_R = 0, V_RH, V_P, V_V, V_S, V_T, V_PH, V_SH, V_X, V_Y, V_Z, NVEC };
-enum Act { A_INIT = 0, A_ALPHA, A_OMEGA, A_RHO, A_STORE, A_MEAN, A_CG_ALPHA, A_CG_RHO, A_CG_START };
+enum Act { A_INIT = 0, A_ALPHA, A_OMEGA, A_RHO, A_STORE, A_MEAN, A_CG_ALPHA, A_CG_RHO, A_CG_START, A_REFINE };
 // Krylov methods of a solve (bit mask): BiCGStab, and CG for symmetric systems (the
 // two-phase pressure stage in the reduced form)
 enum Meth { M_BICG = 1, M_CG = 2 };
@@ -390,6 +390,17 @@ __device__ void fin_body(const Desc& d, const double* part, int nb, const Fin& f
         case A_MEAN:   // Krylov right-hand side sum -> its mean over the cells (singular systems)
             sc[S_MEAN] = res[0] / (double)(d.ns == 1 ? d.n / 2 : d.n);   // black cells or all cells
             break;
+        case A_REFINE: {   // the refinement defect's ||r||^2, then k_refine_flags' mode 1 (maxit: max_refine)
+            sc[S_FN] = res[0];
+            if (!a1) break;
+            const double rn = sqrt(res[0]);
+            const bool progress = rn < sc[S_RN];
+            sc[S_RN] = rn;
+            sc[S_REF] += 1.0;
+            d.act[1] = isfinite(rn) && progress && rn > sc[S_LTOL] * sc[S_BN] && sc[S_REF] < maxit;
+            sc[S_CONT] = d.act[1] ? 1.0 : 0.0;   // another round before the Newton update
+            break;
+        }
         default:
             sc[slot] = res[0];
             break;
@@ -1925,9 +1936,9 @@ __global__ void k_count_active(const Desc* __restrict__ D, int ninst, int which,
 // refinement bookkeeping (one thread per instance):
 //   mode 0: start  -> act[1] = ||F|| > 0; S_BN = S_RN = ||F||; counters reset (skipped while
 //           S_CONT marks a refinement that continues from the previous round)
-//   mode 1: after a defect: S_RN = sqrt(S_FN); act[1] = S_RN > tol*S_BN && S_REF < max_refine
-//           and the defect decreased (stagnation stops the refinement); S_CONT = act[1]: the
-//           Newton update waits for the next round
+//   (after a defect, the finaliser's A_REFINE: S_RN = sqrt(S_FN); act[1] = S_RN > S_LTOL
+//   S_BN && S_REF < max_refine and the defect decreased (stagnation stops the refinement);
+//   S_CONT = act[1]: the Newton update waits for the next round)
 struct Forcing {
     double rel, abs, forcing, eta_max;
     double forcing0;   // first iteration of a step: forcing0 * (||F_1|| / ||F_0|| of the latest step)
@@ -1939,7 +1950,6 @@ __global__ void k_refine_flags(const Desc* __restrict__ D, int ninst, int mode, 
     if (i >= ninst) return;
     const Desc& d = D[i];
     double* sc = d.sc;
-    if (mode == 1 && gated_off(d)) return;
     if (mode == 0) {
         if (sc[S_CONT] != 0.0) return;   // refinement of the current Newton iteration continues
         sc[S_BN] = sqrt(sc[S_FN]);
@@ -1967,14 +1977,6 @@ __global__ void k_refine_flags(const Desc* __restrict__ D, int ninst, int mode, 
         }
         sc[S_LTOL] = eta;
         sc[S_TOL2] = fc.mixed ? fmax(sc[S_ITOL], eta * eta) : eta * eta;
-    } else {
-        if (!d.act[1]) return;
-        const double rn = sqrt(sc[S_FN]);   // defect norm stored by k_fin(A_STORE, S_FN)
-        const bool progress = rn < sc[S_RN];
-        sc[S_RN] = rn;
-        sc[S_REF] += 1.0;
-        d.act[1] = isfinite(rn) && progress && rn > sc[S_LTOL] * sc[S_BN] && sc[S_REF] < max_refine;
-        sc[S_CONT] = d.act[1] ? 1.0 : 0.0;   // another round before the Newton update
     }
 }
 
@@ -3261,9 +3263,12 @@ struct HeatBatch::Impl {
         const unsigned ni = (unsigned)t.inst.size();
         HLP(C_PC, (k_accum<T, CP>), 1, -1);
         launch_defect(t);
-        fin(t, Fcell(S_FN));
-        hipLaunchKernelGGL(k_refine_flags, dim3((ni + 63) / 64), dim3(64), 0, S(), t.desc->p, (int)ni, 1,
-                           cfg.solver_tol, cfg.max_refine, forcing());
+        // ||r64||^2 and the refinement flags in the finaliser (k_refine_flags' mode 1)
+        Fin f = Fcell(S_FN);
+        f.act = A_REFINE;
+        f.maxit = cfg.max_refine;
+        fin(t, f);
+        (void)ni;
     }
     double tol_call = 0.0;   // > 0: the tolerance of the current solve_linear call (linear-system instances)
     Forcing forcing() const {
