@@ -774,6 +774,7 @@ static FusedConfig fused_config(const char* params) {
     c.krylov_form_heat = m.get_string("krylov_form_heat", c.krylov_form_heat);
     c.krylov_form_twophase = m.get_string("krylov_form_twophase", c.krylov_form_twophase);
     c.pair_vectors = m.get_int("pair_vectors", c.pair_vectors ? 1 : 0) != 0;
+    c.operator_tiles = m.get_int("operator_tiles", c.operator_tiles ? 1 : 0) != 0;
     c.twophase_pressure_cg = m.get_int("twophase_pressure_cg", c.twophase_pressure_cg ? 1 : 0) != 0;
     c.twophase_pressure_mg = m.get_int("twophase_pressure_mg", c.twophase_pressure_mg ? 1 : 0) != 0;
     c.mg_scale = m.get_double("mg_scale", c.mg_scale);

@@ -61,6 +61,7 @@ struct Desc {
     int ns;                       // Krylov slots per pair rank: 1 (reduced system) or 2 (full system)
     int ngrpk;                    // Krylov partials: groups [gbase, gbase + ngrpk) of BS pairs
     int vec;                      // Krylov kernels in the four-ranks-per-lane form (2D heat, nx % 8 == 0)
+    int tile;                     // vec, nx % 512 == 0: the reduced operator in 4-row tiles (k_schur_tile)
     int cgp;                      // two-phase, reduced form: the pressure stage (symmetric) is solved by CG
     int mgp;                      // two-phase, full form: pressure by CG with the multigrid V-cycle
     int mgl;                      // multigrid: levels below this one (0 on the coarsest level)
@@ -1073,7 +1074,7 @@ __global__ __launch_bounds__(BS) void k_schur_vec(const Desc* __restrict__ D, co
                                                   int meth) {
     LOCATE();
     (void)base;
-    if (d.pcm != PM_SCHUR || !d.vec || !(meth & meth_of(d))) return;   // block-uniform
+    if (d.pcm != PM_SCHUR || !d.vec || d.tile || !(meth & meth_of(d))) return;   // block-uniform
     const int h = d.h, hr = d.nx >> 1, ny = d.ny;
     const T* __restrict__ v = (const T*)d.v[iv];
     const T* __restrict__ dvr = (const T*)d.dinv;
@@ -1132,6 +1133,117 @@ __global__ __launch_bounds__(BS) void k_schur_vec(const Desc* __restrict__ D, co
             vgroup_store(d, bi, CPT, g, s2, part + 2 * (long)ng);
             vgroup_store(d, bi, CPT, g, s3, part + 3 * (long)ng);
             vgroup_store(d, bi, CPT, g, s4, part + 4 * (long)ng);
+        }
+    }
+}
+// The reduced operator in tiles (Desc::tile: four ranks per lane, nx % 512 == 0, so a row
+// holds a whole number of 256-rank groups).  Block b of an instance takes tile b: the 256
+// pair ranks of group column b % G (G = nx / 512 groups per row) in the four rows from
+// 4 (b / G).  Phase 1: the block's threads form y_r (k_schur_vec's red unknowns, the same
+// arithmetic) of the six red rows the tile touches, four ranks each, into LDS, and four
+// threads the red cell just outside the tile in each of its rows (west or east, by the
+// row's colours).  Phase 2: wave w forms black row 4 (b / G) + w from LDS with the fused
+// dots, one partial per 256-rank group as k_schur_vec stores it; w is bit-identical to
+// k_schur_vec's.  Each red unknown is formed 1.5 times per tile instead of 3 times, and
+// the neighbour values come from LDS instead of L2.
+constexpr int TILE_ROW = 264;   // LDS values per tile row: [3] west cell, [4, 260) the tile's ranks, [260] east cell
+template <class T>
+__global__ __launch_bounds__(BS) void k_schur_tile(const Desc* __restrict__ D, const int2* __restrict__ blk, int iv,
+                                                   int iw, int nd, int a0, double* __restrict__ part, int ng,
+                                                   int meth) {
+    const int lb = D[0].xcd ? xcd_block((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x;
+    const int2 bi = blk[lb];
+    const Desc& d = D[bi.x];
+    if (d.pcm != PM_SCHUR || !d.tile || !(meth & meth_of(d))) return;   // block-uniform
+    __shared__ __align__(16) T ylds[6][TILE_ROW];
+    const int hr = d.nx >> 1, ny = d.ny, G = hr >> 8;
+    const int band = bi.y / G, col = bi.y - band * G, j0 = 4 * band, qt = col * 256;
+    const int w = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
+    const int j = j0 + w, q0 = qt + 4 * lane, p0 = j * hr + q0;
+    if (!d.act[0]) {   // block-uniform: zero partials, as the other operator kernels store them
+        if (lane == 0 && j < ny) {
+            const long gi = d.gbase + (p0 >> 8);
+            for (int q = 0; q < nd && q < 5; ++q) part[q * (long)ng + gi] = 0.0;
+        }
+        return;
+    }
+    const T* __restrict__ v = (const T*)d.v[iv];
+    const T* __restrict__ dvr = (const T*)d.dinv;
+    const T* kr[5] = {nullptr, (const T*)d.K[1], (const T*)d.K[2], (const T*)d.K[3], (const T*)d.K[4]};
+    const V4<T> zero{{(T)0, (T)0, (T)0, (T)0}};
+    // phase 2's own-row operands, loaded first so their latency overlaps phase 1
+    const int h = d.h, pl = j < ny ? p0 : q0;   // (rows below the grid: an in-bounds stand-in)
+    const V4<T> kw = ld4((const T*)d.K[1] + h + pl), ke = ld4((const T*)d.K[2] + h + pl);
+    const V4<T> ks = ld4((const T*)d.K[3] + h + pl), kn = ld4((const T*)d.K[4] + h + pl);
+    const V4<T> cb = ld4((const T*)d.K[0] + h + pl), vv = ld4(v + pl), aa = ld4((const T*)d.v[a0] + pl);
+    // phase 1: red rows j0 - 1 .. j0 + 4 (window rows 0 .. 5; zero outside the grid)
+    for (int c = (int)threadIdx.x; c < 6 * 64; c += BS) {
+        const int r = c >> 6, l = c & 63, jr = j0 - 1 + r, qc = qt + 4 * l;
+        V4<T> y = zero;
+        if (jr >= 0 && jr < ny) {
+            const int pr = jr * hr + qc;
+            const V4<T> o = voffdiag<T>(d, kr, pr, jr, qc, 0, v);
+            const V4<T> dd = ld4(dvr + pr);
+#pragma unroll
+            for (int m = 0; m < 4; ++m) y.e[m] = o.e[m] * dd.e[m];
+        }
+        *reinterpret_cast<V4<T>*>(&ylds[r][4 + 4 * l]) = y;
+    }
+    if (threadIdx.x < 4) {   // the red cells outside the tile in black row j0 + threadIdx.x
+        const int jb = j0 + (int)threadIdx.x;
+        T yw = (T)0, ye = (T)0;
+        if (jb < ny) {
+            const int pb = jb * hr + qt;
+            if ((jb & 1) != 0) {   // black cells at even columns: the west neighbour of the first
+                if (qt > 0) yw = red_y1<T>(d, kr, dvr, v, pb - 1, jb, false);
+            } else if (qt + 256 < hr) {   // odd columns: the east neighbour of the last
+                ye = red_y1<T>(d, kr, dvr, v, pb + 256, jb, true);
+            }
+        }
+        ylds[1 + threadIdx.x][3] = yw;
+        ylds[1 + threadIdx.x][4 + 256] = ye;
+    }
+    __syncthreads();
+    if (j >= ny) return;   // wave-uniform: the tile's rows below the grid
+    // phase 2: black row j (window row w + 1)
+    const int rj = w + 1, o0 = 4 + 4 * lane;
+    const V4<T> yc = *reinterpret_cast<const V4<T>*>(&ylds[rj][o0]);
+    const V4<T> ys = *reinterpret_cast<const V4<T>*>(&ylds[rj - 1][o0]);
+    const V4<T> yn = *reinterpret_cast<const V4<T>*>(&ylds[rj + 1][o0]);
+    const bool even = (j & 1) != 0;   // black cells at the even columns of row j
+    const T ye = even ? ylds[rj][o0 - 1] : ylds[rj][o0 + 4];
+    V4<T> wc;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        const T xw = even ? (m == 0 ? ye : yc.e[m - 1]) : yc.e[m];
+        const T xe = even ? yc.e[m] : (m == 3 ? ye : yc.e[m + 1]);
+        const T o = stencil4(kw.e[m], xw, ke.e[m], xe, ks.e[m], ys.e[m], kn.e[m], yn.e[m]);
+        wc.e[m] = cb.e[m] * vv.e[m] - o;
+    }
+    st4((T*)d.v[iw] + p0, wc);
+    double s0 = dot4d(aa, wc), s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0;
+    if (nd > 1) s1 = dot4d(wc, wc);
+    if (nd > 2) {
+        const V4<T> rh = ld4((const T*)d.v[V_RH] + p0);
+        s2 = dot4d(rh, wc);
+        s3 = dot4d(rh, aa);
+        s4 = dot4d(aa, aa);
+    }
+    const long gi = d.gbase + (p0 >> 8);
+    s0 = wave_sum(s0);
+    if (nd > 1) s1 = wave_sum(s1);
+    if (nd > 2) {
+        s2 = wave_sum(s2);
+        s3 = wave_sum(s3);
+        s4 = wave_sum(s4);
+    }
+    if (lane == 0) {
+        part[gi] = s0;
+        if (nd > 1) part[ng + gi] = s1;
+        if (nd > 2) {
+            part[2 * (long)ng + gi] = s2;
+            part[3 * (long)ng + gi] = s3;
+            part[4 * (long)ng + gi] = s4;
         }
     }
 }
@@ -2327,12 +2439,15 @@ struct Table {
     int nb[4] = {};          // blocks of each map
     DevPtr<int2> pblk[4];    // pair kernels (Krylov space): the same over each instance's h pair ranks
     int pnb[4] = {};
+    DevPtr<int2> tblk;       // tiled operator: block -> (instance, tile) (k_schur_tile)
+    int tnb = 0;
     int ng = 0;              // partial groups over all instances (an instance has ceil(n / BS))
     bool has_schur = false, has_full = false, has_ilu = false;   // Krylov forms present
     bool has_tp = false;                                         // two-phase instances present
     // instances whose Krylov kernels run in the four-ranks-per-lane form, and the others
     bool has_vec = false, has_nonvec = false;
     bool has_schur_strided = false;   // reduced-form instances in the strided form
+    bool has_tile = false;            // four-ranks-per-lane instances with the tiled operator
     // Multigrid plan over the instances with a hierarchy; index l = 1 .. levels - 1: the
     // instances that have level l, their level l - 1 (fine) and level l (coarse)
     // descriptors, blocks over the coarse cells, the fine pairs and the coarse pairs.
@@ -2606,6 +2721,7 @@ struct HeatBatch::Impl {
         d.ns = d.pcm == PM_SCHUR ? 1 : 2;
         d.vec = cfg.pair_vectors && (I.kind == K_HEAT || (I.kind == K_LINEAR && I.nz == 1)) && I.pcm == PM_SCHUR &&
                 I.nx % 8 == 0;
+        d.tile = d.vec && cfg.operator_tiles && I.nx % 512 == 0;
         d.cgp = cg_capable(I);
         d.mgp = I.mg_on;
         d.mgs = cfg.mg_scale;
@@ -2644,6 +2760,7 @@ struct HeatBatch::Impl {
             t.has_vec |= d.vec != 0;
             t.has_nonvec |= d.vec == 0;
             t.has_schur_strided |= d.pcm == PM_SCHUR && d.vec == 0;
+            t.has_tile |= d.tile != 0;
         }
         t.ng = g;
         for (int ci = 0; ci < 4; ++ci) {
@@ -2658,6 +2775,16 @@ struct HeatBatch::Impl {
             t.blk[ci] = dev_from_host(blk.data(), blk.size());
             t.pnb[ci] = (int)pblk.size();
             t.pblk[ci] = dev_from_host(pblk.data(), pblk.size());
+        }
+        {   // tiles of four rows x 256 pair ranks (k_schur_tile)
+            std::vector<int2> tb;
+            for (size_t q = 0; q < list.size(); ++q) {
+                if (!ds[q].tile) continue;
+                const int tiles = (ds[q].nx / 512) * ((ds[q].ny + 3) / 4);
+                for (int b = 0; b < tiles; ++b) tb.push_back(make_int2((int)q, b));
+            }
+            t.tnb = (int)tb.size();
+            if (!tb.empty()) t.tblk = dev_from_host(tb.data(), tb.size());
         }
         t.desc = dev_from_host(ds.data(), ds.size());
         t.part = dev_alloc<double>(5 * (size_t)t.ng + 2);   // up to five dot products per launch
@@ -2902,6 +3029,11 @@ struct HeatBatch::Impl {
             HLP(C_SPMV, (k_schur_black<T, CP>), iv, iw, nd, a0, a1, t.part->p, t.ng, meth);
         }
         if (t.has_vec) HLP(C_SPMV, (k_schur_vec<T, CP>), iv, iw, nd, a0, t.part->p, t.ng, meth);
+        if (t.has_tile) {
+            hipLaunchKernelGGL((k_schur_tile<T>), dim3((unsigned)t.tnb), dim3(BS), 0, S(), t.desc->p, t.tblk->p, iv,
+                               iw, nd, a0, t.part->p, t.ng, meth);
+            EQ_HIP(hipGetLastError());
+        }
         if (t.has_full) HLP(C_SPMV, (k_spmv_rb<T, CP>), iv, iw, nd, a0, a1, t.part->p, t.ng, meth);
     }
     // Multigrid: coarse operators and coarsest inverses of the instances about to solve a
@@ -3923,9 +4055,20 @@ double HeatBatch::time_kernel(const std::string& name, int reps, double* event_m
     }
     bool red = false, full = false, ilu = false;
     for (Inst* I : all) red |= I->pcm == PM_SCHUR, full |= I->pcm != PM_SCHUR, ilu |= I->pcm == PM_ILU;
-    bool vec = false;
-    for (Inst* I : all) vec |= P.desc_of(*I).vec != 0;
-    if (name == "operator" && vec) {
+    bool vec = false, tile = false;
+    for (Inst* I : all) vec |= P.desc_of(*I).vec != 0, tile |= P.desc_of(*I).tile != 0;
+    if (name == "operator" && tile) {
+        launch = [&](hipEvent_t s0, hipEvent_t s1) {
+            if (P.mixed)
+                hipExtLaunchKernelGGL((k_schur_tile<float>), dim3((unsigned)t.tnb), dim3(BS), 0, S(), s0, s1, 0,
+                                      t.desc->p, t.tblk->p, (int)V_PH, (int)V_V, 1, (int)V_RH, t.part->p, t.ng,
+                                      (int)(M_BICG | M_CG));
+            else
+                hipExtLaunchKernelGGL((k_schur_tile<double>), dim3((unsigned)t.tnb), dim3(BS), 0, S(), s0, s1, 0,
+                                      t.desc->p, t.tblk->p, (int)V_PH, (int)V_V, 1, (int)V_RH, t.part->p, t.ng,
+                                      (int)(M_BICG | M_CG));
+        };
+    } else if (name == "operator" && vec) {
         if (P.mixed)
             TLP(C_SPMV, (k_schur_vec<float, CP>), (int)V_PH, (int)V_V, 1, (int)V_RH, t.part->p, t.ng,
                 (int)(M_BICG | M_CG));

@@ -67,6 +67,9 @@ struct FusedConfig {
     // four consecutive pair ranks of one row (16-byte loads) instead of ranks strided by the
     // block size.  Other instances always take the strided form.
     bool pair_vectors = true;
+    // Of those, instances with nx % 512 == 0 apply the reduced operator in tiles of four
+    // rows x 256 pair ranks per workgroup (red unknowns shared through LDS).
+    bool operator_tiles = true;
     // Two-phase instances solve the pressure stage (a symmetric M-matrix) by preconditioned
     // CG instead of BiCGStab: in the reduced form with the Schur complement's diagonal, in
     // the full form with an aggregation multigrid V-cycle (twophase_pressure_mg).

@@ -216,7 +216,9 @@ int eqhip_twophase_destroy(void* sim);
  * red-black ILU(0).  twophase_pressure_cg (0|1, default 1): CG for the symmetric pressure
  * stage of two-phase instances; twophase_pressure_mg (0|1, default 1): with the aggregation
  * multigrid V-cycle as its preconditioner (full form).  pair_vectors (0|1, default 1): four
- * pair ranks per lane in the Krylov kernels of 2D heat instances with nx % 8 == 0. */
+ * pair ranks per lane in the Krylov kernels of 2D heat instances with nx % 8 == 0;
+ * operator_tiles (0|1, default 1): of those, instances with nx % 512 == 0 apply the reduced
+ * operator in tiles of four rows x 256 pair ranks per workgroup. */
 int eqhip_batch_create(const char* params, void** batch);
 int eqhip_batch_destroy(void* batch);
 int eqhip_batch_add(void* batch, int nx, int ny, double k, double cv, const double* u, int* slot);

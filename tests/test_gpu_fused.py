@@ -389,6 +389,36 @@ def test_pair_vectors_match_strided_form(hip_api, precision, tol):
         assert np.linalg.norm(a - v) <= tol * np.linalg.norm(a), np.linalg.norm(a - v) / np.linalg.norm(a)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_operator_tiles_match_one_pass_operator(hip_api, precision):
+    """The reduced operator in tiles of four rows x 256 pair ranks (nx % 512 == 0, red
+    unknowns shared through LDS) gives steps bit-identical to the one-pass operator, next
+    to instances that take the one-pass operator (nx = 80) and the strided kernels (nx = 70)
+    in the same batch; the heights are not multiples of four (tiles below the grid)."""
+    sizes = [(512, 94, 0.2), (1024, 37, 0.1), (80, 64, 0.3), (70, 45, 0.4)]
+    out = {}
+    for tiles in (0, 1):
+        b = make_batch(hip_api, f"precision={precision} preconditioner=ilu0 rel_res_tol=1e-9 abs_res_tol=0 "
+                                f"solver_tol=1e-6 operator_tiles={tiles}")
+        try:
+            slots = [add(hip_api, b, nx, ny, k, gaussian_bumps(nx, ny, 50 + q, width_scale=0.05))
+                     for q, (nx, ny, k) in enumerate(sizes)]
+            arr = np.array(slots, dtype=np.int32)
+            res = []
+            for _ in range(2):
+                hip_api.batch_step(b, iptr(arr), len(arr), 1.0)
+                for sl, (nx, ny, _) in zip(slots, sizes):
+                    st, _ = stats(hip_api, b, sl)
+                    assert st[3] != 0 and st[4] == 0, (tiles, st)
+                    res.append((state(hip_api, b, sl, nx * ny), st[:3].tolist()))
+            out[tiles] = res
+        finally:
+            hip_api.batch_destroy(b)
+    for (ua, sa), (ub, sb) in zip(out[0], out[1]):
+        assert sa == sb
+        assert np.array_equal(ua, ub)
+
+
 def test_launch_shapes_give_identical_results(hip_api):
     """Every launch shape (cells per thread 1/2/4/8, XCD block order on/off) gives
     bit-identical steps: reductions use fixed groups of cells, not blocks."""
