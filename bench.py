@@ -117,6 +117,7 @@ def run_fused(lib, precision, u0, nx, ny, dt, steps, warmup, precond="ilu0", gra
         # iterations end in one fused update (b31) instead of b1 and b3
         vec = precond == "ilu0" and nx % 8 == 0
         kern["tiles"] = vec and nx % 512 == 0
+        kern["vec"] = vec
         names = (("operator",) if vec else ("operator_red", "operator_black") if precond == "ilu0" else ("spmv",)) + \
             ("assemble",) + (("defect",) if precision == "mixed" else ()) + \
             (("b2", "b31", "setup") if vec else ("b1", "b2", "b3", "load_rhs"))
@@ -763,6 +764,17 @@ KERNELS = {
 }
 
 
+# The four-ranks-per-lane form solves the column-scaled reduced system (the preconditioner
+# lives in the black columns): its vector kernels carry no preconditioned vectors.
+KERNELS_VEC = {
+    "b2": (lambda n, h, ks: 3 * ks * h, "r, v read; s written"),
+    "b31": (lambda n, h, ks: 8 * ks * h, "y, p, s, t, v read; y, r, p written"),
+    "setup": (lambda n, h, ks: 8 * n + 21 * ks * h,
+              "r64 (FP64) and the 10 coefficient arrays read; 1/C_r, 1/P_b, the 5 scaled black columns, "
+              "r, rh, p, y written"),
+}
+
+
 def kernel_table(run, n, ks, steps):
     h = (n + 1) // 2
     kern = run["kern"]
@@ -778,6 +790,8 @@ def kernel_table(run, n, ks, steps):
             continue
         if name == "operator" and kern.get("tiles"):   # nx % 512 == 0: the tiled form of the operator
             kname = "k_schur_tile<T>"
+        if kern.get("vec") and name in KERNELS_VEC:   # the column-scaled four-ranks-per-lane form
+            fb, text = KERNELS_VEC[name]
         _, ms = kern[name]
         bytes_ = fb(n, h, ks)
         gbs = bytes_ / (ms * 1e-3) / 1e9

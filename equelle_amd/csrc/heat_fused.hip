@@ -1562,18 +1562,14 @@ __global__ __launch_bounds__(BS) void k_b2(const Desc* __restrict__ D, const int
     if (!d.act[0] || meth_of(d) != M_BICG) return;
     const T alpha = (T)d.sc[S_ALPHA];
     const T* dv = slot_dinv<T>(d);
-    if (d.vec) {
+    if (d.vec) {   // the columns carry the preconditioner (k_setup_vec): no sh
         FOR_VGROUPS(g, p0) {
             if (p0 >= d.h) continue;
-            const V4<T> r = ld4((const T*)d.v[V_R] + p0), vv = ld4((const T*)d.v[V_V] + p0), dd = ld4(dv + p0);
-            V4<T> sv, sh;
+            const V4<T> r = ld4((const T*)d.v[V_R] + p0), vv = ld4((const T*)d.v[V_V] + p0);
+            V4<T> sv;
 #pragma unroll
-            for (int m = 0; m < 4; ++m) {
-                sv.e[m] = r.e[m] - alpha * vv.e[m];
-                sh.e[m] = dd.e[m] * sv.e[m];
-            }
+            for (int m = 0; m < 4; ++m) sv.e[m] = r.e[m] - alpha * vv.e[m];
             st4((T*)d.v[V_S] + p0, sv);
-            st4((T*)d.v[V_SH] + p0, sh);
         }
         return;
     }
@@ -1615,9 +1611,10 @@ __global__ __launch_bounds__(BS) void k_b3(const Desc* __restrict__ D, const int
 }
 
 // Four-ranks-per-lane form, the end of a BiCGStab iteration and the start of the next in
-// one pass (rho, ||r||^2 and beta come from the second finaliser, A_OMEGA):
-//   x += alpha ph + omega sh ; r = s - omega t ;
-//   while the solve goes on: p = r + beta (p - omega v), ph = M^-1 p.
+// one pass (rho, ||r||^2 and beta come from the second finaliser, A_OMEGA), on the column-
+// scaled system (no preconditioner vectors: p and s are the operator's operands):
+//   y += alpha p + omega s ; r = s - omega t ;
+//   while the solve goes on: p = r + beta (p - omega v).
 // S_XUPD says the iteration's update is pending (the instance was active when it started).
 template <class T, int CPT>
 __global__ __launch_bounds__(BS) void k_b31(const Desc* __restrict__ D, const int2* __restrict__ blk) {
@@ -1631,30 +1628,24 @@ __global__ __launch_bounds__(BS) void k_b31(const Desc* __restrict__ D, const in
     T* __restrict__ x = (T*)d.v[V_X];
     T* __restrict__ r = (T*)d.v[V_R];
     T* __restrict__ p = (T*)d.v[V_P];
-    T* __restrict__ ph = (T*)d.v[V_PH];
-    const T* __restrict__ dv = slot_dinv<T>(d);
     FOR_VGROUPS(g, p0) {
         if (p0 >= d.h) continue;
-        const V4<T> xo = ld4(x + p0), pho = ld4(ph + p0), sh = ld4((const T*)d.v[V_SH] + p0);
+        const V4<T> xo = ld4(x + p0), po = ld4(p + p0);
         const V4<T> sv = ld4((const T*)d.v[V_S] + p0), tv = ld4((const T*)d.v[V_T] + p0);
         V4<T> xn, rn;
 #pragma unroll
         for (int m = 0; m < 4; ++m) {
-            xn.e[m] = xo.e[m] + (alpha * pho.e[m] + omega * sh.e[m]);
+            xn.e[m] = xo.e[m] + (alpha * po.e[m] + omega * sv.e[m]);
             rn.e[m] = sv.e[m] - omega * tv.e[m];
         }
         st4(x + p0, xn);
         st4(r + p0, rn);
         if (more) {
-            const V4<T> po = ld4(p + p0), vv = ld4((const T*)d.v[V_V] + p0), dd = ld4(dv + p0);
-            V4<T> pn, phn;
+            const V4<T> vv = ld4((const T*)d.v[V_V] + p0);
+            V4<T> pn;
 #pragma unroll
-            for (int m = 0; m < 4; ++m) {
-                pn.e[m] = rn.e[m] + beta * (po.e[m] - omega * vv.e[m]);
-                phn.e[m] = dd.e[m] * pn.e[m];
-            }
+            for (int m = 0; m < 4; ++m) pn.e[m] = rn.e[m] + beta * (po.e[m] - omega * vv.e[m]);
             st4(p + p0, pn);
-            st4(ph + p0, phn);
         }
     }
 }
@@ -2143,23 +2134,31 @@ __global__ __launch_bounds__(BS) void k_load_rhs(const Desc* __restrict__ D, con
 // Four-ranks-per-lane form, the start of a solve in one pass (k_precond_setup, k_load_rhs
 // and k_kinit of the strided forms): a lane takes the four red and the four black cells of
 // pair ranks p0 .. p0 + 3 of row j and forms
-//   1 / C_r of its red cells and 1 / P_b of its black ones, P_b = C_b - sum over the red
-//     neighbours m of A_bm A_mb / C_m (the red-black ILU(0) pivot), with the neighbours'
-//     1 / C_m formed from their C_m as their own lanes form them;
+//   1 / C_r of its red cells and, in the first refinement round of a Newton iteration
+//     (scale_cols), 1 / P_b of its black ones, P_b = C_b - sum over the red neighbours m of
+//     A_bm A_mb / C_m (the red-black ILU(0) pivot, the neighbours' 1 / C_m formed from
+//     their C_m as their own lanes form them), and then the black columns scaled by 1 / P_b
+//     in place: C_b and the red coefficients A_mb.  The Krylov solve then runs on S D
+//     (D = diag(1 / P_b)), the right-preconditioned reduced system, without a
+//     preconditioner vector: its iterate y gives x_b = D y (k_accum);
 //   g_b = b_b - sum_m A_bm (b_m / C_m), b = (T)(src scale) (src = F in the first refinement
-//     round, r64 after it; scale = 1 / ||src|| in mixed precision);
-//   r = rh = p = g, ph = g / P_b, x = 0, and the partial dots (r.r, r.r) for A_INIT.
-// Reads the 10 coefficient arrays and src once, writes dinv and the five vectors.
+//     round, r64 after it; scale = 1 / ||src|| in mixed precision; A_bm is the black row's,
+//     which the scaling leaves alone);
+//   r = rh = p = g, y = 0, and the partial dots (r.r, r.r) for A_INIT.
 template <class T>
 __device__ inline T inv_or_one(T c) {
     return c != (T)0 ? (T)1 / c : (T)1;
 }
 template <class T, int CPT>
 __global__ __launch_bounds__(BS) void k_setup_vec(const Desc* __restrict__ D, const int2* __restrict__ blk,
-                                                  double* __restrict__ part, int ng, int scaled, int first) {
+                                                  double* __restrict__ part, int ng, int scaled, int first,
+                                                  int scale_cols) {
     LOCATE();
     (void)base;
     if (!d.vec) return;   // block-uniform
+    // the first refinement round of a Newton iteration (the coefficients as assembled):
+    // pivots, then the columns scaled; later rounds reuse both
+    const bool cols = scale_cols && d.sc[S_REF] == 0.0;
     const bool on = d.act[1] != 0;
     const double scale = scaled ? 1.0 / d.sc[S_RN] : 1.0;
     const bool fst = first < 0 ? d.sc[S_REF] == 0.0 : first != 0;   // first < 0: the instance's own round
@@ -2171,6 +2170,11 @@ __global__ __launch_bounds__(BS) void k_setup_vec(const Desc* __restrict__ D, co
     const T* __restrict__ K3 = (const T*)d.K[3];
     const T* __restrict__ K4 = (const T*)d.K[4];
     T* __restrict__ dv = (T*)d.dinv;
+    T* __restrict__ K0w = (T*)d.K[0];
+    T* __restrict__ K1w = (T*)d.K[1];
+    T* __restrict__ K2w = (T*)d.K[2];
+    T* __restrict__ K3w = (T*)d.K[3];
+    T* __restrict__ K4w = (T*)d.K[4];
     FOR_VGROUPS(g, p0) {
         double rr = 0.0;
         if (p0 < h) {
@@ -2195,21 +2199,61 @@ __global__ __launch_bounds__(BS) void k_setup_vec(const Desc* __restrict__ D, co
             const V4<T> cb = ld4(K0 + h + p0), kw = ld4(K1 + h + p0), ke = ld4(K2 + h + p0);
             const V4<T> ks = ld4(K3 + h + p0), kn = ld4(K4 + h + p0);
             V4<T> db;
+            if (cols) {
 #pragma unroll
-            for (int m = 0; m < 4; ++m) {
-                // the black cell's red W / E neighbours: (C, the coefficient towards the black cell)
-                const T cW = evb ? (m == 0 ? cre : cr.e[m > 0 ? m - 1 : 0]) : cr.e[m];
-                const T aW = evb ? (m == 0 ? kee : ker.e[m > 0 ? m - 1 : 0]) : ker.e[m];
-                const T cE = evb ? cr.e[m] : (m == 3 ? cre : cr.e[m < 3 ? m + 1 : 3]);
-                const T aE = evb ? kwr.e[m] : (m == 3 ? kwe : kwr.e[m < 3 ? m + 1 : 3]);
-                T piv = cb.e[m];
-                piv -= kw.e[m] * aW / cW;
-                piv -= ke.e[m] * aE / cE;
-                piv -= ks.e[m] * kns.e[m] / crs.e[m];
-                piv -= kn.e[m] * ksn.e[m] / crn.e[m];
-                db.e[m] = inv_or_one(piv);
+                for (int m = 0; m < 4; ++m) {
+                    // the black cell's red W / E neighbours: (C, the coefficient towards the black cell)
+                    const T cW = evb ? (m == 0 ? cre : cr.e[m > 0 ? m - 1 : 0]) : cr.e[m];
+                    const T aW = evb ? (m == 0 ? kee : ker.e[m > 0 ? m - 1 : 0]) : ker.e[m];
+                    const T cE = evb ? cr.e[m] : (m == 3 ? cre : cr.e[m < 3 ? m + 1 : 3]);
+                    const T aE = evb ? kwr.e[m] : (m == 3 ? kwe : kwr.e[m < 3 ? m + 1 : 3]);
+                    T piv = cb.e[m];
+                    piv -= kw.e[m] * aW / cW;
+                    piv -= ke.e[m] * aE / cE;
+                    piv -= ks.e[m] * kns.e[m] / crs.e[m];
+                    piv -= kn.e[m] * ksn.e[m] / crn.e[m];
+                    db.e[m] = inv_or_one(piv);
+                }
+                st4(dv + h + p0, db);
+                // right preconditioning by column scaling: the columns of the black cells (C_b
+                // and the four red coefficients towards each black cell) times 1 / P_b.  Each
+                // of those red coefficients couples its red cell to one black cell only, whose
+                // lane alone reads it (for the pivot above) and writes it here.
+                V4<T> cbs, kwo, keo, kso, kno;
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    cbs.e[m] = cb.e[m] * db.e[m];
+                    kso.e[m] = kns.e[m] * db.e[m];   // red S neighbour's N coefficient
+                    kno.e[m] = ksn.e[m] * db.e[m];   // red N neighbour's S coefficient
+                }
+                st4(K0w + h + p0, cbs);
+                if (j > 0) st4(K4w + p0 - hr, kso);
+                if (j < ny - 1) st4(K3w + p0 + hr, kno);
+                // row j.  evb: red rank p0 + m (an odd column) couples by its W coefficient to
+                // black m and by its E coefficient to black m + 1, so this lane owns K1 of red
+                // p0 .. p0 + 3 and K2 of red p0 - 1 .. p0 + 2.  Otherwise red p0 + m (an even
+                // column) couples by its E coefficient to black m and by its W one to black
+                // m - 1: this lane owns K2 of red p0 .. p0 + 3 and K1 of red p0 + 1 .. p0 + 4.
+                // (Each lane read these values above; the ones it loaded but does not own are
+                // not used.)
+                if (evb) {
+#pragma unroll
+                    for (int m = 0; m < 4; ++m) kwo.e[m] = kwr.e[m] * db.e[m];
+                    st4(K1w + p0, kwo);
+                    if (has_e) K2w[p0 - 1] = kee * db.e[0];
+                    K2w[p0] = ker.e[0] * db.e[1];
+                    K2w[p0 + 1] = ker.e[1] * db.e[2];
+                    K2w[p0 + 2] = ker.e[2] * db.e[3];
+                } else {
+#pragma unroll
+                    for (int m = 0; m < 4; ++m) keo.e[m] = ker.e[m] * db.e[m];
+                    st4(K2w + p0, keo);
+                    K1w[p0 + 1] = kwr.e[1] * db.e[0];
+                    K1w[p0 + 2] = kwr.e[2] * db.e[1];
+                    K1w[p0 + 3] = kwr.e[3] * db.e[2];
+                    if (has_e) K1w[p0 + 4] = kwe * db.e[3];
+                }
             }
-            st4(dv + h + p0, db);
             if (on) {
                 // right-hand side: b at the row's eight cells, the red cells above and below,
                 // and the outside red cell
@@ -2219,7 +2263,7 @@ __global__ __launch_bounds__(BS) void k_setup_vec(const Desc* __restrict__ D, co
                 ld8(src + (j > 0 ? c0 - nx : c0), ss);
                 ld8(src + (j < ny - 1 ? c0 + nx : c0), sn);
                 const T be = has_e ? (T)(src[evb ? c0 - 1 : c0 + 8] * scale) : (T)0;
-                V4<T> gv, ph;
+                V4<T> gv;
 #pragma unroll
                 for (int m = 0; m < 4; ++m) {
                     const int bc = evb ? 2 * m : 2 * m + 1;   // the black cell's place among the eight
@@ -2231,12 +2275,10 @@ __global__ __launch_bounds__(BS) void k_setup_vec(const Desc* __restrict__ D, co
                     const T yS = (T)(ss[bc] * scale) * inv_or_one(crs.e[m]);
                     const T yN = (T)(sn[bc] * scale) * inv_or_one(crn.e[m]);
                     gv.e[m] = (T)(sc[bc] * scale) - stencil4(kw.e[m], yW, ke.e[m], yE, ks.e[m], yS, kn.e[m], yN);
-                    ph.e[m] = db.e[m] * gv.e[m];
                 }
                 st4((T*)d.v[V_R] + p0, gv);
                 st4((T*)d.v[V_RH] + p0, gv);
                 st4((T*)d.v[V_P] + p0, gv);
-                st4((T*)d.v[V_PH] + p0, ph);
                 st4((T*)d.v[V_X] + p0, zero);
                 rr = dot4d(gv, gv);
             }
@@ -2282,16 +2324,20 @@ __global__ __launch_bounds__(BS) void k_accum(const Desc* __restrict__ D, const 
                 vrow(d, p0, j, q0);
                 const bool evr = (j & 1) == 0;   // red cells at the even columns of the row
                 // red unknowns recovered: x_r = C_r^-1 (b_r - A_rb x_b)
+                // the Krylov iterate is y of the column-scaled system (k_setup_vec): the red
+                // coefficients towards black cells are scaled, so A_rb x_b = (scaled A_rb) y,
+                // and x_b = y / P_b
                 const V4<T> o = voffdiag<T>(d, kq, p0, j, q0, 0, x);
-                const V4<T> dc = ld4(dv + p0), xb = ld4(x + p0);
+                const V4<T> dc = ld4(dv + p0), yb = ld4(x + p0), db = ld4(dv + h + p0);
                 double sc[8];
                 ld8(src + c0, sc);
 #pragma unroll
                 for (int m = 0; m < 4; ++m) {
                     const int rc = evr ? 2 * m : 2 * m + 1;
                     const T xr = ((T)(sc[rc] * scale) - o.e[m]) * dc.e[m];
+                    const T xb = yb.e[m] * db.e[m];
                     v[rc] = scaled ? rn * (double)xr : (double)xr;
-                    v[rc ^ 1] = scaled ? rn * (double)xb.e[m] : (double)xb.e[m];
+                    v[rc ^ 1] = scaled ? rn * (double)xb : (double)xb;
                 }
             }
             V4<double> a{{v[0], v[1], v[2], v[3]}}, b{{v[4], v[5], v[6], v[7]}};
@@ -3020,7 +3066,8 @@ struct HeatBatch::Impl {
         HL(C_DEF, (k_defect<CP>), t.part->p);
     }
     // w = A v (full-system instances) and S v (reduced-system instances) with the fused dots
-    // (reduced-system instances of the methods in meth only)
+    // (reduced-system instances of the methods in meth only).  The four-ranks-per-lane
+    // instances solve the column-scaled system: their operand for ph (sh) is p (s).
     template <class T>
     void launch_op(const Table& t, int iv, int iw, int nd, int a0, int a1, int meth = M_BICG | M_CG) {
         TimedLaunch tl(sizeof(T) == 4 ? "operator_fp32" : "operator_fp64");
@@ -3028,9 +3075,10 @@ struct HeatBatch::Impl {
             HLP(C_SPMV, (k_schur_red<T, CP>), iv, meth);
             HLP(C_SPMV, (k_schur_black<T, CP>), iv, iw, nd, a0, a1, t.part->p, t.ng, meth);
         }
-        if (t.has_vec) HLP(C_SPMV, (k_schur_vec<T, CP>), iv, iw, nd, a0, t.part->p, t.ng, meth);
+        const int ivv = iv == V_PH ? (int)V_P : iv == V_SH ? (int)V_S : iv;
+        if (t.has_vec) HLP(C_SPMV, (k_schur_vec<T, CP>), ivv, iw, nd, a0, t.part->p, t.ng, meth);
         if (t.has_tile) {
-            hipLaunchKernelGGL((k_schur_tile<T>), dim3((unsigned)t.tnb), dim3(BS), 0, S(), t.desc->p, t.tblk->p, iv,
+            hipLaunchKernelGGL((k_schur_tile<T>), dim3((unsigned)t.tnb), dim3(BS), 0, S(), t.desc->p, t.tblk->p, ivv,
                                iw, nd, a0, t.part->p, t.ng, meth);
             EQ_HIP(hipGetLastError());
         }
@@ -3106,7 +3154,7 @@ struct HeatBatch::Impl {
     template <class T>
     void conversions(const Table& t) {
         if (t.has_nonvec) HLP(C_PC, (k_load_rhs<T, CP>), 1, 0);
-        if (t.has_vec) HLP(C_PC, (k_setup_vec<T, CP>), t.part->p, t.ng, 1, 0);
+        if (t.has_vec) HLP(C_PC, (k_setup_vec<T, CP>), t.part->p, t.ng, 1, 0, 0);   // (no column scaling)
     }
     template <class T>
     void vec_round(const Table& t) {
@@ -3185,9 +3233,9 @@ struct HeatBatch::Impl {
             HLP(C_VEC, (k_rhs_sum<T, CP>), t.part->p);
             fin(t, F(A_MEAN, 1));
         }
-        if (t.has_vec) {   // the four-ranks-per-lane instances' pivots, right-hand side and start
-            if (mixed) HLP(C_PC, (k_setup_vec<T, CP>), t.part->p, t.ng, 1, -1);
-            else HLP(C_PC, (k_setup_vec<T, CP>), t.part->p, t.ng, 0, 1);
+        if (t.has_vec) {   // the four-ranks-per-lane instances' pivots, column scaling, right-hand side, start
+            if (mixed) HLP(C_PC, (k_setup_vec<T, CP>), t.part->p, t.ng, 1, -1, 1);
+            else HLP(C_PC, (k_setup_vec<T, CP>), t.part->p, t.ng, 0, 1, 1);
         }
         if (t.has_nonvec) HLP(C_VEC, (k_kinit<T, CP>), t.part->p, t.ng);
         fin(t, F(A_INIT, 2));
@@ -4061,19 +4109,19 @@ double HeatBatch::time_kernel(const std::string& name, int reps, double* event_m
         launch = [&](hipEvent_t s0, hipEvent_t s1) {
             if (P.mixed)
                 hipExtLaunchKernelGGL((k_schur_tile<float>), dim3((unsigned)t.tnb), dim3(BS), 0, S(), s0, s1, 0,
-                                      t.desc->p, t.tblk->p, (int)V_PH, (int)V_V, 1, (int)V_RH, t.part->p, t.ng,
+                                      t.desc->p, t.tblk->p, (int)V_P, (int)V_V, 1, (int)V_RH, t.part->p, t.ng,
                                       (int)(M_BICG | M_CG));
             else
                 hipExtLaunchKernelGGL((k_schur_tile<double>), dim3((unsigned)t.tnb), dim3(BS), 0, S(), s0, s1, 0,
-                                      t.desc->p, t.tblk->p, (int)V_PH, (int)V_V, 1, (int)V_RH, t.part->p, t.ng,
+                                      t.desc->p, t.tblk->p, (int)V_P, (int)V_V, 1, (int)V_RH, t.part->p, t.ng,
                                       (int)(M_BICG | M_CG));
         };
     } else if (name == "operator" && vec) {
         if (P.mixed)
-            TLP(C_SPMV, (k_schur_vec<float, CP>), (int)V_PH, (int)V_V, 1, (int)V_RH, t.part->p, t.ng,
+            TLP(C_SPMV, (k_schur_vec<float, CP>), (int)V_P, (int)V_V, 1, (int)V_RH, t.part->p, t.ng,
                 (int)(M_BICG | M_CG));
         else
-            TLP(C_SPMV, (k_schur_vec<double, CP>), (int)V_PH, (int)V_V, 1, (int)V_RH, t.part->p, t.ng,
+            TLP(C_SPMV, (k_schur_vec<double, CP>), (int)V_P, (int)V_V, 1, (int)V_RH, t.part->p, t.ng,
                 (int)(M_BICG | M_CG));
     } else if (name == "operator_red" && red) {
         if (P.mixed) TLP(C_SPMV, (k_schur_red<float, CP>), (int)V_PH, (int)(M_BICG | M_CG));
@@ -4119,8 +4167,8 @@ double HeatBatch::time_kernel(const std::string& name, int reps, double* event_m
         if (P.mixed) TLP(C_PC, (k_load_rhs<float, CP>), 1, 0);
         else TLP(C_PC, (k_load_rhs<double, CP>), 1, 0);
     } else if (name == "setup" && vec) {
-        if (P.mixed) TLP(C_PC, (k_setup_vec<float, CP>), t.part->p, t.ng, 1, 0);
-        else TLP(C_PC, (k_setup_vec<double, CP>), t.part->p, t.ng, 1, 0);
+        if (P.mixed) TLP(C_PC, (k_setup_vec<float, CP>), t.part->p, t.ng, 1, 0, 0);
+        else TLP(C_PC, (k_setup_vec<double, CP>), t.part->p, t.ng, 1, 0, 0);
     } else {
         EQ_REQUIRE(false, "unknown kernel " << name << " (operator [reduced system, four ranks per lane], "
                                                        "operator_red, operator_black [reduced system], spmv, "
