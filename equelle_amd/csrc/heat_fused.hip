@@ -29,8 +29,10 @@ inline hipStream_t S() { return Device::get().stream(); }
 // pressure / transport stage), 0 until measured
 // S_XUPD: the four-ranks-per-lane BiCGStab's pending solution update (set when the
 // iteration's second finaliser ran for the instance, cleared by the next first finaliser)
+// S_U0D: heat, 0 from the start of a step until its first assembly has copied u0 = u (the
+// assembly's finaliser sets it); next to S_F0, which the step start clears with it
 enum Slot { S_RHO = 0, S_RHO_OLD, S_ALPHA, S_OMEGA, S_RR, S_BB, S_BREAK, S_IT, S_FN, S_RN, S_BN, S_REF, S_ITT, S_TOL2,
-            S_MEAN, S_BETA, S_CONT, S_F0, S_LTOL, S_ITOL, S_NK, S_R1, S_R1B, S_XUPD, NSLOT = 24 };
+            S_MEAN, S_BETA, S_CONT, S_F0, S_U0D, S_LTOL, S_ITOL, S_NK, S_R1, S_R1B, S_XUPD, NSLOT = 32 };
 // Krylov vectors; V_Y is the red-rank scratch of the reduced operator (ilu0 only), V_Z the
 // V-cycle's z = M^-1 r (two-phase pressure by CG with multigrid)
 enum Vec { V_R = 0, V_RH, V_P, V_V, V_S, V_T, V_PH, V_SH, V_X, V_Y, V_Z, NVEC };
@@ -404,6 +406,7 @@ __device__ void fin_body(const Desc& d, const double* part, int nb, const Fin& f
         }
         default:
             sc[slot] = res[0];
+            if (slot == S_FN && !f.kry) sc[S_U0D] = 1.0;   // an assembly has run: u0 is in place
             break;
     }
 }
@@ -706,14 +709,18 @@ __global__ __launch_bounds__(BS) void k_assemble(const Desc* __restrict__ D, con
     const double* __restrict__ u0 = d.u0;
     double* __restrict__ Fo = d.F;
     K* __restrict__ co_out[5] = {(K*)d.K[0], (K*)d.K[1], (K*)d.K[2], (K*)d.K[3], (K*)d.K[4]};
+    // the step's first assembly: u0 = u, copied here (S_U0D, set by this launch's finaliser)
+    const bool cpy = d.sc[S_U0D] == 0.0;
+    double* __restrict__ u0w = d.u0;
     FOR_GROUPS(m, c) {
         if (c >= d.n) continue;
         int i, j, kk;
         cell_ijk(d, c, i, j, kk);
         const Nb5 x = load_nb5(u, c, i, j, nx, ny);
         double co[5];
-        const double F = heat_row(d, i, j, x.c, x.w, x.e, x.s, x.n, u0[c], co);
+        const double F = heat_row(d, i, j, x.c, x.w, x.e, x.s, x.n, cpy ? x.c : u0[c], co);
         f2[m] = F * F;
+        if (cpy) u0w[c] = x.c;
         if (norm_only) continue;
         Fo[c] = F;
         const int sl = ((i + j) & 1) * d.h + (c >> 1);
@@ -2423,6 +2430,7 @@ struct Inst {
     int max_newton = 0;            // per-slot Newton limit per solve (0: the batch's max_iter)
     int round = 0;                 // refinement round of the current Newton iteration (host count)
     int newton_prev = 0;           // Newton iterations of the instance's previous step (0: none yet)
+    bool u0_pending = false;       // heat: u0 = u is left to the step's first assembly (S_U0D = 0)
     DevPtr<double> tp;             // tx, ty, tz, pv, src (n each)
     DevPtr<double> p, sw, sw0;
     DevPtr<double> u, u0, F, du, r64, C, W, E, S, N, B, T, sc;
@@ -3915,7 +3923,7 @@ void HeatBatch::begin_step(int slot, double dt) {
     if (I.dt != dt) ++p_->version;
     I.dt = dt;
     if (I.kind == K_HEAT) {
-        p_->pend.push_back({I.u0->p, I.u->p, (long)I.n, 0});   // u0 = u, issued by flush_copies
+        I.u0_pending = true;   // u0 = u in the step's first assembly (S_U0D = 0, cleared with S_F0 below)
     } else {
         // sw0 = sw; pressure solve first, from the previous step's pressure
         p_->pend.push_back({I.sw0->p, I.sw->p, (long)I.n, 0});
@@ -3924,7 +3932,8 @@ void HeatBatch::begin_step(int slot, double dt) {
     }
     I.st = StepStats();
     I.state = Inst::START;
-    launch_fill(I.sc->p + S_F0, 0.0, 1);   // the step's first Newton iteration sets it
+    // S_F0: the step's first Newton iteration sets it; S_U0D (heat): its first assembly
+    launch_fill(I.sc->p + S_F0, 0.0, I.kind == K_HEAT ? 2 : 1);
 }
 
 std::vector<int> HeatBatch::iterate() {
@@ -3973,6 +3982,7 @@ std::vector<int> HeatBatch::iterate() {
         P.assemble(t, norm_only);
         g = P.gather(t, make_int4(S_FN, S_ITT, S_REF, S_RN), S_CONT);
     }
+    for (Inst* I : asmb) I->u0_pending = false;   // the assembly copied it (S_U0D)
     if (P.spec.on && (P.mixed ? P.finish_spec<float>() : P.finish_spec<double>())) {
         // the blind iterations were too few: the work after the solve was gated off and is
         // queued again now that the solve has finished
@@ -4229,6 +4239,13 @@ void HeatBatch::get_twophase_state(int slot, double* p, double* sw) const {
 // captured graphs keep their pointers, so nothing is rebuilt.  A queued step-start copy
 // (u0 = u, sw0 = sw) is issued first, so an upload inside a step changes the Newton
 // start, not the state the step began from.
+// A step-start copy left to the first assembly (heat) is done now.
+static void settle_u0(Inst& I) {
+    if (!I.u0_pending) return;
+    EQ_HIP(hipMemcpyAsync(I.u0->p, I.u->p, (size_t)I.n * sizeof(double), hipMemcpyDeviceToDevice, S()));
+    launch_fill(I.sc->p + S_U0D, 1.0, 1);
+    I.u0_pending = false;
+}
 static void upload(double* dst, const std::vector<double>& h) {
     EQ_HIP(hipMemcpyAsync(dst, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, S()));
     Device::get().sync();   // pageable source
@@ -4237,6 +4254,7 @@ void HeatBatch::set_state(int slot, const double* u) {
     p_->flush_copies();
     Inst& I = get_inst(p_, slot);
     EQ_REQUIRE(I.kind == K_HEAT, "set_state: slot " << slot << " is not a heat instance");
+    settle_u0(I);
     upload(I.u->p, pad_rows(I, u));
 }
 void HeatBatch::set_twophase_state(int slot, const double* p, const double* sw) {
@@ -4303,6 +4321,8 @@ void HeatBatch::export_residual(int slot, const double* u, const double* u0, dou
     EQ_REQUIRE(I.kind == K_HEAT, "slot " << slot << " is not a heat instance");
     I.u = dev_from_host(pad_rows(I, u).data(), I.n);
     I.u0 = dev_from_host(pad_rows(I, u0).data(), I.n);
+    I.u0_pending = false;
+    launch_fill(I.sc->p + S_U0D, 1.0, 1);   // the given u0, not a step start
     I.dt = dt;
     ++p_->version;
     TableP tp = p_->table({&I});
