@@ -2066,7 +2066,9 @@ __global__ void k_refine_flags(const Desc* __restrict__ D, int ninst, int mode, 
         sc[S_RN] = sc[S_BN];
         sc[S_REF] = 0.0;
         sc[S_ITT] = 0.0;
-        d.act[1] = sc[S_BN] > 0.0 && isfinite(sc[S_BN]);
+        // (a Newton instance whose ||F|| meets abs_res_tol solves nothing: a step start that
+        // runs its first solve before the host has checked ||F_0|| leaves u as it is)
+        d.act[1] = sc[S_BN] > 0.0 && isfinite(sc[S_BN]) && (d.kind == K_LINEAR || sc[S_BN] > fc.abs);
         // this Newton iteration's linear tolerance (inexact Newton; FusedConfig::forcing)
         double eta = tol;
         if (fc.forcing > 0.0 && d.kind != K_LINEAR) {
@@ -2391,8 +2393,9 @@ __global__ __launch_bounds__(BS) void k_newton_update(const Desc* __restrict__ D
     double* x = d.kind == K_TWOPHASE ? (d.act[2] == 0 ? d.p : d.sw) : d.u;
     FOR_CELLS(c) x[c] -= d.du[c];
 }
-// out[q * ninst + i] = scalar slots.x, .y, .z, .w, s4 (q = 0..4) of instance i
-__global__ void k_gather_slots(const Desc* __restrict__ D, int ninst, int4 slots, int s4, double* __restrict__ out) {
+// out[q * ninst + i] = scalar slots.x, .y, .z, .w, s4, s5 (q = 0..5) of instance i
+__global__ void k_gather_slots(const Desc* __restrict__ D, int ninst, int4 slots, int s4, int s5,
+                               double* __restrict__ out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ninst || gated_off(D[i])) return;
     const double* sc = D[i].sc;
@@ -2401,6 +2404,7 @@ __global__ void k_gather_slots(const Desc* __restrict__ D, int ninst, int4 slots
     out[2 * ninst + i] = sc[slots.z];
     out[3 * ninst + i] = sc[slots.w];
     out[4 * ninst + i] = sc[s4];
+    out[5 * ninst + i] = sc[s5];
 }
 #undef FOR_GROUPS
 #undef FOR_CELLS
@@ -2533,7 +2537,7 @@ struct Table {
     std::map<std::tuple<double, int, int>, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
     std::map<std::tuple<double, int, int, int>, int> pred;
     // whole speculative Newton iterations, keyed by (blind Krylov iterations, methods + 8 x
-    // norm-only assembly, Krylov precision, the pinned gather buffer)
+    // norm-only assembly + 16 x step start, Krylov precision, the pinned gather buffer)
     std::map<std::tuple<int, int, int, uintptr_t>, std::pair<hipGraph_t, hipGraphExec_t>> newton_graphs;
     ~Table() {
         for (auto* m : {&graphs})
@@ -3400,7 +3404,7 @@ struct HeatBatch::Impl {
     // path, with one graph launch and one wait.  False (nothing launched) when the solve
     // has no prediction yet or is not speculative; the caller then takes the separate path.
     template <class T>
-    bool newton_graph_iteration(Table& t, std::vector<double>& g, int norm_only) {
+    bool newton_graph_iteration(Table& t, std::vector<double>& g, int norm_only, int start = 0) {
         if (!allow_spec || !use_graph || timing_enabled() || t.solves == 0 || tol_call > 0.0) return false;
         const int maxit = cfg.solver_max_iter, meth = methods(t), ni = (int)t.inst.size();
         const auto pkey = pred_key(t, -1.0, maxit);
@@ -3408,12 +3412,16 @@ struct HeatBatch::Impl {
         if (it == t.pred.end() || it->second < 1) return false;
         const int nb = std::min(maxit, it->second);
         ensure_gpin(ni);
-        const auto gk = std::make_tuple(nb, meth + 8 * norm_only, (int)sizeof(T), (uintptr_t)gpin);
+        const auto gk = std::make_tuple(nb, meth + 8 * norm_only + 16 * start, (int)sizeof(T), (uintptr_t)gpin);
         auto gi = t.newton_graphs.find(gk);
         if (gi == t.newton_graphs.end()) {
             hipGraph_t graph;
             hipGraphExec_t exec;
             EQ_HIP(hipStreamBeginCapture(S(), hipStreamCaptureModeThreadLocal));
+            if (start) {   // the step's first assembly (u0 = u) ahead of its first solve
+                launch_assemble(t, 0);
+                fin(t, Fcell(S_FN));
+            }
             solve_front<T>(t);
             krylov_start<T>(t, meth);
             for (int k = 0; k < nb; ++k) krylov_iteration<T>(t, -1.0, maxit, meth);
@@ -3439,7 +3447,7 @@ struct HeatBatch::Impl {
         spec.meth = meth;
         spec.k = nb;
         Device::get().sync();
-        g.assign(gpin, gpin + 5 * (size_t)ni);
+        g.assign(gpin, gpin + 6 * (size_t)ni);
         return true;
     }
     template <class T>
@@ -3516,24 +3524,26 @@ struct HeatBatch::Impl {
 
     // per-instance scalars of five slots in one launch, stored into pinned host memory:
     // result[q * ninst + i] = scalar slot q of instance i
-    std::vector<double> gather(const Table& t, int4 slots, int s4) {
+    // six slots per instance (the sixth: S_BN, ||F|| at the start of the latest solve)
+    std::vector<double> gather(const Table& t, int4 slots, int s4, int s5 = S_BN) {
         const int ni = (int)t.inst.size();
         ensure_gpin(ni);
-        launch_gather(t, slots, s4);
+        launch_gather(t, slots, s4, s5);
         Device::get().sync();
-        return std::vector<double>(gpin, gpin + 5 * (size_t)ni);
+        return std::vector<double>(gpin, gpin + 6 * (size_t)ni);
     }
     void ensure_gpin(int ni) {
-        if (gcap >= 5 * (size_t)ni) return;
+        if (gcap >= 6 * (size_t)ni) return;
         Device::get().sync();
         if (gpin) EQ_HIP(hipHostFree(gpin));
         gpin = nullptr;
-        gcap = 5 * (size_t)ni;
+        gcap = 6 * (size_t)ni;
         EQ_HIP(hipHostMalloc((void**)&gpin, gcap * sizeof(double), hipHostMallocCoherent));
     }
-    void launch_gather(const Table& t, int4 slots, int s4) {
+    void launch_gather(const Table& t, int4 slots, int s4, int s5 = S_BN) {
         const int ni = (int)t.inst.size();
-        hipLaunchKernelGGL(k_gather_slots, dim3((ni + 63) / 64), dim3(64), 0, S(), t.desc->p, ni, slots, s4, gpin);
+        hipLaunchKernelGGL(k_gather_slots, dim3((ni + 63) / 64), dim3(64), 0, S(), t.desc->p, ni, slots, s4, s5,
+                           gpin);
         EQ_HIP(hipGetLastError());
     }
 };
@@ -3959,6 +3969,27 @@ std::vector<int> HeatBatch::iterate() {
     int norm_only = !solve.empty() && solve.size() == asmb.size();
     for (const Inst* I : solve)
         if (I->kind != K_HEAT || I->newton_prev <= 0 || I->st.newton_stage + 1 < I->newton_prev) norm_only = 0;
+    // A step start of heat instances whose first solve the previous steps predict: the
+    // first assembly and the first Newton iteration in one graph; the host reads ||F_0||
+    // (S_BN) afterwards, and an instance that had already converged (||F_0|| <=
+    // abs_res_tol) found its solve a no-op on the device.
+    bool merged = false;
+    if (solve.empty() && !asmb.empty() && P.cfg.speculate && P.cfg.max_iter >= 1) {
+        bool ok = true;
+        int no = 1;   // norm-only assembly: each previous step took one Newton iteration
+        for (const Inst* I : asmb) {
+            ok &= I->state == Inst::START && I->kind == K_HEAT;
+            if (I->newton_prev != 1) no = 0;
+        }
+        if (ok) {
+            tps = P.table(asmb);
+            P.allow_spec = true;
+            merged = P.mixed ? P.newton_graph_iteration<float>(*tps, g, no, 1)
+                             : P.newton_graph_iteration<double>(*tps, g, no, 1);
+            P.allow_spec = false;
+            if (merged) whole = true, norm_only = no;
+        }
+    }
     if (!solve.empty()) {
         tps = P.table(solve);
         Table& t = *tps;
@@ -3993,11 +4024,11 @@ std::vector<int> HeatBatch::iterate() {
         g = P.gather(t, make_int4(S_FN, S_ITT, S_REF, S_RN), S_CONT);
     }
     const FusedConfig& c = P.cfg;
-    for (size_t q = 0; q < na; ++q) {
+    auto process = [&](size_t q, const std::vector<double>& g) {
         Inst& I = *asmb[q];
         if (I.state == Inst::SOLVE && g[4 * na + q] != 0.0) {   // next refinement round
             ++I.round;
-            continue;
+            return;
         }
         I.round = 0;
         if (I.state == Inst::SOLVE) {   // it was solved and updated in this call
@@ -4033,7 +4064,7 @@ std::vector<int> HeatBatch::iterate() {
             launch_fill(I.sw->p, 0.5, I.n);
             launch_fill(I.sc->p + S_F0, 0.0, 1);   // the transport stage's own first iteration
             I.state = Inst::START;
-            continue;
+            return;
         }
         if (stop || bad || out) {
             I.st.stop = bad ? 0 : stop;
@@ -4045,6 +4076,15 @@ std::vector<int> HeatBatch::iterate() {
         } else {
             I.state = Inst::SOLVE;
         }
+    };
+    if (merged) {   // the step start as its own assembly's check would have seen it, then the solve
+        std::vector<double> g0(6 * na, 0.0);
+        for (size_t q = 0; q < na; ++q) g0[q] = g[5 * na + q] * g[5 * na + q];
+        for (size_t q = 0; q < na; ++q) process(q, g0);
+        for (size_t q = 0; q < na; ++q)
+            if (asmb[q]->state == Inst::SOLVE) process(q, g);
+    } else {
+        for (size_t q = 0; q < na; ++q) process(q, g);
     }
     if (norm_only) {   // an instance goes on: its next solve reads F and the coefficients
         bool more = false;

@@ -361,6 +361,35 @@ def test_predicted_last_newton_check_matches_full_assembly(hip_api, precision):
     assert np.array_equal(ua, ub)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_converged_step_start_leaves_state_unchanged(hip_api, precision):
+    """A step start whose ||F_0|| already meets abs_res_tol: with the previous steps' prediction
+    the first assembly and the first solve run as one graph before the host sees ||F_0||; the
+    solve must then be a no-op (no Newton iteration counted, u unchanged, converged)."""
+    nx, ny = 96, 64
+    params = f"precision={precision} preconditioner=ilu0 rel_res_tol=1e-9 abs_res_tol=1e-12 solver_tol=1e-6"
+    b = make_batch(hip_api, params)
+    try:
+        s = add(hip_api, b, nx, ny, 0.2, gaussian_bumps(nx, ny, 3, width_scale=0.1))
+        arr = np.array([s], dtype=np.int32)
+        for _ in range(2):
+            hip_api.batch_step(b, iptr(arr), 1, 1.0)
+        assert stats(hip_api, b, s)[0][0] >= 1
+        flat = np.full(nx * ny, 0.7)   # uniform: no flux, F_0 = 0
+        hip_api.batch_set_state(b, s, dptr(flat))
+        hip_api.batch_step(b, iptr(arr), 1, 1.0)
+        st, nrm = stats(hip_api, b, s)
+        assert st[0] == 0 and st[3] == 1 and st[4] == 0, (st, nrm)
+        assert np.array_equal(state(hip_api, b, s, nx * ny), flat)
+        # and the next non-trivial step runs normally
+        hip_api.batch_set_state(b, s, dptr(gaussian_bumps(nx, ny, 4, width_scale=0.1)))
+        hip_api.batch_step(b, iptr(arr), 1, 1.0)
+        st, nrm = stats(hip_api, b, s)
+        assert st[0] >= 1 and st[3] != 0 and st[4] == 0, (st, nrm)
+    finally:
+        hip_api.batch_destroy(b)
+
+
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("mixed", 1e-9)])
 def test_pair_vectors_match_strided_form(hip_api, precision, tol):
     """The four-ranks-per-lane Krylov kernels (2D heat, nx % 8 == 0) and the strided
