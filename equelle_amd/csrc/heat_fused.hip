@@ -2506,6 +2506,7 @@ struct Table {
     bool has_vec = false, has_nonvec = false;
     bool has_schur_strided = false;   // reduced-form instances in the strided form
     bool has_tile = false;            // four-ranks-per-lane instances with the tiled operator
+    bool has_vec_onepass = false;     // four-ranks-per-lane instances with the one-pass operator
     // Multigrid plan over the instances with a hierarchy; index l = 1 .. levels - 1: the
     // instances that have level l, their level l - 1 (fine) and level l (coarse)
     // descriptors, blocks over the coarse cells, the fine pairs and the coarse pairs.
@@ -2819,6 +2820,7 @@ struct HeatBatch::Impl {
             t.has_nonvec |= d.vec == 0;
             t.has_schur_strided |= d.pcm == PM_SCHUR && d.vec == 0;
             t.has_tile |= d.tile != 0;
+            t.has_vec_onepass |= d.vec && !d.tile;
         }
         t.ng = g;
         for (int ci = 0; ci < 4; ++ci) {
@@ -3088,7 +3090,7 @@ struct HeatBatch::Impl {
             HLP(C_SPMV, (k_schur_black<T, CP>), iv, iw, nd, a0, a1, t.part->p, t.ng, meth);
         }
         const int ivv = iv == V_PH ? (int)V_P : iv == V_SH ? (int)V_S : iv;
-        if (t.has_vec) HLP(C_SPMV, (k_schur_vec<T, CP>), ivv, iw, nd, a0, t.part->p, t.ng, meth);
+        if (t.has_vec_onepass) HLP(C_SPMV, (k_schur_vec<T, CP>), ivv, iw, nd, a0, t.part->p, t.ng, meth);
         if (t.has_tile) {
             hipLaunchKernelGGL((k_schur_tile<T>), dim3((unsigned)t.tnb), dim3(BS), 0, S(), t.desc->p, t.tblk->p, ivv,
                                iw, nd, a0, t.part->p, t.ng, meth);
