@@ -2393,18 +2393,29 @@ __global__ __launch_bounds__(BS) void k_newton_update(const Desc* __restrict__ D
     double* x = d.kind == K_TWOPHASE ? (d.act[2] == 0 ? d.p : d.sw) : d.u;
     FOR_CELLS(c) x[c] -= d.du[c];
 }
-// out[q * ninst + i] = scalar slots.x, .y, .z, .w, s4, s5 (q = 0..5) of instance i
-__global__ void k_gather_slots(const Desc* __restrict__ D, int ninst, int4 slots, int s4, int s5,
-                               double* __restrict__ out) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ninst || gated_off(D[i])) return;
-    const double* sc = D[i].sc;
-    out[i] = sc[slots.x];
-    out[ninst + i] = sc[slots.y];
-    out[2 * ninst + i] = sc[slots.z];
-    out[3 * ninst + i] = sc[slots.w];
-    out[4 * ninst + i] = sc[s4];
-    out[5 * ninst + i] = sc[s5];
+// out[q * ninst + i] = scalar slots.x, .y, .z, .w, s4, s5 (q = 0..5) of instance i, into
+// pinned host memory; one block, which then stores seq into *flag (pinned too) after a
+// system-scope fence, so the host can wait for the values by watching the flag.
+constexpr int GATHER_BS = 1024;
+__global__ __launch_bounds__(GATHER_BS) void k_gather_slots(const Desc* __restrict__ D, int ninst, int4 slots,
+                                                             int s4, int s5, double* __restrict__ out,
+                                                             int* __restrict__ flag, int seq) {
+    for (int i = (int)threadIdx.x; i < ninst; i += (int)blockDim.x) {
+        if (gated_off(D[i])) continue;
+        const double* sc = D[i].sc;
+        out[i] = sc[slots.x];
+        out[ninst + i] = sc[slots.y];
+        out[2 * ninst + i] = sc[slots.z];
+        out[3 * ninst + i] = sc[slots.w];
+        out[4 * ninst + i] = sc[s4];
+        out[5 * ninst + i] = sc[s5];
+    }
+    __threadfence_system();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence_system();
+        __hip_atomic_store(flag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
 }
 #undef FOR_GROUPS
 #undef FOR_CELLS
@@ -2592,6 +2603,10 @@ struct HeatBatch::Impl {
     }
     double* gpin = nullptr;
     size_t gcap = 0;
+    int* gflag = nullptr;   // pinned: the sequence number of the latest gather (k_gather_slots)
+    int gseq = 0;           // the latest direct gather's number (1 .. 2^30)
+    // the number every gather captured in a graph stores (the flag is cleared before a replay)
+    static constexpr int GSEQ_GRAPH = -7;
     // step-start copies queued by begin_step, issued together before the next device work
     // that reads or replaces those fields (flush_copies)
     std::vector<McJob> pend;
@@ -3434,12 +3449,13 @@ struct HeatBatch::Impl {
             newton_update(t);
             launch_assemble(t, norm_only);
             fin(t, Fcell(S_FN));
-            launch_gather(t, make_int4(S_FN, S_ITT, S_REF, S_RN), S_CONT);
+            launch_gather(t, make_int4(S_FN, S_ITT, S_REF, S_RN), S_CONT, S_BN, GSEQ_GRAPH);
             EQ_HIP(hipStreamEndCapture(S(), &graph));
             EQ_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
             gi = t.newton_graphs.emplace(gk, std::make_pair(graph, exec)).first;
         }
         ++t.solves;
+        __atomic_store_n(gflag, 0, __ATOMIC_RELEASE);   // every earlier gather has been waited for
         EQ_HIP(hipGraphLaunch(gi->second.second, S()));
         spec.on = true;
         spec.t = &t;
@@ -3448,7 +3464,7 @@ struct HeatBatch::Impl {
         spec.maxit = maxit;
         spec.meth = meth;
         spec.k = nb;
-        Device::get().sync();
+        wait_gather(GSEQ_GRAPH);
         g.assign(gpin, gpin + 6 * (size_t)ni);
         return true;
     }
@@ -3530,9 +3546,27 @@ struct HeatBatch::Impl {
     std::vector<double> gather(const Table& t, int4 slots, int s4, int s5 = S_BN) {
         const int ni = (int)t.inst.size();
         ensure_gpin(ni);
-        launch_gather(t, slots, s4, s5);
-        Device::get().sync();
+        gseq = gseq >= (1 << 30) ? 1 : gseq + 1;
+        launch_gather(t, slots, s4, s5, gseq);
+        wait_gather(gseq);
         return std::vector<double>(gpin, gpin + 6 * (size_t)ni);
+    }
+    // Waits for the latest gather by watching its flag (the values are in place once it
+    // shows the gather's number), with a stream query every so often: a stream that has
+    // finished without the flag (an error) is reported by synchronising it.
+    void wait_gather(int seq) {
+        for (long it = 1;; ++it) {
+            if (__atomic_load_n(gflag, __ATOMIC_ACQUIRE) == seq) return;
+            if ((it & 255) == 0) {
+                const hipError_t e = hipStreamQuery(S());
+                if (e == hipErrorNotReady) continue;
+                EQ_HIP(e);
+                if (__atomic_load_n(gflag, __ATOMIC_ACQUIRE) == seq) return;
+                Device::get().sync();
+                EQ_REQUIRE(__atomic_load_n(gflag, __ATOMIC_ACQUIRE) == seq, "gathered scalars missing");
+                return;
+            }
+        }
     }
     void ensure_gpin(int ni) {
         if (gcap >= 6 * (size_t)ni) return;
@@ -3542,10 +3576,11 @@ struct HeatBatch::Impl {
         gcap = 6 * (size_t)ni;
         EQ_HIP(hipHostMalloc((void**)&gpin, gcap * sizeof(double), hipHostMallocCoherent));
     }
-    void launch_gather(const Table& t, int4 slots, int s4, int s5 = S_BN) {
+    void launch_gather(const Table& t, int4 slots, int s4, int s5, int seq) {
         const int ni = (int)t.inst.size();
-        hipLaunchKernelGGL(k_gather_slots, dim3((ni + 63) / 64), dim3(64), 0, S(), t.desc->p, ni, slots, s4, s5,
-                           gpin);
+        const int nt = std::min(GATHER_BS, std::max(64, (ni + 63) / 64 * 64));
+        hipLaunchKernelGGL(k_gather_slots, dim3(1), dim3(nt), 0, S(), t.desc->p, ni, slots, s4, s5, gpin, gflag,
+                           seq);
         EQ_HIP(hipGetLastError());
     }
 };
@@ -3565,6 +3600,8 @@ HeatBatch::HeatBatch(const FusedConfig& cfg) : p_(new Impl()) {
                "krylov_form_twophase must be reduced, full or auto");
     p_->use_graph = cfg.graph;
     EQ_HIP(hipHostMalloc((void**)&p_->host_count, 2 * (Impl::NRING + 1) * sizeof(int), hipHostMallocCoherent));
+    EQ_HIP(hipHostMalloc((void**)&p_->gflag, sizeof(int), hipHostMallocCoherent));
+    *p_->gflag = 0;
     p_->gate = dev_alloc<int>(1);
     const int one = 1;
     EQ_HIP(hipMemcpy(p_->gate->p, &one, sizeof(int), hipMemcpyHostToDevice));
@@ -3578,6 +3615,7 @@ HeatBatch::~HeatBatch() {
         if (p_->ev[i]) (void)hipEventDestroy(p_->ev[i]);
     if (p_->host_count) (void)hipHostFree(p_->host_count);
     if (p_->gpin) (void)hipHostFree(p_->gpin);
+    if (p_->gflag) (void)hipHostFree(p_->gflag);
     if (p_->mc_ev) (void)hipEventDestroy(p_->mc_ev);
     if (p_->mc_host) (void)hipHostFree(p_->mc_host);
     delete p_;
