@@ -783,6 +783,7 @@ static FusedConfig fused_config(const char* params) {
     c.max_refine = m.get_int("max_refine", c.max_refine);
     c.graph = m.get_int("graph", 1) != 0;
     c.speculate = m.get_int("speculate", 1) != 0;
+    c.newton_graph = m.get_int("newton_graph", c.newton_graph ? 1 : 0) != 0;
     c.cpt = m.get_int("cpt", c.cpt);
     EQ_REQUIRE(c.cpt == 0 || c.cpt == 1 || c.cpt == 2 || c.cpt == 4 || c.cpt == 8, "cpt must be 0, 1, 2, 4 or 8");
     c.xcd = m.get_int("xcd", c.xcd) != 0 ? 1 : 0;

@@ -2552,6 +2552,9 @@ struct Table {
     // norm-only assembly + 16 x step start, Krylov precision, the pinned gather buffer)
     std::map<std::tuple<int, int, int, uintptr_t>, std::pair<hipGraph_t, hipGraphExec_t>> newton_graphs;
     ~Table() {
+        // a replay's last kernel may still be retiring after the host saw its gather flag:
+        // no executable graph is destroyed before the stream has drained
+        if (!graphs.empty() || !newton_graphs.empty()) (void)hipStreamSynchronize(S());
         for (auto* m : {&graphs})
             for (auto& g : *m) {
                 (void)hipGraphExecDestroy(g.second.second);
@@ -3429,12 +3432,8 @@ struct HeatBatch::Impl {
         if (it == t.pred.end() || it->second < 1) return false;
         const int nb = std::min(maxit, it->second);
         ensure_gpin(ni);
-        const auto gk = std::make_tuple(nb, meth + 8 * norm_only + 16 * start, (int)sizeof(T), (uintptr_t)gpin);
-        auto gi = t.newton_graphs.find(gk);
-        if (gi == t.newton_graphs.end()) {
-            hipGraph_t graph;
-            hipGraphExec_t exec;
-            EQ_HIP(hipStreamBeginCapture(S(), hipStreamCaptureModeThreadLocal));
+        // the whole iteration's launches (captured once, or issued directly)
+        auto launches = [&](int seq) {
             if (start) {   // the step's first assembly (u0 = u) ahead of its first solve
                 launch_assemble(t, 0);
                 fin(t, Fcell(S_FN));
@@ -3449,14 +3448,29 @@ struct HeatBatch::Impl {
             newton_update(t);
             launch_assemble(t, norm_only);
             fin(t, Fcell(S_FN));
-            launch_gather(t, make_int4(S_FN, S_ITT, S_REF, S_RN), S_CONT, S_BN, GSEQ_GRAPH);
-            EQ_HIP(hipStreamEndCapture(S(), &graph));
-            EQ_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-            gi = t.newton_graphs.emplace(gk, std::make_pair(graph, exec)).first;
+            launch_gather(t, make_int4(S_FN, S_ITT, S_REF, S_RN), S_CONT, S_BN, seq);
+        };
+        int wait_seq = GSEQ_GRAPH;
+        if (cfg.newton_graph) {
+            const auto gk = std::make_tuple(nb, meth + 8 * norm_only + 16 * start, (int)sizeof(T), (uintptr_t)gpin);
+            auto gi = t.newton_graphs.find(gk);
+            if (gi == t.newton_graphs.end()) {
+                hipGraph_t graph;
+                hipGraphExec_t exec;
+                EQ_HIP(hipStreamBeginCapture(S(), hipStreamCaptureModeThreadLocal));
+                launches(GSEQ_GRAPH);
+                EQ_HIP(hipStreamEndCapture(S(), &graph));
+                EQ_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+                gi = t.newton_graphs.emplace(gk, std::make_pair(graph, exec)).first;
+            }
+            __atomic_store_n(gflag, 0, __ATOMIC_RELEASE);   // every earlier gather has been waited for
+            EQ_HIP(hipGraphLaunch(gi->second.second, S()));
+        } else {   // the same launches one by one: the first starts at once on an idle stream
+            gseq = gseq >= (1 << 30) ? 1 : gseq + 1;
+            wait_seq = gseq;
+            launches(gseq);
         }
         ++t.solves;
-        __atomic_store_n(gflag, 0, __ATOMIC_RELEASE);   // every earlier gather has been waited for
-        EQ_HIP(hipGraphLaunch(gi->second.second, S()));
         spec.on = true;
         spec.t = &t;
         spec.pkey = pkey;
@@ -3464,7 +3478,7 @@ struct HeatBatch::Impl {
         spec.maxit = maxit;
         spec.meth = meth;
         spec.k = nb;
-        wait_gather(GSEQ_GRAPH);
+        wait_gather(wait_seq);
         g.assign(gpin, gpin + 6 * (size_t)ni);
         return true;
     }

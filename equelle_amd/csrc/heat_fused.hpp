@@ -91,6 +91,9 @@ struct FusedConfig {
     // iterations without waiting for the convergence count (gated on device; resolved at
     // the iteration's norm check)
     bool speculate = true;
+    // A whole speculative Newton iteration as one captured graph (1) or as direct launches
+    // (0: on an idle stream the first kernel starts at once, where a graph launch waits).
+    bool newton_graph = false;
     int cpt = 0;                       // cells per thread (1, 2, 4, 8); 0 = chosen by the tuner
     // a fixed shape per kernel class (assembly, defect, SpMV, preconditioner, vector
     // updates; 0 = none): set with the XCD order by parameter "shape=a,d,s,p,v,xcd"
