@@ -112,7 +112,20 @@ struct Fin {
     int act, nd, maxit, slot;
     int kry;                      // partials of a Krylov kernel (groups of pairs) or of a cell kernel
     double tol2;
+    // a one-instance table's gather in the same launch (k_gather_slots' stores and flag):
+    // gout non-null -> after the action, gout[q] = sc[gs[q]] (q = 0..5), then *gflag = gseq
+    double* gout;
+    int* gflag;
+    int gs[6];
+    int gseq;
 };
+// The gather of a one-instance table from a finaliser's thread 0 (see k_gather_slots).
+__device__ inline void fin_gather(const Desc& d, const Fin& f) {
+    if (!gated_off(d))
+        for (int q = 0; q < 6; ++q) f.gout[q] = d.sc[f.gs[q]];
+    __threadfence_system();
+    __hip_atomic_store(f.gflag, f.gseq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
 
 template <int NT = BS>
 __device__ inline void bred_store(double v, double* out) {
@@ -229,7 +242,10 @@ __device__ inline void cell_ijk(const Desc& d, int c, int& i, int& j, int& k) {
 // each set is summed in the same order for either ND.
 template <int ND>
 __device__ void fin_body(const Desc& d, const double* part, int nb, const Fin& f) {
-    if (!f.kry && gated_off(d)) return;   // norms of gated cell kernels (assembly, defect)
+    if (!f.kry && gated_off(d)) {   // norms of gated cell kernels (assembly, defect)
+        if (f.gout && threadIdx.x == 0) fin_gather(d, f);
+        return;
+    }
     const int2 r = make_int2(d.gbase, d.gbase + (f.kry ? d.ngrpk : d.ngrp));
     __shared__ double res[5];
     __shared__ double lds[ND][FBS / 64];
@@ -409,6 +425,7 @@ __device__ void fin_body(const Desc& d, const double* part, int nb, const Fin& f
             if (slot == S_FN && !f.kry) sc[S_U0D] = 1.0;   // an assembly has run: u0 is in place
             break;
     }
+    if (f.gout) fin_gather(d, f);
 }
 
 
@@ -3049,7 +3066,7 @@ struct HeatBatch::Impl {
     }
     // finaliser of a Krylov reduction (partials over pairs) ...
     static Fin F(int act, int nd, double tol2 = 0, int maxit = 0, int slot = 0) {
-        Fin f;
+        Fin f{};
         f.act = act;
         f.nd = nd;
         f.kry = 1;
@@ -3447,8 +3464,18 @@ struct HeatBatch::Impl {
             linear_tail<T>(t);
             newton_update(t);
             launch_assemble(t, norm_only);
-            fin(t, Fcell(S_FN));
-            launch_gather(t, make_int4(S_FN, S_ITT, S_REF, S_RN), S_CONT, S_BN, seq);
+            if (ni == 1) {   // one instance: the norm's finaliser also stores the gathered scalars
+                Fin f = Fcell(S_FN);
+                const int gs[6] = {S_FN, S_ITT, S_REF, S_RN, S_CONT, S_BN};
+                f.gout = gpin;
+                f.gflag = gflag;
+                for (int q = 0; q < 6; ++q) f.gs[q] = gs[q];
+                f.gseq = seq;
+                fin(t, f);
+            } else {
+                fin(t, Fcell(S_FN));
+                launch_gather(t, make_int4(S_FN, S_ITT, S_REF, S_RN), S_CONT, S_BN, seq);
+            }
         };
         int wait_seq = GSEQ_GRAPH;
         if (cfg.newton_graph) {
