@@ -1171,10 +1171,14 @@ __global__ __launch_bounds__(BS) void k_schur_vec(const Desc* __restrict__ D, co
 // k_schur_vec's.  Each red unknown is formed 1.5 times per tile instead of 3 times, and
 // the neighbour values come from LDS instead of L2.
 constexpr int TILE_ROW = 264;   // LDS values per tile row: [3] west cell, [4, 260) the tile's ranks, [260] east cell
+// FP32: at most 64 VGPRs, so eight waves per SIMD fit and a 2048 x 2048 grid's 8192 waves
+// run in one round on 256 CUs (at six waves the last quarter ran as a second, sparse round).
 template <class T>
-__global__ __launch_bounds__(BS) void k_schur_tile(const Desc* __restrict__ D, const int2* __restrict__ blk, int iv,
-                                                   int iw, int nd, int a0, double* __restrict__ part, int ng,
-                                                   int meth) {
+__global__ __launch_bounds__(BS, sizeof(T) == 4 ? 8 : 1) void k_schur_tile(const Desc* __restrict__ D,
+                                                                           const int2* __restrict__ blk, int iv,
+                                                                           int iw, int nd, int a0,
+                                                                           double* __restrict__ part, int ng,
+                                                                           int meth) {
     const int lb = D[0].xcd ? xcd_block((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x;
     const int2 bi = blk[lb];
     const Desc& d = D[bi.x];
@@ -1195,10 +1199,10 @@ __global__ __launch_bounds__(BS) void k_schur_tile(const Desc* __restrict__ D, c
     const T* __restrict__ dvr = (const T*)d.dinv;
     const T* kr[5] = {nullptr, (const T*)d.K[1], (const T*)d.K[2], (const T*)d.K[3], (const T*)d.K[4]};
     const V4<T> zero{{(T)0, (T)0, (T)0, (T)0}};
-    // phase 2's own-row operands, loaded first so their latency overlaps phase 1
+    // phase 2's diagonal, operand and dot operand, loaded first so their latency overlaps
+    // phase 1 (the four coefficients wait until after it: with them the kernel needs more
+    // than 64 VGPRs)
     const int h = d.h, pl = j < ny ? p0 : q0;   // (rows below the grid: an in-bounds stand-in)
-    const V4<T> kw = ld4((const T*)d.K[1] + h + pl), ke = ld4((const T*)d.K[2] + h + pl);
-    const V4<T> ks = ld4((const T*)d.K[3] + h + pl), kn = ld4((const T*)d.K[4] + h + pl);
     const V4<T> cb = ld4((const T*)d.K[0] + h + pl), vv = ld4(v + pl), aa = ld4((const T*)d.v[a0] + pl);
     // phase 1: red rows j0 - 1 .. j0 + 4 (window rows 0 .. 5; zero outside the grid)
     for (int c = (int)threadIdx.x; c < 6 * 64; c += BS) {
@@ -1230,6 +1234,8 @@ __global__ __launch_bounds__(BS) void k_schur_tile(const Desc* __restrict__ D, c
     __syncthreads();
     if (j >= ny) return;   // wave-uniform: the tile's rows below the grid
     // phase 2: black row j (window row w + 1)
+    const V4<T> kw = ld4((const T*)d.K[1] + h + pl), ke = ld4((const T*)d.K[2] + h + pl);
+    const V4<T> ks = ld4((const T*)d.K[3] + h + pl), kn = ld4((const T*)d.K[4] + h + pl);
     const int rj = w + 1, o0 = 4 + 4 * lane;
     const V4<T> yc = *reinterpret_cast<const V4<T>*>(&ylds[rj][o0]);
     const V4<T> ys = *reinterpret_cast<const V4<T>*>(&ylds[rj - 1][o0]);
@@ -2319,9 +2325,11 @@ __global__ __launch_bounds__(BS) void k_setup_vec(const Desc* __restrict__ D, co
 // Reduced system: the red unknowns are recovered first, x_r = C_r^-1 (b_r - A_rb x_b).
 // A solve whose last residual is not finite (breakdown, overflow) contributes nothing:
 // the refinement defect then shows no progress and the refinement stops.
+// (FP32 up to four ranks per lane: eight waves per SIMD, as k_schur_tile.)
 template <class T, int CPT>
-__global__ __launch_bounds__(BS) void k_accum(const Desc* __restrict__ D, const int2* __restrict__ blk, int scaled,
-                                              int first) {
+__global__ __launch_bounds__(BS, sizeof(T) == 4 && CPT <= 4 ? 8 : 1) void k_accum(const Desc* __restrict__ D,
+                                                                                  const int2* __restrict__ blk,
+                                                                                  int scaled, int first) {
     LOCATE();
     if (gated_off(d)) return;
     const bool first_round = first < 0 ? d.sc[S_REF] == 0.0 : first != 0;   // first < 0: the instance's own round
