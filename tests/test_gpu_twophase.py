@@ -281,6 +281,38 @@ def test_multigrid_levels_above_the_tail_match_bicgstab(hip_api, precision):
     assert it1 < it0, (it1, it0)   # fewer pressure iterations with the V-cycle
 
 
+@pytest.mark.parametrize("precision,newton_tol,solver_tol,bar", [("fp64", 1e-10, 1e-10, 1e-7),
+                                                                  ("mixed", 1e-8, 1e-6, 1e-5)])
+def test_multigrid_levels_above_the_tail_match_oracle(hip_api, oracle_api, precision, newton_tol, solver_tol, bar):
+    """18x16x16 (4608 cells, above the 4096-cell workgroup tail): the fine level of the
+    pressure V-cycle runs the one-level multigrid kernels (smoothing, restriction and
+    prolongation folded into the up-sweep), the 9x8x8 level and below the tail.  Two steps
+    of the default pressure stage (CG with the V-cycle) against the oracle's BiCGStab."""
+    nx, ny, nz = 18, 16, 16
+    perm, poro, sw0, sc, sv = case(nx, ny, nz, seed=3)
+    n = perm.size
+    dts = [1.0, 1.0]
+    po, so, _ = oracle_run(oracle_api, twophase_params(nx, ny, nz, rel_res_tol=1e-11, max_iter=40).encode(),
+                           perm, poro, sw0, sc, sv, dts)
+    b = fused_batch(hip_api, f"precision={precision} preconditioner=ilu0 rel_res_tol={newton_tol} abs_res_tol=0 "
+                             f"solver_tol={solver_tol} max_iter=40 solver_max_iter=5000 twophase_pressure_mg=1 "
+                             "krylov_form_twophase=full")
+    try:
+        slot = fused_add(hip_api, b, nx, ny, nz, perm, poro, sw0, sc, sv)
+        st, tps = np.zeros(5, dtype=np.int32), np.zeros(3, dtype=np.int32)
+        for dt in dts:
+            hip_api.batch_step(b, iptr(np.array([slot], dtype=np.int32)), 1, dt)
+            hip_api.batch_stats(b, slot, iptr(st), None)
+            hip_api.batch_twophase_stats(b, slot, iptr(tps))
+            assert st[3] in (1, 2) and tps[2] in (1, 2) and st[4] == 0, (st, tps)
+        ph, sh = np.zeros(n), np.zeros(n)
+        hip_api.batch_twophase_state(b, slot, dptr(ph), dptr(sh))
+    finally:
+        hip_api.batch_destroy(b)
+    assert np.allclose(sh, so, rtol=bar, atol=0.1 * bar)
+    assert np.allclose(ph - ph[-1], po - po[-1], rtol=bar, atol=bar * np.abs(po).max())
+
+
 @pytest.mark.parametrize("precision", ["mixed", "fp64"])   # fp64: both kinds in the reduced form
 def test_fused_twophase_and_heat_share_a_batch(hip_api, precision):
     """Two-phase and heat instances in one batch give bit-identical results to each run
