@@ -63,7 +63,7 @@ struct Desc {
     int ns;                       // Krylov slots per pair rank: 1 (reduced system) or 2 (full system)
     int ngrpk;                    // Krylov partials: groups [gbase, gbase + ngrpk) of BS pairs
     int vec;                      // Krylov kernels in the four-ranks-per-lane form (2D heat, nx % 8 == 0)
-    int tile;                     // vec, nx % 512 == 0: the reduced operator in 4-row tiles (k_schur_tile)
+    int tile;                     // vec, nx % 512 == 0: the reduced operator in 8-row tiles (k_schur_tile)
     int cgp;                      // two-phase, reduced form: the pressure stage (symmetric) is solved by CG
     int mgp;                      // two-phase, full form: pressure by CG with the multigrid V-cycle
     int mgl;                      // multigrid: levels below this one (0 on the coarsest level)
@@ -1162,19 +1162,22 @@ __global__ __launch_bounds__(BS) void k_schur_vec(const Desc* __restrict__ D, co
 }
 // The reduced operator in tiles (Desc::tile: four ranks per lane, nx % 512 == 0, so a row
 // holds a whole number of 256-rank groups).  Block b of an instance takes tile b: the 256
-// pair ranks of group column b % G (G = nx / 512 groups per row) in the four rows from
-// 4 (b / G).  Phase 1: the block's threads form y_r (k_schur_vec's red unknowns, the same
-// arithmetic) of the six red rows the tile touches, four ranks each, into LDS, and four
-// threads the red cell just outside the tile in each of its rows (west or east, by the
-// row's colours).  Phase 2: wave w forms black row 4 (b / G) + w from LDS with the fused
-// dots, one partial per 256-rank group as k_schur_vec stores it; w is bit-identical to
-// k_schur_vec's.  Each red unknown is formed 1.5 times per tile instead of 3 times, and
-// the neighbour values come from LDS instead of L2.
+// pair ranks of group column b % G (G = nx / 512 groups per row) in the TILE_ROWS rows from
+// TILE_ROWS (b / G).  Phase 1: the block's threads form y_r (k_schur_vec's red unknowns,
+// the same arithmetic) of the TILE_ROWS + 2 red rows the tile touches, four ranks each,
+// into LDS, and TILE_ROWS threads the red cell just outside the tile in each of its rows
+// (west or east, by the row's colours).  Phase 2: wave w forms black row
+// TILE_ROWS (b / G) + w from LDS with the fused dots, one partial per 256-rank group as
+// k_schur_vec stores it; w is bit-identical to k_schur_vec's.  Each red unknown is formed
+// 1.25 times per tile instead of 3 times, and the neighbour values come from LDS instead
+// of L2.
 constexpr int TILE_ROW = 264;   // LDS values per tile row: [3] west cell, [4, 260) the tile's ranks, [260] east cell
-// FP32: at most 64 VGPRs, so eight waves per SIMD fit and a 2048 x 2048 grid's 8192 waves
-// run in one round on 256 CUs (at six waves the last quarter ran as a second, sparse round).
+constexpr int TILE_ROWS = 8;    // black rows per tile: one wave each
+constexpr int TBS = 64 * TILE_ROWS;
+// FP32: at most 64 VGPRs, so eight waves per SIMD fit: four 8-wave blocks per CU, and a
+// 2048 x 2048 grid's 1024 tiles run in one round on 256 CUs.
 template <class T>
-__global__ __launch_bounds__(BS, sizeof(T) == 4 ? 8 : 1) void k_schur_tile(const Desc* __restrict__ D,
+__global__ __launch_bounds__(TBS, sizeof(T) == 4 ? 8 : 1) void k_schur_tile(const Desc* __restrict__ D,
                                                                            const int2* __restrict__ blk, int iv,
                                                                            int iw, int nd, int a0,
                                                                            double* __restrict__ part, int ng,
@@ -1183,9 +1186,9 @@ __global__ __launch_bounds__(BS, sizeof(T) == 4 ? 8 : 1) void k_schur_tile(const
     const int2 bi = blk[lb];
     const Desc& d = D[bi.x];
     if (d.pcm != PM_SCHUR || !d.tile || !(meth & meth_of(d))) return;   // block-uniform
-    __shared__ __align__(16) T ylds[6][TILE_ROW];
+    __shared__ __align__(16) T ylds[TILE_ROWS + 2][TILE_ROW];
     const int hr = d.nx >> 1, ny = d.ny, G = hr >> 8;
-    const int band = bi.y / G, col = bi.y - band * G, j0 = 4 * band, qt = col * 256;
+    const int band = bi.y / G, col = bi.y - band * G, j0 = TILE_ROWS * band, qt = col * 256;
     const int w = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
     const int j = j0 + w, q0 = qt + 4 * lane, p0 = j * hr + q0;
     if (!d.act[0]) {   // block-uniform: zero partials, as the other operator kernels store them
@@ -1204,8 +1207,9 @@ __global__ __launch_bounds__(BS, sizeof(T) == 4 ? 8 : 1) void k_schur_tile(const
     // than 64 VGPRs)
     const int h = d.h, pl = j < ny ? p0 : q0;   // (rows below the grid: an in-bounds stand-in)
     const V4<T> cb = ld4((const T*)d.K[0] + h + pl), vv = ld4(v + pl), aa = ld4((const T*)d.v[a0] + pl);
-    // phase 1: red rows j0 - 1 .. j0 + 4 (window rows 0 .. 5; zero outside the grid)
-    for (int c = (int)threadIdx.x; c < 6 * 64; c += BS) {
+    // phase 1: red rows j0 - 1 .. j0 + TILE_ROWS (window rows 0 .. TILE_ROWS + 1; zero
+    // outside the grid)
+    for (int c = (int)threadIdx.x; c < (TILE_ROWS + 2) * 64; c += TBS) {
         const int r = c >> 6, l = c & 63, jr = j0 - 1 + r, qc = qt + 4 * l;
         V4<T> y = zero;
         if (jr >= 0 && jr < ny) {
@@ -1217,7 +1221,7 @@ __global__ __launch_bounds__(BS, sizeof(T) == 4 ? 8 : 1) void k_schur_tile(const
         }
         *reinterpret_cast<V4<T>*>(&ylds[r][4 + 4 * l]) = y;
     }
-    if (threadIdx.x < 4) {   // the red cells outside the tile in black row j0 + threadIdx.x
+    if (threadIdx.x < TILE_ROWS) {   // the red cells outside the tile in black row j0 + threadIdx.x
         const int jb = j0 + (int)threadIdx.x;
         T yw = (T)0, ye = (T)0;
         if (jb < ny) {
@@ -2879,11 +2883,11 @@ struct HeatBatch::Impl {
             t.pnb[ci] = (int)pblk.size();
             t.pblk[ci] = dev_from_host(pblk.data(), pblk.size());
         }
-        {   // tiles of four rows x 256 pair ranks (k_schur_tile)
+        {   // tiles of TILE_ROWS rows x 256 pair ranks (k_schur_tile)
             std::vector<int2> tb;
             for (size_t q = 0; q < list.size(); ++q) {
                 if (!ds[q].tile) continue;
-                const int tiles = (ds[q].nx / 512) * ((ds[q].ny + 3) / 4);
+                const int tiles = (ds[q].nx / 512) * ((ds[q].ny + TILE_ROWS - 1) / TILE_ROWS);
                 for (int b = 0; b < tiles; ++b) tb.push_back(make_int2((int)q, b));
             }
             t.tnb = (int)tb.size();
@@ -3135,7 +3139,7 @@ struct HeatBatch::Impl {
         const int ivv = iv == V_PH ? (int)V_P : iv == V_SH ? (int)V_S : iv;
         if (t.has_vec_onepass) HLP(C_SPMV, (k_schur_vec<T, CP>), ivv, iw, nd, a0, t.part->p, t.ng, meth);
         if (t.has_tile) {
-            hipLaunchKernelGGL((k_schur_tile<T>), dim3((unsigned)t.tnb), dim3(BS), 0, S(), t.desc->p, t.tblk->p, ivv,
+            hipLaunchKernelGGL((k_schur_tile<T>), dim3((unsigned)t.tnb), dim3(TBS), 0, S(), t.desc->p, t.tblk->p, ivv,
                                iw, nd, a0, t.part->p, t.ng, meth);
             EQ_HIP(hipGetLastError());
         }
@@ -4247,11 +4251,11 @@ double HeatBatch::time_kernel(const std::string& name, int reps, double* event_m
     if (name == "operator" && tile) {
         launch = [&](hipEvent_t s0, hipEvent_t s1) {
             if (P.mixed)
-                hipExtLaunchKernelGGL((k_schur_tile<float>), dim3((unsigned)t.tnb), dim3(BS), 0, S(), s0, s1, 0,
+                hipExtLaunchKernelGGL((k_schur_tile<float>), dim3((unsigned)t.tnb), dim3(TBS), 0, S(), s0, s1, 0,
                                       t.desc->p, t.tblk->p, (int)V_P, (int)V_V, 1, (int)V_RH, t.part->p, t.ng,
                                       (int)(M_BICG | M_CG));
             else
-                hipExtLaunchKernelGGL((k_schur_tile<double>), dim3((unsigned)t.tnb), dim3(BS), 0, S(), s0, s1, 0,
+                hipExtLaunchKernelGGL((k_schur_tile<double>), dim3((unsigned)t.tnb), dim3(TBS), 0, S(), s0, s1, 0,
                                       t.desc->p, t.tblk->p, (int)V_P, (int)V_V, 1, (int)V_RH, t.part->p, t.ng,
                                       (int)(M_BICG | M_CG));
         };
