@@ -63,7 +63,7 @@ struct Desc {
     int ns;                       // Krylov slots per pair rank: 1 (reduced system) or 2 (full system)
     int ngrpk;                    // Krylov partials: groups [gbase, gbase + ngrpk) of BS pairs
     int vec;                      // Krylov kernels in the four-ranks-per-lane form (2D heat, nx % 8 == 0)
-    int tile;                     // vec, nx % 512 == 0: the reduced operator in 8-row tiles (k_schur_tile)
+    int tile;                     // vec, nx % 512 == 0: the reduced operator in 16-row tiles (k_schur_tile)
     int cgp;                      // two-phase, reduced form: the pressure stage (symmetric) is solved by CG
     int mgp;                      // two-phase, full form: pressure by CG with the multigrid V-cycle
     int mgl;                      // multigrid: levels below this one (0 on the coarsest level)
@@ -1169,13 +1169,13 @@ __global__ __launch_bounds__(BS) void k_schur_vec(const Desc* __restrict__ D, co
 // (west or east, by the row's colours).  Phase 2: wave w forms black row
 // TILE_ROWS (b / G) + w from LDS with the fused dots, one partial per 256-rank group as
 // k_schur_vec stores it; w is bit-identical to k_schur_vec's.  Each red unknown is formed
-// 1.25 times per tile instead of 3 times, and the neighbour values come from LDS instead
+// 1.125 times per tile instead of 3 times, and the neighbour values come from LDS instead
 // of L2.
 constexpr int TILE_ROW = 264;   // LDS values per tile row: [3] west cell, [4, 260) the tile's ranks, [260] east cell
-constexpr int TILE_ROWS = 8;    // black rows per tile: one wave each
+constexpr int TILE_ROWS = 16;   // black rows per tile: one wave each
 constexpr int TBS = 64 * TILE_ROWS;
-// FP32: at most 64 VGPRs, so eight waves per SIMD fit: four 8-wave blocks per CU, and a
-// 2048 x 2048 grid's 1024 tiles run in one round on 256 CUs.
+// FP32: at most 64 VGPRs, so eight waves per SIMD fit: two 16-wave blocks per CU, and a
+// 2048 x 2048 grid's 512 tiles run in one round on 256 CUs.
 template <class T>
 __global__ __launch_bounds__(TBS, sizeof(T) == 4 ? 8 : 1) void k_schur_tile(const Desc* __restrict__ D,
                                                                            const int2* __restrict__ blk, int iv,
